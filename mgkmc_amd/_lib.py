@@ -1,0 +1,93 @@
+"""ctypes binding of libmgkmc_hip.so (C ABI declared in include/mgkmc_hip.h).
+
+The library is loaded from this package directory only.  There is no CPU
+fallback: if the library or a GPU is missing, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmgkmc_hip.so")
+
+MGK_OK = 0
+MGK_NOT_CONVERGED = 1
+MGK_ERR_ARG = -1
+MGK_ERR_HIP = -2
+MGK_ERR_WORKSPACE = -3
+MGK_ERR_NONFINITE = -4
+MGK_ERR_SOLVER = -5
+
+_lib = None
+
+
+class MgkLsInfo(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int), ("status", ctypes.c_int), ("diff", ctypes.c_double),
+                ("eps_mean", ctypes.c_double * 9), ("sig_mean", ctypes.c_double * 9)]
+
+
+_c_p = ctypes.c_void_p
+_c_i = ctypes.c_int
+_c_d = ctypes.c_double
+_c_sz = ctypes.c_size_t
+_c_i64p = ctypes.POINTER(ctypes.c_int64)
+_c_dp = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); every symbol of include/mgkmc_hip.h
+SIGNATURES = {
+    "mgk_version": (_c_i, []),
+    "mgk_last_error": (ctypes.c_char_p, []),
+    "mgk_device_info": (_c_i, [ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
+    "mgk_ls_workspace_bytes": (_c_i, [_c_i, _c_i64p, _c_i, ctypes.POINTER(_c_sz)]),
+    "mgk_ls_solve": (_c_i, [_c_i, _c_i64p, _c_i, _c_p, _c_p, _c_d, _c_d, _c_dp, _c_p, _c_p, _c_p,
+                            _c_i, _c_i, _c_d, _c_p, _c_sz, _c_p, ctypes.POINTER(MgkLsInfo)]),
+}
+
+
+def load():
+    """Load the library (once) and bind every exported symbol."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} not built: run `python -m mgkmc_amd.build` "
+                           "(there is no CPU fallback for the HIP path)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error():
+    return load().mgk_last_error().decode(errors="replace")
+
+
+def check(rc, what):
+    """Map a C-ABI status to Python exceptions (MGK_NOT_CONVERGED is silent,
+    as the reference's LS loop stops silently at max_iter)."""
+    if rc in (MGK_OK, MGK_NOT_CONVERGED):
+        return rc
+    if rc == MGK_ERR_NONFINITE:
+        return rc
+    raise RuntimeError(f"{what} failed with status {rc}: {last_error()}")
+
+
+def shape_arg(shape):
+    arr = (ctypes.c_int64 * len(shape))(*[int(s) for s in shape])
+    return arr
+
+
+def dptr(t):
+    """Device pointer of a torch tensor (or None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def host_dbl(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_c_dp)

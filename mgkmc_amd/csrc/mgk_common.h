@@ -1,0 +1,280 @@
+// Common device/host utilities for the mgkmc_amd HIP library (gfx950).
+//
+// Layout conventions used by every kernel in this library
+// ---------------------------------------------------------
+//  * A grid is (nx, P, Q) voxels, Q contiguous.  3D: P = ny, Q = nz.
+//    2D: P = 1, Q = ny (so the x axis is always the outermost one).
+//  * Symmetric tensor fields are SoA, component-major:  f[c][x][p][q].
+//      3D components: xx, yy, zz, yz, xz, xy   (C = 6)
+//      2D components: xx, yy, xy               (C = 3)
+//    9-component (finite strain) fields: f[3*i+j][x][p][q], row-major ij.
+//  * Half spectra (r2c along x) are complex, f[c][kx][p][q] with
+//    kx in [0, nx/2].  y and z are full complex axes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#define MGK_NT 256  // threads per workgroup for every tiled kernel
+
+template <typename T>
+struct cplx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmk(T a, T b) {
+  cplx<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return cmk(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return cmk(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) { return cmk(a.x * s, a.y * s); }
+template <typename T>
+__device__ __forceinline__ cplx<T> cconj(cplx<T> a) { return cmk(a.x, -a.y); }
+// multiply by -i (forward) or +i (inverse)
+template <typename T>
+__device__ __forceinline__ cplx<T> mul_mi(cplx<T> a) { return cmk(a.y, -a.x); }
+template <typename T>
+__device__ __forceinline__ cplx<T> mul_pi(cplx<T> a) { return cmk(-a.y, a.x); }
+
+// ---------------------------------------------------------------------------
+// FFT plan: length n factored into radices 8,5,4,3,2 (applied in order).
+// ---------------------------------------------------------------------------
+struct FftPlan {
+  int n;
+  int npass;
+  int radix[12];
+};
+
+// Small fixed-size DFTs.  inv=false: exp(-2 pi i pq/R); inv=true: exp(+...).
+template <typename T>
+__device__ __forceinline__ void dft2(cplx<T>* v) {
+  cplx<T> a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(cplx<T>* v, bool inv) {
+  cplx<T> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+  cplx<T> s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+  cplx<T> r = inv ? mul_pi(d13) : mul_mi(d13);
+  v[0] = cadd(s02, s13);
+  v[2] = csub(s02, s13);
+  v[1] = cadd(d02, r);
+  v[3] = csub(d02, r);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft3(cplx<T>* v, bool inv) {
+  const T c = T(-0.5);
+  const T s = inv ? T(0.86602540378443864676372317075294) : T(-0.86602540378443864676372317075294);
+  cplx<T> a = v[0], b = v[1], d = v[2];
+  cplx<T> sbd = cadd(b, d), dbd = csub(b, d);
+  cplx<T> t = cmk(a.x + c * sbd.x, a.y + c * sbd.y);
+  // i*s*(b-d)
+  cplx<T> u = cmk(-s * dbd.y, s * dbd.x);
+  v[0] = cadd(a, sbd);
+  v[1] = cadd(t, u);
+  v[2] = csub(t, u);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft5(cplx<T>* v, bool inv) {
+  // w = exp(-+2 pi i/5); c1 = cos(2pi/5), c2 = cos(4pi/5), s1 = sin(2pi/5), s2 = sin(4pi/5)
+  const T c1 = T(0.30901699437494742410229341718282);
+  const T c2 = T(-0.80901699437494742410229341718282);
+  const T s1 = inv ? T(0.95105651629515357211643933337938) : T(-0.95105651629515357211643933337938);
+  const T s2 = inv ? T(0.58778525229247312916870595463907) : T(-0.58778525229247312916870595463907);
+  cplx<T> x0 = v[0];
+  cplx<T> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+  cplx<T> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+  cplx<T> t1 = cmk(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+  cplx<T> t2 = cmk(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+  // i*(s1*b1 + s2*b2), i*(s2*b1 - s1*b2)
+  cplx<T> u1 = cmk(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y);
+  cplx<T> u2 = cmk(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y);
+  u1 = mul_pi(u1);
+  u2 = mul_pi(u2);
+  v[0] = cadd(x0, cadd(a1, a2));
+  v[1] = cadd(t1, u1);
+  v[4] = csub(t1, u1);
+  v[2] = cadd(t2, u2);
+  v[3] = csub(t2, u2);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft8(cplx<T>* v, bool inv) {
+  const T r = T(0.70710678118654752440084436210485);
+  // radix-2 on (q, q+4)
+  cplx<T> a[4], b[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    a[q] = cadd(v[q], v[q + 4]);
+    b[q] = csub(v[q], v[q + 4]);
+  }
+  // twiddle b[q] by w8^q
+  if (!inv) {
+    b[1] = cmk(r * (b[1].x + b[1].y), r * (b[1].y - b[1].x));  // * (1-i)/sqrt2
+    b[2] = mul_mi(b[2]);
+    b[3] = cmk(r * (b[3].y - b[3].x), -r * (b[3].x + b[3].y));  // * (-1-i)/sqrt2
+  } else {
+    b[1] = cmk(r * (b[1].x - b[1].y), r * (b[1].y + b[1].x));  // * (1+i)/sqrt2
+    b[2] = mul_pi(b[2]);
+    b[3] = cmk(-r * (b[3].x + b[3].y), r * (b[3].x - b[3].y));  // * (-1+i)/sqrt2
+  }
+  dft4(a, inv);
+  dft4(b, inv);
+  v[0] = a[0];
+  v[2] = a[1];
+  v[4] = a[2];
+  v[6] = a[3];
+  v[1] = b[0];
+  v[3] = b[1];
+  v[5] = b[2];
+  v[7] = b[3];
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void dftR(cplx<T>* v, bool inv) {
+  if constexpr (R == 2) dft2(v);
+  else if constexpr (R == 3) dft3(v, inv);
+  else if constexpr (R == 4) dft4(v, inv);
+  else if constexpr (R == 5) dft5(v, inv);
+  else dft8(v, inv);
+}
+
+// ---------------------------------------------------------------------------
+// LDS tile descriptor and Stockham passes.
+//   element (c, j, b) lives at buf[c*cstride + j*ldb + b]
+//   j in [0,n) is the transform axis, b in [0,B) the column (B = 2^logB),
+//   c in [0,C) a batch of independent tiles.
+// tw: n twiddles exp(-2 pi i t/n) in LDS.
+// Every thread of the workgroup must call these (they contain barriers).
+// ---------------------------------------------------------------------------
+struct Tile {
+  int n, logB, ldb, C, cstride;
+};
+
+template <int R, int MAXV, typename T>
+__device__ __forceinline__ void stockham_pass(cplx<T>* __restrict__ buf, const Tile& tl, int L,
+                                              const cplx<T>* __restrict__ tw, bool inv) {
+  constexpr int S = (MAXV + R - 1) / R;
+  const int n = tl.n;
+  const int m = n / R;
+  const int B = 1 << tl.logB;
+  const int nb = tl.C * m * B;
+  const int tstride = n / (L * R);
+  cplx<T> v[S][R];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int t = threadIdx.x + s * MGK_NT;
+    if (t < nb) {
+      const int b = t & (B - 1);
+      const int u = t >> tl.logB;
+      const int j = u % m;
+      const int c = u / m;
+      cplx<T>* base = buf + c * tl.cstride + b;
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[s][q] = base[(j + q * m) * tl.ldb];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int t = threadIdx.x + s * MGK_NT;
+    if (t < nb) {
+      const int b = t & (B - 1);
+      const int u = t >> tl.logB;
+      const int j = u % m;
+      const int c = u / m;
+      const int k = j % L;
+      if (L > 1) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          cplx<T> w = tw[q * k * tstride];
+          if (inv) w.y = -w.y;
+          v[s][q] = cmul(v[s][q], w);
+        }
+      }
+      dftR<R>(v[s], inv);
+      cplx<T>* base = buf + c * tl.cstride + b;
+      const int o = (j - k) * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) base[(o + q * L) * tl.ldb] = v[s][q];
+    }
+  }
+  __syncthreads();
+}
+
+// Full in-LDS FFT of every column of the tile (forward: exp(-), inverse: exp(+),
+// both unnormalised).  MAXV bounds (C*n*B)/MGK_NT, the values a thread stages.
+template <int MAXV, typename T>
+__device__ void tile_fft(cplx<T>* __restrict__ buf, const Tile& tl, const FftPlan& pl,
+                         const cplx<T>* __restrict__ tw, bool inv) {
+  int L = 1;
+  for (int p = 0; p < pl.npass; ++p) {
+    const int r = pl.radix[p];
+    switch (r) {
+      case 8: stockham_pass<8, MAXV>(buf, tl, L, tw, inv); break;
+      case 4: stockham_pass<4, MAXV>(buf, tl, L, tw, inv); break;
+      case 2: stockham_pass<2, MAXV>(buf, tl, L, tw, inv); break;
+      case 3: stockham_pass<3, MAXV>(buf, tl, L, tw, inv); break;
+      default: stockham_pass<5, MAXV>(buf, tl, L, tw, inv); break;
+    }
+    L *= r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block reductions in FP64 (wavefront = 64 lanes on CDNA).
+// Deterministic: fixed shuffle tree, then fixed order over waves.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sums NV values across the block; result valid in thread 0.  red: LDS of
+// NV * (MGK_NT/64) doubles.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[i * (MGK_NT / 64) + w] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < MGK_NT / 64; ++k) s += red[i * (MGK_NT / 64) + k];
+      v[i] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Host-side helpers ---------------------------------------------------------
+#define MGK_OK 0
+#define MGK_NOT_CONVERGED 1
+#define MGK_ERR_ARG -1
+#define MGK_ERR_HIP -2
+#define MGK_ERR_WORKSPACE -3
+#define MGK_ERR_NONFINITE -4
+#define MGK_ERR_SOLVER -5

@@ -1,0 +1,146 @@
+// Host-side helpers: error string, FFT plans, twiddle cache, pinned scratch.
+#include "mgk_host.h"
+#include <string.h>
+#include <stdio.h>
+#include <string>
+
+static thread_local std::string g_err;
+
+void mgk_set_error(const char* msg, const char* file, int line) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), "%s (%s:%d)", msg, file, line);
+  g_err = buf;
+}
+
+extern "C" const char* mgk_last_error(void) { return g_err.c_str(); }
+
+bool mgk_make_plan(int n, FftPlan* p) {
+  if (n < 1) return false;
+  p->n = n;
+  p->npass = 0;
+  int m = n;
+  int twos = 0;
+  while (m % 2 == 0) {
+    m /= 2;
+    ++twos;
+  }
+  std::vector<int> r;
+  while (twos >= 3) {
+    r.push_back(8);
+    twos -= 3;
+  }
+  if (twos == 2) r.push_back(4);
+  if (twos == 1) {
+    // prefer 8*4*4 style splits over a trailing radix-2 when possible
+    if (!r.empty() && r.back() == 8) {
+      r.back() = 4;
+      r.push_back(4);
+    } else {
+      r.push_back(2);
+    }
+  }
+  while (m % 3 == 0) {
+    r.push_back(3);
+    m /= 3;
+  }
+  while (m % 5 == 0) {
+    r.push_back(5);
+    m /= 5;
+  }
+  if (m != 1 || r.size() > 12) return false;
+  for (size_t i = 0; i < r.size(); ++i) p->radix[i] = r[i];
+  p->npass = (int)r.size();
+  return true;
+}
+
+namespace {
+struct TwKey {
+  int dev, n, prec;
+  bool operator<(const TwKey& o) const {
+    if (dev != o.dev) return dev < o.dev;
+    if (n != o.n) return n < o.n;
+    return prec < o.prec;
+  }
+};
+std::mutex g_tw_mu;
+std::map<TwKey, void*> g_tw;
+std::map<int, size_t> g_lds;
+}  // namespace
+
+const void* mgk_twiddles(int n, int prec) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  TwKey k{dev, n, prec};
+  auto it = g_tw.find(k);
+  if (it != g_tw.end()) return it->second;
+  // exp(-2 pi i t/n) from long double angles, rounded once to the target type
+  const long double pi = 3.141592653589793238462643383279502884L;
+  void* d = nullptr;
+  if (prec == 0) {
+    std::vector<double> h(2 * n);
+    for (int t = 0; t < n; ++t) {
+      long double a = 2.0L * pi * (long double)t / (long double)n;
+      h[2 * t] = (double)cosl(a);
+      h[2 * t + 1] = (double)(-sinl(a));
+    }
+    if (hipMalloc(&d, h.size() * sizeof(double)) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  } else {
+    std::vector<float> h(2 * n);
+    for (int t = 0; t < n; ++t) {
+      long double a = 2.0L * pi * (long double)t / (long double)n;
+      h[2 * t] = (float)cosl(a);
+      h[2 * t + 1] = (float)(-sinl(a));
+    }
+    if (hipMalloc(&d, h.size() * sizeof(float)) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  }
+  g_tw[k] = d;
+  return d;
+}
+
+size_t mgk_lds_limit() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 65536;
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto it = g_lds.find(dev);
+  if (it != g_lds.end()) return it->second;
+  int v = 65536;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
+  g_lds[dev] = (size_t)v;
+  return (size_t)v;
+}
+
+namespace {
+struct Pinned {
+  void* p = nullptr;
+  size_t n = 0;
+  ~Pinned() {
+    // Left to process teardown: freeing pinned memory after the runtime has
+    // shut down is unsafe, and the buffer is small.
+  }
+};
+thread_local Pinned g_pin;
+}  // namespace
+
+void* mgk_pinned(size_t bytes) {
+  if (g_pin.n >= bytes) return g_pin.p;
+  if (g_pin.p) (void)hipHostFree(g_pin.p);
+  g_pin.p = nullptr;
+  g_pin.n = 0;
+  size_t want = bytes < 4096 ? 4096 : bytes;
+  if (hipHostMalloc(&g_pin.p, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+  g_pin.n = want;
+  return g_pin.p;
+}
+
+extern "C" int mgk_version(void) { return 1; }
+
+extern "C" int mgk_device_info(int* lds_per_block, int* cu_count) {
+  int dev = 0;
+  MGK_HIP(hipGetDevice(&dev));
+  MGK_HIP(hipDeviceGetAttribute(lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  MGK_HIP(hipDeviceGetAttribute(cu_count, hipDeviceAttributeMultiprocessorCount, dev));
+  return MGK_OK;
+}

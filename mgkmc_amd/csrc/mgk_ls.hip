@@ -1,0 +1,849 @@
+// Small-strain Lippmann-Schwinger fixed point on gfx950.
+//
+// One LS iteration (reference: mgkmc/linear_elastic_simulator.py:307-364 for
+// 3D, :43-81 for 2D) is five sweeps over HBM:
+//   1. k_ls_tau_xfwd   : tau = C(x):(eps-eps_p) - C0:eps per voxel, then a
+//                        real-to-complex FFT along x (two components packed
+//                        as one complex sequence), half spectrum out.
+//   2. k_fft_mid (fwd) : complex FFT along y (3D only), strided tiles.
+//   3. k_ls_pencil     : complex FFT along the contiguous axis for all
+//                        components of a pencil, eps_hat = -Gamma0(k):tau_hat
+//                        evaluated from k in registers (no materialised
+//                        Gamma), inverse FFT along the same axis.
+//   4. k_fft_mid (inv) : inverse along y (3D only).
+//   5. k_ls_xinv_update: complex-to-real along x, eps_new = eps_bar + eps~,
+//                        FP64 partial sums of |eps_new-eps|^2 and |eps|^2.
+//   6. k_ls_finalize   : one workgroup, FP64 diff and the stop decision,
+//                        kept in device memory so that later launches of the
+//                        same solve exit at once (no host round trip per
+//                        iteration).
+#include "mgk_common.h"
+#include "mgk_host.h"
+#include <algorithm>
+
+struct LsState {
+  int done;    // set once the solve has stopped
+  int iters;   // iterations executed
+  int status;  // MGK_OK (converged), MGK_NOT_CONVERGED (max_iter), MGK_ERR_NONFINITE
+  int pad;
+  double diff;
+  double d2, e2;
+  double mean[12];  // eps means [0..C), sig means [C..2C) (after the stress pass)
+};
+
+template <typename T>
+struct LsGeom {
+  int nx, P, Q, nxh, C;
+  long long NV;
+  int logBx;
+  FftPlan plx;
+  const cplx<T>* twx;
+  int logBy;
+  FftPlan ply;
+  const cplx<T>* twy;
+  int logPP;
+  FftPlan plq;
+  const cplx<T>* twq;
+  int npencil;
+};
+
+struct Sym6 {
+  double v[6];
+};
+
+// ---------------------------------------------------------------------------
+// frequency helpers
+// ---------------------------------------------------------------------------
+// signed frequency index of FFT bin i on an axis of length n (numpy fftfreq order)
+__device__ __forceinline__ int fbin(int i, int n) { return (i <= (n - 1) / 2) ? i : i - n; }
+__device__ __forceinline__ bool fnyq(int i, int n) { return (n % 2 == 0) && (i == n / 2); }
+
+// eps_hat_kh = -[ (1/(2 mu0 |q|^2)) (q_h (tau q)_k + q_k (tau q)_h) - B q_k q_h (q.tau.q)/|q|^4 ]
+// (mgkmc/elasticity.py:70-105 contracted with a symmetric tau).  3D components
+// xx,yy,zz,yz,xz,xy.  Works on real and imaginary parts independently.
+template <typename T>
+__device__ __forceinline__ void gamma3_apply(const T q0, const T q1, const T q2, const cplx<T>* t, cplx<T>* o,
+                                             T A2, T Bc, T scale) {
+  const T k2 = q0 * q0 + q1 * q1 + q2 * q2;
+  const T ik2 = T(1) / k2;
+  const T a = A2 * ik2;
+  const T bb = Bc * ik2 * ik2;
+#pragma unroll
+  for (int ri = 0; ri < 2; ++ri) {
+    const T xx = ri ? t[0].y : t[0].x, yy = ri ? t[1].y : t[1].x, zz = ri ? t[2].y : t[2].x;
+    const T yz = ri ? t[3].y : t[3].x, xz = ri ? t[4].y : t[4].x, xy = ri ? t[5].y : t[5].x;
+    const T u0 = xx * q0 + xy * q1 + xz * q2;
+    const T u1 = xy * q0 + yy * q1 + yz * q2;
+    const T u2 = xz * q0 + yz * q1 + zz * q2;
+    const T qtq = (q0 * u0 + q1 * u1 + q2 * u2) * bb;
+    T r[6];
+    r[0] = -(a * (2 * q0 * u0) - q0 * q0 * qtq);
+    r[1] = -(a * (2 * q1 * u1) - q1 * q1 * qtq);
+    r[2] = -(a * (2 * q2 * u2) - q2 * q2 * qtq);
+    r[3] = -(a * (q2 * u1 + q1 * u2) - q1 * q2 * qtq);
+    r[4] = -(a * (q2 * u0 + q0 * u2) - q0 * q2 * qtq);
+    r[5] = -(a * (q1 * u0 + q0 * u1) - q0 * q1 * qtq);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      if (ri) o[c].y = r[c] * scale;
+      else o[c].x = r[c] * scale;
+    }
+  }
+}
+
+// 2D: components xx, yy, xy (mgkmc/elasticity.py:20-53)
+template <typename T>
+__device__ __forceinline__ void gamma2_apply(const T q0, const T q1, const cplx<T>* t, cplx<T>* o, T A2, T Bc,
+                                             T scale) {
+  const T k2 = q0 * q0 + q1 * q1;
+  const T ik2 = T(1) / k2;
+  const T a = A2 * ik2;
+  const T bb = Bc * ik2 * ik2;
+#pragma unroll
+  for (int ri = 0; ri < 2; ++ri) {
+    const T xx = ri ? t[0].y : t[0].x, yy = ri ? t[1].y : t[1].x, xy = ri ? t[2].y : t[2].x;
+    const T u0 = xx * q0 + xy * q1;
+    const T u1 = xy * q0 + yy * q1;
+    const T qtq = (q0 * u0 + q1 * u1) * bb;
+    T r[3];
+    r[0] = -(a * (2 * q0 * u0) - q0 * q0 * qtq);
+    r[1] = -(a * (2 * q1 * u1) - q1 * q1 * qtq);
+    r[2] = -(a * (q1 * u0 + q0 * u1) - q0 * q1 * qtq);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (ri) o[c].y = r[c] * scale;
+      else o[c].x = r[c] * scale;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x pass helpers
+// ---------------------------------------------------------------------------
+// After a forward FFT of z = a + i b along x (tile rows), split into the half
+// spectra of a and b and store them.
+template <typename T>
+__device__ __forceinline__ void xsplit_store(const cplx<T>* tile, int ldb, int nx, int nxh, int logB, int q0, int Q,
+                                             int P, int p, cplx<T>* __restrict__ sa, cplx<T>* __restrict__ sb) {
+  const int B = 1 << logB;
+  for (int e = threadIdx.x; e < nxh * B; e += MGK_NT) {
+    const int b = e & (B - 1), k = e >> logB, q = q0 + b;
+    if (q >= Q) continue;
+    const cplx<T> z1 = tile[k * ldb + b];
+    const int kr = (k == 0) ? 0 : nx - k;
+    const cplx<T> z2 = tile[kr * ldb + b];
+    const size_t idx = ((size_t)k * P + p) * Q + q;
+    sa[idx] = cmk(T(0.5) * (z1.x + z2.x), T(0.5) * (z1.y - z2.y));
+    if (sb) sb[idx] = cmk(T(0.5) * (z1.y + z2.y), T(-0.5) * (z1.x - z2.x));
+  }
+}
+
+// Build the full x-spectrum of z = a + i b from half spectra of a and b.
+template <typename T>
+__device__ __forceinline__ void xmerge_load(cplx<T>* tile, int ldb, int nx, int nxh, int logB, int q0, int Q, int P,
+                                            int p, const cplx<T>* __restrict__ sa, const cplx<T>* __restrict__ sb) {
+  const int B = 1 << logB;
+  for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
+    const int b = e & (B - 1), k = e >> logB, q = q0 + b;
+    cplx<T> z = cmk(T(0), T(0));
+    if (q < Q) {
+      if (k < nxh) {
+        const size_t idx = ((size_t)k * P + p) * Q + q;
+        const cplx<T> A = sa[idx];
+        const cplx<T> Bv = sb ? sb[idx] : cmk(T(0), T(0));
+        z = cmk(A.x - Bv.y, A.y + Bv.x);
+      } else {
+        const size_t idx = ((size_t)(nx - k) * P + p) * Q + q;
+        const cplx<T> A = sa[idx];
+        const cplx<T> Bv = sb ? sb[idx] : cmk(T(0), T(0));
+        z = cmk(A.x + Bv.y, -A.y + Bv.x);
+      }
+    }
+    tile[k * ldb + b] = z;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// sweep 1: polarisation + forward x FFT
+// ---------------------------------------------------------------------------
+template <typename T, int D, int MAXV>
+__global__ __launch_bounds__(MGK_NT) void k_ls_tau_xfwd(LsGeom<T> g, const T* __restrict__ eps,
+                                                        const T* __restrict__ epsp, const T* __restrict__ lam,
+                                                        const T* __restrict__ mu, T lam0, T mu0,
+                                                        cplx<T>* __restrict__ spec, const LsState* __restrict__ st) {
+  if (st->done) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* tile = tw + g.nx;
+  const int nx = g.nx, P = g.P, Q = g.Q, logB = g.logBx, B = 1 << logB, ldb = B;
+  const int q0 = blockIdx.x * B, p = blockIdx.y;
+  const size_t NV = (size_t)g.NV;
+  const size_t cs = (size_t)g.nxh * P * Q;  // spectrum component stride
+  for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
+  const Tile tl{nx, logB, ldb, 1, 0};
+  const T m02 = T(2) * mu0;
+
+  T keep[MAXV], mus[MAXV];
+  // pass 1: diagonal components, traces; (xx, yy) into the tile
+#pragma unroll
+  for (int s = 0; s < MAXV; ++s) {
+    keep[s] = T(0);
+    mus[s] = T(0);
+    const int e = threadIdx.x + s * MGK_NT;
+    if (e < nx * B) {
+      const int b = e & (B - 1), x = e >> logB, q = q0 + b;
+      T txx = T(0), tyy = T(0);
+      if (q < Q) {
+        const size_t v = ((size_t)x * P + p) * Q + q;
+        const T exx = eps[v], eyy = eps[NV + v];
+        const T pxx = epsp ? epsp[v] : T(0), pyy = epsp ? epsp[NV + v] : T(0);
+        const T l = lam[v], m = mu[v];
+        const T dxx = exx - pxx, dyy = eyy - pyy;
+        T tre, tr0, ezz = T(0), dzz = T(0);
+        if (D == 3) {
+          ezz = eps[2 * NV + v];
+          const T pzz = epsp ? epsp[2 * NV + v] : T(0);
+          dzz = ezz - pzz;
+          tre = (dxx + dyy) + dzz;
+          tr0 = (exx + eyy) + ezz;
+        } else {
+          tre = dxx + dyy;
+          tr0 = exx + eyy;
+        }
+        const T m2 = T(2) * m;
+        const T ltr = l * tre, l0tr = lam0 * tr0;
+        txx = (m2 * dxx + ltr) - (m02 * exx + l0tr);
+        tyy = (m2 * dyy + ltr) - (m02 * eyy + l0tr);
+        if (D == 3) keep[s] = (m2 * dzz + ltr) - (m02 * ezz + l0tr);
+        mus[s] = m;
+      }
+      tile[x * ldb + b] = cmk(txx, tyy);
+    }
+  }
+  __syncthreads();
+  tile_fft<MAXV>(tile, tl, g.plx, tw, false);
+  xsplit_store(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec, spec + cs);
+  __syncthreads();
+
+  // remaining pairs: 3D (zz, yz), (xz, xy); 2D (xy, -)
+  constexpr int NPAIR = (D == 3) ? 2 : 1;
+#pragma unroll
+  for (int pr = 0; pr < NPAIR; ++pr) {
+    const int ca = (D == 3) ? (pr == 0 ? 2 : 4) : 2;
+    const int cb = (D == 3) ? (pr == 0 ? 3 : 5) : -1;
+#pragma unroll
+    for (int s = 0; s < MAXV; ++s) {
+      const int e = threadIdx.x + s * MGK_NT;
+      if (e < nx * B) {
+        const int b = e & (B - 1), x = e >> logB, q = q0 + b;
+        T ta = T(0), tb = T(0);
+        if (q < Q) {
+          const size_t v = ((size_t)x * P + p) * Q + q;
+          const T m2 = T(2) * mus[s];
+          if (D == 3 && pr == 0) {
+            ta = keep[s];
+          } else {
+            const T ea = eps[(size_t)ca * NV + v];
+            const T pa = epsp ? epsp[(size_t)ca * NV + v] : T(0);
+            ta = m2 * (ea - pa) - m02 * ea;
+          }
+          if (cb >= 0) {
+            const T eb = eps[(size_t)cb * NV + v];
+            const T pb = epsp ? epsp[(size_t)cb * NV + v] : T(0);
+            tb = m2 * (eb - pb) - m02 * eb;
+          }
+        }
+        tile[x * ldb + b] = cmk(ta, tb);
+      }
+    }
+    __syncthreads();
+    tile_fft<MAXV>(tile, tl, g.plx, tw, false);
+    xsplit_store(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
+                 cb >= 0 ? spec + (size_t)cb * cs : (cplx<T>*)nullptr);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sweep 2/4: complex FFT along the middle axis (length P, stride Q)
+// grid (ceil(Q/B), nxh, C)
+// ---------------------------------------------------------------------------
+template <typename T, int MAXV>
+__global__ __launch_bounds__(MGK_NT) void k_fft_mid(int P, int Q, int nxh, int logB, FftPlan pl,
+                                                    const cplx<T>* __restrict__ twg, cplx<T>* __restrict__ spec,
+                                                    int inv, const int* __restrict__ done) {
+  if (done && *done) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* tile = tw + P;
+  const int B = 1 << logB, ldb = B;
+  const int q0 = blockIdx.x * B, kx = blockIdx.y, c = blockIdx.z;
+  cplx<T>* base = spec + ((size_t)c * nxh + kx) * (size_t)P * Q;
+  for (int t = threadIdx.x; t < P; t += MGK_NT) tw[t] = twg[t];
+  for (int e = threadIdx.x; e < P * B; e += MGK_NT) {
+    const int b = e & (B - 1), y = e >> logB, q = q0 + b;
+    tile[y * ldb + b] = (q < Q) ? base[(size_t)y * Q + q] : cmk(T(0), T(0));
+  }
+  __syncthreads();
+  tile_fft<MAXV>(tile, Tile{P, logB, ldb, 1, 0}, pl, tw, inv != 0);
+  for (int e = threadIdx.x; e < P * B; e += MGK_NT) {
+    const int b = e & (B - 1), y = e >> logB, q = q0 + b;
+    if (q < Q) base[(size_t)y * Q + q] = tile[y * ldb + b];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sweep 3: pencil FFT along the contiguous axis + Green operator + inverse
+// grid ceil(npencil/PP); pencil id = kx*P + p
+// ---------------------------------------------------------------------------
+template <typename T, int D, int MAXV>
+__global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __restrict__ spec, T A2, T Bc, T scale,
+                                                      const LsState* __restrict__ st) {
+  if (st->done) return;
+  constexpr int C = (D == 3) ? 6 : 3;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* tile = tw + g.Q;
+  const int Q = g.Q, P = g.P, logPP = g.logPP, PP = 1 << logPP;
+  const int ldb = PP, cstride = Q * PP;
+  const int pid0 = blockIdx.x * PP;
+  const int npen = g.npencil;
+  for (int t = threadIdx.x; t < Q; t += MGK_NT) tw[t] = g.twq[t];
+  const int chunk = PP * Q;
+  for (int e = threadIdx.x; e < C * chunk; e += MGK_NT) {
+    const int c = e / chunk;
+    const int r = e - c * chunk;
+    const int pp = r / Q;
+    const int q = r - pp * Q;
+    const int pid = pid0 + pp;
+    tile[c * cstride + q * ldb + pp] =
+        (pid < npen) ? spec[((size_t)c * npen + pid) * Q + q] : cmk(T(0), T(0));
+  }
+  __syncthreads();
+  const Tile tl{Q, logPP, ldb, C, cstride};
+  tile_fft<MAXV>(tile, tl, g.plq, tw, false);
+
+  for (int e = threadIdx.x; e < Q * PP; e += MGK_NT) {
+    const int pp = e & (PP - 1), q = e >> logPP;
+    const int pid = pid0 + pp;
+    if (pid >= npen) continue;
+    const int kx = pid / P, p = pid - kx * P;
+    cplx<T> t[C], o[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) t[c] = tile[c * cstride + q * ldb + pp];
+    const bool nyx = fnyq(kx, g.nx);
+    if (D == 3) {
+      const bool nyy = fnyq(p, P), nyz = fnyq(q, Q);
+      const T qx = T(fbin(kx, g.nx)) / T(g.nx), qy = T(fbin(p, P)) / T(P), qz = T(fbin(q, Q)) / T(Q);
+      if (kx == 0 && p == 0 && q == 0) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) o[c] = cmk(T(0), T(0));
+      } else if (!(nyx || nyy || nyz)) {
+        gamma3_apply(qx, qy, qz, t, o, A2, Bc, scale);
+      } else {
+        // Hermitian part of the reference's full-spectrum operator: average
+        // Gamma over q and q with its Nyquist components negated.
+        cplx<T> o2[C];
+        gamma3_apply(qx, qy, qz, t, o, A2, Bc, scale);
+        gamma3_apply(nyx ? -qx : qx, nyy ? -qy : qy, nyz ? -qz : qz, t, o2, A2, Bc, scale);
+#pragma unroll
+        for (int c = 0; c < C; ++c) o[c] = cmk(T(0.5) * (o[c].x + o2[c].x), T(0.5) * (o[c].y + o2[c].y));
+      }
+    } else {
+      const bool nyy = fnyq(q, Q);
+      const T qx = T(fbin(kx, g.nx)) / T(g.nx), qy = T(fbin(q, Q)) / T(Q);
+      if (kx == 0 && q == 0) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) o[c] = cmk(T(0), T(0));
+      } else if (!(nyx || nyy)) {
+        gamma2_apply(qx, qy, t, o, A2, Bc, scale);
+      } else {
+        cplx<T> o2[C];
+        gamma2_apply(qx, qy, t, o, A2, Bc, scale);
+        gamma2_apply(nyx ? -qx : qx, nyy ? -qy : qy, t, o2, A2, Bc, scale);
+#pragma unroll
+        for (int c = 0; c < C; ++c) o[c] = cmk(T(0.5) * (o[c].x + o2[c].x), T(0.5) * (o[c].y + o2[c].y));
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) tile[c * cstride + q * ldb + pp] = o[c];
+  }
+  __syncthreads();
+  tile_fft<MAXV>(tile, tl, g.plq, tw, true);
+  for (int e = threadIdx.x; e < C * chunk; e += MGK_NT) {
+    const int c = e / chunk;
+    const int r = e - c * chunk;
+    const int pp = r / Q;
+    const int q = r - pp * Q;
+    const int pid = pid0 + pp;
+    if (pid < npen) spec[((size_t)c * npen + pid) * Q + q] = tile[c * cstride + q * ldb + pp];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sweep 5: inverse x FFT (complex-to-real) + update + FP64 partial norms
+// ---------------------------------------------------------------------------
+template <typename T, int D, int MAXV>
+__global__ __launch_bounds__(MGK_NT) void k_ls_xinv_update(LsGeom<T> g, const cplx<T>* __restrict__ spec,
+                                                           T* __restrict__ eps, Sym6 ebar,
+                                                           double* __restrict__ partials,
+                                                           const LsState* __restrict__ st) {
+  if (st->done) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* tile = tw + g.nx;
+  const int nx = g.nx, P = g.P, Q = g.Q, logB = g.logBx, B = 1 << logB, ldb = B;
+  const int q0 = blockIdx.x * B, p = blockIdx.y;
+  const size_t NV = (size_t)g.NV;
+  const size_t cs = (size_t)g.nxh * P * Q;
+  for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
+  const Tile tl{nx, logB, ldb, 1, 0};
+  double acc[2] = {0.0, 0.0};
+  constexpr int NPAIR = (D == 3) ? 3 : 2;
+#pragma unroll
+  for (int pr = 0; pr < NPAIR; ++pr) {
+    const int ca = 2 * pr;
+    const int cb = (D == 3 || pr == 0) ? 2 * pr + 1 : -1;
+    xmerge_load(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
+                cb >= 0 ? spec + (size_t)cb * cs : (const cplx<T>*)nullptr);
+    tile_fft<MAXV>(tile, tl, g.plx, tw, true);
+    const double wa = (D == 3) ? (ca >= 3 ? 2.0 : 1.0) : (ca >= 2 ? 2.0 : 1.0);
+    const double wb = (D == 3) ? (cb >= 3 ? 2.0 : 1.0) : (cb >= 2 ? 2.0 : 1.0);
+    const T eba = T(ebar.v[ca]);
+    const T ebb = cb >= 0 ? T(ebar.v[cb]) : T(0);
+#pragma unroll
+    for (int s = 0; s < MAXV; ++s) {
+      const int e = threadIdx.x + s * MGK_NT;
+      if (e < nx * B) {
+        const int b = e & (B - 1), x = e >> logB, q = q0 + b;
+        if (q < Q) {
+          const size_t v = ((size_t)x * P + p) * Q + q;
+          const cplx<T> z = tile[x * ldb + b];
+          {
+            const T nw = eba + z.x;
+            const T od = eps[(size_t)ca * NV + v];
+            const double dd = (double)nw - (double)od;
+            acc[0] += wa * dd * dd;
+            acc[1] += wa * (double)od * (double)od;
+            eps[(size_t)ca * NV + v] = nw;
+          }
+          if (cb >= 0) {
+            const T nw = ebb + z.y;
+            const T od = eps[(size_t)cb * NV + v];
+            const double dd = (double)nw - (double)od;
+            acc[0] += wb * dd * dd;
+            acc[1] += wb * (double)od * (double)od;
+            eps[(size_t)cb * NV + v] = nw;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double* red = reinterpret_cast<double*>(tile);
+  block_sum<2>(acc, red);
+  if (threadIdx.x == 0) {
+    const size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+    partials[2 * bid] = acc[0];
+    partials[2 * bid + 1] = acc[1];
+  }
+}
+
+__global__ __launch_bounds__(MGK_NT) void k_ls_finalize(const double* __restrict__ partials, int nparts,
+                                                        LsState* __restrict__ st, double tol, int max_iter,
+                                                        double e2_extra) {
+  if (st->done) return;
+  __shared__ double red[2 * (MGK_NT / 64)];
+  double acc[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < nparts; i += MGK_NT) {
+    acc[0] += partials[2 * i];
+    acc[1] += partials[2 * i + 1];
+  }
+  block_sum<2>(acc, red);
+  if (threadIdx.x == 0) {
+    const double diff = sqrt(acc[0]) / (sqrt(acc[1] + e2_extra) + 1e-20);
+    st->d2 = acc[0];
+    st->e2 = acc[1];
+    st->diff = diff;
+    st->iters += 1;
+    if (!isfinite(diff)) {
+      st->done = 1;
+      st->status = MGK_ERR_NONFINITE;
+    } else if (diff < tol) {
+      st->done = 1;
+      st->status = MGK_OK;
+    } else if (st->iters >= max_iter) {
+      st->done = 1;
+      st->status = MGK_NOT_CONVERGED;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// init / warm start / stress / means
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_ls_fill(T* __restrict__ eps, int C, size_t NV, Sym6 ebar) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < NV; i += (size_t)gridDim.x * blockDim.x) {
+    for (int c = 0; c < C; ++c) eps[(size_t)c * NV + i] = T(ebar.v[c]);
+  }
+}
+
+// per-block FP64 sums of C components (fixed grid => deterministic)
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_comp_sums(const T* __restrict__ f, int C, size_t NV,
+                                                      double* __restrict__ partials) {
+  __shared__ double red[6 * (MGK_NT / 64)];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < NV; i += (size_t)gridDim.x * MGK_NT) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      if (c < C) acc[c] += (double)f[(size_t)c * NV + i];
+  }
+  block_sum<6>(acc, red);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < C; ++c) partials[(size_t)blockIdx.x * C + c] = acc[c];
+}
+
+// means[off + c] = sum(partials[:, c]) / NV
+__global__ __launch_bounds__(MGK_NT) void k_means_finalize(const double* __restrict__ partials, int nparts, int C,
+                                                           double NV, double* __restrict__ out) {
+  __shared__ double red[12 * (MGK_NT / 64)];
+  double acc[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) acc[c] = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += MGK_NT) {
+#pragma unroll
+    for (int c = 0; c < 12; ++c)
+      if (c < C) acc[c] += partials[(size_t)i * C + c];
+  }
+  block_sum<12>(acc, red);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < C; ++c) out[c] = acc[c] / NV;
+}
+
+template <typename T>
+__global__ void k_ls_warm_adjust(T* __restrict__ eps, int C, size_t NV, Sym6 ebar,
+                                 const double* __restrict__ means) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < NV; i += (size_t)gridDim.x * blockDim.x) {
+    for (int c = 0; c < C; ++c) {
+      const T add = T(ebar.v[c] - means[c]);
+      eps[(size_t)c * NV + i] += add;
+    }
+  }
+}
+
+// sig = 2 mu (eps-eps_p) + lam tr(eps-eps_p) I  (elasticity.py:12-18, 62-67),
+// plus per-block FP64 sums of eps and sig components.
+template <typename T, int D>
+__global__ __launch_bounds__(MGK_NT) void k_ls_stress(const T* __restrict__ eps, const T* __restrict__ epsp,
+                                                      const T* __restrict__ lam, const T* __restrict__ mu,
+                                                      T* __restrict__ sig, size_t NV, double* __restrict__ partials) {
+  constexpr int C = (D == 3) ? 6 : 3;
+  __shared__ double red[2 * C * (MGK_NT / 64)];
+  double acc[2 * C];
+#pragma unroll
+  for (int c = 0; c < 2 * C; ++c) acc[c] = 0.0;
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < NV; i += (size_t)gridDim.x * MGK_NT) {
+    T e[C], d[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      e[c] = eps[(size_t)c * NV + i];
+      d[c] = e[c] - (epsp ? epsp[(size_t)c * NV + i] : T(0));
+    }
+    const T tr = (D == 3) ? ((d[0] + d[1]) + d[2]) : (d[0] + d[1]);
+    const T m2 = T(2) * mu[i], ltr = lam[i] * tr;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const bool diag = c < D;
+      const T s = diag ? (m2 * d[c] + ltr) : (m2 * d[c]);
+      if (sig) sig[(size_t)c * NV + i] = s;
+      acc[c] += (double)e[c];
+      acc[C + c] += (double)s;
+    }
+  }
+  block_sum<2 * C>(acc, red);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 2 * C; ++c) partials[(size_t)blockIdx.x * 2 * C + c] = acc[c];
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+extern "C" {
+typedef struct {
+  int iters;
+  int status;
+  double diff;
+  double eps_mean[9];  // full d x d row-major (symmetric part + constant antisymmetric part)
+  double sig_mean[9];
+} mgk_ls_info;
+}
+
+namespace {
+
+template <typename T>
+constexpr int maxv() {
+  return sizeof(T) == 8 ? 8 : 16;
+}
+
+struct LsPlan {
+  int D, C, nx, P, Q, nxh;
+  long long NV;
+  int logBx, logBy, logPP;
+  size_t lds_x, lds_y, lds_q;
+  int nblk_upd, nblk_red;
+  size_t off_spec, off_part, off_state, total;
+};
+
+template <typename T>
+int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
+  if (ndim != 2 && ndim != 3) return MGK_ERR_ARG;
+  pl->D = ndim;
+  pl->C = ndim == 3 ? 6 : 3;
+  pl->nx = (int)shape[0];
+  pl->P = ndim == 3 ? (int)shape[1] : 1;
+  pl->Q = ndim == 3 ? (int)shape[2] : (int)shape[1];
+  if (pl->nx < 1 || pl->P < 1 || pl->Q < 1) return MGK_ERR_ARG;
+  pl->nxh = pl->nx / 2 + 1;
+  pl->NV = (long long)pl->nx * pl->P * pl->Q;
+  FftPlan t;
+  if (!mgk_make_plan(pl->nx, &t) || !mgk_make_plan(pl->P, &t) || !mgk_make_plan(pl->Q, &t)) return MGK_ERR_ARG;
+  const int TILE = maxv<T>() * MGK_NT;
+  const size_t lim = mgk_lds_limit();
+  const size_t cb = sizeof(cplx<T>);
+  // x pass: tile nx x B
+  if (pl->nx > TILE) return MGK_ERR_ARG;
+  int B = 1;
+  while (2 * B * pl->nx <= TILE && 2 * B <= mgk_pow2_ceil(pl->Q)) B *= 2;
+  while (B > 1 && (size_t)(pl->nx + pl->nx * B) * cb > lim) B /= 2;
+  pl->logBx = mgk_ilog2(B);
+  pl->lds_x = (size_t)(pl->nx + pl->nx * B) * cb;
+  if (pl->lds_x > lim) return MGK_ERR_ARG;
+  // mid pass
+  B = 1;
+  if (pl->P > TILE) return MGK_ERR_ARG;
+  while (2 * B * pl->P <= TILE && 2 * B <= mgk_pow2_ceil(pl->Q)) B *= 2;
+  while (B > 1 && (size_t)(pl->P + pl->P * B) * cb > lim) B /= 2;
+  pl->logBy = mgk_ilog2(B);
+  pl->lds_y = (size_t)(pl->P + pl->P * B) * cb;
+  // pencil pass: C x Q x PP
+  if (pl->C * pl->Q > TILE) return MGK_ERR_ARG;
+  int PP = 1;
+  const int npen = pl->nxh * pl->P;
+  while (2 * PP * pl->C * pl->Q <= TILE && 2 * PP <= mgk_pow2_ceil(npen)) PP *= 2;
+  while (PP > 1 && (size_t)(pl->Q + pl->C * pl->Q * PP) * cb > lim) PP /= 2;
+  pl->logPP = mgk_ilog2(PP);
+  pl->lds_q = (size_t)(pl->Q + pl->C * pl->Q * PP) * cb;
+  if (pl->lds_q > lim) return MGK_ERR_ARG;
+  const int nqt = (pl->Q + (1 << pl->logBx) - 1) >> pl->logBx;
+  pl->nblk_upd = nqt * pl->P;
+  long long nr = (pl->NV + MGK_NT - 1) / MGK_NT;
+  pl->nblk_red = (int)std::min<long long>(nr, 1024);
+  pl->off_spec = 0;
+  pl->off_part = mgk_align((size_t)pl->C * pl->nxh * pl->P * pl->Q * cb);
+  size_t npart = (size_t)std::max(pl->nblk_upd * 2, pl->nblk_red * 2 * pl->C);
+  pl->off_state = pl->off_part + mgk_align(npart * sizeof(double));
+  pl->total = pl->off_state + mgk_align(sizeof(LsState));
+  return MGK_OK;
+}
+
+template <typename T, int D>
+int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double mu0, const double* eps_bar,
+               const T* epsp, T* eps, T* sig, int warm, int max_iter, double tol, char* ws, hipStream_t stream,
+               mgk_ls_info* info) {
+  constexpr int MV = maxv<T>();
+  constexpr int C = (D == 3) ? 6 : 3;
+  const int prec = sizeof(T) == 8 ? 0 : 1;
+  LsGeom<T> g;
+  g.nx = pl.nx;
+  g.P = pl.P;
+  g.Q = pl.Q;
+  g.nxh = pl.nxh;
+  g.C = C;
+  g.NV = pl.NV;
+  g.logBx = pl.logBx;
+  g.logBy = pl.logBy;
+  g.logPP = pl.logPP;
+  g.npencil = pl.nxh * pl.P;
+  mgk_make_plan(pl.nx, &g.plx);
+  mgk_make_plan(pl.P, &g.ply);
+  mgk_make_plan(pl.Q, &g.plq);
+  g.twx = (const cplx<T>*)mgk_twiddles(pl.nx, prec);
+  g.twy = (const cplx<T>*)mgk_twiddles(pl.P, prec);
+  g.twq = (const cplx<T>*)mgk_twiddles(pl.Q, prec);
+  if (!g.twx || !g.twy || !g.twq) {
+    mgk_set_error("twiddle table allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
+  cplx<T>* spec = (cplx<T>*)(ws + pl.off_spec);
+  double* part = (double*)(ws + pl.off_part);
+  LsState* st = (LsState*)(ws + pl.off_state);
+
+  // symmetric part of eps_bar in component order, antisymmetric norm extra
+  Sym6 eb{};
+  double anti2 = 0.0;
+  if (D == 3) {
+    const double* e = eps_bar;
+    eb.v[0] = e[0];
+    eb.v[1] = e[4];
+    eb.v[2] = e[8];
+    eb.v[3] = 0.5 * (e[5] + e[7]);
+    eb.v[4] = 0.5 * (e[2] + e[6]);
+    eb.v[5] = 0.5 * (e[1] + e[3]);
+    const double a1 = 0.5 * (e[5] - e[7]), a2 = 0.5 * (e[2] - e[6]), a3 = 0.5 * (e[1] - e[3]);
+    anti2 = 2.0 * (a1 * a1 + a2 * a2 + a3 * a3);
+  } else {
+    const double* e = eps_bar;
+    eb.v[0] = e[0];
+    eb.v[1] = e[3];
+    eb.v[2] = 0.5 * (e[1] + e[2]);
+    const double a = 0.5 * (e[1] - e[2]);
+    anti2 = 2.0 * a * a;
+  }
+  const double e2_extra = anti2 * (double)pl.NV;
+  const size_t NV = (size_t)pl.NV;
+
+  MGK_HIP(hipMemsetAsync(st, 0, sizeof(LsState), stream));
+  const int nfill = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 4096);
+  if (!warm) {
+    k_ls_fill<T><<<nfill, MGK_NT, 0, stream>>>(eps, C, NV, eb);
+  } else {
+    k_comp_sums<T><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, C, NV, part);
+    k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, C, (double)NV, st->mean);
+    k_ls_warm_adjust<T><<<nfill, MGK_NT, 0, stream>>>(eps, C, NV, eb, st->mean);
+  }
+  MGK_HIP(hipGetLastError());
+
+  const T A2 = T(1.0 / (2.0 * mu0));
+  const T Bc = T((lam0 + mu0) / (mu0 * (lam0 + 2.0 * mu0)));
+  const T scale = T(1.0 / (double)pl.NV);
+  const int nqx = (pl.Q + (1 << pl.logBx) - 1) >> pl.logBx;
+  const int nqy = (pl.Q + (1 << pl.logBy) - 1) >> pl.logBy;
+  const int npb = (g.npencil + (1 << pl.logPP) - 1) >> pl.logPP;
+  const dim3 gx(nqx, pl.P), gy(nqy, pl.nxh, C);
+
+  auto enqueue_iter = [&]() {
+    k_ls_tau_xfwd<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st);
+    if (D == 3 && pl.P > 1)
+      k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, g.ply, g.twy, spec, 0,
+                                                          &st->done);
+    k_ls_pencil<T, D, MV><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st);
+    if (D == 3 && pl.P > 1)
+      k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, g.ply, g.twy, spec, 1,
+                                                          &st->done);
+    k_ls_xinv_update<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st);
+    k_ls_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_upd, st, tol, max_iter, e2_extra);
+  };
+
+  LsState* hst = (LsState*)mgk_pinned(2 * sizeof(LsState));
+  if (!hst) {
+    mgk_set_error("pinned host allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
+  hipEvent_t ev[2];
+  MGK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  MGK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int enq = 0, chunk = 2, slot = 0;
+  bool pending = false, stop = false;
+  while (!stop && enq < max_iter) {
+    const int k = std::min(chunk, max_iter - enq);
+    for (int i = 0; i < k; ++i) enqueue_iter();
+    enq += k;
+    MGK_HIP(hipGetLastError());
+    MGK_HIP(hipMemcpyAsync(&hst[slot], st, sizeof(LsState), hipMemcpyDeviceToHost, stream));
+    MGK_HIP(hipEventRecord(ev[slot], stream));
+    if (pending) {
+      MGK_HIP(hipEventSynchronize(ev[slot ^ 1]));
+      if (hst[slot ^ 1].done) stop = true;
+    }
+    pending = true;
+    slot ^= 1;
+    chunk = std::min(chunk * 2, 16);
+  }
+  // final stress and macro means
+  k_ls_stress<T, D><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, sig, NV, part);
+  k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, 2 * C, (double)NV, st->mean);
+  MGK_HIP(hipGetLastError());
+  MGK_HIP(hipMemcpyAsync(&hst[0], st, sizeof(LsState), hipMemcpyDeviceToHost, stream));
+  MGK_HIP(hipStreamSynchronize(stream));
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  const LsState& s = hst[0];
+  if (info) {
+    info->iters = s.iters;
+    info->status = s.status;
+    info->diff = s.diff;
+    const double* em = s.mean;
+    const double* sm = s.mean + C;
+    if (D == 3) {
+      const double a1 = 0.5 * (eps_bar[5] - eps_bar[7]), a2 = 0.5 * (eps_bar[2] - eps_bar[6]),
+                   a3 = 0.5 * (eps_bar[1] - eps_bar[3]);
+      const double E[9] = {em[0], em[5] + a3, em[4] + a2, em[5] - a3, em[1], em[3] + a1, em[4] - a2, em[3] - a1, em[2]};
+      const double S[9] = {sm[0], sm[5], sm[4], sm[5], sm[1], sm[3], sm[4], sm[3], sm[2]};
+      for (int i = 0; i < 9; ++i) {
+        info->eps_mean[i] = E[i];
+        info->sig_mean[i] = S[i];
+      }
+    } else {
+      const double a = 0.5 * (eps_bar[1] - eps_bar[2]);
+      const double E[4] = {em[0], em[2] + a, em[2] - a, em[1]};
+      const double S[4] = {sm[0], sm[2], sm[2], sm[1]};
+      for (int i = 0; i < 9; ++i) {
+        info->eps_mean[i] = i < 4 ? E[i] : 0.0;
+        info->sig_mean[i] = i < 4 ? S[i] : 0.0;
+      }
+    }
+  }
+  return s.status;
+}
+
+}  // namespace
+
+extern "C" int mgk_ls_workspace_bytes(int ndim, const int64_t* shape, int precision, size_t* bytes) {
+  LsPlan pl;
+  int rc = precision == 0 ? ls_plan<double>(ndim, shape, &pl) : ls_plan<float>(ndim, shape, &pl);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the LS solver", __FILE__, __LINE__);
+    return rc;
+  }
+  *bytes = pl.total;
+  return MGK_OK;
+}
+
+extern "C" int mgk_ls_solve(int ndim, const int64_t* shape, int precision, const void* lam, const void* mu,
+                            double lam0, double mu0, const double* eps_bar, const void* eps_plastic, void* eps,
+                            void* sig, int warm, int max_iter, double tol, void* workspace, size_t workspace_bytes,
+                            void* stream, mgk_ls_info* info) {
+  if (!lam || !mu || !eps || !eps_bar || !workspace || max_iter < 1) {
+    mgk_set_error("null argument or max_iter < 1", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  LsPlan pl;
+  int rc = precision == 0 ? ls_plan<double>(ndim, shape, &pl) : ls_plan<float>(ndim, shape, &pl);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the LS solver", __FILE__, __LINE__);
+    return rc;
+  }
+  if (workspace_bytes < pl.total) {
+    mgk_set_error("workspace too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  if (precision == 0) {
+    if (ndim == 3)
+      return ls_solve_t<double, 3>(pl, (const double*)lam, (const double*)mu, lam0, mu0, eps_bar,
+                                   (const double*)eps_plastic, (double*)eps, (double*)sig, warm, max_iter, tol, ws, s,
+                                   info);
+    return ls_solve_t<double, 2>(pl, (const double*)lam, (const double*)mu, lam0, mu0, eps_bar,
+                                 (const double*)eps_plastic, (double*)eps, (double*)sig, warm, max_iter, tol, ws, s,
+                                 info);
+  }
+  if (ndim == 3)
+    return ls_solve_t<float, 3>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar,
+                                (const float*)eps_plastic, (float*)eps, (float*)sig, warm, max_iter, tol, ws, s, info);
+  return ls_solve_t<float, 2>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar, (const float*)eps_plastic,
+                              (float*)eps, (float*)sig, warm, max_iter, tol, ws, s, info);
+}

@@ -1,0 +1,238 @@
+"""Oracle: small-strain Lippmann-Schwinger fixed point (CPU restatement).
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Follows, step for step:
+  mgkmc/elasticity.py:3-10      compute_lame_2d
+  mgkmc/elasticity.py:12-18     stress_from_strain_2d
+  mgkmc/elasticity.py:20-53     green_operator_2d
+  mgkmc/elasticity.py:55-59     compute_lame_3d
+  mgkmc/elasticity.py:62-67     stress_from_strain_3d
+  mgkmc/elasticity.py:70-105    green_operator_3d
+  mgkmc/fft.py:9-31             wave vectors, fft_field / ifft_field (ortho)
+  mgkmc/linear_elastic_simulator.py:12-88    spectral_solver_2d
+  mgkmc/linear_elastic_simulator.py:274-376  spectral_solver_3d
+  mgkmc/stz/update_fft.py:4-22  update_stress_fft_full
+
+The reference uses pyfftw (FFTW3) complex fftn with norm="ortho"; this file
+uses scipy.fft.fftn/ifftn (same DFT definition and normalisation; FFTW is not
+installable here).  ``FFT_WORKERS`` sets scipy's thread count, the stand-in
+for pyfftw.config.NUM_THREADS.
+
+Pinned by (tests/test_oracle_ls.py): homogeneous closed form
+sigma_xx = (lam+2mu)*eps (tests/test_barrier_physics.py:49-98 of the
+reference), the Eshelby sign KAT (tests/test_plasticity.py:10-61), the exact
+laminate solution (uniform sigma_xx, strain ratio = inverse stiffness ratio),
+Gamma projector identities (Gamma:C0:Gamma = Gamma, Gamma(0) = 0), and the
+two conditions that fix the converged field independently of the iteration:
+divergence-free stress and compatible strain with mean eps_bar (checked in
+Fourier space away from Nyquist planes).
+"""
+import numpy as np
+import scipy.fft as _sfft
+
+FFT_WORKERS = 1
+
+
+def compute_lame_2d(E, nu, plane_mode="plane_strain"):
+    """Lame pair for 2D (elasticity.py:3-10)."""
+    mu = E / (2 * (1 + nu))
+    if plane_mode == "plane_stress":
+        lam = E * nu / (1 - nu ** 2)
+    else:
+        lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    return lam, mu
+
+
+def compute_lame_3d(E, nu):
+    """Lame pair for 3D (elasticity.py:55-59)."""
+    mu = E / (2 * (1 + nu))
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    return lam, mu
+
+
+def stress_from_strain_2d(eps, E, nu, plane_mode="plane_strain"):
+    """sigma = 2 mu eps + lam tr(eps) I on (nx,ny,2,2) (elasticity.py:12-18)."""
+    lam, mu = compute_lame_2d(E, nu, plane_mode)
+    tr = np.trace(eps, axis1=2, axis2=3)[..., None, None]
+    return 2 * mu[..., None, None] * eps + lam[..., None, None] * tr * np.eye(2)[None, None]
+
+
+def stress_from_strain_3d(eps, E, nu):
+    """sigma = 2 mu eps + lam tr(eps) I on (nx,ny,nz,3,3) (elasticity.py:62-67)."""
+    lam, mu = compute_lame_3d(E, nu)
+    tr = np.trace(eps, axis1=3, axis2=4)[..., None, None]
+    return 2 * mu[..., None, None] * eps + lam[..., None, None] * tr * np.eye(3)[None, None, None]
+
+
+def wave_vectors(shape, pixel=1.0):
+    """k_i = 2 pi fftfreq(n_i, d=L_i/n_i), meshgrid 'ij' (fft.py:9-20)."""
+    ks = [2 * np.pi * np.fft.fftfreq(n, d=(n * pixel) / n) for n in shape]
+    return np.meshgrid(*ks, indexing="ij")
+
+
+def green_operator(q, lam0, mu0):
+    """Materialised isotropic strain Green operator Gamma0_khij(k), spatial
+    dims first, Gamma(0) = 0, Nyquist kept (elasticity.py:20-53, 70-105)."""
+    d = len(q)
+    shape = q[0].shape
+    k2 = sum(qi * qi for qi in q)
+    k2s = k2.copy()
+    k2s[(0,) * d] = 1.0
+    A = 1.0 / (4 * mu0)
+    B = (lam0 + mu0) / (mu0 * (lam0 + 2 * mu0))
+    G = np.zeros(shape + (d, d, d, d))
+    for k in range(d):
+        for h in range(d):
+            for i in range(d):
+                for j in range(d):
+                    t1 = 0.0
+                    if k == i:
+                        t1 = t1 + q[h] * q[j]
+                    if h == i:
+                        t1 = t1 + q[k] * q[j]
+                    if k == j:
+                        t1 = t1 + q[h] * q[i]
+                    if h == j:
+                        t1 = t1 + q[k] * q[i]
+                    t1 = A * t1 / k2s
+                    t2 = B * (q[k] * q[h] * q[i] * q[j]) / (k2s * k2s)
+                    G[..., k, h, i, j] = t1 - t2
+    G[(0,) * d] = 0
+    return G
+
+
+def fft_field(f):
+    """Complex fftn, norm='ortho' (fft.py:22-25)."""
+    return _sfft.fftn(f, norm="ortho", workers=FFT_WORKERS)
+
+
+def ifft_field(f_hat):
+    """Complex ifftn, norm='ortho', real part (fft.py:28-31)."""
+    return _sfft.ifftn(f_hat, norm="ortho", workers=FFT_WORKERS).real
+
+
+def _ls_solve(d, E, nu, eps_bar, eps_plastic, max_iter, tol, pixel, lame, stress,
+              Gamma, eps_init, record):
+    shape = E.shape
+    E_avg = E.mean()
+    nu_avg = nu.mean()
+    if Gamma is None:
+        lam0, mu0 = lame(E_avg, nu_avg)
+        Gamma = green_operator(wave_vectors(shape, pixel), lam0, mu0)
+    if eps_init is not None:
+        eps = eps_init.copy()
+        for i in range(d):
+            for j in range(d):
+                eps[..., i, j] += eps_bar[i, j] - eps_init[..., i, j].mean()
+    else:
+        eps = np.zeros(shape + (d, d))
+        for i in range(d):
+            for j in range(d):
+                eps[..., i, j] = eps_bar[i, j]
+    letters = "xyz"[:d]
+    ein = f"{letters}khij,{letters}ij->{letters}kh"
+    it_done = 0
+    diff = np.inf
+    for it in range(max_iter):
+        if eps_plastic is not None:
+            sig = stress(eps - eps_plastic, E, nu)
+        else:
+            sig = stress(eps, E, nu)
+        sig0 = stress(eps, E * 0 + E_avg, nu * 0 + nu_avg)
+        tau = sig - sig0
+        tau_hat = np.zeros(shape + (d, d), dtype=complex)
+        for i in range(d):
+            for j in range(d):
+                tau_hat[..., i, j] = fft_field(tau[..., i, j])
+        et_hat = -np.einsum(ein, Gamma, tau_hat)
+        et = np.zeros(shape + (d, d))
+        for i in range(d):
+            for j in range(d):
+                et[..., i, j] = ifft_field(et_hat[..., i, j])
+        for i in range(d):
+            for j in range(d):
+                et[..., i, j] -= et[..., i, j].mean()
+        eps_new = np.zeros(shape + (d, d))
+        for i in range(d):
+            for j in range(d):
+                eps_new[..., i, j] = eps_bar[i, j] + et[..., i, j]
+        diff = np.linalg.norm(eps_new - eps) / (np.linalg.norm(eps) + 1e-20)
+        eps = eps_new
+        it_done = it + 1
+        if record is not None:
+            record.append(diff)
+        if diff < tol:
+            break
+    if eps_plastic is not None:
+        sig_out = stress(eps - eps_plastic, E, nu)
+    else:
+        sig_out = stress(eps, E, nu)
+    axes = tuple(range(d))
+    return eps, sig_out, eps.mean(axis=axes), sig_out.mean(axis=axes), it_done, diff
+
+
+def spectral_solver_2d(E, nu, eps_bar, eps_plastic=None, max_iter=200, tol=1e-6,
+                       verbose=False, pixel=1.0, plane_mode="plane_strain",
+                       Gamma=None, eps_init=None, info=None):
+    """2D LS fixed point (linear_elastic_simulator.py:12-88).  ``info`` (a dict,
+    oracle-only extension) receives the iteration count and per-iteration diffs."""
+    rec = [] if info is not None else None
+    out = _ls_solve(2, np.asarray(E, float), np.asarray(nu, float), np.asarray(eps_bar, float),
+                    eps_plastic, max_iter, tol, pixel,
+                    lambda E_, nu_: compute_lame_2d(E_, nu_, plane_mode),
+                    lambda e, E_, nu_: stress_from_strain_2d(e, E_, nu_, plane_mode),
+                    Gamma, eps_init, rec)
+    if info is not None:
+        info.update(iters=out[4], diff=out[5], diffs=rec)
+    return out[:4]
+
+
+def spectral_solver_3d(E, nu, eps_bar, eps_plastic=None, max_iter=200, tol=1e-6,
+                       verbose=False, pixel=1.0, Gamma=None, eps_init=None, info=None):
+    """3D LS fixed point (linear_elastic_simulator.py:274-376)."""
+    rec = [] if info is not None else None
+    out = _ls_solve(3, np.asarray(E, float), np.asarray(nu, float), np.asarray(eps_bar, float),
+                    eps_plastic, max_iter, tol, pixel, compute_lame_3d, stress_from_strain_3d,
+                    Gamma, eps_init, rec)
+    if info is not None:
+        info.update(iters=out[4], diff=out[5], diffs=rec)
+    return out[:4]
+
+
+def update_stress_fft_full(eps_plastic_field, eps_macro, E, nu, pixel=1.0,
+                           max_iter=200, tol=1e-6, verbose=False):
+    """stz/update_fft.py:4-22: cold-start 3D LS, Gamma rebuilt."""
+    return spectral_solver_3d(E, nu, eps_macro, eps_plastic=eps_plastic_field,
+                              max_iter=max_iter, tol=tol, verbose=verbose, pixel=pixel)
+
+
+def equilibrium_residual(sig, pixel=1.0):
+    """FP64 equilibrium residual of a stress field (nx,..,d,d):
+    sqrt(sum_k |q_j sig_hat_ij|^2 / |q|^2) / sqrt(sum_k |sig_hat|^2), k != 0 and
+    k off the Nyquist planes.
+    A build-defined measure (the reference has none); used for the FP32 +
+    refinement form's acceptance test."""
+    d = sig.shape[-1]
+    shape = sig.shape[:-2]
+    q = wave_vectors(shape, pixel)
+    k2 = sum(qi * qi for qi in q)
+    k2[(0,) * d] = 1.0
+    sh = np.stack([np.stack([np.fft.fftn(sig[..., i, j]) for j in range(d)], -1)
+                   for i in range(d)], -2)
+    sh[(0,) * d] = 0
+    # Nyquist planes of even axes are excluded: there the reference's
+    # full-spectrum Re(ifft) applies the average of Gamma over +-q_nyq, whose
+    # kernel is not {q.sigma = 0}.
+    for a, n in enumerate(shape):
+        if n % 2 == 0:
+            sl = [slice(None)] * d
+            sl[a] = n // 2
+            sh[tuple(sl)] = 0
+    div = np.zeros(shape + (d,), complex)
+    for i in range(d):
+        for j in range(d):
+            div[..., i] += q[j] * sh[..., i, j]
+    num = np.sum(np.abs(div) ** 2 / k2[..., None])
+    den = np.sum(np.abs(sh) ** 2)
+    return float(np.sqrt(num) / (np.sqrt(den) + 1e-300))

@@ -1,0 +1,142 @@
+"""GPU parity of the LS solver against the oracle (FP64 <= 1e-10 relative,
+identical iteration counts), plus size-independent properties at full size."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import ls as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(shape, seed, with_ep=True, E0=70e9, scale=1e-3):
+    rng = np.random.default_rng(seed)
+    d = len(shape)
+    E = E0 * rng.uniform(0.9, 1.1, shape)
+    nu = np.full(shape, 0.35)
+    ep = None
+    if with_ep:
+        g = rng.standard_normal(shape + (d, d)) * scale
+        ep = 0.5 * (g + np.swapaxes(g, -1, -2))
+    return E, nu, ep
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (16, 12), (5, 6), (32, 96)])
+def test_ls2d_parity(gpu, shape):
+    from mgkmc_amd.linear_elastic_simulator import spectral_solver_2d
+    print(f"[ls2d] shape={shape}", flush=True)
+    E, nu, ep = _case(shape, 0)
+    eb = np.array([[0.01, 0.002], [0.002, -0.003]])
+    for mode in ("plane_strain", "plane_stress"):
+        info = {}
+        ref = ol.spectral_solver_2d(E, nu, eb, eps_plastic=ep, tol=1e-8, plane_mode=mode, info=info)
+        got = spectral_solver_2d(E, nu, eb, eps_plastic=ep, tol=1e-8, plane_mode=mode)
+        for a, b in zip(got, ref):
+            assert _rel(a, b) < 1e-10, (mode, _rel(a, b))
+    # iteration count: rerun through the device object to read it
+    from mgkmc_amd.linear_elastic_simulator import _drop_in
+    inf = {}
+    _drop_in(2, E, nu, eb, ep, 200, 1e-8, "plane_strain", None, info=inf)
+    info = {}
+    ol.spectral_solver_2d(E, nu, eb, eps_plastic=ep, tol=1e-8, info=info)
+    assert inf["iters"] == info["iters"]
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 6, 10), (5, 4, 3), (16, 16, 1), (12, 8, 32)])
+def test_ls3d_parity(gpu, shape):
+    from mgkmc_amd.linear_elastic_simulator import spectral_solver_3d, _drop_in
+    print(f"[ls3d] shape={shape}", flush=True)
+    E, nu, ep = _case(shape, 1)
+    eb = np.zeros((3, 3))
+    eb[0, 0] = 0.01
+    eb[1, 2] = eb[2, 1] = 0.001
+    info = {}
+    ref = ol.spectral_solver_3d(E, nu, eb, eps_plastic=ep, tol=1e-8, info=info)
+    got = spectral_solver_3d(E, nu, eb, eps_plastic=ep, tol=1e-8)
+    for a, b in zip(got, ref):
+        assert _rel(a, b) < 1e-10, _rel(a, b)
+    inf = {}
+    _drop_in(3, E, nu, eb, ep, 200, 1e-8, "plane_strain", None, info=inf)
+    assert inf["iters"] == info["iters"]
+
+
+def test_ls3d_warm_start_and_nonsymmetric_bar(gpu):
+    from mgkmc_amd.linear_elastic_simulator import spectral_solver_3d
+    shape = (8, 8, 8)
+    E, nu, ep = _case(shape, 2)
+    eb = np.zeros((3, 3))
+    eb[0, 0] = 0.004
+    eb[0, 1] = 0.003  # non-symmetric macro strain
+    rng = np.random.default_rng(5)
+    g = rng.standard_normal(shape + (3, 3)) * 1e-4
+    init = 0.5 * (g + np.swapaxes(g, -1, -2))
+    ref = ol.spectral_solver_3d(E, nu, eb, eps_plastic=ep, tol=1e-9, eps_init=init)
+    got = spectral_solver_3d(E, nu, eb, eps_plastic=ep, tol=1e-9, eps_init=init)
+    for a, b in zip(got, ref):
+        assert _rel(a, b) < 1e-10, _rel(a, b)
+
+
+def test_ls3d_max_iter_silent_and_homogeneous(gpu):
+    from mgkmc_amd.linear_elastic_simulator import spectral_solver_3d, _drop_in
+    shape = (16, 16, 1)
+    E = np.full(shape, 70e9)
+    nu = np.full(shape, 0.3)
+    eb = np.zeros((3, 3))
+    eb[0, 0] = 0.01
+    eps, sig, em, sm = spectral_solver_3d(E, nu, eb, max_iter=50, tol=1e-6)
+    assert abs(sm[0, 0] / 1e9 - 0.9423077) < 1e-6
+    E2, nu2, ep2 = _case((8, 8, 8), 4)
+    inf = {}
+    _drop_in(3, E2, nu2, eb, ep2, 3, 1e-30, "plane_strain", None, info=inf)
+    assert inf["iters"] == 3 and inf["status"] == 1
+
+
+def test_ls3d_fp32_matches_fp64_within_stated_error(gpu):
+    # FP32 form: field error vs the FP64 oracle stays near FP32 resolution
+    import torch
+    from mgkmc_amd.linear_elastic_simulator import LsSolver, FP32
+    from mgkmc_amd import device as dev
+    shape = (32, 32, 32)
+    E, nu, ep = _case(shape, 6)
+    eb = np.diag([0.01, 0.0, 0.0])
+    ref = ol.spectral_solver_3d(E, nu, eb, eps_plastic=ep, tol=1e-10)
+    s = LsSolver(E, nu, precision=FP32)
+    sym, _ = dev.to_sym_soa(ep, 3)
+    inf = s.solve(eb, dev.upload(sym, FP32), tol=1e-7, max_iter=100)
+    eps = dev.from_sym_soa(dev.download(s.eps), 3)
+    err = _rel(eps, ref[0])
+    print(f"[fp32] iters={inf.iters} diff={inf.diff:.2e} field rel err vs FP64 oracle={err:.2e}", flush=True)
+    assert err < 1e-5
+
+
+def test_ls3d_full_size_properties(gpu):
+    # 128^3: homogeneous medium => exact uniform field; heterogeneous => the
+    # converged field has mean eps_bar and the solve is deterministic
+    import torch
+    from mgkmc_amd.linear_elastic_simulator import LsSolver, FP64
+    from mgkmc_amd import device as dev
+    n = 128
+    print(f"[ls3d full] n={n}", flush=True)
+    E = np.full((n, n, n), 70e9)
+    nu = np.full((n, n, n), 0.3)
+    s = LsSolver(E, nu, precision=FP64)
+    eb = np.diag([0.01, -0.002, 0.0])
+    inf = s.solve(eb, None, tol=1e-8)
+    assert inf.iters == 1
+    eps = dev.download(s.eps)
+    assert np.abs(eps[0] - 0.01).max() < 1e-15 and np.abs(eps[3:]).max() < 1e-15
+    rng = np.random.default_rng(7)
+    E = 70e9 * rng.uniform(0.9, 1.1, (n, n, n))
+    s = LsSolver(E, nu, precision=FP64)
+    ep = torch.from_numpy(rng.standard_normal((6, n, n, n)) * 1e-3).cuda()
+    inf = s.solve(eb, ep, tol=1e-8)
+    it1, m1 = inf.iters, list(inf.eps_mean)
+    e1 = s.eps.clone()
+    inf = s.solve(eb, ep, tol=1e-8)
+    assert it1 == inf.iters and it1 > 2 and torch.equal(e1, s.eps)
+    assert np.allclose(m1, eb.ravel(), atol=1e-14)

@@ -1,0 +1,137 @@
+"""Pin the LS oracle (oracle/ls.py) against facts that do not come from its
+own formulas: the reference's KATs, an exact laminate solution, and the
+equilibrium / compatibility conditions that fix the converged field."""
+import numpy as np
+import pytest
+
+from oracle import ls as ol
+
+
+def _rng(seed=0):
+    return np.random.default_rng(seed)
+
+
+def test_homogeneous_kat_sigma_xx():
+    # reference tests/test_barrier_physics.py:49-98: 16x16x1, E=70 GPa, nu=0.3,
+    # eps_xx = 1% => sigma_xx = (lam+2mu)*0.01 = 0.94231 GPa
+    shape = (16, 16, 1)
+    E = np.full(shape, 70e9)
+    nu = np.full(shape, 0.3)
+    eb = np.zeros((3, 3))
+    eb[0, 0] = 0.01
+    info = {}
+    eps, sig, em, sm = ol.spectral_solver_3d(E, nu, eb, max_iter=50, tol=1e-6, info=info)
+    lam = 70e9 * 0.3 / (1.3 * 0.4)
+    mu = 70e9 / 2.6
+    assert abs(sm[0, 0] - (lam + 2 * mu) * 0.01) < 1e-6 * abs(sm[0, 0])
+    assert abs(sm[0, 0] / 1e9 - 0.9423077) < 1e-6
+    assert np.allclose(eps, eb)
+    assert info["iters"] == 1
+
+
+def test_eshelby_sign_kat():
+    # reference tests/test_plasticity.py:10-61: positive plastic shear at one
+    # voxel => negative residual shear stress there
+    shape = (16, 16, 1)
+    E = np.full(shape, 70e9)
+    nu = np.full(shape, 0.3)
+    ep = np.zeros(shape + (3, 3))
+    ep[8, 8, 0, 0, 1] = ep[8, 8, 0, 1, 0] = 0.05
+    eps, sig, _, _ = ol.spectral_solver_3d(E, nu, np.zeros((3, 3)), eps_plastic=ep)
+    assert sig[8, 8, 0, 0, 1] < 0
+    assert 0 < eps[8, 8, 0, 0, 1] < 0.05
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_laminate_exact(d):
+    # layers normal to x: sigma_xx uniform, eps_xx(x) = S/(lam+2mu)(x),
+    # transverse strains uniform, S fixed by <eps_xx> = e
+    rng = _rng(1)
+    n = 16
+    shape = (n,) * d
+    Ex = 70e9 * rng.uniform(0.5, 1.5, n)
+    E = np.broadcast_to(Ex.reshape((n,) + (1,) * (d - 1)), shape).copy()
+    nu = np.full(shape, 0.3)
+    e = 0.01
+    eb = np.zeros((d, d))
+    eb[0, 0] = e
+    if d == 3:
+        eps, sig, em, sm = ol.spectral_solver_3d(E, nu, eb, tol=1e-14, max_iter=2000)
+        lam, mu = ol.compute_lame_3d(Ex, 0.3)
+    else:
+        eps, sig, em, sm = ol.spectral_solver_2d(E, nu, eb, tol=1e-14, max_iter=2000)
+        lam, mu = ol.compute_lame_2d(Ex, 0.3)
+    M = lam + 2 * mu
+    S = e / np.mean(1.0 / M)
+    exx = S / M
+    got = eps[(slice(None),) + (0,) * (d - 1) + (0, 0)]
+    assert np.allclose(got, exx, rtol=1e-9, atol=0)
+    assert np.allclose(sig[..., 0, 0], S, rtol=1e-9)
+
+
+@pytest.mark.parametrize("shape", [(8, 8), (6, 5, 4), (4, 4, 4)])
+def test_converged_field_is_equilibrated_and_compatible(shape):
+    # the LS solution is the unique field with mean eps_bar, compatible eps and
+    # divergence-free sigma; check both conditions in Fourier space
+    d = len(shape)
+    rng = _rng(2)
+    E = 70e9 * rng.uniform(0.8, 1.2, shape)
+    nu = np.full(shape, 0.35)
+    g = rng.standard_normal(shape + (d, d)) * 1e-3
+    ep = 0.5 * (g + np.swapaxes(g, -1, -2))
+    eb = np.eye(d) * 0.002
+    eb[0, 1] = eb[1, 0] = 0.001
+    solve = ol.spectral_solver_3d if d == 3 else ol.spectral_solver_2d
+    eps, sig, em, sm = solve(E, nu, eb, eps_plastic=ep, tol=1e-14, max_iter=5000)
+    assert np.allclose(em, eb, atol=1e-15)
+    # equilibrium: residual relative to stress magnitude
+    assert ol.equilibrium_residual(sig) < 1e-10
+    # compatibility: for k != 0 (non-Nyquist), eps_hat = sym(k (x) u) for some u,
+    # i.e. the projection onto compatible fields leaves it unchanged
+    q = ol.wave_vectors(shape)
+    eh = np.stack([np.stack([np.fft.fftn(eps[..., i, j]) for j in range(d)], -1) for i in range(d)], -2)
+    k2 = sum(qi * qi for qi in q)
+    for idx in np.ndindex(shape):
+        if all(i == 0 for i in idx):
+            continue
+        if any(n % 2 == 0 and i == n // 2 for i, n in zip(idx, shape)):
+            continue
+        kv = np.array([qi[idx] for qi in q])
+        E_ = eh[idx]
+        # u = 2 E k / |k|^2 - (k.E.k) k / |k|^4 (unique u with sym(k u) = E when compatible)
+        u = (2 * E_ @ kv) / k2[idx] - (kv @ E_ @ kv) * kv / k2[idx] ** 2
+        rec = 0.5 * (np.outer(kv, u) + np.outer(u, kv))
+        assert np.abs(rec - E_).max() <= 1e-9 * np.abs(eh).max()
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_gamma_projector(d):
+    # Gamma : C0 : Gamma = Gamma and Gamma(0) = 0
+    shape = (6,) * d
+    q = ol.wave_vectors(shape)
+    lam0, mu0 = 40e9, 27e9
+    G = ol.green_operator(q, lam0, mu0)
+    I = np.eye(d)
+    C0 = (lam0 * np.einsum("ij,kl->ijkl", I, I)
+          + mu0 * (np.einsum("ik,jl->ijkl", I, I) + np.einsum("il,jk->ijkl", I, I)))
+    assert np.all(G[(0,) * d] == 0)
+    for idx in [(1,) + (2,) * (d - 1), (3,) + (0,) * (d - 1), (2,) * d]:
+        g = G[idx]
+        gcg = np.einsum("khij,ijab,abmn->khmn", g, C0, g)
+        assert np.allclose(gcg, g, rtol=1e-12, atol=1e-12 * np.abs(g).max())
+
+
+def test_iteration_trajectory_stops_like_reference():
+    # the stop rule is diff < tol after the update, silent at max_iter
+    shape = (8, 8)
+    rng = _rng(3)
+    E = 70e9 * rng.uniform(0.9, 1.1, shape)
+    nu = np.full(shape, 0.35)
+    eb = np.diag([0.01, 0.0])
+    info = {}
+    ol.spectral_solver_2d(E, nu, eb, tol=1e-8, info=info)
+    diffs = info["diffs"]
+    assert diffs[-1] < 1e-8 and all(x >= 1e-8 for x in diffs[:-1])
+    info2 = {}
+    ol.spectral_solver_2d(E, nu, eb, tol=1e-30, max_iter=3, info=info2)
+    assert info2["iters"] == 3
