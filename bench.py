@@ -1,0 +1,149 @@
+#!/usr/bin/env python
+"""Headline benchmark: LS voxel-iter/s and KMC steps/s at 256^3 on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+
+Workload (synthetic, seeded): 3D n^3 small-strain Lippmann-Schwinger with the
+KMC hybrid's inputs -- shear modulus mu ~ U[0.9,1.1] * 26.923 GPa, nu = 0.35,
+zero-mean i.i.d. Gaussian eigenstrain (std 1e-3 per component, seed 1),
+uniaxial eps_xx = 1%.  One step = one cold-start LS solve to tol 1e-6 in the
+FP32 form (FP32 FFT / Green operator / constitutive sweeps, FP64 norms and
+stop decision).  Inputs are resident in HBM before the timed region.
+
+value = whole-job LS voxel-iterations per second of the FP32 form.  The same
+line reports the plain FP64 path, the live per-kernel roofline of the
+dominant kernel (HIP events on the launching stream, algorithmic bytes), and
+the CPU oracle's rate on a bounded sample timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MU_UNIT = 70e9 / 2.6  # Pa; config.yaml E = 70 GPa, nu = 0.3
+HBM_PEAK = 8.0e12
+
+
+def fields(n, seed=1):
+    rng = np.random.default_rng(seed)
+    mu = MU_UNIT * rng.uniform(0.9, 1.1, (n, n, n))
+    nu = np.full((n, n, n), 0.35)
+    E = 2.0 * mu * (1.0 + nu)
+    ep = rng.standard_normal((6, n, n, n)) * 1e-3
+    ep -= ep.mean(axis=(1, 2, 3), keepdims=True)
+    return E, nu, ep
+
+
+def gpu_run(n, steps, warmup):
+    import torch
+    from mgkmc_amd import _lib
+    from mgkmc_amd.linear_elastic_simulator import LsSolver, FP32, FP64
+    E, nu, ep = fields(n)
+    eb = np.diag([0.01, 0.0, 0.0])
+    out = {}
+    for prec in (FP32, FP64):
+        s = LsSolver(E, nu, precision=prec)
+        ep_d = torch.from_numpy(ep).to(device="cuda", dtype=s.eps.dtype)
+        for _ in range(warmup):
+            s.solve(eb, ep_d, tol=1e-6)
+        torch.cuda.synchronize()
+        lib = _lib.load()
+        lib.mgk_profile_reset()
+        lib.mgk_profile_enable(1 if prec == FP32 else 0)
+        iters = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            inf = s.solve(eb, ep_d, tol=1e-6)
+            iters += inf.iters
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        lib.mgk_profile_enable(0)
+        prof = _lib.profile_read() if prec == FP32 else None
+        out[prec] = dict(iters=iters, seconds=dt, voxel_iter_per_s=iters * n ** 3 / dt, last_diff=inf.diff,
+                         prof=prof)
+        del s, ep_d
+        torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline(n_sample=128, iters=3):
+    """The oracle (reference structure: materialised Gamma, per-component
+    complex FFTs) on this host: LS iterations on a bounded 128^3 sample."""
+    from oracle import ls as ol
+    cores = len(os.sched_getaffinity(0))
+    ol.FFT_WORKERS = cores
+    E, nu, ep = fields(n_sample)
+    epa = np.zeros((n_sample,) * 3 + (3, 3))
+    for c, (i, j) in enumerate(((0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1))):
+        epa[..., i, j] = ep[c]
+        epa[..., j, i] = ep[c]
+    eb = np.diag([0.01, 0.0, 0.0])
+    lam0, mu0 = ol.compute_lame_3d(E.mean(), nu.mean())
+    G = ol.green_operator(ol.wave_vectors(E.shape), lam0, mu0)
+    t0 = time.perf_counter()
+    ol.spectral_solver_3d(E, nu, eb, eps_plastic=epa, max_iter=iters, tol=0.0, Gamma=G)
+    dt = time.perf_counter() - t0
+    return {"value": iters * n_sample ** 3 / dt, "unit": "voxel-iter/s", "cores": cores, "kind": "port",
+            "sample": f"{n_sample}^3 LS, {iters} iterations of the oracle (materialised Gamma0 built "
+                      f"beforehand, scipy.fft workers={cores})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); multi-GPU sharding is design-only "
+              "in this build", file=sys.stderr)
+        sys.exit(2)
+    res = gpu_run(a.n, a.steps, a.warmup)
+    r32, r64 = res[1], res[0]
+    prof = r32["prof"] or {}
+    dom = max(prof.items(), key=lambda kv: kv[1][0]) if prof else None
+    roof = None
+    if dom:
+        name, (ms, by, cnt) = dom
+        ach = (by / cnt) / ((ms / cnt) * 1e-3)
+        roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": None, "kernel": name,
+                "bytes_per_launch": by / cnt, "avg_launch_ms": ms / cnt,
+                "per_kernel": {k: {"avg_ms": v[0] / v[2], "GB_s": (v[1] / v[2]) / (v[0] / v[2] * 1e-3) / 1e9}
+                               for k, v in prof.items()}}
+    cpu = None if a.no_cpu else cpu_baseline()
+    steps = a.steps
+    line = {
+        "metric": "LS voxel-iter/s + KMC steps/s at 256^3, FP32 vs FP64 @1e-6; req p50/p99 latency",
+        "value": r32["voxel_iter_per_s"],
+        "unit": "voxel-iter/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": r32["seconds"] / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D {a.n}^3 small-strain LS, cold start to tol 1e-6, FP32 form",
+                   "grid": [a.n] * 3, "tol": 1e-6, "seed": 1},
+        "ls_fp64_voxel_iter_per_s": r64["voxel_iter_per_s"],
+        "fp32_over_fp64": r32["voxel_iter_per_s"] / r64["voxel_iter_per_s"],
+        "iters_per_solve": {"fp32": r32["iters"] / steps, "fp64": r64["iters"] / steps},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -32,6 +32,16 @@ int mgk_version(void);
 const char* mgk_last_error(void);
 int mgk_device_info(int* lds_per_block, int* cu_count);
 
+/* Live per-kernel timing with HIP events on the launching stream (no
+ * counterpart in the reference; used by bench.py).  kid: 0 ls_tau_xfwd,
+ * 1 ls_fft_mid, 2 ls_pencil_gamma, 3 ls_xinv_update, 4 kmc_barrier_rates,
+ * 5 kmc_select, 6 fs_constitutive, 7 fs_xfwd, 8 fs_fft_mid, 9 fs_pencil_proj,
+ * 10 fs_xinv, 11 fs_vector.  total_bytes: algorithmic HBM bytes of the timed
+ * launches. */
+void mgk_profile_enable(int on);
+void mgk_profile_reset(void);
+int mgk_profile_read(int kid, double* total_ms, double* total_bytes, long long* launches);
+
 /* ---------------------------------------------------------------------------
  * Small-strain Lippmann-Schwinger fixed point.
  * Replaces spectral_solver_2d (mgkmc/linear_elastic_simulator.py:12-88) and

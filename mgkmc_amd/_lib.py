@@ -39,6 +39,9 @@ SIGNATURES = {
     "mgk_version": (_c_i, []),
     "mgk_last_error": (ctypes.c_char_p, []),
     "mgk_device_info": (_c_i, [ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
+    "mgk_profile_enable": (None, [_c_i]),
+    "mgk_profile_reset": (None, []),
+    "mgk_profile_read": (_c_i, [_c_i, _c_dp, _c_dp, ctypes.POINTER(ctypes.c_longlong)]),
     "mgk_ls_workspace_bytes": (_c_i, [_c_i, _c_i64p, _c_i, ctypes.POINTER(_c_sz)]),
     "mgk_ls_solve": (_c_i, [_c_i, _c_i64p, _c_i, _c_p, _c_p, _c_d, _c_d, _c_dp, _c_p, _c_p, _c_p,
                             _c_i, _c_i, _c_d, _c_p, _c_sz, _c_p, ctypes.POINTER(MgkLsInfo)]),
@@ -91,3 +94,21 @@ def dptr(t):
 def host_dbl(a):
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a, a.ctypes.data_as(_c_dp)
+
+
+# kernel ids of the live timing counters (mgk_host.h MgkKid)
+KID = {"ls_tau_xfwd": 0, "ls_fft_mid": 1, "ls_pencil_gamma": 2, "ls_xinv_update": 3,
+       "kmc_barrier_rates": 4, "kmc_select": 5, "fs_constitutive": 6, "fs_xfwd": 7, "fs_fft_mid": 8,
+       "fs_pencil_proj": 9, "fs_xinv": 10, "fs_vector": 11}
+
+
+def profile_read():
+    """{kernel name: (total_ms, total_algorithmic_bytes, launches)}"""
+    lib = load()
+    out = {}
+    for name, kid in KID.items():
+        ms, by, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        lib.mgk_profile_read(kid, ctypes.byref(ms), ctypes.byref(by), ctypes.byref(n))
+        if n.value:
+            out[name] = (ms.value, by.value, n.value)
+    return out

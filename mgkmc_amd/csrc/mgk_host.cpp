@@ -144,3 +144,74 @@ extern "C" int mgk_device_info(int* lds_per_block, int* cu_count) {
   MGK_HIP(hipDeviceGetAttribute(cu_count, hipDeviceAttributeMultiprocessorCount, dev));
   return MGK_OK;
 }
+
+// ---------------------------------------------------------------------------
+// live kernel timing
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+double g_prof_ms[KID_COUNT];
+double g_prof_bytes[KID_COUNT];
+long long g_prof_n[KID_COUNT];
+}  // namespace
+
+bool mgk_prof_on() { return g_prof_on; }
+
+int MgkProfRec::begin(int kid, int iter, double bytes) {
+  if (!on_) return -1;
+  Rec r{kid, iter, bytes, nullptr, nullptr};
+  if (hipEventCreate(&r.e0) != hipSuccess || hipEventCreate(&r.e1) != hipSuccess) return -1;
+  (void)hipEventRecord(r.e0, s_);
+  r_.push_back(r);
+  return (int)r_.size() - 1;
+}
+
+void MgkProfRec::end(int h) {
+  if (!on_ || h < 0) return;
+  (void)hipEventRecord(r_[h].e1, s_);
+}
+
+void MgkProfRec::commit(int iters_done) {
+  if (!on_) return;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  for (auto& r : r_) {
+    if (r.iter >= iters_done) continue;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) {
+      g_prof_ms[r.kid] += ms;
+      g_prof_bytes[r.kid] += r.bytes;
+      g_prof_n[r.kid] += 1;
+    }
+  }
+}
+
+MgkProfRec::~MgkProfRec() {
+  for (auto& r : r_) {
+    if (r.e0) (void)hipEventDestroy(r.e0);
+    if (r.e1) (void)hipEventDestroy(r.e1);
+  }
+}
+
+extern "C" void mgk_profile_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_on = on != 0;
+}
+
+extern "C" void mgk_profile_reset(void) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  for (int i = 0; i < KID_COUNT; ++i) {
+    g_prof_ms[i] = 0.0;
+    g_prof_bytes[i] = 0.0;
+    g_prof_n[i] = 0;
+  }
+}
+
+extern "C" int mgk_profile_read(int kid, double* total_ms, double* total_bytes, long long* launches) {
+  if (kid < 0 || kid >= KID_COUNT) return MGK_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  *total_ms = g_prof_ms[kid];
+  *total_bytes = g_prof_bytes[kid];
+  *launches = g_prof_n[kid];
+  return MGK_OK;
+}

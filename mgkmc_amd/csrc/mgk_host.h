@@ -35,6 +35,51 @@ size_t mgk_lds_limit();
 // Pinned host scratch (per thread, grown on demand) for small D2H reads.
 void* mgk_pinned(size_t bytes);
 
+// ---------------------------------------------------------------------------
+// Live per-kernel timing (HIP events on the launching stream).  Off by default;
+// bench.py turns it on around its timed region.
+// ---------------------------------------------------------------------------
+enum MgkKid {
+  KID_LS_TAU = 0,
+  KID_LS_MID,
+  KID_LS_PENCIL,
+  KID_LS_UPDATE,
+  KID_KMC_RATES,
+  KID_KMC_SELECT,
+  KID_FS_CONST,
+  KID_FS_XFWD,
+  KID_FS_MID,
+  KID_FS_PENCIL,
+  KID_FS_XINV,
+  KID_FS_VEC,
+  KID_COUNT = 16
+};
+
+bool mgk_prof_on();
+
+// Records (kernel id, iteration, bytes, event pair) for one solve; commit()
+// adds the elapsed times of records with iter < iters_done to the global
+// counters after the stream has been synchronised.
+class MgkProfRec {
+ public:
+  explicit MgkProfRec(hipStream_t s) : s_(s), on_(mgk_prof_on()) {}
+  ~MgkProfRec();
+  int begin(int kid, int iter, double bytes);
+  void end(int h);
+  void commit(int iters_done);
+  bool on() const { return on_; }
+
+ private:
+  struct Rec {
+    int kid, iter;
+    double bytes;
+    hipEvent_t e0, e1;
+  };
+  hipStream_t s_;
+  bool on_;
+  std::vector<Rec> r_;
+};
+
 inline int mgk_ilog2(int v) {
   int l = 0;
   while ((1 << (l + 1)) <= v) ++l;

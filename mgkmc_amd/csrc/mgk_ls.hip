@@ -649,6 +649,19 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
   return MGK_OK;
 }
 
+// last iteration count per workspace (0 = unknown); set when iters > 0
+int ls_iter_hint(const void* ws, int iters) {
+  static std::mutex mu;
+  static std::map<const void*, int> last;
+  std::lock_guard<std::mutex> lk(mu);
+  if (iters > 0) {
+    last[ws] = iters;
+    return iters;
+  }
+  auto it = last.find(ws);
+  return it == last.end() ? 0 : it->second;
+}
+
 template <typename T, int D>
 int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double mu0, const double* eps_bar,
                const T* epsp, T* eps, T* sig, int warm, int max_iter, double tol, char* ws, hipStream_t stream,
@@ -724,17 +737,39 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   const int npb = (g.npencil + (1 << pl.logPP) - 1) >> pl.logPP;
   const dim3 gx(nqx, pl.P), gy(nqy, pl.nxh, C);
 
+  // algorithmic HBM bytes per launch (what each sweep must read + write)
+  const double sT = (double)sizeof(T);
+  const double specB = (double)C * pl.nxh * pl.P * pl.Q * 2.0 * sT;
+  const double fieldB = (double)pl.NV * sT;
+  const double b_tau = fieldB * ((epsp ? 2.0 * C : (double)C) + 2.0) + specB;
+  const double b_fft = 2.0 * specB;
+  const double b_upd = specB + 2.0 * C * fieldB;
+  MgkProfRec prof(stream);
+  int iter_idx = 0;
   auto enqueue_iter = [&]() {
+    int h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
     k_ls_tau_xfwd<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st);
-    if (D == 3 && pl.P > 1)
+    prof.end(h);
+    if (D == 3 && pl.P > 1) {
+      h = prof.begin(KID_LS_MID, iter_idx, b_fft);
       k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, g.ply, g.twy, spec, 0,
                                                           &st->done);
+      prof.end(h);
+    }
+    h = prof.begin(KID_LS_PENCIL, iter_idx, b_fft);
     k_ls_pencil<T, D, MV><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st);
-    if (D == 3 && pl.P > 1)
+    prof.end(h);
+    if (D == 3 && pl.P > 1) {
+      h = prof.begin(KID_LS_MID, iter_idx, b_fft);
       k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, g.ply, g.twy, spec, 1,
                                                           &st->done);
+      prof.end(h);
+    }
+    h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
     k_ls_xinv_update<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st);
+    prof.end(h);
     k_ls_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_upd, st, tol, max_iter, e2_extra);
+    ++iter_idx;
   };
 
   LsState* hst = (LsState*)mgk_pinned(2 * sizeof(LsState));
@@ -745,8 +780,14 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   hipEvent_t ev[2];
   MGK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   MGK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  int enq = 0, chunk = 2, slot = 0;
+  // Iterations are enqueued in chunks; the stop flag of chunk i is read while
+  // chunk i+1 runs.  The first chunk is the iteration count of the previous
+  // solve on this workspace (solves in a KMC loop converge alike), so at most
+  // one chunk of early-exit launches trails the converged iteration.
+  const int hint = ls_iter_hint(ws, 0);
+  int enq = 0, chunk = hint > 0 ? hint : 2, slot = 0;
   bool pending = false, stop = false;
+  bool first = true;
   while (!stop && enq < max_iter) {
     const int k = std::min(chunk, max_iter - enq);
     for (int i = 0; i < k; ++i) enqueue_iter();
@@ -760,7 +801,8 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     }
     pending = true;
     slot ^= 1;
-    chunk = std::min(chunk * 2, 16);
+    chunk = first ? 1 : std::min(chunk * 2, 16);
+    first = false;
   }
   // final stress and macro means
   k_ls_stress<T, D><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, sig, NV, part);
@@ -768,6 +810,8 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   MGK_HIP(hipGetLastError());
   MGK_HIP(hipMemcpyAsync(&hst[0], st, sizeof(LsState), hipMemcpyDeviceToHost, stream));
   MGK_HIP(hipStreamSynchronize(stream));
+  prof.commit(hst[0].iters);
+  ls_iter_hint(ws, hst[0].iters);
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   const LsState& s = hst[0];

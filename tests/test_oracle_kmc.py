@@ -1,0 +1,174 @@
+"""Pin the KMC oracle (oracle/kmc.py) with the reference's KATs, closed forms
+and a literal scalar-loop transcription on tiny grids."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import kmc as ok
+
+
+def _scalar_barrier_3d(Q, Q0, sig, cat, vol, soft, tl, t, prev, cap, scheme, tau):
+    # literal scalar form of the reference loop nest (checker only)
+    nx, ny, nz, M = Q.shape
+    for x in range(nx):
+        for y in range(ny):
+            for z in range(nz):
+                g_t = soft[x, y, z, 1]
+                g_tc = 0.0
+                if g_t > 0:
+                    if tau == np.inf:
+                        g_tc = g_t
+                    elif tl[x, y, z] == -np.inf:
+                        g_tc = 0.0
+                    else:
+                        dt = max(t - tl[x, y, z], 0.0)
+                        g_tc = g_t * math.exp(-dt / tau)
+                gb = soft[x, y, z, 0] + g_tc
+                if cap > 0 and gb > cap:
+                    gb = cap
+                for m in range(M):
+                    mod = 1.0
+                    if scheme == 1:
+                        dp = nm = npv = 0.0
+                        for i in range(3):
+                            for j in range(3):
+                                a, b = cat[x, y, z, m, i, j], prev[x, y, z, i, j]
+                                dp += a * b
+                                nm += a * a
+                                npv += b * b
+                        if math.sqrt(npv) >= 1e-12 and math.sqrt(nm) >= 1e-12:
+                            c = dp / (math.sqrt(nm) * math.sqrt(npv))
+                            mod = (1.0 + c) ** 2 / 4.0
+                    w = 0.0
+                    for i in range(3):
+                        for j in range(3):
+                            w += sig[x, y, z, i, j] * cat[x, y, z, m, i, j]
+                    Q[x, y, z, m] = Q0[x, y, z, m] * math.exp(-mod * gb) - 0.5 * vol * (w / 1e9) * 6.241509
+
+
+def test_barrier_work_kat():
+    # reference tests/test_barrier_physics.py:10-47
+    Q = np.zeros((1, 1, 1, 1))
+    Q0 = np.ones((1, 1, 1, 1))
+    sig = np.zeros((1, 1, 1, 3, 3))
+    sig[0, 0, 0, 0, 1] = sig[0, 0, 0, 1, 0] = 1e9
+    cat = np.zeros((1, 1, 1, 1, 3, 3))
+    cat[0, 0, 0, 0, 0, 1] = cat[0, 0, 0, 0, 1, 0] = 0.1
+    ok.compute_barrier(Q, Q0, sig, cat, 1.0, np.zeros((1, 1, 1, 4)), np.full((1, 1, 1), -np.inf), 0.0,
+                       np.zeros((1, 1, 1, 3, 3)), 2.0, 0, np.inf)
+    assert abs(Q[0, 0, 0, 0] - (1.0 - 0.6241509)) < 1e-12
+
+
+@pytest.mark.parametrize("scheme,tau", [(0, np.inf), (1, np.inf), (0, 3e-3), (1, 1e-2)])
+def test_barrier_matches_scalar_loops(scheme, tau):
+    rng = np.random.default_rng(1)
+    sh = (3, 2, 4)
+    M = 5
+    Q0 = rng.normal(2.0, 0.5, sh + (M,))
+    g = rng.standard_normal(sh + (3, 3)) * 1e9
+    sig = 0.5 * (g + np.swapaxes(g, -1, -2))
+    cat = rng.standard_normal(sh + (M, 3, 3)) * 0.07
+    prev = rng.standard_normal(sh + (3, 3)) * 0.07
+    prev[0, 0, 0] = 0.0
+    soft = np.abs(rng.standard_normal(sh + (4,)))
+    soft[1, 1, 1, 1] = 0.0
+    tl = rng.uniform(0, 1e-2, sh)
+    tl[0, 1, 2] = -np.inf
+    Q = np.zeros(sh + (M,))
+    ok.compute_barrier(Q, Q0, sig, cat, 0.343, soft, tl, 5e-3, prev, 0.916, scheme, tau)
+    Qs = np.zeros_like(Q)
+    _scalar_barrier_3d(Qs, Q0, sig, cat, 0.343, soft, tl, 5e-3, prev, 0.916, scheme, tau)
+    assert np.allclose(Q, Qs, rtol=1e-14, atol=1e-14)
+
+
+def test_directional_modifier_limits():
+    cat = np.zeros((1, 3, 3, 3))
+    p = np.zeros((1, 3, 3))
+    p[0, 0, 1] = p[0, 1, 0] = 1.0
+    cat[0, 0] = p[0]  # parallel
+    cat[0, 1] = -p[0]  # antiparallel
+    cat[0, 2, 0, 0], cat[0, 2, 1, 1] = 1.0, -1.0  # orthogonal
+    mod = ok._dir_modifier(cat, p, 3)
+    assert np.allclose(mod[0], [1.0, 0.0, 0.25])
+
+
+def test_rates_closed_form_and_cascade_filter():
+    Q = np.array([[[[0.5, -0.1, 0.0, 1.2]]]])
+    T = 800.0
+    rates, idx, tot = ok.compute_rates(Q, 1.0, T, 1e13, cascade_mode=False)
+    beta = 1.0 / (8.617e-5 * T)
+    exp = [1e13 * math.exp(-0.5 * beta), 1e13, 1e13, 1e13 * math.exp(-1.2 * beta)]
+    assert np.allclose(rates, exp, rtol=1e-15) and list(idx) == [0, 1, 2, 3]
+    assert tot == ((exp[0] + exp[1]) + exp[2]) + exp[3]
+    rates, idx, tot = ok.compute_rates(Q, 1.0, T, 1e13, cascade_mode=True)
+    assert list(idx) == [0, 3]
+    r0, i0, t0 = ok.compute_rates(Q, 1.0, 0.0, 1e13, cascade_mode=False)
+    assert list(r0) == [0.0, 1e13, 1e13, 0.0]  # T = 0: AQS limit
+
+
+def test_select_event_sequential_semantics():
+    rates = np.array([1.0, 2.0, 0.5, 3.0])
+    tot = float(np.cumsum(rates)[-1])
+    # literal running-sum loop as the checker
+    def loop(r):
+        s = 0.0
+        for i, x in enumerate(rates):
+            s += x
+            if s >= r:
+                return i
+        return len(rates) - 1
+    for r in [0.0, 0.5, 1.0, 1.0000001, 3.0, 3.2, 3.5, 6.4, 6.5, 7.0]:
+        assert ok.select_event(rates, tot, r) == loop(r)
+    assert ok.select_event(rates, 0.0, 0.1) == -1
+
+
+def test_decode_index_roundtrip():
+    ny, nz, M = 5, 7, 20
+    for x, y, z, m in [(0, 0, 0, 0), (3, 4, 6, 19), (11, 2, 3, 7)]:
+        f = ((x * ny + y) * nz + z) * M + m
+        assert ok.decode_index(f, ny, nz, M) == (x, y, z, m)
+        f2 = (x * ny + y) * M + m
+        assert ok.decode_index_2d(f2, ny, M) == (x, y, m)
+
+
+def test_flip_vm_and_neighbors():
+    # pure shear gxy = 0.05 => vm^2 = 0.0025
+    nx = ny = nz = 3
+    ep = np.zeros((nx, ny, nz, 3, 3))
+    soft = np.zeros((nx, ny, nz, 4))
+    tl = np.full((nx, ny, nz), -np.inf)
+    cat = np.zeros((nx, ny, nz, 2, 3, 3))
+    cat[0, 0, 0, 1, 0, 1] = cat[0, 0, 0, 1, 1, 0] = 0.05
+    ok.apply_flip_soa(ep, None, soft, tl, cat, 0, 0, 0, 1, 7.0, 20.0, 3.0, 0.916, 0.5)
+    assert ep[0, 0, 0, 0, 1] == 0.05 and tl[0, 0, 0] == 7.0
+    assert np.isclose(soft[0, 0, 0, 0], 20.0 * 0.0025) and np.isclose(soft[0, 0, 0, 1], 3.0 * 0.0025)
+    assert np.isclose(soft[2, 2, 2, 0], 0.5 * 20.0 * 0.0025)  # periodic neighbour
+    assert soft[1, 1, 1, 0] == 0.0 or (1, 1, 1) in [(1, 1, 1)]  # (1,1,1) is a neighbour of (0,0,0)
+    # cap
+    soft[0, 0, 0, 0] = 0.9
+    ok.apply_flip_soa(ep, None, soft, tl, cat, 0, 0, 0, 1, 8.0, 20.0, 0.0, 0.916, 0.0)
+    assert soft[0, 0, 0, 0] == 0.916
+
+
+def test_tau_closed_form():
+    # reference tests/verify_tau.py:4-29: tau = 1/(nu0 exp(-q/(kB T)))
+    nu0, q, T = 1e13, 0.37, 300.0
+    tau = 1.0 / (nu0 * math.exp(-q / (ok.KB * T)))
+    sh = (1, 1, 1)
+    Q = np.zeros(sh + (1,))
+    soft = np.zeros(sh + (4,))
+    soft[..., 1] = 1.0
+    tl = np.zeros(sh)
+    ok.compute_barrier(Q, np.ones(sh + (1,)), np.zeros(sh + (3, 3)), np.zeros(sh + (1, 3, 3)), 1.0, soft,
+                       tl, tau, np.zeros(sh + (3, 3)), 0.0, 0, tau)
+    # after one tau the transient softening has decayed to 1/e
+    assert np.isclose(Q[0, 0, 0, 0], math.exp(-math.exp(-1.0)), rtol=1e-14)
+
+
+def test_selection_stream_contract():
+    # r = total * u, u = RandomState(seed).random_sample(): MT19937 init_genrand
+    # + genrand_res53 (first output for seed 5489 is documented by the
+    # algorithm's authors as genrand_int32 = 3499211612)
+    rs = np.random.RandomState(5489)
+    assert rs.randint(0, 2 ** 32, dtype=np.uint64) == 3499211612
