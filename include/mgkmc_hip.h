@@ -69,6 +69,77 @@ int mgk_ls_solve(int ndim, const int64_t* shape, int precision, const void* lam,
                  void* sig, int warm, int max_iter, double tol, void* workspace, size_t workspace_bytes,
                  void* stream, mgk_ls_info* info);
 
+/* ---------------------------------------------------------------------------
+ * KMC: barriers, rates, selection, compaction, flips.
+ * Device layouts: Q, Q0 [m][v]; catalog [k][m][v] with k over 2 planar
+ * parameters (gxx, gxy) (cat_kind 0) or d*d row-major entries (cat_kind 1);
+ * sigma symmetric SoA FP64 (sig_kind 0), FP32 (1) or d*d row-major FP64 (2);
+ * soft [2][v] = (g_p, g_t); tlast [v]; prev [d*d][v].
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  double volume;       /* barrier work volume (pixel^3)                        */
+  double rate_volume;  /* rate prefactor volume (1 unless scale_rate_by_volume) */
+  double nu0;          /* attempt frequency                                    */
+  double temperature;  /* K, used when Tfield == NULL                          */
+  double cap;          /* softening cap                                        */
+  double tau;          /* transient softening decay time (inf allowed)         */
+  double time;         /* current simulation time                              */
+  int scheme;          /* 0 isotropic, 1 directional                           */
+  int cascade;         /* 1: only q > 0 modes are events                       */
+} mgk_kmc_params;
+
+typedef struct {
+  long long item;   /* selected voxel (or array index)                        */
+  int mode;         /* selected mode within the voxel                         */
+  int fallthrough;  /* nonzero if the running sum never reached r             */
+  double r;         /* the draw U(0, total) actually used                     */
+  double cum_before, cum_at;  /* running sum before / including the event    */
+} mgk_kmc_sel;
+
+int mgk_kmc_workspace_bytes(long long nvox, size_t* bytes);
+
+/* Fused barrier + rate pass.  Replaces compute_barrier (mgkmc/stz/barriers.py:4-100;
+ * 2D kmc_simulator_functions.py:100-160) followed by compute_rates
+ * (mgkmc/stz/kmc.py:4-73; 2D kmc_simulator_functions.py:5-56): writes Q (if not
+ * NULL), per-voxel FP64 rate sums and the total (returned if total_out). */
+int mgk_kmc_barrier_rates(int ndim, long long nvox, int M, int sig_kind, int cat_kind, const void* sig,
+                          const double* Q0, const double* cat, const double* soft, const double* tlast,
+                          const double* prev, const double* Tfield, const mgk_kmc_params* params, double* Q,
+                          void* workspace, void* stream, double* total_out);
+
+/* Event selection after mgk_kmc_barrier_rates: r = total * u, first (voxel, mode)
+ * in (x,y,z,m) order whose running FP64 sum reaches r.  Replaces select_event +
+ * decode_index (mgkmc/stz/kmc.py:76-103; 2D kmc_simulator_functions.py:58-76). */
+int mgk_kmc_select(int ndim, long long nvox, int M, int sig_kind, int cat_kind, double u, const double* Q,
+                   const double* Tfield, const mgk_kmc_params* params, void* workspace, void* stream,
+                   mgk_kmc_sel* out);
+
+/* compute_rates on a Q field in the reference (v, m) layout: full per-mode
+ * rates (-1 = not an event), compacted rates/indices in reference order, total
+ * (mgkmc/stz/kmc.py:4-73). */
+int mgk_kmc_rates_from_q(long long nvox, int M, const double* Qvm, const double* Tfield,
+                         const mgk_kmc_params* params, double* rates_full, long long* idx_out, double* val_out,
+                         void* workspace, void* stream, long long* count_out, double* total_out);
+
+/* select_event over a rates array with a given draw r (mgkmc/stz/kmc.py:76-92). */
+int mgk_kmc_select_array(long long n, const double* rates, double r, void* workspace, void* stream,
+                         mgk_kmc_sel* out);
+
+/* Indices of a[i] < thr in order (find_unstable, mgkmc/stz/cascade.py:4-41). */
+int mgk_find_below(long long n, const double* a, double thr, long long* idx_out, void* stream,
+                   long long* count_out);
+
+/* apply_flip_soa (mgkmc/stz/cascade.py:43-113; 2D kmc_simulator_functions.py:183-218)
+ * on device state; also sets prev (if not NULL) to the mode, as the callers do
+ * (mgkmc/aqs.py:999, kmc_simulator.py:1026). */
+int mgk_kmc_flip(int ndim, const int* shape, int M, int cat_kind, long long v, int m, double time, double jp,
+                 double jt, double gmax, double jn, const double* cat, void* epsp, int precision, double* soft,
+                 double* tlast, double* prev, double* gamma_out, void* stream);
+
+/* Redraw of one voxel's modes (mgkmc/aqs.py:1007-1019): newcat [m][k], newq0 [m]. */
+int mgk_kmc_set_voxel(long long nvox, int M, int CK, long long v, const double* newcat, const double* newq0,
+                      double* cat, double* Q0, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,17 @@ class MgkLsInfo(ctypes.Structure):
                 ("eps_mean", ctypes.c_double * 9), ("sig_mean", ctypes.c_double * 9)]
 
 
+class MgkKmcParams(ctypes.Structure):
+    _fields_ = [("volume", ctypes.c_double), ("rate_volume", ctypes.c_double), ("nu0", ctypes.c_double),
+                ("temperature", ctypes.c_double), ("cap", ctypes.c_double), ("tau", ctypes.c_double),
+                ("time", ctypes.c_double), ("scheme", ctypes.c_int), ("cascade", ctypes.c_int)]
+
+
+class MgkKmcSel(ctypes.Structure):
+    _fields_ = [("item", ctypes.c_longlong), ("mode", ctypes.c_int), ("fallthrough", ctypes.c_int),
+                ("r", ctypes.c_double), ("cum_before", ctypes.c_double), ("cum_at", ctypes.c_double)]
+
+
 _c_p = ctypes.c_void_p
 _c_i = ctypes.c_int
 _c_d = ctypes.c_double
@@ -45,6 +56,19 @@ SIGNATURES = {
     "mgk_ls_workspace_bytes": (_c_i, [_c_i, _c_i64p, _c_i, ctypes.POINTER(_c_sz)]),
     "mgk_ls_solve": (_c_i, [_c_i, _c_i64p, _c_i, _c_p, _c_p, _c_d, _c_d, _c_dp, _c_p, _c_p, _c_p,
                             _c_i, _c_i, _c_d, _c_p, _c_sz, _c_p, ctypes.POINTER(MgkLsInfo)]),
+    "mgk_kmc_workspace_bytes": (_c_i, [ctypes.c_longlong, ctypes.POINTER(_c_sz)]),
+    "mgk_kmc_barrier_rates": (_c_i, [_c_i, ctypes.c_longlong, _c_i, _c_i, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p,
+                                     _c_p, _c_p, ctypes.POINTER(MgkKmcParams), _c_p, _c_p, _c_p, _c_dp]),
+    "mgk_kmc_select": (_c_i, [_c_i, ctypes.c_longlong, _c_i, _c_i, _c_i, _c_d, _c_p, _c_p,
+                              ctypes.POINTER(MgkKmcParams), _c_p, _c_p, ctypes.POINTER(MgkKmcSel)]),
+    "mgk_kmc_rates_from_q": (_c_i, [ctypes.c_longlong, _c_i, _c_p, _c_p, ctypes.POINTER(MgkKmcParams), _c_p,
+                                    _c_p, _c_p, _c_p, _c_p, ctypes.POINTER(ctypes.c_longlong), _c_dp]),
+    "mgk_kmc_select_array": (_c_i, [ctypes.c_longlong, _c_p, _c_d, _c_p, _c_p, ctypes.POINTER(MgkKmcSel)]),
+    "mgk_find_below": (_c_i, [ctypes.c_longlong, _c_p, _c_d, _c_p, _c_p, ctypes.POINTER(ctypes.c_longlong)]),
+    "mgk_kmc_flip": (_c_i, [_c_i, ctypes.POINTER(_c_i), _c_i, _c_i, ctypes.c_longlong, _c_i, _c_d, _c_d, _c_d,
+                            _c_d, _c_d, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "mgk_kmc_set_voxel": (_c_i, [ctypes.c_longlong, _c_i, _c_i, ctypes.c_longlong, _c_p, _c_p, _c_p, _c_p,
+                                 _c_p]),
 }
 
 

@@ -1,0 +1,158 @@
+"""GPU parity of the KMC kernels with the oracle: barriers and rates to the
+last bits of exp(), identical selections, compaction and flips, and an
+identical event sequence of the device-resident KMC loop."""
+import numpy as np
+import pytest
+
+from oracle import kmc as ok
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(shape, M, seed, d=3, general=False):
+    rng = np.random.default_rng(seed)
+    Q0 = rng.normal(2.2, 0.45, shape + (M,))
+    g = rng.standard_normal(shape + (d, d)) * 5e8
+    sig = 0.5 * (g + np.swapaxes(g, -1, -2))
+    if general:
+        cat = rng.standard_normal(shape + (M, d, d)) * 0.07
+    else:
+        gx = rng.standard_normal(shape + (M,)) * 0.07
+        gy = rng.standard_normal(shape + (M,)) * 0.07
+        cat = np.zeros(shape + (M, d, d))
+        cat[..., 0, 0], cat[..., 1, 1], cat[..., 0, 1], cat[..., 1, 0] = gx, -gx, gy, gy
+    soft = np.abs(rng.standard_normal(shape + (4 if d == 3 else 2,))) * 0.3
+    tl = rng.uniform(0, 1e-6, shape)
+    tl.reshape(-1)[::3] = -np.inf
+    prev = rng.standard_normal(shape + (d, d)) * 0.07
+    return Q0, sig, cat, soft, tl, prev
+
+
+def _close(a, b, rtol):
+    scale = np.maximum(np.abs(b), 1e-300)
+    return np.all(np.abs(a - b) <= rtol * scale)
+
+
+@pytest.mark.parametrize("d,scheme,tau,general", [(3, 0, np.inf, False), (3, 1, 3e-7, False), (3, 1, np.inf, True),
+                                                  (2, 0, np.inf, False), (2, 1, 5e-7, True)])
+def test_barrier_parity(gpu, d, scheme, tau, general):
+    from mgkmc_amd.stz.barriers import compute_barrier
+    from mgkmc_amd.kmc_simulator_functions import compute_barrier_2d
+    shape = (6, 5, 4) if d == 3 else (9, 7)
+    M = 20
+    Q0, sig, cat, soft, tl, prev = _state(shape, M, 1, d, general)
+    Qr = np.zeros(shape + (M,))
+    Qg = np.zeros(shape + (M,))
+    args = (0.343, soft, tl, 5e-7, prev, 0.916, scheme, tau)
+    if d == 3:
+        ok.compute_barrier(Qr, Q0, sig, cat, *args)
+        compute_barrier(Qg, Q0, sig, cat, *args)
+    else:
+        ok.compute_barrier_2d(Qr, Q0, sig, cat, *args)
+        compute_barrier_2d(Qg, Q0, sig, cat, *args)
+    # identical op order; only exp() may differ by an ulp
+    assert np.abs(Qg - Qr).max() <= 4e-16 * np.abs(Qr).max() * 8, np.abs(Qg - Qr).max()
+
+
+@pytest.mark.parametrize("cascade", [False, True])
+def test_rates_parity(gpu, cascade):
+    from mgkmc_amd.stz.kmc import compute_rates
+    rng = np.random.default_rng(2)
+    Q = rng.normal(0.8, 0.6, (7, 6, 5, 20))
+    r0, i0, t0 = ok.compute_rates(Q, 1.0, 1276.5, 1e13, cascade_mode=cascade)
+    r1, i1, t1 = compute_rates(Q, 1.0, 1276.5, 1e13, cascade_mode=cascade)
+    assert np.array_equal(i0, i1)
+    assert _close(r1, r0, 4e-15)
+    assert abs(t1 - t0) <= 1e-12 * t0
+
+
+def test_select_array_parity(gpu):
+    from mgkmc_amd.kmc_device import select_host
+    rng = np.random.default_rng(3)
+    rates = np.exp(rng.normal(0, 3, 50000))
+    rates[::7] = 0.0
+    tot = float(np.cumsum(rates)[-1])
+    us = np.concatenate([rng.random(200), [0.0, 1e-12, 0.5, 1 - 1e-15]])
+    for u in us:
+        r = tot * u
+        assert select_host(rates, r) == ok.select_event(rates, tot, r), u
+
+
+def test_find_unstable_parity(gpu):
+    from mgkmc_amd.stz.cascade import find_unstable
+    rng = np.random.default_rng(4)
+    Q = rng.normal(0.5, 0.5, (9, 8, 7, 20))
+    assert np.array_equal(find_unstable(Q, 0.0), ok.find_unstable(Q, 0.0))
+    assert find_unstable(Q, -100.0).shape == (0, 4)
+
+
+@pytest.mark.parametrize("jn", [0.0, 0.3])
+def test_flip_parity(gpu, jn):
+    from mgkmc_amd.stz.cascade import apply_flip_soa
+    shape = (5, 4, 3)
+    M = 6
+    Q0, sig, cat, soft, tl, prev = _state(shape, M, 5)
+    ep = np.zeros(shape + (3, 3))
+    a = (ep.copy(), None, soft.copy(), tl.copy())
+    b = (ep.copy(), None, soft.copy(), tl.copy())
+    ok.apply_flip_soa(*a, cat, 4, 0, 2, 3, 1e-6, 20.0, 3.0, 0.916, jn)
+    apply_flip_soa(*b, cat, 4, 0, 2, 3, 1e-6, 20.0, 3.0, 0.916, jn)
+    assert np.array_equal(a[0], b[0])
+    assert np.allclose(a[2][..., :2], b[2][..., :2], rtol=1e-15, atol=0)
+    assert np.array_equal(a[3], b[3])
+
+
+def test_fused_select_matches_oracle(gpu):
+    # device barrier+rate pass and two-level selection vs the oracle's
+    # compute_barrier -> compute_rates -> select_event on the same draws
+    import torch
+    from mgkmc_amd.kmc_device import KmcState, params, SIG_GEN64
+    from mgkmc_amd import device as dev
+    shape = (10, 9, 8)
+    M = 20
+    Q0, sig, cat, soft, tl, prev = _state(shape, M, 6)
+    st = KmcState(shape, M, Q0, cat, soft_prop=soft, last_event_time=tl, prev_strain_dir=prev, directional=True)
+    sig_d = dev.upload(np.ascontiguousarray(sig.reshape(-1, 9).T), 0)
+    T = 1276.5
+    p = params(volume=0.343, temperature=T, cap=0.916, tau=3e-7, time=5e-7, scheme=1)
+    total = st.barrier_rates(sig_d, SIG_GEN64, p)
+    Q = np.zeros(shape + (M,))
+    ok.compute_barrier(Q, Q0, sig, cat, 0.343, soft, tl, 5e-7, prev, 0.916, 1, 3e-7)
+    rates, idx, tot = ok.compute_rates(Q, 1.0, T, 1e13, cascade_mode=False)
+    assert abs(total - tot) <= 1e-12 * tot
+    rng = np.random.default_rng(7)
+    for u in list(rng.random(300)) + [0.0, 0.999999999]:
+        v, m = st.select(u, SIG_GEN64, p)
+        i = ok.select_event(rates, tot, tot * u)
+        assert (v * M + m) == idx[i], (u, v, m, idx[i])
+
+
+def test_kmc_event_sequence_matches_oracle(gpu):
+    # device-resident loop vs the oracle loop: same event list for fixed seeds
+    from mgkmc_amd.aqs import ThermalSimulation
+    from mgkmc_amd.stz import kmc as kmod
+    from oracle.aqs_loop import run_events
+    n = 8
+    M = 20
+    rng = np.random.default_rng(1)
+    mu = 26.923e9 * rng.uniform(0.9, 1.1, (n, n, n))
+    nu = np.full((n, n, n), 0.35)
+    E = 2 * mu * (1 + nu)
+    T = 1276.5
+    kw = dict(mean=2.2, std=0.45, min_cutoff=0.0)
+    ev_ref, st_ref = run_events(n, n, n, M, 0.14, E, nu, 0.7, T, 1e7, 1e-4, (0, 0), 40, seed=11, sel_seed=12,
+                                barrier_kwargs=kw, softening_cap=0.916, jp=20.0, jt=0.0)
+    np.random.seed(11)
+    kmod.seed_selection(12)
+    sim = ThermalSimulation(n, n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                            softening_cap=0.916, jp=20.0, jt=0.0, temperature=T, strain_rate=1e7)
+    ev = sim.run_simulation(10 ** 9, 1e-4, component=(0, 0), max_events=40)
+    got = [e[:4] for e in ev]
+    ref = [e[:4] for e in ev_ref]
+    print("events:", got[:10], flush=True)
+    assert got == ref
+    assert np.allclose([e[4] for e in ev], [e[4] for e in ev_ref], rtol=1e-9)
+    ep = sim.eps_plastic
+    assert np.allclose(ep, st_ref["eps_plastic"], rtol=0, atol=1e-15)
+    sig = sim.sig_field
+    assert np.abs(sig - st_ref["sig"]).max() <= 1e-9 * np.abs(st_ref["sig"]).max()

@@ -172,3 +172,20 @@ def test_selection_stream_contract():
     # algorithm's authors as genrand_int32 = 3499211612)
     rs = np.random.RandomState(5489)
     assert rs.randint(0, 2 ** 32, dtype=np.uint64) == 3499211612
+
+
+def test_vectorised_draws_match_reference_loop():
+    # the device driver draws a voxel's modes with one normal(size=2M) call;
+    # the reference loops normal() per component: same legacy-stream sequence
+    from mgkmc_amd.aqs import planar_catalog, planar_to_full
+    np.random.seed(3)
+    loop = ok.stz_catalog_glass(7, 0.14, np.random)
+    tail_loop = np.random.uniform()
+    np.random.seed(3)
+    vec = planar_to_full(planar_catalog(7, 0.14))
+    tail_vec = np.random.uniform()
+    assert np.array_equal(loop, vec) and tail_loop == tail_vec
+    np.random.seed(4)
+    a = [np.random.normal() for _ in range(11)]
+    np.random.seed(4)
+    assert a == list(np.random.normal(size=11))
