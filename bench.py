@@ -72,6 +72,45 @@ def gpu_run(n, steps, warmup):
     return out
 
 
+def gpu_kmc(n, events, warmup, precision):
+    """KMC steps/s at n^3: barrier + rates + selection + flip + redraw + LS
+    re-equilibration per event (config-1 recipe: kT/E0 = 0.05 -> 1276.5 K,
+    M = 20, gamma0 = 0.14, pixel 0.7 nm, gaussian barriers 2.2 +- 0.45 eV)."""
+    import torch
+    from mgkmc_amd import _lib
+    from mgkmc_amd.aqs import ThermalSimulation
+    from mgkmc_amd.stz import kmc as kmod
+    E, nu, _ = fields(n)
+    np.random.seed(1)
+    kmod.seed_selection(1)
+    sim = ThermalSimulation(n, n, n, 20, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian",
+                            barrier_kwargs=dict(mean=2.2, std=0.45, min_cutoff=0.0), softening_cap=0.916,
+                            jp=20.0, jt=0.0, temperature=0.05 * 2.2 / 8.617e-5, strain_rate=1e7,
+                            ls_precision=precision)
+    sim.update_stresses()
+    unit = np.zeros((3, 3))
+    unit[0, 0] = 1.0
+    rem = 1e30
+    for _ in range(warmup):
+        sim.kmc_event(rem, unit)
+    torch.cuda.synchronize()
+    lib = _lib.load()
+    lib.mgk_profile_reset()
+    lib.mgk_profile_enable(1)
+    iters = 0
+    t0 = time.perf_counter()
+    for _ in range(events):
+        sim.kmc_event(rem, unit)
+        iters += sim.ls_info.iters
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    lib.mgk_profile_enable(0)
+    prof = _lib.profile_read()
+    del sim
+    torch.cuda.empty_cache()
+    return {"kmc_steps_per_s": events / dt, "ls_iters_per_event": iters / events, "prof": prof}
+
+
 def cpu_baseline(n_sample=128, iters=3):
     """The oracle (reference structure: materialised Gamma, per-component
     complex FFTs) on this host: LS iterations on a bounded 128^3 sample."""
@@ -101,6 +140,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kmc-events", type=int, default=20)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); multi-GPU sharding is design-only "
@@ -108,7 +148,12 @@ def main():
         sys.exit(2)
     res = gpu_run(a.n, a.steps, a.warmup)
     r32, r64 = res[1], res[0]
-    prof = r32["prof"] or {}
+    prof = dict(r32["prof"] or {})
+    kmc32 = gpu_kmc(a.n, a.kmc_events, 2, "fp32")
+    kmc64 = gpu_kmc(a.n, a.kmc_events, 2, "fp64")
+    for k in ("kmc_barrier_rates", "kmc_select"):
+        if k in kmc32["prof"]:
+            prof[k] = kmc32["prof"][k]
     dom = max(prof.items(), key=lambda kv: kv[1][0]) if prof else None
     roof = None
     if dom:
@@ -139,6 +184,9 @@ def main():
         "ls_fp64_voxel_iter_per_s": r64["voxel_iter_per_s"],
         "fp32_over_fp64": r32["voxel_iter_per_s"] / r64["voxel_iter_per_s"],
         "iters_per_solve": {"fp32": r32["iters"] / steps, "fp64": r64["iters"] / steps},
+        "kmc_steps_per_s": {"fp32_ls": kmc32["kmc_steps_per_s"], "fp64_ls": kmc64["kmc_steps_per_s"],
+                            "events": a.kmc_events, "ls_iters_per_event": kmc32["ls_iters_per_event"],
+                            "workload": f"{a.n}^3, M=20, kT/E0=0.05, cold-start LS tol 1e-6 per event"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -1,0 +1,191 @@
+"""Pin the finite-strain oracle (oracle/finite_strain.py) and the matrix-free
+tangent action the GPU implements, against finite differences, the Mandel /
+einsum equivalence of the reference, projector identities and closed forms."""
+import numpy as np
+import pytest
+
+from oracle import finite_strain as fs
+
+LANDAU = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9)
+
+
+def _setup(sh, seed=0, amp=0.02):
+    rng = np.random.default_rng(seed)
+    E = 70e9 * rng.uniform(0.8, 1.2, sh)
+    nu = np.full(sh, 0.32)
+    I2, I4, I4rt, I4s, II = fs.identity_tensors(*sh)
+    C4 = fs.build_C4(E, nu, I4s, II)
+    F = I2 + amp * rng.standard_normal((3, 3) + sh)
+    return rng, E, nu, I2, I4, I4rt, C4, F
+
+
+def _matvec(K4, dF):
+    return np.einsum("jiklxyz,klxyz->ijxyz", K4, dF)
+
+
+def matrix_free_dP(F, dF, lam, mu, model, Fp=None, mat=None, capping=None):
+    """The GPU's tangent action in numpy: dP = dFe S Fpi^T + Fe (C:dE) Fpi^T,
+    dE = sym(Fe^T dF Fpi); the reference's Fp branch returns its transpose."""
+    mat = mat or {}
+    Fpi = fs.invert_Fp(Fp) if Fp is not None else None
+    Fe = np.einsum("ijxyz,jkxyz->ikxyz", F, Fpi) if Fp is not None else F
+    dFe = np.einsum("ijxyz,jkxyz->ikxyz", dF, Fpi) if Fp is not None else dF
+    I = np.eye(3)[:, :, None, None, None]
+    E = 0.5 * (np.einsum("jixyz,jkxyz->ikxyz", Fe, Fe) - I)
+    X = np.einsum("jixyz,jkxyz->ikxyz", Fe, dFe)
+    X = 0.5 * (X + np.swapaxes(X, 0, 1))
+    trX = X[0, 0] + X[1, 1] + X[2, 2]
+    if model == "svk":
+        trE = E[0, 0] + E[1, 1] + E[2, 2]
+        S = lam * trE * I + 2 * mu * E
+        dS = lam * trX * I + 2 * mu * X
+    elif model == "landau":
+        v1, v2, v3, g1, g2, g3, g4 = (mat[k] for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4"))
+        trE = E[0, 0] + E[1, 1] + E[2, 2]
+        E2 = np.einsum("ijxyz,jkxyz->ikxyz", E, E)
+        trE2 = np.einsum("ijxyz,jixyz->xyz", E, E)
+        trE3 = np.einsum("ijxyz,jkxyz,kixyz->xyz", E, E, E)
+        A = lam * trE + 0.5 * v1 * trE ** 2 + v2 * trE2 + g1 / 6 * trE ** 3 + g2 * trE * trE2 + 4 / 3 * g3 * trE3
+        B = 2 * (mu + v2 * trE + 0.5 * g2 * trE ** 2 + g4 * trE2)
+        Cc = 4 * (v3 + g3 * trE)
+        S = A * I + B * E + Cc * E2
+        al = lam + v1 * trE + 0.5 * g1 * trE ** 2 + g2 * trE2
+        be = 2 * (v2 + g2 * trE)
+        ga = 4 * g3
+        EX = np.einsum("ijxyz,ijxyz->xyz", E, X)
+        E2X = np.einsum("ijxyz,ijxyz->xyz", E2, X)
+        dS = (al * trX * I + be * (trX * E + EX * I) + ga * (trX * E2 + E2X * I) + 4 * g4 * EX * E + B * X
+              + Cc * (np.einsum("ijxyz,jkxyz->ikxyz", E, X) + np.einsum("ijxyz,jkxyz->ikxyz", X, E)))
+    else:
+        raise ValueError(model)
+    dP = np.einsum("ijxyz,jkxyz->ikxyz", dFe, S) + np.einsum("ijxyz,jkxyz->ikxyz", Fe, dS)
+    if Fp is not None:
+        dP = np.einsum("ijxyz,kjxyz->ikxyz", dP, Fpi)
+        dP = np.swapaxes(dP, 0, 1)
+    return dP
+
+
+@pytest.mark.parametrize("model", ["svk", "landau", "murnaghan"])
+def test_tangent_is_derivative_without_fp(model):
+    sh = (2, 3, 2)
+    rng, E, nu, I2, I4, I4rt, C4, F = _setup(sh)
+    mat = dict(LANDAU) if model == "landau" else (dict(A_m=-100e9, B_m=-100e9, C_m=-100e9) if model == "murnaghan"
+                                                   else {})
+    P, K4 = fs.constitutive(F, C4, I2, I4, I4rt, model_type=model, **mat)
+    dF = rng.standard_normal((3, 3) + sh)
+    h = 1e-6
+    Pp, _ = fs.constitutive(F + h * dF, C4, I2, I4, I4rt, model_type=model, need_tangent=False, **mat)
+    Pm, _ = fs.constitutive(F - h * dF, C4, I2, I4, I4rt, model_type=model, need_tangent=False, **mat)
+    fd = (Pp - Pm) / (2 * h)
+    mv = _matvec(K4, dF)
+    assert np.abs(mv - fd).max() < 1e-6 * np.abs(fd).max()
+
+
+@pytest.mark.parametrize("capping,ctype", [(False, "piecewise"), (True, "piecewise"), (True, "smooth")])
+def test_mandel_equals_einsum_tangent(capping, ctype):
+    # reference tests/test_landau_tangent_mandel.py:65-131 (< 1e-10 relative)
+    sh = (3, 2, 2)
+    rng, E, nu, I2, I4, I4rt, C4, F = _setup(sh, amp=0.08)
+    kw = dict(model_type="landau", strain_capping_enabled=capping, strain_capping_type=ctype,
+              strain_capping_limit=0.05, **LANDAU)
+    P1, K1 = fs.constitutive(F, C4, I2, I4, I4rt, tangent_method="mandel", **kw)
+    P2, K2 = fs.constitutive(F, C4, I2, I4, I4rt, tangent_method="reference", **kw)
+    assert np.abs(P1 - P2).max() <= 1e-12 * np.abs(P2).max()
+    assert np.abs(K1 - K2).max() <= 1e-10 * np.abs(K2).max()
+
+
+def test_landau_reduces_to_svk():
+    sh = (2, 2, 2)
+    rng, E, nu, I2, I4, I4rt, C4, F = _setup(sh)
+    P1, K1 = fs.constitutive(F, C4, I2, I4, I4rt, model_type="landau")
+    P2, K2 = fs.constitutive(F, C4, I2, I4, I4rt, model_type="svk")
+    assert np.allclose(P1, P2, rtol=1e-12, atol=1e-3)
+    assert np.allclose(_matvec(K1, F), _matvec(K2, F), rtol=1e-10, atol=1e-2)
+
+
+@pytest.mark.parametrize("model,with_fp", [("svk", False), ("landau", False), ("svk", True), ("landau", True)])
+def test_matrix_free_action_equals_assembled_k4(model, with_fp):
+    sh = (3, 2, 2)
+    rng, E, nu, I2, I4, I4rt, C4, F = _setup(sh, seed=3)
+    mat = dict(LANDAU) if model == "landau" else {}
+    Fp = I2 + 0.05 * rng.standard_normal((3, 3) + sh) if with_fp else None
+    P, K4 = fs.constitutive(F, C4, I2, I4, I4rt, Fp=Fp, model_type=model, **mat)
+    dF = rng.standard_normal((3, 3) + sh)
+    lam = C4[0, 0, 1, 1]
+    mu = C4[0, 1, 0, 1]
+    mf = matrix_free_dP(F, dF, lam, mu, model, Fp=Fp, mat=mat)
+    mv = _matvec(K4, dF)
+    assert np.abs(mf - mv).max() <= 1e-10 * np.abs(mv).max()
+
+
+def test_fp_branch_is_transposed_derivative():
+    # documents the reference behaviour mirrored by the GPU: with Fp the
+    # assembled K4 under the (K4 : dF^T)^T matvec gives (dP/dF . dF)^T
+    sh = (2, 2, 2)
+    rng, E, nu, I2, I4, I4rt, C4, F = _setup(sh, seed=4)
+    Fp = I2 + 0.05 * rng.standard_normal((3, 3) + sh)
+    P, K4 = fs.constitutive(F, C4, I2, I4, I4rt, Fp=Fp, model_type="svk")
+    dF = rng.standard_normal((3, 3) + sh)
+    h = 1e-6
+    Pp, _ = fs.constitutive(F + h * dF, C4, I2, I4, I4rt, Fp=Fp, model_type="svk", need_tangent=False)
+    Pm, _ = fs.constitutive(F - h * dF, C4, I2, I4, I4rt, Fp=Fp, model_type="svk", need_tangent=False)
+    fd = (Pp - Pm) / (2 * h)
+    mv = _matvec(K4, dF)
+    assert np.abs(mv - np.swapaxes(fd, 0, 1)).max() < 1e-6 * np.abs(fd).max()
+
+
+def test_ghat_projector_identities():
+    n = (6, 5, 4)
+    G = fs.build_ghat4(*n, 1.0, 1.0, 1.0)  # unit box, matching the field below
+    GG = np.einsum("ijklxyz,lkmnxyz->ijmnxyz", G, G)
+    assert np.allclose(GG, G, atol=1e-14)
+    # a smooth compatible field du/dx_j with zero mean is left unchanged
+    x = [np.arange(k) / k for k in n]
+    X = np.meshgrid(*x, indexing="ij")
+    u = [np.sin(2 * np.pi * X[0]) * np.cos(2 * np.pi * X[2]), np.cos(2 * np.pi * X[1]), np.sin(2 * np.pi * X[2])]
+    # spectral gradient (no Nyquist content for these modes)
+    grad = np.zeros((3, 3) + n)
+    for i in range(3):
+        uh = np.fft.fftn(u[i])
+        for j in range(3):
+            k = 2j * np.pi * np.fft.fftfreq(n[j], d=1.0 / n[j])
+            sh = [1, 1, 1]
+            sh[j] = n[j]
+            grad[i, j] = np.fft.ifftn(uh * k.reshape(sh)).real
+    assert np.allclose(fs.project(grad, G), grad, atol=1e-12)
+
+
+def test_homogeneous_simple_shear_closed_form():
+    sh = (4, 4, 4)
+    E = np.full(sh, 70e9)
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = fs.identity_tensors(*sh)
+    C4 = fs.build_C4(E, nu, I4s, II)
+    G = fs.build_ghat4(*sh, *sh)
+    Fbar = np.eye(3)
+    Fbar[0, 1] = 0.05
+    F, P, Sig, K4, Fb = fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, tol_NW=1e-12, tol_CG=1e-12)
+    assert np.allclose(F, Fbar[:, :, None, None, None], atol=1e-13)
+    lam, mu = 70e9 * 0.3 / (1.3 * 0.4), 70e9 / 2.6
+    Eg = 0.5 * (Fbar.T @ Fbar - np.eye(3))
+    S = lam * np.trace(Eg) * np.eye(3) + 2 * mu * Eg
+    assert np.allclose(P[:, :, 0, 0, 0], Fbar @ S, rtol=1e-12)
+
+
+def test_heterogeneous_solution_is_equilibrated():
+    sh = (6, 4, 5)
+    rng = np.random.default_rng(5)
+    E = 70e9 * rng.uniform(0.8, 1.2, sh)
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = fs.identity_tensors(*sh)
+    C4 = fs.build_C4(E, nu, I4s, II)
+    G = fs.build_ghat4(*sh, *sh)
+    Fbar = np.eye(3)
+    Fbar[0, 1] = 0.02
+    # tolerances of the 256^3 workload (CG 1e-8); tighter ones hit the
+    # BiCGSTAB round-off floor and make the reference's line search fail
+    F, P, Sig, K4, Fb = fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, model_type="landau",
+                                       tol_NW=1e-8, tol_CG=1e-8, **LANDAU)
+    assert np.allclose(F.mean(axis=(2, 3, 4)), Fbar, atol=1e-12)
+    GP = fs.project(P, G)
+    assert np.abs(GP).max() < 1e-8 * np.abs(P).max()
