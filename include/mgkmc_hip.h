@@ -140,6 +140,49 @@ int mgk_kmc_flip(int ndim, const int* shape, int M, int cat_kind, long long v, i
 int mgk_kmc_set_voxel(long long nvox, int M, int CK, long long v, const double* newcat, const double* newq0,
                       double* cat, double* Q0, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Finite strain (de Geus Newton-Krylov).  Fields: 9 components F[3*i+j][v].
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int model;       /* 0 svk, 1 landau, 2 murnaghan                          */
+  int capping;     /* landau strain capping on/off                          */
+  int cap_smooth;  /* 1 smooth (tanh), 0 piecewise                          */
+  double v1, v2, v3, g1, g2, g3, g4;  /* Landau constants in Pa             */
+  double Am, Bm, Cm;                  /* Murnaghan constants                */
+  double cap_limit;                   /* resolved capping limit E_cap       */
+  double cap_power, cap_ratio;        /* smooth power, tangent ratio        */
+} mgk_fs_material;
+
+typedef struct {
+  int status;        /* MGK_OK or MGK_ERR_SOLVER                             */
+  int fail_kind;     /* 1 line search, 2 Newton, 3 macro BC loop             */
+  int newton_iters, krylov_iters, macro_iters;
+  double max_err;    /* last macro stress error (Pa)                         */
+  double F_mean[9], sig_mean[9], F_bar[9];
+} mgk_fs_info;
+
+int mgk_fs_workspace_bytes(const int64_t* shape, int precision, size_t* bytes);
+
+/* One load step: replaces finite_strain_solver_step_3d, solver="newton_cg"
+ * (mgkmc/finite_strain_simulator.py:2967-3168).  F in/out (device, 9 comps);
+ * Fp optional; lam/mu per voxel (the reference reads C4[0,0,1,1], C4[0,1,0,1]);
+ * F_bar, P_target row-major 9 (host); P_mask 9 ints (host).  BiCGSTAB follows
+ * SciPy 1.15.3 (x0 = 0, maxiter 150, rtol = tol_CG). */
+int mgk_fs_solver_step(const int64_t* shape, int precision, void* F, const void* Fp, const void* lam,
+                       const void* mu, const mgk_fs_material* mat, const double* F_bar, const double* P_target,
+                       const int* P_mask, int i_drv, int j_drv, double E_avg, double nu_avg, double tol_NW,
+                       double tol_CG, int max_NW, double tol_macro, int max_iter_macro, void* P_out,
+                       void* sig_out, void* workspace, size_t workspace_bytes, void* stream, mgk_fs_info* info);
+
+/* P(F) and optionally dP = K:dF (reference matvec convention) per voxel
+ * (constitutive_hyperelastic_3d, mgkmc/finite_strain_simulator.py:1647-1885). */
+int mgk_fs_constitutive(long long nvox, int precision, const void* F, const void* Fp, const void* lam,
+                        const void* mu, const mgk_fs_material* mat, void* P, const void* dF, void* dP,
+                        void* stream);
+
+/* sigma = P F^T / det F (cauchy_from_P_3d, mgkmc/finite_strain_simulator.py:1892-1904). */
+int mgk_fs_cauchy(long long nvox, int precision, const void* P, const void* F, void* sig, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,6 +38,18 @@ class MgkKmcSel(ctypes.Structure):
                 ("r", ctypes.c_double), ("cum_before", ctypes.c_double), ("cum_at", ctypes.c_double)]
 
 
+class MgkFsMaterial(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int), ("capping", ctypes.c_int), ("cap_smooth", ctypes.c_int)] + \
+               [(k, ctypes.c_double) for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4", "Am", "Bm", "Cm",
+                                               "cap_limit", "cap_power", "cap_ratio")]
+
+
+class MgkFsInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("fail_kind", ctypes.c_int), ("newton_iters", ctypes.c_int),
+                ("krylov_iters", ctypes.c_int), ("macro_iters", ctypes.c_int), ("max_err", ctypes.c_double),
+                ("F_mean", ctypes.c_double * 9), ("sig_mean", ctypes.c_double * 9), ("F_bar", ctypes.c_double * 9)]
+
+
 _c_p = ctypes.c_void_p
 _c_i = ctypes.c_int
 _c_d = ctypes.c_double
@@ -69,6 +81,13 @@ SIGNATURES = {
                             _c_d, _c_d, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "mgk_kmc_set_voxel": (_c_i, [ctypes.c_longlong, _c_i, _c_i, ctypes.c_longlong, _c_p, _c_p, _c_p, _c_p,
                                  _c_p]),
+    "mgk_fs_workspace_bytes": (_c_i, [_c_i64p, _c_i, ctypes.POINTER(_c_sz)]),
+    "mgk_fs_solver_step": (_c_i, [_c_i64p, _c_i, _c_p, _c_p, _c_p, _c_p, ctypes.POINTER(MgkFsMaterial), _c_dp,
+                                  _c_dp, ctypes.POINTER(_c_i), _c_i, _c_i, _c_d, _c_d, _c_d, _c_d, _c_i, _c_d, _c_i,
+                                  _c_p, _c_p, _c_p, _c_sz, _c_p, ctypes.POINTER(MgkFsInfo)]),
+    "mgk_fs_constitutive": (_c_i, [ctypes.c_longlong, _c_i, _c_p, _c_p, _c_p, _c_p, ctypes.POINTER(MgkFsMaterial),
+                                   _c_p, _c_p, _c_p, _c_p]),
+    "mgk_fs_cauchy": (_c_i, [ctypes.c_longlong, _c_i, _c_p, _c_p, _c_p, _c_p]),
 }
 
 

@@ -1,0 +1,1468 @@
+// De Geus finite-strain Newton-Krylov solve on gfx950.
+//
+// Reference: mgkmc/finite_strain_simulator.py:2967-3168
+// (finite_strain_solver_step_3d, solver="newton_cg"), constitutive model
+// :1647-1885 (svk / landau with capping / murnaghan, optional Fp), projection
+// :1121-1188, and SciPy 1.15.3 bicgstab (scipy/sparse/linalg/_isolve/
+// iterative.py:157-260) which the reference calls.
+//
+// Matrix-free design: no K4 (81 comps) and no G-hat (81 comps) in HBM.
+//   * k_fs_dP       : tangent action dP = dFe S Fpi^T + Fe (C:dE) Fpi^T per voxel,
+//                     rebuilt from F in registers (with Fp the reference's
+//                     assembled K4 yields the transpose of this; mirrored).
+//   * k_fs_xr2c     : 9 real comps -> half spectra along x (pairs packed).
+//   * k_fft_mid     : y axis.
+//   * k_fs_proj     : z pencils, (G:A)_ij = (A xi)_i xi_j / |xi|^2, zero at DC and
+//                     on even-axis Nyquist planes; rows of A are independent.
+//   * k_fs_xc2r     : back to real comps, fused FP64 dot products for BiCGSTAB.
+// BiCGSTAB scalars (rho, alpha, omega, beta) and the stop tests live in device
+// memory in FP64; every kernel of an iteration exits at once after a stop.
+#include "mgk_common.h"
+#include "mgk_host.h"
+#include "mgk_fftpass.h"
+#include <algorithm>
+#include <vector>
+#include <string.h>
+#include <cmath>
+
+struct FsMat {
+  int model;  // 0 svk, 1 landau, 2 murnaghan
+  int capping;
+  int cap_smooth;
+  int pad;
+  double v1, v2, v3, g1, g2, g3, g4;
+  double Am, Bm, Cm;
+  double Ecap, cap_p, cap_ratio;
+};
+
+template <typename T>
+struct FsVox {
+  T Fe[9], S[9], E[9], E2[9], Fpi[9];
+  T lam, mu, trE, al, be, ga, B, C, cw;
+};
+
+template <typename T>
+__device__ __forceinline__ void mm3(const T* A, const T* B, T* C) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) C[3 * i + k] = A[3 * i] * B[k] + A[3 * i + 1] * B[3 + k] + A[3 * i + 2] * B[6 + k];
+}
+// C = A^T B
+template <typename T>
+__device__ __forceinline__ void mtm3(const T* A, const T* B, T* C) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) C[3 * i + k] = A[i] * B[k] + A[3 + i] * B[3 + k] + A[6 + i] * B[6 + k];
+}
+// C = A B^T
+template <typename T>
+__device__ __forceinline__ void mmt3(const T* A, const T* B, T* C) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) C[3 * i + k] = A[3 * i] * B[3 * k] + A[3 * i + 1] * B[3 * k + 1] + A[3 * i + 2] * B[3 * k + 2];
+}
+template <typename T>
+__device__ __forceinline__ T det3(const T* F) {
+  return F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) + F[2] * (F[3] * F[7] - F[4] * F[6]);
+}
+// pointwise inverse with the reference's det guard (_invert_Fp_3d)
+template <typename T>
+__device__ __forceinline__ void inv3(const T* A, T* R) {
+  T d = det3(A);
+  if (fabs((double)d) < 1e-14) d = T(1e-14);
+  R[0] = (A[4] * A[8] - A[5] * A[7]) / d;
+  R[1] = -(A[1] * A[8] - A[2] * A[7]) / d;
+  R[2] = (A[1] * A[5] - A[2] * A[4]) / d;
+  R[3] = -(A[3] * A[8] - A[5] * A[6]) / d;
+  R[4] = (A[0] * A[8] - A[2] * A[6]) / d;
+  R[5] = -(A[0] * A[5] - A[2] * A[3]) / d;
+  R[6] = (A[3] * A[7] - A[4] * A[6]) / d;
+  R[7] = -(A[0] * A[7] - A[1] * A[6]) / d;
+  R[8] = (A[0] * A[4] - A[1] * A[3]) / d;
+}
+
+// stress P and the tangent coefficients at deformation F
+template <typename T>
+__device__ void fs_state(const T* F, const T* Fpi, T lam, T mu, const FsMat& m, FsVox<T>& s, T* P) {
+  s.lam = lam;
+  s.mu = mu;
+  if (Fpi) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s.Fpi[k] = Fpi[k];
+    mm3(F, s.Fpi, s.Fe);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s.Fe[k] = F[k];
+  }
+  T Ce[9];
+  mtm3(s.Fe, s.Fe, Ce);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) s.E[k] = T(0.5) * (Ce[k] - ((k % 4 == 0) ? T(1) : T(0)));
+  s.al = s.be = s.ga = s.B = s.C = s.cw = T(0);
+  if (m.model == 0) {
+    const T trE = s.E[0] + s.E[4] + s.E[8];
+    s.trE = trE;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s.S[k] = T(2) * mu * s.E[k] + ((k % 4 == 0) ? lam * trE : T(0));
+  } else if (m.model == 1) {
+    T Eorig[9];
+    T w = T(0);
+    if (m.capping) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Eorig[k] = s.E[k];
+      const T tr0 = s.E[0] + s.E[4] + s.E[8];
+      const T t3 = tr0 / T(3);
+      T Ed[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Ed[k] = s.E[k] - ((k % 4 == 0) ? t3 : T(0));
+      T dd = T(0);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) dd += Ed[3 * i + j] * Ed[3 * j + i];
+      const T Eeq = sqrt(fmax(T(0), (T(2) / T(3)) * dd));
+      const T Ecap = T(m.Ecap);
+      T f = T(1);
+      if (m.cap_smooth) {
+        const T p = T(m.cap_p);
+        if (Eeq > T(1e-12)) {
+          const T rp = fmin(T(100), pow(Eeq / Ecap, p));
+          f = (Ecap / Eeq) * pow(tanh(rp), T(1) / p);
+        }
+        const T tw = tanh(fmin(T(100), pow(Eeq / Ecap, p)));
+        w = tw * tw;
+      } else {
+        if (Eeq > Ecap) {
+          f = Ecap / Eeq;
+          w = T(1);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 9; ++k) s.E[k] = Ed[k] * f + ((k % 4 == 0) ? t3 : T(0));
+    }
+    const T* E = s.E;
+    mm3(E, E, s.E2);
+    const T trE = E[0] + E[4] + E[8];
+    T trE2 = T(0), trE3 = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) trE2 += E[3 * i + j] * E[3 * j + i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) trE3 += s.E2[3 * i + k] * E[3 * k + i];
+    const T v1 = T(m.v1), v2 = T(m.v2), v3 = T(m.v3), g1 = T(m.g1), g2 = T(m.g2), g3 = T(m.g3), g4 = T(m.g4);
+    const T A = lam * trE + T(0.5) * v1 * trE * trE + v2 * trE2 + (T(1) / T(6)) * g1 * trE * trE * trE +
+                g2 * trE * trE2 + (T(4) / T(3)) * g3 * trE3;
+    const T B = T(2) * (mu + v2 * trE + T(0.5) * g2 * trE * trE + g4 * trE2);
+    const T C = T(4) * (v3 + g3 * trE);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s.S[k] = ((k % 4 == 0) ? A : T(0)) + B * E[k] + C * s.E2[k];
+    if (m.capping) {
+      const T Gt = T(m.cap_ratio) * mu;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) s.S[k] += T(2) * Gt * (Eorig[k] - E[k]) * w;
+      s.cw = T(2) * Gt * w;
+    }
+    s.trE = trE;
+    s.al = lam + v1 * trE + T(0.5) * g1 * trE * trE + g2 * trE2;
+    s.be = T(2) * (v2 + g2 * trE);
+    s.ga = T(4) * g3;
+    s.B = B;
+    s.C = C;
+  } else {
+    const T* E = s.E;
+    mm3(E, E, s.E2);
+    const T trE = E[0] + E[4] + E[8];
+    T trE2 = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) trE2 += E[3 * i + j] * E[3 * j + i];
+    const T Am = T(m.Am), Bm = T(m.Bm), Cm = T(m.Cm);
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      s.S[k] = ((k % 4 == 0) ? (lam * trE + Am * trE * trE + Bm * trE2) : T(0)) + T(2) * (mu + Bm * trE) * E[k] +
+               Cm * s.E2[k];
+    s.trE = trE;
+  }
+  if (P) {
+    if (Fpi) {
+      T FS[9];
+      mm3(s.Fe, s.S, FS);
+      mmt3(FS, s.Fpi, P);
+    } else {
+      mm3(s.Fe, s.S, P);
+    }
+  }
+}
+
+// dP for direction dF at the state s
+template <typename T>
+__device__ void fs_tangent(const FsVox<T>& s, const FsMat& m, const T* dF, bool hasFp, T* dP) {
+  T dFe[9];
+  if (hasFp) mm3(dF, s.Fpi, dFe);
+  else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dFe[k] = dF[k];
+  }
+  T Y[9], X[9];
+  mtm3(s.Fe, dFe, Y);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) X[3 * i + j] = T(0.5) * (Y[3 * i + j] + Y[3 * j + i]);
+  const T trX = X[0] + X[4] + X[8];
+  T dS[9];
+  if (m.model == 0) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dS[k] = T(2) * s.mu * X[k] + ((k % 4 == 0) ? s.lam * trX : T(0));
+  } else if (m.model == 1) {
+    T EX = T(0), E2X = T(0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      EX += s.E[k] * X[k];
+      E2X += s.E2[k] * X[k];
+    }
+    T EXm[9], XEm[9];
+    mm3(s.E, X, EXm);
+    mm3(X, s.E, XEm);
+    const T g4 = T(m.g4);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const T dia = (k % 4 == 0) ? T(1) : T(0);
+      dS[k] = s.al * trX * dia + s.be * (trX * s.E[k] + EX * dia) + s.ga * (trX * s.E2[k] + E2X * dia) +
+              T(4) * g4 * EX * s.E[k] + s.B * X[k] + s.C * (EXm[k] + XEm[k]) + s.cw * (X[k] - trX / T(3) * dia);
+    }
+  } else {
+    T EX = T(0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) EX += s.E[k] * X[k];
+    T EXm[9], XEm[9];
+    mm3(s.E, X, EXm);
+    mm3(X, s.E, XEm);
+    const T Am = T(m.Am), Bm = T(m.Bm), Cm = T(m.Cm);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const T dia = (k % 4 == 0) ? T(1) : T(0);
+      dS[k] = (s.lam + T(2) * Am * s.trE) * trX * dia + T(2) * Bm * (EX * dia + trX * s.E[k]) +
+              T(2) * (s.mu + Bm * s.trE) * X[k] + Cm * (EXm[k] + XEm[k]);
+    }
+  }
+  T a1[9], a2[9], sum[9];
+  mm3(dFe, s.S, a1);
+  mm3(s.Fe, dS, a2);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) sum[k] = a1[k] + a2[k];
+  if (hasFp) {
+    T t[9];
+    mmt3(sum, s.Fpi, t);
+    // the reference's Fp-branch K4 under its (K4:dF^T)^T matvec is transposed
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) dP[3 * i + j] = t[3 * j + i];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dP[k] = sum[k];
+  }
+}
+
+struct Nine {
+  double v[9];
+};
+
+// constitutive pass at F = F_base + alpha*(dXu + dX) (trial) or F_base:
+// writes F (trial), P, and per-block [min det Fe, nonfinite count, |F|^2, sig sums x9]
+#define FS_NPART 12
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_fs_const(size_t NV, const T* __restrict__ Fb, const T* __restrict__ dX,
+                                                     Nine dXu, double alpha, int trial, T* __restrict__ Fout,
+                                                     const T* __restrict__ Fpi, const T* __restrict__ lam,
+                                                     const T* __restrict__ mu, FsMat m, T* __restrict__ P,
+                                                     double* __restrict__ partials) {
+  __shared__ double red[FS_NPART * (MGK_NT / 64)];
+  double acc[FS_NPART];
+#pragma unroll
+  for (int k = 0; k < FS_NPART; ++k) acc[k] = 0.0;
+  double jmin = 1e300;
+  for (size_t v = blockIdx.x * (size_t)MGK_NT + threadIdx.x; v < NV; v += (size_t)gridDim.x * MGK_NT) {
+    T F[9], fpi[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      T f = Fb[(size_t)k * NV + v];
+      if (trial) {
+        const T x = (T)dXu.v[k] + (dX ? dX[(size_t)k * NV + v] : T(0));
+        f = f + (T)alpha * x;
+        Fout[(size_t)k * NV + v] = f;
+      }
+      F[k] = f;
+      if (Fpi) fpi[k] = Fpi[(size_t)k * NV + v];
+    }
+    FsVox<T> s;
+    T Pv[9];
+    fs_state(F, Fpi ? fpi : (const T*)nullptr, lam[v], mu[v], m, s, Pv);
+    const double Je = (double)det3(s.Fe);
+    bool bad = isnan(Je);
+    if (Je < jmin || bad) jmin = bad ? -1e300 : Je;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      P[(size_t)k * NV + v] = Pv[k];
+      bad = bad || isnan((double)Pv[k]);
+    }
+    // the reference also rejects a NaN tangent; its entries come from these coefficients
+    bad = bad || isnan((double)s.al) || isnan((double)s.B) || isnan((double)s.C) || isnan((double)s.cw);
+    acc[1] += bad ? 1.0 : 0.0;
+    double f2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f2 += (double)F[k] * (double)F[k];
+    acc[2] += f2;
+    // Cauchy sigma = P F^T / J(F)  (cauchy_from_P_3d)
+    double J = (double)det3(F);
+    if (fabs(J) < 1e-14) J = 1e-14;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double pf = (double)Pv[3 * i] * (double)F[3 * j] + (double)Pv[3 * i + 1] * (double)F[3 * j + 1] +
+                          (double)Pv[3 * i + 2] * (double)F[3 * j + 2];
+        acc[3 + 3 * i + j] += pf / J;
+      }
+  }
+  // min over the block for the det
+  acc[0] = 0.0;
+  {
+    double v = jmin;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    __shared__ double mins[MGK_NT / 64];
+    if ((threadIdx.x & 63) == 0) mins[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double mn = mins[0];
+      for (int k = 1; k < MGK_NT / 64; ++k) mn = fmin(mn, mins[k]);
+      mins[0] = mn;
+    }
+    __syncthreads();
+    jmin = mins[0];
+  }
+  block_sum<FS_NPART>(acc, red);
+  if (threadIdx.x == 0) {
+    acc[0] = jmin;
+#pragma unroll
+    for (int k = 0; k < FS_NPART; ++k) partials[(size_t)blockIdx.x * FS_NPART + k] = acc[k];
+  }
+}
+
+// Cauchy stress field (for the returned Sig)
+template <typename T>
+__global__ void k_fs_cauchy(size_t NV, const T* __restrict__ P, const T* __restrict__ F, T* __restrict__ sig) {
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < NV; v += (size_t)gridDim.x * blockDim.x) {
+    T f[9], p[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      f[k] = F[(size_t)k * NV + v];
+      p[k] = P[(size_t)k * NV + v];
+    }
+    T J = det3(f);
+    if (fabs((double)J) < 1e-14) J = T(1e-14);
+    T o[9];
+    mmt3(p, f, o);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sig[(size_t)k * NV + v] = o[k] / J;
+  }
+}
+
+// tangent action on a field (or a uniform direction dXu when dX == NULL)
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_fs_dP(size_t NV, const T* __restrict__ F, const T* __restrict__ Fpi,
+                                                  const T* __restrict__ lam, const T* __restrict__ mu, FsMat m,
+                                                  const T* __restrict__ dX, Nine dXu, T* __restrict__ dP,
+                                                  const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (size_t v = blockIdx.x * (size_t)MGK_NT + threadIdx.x; v < NV; v += (size_t)gridDim.x * MGK_NT) {
+    T f[9], fpi[9], d[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      f[k] = F[(size_t)k * NV + v];
+      if (Fpi) fpi[k] = Fpi[(size_t)k * NV + v];
+      d[k] = dX ? dX[(size_t)k * NV + v] : (T)dXu.v[k];
+    }
+    FsVox<T> s;
+    fs_state(f, Fpi ? fpi : (const T*)nullptr, lam[v], mu[v], m, s, (T*)nullptr);
+    T o[9];
+    fs_tangent(s, m, d, Fpi != nullptr, o);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dP[(size_t)k * NV + v] = o[k];
+  }
+}
+
+// per-voxel inverse of Fp
+template <typename T>
+__global__ void k_fs_invfp(size_t NV, const T* __restrict__ Fp, T* __restrict__ Fpi) {
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < NV; v += (size_t)gridDim.x * blockDim.x) {
+    T a[9], r[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = Fp[(size_t)k * NV + v];
+    inv3(a, r);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fpi[(size_t)k * NV + v] = r[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FFT sweeps for 9-component real fields (pairs (0,1) (2,3) (4,5) (6,7) (8,-))
+// ---------------------------------------------------------------------------
+template <typename T>
+struct FsGeom {
+  int nx, P, Q, nxh;
+  long long NV;
+  int logBx;
+  FftPlan plx;
+  const cplx<T>* twx;
+  int logBy;
+  FftPlan ply;
+  const cplx<T>* twy;
+  int logPP;
+  FftPlan plq;
+  const cplx<T>* twq;
+  int npencil;
+};
+
+template <typename T, int MAXV>
+__global__ __launch_bounds__(MGK_NT) void k_fs_xr2c(FsGeom<T> g, const T* __restrict__ f, cplx<T>* __restrict__ spec,
+                                                    const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* tile = tw + g.nx;
+  const int nx = g.nx, P = g.P, Q = g.Q, logB = g.logBx, B = 1 << logB, ldb = B;
+  const int q0 = blockIdx.x * B, p = blockIdx.y, pr = blockIdx.z;
+  const int ca = 2 * pr, cb = (2 * pr + 1 < 9) ? 2 * pr + 1 : -1;
+  const size_t NV = (size_t)g.NV;
+  const size_t cs = (size_t)g.nxh * P * Q;
+  for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
+  for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
+    const int b = e & (B - 1), x = e >> logB, q = q0 + b;
+    T a = T(0), bb = T(0);
+    if (q < Q) {
+      const size_t v = ((size_t)x * P + p) * Q + q;
+      a = f[(size_t)ca * NV + v];
+      if (cb >= 0) bb = f[(size_t)cb * NV + v];
+    }
+    tile[x * ldb + b] = cmk(a, bb);
+  }
+  __syncthreads();
+  tile_fft<MAXV>(tile, Tile{nx, logB, ldb, 1, 0}, g.plx, tw, false);
+  xsplit_store(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
+               cb >= 0 ? spec + (size_t)cb * cs : (cplx<T>*)nullptr);
+}
+
+// c2r along x into real comps; optional fused dots: part[2*blk] += out.a,
+// part[2*blk+1] += out.out (FP64), blk over (x-tile, plane, pair)
+template <typename T, int MAXV>
+__global__ __launch_bounds__(MGK_NT) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* __restrict__ spec, T* __restrict__ out,
+                                                    const T* __restrict__ dota, int selfdot,
+                                                    double* __restrict__ partials, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* tile = tw + g.nx;
+  const int nx = g.nx, P = g.P, Q = g.Q, logB = g.logBx, B = 1 << logB, ldb = B;
+  const int q0 = blockIdx.x * B, p = blockIdx.y, pr = blockIdx.z;
+  const int ca = 2 * pr, cb = (2 * pr + 1 < 9) ? 2 * pr + 1 : -1;
+  const size_t NV = (size_t)g.NV;
+  const size_t cs = (size_t)g.nxh * P * Q;
+  for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
+  xmerge_load(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
+              cb >= 0 ? spec + (size_t)cb * cs : (const cplx<T>*)nullptr);
+  tile_fft<MAXV>(tile, Tile{nx, logB, ldb, 1, 0}, g.plx, tw, true);
+  double acc[2] = {0.0, 0.0};
+  for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
+    const int b = e & (B - 1), x = e >> logB, q = q0 + b;
+    if (q >= Q) continue;
+    const size_t v = ((size_t)x * P + p) * Q + q;
+    const cplx<T> z = tile[x * ldb + b];
+    out[(size_t)ca * NV + v] = z.x;
+    if (dota) acc[0] += (double)z.x * (double)dota[(size_t)ca * NV + v];
+    if (selfdot) acc[1] += (double)z.x * (double)z.x;
+    if (cb >= 0) {
+      out[(size_t)cb * NV + v] = z.y;
+      if (dota) acc[0] += (double)z.y * (double)dota[(size_t)cb * NV + v];
+      if (selfdot) acc[1] += (double)z.y * (double)z.y;
+    }
+  }
+  if (partials) {
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(tile);
+    block_sum<2>(acc, red);
+    if (threadIdx.x == 0) {
+      const size_t bid = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+      partials[2 * bid] = acc[0];
+      partials[2 * bid + 1] = acc[1];
+    }
+  }
+}
+
+__device__ __forceinline__ int fbin_fs(int i, int n) { return (i <= (n - 1) / 2) ? i : i - n; }
+__device__ __forceinline__ bool fnyq_fs(int i, int n) { return (n % 2 == 0) && (i == n / 2); }
+
+// z pencils of one row i (comps 3i..3i+2): FFT, (A xi)_i xi_j / |xi|^2 * scale, iFFT
+template <typename T, int MAXV>
+__global__ __launch_bounds__(MGK_NT) void k_fs_proj(FsGeom<T> g, cplx<T>* __restrict__ spec, T scale,
+                                                    const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* tile = tw + g.Q;
+  const int Q = g.Q, P = g.P, logPP = g.logPP, PP = 1 << logPP;
+  const int ldb = PP, cstride = Q * PP;
+  const int pid0 = blockIdx.x * PP;
+  const int row = blockIdx.y;
+  const int npen = g.npencil;
+  for (int t = threadIdx.x; t < Q; t += MGK_NT) tw[t] = g.twq[t];
+  const int chunk = PP * Q;
+  for (int e = threadIdx.x; e < 3 * chunk; e += MGK_NT) {
+    const int c = e / chunk;
+    const int r = e - c * chunk;
+    const int pp = r / Q;
+    const int q = r - pp * Q;
+    const int pid = pid0 + pp;
+    tile[c * cstride + q * ldb + pp] =
+        (pid < npen) ? spec[((size_t)(3 * row + c) * npen + pid) * Q + q] : cmk(T(0), T(0));
+  }
+  __syncthreads();
+  const Tile tl{Q, logPP, ldb, 3, cstride};
+  tile_fft<MAXV>(tile, tl, g.plq, tw, false);
+  for (int e = threadIdx.x; e < Q * PP; e += MGK_NT) {
+    const int pp = e & (PP - 1), q = e >> logPP;
+    const int pid = pid0 + pp;
+    if (pid >= npen) continue;
+    const int kx = pid / P, p = pid - kx * P;
+    const bool zero = (kx == 0 && p == 0 && q == 0) || fnyq_fs(kx, g.nx) || fnyq_fs(p, P) || fnyq_fs(q, Q);
+    cplx<T> a[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a[c] = tile[c * cstride + q * ldb + pp];
+    cplx<T> o[3];
+    if (zero) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) o[c] = cmk(T(0), T(0));
+    } else {
+      const T xi[3] = {T(fbin_fs(kx, g.nx)) / T(g.nx), T(fbin_fs(p, P)) / T(P), T(fbin_fs(q, Q)) / T(Q)};
+      const T ik2 = T(1) / (xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+      const cplx<T> s = cmk(a[0].x * xi[0] + a[1].x * xi[1] + a[2].x * xi[2],
+                            a[0].y * xi[0] + a[1].y * xi[1] + a[2].y * xi[2]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T f = xi[c] * ik2 * scale;
+        o[c] = cmk(s.x * f, s.y * f);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tile[c * cstride + q * ldb + pp] = o[c];
+  }
+  __syncthreads();
+  tile_fft<MAXV>(tile, tl, g.plq, tw, true);
+  for (int e = threadIdx.x; e < 3 * chunk; e += MGK_NT) {
+    const int c = e / chunk;
+    const int r = e - c * chunk;
+    const int pp = r / Q;
+    const int q = r - pp * Q;
+    const int pid = pid0 + pp;
+    if (pid < npen) spec[((size_t)(3 * row + c) * npen + pid) * Q + q] = tile[c * cstride + q * ldb + pp];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB (SciPy 1.15.3 recurrence) with device-resident scalars
+// ---------------------------------------------------------------------------
+struct KryState {
+  int done;    // stop: every kernel of later iterations exits
+  int done2;   // done || s-converged (skips the second matvec)
+  int sconv;
+  int status;  // 0 converged, maxiter, -10 rho breakdown, -11 omega/rv breakdown
+  int iters;
+  int maxiter;
+  int pad0, pad1;
+  double rtol, atol, bnrm2, rho, rho_prev, alpha, omega, beta, rv, ts, tt, rnorm, snorm, rhotol;
+};
+
+#define KRY_NBLK 1024
+
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_kry_init(size_t n, const T* __restrict__ b, T* __restrict__ x,
+                                                     T* __restrict__ r, T* __restrict__ rt,
+                                                     double* __restrict__ partials) {
+  __shared__ double red[MGK_NT / 64];
+  double acc[1] = {0.0};
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT) {
+    const T v = b[i];
+    x[i] = T(0);
+    r[i] = v;
+    rt[i] = v;
+    acc[0] += (double)v * (double)v;
+  }
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
+}
+
+__device__ __forceinline__ void kry_sum(const double* __restrict__ partials, int n, int stride, int nv, double* out,
+                                        double* red) {
+  double acc[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < n; i += MGK_NT) {
+    acc[0] += partials[(size_t)i * stride];
+    if (nv > 1) acc[1] += partials[(size_t)i * stride + 1];
+  }
+  block_sum<2>(acc, red);
+  if (threadIdx.x == 0) {
+    out[0] = acc[0];
+    out[1] = acc[1];
+  }
+}
+
+__global__ __launch_bounds__(MGK_NT) void k_kry_fin_init(const double* __restrict__ partials, int n,
+                                                          KryState* __restrict__ st) {
+  __shared__ double red[2 * (MGK_NT / 64)];
+  double s[2];
+  kry_sum(partials, n, 1, 1, s, red);
+  if (threadIdx.x == 0) {
+    st->bnrm2 = sqrt(s[0]);
+    st->atol = fmax(0.0, st->rtol * st->bnrm2);
+    st->rnorm = st->bnrm2;
+    st->iters = 0;
+    st->sconv = 0;
+    st->done = 0;
+    st->status = 0;
+    if (st->bnrm2 == 0.0) {
+      st->done = 1;  // x = 0 (= b)
+    } else if (st->rnorm < st->atol) {
+      st->done = 1;
+    } else {
+      st->rho = s[0];  // (rtilde, r) with rtilde = r = b
+      if (fabs(st->rho) < st->rhotol) {
+        st->done = 1;
+        st->status = -10;
+      }
+    }
+    st->done2 = st->done;
+  }
+}
+
+// p = r (first spin) or p = ((p - omega v) * beta) + r
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_kry_p(size_t n, const KryState* __restrict__ st, T* __restrict__ p,
+                                                  const T* __restrict__ r, const T* __restrict__ v) {
+  if (st->done) return;
+  const int first = st->iters == 0;
+  const T om = (T)st->omega, be = (T)st->beta;
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT) {
+    if (first) p[i] = r[i];
+    else {
+      T t = p[i] - om * v[i];
+      t = t * be;
+      p[i] = t + r[i];
+    }
+  }
+}
+
+// after v = A p: rv = (rtilde, v), alpha = rho / rv
+__global__ __launch_bounds__(MGK_NT) void k_kry_fin_rv(const double* __restrict__ partials, int n,
+                                                        KryState* __restrict__ st) {
+  if (st->done) return;
+  __shared__ double red[2 * (MGK_NT / 64)];
+  double s[2];
+  kry_sum(partials, n, 2, 1, s, red);
+  if (threadIdx.x == 0) {
+    st->rv = s[0];
+    if (s[0] == 0.0) {
+      st->done = 1;
+      st->done2 = 1;
+      st->status = -11;
+    } else {
+      st->alpha = st->rho / s[0];
+    }
+  }
+}
+
+// s = r - alpha v (in place in r), |s|^2
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_kry_s(size_t n, const KryState* __restrict__ st, T* __restrict__ r,
+                                                  const T* __restrict__ v, double* __restrict__ partials) {
+  if (st->done) return;
+  __shared__ double red[MGK_NT / 64];
+  const T al = (T)st->alpha;
+  double acc[1] = {0.0};
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT) {
+    const T s = r[i] - al * v[i];
+    r[i] = s;
+    acc[0] += (double)s * (double)s;
+  }
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
+}
+
+__global__ __launch_bounds__(MGK_NT) void k_kry_fin_s(const double* __restrict__ partials, int n,
+                                                       KryState* __restrict__ st) {
+  if (st->done) return;
+  __shared__ double red[2 * (MGK_NT / 64)];
+  double s[2];
+  kry_sum(partials, n, 1, 1, s, red);
+  if (threadIdx.x == 0) {
+    st->snorm = sqrt(s[0]);
+    if (st->snorm < st->atol) {
+      st->sconv = 1;
+      st->done2 = 1;
+    }
+  }
+}
+
+// after t = A s: omega = (t, s) / (t, t)
+__global__ __launch_bounds__(MGK_NT) void k_kry_fin_t(const double* __restrict__ partials, int n,
+                                                       KryState* __restrict__ st) {
+  if (st->done2) return;
+  __shared__ double red[2 * (MGK_NT / 64)];
+  double s[2];
+  kry_sum(partials, n, 2, 2, s, red);
+  if (threadIdx.x == 0) {
+    st->ts = s[0];
+    st->tt = s[1];
+    st->omega = s[0] / s[1];
+  }
+}
+
+// x += alpha p; x += omega s; r = s - omega t; partials |r|^2, (rtilde, r)
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_kry_xr(size_t n, const KryState* __restrict__ st, T* __restrict__ x,
+                                                   const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ t,
+                                                   const T* __restrict__ rt, double* __restrict__ partials) {
+  if (st->done) return;
+  __shared__ double red[2 * (MGK_NT / 64)];
+  const T al = (T)st->alpha, om = (T)st->omega;
+  const int sc = st->sconv;
+  double acc[2] = {0.0, 0.0};
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT) {
+    if (sc) {
+      x[i] = x[i] + al * p[i];
+    } else {
+      const T s = r[i];
+      T xv = x[i] + al * p[i];
+      xv = xv + om * s;
+      x[i] = xv;
+      const T rn = s - om * t[i];
+      r[i] = rn;
+      acc[0] += (double)rn * (double)rn;
+      acc[1] += (double)rt[i] * (double)rn;
+    }
+  }
+  block_sum<2>(acc, red);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = acc[0];
+    partials[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+__global__ __launch_bounds__(MGK_NT) void k_kry_fin_xr(const double* __restrict__ partials, int n,
+                                                        KryState* __restrict__ st) {
+  if (st->done) return;
+  __shared__ double red[2 * (MGK_NT / 64)];
+  double s[2];
+  kry_sum(partials, n, 2, 2, s, red);
+  if (threadIdx.x == 0) {
+    if (st->sconv) {
+      st->done = 1;
+      st->done2 = 1;
+      st->status = 0;
+      return;
+    }
+    st->iters += 1;
+    st->rnorm = sqrt(s[0]);
+    const double rho_new = s[1];
+    st->rho_prev = st->rho;
+    if (st->iters >= st->maxiter) {
+      st->done = 1;
+      st->status = st->maxiter;
+    } else if (st->rnorm < st->atol) {
+      st->done = 1;
+      st->status = 0;
+    } else if (fabs(rho_new) < st->rhotol) {
+      st->done = 1;
+      st->status = -10;
+    } else if (fabs(st->omega) < st->rhotol) {
+      st->done = 1;
+      st->status = -11;
+    } else {
+      st->beta = (rho_new / st->rho_prev) * (st->alpha / st->omega);
+      st->rho = rho_new;
+    }
+    st->done2 = st->done;
+  }
+}
+
+// |x|^2 (FP64) into out[0]
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_norm2(size_t n, const T* __restrict__ x, double* __restrict__ partials) {
+  __shared__ double red[MGK_NT / 64];
+  double acc[1] = {0.0};
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT)
+    acc[0] += (double)x[i] * (double)x[i];
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
+}
+
+__global__ __launch_bounds__(MGK_NT) void k_sum_parts(const double* __restrict__ partials, int n, int nv,
+                                                       double* __restrict__ out) {
+  __shared__ double red[16 * (MGK_NT / 64)];
+  double acc[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = 0.0;
+  for (int i = threadIdx.x; i < n; i += MGK_NT)
+    for (int k = 0; k < nv && k < 16; ++k) acc[k] += partials[(size_t)i * nv + k];
+  block_sum<16>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < nv && k < 16; ++k) out[k] = acc[k];
+}
+
+// per-block FP64 sums of the 9 components
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_comp_means9(size_t NV, const T* __restrict__ f,
+                                                        double* __restrict__ partials) {
+  __shared__ double red[9 * (MGK_NT / 64)];
+  double acc[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+  for (size_t v = blockIdx.x * (size_t)MGK_NT + threadIdx.x; v < NV; v += (size_t)gridDim.x * MGK_NT) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc[k] += (double)f[(size_t)k * NV + v];
+  }
+  block_sum<9>(acc, red);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) partials[(size_t)blockIdx.x * 9 + k] = acc[k];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+extern "C" {
+typedef struct {
+  int model;  // 0 svk, 1 landau, 2 murnaghan
+  int capping;
+  int cap_smooth;
+  double v1, v2, v3, g1, g2, g3, g4;  // Pa (the caller applies the GPa autoconversion)
+  double Am, Bm, Cm;
+  double cap_limit;  // resolved E_cap
+  double cap_power, cap_ratio;
+} mgk_fs_material;
+
+typedef struct {
+  int status;        // 0 ok; MGK_ERR_SOLVER with fail_kind set
+  int fail_kind;     // 1 line search, 2 Newton not converged, 3 macro BC not converged
+  int newton_iters;  // total over macro iterations
+  int krylov_iters;  // total BiCGSTAB iterations
+  int macro_iters;
+  double max_err;
+  double F_mean[9];
+  double sig_mean[9];
+  double F_bar[9];
+} mgk_fs_info;
+}
+
+namespace {
+
+template <typename T>
+constexpr int fs_maxv() {
+  return sizeof(T) == 8 ? 8 : 16;
+}
+
+struct FsPlan {
+  int nx, P, Q, nxh;
+  long long NV;
+  int logBx, logBy, logPP;
+  size_t lds_x, lds_y, lds_q;
+  int nqx, nqy, npb;
+  size_t fld, spec;  // bytes per 9-comp field and spectrum
+  int nred;
+  size_t total;
+};
+
+template <typename T>
+int fs_plan(const int64_t* shape, FsPlan* pl) {
+  pl->nx = (int)shape[0];
+  pl->P = (int)shape[1];
+  pl->Q = (int)shape[2];
+  if (pl->nx < 1 || pl->P < 1 || pl->Q < 1) return MGK_ERR_ARG;
+  pl->nxh = pl->nx / 2 + 1;
+  pl->NV = (long long)pl->nx * pl->P * pl->Q;
+  FftPlan t;
+  if (!mgk_make_plan(pl->nx, &t) || !mgk_make_plan(pl->P, &t) || !mgk_make_plan(pl->Q, &t)) return MGK_ERR_ARG;
+  const int TILE = fs_maxv<T>() * MGK_NT;
+  const size_t lim = mgk_lds_limit();
+  const size_t cb = sizeof(cplx<T>);
+  if (pl->nx > TILE || pl->P > TILE || 3 * pl->Q > TILE) return MGK_ERR_ARG;
+  int B = 1;
+  while (2 * B * pl->nx <= TILE && 2 * B <= mgk_pow2_ceil(pl->Q)) B *= 2;
+  while (B > 1 && (size_t)(pl->nx + pl->nx * B) * cb > lim) B /= 2;
+  pl->logBx = mgk_ilog2(B);
+  pl->lds_x = (size_t)(pl->nx + pl->nx * B) * cb;
+  B = 1;
+  while (2 * B * pl->P <= TILE && 2 * B <= mgk_pow2_ceil(pl->Q)) B *= 2;
+  while (B > 1 && (size_t)(pl->P + pl->P * B) * cb > lim) B /= 2;
+  pl->logBy = mgk_ilog2(B);
+  pl->lds_y = (size_t)(pl->P + pl->P * B) * cb;
+  int PP = 1;
+  const int npen = pl->nxh * pl->P;
+  while (2 * PP * 3 * pl->Q <= TILE && 2 * PP <= mgk_pow2_ceil(npen)) PP *= 2;
+  while (PP > 1 && (size_t)(pl->Q + 3 * pl->Q * PP) * cb > lim) PP /= 2;
+  pl->logPP = mgk_ilog2(PP);
+  pl->lds_q = (size_t)(pl->Q + 3 * pl->Q * PP) * cb;
+  if (pl->lds_x > lim || pl->lds_y > lim || pl->lds_q > lim) return MGK_ERR_ARG;
+  pl->nqx = (pl->Q + (1 << pl->logBx) - 1) >> pl->logBx;
+  pl->nqy = (pl->Q + (1 << pl->logBy) - 1) >> pl->logBy;
+  pl->npb = (npen + (1 << pl->logPP) - 1) >> pl->logPP;
+  pl->fld = mgk_align((size_t)9 * pl->NV * sizeof(T));
+  pl->spec = mgk_align((size_t)9 * pl->nxh * pl->P * pl->Q * cb);
+  pl->nred = (int)std::min<long long>((9 * pl->NV + MGK_NT - 1) / MGK_NT, KRY_NBLK);
+  // fields: Ftrial P Ptrial b x r rt p v t dP Fpi  (12) + spectrum + partials + states
+  const size_t nparts = std::max<size_t>((size_t)pl->nqx * pl->P * 5 * 2, (size_t)KRY_NBLK * FS_NPART);
+  pl->total = 12 * pl->fld + pl->spec + mgk_align(nparts * sizeof(double)) + 4096;
+  return MGK_OK;
+}
+
+template <typename T>
+struct FsCtx {
+  FsPlan pl;
+  FsGeom<T> g;
+  hipStream_t s;
+  T *Ftr, *P, *Ptr, *b, *x, *r, *rt, *p, *v, *t, *dP, *Fpi;
+  cplx<T>* spec;
+  double* part;
+  KryState* kst;
+  double* scal;  // 16 doubles of reductions
+  const T *lam, *mu;
+  FsMat mat;
+  MgkProfRec* prof;
+  int kry_it;
+};
+
+template <typename T>
+void fs_project(FsCtx<T>& c, const T* in, T* out, T scale, const T* dota, int selfdot, double* partials,
+                const int* skip) {
+  constexpr int MV = fs_maxv<T>();
+  const FsPlan& pl = c.pl;
+  int h = c.prof->begin(KID_FS_XFWD, c.kry_it, (double)pl.NV * 9 * sizeof(T) + pl.spec);
+  k_fs_xr2c<T, MV><<<dim3(pl.nqx, pl.P, 5), MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip);
+  c.prof->end(h);
+  if (pl.P > 1) {
+    h = c.prof->begin(KID_FS_MID, c.kry_it, 2.0 * pl.spec);
+    k_fft_mid<T, MV><<<dim3(pl.nqy, pl.nxh, 9), MGK_NT, pl.lds_y, c.s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, c.g.ply,
+                                                                          c.g.twy, c.spec, 0, skip);
+    c.prof->end(h);
+  }
+  h = c.prof->begin(KID_FS_PENCIL, c.kry_it, 2.0 * pl.spec);
+  k_fs_proj<T, MV><<<dim3(pl.npb, 3), MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip);
+  c.prof->end(h);
+  if (pl.P > 1) {
+    h = c.prof->begin(KID_FS_MID, c.kry_it, 2.0 * pl.spec);
+    k_fft_mid<T, MV><<<dim3(pl.nqy, pl.nxh, 9), MGK_NT, pl.lds_y, c.s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, c.g.ply,
+                                                                          c.g.twy, c.spec, 1, skip);
+    c.prof->end(h);
+  }
+  h = c.prof->begin(KID_FS_XINV, c.kry_it,
+                    pl.spec + (double)pl.NV * 9 * sizeof(T) * (1 + (dota ? 1 : 0)));
+  k_fs_xc2r<T, MV><<<dim3(pl.nqx, pl.P, 5), MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials,
+                                                                      skip);
+  c.prof->end(h);
+}
+
+template <typename T>
+int fs_nblk(long long n) {
+  return (int)std::min<long long>((n + MGK_NT - 1) / MGK_NT, KRY_NBLK);
+}
+
+// A x = G((K : x^T)^T) (matrix-free); F = current deformation (Fcur)
+template <typename T>
+void fs_matvec(FsCtx<T>& c, const T* Fcur, const T* in, T* out, const T* dota, int selfdot, double* partials,
+               const int* skip) {
+  const FsPlan& pl = c.pl;
+  Nine z{};
+  const int h = c.prof->begin(KID_FS_CONST, c.kry_it, (double)pl.NV * sizeof(T) * (27 + 2 + (c.Fpi ? 9 : 0)));
+  k_fs_dP<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, c.s>>>((size_t)pl.NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, in, z, c.dP,
+                                                    skip);
+  c.prof->end(h);
+  fs_project(c, c.dP, out, T(1.0 / (double)pl.NV), dota, selfdot, partials, skip);
+}
+
+// SciPy bicgstab(A, b, x0=None, rtol, maxiter) -> c.x
+template <typename T>
+int fs_bicgstab(FsCtx<T>& c, const T* Fcur, double rtol, int maxiter, int* iters, int* status) {
+  const FsPlan& pl = c.pl;
+  const size_t n = (size_t)9 * pl.NV;
+  const int nb = fs_nblk<T>((long long)n);
+  KryState init{};
+  init.rtol = rtol;
+  init.maxiter = maxiter;
+  init.rhotol = 4.930380657631324e-32;  // eps(double)^2, as SciPy for float64
+  MGK_HIP(hipMemcpyAsync(c.kst, &init, sizeof(KryState), hipMemcpyHostToDevice, c.s));
+  k_kry_init<T><<<nb, MGK_NT, 0, c.s>>>(n, c.b, c.x, c.r, c.rt, c.part);
+  k_kry_fin_init<<<1, MGK_NT, 0, c.s>>>(c.part, nb, c.kst);
+  const int nxp = pl.nqx * pl.P * 5;
+  auto enqueue = [&]() {
+    const int h = c.prof->begin(KID_FS_VEC, c.kry_it, 3.0 * n * sizeof(T));
+    k_kry_p<T><<<nb, MGK_NT, 0, c.s>>>(n, c.kst, c.p, c.r, c.v);
+    c.prof->end(h);
+    fs_matvec(c, Fcur, c.p, c.v, c.rt, 0, c.part, &c.kst->done);
+    k_kry_fin_rv<<<1, MGK_NT, 0, c.s>>>(c.part, nxp, c.kst);
+    k_kry_s<T><<<nb, MGK_NT, 0, c.s>>>(n, c.kst, c.r, c.v, c.part);
+    k_kry_fin_s<<<1, MGK_NT, 0, c.s>>>(c.part, nb, c.kst);
+    fs_matvec(c, Fcur, c.r, c.t, c.r, 1, c.part, &c.kst->done2);
+    k_kry_fin_t<<<1, MGK_NT, 0, c.s>>>(c.part, nxp, c.kst);
+    const int h2 = c.prof->begin(KID_FS_VEC, c.kry_it, 6.0 * n * sizeof(T));
+    k_kry_xr<T><<<nb, MGK_NT, 0, c.s>>>(n, c.kst, c.x, c.p, c.r, c.t, c.rt, c.part);
+    c.prof->end(h2);
+    k_kry_fin_xr<<<1, MGK_NT, 0, c.s>>>(c.part, nb, c.kst);
+    ++c.kry_it;
+  };
+  KryState* hs = (KryState*)mgk_pinned(2 * sizeof(KryState));
+  hipEvent_t ev[2];
+  MGK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  MGK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  const int start_it = c.kry_it;
+  int enq = 0, chunk = 4, slot = 0;
+  bool pending = false, stop = false;
+  while (!stop && enq < maxiter) {
+    const int k = std::min(chunk, maxiter - enq);
+    for (int i = 0; i < k; ++i) enqueue();
+    enq += k;
+    MGK_HIP(hipGetLastError());
+    MGK_HIP(hipMemcpyAsync(&hs[slot], c.kst, sizeof(KryState), hipMemcpyDeviceToHost, c.s));
+    MGK_HIP(hipEventRecord(ev[slot], c.s));
+    if (pending) {
+      MGK_HIP(hipEventSynchronize(ev[slot ^ 1]));
+      if (hs[slot ^ 1].done) stop = true;
+    }
+    pending = true;
+    slot ^= 1;
+    chunk = std::min(chunk * 2, 32);
+  }
+  MGK_HIP(hipMemcpyAsync(&hs[0], c.kst, sizeof(KryState), hipMemcpyDeviceToHost, c.s));
+  MGK_HIP(hipStreamSynchronize(c.s));
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  *iters = hs[0].iters;
+  *status = hs[0].status;
+  // launches queued after the stop exited at once: not timed
+  c.prof->invalidate(start_it + hs[0].iters + 1, start_it + enq);
+  return MGK_OK;
+}
+
+template <typename T>
+int fs_read_scal(FsCtx<T>& c, int nparts, int nv, double* out) {
+  k_sum_parts<<<1, MGK_NT, 0, c.s>>>(c.part, nparts, nv, c.scal);
+  double* h = (double*)mgk_pinned(16 * sizeof(double));
+  MGK_HIP(hipMemcpyAsync(h, c.scal, 16 * sizeof(double), hipMemcpyDeviceToHost, c.s));
+  MGK_HIP(hipStreamSynchronize(c.s));
+  memcpy(out, h, nv * sizeof(double));
+  return MGK_OK;
+}
+
+// constitutive pass; returns [minJe, nonfinite, |F|^2, sig mean x9] (means divided)
+template <typename T>
+int fs_const(FsCtx<T>& c, const T* Fb, const T* dX, const Nine& dXu, double alpha, int trial, T* Fout, T* P,
+             double* res) {
+  const int nb = fs_nblk<T>(c.pl.NV);
+  k_fs_const<T><<<nb, MGK_NT, 0, c.s>>>((size_t)c.pl.NV, Fb, dX, dXu, alpha, trial, Fout, c.Fpi, c.lam, c.mu, c.mat,
+                                        P, c.part);
+  MGK_HIP(hipGetLastError());
+  // min over blocks of column 0 is needed, the rest are sums
+  std::vector<double> hp((size_t)nb * FS_NPART);
+  MGK_HIP(hipMemcpyAsync(hp.data(), c.part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, c.s));
+  MGK_HIP(hipStreamSynchronize(c.s));
+  double mn = 1e300;
+  for (int k = 0; k < FS_NPART; ++k) res[k] = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    mn = std::min(mn, hp[(size_t)i * FS_NPART]);
+    for (int k = 1; k < FS_NPART; ++k) res[k] += hp[(size_t)i * FS_NPART + k];
+  }
+  res[0] = mn;
+  for (int k = 3; k < FS_NPART; ++k) res[k] /= (double)c.pl.NV;
+  return MGK_OK;
+}
+
+template <typename T>
+int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu, const mgk_fs_material* mt,
+              const double* F_bar, const double* P_target, const int* P_mask, int i_drv, int j_drv, double E_avg,
+              double nu_avg, double tol_NW, double tol_CG, int max_NW, double tol_macro, int max_iter_macro,
+              T* P_out, T* sig_out, char* ws, hipStream_t s, mgk_fs_info* info) {
+  constexpr int prec = sizeof(T) == 8 ? 0 : 1;
+  FsCtx<T> c;
+  int rc = fs_plan<T>(shape, &c.pl);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the finite-strain solver", __FILE__, __LINE__);
+    return rc;
+  }
+  const FsPlan& pl = c.pl;
+  c.s = s;
+  c.lam = lam;
+  c.mu = mu;
+  c.kry_it = 0;
+  FsMat& m = c.mat;
+  m.model = mt->model;
+  m.capping = mt->capping;
+  m.cap_smooth = mt->cap_smooth;
+  m.v1 = mt->v1;
+  m.v2 = mt->v2;
+  m.v3 = mt->v3;
+  m.g1 = mt->g1;
+  m.g2 = mt->g2;
+  m.g3 = mt->g3;
+  m.g4 = mt->g4;
+  m.Am = mt->Am;
+  m.Bm = mt->Bm;
+  m.Cm = mt->Cm;
+  m.Ecap = mt->cap_limit;
+  m.cap_p = mt->cap_power;
+  m.cap_ratio = mt->cap_ratio;
+  FsGeom<T>& g = c.g;
+  g.nx = pl.nx;
+  g.P = pl.P;
+  g.Q = pl.Q;
+  g.nxh = pl.nxh;
+  g.NV = pl.NV;
+  g.logBx = pl.logBx;
+  g.logBy = pl.logBy;
+  g.logPP = pl.logPP;
+  g.npencil = pl.nxh * pl.P;
+  mgk_make_plan(pl.nx, &g.plx);
+  mgk_make_plan(pl.P, &g.ply);
+  mgk_make_plan(pl.Q, &g.plq);
+  g.twx = (const cplx<T>*)mgk_twiddles(pl.nx, prec);
+  g.twy = (const cplx<T>*)mgk_twiddles(pl.P, prec);
+  g.twq = (const cplx<T>*)mgk_twiddles(pl.Q, prec);
+  if (!g.twx || !g.twy || !g.twq) {
+    mgk_set_error("twiddle table allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
+  char* w = ws;
+  T** fl[12] = {&c.Ftr, &c.P, &c.Ptr, &c.b, &c.x, &c.r, &c.rt, &c.p, &c.v, &c.t, &c.dP, &c.Fpi};
+  for (int i = 0; i < 12; ++i) {
+    *fl[i] = (T*)w;
+    w += pl.fld;
+  }
+  c.spec = (cplx<T>*)w;
+  w += pl.spec;
+  c.part = (double*)w;
+  w += mgk_align(std::max<size_t>((size_t)pl.nqx * pl.P * 5 * 2, (size_t)KRY_NBLK * FS_NPART) * sizeof(double));
+  c.kst = (KryState*)w;
+  w += mgk_align(sizeof(KryState));
+  c.scal = (double*)w;
+  if (!Fp) c.Fpi = nullptr;
+  MgkProfRec prof(s);
+  c.prof = &prof;
+  const size_t NV = (size_t)pl.NV;
+  if (Fp) k_fs_invfp<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, Fp, c.Fpi);
+
+  // F is updated in place; the accepted trial is copied back into F
+  double Fbc[9];
+  for (int k = 0; k < 9; ++k) Fbc[k] = F_bar[k];
+  int upd[9] = {0};
+  int any_mask = 0;
+  for (int k = 0; k < 9; ++k) any_mask |= P_mask[k];
+  for (int a = 0; a < 3; ++a)
+    if (P_mask[4 * a] && !(a == i_drv && a == j_drv)) upd[4 * a] = 1;
+  bool have_prev = false;
+  double prevF[3] = {0, 0, 0}, prevS[3] = {0, 0, 0};
+  double max_err = 0.0;
+  double res[FS_NPART];
+  info->newton_iters = 0;
+  info->krylov_iters = 0;
+  info->macro_iters = 0;
+  info->fail_kind = 0;
+  int it_mac = 0;
+  bool macro_ok = false;
+  for (it_mac = 0; it_mac < max_iter_macro; ++it_mac) {
+    info->macro_iters = it_mac + 1;
+    // DbarF = F_bar - mean(F_start)
+    double Fm[9];
+    {
+      const int nb = fs_nblk<T>(pl.NV);
+      k_comp_means9<T><<<nb, MGK_NT, 0, s>>>(NV, F, c.part);
+      rc = fs_read_scal(c, nb, 9, Fm);
+      if (rc) return rc;
+      for (int k = 0; k < 9; ++k) Fm[k] /= (double)NV;
+    }
+    Nine DbarF;
+    for (int k = 0; k < 9; ++k) DbarF.v[k] = Fbc[k] - Fm[k];
+    Nine zero{};
+    // P at F_start
+    rc = fs_const(c, F, (const T*)nullptr, zero, 0.0, 0, (T*)nullptr, c.P, res);
+    if (rc) return rc;
+    bool newton_ok = false;
+    for (int i_NW = 0; i_NW < max_NW; ++i_NW) {
+      info->newton_iters += 1;
+      // rhs
+      if (i_NW == 0) {
+        k_fs_dP<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, F, c.Fpi, c.lam, c.mu, c.mat, nullptr, DbarF, c.dP,
+                                                       nullptr);
+        fs_project(c, c.dP, c.b, T(-1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
+      } else {
+        fs_project(c, c.P, c.b, T(-1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
+      }
+      int kit = 0, kst = 0;
+      rc = fs_bicgstab(c, F, tol_CG, 150, &kit, &kst);
+      if (rc) return rc;
+      info->krylov_iters += kit;
+      // |dF|
+      double dn2;
+      {
+        const int nb = fs_nblk<T>(9 * pl.NV);
+        k_norm2<T><<<nb, MGK_NT, 0, s>>>((size_t)9 * NV, c.x, c.part);
+        rc = fs_read_scal(c, nb, 1, &dn2);
+        if (rc) return rc;
+      }
+      // line search
+      double alpha = 1.0;
+      bool accepted = false;
+      const Nine dXu = (i_NW == 0) ? DbarF : zero;
+      for (int ls = 0; ls < 16; ++ls) {
+        rc = fs_const(c, F, c.x, dXu, alpha, 1, c.Ftr, c.Ptr, res);
+        if (rc) return rc;
+        if (!(res[0] > 1e-4) || res[1] > 0.0) {
+          alpha *= 0.5;
+          continue;
+        }
+        accepted = true;
+        break;
+      }
+      if (!accepted) {
+        info->fail_kind = 1;
+        mgk_set_error("Newton-CG line search failed to find a valid step.", __FILE__, __LINE__);
+        return MGK_ERR_SOLVER;
+      }
+      std::swap(c.Ptr, c.P);
+      MGK_HIP(hipMemcpyAsync(F, c.Ftr, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
+      const double fnorm = sqrt(res[2]);
+      for (int k = 0; k < 9; ++k) info->sig_mean[k] = res[3 + k];
+      if (sqrt(dn2) / (fnorm + 1e-20) < tol_NW && i_NW > 0) {
+        newton_ok = true;
+        break;
+      }
+    }
+    if (!newton_ok) {
+      info->fail_kind = 2;
+      mgk_set_error("Newton-CG inner solver did not converge within max_NW iterations.", __FILE__, __LINE__);
+      return MGK_ERR_SOLVER;
+    }
+    if (!any_mask) {
+      macro_ok = true;
+      break;
+    }
+    // mixed-BC macro update (finite_strain_simulator.py:3130-3161)
+    double err[9] = {0};
+    max_err = 0.0;
+    for (int k = 0; k < 9; ++k)
+      if (P_mask[k]) {
+        err[k] = P_target[k] - info->sig_mean[k];
+        max_err = std::max(max_err, fabs(err[k]));
+      }
+    if (max_err < tol_macro) {
+      macro_ok = true;
+      break;
+    }
+    const double tr = err[0] + err[4] + err[8];
+    double dF[9];
+    for (int k = 0; k < 9; ++k) {
+      double v = (err[k] - nu_avg * tr * ((k % 4 == 0) ? 1.0 : 0.0)) / E_avg;
+      dF[k] = std::min(0.01, std::max(-0.01, v));
+    }
+    double curF[3], curS[3], dsc[3];
+    int nu_ = 0;
+    int idx[3];
+    for (int a = 0; a < 3; ++a)
+      if (upd[4 * a]) {
+        idx[nu_] = 4 * a;
+        curF[nu_] = Fbc[4 * a];
+        curS[nu_] = info->sig_mean[4 * a];
+        dsc[nu_] = dF[4 * a];
+        ++nu_;
+      }
+    if (have_prev) {
+      for (int q = 0; q < nu_; ++q) {
+        const double dh = curF[q] - prevF[q], dsh = curS[q] - prevS[q];
+        const double slope = dsh / dh;
+        if (std::isfinite(slope) && fabs(dh) > 1e-14 && slope > 0.01 * E_avg)
+          dsc[q] = std::min(0.01, std::max(-0.01, err[idx[q]] / slope));
+      }
+    }
+    for (int q = 0; q < nu_; ++q) {
+      prevF[q] = curF[q];
+      prevS[q] = curS[q];
+    }
+    have_prev = true;
+    for (int q = 0; q < nu_; ++q) Fbc[idx[q]] += dsc[q];
+  }
+  info->max_err = max_err;
+  for (int k = 0; k < 9; ++k) info->F_bar[k] = Fbc[k];
+  if (!macro_ok && any_mask) {
+    info->fail_kind = 3;
+    mgk_set_error("Outer BC loop did not converge after max_iter_macro iterations.", __FILE__, __LINE__);
+    return MGK_ERR_SOLVER;
+  }
+  if (P_out) MGK_HIP(hipMemcpyAsync(P_out, c.P, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
+  if (sig_out) k_fs_cauchy<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, c.P, F, sig_out);
+  {
+    const int nb = fs_nblk<T>(pl.NV);
+    k_comp_means9<T><<<nb, MGK_NT, 0, s>>>(NV, F, c.part);
+    double Fm[9];
+    rc = fs_read_scal(c, nb, 9, Fm);
+    if (rc) return rc;
+    for (int k = 0; k < 9; ++k) info->F_mean[k] = Fm[k] / (double)NV;
+  }
+  MGK_HIP(hipStreamSynchronize(s));
+  prof.commit(1 << 30);
+  info->status = MGK_OK;
+  return MGK_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int mgk_fs_workspace_bytes(const int64_t* shape, int precision, size_t* bytes) {
+  FsPlan pl;
+  const int rc = precision == 0 ? fs_plan<double>(shape, &pl) : fs_plan<float>(shape, &pl);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the finite-strain solver", __FILE__, __LINE__);
+    return rc;
+  }
+  *bytes = pl.total;
+  return MGK_OK;
+}
+
+extern "C" int mgk_fs_solver_step(const int64_t* shape, int precision, void* F, const void* Fp, const void* lam,
+                                  const void* mu, const mgk_fs_material* mat, const double* F_bar,
+                                  const double* P_target, const int* P_mask, int i_drv, int j_drv, double E_avg,
+                                  double nu_avg, double tol_NW, double tol_CG, int max_NW, double tol_macro,
+                                  int max_iter_macro, void* P_out, void* sig_out, void* workspace,
+                                  size_t workspace_bytes, void* stream, mgk_fs_info* info) {
+  if (!shape || !F || !lam || !mu || !mat || !F_bar || !P_target || !P_mask || !workspace || !info) {
+    mgk_set_error("mgk_fs_solver_step: null argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  size_t need = 0;
+  int rc = mgk_fs_workspace_bytes(shape, precision, &need);
+  if (rc) return rc;
+  if (workspace_bytes < need) {
+    mgk_set_error("workspace too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  memset(info, 0, sizeof(*info));
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == 0)
+    rc = fs_step_t<double>(shape, (double*)F, (const double*)Fp, (const double*)lam, (const double*)mu, mat, F_bar,
+                           P_target, P_mask, i_drv, j_drv, E_avg, nu_avg, tol_NW, tol_CG, max_NW, tol_macro,
+                           max_iter_macro, (double*)P_out, (double*)sig_out, (char*)workspace, s, info);
+  else
+    rc = fs_step_t<float>(shape, (float*)F, (const float*)Fp, (const float*)lam, (const float*)mu, mat, F_bar,
+                          P_target, P_mask, i_drv, j_drv, E_avg, nu_avg, tol_NW, tol_CG, max_NW, tol_macro,
+                          max_iter_macro, (float*)P_out, (float*)sig_out, (char*)workspace, s, info);
+  info->status = rc;
+  return rc;
+}
+
+static FsMat to_fsmat(const mgk_fs_material* mt) {
+  FsMat m;
+  m.model = mt->model;
+  m.capping = mt->capping;
+  m.cap_smooth = mt->cap_smooth;
+  m.pad = 0;
+  m.v1 = mt->v1;
+  m.v2 = mt->v2;
+  m.v3 = mt->v3;
+  m.g1 = mt->g1;
+  m.g2 = mt->g2;
+  m.g3 = mt->g3;
+  m.g4 = mt->g4;
+  m.Am = mt->Am;
+  m.Bm = mt->Bm;
+  m.Cm = mt->Cm;
+  m.Ecap = mt->cap_limit;
+  m.cap_p = mt->cap_power;
+  m.cap_ratio = mt->cap_ratio;
+  return m;
+}
+
+template <typename T>
+static __global__ void k_fs_pointwise(size_t NV, const T* __restrict__ F, const T* __restrict__ Fp,
+                                      const T* __restrict__ lam, const T* __restrict__ mu, FsMat m, T* __restrict__ P,
+                                      const T* __restrict__ dF, T* __restrict__ dP) {
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < NV; v += (size_t)gridDim.x * blockDim.x) {
+    T f[9], fpi[9], d[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f[k] = F[(size_t)k * NV + v];
+    if (Fp) {
+      T a[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) a[k] = Fp[(size_t)k * NV + v];
+      inv3(a, fpi);
+    }
+    FsVox<T> s;
+    T Pv[9];
+    fs_state(f, Fp ? fpi : (const T*)nullptr, lam[v], mu[v], m, s, Pv);
+    if (P)
+#pragma unroll
+      for (int k = 0; k < 9; ++k) P[(size_t)k * NV + v] = Pv[k];
+    if (dF && dP) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) d[k] = dF[(size_t)k * NV + v];
+      T o[9];
+      fs_tangent(s, m, d, Fp != nullptr, o);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) dP[(size_t)k * NV + v] = o[k];
+    }
+  }
+}
+
+// Pointwise constitutive model: P(F) and optionally the tangent action
+// dP = K:dF in the reference's (K4 : dF^T)^T convention
+// (constitutive_hyperelastic_3d, mgkmc/finite_strain_simulator.py:1647-1885).
+extern "C" int mgk_fs_constitutive(long long nvox, int precision, const void* F, const void* Fp, const void* lam,
+                                   const void* mu, const mgk_fs_material* mat, void* P, const void* dF, void* dP,
+                                   void* stream) {
+  if (nvox < 1 || !F || !lam || !mu || !mat) {
+    mgk_set_error("mgk_fs_constitutive: bad argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  const FsMat m = to_fsmat(mat);
+  const int nb = (int)std::min<long long>((nvox + 255) / 256, 4096);
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == 0)
+    k_fs_pointwise<double><<<nb, 256, 0, s>>>((size_t)nvox, (const double*)F, (const double*)Fp, (const double*)lam,
+                                              (const double*)mu, m, (double*)P, (const double*)dF, (double*)dP);
+  else
+    k_fs_pointwise<float><<<nb, 256, 0, s>>>((size_t)nvox, (const float*)F, (const float*)Fp, (const float*)lam,
+                                             (const float*)mu, m, (float*)P, (const float*)dF, (float*)dP);
+  MGK_HIP(hipGetLastError());
+  return MGK_OK;
+}
+
+// Cauchy stress sigma = P F^T / det F per voxel (cauchy_from_P_3d,
+// mgkmc/finite_strain_simulator.py:1892-1904).
+extern "C" int mgk_fs_cauchy(long long nvox, int precision, const void* P, const void* F, void* sig, void* stream) {
+  if (nvox < 1 || !P || !F || !sig) {
+    mgk_set_error("mgk_fs_cauchy: bad argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  const int nb = (int)std::min<long long>((nvox + 255) / 256, 4096);
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == 0)
+    k_fs_cauchy<double><<<nb, 256, 0, s>>>((size_t)nvox, (const double*)P, (const double*)F, (double*)sig);
+  else
+    k_fs_cauchy<float><<<nb, 256, 0, s>>>((size_t)nvox, (const float*)P, (const float*)F, (float*)sig);
+  MGK_HIP(hipGetLastError());
+  return MGK_OK;
+}

@@ -172,11 +172,16 @@ void MgkProfRec::end(int h) {
   (void)hipEventRecord(r_[h].e1, s_);
 }
 
+void MgkProfRec::invalidate(int lo, int hi) {
+  for (auto& r : r_)
+    if (r.iter >= lo && r.iter < hi) r.iter = -1 - r.iter;
+}
+
 void MgkProfRec::commit(int iters_done) {
   if (!on_) return;
   std::lock_guard<std::mutex> lk(g_prof_mu);
   for (auto& r : r_) {
-    if (r.iter >= iters_done) continue;
+    if (r.iter < 0 || r.iter >= iters_done) continue;
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) {
       g_prof_ms[r.kid] += ms;

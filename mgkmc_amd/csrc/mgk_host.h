@@ -67,6 +67,8 @@ class MgkProfRec {
   int begin(int kid, int iter, double bytes);
   void end(int h);
   void commit(int iters_done);
+  // drop records with lo <= iter < hi (launches that exited early)
+  void invalidate(int lo, int hi);
   bool on() const { return on_; }
 
  private:

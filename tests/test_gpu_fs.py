@@ -1,0 +1,136 @@
+"""GPU parity of the finite-strain path with the oracle: constitutive P and
+tangent (K4) to round-off, Newton-BiCGSTAB load steps to the solver's
+tolerance (tol_CG = tol_NW = 1e-8), mixed BCs, Fp, and full-size properties."""
+import numpy as np
+import pytest
+
+from oracle import finite_strain as fs
+
+pytestmark = pytest.mark.gpu
+
+LANDAU = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9)
+
+
+def _grid(sh, seed, amp=0.02, spread=0.2):
+    rng = np.random.default_rng(seed)
+    E = 70e9 * rng.uniform(1 - spread, 1 + spread, sh)
+    nu = np.full(sh, 0.32)
+    I2, I4, I4rt, I4s, II = fs.identity_tensors(*sh)
+    C4 = fs.build_C4(E, nu, I4s, II)
+    F = I2 + amp * rng.standard_normal((3, 3) + sh)
+    return rng, E, nu, I2, I4, I4rt, C4, F
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("model,kw,with_fp", [
+    ("svk", {}, False), ("svk", {}, True), ("landau", {}, False), ("landau", {}, True),
+    ("landau", dict(strain_capping_enabled=True, strain_capping_type="piecewise", strain_capping_limit=0.03), False),
+    ("landau", dict(strain_capping_enabled=True, strain_capping_type="smooth", strain_capping_limit=None), False),
+    ("murnaghan", dict(A_m=-100.0e9, B_m=-100.0e9, C_m=-100.0e9), False)])
+def test_constitutive_parity(gpu, model, kw, with_fp):
+    from mgkmc_amd.finite_strain_simulator import constitutive_hyperelastic_3d
+    sh = (4, 3, 5)
+    rng, E, nu, I2, I4, I4rt, C4, F = _grid(sh, 1, amp=0.06)
+    Fp = I2 + 0.04 * rng.standard_normal((3, 3) + sh) if with_fp else None
+    args = dict(kw)
+    if model == "landau":
+        args.update(LANDAU)
+    P0, K0 = fs.constitutive(F, C4, I2, I4, I4rt, Fp=Fp, model_type=model, **args)
+    P1, K1 = constitutive_hyperelastic_3d(F, C4, I2, I4, I4rt, Fp=Fp, model_type=model, **args)
+    assert _rel(P1, P0) < 1e-12, _rel(P1, P0)
+    assert _rel(K1, K0) < 1e-10, _rel(K1, K0)
+
+
+def test_cauchy_parity(gpu):
+    from mgkmc_amd.finite_strain_simulator import cauchy_from_P_3d
+    rng, E, nu, I2, I4, I4rt, C4, F = _grid((5, 4, 3), 2)
+    P = rng.standard_normal((3, 3, 5, 4, 3)) * 1e9
+    assert _rel(cauchy_from_P_3d(P, F), fs.cauchy_from_P(P, F)) < 1e-14
+
+
+def _step_pair(sh, seed, model, Fbar, Fp=None, P_target=None, P_mask=None, drv=(0, 1), tol=1e-8, kw=None):
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_3d
+    rng, E, nu, I2, I4, I4rt, C4, F0 = _grid(sh, seed)
+    G = fs.build_ghat4(*sh, *sh)
+    args = dict(kw or {})
+    if model == "landau":
+        args.update(LANDAU)
+    st = {}
+    ref = fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, Fp=Fp, driving_component=drv, P_target=P_target,
+                         P_mask=P_mask, E_avg=E.mean(), nu_avg=nu.mean(), tol_NW=tol, tol_CG=tol,
+                         model_type=model, stats=st, **args)
+    got = finite_strain_solver_step_3d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, Fp=Fp, driving_component=drv,
+                                       P_target=P_target, P_mask=P_mask, E_avg=E.mean(), nu_avg=nu.mean(),
+                                       tol_NW=tol, tol_CG=tol, model_type=model, solver="newton_cg", **args)
+    return ref, got, st
+
+
+@pytest.mark.parametrize("sh,model", [((6, 4, 5), "svk"), ((8, 8, 8), "landau"), ((16, 16, 16), "landau"),
+                                      ((12, 6, 10), "svk")])
+def test_newton_step_parity(gpu, sh, model):
+    Fbar = np.eye(3)
+    Fbar[0, 1] = 0.02
+    print(f"[fs] shape={sh} model={model}", flush=True)
+    ref, got, st = _step_pair(sh, 3, model, Fbar)
+    eF, eP, eS = _rel(got[0], ref[0]), _rel(got[1], ref[1]), _rel(got[2], ref[2])
+    print(f"[fs] newton(ref)={st.get('newton')} relF={eF:.2e} relP={eP:.2e} relSig={eS:.2e}", flush=True)
+    assert eF < 1e-9 and eP < 1e-7 and eS < 1e-7
+    assert np.allclose(got[4], ref[4])
+
+
+def test_newton_step_mixed_bc_parity(gpu):
+    sh = (8, 6, 8)
+    Fbar = np.eye(3)
+    Fbar[0, 0] = 1.01
+    Pt = np.zeros((3, 3))
+    Pm = np.zeros((3, 3), bool)
+    Pm[1, 1] = Pm[2, 2] = True
+    ref, got, st = _step_pair(sh, 4, "svk", Fbar, P_target=Pt, P_mask=Pm, drv=(0, 0))
+    print(f"[fs mixed] Fbar ref={np.diag(ref[4])} got={np.diag(got[4])}", flush=True)
+    assert np.abs(got[4] - ref[4]).max() < 1e-10
+    assert _rel(got[0], ref[0]) < 1e-9
+
+
+def test_newton_step_with_fp(gpu):
+    sh = (6, 6, 6)
+    rng = np.random.default_rng(8)
+    I2 = fs.identity_tensors(*sh)[0]
+    Fp = I2 + 0.01 * rng.standard_normal((3, 3) + sh)
+    Fbar = np.eye(3)
+    Fbar[0, 1] = 0.01
+    ref, got, st = _step_pair(sh, 5, "svk", Fbar, Fp=Fp)
+    print(f"[fs Fp] newton(ref)={st.get('newton')} relF={_rel(got[0], ref[0]):.2e}", flush=True)
+    assert _rel(got[0], ref[0]) < 1e-8
+
+
+def test_line_search_failure_raises_value_error(gpu):
+    # an unreachable macro strain makes det(Fe) <= 1e-4 on every trial
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_3d
+    sh = (4, 4, 4)
+    rng, E, nu, I2, I4, I4rt, C4, F0 = _grid(sh, 6)
+    Fbar = np.eye(3)
+    Fbar[0, 0] = -0.5
+    with pytest.raises(ValueError):
+        finite_strain_solver_step_3d(I2.copy(), Fbar, None, C4, I2, I4, I4rt, driving_component=(0, 0),
+                                     solver="newton_cg", max_NW=3)
+
+
+def test_full_size_properties(gpu):
+    import torch
+    from mgkmc_amd.finite_strain_simulator import FsSolver, material
+    n = 64
+    rng = np.random.default_rng(9)
+    mu = 26.9e9 * np.exp(rng.normal(-0.005, 0.1, (n, n, n)))
+    lam = 1.83 * mu
+    s = FsSolver(lam, mu)
+    mat = material("landau", **LANDAU, strain_capping_enabled=True, strain_capping_type="smooth",
+                   strain_capping_limit=0.10, mu_mean=mu.mean())
+    Fbar = np.eye(3)
+    Fbar[0, 1] = 0.01
+    inf = s.step(Fbar, mat, (0, 1), tol_NW=1e-8, tol_CG=1e-8)
+    print(f"[fs full] newton={inf.newton_iters} krylov={inf.krylov_iters}", flush=True)
+    assert np.allclose(np.array(inf.F_mean[:]).reshape(3, 3), Fbar, atol=1e-12)
+    assert inf.newton_iters <= 6
