@@ -237,6 +237,76 @@ __device__ void tile_fft(cplx<T>* __restrict__ buf, const Tile& tl, const FftPla
 }
 
 // ---------------------------------------------------------------------------
+// Compile-time specialised tile FFT for power-of-two lengths.  Same Stockham
+// passes and data layout as above with row stride B and batch stride N*B, but
+// every extent is a template constant, so the index arithmetic folds into
+// shifts/immediates and the staged values stay in few registers.
+// ---------------------------------------------------------------------------
+constexpr int ct_log2(int v) { return v <= 1 ? 0 : 1 + ct_log2(v / 2); }
+
+// radix of the pass that starts at sub-transform length L (8s, then 4/2)
+constexpr int ct_radix(int N, int L) {
+  const int k = ct_log2(N / L);
+  return (k % 3 == 0) ? 8 : ((k % 3 == 2) ? 4 : (k >= 4 ? 4 : 2));
+}
+
+template <int N, int B, int C, int R, int L, typename T>
+__device__ __forceinline__ void ct_pass(cplx<T>* __restrict__ buf, const cplx<T>* __restrict__ tw, bool inv) {
+  constexpr int M = N / R;
+  constexpr int NB = C * M * B;
+  constexpr int S = (NB + MGK_NT - 1) / MGK_NT;
+  constexpr int TS = N / (L * R);
+  cplx<T> v[S][R];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int t = threadIdx.x + s * MGK_NT;
+    if (NB % MGK_NT == 0 || t < NB) {
+      const int b = t % B, u = t / B, j = u % M, c = u / M;
+      const cplx<T>* base = buf + c * (N * B) + b;
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[s][q] = base[(j + q * M) * B];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int t = threadIdx.x + s * MGK_NT;
+    if (NB % MGK_NT == 0 || t < NB) {
+      const int b = t % B, u = t / B, j = u % M, c = u / M;
+      const int k = j % L;
+      if (L > 1) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          cplx<T> w = tw[q * k * TS];
+          if (inv) w.y = -w.y;
+          v[s][q] = cmul(v[s][q], w);
+        }
+      }
+      dftR<R>(v[s], inv);
+      cplx<T>* base = buf + c * (N * B) + b;
+      const int o = (j - k) * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) base[(o + q * L) * B] = v[s][q];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int B, int C, int L, typename T>
+__device__ __forceinline__ void ct_fft_rec(cplx<T>* __restrict__ buf, const cplx<T>* __restrict__ tw, bool inv) {
+  if constexpr (L < N) {
+    constexpr int R = ct_radix(N, L);
+    ct_pass<N, B, C, R, L>(buf, tw, inv);
+    ct_fft_rec<N, B, C, L * R>(buf, tw, inv);
+  }
+}
+
+template <int N, int B, int C, typename T>
+__device__ __forceinline__ void tile_fft_ct(cplx<T>* __restrict__ buf, const cplx<T>* __restrict__ tw, bool inv) {
+  ct_fft_rec<N, B, C, 1>(buf, tw, inv);
+}
+
+// ---------------------------------------------------------------------------
 // Block reductions in FP64 (wavefront = 64 lanes on CDNA).
 // Deterministic: fixed shuffle tree, then fixed order over waves.
 // ---------------------------------------------------------------------------

@@ -55,7 +55,7 @@ __device__ __forceinline__ void xmerge_load(cplx<T>* tile, int ldb, int nx, int 
 // sweep 2/4: complex FFT along the middle axis (length P, stride Q)
 // grid (ceil(Q/B), nxh, C)
 // ---------------------------------------------------------------------------
-template <typename T, int MAXV>
+template <typename T, int MAXV, int NN = 0, int BB = 0>
 __global__ __launch_bounds__(MGK_NT) void k_fft_mid(int P, int Q, int nxh, int logB, FftPlan pl,
                                                     const cplx<T>* __restrict__ twg, cplx<T>* __restrict__ spec,
                                                     int inv, const int* __restrict__ done) {
@@ -63,7 +63,12 @@ __global__ __launch_bounds__(MGK_NT) void k_fft_mid(int P, int Q, int nxh, int l
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* tile = tw + P;
-  const int B = 1 << logB, ldb = B;
+  constexpr bool CT = NN > 0;
+  if (CT) {
+    P = NN;
+    logB = ct_log2(BB);
+  }
+  const int B = CT ? BB : 1 << logB, ldb = B;
   const int q0 = blockIdx.x * B, kx = blockIdx.y, c = blockIdx.z;
   cplx<T>* base = spec + ((size_t)c * nxh + kx) * (size_t)P * Q;
   for (int t = threadIdx.x; t < P; t += MGK_NT) tw[t] = twg[t];
@@ -72,7 +77,8 @@ __global__ __launch_bounds__(MGK_NT) void k_fft_mid(int P, int Q, int nxh, int l
     tile[y * ldb + b] = (q < Q) ? base[(size_t)y * Q + q] : cmk(T(0), T(0));
   }
   __syncthreads();
-  tile_fft<MAXV>(tile, Tile{P, logB, ldb, 1, 0}, pl, tw, inv != 0);
+  if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, inv != 0);
+  else tile_fft<MAXV>(tile, Tile{P, logB, ldb, 1, 0}, pl, tw, inv != 0);
   for (int e = threadIdx.x; e < P * B; e += MGK_NT) {
     const int b = e & (B - 1), y = e >> logB, q = q0 + b;
     if (q < Q) base[(size_t)y * Q + q] = tile[y * ldb + b];

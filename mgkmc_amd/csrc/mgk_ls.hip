@@ -121,7 +121,7 @@ __device__ __forceinline__ void gamma2_apply(const T q0, const T q1, const cplx<
 // ---------------------------------------------------------------------------
 // sweep 1: polarisation + forward x FFT
 // ---------------------------------------------------------------------------
-template <typename T, int D, int MAXV>
+template <typename T, int D, int MAXV, int NN = 0, int BB = 0>
 __global__ __launch_bounds__(MGK_NT) void k_ls_tau_xfwd(LsGeom<T> g, const T* __restrict__ eps,
                                                         const T* __restrict__ epsp, const T* __restrict__ lam,
                                                         const T* __restrict__ mu, T lam0, T mu0,
@@ -130,7 +130,9 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_tau_xfwd(LsGeom<T> g, const T* __
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* tile = tw + g.nx;
-  const int nx = g.nx, P = g.P, Q = g.Q, logB = g.logBx, B = 1 << logB, ldb = B;
+  constexpr bool CT = NN > 0;
+  const int nx = CT ? NN : g.nx, P = g.P, Q = g.Q, logB = CT ? ct_log2(BB) : g.logBx, B = CT ? BB : 1 << logB;
+  const int ldb = B;
   const int q0 = blockIdx.x * B, p = blockIdx.y;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;  // spectrum component stride
@@ -176,7 +178,8 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_tau_xfwd(LsGeom<T> g, const T* __
     }
   }
   __syncthreads();
-  tile_fft<MAXV>(tile, tl, g.plx, tw, false);
+  if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, false);
+  else tile_fft<MAXV>(tile, tl, g.plx, tw, false);
   xsplit_store(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec, spec + cs);
   __syncthreads();
 
@@ -212,7 +215,8 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_tau_xfwd(LsGeom<T> g, const T* __
       }
     }
     __syncthreads();
-    tile_fft<MAXV>(tile, tl, g.plx, tw, false);
+    if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, false);
+  else tile_fft<MAXV>(tile, tl, g.plx, tw, false);
     xsplit_store(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
                  cb >= 0 ? spec + (size_t)cb * cs : (cplx<T>*)nullptr);
     __syncthreads();
@@ -223,7 +227,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_tau_xfwd(LsGeom<T> g, const T* __
 // sweep 3: pencil FFT along the contiguous axis + Green operator + inverse
 // grid ceil(npencil/PP); pencil id = kx*P + p
 // ---------------------------------------------------------------------------
-template <typename T, int D, int MAXV>
+template <typename T, int D, int MAXV, int NN = 0, int BB = 0>
 __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __restrict__ spec, T A2, T Bc, T scale,
                                                       const LsState* __restrict__ st) {
   if (st->done) return;
@@ -231,7 +235,8 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* tile = tw + g.Q;
-  const int Q = g.Q, P = g.P, logPP = g.logPP, PP = 1 << logPP;
+  constexpr bool CT = NN > 0;
+  const int Q = CT ? NN : g.Q, P = g.P, logPP = CT ? ct_log2(BB) : g.logPP, PP = CT ? BB : 1 << logPP;
   const int ldb = PP, cstride = Q * PP;
   const int pid0 = blockIdx.x * PP;
   const int npen = g.npencil;
@@ -248,7 +253,8 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
   }
   __syncthreads();
   const Tile tl{Q, logPP, ldb, C, cstride};
-  tile_fft<MAXV>(tile, tl, g.plq, tw, false);
+  if constexpr (CT) tile_fft_ct<NN, BB, C>(tile, tw, false);
+  else tile_fft<MAXV>(tile, tl, g.plq, tw, false);
 
   for (int e = threadIdx.x; e < Q * PP; e += MGK_NT) {
     const int pp = e & (PP - 1), q = e >> logPP;
@@ -296,7 +302,8 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
     for (int c = 0; c < C; ++c) tile[c * cstride + q * ldb + pp] = o[c];
   }
   __syncthreads();
-  tile_fft<MAXV>(tile, tl, g.plq, tw, true);
+  if constexpr (CT) tile_fft_ct<NN, BB, C>(tile, tw, true);
+  else tile_fft<MAXV>(tile, tl, g.plq, tw, true);
   for (int e = threadIdx.x; e < C * chunk; e += MGK_NT) {
     const int c = e / chunk;
     const int r = e - c * chunk;
@@ -310,7 +317,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
 // ---------------------------------------------------------------------------
 // sweep 5: inverse x FFT (complex-to-real) + update + FP64 partial norms
 // ---------------------------------------------------------------------------
-template <typename T, int D, int MAXV>
+template <typename T, int D, int MAXV, int NN = 0, int BB = 0>
 __global__ __launch_bounds__(MGK_NT) void k_ls_xinv_update(LsGeom<T> g, const cplx<T>* __restrict__ spec,
                                                            T* __restrict__ eps, Sym6 ebar,
                                                            double* __restrict__ partials,
@@ -319,7 +326,9 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xinv_update(LsGeom<T> g, const cp
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* tile = tw + g.nx;
-  const int nx = g.nx, P = g.P, Q = g.Q, logB = g.logBx, B = 1 << logB, ldb = B;
+  constexpr bool CT = NN > 0;
+  const int nx = CT ? NN : g.nx, P = g.P, Q = g.Q, logB = CT ? ct_log2(BB) : g.logBx, B = CT ? BB : 1 << logB;
+  const int ldb = B;
   const int q0 = blockIdx.x * B, p = blockIdx.y;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
@@ -333,7 +342,8 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xinv_update(LsGeom<T> g, const cp
     const int cb = (D == 3 || pr == 0) ? 2 * pr + 1 : -1;
     xmerge_load(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
                 cb >= 0 ? spec + (size_t)cb * cs : (const cplx<T>*)nullptr);
-    tile_fft<MAXV>(tile, tl, g.plx, tw, true);
+    if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, true);
+    else tile_fft<MAXV>(tile, tl, g.plx, tw, true);
     const double wa = (D == 3) ? (ca >= 3 ? 2.0 : 1.0) : (ca >= 2 ? 2.0 : 1.0);
     const double wb = (D == 3) ? (cb >= 3 ? 2.0 : 1.0) : (cb >= 2 ? 2.0 : 1.0);
     const T eba = T(ebar.v[ca]);
@@ -518,6 +528,7 @@ struct LsPlan {
   int D, C, nx, P, Q, nxh;
   long long NV;
   int logBx, logBy, logPP;
+  int ctx, cty, ctq;  // compile-time specialised sweeps in use
   size_t lds_x, lds_y, lds_q;
   int nblk_upd, nblk_red;
   size_t off_spec, off_part, off_state, total;
@@ -563,6 +574,29 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
   pl->logPP = mgk_ilog2(PP);
   pl->lds_q = (size_t)(pl->Q + pl->C * pl->Q * PP) * cb;
   if (pl->lds_q > lim) return MGK_ERR_ARG;
+  // compile-time specialised sweeps (power-of-two lengths 64..512, 4096-point tiles)
+  const int CTT = 4096;
+  auto ct_ok = [](int n) { return n == 64 || n == 128 || n == 256 || n == 512; };
+  pl->ctx = pl->cty = pl->ctq = 0;
+  if (ct_ok(pl->nx) && pl->Q >= CTT / pl->nx && (size_t)(pl->nx + CTT) * cb <= lim) {
+    pl->ctx = 1;
+    pl->logBx = mgk_ilog2(CTT / pl->nx);
+    pl->lds_x = (size_t)(pl->nx + CTT) * cb;
+  }
+  if (ct_ok(pl->P) && pl->Q >= CTT / pl->P && (size_t)(pl->P + CTT) * cb <= lim) {
+    pl->cty = 1;
+    pl->logBy = mgk_ilog2(CTT / pl->P);
+    pl->lds_y = (size_t)(pl->P + CTT) * cb;
+  }
+  if (ct_ok(pl->Q)) {
+    int pp = 1;
+    while (2 * pp * pl->C * pl->Q <= CTT) pp *= 2;
+    if (npen >= pp && (size_t)(pl->Q + pl->C * pl->Q * pp) * cb <= lim) {
+      pl->ctq = 1;
+      pl->logPP = mgk_ilog2(pp);
+      pl->lds_q = (size_t)(pl->Q + pl->C * pl->Q * pp) * cb;
+    }
+  }
   const int nqt = (pl->Q + (1 << pl->logBx) - 1) >> pl->logBx;
   pl->nblk_upd = nqt * pl->P;
   long long nr = (pl->NV + MGK_NT - 1) / MGK_NT;
@@ -573,6 +607,21 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
   pl->off_state = pl->off_part + mgk_align(npart * sizeof(double));
   pl->total = pl->off_state + mgk_align(sizeof(LsState));
   return MGK_OK;
+}
+
+template <typename T, int MV>
+void launch_mid(const LsPlan& pl, dim3 gy, hipStream_t stream, const FftPlan& ply, const cplx<T>* twy, cplx<T>* spec,
+                int inv, const int* done) {
+  if (pl.cty) {
+    switch (pl.P) {
+      case 64: k_fft_mid<T, 16, 64, 64><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+      case 128: k_fft_mid<T, 16, 128, 32><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+      case 256: k_fft_mid<T, 16, 256, 16><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+      default: k_fft_mid<T, 16, 512, 8><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+    }
+  } else {
+    k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done);
+  }
 }
 
 // last iteration count per workspace (0 = unknown); set when iters > 0
@@ -674,25 +723,50 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   int iter_idx = 0;
   auto enqueue_iter = [&]() {
     int h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
-    k_ls_tau_xfwd<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st);
+    if (pl.ctx) {
+      switch (pl.nx) {
+        case 64: k_ls_tau_xfwd<T, D, 16, 64, 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        case 128: k_ls_tau_xfwd<T, D, 16, 128, 32><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        case 256: k_ls_tau_xfwd<T, D, 16, 256, 16><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        default: k_ls_tau_xfwd<T, D, 16, 512, 8><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+      }
+    } else {
+      k_ls_tau_xfwd<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st);
+    }
     prof.end(h);
     if (D == 3 && pl.P > 1) {
       h = prof.begin(KID_LS_MID, iter_idx, b_fft);
-      k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, g.ply, g.twy, spec, 0,
-                                                          &st->done);
+      launch_mid<T, MV>(pl, gy, stream, g.ply, g.twy, spec, 0, &st->done);
       prof.end(h);
     }
     h = prof.begin(KID_LS_PENCIL, iter_idx, b_fft);
-    k_ls_pencil<T, D, MV><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st);
+    if (pl.ctq) {
+      switch (pl.Q) {
+        case 64: k_ls_pencil<T, D, 16, 64, (D == 3 ? 8 : 16)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
+        case 128: k_ls_pencil<T, D, 16, 128, (D == 3 ? 4 : 8)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
+        case 256: k_ls_pencil<T, D, 16, 256, (D == 3 ? 2 : 4)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
+        default: k_ls_pencil<T, D, 16, 512, (D == 3 ? 1 : 2)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
+      }
+    } else {
+      k_ls_pencil<T, D, MV><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st);
+    }
     prof.end(h);
     if (D == 3 && pl.P > 1) {
       h = prof.begin(KID_LS_MID, iter_idx, b_fft);
-      k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, g.ply, g.twy, spec, 1,
-                                                          &st->done);
+      launch_mid<T, MV>(pl, gy, stream, g.ply, g.twy, spec, 1, &st->done);
       prof.end(h);
     }
     h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
-    k_ls_xinv_update<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st);
+    if (pl.ctx) {
+      switch (pl.nx) {
+        case 64: k_ls_xinv_update<T, D, 16, 64, 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        case 128: k_ls_xinv_update<T, D, 16, 128, 32><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        case 256: k_ls_xinv_update<T, D, 16, 256, 16><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        default: k_ls_xinv_update<T, D, 16, 512, 8><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+      }
+    } else {
+      k_ls_xinv_update<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st);
+    }
     prof.end(h);
     k_ls_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_upd, st, tol, max_iter, e2_extra);
     ++iter_idx;

@@ -88,7 +88,9 @@ def test_newton_step_mixed_bc_parity(gpu):
     Pt = np.zeros((3, 3))
     Pm = np.zeros((3, 3), bool)
     Pm[1, 1] = Pm[2, 2] = True
-    ref, got, st = _step_pair(sh, 4, "svk", Fbar, P_target=Pt, P_mask=Pm, drv=(0, 0))
+    # reference default tolerances: at 1e-8 the reference's own macro loop
+    # ends in a line-search failure for this case (the oracle reproduces it)
+    ref, got, st = _step_pair(sh, 4, "svk", Fbar, P_target=Pt, P_mask=Pm, drv=(0, 0), tol=1e-6)
     print(f"[fs mixed] Fbar ref={np.diag(ref[4])} got={np.diag(got[4])}", flush=True)
     assert np.abs(got[4] - ref[4]).max() < 1e-10
     assert _rel(got[0], ref[0]) < 1e-9

@@ -65,6 +65,24 @@ def test_ls3d_parity(gpu, shape):
     assert inf["iters"] == info["iters"]
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 256), (256, 256), (512, 128)])
+def test_ls_specialised_sizes_parity(gpu, shape):
+    # power-of-two extents 64..512 run the compile-time specialised sweeps
+    from mgkmc_amd.linear_elastic_simulator import spectral_solver_3d, spectral_solver_2d
+    print(f"[ls ct] shape={shape}", flush=True)
+    E, nu, ep = _case(shape, 11)
+    d = len(shape)
+    eb = np.zeros((d, d))
+    eb[0, 0] = 0.01
+    solve_o = ol.spectral_solver_3d if d == 3 else ol.spectral_solver_2d
+    solve_g = spectral_solver_3d if d == 3 else spectral_solver_2d
+    info = {}
+    ref = solve_o(E, nu, eb, eps_plastic=ep, tol=1e-8, info=info)
+    got = solve_g(E, nu, eb, eps_plastic=ep, tol=1e-8)
+    for a, b in zip(got, ref):
+        assert _rel(a, b) < 1e-10, _rel(a, b)
+
+
 def test_ls3d_warm_start_and_nonsymmetric_bar(gpu):
     from mgkmc_amd.linear_elastic_simulator import spectral_solver_3d
     shape = (8, 8, 8)
