@@ -111,6 +111,48 @@ def gpu_kmc(n, events, warmup, precision):
     return {"kmc_steps_per_s": events / dt, "ls_iters_per_event": iters / events, "prof": prof}
 
 
+LANDAU = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9)
+
+
+def gpu_fs(n, increments):
+    """Finite-strain Newton-BiCGSTAB at n^3 (256^3 config recipe): mu ~ mu_unit *
+    lognormal(mean 1, sigma 0.1) (seed 2), kappa = 2.5 mu, Landau constants in Pa
+    with smooth strain capping (limit 0.10), simple shear F_xy ramped by 0.1% per
+    increment (the first `increments` of the 50 to 5%), tol_CG 1e-8, tol_NW 1e-5,
+    FP64.  voxel-matvec = one Krylov operator application over one voxel."""
+    import torch
+    from mgkmc_amd import _lib
+    from mgkmc_amd.finite_strain_simulator import FsSolver, material
+    rng = np.random.default_rng(2)
+    mu = MU_UNIT * np.exp(rng.normal(-0.005, 0.1, (n, n, n)))
+    lam = (2.5 - 2.0 / 3.0) * mu
+    s = FsSolver(lam, mu, precision=0)
+    mat = material("landau", **LANDAU, strain_capping_enabled=True, strain_capping_type="smooth",
+                   strain_capping_limit=0.10, mu_mean=float(mu.mean()))
+    eps = np.linspace(0.0, 0.05, 51)
+    Fb = np.eye(3)
+    s.step(Fb, mat, (0, 1), tol_NW=1e-5, tol_CG=1e-8)  # increment 0 (F = I), warm-up
+    torch.cuda.synchronize()
+    lib = _lib.load()
+    lib.mgk_profile_reset()
+    lib.mgk_profile_enable(1)
+    mv = 0
+    t0 = time.perf_counter()
+    for k in range(1, increments + 1):
+        Fb = np.eye(3)
+        Fb[0, 1] = eps[k]
+        inf = s.step(Fb, mat, (0, 1), tol_NW=1e-5, tol_CG=1e-8, max_NW=20)
+        mv += 2 * inf.krylov_iters + inf.newton_iters
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    lib.mgk_profile_enable(0)
+    prof = _lib.profile_read()
+    del s
+    torch.cuda.empty_cache()
+    return {"voxel_matvec_per_s": mv * n ** 3 / dt, "increments": increments, "matvecs": mv,
+            "seconds_per_increment": dt / increments, "prof": prof}
+
+
 def cpu_baseline(n_sample=128, iters=3):
     """The oracle (reference structure: materialised Gamma, per-component
     complex FFTs) on this host: LS iterations on a bounded 128^3 sample."""
@@ -141,6 +183,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kmc-events", type=int, default=20)
+    ap.add_argument("--fs-increments", type=int, default=2)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); multi-GPU sharding is design-only "
@@ -154,6 +197,7 @@ def main():
     for k in ("kmc_barrier_rates", "kmc_select"):
         if k in kmc32["prof"]:
             prof[k] = kmc32["prof"][k]
+    fsr = gpu_fs(a.n, a.fs_increments) if a.fs_increments > 0 else None
     dom = max(prof.items(), key=lambda kv: kv[1][0]) if prof else None
     roof = None
     if dom:
@@ -187,6 +231,12 @@ def main():
         "kmc_steps_per_s": {"fp32_ls": kmc32["kmc_steps_per_s"], "fp64_ls": kmc64["kmc_steps_per_s"],
                             "events": a.kmc_events, "ls_iters_per_event": kmc32["ls_iters_per_event"],
                             "workload": f"{a.n}^3, M=20, kT/E0=0.05, cold-start LS tol 1e-6 per event"},
+        "finite_strain_fp64": None if fsr is None else {
+            "voxel_matvec_per_s": fsr["voxel_matvec_per_s"], "increments": fsr["increments"],
+            "matvecs": fsr["matvecs"], "seconds_per_increment": fsr["seconds_per_increment"],
+            "per_kernel": {k: {"avg_ms": v[0] / v[2], "GB_s": (v[1] / v[2]) / (v[0] / v[2] * 1e-3) / 1e9}
+                           for k, v in fsr["prof"].items()},
+            "workload": f"{a.n}^3 Landau + smooth capping, simple shear F_xy 0.1%/increment, tol_CG 1e-8"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

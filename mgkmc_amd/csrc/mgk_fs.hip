@@ -433,14 +433,16 @@ struct FsGeom {
   int npencil;
 };
 
-template <typename T, int MAXV>
+template <typename T, int MAXV, int NN = 0, int BB = 0>
 __global__ __launch_bounds__(MGK_NT) void k_fs_xr2c(FsGeom<T> g, const T* __restrict__ f, cplx<T>* __restrict__ spec,
                                                     const int* __restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* tile = tw + g.nx;
-  const int nx = g.nx, P = g.P, Q = g.Q, logB = g.logBx, B = 1 << logB, ldb = B;
+  constexpr bool CT = NN > 0;
+  const int nx = CT ? NN : g.nx, P = g.P, Q = g.Q, logB = CT ? ct_log2(BB) : g.logBx, B = CT ? BB : 1 << logB;
+  const int ldb = B;
   const int q0 = blockIdx.x * B, p = blockIdx.y, pr = blockIdx.z;
   const int ca = 2 * pr, cb = (2 * pr + 1 < 9) ? 2 * pr + 1 : -1;
   const size_t NV = (size_t)g.NV;
@@ -457,14 +459,15 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xr2c(FsGeom<T> g, const T* __rest
     tile[x * ldb + b] = cmk(a, bb);
   }
   __syncthreads();
-  tile_fft<MAXV>(tile, Tile{nx, logB, ldb, 1, 0}, g.plx, tw, false);
+  if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, false);
+  else tile_fft<MAXV>(tile, Tile{nx, logB, ldb, 1, 0}, g.plx, tw, false);
   xsplit_store(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
                cb >= 0 ? spec + (size_t)cb * cs : (cplx<T>*)nullptr);
 }
 
 // c2r along x into real comps; optional fused dots: part[2*blk] += out.a,
 // part[2*blk+1] += out.out (FP64), blk over (x-tile, plane, pair)
-template <typename T, int MAXV>
+template <typename T, int MAXV, int NN = 0, int BB = 0>
 __global__ __launch_bounds__(MGK_NT) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* __restrict__ spec, T* __restrict__ out,
                                                     const T* __restrict__ dota, int selfdot,
                                                     double* __restrict__ partials, const int* __restrict__ skip) {
@@ -472,7 +475,9 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* 
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* tile = tw + g.nx;
-  const int nx = g.nx, P = g.P, Q = g.Q, logB = g.logBx, B = 1 << logB, ldb = B;
+  constexpr bool CT = NN > 0;
+  const int nx = CT ? NN : g.nx, P = g.P, Q = g.Q, logB = CT ? ct_log2(BB) : g.logBx, B = CT ? BB : 1 << logB;
+  const int ldb = B;
   const int q0 = blockIdx.x * B, p = blockIdx.y, pr = blockIdx.z;
   const int ca = 2 * pr, cb = (2 * pr + 1 < 9) ? 2 * pr + 1 : -1;
   const size_t NV = (size_t)g.NV;
@@ -480,7 +485,8 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* 
   for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
   xmerge_load(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
               cb >= 0 ? spec + (size_t)cb * cs : (const cplx<T>*)nullptr);
-  tile_fft<MAXV>(tile, Tile{nx, logB, ldb, 1, 0}, g.plx, tw, true);
+  if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, true);
+  else tile_fft<MAXV>(tile, Tile{nx, logB, ldb, 1, 0}, g.plx, tw, true);
   double acc[2] = {0.0, 0.0};
   for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
     const int b = e & (B - 1), x = e >> logB, q = q0 + b;
@@ -512,14 +518,15 @@ __device__ __forceinline__ int fbin_fs(int i, int n) { return (i <= (n - 1) / 2)
 __device__ __forceinline__ bool fnyq_fs(int i, int n) { return (n % 2 == 0) && (i == n / 2); }
 
 // z pencils of one row i (comps 3i..3i+2): FFT, (A xi)_i xi_j / |xi|^2 * scale, iFFT
-template <typename T, int MAXV>
+template <typename T, int MAXV, int NN = 0, int BB = 0>
 __global__ __launch_bounds__(MGK_NT) void k_fs_proj(FsGeom<T> g, cplx<T>* __restrict__ spec, T scale,
                                                     const int* __restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* tile = tw + g.Q;
-  const int Q = g.Q, P = g.P, logPP = g.logPP, PP = 1 << logPP;
+  constexpr bool CT = NN > 0;
+  const int Q = CT ? NN : g.Q, P = g.P, logPP = CT ? ct_log2(BB) : g.logPP, PP = CT ? BB : 1 << logPP;
   const int ldb = PP, cstride = Q * PP;
   const int pid0 = blockIdx.x * PP;
   const int row = blockIdx.y;
@@ -537,7 +544,8 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_proj(FsGeom<T> g, cplx<T>* __rest
   }
   __syncthreads();
   const Tile tl{Q, logPP, ldb, 3, cstride};
-  tile_fft<MAXV>(tile, tl, g.plq, tw, false);
+  if constexpr (CT) tile_fft_ct<NN, BB, 3>(tile, tw, false);
+  else tile_fft<MAXV>(tile, tl, g.plq, tw, false);
   for (int e = threadIdx.x; e < Q * PP; e += MGK_NT) {
     const int pp = e & (PP - 1), q = e >> logPP;
     const int pid = pid0 + pp;
@@ -566,7 +574,8 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_proj(FsGeom<T> g, cplx<T>* __rest
     for (int c = 0; c < 3; ++c) tile[c * cstride + q * ldb + pp] = o[c];
   }
   __syncthreads();
-  tile_fft<MAXV>(tile, tl, g.plq, tw, true);
+  if constexpr (CT) tile_fft_ct<NN, BB, 3>(tile, tw, true);
+  else tile_fft<MAXV>(tile, tl, g.plq, tw, true);
   for (int e = threadIdx.x; e < 3 * chunk; e += MGK_NT) {
     const int c = e / chunk;
     const int r = e - c * chunk;
@@ -882,6 +891,7 @@ struct FsPlan {
   int nx, P, Q, nxh;
   long long NV;
   int logBx, logBy, logPP;
+  int ctx, cty, ctq;
   size_t lds_x, lds_y, lds_q;
   int nqx, nqy, npb;
   size_t fld, spec;  // bytes per 9-comp field and spectrum
@@ -920,6 +930,30 @@ int fs_plan(const int64_t* shape, FsPlan* pl) {
   pl->logPP = mgk_ilog2(PP);
   pl->lds_q = (size_t)(pl->Q + 3 * pl->Q * PP) * cb;
   if (pl->lds_x > lim || pl->lds_y > lim || pl->lds_q > lim) return MGK_ERR_ARG;
+  {
+    const int CTT = 4096;
+    auto ct_ok = [](int n) { return n == 64 || n == 128 || n == 256 || n == 512; };
+    pl->ctx = pl->cty = pl->ctq = 0;
+    if (ct_ok(pl->nx) && pl->Q >= CTT / pl->nx && (size_t)(pl->nx + CTT) * cb <= lim) {
+      pl->ctx = 1;
+      pl->logBx = mgk_ilog2(CTT / pl->nx);
+      pl->lds_x = (size_t)(pl->nx + CTT) * cb;
+    }
+    if (ct_ok(pl->P) && pl->Q >= CTT / pl->P && (size_t)(pl->P + CTT) * cb <= lim) {
+      pl->cty = 1;
+      pl->logBy = mgk_ilog2(CTT / pl->P);
+      pl->lds_y = (size_t)(pl->P + CTT) * cb;
+    }
+    if (ct_ok(pl->Q)) {
+      int pp = 1;
+      while (2 * pp * 3 * pl->Q <= CTT) pp *= 2;
+      if (npen >= pp && (size_t)(pl->Q + 3 * pl->Q * pp) * cb <= lim) {
+        pl->ctq = 1;
+        pl->logPP = mgk_ilog2(pp);
+        pl->lds_q = (size_t)(pl->Q + 3 * pl->Q * pp) * cb;
+      }
+    }
+  }
   pl->nqx = (pl->Q + (1 << pl->logBx) - 1) >> pl->logBx;
   pl->nqy = (pl->Q + (1 << pl->logBy) - 1) >> pl->logBy;
   pl->npb = (npen + (1 << pl->logPP) - 1) >> pl->logPP;
@@ -948,33 +982,82 @@ struct FsCtx {
   int kry_it;
 };
 
+template <typename T, int MV>
+void fs_launch_mid(const FsPlan& pl, hipStream_t s, const FftPlan& ply, const cplx<T>* twy, cplx<T>* spec, int inv,
+                   const int* skip) {
+  const dim3 gy(pl.nqy, pl.nxh, 9);
+  if (pl.cty) {
+    switch (pl.P) {
+      case 64: k_fft_mid<T, 16, 64, 64><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+      case 128: k_fft_mid<T, 16, 128, 32><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+      case 256: k_fft_mid<T, 16, 256, 16><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+      default: k_fft_mid<T, 16, 512, 8><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+    }
+  } else {
+    k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip);
+  }
+}
+
 template <typename T>
 void fs_project(FsCtx<T>& c, const T* in, T* out, T scale, const T* dota, int selfdot, double* partials,
                 const int* skip) {
   constexpr int MV = fs_maxv<T>();
   const FsPlan& pl = c.pl;
   int h = c.prof->begin(KID_FS_XFWD, c.kry_it, (double)pl.NV * 9 * sizeof(T) + pl.spec);
-  k_fs_xr2c<T, MV><<<dim3(pl.nqx, pl.P, 5), MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip);
+  {
+    const dim3 gx(pl.nqx, pl.P, 5);
+    if (pl.ctx) {
+      switch (pl.nx) {
+        case 64: k_fs_xr2c<T, 16, 64, 64><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
+        case 128: k_fs_xr2c<T, 16, 128, 32><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
+        case 256: k_fs_xr2c<T, 16, 256, 16><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
+        default: k_fs_xr2c<T, 16, 512, 8><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
+      }
+    } else {
+      k_fs_xr2c<T, MV><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip);
+    }
+  }
   c.prof->end(h);
   if (pl.P > 1) {
     h = c.prof->begin(KID_FS_MID, c.kry_it, 2.0 * pl.spec);
-    k_fft_mid<T, MV><<<dim3(pl.nqy, pl.nxh, 9), MGK_NT, pl.lds_y, c.s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, c.g.ply,
-                                                                          c.g.twy, c.spec, 0, skip);
+    fs_launch_mid<T, MV>(pl, c.s, c.g.ply, c.g.twy, c.spec, 0, skip);
     c.prof->end(h);
   }
   h = c.prof->begin(KID_FS_PENCIL, c.kry_it, 2.0 * pl.spec);
-  k_fs_proj<T, MV><<<dim3(pl.npb, 3), MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip);
+  {
+    const dim3 gq(pl.npb, 3);
+    if (pl.ctq) {
+      switch (pl.Q) {
+        case 64: k_fs_proj<T, 16, 64, 16><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip); break;
+        case 128: k_fs_proj<T, 16, 128, 8><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip); break;
+        case 256: k_fs_proj<T, 16, 256, 4><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip); break;
+        default: k_fs_proj<T, 16, 512, 2><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip); break;
+      }
+    } else {
+      k_fs_proj<T, MV><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip);
+    }
+  }
   c.prof->end(h);
   if (pl.P > 1) {
     h = c.prof->begin(KID_FS_MID, c.kry_it, 2.0 * pl.spec);
-    k_fft_mid<T, MV><<<dim3(pl.nqy, pl.nxh, 9), MGK_NT, pl.lds_y, c.s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, c.g.ply,
-                                                                          c.g.twy, c.spec, 1, skip);
+    fs_launch_mid<T, MV>(pl, c.s, c.g.ply, c.g.twy, c.spec, 1, skip);
     c.prof->end(h);
   }
   h = c.prof->begin(KID_FS_XINV, c.kry_it,
                     pl.spec + (double)pl.NV * 9 * sizeof(T) * (1 + (dota ? 1 : 0)));
-  k_fs_xc2r<T, MV><<<dim3(pl.nqx, pl.P, 5), MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials,
-                                                                      skip);
+  {
+    const dim3 gx(pl.nqx, pl.P, 5);
+    if (pl.ctx) {
+      switch (pl.nx) {
+        case 64: k_fs_xc2r<T, 16, 64, 64><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
+        case 128: k_fs_xc2r<T, 16, 128, 32><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
+        case 256: k_fs_xc2r<T, 16, 256, 16><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
+        default: k_fs_xc2r<T, 16, 512, 8><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
+      }
+    } else {
+      k_fs_xc2r<T, MV><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip);
+    }
+  }
   c.prof->end(h);
 }
 

@@ -69,7 +69,7 @@ def _step_pair(sh, seed, model, Fbar, Fp=None, P_target=None, P_mask=None, drv=(
 
 
 @pytest.mark.parametrize("sh,model", [((6, 4, 5), "svk"), ((8, 8, 8), "landau"), ((16, 16, 16), "landau"),
-                                      ((12, 6, 10), "svk")])
+                                      ((12, 6, 10), "svk"), ((64, 8, 64), "landau"), ((8, 64, 64), "svk")])
 def test_newton_step_parity(gpu, sh, model):
     Fbar = np.eye(3)
     Fbar[0, 1] = 0.02
