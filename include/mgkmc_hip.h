@@ -94,6 +94,7 @@ typedef struct {
   int fallthrough;  /* nonzero if the running sum never reached r             */
   double r;         /* the draw U(0, total) actually used                     */
   double cum_before, cum_at;  /* running sum before / including the event    */
+  double q;         /* barrier of the selected mode (is_instab test)          */
 } mgk_kmc_sel;
 
 int mgk_kmc_workspace_bytes(long long nvox, size_t* bytes);
@@ -107,12 +108,15 @@ int mgk_kmc_barrier_rates(int ndim, long long nvox, int M, int sig_kind, int cat
                           const double* prev, const double* Tfield, const mgk_kmc_params* params, double* Q,
                           void* workspace, void* stream, double* total_out);
 
-/* Event selection after mgk_kmc_barrier_rates: r = total * u, first (voxel, mode)
- * in (x,y,z,m) order whose running FP64 sum reaches r.  Replaces select_event +
- * decode_index (mgkmc/stz/kmc.py:76-103; 2D kmc_simulator_functions.py:58-76). */
-int mgk_kmc_select(int ndim, long long nvox, int M, int sig_kind, int cat_kind, double u, const double* Q,
-                   const double* Tfield, const mgk_kmc_params* params, void* workspace, void* stream,
-                   mgk_kmc_sel* out);
+/* Event selection after mgk_kmc_barrier_rates (same inputs): r = total * u, first
+ * (voxel, mode) in (x,y,z,m) order whose running FP64 sum reaches r; the voxel's
+ * barriers are recomputed bit-identically, so Q need not be stored.  Replaces
+ * select_event + decode_index (mgkmc/stz/kmc.py:76-103; 2D
+ * kmc_simulator_functions.py:58-76). */
+int mgk_kmc_select(int ndim, long long nvox, int M, int sig_kind, int cat_kind, double u, const void* sig,
+                   const double* Q0, const double* cat, const double* soft, const double* tlast,
+                   const double* prev, const double* Tfield, const mgk_kmc_params* params, void* workspace,
+                   void* stream, mgk_kmc_sel* out);
 
 /* compute_rates on a Q field in the reference (v, m) layout: full per-mode
  * rates (-1 = not an event), compacted rates/indices in reference order, total

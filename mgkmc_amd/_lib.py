@@ -35,7 +35,8 @@ class MgkKmcParams(ctypes.Structure):
 
 class MgkKmcSel(ctypes.Structure):
     _fields_ = [("item", ctypes.c_longlong), ("mode", ctypes.c_int), ("fallthrough", ctypes.c_int),
-                ("r", ctypes.c_double), ("cum_before", ctypes.c_double), ("cum_at", ctypes.c_double)]
+                ("r", ctypes.c_double), ("cum_before", ctypes.c_double), ("cum_at", ctypes.c_double),
+                ("q", ctypes.c_double)]
 
 
 class MgkFsMaterial(ctypes.Structure):
@@ -71,8 +72,8 @@ SIGNATURES = {
     "mgk_kmc_workspace_bytes": (_c_i, [ctypes.c_longlong, ctypes.POINTER(_c_sz)]),
     "mgk_kmc_barrier_rates": (_c_i, [_c_i, ctypes.c_longlong, _c_i, _c_i, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p,
                                      _c_p, _c_p, ctypes.POINTER(MgkKmcParams), _c_p, _c_p, _c_p, _c_dp]),
-    "mgk_kmc_select": (_c_i, [_c_i, ctypes.c_longlong, _c_i, _c_i, _c_i, _c_d, _c_p, _c_p,
-                              ctypes.POINTER(MgkKmcParams), _c_p, _c_p, ctypes.POINTER(MgkKmcSel)]),
+    "mgk_kmc_select": (_c_i, [_c_i, ctypes.c_longlong, _c_i, _c_i, _c_i, _c_d, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
+                              _c_p, ctypes.POINTER(MgkKmcParams), _c_p, _c_p, ctypes.POINTER(MgkKmcSel)]),
     "mgk_kmc_rates_from_q": (_c_i, [ctypes.c_longlong, _c_i, _c_p, _c_p, ctypes.POINTER(MgkKmcParams), _c_p,
                                     _c_p, _c_p, _c_p, _c_p, ctypes.POINTER(ctypes.c_longlong), _c_dp]),
     "mgk_kmc_select_array": (_c_i, [ctypes.c_longlong, _c_p, _c_d, _c_p, _c_p, ctypes.POINTER(MgkKmcSel)]),

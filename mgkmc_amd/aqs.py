@@ -152,7 +152,10 @@ class ThermalSimulation:
 
     @property
     def Q(self):
+        """Barriers of the current state (evaluated on access; the event loop
+        does not store them)."""
         from .kmc_device import modes_from_device
+        self.kmc.barrier_rates(self.ls.sig, self.sig_kind, self._params(), write_q=True)
         return modes_from_device(self.kmc.Q, self.grid_shape)
 
     @property
@@ -193,7 +196,8 @@ class ThermalSimulation:
         """One pass of the inner KMC loop body (mgkmc/aqs.py:852-1087).
         Returns (event or None, remaining_time)."""
         p = self._params()
-        total = self.kmc.barrier_rates(self.ls.sig, self.sig_kind, p)
+        # barriers are not stored: the selection recomputes the chosen voxel's
+        total = self.kmc.barrier_rates(self.ls.sig, self.sig_kind, p, write_q=False)
         if total > 0:
             u = np.random.uniform()
             t_wait = -np.log(u) / total
@@ -202,10 +206,9 @@ class ThermalSimulation:
                 self.eps_macro += strain_unit_tensor * (self.strain_rate * t_wait)
                 remaining_time -= t_wait
                 us = _kmcmod.SELECTION_STREAM.random_sample()
-                v, m = self.kmc.select(us, self.sig_kind, p)
+                v, m = self.kmc.select(us, self.ls.sig, self.sig_kind, p)
                 x, y, z = (int(c) for c in self.kmc.voxel_coords(v))
-                qsel = float(self.kmc.Q[m, v].item())
-                is_instab = qsel <= self.stability_threshold
+                is_instab = self.kmc.sel.q <= self.stability_threshold
                 self.kmc.flip(v, m, self.time, self.jp, self.jt, self.softening_cap, self.neighbor_softening_fraction)
                 newcat = newq0 = None
                 if self.redraw_directions or self.redraw_barriers:

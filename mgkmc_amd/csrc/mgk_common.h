@@ -307,6 +307,20 @@ __device__ __forceinline__ void tile_fft_ct(cplx<T>* __restrict__ buf, const cpl
 }
 
 // ---------------------------------------------------------------------------
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8 XCDs
+// of the MI355X, each with its own L2.  Mapping the linear workgroup id w to
+// tile (w % 8) * per + w / 8 hands every XCD a contiguous range of tiles, so
+// neighbouring tiles that share 128-byte lines are served by one L2 instead of
+// being fetched from HBM once per XCD.
+// ---------------------------------------------------------------------------
+#define MGK_NXCD 8
+__device__ __forceinline__ int xcd_tile(int w, int n) {
+  const int per = n / MGK_NXCD;
+  if (w >= per * MGK_NXCD) return w;
+  return (w % MGK_NXCD) * per + w / MGK_NXCD;
+}
+
+// ---------------------------------------------------------------------------
 // Block reductions in FP64 (wavefront = 64 lanes on CDNA).
 // Deterministic: fixed shuffle tree, then fixed order over waves.
 // ---------------------------------------------------------------------------

@@ -24,6 +24,8 @@
 #include <vector>
 #include <string.h>
 #include <cmath>
+#include <map>
+#include <mutex>
 
 struct FsMat {
   int model;  // 0 svk, 1 landau, 2 murnaghan
@@ -443,7 +445,9 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xr2c(FsGeom<T> g, const T* __rest
   constexpr bool CT = NN > 0;
   const int nx = CT ? NN : g.nx, P = g.P, Q = g.Q, logB = CT ? ct_log2(BB) : g.logBx, B = CT ? BB : 1 << logB;
   const int ldb = B;
-  const int q0 = blockIdx.x * B, p = blockIdx.y, pr = blockIdx.z;
+  const int tid_lin = xcd_tile((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x,
+                               gridDim.x * gridDim.y * gridDim.z);
+  const int q0 = (tid_lin % gridDim.x) * B, p = (tid_lin / gridDim.x) % gridDim.y, pr = tid_lin / (gridDim.x * gridDim.y);
   const int ca = 2 * pr, cb = (2 * pr + 1 < 9) ? 2 * pr + 1 : -1;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
@@ -478,7 +482,9 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* 
   constexpr bool CT = NN > 0;
   const int nx = CT ? NN : g.nx, P = g.P, Q = g.Q, logB = CT ? ct_log2(BB) : g.logBx, B = CT ? BB : 1 << logB;
   const int ldb = B;
-  const int q0 = blockIdx.x * B, p = blockIdx.y, pr = blockIdx.z;
+  const int tid_lin = xcd_tile((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x,
+                               gridDim.x * gridDim.y * gridDim.z);
+  const int q0 = (tid_lin % gridDim.x) * B, p = (tid_lin / gridDim.x) % gridDim.y, pr = tid_lin / (gridDim.x * gridDim.y);
   const int ca = 2 * pr, cb = (2 * pr + 1 < 9) ? 2 * pr + 1 : -1;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
@@ -1079,6 +1085,19 @@ void fs_matvec(FsCtx<T>& c, const T* Fcur, const T* in, T* out, const T* dota, i
   fs_project(c, c.dP, out, T(1.0 / (double)pl.NV), dota, selfdot, partials, skip);
 }
 
+// last Krylov iteration count per workspace (-1 query)
+int kry_hint(const void* key, int iters) {
+  static std::mutex mu;
+  static std::map<const void*, int> last;
+  std::lock_guard<std::mutex> lk(mu);
+  if (iters >= 0) {
+    last[key] = iters;
+    return iters;
+  }
+  auto it = last.find(key);
+  return it == last.end() ? 0 : it->second;
+}
+
 // SciPy bicgstab(A, b, x0=None, rtol, maxiter) -> c.x
 template <typename T>
 int fs_bicgstab(FsCtx<T>& c, const T* Fcur, double rtol, int maxiter, int* iters, int* status) {
@@ -1114,8 +1133,11 @@ int fs_bicgstab(FsCtx<T>& c, const T* Fcur, double rtol, int maxiter, int* iters
   MGK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   MGK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   const int start_it = c.kry_it;
-  int enq = 0, chunk = 4, slot = 0;
-  bool pending = false, stop = false;
+  // first chunk: the iteration count of the previous solve on this workspace
+  // (Newton steps of one load path need similar Krylov counts)
+  const int hint = kry_hint(c.kst, -1);
+  int enq = 0, chunk = hint > 0 ? hint + 1 : 4, slot = 0;
+  bool pending = false, stop = false, first = true;
   while (!stop && enq < maxiter) {
     const int k = std::min(chunk, maxiter - enq);
     for (int i = 0; i < k; ++i) enqueue();
@@ -1129,7 +1151,8 @@ int fs_bicgstab(FsCtx<T>& c, const T* Fcur, double rtol, int maxiter, int* iters
     }
     pending = true;
     slot ^= 1;
-    chunk = std::min(chunk * 2, 32);
+    chunk = first ? 2 : std::min(chunk * 2, 32);
+    first = false;
   }
   MGK_HIP(hipMemcpyAsync(&hs[0], c.kst, sizeof(KryState), hipMemcpyDeviceToHost, c.s));
   MGK_HIP(hipStreamSynchronize(c.s));
@@ -1137,6 +1160,7 @@ int fs_bicgstab(FsCtx<T>& c, const T* Fcur, double rtol, int maxiter, int* iters
   (void)hipEventDestroy(ev[1]);
   *iters = hs[0].iters;
   *status = hs[0].status;
+  kry_hint(c.kst, hs[0].iters);
   // launches queued after the stop exited at once: not timed
   c.prof->invalidate(start_it + hs[0].iters + 1, start_it + enq);
   return MGK_OK;

@@ -57,26 +57,21 @@ __device__ __forceinline__ int sym_index(int i, int j) {
 
 #pragma clang fp contract(off)
 
-// Fused barrier + rate kernel: one thread per voxel (KMC_VPT voxels/thread),
-// per-voxel FP64 rate sums and per-block sums.
+// Per-voxel barrier state: sigma in (i, j) order, softening, previous
+// direction, beta.  mode_q() is the barrier of one mode; the fused pass and the
+// selection both call it, so their Q values are bit-identical.
 template <int D, int CK, int SIGK, typename TS>
-__global__ __launch_bounds__(MGK_NT) void k_kmc_barrier_rates(
-    size_t NV, int M, const TS* __restrict__ sig, const double* __restrict__ Q0, const double* __restrict__ cat,
-    const double* __restrict__ soft, const double* __restrict__ tlast, const double* __restrict__ prev,
-    const double* __restrict__ Tfield, KmcParams P, double* __restrict__ Q, double* __restrict__ Rv,
-    double* __restrict__ blocksums) {
-  constexpr int DD = D * D;
-  constexpr int C = (D == 3) ? 6 : 3;
-  const double kB = 8.617e-5;
-  const size_t MNV = (size_t)M * NV;
-  __shared__ double red[MGK_NT / 64];
-  double bsum = 0.0;
-  const size_t base = (size_t)blockIdx.x * KMC_VB;
-  for (int s = 0; s < KMC_VPT; ++s) {
-    const size_t v = base + (size_t)s * MGK_NT + threadIdx.x;
-    if (v >= NV) continue;
-    // sigma in (i, j) order
-    double sg[DD];
+struct KmcVox {
+  double sg[D * D];
+  double pv[D * D];
+  double g_base, beta;
+  bool dir;
+
+  __device__ __forceinline__ void load(size_t v, size_t NV, const TS* __restrict__ sig,
+                                       const double* __restrict__ soft, const double* __restrict__ tlast,
+                                       const double* __restrict__ prev, const double* __restrict__ Tfield,
+                                       const KmcParams& P) {
+    const double kB = 8.617e-5;
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
@@ -84,8 +79,7 @@ __global__ __launch_bounds__(MGK_NT) void k_kmc_barrier_rates(
         if (SIGK == KMC_SIG_SYM) sg[i * D + j] = (double)sig[(size_t)sym_index<D>(i, j) * NV + v];
         else sg[i * D + j] = (double)sig[(size_t)(i * D + j) * NV + v];
       }
-    (void)C;
-    // transient softening
+    // transient softening (barriers.py:37-49)
     const double g_t = soft[NV + v];
     const double t_last = tlast[v];
     double g_tc = 0.0;
@@ -100,62 +94,92 @@ __global__ __launch_bounds__(MGK_NT) void k_kmc_barrier_rates(
         g_tc = g_t * exp(-dt / P.tau);
       }
     }
-    double g_base = soft[v] + g_tc;
+    g_base = soft[v] + g_tc;
     if (P.cap > 0 && g_base > P.cap) g_base = P.cap;
-    const bool dir = P.scheme == 1;
-    double pv[DD];
-    double npv2 = 0.0;
+    dir = P.scheme == 1;
     if (dir) {
 #pragma unroll
-      for (int k = 0; k < DD; ++k) pv[k] = prev[(size_t)k * NV + v];
+      for (int k = 0; k < D * D; ++k) pv[k] = prev[(size_t)k * NV + v];
     }
     const double T = Tfield ? Tfield[v] : P.temperature;
-    const double beta = (T > 0.0) ? 1.0 / (kB * T) : (double)INFINITY;
+    beta = (T > 0.0) ? 1.0 / (kB * T) : (double)INFINITY;
+  }
+
+  __device__ __forceinline__ double mode_q(int m, size_t v, size_t NV, size_t MNV, const double* __restrict__ Q0,
+                                           const double* __restrict__ cat, const KmcParams& P) const {
+    constexpr int DD = D * D;
+    const size_t mo = (size_t)m * NV + v;
+    double c[DD];
+    if (CK == 2) {
+      const double gxx = cat[mo], gxy = cat[MNV + mo];
+#pragma unroll
+      for (int k = 0; k < DD; ++k) c[k] = 0.0;
+      c[0] = gxx;
+      c[1] = gxy;
+      c[D] = gxy;
+      c[D + 1] = -gxx;
+    } else {
+#pragma unroll
+      for (int k = 0; k < DD; ++k) c[k] = cat[(size_t)k * MNV + mo];
+    }
+    double modifier = 1.0;
+    if (dir) {
+      double dp = 0.0, nm = 0.0, npv2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < DD; ++k) {
+        dp = dp + c[k] * pv[k];
+        nm = nm + c[k] * c[k];
+        npv2 = npv2 + pv[k] * pv[k];
+      }
+      const double n_prev = sqrt(npv2), n_mode = sqrt(nm);
+      if (!(n_prev < 1e-12 || n_mode < 1e-12)) {
+        const double ct = dp / (n_mode * n_prev);
+        const double a = 1.0 + ct;
+        modifier = (a * a) / 4.0;
+      }
+    }
+    const double g_eff = modifier * g_base;
+    double w = 0.0;
+#pragma unroll
+    for (int k = 0; k < DD; ++k) w = w + sg[k] * c[k];
+    const double w_val = ((0.5 * P.volume) * (w / 1e9)) * 6.241509;
+    return Q0[mo] * exp(-g_eff) - w_val;
+  }
+};
+
+// Arrhenius rate of a mode (kmc.py:52-60); 0 and *inc = false for a non-event
+__device__ __forceinline__ double kmc_rate(double q, double beta, const KmcParams& P, bool* inc) {
+  *inc = !P.cascade || q > 0;
+  if (!*inc) return 0.0;
+  return (q <= 0) ? (P.rate_volume * P.nu0) : ((P.rate_volume * P.nu0) * exp(-q * beta));
+}
+
+// Fused barrier + rate kernel: one thread per voxel (KMC_VPT voxels/thread),
+// per-voxel FP64 rate sums (modes in order) and per-segment sums.  Q is
+// written only when requested (the drop-in compute_barrier needs it; the
+// device KMC loop does not).
+template <int D, int CK, int SIGK, typename TS>
+__global__ __launch_bounds__(MGK_NT) void k_kmc_barrier_rates(
+    size_t NV, int M, const TS* __restrict__ sig, const double* __restrict__ Q0, const double* __restrict__ cat,
+    const double* __restrict__ soft, const double* __restrict__ tlast, const double* __restrict__ prev,
+    const double* __restrict__ Tfield, KmcParams P, double* __restrict__ Q, double* __restrict__ Rv,
+    double* __restrict__ blocksums) {
+  const size_t MNV = (size_t)M * NV;
+  __shared__ double red[MGK_NT / 64];
+  double bsum = 0.0;
+  const size_t base = (size_t)blockIdx.x * KMC_VB;
+  for (int s = 0; s < KMC_VPT; ++s) {
+    const size_t v = base + (size_t)s * MGK_NT + threadIdx.x;
+    if (v >= NV) continue;
+    KmcVox<D, CK, SIGK, TS> vx;
+    vx.load(v, NV, sig, soft, tlast, prev, Tfield, P);
     double rsum = 0.0;
     for (int m = 0; m < M; ++m) {
-      const size_t mo = (size_t)m * NV + v;
-      double c[DD];
-      if (CK == 2) {
-        const double gxx = cat[mo], gxy = cat[MNV + mo];
-#pragma unroll
-        for (int k = 0; k < DD; ++k) c[k] = 0.0;
-        c[0] = gxx;
-        c[1] = gxy;
-        c[D] = gxy;
-        c[D + 1] = -gxx;
-      } else {
-#pragma unroll
-        for (int k = 0; k < DD; ++k) c[k] = cat[(size_t)k * MNV + mo];
-      }
-      double modifier = 1.0;
-      if (dir) {
-        double dp = 0.0, nm = 0.0;
-        npv2 = 0.0;
-#pragma unroll
-        for (int k = 0; k < DD; ++k) {
-          dp = dp + c[k] * pv[k];
-          nm = nm + c[k] * c[k];
-          npv2 = npv2 + pv[k] * pv[k];
-        }
-        const double n_prev = sqrt(npv2), n_mode = sqrt(nm);
-        if (!(n_prev < 1e-12 || n_mode < 1e-12)) {
-          const double ct = dp / (n_mode * n_prev);
-          const double a = 1.0 + ct;
-          modifier = (a * a) / 4.0;
-        }
-      }
-      const double g_eff = modifier * g_base;
-      double w = 0.0;
-#pragma unroll
-      for (int k = 0; k < DD; ++k) w = w + sg[k] * c[k];
-      const double w_val = ((0.5 * P.volume) * (w / 1e9)) * 6.241509;
-      const double q = Q0[mo] * exp(-g_eff) - w_val;
-      if (Q) Q[mo] = q;
-      const bool inc = !P.cascade || q > 0;
-      if (inc) {
-        const double r = (q <= 0) ? (P.rate_volume * P.nu0) : ((P.rate_volume * P.nu0) * exp(-q * beta));
-        rsum = rsum + r;
-      }
+      const double q = vx.mode_q(m, v, NV, MNV, Q0, cat, P);
+      if (Q) Q[(size_t)m * NV + v] = q;
+      bool inc;
+      const double r = kmc_rate(q, vx.beta, P, &inc);
+      if (inc) rsum = rsum + r;
     }
     Rv[v] = rsum;
     bsum += rsum;
@@ -230,6 +254,7 @@ struct KmcSel {
   double r;
   double cum_before;  // running sum before the selected event
   double cum_at;      // running sum including it
+  double q;           // barrier of the selected mode
 };
 
 // Find the first index i with cum[i] >= r over `n` items with per-item sums
@@ -347,15 +372,15 @@ __device__ void first_reaching(const double* __restrict__ a, size_t n, const dou
 }
 
 // Selection for the fused path: voxel via the two-level search over Rv, then
-// the mode by recomputing that voxel's rates in order.  r = total * u with
-// total read from `total` (device) so the host and device use one value.
+// the mode by recomputing that voxel's barriers and rates in order with the
+// same code as the fused pass.  r = total * u with total read from `total`
+// (device) so the host and device use one value.
 template <int D, int CK, int SIGK, typename TS>
-__global__ __launch_bounds__(MGK_NT) void k_kmc_select_voxel(size_t NV, int M, const double* __restrict__ Rv,
-                                                              const double* __restrict__ bs, int nb,
-                                                              const double* __restrict__ total, double u,
-                                                              const double* __restrict__ Q,
-                                                              const double* __restrict__ Tfield, KmcParams P,
-                                                              KmcSel* __restrict__ out) {
+__global__ __launch_bounds__(MGK_NT) void k_kmc_select_voxel(
+    size_t NV, int M, const double* __restrict__ Rv, const double* __restrict__ bs, int nb,
+    const double* __restrict__ total, double u, const TS* __restrict__ sig, const double* __restrict__ Q0,
+    const double* __restrict__ cat, const double* __restrict__ soft, const double* __restrict__ tlast,
+    const double* __restrict__ prev, const double* __restrict__ Tfield, KmcParams P, KmcSel* __restrict__ out) {
   __shared__ long long item;
   __shared__ double before;
   double r = (*total) * u;
@@ -363,7 +388,7 @@ __global__ __launch_bounds__(MGK_NT) void k_kmc_select_voxel(size_t NV, int M, c
   first_reaching(Rv, NV, bs, nb, r, &item, &before);
   __syncthreads();
   if (threadIdx.x != 0) return;
-  const double kB = 8.617e-5;
+  const size_t MNV = (size_t)M * NV;
   long long v = item;
   double c = before;
   int fall = 0;
@@ -375,27 +400,30 @@ __global__ __launch_bounds__(MGK_NT) void k_kmc_select_voxel(size_t NV, int M, c
     while (v > 0 && !(Rv[v] > 0.0)) --v;
     c = 0.0;
   }
-  const double T = Tfield ? Tfield[v] : P.temperature;
-  const double beta = (T > 0.0) ? 1.0 / (kB * T) : (double)INFINITY;
+  KmcVox<D, CK, SIGK, TS> vx;
+  vx.load((size_t)v, NV, sig, soft, tlast, prev, Tfield, P);
   int mode = -1, last_inc = -1;
-  double cb = c, ca = c;
+  double cb = c, ca = c, qsel = 0.0, qlast = 0.0;
   for (int m = 0; m < M; ++m) {
-    const double q = Q[(size_t)m * NV + v];
-    const bool inc = !P.cascade || q > 0;
+    const double q = vx.mode_q(m, (size_t)v, NV, MNV, Q0, cat, P);
+    bool inc;
+    const double rr = kmc_rate(q, vx.beta, P, &inc);
     if (!inc) continue;
     last_inc = m;
-    const double rr = (q <= 0) ? (P.rate_volume * P.nu0) : ((P.rate_volume * P.nu0) * exp(-q * beta));
+    qlast = q;
     const double nc = c + rr;
     if (!fall && nc >= r) {
       mode = m;
       cb = c;
       ca = nc;
+      qsel = q;
       break;
     }
     c = nc;
   }
   if (mode < 0) {
     mode = last_inc < 0 ? 0 : last_inc;
+    qsel = qlast;
     ca = c;
     fall = fall | 2;
   }
@@ -405,6 +433,7 @@ __global__ __launch_bounds__(MGK_NT) void k_kmc_select_voxel(size_t NV, int M, c
   out->r = r;
   out->cum_before = cb;
   out->cum_at = ca;
+  out->q = qsel;
 }
 
 // Selection over a plain rates array (drop-in select_event): r given.
@@ -422,6 +451,7 @@ __global__ __launch_bounds__(MGK_NT) void k_kmc_select_array(size_t n, const dou
     out->r = r;
     out->cum_before = before;
     out->cum_at = item < 0 ? before : before + a[item];
+    out->q = 0.0;
   }
 }
 
@@ -639,6 +669,7 @@ typedef struct {
   double r;
   double cum_before;
   double cum_at;
+  double q;
 } mgk_kmc_sel;
 }
 
@@ -703,10 +734,12 @@ static void launch_rates(size_t NV, int M, const void* sig, const double* Q0, co
 }
 
 template <int D, int CK, int SIGK, typename TS>
-static void launch_select(size_t NV, int M, double u, const double* Q, const double* Tf, const KmcParams& P,
-                          const KmcWs& w, hipStream_t s) {
+static void launch_select(size_t NV, int M, double u, const void* sig, const double* Q0, const double* cat,
+                          const double* soft, const double* tlast, const double* prev, const double* Tf,
+                          const KmcParams& P, const KmcWs& w, hipStream_t s) {
   const int nb = kmc_nblocks(NV);
-  k_kmc_select_voxel<D, CK, SIGK, TS><<<1, MGK_NT, 0, s>>>(NV, M, w.Rv, w.bs, nb, w.total, u, Q, Tf, P, w.sel);
+  k_kmc_select_voxel<D, CK, SIGK, TS><<<1, MGK_NT, 0, s>>>(NV, M, w.Rv, w.bs, nb, w.total, u, (const TS*)sig, Q0,
+                                                           cat, soft, tlast, prev, Tf, P, w.sel);
 }
 
 // sig_kind: 0 sym FP64, 1 sym FP32, 2 general FP64.  cat_kind: 0 planar, 1 general.
@@ -772,10 +805,12 @@ extern "C" int mgk_kmc_barrier_rates(int ndim, long long nvox, int M, int sig_ki
   return MGK_OK;
 }
 
-extern "C" int mgk_kmc_select(int ndim, long long nvox, int M, int sig_kind, int cat_kind, double u, const double* Q,
-                              const double* Tfield, const mgk_kmc_params* params, void* workspace, void* stream,
-                              mgk_kmc_sel* out) {
-  if ((ndim != 2 && ndim != 3) || nvox < 1 || M < 1 || !Q || !params || !workspace || !out) {
+extern "C" int mgk_kmc_select(int ndim, long long nvox, int M, int sig_kind, int cat_kind, double u, const void* sig,
+                              const double* Q0, const double* cat, const double* soft, const double* tlast,
+                              const double* prev, const double* Tfield, const mgk_kmc_params* params,
+                              void* workspace, void* stream, mgk_kmc_sel* out) {
+  if ((ndim != 2 && ndim != 3) || nvox < 1 || M < 1 || !sig || !Q0 || !cat || !soft || !tlast || !params ||
+      !workspace || !out || (params->scheme == 1 && !prev)) {
     mgk_set_error("mgk_kmc_select: bad argument", __FILE__, __LINE__);
     return MGK_ERR_ARG;
   }
@@ -784,7 +819,7 @@ extern "C" int mgk_kmc_select(int ndim, long long nvox, int M, int sig_kind, int
   const KmcWs w = kmc_ws(workspace, (size_t)nvox);
   MgkProfRec prof(s);
   const int h = prof.begin(KID_KMC_SELECT, 0, 8.0 * (KMC_VB + kmc_nblocks((size_t)nvox) + 2 * M));
-  KMC_DISPATCH(launch_select, (size_t)nvox, M, u, Q, Tfield, P, w, s);
+  KMC_DISPATCH(launch_select, (size_t)nvox, M, u, sig, Q0, cat, soft, tlast, prev, Tfield, P, w, s);
   prof.end(h);
   MGK_HIP(hipGetLastError());
   KmcSel* hp = (KmcSel*)mgk_pinned(sizeof(KmcSel));
@@ -797,6 +832,7 @@ extern "C" int mgk_kmc_select(int ndim, long long nvox, int M, int sig_kind, int
   out->r = hp->r;
   out->cum_before = hp->cum_before;
   out->cum_at = hp->cum_at;
+  out->q = hp->q;
   return MGK_OK;
 }
 
@@ -861,6 +897,7 @@ extern "C" int mgk_kmc_select_array(long long n, const double* rates, double r, 
   out->r = hp->r;
   out->cum_before = hp->cum_before;
   out->cum_at = hp->cum_at;
+  out->q = hp->q;
   return MGK_OK;
 }
 

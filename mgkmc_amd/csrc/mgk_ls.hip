@@ -133,7 +133,8 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_tau_xfwd(LsGeom<T> g, const T* __
   constexpr bool CT = NN > 0;
   const int nx = CT ? NN : g.nx, P = g.P, Q = g.Q, logB = CT ? ct_log2(BB) : g.logBx, B = CT ? BB : 1 << logB;
   const int ldb = B;
-  const int q0 = blockIdx.x * B, p = blockIdx.y;
+  const int tid_lin = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int q0 = (tid_lin % gridDim.x) * B, p = tid_lin / gridDim.x;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;  // spectrum component stride
   for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
@@ -329,7 +330,8 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xinv_update(LsGeom<T> g, const cp
   constexpr bool CT = NN > 0;
   const int nx = CT ? NN : g.nx, P = g.P, Q = g.Q, logB = CT ? ct_log2(BB) : g.logBx, B = CT ? BB : 1 << logB;
   const int ldb = B;
-  const int q0 = blockIdx.x * B, p = blockIdx.y;
+  const int tid_lin = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int q0 = (tid_lin % gridDim.x) * B, p = tid_lin / gridDim.x;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
   for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];

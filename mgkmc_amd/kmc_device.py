@@ -127,11 +127,15 @@ class KmcState:
         _lib.check(rc, "mgk_kmc_barrier_rates")
         return tot.value
 
-    def select(self, u, sig_kind, p):
-        """(voxel, mode) of the event for draw u in [0, 1): r = total * u."""
+    def select(self, u, sig, sig_kind, p):
+        """(voxel, mode) of the event for draw u in [0, 1): r = total * u, after
+        barrier_rates with the same sig and parameters.  self.sel.q holds the
+        selected mode's barrier."""
         rc = _lib.load().mgk_kmc_select(self.d, self.NV, self.M, sig_kind, self.cat_kind, float(u),
-                                        _lib.dptr(self.Q), None, ctypes.byref(p), self.ws.ptr,
-                                        ctypes.c_void_p(dev.stream_ptr()), ctypes.byref(self.sel))
+                                        _lib.dptr(sig), _lib.dptr(self.Q0), _lib.dptr(self.cat),
+                                        _lib.dptr(self.soft), _lib.dptr(self.tlast), _lib.dptr(self.prev), None,
+                                        ctypes.byref(p), self.ws.ptr, ctypes.c_void_p(dev.stream_ptr()),
+                                        ctypes.byref(self.sel))
         _lib.check(rc, "mgk_kmc_select")
         return int(self.sel.item), int(self.sel.mode)
 

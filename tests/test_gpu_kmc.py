@@ -122,9 +122,11 @@ def test_fused_select_matches_oracle(gpu):
     assert abs(total - tot) <= 1e-12 * tot
     rng = np.random.default_rng(7)
     for u in list(rng.random(300)) + [0.0, 0.999999999]:
-        v, m = st.select(u, SIG_GEN64, p)
+        v, m = st.select(u, sig_d, SIG_GEN64, p)
         i = ok.select_event(rates, tot, tot * u)
         assert (v * M + m) == idx[i], (u, v, m, idx[i])
+        qref = Q.reshape(-1, M)[v, m]
+        assert abs(st.sel.q - qref) <= 1e-14 * max(1.0, abs(qref))
 
 
 def test_kmc_event_sequence_matches_oracle(gpu):
