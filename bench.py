@@ -175,6 +175,41 @@ def cpu_baseline(n_sample=128, iters=3):
                       f"beforehand, scipy.fft workers={cores})"}
 
 
+def pmc_workload(n):
+    """Short workload for rocprofv3 --pmc runs: a 1 GiB device copy (counter
+    calibration), then 3 LS iterations at n^3 in FP32 and in FP64."""
+    import torch
+    from mgkmc_amd.linear_elastic_simulator import LsSolver, FP32, FP64
+    E, nu, ep = fields(n)
+    a = torch.empty(2 ** 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    for prec in (FP32, FP64):
+        s = LsSolver(E, nu, precision=prec)
+        ep_d = torch.from_numpy(ep).to(device="cuda", dtype=s.eps.dtype)
+        s.solve(np.diag([0.01, 0.0, 0.0]), ep_d, tol=1e-30, max_iter=3)
+        torch.cuda.synchronize()
+        print("pmc workload done", "fp32" if prec == FP32 else "fp64", flush=True)
+        del s, ep_d
+        torch.cuda.empty_cache()
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at n^3 from the committed PMC summary
+    (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated on a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"ls_{n}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        f = ln.split()
+        if len(f) == 4 and f[0] == "traffic_bytes" and f[1] == kernel and f[2] == "fp32":
+            return float(f[3])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,27 +219,34 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kmc-events", type=int, default=20)
     ap.add_argument("--fs-increments", type=int, default=2)
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="run only the short LS workload used for rocprofv3 --pmc counter runs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); multi-GPU sharding is design-only "
               "in this build", file=sys.stderr)
         sys.exit(2)
+    if a.pmc_workload:
+        pmc_workload(a.n)
+        return
     res = gpu_run(a.n, a.steps, a.warmup)
     r32, r64 = res[1], res[0]
     prof = dict(r32["prof"] or {})
     kmc32 = gpu_kmc(a.n, a.kmc_events, 2, "fp32")
     kmc64 = gpu_kmc(a.n, a.kmc_events, 2, "fp64")
+    # the roofline names the dominant kernel of the value's workload (FP32 LS);
+    # the KMC kernels are listed beside it
+    dom = max(prof.items(), key=lambda kv: kv[1][0]) if prof else None
     for k in ("kmc_barrier_rates", "kmc_select"):
         if k in kmc32["prof"]:
             prof[k] = kmc32["prof"][k]
     fsr = gpu_fs(a.n, a.fs_increments) if a.fs_increments > 0 else None
-    dom = max(prof.items(), key=lambda kv: kv[1][0]) if prof else None
     roof = None
     if dom:
         name, (ms, by, cnt) = dom
         ach = (by / cnt) / ((ms / cnt) * 1e-3)
         roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": ach / HBM_PEAK, "traffic": None, "kernel": name,
+                "frac": ach / HBM_PEAK, "traffic": measured_traffic(name, a.n), "kernel": name,
                 "bytes_per_launch": by / cnt, "avg_launch_ms": ms / cnt,
                 "per_kernel": {k: {"avg_ms": v[0] / v[2], "GB_s": (v[1] / v[2]) / (v[0] / v[2] * 1e-3) / 1e9}
                                for k, v in prof.items()}}
