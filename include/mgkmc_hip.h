@@ -69,6 +69,21 @@ int mgk_ls_solve(int ndim, const int64_t* shape, int precision, const void* lam,
                  void* sig, int warm, int max_iter, double tol, void* workspace, size_t workspace_bytes,
                  void* stream, mgk_ls_info* info);
 
+/* FP32 form with FP64 refinement of the same solve (replaces the same reference
+ * functions as mgk_ls_solve; the precision split has no reference counterpart).
+ * Stage 1: FP32 fixed point on (lam32, mu32, epsp32) from eps32 (warm != 0: eps32 is
+ * the initial guess) until diff < tol32, without the final stress.  Stage 2: eps32 is
+ * widened into eps64 and the FP64 fixed point continues from it (warm start) until the
+ * reference's stop test holds at tol in FP64; sig64 and info describe stage 2 (info->iters
+ * = FP64 iterations), *iters32 the FP32 iterations.  epsp32/epsp64 are both NULL or both
+ * given; ws32/ws64 are sized by mgk_ls_workspace_bytes for precision 1 and 0.  Blocks
+ * until done. */
+int mgk_ls_solve_refined(int ndim, const int64_t* shape, const void* lam32, const void* mu32, const void* lam64,
+                         const void* mu64, double lam0, double mu0, const double* eps_bar, const void* epsp32,
+                         const void* epsp64, void* eps32, void* eps64, void* sig64, int warm, int max_iter, double tol,
+                         double tol32, void* ws32, size_t ws32_bytes, void* ws64, size_t ws64_bytes, void* stream,
+                         mgk_ls_info* info, int* iters32);
+
 /* ---------------------------------------------------------------------------
  * KMC: barriers, rates, selection, compaction, flips.
  * Device layouts: Q, Q0 [m][v]; catalog [k][m][v] with k over 2 planar
@@ -135,10 +150,11 @@ int mgk_find_below(long long n, const double* a, double thr, long long* idx_out,
 
 /* apply_flip_soa (mgkmc/stz/cascade.py:43-113; 2D kmc_simulator_functions.py:183-218)
  * on device state; also sets prev (if not NULL) to the mode, as the callers do
- * (mgkmc/aqs.py:999, kmc_simulator.py:1026). */
+ * (mgkmc/aqs.py:999, kmc_simulator.py:1026).  epsp_shadow (FP64 epsp only, may be
+ * NULL): an FP32 copy of epsp whose flipped voxel receives the rounded new values. */
 int mgk_kmc_flip(int ndim, const int* shape, int M, int cat_kind, long long v, int m, double time, double jp,
-                 double jt, double gmax, double jn, const double* cat, void* epsp, int precision, double* soft,
-                 double* tlast, double* prev, double* gamma_out, void* stream);
+                 double jt, double gmax, double jn, const double* cat, void* epsp, int precision, float* epsp_shadow,
+                 double* soft, double* tlast, double* prev, double* gamma_out, void* stream);
 
 /* Redraw of one voxel's modes (mgkmc/aqs.py:1007-1019): newcat [m][k], newq0 [m]. */
 int mgk_kmc_set_voxel(long long nvox, int M, int CK, long long v, const double* newcat, const double* newq0,

@@ -69,6 +69,9 @@ SIGNATURES = {
     "mgk_ls_workspace_bytes": (_c_i, [_c_i, _c_i64p, _c_i, ctypes.POINTER(_c_sz)]),
     "mgk_ls_solve": (_c_i, [_c_i, _c_i64p, _c_i, _c_p, _c_p, _c_d, _c_d, _c_dp, _c_p, _c_p, _c_p,
                             _c_i, _c_i, _c_d, _c_p, _c_sz, _c_p, ctypes.POINTER(MgkLsInfo)]),
+    "mgk_ls_solve_refined": (_c_i, [_c_i, _c_i64p, _c_p, _c_p, _c_p, _c_p, _c_d, _c_d, _c_dp, _c_p, _c_p, _c_p,
+                                    _c_p, _c_p, _c_i, _c_i, _c_d, _c_d, _c_p, _c_sz, _c_p, _c_sz, _c_p,
+                                    ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i)]),
     "mgk_kmc_workspace_bytes": (_c_i, [ctypes.c_longlong, ctypes.POINTER(_c_sz)]),
     "mgk_kmc_barrier_rates": (_c_i, [_c_i, ctypes.c_longlong, _c_i, _c_i, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p,
                                      _c_p, _c_p, ctypes.POINTER(MgkKmcParams), _c_p, _c_p, _c_p, _c_dp]),
@@ -79,7 +82,7 @@ SIGNATURES = {
     "mgk_kmc_select_array": (_c_i, [ctypes.c_longlong, _c_p, _c_d, _c_p, _c_p, ctypes.POINTER(MgkKmcSel)]),
     "mgk_find_below": (_c_i, [ctypes.c_longlong, _c_p, _c_d, _c_p, _c_p, ctypes.POINTER(ctypes.c_longlong)]),
     "mgk_kmc_flip": (_c_i, [_c_i, ctypes.POINTER(_c_i), _c_i, _c_i, ctypes.c_longlong, _c_i, _c_d, _c_d, _c_d,
-                            _c_d, _c_d, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p]),
+                            _c_d, _c_d, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "mgk_kmc_set_voxel": (_c_i, [ctypes.c_longlong, _c_i, _c_i, ctypes.c_longlong, _c_p, _c_p, _c_p, _c_p,
                                  _c_p]),
     "mgk_fs_workspace_bytes": (_c_i, [_c_i64p, _c_i, ctypes.POINTER(_c_sz)]),

@@ -25,10 +25,12 @@ import numpy as np
 
 from . import device as dev
 from .kmc_device import KmcState, params as kmc_params, SIG_SYM64, SIG_SYM32
-from .linear_elastic_simulator import LsSolver, FP64, FP32
+from .linear_elastic_simulator import LsSolver, FP64, FP32, MIXED
 from .stz import kmc as _kmcmod
 
 KB = 8.617e-5
+# "mixed": FP32 iterations, then FP64 refinement to the FP64 stop test
+LS_PRECISIONS = {"fp64": FP64, "fp32": FP32, "mixed": MIXED}
 
 
 def gaussian_generator(mean=2.0, std=0.6, min_cutoff=0.1, max_cutoff=None):
@@ -123,11 +125,14 @@ class ThermalSimulation:
         # initial draws in the reference's order: Q0, then the catalog
         Q0 = self.barrier_generator((nx, ny, nz, M))
         cat = planar_to_full(planar_catalog(self.total_voxels * M, gamma0)).reshape(nx, ny, nz, M, 3, 3)
-        self.precision = FP32 if ls_precision == "fp32" else FP64
+        if ls_precision not in LS_PRECISIONS:
+            raise ValueError(f"ls_precision must be one of {sorted(LS_PRECISIONS)}")
+        self.precision = LS_PRECISIONS[ls_precision]
         self.sig_kind = SIG_SYM32 if self.precision == FP32 else SIG_SYM64
         self.ls = LsSolver(self.E_field, self.nu_field, precision=self.precision)
         self.kmc = KmcState((nx, ny, nz), M, Q0, cat, directional=(softening_scheme == "directional"),
-                            eps_precision=self.precision)
+                            eps_precision=FP32 if self.precision == FP32 else FP64,
+                            eps_shadow32=self.precision == MIXED)
         self.ls_warm_start = ls_warm_start
         self._solved_once = False
         self.eps_macro = np.zeros((3, 3))
@@ -179,7 +184,7 @@ class ThermalSimulation:
         args = dict(max_iter=200, tol=1e-6)
         args.update(self.solver_args)
         self.ls_info = self.ls.solve(self.eps_macro, self.kmc.eps_p, warm=warm,
-                                     max_iter=args["max_iter"], tol=args["tol"])
+                                     max_iter=args["max_iter"], tol=args["tol"], eps_plastic32=self.kmc.eps_p32)
         self._solved_once = True
         return self.ls_info
 

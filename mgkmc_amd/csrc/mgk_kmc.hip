@@ -558,7 +558,7 @@ struct KmcFlip {
 
 template <int D, int CK, typename TE>
 __global__ void k_kmc_flip(KmcFlip F, const double* __restrict__ cat, TE* __restrict__ epsp,
-                           double* __restrict__ soft, double* __restrict__ tlast, double* __restrict__ prev,
+                           float* __restrict__ shadow, double* __restrict__ soft, double* __restrict__ tlast, double* __restrict__ prev,
                            double* __restrict__ gamma_out) {
   constexpr int DD = D * D;
   const size_t NV = (size_t)F.nx * F.ny * F.nz;
@@ -593,18 +593,27 @@ __global__ void k_kmc_flip(KmcFlip F, const double* __restrict__ cat, TE* __rest
   if (t == 0) {
     const size_t v = (size_t)F.v;
     // eps_p += gamma (symmetric SoA storage)
+    // symmetric part of the mode per stored component (exact for symmetric
+    // modes: 0.5*(a+a) == a); the FP32 shadow, if any, gets the rounded sum
+    double add[6];
     if (D == 3) {
-      epsp[0 * NV + v] = (TE)((double)epsp[0 * NV + v] + c[0]);
-      epsp[1 * NV + v] = (TE)((double)epsp[1 * NV + v] + c[4]);
-      epsp[2 * NV + v] = (TE)((double)epsp[2 * NV + v] + c[8]);
-      // symmetric part of the mode (exact for symmetric modes: 0.5*(a+a) == a)
-      epsp[3 * NV + v] = (TE)((double)epsp[3 * NV + v] + 0.5 * (c[5] + c[7]));
-      epsp[4 * NV + v] = (TE)((double)epsp[4 * NV + v] + 0.5 * (c[2] + c[6]));
-      epsp[5 * NV + v] = (TE)((double)epsp[5 * NV + v] + 0.5 * (c[1] + c[3]));
+      add[0] = c[0];
+      add[1] = c[4];
+      add[2] = c[8];
+      add[3] = 0.5 * (c[5] + c[7]);
+      add[4] = 0.5 * (c[2] + c[6]);
+      add[5] = 0.5 * (c[1] + c[3]);
     } else {
-      epsp[0 * NV + v] = (TE)((double)epsp[0 * NV + v] + c[0]);
-      epsp[1 * NV + v] = (TE)((double)epsp[1 * NV + v] + c[3]);
-      epsp[2 * NV + v] = (TE)((double)epsp[2 * NV + v] + 0.5 * (c[1] + c[2]));
+      add[0] = c[0];
+      add[1] = c[3];
+      add[2] = 0.5 * (c[1] + c[2]);
+    }
+    constexpr int CS = (D == 3) ? 6 : 3;
+#pragma unroll
+    for (int k = 0; k < CS; ++k) {
+      const TE nv = (TE)((double)epsp[k * NV + v] + add[k]);
+      epsp[k * NV + v] = nv;
+      if (shadow) shadow[k * NV + v] = (float)nv;
     }
     double gp = soft[v] + F.jp * s;
     if (F.gmax > 0 && gp > F.gmax) gp = F.gmax;
@@ -929,9 +938,9 @@ extern "C" int mgk_find_below(long long n, const double* a, double thr, long lon
 // gamma_out (nullable, device): the D*D mode tensor.
 extern "C" int mgk_kmc_flip(int ndim, const int* shape, int M, int cat_kind, long long v, int m, double time,
                             double jp, double jt, double gmax, double jn, const double* cat, void* epsp,
-                            int precision, double* soft, double* tlast, double* prev, double* gamma_out,
+                            int precision, float* epsp_shadow, double* soft, double* tlast, double* prev, double* gamma_out,
                             void* stream) {
-  if ((ndim != 2 && ndim != 3) || !shape || !cat || !epsp || !soft || !tlast) {
+  if ((ndim != 2 && ndim != 3) || !shape || !cat || !epsp || !soft || !tlast || (precision != 0 && epsp_shadow)) {
     mgk_set_error("mgk_kmc_flip: bad argument", __FILE__, __LINE__);
     return MGK_ERR_ARG;
   }
@@ -955,19 +964,19 @@ extern "C" int mgk_kmc_flip(int ndim, const int* shape, int M, int cat_kind, lon
   hipStream_t s = (hipStream_t)stream;
   if (ndim == 3) {
     if (cat_kind == 0) {
-      if (precision == 0) k_kmc_flip<3, 2, double><<<1, 64, 0, s>>>(F, cat, (double*)epsp, soft, tlast, prev, gamma_out);
-      else k_kmc_flip<3, 2, float><<<1, 64, 0, s>>>(F, cat, (float*)epsp, soft, tlast, prev, gamma_out);
+      if (precision == 0) k_kmc_flip<3, 2, double><<<1, 64, 0, s>>>(F, cat, (double*)epsp, epsp_shadow, soft, tlast, prev, gamma_out);
+      else k_kmc_flip<3, 2, float><<<1, 64, 0, s>>>(F, cat, (float*)epsp, epsp_shadow, soft, tlast, prev, gamma_out);
     } else {
-      if (precision == 0) k_kmc_flip<3, 9, double><<<1, 64, 0, s>>>(F, cat, (double*)epsp, soft, tlast, prev, gamma_out);
-      else k_kmc_flip<3, 9, float><<<1, 64, 0, s>>>(F, cat, (float*)epsp, soft, tlast, prev, gamma_out);
+      if (precision == 0) k_kmc_flip<3, 9, double><<<1, 64, 0, s>>>(F, cat, (double*)epsp, epsp_shadow, soft, tlast, prev, gamma_out);
+      else k_kmc_flip<3, 9, float><<<1, 64, 0, s>>>(F, cat, (float*)epsp, epsp_shadow, soft, tlast, prev, gamma_out);
     }
   } else {
     if (cat_kind == 0) {
-      if (precision == 0) k_kmc_flip<2, 2, double><<<1, 64, 0, s>>>(F, cat, (double*)epsp, soft, tlast, prev, gamma_out);
-      else k_kmc_flip<2, 2, float><<<1, 64, 0, s>>>(F, cat, (float*)epsp, soft, tlast, prev, gamma_out);
+      if (precision == 0) k_kmc_flip<2, 2, double><<<1, 64, 0, s>>>(F, cat, (double*)epsp, epsp_shadow, soft, tlast, prev, gamma_out);
+      else k_kmc_flip<2, 2, float><<<1, 64, 0, s>>>(F, cat, (float*)epsp, epsp_shadow, soft, tlast, prev, gamma_out);
     } else {
-      if (precision == 0) k_kmc_flip<2, 4, double><<<1, 64, 0, s>>>(F, cat, (double*)epsp, soft, tlast, prev, gamma_out);
-      else k_kmc_flip<2, 4, float><<<1, 64, 0, s>>>(F, cat, (float*)epsp, soft, tlast, prev, gamma_out);
+      if (precision == 0) k_kmc_flip<2, 4, double><<<1, 64, 0, s>>>(F, cat, (double*)epsp, epsp_shadow, soft, tlast, prev, gamma_out);
+      else k_kmc_flip<2, 4, float><<<1, 64, 0, s>>>(F, cat, (float*)epsp, epsp_shadow, soft, tlast, prev, gamma_out);
     }
   }
   MGK_HIP(hipGetLastError());

@@ -506,6 +506,11 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_stress(const T* __restrict__ eps,
     for (int c = 0; c < 2 * C; ++c) partials[(size_t)blockIdx.x * 2 * C + c] = acc[c];
 }
 
+__global__ void k_widen(const float* __restrict__ a, double* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = (double)a[i];
+}
+
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
@@ -642,7 +647,7 @@ int ls_iter_hint(const void* ws, int iters) {
 template <typename T, int D>
 int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double mu0, const double* eps_bar,
                const T* epsp, T* eps, T* sig, int warm, int max_iter, double tol, char* ws, hipStream_t stream,
-               mgk_ls_info* info) {
+               mgk_ls_info* info, bool final_stress = true) {
   constexpr int MV = maxv<T>();
   constexpr int C = (D == 3) ? 6 : 3;
   const int prec = sizeof(T) == 8 ? 0 : 1;
@@ -699,7 +704,7 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   const int nfill = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 4096);
   if (!warm) {
     k_ls_fill<T><<<nfill, MGK_NT, 0, stream>>>(eps, C, NV, eb);
-  } else {
+  } else if (warm == 1) {
     k_comp_sums<T><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, C, NV, part);
     k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, C, (double)NV, st->mean);
     k_ls_warm_adjust<T><<<nfill, MGK_NT, 0, stream>>>(eps, C, NV, eb, st->mean);
@@ -807,9 +812,11 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     first = false;
   }
   // final stress and macro means
-  k_ls_stress<T, D><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, sig, NV, part);
-  k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, 2 * C, (double)NV, st->mean);
-  MGK_HIP(hipGetLastError());
+  if (final_stress) {
+    k_ls_stress<T, D><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, sig, NV, part);
+    k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, 2 * C, (double)NV, st->mean);
+    MGK_HIP(hipGetLastError());
+  }
   MGK_HIP(hipMemcpyAsync(&hst[0], st, sizeof(LsState), hipMemcpyDeviceToHost, stream));
   MGK_HIP(hipStreamSynchronize(stream));
   prof.commit(hst[0].iters);
@@ -878,6 +885,7 @@ extern "C" int mgk_ls_solve(int ndim, const int64_t* shape, int precision, const
   }
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)workspace;
+  warm = warm ? 1 : 0;
   if (precision == 0) {
     if (ndim == 3)
       return ls_solve_t<double, 3>(pl, (const double*)lam, (const double*)mu, lam0, mu0, eps_bar,
@@ -892,4 +900,58 @@ extern "C" int mgk_ls_solve(int ndim, const int64_t* shape, int precision, const
                                 (const float*)eps_plastic, (float*)eps, (float*)sig, warm, max_iter, tol, ws, s, info);
   return ls_solve_t<float, 2>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar, (const float*)eps_plastic,
                               (float*)eps, (float*)sig, warm, max_iter, tol, ws, s, info);
+}
+
+extern "C" int mgk_ls_solve_refined(int ndim, const int64_t* shape, const void* lam32, const void* mu32,
+                                    const void* lam64, const void* mu64, double lam0, double mu0,
+                                    const double* eps_bar, const void* epsp32, const void* epsp64, void* eps32,
+                                    void* eps64, void* sig64, int warm, int max_iter, double tol, double tol32,
+                                    void* ws32, size_t ws32_bytes, void* ws64, size_t ws64_bytes, void* stream,
+                                    mgk_ls_info* info, int* iters32) {
+  if (!lam32 || !mu32 || !lam64 || !mu64 || !eps32 || !eps64 || !eps_bar || !ws32 || !ws64 || max_iter < 1 ||
+      (epsp32 == nullptr) != (epsp64 == nullptr)) {
+    mgk_set_error("null argument, max_iter < 1, or only one of the eps_plastic copies given", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  LsPlan p32, p64;
+  int rc = ls_plan<float>(ndim, shape, &p32);
+  if (rc == MGK_OK) rc = ls_plan<double>(ndim, shape, &p64);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the LS solver", __FILE__, __LINE__);
+    return rc;
+  }
+  if (ws32_bytes < p32.total || ws64_bytes < p64.total) {
+    mgk_set_error("workspace too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  warm = warm ? 1 : 0;
+  mgk_ls_info i32{};
+  // stage 1: FP32 fixed point to tol32 (no stress / means)
+  if (ndim == 3)
+    rc = ls_solve_t<float, 3>(p32, (const float*)lam32, (const float*)mu32, lam0, mu0, eps_bar, (const float*)epsp32,
+                              (float*)eps32, nullptr, warm, max_iter, tol32, (char*)ws32, s, &i32, false);
+  else
+    rc = ls_solve_t<float, 2>(p32, (const float*)lam32, (const float*)mu32, lam0, mu0, eps_bar, (const float*)epsp32,
+                              (float*)eps32, nullptr, warm, max_iter, tol32, (char*)ws32, s, &i32, false);
+  if (iters32) *iters32 = i32.iters;
+  if (rc < 0) {
+    if (info) *info = i32;
+    return rc;
+  }
+  // stage 2: widen and refine in FP64 with the reference's stop test at tol.
+  // The FP32 iterate already has mean eps_bar up to FP32 rounding, and every
+  // update restores the exact mean (Gamma0(0) = 0), so the stage starts from
+  // the widened field as is (warm = 2: no mean adjustment pass).
+  const size_t n = (size_t)p64.C * (size_t)p64.NV;
+  const int nb = (int)std::min<size_t>((n + MGK_NT - 1) / MGK_NT, 8192);
+  k_widen<<<nb, MGK_NT, 0, s>>>((const float*)eps32, (double*)eps64, n);
+  MGK_HIP(hipGetLastError());
+  if (ndim == 3)
+    return ls_solve_t<double, 3>(p64, (const double*)lam64, (const double*)mu64, lam0, mu0, eps_bar,
+                                 (const double*)epsp64, (double*)eps64, (double*)sig64, 2, max_iter, tol, (char*)ws64,
+                                 s, info);
+  return ls_solve_t<double, 2>(p64, (const double*)lam64, (const double*)mu64, lam0, mu0, eps_bar,
+                               (const double*)epsp64, (double*)eps64, (double*)sig64, 2, max_iter, tol, (char*)ws64, s,
+                               info);
 }

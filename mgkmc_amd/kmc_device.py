@@ -85,7 +85,7 @@ class KmcState:
     """All per-voxel STZ/KMC arrays of one simulation, resident in HBM."""
 
     def __init__(self, shape, M, Q0, catalog, soft_prop=None, last_event_time=None, prev_strain_dir=None,
-                 directional=False, eps_precision=0):
+                 directional=False, eps_precision=0, eps_shadow32=False):
         dev.require_gpu()
         torch = dev.torch
         self.shape = tuple(int(s) for s in shape)
@@ -97,7 +97,7 @@ class KmcState:
         self.cat_kind = CAT_PLANAR if self.planar else CAT_GENERAL
         self.CK = 2 if self.planar else d * d
         self.Q0 = modes_to_device(Q0)
-        self.Q = torch.zeros_like(self.Q0)
+        self.Q = None  # allocated on the first barrier pass that stores Q
         self.cat = catalog_to_device(catalog, d, self.planar)
         soft = np.zeros(self.shape + (2,)) if soft_prop is None else np.asarray(soft_prop)[..., :2]
         self.soft = dev.upload(np.ascontiguousarray(soft.reshape(-1, 2).T), 0)
@@ -111,13 +111,22 @@ class KmcState:
         C = 6 if d == 3 else 3
         self.eps_precision = eps_precision
         self.eps_p = torch.zeros((C,) + self.shape, dtype=dev.torch_dtype(eps_precision), device="cuda")
+        # FP32 copy of an FP64 eps_p kept in step by the flip kernel (FP32 + FP64 refinement LS)
+        self.eps_p32 = None
+        if eps_shadow32:
+            if eps_precision != 0:
+                raise ValueError("eps_shadow32 needs an FP64 eps_p")
+            self.eps_p32 = torch.zeros((C,) + self.shape, dtype=torch.float32, device="cuda")
         self.ws = KmcWorkspace(self.NV)
         self.sel = _lib.MgkKmcSel()
         self._shape_c = (ctypes.c_int * d)(*self.shape)
         self.gamma = torch.zeros(d * d, dtype=torch.float64, device="cuda")
 
     def barrier_rates(self, sig, sig_kind, p, write_q=True):
-        """Fused compute_barrier + compute_rates; returns the FP64 total rate."""
+        """Fused compute_barrier + compute_rates; returns the FP64 total rate.
+        write_q: also store the barriers in self.Q ([m][v] FP64)."""
+        if write_q and self.Q is None:
+            self.Q = dev.torch.zeros_like(self.Q0)
         tot = ctypes.c_double()
         rc = _lib.load().mgk_kmc_barrier_rates(
             self.d, self.NV, self.M, sig_kind, self.cat_kind, _lib.dptr(sig), _lib.dptr(self.Q0),
@@ -142,7 +151,8 @@ class KmcState:
     def flip(self, v, m, time, jp, jt, gmax, jn):
         rc = _lib.load().mgk_kmc_flip(self.d, self._shape_c, self.M, self.cat_kind, int(v), int(m), float(time),
                                       float(jp), float(jt), float(gmax), float(jn), _lib.dptr(self.cat),
-                                      _lib.dptr(self.eps_p), self.eps_precision, _lib.dptr(self.soft),
+                                      _lib.dptr(self.eps_p), self.eps_precision, _lib.dptr(self.eps_p32),
+                                      _lib.dptr(self.soft),
                                       _lib.dptr(self.tlast), _lib.dptr(self.prev), _lib.dptr(self.gamma),
                                       ctypes.c_void_p(dev.stream_ptr()))
         _lib.check(rc, "mgk_kmc_flip")

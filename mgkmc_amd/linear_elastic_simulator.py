@@ -25,6 +25,7 @@ from .elasticity import compute_lame_2d, compute_lame_3d
 
 FP64 = 0
 FP32 = 1
+MIXED = 2  # FP32 iterations, then FP64 refinement iterations to the FP64 stop test
 
 
 class LsSolver:
@@ -54,28 +55,77 @@ class LsSolver:
         self.lam0 = float(l0 if lam0 is None else lam0)
         self.mu0 = float(m0 if mu0 is None else mu0)
         self.mu_host = mu
-        self.lam = dev.upload(lam, precision)
-        self.mu = dev.upload(mu, precision)
-        dt = dev.torch_dtype(precision)
+        fp = FP64 if precision == MIXED else precision  # precision of lam, mu, eps, sig
+        self.lam = dev.upload(lam, fp)
+        self.mu = dev.upload(mu, fp)
+        dt = dev.torch_dtype(fp)
         self.eps = dev.torch.zeros((self.C,) + self.shape, dtype=dt, device="cuda")
         self.sig = dev.torch.zeros((self.C,) + self.shape, dtype=dt, device="cuda")
         self._shape_c = _lib.shape_arg(self.shape)
+        self.ws = self._workspace(fp)
+        self.info = _lib.MgkLsInfo()
+        self.iters32 = 0
+        if precision == MIXED:
+            self.lam32 = dev.upload(lam, FP32)
+            self.mu32 = dev.upload(mu, FP32)
+            self.eps32 = dev.torch.zeros((self.C,) + self.shape, dtype=dev.torch.float32, device="cuda")
+            self.ws32 = self._workspace(FP32)
+            self._iters32 = ctypes.c_int()
+
+    def _workspace(self, precision):
         nbytes = ctypes.c_size_t()
         rc = _lib.load().mgk_ls_workspace_bytes(self.d, self._shape_c, precision, ctypes.byref(nbytes))
         _lib.check(rc, "mgk_ls_workspace_bytes")
-        self.ws = dev.torch.empty(int(nbytes.value), dtype=dev.torch.uint8, device="cuda")
-        self.info = _lib.MgkLsInfo()
+        return dev.torch.empty(int(nbytes.value), dtype=dev.torch.uint8, device="cuda")
 
-    def solve(self, eps_bar, eps_plastic=None, warm=False, max_iter=200, tol=1e-6, stream=None):
+    @staticmethod
+    def footprint_bytes(shape, precision):
+        """HBM bytes a solver of this grid and precision holds (fields + workspaces)."""
+        d = len(shape)
+        C = 6 if d == 3 else 3
+        NV = int(np.prod(shape))
+        sc = _lib.shape_arg(shape)
+        lib = _lib.load()
+
+        def ws(p):
+            nb = ctypes.c_size_t()
+            _lib.check(lib.mgk_ls_workspace_bytes(d, sc, p, ctypes.byref(nb)), "mgk_ls_workspace_bytes")
+            return int(nb.value)
+        if precision == MIXED:
+            return NV * 8 * (2 + 2 * C) + ws(FP64) + NV * 4 * (2 + C) + ws(FP32)
+        s = 8 if precision == FP64 else 4
+        return NV * s * (2 + 2 * C) + ws(precision)
+
+    def solve(self, eps_bar, eps_plastic=None, warm=False, max_iter=200, tol=1e-6, stream=None,
+              eps_plastic32=None, tol32=None):
         """Run the LS fixed point; eps/sig are updated in place on the device.
         eps_plastic: device tensor (C, ...) or None.  warm: start from self.eps
-        (the reference's eps_init).  Returns the mgk_ls_info struct."""
+        (the reference's eps_init; in the MIXED form from self.eps32).
+        MIXED form: FP32 iterations to tol32 (default max(tol, 1e-6)), then FP64
+        iterations to tol; eps_plastic32 is the FP32 copy of eps_plastic (made
+        here if not given).  Returns the mgk_ls_info struct of the final stage."""
         eb, ebp = _lib.host_dbl(np.asarray(eps_bar, dtype=np.float64).reshape(self.d, self.d))
         if eps_plastic is not None:
             if eps_plastic.dtype != self.eps.dtype or tuple(eps_plastic.shape) != tuple(self.eps.shape):
                 raise ValueError("eps_plastic must be a device tensor of shape (C, ...) and solver dtype")
             eps_plastic = eps_plastic.contiguous()
         s = dev.stream_ptr() if stream is None else stream
+        if self.precision == MIXED:
+            if eps_plastic is not None and eps_plastic32 is None:
+                eps_plastic32 = eps_plastic.to(dev.torch.float32)
+            if eps_plastic32 is not None and (eps_plastic32.dtype != dev.torch.float32
+                                              or tuple(eps_plastic32.shape) != tuple(self.eps.shape)):
+                raise ValueError("eps_plastic32 must be an FP32 device tensor of shape (C, ...)")
+            t32 = max(float(tol), 1e-6) if tol32 is None else float(tol32)
+            rc = _lib.load().mgk_ls_solve_refined(
+                self.d, self._shape_c, _lib.dptr(self.lam32), _lib.dptr(self.mu32), _lib.dptr(self.lam),
+                _lib.dptr(self.mu), self.lam0, self.mu0, ebp, _lib.dptr(eps_plastic32), _lib.dptr(eps_plastic),
+                _lib.dptr(self.eps32), _lib.dptr(self.eps), _lib.dptr(self.sig), 1 if warm else 0, int(max_iter),
+                float(tol), t32, _lib.dptr(self.ws32), self.ws32.numel(), _lib.dptr(self.ws), self.ws.numel(),
+                ctypes.c_void_p(s), ctypes.byref(self.info), ctypes.byref(self._iters32))
+            _lib.check(rc, "mgk_ls_solve_refined")
+            self.iters32 = self._iters32.value
+            return self.info
         rc = _lib.load().mgk_ls_solve(
             self.d, self._shape_c, self.precision, _lib.dptr(self.lam), _lib.dptr(self.mu),
             self.lam0, self.mu0, ebp, _lib.dptr(eps_plastic), _lib.dptr(self.eps), _lib.dptr(self.sig),

@@ -132,6 +132,41 @@ def test_ls3d_fp32_matches_fp64_within_stated_error(gpu):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("shape,tol", [((32, 32, 32), 1e-6), ((24, 20, 16), 1e-8), ((64, 48), 1e-8)])
+def test_ls_refined_matches_fp64(gpu, shape, tol):
+    # FP32 iterations + FP64 refinement: the FP64 stop test holds at tol, and the
+    # field agrees with the plain FP64 path (and the oracle) to the accuracy the
+    # stop test implies (a few tol) -- stated bound 10 * tol relative
+    from mgkmc_amd.linear_elastic_simulator import LsSolver, FP64, MIXED
+    from mgkmc_amd import device as dev
+    print(f"[ls refined] shape={shape} tol={tol}", flush=True)
+    d = len(shape)
+    E, nu, ep = _case(shape, 8)
+    eb = np.diag([0.01, -0.002, 0.0][:d])
+    sym, _ = dev.to_sym_soa(ep, d)
+    ref = (ol.spectral_solver_3d if d == 3 else ol.spectral_solver_2d)(E, nu, eb, eps_plastic=ep, tol=1e-12)
+    s64 = LsSolver(E, nu, precision=FP64)
+    i64 = s64.solve(eb, dev.upload(sym, FP64), tol=tol)
+    it64 = i64.iters
+    sm = LsSolver(E, nu, precision=MIXED)
+    im = sm.solve(eb, dev.upload(sym, FP64), tol=tol)
+    assert im.status == 0 and im.diff < tol and im.iters >= 1 and sm.iters32 >= 1
+    eps_m = dev.from_sym_soa(dev.download(sm.eps), d)
+    eps_64 = dev.from_sym_soa(dev.download(s64.eps), d)
+    sig_m = dev.from_sym_soa(dev.download(sm.sig), d)
+    e_ref = _rel(eps_m, ref[0])
+    e_64 = _rel(eps_m, eps_64)
+    print(f"[ls refined] fp32 iters={sm.iters32} fp64 iters={im.iters} (plain fp64 {it64}) "
+          f"err vs oracle(1e-12)={e_ref:.2e} vs fp64@tol={e_64:.2e}", flush=True)
+    assert e_ref < 10 * tol and e_64 < 10 * tol
+    assert _rel(sig_m, ref[1]) < 10 * tol
+    r_eq = ol.equilibrium_residual(sig_m)
+    assert r_eq < 10 * tol, r_eq
+    # warm re-solve of the same problem: FP32 stage starts converged
+    im2 = sm.solve(eb, dev.upload(sym, FP64), tol=tol, warm=True)
+    assert im2.diff < tol and sm.iters32 <= 2
+
+
 def test_ls3d_full_size_properties(gpu):
     # 128^3: homogeneous medium => exact uniform field; heterogeneous => the
     # converged field has mean eps_bar and the solve is deterministic
