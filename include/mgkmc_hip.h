@@ -160,6 +160,16 @@ int mgk_kmc_flip(int ndim, const int* shape, int M, int cat_kind, long long v, i
 int mgk_kmc_set_voxel(long long nvox, int M, int CK, long long v, const double* newcat, const double* newq0,
                       double* cat, double* Q0, void* stream);
 
+/* Initial modes drawn on the device (no counterpart in the reference, which draws
+ * Q0 and the catalog on the host with the legacy stream: mgkmc/aqs.py:101-120,
+ * kmc_simulator.py:117-127).  For callers that start many large simulations (the
+ * serving loop).  Q0 [m][v] = clip(mean + std*N(0,1)); planar catalog [k][m][v],
+ * k = gxx, gxy = 0.5*gamma0*N(0,1); Philox4x32-10 keyed by seed, counter (m, v, draw),
+ * Box-Muller normals: deterministic in (seed, v, m). */
+int mgk_kmc_init_modes(long long nvox, int M, uint64_t seed, double mean, double std_dev, int has_min,
+                       double min_cutoff, int has_max, double max_cutoff, double gamma0, double* Q0, double* cat,
+                       void* stream);
+
 /* ---------------------------------------------------------------------------
  * Finite strain (de Geus Newton-Krylov).  Fields: 9 components F[3*i+j][v].
  * ------------------------------------------------------------------------- */

@@ -995,3 +995,89 @@ extern "C" int mgk_kmc_set_voxel(long long nvox, int M, int CK, long long v, con
   MGK_HIP(hipGetLastError());
   return MGK_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Device initial modes: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel
+// random numbers: as easy as 1, 2, 3", SC'11) keyed by the seed, counter
+// (mode, voxel lo, voxel hi, draw), Box-Muller normals.  No reference
+// counterpart: the reference draws these on the host from the legacy stream.
+// ---------------------------------------------------------------------------
+struct Philox4 {
+  uint32_t v[4];
+};
+
+__device__ __forceinline__ Philox4 philox4x32_10(Philox4 c, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)M0 * c.v[0];
+    const uint64_t p1 = (uint64_t)M1 * c.v[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    Philox4 n;
+    n.v[0] = hi1 ^ c.v[1] ^ k0;
+    n.v[1] = lo1;
+    n.v[2] = hi0 ^ c.v[3] ^ k1;
+    n.v[3] = lo0;
+    c = n;
+    k0 += W0;
+    k1 += W1;
+  }
+  return c;
+}
+
+// 53-bit uniform in (0, 1) from two 32-bit words
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
+  const uint64_t x = ((uint64_t)a << 21) ^ (uint64_t)(b >> 11);
+  return ((double)x + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ void box_muller(double u1, double u2, double* z0, double* z1) {
+  const double r = sqrt(-2.0 * log(u1));
+  double s, c;
+  sincospi(2.0 * u2, &s, &c);
+  *z0 = r * c;
+  *z1 = r * s;
+}
+
+__global__ void k_kmc_init_modes(size_t NV, int M, uint32_t k0, uint32_t k1, double mean, double sd, int has_lo,
+                                 double lo, int has_hi, double hi, double half_g0, double* __restrict__ Q0,
+                                 double* __restrict__ cat) {
+  const size_t MNV = (size_t)M * NV;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < MNV; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t m = i / NV, v = i - m * NV;
+    Philox4 c;
+    c.v[0] = (uint32_t)m;
+    c.v[1] = (uint32_t)v;
+    c.v[2] = (uint32_t)(v >> 32);
+    c.v[3] = 0;
+    const Philox4 a = philox4x32_10(c, k0, k1);
+    c.v[3] = 1;
+    const Philox4 b = philox4x32_10(c, k0, k1);
+    double gxx, gxy, zq, unused;
+    box_muller(u53(a.v[0], a.v[1]), u53(a.v[2], a.v[3]), &gxx, &gxy);
+    box_muller(u53(b.v[0], b.v[1]), u53(b.v[2], b.v[3]), &zq, &unused);
+    double q = mean + sd * zq;
+    if (has_lo && q < lo) q = lo;
+    if (has_hi && q > hi) q = hi;
+    Q0[i] = q;
+    cat[i] = half_g0 * gxx;
+    cat[MNV + i] = half_g0 * gxy;
+  }
+}
+
+extern "C" int mgk_kmc_init_modes(long long nvox, int M, uint64_t seed, double mean, double std_dev, int has_min,
+                                  double min_cutoff, int has_max, double max_cutoff, double gamma0, double* Q0,
+                                  double* cat, void* stream) {
+  if (nvox < 1 || M < 1 || !Q0 || !cat) {
+    mgk_set_error("mgk_kmc_init_modes: bad argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  const size_t n = (size_t)nvox * (size_t)M;
+  const int nb = (int)std::min<size_t>((n + MGK_NT - 1) / MGK_NT, 16384);
+  k_kmc_init_modes<<<nb, MGK_NT, 0, (hipStream_t)stream>>>((size_t)nvox, M, (uint32_t)seed, (uint32_t)(seed >> 32),
+                                                             mean, std_dev, has_min, min_cutoff, has_max, max_cutoff,
+                                                             0.5 * gamma0, Q0, cat);
+  MGK_HIP(hipGetLastError());
+  return MGK_OK;
+}

@@ -93,12 +93,20 @@ class KmcState:
         self.M = int(M)
         self.NV = int(np.prod(self.shape))
         d = self.d
-        self.planar = is_planar(catalog, d)
-        self.cat_kind = CAT_PLANAR if self.planar else CAT_GENERAL
-        self.CK = 2 if self.planar else d * d
-        self.Q0 = modes_to_device(Q0)
+        if Q0 is None and catalog is None:
+            # zero planar modes, filled on the device afterwards
+            self.planar = True
+            self.cat_kind = CAT_PLANAR
+            self.CK = 2
+            self.Q0 = torch.zeros((self.M, self.NV), dtype=torch.float64, device="cuda")
+            self.cat = torch.zeros((2, self.M, self.NV), dtype=torch.float64, device="cuda")
+        else:
+            self.planar = is_planar(catalog, d)
+            self.cat_kind = CAT_PLANAR if self.planar else CAT_GENERAL
+            self.CK = 2 if self.planar else d * d
+            self.Q0 = modes_to_device(Q0)
+            self.cat = catalog_to_device(catalog, d, self.planar)
         self.Q = None  # allocated on the first barrier pass that stores Q
-        self.cat = catalog_to_device(catalog, d, self.planar)
         soft = np.zeros(self.shape + (2,)) if soft_prop is None else np.asarray(soft_prop)[..., :2]
         self.soft = dev.upload(np.ascontiguousarray(soft.reshape(-1, 2).T), 0)
         tl = np.full(self.shape, -np.inf) if last_event_time is None else np.asarray(last_event_time, float)

@@ -1,0 +1,293 @@
+"""Device-resident STZ/KMC event loop shared by the 2D and 3D simulators.
+
+The per-event work of the reference's KMC loops (mgkmc/aqs.py:848-1087 in 3D,
+mgkmc/kmc_simulator.py:871-1101 in 2D) stays on the GPU: fused barrier + rate
+pass, FP64 rate total, two-level event selection (the chosen voxel's barriers
+are recomputed, Q is not stored), flip, mode redraw upload and the LS
+re-equilibration.  The host keeps the scalar state (time, macro strain) and
+the random streams:
+
+  * ``rng``: numpy's legacy stream (the np.random module by default, exactly
+    as the reference uses it; a RandomState for an isolated simulation):
+    initial Q0 (generator), initial catalog (2 normals per mode, voxel order),
+    then per event one uniform (waiting time), 2M normals (new modes) and M
+    normals (new Q0);
+  * ``selection_stream``: r = total * u for the event selection, the
+    reference's numba-internal draw made seedable (default: the module-level
+    ``mgkmc_amd.stz.kmc.SELECTION_STREAM``).
+
+``init_modes=("device", seed)`` draws the initial Q0 and catalog on the GPU
+with a counter-based generator instead (no reference counterpart: used by the
+serving loop, where host draws of M modes per voxel would dominate start-up).
+
+Supported: small strain, linear elasticity, KMC mode (cascade_mode=False),
+isotropic or directional softening, redraw of directions and barriers,
+planar (pure-shear) catalogs.  Options the device path does not cover raise
+NotImplementedError instead of falling back to a CPU path.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from . import device as dev
+from .kmc_device import KmcState, params as kmc_params, SIG_SYM64, SIG_SYM32, modes_from_device
+from .linear_elastic_simulator import LsSolver, FP64, FP32, MIXED
+from .stz import kmc as _kmcmod
+
+KB = 8.617e-5
+# "mixed": FP32 iterations, then FP64 refinement to the FP64 stop test
+LS_PRECISIONS = {"fp64": FP64, "fp32": FP32, "mixed": MIXED}
+
+
+def gaussian_generator(mean=2.0, std=0.6, min_cutoff=0.1, max_cutoff=None, rng=None):
+    """normal + clip on the legacy stream (mgkmc/stz/barrier_generators.py:4-30,
+    kmc_simulator_functions.py:222-229)."""
+    def gen(shape):
+        r = np.random if rng is None else rng
+        b = r.normal(loc=mean, scale=std, size=shape)
+        if min_cutoff is not None or max_cutoff is not None:
+            b = np.clip(b, a_min=min_cutoff, a_max=max_cutoff)
+        return b
+    gen.params = dict(mean=mean, std=std, min_cutoff=min_cutoff, max_cutoff=max_cutoff)
+    return gen
+
+
+def planar_catalog(n_modes_total, gamma0, rng=None):
+    """n planar pure-shear modes drawn as the reference's per-mode loops do
+    (gxx then gxy; mgkmc/stz/catalog.py:7-40, kmc_simulator_functions.py:78-98):
+    returns (n, 2) (gxx, gxy).  One vectorised call consumes the legacy stream
+    exactly like the loop."""
+    r = np.random if rng is None else rng
+    g = r.normal(size=2 * n_modes_total).reshape(n_modes_total, 2)
+    return 0.5 * gamma0 * g
+
+
+def planar_to_full(p, d=3):
+    out = np.zeros(p.shape[:-1] + (d, d))
+    out[..., 0, 0] = p[..., 0]
+    out[..., 1, 1] = -p[..., 0]
+    out[..., 0, 1] = p[..., 1]
+    out[..., 1, 0] = p[..., 1]
+    return out
+
+
+def device_init_modes(shape, M, gamma0, gen_params, seed, Q0, cat, stream=None):
+    """Counter-based (Philox4x32-10) initial Q0 ~ clip(N(mean, std)) and planar
+    catalog gxx, gxy ~ 0.5 gamma0 N(0, 1), written into the device arrays Q0
+    [m][v] and cat [k][m][v].  Deterministic in (seed, voxel, mode)."""
+    nv = int(np.prod(shape))
+    lo = gen_params.get("min_cutoff")
+    hi = gen_params.get("max_cutoff")
+    rc = _lib.load().mgk_kmc_init_modes(
+        nv, int(M), ctypes.c_uint64(int(seed) & (2 ** 64 - 1)), float(gen_params.get("mean", 2.0)),
+        float(gen_params.get("std", 0.6)), 1 if lo is not None else 0, float(lo if lo is not None else 0.0),
+        1 if hi is not None else 0, float(hi if hi is not None else 0.0), float(gamma0), _lib.dptr(Q0),
+        _lib.dptr(cat), ctypes.c_void_p(dev.stream_ptr() if stream is None else stream))
+    _lib.check(rc, "mgk_kmc_init_modes")
+
+
+class DeviceKmc:
+    """Grid-dimension-independent state and per-event step."""
+
+    def _unsupported(self, cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model,
+                     extra=()):
+        bad = list(extra)
+        if cascade_mode:
+            bad.append("cascade_mode")
+        if fast_patching and fast_patching.get("enabled", False):
+            bad.append("fast_patching")
+        if enable_thermal:
+            bad.append("enable_thermal")
+        if strain_assumption != "small_strain":
+            bad.append("finite_strain KMC")
+        if hyperelastic_model in ("landau", "secant_degradation", "rose"):
+            bad.append(f"hyperelastic_model={hyperelastic_model}")
+        if bad:
+            raise NotImplementedError("device KMC path does not cover: " + ", ".join(bad))
+
+    def _setup(self, shape, M, gamma0, E_field, nu_field, pixel, barrier_generator, barrier_kwargs,
+               softening_scheme, softening_cap, jp, jt, neighbor_softening_fraction, temperature, strain_rate,
+               stability_threshold, redraw_directions, redraw_barriers, nu0, q_act_temp, cascade_mode,
+               scale_rate_by_volume, plane_mode, ls_precision, ls_warm_start, rng, selection_stream, init_modes,
+               barrier_defaults):
+        self.rng = np.random if rng is None else rng
+        self.selection_stream = _kmcmod.SELECTION_STREAM if selection_stream is None else selection_stream
+        self.M, self.gamma0 = M, gamma0
+        self.pixel = pixel
+        self.volume = pixel ** 3
+        self.softening_scheme = softening_scheme
+        self.softening_cap = softening_cap
+        self.jp, self.jt = jp, jt
+        self.neighbor_softening_fraction = neighbor_softening_fraction
+        self.temperature = temperature
+        self.strain_rate = strain_rate
+        self.stability_threshold = stability_threshold
+        self.redraw_directions, self.redraw_barriers = redraw_directions, redraw_barriers
+        self.nu0 = nu0
+        self.q_act_temp = q_act_temp
+        self.cascade_mode = cascade_mode
+        self.scale_rate_by_volume = scale_rate_by_volume
+        self.plane_mode = plane_mode
+        if self.temperature > 0:
+            self.tau = 1.0 / (self.nu0 * np.exp(-self.q_act_temp / (KB * self.temperature)))
+        else:
+            self.tau = np.inf
+        self.grid_shape = tuple(int(s) for s in shape)
+        self.d = len(self.grid_shape)
+        self.total_voxels = int(np.prod(self.grid_shape))
+        d = self.d
+        E = np.asarray(E_field, dtype=np.float64)
+        if E.ndim == 0:
+            E = np.full(self.grid_shape, float(E))
+        E = E * 1e9 if E.mean() < 1e6 else E
+        nu = np.asarray(nu_field, dtype=np.float64)
+        if nu.ndim == 0:
+            nu = np.full(self.grid_shape, float(nu))
+        self.E_field, self.nu_field = E, nu
+        self.E, self.nu = E, nu
+        if barrier_generator is None:
+            self.barrier_generator = gaussian_generator(**barrier_defaults, rng=rng)
+        elif isinstance(barrier_generator, str):
+            if barrier_generator.lower() != "gaussian":
+                raise NotImplementedError(f"barrier generator {barrier_generator!r}")
+            self.barrier_generator = gaussian_generator(**barrier_kwargs, rng=rng)
+        else:
+            self.barrier_generator = barrier_generator
+        if ls_precision not in LS_PRECISIONS:
+            raise ValueError(f"ls_precision must be one of {sorted(LS_PRECISIONS)}")
+        self.precision = LS_PRECISIONS[ls_precision]
+        self.sig_kind = SIG_SYM32 if self.precision == FP32 else SIG_SYM64
+        self.ls = LsSolver(self.E_field, self.nu_field, plane_mode=plane_mode, precision=self.precision)
+        directional = softening_scheme == "directional"
+        eps_prec = FP32 if self.precision == FP32 else FP64
+        if init_modes is not None and init_modes[0] == "device":
+            params = getattr(self.barrier_generator, "params", None)
+            if params is None:
+                raise NotImplementedError("device mode initialisation needs a gaussian barrier generator")
+            self.kmc = KmcState(self.grid_shape, M, None, None, directional=directional, eps_precision=eps_prec,
+                                eps_shadow32=self.precision == MIXED)
+            device_init_modes(self.grid_shape, M, gamma0, params, init_modes[1], self.kmc.Q0, self.kmc.cat)
+        else:
+            # the reference's order: Q0, then the catalog voxel by voxel
+            Q0 = self.barrier_generator(self.grid_shape + (M,))
+            cat = planar_to_full(planar_catalog(self.total_voxels * M, gamma0, rng), d).reshape(
+                self.grid_shape + (M, d, d))
+            self.kmc = KmcState(self.grid_shape, M, Q0, cat, directional=directional, eps_precision=eps_prec,
+                                eps_shadow32=self.precision == MIXED)
+        self.ls_warm_start = ls_warm_start
+        self._solved_once = False
+        self.eps_macro = np.zeros((d, d))
+        self.time = 0.0
+        self.solver_args = {}
+        self.flip_event_history = []
+        self.ls_info = None
+
+    # ----- state mirrors (synchronised on access) --------------------------
+    @property
+    def sig_field(self):
+        return dev.from_sym_soa(dev.download(self.ls.sig), self.d)
+
+    @property
+    def eps_field(self):
+        return dev.from_sym_soa(dev.download(self.ls.eps), self.d)
+
+    @property
+    def eps_plastic(self):
+        return dev.from_sym_soa(dev.download(self.kmc.eps_p), self.d)
+
+    @eps_plastic.setter
+    def eps_plastic(self, value):
+        """Set the eigenstrain field from a host (..., d, d) array (its
+        symmetric part is stored)."""
+        sym, _ = dev.to_sym_soa(np.asarray(value, dtype=np.float64), self.d)
+        self.kmc.eps_p.copy_(dev.upload(sym, FP64).to(self.kmc.eps_p.dtype))
+        if self.kmc.eps_p32 is not None:
+            self.kmc.eps_p32.copy_(self.kmc.eps_p.to(dev.torch.float32))
+
+    @property
+    def Q(self):
+        """Barriers of the current state (evaluated on access; the event loop
+        does not store them)."""
+        self.kmc.barrier_rates(self.ls.sig, self.sig_kind, self._params(), write_q=True)
+        return modes_from_device(self.kmc.Q, self.grid_shape)
+
+    @property
+    def Q0(self):
+        return modes_from_device(self.kmc.Q0, self.grid_shape)
+
+    @property
+    def soft_prop(self):
+        s = dev.download(self.kmc.soft)
+        out = np.zeros(self.grid_shape + ((4,) if self.d == 3 else (2,)))
+        out[..., 0] = s[0].reshape(self.grid_shape)
+        out[..., 1] = s[1].reshape(self.grid_shape)
+        return out
+
+    # ----- seams -----------------------------------------------------------
+    def _solve(self, eps_macro):
+        warm = self.ls_warm_start and self._solved_once
+        args = dict(max_iter=200, tol=1e-6)
+        args.update(self.solver_args)
+        self.ls_info = self.ls.solve(eps_macro, self.kmc.eps_p, warm=warm, max_iter=args["max_iter"],
+                                     tol=args["tol"], eps_plastic32=self.kmc.eps_p32)
+        self._solved_once = True
+        return self.ls_info
+
+    def sig_mean(self):
+        d = self.d
+        return np.array(self.ls_info.sig_mean[: d * d]).reshape(d, d)
+
+    def _params(self):
+        eff_volume = self.volume if self.scale_rate_by_volume else 1.0
+        return kmc_params(volume=self.volume, rate_volume=eff_volume, nu0=self.nu0, temperature=self.temperature,
+                          cap=self.softening_cap, tau=self.tau, time=self.time,
+                          scheme=1 if self.softening_scheme == "directional" else 0, cascade=self.cascade_mode)
+
+    def _after_flip_solve(self):
+        self.update_stresses()
+
+    def kmc_event(self, remaining_time, strain_unit_tensor):
+        """One pass of the inner KMC loop body.  Returns (event or None,
+        remaining_time); an event is (x, y[, z], m, time, is_instab)."""
+        p = self._params()
+        total = self.kmc.barrier_rates(self.ls.sig, self.sig_kind, p, write_q=False)
+        if total > 0:
+            u = self.rng.random_sample()
+            t_wait = -np.log(u) / total
+            if t_wait < remaining_time:
+                self.time += t_wait
+                self.eps_macro += strain_unit_tensor * (self.strain_rate * t_wait)
+                remaining_time -= t_wait
+                us = self.selection_stream.random_sample()
+                v, m = self.kmc.select(us, self.ls.sig, self.sig_kind, p)
+                coords = tuple(int(c) for c in self.kmc.voxel_coords(v))
+                is_instab = self.kmc.sel.q <= self.stability_threshold
+                self.kmc.flip(v, m, self.time, self.jp, self.jt, self.softening_cap, self.neighbor_softening_fraction)
+                if not (self.redraw_directions or self.redraw_barriers):
+                    raise NotImplementedError("single-mode redraw")
+                newcat = newq0 = None
+                if self.redraw_directions:
+                    newcat = planar_to_full(planar_catalog(self.M, self.gamma0, self.rng), self.d)
+                if self.redraw_barriers:
+                    newq0 = self.barrier_generator((self.M,))
+                self.kmc.set_voxel(v, newcat, newq0)
+                self._after_flip_solve()
+                ev = coords + (m, self.time, is_instab)
+                self.flip_event_history.append(ev)
+                return ev, remaining_time
+        self.eps_macro += strain_unit_tensor * (self.strain_rate * remaining_time)
+        self.time += remaining_time
+        return None, 0.0
+
+    def footprint_bytes(self):
+        """HBM bytes held by this simulation's device state."""
+        t = 0
+        for x in (self.ls.lam, self.ls.mu, self.ls.eps, self.ls.sig, self.ls.ws,
+                  getattr(self.ls, "lam32", None), getattr(self.ls, "mu32", None), getattr(self.ls, "eps32", None),
+                  getattr(self.ls, "ws32", None), self.kmc.Q0, self.kmc.Q, self.kmc.cat, self.kmc.soft,
+                  self.kmc.tlast, self.kmc.prev, self.kmc.eps_p, self.kmc.eps_p32, self.kmc.ws.buf):
+            if x is not None:
+                t += x.numel() * x.element_size()
+        return t
+

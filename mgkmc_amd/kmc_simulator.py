@@ -1,0 +1,122 @@
+"""Device-resident 2D STZ/KMC simulation (drop-in for mgkmc/kmc_simulator.py
+KmcSimulation2D).
+
+Same constructor and run_simulation signatures as the reference
+(mgkmc/kmc_simulator.py:14-35, :796-803).  The per-event work of its KMC loop
+(:871-1101, small strain, linear elasticity) runs on the GPU through
+``mgkmc_amd.kmc_engine.DeviceKmc``; elastic_run (:387-525, the linear branch)
+is the device LS solve with the reference's plane mode and its
+unstable-stress check; the mixed-BC loop (:1106-1143) keeps the reference's
+compliance-then-secant update on the host.  Extensions as in
+``mgkmc_amd.aqs.ThermalSimulation``.
+"""
+import numpy as np
+
+from . import _lib
+from .kmc_engine import DeviceKmc
+
+
+class KmcSimulation2D(DeviceKmc):
+    def __init__(self, nx, ny, M, gamma0, E_field, nu_field, pixel=1.0, barrier_generator=None, barrier_kwargs={},
+                 softening_scheme="isotropic", softening_cap=2.0, jp=10.0, jt=30.0, neighbor_softening_fraction=0.0,
+                 q_act_temp=0.37, output_dir="output", temperature=0.0, strain_rate=1.0, stability_threshold=0.0,
+                 nu0=1e13, plane_mode="plane_strain", fast_patching=None, cascade_mode=False,
+                 scale_rate_by_volume=False, redraw_directions=True, redraw_barriers=True, enable_thermal=False,
+                 Cp=420.0, rho=6125.0, thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0,
+                 thermostat=False, tau_bath=0.0, strain_assumption="small_strain", hyperelastic_model="svk",
+                 A_m=0.0, B_m=0.0, C_m=0.0, solver="al", stz_mode="pure_shear", d=0.0, k=0.0, eta=5.0,
+                 mu_floor_fraction=0.1, v1=0.0, v2=0.0, v3=0.0, g1=0.0, g2=0.0, g3=0.0, g4=0.0,
+                 strain_capping_enabled=False, strain_capping_limit=None, strain_capping_tangent_ratio=0.1,
+                 strain_capping_type="piecewise", strain_capping_smooth_power=1.0,
+                 ls_precision="fp64", ls_warm_start=False, rng=None, selection_stream=None, init_modes=None):
+        self._unsupported(cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model,
+                          [f"stz_mode={stz_mode}"] if stz_mode != "pure_shear" else [])
+        self.nx, self.ny = nx, ny
+        self.output_dir = output_dir
+        self.strain_assumption = strain_assumption
+        self.hyperelastic_model = hyperelastic_model
+        self.stz_mode = stz_mode
+        self._setup((nx, ny), M, gamma0, E_field, nu_field, pixel, barrier_generator, barrier_kwargs,
+                    softening_scheme, softening_cap, jp, jt, neighbor_softening_fraction, temperature, strain_rate,
+                    stability_threshold, redraw_directions, redraw_barriers, nu0, q_act_temp, cascade_mode,
+                    scale_rate_by_volume, plane_mode, ls_precision, ls_warm_start, rng, selection_stream,
+                    init_modes, dict(mean=2.0, std=0.6, min_cutoff=0.1))
+
+    def elastic_run(self, eps_macro):
+        """Small-strain linear elastic equilibrium (mgkmc/kmc_simulator.py:515-525):
+        LS solve, then ValueError on a NaN stress or |<sigma>| > 20 GPa.
+        Returns <sigma> (2, 2)."""
+        inf = self._solve(eps_macro)
+        sig_mean = self.sig_mean()
+        if inf.status == _lib.MGK_ERR_NONFINITE or not np.all(np.isfinite(sig_mean)) or \
+                np.max(np.abs(sig_mean)) / 1e9 > 20.0:
+            raise ValueError(f"Unstable stress in small-strain elastic_run: {sig_mean / 1e9} GPa")
+        return sig_mean
+
+    def update_stresses(self):
+        return self.elastic_run(self.eps_macro)
+
+    def mixed_bc(self, stress_targets, mixed_tol, mixed_max_iter):
+        """Macro loop after an increment (mgkmc/kmc_simulator.py:1106-1143):
+        solve, stop if every targeted stress is within mixed_tol, else the
+        Poisson-corrected compliance step, replaced per component by the secant
+        slope once two iterates exist (and the slope exceeds 0.01 E_avg)."""
+        E_avg = self.E_field.mean()
+        nu_avg = self.nu_field.mean()
+        prev_eps, prev_sig = {}, {}
+        for _ in range(mixed_max_iter):
+            sigM = self.elastic_run(self.eps_macro)
+            err = np.zeros((2, 2))
+            err_max = 0.0
+            for idx, target in stress_targets.items():
+                if idx[0] < 2 and idx[1] < 2:
+                    e = target - sigM[idx]
+                    err[idx] = e
+                    err_max = max(err_max, abs(e))
+            if err_max < mixed_tol:
+                break
+            d_eps = (err - nu_avg * np.trace(err) * np.eye(2)) / E_avg
+            for idx in stress_targets.keys():
+                if idx[0] < 2 and idx[1] < 2:
+                    step = d_eps[idx]
+                    if idx in prev_eps:
+                        d_e = self.eps_macro[idx] - prev_eps[idx]
+                        d_s = sigM[idx] - prev_sig[idx]
+                        if abs(d_e) > 1e-14 and np.isfinite(d_s):
+                            slope = d_s / d_e
+                            if np.isfinite(slope) and slope > 0.01 * E_avg:
+                                step = err[idx] / slope
+                    prev_eps[idx] = self.eps_macro[idx]
+                    prev_sig[idx] = sigM[idx]
+                    self.eps_macro[idx] += step
+
+    def run_simulation(self, n_global_steps, step_size, component=(0, 0), stress_targets={}, mixed_tol=1e-4,
+                       mixed_max_iter=50, max_kmc_steps_pct=0.3, max_events=None, **kwargs):
+        """KMC + elastic increments (mgkmc/kmc_simulator.py:796-1143, KMC mode).
+        Logging, checkpoint and VTK options are accepted and ignored.
+        ``max_events`` (extension) stops after that many events.  Returns the
+        event list (x, y, m, time, is_instab)."""
+        self.driving_component = component
+        self.stress_targets = stress_targets
+        unit = np.zeros((2, 2))
+        unit[component] = 1.0
+        self.elastic_run(self.eps_macro)
+        dt_step = abs(step_size) / self.strain_rate if self.strain_rate > 0 else 1.0
+        max_seq = int(max_kmc_steps_pct * self.nx * self.ny)
+        seq = 0
+        remaining = 0.0
+        done = 0
+        events = 0
+        while done < n_global_steps:
+            remaining += dt_step
+            while remaining > 0:
+                ev, remaining = self.kmc_event(remaining, unit)
+                if ev is None:
+                    break
+                events += 1
+                seq = seq + 1 if ev[-1] else 0
+                if seq > max_seq or (max_events is not None and events >= max_events):
+                    return self.flip_event_history
+            self.mixed_bc(stress_targets, mixed_tol, mixed_max_iter)
+            done += 1
+        return self.flip_event_history

@@ -158,3 +158,76 @@ def test_kmc_event_sequence_matches_oracle(gpu):
     assert np.allclose(ep, st_ref["eps_plastic"], rtol=0, atol=1e-15)
     sig = sim.sig_field
     assert np.abs(sig - st_ref["sig"]).max() <= 1e-9 * np.abs(st_ref["sig"]).max()
+
+
+@pytest.mark.parametrize("scheme,plane_mode", [("isotropic", "plane_strain"), ("directional", "plane_stress")])
+def test_kmc2d_event_sequence_matches_oracle(gpu, scheme, plane_mode):
+    # 2D device loop (KmcSimulation2D) vs the oracle 2D loop, with the mixed-BC
+    # secant loop active between increments
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    from oracle.kmc2d_loop import run_events_2d
+    print(f"[kmc2d] {scheme} {plane_mode}", flush=True)
+    n, M = 16, 20
+    rng = np.random.default_rng(2)
+    mu = 26.923e9 * rng.uniform(0.9, 1.1, (n, n))
+    nu = np.full((n, n), 0.35)
+    E = 2 * mu * (1 + nu)
+    kw = dict(mean=2.2, std=0.45, min_cutoff=0.0)
+    args = dict(softening_scheme=scheme, softening_cap=0.916, jp=20.0, jt=0.0)
+    ev_ref, st_ref = run_events_2d(n, n, M, 0.14, E, nu, 0.7, 1276.5, 1e7, 1e-2, (0, 0), 30, seed=21, sel_seed=22,
+                                   barrier_kwargs=kw, plane_mode=plane_mode, stress_targets={(1, 1): 0.0},
+                                   mixed_tol=1e6, **args)
+    sim = KmcSimulation2D(n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                          temperature=1276.5, strain_rate=1e7, plane_mode=plane_mode,
+                          rng=np.random.RandomState(21), selection_stream=np.random.RandomState(22), **args)
+    # the oracle seeds the global stream; the simulator was given an equal private one
+    ev = sim.run_simulation(10 ** 9, 1e-2, component=(0, 0), stress_targets={(1, 1): 0.0}, mixed_tol=1e6,
+                            max_events=30)
+    got = [e[:3] for e in ev]
+    ref = [e[:3] for e in ev_ref]
+    print("events:", got[:8], flush=True)
+    assert got == ref
+    assert np.allclose([e[3] for e in ev], [e[3] for e in ev_ref], rtol=1e-9)
+    assert [e[4] for e in ev] == [e[4] for e in ev_ref]
+    assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
+    assert np.abs(sim.eps_macro - st_ref["eps_macro"]).max() <= 1e-9 * np.abs(st_ref["eps_macro"]).max()
+    sig = sim.sig_field
+    assert np.abs(sig - st_ref["sig"]).max() <= 1e-9 * np.abs(st_ref["sig"]).max()
+
+
+def test_device_init_modes_match_oracle(gpu):
+    from mgkmc_amd.kmc_device import KmcState
+    from mgkmc_amd.kmc_engine import device_init_modes
+    from oracle.philox import init_modes
+    shape, M = (12, 10, 9), 7
+    st = KmcState(shape, M, None, None)
+    prm = dict(mean=2.2, std=0.45, min_cutoff=0.0, max_cutoff=None)
+    device_init_modes(shape, M, 0.14, prm, 987654321987, st.Q0, st.cat)
+    q, cat = init_modes(int(np.prod(shape)), M, 987654321987, 2.2, 0.45, 0.0, None, 0.14)
+    gq = st.Q0.cpu().numpy()
+    gc = st.cat.cpu().numpy()
+    assert np.abs(gq - q).max() <= 1e-12 * np.abs(q).max()
+    assert np.abs(gc - cat).max() <= 1e-12 * np.abs(cat).max()
+
+
+def test_kmc_refined_ls_event_sequence_matches_fp64(gpu):
+    # the FP32 + FP64 refinement LS changes sigma by ~tol; the event sequence of
+    # the FP64 path is reproduced (selection differs only for draws within
+    # ~1e-8 relative of a boundary)
+    from mgkmc_amd.aqs import ThermalSimulation
+    n, M = 16, 20
+    rng = np.random.default_rng(5)
+    mu = 26.923e9 * rng.uniform(0.9, 1.1, (n, n, n))
+    nu = np.full((n, n, n), 0.35)
+    E = 2 * mu * (1 + nu)
+    kw = dict(mean=2.2, std=0.45, min_cutoff=0.0)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        sim = ThermalSimulation(n, n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                                softening_cap=0.916, jp=20.0, jt=0.0, temperature=1276.5, strain_rate=1e7,
+                                ls_precision=prec, rng=np.random.RandomState(31),
+                                selection_stream=np.random.RandomState(32))
+        out[prec] = (sim.run_simulation(10 ** 9, 1e-4, component=(0, 0), max_events=25), sim.sig_field)
+    assert [e[:4] for e in out["fp64"][0]] == [e[:4] for e in out["mixed"][0]]
+    s64, smx = out["fp64"][1], out["mixed"][1]
+    assert np.abs(smx - s64).max() <= 1e-5 * np.abs(s64).max()

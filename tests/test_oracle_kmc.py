@@ -189,3 +189,46 @@ def test_vectorised_draws_match_reference_loop():
     a = [np.random.normal() for _ in range(11)]
     np.random.seed(4)
     assert a == list(np.random.normal(size=11))
+
+
+def test_philox_known_answers():
+    # Random123 known-answer vectors for Philox4x32-10
+    from oracle.philox import philox4x32_10
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for ctr, key, want in kat:
+        out = philox4x32_10([np.array([c], dtype=np.uint64) for c in ctr], key)
+        assert tuple(int(o[0]) for o in out) == want
+
+
+def test_device_init_modes_statistics():
+    # the oracle's draws: Q0 ~ clip(N(mean, std)), catalog ~ 0.5 gamma0 N(0,1)
+    from oracle.philox import init_modes
+    q, cat = init_modes(4096, 20, 12345, 2.2, 0.45, 0.0, None, 0.14)
+    assert q.shape == (20, 4096) and cat.shape == (2, 20, 4096)
+    assert abs(q.mean() - 2.2) < 0.01 and abs(q.std() - 0.45) < 0.01 and q.min() >= 0.0
+    g = cat / 0.07
+    assert abs(g.mean()) < 0.01 and abs(g.std() - 1.0) < 0.01
+    assert abs(np.corrcoef(g[0].ravel(), g[1].ravel())[0, 1]) < 0.01
+    q2, _ = init_modes(4096, 20, 12346, 2.2, 0.45, 0.0, None, 0.14)
+    assert not np.array_equal(q, q2)
+
+
+def test_oracle_2d_loop_invariants():
+    # event times increase; the driven macro strain is strain_rate * time between
+    # mixed-BC corrections of the other components; stress targets are met
+    from oracle.kmc2d_loop import run_events_2d
+    n, M = 6, 4
+    rng = np.random.default_rng(0)
+    mu = 26.923e9 * rng.uniform(0.9, 1.1, (n, n))
+    nu = np.full((n, n), 0.35)
+    E = 2 * mu * (1 + nu)
+    ev, st = run_events_2d(n, n, M, 0.14, E, nu, 0.7, 1276.5, 1e7, 2e-2, (0, 0), 6, seed=3, sel_seed=4,
+                           barrier_kwargs=dict(mean=2.2, std=0.45, min_cutoff=0.0), stress_targets={(1, 1): 0.0},
+                           mixed_tol=1e6, ls_tol=1e-8)
+    times = [e[3] for e in ev]
+    assert len(ev) == 6 and all(b > a for a, b in zip(times, times[1:]))
+    assert abs(st["eps_macro"][0, 0] - 1e7 * st["time"]) <= 1e-12 * abs(st["eps_macro"][0, 0])
+    assert np.all(np.abs(st["eps_plastic"]) < 1.0)
