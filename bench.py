@@ -30,6 +30,11 @@ MU_UNIT = 70e9 / 2.6  # Pa; config.yaml E = 70 GPa, nu = 0.3
 HBM_PEAK = 8.0e12
 
 
+def log(msg):
+    """Progress line on stderr (the JSON result line goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def fields(n, seed=1):
     rng = np.random.default_rng(seed)
     mu = MU_UNIT * rng.uniform(0.9, 1.1, (n, n, n))
@@ -200,6 +205,20 @@ def gpu_fs(n, increments):
             "seconds_per_increment": dt / increments, "prof": prof}
 
 
+def gpu_serving(n_requests, rate, events):
+    """Serving stream (Poisson arrivals at `rate`/s, seed 4; grids uniform over
+    64^2..512^2 and 64^3..192^3; kT/E0 ~ U[0.01, 0.08]; `events` KMC events
+    per request with FP32 + FP64 refinement LS), 8 slots, HBM admission."""
+    from mgkmc_amd.serving import SimServer, request_stream, summarize
+    reqs = request_stream(n_requests, rate, 4, events)
+    srv = SimServer(max_active=8)
+    res = srv.serve(reqs)
+    out = summarize(res, srv.wall)
+    out.update(slots=8, rate_per_s=rate, events_per_request=events, peak_active=srv.peak_active,
+               peak_hbm_gb=srv.peak_used / 1e9)
+    return out
+
+
 def cpu_baseline(n_sample=128, iters=3):
     """The oracle (reference structure: materialised Gamma, per-component
     complex FFTs) on this host: LS iterations on a bounded 128^3 sample."""
@@ -266,6 +285,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kmc-events", type=int, default=20)
     ap.add_argument("--fs-increments", type=int, default=2)
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-events", type=int, default=100)
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run only the short LS workload used for rocprofv3 --pmc counter runs")
     a = ap.parse_args()
@@ -276,9 +298,11 @@ def main():
     if a.pmc_workload:
         pmc_workload(a.n)
         return
+    log(f"LS forms at {a.n}^3")
     res = gpu_run(a.n, a.steps, a.warmup)
     rmx, r64, r32 = res[2], res[0], res[1]
     prof = dict(r32["prof"] or {})
+    log("KMC legs")
     kmcmx = gpu_kmc(a.n, a.kmc_events, 2, "mixed")
     kmc32 = gpu_kmc(a.n, a.kmc_events, 2, "fp32")
     kmc64 = gpu_kmc(a.n, a.kmc_events, 2, "fp64")
@@ -288,7 +312,10 @@ def main():
     for k in ("kmc_barrier_rates", "kmc_select"):
         if k in kmc32["prof"]:
             prof[k] = kmc32["prof"][k]
+    log("finite strain")
     fsr = gpu_fs(a.n, a.fs_increments) if a.fs_increments > 0 else None
+    log("serving stream")
+    srv = gpu_serving(a.serve_requests, a.serve_rate, a.serve_events) if a.serve_requests > 0 else None
     roof = None
     if dom:
         name, (ms, by, cnt) = dom
@@ -298,6 +325,7 @@ def main():
                 "bytes_per_launch": by / cnt, "avg_launch_ms": ms / cnt,
                 "per_kernel": {k: {"avg_ms": v[0] / v[2], "GB_s": (v[1] / v[2]) / (v[0] / v[2] * 1e-3) / 1e9}
                                for k, v in prof.items()}}
+    log("CPU baseline")
     cpu = None if a.no_cpu else cpu_baseline()
     steps = a.steps
     line = {
@@ -340,6 +368,7 @@ def main():
             "per_kernel": {k: {"avg_ms": v[0] / v[2], "GB_s": (v[1] / v[2]) / (v[0] / v[2] * 1e-3) / 1e9}
                            for k, v in fsr["prof"].items()},
             "workload": f"{a.n}^3 Landau + smooth capping, simple shear F_xy 0.1%/increment, tol_CG 1e-8"},
+        "serving": srv,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -1,0 +1,278 @@
+"""Serving loop: many KMC simulations resident on one GPU, continuously admitted.
+
+A request is one simulation (2D or 3D grid, temperature, seed) that runs a
+fixed number of KMC events, each with its LS re-equilibration in the FP32 +
+FP64 refinement form.  The server
+
+  * admits arrived requests in arrival order while the estimated HBM footprint
+    of the admitted set stays within the budget and a slot is free (requests
+    that arrive while others run are admitted as soon as slots and memory free
+    up: continuous batching, not a queue in front of one simulation);
+  * runs every admitted simulation on its own worker thread and HIP stream, so
+    the kernels of different simulations overlap on the device (small grids
+    are launch-latency bound and leave most CUs idle on their own);
+  * keeps each simulation's fields, modes and KMC state in HBM for its whole
+    life; its random streams are private (numpy RandomState for waiting times,
+    selection and redraws; Philox on the device for the initial modes), so a
+    request's result does not depend on what else is running.
+
+There is no counterpart in the reference (its simulators run one at a time).
+"""
+import threading
+import time
+from collections import deque
+from concurrent.futures import ThreadPoolExecutor, wait, FIRST_COMPLETED
+from dataclasses import dataclass, field
+
+import ctypes
+import numpy as np
+
+from . import _lib
+from . import device as dev
+from .linear_elastic_simulator import LsSolver, MIXED, FP64, FP32
+
+MU_UNIT = 70e9 / 2.6   # Pa: E = 70 GPa, nu = 0.3
+E0_EV = 2.2            # barrier mean (eV): T = (kT/E0) * E0 / kB
+KB = 8.617e-5
+SIZES_2D = ((64, 64), (128, 128), (256, 256), (512, 512))
+SIZES_3D = ((64, 64, 64), (96, 96, 96), (128, 128, 128), (192, 192, 192))
+
+
+@dataclass
+class SimRequest:
+    rid: int
+    arrival: float          # seconds after the start of the stream
+    shape: tuple
+    kT: float               # kT / E0
+    seed: int
+    events: int = 100
+    ls_precision: str = "mixed"
+    M: int = 20
+
+
+@dataclass
+class SimResult:
+    rid: int
+    shape: tuple
+    events: list
+    eps_macro: np.ndarray
+    sig_mean: np.ndarray
+    time: float
+    eps_p_sum: float        # FP64 sum of the eigenstrain field (a fingerprint of the flips)
+    ls_iters: int
+    footprint: int
+    t_arrival: float = 0.0
+    t_admit: float = 0.0
+    t_done: float = 0.0
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def latency(self):
+        return self.t_done - self.t_arrival
+
+
+def request_stream(n, rate=4.0, seed=4, events=100, sizes=SIZES_2D + SIZES_3D, kT=(0.01, 0.08),
+                   ls_precision="mixed"):
+    """n requests with Poisson arrivals (mean rate `rate` per second), grid
+    sizes uniform over `sizes`, kT/E0 ~ U[kT], per-request seeds, from seed."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n)
+    arrivals = np.cumsum(gaps)
+    idx = rng.integers(0, len(sizes), n)
+    kts = rng.uniform(kT[0], kT[1], n)
+    seeds = rng.integers(1, 2 ** 31 - 1, n)
+    return [SimRequest(i, float(arrivals[i]), tuple(sizes[idx[i]]), float(kts[i]), int(seeds[i]), events,
+                       ls_precision) for i in range(n)]
+
+
+def _kmc_bytes(shape, M, eps_prec):
+    d = len(shape)
+    C = 6 if d == 3 else 3
+    NV = int(np.prod(shape))
+    n = ctypes.c_size_t()
+    _lib.check(_lib.load().mgk_kmc_workspace_bytes(NV, ctypes.byref(n)), "mgk_kmc_workspace_bytes")
+    es = 4 if eps_prec == FP32 else 8
+    shadow = 4 * C * NV if eps_prec == MIXED else 0
+    return NV * M * 8 * 3 + NV * 8 * 3 + C * NV * es + shadow + int(n.value)
+
+
+def footprint_bytes(req):
+    """Estimated HBM bytes of a request's resident state (LS fields and
+    workspaces, modes, KMC state) plus transient upload buffers."""
+    prec = {"mixed": MIXED, "fp64": FP64, "fp32": FP32}[req.ls_precision]
+    NV = int(np.prod(req.shape))
+    C = 6 if len(req.shape) == 3 else 3
+    ls = LsSolver.footprint_bytes(req.shape, prec)
+    kmc = _kmc_bytes(req.shape, req.M, prec)
+    transient = NV * 8 * (C + 2)
+    return int(1.05 * (ls + kmc + transient)) + (16 << 20)
+
+
+def build_simulation(req):
+    """The request's simulation with the serving recipe: mu ~ U[0.9, 1.1] *
+    26.92 GPa, nu = 0.35, zero-mean Gaussian eigenstrain (std 1e-3 per stored
+    component), M modes, gamma0 0.14, pixel 0.7 nm, Gaussian barriers (2.2,
+    0.45, clipped at 0) drawn on the device, jp 20, jt 0, cap 0.916,
+    strain rate 1e7 /s along xx."""
+    from .aqs import ThermalSimulation
+    from .kmc_simulator import KmcSimulation2D
+    rng = np.random.default_rng(req.seed)
+    shape = tuple(req.shape)
+    d = len(shape)
+    mu = MU_UNIT * rng.uniform(0.9, 1.1, shape)
+    nu = np.full(shape, 0.35)
+    E = 2.0 * mu * (1.0 + nu)
+    C = 6 if d == 3 else 3
+    ep = rng.standard_normal((C,) + shape) * 1e-3
+    ep -= ep.mean(axis=tuple(range(1, d + 1)), keepdims=True)
+    T = req.kT * E0_EV / KB
+    common = dict(pixel=0.7, barrier_generator="gaussian", barrier_kwargs=dict(mean=2.2, std=0.45, min_cutoff=0.0),
+                  softening_cap=0.916, jp=20.0, jt=0.0, temperature=T, strain_rate=1e7,
+                  ls_precision=req.ls_precision, rng=np.random.RandomState(req.seed),
+                  selection_stream=np.random.RandomState(req.seed + 1), init_modes=("device", req.seed))
+    if d == 3:
+        sim = ThermalSimulation(*shape, req.M, 0.14, E, nu, **common)
+    else:
+        sim = KmcSimulation2D(*shape, req.M, 0.14, E, nu, **common)
+    prec = sim.kmc.eps_p.dtype
+    sim.kmc.eps_p.copy_(dev.torch.from_numpy(ep).to(device="cuda", dtype=prec))
+    if sim.kmc.eps_p32 is not None:
+        sim.kmc.eps_p32.copy_(sim.kmc.eps_p.to(dev.torch.float32))
+    return sim
+
+
+_tls = threading.local()
+
+
+def _worker_stream():
+    s = getattr(_tls, "stream", None)
+    if s is None:
+        s = dev.torch.cuda.Stream()  # default priority
+        _tls.stream = s
+    return s
+
+
+def run_request(req):
+    """Run one request to completion on the calling thread's HIP stream."""
+    torch = dev.torch
+    s = _worker_stream()
+    with torch.cuda.stream(s):
+        sim = build_simulation(req)
+        fp = sim.footprint_bytes()
+        sim.update_stresses()
+        d = sim.d
+        unit = np.zeros((d, d))
+        unit[0, 0] = 1.0
+        iters = 0
+        for _ in range(req.events):
+            ev, _ = sim.kmc_event(1e30, unit)
+            if ev is None:
+                break
+            iters += sim.ls_info.iters + sim.ls.iters32
+        eps_p_sum = float(sim.kmc.eps_p.to(torch.float64).sum())
+        res = SimResult(req.rid, tuple(req.shape), list(sim.flip_event_history), sim.eps_macro.copy(),
+                        sim.sig_mean().copy(), sim.time, eps_p_sum, iters, fp)
+        s.synchronize()
+        del sim
+    return res
+
+
+class SimServer:
+    """Admission by HBM footprint, one worker thread + HIP stream per slot."""
+
+    def __init__(self, hbm_budget=None, max_active=8):
+        dev.require_gpu()
+        if hbm_budget is None:
+            free, _total = dev.torch.cuda.mem_get_info()
+            hbm_budget = int(0.85 * free)
+        self.hbm_budget = int(hbm_budget)
+        self.max_active = int(max_active)
+        self.peak_used = 0
+        self.peak_active = 0
+
+    def serve(self, requests, time_scale=1.0, on_done=None):
+        """Replay `requests` against the wall clock (arrival * time_scale) and
+        return their SimResults in completion order."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        queue = deque()
+        active = {}
+        used = 0
+        results = []
+        for r in requests:
+            if footprint_bytes(r) > self.hbm_budget:
+                raise ValueError(f"request {r.rid} ({r.shape}) needs more than the HBM budget")
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=self.max_active) as pool:
+            while pending or queue or active:
+                now = time.perf_counter() - t0
+                while pending and pending[0].arrival * time_scale <= now:
+                    queue.append(pending.popleft())
+                while queue and len(active) < self.max_active:
+                    fp = footprint_bytes(queue[0])
+                    if used + fp > self.hbm_budget:
+                        break
+                    r = queue.popleft()
+                    used += fp
+                    fut = pool.submit(run_request, r)
+                    active[fut] = (r, fp, time.perf_counter() - t0)
+                self.peak_used = max(self.peak_used, used)
+                self.peak_active = max(self.peak_active, len(active))
+                timeout = None
+                if pending:
+                    timeout = max(0.0, pending[0].arrival * time_scale - (time.perf_counter() - t0))
+                if active:
+                    done, _ = wait(list(active), timeout=timeout, return_when=FIRST_COMPLETED)
+                    for fut in done:
+                        r, fp, t_admit = active.pop(fut)
+                        used -= fp
+                        res = fut.result()
+                        res.t_arrival = r.arrival * time_scale
+                        res.t_admit = t_admit
+                        res.t_done = time.perf_counter() - t0
+                        results.append(res)
+                        if on_done is not None:
+                            on_done(res)
+                elif timeout:
+                    time.sleep(timeout)
+        self.wall = time.perf_counter() - t0
+        return results
+
+
+def summarize(results, wall):
+    lat = np.array([r.latency for r in results])
+    ev = sum(len(r.events) for r in results)
+    return {"requests": len(results), "wall_s": wall, "requests_per_s": len(results) / wall,
+            "kmc_events_per_s": ev / wall, "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)), "latency_mean_s": float(lat.mean()),
+            "latency_max_s": float(lat.max())}
+
+
+def main(argv=None):
+    """python -m mgkmc_amd.serving [--requests N] [--rate R] [--events E] [--slots S]:
+    replay a Poisson request stream and print one JSON summary line."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--events", type=int, default=100)
+    ap.add_argument("--slots", type=int, default=8)
+    a = ap.parse_args(argv)
+    reqs = request_stream(a.requests, a.rate, a.seed, a.events)
+    srv = SimServer(max_active=a.slots)
+    done = [0]
+
+    def progress(res):
+        done[0] += 1
+        if done[0] % 16 == 0:
+            print(f"[serving] {done[0]}/{len(reqs)} done", flush=True)
+    res = srv.serve(reqs, on_done=progress)
+    out = summarize(res, srv.wall)
+    out.update(slots=a.slots, rate=a.rate, events_per_request=a.events, peak_active=srv.peak_active,
+               peak_hbm_bytes=srv.peak_used, hbm_budget=srv.hbm_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
