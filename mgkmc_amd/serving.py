@@ -34,6 +34,7 @@ from .linear_elastic_simulator import LsSolver, MIXED, FP64, FP32
 MU_UNIT = 70e9 / 2.6   # Pa: E = 70 GPa, nu = 0.3
 E0_EV = 2.2            # barrier mean (eV): T = (kT/E0) * E0 / kB
 KB = 8.617e-5
+APPLIED_STRAIN = 0.005
 SIZES_2D = ((64, 64), (128, 128), (256, 256), (512, 512))
 SIZES_3D = ((64, 64, 64), (96, 96, 96), (128, 128, 128), (192, 192, 192))
 
@@ -112,8 +113,10 @@ def build_simulation(req):
     """The request's simulation with the serving recipe: mu ~ U[0.9, 1.1] *
     26.92 GPa, nu = 0.35, zero-mean Gaussian eigenstrain (std 1e-3 per stored
     component), M modes, gamma0 0.14, pixel 0.7 nm, Gaussian barriers (2.2,
-    0.45, clipped at 0) drawn on the device, jp 20, jt 0, cap 0.916,
-    strain rate 1e7 /s along xx."""
+    0.45, clipped at 0) drawn on the device, jp 20, jt 0, cap 0.916, and a
+    fixed macro strain eps_xx = APPLIED_STRAIN (strain rate 0: thermally
+    activated relaxation under load; a driven strain rate would strain a
+    small cold grid by ~10% per event, since waiting times grow as e^(Q/kT))."""
     from .aqs import ThermalSimulation
     from .kmc_simulator import KmcSimulation2D
     rng = np.random.default_rng(req.seed)
@@ -127,7 +130,7 @@ def build_simulation(req):
     ep -= ep.mean(axis=tuple(range(1, d + 1)), keepdims=True)
     T = req.kT * E0_EV / KB
     common = dict(pixel=0.7, barrier_generator="gaussian", barrier_kwargs=dict(mean=2.2, std=0.45, min_cutoff=0.0),
-                  softening_cap=0.916, jp=20.0, jt=0.0, temperature=T, strain_rate=1e7,
+                  softening_cap=0.916, jp=20.0, jt=0.0, temperature=T, strain_rate=0.0,
                   ls_precision=req.ls_precision, rng=np.random.RandomState(req.seed),
                   selection_stream=np.random.RandomState(req.seed + 1), init_modes=("device", req.seed))
     if d == 3:
@@ -138,6 +141,7 @@ def build_simulation(req):
     sim.kmc.eps_p.copy_(dev.torch.from_numpy(ep).to(device="cuda", dtype=prec))
     if sim.kmc.eps_p32 is not None:
         sim.kmc.eps_p32.copy_(sim.kmc.eps_p.to(dev.torch.float32))
+    sim.eps_macro[0, 0] = APPLIED_STRAIN
     return sim
 
 
@@ -159,19 +163,25 @@ def run_request(req):
     with torch.cuda.stream(s):
         sim = build_simulation(req)
         fp = sim.footprint_bytes()
-        sim.update_stresses()
         d = sim.d
         unit = np.zeros((d, d))
         unit[0, 0] = 1.0
         iters = 0
-        for _ in range(req.events):
-            ev, _ = sim.kmc_event(1e30, unit)
-            if ev is None:
-                break
-            iters += sim.ls_info.iters + sim.ls.iters32
+        err = None
+        try:
+            sim.update_stresses()
+            for _ in range(req.events):
+                ev, _ = sim.kmc_event(1e30, unit)
+                if ev is None:
+                    break
+                iters += sim.ls_info.iters + sim.ls.iters32
+        except (ValueError, FloatingPointError) as e:  # the simulators' solver-failure errors
+            err = str(e)
         eps_p_sum = float(sim.kmc.eps_p.to(torch.float64).sum())
         res = SimResult(req.rid, tuple(req.shape), list(sim.flip_event_history), sim.eps_macro.copy(),
-                        sim.sig_mean().copy(), sim.time, eps_p_sum, iters, fp)
+                        sim.sig_mean().copy() if sim.ls_info is not None else None, sim.time, eps_p_sum, iters, fp)
+        if err is not None:
+            res.extra["error"] = err
         s.synchronize()
         del sim
     return res
@@ -241,7 +251,8 @@ class SimServer:
 def summarize(results, wall):
     lat = np.array([r.latency for r in results])
     ev = sum(len(r.events) for r in results)
-    return {"requests": len(results), "wall_s": wall, "requests_per_s": len(results) / wall,
+    return {"requests": len(results), "failed": sum(1 for r in results if "error" in r.extra),
+            "wall_s": wall, "requests_per_s": len(results) / wall,
             "kmc_events_per_s": ev / wall, "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)), "latency_mean_s": float(lat.mean()),
             "latency_max_s": float(lat.max())}

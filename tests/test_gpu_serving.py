@@ -27,7 +27,7 @@ def test_solo_vs_mix_identical(gpu):
         m = mix[q.rid]
         print(f"[serving] rid={q.rid} shape={q.shape} events={len(m.events)} solo==mix: {_same(solo, m)}",
               flush=True)
-        assert len(m.events) == q.events
+        assert len(m.events) == q.events and "error" not in m.extra
         assert _same(solo, m)
 
 
