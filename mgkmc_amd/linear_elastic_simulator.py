@@ -202,3 +202,88 @@ def spectral_solver_3d(E, nu, eps_bar, eps_plastic=None, max_iter=200, tol=1e-6,
                        verbose=False, pixel=1.0, Gamma=None, eps_init=None):
     """3D LS solver (mgkmc/linear_elastic_simulator.py:274-376) on the GPU."""
     return _drop_in(3, E, nu, eps_bar, eps_plastic, max_iter, tol, "plane_strain", eps_init)
+
+
+def _simulation(d, E, nu, target_strain_mask, target_values, n_steps, pixel, tol_ls, max_iter_ls, tol_macro,
+                max_iter_macro, store, plane_mode, precision=FP64, info=None):
+    """Mixed stress/strain load stepping on one device solver (the reference's
+    Gamma-once + warm-start structure): per step, macro iterations restart from
+    the previous step's converged field; the stress-controlled entries of
+    eps_bar get (err - nu_avg tr(err) I) / E_avg until max |err| < tol_macro."""
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    nu = np.ascontiguousarray(nu, dtype=np.float64)
+    if E.ndim != d:
+        raise ValueError(f"E must be {d}D")
+    solver = LsSolver(E, nu, plane_mode=plane_mode, precision=precision)
+    torch = dev.torch
+    E_avg, nu_avg = E.mean(), nu.mean()
+    mu_mean = float(solver.mu_host.mean())
+    mask = np.asarray(target_strain_mask, dtype=bool)
+    smask = ~mask
+    tv = np.asarray(target_values, dtype=np.float64)
+    eb = np.zeros((d, d))
+    warm = None  # device copy of the previous step's field
+    warm32 = None
+    em_l, sm_l, e_l, s_l, its = [], [], [], [], []
+    for s in range(n_steps + 1):
+        tgt = tv * (s / n_steps)
+        max_err = 0.0
+        for _ in range(max_iter_macro):
+            eb[mask] = tgt[mask]
+            if warm is not None:
+                solver.eps.copy_(warm)
+                if precision == MIXED:
+                    solver.eps32.copy_(warm32)
+            inf = solver.solve(eb, None, warm=warm is not None, max_iter=max_iter_ls, tol=tol_ls)
+            its.append(inf.iters + solver.iters32)
+            a_bar = 0.5 * (eb - eb.T)
+            epsM = np.array(inf.eps_mean[: d * d]).reshape(d, d)
+            sigM = np.array(inf.sig_mean[: d * d]).reshape(d, d) + 2.0 * mu_mean * a_bar
+            err = np.zeros((d, d))
+            err[smask] = tgt[smask] - sigM[smask]
+            max_err = np.max(np.abs(err[smask])) if np.any(smask) else 0.0
+            if max_err < tol_macro:
+                break
+            de = (err - nu_avg * np.trace(err) * np.eye(d)) / E_avg
+            eb[smask] += de[smask]
+        warm = solver.eps.clone()
+        if precision == MIXED:
+            warm32 = solver.eps32.clone()
+        em_l.append(epsM)
+        sm_l.append(sigM)
+        if store:
+            eps = dev.from_sym_soa(dev.download(solver.eps), d)
+            sig = dev.from_sym_soa(dev.download(solver.sig), d)
+            if np.any(a_bar):
+                eps += a_bar
+                sig += 2.0 * solver.mu_host[..., None, None] * a_bar
+            e_l.append(eps)
+            s_l.append(sig)
+    if info is not None:
+        info["ls_iters"] = its
+    return np.array(em_l), np.array(sm_l), e_l, s_l
+
+
+def linear_elastic_simulation_2d(E, nu, target_strain_mask, target_values, n_steps=20, pixel=1.0, tol_ls=1e-6,
+                                 max_iter_ls=200, tol_macro=1e-4, max_iter_macro=20, store=True,
+                                 plane_mode="plane_strain", log_path=None, global_log_path=None,
+                                 driving_component=(0, 0), enable_console=True, vtk_path=None, vtk_interval="none",
+                                 **solver_kw):
+    """2D mixed-BC load stepping (mgkmc/linear_elastic_simulator.py:90-272) on
+    the GPU.  Returns (eps_macro (n+1,2,2), sig_macro (n+1,2,2), eps list, sig
+    list).  Log, VTK and checkpoint options are accepted and ignored;
+    ``precision`` / ``info`` (extensions) select the LS form / receive the LS
+    iteration counts."""
+    return _simulation(2, E, nu, target_strain_mask, target_values, n_steps, pixel, tol_ls, max_iter_ls, tol_macro,
+                       max_iter_macro, store, plane_mode, solver_kw.get("precision", FP64), solver_kw.get("info"))
+
+
+def linear_elastic_simulation_3d(E, nu, target_strain_mask, target_values, n_steps=20, pixel=1.0, tol_ls=1e-6,
+                                 max_iter_ls=200, tol_macro=1e-4, max_iter_macro=20, store=True, log_path=None,
+                                 global_log_path=None, driving_component=(0, 0), enable_console=True, vtk_path=None,
+                                 vtk_interval="none", **solver_kw):
+    """3D mixed-BC load stepping (mgkmc/linear_elastic_simulator.py:378-566) on
+    the GPU; as linear_elastic_simulation_2d."""
+    return _simulation(3, E, nu, target_strain_mask, target_values, n_steps, pixel, tol_ls, max_iter_ls, tol_macro,
+                       max_iter_macro, store, "plane_strain", solver_kw.get("precision", FP64),
+                       solver_kw.get("info"))

@@ -236,3 +236,54 @@ def equilibrium_residual(sig, pixel=1.0):
     num = np.sum(np.abs(div) ** 2 / k2[..., None])
     den = np.sum(np.abs(sh) ** 2)
     return float(np.sqrt(num) / (np.sqrt(den) + 1e-300))
+
+
+def linear_elastic_simulation(d, E, nu, target_strain_mask, target_values, n_steps=20, pixel=1.0, tol_ls=1e-6,
+                              max_iter_ls=200, tol_macro=1e-4, max_iter_macro=20, store=True,
+                              plane_mode="plane_strain"):
+    """Mixed stress/strain load stepping (linear_elastic_simulator.py:90-272 in
+    2D, :378-566 in 3D): targets ramp linearly over n_steps + 1 steps; per step
+    a macro loop sets the strain-controlled entries, solves warm-started from
+    the previous step's field (eps_init), and corrects the stress-controlled
+    entries by (err - nu_avg tr(err) I) / E_avg until max |err| < tol_macro.
+    Returns (eps_macro (n+1,d,d), sig_macro (n+1,d,d), eps list, sig list, LS
+    iteration counts per solve)."""
+    E = np.asarray(E, float)
+    nu = np.asarray(nu, float)
+    shape = E.shape
+    if d == 3:
+        lam0, mu0 = compute_lame_3d(E.mean(), nu.mean())
+    else:
+        lam0, mu0 = compute_lame_2d(E.mean(), nu.mean(), plane_mode)
+    G = green_operator(wave_vectors(shape, pixel), lam0, mu0)
+    E_avg, nu_avg = E.mean(), nu.mean()
+    mask = np.asarray(target_strain_mask, bool)
+    smask = ~mask
+    tv = np.asarray(target_values, float)
+    em_l, sm_l, e_l, s_l, its = [], [], [], [], []
+    eb = np.zeros((d, d))
+    warm = None
+    solve = spectral_solver_3d if d == 3 else (
+        lambda *a, **k: spectral_solver_2d(*a, plane_mode=plane_mode, **k))
+    for s in range(n_steps + 1):
+        tgt = tv * (s / n_steps)
+        for _ in range(max_iter_macro):
+            eb[mask] = tgt[mask]
+            info = {}
+            eps, sig, epsM, sigM = solve(E, nu, eb, max_iter=max_iter_ls, tol=tol_ls, pixel=pixel, Gamma=G,
+                                         eps_init=warm, info=info)
+            its.append(info["iters"])
+            err = np.zeros((d, d))
+            err[smask] = tgt[smask] - sigM[smask]
+            mx = np.max(np.abs(err[smask])) if np.any(smask) else 0.0
+            if mx < tol_macro:
+                break
+            de = (err - nu_avg * np.trace(err) * np.eye(d)) / E_avg
+            eb[smask] += de[smask]
+        warm = eps
+        em_l.append(epsM)
+        sm_l.append(sigM)
+        if store:
+            e_l.append(eps)
+            s_l.append(sig)
+    return np.array(em_l), np.array(sm_l), e_l, s_l, its

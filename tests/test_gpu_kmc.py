@@ -231,3 +231,38 @@ def test_kmc_refined_ls_event_sequence_matches_fp64(gpu):
     assert [e[:4] for e in out["fp64"][0]] == [e[:4] for e in out["mixed"][0]]
     s64, smx = out["fp64"][1], out["mixed"][1]
     assert np.abs(smx - s64).max() <= 1e-5 * np.abs(s64).max()
+
+
+def test_kmc2d_athermal_aqs_with_eigenstrain_matches_oracle(gpu):
+    # the 2D plain-method recipe at reduced size: T = 0 (only q <= 0 modes are
+    # events), initial zero-mean Gaussian eigenstrain, LS tol 1e-8, strain
+    # increments along xx with sig_yy = 0
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    from oracle.kmc2d_loop import run_events_2d
+    n, M = 16, 20
+    rng = np.random.default_rng(0)
+    mu = 26.923e9 * rng.uniform(0.9, 1.1, (n, n))
+    nu = np.full((n, n), 0.35)
+    E = 2 * mu * (1 + nu)
+    g = rng.standard_normal((3, n, n)) * 1e-3
+    g -= g.mean(axis=(1, 2), keepdims=True)
+    ep = np.zeros((n, n, 2, 2))
+    ep[..., 0, 0], ep[..., 1, 1], ep[..., 0, 1] = g[0], g[1], g[2]
+    ep[..., 1, 0] = g[2]
+    kw = dict(mean=0.55, std=0.15, min_cutoff=0.0)  # low barriers: ~60 athermal events in 8 increments
+    args = dict(softening_cap=0.916, jp=20.0, jt=0.0)
+    ev_ref, st_ref = run_events_2d(n, n, M, 0.14, E, nu, 0.7, 0.0, 1e7, 1e-3, (0, 0), 10 ** 9, seed=5, sel_seed=6,
+                                   barrier_kwargs=kw, stress_targets={(1, 1): 0.0}, mixed_tol=1e6, ls_tol=1e-8,
+                                   n_global_steps=8, eps_plastic=ep, **args)
+    sim = KmcSimulation2D(n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                          temperature=0.0, strain_rate=1e7, rng=np.random.RandomState(5),
+                          selection_stream=np.random.RandomState(6), **args)
+    sim.eps_plastic = ep
+    sim.solver_args = {"tol": 1e-8}
+    ev = sim.run_simulation(8, 1e-3, component=(0, 0), stress_targets={(1, 1): 0.0}, mixed_tol=1e6)
+    print(f"[kmc2d aqs] events={len(ev)} (oracle {len(ev_ref)})", flush=True)
+    assert len(ev_ref) > 0
+    assert [e[:3] for e in ev] == [e[:3] for e in ev_ref]
+    assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
+    assert np.abs(sim.eps_macro - st_ref["eps_macro"]).max() <= 1e-9 * np.abs(st_ref["eps_macro"]).max()
+    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-9 * np.abs(st_ref["sig"]).max()

@@ -193,3 +193,32 @@ def test_ls3d_full_size_properties(gpu):
     inf = s.solve(eb, ep, tol=1e-8)
     assert it1 == inf.iters and it1 > 2 and torch.equal(e1, s.eps)
     assert np.allclose(m1, eb.ravel(), atol=1e-14)
+
+
+@pytest.mark.parametrize("shape,plane_mode", [((16, 12), "plane_strain"), ((16, 16), "plane_stress"),
+                                              ((8, 6, 10), "plane_strain")])
+def test_mixed_bc_simulation_parity(gpu, shape, plane_mode):
+    # linear_elastic_simulation_2d/3d: macro lists, fields and per-solve LS
+    # iteration counts equal the oracle's (warm starts from the previous step)
+    from mgkmc_amd import linear_elastic_simulator as les
+    print(f"[mixed-bc sim] shape={shape} {plane_mode}", flush=True)
+    d = len(shape)
+    E, nu, _ = _case(shape, 9, with_ep=False)
+    mask = np.ones((d, d), bool)
+    mask[1, 1] = False
+    if d == 3:
+        mask[2, 2] = False
+    tv = np.zeros((d, d))
+    tv[0, 0] = 0.02
+    ref = ol.linear_elastic_simulation(d, E, nu, mask, tv, n_steps=4, tol_ls=1e-8, tol_macro=1e3,
+                                       plane_mode=plane_mode)
+    info = {}
+    if d == 3:
+        got = les.linear_elastic_simulation_3d(E, nu, mask, tv, n_steps=4, tol_ls=1e-8, tol_macro=1e3, info=info)
+    else:
+        got = les.linear_elastic_simulation_2d(E, nu, mask, tv, n_steps=4, tol_ls=1e-8, tol_macro=1e3,
+                                               plane_mode=plane_mode, info=info)
+    assert info["ls_iters"] == ref[4]
+    assert _rel(got[0], ref[0]) < 1e-10 and _rel(got[1], ref[1]) < 1e-10
+    for a, b in zip(got[2] + got[3], ref[2] + ref[3]):
+        assert _rel(a, b) < 1e-10

@@ -135,3 +135,35 @@ def test_iteration_trajectory_stops_like_reference():
     info2 = {}
     ol.spectral_solver_2d(E, nu, eb, tol=1e-30, max_iter=3, info=info2)
     assert info2["iters"] == 3
+
+
+@pytest.mark.parametrize("d,plane_mode", [(2, "plane_strain"), (2, "plane_stress"), (3, "plane_strain")])
+def test_mixed_bc_driver_homogeneous_closed_form(d, plane_mode):
+    # homogeneous medium, eps_xx driven, other normal stresses zero: the macro
+    # loop must reach uniaxial stress: sig_xx = E' eps_xx, eps_yy = -nu' eps_xx
+    E0, nu0 = 70e9, 0.3
+    shape = (8,) * d
+    E = np.full(shape, E0)
+    nu = np.full(shape, nu0)
+    mask = np.zeros((d, d), bool)
+    mask[0, 0] = True
+    for i in range(d):
+        for j in range(d):
+            if i != j:
+                mask[i, j] = True  # shear strains held at 0
+    tv = np.zeros((d, d))
+    tv[0, 0] = 0.01
+    em, sm, el, sl, its = ol.linear_elastic_simulation(d, E, nu, mask, tv, n_steps=2, tol_ls=1e-10, tol_macro=1.0,
+                                                      plane_mode=plane_mode)
+    if d == 3 or plane_mode == "plane_stress":
+        Ep, nup = E0, nu0
+    else:
+        Ep, nup = E0 / (1 - nu0 ** 2), nu0 / (1 - nu0)
+    exx = em[-1][0, 0]
+    assert abs(exx - 0.01) < 1e-15
+    assert abs(sm[-1][0, 0] - Ep * exx) < 1e-6 * Ep * exx
+    for k in range(1, d):
+        assert abs(em[-1][k, k] + nup * exx) < 1e-6 * exx
+        assert abs(sm[-1][k, k]) < 1.0
+    assert np.allclose(em[1], 0.5 * em[2], rtol=1e-6, atol=1e-14)
+    assert len(el) == 3 and el[-1].shape == shape + (d, d)
