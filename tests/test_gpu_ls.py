@@ -83,6 +83,28 @@ def test_ls_specialised_sizes_parity(gpu, shape):
         assert _rel(a, b) < 1e-10, _rel(a, b)
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 256), (64, 64, 75), (256, 256), (512, 128)])
+def test_ls_fp32_specialised_sizes(gpu, shape):
+    # FP32 sweeps (vectorised 16-byte tile loads/stores on full, even rows;
+    # per-element on partial or odd rows) agree with the FP64 oracle to FP32
+    # resolution
+    from mgkmc_amd.linear_elastic_simulator import LsSolver, FP32
+    from mgkmc_amd import device as dev
+    print(f"[ls ct fp32] shape={shape}", flush=True)
+    E, nu, ep = _case(shape, 12)
+    d = len(shape)
+    eb = np.zeros((d, d))
+    eb[0, 0] = 0.01
+    solve_o = ol.spectral_solver_3d if d == 3 else ol.spectral_solver_2d
+    ref = solve_o(E, nu, eb, eps_plastic=ep, tol=1e-10)
+    s = LsSolver(E, nu, precision=FP32)
+    sym, _ = dev.to_sym_soa(ep, d)
+    inf = s.solve(eb, dev.upload(sym, FP32), tol=1e-7, max_iter=100)
+    err = _rel(dev.from_sym_soa(dev.download(s.eps), d), ref[0])
+    print(f"[ls ct fp32] iters={inf.iters} err={err:.2e}", flush=True)
+    assert err < 1e-5
+
+
 def test_ls3d_warm_start_and_nonsymmetric_bar(gpu):
     from mgkmc_amd.linear_elastic_simulator import spectral_solver_3d
     shape = (8, 8, 8)
