@@ -48,6 +48,37 @@ template <typename T>
 __device__ __forceinline__ cplx<T> mul_pi(cplx<T> a) { return cmk(-a.y, a.x); }
 
 // ---------------------------------------------------------------------------
+// 16-byte vector access (one global_load/store_dwordx4 per lane): VW(T)
+// consecutive reals.  The address must be 16-byte aligned.
+// ---------------------------------------------------------------------------
+template <typename T>
+constexpr int vw16() { return 16 / (int)sizeof(T); }
+
+template <typename T>
+__device__ __forceinline__ void vload16(const T* __restrict__ p, T (&o)[16 / sizeof(T)]) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+  } else {
+    const double2 v = *reinterpret_cast<const double2*>(p);
+    o[0] = v.x;
+    o[1] = v.y;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void vstore16(T* __restrict__ p, const T (&a)[16 / sizeof(T)]) {
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4(a[0], a[1], a[2], a[3]);
+  } else {
+    *reinterpret_cast<double2*>(p) = make_double2(a[0], a[1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FFT plan: length n factored into radices 8,5,4,3,2 (applied in order).
 // ---------------------------------------------------------------------------
 struct FftPlan {
@@ -249,6 +280,16 @@ constexpr int ct_radix(int N, int L) {
   const int k = ct_log2(N / L);
   return (k % 3 == 0) ? 8 : ((k % 3 == 2) ? 4 : (k >= 4 ? 4 : 2));
 }
+
+// points per compile-time tile: 32 KB of LDS in FP32, 32 KB in FP64 (so four
+// tiles fit the 160 KB of a CU in either precision)
+template <typename T>
+constexpr int ct_tile() {
+  return sizeof(T) == 8 ? 2048 : 4096;
+}
+
+// pencils per pencil-pass tile: largest power of two pp with pp*C*Q <= tile
+constexpr int ct_pp(int tile, int C, int Q) { return (2 * C * Q <= tile) ? 2 * ct_pp(tile / 2, C, Q) : 1; }
 
 template <int N, int B, int C, int R, int L, typename T>
 __device__ __forceinline__ void ct_pass(cplx<T>* __restrict__ buf, const cplx<T>* __restrict__ tw, bool inv) {

@@ -25,6 +25,62 @@ __device__ __forceinline__ void xsplit_store(const cplx<T>* tile, int ldb, int n
   }
 }
 
+// Same, FP32 with a compile-time tile (nx = NN, B = BB columns) whose columns
+// are all inside the grid and whose rows start 16-byte aligned: two complex
+// values (16 bytes) per lane and access.
+template <int NN, int BB>
+__device__ __forceinline__ void xsplit_store_v(const cplx<float>* tile, int q0, int Q, int P, int p,
+                                               cplx<float>* __restrict__ sa, cplx<float>* __restrict__ sb) {
+  constexpr int H = BB / 2, NXH = NN / 2 + 1;
+  for (int e = threadIdx.x; e < NXH * H; e += MGK_NT) {
+    const int h = e % H, k = e / H;
+    const int kr = (k == 0) ? 0 : NN - k;
+    const float4 z1 = *reinterpret_cast<const float4*>(tile + k * BB + 2 * h);
+    const float4 z2 = *reinterpret_cast<const float4*>(tile + kr * BB + 2 * h);
+    const size_t idx = ((size_t)k * P + p) * Q + q0 + 2 * h;
+    *reinterpret_cast<float4*>(sa + idx) = make_float4(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y),
+                                                       0.5f * (z1.z + z2.z), 0.5f * (z1.w - z2.w));
+    if (sb)
+      *reinterpret_cast<float4*>(sb + idx) = make_float4(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x),
+                                                         0.5f * (z1.w + z2.w), -0.5f * (z1.z - z2.z));
+  }
+}
+
+template <int NN, int BB>
+__device__ __forceinline__ void xmerge_load_v(cplx<float>* tile, int q0, int Q, int P, int p,
+                                              const cplx<float>* __restrict__ sa,
+                                              const cplx<float>* __restrict__ sb) {
+  constexpr int H = BB / 2, NXH = NN / 2 + 1;
+  for (int e = threadIdx.x; e < NN * H; e += MGK_NT) {
+    const int h = e % H, k = e / H;
+    const bool lo = k < NXH;
+    const int kk = lo ? k : NN - k;
+    const size_t idx = ((size_t)kk * P + p) * Q + q0 + 2 * h;
+    const float4 A = *reinterpret_cast<const float4*>(sa + idx);
+    const float4 Bv = sb ? *reinterpret_cast<const float4*>(sb + idx) : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 z;
+    if (lo) z = make_float4(A.x - Bv.y, A.y + Bv.x, A.z - Bv.w, A.w + Bv.z);
+    else z = make_float4(A.x + Bv.y, -A.y + Bv.x, A.z + Bv.w, -A.w + Bv.z);
+    *reinterpret_cast<float4*>(tile + k * BB + 2 * h) = z;
+  }
+  __syncthreads();
+}
+
+// xsplit_store, through the vector form when the tile allows it (vec: every
+// column inside the grid, rows 16-byte aligned).
+template <typename T, int NN, int BB>
+__device__ __forceinline__ void store_half(const cplx<T>* tile, bool vec, int ldb, int nx, int nxh, int logB, int q0,
+                                           int Q, int P, int p, cplx<T>* __restrict__ sa,
+                                           cplx<T>* __restrict__ sb) {
+  if constexpr (sizeof(T) == 4 && NN > 0 && BB % 2 == 0) {
+    if (vec) {
+      xsplit_store_v<NN, BB>(tile, q0, Q, P, p, sa, sb);
+      return;
+    }
+  }
+  xsplit_store(tile, ldb, nx, nxh, logB, q0, Q, P, p, sa, sb);
+}
+
 // Build the full x-spectrum of z = a + i b from half spectra of a and b.
 template <typename T>
 __device__ __forceinline__ void xmerge_load(cplx<T>* tile, int ldb, int nx, int nxh, int logB, int q0, int Q, int P,

@@ -937,7 +937,7 @@ int fs_plan(const int64_t* shape, FsPlan* pl) {
   pl->lds_q = (size_t)(pl->Q + 3 * pl->Q * PP) * cb;
   if (pl->lds_x > lim || pl->lds_y > lim || pl->lds_q > lim) return MGK_ERR_ARG;
   {
-    const int CTT = 4096;
+    const int CTT = ct_tile<T>();
     auto ct_ok = [](int n) { return n == 64 || n == 128 || n == 256 || n == 512; };
     pl->ctx = pl->cty = pl->ctq = 0;
     if (ct_ok(pl->nx) && pl->Q >= CTT / pl->nx && (size_t)(pl->nx + CTT) * cb <= lim) {
@@ -994,10 +994,10 @@ void fs_launch_mid(const FsPlan& pl, hipStream_t s, const FftPlan& ply, const cp
   const dim3 gy(pl.nqy, pl.nxh, 9);
   if (pl.cty) {
     switch (pl.P) {
-      case 64: k_fft_mid<T, 16, 64, 64><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
-      case 128: k_fft_mid<T, 16, 128, 32><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
-      case 256: k_fft_mid<T, 16, 256, 16><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
-      default: k_fft_mid<T, 16, 512, 8><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+      case 64: k_fft_mid<T, ct_tile<T>() / MGK_NT, 64, ct_tile<T>() / 64><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+      case 128: k_fft_mid<T, ct_tile<T>() / MGK_NT, 128, ct_tile<T>() / 128><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+      case 256: k_fft_mid<T, ct_tile<T>() / MGK_NT, 256, ct_tile<T>() / 256><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+      default: k_fft_mid<T, ct_tile<T>() / MGK_NT, 512, ct_tile<T>() / 512><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
     }
   } else {
     k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip);
@@ -1008,16 +1008,17 @@ template <typename T>
 void fs_project(FsCtx<T>& c, const T* in, T* out, T scale, const T* dota, int selfdot, double* partials,
                 const int* skip) {
   constexpr int MV = fs_maxv<T>();
+  constexpr int TT = ct_tile<T>();  // points per compile-time tile
   const FsPlan& pl = c.pl;
   int h = c.prof->begin(KID_FS_XFWD, c.kry_it, (double)pl.NV * 9 * sizeof(T) + pl.spec);
   {
     const dim3 gx(pl.nqx, pl.P, 5);
     if (pl.ctx) {
       switch (pl.nx) {
-        case 64: k_fs_xr2c<T, 16, 64, 64><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
-        case 128: k_fs_xr2c<T, 16, 128, 32><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
-        case 256: k_fs_xr2c<T, 16, 256, 16><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
-        default: k_fs_xr2c<T, 16, 512, 8><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
+        case 64: k_fs_xr2c<T, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
+        case 128: k_fs_xr2c<T, TT / MGK_NT, 128, TT / 128><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
+        case 256: k_fs_xr2c<T, TT / MGK_NT, 256, TT / 256><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
+        default: k_fs_xr2c<T, TT / MGK_NT, 512, TT / 512><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
       }
     } else {
       k_fs_xr2c<T, MV><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip);
@@ -1034,10 +1035,10 @@ void fs_project(FsCtx<T>& c, const T* in, T* out, T scale, const T* dota, int se
     const dim3 gq(pl.npb, 3);
     if (pl.ctq) {
       switch (pl.Q) {
-        case 64: k_fs_proj<T, 16, 64, 16><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip); break;
-        case 128: k_fs_proj<T, 16, 128, 8><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip); break;
-        case 256: k_fs_proj<T, 16, 256, 4><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip); break;
-        default: k_fs_proj<T, 16, 512, 2><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip); break;
+        case 64: k_fs_proj<T, TT / MGK_NT, 64, ct_pp(TT, 3, 64)><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip); break;
+        case 128: k_fs_proj<T, TT / MGK_NT, 128, ct_pp(TT, 3, 128)><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip); break;
+        case 256: k_fs_proj<T, TT / MGK_NT, 256, ct_pp(TT, 3, 256)><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip); break;
+        default: k_fs_proj<T, TT / MGK_NT, 512, ct_pp(TT, 3, 512)><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip); break;
       }
     } else {
       k_fs_proj<T, MV><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, c.spec, scale, skip);
@@ -1055,10 +1056,10 @@ void fs_project(FsCtx<T>& c, const T* in, T* out, T scale, const T* dota, int se
     const dim3 gx(pl.nqx, pl.P, 5);
     if (pl.ctx) {
       switch (pl.nx) {
-        case 64: k_fs_xc2r<T, 16, 64, 64><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
-        case 128: k_fs_xc2r<T, 16, 128, 32><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
-        case 256: k_fs_xc2r<T, 16, 256, 16><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
-        default: k_fs_xc2r<T, 16, 512, 8><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
+        case 64: k_fs_xc2r<T, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
+        case 128: k_fs_xc2r<T, TT / MGK_NT, 128, TT / 128><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
+        case 256: k_fs_xc2r<T, TT / MGK_NT, 256, TT / 256><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
+        default: k_fs_xc2r<T, TT / MGK_NT, 512, TT / 512><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
       }
     } else {
       k_fs_xc2r<T, MV><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip);

@@ -122,7 +122,7 @@ __device__ __forceinline__ void gamma2_apply(const T q0, const T q1, const cplx<
 // sweep 1: polarisation + forward x FFT
 // ---------------------------------------------------------------------------
 template <typename T, int D, int MAXV, int NN = 0, int BB = 0>
-__global__ __launch_bounds__(MGK_NT) void k_ls_tau_xfwd(LsGeom<T> g, const T* __restrict__ eps,
+__global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_ls_tau_xfwd(LsGeom<T> g, const T* __restrict__ eps,
                                                         const T* __restrict__ epsp, const T* __restrict__ lam,
                                                         const T* __restrict__ mu, T lam0, T mu0,
                                                         cplx<T>* __restrict__ spec, const LsState* __restrict__ st) {
@@ -142,46 +142,98 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_tau_xfwd(LsGeom<T> g, const T* __
   const T m02 = T(2) * mu0;
 
   T keep[MAXV], mus[MAXV];
-  // pass 1: diagonal components, traces; (xx, yy) into the tile
+  // 16-byte vector path: compile-time tile, every column inside the grid and
+  // rows 16-byte aligned; each lane then handles VW consecutive voxels
+  constexpr int VW = vw16<T>();
+  constexpr bool VOK = CT && (BB % VW == 0) && (MAXV % VW == 0);
+  const bool vec = VOK && q0 + B <= Q && Q % VW == 0;
+  if (vec) {
+    constexpr int GPR = VOK ? BB / VW : 1;  // vector groups per tile row
+    constexpr int SV = MAXV / VW;
+    // pass 1: diagonal components, traces; (xx, yy) into the tile
 #pragma unroll
-  for (int s = 0; s < MAXV; ++s) {
-    keep[s] = T(0);
-    mus[s] = T(0);
-    const int e = threadIdx.x + s * MGK_NT;
-    if (e < nx * B) {
-      const int b = e & (B - 1), x = e >> logB, q = q0 + b;
-      T txx = T(0), tyy = T(0);
-      if (q < Q) {
-        const size_t v = ((size_t)x * P + p) * Q + q;
-        const T exx = eps[v], eyy = eps[NV + v];
-        const T pxx = epsp ? epsp[v] : T(0), pyy = epsp ? epsp[NV + v] : T(0);
-        const T l = lam[v], m = mu[v];
-        const T dxx = exx - pxx, dyy = eyy - pyy;
-        T tre, tr0, ezz = T(0), dzz = T(0);
+    for (int s = 0; s < SV; ++s) {
+      const int e = threadIdx.x + s * MGK_NT;
+      const int bg = e % GPR, x = e / GPR;
+      const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
+      T exx[VW], eyy[VW], ezz[VW], pxx[VW], pyy[VW], pzz[VW], l[VW], m[VW];
+      vload16(eps + v, exx);
+      vload16(eps + NV + v, eyy);
+      vload16(lam + v, l);
+      vload16(mu + v, m);
+      if (D == 3) vload16(eps + 2 * NV + v, ezz);
+      if (epsp) {
+        vload16(epsp + v, pxx);
+        vload16(epsp + NV + v, pyy);
+        if (D == 3) vload16(epsp + 2 * NV + v, pzz);
+      } else {
+#pragma unroll
+        for (int i = 0; i < VW; ++i) pxx[i] = pyy[i] = pzz[i] = T(0);
+      }
+#pragma unroll
+      for (int i = 0; i < VW; ++i) {
+        const T dxx = exx[i] - pxx[i], dyy = eyy[i] - pyy[i];
+        T tre, tr0, ez = T(0), dzz = T(0);
         if (D == 3) {
-          ezz = eps[2 * NV + v];
-          const T pzz = epsp ? epsp[2 * NV + v] : T(0);
-          dzz = ezz - pzz;
+          ez = ezz[i];
+          dzz = ez - pzz[i];
           tre = (dxx + dyy) + dzz;
-          tr0 = (exx + eyy) + ezz;
+          tr0 = (exx[i] + eyy[i]) + ez;
         } else {
           tre = dxx + dyy;
-          tr0 = exx + eyy;
+          tr0 = exx[i] + eyy[i];
         }
-        const T m2 = T(2) * m;
-        const T ltr = l * tre, l0tr = lam0 * tr0;
-        txx = (m2 * dxx + ltr) - (m02 * exx + l0tr);
-        tyy = (m2 * dyy + ltr) - (m02 * eyy + l0tr);
-        if (D == 3) keep[s] = (m2 * dzz + ltr) - (m02 * ezz + l0tr);
-        mus[s] = m;
+        const T m2 = T(2) * m[i];
+        const T ltr = l[i] * tre, l0tr = lam0 * tr0;
+        const T txx = (m2 * dxx + ltr) - (m02 * exx[i] + l0tr);
+        const T tyy = (m2 * dyy + ltr) - (m02 * eyy[i] + l0tr);
+        keep[s * VW + i] = (D == 3) ? (m2 * dzz + ltr) - (m02 * ez + l0tr) : T(0);
+        mus[s * VW + i] = m[i];
+        tile[x * ldb + bg * VW + i] = cmk(txx, tyy);
       }
-      tile[x * ldb + b] = cmk(txx, tyy);
+    }
+  } else {
+    // pass 1: diagonal components, traces; (xx, yy) into the tile
+#pragma unroll
+    for (int s = 0; s < MAXV; ++s) {
+      keep[s] = T(0);
+      mus[s] = T(0);
+      const int e = threadIdx.x + s * MGK_NT;
+      if (e < nx * B) {
+        const int b = e & (B - 1), x = e >> logB, q = q0 + b;
+        T txx = T(0), tyy = T(0);
+        if (q < Q) {
+          const size_t v = ((size_t)x * P + p) * Q + q;
+          const T exx = eps[v], eyy = eps[NV + v];
+          const T pxx = epsp ? epsp[v] : T(0), pyy = epsp ? epsp[NV + v] : T(0);
+          const T l = lam[v], m = mu[v];
+          const T dxx = exx - pxx, dyy = eyy - pyy;
+          T tre, tr0, ezz = T(0), dzz = T(0);
+          if (D == 3) {
+            ezz = eps[2 * NV + v];
+            const T pzz = epsp ? epsp[2 * NV + v] : T(0);
+            dzz = ezz - pzz;
+            tre = (dxx + dyy) + dzz;
+            tr0 = (exx + eyy) + ezz;
+          } else {
+            tre = dxx + dyy;
+            tr0 = exx + eyy;
+          }
+          const T m2 = T(2) * m;
+          const T ltr = l * tre, l0tr = lam0 * tr0;
+          txx = (m2 * dxx + ltr) - (m02 * exx + l0tr);
+          tyy = (m2 * dyy + ltr) - (m02 * eyy + l0tr);
+          if (D == 3) keep[s] = (m2 * dzz + ltr) - (m02 * ezz + l0tr);
+          mus[s] = m;
+        }
+        tile[x * ldb + b] = cmk(txx, tyy);
+      }
     }
   }
   __syncthreads();
   if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, false);
   else tile_fft<MAXV>(tile, tl, g.plx, tw, false);
-  xsplit_store(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec, spec + cs);
+  store_half<T, NN, BB>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec, spec + cs);
   __syncthreads();
 
   // remaining pairs: 3D (zz, yz), (xz, xy); 2D (xy, -)
@@ -190,36 +242,65 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_tau_xfwd(LsGeom<T> g, const T* __
   for (int pr = 0; pr < NPAIR; ++pr) {
     const int ca = (D == 3) ? (pr == 0 ? 2 : 4) : 2;
     const int cb = (D == 3) ? (pr == 0 ? 3 : 5) : -1;
+    if (vec) {
+      constexpr int GPR = VOK ? BB / VW : 1;
+      constexpr int SV = MAXV / VW;
 #pragma unroll
-    for (int s = 0; s < MAXV; ++s) {
-      const int e = threadIdx.x + s * MGK_NT;
-      if (e < nx * B) {
-        const int b = e & (B - 1), x = e >> logB, q = q0 + b;
-        T ta = T(0), tb = T(0);
-        if (q < Q) {
-          const size_t v = ((size_t)x * P + p) * Q + q;
-          const T m2 = T(2) * mus[s];
-          if (D == 3 && pr == 0) {
-            ta = keep[s];
-          } else {
-            const T ea = eps[(size_t)ca * NV + v];
-            const T pa = epsp ? epsp[(size_t)ca * NV + v] : T(0);
-            ta = m2 * (ea - pa) - m02 * ea;
-          }
-          if (cb >= 0) {
-            const T eb = eps[(size_t)cb * NV + v];
-            const T pb = epsp ? epsp[(size_t)cb * NV + v] : T(0);
-            tb = m2 * (eb - pb) - m02 * eb;
-          }
+      for (int s = 0; s < SV; ++s) {
+        const int e = threadIdx.x + s * MGK_NT;
+        const int bg = e % GPR, x = e / GPR;
+        const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
+        T ea[VW], pa[VW], eb[VW], pb[VW];
+        if (!(D == 3 && pr == 0)) {
+          vload16(eps + (size_t)ca * NV + v, ea);
+          if (epsp) vload16(epsp + (size_t)ca * NV + v, pa);
         }
-        tile[x * ldb + b] = cmk(ta, tb);
+        if (cb >= 0) {
+          vload16(eps + (size_t)cb * NV + v, eb);
+          if (epsp) vload16(epsp + (size_t)cb * NV + v, pb);
+        }
+#pragma unroll
+        for (int i = 0; i < VW; ++i) {
+          const T m2 = T(2) * mus[s * VW + i];
+          T ta, tb = T(0);
+          if (D == 3 && pr == 0) ta = keep[s * VW + i];
+          else ta = m2 * (ea[i] - (epsp ? pa[i] : T(0))) - m02 * ea[i];
+          if (cb >= 0) tb = m2 * (eb[i] - (epsp ? pb[i] : T(0))) - m02 * eb[i];
+          tile[x * ldb + bg * VW + i] = cmk(ta, tb);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < MAXV; ++s) {
+        const int e = threadIdx.x + s * MGK_NT;
+        if (e < nx * B) {
+          const int b = e & (B - 1), x = e >> logB, q = q0 + b;
+          T ta = T(0), tb = T(0);
+          if (q < Q) {
+            const size_t v = ((size_t)x * P + p) * Q + q;
+            const T m2 = T(2) * mus[s];
+            if (D == 3 && pr == 0) {
+              ta = keep[s];
+            } else {
+              const T ea = eps[(size_t)ca * NV + v];
+              const T pa = epsp ? epsp[(size_t)ca * NV + v] : T(0);
+              ta = m2 * (ea - pa) - m02 * ea;
+            }
+            if (cb >= 0) {
+              const T eb = eps[(size_t)cb * NV + v];
+              const T pb = epsp ? epsp[(size_t)cb * NV + v] : T(0);
+              tb = m2 * (eb - pb) - m02 * eb;
+            }
+          }
+          tile[x * ldb + b] = cmk(ta, tb);
+        }
       }
     }
     __syncthreads();
     if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, false);
-  else tile_fft<MAXV>(tile, tl, g.plx, tw, false);
-    xsplit_store(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
-                 cb >= 0 ? spec + (size_t)cb * cs : (cplx<T>*)nullptr);
+    else tile_fft<MAXV>(tile, tl, g.plx, tw, false);
+    store_half<T, NN, BB>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
+                          cb >= 0 ? spec + (size_t)cb * cs : (cplx<T>*)nullptr);
     __syncthreads();
   }
 }
@@ -243,14 +324,28 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
   const int npen = g.npencil;
   for (int t = threadIdx.x; t < Q; t += MGK_NT) tw[t] = g.twq[t];
   const int chunk = PP * Q;
-  for (int e = threadIdx.x; e < C * chunk; e += MGK_NT) {
-    const int c = e / chunk;
-    const int r = e - c * chunk;
-    const int pp = r / Q;
-    const int q = r - pp * Q;
-    const int pid = pid0 + pp;
-    tile[c * cstride + q * ldb + pp] =
-        (pid < npen) ? spec[((size_t)c * npen + pid) * Q + q] : cmk(T(0), T(0));
+  // FP32 with compile-time pencils all inside the grid: two complex values
+  // (16 bytes) per lane and global access
+  constexpr bool VOK = CT && sizeof(T) == 4;
+  const bool vec = VOK && pid0 + PP <= npen;
+  if (vec) {
+    constexpr int QH = VOK ? NN / 2 : 1;
+    for (int e = threadIdx.x; e < C * PP * QH; e += MGK_NT) {
+      const int h = e % QH, r = e / QH, pp = r % PP, c = r / PP;
+      const float4 v = *reinterpret_cast<const float4*>(spec + ((size_t)c * npen + pid0 + pp) * Q + 2 * h);
+      tile[c * cstride + (2 * h) * ldb + pp] = cmk(T(v.x), T(v.y));
+      tile[c * cstride + (2 * h + 1) * ldb + pp] = cmk(T(v.z), T(v.w));
+    }
+  } else {
+    for (int e = threadIdx.x; e < C * chunk; e += MGK_NT) {
+      const int c = e / chunk;
+      const int r = e - c * chunk;
+      const int pp = r / Q;
+      const int q = r - pp * Q;
+      const int pid = pid0 + pp;
+      tile[c * cstride + q * ldb + pp] =
+          (pid < npen) ? spec[((size_t)c * npen + pid) * Q + q] : cmk(T(0), T(0));
+    }
   }
   __syncthreads();
   const Tile tl{Q, logPP, ldb, C, cstride};
@@ -305,13 +400,24 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
   __syncthreads();
   if constexpr (CT) tile_fft_ct<NN, BB, C>(tile, tw, true);
   else tile_fft<MAXV>(tile, tl, g.plq, tw, true);
-  for (int e = threadIdx.x; e < C * chunk; e += MGK_NT) {
-    const int c = e / chunk;
-    const int r = e - c * chunk;
-    const int pp = r / Q;
-    const int q = r - pp * Q;
-    const int pid = pid0 + pp;
-    if (pid < npen) spec[((size_t)c * npen + pid) * Q + q] = tile[c * cstride + q * ldb + pp];
+  if (vec) {
+    constexpr int QH = VOK ? NN / 2 : 1;
+    for (int e = threadIdx.x; e < C * PP * QH; e += MGK_NT) {
+      const int h = e % QH, r = e / QH, pp = r % PP, c = r / PP;
+      const cplx<T> a = tile[c * cstride + (2 * h) * ldb + pp];
+      const cplx<T> b = tile[c * cstride + (2 * h + 1) * ldb + pp];
+      *reinterpret_cast<float4*>(spec + ((size_t)c * npen + pid0 + pp) * Q + 2 * h) =
+          make_float4(float(a.x), float(a.y), float(b.x), float(b.y));
+    }
+  } else {
+    for (int e = threadIdx.x; e < C * chunk; e += MGK_NT) {
+      const int c = e / chunk;
+      const int r = e - c * chunk;
+      const int pp = r / Q;
+      const int q = r - pp * Q;
+      const int pid = pid0 + pp;
+      if (pid < npen) spec[((size_t)c * npen + pid) * Q + q] = tile[c * cstride + q * ldb + pp];
+    }
   }
 }
 
@@ -319,7 +425,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
 // sweep 5: inverse x FFT (complex-to-real) + update + FP64 partial norms
 // ---------------------------------------------------------------------------
 template <typename T, int D, int MAXV, int NN = 0, int BB = 0>
-__global__ __launch_bounds__(MGK_NT) void k_ls_xinv_update(LsGeom<T> g, const cplx<T>* __restrict__ spec,
+__global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_ls_xinv_update(LsGeom<T> g, const cplx<T>* __restrict__ spec,
                                                            T* __restrict__ eps, Sym6 ebar,
                                                            double* __restrict__ partials,
                                                            const LsState* __restrict__ st) {
@@ -337,19 +443,61 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xinv_update(LsGeom<T> g, const cp
   for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
   const Tile tl{nx, logB, ldb, 1, 0};
   double acc[2] = {0.0, 0.0};
+  constexpr int VW = vw16<T>();
+  constexpr bool VOK = CT && (BB % VW == 0) && (MAXV % VW == 0);
+  const bool vec = VOK && q0 + B <= Q && Q % VW == 0;
   constexpr int NPAIR = (D == 3) ? 3 : 2;
 #pragma unroll
   for (int pr = 0; pr < NPAIR; ++pr) {
     const int ca = 2 * pr;
     const int cb = (D == 3 || pr == 0) ? 2 * pr + 1 : -1;
-    xmerge_load(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
-                cb >= 0 ? spec + (size_t)cb * cs : (const cplx<T>*)nullptr);
+    const cplx<T>* sa = spec + (size_t)ca * cs;
+    const cplx<T>* sb = cb >= 0 ? spec + (size_t)cb * cs : (const cplx<T>*)nullptr;
+    bool merged = false;
+    if constexpr (sizeof(T) == 4 && CT && BB % 2 == 0) {
+      if (vec) {
+        xmerge_load_v<NN, BB>(tile, q0, Q, P, p, sa, sb);
+        merged = true;
+      }
+    }
+    if (!merged) xmerge_load(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, sa, sb);
     if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, true);
     else tile_fft<MAXV>(tile, tl, g.plx, tw, true);
     const double wa = (D == 3) ? (ca >= 3 ? 2.0 : 1.0) : (ca >= 2 ? 2.0 : 1.0);
     const double wb = (D == 3) ? (cb >= 3 ? 2.0 : 1.0) : (cb >= 2 ? 2.0 : 1.0);
     const T eba = T(ebar.v[ca]);
     const T ebb = cb >= 0 ? T(ebar.v[cb]) : T(0);
+    if (vec) {
+      constexpr int GPR = VOK ? BB / VW : 1;
+      constexpr int SV = MAXV / VW;
+#pragma unroll
+      for (int s = 0; s < SV; ++s) {
+        const int e = threadIdx.x + s * MGK_NT;
+        const int bg = e % GPR, x = e / GPR;
+        const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
+        T oa[VW], na[VW], ob[VW], nb[VW];
+        vload16(eps + (size_t)ca * NV + v, oa);
+        if (cb >= 0) vload16(eps + (size_t)cb * NV + v, ob);
+#pragma unroll
+        for (int i = 0; i < VW; ++i) {
+          const cplx<T> z = tile[x * ldb + bg * VW + i];
+          na[i] = eba + z.x;
+          const double dd = (double)na[i] - (double)oa[i];
+          acc[0] += wa * dd * dd;
+          acc[1] += wa * (double)oa[i] * (double)oa[i];
+          if (cb >= 0) {
+            nb[i] = ebb + z.y;
+            const double db = (double)nb[i] - (double)ob[i];
+            acc[0] += wb * db * db;
+            acc[1] += wb * (double)ob[i] * (double)ob[i];
+          }
+        }
+        vstore16(eps + (size_t)ca * NV + v, na);
+        if (cb >= 0) vstore16(eps + (size_t)cb * NV + v, nb);
+      }
+      __syncthreads();
+      continue;
+    }
 #pragma unroll
     for (int s = 0; s < MAXV; ++s) {
       const int e = threadIdx.x + s * MGK_NT;
@@ -582,7 +730,7 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
   pl->lds_q = (size_t)(pl->Q + pl->C * pl->Q * PP) * cb;
   if (pl->lds_q > lim) return MGK_ERR_ARG;
   // compile-time specialised sweeps (power-of-two lengths 64..512, 4096-point tiles)
-  const int CTT = 4096;
+  const int CTT = ct_tile<T>();
   auto ct_ok = [](int n) { return n == 64 || n == 128 || n == 256 || n == 512; };
   pl->ctx = pl->cty = pl->ctq = 0;
   if (ct_ok(pl->nx) && pl->Q >= CTT / pl->nx && (size_t)(pl->nx + CTT) * cb <= lim) {
@@ -621,10 +769,10 @@ void launch_mid(const LsPlan& pl, dim3 gy, hipStream_t stream, const FftPlan& pl
                 int inv, const int* done) {
   if (pl.cty) {
     switch (pl.P) {
-      case 64: k_fft_mid<T, 16, 64, 64><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
-      case 128: k_fft_mid<T, 16, 128, 32><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
-      case 256: k_fft_mid<T, 16, 256, 16><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
-      default: k_fft_mid<T, 16, 512, 8><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+      case 64: k_fft_mid<T, ct_tile<T>() / MGK_NT, 64, ct_tile<T>() / 64><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+      case 128: k_fft_mid<T, ct_tile<T>() / MGK_NT, 128, ct_tile<T>() / 128><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+      case 256: k_fft_mid<T, ct_tile<T>() / MGK_NT, 256, ct_tile<T>() / 256><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+      default: k_fft_mid<T, ct_tile<T>() / MGK_NT, 512, ct_tile<T>() / 512><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
     }
   } else {
     k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done);
@@ -649,6 +797,7 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
                const T* epsp, T* eps, T* sig, int warm, int max_iter, double tol, char* ws, hipStream_t stream,
                mgk_ls_info* info, bool final_stress = true) {
   constexpr int MV = maxv<T>();
+  constexpr int TT = ct_tile<T>();  // points per compile-time tile
   constexpr int C = (D == 3) ? 6 : 3;
   const int prec = sizeof(T) == 8 ? 0 : 1;
   LsGeom<T> g;
@@ -732,10 +881,10 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     int h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
     if (pl.ctx) {
       switch (pl.nx) {
-        case 64: k_ls_tau_xfwd<T, D, 16, 64, 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-        case 128: k_ls_tau_xfwd<T, D, 16, 128, 32><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-        case 256: k_ls_tau_xfwd<T, D, 16, 256, 16><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-        default: k_ls_tau_xfwd<T, D, 16, 512, 8><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        case 64: k_ls_tau_xfwd<T, D, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        case 128: k_ls_tau_xfwd<T, D, TT / MGK_NT, 128, TT / 128><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        case 256: k_ls_tau_xfwd<T, D, TT / MGK_NT, 256, TT / 256><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        default: k_ls_tau_xfwd<T, D, TT / MGK_NT, 512, TT / 512><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
       }
     } else {
       k_ls_tau_xfwd<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st);
@@ -749,10 +898,10 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     h = prof.begin(KID_LS_PENCIL, iter_idx, b_fft);
     if (pl.ctq) {
       switch (pl.Q) {
-        case 64: k_ls_pencil<T, D, 16, 64, (D == 3 ? 8 : 16)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
-        case 128: k_ls_pencil<T, D, 16, 128, (D == 3 ? 4 : 8)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
-        case 256: k_ls_pencil<T, D, 16, 256, (D == 3 ? 2 : 4)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
-        default: k_ls_pencil<T, D, 16, 512, (D == 3 ? 1 : 2)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
+        case 64: k_ls_pencil<T, D, TT / MGK_NT, 64, ct_pp(TT, C, 64)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
+        case 128: k_ls_pencil<T, D, TT / MGK_NT, 128, ct_pp(TT, C, 128)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
+        case 256: k_ls_pencil<T, D, TT / MGK_NT, 256, ct_pp(TT, C, 256)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
+        default: k_ls_pencil<T, D, TT / MGK_NT, 512, ct_pp(TT, C, 512)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
       }
     } else {
       k_ls_pencil<T, D, MV><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st);
@@ -766,10 +915,10 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
     if (pl.ctx) {
       switch (pl.nx) {
-        case 64: k_ls_xinv_update<T, D, 16, 64, 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-        case 128: k_ls_xinv_update<T, D, 16, 128, 32><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-        case 256: k_ls_xinv_update<T, D, 16, 256, 16><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-        default: k_ls_xinv_update<T, D, 16, 512, 8><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        case 64: k_ls_xinv_update<T, D, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        case 128: k_ls_xinv_update<T, D, TT / MGK_NT, 128, TT / 128><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        case 256: k_ls_xinv_update<T, D, TT / MGK_NT, 256, TT / 256><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        default: k_ls_xinv_update<T, D, TT / MGK_NT, 512, TT / 512><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
       }
     } else {
       k_ls_xinv_update<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st);
