@@ -291,7 +291,17 @@ constexpr int ct_tile() {
 // pencils per pencil-pass tile: largest power of two pp with pp*C*Q <= tile
 constexpr int ct_pp(int tile, int C, int Q) { return (2 * C * Q <= tile) ? 2 * ct_pp(tile / 2, C, Q) : 1; }
 
-template <int N, int B, int C, int R, int L, typename T>
+// Optional LDS padding of the compile-time tiles: one spare complex slot per
+// 16 (index i lives at i + i/16).  Stockham passes write with stride R*B; for
+// narrow tiles (the pencil pass has B = 1 or 2 columns) that stride maps most
+// lanes of a wave onto the same few LDS banks, and the padding spreads them.
+template <bool PAD>
+__device__ __forceinline__ int tpad(int i) {
+  return PAD ? i + (i >> 4) : i;
+}
+constexpr int tpad_size(int n) { return n + n / 16 + 1; }
+
+template <int N, int B, int C, int R, int L, bool PAD, typename T>
 __device__ __forceinline__ void ct_pass(cplx<T>* __restrict__ buf, const cplx<T>* __restrict__ tw, bool inv) {
   constexpr int M = N / R;
   constexpr int NB = C * M * B;
@@ -303,9 +313,9 @@ __device__ __forceinline__ void ct_pass(cplx<T>* __restrict__ buf, const cplx<T>
     const int t = threadIdx.x + s * MGK_NT;
     if (NB % MGK_NT == 0 || t < NB) {
       const int b = t % B, u = t / B, j = u % M, c = u / M;
-      const cplx<T>* base = buf + c * (N * B) + b;
+      const int base = c * (N * B) + b;
 #pragma unroll
-      for (int q = 0; q < R; ++q) v[s][q] = base[(j + q * M) * B];
+      for (int q = 0; q < R; ++q) v[s][q] = buf[tpad<PAD>(base + (j + q * M) * B)];
     }
   }
   __syncthreads();
@@ -324,27 +334,28 @@ __device__ __forceinline__ void ct_pass(cplx<T>* __restrict__ buf, const cplx<T>
         }
       }
       dftR<R>(v[s], inv);
-      cplx<T>* base = buf + c * (N * B) + b;
+      const int base = c * (N * B) + b;
       const int o = (j - k) * R + k;
 #pragma unroll
-      for (int q = 0; q < R; ++q) base[(o + q * L) * B] = v[s][q];
+      for (int q = 0; q < R; ++q) buf[tpad<PAD>(base + (o + q * L) * B)] = v[s][q];
     }
   }
   __syncthreads();
 }
 
-template <int N, int B, int C, int L, typename T>
+template <int N, int B, int C, int L, bool PAD, typename T>
 __device__ __forceinline__ void ct_fft_rec(cplx<T>* __restrict__ buf, const cplx<T>* __restrict__ tw, bool inv) {
   if constexpr (L < N) {
     constexpr int R = ct_radix(N, L);
-    ct_pass<N, B, C, R, L>(buf, tw, inv);
-    ct_fft_rec<N, B, C, L * R>(buf, tw, inv);
+    ct_pass<N, B, C, R, L, PAD>(buf, tw, inv);
+    ct_fft_rec<N, B, C, L * R, PAD>(buf, tw, inv);
   }
 }
 
-template <int N, int B, int C, typename T>
+// element (c, j, b) at buf[tpad<PAD>(c*N*B + j*B + b)]
+template <int N, int B, int C, bool PAD = false, typename T>
 __device__ __forceinline__ void tile_fft_ct(cplx<T>* __restrict__ buf, const cplx<T>* __restrict__ tw, bool inv) {
-  ct_fft_rec<N, B, C, 1>(buf, tw, inv);
+  ct_fft_rec<N, B, C, 1, PAD>(buf, tw, inv);
 }
 
 // ---------------------------------------------------------------------------

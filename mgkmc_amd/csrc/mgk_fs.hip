@@ -539,18 +539,20 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_proj(FsGeom<T> g, cplx<T>* __rest
   const int npen = g.npencil;
   for (int t = threadIdx.x; t < Q; t += MGK_NT) tw[t] = g.twq[t];
   const int chunk = PP * Q;
+  // tile index (padded for the compile-time passes)
+  auto ti = [&](int c, int q, int pp) { return tpad<CT>(c * cstride + q * ldb + pp); };
   for (int e = threadIdx.x; e < 3 * chunk; e += MGK_NT) {
     const int c = e / chunk;
     const int r = e - c * chunk;
     const int pp = r / Q;
     const int q = r - pp * Q;
     const int pid = pid0 + pp;
-    tile[c * cstride + q * ldb + pp] =
+    tile[ti(c, q, pp)] =
         (pid < npen) ? spec[((size_t)(3 * row + c) * npen + pid) * Q + q] : cmk(T(0), T(0));
   }
   __syncthreads();
   const Tile tl{Q, logPP, ldb, 3, cstride};
-  if constexpr (CT) tile_fft_ct<NN, BB, 3>(tile, tw, false);
+  if constexpr (CT) tile_fft_ct<NN, BB, 3, true>(tile, tw, false);
   else tile_fft<MAXV>(tile, tl, g.plq, tw, false);
   for (int e = threadIdx.x; e < Q * PP; e += MGK_NT) {
     const int pp = e & (PP - 1), q = e >> logPP;
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_proj(FsGeom<T> g, cplx<T>* __rest
     const bool zero = (kx == 0 && p == 0 && q == 0) || fnyq_fs(kx, g.nx) || fnyq_fs(p, P) || fnyq_fs(q, Q);
     cplx<T> a[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) a[c] = tile[c * cstride + q * ldb + pp];
+    for (int c = 0; c < 3; ++c) a[c] = tile[ti(c, q, pp)];
     cplx<T> o[3];
     if (zero) {
 #pragma unroll
@@ -577,10 +579,10 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_proj(FsGeom<T> g, cplx<T>* __rest
       }
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) tile[c * cstride + q * ldb + pp] = o[c];
+    for (int c = 0; c < 3; ++c) tile[ti(c, q, pp)] = o[c];
   }
   __syncthreads();
-  if constexpr (CT) tile_fft_ct<NN, BB, 3>(tile, tw, true);
+  if constexpr (CT) tile_fft_ct<NN, BB, 3, true>(tile, tw, true);
   else tile_fft<MAXV>(tile, tl, g.plq, tw, true);
   for (int e = threadIdx.x; e < 3 * chunk; e += MGK_NT) {
     const int c = e / chunk;
@@ -588,7 +590,7 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_proj(FsGeom<T> g, cplx<T>* __rest
     const int pp = r / Q;
     const int q = r - pp * Q;
     const int pid = pid0 + pp;
-    if (pid < npen) spec[((size_t)(3 * row + c) * npen + pid) * Q + q] = tile[c * cstride + q * ldb + pp];
+    if (pid < npen) spec[((size_t)(3 * row + c) * npen + pid) * Q + q] = tile[ti(c, q, pp)];
   }
 }
 
@@ -953,10 +955,10 @@ int fs_plan(const int64_t* shape, FsPlan* pl) {
     if (ct_ok(pl->Q)) {
       int pp = 1;
       while (2 * pp * 3 * pl->Q <= CTT) pp *= 2;
-      if (npen >= pp && (size_t)(pl->Q + 3 * pl->Q * pp) * cb <= lim) {
+      if (npen >= pp && (size_t)(pl->Q + tpad_size(3 * pl->Q * pp)) * cb <= lim) {
         pl->ctq = 1;
         pl->logPP = mgk_ilog2(pp);
-        pl->lds_q = (size_t)(pl->Q + 3 * pl->Q * pp) * cb;
+        pl->lds_q = (size_t)(pl->Q + tpad_size(3 * pl->Q * pp)) * cb;
       }
     }
   }

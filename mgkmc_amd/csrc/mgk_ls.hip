@@ -324,6 +324,8 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
   const int npen = g.npencil;
   for (int t = threadIdx.x; t < Q; t += MGK_NT) tw[t] = g.twq[t];
   const int chunk = PP * Q;
+  // tile index (padded for the compile-time passes)
+  auto ti = [&](int c, int q, int pp) { return tpad<CT>(c * cstride + q * ldb + pp); };
   // FP32 with compile-time pencils all inside the grid: two complex values
   // (16 bytes) per lane and global access
   constexpr bool VOK = CT && sizeof(T) == 4;
@@ -333,8 +335,8 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
     for (int e = threadIdx.x; e < C * PP * QH; e += MGK_NT) {
       const int h = e % QH, r = e / QH, pp = r % PP, c = r / PP;
       const float4 v = *reinterpret_cast<const float4*>(spec + ((size_t)c * npen + pid0 + pp) * Q + 2 * h);
-      tile[c * cstride + (2 * h) * ldb + pp] = cmk(T(v.x), T(v.y));
-      tile[c * cstride + (2 * h + 1) * ldb + pp] = cmk(T(v.z), T(v.w));
+      tile[ti(c, 2 * h, pp)] = cmk(T(v.x), T(v.y));
+      tile[ti(c, 2 * h + 1, pp)] = cmk(T(v.z), T(v.w));
     }
   } else {
     for (int e = threadIdx.x; e < C * chunk; e += MGK_NT) {
@@ -343,13 +345,13 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
       const int pp = r / Q;
       const int q = r - pp * Q;
       const int pid = pid0 + pp;
-      tile[c * cstride + q * ldb + pp] =
+      tile[ti(c, q, pp)] =
           (pid < npen) ? spec[((size_t)c * npen + pid) * Q + q] : cmk(T(0), T(0));
     }
   }
   __syncthreads();
   const Tile tl{Q, logPP, ldb, C, cstride};
-  if constexpr (CT) tile_fft_ct<NN, BB, C>(tile, tw, false);
+  if constexpr (CT) tile_fft_ct<NN, BB, C, true>(tile, tw, false);
   else tile_fft<MAXV>(tile, tl, g.plq, tw, false);
 
   for (int e = threadIdx.x; e < Q * PP; e += MGK_NT) {
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
     const int kx = pid / P, p = pid - kx * P;
     cplx<T> t[C], o[C];
 #pragma unroll
-    for (int c = 0; c < C; ++c) t[c] = tile[c * cstride + q * ldb + pp];
+    for (int c = 0; c < C; ++c) t[c] = tile[ti(c, q, pp)];
     const bool nyx = fnyq(kx, g.nx);
     if (D == 3) {
       const bool nyy = fnyq(p, P), nyz = fnyq(q, Q);
@@ -395,17 +397,17 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
       }
     }
 #pragma unroll
-    for (int c = 0; c < C; ++c) tile[c * cstride + q * ldb + pp] = o[c];
+    for (int c = 0; c < C; ++c) tile[ti(c, q, pp)] = o[c];
   }
   __syncthreads();
-  if constexpr (CT) tile_fft_ct<NN, BB, C>(tile, tw, true);
+  if constexpr (CT) tile_fft_ct<NN, BB, C, true>(tile, tw, true);
   else tile_fft<MAXV>(tile, tl, g.plq, tw, true);
   if (vec) {
     constexpr int QH = VOK ? NN / 2 : 1;
     for (int e = threadIdx.x; e < C * PP * QH; e += MGK_NT) {
       const int h = e % QH, r = e / QH, pp = r % PP, c = r / PP;
-      const cplx<T> a = tile[c * cstride + (2 * h) * ldb + pp];
-      const cplx<T> b = tile[c * cstride + (2 * h + 1) * ldb + pp];
+      const cplx<T> a = tile[ti(c, 2 * h, pp)];
+      const cplx<T> b = tile[ti(c, 2 * h + 1, pp)];
       *reinterpret_cast<float4*>(spec + ((size_t)c * npen + pid0 + pp) * Q + 2 * h) =
           make_float4(float(a.x), float(a.y), float(b.x), float(b.y));
     }
@@ -416,7 +418,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
       const int pp = r / Q;
       const int q = r - pp * Q;
       const int pid = pid0 + pp;
-      if (pid < npen) spec[((size_t)c * npen + pid) * Q + q] = tile[c * cstride + q * ldb + pp];
+      if (pid < npen) spec[((size_t)c * npen + pid) * Q + q] = tile[ti(c, q, pp)];
     }
   }
 }
@@ -746,10 +748,10 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
   if (ct_ok(pl->Q)) {
     int pp = 1;
     while (2 * pp * pl->C * pl->Q <= CTT) pp *= 2;
-    if (npen >= pp && (size_t)(pl->Q + pl->C * pl->Q * pp) * cb <= lim) {
+    if (npen >= pp && (size_t)(pl->Q + tpad_size(pl->C * pl->Q * pp)) * cb <= lim) {
       pl->ctq = 1;
       pl->logPP = mgk_ilog2(pp);
-      pl->lds_q = (size_t)(pl->Q + pl->C * pl->Q * pp) * cb;
+      pl->lds_q = (size_t)(pl->Q + tpad_size(pl->C * pl->Q * pp)) * cb;
     }
   }
   const int nqt = (pl->Q + (1 << pl->logBx) - 1) >> pl->logBx;
