@@ -84,6 +84,27 @@ int mgk_ls_solve_refined(int ndim, const int64_t* shape, const void* lam32, cons
                          double tol32, void* ws32, size_t ws32_bytes, void* ws64, size_t ws64_bytes, void* stream,
                          mgk_ls_info* info, int* iters32);
 
+/* Landau small-strain model (3D).  Replaces spectral_solver_landau_3d, default
+ * "dbfft" path (mgkmc/linear_elastic_simulator.py:1231-1350) with the stress of
+ * stress_from_strain_landau_3d (mgkmc/elasticity.py:297-335, :811-861).  Constants in
+ * Pa as given; cap_limit is the resolved E_cap (the reference's default when its
+ * limit is None or <= 0: sqrt(0.2 <mu> / |g4|) if g4 < 0, else 999). */
+typedef struct {
+  double v1, v2, v3, g1, g2, g3, g4;
+  int capping;     /* strain capping on/off                       */
+  int cap_smooth;  /* 1 smooth (tanh), 0 piecewise                */
+  double cap_limit, cap_power, cap_ratio;
+} mgk_landau_params;
+
+/* lam, mu per voxel; lam0, mu0 = their means (the reference medium of this solver);
+ * tau_buf: 6*NV scratch values of the field precision; other arguments as mgk_ls_solve.
+ * Returns MGK_ERR_NONFINITE when an update is not finite (the reference raises
+ * FloatingPointError). */
+int mgk_ls_solve_landau(const int64_t* shape, int precision, const void* lam, const void* mu, double lam0,
+                        double mu0, const mgk_landau_params* params, const double* eps_bar, const void* eps_plastic,
+                        void* eps, void* sig, void* tau_buf, int warm, int max_iter, double tol, void* workspace,
+                        size_t workspace_bytes, void* stream, mgk_ls_info* info);
+
 /* ---------------------------------------------------------------------------
  * KMC: barriers, rates, selection, compaction, flips.
  * Device layouts: Q, Q0 [m][v]; catalog [k][m][v] with k over 2 planar

@@ -27,6 +27,12 @@ class MgkLsInfo(ctypes.Structure):
                 ("eps_mean", ctypes.c_double * 9), ("sig_mean", ctypes.c_double * 9)]
 
 
+class MgkLandauParams(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4")] + \
+               [("capping", ctypes.c_int), ("cap_smooth", ctypes.c_int), ("cap_limit", ctypes.c_double),
+                ("cap_power", ctypes.c_double), ("cap_ratio", ctypes.c_double)]
+
+
 class MgkKmcParams(ctypes.Structure):
     _fields_ = [("volume", ctypes.c_double), ("rate_volume", ctypes.c_double), ("nu0", ctypes.c_double),
                 ("temperature", ctypes.c_double), ("cap", ctypes.c_double), ("tau", ctypes.c_double),
@@ -72,6 +78,8 @@ SIGNATURES = {
     "mgk_ls_solve_refined": (_c_i, [_c_i, _c_i64p, _c_p, _c_p, _c_p, _c_p, _c_d, _c_d, _c_dp, _c_p, _c_p, _c_p,
                                     _c_p, _c_p, _c_i, _c_i, _c_d, _c_d, _c_p, _c_sz, _c_p, _c_sz, _c_p,
                                     ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i)]),
+    "mgk_ls_solve_landau": (_c_i, [_c_i64p, _c_i, _c_p, _c_p, _c_d, _c_d, ctypes.POINTER(MgkLandauParams), _c_dp, _c_p,
+                                   _c_p, _c_p, _c_p, _c_i, _c_i, _c_d, _c_p, _c_sz, _c_p, ctypes.POINTER(MgkLsInfo)]),
     "mgk_kmc_workspace_bytes": (_c_i, [ctypes.c_longlong, ctypes.POINTER(_c_sz)]),
     "mgk_kmc_barrier_rates": (_c_i, [_c_i, ctypes.c_longlong, _c_i, _c_i, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p,
                                      _c_p, _c_p, ctypes.POINTER(MgkKmcParams), _c_p, _c_p, _c_p, _c_dp]),

@@ -20,6 +20,7 @@
 #include "mgk_common.h"
 #include "mgk_host.h"
 #include "mgk_fftpass.h"
+#include "mgk_landau.h"
 #include <algorithm>
 
 struct LsState {
@@ -656,6 +657,115 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_stress(const T* __restrict__ eps,
     for (int c = 0; c < 2 * C; ++c) partials[(size_t)blockIdx.x * 2 * C + c] = acc[c];
 }
 
+// ---------------------------------------------------------------------------
+// Landau small-strain model (3D): polarisation tau = sigma_L(eps - eps_p) -
+// C0:eps into a 6-component field, r2c of a real 6-component field along x,
+// final Landau stress with FP64 sums of eps and sigma.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_ls_landau_tau(const T* __restrict__ eps, const T* __restrict__ epsp,
+                                                          const T* __restrict__ lam, const T* __restrict__ mu,
+                                                          T lam0, T mu0, LandauPar P, T* __restrict__ tau,
+                                                          size_t NV, const LsState* __restrict__ st) {
+  if (st->done) return;
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < NV; i += (size_t)gridDim.x * MGK_NT) {
+    T e[6], d[6], sg[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      e[c] = eps[(size_t)c * NV + i];
+      d[c] = e[c] - (epsp ? epsp[(size_t)c * NV + i] : T(0));
+    }
+    landau_sigma_sym(d, lam[i], mu[i], P, sg);
+    const T l0tr = lam0 * ((e[0] + e[1]) + e[2]);
+    const T m02 = T(2) * mu0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) tau[(size_t)c * NV + i] = sg[c] - ((c < 3 ? l0tr : T(0)) + m02 * e[c]);
+  }
+}
+
+template <typename T, int MAXV, int NN = 0, int BB = 0>
+__global__ __launch_bounds__(MGK_NT) void k_ls_xfwd_real(LsGeom<T> g, const T* __restrict__ f,
+                                                         cplx<T>* __restrict__ spec, const LsState* __restrict__ st) {
+  if (st->done) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* tile = tw + g.nx;
+  constexpr bool CT = NN > 0;
+  const int nx = CT ? NN : g.nx, P = g.P, Q = g.Q, logB = CT ? ct_log2(BB) : g.logBx, B = CT ? BB : 1 << logB;
+  const int ldb = B;
+  const int tid_lin = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int q0 = (tid_lin % gridDim.x) * B, p = tid_lin / gridDim.x;
+  const size_t NV = (size_t)g.NV;
+  const size_t cs = (size_t)g.nxh * P * Q;
+  for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
+  const Tile tl{nx, logB, ldb, 1, 0};
+  constexpr int VW = vw16<T>();
+  constexpr bool VOK = CT && (BB % VW == 0);
+  const bool vec = VOK && q0 + B <= Q && Q % VW == 0;
+#pragma unroll 1
+  for (int pr = 0; pr < 3; ++pr) {
+    const T* fa = f + (size_t)(2 * pr) * NV;
+    const T* fb = f + (size_t)(2 * pr + 1) * NV;
+    if (vec) {
+      constexpr int GPR = VOK ? BB / VW : 1;
+      for (int e = threadIdx.x; e < nx * GPR; e += MGK_NT) {
+        const int bg = e % GPR, x = e / GPR;
+        const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
+        T a[VW], b[VW];
+        vload16(fa + v, a);
+        vload16(fb + v, b);
+#pragma unroll
+        for (int i = 0; i < VW; ++i) tile[x * ldb + bg * VW + i] = cmk(a[i], b[i]);
+      }
+    } else {
+      for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
+        const int b = e & (B - 1), x = e >> logB, q = q0 + b;
+        cplx<T> z = cmk(T(0), T(0));
+        if (q < Q) {
+          const size_t v = ((size_t)x * P + p) * Q + q;
+          z = cmk(fa[v], fb[v]);
+        }
+        tile[x * ldb + b] = z;
+      }
+    }
+    __syncthreads();
+    if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, false);
+    else tile_fft<MAXV>(tile, tl, g.plx, tw, false);
+    store_half<T, NN, BB>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)(2 * pr) * cs,
+                          spec + (size_t)(2 * pr + 1) * cs);
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_ls_landau_stress(const T* __restrict__ eps, const T* __restrict__ epsp,
+                                                             const T* __restrict__ lam, const T* __restrict__ mu,
+                                                             LandauPar P, T* __restrict__ sig, size_t NV,
+                                                             double* __restrict__ partials) {
+  __shared__ double red[12 * (MGK_NT / 64)];
+  double acc[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) acc[c] = 0.0;
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < NV; i += (size_t)gridDim.x * MGK_NT) {
+    T e[6], d[6], sg[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      e[c] = eps[(size_t)c * NV + i];
+      d[c] = e[c] - (epsp ? epsp[(size_t)c * NV + i] : T(0));
+    }
+    landau_sigma_sym(d, lam[i], mu[i], P, sg);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      if (sig) sig[(size_t)c * NV + i] = sg[c];
+      acc[c] += (double)e[c];
+      acc[6 + c] += (double)sg[c];
+    }
+  }
+  block_sum<12>(acc, red);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 12; ++c) partials[(size_t)blockIdx.x * 12 + c] = acc[c];
+}
+
 __global__ void k_widen(const float* __restrict__ a, double* __restrict__ b, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     b[i] = (double)a[i];
@@ -672,6 +782,11 @@ typedef struct {
   double eps_mean[9];  // full d x d row-major (symmetric part + constant antisymmetric part)
   double sig_mean[9];
 } mgk_ls_info;
+typedef struct {
+  double v1, v2, v3, g1, g2, g3, g4;
+  int capping, cap_smooth;
+  double cap_limit, cap_power, cap_ratio;
+} mgk_landau_params;
 }
 
 namespace {
@@ -797,7 +912,7 @@ int ls_iter_hint(const void* ws, int iters) {
 template <typename T, int D>
 int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double mu0, const double* eps_bar,
                const T* epsp, T* eps, T* sig, int warm, int max_iter, double tol, char* ws, hipStream_t stream,
-               mgk_ls_info* info, bool final_stress = true) {
+               mgk_ls_info* info, bool final_stress = true, const LandauPar* lp = nullptr, T* taubuf = nullptr) {
   constexpr int MV = maxv<T>();
   constexpr int TT = ct_tile<T>();  // points per compile-time tile
   constexpr int C = (D == 3) ? 6 : 3;
@@ -879,8 +994,24 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   const double b_upd = specB + 2.0 * C * fieldB;
   MgkProfRec prof(stream);
   int iter_idx = 0;
+  const int nel = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 8192);
   auto enqueue_iter = [&]() {
-    int h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
+    int h = -1;
+    if (lp) {
+      // Landau: elementwise polarisation, then r2c of the 6-component field
+      k_ls_landau_tau<T><<<nel, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, T(lam0), T(mu0), *lp, taubuf, NV, st);
+      if (pl.ctx) {
+        switch (pl.nx) {
+          case 64: k_ls_xfwd_real<T, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
+          case 128: k_ls_xfwd_real<T, TT / MGK_NT, 128, TT / 128><<<gx, MGK_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
+          case 256: k_ls_xfwd_real<T, TT / MGK_NT, 256, TT / 256><<<gx, MGK_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
+          default: k_ls_xfwd_real<T, TT / MGK_NT, 512, TT / 512><<<gx, MGK_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
+        }
+      } else {
+        k_ls_xfwd_real<T, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, taubuf, spec, st);
+      }
+    } else {
+    h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
     if (pl.ctx) {
       switch (pl.nx) {
         case 64: k_ls_tau_xfwd<T, D, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
@@ -892,6 +1023,7 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
       k_ls_tau_xfwd<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st);
     }
     prof.end(h);
+    }
     if (D == 3 && pl.P > 1) {
       h = prof.begin(KID_LS_MID, iter_idx, b_fft);
       launch_mid<T, MV>(pl, gy, stream, g.ply, g.twy, spec, 0, &st->done);
@@ -963,7 +1095,11 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     first = false;
   }
   // final stress and macro means
-  if (final_stress) {
+  if (final_stress && lp) {
+    k_ls_landau_stress<T><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, *lp, sig, NV, part);
+    k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, 2 * C, (double)NV, st->mean);
+    MGK_HIP(hipGetLastError());
+  } else if (final_stress) {
     k_ls_stress<T, D><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, sig, NV, part);
     k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, 2 * C, (double)NV, st->mean);
     MGK_HIP(hipGetLastError());
@@ -1105,4 +1241,47 @@ extern "C" int mgk_ls_solve_refined(int ndim, const int64_t* shape, const void* 
   return ls_solve_t<double, 2>(p64, (const double*)lam64, (const double*)mu64, lam0, mu0, eps_bar,
                                (const double*)epsp64, (double*)eps64, (double*)sig64, 2, max_iter, tol, (char*)ws64, s,
                                info);
+}
+
+extern "C" int mgk_ls_solve_landau(const int64_t* shape, int precision, const void* lam, const void* mu, double lam0,
+                                   double mu0, const mgk_landau_params* lpar, const double* eps_bar,
+                                   const void* eps_plastic, void* eps, void* sig, void* tau_buf, int warm,
+                                   int max_iter, double tol, void* workspace, size_t workspace_bytes, void* stream,
+                                   mgk_ls_info* info) {
+  if (!lam || !mu || !eps || !eps_bar || !workspace || !tau_buf || !lpar || max_iter < 1) {
+    mgk_set_error("null argument or max_iter < 1", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  LsPlan pl;
+  int rc = precision == 0 ? ls_plan<double>(3, shape, &pl) : ls_plan<float>(3, shape, &pl);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the LS solver", __FILE__, __LINE__);
+    return rc;
+  }
+  if (workspace_bytes < pl.total) {
+    mgk_set_error("workspace too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  LandauPar P;
+  P.v1 = lpar->v1;
+  P.v2 = lpar->v2;
+  P.v3 = lpar->v3;
+  P.g1 = lpar->g1;
+  P.g2 = lpar->g2;
+  P.g3 = lpar->g3;
+  P.g4 = lpar->g4;
+  P.Ecap = lpar->cap_limit;
+  P.cap_p = lpar->cap_power;
+  P.cap_ratio = lpar->cap_ratio;
+  P.capping = lpar->capping;
+  P.cap_smooth = lpar->cap_smooth;
+  hipStream_t s = (hipStream_t)stream;
+  warm = warm ? 1 : 0;
+  if (precision == 0)
+    return ls_solve_t<double, 3>(pl, (const double*)lam, (const double*)mu, lam0, mu0, eps_bar,
+                                 (const double*)eps_plastic, (double*)eps, (double*)sig, warm, max_iter, tol,
+                                 (char*)workspace, s, info, true, &P, (double*)tau_buf);
+  return ls_solve_t<float, 3>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar, (const float*)eps_plastic,
+                              (float*)eps, (float*)sig, warm, max_iter, tol, (char*)workspace, s, info, true, &P,
+                              (float*)tau_buf);
 }

@@ -287,3 +287,97 @@ def linear_elastic_simulation_3d(E, nu, target_strain_mask, target_values, n_ste
     return _simulation(3, E, nu, target_strain_mask, target_values, n_steps, pixel, tol_ls, max_iter_ls, tol_macro,
                        max_iter_macro, store, "plane_strain", solver_kw.get("precision", FP64),
                        solver_kw.get("info"))
+
+
+def landau_params(v1, v2, v3, g1, g2, g3, g4, mu, strain_capping_enabled=False, strain_capping_limit=None,
+                  strain_capping_tangent_ratio=0.1, strain_capping_type="piecewise", strain_capping_smooth_power=1.0):
+    """mgk_landau_params with the reference's resolved capping strain
+    (mgkmc/elasticity.py:312-319: sqrt(0.2 <mu> / |g4|) if g4 < 0 else 999 when
+    no positive limit is given)."""
+    if strain_capping_limit is None or strain_capping_limit <= 0:
+        mu_val = float(np.mean(mu))
+        ecap = float(np.sqrt(0.2 * mu_val / abs(g4))) if g4 < 0 else 999.0
+    else:
+        ecap = float(strain_capping_limit)
+    return _lib.MgkLandauParams(float(v1), float(v2), float(v3), float(g1), float(g2), float(g3), float(g4),
+                                1 if strain_capping_enabled else 0, 1 if strain_capping_type == "smooth" else 0,
+                                ecap, float(strain_capping_smooth_power), float(strain_capping_tangent_ratio))
+
+
+class LandauSolver:
+    """Device-resident 3D Landau LS solver (moduli uploaded once)."""
+
+    def __init__(self, lam, mu, precision=FP64):
+        dev.require_gpu()
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        if lam.ndim != 3 or lam.shape != mu.shape:
+            raise ValueError("lam and mu must be 3D fields of one shape")
+        self.shape = lam.shape
+        self.precision = precision
+        self.lam0, self.mu0 = float(lam.mean()), float(mu.mean())
+        self.mu_host = mu
+        self.lam = dev.upload(lam, precision)
+        self.mu = dev.upload(mu, precision)
+        dt = dev.torch_dtype(precision)
+        self.eps = dev.torch.zeros((6,) + self.shape, dtype=dt, device="cuda")
+        self.sig = dev.torch.zeros((6,) + self.shape, dtype=dt, device="cuda")
+        self.tau = dev.torch.empty((6,) + self.shape, dtype=dt, device="cuda")
+        self._shape_c = _lib.shape_arg(self.shape)
+        nb = ctypes.c_size_t()
+        _lib.check(_lib.load().mgk_ls_workspace_bytes(3, self._shape_c, precision, ctypes.byref(nb)),
+                   "mgk_ls_workspace_bytes")
+        self.ws = dev.torch.empty(int(nb.value), dtype=dev.torch.uint8, device="cuda")
+        self.info = _lib.MgkLsInfo()
+
+    def solve(self, params, eps_bar, eps_plastic=None, warm=False, max_iter=400, tol=1e-6, stream=None):
+        eb, ebp = _lib.host_dbl(np.asarray(eps_bar, dtype=np.float64).reshape(3, 3))
+        s = dev.stream_ptr() if stream is None else stream
+        rc = _lib.load().mgk_ls_solve_landau(
+            self._shape_c, self.precision, _lib.dptr(self.lam), _lib.dptr(self.mu), self.lam0, self.mu0,
+            ctypes.byref(params), ebp, _lib.dptr(eps_plastic), _lib.dptr(self.eps), _lib.dptr(self.sig),
+            _lib.dptr(self.tau), 1 if warm else 0, int(max_iter), float(tol), _lib.dptr(self.ws), self.ws.numel(),
+            ctypes.c_void_p(s), ctypes.byref(self.info))
+        _lib.check(rc, "mgk_ls_solve_landau")
+        if rc == _lib.MGK_ERR_NONFINITE:
+            raise FloatingPointError(
+                "spectral_solver_landau_3d: non-finite strain update in LS loop (Landau energy likely unstable at "
+                "this strain; consider strain capping or smaller load steps).")
+        return self.info
+
+
+def spectral_solver_landau_3d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_plastic=None, max_iter=400, tol=1e-6,
+                              verbose=False, pixel=1.0, Gamma=None, eps_init=None, strain_capping_enabled=False,
+                              strain_capping_limit=None, strain_capping_tangent_ratio=0.1,
+                              strain_capping_type="piecewise", strain_capping_smooth_power=1.0, solver="dbfft"):
+    """Nonlinear LS with the 3D Landau stress on the GPU
+    (mgkmc/linear_elastic_simulator.py:1231-1350); reference medium from
+    (mean lam, mean mu); FloatingPointError on a non-finite update.  Strains
+    must be symmetric (the simulators' are)."""
+    if solver == "newton_cg":
+        raise NotImplementedError("Newton-CG solver is not yet implemented for 3D small-strain. Please use 2D or "
+                                  "default to dbfft solver.")
+    lam = np.asarray(lam, dtype=np.float64)
+    mu = np.asarray(mu, dtype=np.float64)
+    eps_bar = np.asarray(eps_bar, dtype=np.float64)
+    if np.any(eps_bar != eps_bar.T):
+        raise ValueError("the device Landau solver takes symmetric eps_bar")
+    solver_ = LandauSolver(lam, mu, FP64)
+    ep = None
+    if eps_plastic is not None:
+        sym, a = dev.to_sym_soa(eps_plastic, 3)
+        if np.any(a):
+            raise ValueError("the device Landau solver takes a symmetric eps_plastic")
+        ep = dev.upload(sym, FP64)
+    warm = eps_init is not None
+    if warm:
+        sym, a = dev.to_sym_soa(eps_init, 3)
+        if np.any(a):
+            raise ValueError("the device Landau solver takes a symmetric eps_init")
+        solver_.eps.copy_(dev.upload(sym, FP64))
+    prm = landau_params(v1, v2, v3, g1, g2, g3, g4, mu, strain_capping_enabled, strain_capping_limit,
+                        strain_capping_tangent_ratio, strain_capping_type, strain_capping_smooth_power)
+    inf = solver_.solve(prm, eps_bar, ep, warm=warm, max_iter=max_iter, tol=tol)
+    eps = dev.from_sym_soa(dev.download(solver_.eps), 3)
+    sig = dev.from_sym_soa(dev.download(solver_.sig), 3)
+    return eps, sig, np.array(inf.eps_mean[:9]).reshape(3, 3), np.array(inf.sig_mean[:9]).reshape(3, 3)

@@ -287,3 +287,133 @@ def linear_elastic_simulation(d, E, nu, target_strain_mask, target_values, n_ste
             e_l.append(eps)
             s_l.append(sig)
     return np.array(em_l), np.array(sm_l), e_l, s_l, its
+
+
+# ---------------------------------------------------------------------------
+# Landau small-strain model, 3D (elasticity.py:297-335, :811-861) and its LS
+# fixed point (linear_elastic_simulator.py:1231-1350, the default "dbfft" path)
+# ---------------------------------------------------------------------------
+def cap_limit(mu, g4, strain_capping_limit):
+    """Resolved capping strain E_cap (elasticity.py:312-319)."""
+    mu_val = mu.mean() if isinstance(mu, np.ndarray) else mu
+    if strain_capping_limit is None or strain_capping_limit <= 0:
+        return np.sqrt(0.2 * mu_val / np.abs(g4)) if g4 < 0 else 999.0
+    return strain_capping_limit
+
+
+def cap_strain_3d(eps, mu, g4, enabled, limit, ratio, kind, power):
+    """(capped strain, capping weight) (elasticity.py:297-335)."""
+    if not enabled:
+        return eps, np.zeros(eps.shape[:-2])
+    I1 = np.trace(eps, axis1=-2, axis2=-1)
+    I = np.broadcast_to(np.eye(3), eps.shape)
+    dev = eps - I1[..., None, None] * I / 3.0
+    dd = np.einsum("...ij,...ji->...", dev, dev)
+    Eeq = np.sqrt(np.maximum(0.0, (2.0 / 3.0) * dd))
+    Ecap = cap_limit(mu, g4, limit)
+    f = np.ones_like(Eeq)
+    nz = Eeq > 1e-12
+    if kind == "smooth":
+        rp = np.minimum(100.0, (Eeq[nz] / Ecap) ** power)
+        f[nz] = (Ecap / Eeq[nz]) * np.tanh(rp) ** (1.0 / power)
+        w = np.tanh(np.minimum(100.0, (Eeq / Ecap) ** power)) ** 2
+    else:
+        capped = Eeq > Ecap
+        np.divide(Ecap, Eeq, out=f, where=capped)
+        w = capped.astype(float)
+    return dev * f[..., None, None] + I1[..., None, None] * I / 3.0, w
+
+
+def stress_landau_3d(eps, lam, mu, v1, v2, v3, g1, g2, g3, g4, strain_capping_enabled=False,
+                     strain_capping_limit=None, strain_capping_tangent_ratio=0.1, strain_capping_type="piecewise",
+                     strain_capping_smooth_power=1.0):
+    """sigma = A I + B eps + C eps^2 with the Landau invariant coefficients,
+    plus the capping tangent term (elasticity.py:811-861)."""
+    e0 = eps
+    e, w = cap_strain_3d(eps, mu, g4, strain_capping_enabled, strain_capping_limit, strain_capping_tangent_ratio,
+                         strain_capping_type, strain_capping_smooth_power)
+    I1 = np.trace(e, axis1=-2, axis2=-1)
+    e2 = np.einsum("...ij,...jk->...ik", e, e)
+    I2 = np.trace(e2, axis1=-2, axis2=-1)
+    I3 = np.trace(np.einsum("...ij,...jk->...ik", e2, e), axis1=-2, axis2=-1)
+    lam = np.asarray(lam)
+    mu = np.asarray(mu)
+    A = lam * I1 + 0.5 * v1 * I1 ** 2 + v2 * I2 + (1.0 / 6.0) * g1 * I1 ** 3 + g2 * I1 * I2 + (4.0 / 3.0) * g3 * I3
+    B = 2.0 * (mu + v2 * I1 + 0.5 * g2 * I1 ** 2 + g4 * I2)
+    C = 4.0 * (v3 + g3 * I1)
+    sig = A[..., None, None] * np.eye(3) + B[..., None, None] * e + C[..., None, None] * e2
+    if strain_capping_enabled:
+        sig = sig + 2.0 * (strain_capping_tangent_ratio * mu)[..., None, None] * (e0 - e) * w[..., None, None]
+    return sig
+
+
+def landau_energy_3d(eps, lam, mu, v1, v2, v3, g1, g2, g3, g4):
+    """Stored energy whose strain derivative is the Landau stress (no capping):
+    W = lam/2 I1^2 + mu I2 + v1/6 I1^3 + v2 I1 I2 + 4/3 v3 I3 + g1/24 I1^4
+        + g2/2 I1^2 I2 + 4/3 g3 I1 I3 + g4/2 I2^2  (used to pin the stress by finite differences)."""
+    I1 = np.trace(eps, axis1=-2, axis2=-1)
+    e2 = np.einsum("...ij,...jk->...ik", eps, eps)
+    I2 = np.trace(e2, axis1=-2, axis2=-1)
+    I3 = np.trace(np.einsum("...ij,...jk->...ik", e2, eps), axis1=-2, axis2=-1)
+    return (0.5 * lam * I1 ** 2 + mu * I2 + v1 / 6.0 * I1 ** 3 + v2 * I1 * I2 + 4.0 / 3.0 * v3 * I3
+            + g1 / 24.0 * I1 ** 4 + 0.5 * g2 * I1 ** 2 * I2 + 4.0 / 3.0 * g3 * I1 * I3 + 0.5 * g4 * I2 ** 2)
+
+
+def spectral_solver_landau_3d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_plastic=None, max_iter=400, tol=1e-6,
+                              verbose=False, pixel=1.0, Gamma=None, eps_init=None, strain_capping_enabled=False,
+                              strain_capping_limit=None, strain_capping_tangent_ratio=0.1,
+                              strain_capping_type="piecewise", strain_capping_smooth_power=1.0, solver="dbfft",
+                              info=None):
+    """Nonlinear LS fixed point with the Landau stress; reference medium from
+    (mean lam, mean mu); FloatingPointError on a non-finite update
+    (linear_elastic_simulator.py:1231-1350)."""
+    if solver == "newton_cg":
+        raise NotImplementedError("newton_cg is not available for 3D small strain")
+    lam = np.asarray(lam, float)
+    mu = np.asarray(mu, float)
+    shape = lam.shape
+    la, ma = float(lam.mean()), float(mu.mean())
+    if Gamma is None:
+        Gamma = green_operator(wave_vectors(shape, pixel), la, ma)
+    if eps_init is not None:
+        eps = eps_init.copy()
+        for i in range(3):
+            for j in range(3):
+                eps[..., i, j] += eps_bar[i, j] - eps_init[..., i, j].mean()
+    else:
+        eps = np.zeros(shape + (3, 3))
+        for i in range(3):
+            for j in range(3):
+                eps[..., i, j] = eps_bar[i, j]
+    cap = dict(strain_capping_enabled=strain_capping_enabled, strain_capping_limit=strain_capping_limit,
+               strain_capping_tangent_ratio=strain_capping_tangent_ratio, strain_capping_type=strain_capping_type,
+               strain_capping_smooth_power=strain_capping_smooth_power)
+    it_done = 0
+    for it in range(max_iter):
+        e_el = eps - eps_plastic if eps_plastic is not None else eps
+        sig = stress_landau_3d(e_el, lam, mu, v1, v2, v3, g1, g2, g3, g4, **cap)
+        tr = np.trace(eps, axis1=3, axis2=4)[..., None, None]
+        tau = sig - (la * tr * np.eye(3) + 2.0 * ma * eps)
+        tau_hat = np.zeros(shape + (3, 3), dtype=complex)
+        for i in range(3):
+            for j in range(3):
+                tau_hat[..., i, j] = fft_field(tau[..., i, j])
+        et_hat = -np.einsum("xyzkhij,xyzij->xyzkh", Gamma, tau_hat)
+        et = np.zeros(shape + (3, 3))
+        for i in range(3):
+            for j in range(3):
+                et[..., i, j] = ifft_field(et_hat[..., i, j])
+                et[..., i, j] -= et[..., i, j].mean()
+        eps_new = eps_bar[None, None, None] + et
+        diff = np.linalg.norm(eps_new - eps) / (np.linalg.norm(eps) + 1e-20)
+        eps = eps_new
+        it_done = it + 1
+        if not np.isfinite(diff):
+            raise FloatingPointError("spectral_solver_landau_3d: non-finite strain update")
+        if diff < tol:
+            break
+    e_el = eps - eps_plastic if eps_plastic is not None else eps
+    sig_out = stress_landau_3d(e_el, lam, mu, v1, v2, v3, g1, g2, g3, g4, **cap)
+    if info is not None:
+        info.update(iters=it_done, diff=diff)
+    return eps, sig_out, eps.mean(axis=(0, 1, 2)), sig_out.mean(axis=(0, 1, 2))

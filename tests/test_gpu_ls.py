@@ -244,3 +244,80 @@ def test_mixed_bc_simulation_parity(gpu, shape, plane_mode):
     assert _rel(got[0], ref[0]) < 1e-10 and _rel(got[1], ref[1]) < 1e-10
     for a, b in zip(got[2] + got[3], ref[2] + ref[3]):
         assert _rel(a, b) < 1e-10
+
+
+LANDAU_PA = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9)
+_LANDAU_CASES = [
+    ("plain", [[0.01, 0.02, 0], [0.02, -0.003, 0], [0, 0, 0]], {}),
+    ("piecewise cap", [[0.01, 0.05, 0], [0.05, -0.003, 0], [0, 0, 0]],
+     dict(strain_capping_enabled=True, strain_capping_limit=0.03)),
+    ("smooth cap", [[0.01, 0.05, 0], [0.05, -0.003, 0], [0, 0, 0]],
+     dict(strain_capping_enabled=True, strain_capping_limit=0.03, strain_capping_type="smooth",
+          strain_capping_smooth_power=2.0)),
+    ("default cap limit", [[0.01, 0.05, 0], [0.05, -0.003, 0], [0, 0, 0]], dict(strain_capping_enabled=True)),
+]
+
+
+@pytest.mark.parametrize("name,eb,cap", _LANDAU_CASES)
+def test_landau_ls_parity(gpu, name, eb, cap):
+    # 3D Landau small-strain LS (FP64) vs the oracle: fields <= 1e-9 relative,
+    # iteration counts equal (+-1 where diff crosses tol within round-off)
+    from mgkmc_amd.linear_elastic_simulator import spectral_solver_landau_3d
+    print(f"[landau] {name}", flush=True)
+    shape = (8, 6, 10)
+    rng = np.random.default_rng(3)
+    E = 70e9 * rng.uniform(0.9, 1.1, shape)
+    nu = np.full(shape, 0.3)
+    lam, mu = ol.compute_lame_3d(E, nu)
+    g = rng.standard_normal(shape + (3, 3)) * 2e-3
+    ep = 0.5 * (g + np.swapaxes(g, -1, -2))
+    eb = np.array(eb, float)
+    info = {}
+    ref = ol.spectral_solver_landau_3d(lam, mu, **LANDAU_PA, eps_bar=eb, eps_plastic=ep, tol=1e-8, info=info, **cap)
+    from mgkmc_amd.linear_elastic_simulator import LandauSolver, landau_params
+    from mgkmc_amd import device as dev
+    got = spectral_solver_landau_3d(lam, mu, **LANDAU_PA, eps_bar=eb, eps_plastic=ep, tol=1e-8, **cap)
+    for a, b in zip(got, ref):
+        assert _rel(a, b) < 1e-9, (name, _rel(a, b))
+    s = LandauSolver(lam, mu)
+    sym, _ = dev.to_sym_soa(ep, 3)
+    inf = s.solve(landau_params(**LANDAU_PA, mu=mu, **cap), eb, dev.upload(sym, 0), tol=1e-8)
+    print(f"[landau] {name}: iters {inf.iters} (oracle {info['iters']})", flush=True)
+    assert abs(inf.iters - info["iters"]) <= 1
+
+
+def test_landau_ls_nonfinite_raises(gpu):
+    from mgkmc_amd.linear_elastic_simulator import spectral_solver_landau_3d
+    shape = (8, 6, 10)
+    rng = np.random.default_rng(3)
+    lam, mu = ol.compute_lame_3d(70e9 * rng.uniform(0.9, 1.1, shape), np.full(shape, 0.3))
+    eb = np.array([[0.01, 0.3, 0], [0.3, -0.003, 0], [0, 0, 0]])
+    with pytest.raises(FloatingPointError):
+        ol.spectral_solver_landau_3d(lam, mu, **LANDAU_PA, eps_bar=eb, tol=1e-8)
+    with pytest.raises(FloatingPointError):
+        spectral_solver_landau_3d(lam, mu, **LANDAU_PA, eps_bar=eb, tol=1e-8)
+
+
+def test_landau_ls_specialised_256(gpu):
+    # 256^3 Landau + smooth capping runs the compile-time sweeps; with v_i = g_i
+    # = 0 it reproduces the linear LS solution of the same field (both FP64)
+    import torch
+    from mgkmc_amd.linear_elastic_simulator import LandauSolver, LsSolver, landau_params, FP64
+    n = 128
+    rng = np.random.default_rng(5)
+    E = 70e9 * rng.uniform(0.9, 1.1, (n, n, n))
+    nu = np.full((n, n, n), 0.3)
+    lam, mu = ol.compute_lame_3d(E, nu)
+    ep = torch.from_numpy(rng.standard_normal((6, n, n, n)) * 1e-3).cuda()
+    eb = np.diag([0.01, 0.0, 0.0])
+    s = LandauSolver(lam, mu)
+    inf = s.solve(landau_params(**{k: 0.0 for k in LANDAU_PA}, mu=mu), eb, ep, tol=1e-10, max_iter=200)
+    lin = LsSolver(E, nu, precision=FP64)
+    inf2 = lin.solve(eb, ep, tol=1e-10)
+    assert inf.status == 0 and inf2.status == 0
+    err = float((s.eps - lin.eps).abs().max() / lin.eps.abs().max())
+    print(f"[landau 128^3] iters {inf.iters} vs linear {inf2.iters}, field diff {err:.2e}", flush=True)
+    assert err < 1e-8
+    inf = s.solve(landau_params(**LANDAU_PA, mu=mu, strain_capping_enabled=True, strain_capping_limit=0.1,
+                                strain_capping_type="smooth"), eb, ep, tol=1e-6, max_iter=400)
+    assert inf.status == 0 and inf.diff < 1e-6

@@ -167,3 +167,51 @@ def test_mixed_bc_driver_homogeneous_closed_form(d, plane_mode):
         assert abs(sm[-1][k, k]) < 1.0
     assert np.allclose(em[1], 0.5 * em[2], rtol=1e-6, atol=1e-14)
     assert len(el) == 3 and el[-1].shape == shape + (d, d)
+
+
+LANDAU_PA = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9)
+
+
+def test_landau_stress_is_energy_derivative():
+    # sigma_ij = dW/d eps_ij (central differences of the independently written energy)
+    rng = np.random.default_rng(4)
+    g = rng.standard_normal((3, 3)) * 0.01
+    eps = 0.5 * (g + g.T)
+    lam, mu = 60e9, 27e9
+    sig = ol.stress_landau_3d(eps, lam, mu, **LANDAU_PA)
+    h = 1e-7
+    for i in range(3):
+        for j in range(3):
+            d = np.zeros((3, 3))
+            d[i, j] += 0.5 * h
+            d[j, i] += 0.5 * h
+            fd = (ol.landau_energy_3d(eps + d, lam, mu, **LANDAU_PA) -
+                  ol.landau_energy_3d(eps - d, lam, mu, **LANDAU_PA)) / (2 * h)
+            assert abs(fd - sig[i, j]) <= 1e-6 * np.abs(sig).max(), (i, j, fd, sig[i, j])
+
+
+def test_landau_ls_linear_limit_and_capping():
+    # v_i = g_i = 0: the Landau LS converges to the linear LS solution (the
+    # reference media differ, the fixed point does not); capping leaves small
+    # strains alone and holds the deviatoric strain at E_cap for large ones
+    shape = (6, 5, 4)
+    rng = np.random.default_rng(2)
+    E = 70e9 * rng.uniform(0.9, 1.1, shape)
+    nu = np.full(shape, 0.3)
+    lam, mu = ol.compute_lame_3d(E, nu)
+    g = rng.standard_normal(shape + (3, 3)) * 1e-3
+    ep = 0.5 * (g + np.swapaxes(g, -1, -2))
+    eb = np.diag([0.01, -0.002, 0.0])
+    zero = {k: 0.0 for k in LANDAU_PA}
+    a = ol.spectral_solver_landau_3d(lam, mu, eps_bar=eb, eps_plastic=ep, tol=1e-12, **zero)
+    b = ol.spectral_solver_3d(E, nu, eb, eps_plastic=ep, tol=1e-12)
+    assert np.abs(a[0] - b[0]).max() < 1e-10 * np.abs(b[0]).max()
+    assert np.abs(a[1] - b[1]).max() < 1e-9 * np.abs(b[1]).max()
+    small = np.diag([1e-4, -5e-5, -5e-5])
+    e, w = ol.cap_strain_3d(small, 27e9, -305e9, True, 0.1, 0.1, "piecewise", 1.0)
+    assert np.array_equal(e, small) and w == 0.0
+    big = np.array([[0.0, 0.3, 0.0], [0.3, 0.0, 0.0], [0.0, 0.0, 0.0]]) + 0.01 * np.eye(3)
+    e, w = ol.cap_strain_3d(big, 27e9, -305e9, True, 0.1, 0.1, "piecewise", 1.0)
+    dev = e - np.trace(e) / 3 * np.eye(3)
+    assert abs(np.sqrt(2.0 / 3.0 * np.sum(dev * dev)) - 0.1) < 1e-14 and w == 1.0
+    assert abs(np.trace(e) - np.trace(big)) < 1e-15
