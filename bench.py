@@ -166,24 +166,26 @@ def gpu_kmc(n, events, warmup, precision):
 LANDAU = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9)
 
 
-def gpu_fs(n, increments):
+def gpu_fs(n, increments, mixed=False):
     """Finite-strain Newton-BiCGSTAB at n^3 (256^3 config recipe): mu ~ mu_unit *
     lognormal(mean 1, sigma 0.1) (seed 2), kappa = 2.5 mu, Landau constants in Pa
     with smooth strain capping (limit 0.10), simple shear F_xy ramped by 0.1% per
-    increment (the first `increments` of the 50 to 5%), tol_CG 1e-8, tol_NW 1e-5,
-    FP64.  voxel-matvec = one Krylov operator application over one voxel."""
+    increment (the first `increments` of the 50 to 5%), tol_NW 1e-5; FP64 with
+    tol_CG 1e-8, or (mixed) the FP64 Newton loop with FP32 Krylov solves at
+    tol_CG 1e-6.  voxel-matvec = one Krylov operator application over one voxel."""
     import torch
     from mgkmc_amd import _lib
-    from mgkmc_amd.finite_strain_simulator import FsSolver, material
+    from mgkmc_amd.finite_strain_simulator import FsSolver, material, MIXED
     rng = np.random.default_rng(2)
     mu = MU_UNIT * np.exp(rng.normal(-0.005, 0.1, (n, n, n)))
     lam = (2.5 - 2.0 / 3.0) * mu
-    s = FsSolver(lam, mu, precision=0)
+    s = FsSolver(lam, mu, precision=MIXED if mixed else 0)
+    tcg = 1e-6 if mixed else 1e-8
     mat = material("landau", **LANDAU, strain_capping_enabled=True, strain_capping_type="smooth",
                    strain_capping_limit=0.10, mu_mean=float(mu.mean()))
     eps = np.linspace(0.0, 0.05, 51)
     Fb = np.eye(3)
-    s.step(Fb, mat, (0, 1), tol_NW=1e-5, tol_CG=1e-8)  # increment 0 (F = I), warm-up
+    s.step(Fb, mat, (0, 1), tol_NW=1e-5, tol_CG=tcg)  # increment 0 (F = I), warm-up
     torch.cuda.synchronize()
     lib = _lib.load()
     lib.mgk_profile_reset()
@@ -193,16 +195,17 @@ def gpu_fs(n, increments):
     for k in range(1, increments + 1):
         Fb = np.eye(3)
         Fb[0, 1] = eps[k]
-        inf = s.step(Fb, mat, (0, 1), tol_NW=1e-5, tol_CG=1e-8, max_NW=20)
+        inf = s.step(Fb, mat, (0, 1), tol_NW=1e-5, tol_CG=tcg, max_NW=20)
         mv += 2 * inf.krylov_iters + inf.newton_iters
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     lib.mgk_profile_enable(0)
     prof = _lib.profile_read()
+    F = s.F.clone()
     del s
     torch.cuda.empty_cache()
     return {"voxel_matvec_per_s": mv * n ** 3 / dt, "increments": increments, "matvecs": mv,
-            "seconds_per_increment": dt / increments, "prof": prof}
+            "seconds_per_increment": dt / increments, "prof": prof, "F": F}
 
 
 def gpu_serving(n_requests, rate, events):
@@ -314,6 +317,9 @@ def main():
             prof[k] = kmc32["prof"][k]
     log("finite strain")
     fsr = gpu_fs(a.n, a.fs_increments) if a.fs_increments > 0 else None
+    fsm = gpu_fs(a.n, a.fs_increments, mixed=True) if a.fs_increments > 0 else None
+    if fsm is not None:
+        fsm["F_err_vs_fp64"] = float((fsm["F"] - fsr["F"]).abs().max() / fsr["F"].abs().max())
     log("serving stream")
     srv = gpu_serving(a.serve_requests, a.serve_rate, a.serve_events) if a.serve_requests > 0 else None
     roof = None
@@ -368,6 +374,12 @@ def main():
             "per_kernel": {k: {"avg_ms": v[0] / v[2], "GB_s": (v[1] / v[2]) / (v[0] / v[2] * 1e-3) / 1e9}
                            for k, v in fsr["prof"].items()},
             "workload": f"{a.n}^3 Landau + smooth capping, simple shear F_xy 0.1%/increment, tol_CG 1e-8"},
+        "finite_strain_refined": None if fsm is None else {
+            "voxel_matvec_per_s": fsm["voxel_matvec_per_s"], "matvecs": fsm["matvecs"],
+            "seconds_per_increment": fsm["seconds_per_increment"],
+            "time_to_solution_fp64_over_refined": fsr["seconds_per_increment"] / fsm["seconds_per_increment"],
+            "F_err_vs_fp64": fsm["F_err_vs_fp64"],
+            "workload": "same increments, FP64 Newton loop + FP32 BiCGSTAB at tol_CG 1e-6"},
         "serving": srv,
         "roofline": roof,
         "cpu_baseline": cpu,

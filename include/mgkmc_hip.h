@@ -225,6 +225,20 @@ int mgk_fs_solver_step(const int64_t* shape, int precision, void* F, const void*
                        double tol_CG, int max_NW, double tol_macro, int max_iter_macro, void* P_out,
                        void* sig_out, void* workspace, size_t workspace_bytes, void* stream, mgk_fs_info* info);
 
+/* FP32 form of the same step (replaces the same reference function; the precision
+ * split has no reference counterpart): the Newton loop, P, the residual -G:P, the line
+ * search and the macro BC update run in FP64 on F, lam, mu (FP64); each BiCGSTAB solve
+ * runs in FP32 (vectors, tangent action, projection FFTs; Krylov scalars in FP64) on
+ * lam32, mu32 with rtol = tol_CG, and its solution is widened into the FP64 update.
+ * ws32: mgk_fs_workspace_bytes(shape, 1) bytes. */
+int mgk_fs_solver_step_refined(const int64_t* shape, void* F, const void* Fp, const void* lam, const void* mu,
+                               const void* lam32, const void* mu32, const mgk_fs_material* mat, const double* F_bar,
+                               const double* P_target, const int* P_mask, int i_drv, int j_drv, double E_avg,
+                               double nu_avg, double tol_NW, double tol_CG, int max_NW, double tol_macro,
+                               int max_iter_macro, void* P_out, void* sig_out, void* workspace,
+                               size_t workspace_bytes, void* ws32, size_t ws32_bytes, void* stream,
+                               mgk_fs_info* info);
+
 /* P(F) and optionally dP = K:dF (reference matvec convention) per voxel
  * (constitutive_hyperelastic_3d, mgkmc/finite_strain_simulator.py:1647-1885). */
 int mgk_fs_constitutive(long long nvox, int precision, const void* F, const void* Fp, const void* lam,

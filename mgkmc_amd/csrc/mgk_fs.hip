@@ -1203,12 +1203,9 @@ int fs_const(FsCtx<T>& c, const T* Fb, const T* dX, const Nine& dXu, double alph
 }
 
 template <typename T>
-int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu, const mgk_fs_material* mt,
-              const double* F_bar, const double* P_target, const int* P_mask, int i_drv, int j_drv, double E_avg,
-              double nu_avg, double tol_NW, double tol_CG, int max_NW, double tol_macro, int max_iter_macro,
-              T* P_out, T* sig_out, char* ws, hipStream_t s, mgk_fs_info* info) {
+int fs_ctx_init(FsCtx<T>& c, const int64_t* shape, const T* lam, const T* mu, const mgk_fs_material* mt,
+                bool has_fp, char* ws, hipStream_t s) {
   constexpr int prec = sizeof(T) == 8 ? 0 : 1;
-  FsCtx<T> c;
   int rc = fs_plan<T>(shape, &c.pl);
   if (rc != MGK_OK) {
     mgk_set_error("unsupported grid shape for the finite-strain solver", __FILE__, __LINE__);
@@ -1269,11 +1266,56 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
   c.kst = (KryState*)w;
   w += mgk_align(sizeof(KryState));
   c.scal = (double*)w;
-  if (!Fp) c.Fpi = nullptr;
+  if (!has_fp) c.Fpi = nullptr;
+  return MGK_OK;
+}
+
+__global__ void k_fs_narrow(const double* __restrict__ a, float* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+__global__ void k_fs_widen(const float* __restrict__ a, double* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = (double)a[i];
+}
+
+// Krylov solve of the Newton system in FP32 (the FP64 Newton loop keeps F, P,
+// the residual and the line search): b and the current F narrowed into the
+// FP32 context, BiCGSTAB there, the solution widened into c.x
+int fs_bicgstab_f32(FsCtx<double>& c, FsCtx<float>& c32, const double* Fcur, double rtol, int maxiter, int* iters,
+                    int* status) {
+  const size_t n = (size_t)9 * c.pl.NV;
+  const int nb = (int)std::min<size_t>((n + MGK_NT - 1) / MGK_NT, 8192);
+  k_fs_narrow<<<nb, MGK_NT, 0, c.s>>>(c.b, c32.b, n);
+  k_fs_narrow<<<nb, MGK_NT, 0, c.s>>>(Fcur, c32.Ftr, n);
+  MGK_HIP(hipGetLastError());
+  const int rc = fs_bicgstab(c32, c32.Ftr, rtol, maxiter, iters, status);
+  if (rc) return rc;
+  k_fs_widen<<<nb, MGK_NT, 0, c.s>>>(c32.x, c.x, n);
+  MGK_HIP(hipGetLastError());
+  return MGK_OK;
+}
+
+template <typename T>
+int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu, const mgk_fs_material* mt,
+              const double* F_bar, const double* P_target, const int* P_mask, int i_drv, int j_drv, double E_avg,
+              double nu_avg, double tol_NW, double tol_CG, int max_NW, double tol_macro, int max_iter_macro,
+              T* P_out, T* sig_out, char* ws, hipStream_t s, mgk_fs_info* info, FsCtx<float>* inner = nullptr) {
+  FsCtx<T> c;
+  int rc = fs_ctx_init(c, shape, lam, mu, mt, Fp != nullptr, ws, s);
+  if (rc) return rc;
+  const FsPlan& pl = c.pl;
   MgkProfRec prof(s);
   c.prof = &prof;
   const size_t NV = (size_t)pl.NV;
   if (Fp) k_fs_invfp<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, Fp, c.Fpi);
+  if constexpr (sizeof(T) == 8) {
+    if (inner && Fp) {
+      const int nb = (int)std::min<size_t>((9 * NV + MGK_NT - 1) / MGK_NT, 8192);
+      k_fs_narrow<<<nb, MGK_NT, 0, s>>>(c.Fpi, inner->Fpi, 9 * NV);
+    }
+  }
 
   // F is updated in place; the accepted trial is copied back into F
   double Fbc[9];
@@ -1322,7 +1364,14 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
         fs_project(c, c.P, c.b, T(-1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
       }
       int kit = 0, kst = 0;
-      rc = fs_bicgstab(c, F, tol_CG, 150, &kit, &kst);
+      bool done_inner = false;
+      if constexpr (sizeof(T) == 8) {
+        if (inner) {
+          rc = fs_bicgstab_f32(c, *inner, F, tol_CG, 150, &kit, &kst);
+          done_inner = true;
+        }
+      }
+      if (!done_inner) rc = fs_bicgstab(c, F, tol_CG, 150, &kit, &kst);
       if (rc) return rc;
       info->krylov_iters += kit;
       // |dF|
@@ -1480,6 +1529,43 @@ extern "C" int mgk_fs_solver_step(const int64_t* shape, int precision, void* F, 
     rc = fs_step_t<float>(shape, (float*)F, (const float*)Fp, (const float*)lam, (const float*)mu, mat, F_bar,
                           P_target, P_mask, i_drv, j_drv, E_avg, nu_avg, tol_NW, tol_CG, max_NW, tol_macro,
                           max_iter_macro, (float*)P_out, (float*)sig_out, (char*)workspace, s, info);
+  info->status = rc;
+  return rc;
+}
+
+extern "C" int mgk_fs_solver_step_refined(const int64_t* shape, void* F, const void* Fp, const void* lam,
+                                          const void* mu, const void* lam32, const void* mu32,
+                                          const mgk_fs_material* mat, const double* F_bar, const double* P_target,
+                                          const int* P_mask, int i_drv, int j_drv, double E_avg, double nu_avg,
+                                          double tol_NW, double tol_CG, int max_NW, double tol_macro,
+                                          int max_iter_macro, void* P_out, void* sig_out, void* workspace,
+                                          size_t workspace_bytes, void* ws32, size_t ws32_bytes, void* stream,
+                                          mgk_fs_info* info) {
+  if (!shape || !F || !lam || !mu || !lam32 || !mu32 || !mat || !F_bar || !P_target || !P_mask || !workspace ||
+      !ws32 || !info) {
+    mgk_set_error("mgk_fs_solver_step_refined: null argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  size_t need = 0, need32 = 0;
+  int rc = mgk_fs_workspace_bytes(shape, 0, &need);
+  if (rc) return rc;
+  rc = mgk_fs_workspace_bytes(shape, 1, &need32);
+  if (rc) return rc;
+  if (workspace_bytes < need || ws32_bytes < need32) {
+    mgk_set_error("workspace too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  memset(info, 0, sizeof(*info));
+  hipStream_t s = (hipStream_t)stream;
+  FsCtx<float> c32;
+  rc = fs_ctx_init(c32, shape, (const float*)lam32, (const float*)mu32, mat, Fp != nullptr, (char*)ws32, s);
+  if (rc) return rc;
+  MgkProfRec prof32(s);
+  c32.prof = &prof32;
+  rc = fs_step_t<double>(shape, (double*)F, (const double*)Fp, (const double*)lam, (const double*)mu, mat, F_bar,
+                         P_target, P_mask, i_drv, j_drv, E_avg, nu_avg, tol_NW, tol_CG, max_NW, tol_macro,
+                         max_iter_macro, (double*)P_out, (double*)sig_out, (char*)workspace, s, info, &c32);
+  prof32.commit(1 << 30);
   info->status = rc;
   return rc;
 }

@@ -65,17 +65,27 @@ def material(model_type="svk", A_m=0.0, B_m=0.0, C_m=0.0, v1=0.0, v2=0.0, v3=0.0
     return m
 
 
+MIXED = 2  # FP64 Newton loop, FP32 Krylov solves
+
+
 class FsSolver:
     """Device-resident finite-strain state for one grid: lam, mu, F, P, Fp."""
 
     def __init__(self, lam, mu, precision=0, Fp=None):
+        """precision: 0 FP64, 1 FP32, MIXED (2): FP64 Newton loop (F, P,
+        residual, line search, macro BC) with the Krylov solves in FP32."""
         dev.require_gpu()
         lam = np.ascontiguousarray(lam, dtype=np.float64)
         self.shape = lam.shape
         self.NV = int(np.prod(self.shape))
-        self.precision = precision
+        self.mixed = precision == MIXED
+        self.precision = 0 if self.mixed else precision
+        precision = self.precision
         self.lam = dev.upload(lam, precision)
         self.mu = dev.upload(np.ascontiguousarray(mu, dtype=np.float64), precision)
+        if self.mixed:
+            self.lam32 = dev.upload(lam, 1)
+            self.mu32 = dev.upload(np.ascontiguousarray(mu, dtype=np.float64), 1)
         self.mu_mean = float(np.mean(mu))
         dt = dev.torch_dtype(precision)
         eye = dev.torch.eye(3, dtype=dt, device="cuda").reshape(3, 3, 1, 1, 1)
@@ -88,6 +98,10 @@ class FsSolver:
         _lib.check(_lib.load().mgk_fs_workspace_bytes(self._shape_c, precision, ctypes.byref(n)),
                    "mgk_fs_workspace_bytes")
         self.ws = dev.torch.empty(int(n.value), dtype=dev.torch.uint8, device="cuda")
+        if self.mixed:
+            _lib.check(_lib.load().mgk_fs_workspace_bytes(self._shape_c, 1, ctypes.byref(n)),
+                       "mgk_fs_workspace_bytes")
+            self.ws32 = dev.torch.empty(int(n.value), dtype=dev.torch.uint8, device="cuda")
         self.info = _lib.MgkFsInfo()
 
     def step(self, F_bar, mat, driving_component=(0, 0), P_target=None, P_mask=None, E_avg=100e9, nu_avg=0.3,
@@ -96,12 +110,21 @@ class FsSolver:
         Pt, Ptp = _lib.host_dbl(np.zeros(9) if P_target is None else np.asarray(P_target, float).reshape(9))
         pm = np.zeros(9, dtype=np.int32) if P_mask is None else np.asarray(P_mask, bool).reshape(9).astype(np.int32)
         pm_c = (ctypes.c_int * 9)(*pm.tolist())
-        rc = _lib.load().mgk_fs_solver_step(
-            self._shape_c, self.precision, _lib.dptr(self.F), _lib.dptr(self.Fp), _lib.dptr(self.lam),
-            _lib.dptr(self.mu), ctypes.byref(mat), Fbp, Ptp, pm_c, int(driving_component[0]),
-            int(driving_component[1]), float(E_avg), float(nu_avg), float(tol_NW), float(tol_CG), int(max_NW),
-            float(tol_macro), int(max_iter_macro), _lib.dptr(self.P), _lib.dptr(self.sig), _lib.dptr(self.ws),
-            self.ws.numel(), ctypes.c_void_p(dev.stream_ptr()), ctypes.byref(self.info))
+        if self.mixed:
+            rc = _lib.load().mgk_fs_solver_step_refined(
+                self._shape_c, _lib.dptr(self.F), _lib.dptr(self.Fp), _lib.dptr(self.lam), _lib.dptr(self.mu),
+                _lib.dptr(self.lam32), _lib.dptr(self.mu32), ctypes.byref(mat), Fbp, Ptp, pm_c,
+                int(driving_component[0]), int(driving_component[1]), float(E_avg), float(nu_avg), float(tol_NW),
+                float(tol_CG), int(max_NW), float(tol_macro), int(max_iter_macro), _lib.dptr(self.P),
+                _lib.dptr(self.sig), _lib.dptr(self.ws), self.ws.numel(), _lib.dptr(self.ws32), self.ws32.numel(),
+                ctypes.c_void_p(dev.stream_ptr()), ctypes.byref(self.info))
+        else:
+            rc = _lib.load().mgk_fs_solver_step(
+                self._shape_c, self.precision, _lib.dptr(self.F), _lib.dptr(self.Fp), _lib.dptr(self.lam),
+                _lib.dptr(self.mu), ctypes.byref(mat), Fbp, Ptp, pm_c, int(driving_component[0]),
+                int(driving_component[1]), float(E_avg), float(nu_avg), float(tol_NW), float(tol_CG), int(max_NW),
+                float(tol_macro), int(max_iter_macro), _lib.dptr(self.P), _lib.dptr(self.sig), _lib.dptr(self.ws),
+                self.ws.numel(), ctypes.c_void_p(dev.stream_ptr()), ctypes.byref(self.info))
         if rc == _lib.MGK_ERR_SOLVER:
             raise ValueError(_lib.last_error().split(" (")[0])
         _lib.check(rc, "mgk_fs_solver_step")
