@@ -208,6 +208,50 @@ def gpu_fs(n, increments, mixed=False):
             "seconds_per_increment": dt / increments, "prof": prof, "F": F}
 
 
+def gpu_config3(n, events):
+    """512^3 KMC with the refined LS under mixed BCs (eps_xx driven, sig_yy =
+    sig_zz = 0): mu ~ U[0.8, 1.2] * mu_unit (seed 3), nu = 0.35, 5% of the
+    voxels directionally softened (g_p = ln 2, previous direction a random
+    planar pure shear), kT/E0 = 0.03, M = 20 modes drawn on the device, strain
+    rate 1e7/s in 1e-4 increments; runs `events` events."""
+    import torch
+    from mgkmc_amd.aqs import ThermalSimulation
+    rng = np.random.default_rng(3)
+    mu = MU_UNIT * rng.uniform(0.8, 1.2, (n, n, n))
+    nu = np.full((n, n, n), 0.35)
+    E = 2.0 * mu * (1.0 + nu)
+    del mu
+    t0 = time.perf_counter()
+    sim = ThermalSimulation(n, n, n, 20, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian",
+                            barrier_kwargs=dict(mean=2.2, std=0.45, min_cutoff=0.0), softening_scheme="directional",
+                            softening_cap=0.916, jp=20.0, jt=0.0, temperature=0.03 * 2.2 / 8.617e-5,
+                            strain_rate=1e7, ls_precision="mixed", rng=np.random.RandomState(3),
+                            selection_stream=np.random.RandomState(4), init_modes=("device", 3))
+    del E, nu
+    NV = n ** 3
+    idx = torch.from_numpy(rng.choice(NV, NV // 20, replace=False)).cuda()
+    th = torch.from_numpy(rng.uniform(0.0, np.pi, NV // 20)).cuda()
+    sim.kmc.soft[0, idx] = float(np.log(2.0))
+    # planar pure shear of unit amplitude along angle th: [[c, s], [s, -c]] (c = cos 2th, s = sin 2th)
+    c2, s2 = torch.cos(2 * th).double(), torch.sin(2 * th).double()
+    prev = sim.kmc.prev
+    prev[0, idx], prev[1, idx], prev[3, idx], prev[4, idx] = c2, s2, s2, -c2
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ev = sim.run_simulation(10 ** 9, 1e-4, component=(0, 0), stress_targets={(1, 1): 0.0, (2, 2): 0.0},
+                            mixed_tol=1e6, max_events=events)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"events": len(ev), "seconds": dt, "kmc_steps_per_s": len(ev) / dt, "setup_s": t_setup,
+           "hbm_gb": sim.footprint_bytes() / 1e9, "eps_xx": float(sim.eps_macro[0, 0]),
+           "sig_mean_gpa": [float(x) / 1e9 for x in np.diag(sim.sig_mean())],
+           "workload": f"{n}^3, M=20, kT/E0=0.03, directional softening on 5% of voxels, refined LS, mixed BC"}
+    del sim
+    torch.cuda.empty_cache()
+    return out
+
+
 def gpu_serving(n_requests, rate, events):
     """Serving stream (Poisson arrivals at `rate`/s, seed 4; grids uniform over
     64^2..512^2 and 64^3..192^3; kT/E0 ~ U[0.01, 0.08]; `events` KMC events
@@ -291,6 +335,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-events", type=int, default=100)
+    ap.add_argument("--config3-events", type=int, default=0,
+                    help="run only the 512^3 KMC leg with this many events and print its JSON")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run only the short LS workload used for rocprofv3 --pmc counter runs")
     a = ap.parse_args()
@@ -300,6 +346,9 @@ def main():
         sys.exit(2)
     if a.pmc_workload:
         pmc_workload(a.n)
+        return
+    if a.config3_events > 0:
+        print(json.dumps(gpu_config3(512, a.config3_events)))
         return
     log(f"LS forms at {a.n}^3")
     res = gpu_run(a.n, a.steps, a.warmup)

@@ -452,21 +452,38 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xr2c(FsGeom<T> g, const T* __rest
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
   for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
-  for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
-    const int b = e & (B - 1), x = e >> logB, q = q0 + b;
-    T a = T(0), bb = T(0);
-    if (q < Q) {
-      const size_t v = ((size_t)x * P + p) * Q + q;
-      a = f[(size_t)ca * NV + v];
-      if (cb >= 0) bb = f[(size_t)cb * NV + v];
+  // 16-byte vector path on full, aligned tiles (as the LS x sweeps)
+  constexpr int VW = vw16<T>();
+  constexpr bool VOK = CT && (BB % VW == 0);
+  const bool vec = VOK && q0 + B <= Q && Q % VW == 0;
+  if (vec) {
+    constexpr int GPR = VOK ? BB / VW : 1;
+    for (int e = threadIdx.x; e < nx * GPR; e += MGK_NT) {
+      const int bg = e % GPR, x = e / GPR;
+      const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
+      T a[VW], b[VW];
+      vload16(f + (size_t)ca * NV + v, a);
+      if (cb >= 0) vload16(f + (size_t)cb * NV + v, b);
+#pragma unroll
+      for (int i = 0; i < VW; ++i) tile[x * ldb + bg * VW + i] = cmk(a[i], cb >= 0 ? b[i] : T(0));
     }
-    tile[x * ldb + b] = cmk(a, bb);
+  } else {
+    for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
+      const int b = e & (B - 1), x = e >> logB, q = q0 + b;
+      T a = T(0), bb = T(0);
+      if (q < Q) {
+        const size_t v = ((size_t)x * P + p) * Q + q;
+        a = f[(size_t)ca * NV + v];
+        if (cb >= 0) bb = f[(size_t)cb * NV + v];
+      }
+      tile[x * ldb + b] = cmk(a, bb);
+    }
   }
   __syncthreads();
   if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, false);
   else tile_fft<MAXV>(tile, Tile{nx, logB, ldb, 1, 0}, g.plx, tw, false);
-  xsplit_store(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
-               cb >= 0 ? spec + (size_t)cb * cs : (cplx<T>*)nullptr);
+  store_half<T, NN, BB>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
+                        cb >= 0 ? spec + (size_t)cb * cs : (cplx<T>*)nullptr);
 }
 
 // c2r along x into real comps; optional fused dots: part[2*blk] += out.a,
@@ -489,11 +506,51 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* 
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
   for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
-  xmerge_load(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
-              cb >= 0 ? spec + (size_t)cb * cs : (const cplx<T>*)nullptr);
+  constexpr int VW = vw16<T>();
+  constexpr bool VOK = CT && (BB % VW == 0);
+  const bool vec = VOK && q0 + B <= Q && Q % VW == 0;
+  const cplx<T>* sa = spec + (size_t)ca * cs;
+  const cplx<T>* sb = cb >= 0 ? spec + (size_t)cb * cs : (const cplx<T>*)nullptr;
+  bool merged = false;
+  if constexpr (sizeof(T) == 4 && CT && BB % 2 == 0) {
+    if (vec) {
+      xmerge_load_v<NN, BB>(tile, q0, Q, P, p, sa, sb);
+      merged = true;
+    }
+  }
+  if (!merged) xmerge_load(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, sa, sb);
   if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, true);
   else tile_fft<MAXV>(tile, Tile{nx, logB, ldb, 1, 0}, g.plx, tw, true);
   double acc[2] = {0.0, 0.0};
+  if (vec) {
+    constexpr int GPR = VOK ? BB / VW : 1;
+    for (int e = threadIdx.x; e < nx * GPR; e += MGK_NT) {
+      const int bg = e % GPR, x = e / GPR;
+      const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
+      T oa[VW], ob[VW], da[VW], db[VW];
+#pragma unroll
+      for (int i = 0; i < VW; ++i) {
+        const cplx<T> z = tile[x * ldb + bg * VW + i];
+        oa[i] = z.x;
+        ob[i] = z.y;
+      }
+      vstore16(out + (size_t)ca * NV + v, oa);
+      if (cb >= 0) vstore16(out + (size_t)cb * NV + v, ob);
+      if (dota) {
+        vload16(dota + (size_t)ca * NV + v, da);
+        if (cb >= 0) vload16(dota + (size_t)cb * NV + v, db);
+      }
+#pragma unroll
+      for (int i = 0; i < VW; ++i) {
+        if (dota) acc[0] += (double)oa[i] * (double)da[i];
+        if (selfdot) acc[1] += (double)oa[i] * (double)oa[i];
+        if (cb >= 0) {
+          if (dota) acc[0] += (double)ob[i] * (double)db[i];
+          if (selfdot) acc[1] += (double)ob[i] * (double)ob[i];
+        }
+      }
+    }
+  } else
   for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
     const int b = e & (B - 1), x = e >> logB, q = q0 + b;
     if (q >= Q) continue;
@@ -541,14 +598,29 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_proj(FsGeom<T> g, cplx<T>* __rest
   const int chunk = PP * Q;
   // tile index (padded for the compile-time passes)
   auto ti = [&](int c, int q, int pp) { return tpad<CT>(c * cstride + q * ldb + pp); };
-  for (int e = threadIdx.x; e < 3 * chunk; e += MGK_NT) {
-    const int c = e / chunk;
-    const int r = e - c * chunk;
-    const int pp = r / Q;
-    const int q = r - pp * Q;
-    const int pid = pid0 + pp;
-    tile[ti(c, q, pp)] =
-        (pid < npen) ? spec[((size_t)(3 * row + c) * npen + pid) * Q + q] : cmk(T(0), T(0));
+  // FP32 with compile-time pencils all inside the grid: two complex values
+  // (16 bytes) per lane and global access
+  constexpr bool VOK = CT && sizeof(T) == 4;
+  const bool vec = VOK && pid0 + PP <= npen;
+  if (vec) {
+    constexpr int QH = VOK ? NN / 2 : 1;
+    for (int e = threadIdx.x; e < 3 * PP * QH; e += MGK_NT) {
+      const int h = e % QH, r = e / QH, pp = r % PP, c = r / PP;
+      const float4 v =
+          *reinterpret_cast<const float4*>(spec + ((size_t)(3 * row + c) * npen + pid0 + pp) * Q + 2 * h);
+      tile[ti(c, 2 * h, pp)] = cmk(T(v.x), T(v.y));
+      tile[ti(c, 2 * h + 1, pp)] = cmk(T(v.z), T(v.w));
+    }
+  } else {
+    for (int e = threadIdx.x; e < 3 * chunk; e += MGK_NT) {
+      const int c = e / chunk;
+      const int r = e - c * chunk;
+      const int pp = r / Q;
+      const int q = r - pp * Q;
+      const int pid = pid0 + pp;
+      tile[ti(c, q, pp)] =
+          (pid < npen) ? spec[((size_t)(3 * row + c) * npen + pid) * Q + q] : cmk(T(0), T(0));
+    }
   }
   __syncthreads();
   const Tile tl{Q, logPP, ldb, 3, cstride};
@@ -584,6 +656,17 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_proj(FsGeom<T> g, cplx<T>* __rest
   __syncthreads();
   if constexpr (CT) tile_fft_ct<NN, BB, 3, true>(tile, tw, true);
   else tile_fft<MAXV>(tile, tl, g.plq, tw, true);
+  if (vec) {
+    constexpr int QH = VOK ? NN / 2 : 1;
+    for (int e = threadIdx.x; e < 3 * PP * QH; e += MGK_NT) {
+      const int h = e % QH, r = e / QH, pp = r % PP, c = r / PP;
+      const cplx<T> a = tile[ti(c, 2 * h, pp)];
+      const cplx<T> b = tile[ti(c, 2 * h + 1, pp)];
+      *reinterpret_cast<float4*>(spec + ((size_t)(3 * row + c) * npen + pid0 + pp) * Q + 2 * h) =
+          make_float4(float(a.x), float(a.y), float(b.x), float(b.y));
+    }
+    return;
+  }
   for (int e = threadIdx.x; e < 3 * chunk; e += MGK_NT) {
     const int c = e / chunk;
     const int r = e - c * chunk;

@@ -107,15 +107,24 @@ class KmcState:
             self.Q0 = modes_to_device(Q0)
             self.cat = catalog_to_device(catalog, d, self.planar)
         self.Q = None  # allocated on the first barrier pass that stores Q
-        soft = np.zeros(self.shape + (2,)) if soft_prop is None else np.asarray(soft_prop)[..., :2]
-        self.soft = dev.upload(np.ascontiguousarray(soft.reshape(-1, 2).T), 0)
-        tl = np.full(self.shape, -np.inf) if last_event_time is None else np.asarray(last_event_time, float)
-        self.tlast = dev.upload(tl.reshape(-1), 0)
+        f64 = dict(dtype=torch.float64, device="cuda")
+        if soft_prop is None:
+            self.soft = torch.zeros((2, self.NV), **f64)
+        else:
+            soft = np.asarray(soft_prop)[..., :2]
+            self.soft = dev.upload(np.ascontiguousarray(soft.reshape(-1, 2).T), 0)
+        if last_event_time is None:
+            self.tlast = torch.full((self.NV,), -np.inf, **f64)
+        else:
+            self.tlast = dev.upload(np.asarray(last_event_time, float).reshape(-1), 0)
         self.directional = bool(directional)
         self.prev = None
         if self.directional:
-            pv = np.zeros(self.shape + (d, d)) if prev_strain_dir is None else np.asarray(prev_strain_dir, float)
-            self.prev = dev.upload(np.ascontiguousarray(pv.reshape(-1, d * d).T), 0)
+            if prev_strain_dir is None:
+                self.prev = torch.zeros((d * d, self.NV), **f64)
+            else:
+                pv = np.asarray(prev_strain_dir, float)
+                self.prev = dev.upload(np.ascontiguousarray(pv.reshape(-1, d * d).T), 0)
         C = 6 if d == 3 else 3
         self.eps_precision = eps_precision
         self.eps_p = torch.zeros((C,) + self.shape, dtype=dev.torch_dtype(eps_precision), device="cuda")
