@@ -837,15 +837,15 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
   while (B > 1 && (size_t)(pl->P + pl->P * B) * cb > lim) B /= 2;
   pl->logBy = mgk_ilog2(B);
   pl->lds_y = (size_t)(pl->P + pl->P * B) * cb;
-  // pencil pass: C x Q x PP
-  if (pl->C * pl->Q > TILE) return MGK_ERR_ARG;
+  // pencil pass: C x Q x PP (the generic path stages at most TILE values per
+  // workgroup; longer pencils need the compile-time path below)
+  const bool gen_q = pl->C * pl->Q <= TILE;
   int PP = 1;
   const int npen = pl->nxh * pl->P;
   while (2 * PP * pl->C * pl->Q <= TILE && 2 * PP <= mgk_pow2_ceil(npen)) PP *= 2;
   while (PP > 1 && (size_t)(pl->Q + pl->C * pl->Q * PP) * cb > lim) PP /= 2;
   pl->logPP = mgk_ilog2(PP);
   pl->lds_q = (size_t)(pl->Q + pl->C * pl->Q * PP) * cb;
-  if (pl->lds_q > lim) return MGK_ERR_ARG;
   // compile-time specialised sweeps (power-of-two lengths 64..512, 4096-point tiles)
   const int CTT = ct_tile<T>();
   auto ct_ok = [](int n) { return n == 64 || n == 128 || n == 256 || n == 512; };
@@ -869,6 +869,7 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
       pl->lds_q = (size_t)(pl->Q + tpad_size(pl->C * pl->Q * pp)) * cb;
     }
   }
+  if (!pl->ctq && (!gen_q || pl->lds_q > lim)) return MGK_ERR_ARG;
   const int nqt = (pl->Q + (1 << pl->logBx) - 1) >> pl->logBx;
   pl->nblk_upd = nqt * pl->P;
   long long nr = (pl->NV + MGK_NT - 1) / MGK_NT;

@@ -65,7 +65,7 @@ def test_ls3d_parity(gpu, shape):
     assert inf["iters"] == info["iters"]
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 256), (256, 256), (512, 128)])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 256), (256, 256), (512, 128), (8, 4, 512)])
 def test_ls_specialised_sizes_parity(gpu, shape):
     # power-of-two extents 64..512 run the compile-time specialised sweeps
     from mgkmc_amd.linear_elastic_simulator import spectral_solver_3d, spectral_solver_2d
