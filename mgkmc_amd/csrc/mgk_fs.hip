@@ -691,7 +691,7 @@ struct KryState {
   double rtol, atol, bnrm2, rho, rho_prev, alpha, omega, beta, rv, ts, tt, rnorm, snorm, rhotol;
 };
 
-#define KRY_NBLK 1024
+#define KRY_NBLK 4096
 
 template <typename T>
 __global__ __launch_bounds__(MGK_NT) void k_kry_init(size_t n, const T* __restrict__ b, T* __restrict__ x,
@@ -699,7 +699,23 @@ __global__ __launch_bounds__(MGK_NT) void k_kry_init(size_t n, const T* __restri
                                                      double* __restrict__ partials) {
   __shared__ double red[MGK_NT / 64];
   double acc[1] = {0.0};
-  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT) {
+  // 16-byte vectors of VW values (the Krylov vectors are 256-byte aligned),
+  // then the tail element by element
+  constexpr int VW = vw16<T>();
+  const size_t nvec = n / VW, gs = (size_t)gridDim.x * MGK_NT, g0 = blockIdx.x * (size_t)MGK_NT + threadIdx.x;
+  for (size_t j = g0; j < nvec; j += gs) {
+    T v[VW], z[VW];
+    vload16(b + j * VW, v);
+#pragma unroll
+    for (int k = 0; k < VW; ++k) {
+      z[k] = T(0);
+      acc[0] += (double)v[k] * (double)v[k];
+    }
+    vstore16(x + j * VW, z);
+    vstore16(r + j * VW, v);
+    vstore16(rt + j * VW, v);
+  }
+  for (size_t i = nvec * VW + g0; i < n; i += gs) {
     const T v = b[i];
     x[i] = T(0);
     r[i] = v;
@@ -759,7 +775,24 @@ __global__ __launch_bounds__(MGK_NT) void k_kry_p(size_t n, const KryState* __re
   if (st->done) return;
   const int first = st->iters == 0;
   const T om = (T)st->omega, be = (T)st->beta;
-  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT) {
+  constexpr int VW = vw16<T>();
+  const size_t nvec = n / VW, gs = (size_t)gridDim.x * MGK_NT, g0 = blockIdx.x * (size_t)MGK_NT + threadIdx.x;
+  for (size_t j = g0; j < nvec; j += gs) {
+    T pv[VW], rv[VW], vv[VW];
+    vload16(r + j * VW, rv);
+    if (!first) {
+      vload16(p + j * VW, pv);
+      vload16(v + j * VW, vv);
+#pragma unroll
+      for (int k = 0; k < VW; ++k) {
+        T t = pv[k] - om * vv[k];
+        t = t * be;
+        rv[k] = t + rv[k];
+      }
+    }
+    vstore16(p + j * VW, rv);
+  }
+  for (size_t i = nvec * VW + g0; i < n; i += gs) {
     if (first) p[i] = r[i];
     else {
       T t = p[i] - om * v[i];
@@ -796,7 +829,20 @@ __global__ __launch_bounds__(MGK_NT) void k_kry_s(size_t n, const KryState* __re
   __shared__ double red[MGK_NT / 64];
   const T al = (T)st->alpha;
   double acc[1] = {0.0};
-  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT) {
+  constexpr int VW = vw16<T>();
+  const size_t nvec = n / VW, gs = (size_t)gridDim.x * MGK_NT, g0 = blockIdx.x * (size_t)MGK_NT + threadIdx.x;
+  for (size_t j = g0; j < nvec; j += gs) {
+    T rv[VW], vv[VW];
+    vload16(r + j * VW, rv);
+    vload16(v + j * VW, vv);
+#pragma unroll
+    for (int k = 0; k < VW; ++k) {
+      rv[k] = rv[k] - al * vv[k];
+      acc[0] += (double)rv[k] * (double)rv[k];
+    }
+    vstore16(r + j * VW, rv);
+  }
+  for (size_t i = nvec * VW + g0; i < n; i += gs) {
     const T s = r[i] - al * v[i];
     r[i] = s;
     acc[0] += (double)s * (double)s;
@@ -844,7 +890,34 @@ __global__ __launch_bounds__(MGK_NT) void k_kry_xr(size_t n, const KryState* __r
   const T al = (T)st->alpha, om = (T)st->omega;
   const int sc = st->sconv;
   double acc[2] = {0.0, 0.0};
-  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT) {
+  constexpr int VW = vw16<T>();
+  const size_t nvec = n / VW, gs = (size_t)gridDim.x * MGK_NT, g0 = blockIdx.x * (size_t)MGK_NT + threadIdx.x;
+  for (size_t j = g0; j < nvec; j += gs) {
+    T xv[VW], pv[VW];
+    vload16(x + j * VW, xv);
+    vload16(p + j * VW, pv);
+    if (sc) {
+#pragma unroll
+      for (int k = 0; k < VW; ++k) xv[k] = xv[k] + al * pv[k];
+    } else {
+      T sv[VW], tv[VW], rtv[VW];
+      vload16(r + j * VW, sv);
+      vload16(t + j * VW, tv);
+      vload16(rt + j * VW, rtv);
+#pragma unroll
+      for (int k = 0; k < VW; ++k) {
+        T q = xv[k] + al * pv[k];
+        xv[k] = q + om * sv[k];
+        const T rn = sv[k] - om * tv[k];
+        sv[k] = rn;
+        acc[0] += (double)rn * (double)rn;
+        acc[1] += (double)rtv[k] * (double)rn;
+      }
+      vstore16(r + j * VW, sv);
+    }
+    vstore16(x + j * VW, xv);
+  }
+  for (size_t i = nvec * VW + g0; i < n; i += gs) {
     if (sc) {
       x[i] = x[i] + al * p[i];
     } else {
@@ -907,8 +980,15 @@ template <typename T>
 __global__ __launch_bounds__(MGK_NT) void k_norm2(size_t n, const T* __restrict__ x, double* __restrict__ partials) {
   __shared__ double red[MGK_NT / 64];
   double acc[1] = {0.0};
-  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT)
-    acc[0] += (double)x[i] * (double)x[i];
+  constexpr int VW = vw16<T>();
+  const size_t nvec = n / VW, gs = (size_t)gridDim.x * MGK_NT, g0 = blockIdx.x * (size_t)MGK_NT + threadIdx.x;
+  for (size_t j = g0; j < nvec; j += gs) {
+    T v[VW];
+    vload16(x + j * VW, v);
+#pragma unroll
+    for (int k = 0; k < VW; ++k) acc[0] += (double)v[k] * (double)v[k];
+  }
+  for (size_t i = nvec * VW + g0; i < n; i += gs) acc[0] += (double)x[i] * (double)x[i];
   block_sum<1>(acc, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
 }
