@@ -51,17 +51,18 @@ __device__ __forceinline__ void xmerge_load_v(cplx<float>* tile, int q0, int Q, 
                                               const cplx<float>* __restrict__ sa,
                                               const cplx<float>* __restrict__ sb) {
   constexpr int H = BB / 2, NXH = NN / 2 + 1;
-  for (int e = threadIdx.x; e < NN * H; e += MGK_NT) {
+  // each stored half-spectrum row k is read once and gives tile rows k and
+  // NN - k (conjugate-symmetric halves)
+  for (int e = threadIdx.x; e < NXH * H; e += MGK_NT) {
     const int h = e % H, k = e / H;
-    const bool lo = k < NXH;
-    const int kk = lo ? k : NN - k;
-    const size_t idx = ((size_t)kk * P + p) * Q + q0 + 2 * h;
+    const size_t idx = ((size_t)k * P + p) * Q + q0 + 2 * h;
     const float4 A = *reinterpret_cast<const float4*>(sa + idx);
     const float4 Bv = sb ? *reinterpret_cast<const float4*>(sb + idx) : make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 z;
-    if (lo) z = make_float4(A.x - Bv.y, A.y + Bv.x, A.z - Bv.w, A.w + Bv.z);
-    else z = make_float4(A.x + Bv.y, -A.y + Bv.x, A.z + Bv.w, -A.w + Bv.z);
-    *reinterpret_cast<float4*>(tile + k * BB + 2 * h) = z;
+    *reinterpret_cast<float4*>(tile + k * BB + 2 * h) =
+        make_float4(A.x - Bv.y, A.y + Bv.x, A.z - Bv.w, A.w + Bv.z);
+    if (k > 0 && k < NN - k)
+      *reinterpret_cast<float4*>(tile + (NN - k) * BB + 2 * h) =
+          make_float4(A.x + Bv.y, -A.y + Bv.x, A.z + Bv.w, -A.w + Bv.z);
   }
   __syncthreads();
 }
@@ -86,23 +87,17 @@ template <typename T>
 __device__ __forceinline__ void xmerge_load(cplx<T>* tile, int ldb, int nx, int nxh, int logB, int q0, int Q, int P,
                                             int p, const cplx<T>* __restrict__ sa, const cplx<T>* __restrict__ sb) {
   const int B = 1 << logB;
-  for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
+  // each stored row k < nxh is read once and gives tile rows k and nx - k
+  for (int e = threadIdx.x; e < nxh * B; e += MGK_NT) {
     const int b = e & (B - 1), k = e >> logB, q = q0 + b;
-    cplx<T> z = cmk(T(0), T(0));
+    cplx<T> A = cmk(T(0), T(0)), Bv = cmk(T(0), T(0));
     if (q < Q) {
-      if (k < nxh) {
-        const size_t idx = ((size_t)k * P + p) * Q + q;
-        const cplx<T> A = sa[idx];
-        const cplx<T> Bv = sb ? sb[idx] : cmk(T(0), T(0));
-        z = cmk(A.x - Bv.y, A.y + Bv.x);
-      } else {
-        const size_t idx = ((size_t)(nx - k) * P + p) * Q + q;
-        const cplx<T> A = sa[idx];
-        const cplx<T> Bv = sb ? sb[idx] : cmk(T(0), T(0));
-        z = cmk(A.x + Bv.y, -A.y + Bv.x);
-      }
+      const size_t idx = ((size_t)k * P + p) * Q + q;
+      A = sa[idx];
+      if (sb) Bv = sb[idx];
     }
-    tile[k * ldb + b] = z;
+    tile[k * ldb + b] = cmk(A.x - Bv.y, A.y + Bv.x);
+    if (k > 0 && k < nx - k) tile[(nx - k) * ldb + b] = cmk(A.x + Bv.y, -A.y + Bv.x);
   }
   __syncthreads();
 }
