@@ -123,7 +123,7 @@ __device__ __forceinline__ void gamma2_apply(const T q0, const T q1, const cplx<
 // sweep 1: polarisation + forward x FFT
 // ---------------------------------------------------------------------------
 template <typename T, int D, int MAXV, int NN = 0, int BB = 0>
-__global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_ls_tau_xfwd(LsGeom<T> g, const T* __restrict__ eps,
+__global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_ls_tau_xfwd(LsGeom<T> g, const T* __restrict__ eps,
                                                         const T* __restrict__ epsp, const T* __restrict__ lam,
                                                         const T* __restrict__ mu, T lam0, T mu0,
                                                         cplx<T>* __restrict__ spec, const LsState* __restrict__ st) {
