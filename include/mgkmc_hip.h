@@ -105,6 +105,20 @@ int mgk_ls_solve_landau(const int64_t* shape, int precision, const void* lam, co
                         void* eps, void* sig, void* tau_buf, int warm, int max_iter, double tol, void* workspace,
                         size_t workspace_bytes, void* stream, mgk_ls_info* info);
 
+/* Landau small-strain model (2D, FP64).  Replaces spectral_solver_landau_2d, default
+ * "dbfft" path with Anderson mixing (mgkmc/linear_elastic_simulator.py:952-1228) and the
+ * stress of stress_from_strain_landau_2d (mgkmc/elasticity.py:487-638).  Fields 3
+ * components (xx, yy, xy).  plane_stress: e33 from sigma33 = 0 (Newton, the reference's
+ * global step count); e33 (NV values) holds the warm start when e33_persist and e33_have
+ * (the reference's e33_state dict) and receives the last e33.  anderson_m in 0..5;
+ * aa_buf: (2 (m + 1) + 1) * 3 * NV doubles (history and previous iterate), NULL if m = 0.
+ * tau_buf: 3 NV doubles.  Other arguments as mgk_ls_solve_landau. */
+int mgk_ls_solve_landau2d(const int64_t* shape, const double* lam, const double* mu, double lam0, double mu0,
+                          const mgk_landau_params* params, int plane_stress, const double* eps_bar,
+                          const double* eps_plastic, double* eps, double* sig, double* e33, int e33_persist,
+                          int e33_have, double* tau_buf, int anderson_m, double* aa_buf, int warm, int max_iter,
+                          double tol, void* workspace, size_t workspace_bytes, void* stream, mgk_ls_info* info);
+
 /* ---------------------------------------------------------------------------
  * KMC: barriers, rates, selection, compaction, flips.
  * Device layouts: Q, Q0 [m][v]; catalog [k][m][v] with k over 2 planar

@@ -80,6 +80,9 @@ SIGNATURES = {
                                     ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i)]),
     "mgk_ls_solve_landau": (_c_i, [_c_i64p, _c_i, _c_p, _c_p, _c_d, _c_d, ctypes.POINTER(MgkLandauParams), _c_dp, _c_p,
                                    _c_p, _c_p, _c_p, _c_i, _c_i, _c_d, _c_p, _c_sz, _c_p, ctypes.POINTER(MgkLsInfo)]),
+    "mgk_ls_solve_landau2d": (_c_i, [_c_i64p, _c_p, _c_p, _c_d, _c_d, ctypes.POINTER(MgkLandauParams), _c_i, _c_dp,
+                                     _c_p, _c_p, _c_p, _c_p, _c_i, _c_i, _c_p, _c_i, _c_p, _c_i, _c_i, _c_d, _c_p,
+                                     _c_sz, _c_p, ctypes.POINTER(MgkLsInfo)]),
     "mgk_kmc_workspace_bytes": (_c_i, [ctypes.c_longlong, ctypes.POINTER(_c_sz)]),
     "mgk_kmc_barrier_rates": (_c_i, [_c_i, ctypes.c_longlong, _c_i, _c_i, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p,
                                      _c_p, _c_p, ctypes.POINTER(MgkKmcParams), _c_p, _c_p, _c_p, _c_dp]),

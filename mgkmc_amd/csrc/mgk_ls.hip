@@ -31,6 +31,10 @@ struct LsState {
   double diff;
   double d2, e2;
   double mean[12];  // eps means [0..C), sig means [C..2C) (after the stress pass)
+  // 2D Landau: plane-stress Newton count (device max), e33 buffer state,
+  // Anderson history length / ring head / mixing fallback flag
+  int nmax, e33_valid, aa_len, aa_head, aa_plain, pad2;
+  double aa_gamma[8];
 };
 
 template <typename T>
@@ -702,10 +706,12 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xfwd_real(LsGeom<T> g, const T* _
   constexpr int VW = vw16<T>();
   constexpr bool VOK = CT && (BB % VW == 0);
   const bool vec = VOK && q0 + B <= Q && Q % VW == 0;
+  const int npair = (g.C + 1) / 2;
 #pragma unroll 1
-  for (int pr = 0; pr < 3; ++pr) {
+  for (int pr = 0; pr < npair; ++pr) {
+    const bool hasb = 2 * pr + 1 < g.C;
     const T* fa = f + (size_t)(2 * pr) * NV;
-    const T* fb = f + (size_t)(2 * pr + 1) * NV;
+    const T* fb = f + (size_t)(hasb ? 2 * pr + 1 : 2 * pr) * NV;
     if (vec) {
       constexpr int GPR = VOK ? BB / VW : 1;
       for (int e = threadIdx.x; e < nx * GPR; e += MGK_NT) {
@@ -715,7 +721,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xfwd_real(LsGeom<T> g, const T* _
         vload16(fa + v, a);
         vload16(fb + v, b);
 #pragma unroll
-        for (int i = 0; i < VW; ++i) tile[x * ldb + bg * VW + i] = cmk(a[i], b[i]);
+        for (int i = 0; i < VW; ++i) tile[x * ldb + bg * VW + i] = cmk(a[i], hasb ? b[i] : T(0));
       }
     } else {
       for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
@@ -723,7 +729,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xfwd_real(LsGeom<T> g, const T* _
         cplx<T> z = cmk(T(0), T(0));
         if (q < Q) {
           const size_t v = ((size_t)x * P + p) * Q + q;
-          z = cmk(fa[v], fb[v]);
+          z = cmk(fa[v], hasb ? fb[v] : T(0));
         }
         tile[x * ldb + b] = z;
       }
@@ -732,7 +738,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xfwd_real(LsGeom<T> g, const T* _
     if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, false);
     else tile_fft<MAXV>(tile, tl, g.plx, tw, false);
     store_half<T, NN, BB>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)(2 * pr) * cs,
-                          spec + (size_t)(2 * pr + 1) * cs);
+                          hasb ? spec + (size_t)(2 * pr + 1) * cs : (cplx<T>*)nullptr);
     __syncthreads();
   }
 }
@@ -764,6 +770,296 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_landau_stress(const T* __restrict
   block_sum<12>(acc, red);
   if (threadIdx.x == 0)
     for (int c = 0; c < 12; ++c) partials[(size_t)blockIdx.x * 12 + c] = acc[c];
+}
+
+// ---------------------------------------------------------------------------
+// 2D Landau (FP64): plane-stress e33 Newton as the reference runs it -- every
+// voxel takes the same number of steps N, the first step at which all steps
+// are below 1e-9 (at most 20) -- found by a first pass (per-voxel first
+// convergence step, device max), then the stress pass takes exactly N steps
+// and evaluates the stress at the state before the last update.
+// e33 buffer: warm start when st->e33_valid (persistent mode), else the
+// reference's initial guess -lam/(lam + 2 mu) (a11 + a22).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double l2_e33_init(const double* e33, int valid, size_t i, double a11, double a22,
+                                              double lam, double mu) {
+  return valid ? e33[i] : -lam / (lam + 2.0 * mu) * (a11 + a22);
+}
+
+__global__ __launch_bounds__(MGK_NT) void k_l2_count(const double* __restrict__ eps,
+                                                     const double* __restrict__ epsp, const double* __restrict__ lam,
+                                                     const double* __restrict__ mu, LandauPar P,
+                                                     const double* __restrict__ e33, size_t NV,
+                                                     LsState* __restrict__ st, int final_pass) {
+  if (st->done && !final_pass) return;
+  const int valid = st->e33_valid;
+  int kmax = 0;
+  bool bad = false;
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < NV; i += (size_t)gridDim.x * MGK_NT) {
+    const double a11 = eps[i] - (epsp ? epsp[i] : 0.0), a22 = eps[NV + i] - (epsp ? epsp[NV + i] : 0.0);
+    const double a12 = eps[2 * NV + i] - (epsp ? epsp[2 * NV + i] : 0.0);
+    const double l = lam[i], m = mu[i];
+    double e = l2_e33_init(e33, valid, i, a11, a22, l, m);
+    int k = 20;
+    for (int it = 1; it <= 20; ++it) {
+      const Landau2<double> sv = landau2_eval(a11, a22, a12, e, l, m, P);
+      const double d = landau2_delta(sv, e, l, m, P);
+      if (!isfinite(d)) {
+        bad = true;
+        break;
+      }
+      e = e + d;
+      if (fabs(d) < 1e-9) {
+        k = it;
+        break;
+      }
+    }
+    kmax = max(kmax, k);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(&st->nmax, kmax);
+  if (bad) {
+    atomicExch(&st->status, MGK_ERR_NONFINITE);
+    atomicExch(&st->done, 1);
+  }
+}
+
+// tau = sigma_L(eps - eps_p) - C0:eps (tau != NULL) and/or the final stress
+// with FP64 sums of eps and sigma (sig != NULL); updates e33
+__global__ __launch_bounds__(MGK_NT) void k_l2_stress(const double* __restrict__ eps,
+                                                      const double* __restrict__ epsp,
+                                                      const double* __restrict__ lam,
+                                                      const double* __restrict__ mu, double lam0, double mu0,
+                                                      LandauPar P, int plane_stress, double* __restrict__ e33,
+                                                      double* __restrict__ tau, double* __restrict__ sig,
+                                                      double* __restrict__ partials, size_t NV,
+                                                      const LsState* __restrict__ st, int final_pass) {
+  if (st->done && !final_pass) return;
+  __shared__ double red[6 * (MGK_NT / 64)];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int valid = st->e33_valid, N = st->nmax;
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < NV; i += (size_t)gridDim.x * MGK_NT) {
+    const double e11 = eps[i], e22 = eps[NV + i], e12 = eps[2 * NV + i];
+    const double a11 = e11 - (epsp ? epsp[i] : 0.0), a22 = e22 - (epsp ? epsp[NV + i] : 0.0);
+    const double a12 = e12 - (epsp ? epsp[2 * NV + i] : 0.0);
+    const double l = lam[i], m = mu[i];
+    Landau2<double> sv;
+    if (plane_stress) {
+      double e = l2_e33_init(e33, valid, i, a11, a22, l, m);
+      sv = landau2_eval(a11, a22, a12, e, l, m, P);
+      for (int it = 1; it <= N; ++it) {
+        if (it > 1) sv = landau2_eval(a11, a22, a12, e, l, m, P);
+        e = e + landau2_delta(sv, e, l, m, P);
+      }
+      e33[i] = e;
+    } else {
+      sv = landau2_eval(a11, a22, a12, 0.0, l, m, P);
+    }
+    double sg[3];
+    landau2_sigma(sv, a11, a22, a12, m, P, sg);
+    if (tau) {
+      const double l0tr = lam0 * (e11 + e22), m02 = 2.0 * mu0;
+      tau[i] = sg[0] - (l0tr + m02 * e11);
+      tau[NV + i] = sg[1] - (l0tr + m02 * e22);
+      tau[2 * NV + i] = sg[2] - m02 * e12;
+    }
+    if (sig) {
+      sig[i] = sg[0];
+      sig[NV + i] = sg[1];
+      sig[2 * NV + i] = sg[2];
+      acc[0] += e11;
+      acc[1] += e22;
+      acc[2] += e12;
+      acc[3] += sg[0];
+      acc[4] += sg[1];
+      acc[5] += sg[2];
+    }
+  }
+  if (partials) {
+    block_sum<6>(acc, red);
+    if (threadIdx.x == 0)
+      for (int c = 0; c < 6; ++c) partials[(size_t)blockIdx.x * 6 + c] = acc[c];
+  }
+}
+
+__global__ void k_l2_after_eval(LsState* __restrict__ st, int persist) {
+  if (persist) st->e33_valid = 1;
+  st->nmax = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Type-II Anderson mixing of the last m+1 iterates (the reference's dbfft path:
+// numpy lstsq on residual differences, mgkmc/linear_elastic_simulator.py:
+// 1178-1212), on the device: history ring of g = G(eps) and f = G(eps) - eps,
+// Gram sums of the residual differences (off-diagonal component weight 2, as
+// the flattened full tensor counts it), a one-thread pseudo-inverse solve, mix.
+// ---------------------------------------------------------------------------
+// Ring of nslot = m + 1 history entries, oldest first: entry a lives in slot
+// (aa_head + a) % nslot; g and f slots are n = 3 NV values apart.
+__device__ __forceinline__ size_t aa_slot(const LsState* st, int a, int nslot) {
+  return (size_t)((st->aa_head + a) % nslot);
+}
+
+// new entry (g = eps, f = eps - eps_old) at the ring's end; skipped once the
+// solve has stopped
+__global__ void k_aa_store(const double* __restrict__ eps, const double* __restrict__ eps_old, double* __restrict__ gb,
+                           double* __restrict__ fb, size_t n, int nslot, const LsState* __restrict__ st) {
+  if (st->done) return;
+  const size_t sl = st->aa_len < nslot ? aa_slot(st, st->aa_len, nslot) : (size_t)st->aa_head;
+  double* gs = gb + sl * n;
+  double* fs = fb + sl * n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double v = eps[i];
+    gs[i] = v;
+    fs[i] = v - eps_old[i];
+  }
+}
+
+__global__ void k_aa_advance(int nslot, LsState* __restrict__ st) {
+  if (st->done) return;
+  if (st->aa_len < nslot) st->aa_len += 1;
+  else st->aa_head = (st->aa_head + 1) % nslot;
+}
+
+// partials per block: G[a][b] (a <= b) of dF_a = f_{a+1} - f_a (a < L-1 <= 5),
+// then r_a = (dF_a, f_{L-1}); the off-diagonal component counts twice
+__global__ __launch_bounds__(MGK_NT) void k_aa_gram(const double* __restrict__ fb, size_t NV, int nslot,
+                                                    double* __restrict__ partials, const LsState* __restrict__ st) {
+  const int L = st->aa_len;
+  if (st->done || L < 2) return;
+  __shared__ double red[20 * (MGK_NT / 64)];
+  double acc[20];
+#pragma unroll
+  for (int k = 0; k < 20; ++k) acc[k] = 0.0;
+  const int K = L - 1;
+  const size_t n = 3 * NV;
+  const double* f[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) f[a] = fb + aa_slot(st, a < L ? a : L - 1, nslot) * n;
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT) {
+    const double wgt = i >= 2 * NV ? 2.0 : 1.0;
+    double d[5];
+    const double fl = f[K][i];
+#pragma unroll
+    for (int a = 0; a < 5; ++a) d[a] = a < K ? f[a + 1][i] - f[a][i] : 0.0;
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+#pragma unroll
+      for (int b = a; b < 5; ++b) acc[k++] += wgt * d[a] * d[b];
+#pragma unroll
+    for (int a = 0; a < 5; ++a) acc[15 + a] += wgt * d[a] * fl;
+  }
+  block_sum<20>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 20; ++k) partials[(size_t)blockIdx.x * 20 + k] = acc[k];
+}
+
+// one thread: reduce the partials, solve the K x K normal equations through a
+// Jacobi eigen-decomposition with the least-squares singular-value cutoff
+// (rcond = eps * rows), store gamma; aa_plain = 1 if it is not finite
+__global__ void k_aa_solve(const double* __restrict__ partials, int nparts, double rows, LsState* __restrict__ st) {
+  const int L = st->aa_len;
+  if (st->done || L < 2 || threadIdx.x != 0) return;
+  const int K = L - 1;
+  double acc[20];
+  for (int k = 0; k < 20; ++k) acc[k] = 0.0;
+  for (int b = 0; b < nparts; ++b)
+    for (int k = 0; k < 20; ++k) acc[k] += partials[(size_t)b * 20 + k];
+  double G[5][5], V[5][5], rhs[5];
+  int k = 0;
+  for (int a = 0; a < 5; ++a)
+    for (int b = a; b < 5; ++b) {
+      G[a][b] = G[b][a] = acc[k++];
+    }
+  for (int a = 0; a < 5; ++a) rhs[a] = acc[15 + a];
+  for (int a = 0; a < 5; ++a)
+    for (int b = 0; b < 5; ++b) V[a][b] = (a == b) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int a = 0; a < K; ++a) {
+      dia += G[a][a] * G[a][a];
+      for (int b = a + 1; b < K; ++b) off += G[a][b] * G[a][b];
+    }
+    if (off <= 1e-32 * dia) break;
+    for (int p = 0; p < K; ++p)
+      for (int q = p + 1; q < K; ++q) {
+        if (G[p][q] == 0.0) continue;
+        const double th = (G[q][q] - G[p][p]) / (2.0 * G[p][q]);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+        for (int r = 0; r < K; ++r) {
+          const double gp = G[r][p], gq = G[r][q];
+          G[r][p] = c * gp - sn * gq;
+          G[r][q] = sn * gp + c * gq;
+        }
+        for (int r = 0; r < K; ++r) {
+          const double gp = G[p][r], gq = G[q][r];
+          G[p][r] = c * gp - sn * gq;
+          G[q][r] = sn * gp + c * gq;
+        }
+        for (int r = 0; r < K; ++r) {
+          const double vp = V[r][p], vq = V[r][q];
+          V[r][p] = c * vp - sn * vq;
+          V[r][q] = sn * vp + c * vq;
+        }
+      }
+  }
+  double smax = 0.0;
+  for (int a = 0; a < K; ++a) smax = fmax(smax, sqrt(fmax(G[a][a], 0.0)));
+  const double cut = 2.220446049250313e-16 * rows * smax;
+  double gam[5] = {0, 0, 0, 0, 0};
+  for (int a = 0; a < K; ++a) {
+    const double sv = sqrt(fmax(G[a][a], 0.0));
+    if (!(sv > cut)) continue;
+    double proj = 0.0;
+    for (int r = 0; r < K; ++r) proj += V[r][a] * rhs[r];
+    proj /= G[a][a];
+    for (int r = 0; r < K; ++r) gam[r] += V[r][a] * proj;
+  }
+  bool ok = true;
+  for (int a = 0; a < K; ++a) {
+    st->aa_gamma[a] = gam[a];
+    ok = ok && isfinite(gam[a]);
+  }
+  st->aa_plain = ok ? 0 : 1;
+}
+
+// eps = g_{L-1} - sum_a gamma_a (g_{a+1} - g_a), or the plain step g_{L-1}
+__global__ void k_aa_mix(const double* __restrict__ gb, size_t n, int nslot, double* __restrict__ eps,
+                         const LsState* __restrict__ st) {
+  if (st->done) return;
+  const int L = st->aa_len, K = L - 1;
+  const bool plain = L < 2 || st->aa_plain;
+  double gam[5] = {0, 0, 0, 0, 0};
+  const double* g[6];
+  for (int a = 0; a < 6; ++a) g[a] = gb + aa_slot(st, a < L ? a : L - 1, nslot) * n;
+  for (int a = 0; a < K && a < 5; ++a) gam[a] = st->aa_gamma[a];
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double v = g[K][i];
+    if (!plain) {
+      double dg = 0.0;
+      for (int a = 0; a < K; ++a) dg += (g[a + 1][i] - g[a][i]) * gam[a];
+      v = v - dg;
+    }
+    eps[i] = v;
+  }
+}
+
+// after a plain fallback the history restarts from its newest entry
+__global__ void k_aa_finish(int nslot, LsState* __restrict__ st) {
+  if (st->done) return;
+  if (st->aa_len >= 2 && st->aa_plain) {
+    st->aa_head = (st->aa_head + st->aa_len - 1) % nslot;
+    st->aa_len = 1;
+  }
+  st->aa_plain = 0;
+}
+
+__global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, size_t n, const LsState* __restrict__ st) {
+  if (st->done) return;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = a[i];
 }
 
 __global__ void k_widen(const float* __restrict__ a, double* __restrict__ b, size_t n) {
@@ -876,7 +1172,8 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
   pl->nblk_red = (int)std::min<long long>(nr, 1024);
   pl->off_spec = 0;
   pl->off_part = mgk_align((size_t)pl->C * pl->nxh * pl->P * pl->Q * cb);
-  size_t npart = (size_t)std::max(pl->nblk_upd * 2, pl->nblk_red * 2 * pl->C);
+  // partial sums: update norms, stress means, Anderson Gram sums (20 per block)
+  size_t npart = (size_t)std::max(pl->nblk_upd * 2, pl->nblk_red * std::max(2 * pl->C, 20));
   pl->off_state = pl->off_part + mgk_align(npart * sizeof(double));
   pl->total = pl->off_state + mgk_align(sizeof(LsState));
   return MGK_OK;
@@ -910,10 +1207,25 @@ int ls_iter_hint(const void* ws, int iters) {
   return it == last.end() ? 0 : it->second;
 }
 
+// 2D Landau mode of the LS loop (FP64)
+struct L2Mode {
+  LandauPar P;
+  int plane_stress;
+  int persist, e33_have;  // e33 buffer kept between evaluations / holds a warm start now
+  double* e33;            // NV
+  double* tau;            // 3 NV
+  int aa_m;               // Anderson depth (0: plain fixed point)
+  double *aa_g, *aa_f;    // (m + 1) x 3 NV each
+  double* eps_old;        // 3 NV
+};
+
+__global__ void k_l2_init(LsState* __restrict__ st, int e33_valid) { st->e33_valid = e33_valid; }
+
 template <typename T, int D>
 int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double mu0, const double* eps_bar,
                const T* epsp, T* eps, T* sig, int warm, int max_iter, double tol, char* ws, hipStream_t stream,
-               mgk_ls_info* info, bool final_stress = true, const LandauPar* lp = nullptr, T* taubuf = nullptr) {
+               mgk_ls_info* info, bool final_stress = true, const LandauPar* lp = nullptr, T* taubuf = nullptr,
+               const L2Mode* l2 = nullptr) {
   constexpr int MV = maxv<T>();
   constexpr int TT = ct_tile<T>();  // points per compile-time tile
   constexpr int C = (D == 3) ? 6 : 3;
@@ -968,6 +1280,7 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   const size_t NV = (size_t)pl.NV;
 
   MGK_HIP(hipMemsetAsync(st, 0, sizeof(LsState), stream));
+  if (l2) k_l2_init<<<1, 1, 0, stream>>>(st, l2->persist && l2->e33_have ? 1 : 0);
   const int nfill = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 4096);
   if (!warm) {
     k_ls_fill<T><<<nfill, MGK_NT, 0, stream>>>(eps, C, NV, eb);
@@ -998,7 +1311,26 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   const int nel = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 8192);
   auto enqueue_iter = [&]() {
     int h = -1;
-    if (lp) {
+    if constexpr (sizeof(T) == 8 && D == 2) {
+      if (l2) {
+        if (l2->plane_stress) k_l2_count<<<nel, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, l2->P, l2->e33, NV, st, 0);
+        k_l2_stress<<<nel, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, lam0, mu0, l2->P, l2->plane_stress, l2->e33,
+                                                l2->tau, nullptr, nullptr, NV, st, 0);
+        k_l2_after_eval<<<1, 1, 0, stream>>>(st, l2->persist);
+        if (pl.ctx) {
+          switch (pl.nx) {
+            case 64: k_ls_xfwd_real<T, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
+            case 128: k_ls_xfwd_real<T, TT / MGK_NT, 128, TT / 128><<<gx, MGK_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
+            case 256: k_ls_xfwd_real<T, TT / MGK_NT, 256, TT / 256><<<gx, MGK_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
+            default: k_ls_xfwd_real<T, TT / MGK_NT, 512, TT / 512><<<gx, MGK_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
+          }
+        } else {
+          k_ls_xfwd_real<T, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st);
+        }
+      }
+    }
+    if (l2) {
+    } else if (lp) {
       // Landau: elementwise polarisation, then r2c of the 6-component field
       k_ls_landau_tau<T><<<nel, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, T(lam0), T(mu0), *lp, taubuf, NV, st);
       if (pl.ctx) {
@@ -1047,6 +1379,10 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
       launch_mid<T, MV>(pl, gy, stream, g.ply, g.twy, spec, 1, &st->done);
       prof.end(h);
     }
+    if constexpr (sizeof(T) == 8) {
+      if (l2 && l2->aa_m > 0)
+        k_copy<<<nel, MGK_NT, 0, stream>>>(eps, l2->eps_old, (size_t)C * NV, st);
+    }
     h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
     if (pl.ctx) {
       switch (pl.nx) {
@@ -1060,6 +1396,18 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     }
     prof.end(h);
     k_ls_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_upd, st, tol, max_iter, e2_extra);
+    if constexpr (sizeof(T) == 8) {
+      if (l2 && l2->aa_m > 0) {
+        const size_t n3 = (size_t)C * NV;
+        const int ns = l2->aa_m + 1;
+        k_aa_store<<<nel, MGK_NT, 0, stream>>>(eps, l2->eps_old, l2->aa_g, l2->aa_f, n3, ns, st);
+        k_aa_advance<<<1, 1, 0, stream>>>(ns, st);
+        k_aa_gram<<<pl.nblk_red, MGK_NT, 0, stream>>>(l2->aa_f, NV, ns, part, st);
+        k_aa_solve<<<1, 64, 0, stream>>>(part, pl.nblk_red, 4.0 * (double)NV, st);
+        k_aa_mix<<<nel, MGK_NT, 0, stream>>>(l2->aa_g, n3, ns, eps, st);
+        k_aa_finish<<<1, 1, 0, stream>>>(ns, st);
+      }
+    }
     ++iter_idx;
   };
 
@@ -1096,7 +1444,20 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     first = false;
   }
   // final stress and macro means
-  if (final_stress && lp) {
+  bool l2_done = false;
+  if constexpr (sizeof(T) == 8 && D == 2) {
+    if (final_stress && l2) {
+      if (l2->plane_stress) k_l2_count<<<nel, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, l2->P, l2->e33, NV, st, 1);
+      k_l2_stress<<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, lam0, mu0, l2->P, l2->plane_stress,
+                                                      l2->e33, nullptr, sig, part, NV, st, 1);
+      k_l2_after_eval<<<1, 1, 0, stream>>>(st, l2->persist);
+      k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, 2 * C, (double)NV, st->mean);
+      MGK_HIP(hipGetLastError());
+      l2_done = true;
+    }
+  }
+  if (l2_done) {
+  } else if (final_stress && lp) {
     k_ls_landau_stress<T><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, *lp, sig, NV, part);
     k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, 2 * C, (double)NV, st->mean);
     MGK_HIP(hipGetLastError());
@@ -1285,4 +1646,52 @@ extern "C" int mgk_ls_solve_landau(const int64_t* shape, int precision, const vo
   return ls_solve_t<float, 3>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar, (const float*)eps_plastic,
                               (float*)eps, (float*)sig, warm, max_iter, tol, (char*)workspace, s, info, true, &P,
                               (float*)tau_buf);
+}
+
+extern "C" int mgk_ls_solve_landau2d(const int64_t* shape, const double* lam, const double* mu, double lam0,
+                                     double mu0, const mgk_landau_params* lpar, int plane_stress,
+                                     const double* eps_bar, const double* eps_plastic, double* eps, double* sig,
+                                     double* e33, int e33_persist, int e33_have, double* tau_buf, int anderson_m,
+                                     double* aa_buf, int warm, int max_iter, double tol, void* workspace,
+                                     size_t workspace_bytes, void* stream, mgk_ls_info* info) {
+  if (!lam || !mu || !eps || !eps_bar || !workspace || !tau_buf || !lpar || max_iter < 1 || anderson_m < 0 ||
+      anderson_m > 5 || (plane_stress && !e33) || (anderson_m > 0 && !aa_buf)) {
+    mgk_set_error("bad argument (null pointer, max_iter < 1 or anderson_m outside 0..5)", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  LsPlan pl;
+  int rc = ls_plan<double>(2, shape, &pl);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the LS solver", __FILE__, __LINE__);
+    return rc;
+  }
+  if (workspace_bytes < pl.total) {
+    mgk_set_error("workspace too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  L2Mode m;
+  m.P.v1 = lpar->v1;
+  m.P.v2 = lpar->v2;
+  m.P.v3 = lpar->v3;
+  m.P.g1 = lpar->g1;
+  m.P.g2 = lpar->g2;
+  m.P.g3 = lpar->g3;
+  m.P.g4 = lpar->g4;
+  m.P.Ecap = lpar->cap_limit;
+  m.P.cap_p = lpar->cap_power;
+  m.P.cap_ratio = lpar->cap_ratio;
+  m.P.capping = lpar->capping;
+  m.P.cap_smooth = lpar->cap_smooth;
+  m.plane_stress = plane_stress ? 1 : 0;
+  m.persist = e33_persist ? 1 : 0;
+  m.e33_have = e33_have ? 1 : 0;
+  m.e33 = e33;
+  m.tau = tau_buf;
+  m.aa_m = anderson_m;
+  const size_t n3 = (size_t)3 * pl.NV;
+  m.aa_g = aa_buf;
+  m.aa_f = aa_buf ? aa_buf + (size_t)(anderson_m + 1) * n3 : nullptr;
+  m.eps_old = aa_buf ? aa_buf + (size_t)2 * (anderson_m + 1) * n3 : nullptr;
+  return ls_solve_t<double, 2>(pl, lam, mu, lam0, mu0, eps_bar, eps_plastic, eps, sig, warm ? 1 : 0, max_iter, tol,
+                               (char*)workspace, (hipStream_t)stream, info, true, nullptr, nullptr, &m);
 }

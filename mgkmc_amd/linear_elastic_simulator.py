@@ -381,3 +381,100 @@ def spectral_solver_landau_3d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_
     eps = dev.from_sym_soa(dev.download(solver_.eps), 3)
     sig = dev.from_sym_soa(dev.download(solver_.sig), 3)
     return eps, sig, np.array(inf.eps_mean[:9]).reshape(3, 3), np.array(inf.sig_mean[:9]).reshape(3, 3)
+
+
+class LandauSolver2D:
+    """Device-resident 2D Landau LS solver (FP64, plane strain or plane stress)."""
+
+    def __init__(self, lam, mu, plane_mode="plane_strain", anderson_m=5):
+        dev.require_gpu()
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        if lam.ndim != 2 or lam.shape != mu.shape:
+            raise ValueError("lam and mu must be 2D fields of one shape")
+        if not 0 <= int(anderson_m) <= 5:
+            raise ValueError("anderson_m must be in 0..5")
+        self.shape = lam.shape
+        NV = lam.size
+        self.plane_stress = plane_mode == "plane_stress"
+        self.anderson_m = int(anderson_m)
+        self.lam0, self.mu0 = float(lam.mean()), float(mu.mean())
+        self.mu_host = mu
+        self.lam = dev.upload(lam, FP64)
+        self.mu = dev.upload(mu, FP64)
+        t = dev.torch
+        self.eps = t.zeros((3,) + self.shape, dtype=t.float64, device="cuda")
+        self.sig = t.zeros((3,) + self.shape, dtype=t.float64, device="cuda")
+        self.tau = t.empty((3,) + self.shape, dtype=t.float64, device="cuda")
+        self.e33 = t.zeros(self.shape, dtype=t.float64, device="cuda")
+        self.e33_have = False
+        m = self.anderson_m
+        self.aa = t.empty((2 * (m + 1) + 1) * 3 * NV, dtype=t.float64, device="cuda") if m > 0 else None
+        self._shape_c = _lib.shape_arg(self.shape)
+        nb = ctypes.c_size_t()
+        _lib.check(_lib.load().mgk_ls_workspace_bytes(2, self._shape_c, FP64, ctypes.byref(nb)),
+                   "mgk_ls_workspace_bytes")
+        self.ws = t.empty(int(nb.value), dtype=t.uint8, device="cuda")
+        self.info = _lib.MgkLsInfo()
+
+    def solve(self, params, eps_bar, eps_plastic=None, warm=False, max_iter=400, tol=1e-6, persist_e33=True,
+              stream=None):
+        eb, ebp = _lib.host_dbl(np.asarray(eps_bar, dtype=np.float64).reshape(2, 2))
+        s = dev.stream_ptr() if stream is None else stream
+        rc = _lib.load().mgk_ls_solve_landau2d(
+            self._shape_c, _lib.dptr(self.lam), _lib.dptr(self.mu), self.lam0, self.mu0, ctypes.byref(params),
+            1 if self.plane_stress else 0, ebp, _lib.dptr(eps_plastic), _lib.dptr(self.eps), _lib.dptr(self.sig),
+            _lib.dptr(self.e33), 1 if persist_e33 else 0, 1 if (persist_e33 and self.e33_have) else 0,
+            _lib.dptr(self.tau), self.anderson_m, _lib.dptr(self.aa), 1 if warm else 0, int(max_iter), float(tol),
+            _lib.dptr(self.ws), self.ws.numel(), ctypes.c_void_p(s), ctypes.byref(self.info))
+        _lib.check(rc, "mgk_ls_solve_landau2d")
+        if persist_e33 and self.plane_stress:
+            self.e33_have = True
+        if rc == _lib.MGK_ERR_NONFINITE:
+            raise FloatingPointError(
+                "spectral_solver_landau_2d: non-finite update (Landau energy likely unstable at this strain; "
+                "consider strain capping or smaller load steps).")
+        return self.info
+
+
+def spectral_solver_landau_2d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_plastic=None, max_iter=400, tol=1e-6,
+                              verbose=False, pixel=1.0, plane_mode="plane_strain", Gamma=None, eps_init=None,
+                              strain_capping_enabled=False, strain_capping_limit=None,
+                              strain_capping_tangent_ratio=0.1, strain_capping_type="piecewise",
+                              strain_capping_smooth_power=1.0, solver="dbfft", e33_state=None, anderson_m=5):
+    """2D nonlinear LS with the Landau stress and Anderson mixing on the GPU
+    (mgkmc/linear_elastic_simulator.py:952-1228, "dbfft" path); e33_state
+    (dict) carries the plane-stress e33 warm start across calls as in the
+    reference.  Strains must be symmetric."""
+    if solver == "newton_cg":
+        raise NotImplementedError("the device path implements the dbfft (fixed point + Anderson) solver")
+    lam = np.asarray(lam, dtype=np.float64)
+    mu = np.asarray(mu, dtype=np.float64)
+    eps_bar = np.asarray(eps_bar, dtype=np.float64)
+    if np.any(eps_bar != eps_bar.T):
+        raise ValueError("the device Landau solver takes symmetric eps_bar")
+    sol = LandauSolver2D(lam, mu, plane_mode, anderson_m)
+    ep = None
+    if eps_plastic is not None:
+        sym, a = dev.to_sym_soa(eps_plastic, 2)
+        if np.any(a):
+            raise ValueError("the device Landau solver takes a symmetric eps_plastic")
+        ep = dev.upload(sym, FP64)
+    warm = eps_init is not None
+    if warm:
+        sym, a = dev.to_sym_soa(eps_init, 2)
+        if np.any(a):
+            raise ValueError("the device Landau solver takes a symmetric eps_init")
+        sol.eps.copy_(dev.upload(sym, FP64))
+    persist = e33_state is not None
+    if persist and e33_state.get("e33") is not None and np.shape(e33_state["e33"]) == lam.shape:
+        sol.e33.copy_(dev.upload(np.ascontiguousarray(e33_state["e33"], dtype=np.float64), FP64))
+        sol.e33_have = True
+    prm = landau_params(v1, v2, v3, g1, g2, g3, g4, mu, strain_capping_enabled, strain_capping_limit,
+                        strain_capping_tangent_ratio, strain_capping_type, strain_capping_smooth_power)
+    inf = sol.solve(prm, eps_bar, ep, warm=warm, max_iter=max_iter, tol=tol, persist_e33=persist)
+    if persist and sol.plane_stress:
+        e33_state["e33"] = dev.download(sol.e33)
+    eps = dev.from_sym_soa(dev.download(sol.eps), 2)
+    sig = dev.from_sym_soa(dev.download(sol.sig), 2)
+    return eps, sig, np.array(inf.eps_mean[:4]).reshape(2, 2), np.array(inf.sig_mean[:4]).reshape(2, 2)

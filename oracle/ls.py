@@ -417,3 +417,180 @@ def spectral_solver_landau_3d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_
     if info is not None:
         info.update(iters=it_done, diff=diff)
     return eps, sig_out, eps.mean(axis=(0, 1, 2)), sig_out.mean(axis=(0, 1, 2))
+
+
+# ---------------------------------------------------------------------------
+# Landau small-strain model, 2D (elasticity.py:487-638, the block-diagonal
+# scalar form) and its LS fixed point with Anderson mixing
+# (linear_elastic_simulator.py:952-1228, default "dbfft" path)
+# ---------------------------------------------------------------------------
+def stress_landau_2d(eps, lam, mu, v1, v2, v3, g1, g2, g3, g4, plane_mode="plane_strain",
+                     strain_capping_enabled=False, strain_capping_limit=None, strain_capping_tangent_ratio=0.1,
+                     strain_capping_type="piecewise", strain_capping_smooth_power=1.0, e33_state=None, info=None):
+    """In-plane Cauchy stress of the 3D Landau law for the block-diagonal strain
+    [[a11, a12, 0], [a21, a22, 0], [0, 0, e33]]: e33 = 0 (plane strain) or the
+    root of sigma33 = 0 by at most 20 Newton steps that stop when every voxel's
+    step is below 1e-9, warm-started from e33_state["e33"] (plane stress).  The
+    stress comes from the last evaluated state (before the final e33 update)."""
+    a11, a22, a12, a21 = eps[..., 0, 0], eps[..., 1, 1], eps[..., 0, 1], eps[..., 1, 0]
+    lam = np.asarray(lam, float)
+    mu = np.asarray(mu, float)
+    cap = strain_capping_enabled
+    if cap:
+        Ecap = cap_limit(mu, g4, strain_capping_limit)
+        Gt = strain_capping_tangent_ratio * mu
+
+    def scal(e33):
+        I1r = a11 + a22 + e33
+        m3 = I1r / 3.0
+        d11, d22, d33 = a11 - m3, a22 - m3, e33 - m3
+        if cap:
+            Eeq = np.sqrt(np.maximum(0.0, (2.0 / 3.0) * (d11 * d11 + d22 * d22 + d33 * d33 + 2.0 * a12 * a21)))
+            f = np.ones_like(Eeq)
+            nz = Eeq > 1e-12
+            if strain_capping_type == "smooth":
+                p = strain_capping_smooth_power
+                rp = np.minimum(100.0, (Eeq[nz] / Ecap) ** p)
+                f[nz] = (Ecap / Eeq[nz]) * np.tanh(rp) ** (1.0 / p)
+                w = np.tanh(np.minimum(100.0, (Eeq / Ecap) ** p)) ** 2
+            else:
+                capped = Eeq > Ecap
+                np.divide(Ecap, Eeq, out=f, where=capped)
+                w = capped.astype(float)
+            c11, c22, c12, c21, c33 = f * d11 + m3, f * d22 + m3, f * a12, f * a21, f * d33 + m3
+        else:
+            w = None
+            c11, c22, c12, c21, c33 = a11, a22, a12, a21, e33
+        I1 = c11 + c22 + c33
+        cross = c12 * c21
+        tr2 = c11 * c11 + c22 * c22 + 2.0 * cross
+        I2 = tr2 + c33 * c33
+        I3 = (c11 + c22) * (tr2 - (c11 * c22 - cross)) + c33 ** 3
+        A = lam * I1 + 0.5 * v1 * I1 ** 2 + v2 * I2 + (1.0 / 6.0) * g1 * I1 ** 3 + g2 * I1 * I2 + (4.0 / 3.0) * g3 * I3
+        B = 2.0 * (mu + v2 * I1 + 0.5 * g2 * I1 ** 2 + g4 * I2)
+        C = 4.0 * (v3 + g3 * I1)
+        return I1, I2, c11, c22, c12, c21, c33, w, A, B, C
+
+    n_newton = 0
+    if plane_mode == "plane_stress":
+        warm = e33_state.get("e33") if e33_state is not None else None
+        e33 = warm.copy() if warm is not None and warm.shape == a11.shape else -lam / (lam + 2.0 * mu) * (a11 + a22)
+        for _ in range(20):
+            I1, I2, c11, c22, c12, c21, c33, w, A, B, C = scal(e33)
+            s33 = A + B * c33 + C * c33 ** 2
+            dA = lam + v1 * I1 + 0.5 * g1 * I1 ** 2 + g2 * I2 + 2.0 * (v2 + g2 * I1) * c33 + 4.0 * g3 * c33 ** 2
+            dB = 2.0 * (v2 + g2 * I1) + 4.0 * g4 * c33
+            ds33 = dA + dB * c33 + B + 4.0 * g3 * c33 ** 2 + 2.0 * C * c33
+            if cap:
+                s33 = s33 + 2.0 * Gt * (e33 - c33) * w
+                ds33 = ds33 + 2.0 * Gt * w
+            delta = -s33 / (ds33 + 1e-12)
+            if not np.all(np.isfinite(delta)):
+                raise FloatingPointError("stress_from_strain_landau_2d: non-finite e33 Newton update")
+            e33 = e33 + delta
+            n_newton += 1
+            if np.all(np.abs(delta) < 1e-9):
+                break
+        if e33_state is not None:
+            e33_state["e33"] = e33
+    else:
+        e33 = np.zeros_like(a11)
+        I1, I2, c11, c22, c12, c21, c33, w, A, B, C = scal(e33)
+    if info is not None:
+        info["newton"] = n_newton
+    cross = c12 * c21
+    t = c11 + c22
+    sig = np.empty(eps.shape)
+    sig[..., 0, 0] = A + B * c11 + C * (c11 * c11 + cross)
+    sig[..., 1, 1] = A + B * c22 + C * (c22 * c22 + cross)
+    sig[..., 0, 1] = B * c12 + C * (c12 * t)
+    sig[..., 1, 0] = B * c21 + C * (c21 * t)
+    if cap:
+        r = 2.0 * Gt * w
+        sig[..., 0, 0] += r * (a11 - c11)
+        sig[..., 1, 1] += r * (a22 - c22)
+        sig[..., 0, 1] += r * (a12 - c12)
+        sig[..., 1, 0] += r * (a21 - c21)
+    return sig
+
+
+def spectral_solver_landau_2d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_plastic=None, max_iter=400, tol=1e-6,
+                              verbose=False, pixel=1.0, plane_mode="plane_strain", Gamma=None, eps_init=None,
+                              strain_capping_enabled=False, strain_capping_limit=None, strain_capping_tangent_ratio=0.1,
+                              strain_capping_type="piecewise", strain_capping_smooth_power=1.0, solver="dbfft",
+                              e33_state=None, anderson_m=5, info=None):
+    """2D nonlinear LS with type-II Anderson mixing of the last m iterates
+    (numpy lstsq on the residual differences; plain step and history restart
+    when the mixed iterate is not finite); reference medium (mean lam, mean mu)."""
+    lam = np.asarray(lam, float)
+    mu = np.asarray(mu, float)
+    shape = lam.shape
+    la, ma = float(lam.mean()), float(mu.mean())
+    if Gamma is None:
+        Gamma = green_operator(wave_vectors(shape, pixel), la, ma)
+    if eps_init is not None:
+        eps = eps_init.copy()
+        for i in range(2):
+            for j in range(2):
+                eps[..., i, j] += eps_bar[i, j] - eps_init[..., i, j].mean()
+    else:
+        eps = np.zeros(shape + (2, 2))
+        for i in range(2):
+            for j in range(2):
+                eps[..., i, j] = eps_bar[i, j]
+    cap = dict(plane_mode=plane_mode, strain_capping_enabled=strain_capping_enabled,
+               strain_capping_limit=strain_capping_limit, strain_capping_tangent_ratio=strain_capping_tangent_ratio,
+               strain_capping_type=strain_capping_type, strain_capping_smooth_power=strain_capping_smooth_power,
+               e33_state=e33_state)
+    aa_g, aa_f = [], []
+    it_done = 0
+    diff = np.inf
+    for it in range(max_iter):
+        e_el = eps - eps_plastic if eps_plastic is not None else eps
+        sig = stress_landau_2d(e_el, lam, mu, v1, v2, v3, g1, g2, g3, g4, **cap)
+        tr = np.trace(eps, axis1=2, axis2=3)[..., None, None]
+        tau = sig - (la * tr * np.eye(2) + 2.0 * ma * eps)
+        tau_hat = np.zeros(shape + (2, 2), dtype=complex)
+        for i in range(2):
+            for j in range(2):
+                tau_hat[..., i, j] = fft_field(tau[..., i, j])
+        et_hat = -np.einsum("xykhij,xyij->xykh", Gamma, tau_hat)
+        et = np.zeros(shape + (2, 2))
+        for i in range(2):
+            for j in range(2):
+                et[..., i, j] = ifft_field(et_hat[..., i, j])
+                et[..., i, j] -= et[..., i, j].mean()
+        eps_new = eps_bar[None, None] + et
+        diff = np.linalg.norm(eps_new - eps) / (np.linalg.norm(eps) + 1e-20)
+        it_done = it + 1
+        if not np.isfinite(diff):
+            raise FloatingPointError("spectral_solver_landau_2d: non-finite strain update")
+        if diff < tol:
+            eps = eps_new
+            break
+        if anderson_m > 0:
+            aa_g.append(eps_new.reshape(-1))
+            aa_f.append((eps_new - eps).reshape(-1))
+            if len(aa_f) > anderson_m + 1:
+                aa_g.pop(0)
+                aa_f.pop(0)
+        if anderson_m > 0 and len(aa_f) >= 2:
+            dF = np.stack([aa_f[i + 1] - aa_f[i] for i in range(len(aa_f) - 1)], axis=1)
+            dG = np.stack([aa_g[i + 1] - aa_g[i] for i in range(len(aa_g) - 1)], axis=1)
+            try:
+                gam, *_ = np.linalg.lstsq(dF, aa_f[-1], rcond=None)
+                eps_aa = aa_g[-1] - dG @ gam
+            except np.linalg.LinAlgError:
+                eps_aa = None
+            if eps_aa is not None and np.all(np.isfinite(eps_aa)):
+                eps = eps_aa.reshape(shape + (2, 2))
+            else:
+                eps = eps_new
+                aa_g, aa_f = aa_g[-1:], aa_f[-1:]
+        else:
+            eps = eps_new
+    e_el = eps - eps_plastic if eps_plastic is not None else eps
+    sig_out = stress_landau_2d(e_el, lam, mu, v1, v2, v3, g1, g2, g3, g4, **cap)
+    if info is not None:
+        info.update(iters=it_done, diff=diff)
+    return eps, sig_out, eps.mean(axis=(0, 1)), sig_out.mean(axis=(0, 1))

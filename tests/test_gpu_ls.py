@@ -321,3 +321,79 @@ def test_landau_ls_specialised_256(gpu):
     inf = s.solve(landau_params(**LANDAU_PA, mu=mu, strain_capping_enabled=True, strain_capping_limit=0.1,
                                 strain_capping_type="smooth"), eb, ep, tol=1e-6, max_iter=400)
     assert inf.status == 0 and inf.diff < 1e-6
+
+
+def _landau2d_case(seed=8, shape=(16, 12)):
+    rng = np.random.default_rng(seed)
+    E = 70e9 * rng.uniform(0.8, 1.2, shape)
+    lam, mu = ol.compute_lame_2d(E, np.full(shape, 0.3), "plane_strain")
+    g = rng.standard_normal(shape + (2, 2)) * 2e-3
+    ep = 0.5 * (g + np.swapaxes(g, -1, -2))
+    return lam, mu, ep
+
+
+@pytest.mark.parametrize("plane_mode,cap", [
+    ("plane_strain", {}),
+    ("plane_stress", {}),
+    ("plane_stress", dict(strain_capping_enabled=True, strain_capping_limit=0.03)),
+    ("plane_strain", dict(strain_capping_enabled=True, strain_capping_type="smooth", strain_capping_smooth_power=2.0)),
+])
+def test_landau2d_plain_fixed_point_parity(gpu, plane_mode, cap):
+    # anderson_m = 0 (plain fixed point): <= 1e-9 relative and identical
+    # iteration counts; the plane-stress e33 state round-trips through the dict
+    from mgkmc_amd.linear_elastic_simulator import spectral_solver_landau_2d
+    print(f"[landau2d plain] {plane_mode} {cap}", flush=True)
+    lam, mu, ep = _landau2d_case()
+    eb = np.array([[0.01, 0.02], [0.02, -0.004]])
+    st_o, st_g, info = {}, {}, {}
+    for call in range(2):
+        ref = ol.spectral_solver_landau_2d(lam, mu, **LANDAU_PA, eps_bar=eb * (1 + 0.1 * call), eps_plastic=ep,
+                                           tol=1e-8, plane_mode=plane_mode, e33_state=st_o, anderson_m=0, info=info,
+                                           **cap)
+        got = spectral_solver_landau_2d(lam, mu, **LANDAU_PA, eps_bar=eb * (1 + 0.1 * call), eps_plastic=ep,
+                                        tol=1e-8, plane_mode=plane_mode, e33_state=st_g, anderson_m=0, **cap)
+        for a, b in zip(got, ref):
+            assert _rel(a, b) < 1e-9, (call, _rel(a, b))
+        if plane_mode == "plane_stress":
+            assert _rel(st_g["e33"], st_o["e33"]) < 1e-9
+    print(f"[landau2d plain] iters (oracle) {info['iters']}", flush=True)
+
+
+@pytest.mark.parametrize("plane_mode", ["plane_strain", "plane_stress"])
+def test_landau2d_anderson_parity(gpu, plane_mode):
+    # Anderson depth 5: the least-squares mixing is solved on the device (normal
+    # equations, eigen pseudo-inverse) instead of LAPACK's SVD, so trajectories
+    # agree to round-off growth: fields <= 1e-6 relative at tol 1e-8, iteration
+    # counts within 3
+    from mgkmc_amd.linear_elastic_simulator import LandauSolver2D, landau_params
+    from mgkmc_amd import device as dev
+    lam, mu, ep = _landau2d_case(9, (24, 20))
+    eb = np.array([[0.01, 0.03], [0.03, -0.004]])
+    cap = dict(strain_capping_enabled=True, strain_capping_limit=0.03)
+    info = {}
+    ref = ol.spectral_solver_landau_2d(lam, mu, **LANDAU_PA, eps_bar=eb, eps_plastic=ep, tol=1e-8, plane_mode=plane_mode,
+                                       e33_state={}, anderson_m=5, info=info, **cap)
+    s = LandauSolver2D(lam, mu, plane_mode, anderson_m=5)
+    sym, _ = dev.to_sym_soa(ep, 2)
+    inf = s.solve(landau_params(**LANDAU_PA, mu=mu, **cap), eb, dev.upload(sym, 0), tol=1e-8)
+    eps = dev.from_sym_soa(dev.download(s.eps), 2)
+    sig = dev.from_sym_soa(dev.download(s.sig), 2)
+    print(f"[landau2d anderson] {plane_mode}: iters {inf.iters} (oracle {info['iters']}), "
+          f"eps {_rel(eps, ref[0]):.2e} sig {_rel(sig, ref[1]):.2e}", flush=True)
+    assert inf.status == 0 and abs(inf.iters - info["iters"]) <= 3
+    assert _rel(eps, ref[0]) < 1e-6 and _rel(sig, ref[1]) < 1e-6
+
+
+def test_landau2d_nonfinite_raises(gpu):
+    # plain fixed point on an unstable load: both overflow within 10 iterations.
+    # (With Anderson mixing the same load wanders for ~100 iterations before it
+    # overflows in the oracle: round-off differences then decide the outcome.)
+    from mgkmc_amd.linear_elastic_simulator import spectral_solver_landau_2d
+    lam, mu, _ = _landau2d_case()
+    eb = np.array([[0.01, 0.3], [0.3, -0.004]])
+    with pytest.raises(FloatingPointError):
+        ol.spectral_solver_landau_2d(lam, mu, **LANDAU_PA, eps_bar=eb, tol=1e-8, plane_mode="plane_stress",
+                                     anderson_m=0)
+    with pytest.raises(FloatingPointError):
+        spectral_solver_landau_2d(lam, mu, **LANDAU_PA, eps_bar=eb, tol=1e-8, plane_mode="plane_stress",
+                                  anderson_m=0)

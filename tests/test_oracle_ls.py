@@ -215,3 +215,47 @@ def test_landau_ls_linear_limit_and_capping():
     dev = e - np.trace(e) / 3 * np.eye(3)
     assert abs(np.sqrt(2.0 / 3.0 * np.sum(dev * dev)) - 0.1) < 1e-14 and w == 1.0
     assert abs(np.trace(e) - np.trace(big)) < 1e-15
+
+
+@pytest.mark.parametrize("cap", [{}, dict(strain_capping_enabled=True, strain_capping_limit=0.03),
+                                 dict(strain_capping_enabled=True, strain_capping_type="smooth",
+                                      strain_capping_smooth_power=2.0)])
+def test_landau_2d_stress_against_3d_law(cap):
+    # plane strain: the 2D block-diagonal form equals the 3D tensor law at e33 = 0;
+    # plane stress: the Newton e33 makes the 3D law's sigma33 vanish, and the
+    # in-plane stress equals the 3D law at that e33 (to the Newton step tolerance)
+    rng = np.random.default_rng(6)
+    shape = (5, 4)
+    g = rng.standard_normal(shape + (2, 2)) * 0.01
+    e2 = 0.5 * (g + np.swapaxes(g, -1, -2))
+    lam = 60e9 * rng.uniform(0.9, 1.1, shape)
+    mu = 27e9 * rng.uniform(0.9, 1.1, shape)
+    s2 = ol.stress_landau_2d(e2, lam, mu, **LANDAU_PA, **cap)
+    e3 = np.zeros(shape + (3, 3))
+    e3[..., :2, :2] = e2
+    s3 = ol.stress_landau_3d(e3, lam, mu, **LANDAU_PA, **cap)
+    assert np.abs(s2 - s3[..., :2, :2]).max() < 1e-9 * np.abs(s3).max()
+    st = {}
+    s2p = ol.stress_landau_2d(e2, lam, mu, **LANDAU_PA, plane_mode="plane_stress", e33_state=st, **cap)
+    e3[..., 2, 2] = st["e33"]
+    s3p = ol.stress_landau_3d(e3, lam, mu, **LANDAU_PA, **cap)
+    assert np.abs(s3p[..., 2, 2]).max() < 1e-6 * np.abs(s3p).max()
+    assert np.abs(s2p - s3p[..., :2, :2]).max() < 1e-6 * np.abs(s3p).max()
+
+
+def test_landau_2d_anderson_reaches_the_picard_fixed_point():
+    shape = (12, 10)
+    rng = np.random.default_rng(8)
+    E = 70e9 * rng.uniform(0.8, 1.2, shape)
+    lam, mu = ol.compute_lame_2d(E, np.full(shape, 0.3), "plane_strain")
+    g = rng.standard_normal(shape + (2, 2)) * 2e-3
+    ep = 0.5 * (g + np.swapaxes(g, -1, -2))
+    eb = np.array([[0.01, 0.03], [0.03, -0.004]])
+    kw = dict(eps_plastic=ep, tol=1e-10, plane_mode="plane_stress", strain_capping_enabled=True,
+              strain_capping_limit=0.03)
+    ia, ip = {}, {}
+    a = ol.spectral_solver_landau_2d(lam, mu, **LANDAU_PA, eps_bar=eb, anderson_m=5, info=ia, e33_state={}, **kw)
+    b = ol.spectral_solver_landau_2d(lam, mu, **LANDAU_PA, eps_bar=eb, anderson_m=0, info=ip, e33_state={}, **kw)
+    assert ia["iters"] < ip["iters"]
+    assert np.abs(a[0] - b[0]).max() < 1e-8 * np.abs(b[0]).max()
+    assert np.abs(a[1] - b[1]).max() < 1e-7 * np.abs(b[1]).max()
