@@ -301,17 +301,17 @@ __device__ __forceinline__ int tpad(int i) {
 }
 constexpr int tpad_size(int n) { return n + n / 16 + 1; }
 
-template <int N, int B, int C, int R, int L, bool PAD, typename T>
+template <int N, int B, int C, int R, int L, bool PAD, int NT, typename T>
 __device__ __forceinline__ void ct_pass(cplx<T>* __restrict__ buf, const cplx<T>* __restrict__ tw, bool inv) {
   constexpr int M = N / R;
   constexpr int NB = C * M * B;
-  constexpr int S = (NB + MGK_NT - 1) / MGK_NT;
+  constexpr int S = (NB + NT - 1) / NT;
   constexpr int TS = N / (L * R);
   cplx<T> v[S][R];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    const int t = threadIdx.x + s * MGK_NT;
-    if (NB % MGK_NT == 0 || t < NB) {
+    const int t = threadIdx.x + s * NT;
+    if (NB % NT == 0 || t < NB) {
       const int b = t % B, u = t / B, j = u % M, c = u / M;
       const int base = c * (N * B) + b;
 #pragma unroll
@@ -321,8 +321,8 @@ __device__ __forceinline__ void ct_pass(cplx<T>* __restrict__ buf, const cplx<T>
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    const int t = threadIdx.x + s * MGK_NT;
-    if (NB % MGK_NT == 0 || t < NB) {
+    const int t = threadIdx.x + s * NT;
+    if (NB % NT == 0 || t < NB) {
       const int b = t % B, u = t / B, j = u % M, c = u / M;
       const int k = j % L;
       if (L > 1) {
@@ -343,19 +343,19 @@ __device__ __forceinline__ void ct_pass(cplx<T>* __restrict__ buf, const cplx<T>
   __syncthreads();
 }
 
-template <int N, int B, int C, int L, bool PAD, typename T>
+template <int N, int B, int C, int L, bool PAD, int NT, typename T>
 __device__ __forceinline__ void ct_fft_rec(cplx<T>* __restrict__ buf, const cplx<T>* __restrict__ tw, bool inv) {
   if constexpr (L < N) {
     constexpr int R = ct_radix(N, L);
-    ct_pass<N, B, C, R, L, PAD>(buf, tw, inv);
-    ct_fft_rec<N, B, C, L * R, PAD>(buf, tw, inv);
+    ct_pass<N, B, C, R, L, PAD, NT>(buf, tw, inv);
+    ct_fft_rec<N, B, C, L * R, PAD, NT>(buf, tw, inv);
   }
 }
 
-// element (c, j, b) at buf[tpad<PAD>(c*N*B + j*B + b)]
-template <int N, int B, int C, bool PAD = false, typename T>
+// element (c, j, b) at buf[tpad<PAD>(c*N*B + j*B + b)]; NT threads in the workgroup
+template <int N, int B, int C, bool PAD = false, int NT = MGK_NT, typename T>
 __device__ __forceinline__ void tile_fft_ct(cplx<T>* __restrict__ buf, const cplx<T>* __restrict__ tw, bool inv) {
-  ct_fft_rec<N, B, C, 1, PAD>(buf, tw, inv);
+  ct_fft_rec<N, B, C, 1, PAD, NT>(buf, tw, inv);
 }
 
 // ---------------------------------------------------------------------------

@@ -106,8 +106,8 @@ __device__ __forceinline__ void xmerge_load(cplx<T>* tile, int ldb, int nx, int 
 // sweep 2/4: complex FFT along the middle axis (length P, stride Q)
 // grid (ceil(Q/B), nxh, C)
 // ---------------------------------------------------------------------------
-template <typename T, int MAXV, int NN = 0, int BB = 0>
-__global__ __launch_bounds__(MGK_NT) void k_fft_mid(int P, int Q, int nxh, int logB, FftPlan pl,
+template <typename T, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
+__global__ __launch_bounds__(NTH) void k_fft_mid(int P, int Q, int nxh, int logB, FftPlan pl,
                                                     const cplx<T>* __restrict__ twg, cplx<T>* __restrict__ spec,
                                                     int inv, const int* __restrict__ done) {
   if (done && *done) return;
@@ -122,14 +122,14 @@ __global__ __launch_bounds__(MGK_NT) void k_fft_mid(int P, int Q, int nxh, int l
   const int B = CT ? BB : 1 << logB, ldb = B;
   const int q0 = blockIdx.x * B, kx = blockIdx.y, c = blockIdx.z;
   cplx<T>* base = spec + ((size_t)c * nxh + kx) * (size_t)P * Q;
-  for (int t = threadIdx.x; t < P; t += MGK_NT) tw[t] = twg[t];
+  for (int t = threadIdx.x; t < P; t += NTH) tw[t] = twg[t];
   // FP32 with compile-time tiles: two complex values (16 bytes) per lane and
   // access, so one wave instruction moves 1 KB instead of 512 B
   constexpr bool VEC = CT && sizeof(T) == 4 && (BB % 2 == 0);
   if constexpr (VEC) {
     constexpr int H = BB / 2;
     const bool full = q0 + BB <= Q && (Q & 1) == 0;  // 16-byte aligned rows
-    for (int e = threadIdx.x; e < NN * H; e += MGK_NT) {
+    for (int e = threadIdx.x; e < NN * H; e += NTH) {
       const int h = e % H, y = e / H, q = q0 + 2 * h;
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
       if (full) v = *reinterpret_cast<const float4*>(base + (size_t)y * Q + q);
@@ -146,18 +146,18 @@ __global__ __launch_bounds__(MGK_NT) void k_fft_mid(int P, int Q, int nxh, int l
       *reinterpret_cast<float4*>(tile + y * ldb + 2 * h) = v;
     }
   } else {
-    for (int e = threadIdx.x; e < P * B; e += MGK_NT) {
+    for (int e = threadIdx.x; e < P * B; e += NTH) {
       const int b = e & (B - 1), y = e >> logB, q = q0 + b;
       tile[y * ldb + b] = (q < Q) ? base[(size_t)y * Q + q] : cmk(T(0), T(0));
     }
   }
   __syncthreads();
-  if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, inv != 0);
+  if constexpr (CT) tile_fft_ct<NN, BB, 1, false, NTH>(tile, tw, inv != 0);
   else tile_fft<MAXV>(tile, Tile{P, logB, ldb, 1, 0}, pl, tw, inv != 0);
   if constexpr (VEC) {
     constexpr int H = BB / 2;
     const bool full = q0 + BB <= Q && (Q & 1) == 0;  // 16-byte aligned rows
-    for (int e = threadIdx.x; e < NN * H; e += MGK_NT) {
+    for (int e = threadIdx.x; e < NN * H; e += NTH) {
       const int h = e % H, y = e / H, q = q0 + 2 * h;
       const float4 v = *reinterpret_cast<const float4*>(tile + y * ldb + 2 * h);
       if (full) *reinterpret_cast<float4*>(base + (size_t)y * Q + q) = v;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(MGK_NT) void k_fft_mid(int P, int Q, int nxh, int l
       }
     }
   } else {
-    for (int e = threadIdx.x; e < P * B; e += MGK_NT) {
+    for (int e = threadIdx.x; e < P * B; e += NTH) {
       const int b = e & (B - 1), y = e >> logB, q = q0 + b;
       if (q < Q) base[(size_t)y * Q + q] = tile[y * ldb + b];
     }

@@ -1159,10 +1159,10 @@ void fs_launch_mid(const FsPlan& pl, hipStream_t s, const FftPlan& ply, const cp
   const dim3 gy(pl.nqy, pl.nxh, 9);
   if (pl.cty) {
     switch (pl.P) {
-      case 64: k_fft_mid<T, ct_tile<T>() / MGK_NT, 64, ct_tile<T>() / 64><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
-      case 128: k_fft_mid<T, ct_tile<T>() / MGK_NT, 128, ct_tile<T>() / 128><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
-      case 256: k_fft_mid<T, ct_tile<T>() / MGK_NT, 256, ct_tile<T>() / 256><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
-      default: k_fft_mid<T, ct_tile<T>() / MGK_NT, 512, ct_tile<T>() / 512><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+      case 64: k_fft_mid<T, ct_tile<T>() / MGK_NT, 64, ct_tile<T>() / 64, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+      case 128: k_fft_mid<T, ct_tile<T>() / MGK_NT, 128, ct_tile<T>() / 128, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+      case 256: k_fft_mid<T, ct_tile<T>() / MGK_NT, 256, ct_tile<T>() / 256, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+      default: k_fft_mid<T, ct_tile<T>() / MGK_NT, 512, ct_tile<T>() / 512, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
     }
   } else {
     k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip);

@@ -314,8 +314,8 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
 // sweep 3: pencil FFT along the contiguous axis + Green operator + inverse
 // grid ceil(npencil/PP); pencil id = kx*P + p
 // ---------------------------------------------------------------------------
-template <typename T, int D, int MAXV, int NN = 0, int BB = 0>
-__global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __restrict__ spec, T A2, T Bc, T scale,
+template <typename T, int D, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
+__global__ __launch_bounds__(NTH) void k_ls_pencil(LsGeom<T> g, cplx<T>* __restrict__ spec, T A2, T Bc, T scale,
                                                       const LsState* __restrict__ st) {
   if (st->done) return;
   constexpr int C = (D == 3) ? 6 : 3;
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
   const int ldb = PP, cstride = Q * PP;
   const int pid0 = blockIdx.x * PP;
   const int npen = g.npencil;
-  for (int t = threadIdx.x; t < Q; t += MGK_NT) tw[t] = g.twq[t];
+  for (int t = threadIdx.x; t < Q; t += NTH) tw[t] = g.twq[t];
   const int chunk = PP * Q;
   // tile index (padded for the compile-time passes)
   auto ti = [&](int c, int q, int pp) { return tpad<CT>(c * cstride + q * ldb + pp); };
@@ -337,14 +337,14 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
   const bool vec = VOK && pid0 + PP <= npen;
   if (vec) {
     constexpr int QH = VOK ? NN / 2 : 1;
-    for (int e = threadIdx.x; e < C * PP * QH; e += MGK_NT) {
+    for (int e = threadIdx.x; e < C * PP * QH; e += NTH) {
       const int h = e % QH, r = e / QH, pp = r % PP, c = r / PP;
       const float4 v = *reinterpret_cast<const float4*>(spec + ((size_t)c * npen + pid0 + pp) * Q + 2 * h);
       tile[ti(c, 2 * h, pp)] = cmk(T(v.x), T(v.y));
       tile[ti(c, 2 * h + 1, pp)] = cmk(T(v.z), T(v.w));
     }
   } else {
-    for (int e = threadIdx.x; e < C * chunk; e += MGK_NT) {
+    for (int e = threadIdx.x; e < C * chunk; e += NTH) {
       const int c = e / chunk;
       const int r = e - c * chunk;
       const int pp = r / Q;
@@ -356,10 +356,10 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
   }
   __syncthreads();
   const Tile tl{Q, logPP, ldb, C, cstride};
-  if constexpr (CT) tile_fft_ct<NN, BB, C, true>(tile, tw, false);
+  if constexpr (CT) tile_fft_ct<NN, BB, C, true, NTH>(tile, tw, false);
   else tile_fft<MAXV>(tile, tl, g.plq, tw, false);
 
-  for (int e = threadIdx.x; e < Q * PP; e += MGK_NT) {
+  for (int e = threadIdx.x; e < Q * PP; e += NTH) {
     const int pp = e & (PP - 1), q = e >> logPP;
     const int pid = pid0 + pp;
     if (pid >= npen) continue;
@@ -405,11 +405,11 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
     for (int c = 0; c < C; ++c) tile[ti(c, q, pp)] = o[c];
   }
   __syncthreads();
-  if constexpr (CT) tile_fft_ct<NN, BB, C, true>(tile, tw, true);
+  if constexpr (CT) tile_fft_ct<NN, BB, C, true, NTH>(tile, tw, true);
   else tile_fft<MAXV>(tile, tl, g.plq, tw, true);
   if (vec) {
     constexpr int QH = VOK ? NN / 2 : 1;
-    for (int e = threadIdx.x; e < C * PP * QH; e += MGK_NT) {
+    for (int e = threadIdx.x; e < C * PP * QH; e += NTH) {
       const int h = e % QH, r = e / QH, pp = r % PP, c = r / PP;
       const cplx<T> a = tile[ti(c, 2 * h, pp)];
       const cplx<T> b = tile[ti(c, 2 * h + 1, pp)];
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pencil(LsGeom<T> g, cplx<T>* __re
           make_float4(float(a.x), float(a.y), float(b.x), float(b.y));
     }
   } else {
-    for (int e = threadIdx.x; e < C * chunk; e += MGK_NT) {
+    for (int e = threadIdx.x; e < C * chunk; e += NTH) {
       const int c = e / chunk;
       const int r = e - c * chunk;
       const int pp = r / Q;
@@ -1179,15 +1179,19 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
   return MGK_OK;
 }
 
+// threads per workgroup of the compile-time mid-axis sweep (its tiles are
+// LDS-bound to four per CU, so wider workgroups give more waves per CU)
+constexpr int MID_NT = 512;
+
 template <typename T, int MV>
 void launch_mid(const LsPlan& pl, dim3 gy, hipStream_t stream, const FftPlan& ply, const cplx<T>* twy, cplx<T>* spec,
                 int inv, const int* done) {
   if (pl.cty) {
     switch (pl.P) {
-      case 64: k_fft_mid<T, ct_tile<T>() / MGK_NT, 64, ct_tile<T>() / 64><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
-      case 128: k_fft_mid<T, ct_tile<T>() / MGK_NT, 128, ct_tile<T>() / 128><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
-      case 256: k_fft_mid<T, ct_tile<T>() / MGK_NT, 256, ct_tile<T>() / 256><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
-      default: k_fft_mid<T, ct_tile<T>() / MGK_NT, 512, ct_tile<T>() / 512><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+      case 64: k_fft_mid<T, ct_tile<T>() / MGK_NT, 64, ct_tile<T>() / 64, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+      case 128: k_fft_mid<T, ct_tile<T>() / MGK_NT, 128, ct_tile<T>() / 128, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+      case 256: k_fft_mid<T, ct_tile<T>() / MGK_NT, 256, ct_tile<T>() / 256, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+      default: k_fft_mid<T, ct_tile<T>() / MGK_NT, 512, ct_tile<T>() / 512, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
     }
   } else {
     k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done);
