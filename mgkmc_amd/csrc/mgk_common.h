@@ -384,7 +384,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Sums NV values across the block; result valid in thread 0.  red: LDS of
 // NV * (MGK_NT/64) doubles.
-template <int NV>
+template <int NV, int NT = MGK_NT>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -392,14 +392,14 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
   __syncthreads();
   if (lane == 0) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) red[i * (MGK_NT / 64) + w] = v[i];
+    for (int i = 0; i < NV; ++i) red[i * (NT / 64) + w] = v[i];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       double s = 0.0;
-      for (int k = 0; k < MGK_NT / 64; ++k) s += red[i * (MGK_NT / 64) + k];
+      for (int k = 0; k < NT / 64; ++k) s += red[i * (NT / 64) + k];
       v[i] = s;
     }
   }

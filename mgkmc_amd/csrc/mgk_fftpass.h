@@ -9,11 +9,11 @@
 // ---------------------------------------------------------------------------
 // After a forward FFT of z = a + i b along x (tile rows), split into the half
 // spectra of a and b and store them.
-template <typename T>
+template <typename T, int NT = MGK_NT>
 __device__ __forceinline__ void xsplit_store(const cplx<T>* tile, int ldb, int nx, int nxh, int logB, int q0, int Q,
                                              int P, int p, cplx<T>* __restrict__ sa, cplx<T>* __restrict__ sb) {
   const int B = 1 << logB;
-  for (int e = threadIdx.x; e < nxh * B; e += MGK_NT) {
+  for (int e = threadIdx.x; e < nxh * B; e += NT) {
     const int b = e & (B - 1), k = e >> logB, q = q0 + b;
     if (q >= Q) continue;
     const cplx<T> z1 = tile[k * ldb + b];
@@ -28,11 +28,11 @@ __device__ __forceinline__ void xsplit_store(const cplx<T>* tile, int ldb, int n
 // Same, FP32 with a compile-time tile (nx = NN, B = BB columns) whose columns
 // are all inside the grid and whose rows start 16-byte aligned: two complex
 // values (16 bytes) per lane and access.
-template <int NN, int BB>
+template <int NN, int BB, int NT = MGK_NT>
 __device__ __forceinline__ void xsplit_store_v(const cplx<float>* tile, int q0, int Q, int P, int p,
                                                cplx<float>* __restrict__ sa, cplx<float>* __restrict__ sb) {
   constexpr int H = BB / 2, NXH = NN / 2 + 1;
-  for (int e = threadIdx.x; e < NXH * H; e += MGK_NT) {
+  for (int e = threadIdx.x; e < NXH * H; e += NT) {
     const int h = e % H, k = e / H;
     const int kr = (k == 0) ? 0 : NN - k;
     const float4 z1 = *reinterpret_cast<const float4*>(tile + k * BB + 2 * h);
@@ -46,14 +46,14 @@ __device__ __forceinline__ void xsplit_store_v(const cplx<float>* tile, int q0, 
   }
 }
 
-template <int NN, int BB>
+template <int NN, int BB, int NT = MGK_NT>
 __device__ __forceinline__ void xmerge_load_v(cplx<float>* tile, int q0, int Q, int P, int p,
                                               const cplx<float>* __restrict__ sa,
                                               const cplx<float>* __restrict__ sb) {
   constexpr int H = BB / 2, NXH = NN / 2 + 1;
   // each stored half-spectrum row k is read once and gives tile rows k and
   // NN - k (conjugate-symmetric halves)
-  for (int e = threadIdx.x; e < NXH * H; e += MGK_NT) {
+  for (int e = threadIdx.x; e < NXH * H; e += NT) {
     const int h = e % H, k = e / H;
     const size_t idx = ((size_t)k * P + p) * Q + q0 + 2 * h;
     const float4 A = *reinterpret_cast<const float4*>(sa + idx);
@@ -69,26 +69,26 @@ __device__ __forceinline__ void xmerge_load_v(cplx<float>* tile, int q0, int Q, 
 
 // xsplit_store, through the vector form when the tile allows it (vec: every
 // column inside the grid, rows 16-byte aligned).
-template <typename T, int NN, int BB>
+template <typename T, int NN, int BB, int NT = MGK_NT>
 __device__ __forceinline__ void store_half(const cplx<T>* tile, bool vec, int ldb, int nx, int nxh, int logB, int q0,
                                            int Q, int P, int p, cplx<T>* __restrict__ sa,
                                            cplx<T>* __restrict__ sb) {
   if constexpr (sizeof(T) == 4 && NN > 0 && BB % 2 == 0) {
     if (vec) {
-      xsplit_store_v<NN, BB>(tile, q0, Q, P, p, sa, sb);
+      xsplit_store_v<NN, BB, NT>(tile, q0, Q, P, p, sa, sb);
       return;
     }
   }
-  xsplit_store(tile, ldb, nx, nxh, logB, q0, Q, P, p, sa, sb);
+  xsplit_store<T, NT>(tile, ldb, nx, nxh, logB, q0, Q, P, p, sa, sb);
 }
 
 // Build the full x-spectrum of z = a + i b from half spectra of a and b.
-template <typename T>
+template <typename T, int NT = MGK_NT>
 __device__ __forceinline__ void xmerge_load(cplx<T>* tile, int ldb, int nx, int nxh, int logB, int q0, int Q, int P,
                                             int p, const cplx<T>* __restrict__ sa, const cplx<T>* __restrict__ sb) {
   const int B = 1 << logB;
   // each stored row k < nxh is read once and gives tile rows k and nx - k
-  for (int e = threadIdx.x; e < nxh * B; e += MGK_NT) {
+  for (int e = threadIdx.x; e < nxh * B; e += NT) {
     const int b = e & (B - 1), k = e >> logB, q = q0 + b;
     cplx<T> A = cmk(T(0), T(0)), Bv = cmk(T(0), T(0));
     if (q < Q) {

@@ -126,8 +126,8 @@ __device__ __forceinline__ void gamma2_apply(const T q0, const T q1, const cplx<
 // ---------------------------------------------------------------------------
 // sweep 1: polarisation + forward x FFT
 // ---------------------------------------------------------------------------
-template <typename T, int D, int MAXV, int NN = 0, int BB = 0>
-__global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_ls_tau_xfwd(LsGeom<T> g, const T* __restrict__ eps,
+template <typename T, int D, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 ? 6 : (sizeof(T) == 4 ? 3 : 2)))) void k_ls_tau_xfwd(LsGeom<T> g, const T* __restrict__ eps,
                                                         const T* __restrict__ epsp, const T* __restrict__ lam,
                                                         const T* __restrict__ mu, T lam0, T mu0,
                                                         cplx<T>* __restrict__ spec, const LsState* __restrict__ st) {
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
   const int q0 = (tid_lin % gridDim.x) * B, p = tid_lin / gridDim.x;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;  // spectrum component stride
-  for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
+  for (int t = threadIdx.x; t < nx; t += NTH) tw[t] = g.twx[t];
   const Tile tl{nx, logB, ldb, 1, 0};
   const T m02 = T(2) * mu0;
 
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
     // pass 1: diagonal components, traces; (xx, yy) into the tile
 #pragma unroll
     for (int s = 0; s < SV; ++s) {
-      const int e = threadIdx.x + s * MGK_NT;
+      const int e = threadIdx.x + s * NTH;
       const int bg = e % GPR, x = e / GPR;
       const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
       T exx[VW], eyy[VW], ezz[VW], pxx[VW], pyy[VW], pzz[VW], l[VW], m[VW];
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
     for (int s = 0; s < MAXV; ++s) {
       keep[s] = T(0);
       mus[s] = T(0);
-      const int e = threadIdx.x + s * MGK_NT;
+      const int e = threadIdx.x + s * NTH;
       if (e < nx * B) {
         const int b = e & (B - 1), x = e >> logB, q = q0 + b;
         T txx = T(0), tyy = T(0);
@@ -236,9 +236,9 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
     }
   }
   __syncthreads();
-  if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, false);
+  if constexpr (CT) tile_fft_ct<NN, BB, 1, false, NTH>(tile, tw, false);
   else tile_fft<MAXV>(tile, tl, g.plx, tw, false);
-  store_half<T, NN, BB>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec, spec + cs);
+  store_half<T, NN, BB, NTH>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec, spec + cs);
   __syncthreads();
 
   // remaining pairs: 3D (zz, yz), (xz, xy); 2D (xy, -)
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
       constexpr int SV = MAXV / VW;
 #pragma unroll
       for (int s = 0; s < SV; ++s) {
-        const int e = threadIdx.x + s * MGK_NT;
+        const int e = threadIdx.x + s * NTH;
         const int bg = e % GPR, x = e / GPR;
         const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
         T ea[VW], pa[VW], eb[VW], pb[VW];
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
     } else {
 #pragma unroll
       for (int s = 0; s < MAXV; ++s) {
-        const int e = threadIdx.x + s * MGK_NT;
+        const int e = threadIdx.x + s * NTH;
         if (e < nx * B) {
           const int b = e & (B - 1), x = e >> logB, q = q0 + b;
           T ta = T(0), tb = T(0);
@@ -302,9 +302,9 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
       }
     }
     __syncthreads();
-    if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, false);
+    if constexpr (CT) tile_fft_ct<NN, BB, 1, false, NTH>(tile, tw, false);
     else tile_fft<MAXV>(tile, tl, g.plx, tw, false);
-    store_half<T, NN, BB>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
+    store_half<T, NN, BB, NTH>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
                           cb >= 0 ? spec + (size_t)cb * cs : (cplx<T>*)nullptr);
     __syncthreads();
   }
@@ -431,8 +431,8 @@ __global__ __launch_bounds__(NTH) void k_ls_pencil(LsGeom<T> g, cplx<T>* __restr
 // ---------------------------------------------------------------------------
 // sweep 5: inverse x FFT (complex-to-real) + update + FP64 partial norms
 // ---------------------------------------------------------------------------
-template <typename T, int D, int MAXV, int NN = 0, int BB = 0>
-__global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_ls_xinv_update(LsGeom<T> g, const cplx<T>* __restrict__ spec,
+template <typename T, int D, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 ? 6 : (sizeof(T) == 4 ? 4 : 2)))) void k_ls_xinv_update(LsGeom<T> g, const cplx<T>* __restrict__ spec,
                                                            T* __restrict__ eps, Sym6 ebar,
                                                            double* __restrict__ partials,
                                                            const LsState* __restrict__ st) {
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
   const int q0 = (tid_lin % gridDim.x) * B, p = tid_lin / gridDim.x;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
-  for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
+  for (int t = threadIdx.x; t < nx; t += NTH) tw[t] = g.twx[t];
   const Tile tl{nx, logB, ldb, 1, 0};
   double acc[2] = {0.0, 0.0};
   constexpr int VW = vw16<T>();
@@ -463,12 +463,12 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
     bool merged = false;
     if constexpr (sizeof(T) == 4 && CT && BB % 2 == 0) {
       if (vec) {
-        xmerge_load_v<NN, BB>(tile, q0, Q, P, p, sa, sb);
+        xmerge_load_v<NN, BB, NTH>(tile, q0, Q, P, p, sa, sb);
         merged = true;
       }
     }
-    if (!merged) xmerge_load(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, sa, sb);
-    if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, true);
+    if (!merged) xmerge_load<T, NTH>(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, sa, sb);
+    if constexpr (CT) tile_fft_ct<NN, BB, 1, false, NTH>(tile, tw, true);
     else tile_fft<MAXV>(tile, tl, g.plx, tw, true);
     const double wa = (D == 3) ? (ca >= 3 ? 2.0 : 1.0) : (ca >= 2 ? 2.0 : 1.0);
     const double wb = (D == 3) ? (cb >= 3 ? 2.0 : 1.0) : (cb >= 2 ? 2.0 : 1.0);
@@ -479,7 +479,7 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
       constexpr int SV = MAXV / VW;
 #pragma unroll
       for (int s = 0; s < SV; ++s) {
-        const int e = threadIdx.x + s * MGK_NT;
+        const int e = threadIdx.x + s * NTH;
         const int bg = e % GPR, x = e / GPR;
         const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
         T oa[VW], na[VW], ob[VW], nb[VW];
@@ -507,7 +507,7 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
     }
 #pragma unroll
     for (int s = 0; s < MAXV; ++s) {
-      const int e = threadIdx.x + s * MGK_NT;
+      const int e = threadIdx.x + s * NTH;
       if (e < nx * B) {
         const int b = e & (B - 1), x = e >> logB, q = q0 + b;
         if (q < Q) {
@@ -535,7 +535,7 @@ __global__ __launch_bounds__(MGK_NT) __attribute__((amdgpu_waves_per_eu(sizeof(T
     __syncthreads();
   }
   double* red = reinterpret_cast<double*>(tile);
-  block_sum<2>(acc, red);
+  block_sum<2, NTH>(acc, red);
   if (threadIdx.x == 0) {
     const size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
     partials[2 * bid] = acc[0];
@@ -1182,6 +1182,9 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
 // threads per workgroup of the compile-time mid-axis sweep (its tiles are
 // LDS-bound to four per CU, so wider workgroups give more waves per CU)
 constexpr int MID_NT = 512;
+// threads per workgroup of the compile-time x sweeps (512 measured slower:
+// the staged values per lane do not shrink enough to fit more waves)
+constexpr int X_NT = MGK_NT;
 
 template <typename T, int MV>
 void launch_mid(const LsPlan& pl, dim3 gy, hipStream_t stream, const FftPlan& ply, const cplx<T>* twy, cplx<T>* spec,
@@ -1351,10 +1354,10 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
     if (pl.ctx) {
       switch (pl.nx) {
-        case 64: k_ls_tau_xfwd<T, D, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-        case 128: k_ls_tau_xfwd<T, D, TT / MGK_NT, 128, TT / 128><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-        case 256: k_ls_tau_xfwd<T, D, TT / MGK_NT, 256, TT / 256><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-        default: k_ls_tau_xfwd<T, D, TT / MGK_NT, 512, TT / 512><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        case 64: k_ls_tau_xfwd<T, D, TT / X_NT, 64, TT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        case 128: k_ls_tau_xfwd<T, D, TT / X_NT, 128, TT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        case 256: k_ls_tau_xfwd<T, D, TT / X_NT, 256, TT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        default: k_ls_tau_xfwd<T, D, TT / X_NT, 512, TT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
       }
     } else {
       k_ls_tau_xfwd<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st);
@@ -1390,10 +1393,10 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
     if (pl.ctx) {
       switch (pl.nx) {
-        case 64: k_ls_xinv_update<T, D, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-        case 128: k_ls_xinv_update<T, D, TT / MGK_NT, 128, TT / 128><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-        case 256: k_ls_xinv_update<T, D, TT / MGK_NT, 256, TT / 256><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-        default: k_ls_xinv_update<T, D, TT / MGK_NT, 512, TT / 512><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        case 64: k_ls_xinv_update<T, D, TT / X_NT, 64, TT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        case 128: k_ls_xinv_update<T, D, TT / X_NT, 128, TT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        case 256: k_ls_xinv_update<T, D, TT / X_NT, 256, TT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        default: k_ls_xinv_update<T, D, TT / X_NT, 512, TT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
       }
     } else {
       k_ls_xinv_update<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st);
