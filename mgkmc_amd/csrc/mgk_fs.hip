@@ -435,8 +435,8 @@ struct FsGeom {
   int npencil;
 };
 
-template <typename T, int MAXV, int NN = 0, int BB = 0>
-__global__ __launch_bounds__(MGK_NT) void k_fs_xr2c(FsGeom<T> g, const T* __restrict__ f, cplx<T>* __restrict__ spec,
+template <typename T, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
+__global__ __launch_bounds__(NTH) void k_fs_xr2c(FsGeom<T> g, const T* __restrict__ f, cplx<T>* __restrict__ spec,
                                                     const int* __restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -451,14 +451,14 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xr2c(FsGeom<T> g, const T* __rest
   const int ca = 2 * pr, cb = (2 * pr + 1 < 9) ? 2 * pr + 1 : -1;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
-  for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
+  for (int t = threadIdx.x; t < nx; t += NTH) tw[t] = g.twx[t];
   // 16-byte vector path on full, aligned tiles (as the LS x sweeps)
   constexpr int VW = vw16<T>();
   constexpr bool VOK = CT && (BB % VW == 0);
   const bool vec = VOK && q0 + B <= Q && Q % VW == 0;
   if (vec) {
     constexpr int GPR = VOK ? BB / VW : 1;
-    for (int e = threadIdx.x; e < nx * GPR; e += MGK_NT) {
+    for (int e = threadIdx.x; e < nx * GPR; e += NTH) {
       const int bg = e % GPR, x = e / GPR;
       const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
       T a[VW], b[VW];
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xr2c(FsGeom<T> g, const T* __rest
       for (int i = 0; i < VW; ++i) tile[x * ldb + bg * VW + i] = cmk(a[i], cb >= 0 ? b[i] : T(0));
     }
   } else {
-    for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
+    for (int e = threadIdx.x; e < nx * B; e += NTH) {
       const int b = e & (B - 1), x = e >> logB, q = q0 + b;
       T a = T(0), bb = T(0);
       if (q < Q) {
@@ -480,16 +480,16 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xr2c(FsGeom<T> g, const T* __rest
     }
   }
   __syncthreads();
-  if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, false);
+  if constexpr (CT) tile_fft_ct<NN, BB, 1, false, NTH>(tile, tw, false);
   else tile_fft<MAXV>(tile, Tile{nx, logB, ldb, 1, 0}, g.plx, tw, false);
-  store_half<T, NN, BB>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
+  store_half<T, NN, BB, NTH>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)ca * cs,
                         cb >= 0 ? spec + (size_t)cb * cs : (cplx<T>*)nullptr);
 }
 
 // c2r along x into real comps; optional fused dots: part[2*blk] += out.a,
 // part[2*blk+1] += out.out (FP64), blk over (x-tile, plane, pair)
-template <typename T, int MAXV, int NN = 0, int BB = 0>
-__global__ __launch_bounds__(MGK_NT) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* __restrict__ spec, T* __restrict__ out,
+template <typename T, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
+__global__ __launch_bounds__(NTH) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* __restrict__ spec, T* __restrict__ out,
                                                     const T* __restrict__ dota, int selfdot,
                                                     double* __restrict__ partials, const int* __restrict__ skip) {
   if (skip && *skip) return;
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* 
   const int ca = 2 * pr, cb = (2 * pr + 1 < 9) ? 2 * pr + 1 : -1;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
-  for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
+  for (int t = threadIdx.x; t < nx; t += NTH) tw[t] = g.twx[t];
   constexpr int VW = vw16<T>();
   constexpr bool VOK = CT && (BB % VW == 0);
   const bool vec = VOK && q0 + B <= Q && Q % VW == 0;
@@ -514,17 +514,17 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* 
   bool merged = false;
   if constexpr (sizeof(T) == 4 && CT && BB % 2 == 0) {
     if (vec) {
-      xmerge_load_v<NN, BB>(tile, q0, Q, P, p, sa, sb);
+      xmerge_load_v<NN, BB, NTH>(tile, q0, Q, P, p, sa, sb);
       merged = true;
     }
   }
-  if (!merged) xmerge_load(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, sa, sb);
-  if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, true);
+  if (!merged) xmerge_load<T, NTH>(tile, ldb, nx, g.nxh, logB, q0, Q, P, p, sa, sb);
+  if constexpr (CT) tile_fft_ct<NN, BB, 1, false, NTH>(tile, tw, true);
   else tile_fft<MAXV>(tile, Tile{nx, logB, ldb, 1, 0}, g.plx, tw, true);
   double acc[2] = {0.0, 0.0};
   if (vec) {
     constexpr int GPR = VOK ? BB / VW : 1;
-    for (int e = threadIdx.x; e < nx * GPR; e += MGK_NT) {
+    for (int e = threadIdx.x; e < nx * GPR; e += NTH) {
       const int bg = e % GPR, x = e / GPR;
       const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
       T oa[VW], ob[VW], da[VW], db[VW];
@@ -551,7 +551,7 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* 
       }
     }
   } else
-  for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
+  for (int e = threadIdx.x; e < nx * B; e += NTH) {
     const int b = e & (B - 1), x = e >> logB, q = q0 + b;
     if (q >= Q) continue;
     const size_t v = ((size_t)x * P + p) * Q + q;
@@ -568,7 +568,7 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* 
   if (partials) {
     __syncthreads();
     double* red = reinterpret_cast<double*>(tile);
-    block_sum<2>(acc, red);
+    block_sum<2, NTH>(acc, red);
     if (threadIdx.x == 0) {
       const size_t bid = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
       partials[2 * bid] = acc[0];
@@ -691,7 +691,7 @@ struct KryState {
   double rtol, atol, bnrm2, rho, rho_prev, alpha, omega, beta, rv, ts, tt, rnorm, snorm, rhotol;
 };
 
-#define KRY_NBLK 4096
+#define KRY_NBLK 1024
 
 template <typename T>
 __global__ __launch_bounds__(MGK_NT) void k_kry_init(size_t n, const T* __restrict__ b, T* __restrict__ x,
@@ -1180,10 +1180,10 @@ void fs_project(FsCtx<T>& c, const T* in, T* out, T scale, const T* dota, int se
     const dim3 gx(pl.nqx, pl.P, 5);
     if (pl.ctx) {
       switch (pl.nx) {
-        case 64: k_fs_xr2c<T, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
-        case 128: k_fs_xr2c<T, TT / MGK_NT, 128, TT / 128><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
-        case 256: k_fs_xr2c<T, TT / MGK_NT, 256, TT / 256><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
-        default: k_fs_xr2c<T, TT / MGK_NT, 512, TT / 512><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
+        case 64: k_fs_xr2c<T, TT / 512, 64, TT / 64, 512><<<gx, 512, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
+        case 128: k_fs_xr2c<T, TT / 512, 128, TT / 128, 512><<<gx, 512, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
+        case 256: k_fs_xr2c<T, TT / 512, 256, TT / 256, 512><<<gx, 512, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
+        default: k_fs_xr2c<T, TT / 512, 512, TT / 512, 512><<<gx, 512, pl.lds_x, c.s>>>(c.g, in, c.spec, skip); break;
       }
     } else {
       k_fs_xr2c<T, MV><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, in, c.spec, skip);
@@ -1221,10 +1221,10 @@ void fs_project(FsCtx<T>& c, const T* in, T* out, T scale, const T* dota, int se
     const dim3 gx(pl.nqx, pl.P, 5);
     if (pl.ctx) {
       switch (pl.nx) {
-        case 64: k_fs_xc2r<T, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
-        case 128: k_fs_xc2r<T, TT / MGK_NT, 128, TT / 128><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
-        case 256: k_fs_xc2r<T, TT / MGK_NT, 256, TT / 256><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
-        default: k_fs_xc2r<T, TT / MGK_NT, 512, TT / 512><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
+        case 64: k_fs_xc2r<T, TT / 512, 64, TT / 64, 512><<<gx, 512, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
+        case 128: k_fs_xc2r<T, TT / 512, 128, TT / 128, 512><<<gx, 512, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
+        case 256: k_fs_xc2r<T, TT / 512, 256, TT / 256, 512><<<gx, 512, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
+        default: k_fs_xc2r<T, TT / 512, 512, TT / 512, 512><<<gx, 512, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip); break;
       }
     } else {
       k_fs_xc2r<T, MV><<<gx, MGK_NT, pl.lds_x, c.s>>>(c.g, c.spec, out, dota, selfdot, partials, skip);
