@@ -119,6 +119,28 @@ int mgk_ls_solve_landau2d(const int64_t* shape, const double* lam, const double*
                           int e33_have, double* tau_buf, int anderson_m, double* aa_buf, int warm, int max_iter,
                           double tol, void* workspace, size_t workspace_bytes, void* stream, mgk_ls_info* info);
 
+/* Secant-modulus small-strain laws.  Replaces spectral_solver_secant_2d/3d
+ * (mgkmc/linear_elastic_simulator.py:614-850; stress_from_strain_secant_2d/3d,
+ * mgkmc/elasticity.py:641-810) for law 0, and spectral_solver_rose_2d
+ * (mgkmc/linear_elastic_simulator.py:852-950; stress_from_strain_rose_2d,
+ * mgkmc/elasticity.py:868-943) for law 1 (2D only).  p1, p2 = (d, k) for law 0,
+ * (eta, mu_floor_fraction) for law 1; plane_stress selects the Rose e33 update. */
+typedef struct {
+  int law;           /* 0 exponential degradation, 1 Rose */
+  int plane_stress;  /* Rose only                          */
+  double p1, p2;
+} mgk_secant_params;
+
+/* lam, mu per voxel; (lam0, mu0) = the reference medium: (mean lam, mean mu) for
+ * law 0, (mean lam, (1 + floor) mean mu / 2) for law 1.  Fields 6 (3D) or 3 (2D)
+ * components; tau_buf: C*NV scratch values of the field precision.  The reference
+ * has no non-finite check here: a non-finite update returns MGK_ERR_NONFINITE with
+ * the fields as they are (non-finite), where the reference runs on to max_iter. */
+int mgk_ls_solve_secant(int ndim, const int64_t* shape, int precision, const void* lam, const void* mu,
+                        double lam0, double mu0, const mgk_secant_params* params, const double* eps_bar,
+                        const void* eps_plastic, void* eps, void* sig, void* tau_buf, int warm, int max_iter,
+                        double tol, void* workspace, size_t workspace_bytes, void* stream, mgk_ls_info* info);
+
 /* ---------------------------------------------------------------------------
  * KMC: barriers, rates, selection, compaction, flips.
  * Device layouts: Q, Q0 [m][v]; catalog [k][m][v] with k over 2 planar

@@ -33,6 +33,11 @@ class MgkLandauParams(ctypes.Structure):
                 ("cap_power", ctypes.c_double), ("cap_ratio", ctypes.c_double)]
 
 
+class MgkSecantParams(ctypes.Structure):
+    _fields_ = [("law", ctypes.c_int), ("plane_stress", ctypes.c_int), ("p1", ctypes.c_double),
+                ("p2", ctypes.c_double)]
+
+
 class MgkKmcParams(ctypes.Structure):
     _fields_ = [("volume", ctypes.c_double), ("rate_volume", ctypes.c_double), ("nu0", ctypes.c_double),
                 ("temperature", ctypes.c_double), ("cap", ctypes.c_double), ("tau", ctypes.c_double),
@@ -83,6 +88,9 @@ SIGNATURES = {
     "mgk_ls_solve_landau2d": (_c_i, [_c_i64p, _c_p, _c_p, _c_d, _c_d, ctypes.POINTER(MgkLandauParams), _c_i, _c_dp,
                                      _c_p, _c_p, _c_p, _c_p, _c_i, _c_i, _c_p, _c_i, _c_p, _c_i, _c_i, _c_d, _c_p,
                                      _c_sz, _c_p, ctypes.POINTER(MgkLsInfo)]),
+    "mgk_ls_solve_secant": (_c_i, [_c_i, _c_i64p, _c_i, _c_p, _c_p, _c_d, _c_d, ctypes.POINTER(MgkSecantParams), _c_dp,
+                                   _c_p, _c_p, _c_p, _c_p, _c_i, _c_i, _c_d, _c_p, _c_sz, _c_p,
+                                   ctypes.POINTER(MgkLsInfo)]),
     "mgk_kmc_workspace_bytes": (_c_i, [ctypes.c_longlong, ctypes.POINTER(_c_sz)]),
     "mgk_kmc_barrier_rates": (_c_i, [_c_i, ctypes.c_longlong, _c_i, _c_i, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p,
                                      _c_p, _c_p, ctypes.POINTER(MgkKmcParams), _c_p, _c_p, _c_p, _c_dp]),

@@ -21,6 +21,7 @@
 #include "mgk_host.h"
 #include "mgk_fftpass.h"
 #include "mgk_landau.h"
+#include "mgk_secant.h"
 #include <algorithm>
 
 struct LsState {
@@ -662,28 +663,39 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_stress(const T* __restrict__ eps,
 }
 
 // ---------------------------------------------------------------------------
-// Landau small-strain model (3D): polarisation tau = sigma_L(eps - eps_p) -
-// C0:eps into a 6-component field, r2c of a real 6-component field along x,
-// final Landau stress with FP64 sums of eps and sigma.
+// Nonlinear small-strain laws (3D Landau, secant-modulus laws): polarisation
+// tau = sigma(eps - eps_p) - C0:eps into a C-component real field, r2c of that
+// field along x, and the final stress with FP64 sums of eps and sigma.  Law
+// objects map C strain components to C stress components per voxel.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(MGK_NT) void k_ls_landau_tau(const T* __restrict__ eps, const T* __restrict__ epsp,
-                                                          const T* __restrict__ lam, const T* __restrict__ mu,
-                                                          T lam0, T mu0, LandauPar P, T* __restrict__ tau,
-                                                          size_t NV, const LsState* __restrict__ st) {
+struct LandauLaw {
+  LandauPar P;
+  static constexpr int C = 6;
+  template <typename T>
+  __device__ __forceinline__ void operator()(const T (&e)[6], T lam, T mu, T (&s)[6]) const {
+    landau_sigma_sym(e, lam, mu, P, s);
+  }
+};
+
+template <typename T, class Law>
+__global__ __launch_bounds__(MGK_NT) void k_ls_pol_tau(const T* __restrict__ eps, const T* __restrict__ epsp,
+                                                       const T* __restrict__ lam, const T* __restrict__ mu,
+                                                       T lam0, T mu0, Law law, T* __restrict__ tau, size_t NV,
+                                                       const LsState* __restrict__ st) {
+  constexpr int C = Law::C, ND = C == 6 ? 3 : 2;
   if (st->done) return;
   for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < NV; i += (size_t)gridDim.x * MGK_NT) {
-    T e[6], d[6], sg[6];
+    T e[C], d[C], sg[C];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
+    for (int c = 0; c < C; ++c) {
       e[c] = eps[(size_t)c * NV + i];
       d[c] = e[c] - (epsp ? epsp[(size_t)c * NV + i] : T(0));
     }
-    landau_sigma_sym(d, lam[i], mu[i], P, sg);
-    const T l0tr = lam0 * ((e[0] + e[1]) + e[2]);
+    law(d, lam[i], mu[i], sg);
+    const T l0tr = lam0 * (C == 6 ? (e[0] + e[1]) + e[2] : e[0] + e[1]);
     const T m02 = T(2) * mu0;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) tau[(size_t)c * NV + i] = sg[c] - ((c < 3 ? l0tr : T(0)) + m02 * e[c]);
+    for (int c = 0; c < C; ++c) tau[(size_t)c * NV + i] = sg[c] - ((c < ND ? l0tr : T(0)) + m02 * e[c]);
   }
 }
 
@@ -743,33 +755,34 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xfwd_real(LsGeom<T> g, const T* _
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(MGK_NT) void k_ls_landau_stress(const T* __restrict__ eps, const T* __restrict__ epsp,
-                                                             const T* __restrict__ lam, const T* __restrict__ mu,
-                                                             LandauPar P, T* __restrict__ sig, size_t NV,
-                                                             double* __restrict__ partials) {
-  __shared__ double red[12 * (MGK_NT / 64)];
-  double acc[12];
+template <typename T, class Law>
+__global__ __launch_bounds__(MGK_NT) void k_ls_pol_stress(const T* __restrict__ eps, const T* __restrict__ epsp,
+                                                          const T* __restrict__ lam, const T* __restrict__ mu,
+                                                          Law law, T* __restrict__ sig, size_t NV,
+                                                          double* __restrict__ partials) {
+  constexpr int C = Law::C;
+  __shared__ double red[2 * C * (MGK_NT / 64)];
+  double acc[2 * C];
 #pragma unroll
-  for (int c = 0; c < 12; ++c) acc[c] = 0.0;
+  for (int c = 0; c < 2 * C; ++c) acc[c] = 0.0;
   for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < NV; i += (size_t)gridDim.x * MGK_NT) {
-    T e[6], d[6], sg[6];
+    T e[C], d[C], sg[C];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
+    for (int c = 0; c < C; ++c) {
       e[c] = eps[(size_t)c * NV + i];
       d[c] = e[c] - (epsp ? epsp[(size_t)c * NV + i] : T(0));
     }
-    landau_sigma_sym(d, lam[i], mu[i], P, sg);
+    law(d, lam[i], mu[i], sg);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
+    for (int c = 0; c < C; ++c) {
       if (sig) sig[(size_t)c * NV + i] = sg[c];
       acc[c] += (double)e[c];
-      acc[6 + c] += (double)sg[c];
+      acc[C + c] += (double)sg[c];
     }
   }
-  block_sum<12>(acc, red);
+  block_sum<2 * C>(acc, red);
   if (threadIdx.x == 0)
-    for (int c = 0; c < 12; ++c) partials[(size_t)blockIdx.x * 12 + c] = acc[c];
+    for (int c = 0; c < 2 * C; ++c) partials[(size_t)blockIdx.x * 2 * C + c] = acc[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -1083,6 +1096,10 @@ typedef struct {
   int capping, cap_smooth;
   double cap_limit, cap_power, cap_ratio;
 } mgk_landau_params;
+typedef struct {
+  int law, plane_stress;
+  double p1, p2;
+} mgk_secant_params;
 }
 
 namespace {
@@ -1232,7 +1249,7 @@ template <typename T, int D>
 int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double mu0, const double* eps_bar,
                const T* epsp, T* eps, T* sig, int warm, int max_iter, double tol, char* ws, hipStream_t stream,
                mgk_ls_info* info, bool final_stress = true, const LandauPar* lp = nullptr, T* taubuf = nullptr,
-               const L2Mode* l2 = nullptr) {
+               const L2Mode* l2 = nullptr, const SecLaw* sl = nullptr) {
   constexpr int MV = maxv<T>();
   constexpr int TT = ct_tile<T>();  // points per compile-time tile
   constexpr int C = (D == 3) ? 6 : 3;
@@ -1337,9 +1354,15 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
       }
     }
     if (l2) {
-    } else if (lp) {
-      // Landau: elementwise polarisation, then r2c of the 6-component field
-      k_ls_landau_tau<T><<<nel, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, T(lam0), T(mu0), *lp, taubuf, NV, st);
+    } else if (lp || sl) {
+      // nonlinear law: elementwise polarisation, then r2c of the C-component field
+      if (sl) {
+        k_ls_pol_tau<T, SecantLaw<D>><<<nel, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, T(lam0), T(mu0),
+                                                                  SecantLaw<D>{*sl}, taubuf, NV, st);
+      } else if constexpr (D == 3) {
+        k_ls_pol_tau<T, LandauLaw><<<nel, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, T(lam0), T(mu0), LandauLaw{*lp},
+                                                               taubuf, NV, st);
+      }
       if (pl.ctx) {
         switch (pl.nx) {
           case 64: k_ls_xfwd_real<T, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
@@ -1464,8 +1487,14 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     }
   }
   if (l2_done) {
-  } else if (final_stress && lp) {
-    k_ls_landau_stress<T><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, *lp, sig, NV, part);
+  } else if (final_stress && (lp || sl)) {
+    if (sl) {
+      k_ls_pol_stress<T, SecantLaw<D>><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, SecantLaw<D>{*sl},
+                                                                           sig, NV, part);
+    } else if constexpr (D == 3) {
+      k_ls_pol_stress<T, LandauLaw><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, LandauLaw{*lp}, sig, NV,
+                                                                        part);
+    }
     k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, 2 * C, (double)NV, st->mean);
     MGK_HIP(hipGetLastError());
   } else if (final_stress) {
@@ -1653,6 +1682,48 @@ extern "C" int mgk_ls_solve_landau(const int64_t* shape, int precision, const vo
   return ls_solve_t<float, 3>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar, (const float*)eps_plastic,
                               (float*)eps, (float*)sig, warm, max_iter, tol, (char*)workspace, s, info, true, &P,
                               (float*)tau_buf);
+}
+
+extern "C" int mgk_ls_solve_secant(int ndim, const int64_t* shape, int precision, const void* lam, const void* mu,
+                                   double lam0, double mu0, const mgk_secant_params* par, const double* eps_bar,
+                                   const void* eps_plastic, void* eps, void* sig, void* tau_buf, int warm,
+                                   int max_iter, double tol, void* workspace, size_t workspace_bytes, void* stream,
+                                   mgk_ls_info* info) {
+  if (!lam || !mu || !eps || !eps_bar || !workspace || !tau_buf || !par || max_iter < 1 || (ndim != 2 && ndim != 3) ||
+      (precision != 0 && precision != 1) || par->law < 0 || par->law > 1 || (par->law == 1 && ndim != 2)) {
+    mgk_set_error("bad argument (null pointer, max_iter < 1, ndim, precision, or a law not defined in 3D)", __FILE__,
+                  __LINE__);
+    return MGK_ERR_ARG;
+  }
+  LsPlan pl;
+  int rc = precision == 0 ? ls_plan<double>(ndim, shape, &pl) : ls_plan<float>(ndim, shape, &pl);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the LS solver", __FILE__, __LINE__);
+    return rc;
+  }
+  if (workspace_bytes < pl.total) {
+    mgk_set_error("workspace too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  SecLaw L{par->law, par->plane_stress ? 1 : 0, par->p1, par->p2};
+  hipStream_t s = (hipStream_t)stream;
+  warm = warm ? 1 : 0;
+  if (precision == 0) {
+    if (ndim == 3)
+      return ls_solve_t<double, 3>(pl, (const double*)lam, (const double*)mu, lam0, mu0, eps_bar,
+                                   (const double*)eps_plastic, (double*)eps, (double*)sig, warm, max_iter, tol,
+                                   (char*)workspace, s, info, true, nullptr, (double*)tau_buf, nullptr, &L);
+    return ls_solve_t<double, 2>(pl, (const double*)lam, (const double*)mu, lam0, mu0, eps_bar,
+                                 (const double*)eps_plastic, (double*)eps, (double*)sig, warm, max_iter, tol,
+                                 (char*)workspace, s, info, true, nullptr, (double*)tau_buf, nullptr, &L);
+  }
+  if (ndim == 3)
+    return ls_solve_t<float, 3>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar,
+                                (const float*)eps_plastic, (float*)eps, (float*)sig, warm, max_iter, tol,
+                                (char*)workspace, s, info, true, nullptr, (float*)tau_buf, nullptr, &L);
+  return ls_solve_t<float, 2>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar, (const float*)eps_plastic,
+                              (float*)eps, (float*)sig, warm, max_iter, tol, (char*)workspace, s, info, true, nullptr,
+                              (float*)tau_buf, nullptr, &L);
 }
 
 extern "C" int mgk_ls_solve_landau2d(const int64_t* shape, const double* lam, const double* mu, double lam0,

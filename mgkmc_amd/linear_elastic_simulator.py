@@ -478,3 +478,206 @@ def spectral_solver_landau_2d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_
     eps = dev.from_sym_soa(dev.download(sol.eps), 2)
     sig = dev.from_sym_soa(dev.download(sol.sig), 2)
     return eps, sig, np.array(inf.eps_mean[:4]).reshape(2, 2), np.array(inf.sig_mean[:4]).reshape(2, 2)
+
+
+# ---------------------------------------------------------------------------
+# Secant-modulus laws: exponential degradation (2D/3D) and Rose (2D)
+# ---------------------------------------------------------------------------
+SECANT_EXP, SECANT_ROSE = 0, 1
+
+
+class SecantSolver:
+    """Device-resident nonlinear LS solver for the secant-modulus laws
+    (mgkmc/linear_elastic_simulator.py:614-950): law SECANT_EXP with
+    (p1, p2) = (d, k) in 2D or 3D, law SECANT_ROSE with (eta, floor
+    fraction) in 2D.  The reference medium is (mean lam, mean mu), with
+    mu_ref = (1 + floor) mean mu / 2 for the Rose law, as the reference sets it."""
+
+    def __init__(self, lam, mu, law, p1, p2, plane_mode="plane_strain", precision=FP64):
+        dev.require_gpu()
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        if lam.shape != mu.shape or lam.ndim not in (2, 3):
+            raise ValueError("lam and mu must be 2D or 3D fields of one shape")
+        if law == SECANT_ROSE and lam.ndim != 2:
+            raise ValueError("the Rose law is defined for 2D grids only")
+        if precision not in (FP64, FP32):
+            raise ValueError("precision must be FP64 or FP32")
+        self.shape = lam.shape
+        self.d = lam.ndim
+        self.C = 6 if self.d == 3 else 3
+        self.precision = precision
+        self.params = _lib.MgkSecantParams(int(law), 1 if plane_mode == "plane_stress" else 0, float(p1), float(p2))
+        self.lam0 = float(lam.mean())
+        m = float(mu.mean())
+        self.mu0 = 0.5 * (m + float(p2) * m) if law == SECANT_ROSE else m
+        self.lam = dev.upload(lam, precision)
+        self.mu = dev.upload(mu, precision)
+        dt = dev.torch_dtype(precision)
+        t = dev.torch
+        self.eps = t.zeros((self.C,) + self.shape, dtype=dt, device="cuda")
+        self.sig = t.zeros((self.C,) + self.shape, dtype=dt, device="cuda")
+        self.tau = t.empty((self.C,) + self.shape, dtype=dt, device="cuda")
+        self._shape_c = _lib.shape_arg(self.shape)
+        nb = ctypes.c_size_t()
+        _lib.check(_lib.load().mgk_ls_workspace_bytes(self.d, self._shape_c, precision, ctypes.byref(nb)),
+                   "mgk_ls_workspace_bytes")
+        self.ws = t.empty(int(nb.value), dtype=t.uint8, device="cuda")
+        self.info = _lib.MgkLsInfo()
+
+    def solve(self, eps_bar, eps_plastic=None, warm=False, max_iter=400, tol=1e-6, stream=None):
+        """One solve; eps_plastic a device SoA field or None.  A non-finite
+        update stops the loop with info.status == MGK_ERR_NONFINITE and
+        non-finite fields (the reference iterates on to max_iter there)."""
+        eb, ebp = _lib.host_dbl(np.asarray(eps_bar, dtype=np.float64).reshape(self.d, self.d))
+        s = dev.stream_ptr() if stream is None else stream
+        rc = _lib.load().mgk_ls_solve_secant(
+            self.d, self._shape_c, self.precision, _lib.dptr(self.lam), _lib.dptr(self.mu), self.lam0, self.mu0,
+            ctypes.byref(self.params), ebp, _lib.dptr(eps_plastic), _lib.dptr(self.eps), _lib.dptr(self.sig),
+            _lib.dptr(self.tau), 1 if warm else 0, int(max_iter), float(tol), _lib.dptr(self.ws), self.ws.numel(),
+            ctypes.c_void_p(s), ctypes.byref(self.info))
+        _lib.check(rc, "mgk_ls_solve_secant")
+        return self.info
+
+
+def _secant_drop_in(d, lam, mu, law, p1, p2, eps_bar, eps_plastic, max_iter, tol, plane_mode, eps_init):
+    lam = np.asarray(lam, dtype=np.float64)
+    mu = np.asarray(mu, dtype=np.float64)
+    shape = lam.shape if lam.ndim == d else mu.shape
+    lam = np.broadcast_to(lam, shape)
+    mu = np.broadcast_to(mu, shape)
+    eps_bar = np.asarray(eps_bar, dtype=np.float64)
+    if np.any(eps_bar != eps_bar.T):
+        raise ValueError("the device secant solvers take a symmetric eps_bar")
+    sol = SecantSolver(lam, mu, law, p1, p2, plane_mode, FP64)
+    ep = None
+    if eps_plastic is not None:
+        sym, a = dev.to_sym_soa(eps_plastic, d)
+        if np.any(a):
+            raise ValueError("the device secant solvers take a symmetric eps_plastic")
+        ep = dev.upload(sym, FP64)
+    warm = eps_init is not None
+    if warm:
+        sym, a = dev.to_sym_soa(eps_init, d)
+        if np.any(a):
+            raise ValueError("the device secant solvers take a symmetric eps_init")
+        sol.eps.copy_(dev.upload(sym, FP64))
+    inf = sol.solve(eps_bar, ep, warm=warm, max_iter=max_iter, tol=tol)
+    eps = dev.from_sym_soa(dev.download(sol.eps), d)
+    sig = dev.from_sym_soa(dev.download(sol.sig), d)
+    n = d * d
+    return eps, sig, np.array(inf.eps_mean[:n]).reshape(d, d), np.array(inf.sig_mean[:n]).reshape(d, d)
+
+
+def spectral_solver_secant_2d(lam, mu, d, k, eps_bar, eps_plastic=None, max_iter=400, tol=1e-6, verbose=False,
+                              pixel=1.0, plane_mode="plane_strain", Gamma=None, eps_init=None):
+    """2D LS with the exponential secant-degradation law on the GPU
+    (mgkmc/linear_elastic_simulator.py:614-765); lam is used as given in
+    either plane mode, as in the reference.  Gamma is ignored (the reference
+    medium is (mean lam, mean mu))."""
+    return _secant_drop_in(2, lam, mu, SECANT_EXP, d, k, eps_bar, eps_plastic, max_iter, tol, plane_mode, eps_init)
+
+
+def spectral_solver_secant_3d(lam, mu, d, k, eps_bar, eps_plastic=None, max_iter=400, tol=1e-6, verbose=False,
+                              pixel=1.0, Gamma=None, eps_init=None):
+    """3D LS with the exponential secant-degradation law on the GPU
+    (mgkmc/linear_elastic_simulator.py:768-849)."""
+    return _secant_drop_in(3, lam, mu, SECANT_EXP, d, k, eps_bar, eps_plastic, max_iter, tol, "plane_strain",
+                           eps_init)
+
+
+def spectral_solver_rose_2d(lam, mu, eta, mu_floor_fraction=0.1, eps_bar=None, eps_plastic=None, max_iter=400,
+                            tol=1e-6, verbose=False, pixel=1.0, plane_mode="plane_strain", Gamma=None,
+                            eps_init=None):
+    """2D LS with the Rose secant law on the GPU
+    (mgkmc/linear_elastic_simulator.py:852-950); reference medium
+    (mean lam, (1 + floor) mean mu / 2)."""
+    return _secant_drop_in(2, lam, mu, SECANT_ROSE, eta, mu_floor_fraction, eps_bar, eps_plastic, max_iter, tol,
+                           plane_mode, eps_init)
+
+
+def _secant_simulation(d, lam, mu, law, p1, p2, target_strain_mask, target_values, n_steps, tol_ls, max_iter_ls,
+                       tol_macro, max_iter_macro, store, plane_mode, precision=FP64, info=None):
+    """Mixed stress/strain load stepping of a secant-modulus law on one device
+    solver (mgkmc/linear_elastic_simulator.py:1353-2052): per step, every macro
+    iteration restarts from the previous step's converged field; the
+    stress-controlled entries get (err - nu_avg tr(err) I) / E_avg with the
+    compliance of the reference medium (mean lam, mu_ref)."""
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    if lam.ndim != d:
+        raise ValueError(f"lam must be {d}D")
+    solver = SecantSolver(lam, mu, law, p1, p2, plane_mode, precision)
+    la, ma = solver.lam0, solver.mu0
+    E_avg = ma * (3.0 * la + 2.0 * ma) / (la + ma)
+    nu_avg = la / (2.0 * (la + ma))
+    mask = np.asarray(target_strain_mask, dtype=bool)
+    smask = ~mask
+    tv = np.asarray(target_values, dtype=np.float64)
+    eb = np.zeros((d, d))
+    warm = None
+    em_l, sm_l, e_l, s_l, its = [], [], [], [], []
+    n = d * d
+    for s in range(n_steps + 1):
+        tgt = tv * (s / n_steps)
+        for _ in range(max_iter_macro):
+            eb[mask] = tgt[mask]
+            if warm is not None:
+                solver.eps.copy_(warm)
+            inf = solver.solve(eb, None, warm=warm is not None, max_iter=max_iter_ls, tol=tol_ls)
+            its.append(inf.iters)
+            epsM = np.array(inf.eps_mean[:n]).reshape(d, d)
+            sigM = np.array(inf.sig_mean[:n]).reshape(d, d)
+            err = np.zeros((d, d))
+            err[smask] = tgt[smask] - sigM[smask]
+            max_err = np.max(np.abs(err[smask])) if np.any(smask) else 0.0
+            if max_err < tol_macro:
+                break
+            de = (err - nu_avg * np.trace(err) * np.eye(d)) / E_avg
+            eb[smask] += de[smask]
+        warm = solver.eps.clone()
+        em_l.append(epsM)
+        sm_l.append(sigM)
+        if store:
+            e_l.append(dev.from_sym_soa(dev.download(solver.eps), d))
+            s_l.append(dev.from_sym_soa(dev.download(solver.sig), d))
+    if info is not None:
+        info["ls_iters"] = its
+    return np.array(em_l), np.array(sm_l), e_l, s_l
+
+
+def secant_elastic_simulation_2d(lam, mu, d, k, target_strain_mask, target_values, n_steps=20, pixel=1.0, tol_ls=1e-6,
+                                 max_iter_ls=400, tol_macro=1e-4, max_iter_macro=20, store=True,
+                                 plane_mode="plane_strain", log_path=None, global_log_path=None,
+                                 driving_component=(0, 0), enable_console=True, vtk_path=None, vtk_interval="none",
+                                 **solver_kw):
+    """2D mixed-BC load stepping with the secant-degradation law
+    (mgkmc/linear_elastic_simulator.py:1353-1600) on the GPU; log, VTK and
+    checkpoint options are accepted and ignored; ``precision`` / ``info``
+    (extensions) as in linear_elastic_simulation_2d."""
+    return _secant_simulation(2, lam, mu, SECANT_EXP, d, k, target_strain_mask, target_values, n_steps, tol_ls,
+                              max_iter_ls, tol_macro, max_iter_macro, store, plane_mode,
+                              solver_kw.get("precision", FP64), solver_kw.get("info"))
+
+
+def secant_elastic_simulation_3d(lam, mu, d, k, target_strain_mask, target_values, n_steps=20, pixel=1.0, tol_ls=1e-6,
+                                 max_iter_ls=400, tol_macro=1e-4, max_iter_macro=20, store=True, log_path=None,
+                                 global_log_path=None, driving_component=(0, 0), enable_console=True, vtk_path=None,
+                                 vtk_interval="none", **solver_kw):
+    """3D mixed-BC load stepping with the secant-degradation law
+    (mgkmc/linear_elastic_simulator.py:1857-2052) on the GPU."""
+    return _secant_simulation(3, lam, mu, SECANT_EXP, d, k, target_strain_mask, target_values, n_steps, tol_ls,
+                              max_iter_ls, tol_macro, max_iter_macro, store, "plane_strain",
+                              solver_kw.get("precision", FP64), solver_kw.get("info"))
+
+
+def rose_elastic_simulation_2d(lam, mu, eta, mu_floor_fraction=0.1, target_strain_mask=None, target_values=None,
+                               n_steps=20, pixel=1.0, tol_ls=1e-6, max_iter_ls=400, tol_macro=1e-4, max_iter_macro=20,
+                               store=True, plane_mode="plane_strain", log_path=None, global_log_path=None,
+                               driving_component=(0, 0), enable_console=True, vtk_path=None, vtk_interval="none",
+                               **solver_kw):
+    """2D mixed-BC load stepping with the Rose law
+    (mgkmc/linear_elastic_simulator.py:1603-1855) on the GPU."""
+    return _secant_simulation(2, lam, mu, SECANT_ROSE, eta, mu_floor_fraction, target_strain_mask, target_values,
+                              n_steps, tol_ls, max_iter_ls, tol_macro, max_iter_macro, store, plane_mode,
+                              solver_kw.get("precision", FP64), solver_kw.get("info"))

@@ -13,6 +13,8 @@ Follows, step for step:
   mgkmc/linear_elastic_simulator.py:12-88    spectral_solver_2d
   mgkmc/linear_elastic_simulator.py:274-376  spectral_solver_3d
   mgkmc/stz/update_fft.py:4-22  update_stress_fft_full
+  mgkmc/elasticity.py:641-810,868-943 + linear_elastic_simulator.py:614-950
+                                secant-degradation and Rose laws and LS loops
 
 The reference uses pyfftw (FFTW3) complex fftn with norm="ortho"; this file
 uses scipy.fft.fftn/ifftn (same DFT definition and normalisation; FFTW is not
@@ -594,3 +596,219 @@ def spectral_solver_landau_2d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_
     if info is not None:
         info.update(iters=it_done, diff=diff)
     return eps, sig_out, eps.mean(axis=(0, 1)), sig_out.mean(axis=(0, 1))
+
+
+# ---------------------------------------------------------------------------
+# Secant-modulus small-strain models and their LS fixed points:
+#   exponential degradation, 2D/3D (elasticity.py:641-810;
+#   linear_elastic_simulator.py:614-850),
+#   Rose law, 2D plane strain / plane stress (elasticity.py:868-943;
+#   linear_elastic_simulator.py:852-950).
+# ---------------------------------------------------------------------------
+def von_mises_strain(eps):
+    """sqrt(2/3 e':e'); a 2D tensor is taken with e33 = 0, whose deviator
+    still has e'33 = -(e11 + e22)/3 (elasticity.py:641-700)."""
+    d = eps.shape[-1]
+    if d == 2:
+        e11, e22, e12, e21 = eps[..., 0, 0], eps[..., 1, 1], eps[..., 0, 1], eps[..., 1, 0]
+        tr3 = (e11 + e22) / 3.0
+        ep11, ep22, ep33 = e11 - tr3, e22 - tr3, -(e11 + e22) / 3.0
+        return np.sqrt((2.0 / 3.0) * (ep11 ** 2 + ep22 ** 2 + ep33 ** 2 + 0.5 * (e12 + e21) ** 2))
+    e = eps
+    tr3 = (e[..., 0, 0] + e[..., 1, 1] + e[..., 2, 2]) / 3.0
+    ep11, ep22, ep33 = e[..., 0, 0] - tr3, e[..., 1, 1] - tr3, e[..., 2, 2] - tr3
+    return np.sqrt((2.0 / 3.0) * (ep11 ** 2 + ep22 ** 2 + ep33 ** 2
+                                  + 0.5 * ((e[..., 0, 1] + e[..., 1, 0]) ** 2 + (e[..., 0, 2] + e[..., 2, 0]) ** 2
+                                           + (e[..., 1, 2] + e[..., 2, 1]) ** 2)))
+
+
+def stress_secant(eps, lam, mu, d, k):
+    """sigma = lam tr(e) I + 2 mu_sec e with mu_sec = mu (1 - d (1 - exp(-k e_eq)))
+    (elasticity.py:703-810; the 2D form takes lam as given for either plane mode)."""
+    dim = eps.shape[-1]
+    mu_sec = mu * (1.0 - d * (1.0 - np.exp(-k * von_mises_strain(eps))))
+    tr = np.trace(eps, axis1=-2, axis2=-1)[..., None, None]
+    return np.asarray(lam)[..., None, None] * tr * np.eye(dim) + 2.0 * mu_sec[..., None, None] * eps
+
+
+def rose_shear(eps_eq, mu, eta, mu_floor_fraction=0.1):
+    """G_sec = max(mu exp(-eta e_eq), floor mu) (elasticity.py:868-876)."""
+    return np.maximum(mu * np.exp(-eta * eps_eq), mu_floor_fraction * mu)
+
+
+def stress_rose_2d(eps, lam, mu, eta, mu_floor_fraction=0.1, plane_mode="plane_strain", e33_out=None):
+    """In-plane stress of the Rose law at constant bulk modulus K = lam + 2/3 mu:
+    sigma = (K - 2/3 G) tr I + 2 G e; plane stress runs five fixed-point updates
+    e33 = -(3K - 2G)/(3K + 4G) (e11 + e22) from the linear guess and evaluates G
+    once more (elasticity.py:879-943)."""
+    exx, eyy, exy = eps[..., 0, 0], eps[..., 1, 1], eps[..., 0, 1]
+    K = lam + (2.0 / 3.0) * mu
+
+    def shear(ezz):
+        tr = exx + eyy + ezz
+        tr3 = tr / 3.0
+        eq = np.sqrt((2.0 / 3.0) * ((exx - tr3) ** 2 + (eyy - tr3) ** 2 + (ezz - tr3) ** 2 + 2.0 * exy ** 2))
+        return tr, rose_shear(eq, mu, eta, mu_floor_fraction)
+
+    if plane_mode == "plane_stress":
+        ezz = -lam / (lam + 2.0 * mu) * (exx + eyy)
+        for _ in range(5):
+            _, G = shear(ezz)
+            ezz = -((3.0 * K - 2.0 * G) / (3.0 * K + 4.0 * G)) * (exx + eyy)
+    else:
+        ezz = np.zeros_like(exx)
+    tr, G = shear(ezz)
+    if e33_out is not None:
+        e33_out[...] = ezz
+    sig = np.zeros_like(eps)
+    le = K - (2.0 / 3.0) * G
+    sig[..., 0, 0] = le * tr + 2.0 * G * exx
+    sig[..., 1, 1] = le * tr + 2.0 * G * eyy
+    sig[..., 0, 1] = 2.0 * G * exy
+    sig[..., 1, 0] = sig[..., 0, 1]
+    return sig
+
+
+def _secant_ls(stress, dim, shape, lam_avg, mu_ref, eps_bar, eps_plastic, max_iter, tol, pixel, Gamma, eps_init,
+               info):
+    """LS fixed point eps = eps_bar - Gamma0 * (sigma(eps - eps_p) - C0:eps) with
+    C0 = (lam_avg, mu_ref) and the relative-change stop test; no non-finite
+    check (linear_elastic_simulator.py:614-950)."""
+    if Gamma is None:
+        Gamma = green_operator(wave_vectors(shape, pixel), lam_avg, mu_ref)
+    eps_bar = np.asarray(eps_bar, float)
+    if eps_init is not None:
+        eps = eps_init.copy()
+        for i in range(dim):
+            for j in range(dim):
+                eps[..., i, j] += eps_bar[i, j] - eps_init[..., i, j].mean()
+    else:
+        eps = np.zeros(shape + (dim, dim))
+        for i in range(dim):
+            for j in range(dim):
+                eps[..., i, j] = eps_bar[i, j]
+    letters = "xyz"[:dim]
+    ein = f"{letters}khij,{letters}ij->{letters}kh"
+    it_done = 0
+    diff = np.inf
+    for it in range(max_iter):
+        e_el = eps - eps_plastic if eps_plastic is not None else eps
+        sig = stress(e_el)
+        tr = np.trace(eps, axis1=-2, axis2=-1)[..., None, None]
+        tau = sig - (lam_avg * tr * np.eye(dim) + 2.0 * mu_ref * eps)
+        tau_hat = np.zeros(shape + (dim, dim), dtype=complex)
+        for i in range(dim):
+            for j in range(dim):
+                tau_hat[..., i, j] = fft_field(tau[..., i, j])
+        et_hat = -np.einsum(ein, Gamma, tau_hat)
+        et = np.zeros(shape + (dim, dim))
+        for i in range(dim):
+            for j in range(dim):
+                et[..., i, j] = ifft_field(et_hat[..., i, j])
+                et[..., i, j] -= et[..., i, j].mean()
+        eps_new = eps_bar + et
+        diff = np.linalg.norm(eps_new - eps) / (np.linalg.norm(eps) + 1e-20)
+        eps = eps_new
+        it_done = it + 1
+        if diff < tol:
+            break
+    e_el = eps - eps_plastic if eps_plastic is not None else eps
+    sig_out = stress(e_el)
+    if info is not None:
+        info.update(iters=it_done, diff=diff)
+    axes = tuple(range(dim))
+    return eps, sig_out, eps.mean(axis=axes), sig_out.mean(axis=axes)
+
+
+def _field_shape(lam, mu, dim):
+    return lam.shape if lam.ndim == dim else mu.shape
+
+
+def spectral_solver_secant_2d(lam, mu, d, k, eps_bar, eps_plastic=None, max_iter=400, tol=1e-6, verbose=False,
+                              pixel=1.0, plane_mode="plane_strain", Gamma=None, eps_init=None, info=None):
+    lam = np.asarray(lam, float)
+    mu = np.asarray(mu, float)
+    shape = _field_shape(lam, mu, 2)
+    la = float(lam.mean()) if lam.ndim > 0 else float(lam)
+    ma = float(mu.mean()) if mu.ndim > 0 else float(mu)
+    return _secant_ls(lambda e: stress_secant(e, lam, mu, d, k), 2, shape, la, ma, eps_bar, eps_plastic, max_iter,
+                      tol, pixel, Gamma, eps_init, info)
+
+
+def spectral_solver_secant_3d(lam, mu, d, k, eps_bar, eps_plastic=None, max_iter=400, tol=1e-6, verbose=False,
+                              pixel=1.0, Gamma=None, eps_init=None, info=None):
+    lam = np.asarray(lam, float)
+    mu = np.asarray(mu, float)
+    shape = _field_shape(lam, mu, 3)
+    return _secant_ls(lambda e: stress_secant(e, lam, mu, d, k), 3, shape, float(lam.mean()), float(mu.mean()),
+                      eps_bar, eps_plastic, max_iter, tol, pixel, Gamma, eps_init, info)
+
+
+def spectral_solver_rose_2d(lam, mu, eta, mu_floor_fraction=0.1, eps_bar=None, eps_plastic=None, max_iter=400,
+                            tol=1e-6, verbose=False, pixel=1.0, plane_mode="plane_strain", Gamma=None, eps_init=None,
+                            info=None):
+    """Reference shear modulus mu_ref = (mu_avg + floor mu_avg)/2, lam_avg as given."""
+    lam = np.asarray(lam, float)
+    mu = np.asarray(mu, float)
+    shape = _field_shape(lam, mu, 2)
+    la = float(lam.mean()) if lam.ndim > 0 else float(lam)
+    ma = float(mu.mean()) if mu.ndim > 0 else float(mu)
+    mu_ref = 0.5 * (ma + mu_floor_fraction * ma)
+    return _secant_ls(lambda e: stress_rose_2d(e, lam, mu, eta, mu_floor_fraction, plane_mode), 2, shape, la, mu_ref,
+                      eps_bar, eps_plastic, max_iter, tol, pixel, Gamma, eps_init, info)
+
+
+def secant_simulation(d, law, lam, mu, p1, p2, target_strain_mask, target_values, n_steps=20, pixel=1.0, tol_ls=1e-6,
+                      max_iter_ls=400, tol_macro=1e-4, max_iter_macro=20, store=True, plane_mode="plane_strain"):
+    """Mixed stress/strain load stepping of the secant-modulus laws
+    (linear_elastic_simulator.py:1353-1600 secant 2D, :1603-1855 Rose 2D,
+    :1857-2052 secant 3D): law "secant" (p1, p2 = d, k) or "rose" (eta,
+    floor).  As linear_elastic_simulation, except that the compliance of the
+    stress correction comes from the reference medium: E_avg = m (3 l + 2 m) /
+    (l + m), nu_avg = l / (2 (l + m)) with (l, m) = (mean lam, mean mu), m =
+    mu_ref for the Rose law.  Returns (eps_macro, sig_macro, eps list, sig
+    list, LS iteration counts per solve)."""
+    lam = np.asarray(lam, float)
+    mu = np.asarray(mu, float)
+    la, ma = float(lam.mean()), float(mu.mean())
+    if law == "rose":
+        ma = 0.5 * (ma + p2 * ma)
+    E_avg = ma * (3.0 * la + 2.0 * ma) / (la + ma)
+    nu_avg = la / (2.0 * (la + ma))
+    mask = np.asarray(target_strain_mask, bool)
+    smask = ~mask
+    tv = np.asarray(target_values, float)
+    em_l, sm_l, e_l, s_l, its = [], [], [], [], []
+    eb = np.zeros((d, d))
+    warm = None
+    for s in range(n_steps + 1):
+        tgt = tv * (s / n_steps)
+        for _ in range(max_iter_macro):
+            eb[mask] = tgt[mask]
+            info = {}
+            if law == "rose":
+                eps, sig, epsM, sigM = spectral_solver_rose_2d(lam, mu, p1, p2, eps_bar=eb, max_iter=max_iter_ls,
+                                                               tol=tol_ls, pixel=pixel, plane_mode=plane_mode,
+                                                               eps_init=warm, info=info)
+            elif d == 3:
+                eps, sig, epsM, sigM = spectral_solver_secant_3d(lam, mu, p1, p2, eb, max_iter=max_iter_ls,
+                                                                 tol=tol_ls, pixel=pixel, eps_init=warm, info=info)
+            else:
+                eps, sig, epsM, sigM = spectral_solver_secant_2d(lam, mu, p1, p2, eb, max_iter=max_iter_ls,
+                                                                 tol=tol_ls, pixel=pixel, plane_mode=plane_mode,
+                                                                 eps_init=warm, info=info)
+            its.append(info["iters"])
+            err = np.zeros((d, d))
+            err[smask] = tgt[smask] - sigM[smask]
+            mx = np.max(np.abs(err[smask])) if np.any(smask) else 0.0
+            if mx < tol_macro:
+                break
+            de = (err - nu_avg * np.trace(err) * np.eye(d)) / E_avg
+            eb[smask] += de[smask]
+        warm = eps
+        em_l.append(epsM)
+        sm_l.append(sigM)
+        if store:
+            e_l.append(eps)
+            s_l.append(sig)
+    return np.array(em_l), np.array(sm_l), e_l, s_l, its

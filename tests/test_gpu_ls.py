@@ -397,3 +397,141 @@ def test_landau2d_nonfinite_raises(gpu):
     with pytest.raises(FloatingPointError):
         spectral_solver_landau_2d(lam, mu, **LANDAU_PA, eps_bar=eb, tol=1e-8, plane_mode="plane_stress",
                                   anderson_m=0)
+
+
+# ---------------------------------------------------------------------------
+# secant-modulus laws (exponential degradation 2D/3D, Rose 2D)
+# ---------------------------------------------------------------------------
+_SECANT_CASES = [
+    ("secant3", (8, 6, 10), None),
+    ("secant3", (5, 6, 4), None),
+    ("secant2", (16, 12), "plane_strain"),
+    ("secant2", (16, 12), "plane_stress"),
+    ("rose", (16, 12), "plane_strain"),
+    ("rose", (12, 20), "plane_stress"),
+]
+
+
+def _secant_run(mod, law, lam, mu, eb, ep, mode, tol, info=None):
+    kw = dict(eps_plastic=ep, tol=tol, max_iter=400)
+    if info is not None:
+        kw["info"] = info
+    if law == "secant3":
+        return mod.spectral_solver_secant_3d(lam, mu, 0.6, 40.0, eb, **kw)
+    if law == "secant2":
+        return mod.spectral_solver_secant_2d(lam, mu, 0.6, 40.0, eb, plane_mode=mode, **kw)
+    return mod.spectral_solver_rose_2d(lam, mu, 25.0, 0.1, eps_bar=eb, plane_mode=mode, **kw)
+
+
+@pytest.mark.parametrize("law,shape,mode", _SECANT_CASES)
+def test_secant_ls_parity(gpu, law, shape, mode):
+    # FP64 fields <= 1e-10 relative vs the oracle, identical iteration counts
+    from mgkmc_amd import linear_elastic_simulator as les
+    from mgkmc_amd import device as dev
+    print(f"[secant] {law} {shape} {mode}", flush=True)
+    d = len(shape)
+    rng = np.random.default_rng(21)
+    E = 70e9 * rng.uniform(0.85, 1.15, shape)
+    nu = np.full(shape, 0.3)
+    lam, mu = ol.compute_lame_3d(E, nu) if d == 3 else ol.compute_lame_2d(E, nu, "plane_strain")
+    g = rng.standard_normal(shape + (d, d)) * 3e-3
+    ep = 0.5 * (g + np.swapaxes(g, -1, -2))
+    eb = np.zeros((d, d))
+    eb[0, 0], eb[0, 1], eb[1, 0] = 0.012, 0.008, 0.008
+    info = {}
+    ref = _secant_run(ol, law, lam, mu, eb, ep, mode, 1e-9, info)
+    got = _secant_run(les, law, lam, mu, eb, ep, mode, 1e-9)
+    for a, b in zip(got, ref):
+        assert _rel(a, b) < 1e-10, (law, _rel(a, b))
+    p = (les.SECANT_ROSE, 25.0, 0.1) if law == "rose" else (les.SECANT_EXP, 0.6, 40.0)
+    s = les.SecantSolver(lam, mu, *p, plane_mode=mode or "plane_strain")
+    sym, _ = dev.to_sym_soa(ep, d)
+    inf = s.solve(eb, dev.upload(sym, 0), tol=1e-9)
+    print(f"[secant] {law}: iters {inf.iters} (oracle {info['iters']})", flush=True)
+    assert inf.status == 0 and inf.iters == info["iters"]
+
+
+def test_secant_scalar_moduli_and_warm_start(gpu):
+    # scalar lam with a mu field (the reference broadcasts), and eps_init
+    from mgkmc_amd import linear_elastic_simulator as les
+    shape = (16, 12)
+    rng = np.random.default_rng(22)
+    mu = 27e9 * rng.uniform(0.8, 1.2, shape)
+    eb = np.array([[0.01, 0.004], [0.004, 0.0]])
+    init = np.broadcast_to(eb, shape + (2, 2)) + 1e-3 * rng.standard_normal(shape + (2, 2))
+    init = 0.5 * (init + np.swapaxes(init, -1, -2))
+    ref = ol.spectral_solver_rose_2d(40e9, mu, 25.0, 0.1, eps_bar=eb, tol=1e-9, eps_init=init.copy())
+    got = les.spectral_solver_rose_2d(40e9, mu, 25.0, 0.1, eps_bar=eb, tol=1e-9, eps_init=init.copy())
+    for a, b in zip(got, ref):
+        assert _rel(a, b) < 1e-10
+    ref = ol.spectral_solver_secant_2d(40e9, mu, 0.5, 30.0, eb, tol=1e-9)
+    got = les.spectral_solver_secant_2d(40e9, mu, 0.5, 30.0, eb, tol=1e-9)
+    for a, b in zip(got, ref):
+        assert _rel(a, b) < 1e-10
+
+
+@pytest.mark.parametrize("n", [64, 128])
+def test_secant_specialised_sizes(gpu, n):
+    # compile-time sweeps: d = 0 reproduces the linear LS field (FP64); the FP32
+    # form of the degrading law stays within 1e-5 of FP64 at tol 1e-7
+    import torch
+    from mgkmc_amd.linear_elastic_simulator import SecantSolver, LsSolver, SECANT_EXP, FP64, FP32
+    rng = np.random.default_rng(23)
+    shape = (n, n, n)
+    E = 70e9 * rng.uniform(0.9, 1.1, shape)
+    nu = np.full(shape, 0.3)
+    lam, mu = ol.compute_lame_3d(E, nu)
+    ep = torch.from_numpy(rng.standard_normal((6,) + shape) * 1e-3).cuda()
+    eb = np.diag([0.01, 0.0, 0.0])
+    s = SecantSolver(lam, mu, SECANT_EXP, 0.0, 40.0)
+    inf = s.solve(eb, ep, tol=1e-10, max_iter=300)
+    lin = LsSolver(E, nu, precision=FP64)
+    inf2 = lin.solve(eb, ep, tol=1e-10)
+    assert inf.status == 0 and inf2.status == 0
+    err = float((s.eps - lin.eps).abs().max() / lin.eps.abs().max())
+    print(f"[secant {n}^3] d=0: iters {inf.iters} vs linear {inf2.iters}, field diff {err:.2e}", flush=True)
+    assert err < 1e-8
+    s64 = SecantSolver(lam, mu, SECANT_EXP, 0.6, 40.0)
+    s32 = SecantSolver(lam, mu, SECANT_EXP, 0.6, 40.0, precision=FP32)
+    i64 = s64.solve(eb, ep, tol=1e-7)
+    i32 = s32.solve(eb, ep.float(), tol=1e-7)
+    assert i64.status == 0 and i32.status == 0
+    err = float((s32.eps.double() - s64.eps).abs().max() / s64.eps.abs().max())
+    print(f"[secant {n}^3] FP32 iters {i32.iters} vs FP64 {i64.iters}, field diff {err:.2e}", flush=True)
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("law,d,plane_mode", [("secant", 3, "plane_strain"), ("secant", 2, "plane_stress"),
+                                              ("rose", 2, "plane_stress")])
+def test_secant_driver_parity(gpu, law, d, plane_mode):
+    # mixed-BC load stepping: macro lists and fields <= 1e-10 relative, LS
+    # iteration counts per solve identical to the oracle driver's
+    from mgkmc_amd import linear_elastic_simulator as les
+    print(f"[secant driver] {law} {d}D {plane_mode}", flush=True)
+    shape = (12, 10, 8) if d == 3 else (16, 12)
+    rng = np.random.default_rng(24)
+    E = 70e9 * rng.uniform(0.8, 1.2, shape)
+    nu = np.full(shape, 0.3)
+    lam, mu = ol.compute_lame_3d(E, nu) if d == 3 else ol.compute_lame_2d(E, nu, "plane_strain")
+    mask = ~np.eye(d, dtype=bool)
+    mask[0, 0] = True
+    tv = np.zeros((d, d))
+    tv[0, 0] = 0.02
+    p1, p2 = (0.6, 40.0) if law == "secant" else (25.0, 0.1)
+    ref = ol.secant_simulation(d, law, lam, mu, p1, p2, mask, tv, n_steps=3, tol_ls=1e-8, tol_macro=1e3,
+                               plane_mode=plane_mode)
+    info = {}
+    if law == "rose":
+        got = les.rose_elastic_simulation_2d(lam, mu, p1, p2, mask, tv, n_steps=3, tol_ls=1e-8, tol_macro=1e3,
+                                             plane_mode=plane_mode, info=info)
+    elif d == 3:
+        got = les.secant_elastic_simulation_3d(lam, mu, p1, p2, mask, tv, n_steps=3, tol_ls=1e-8, tol_macro=1e3,
+                                               info=info)
+    else:
+        got = les.secant_elastic_simulation_2d(lam, mu, p1, p2, mask, tv, n_steps=3, tol_ls=1e-8, tol_macro=1e3,
+                                               plane_mode=plane_mode, info=info)
+    print(f"[secant driver] LS iterations {info['ls_iters']} (oracle {ref[4]})", flush=True)
+    assert info["ls_iters"] == ref[4]
+    assert _rel(got[0], ref[0]) < 1e-10 and _rel(got[1], ref[1]) < 1e-10
+    for a, b in zip(got[2] + got[3], ref[2] + ref[3]):
+        assert _rel(a, b) < 1e-10

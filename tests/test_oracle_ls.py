@@ -259,3 +259,236 @@ def test_landau_2d_anderson_reaches_the_picard_fixed_point():
     assert ia["iters"] < ip["iters"]
     assert np.abs(a[0] - b[0]).max() < 1e-8 * np.abs(b[0]).max()
     assert np.abs(a[1] - b[1]).max() < 1e-7 * np.abs(b[1]).max()
+
+
+# ---------------------------------------------------------------------------
+# secant-degradation and Rose laws (oracle/ls.py stress_secant, stress_rose_2d)
+# ---------------------------------------------------------------------------
+def _vm_3x3(eps):
+    """Von Mises strain from the full 3x3 deviator (a 2D strain padded with
+    e33 = e13 = e23 = 0), written independently of the oracle's component form."""
+    d = eps.shape[-1]
+    E3 = np.zeros(eps.shape[:-2] + (3, 3))
+    E3[..., :d, :d] = eps
+    dev = E3 - np.trace(E3, axis1=-2, axis2=-1)[..., None, None] / 3.0 * np.eye(3)
+    return np.sqrt(2.0 / 3.0 * np.einsum("...ij,...ij->...", dev, dev))
+
+
+def _sym(rng, shape, d, s=0.01):
+    a = rng.normal(0, s, shape + (d, d))
+    return 0.5 * (a + np.swapaxes(a, -1, -2))
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_secant_law_closed_forms(d):
+    rng = _rng(11)
+    shape = (5, 4) if d == 2 else (3, 4, 2)
+    lam = rng.uniform(30e9, 60e9, shape)
+    mu = rng.uniform(20e9, 40e9, shape)
+    e = _sym(rng, shape, d)
+    assert np.allclose(ol.von_mises_strain(e), _vm_3x3(e), rtol=1e-13, atol=0)
+    lin = lam[..., None, None] * np.trace(e, axis1=-2, axis2=-1)[..., None, None] * np.eye(d) \
+        + 2 * mu[..., None, None] * e
+    assert np.allclose(ol.stress_secant(e, lam, mu, 0.0, 50.0), lin, rtol=1e-14, atol=0)
+    # pure shear gamma: e_eq = gamma/sqrt3; k e_eq = ln 2 => mu_sec = mu (1 - d/2)
+    g, dd = 0.02, 0.6
+    es = np.zeros((d, d))
+    es[0, 1] = es[1, 0] = g / 2
+    k = np.log(2.0) * np.sqrt(3.0) / g
+    s = ol.stress_secant(es, 40e9, 30e9, dd, k)
+    assert abs(s[0, 1] - 2 * 30e9 * (1 - dd / 2) * g / 2) < 1e-12 * 30e9 * g
+    assert np.all(s[np.eye(d) > 0] == 0)
+    # equibiaxial / hydrostatic a I: 3D deviator zero (mu_sec = mu); 2D keeps
+    # e33 = 0 in the deviator, e_eq = 2a/3
+    a = 0.01
+    s = ol.stress_secant(a * np.eye(d), 40e9, 30e9, dd, 20.0)
+    if d == 3:
+        assert np.allclose(np.diag(s), (3 * 40e9 + 2 * 30e9) * a, rtol=1e-14)
+    else:
+        ms = 30e9 * (1 - dd * (1 - np.exp(-20.0 * 2 * a / 3)))
+        assert np.allclose(np.diag(s), 2 * 40e9 * a + 2 * ms * a, rtol=1e-14)
+    # full degradation at large k: sigma -> lam tr(e) I
+    s = ol.stress_secant(e, lam, mu, 1.0, 1e9)
+    assert np.allclose(s, lam[..., None, None] * np.trace(e, axis1=-2, axis2=-1)[..., None, None] * np.eye(d),
+                       rtol=0, atol=1e-6 * np.abs(s).max())
+
+
+def test_rose_law_closed_forms():
+    rng = _rng(12)
+    shape = (6, 5)
+    lam = rng.uniform(30e9, 60e9, shape)
+    mu = rng.uniform(20e9, 40e9, shape)
+    e = _sym(rng, shape, 2)
+    tr = np.trace(e, axis1=-2, axis2=-1)[..., None, None]
+    L, M = lam[..., None, None], mu[..., None, None]
+    # eta = 0: linear plane strain, and exact linear plane stress with
+    # lam* = 2 lam mu / (lam + 2 mu)
+    assert np.allclose(ol.stress_rose_2d(e, lam, mu, 0.0), L * tr * np.eye(2) + 2 * M * e, rtol=1e-13, atol=0)
+    ls = 2 * L * M / (L + 2 * M)
+    assert np.allclose(ol.stress_rose_2d(e, lam, mu, 0.0, plane_mode="plane_stress"),
+                       ls * tr * np.eye(2) + 2 * M * e, rtol=1e-12, atol=1e-9 * np.abs(L * e).max())
+    # pure shear: eta e_eq = ln 2 => G = mu/2; a huge eta hits the floor
+    g = 0.02
+    es = np.zeros((2, 2))
+    es[0, 1] = es[1, 0] = g / 2
+    eta = np.log(2.0) * np.sqrt(3.0) / g
+    for mode in ("plane_strain", "plane_stress"):
+        s = ol.stress_rose_2d(es, 40e9, 30e9, eta, 0.1, plane_mode=mode)
+        assert abs(s[0, 1] - 30e9 * g / 2) < 1e-12 * 30e9 * g
+        s = ol.stress_rose_2d(es, 40e9, 30e9, 1e6, 0.1, plane_mode=mode)
+        assert abs(s[0, 1] - 2 * 0.1 * 30e9 * g / 2) < 1e-12 * 30e9 * g
+    # constant bulk modulus: the in-plane trace of the plane-strain stress is
+    # 2 (K - 2/3 G) tr + 2 G tr with G from the 3x3 deviator
+    eta = 40.0
+    s = ol.stress_rose_2d(e, lam, mu, eta)
+    G = np.maximum(mu * np.exp(-eta * _vm_3x3(e)), 0.1 * mu)
+    K = lam + 2 * mu / 3
+    t = np.trace(e, axis1=-2, axis2=-1)
+    assert np.allclose(s[..., 0, 0] + s[..., 1, 1], 2 * (K - 2 * G / 3) * t + 2 * G * t, rtol=1e-12)
+    assert np.allclose(s[..., 0, 1], 2 * G * e[..., 0, 1], rtol=1e-12)
+    # plane stress: five updates approach the root of sigma_33(e33) = 0 of the 3D law
+    from scipy.optimize import brentq
+    e33 = np.zeros(shape)
+    ol.stress_rose_2d(e, lam, mu, eta, plane_mode="plane_stress", e33_out=e33)
+    for i in np.ndindex(shape):
+        def s33(z, i=i):
+            E3 = np.zeros((3, 3))
+            E3[:2, :2] = e[i]
+            E3[2, 2] = z
+            dev = E3 - np.trace(E3) / 3 * np.eye(3)
+            Gz = max(mu[i] * np.exp(-eta * np.sqrt(2 / 3 * np.sum(dev * dev))), 0.1 * mu[i])
+            return (K[i] - 2 * Gz / 3) * np.trace(E3) + 2 * Gz * z
+        root = brentq(s33, -0.1, 0.1, xtol=1e-16)
+        assert abs(e33[i] - root) <= 1e-3 * (abs(root) + 1e-4)
+        # exactly five updates from the linear guess (the 3x3 form above,
+        # not the oracle's components): the truncation error is visible
+        z = -lam[i] / (lam[i] + 2 * mu[i]) * (e[i][0, 0] + e[i][1, 1])
+        for _ in range(5):
+            E3 = np.zeros((3, 3))
+            E3[:2, :2] = e[i]
+            E3[2, 2] = z
+            dev = E3 - np.trace(E3) / 3 * np.eye(3)
+            Gz = max(mu[i] * np.exp(-eta * np.sqrt(2 / 3 * np.sum(dev * dev))), 0.1 * mu[i])
+            z = -(3 * K[i] - 2 * Gz) / (3 * K[i] + 4 * Gz) * (e[i][0, 0] + e[i][1, 1])
+        assert abs(e33[i] - z) <= 1e-14 * abs(z) + 1e-20
+
+
+def _shear_laminate(law_shear, mus, gbar):
+    """Two-layer laminate (layers normal to x, equal fractions) under mean
+    shear strain gbar/2: the shear stress is uniform, the layer shear strains
+    average to gbar/2, normal strains vanish.  Returns the layer e_xy."""
+    from scipy.optimize import brentq
+    ea_of = lambda eb: gbar - eb  # e_a + e_b = 2 * (gbar/2)
+
+    def f(eb):
+        return law_shear(ea_of(eb), mus[0]) - law_shear(eb, mus[1])
+    eb = brentq(f, 1e-9 * gbar, gbar - 1e-9 * gbar, xtol=1e-18, rtol=1e-15)
+    return ea_of(eb), eb
+
+
+@pytest.mark.parametrize("law", ["secant3", "secant2", "rose_strain", "rose_stress"])
+def test_secant_ls_laminate_exact(law):
+    mus = (20e9, 36e9)
+    lam_v = 45e9
+    gbar = 0.02
+    d = 3 if law == "secant3" else 2
+    n = 8
+    shape = (n, 2, 2) if d == 3 else (n, 2)
+    mu = np.empty(shape)
+    mu[: n // 2] = mus[0]
+    mu[n // 2:] = mus[1]
+    lam = np.full(shape, lam_v)
+    eb = np.zeros((d, d))
+    eb[0, 1] = eb[1, 0] = gbar / 2
+    info = {}
+    if law.startswith("secant"):
+        dd, k = 0.7, 60.0
+        shear = lambda e, m: 2 * m * (1 - dd * (1 - np.exp(-k * 2 * abs(e) / np.sqrt(3)))) * e
+        fn = ol.spectral_solver_secant_3d if d == 3 else ol.spectral_solver_secant_2d
+        eps, sig, em, sm = fn(lam, mu, dd, k, eb, tol=1e-13, max_iter=3000, info=info)
+    else:
+        eta, fl = 30.0, 0.1
+        shear = lambda e, m: 2 * max(m * np.exp(-eta * 2 * abs(e) / np.sqrt(3)), fl * m) * e
+        mode = "plane_stress" if law == "rose_stress" else "plane_strain"
+        eps, sig, em, sm = ol.spectral_solver_rose_2d(lam, mu, eta, fl, eps_bar=eb, tol=1e-13, max_iter=3000,
+                                                      plane_mode=mode, info=info)
+    ea, ebb = _shear_laminate(shear, mus, gbar)
+    assert info["iters"] < 3000
+    assert np.allclose(eps[: n // 2, ..., 0, 1], ea, rtol=1e-9, atol=0)
+    assert np.allclose(eps[n // 2:, ..., 0, 1], ebb, rtol=1e-9, atol=0)
+    assert np.abs(eps[..., 0, 0]).max() < 1e-12
+    assert np.allclose(sig[..., 0, 1], shear(ea, mus[0]), rtol=1e-8)
+    assert np.allclose(em, eb, atol=1e-15)
+
+
+def test_secant_ls_linear_limit_and_equilibrium():
+    rng = _rng(13)
+    shape = (8, 6, 4)
+    E = 70e9 * rng.uniform(0.7, 1.3, shape)
+    nu = np.full(shape, 0.3)
+    lam, mu = ol.compute_lame_3d(E, nu)
+    eb = np.zeros((3, 3))
+    eb[0, 0], eb[0, 1], eb[1, 0] = 0.01, 0.004, 0.004
+    ep = _sym(rng, shape, 3, 2e-3)
+    a = ol.spectral_solver_secant_3d(lam, mu, 0.0, 30.0, eb, eps_plastic=ep, tol=1e-12, max_iter=2000)
+    b = ol.spectral_solver_3d(E, nu, eb, eps_plastic=ep, tol=1e-12, max_iter=2000)
+    assert np.allclose(a[0], b[0], rtol=0, atol=1e-9 * np.abs(b[0]).max())
+    c = ol.spectral_solver_secant_3d(lam, mu, 0.5, 30.0, eb, eps_plastic=ep, tol=1e-11, max_iter=2000)
+    assert ol.equilibrium_residual(c[1]) < 1e-8
+    assert np.allclose(c[2], eb, atol=1e-15)
+    e2 = _sym(rng, (8, 8), 2, 2e-3)
+    lam2, mu2 = lam[:, :, 0].repeat(2, 1)[:, :8], mu[:, :, 0].repeat(2, 1)[:, :8]
+    r = ol.spectral_solver_rose_2d(lam2, mu2, 30.0, 0.1, eps_bar=eb[:2, :2], eps_plastic=e2, tol=1e-11,
+                                   max_iter=3000, plane_mode="plane_stress")
+    assert ol.equilibrium_residual(r[1]) < 1e-8
+
+
+@pytest.mark.parametrize("law,d,plane_mode", [("secant", 3, "plane_strain"), ("secant", 2, "plane_strain"),
+                                              ("rose", 2, "plane_strain"), ("rose", 2, "plane_stress")])
+def test_secant_driver_uniaxial_closed_form(law, d, plane_mode):
+    # homogeneous nonlinear medium, eps_xx driven, transverse normal stresses
+    # zero, shear strains held: the macro loop reaches the uniaxial state whose
+    # transverse strain t solves sigma_yy(eps_xx, t) = 0 of the law (brentq on
+    # the 3x3 deviator form), at each of the ramped steps
+    from scipy.optimize import brentq
+    lam0, mu0 = 45e9, 27e9
+    shape = (6,) * d
+    lam = np.full(shape, lam0)
+    mu = np.full(shape, mu0)
+    mask = ~np.eye(d, dtype=bool)
+    mask[0, 0] = True
+    tv = np.zeros((d, d))
+    tv[0, 0] = 0.02
+    p1, p2 = (0.6, 40.0) if law == "secant" else (25.0, 0.1)
+    em, sm, el, sl, its = ol.secant_simulation(d, law, lam, mu, p1, p2, mask, tv, n_steps=2, tol_ls=1e-10,
+                                               tol_macro=1.0, plane_mode=plane_mode)
+
+    def stress(ex, t):
+        E3 = np.diag([ex] + [t] * (d - 1) + [0.0] * (3 - d))
+        if law == "rose" and plane_mode == "plane_stress":
+            # sigma_33 = 0 fixes e33 (exact root here; the law's five updates
+            # are within 1e-3 of it, far below the tolerance below)
+            def s33(z):
+                E3[2, 2] = z
+                dev_ = E3 - np.trace(E3) / 3 * np.eye(3)
+                G = max(mu0 * np.exp(-p1 * np.sqrt(2 / 3 * np.sum(dev_ * dev_))), p2 * mu0)
+                return (lam0 + 2 * mu0 / 3 - 2 * G / 3) * np.trace(E3) + 2 * G * z
+            E3[2, 2] = brentq(s33, -0.1, 0.1, xtol=1e-18)
+        dev_ = E3 - np.trace(E3) / 3 * np.eye(3)
+        eq = np.sqrt(2 / 3 * np.sum(dev_ * dev_))
+        if law == "secant":
+            ms = mu0 * (1 - p1 * (1 - np.exp(-p2 * eq)))
+            return lam0 * np.trace(E3[:d, :d]) + 2 * ms * E3[1, 1], lam0 * np.trace(E3[:d, :d]) + 2 * ms * ex
+        G = max(mu0 * np.exp(-p1 * eq), p2 * mu0)
+        le = lam0 + 2 * mu0 / 3 - 2 * G / 3
+        return le * np.trace(E3) + 2 * G * E3[1, 1], le * np.trace(E3) + 2 * G * ex
+
+    rt = 1e-4 if plane_mode == "plane_stress" else 1e-8
+    for s, ex in ((1, 0.01), (2, 0.02)):
+        t = brentq(lambda t: stress(ex, t)[0], -0.05, 0.05, xtol=1e-18)
+        assert abs(em[s][0, 0] - ex) < 1e-15
+        for k in range(1, d):
+            assert abs(em[s][k, k] - t) < rt * abs(t)
+            assert abs(sm[s][k, k]) < 1.0
+        assert abs(sm[s][0, 0] - stress(ex, t)[1]) < rt * abs(sm[s][0, 0])
+    assert len(el) == 3 and el[-1].shape == shape + (d, d)
