@@ -596,21 +596,17 @@ def spectral_solver_rose_2d(lam, mu, eta, mu_floor_fraction=0.1, eps_bar=None, e
                            plane_mode, eps_init)
 
 
-def _secant_simulation(d, lam, mu, law, p1, p2, target_strain_mask, target_values, n_steps, tol_ls, max_iter_ls,
-                       tol_macro, max_iter_macro, store, plane_mode, precision=FP64, info=None):
-    """Mixed stress/strain load stepping of a secant-modulus law on one device
-    solver (mgkmc/linear_elastic_simulator.py:1353-2052): per step, every macro
+def _nl_stepping(d, solver, solve, E_avg, nu_avg, target_strain_mask, target_values, n_steps, tol_macro,
+                 max_iter_macro, store, secant_slope, info):
+    """Mixed stress/strain load stepping of a nonlinear law on one device
+    solver (mgkmc/linear_elastic_simulator.py:1353-2516): per step, every macro
     iteration restarts from the previous step's converged field; the
     stress-controlled entries get (err - nu_avg tr(err) I) / E_avg with the
-    compliance of the reference medium (mean lam, mu_ref)."""
-    lam = np.ascontiguousarray(lam, dtype=np.float64)
-    mu = np.ascontiguousarray(mu, dtype=np.float64)
-    if lam.ndim != d:
-        raise ValueError(f"lam must be {d}D")
-    solver = SecantSolver(lam, mu, law, p1, p2, plane_mode, precision)
-    la, ma = solver.lam0, solver.mu0
-    E_avg = ma * (3.0 * la + 2.0 * ma) / (la + ma)
-    nu_avg = la / (2.0 * (la + ma))
+    compliance of the reference medium, or, with secant_slope (the Landau
+    drivers, :2155-2205 and :2390-2438), err / slope for entries whose last two
+    macro iterates give a finite slope above 0.01 E_avg and a strain change
+    above 1e-14 (history restarted each step).  solve(eps_bar, warm) runs one
+    solve on ``solver`` and returns its info."""
     mask = np.asarray(target_strain_mask, dtype=bool)
     smask = ~mask
     tv = np.asarray(target_values, dtype=np.float64)
@@ -620,11 +616,12 @@ def _secant_simulation(d, lam, mu, law, p1, p2, target_strain_mask, target_value
     n = d * d
     for s in range(n_steps + 1):
         tgt = tv * (s / n_steps)
+        prev_e = prev_s = None
         for _ in range(max_iter_macro):
             eb[mask] = tgt[mask]
             if warm is not None:
                 solver.eps.copy_(warm)
-            inf = solver.solve(eb, None, warm=warm is not None, max_iter=max_iter_ls, tol=tol_ls)
+            inf = solve(eb, warm is not None)
             its.append(inf.iters)
             epsM = np.array(inf.eps_mean[:n]).reshape(d, d)
             sigM = np.array(inf.sig_mean[:n]).reshape(d, d)
@@ -634,7 +631,18 @@ def _secant_simulation(d, lam, mu, law, p1, p2, target_strain_mask, target_value
             if max_err < tol_macro:
                 break
             de = (err - nu_avg * np.trace(err) * np.eye(d)) / E_avg
-            eb[smask] += de[smask]
+            dsc = de[smask]
+            if secant_slope:
+                e_sc = eb[smask].copy()
+                s_sc = sigM[smask].copy()
+                if prev_e is not None:
+                    dh = e_sc - prev_e
+                    with np.errstate(divide="ignore", invalid="ignore"):
+                        slope = (s_sc - prev_s) / dh
+                    ok = np.isfinite(slope) & (np.abs(dh) > 1e-14) & (slope > 0.01 * E_avg)
+                    dsc[ok] = err[smask][ok] / slope[ok]
+                prev_e, prev_s = e_sc, s_sc
+            eb[smask] += dsc
         warm = solver.eps.clone()
         em_l.append(epsM)
         sm_l.append(sigM)
@@ -644,6 +652,25 @@ def _secant_simulation(d, lam, mu, law, p1, p2, target_strain_mask, target_value
     if info is not None:
         info["ls_iters"] = its
     return np.array(em_l), np.array(sm_l), e_l, s_l
+
+
+def _medium_compliance(la, ma):
+    """(E, nu) of the reference medium (lam0, mu0), the stress-correction compliance."""
+    return ma * (3.0 * la + 2.0 * ma) / (la + ma), la / (2.0 * (la + ma))
+
+
+def _secant_simulation(d, lam, mu, law, p1, p2, target_strain_mask, target_values, n_steps, tol_ls, max_iter_ls,
+                       tol_macro, max_iter_macro, store, plane_mode, precision=FP64, info=None):
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    if lam.ndim != d:
+        raise ValueError(f"lam must be {d}D")
+    solver = SecantSolver(lam, mu, law, p1, p2, plane_mode, precision)
+    E_avg, nu_avg = _medium_compliance(solver.lam0, solver.mu0)
+    return _nl_stepping(d, solver, lambda eb, warm: solver.solve(eb, None, warm=warm, max_iter=max_iter_ls,
+                                                                 tol=tol_ls),
+                        E_avg, nu_avg, target_strain_mask, target_values, n_steps, tol_macro, max_iter_macro, store,
+                        False, info)
 
 
 def secant_elastic_simulation_2d(lam, mu, d, k, target_strain_mask, target_values, n_steps=20, pixel=1.0, tol_ls=1e-6,
@@ -681,3 +708,54 @@ def rose_elastic_simulation_2d(lam, mu, eta, mu_floor_fraction=0.1, target_strai
     return _secant_simulation(2, lam, mu, SECANT_ROSE, eta, mu_floor_fraction, target_strain_mask, target_values,
                               n_steps, tol_ls, max_iter_ls, tol_macro, max_iter_macro, store, plane_mode,
                               solver_kw.get("precision", FP64), solver_kw.get("info"))
+
+
+def _landau_simulation(d, lam, mu, v, target_strain_mask, target_values, n_steps, tol_ls, max_iter_ls, tol_macro,
+                       max_iter_macro, store, plane_mode, solver_kw):
+    solver_name = solver_kw.get("solver", "dbfft")
+    if solver_name == "newton_cg":
+        raise NotImplementedError("the device path implements the dbfft (fixed point + Anderson) solver")
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    if lam.ndim != d:
+        raise ValueError(f"lam must be {d}D")
+    cap = {k: solver_kw[k] for k in ("strain_capping_enabled", "strain_capping_limit", "strain_capping_tangent_ratio",
+                                     "strain_capping_type", "strain_capping_smooth_power") if k in solver_kw}
+    prm = landau_params(*v, mu=mu, **cap)
+    if d == 3:
+        solver = LandauSolver(lam, mu, solver_kw.get("precision", FP64))
+        solve = lambda eb, warm: solver.solve(prm, eb, None, warm=warm, max_iter=max_iter_ls, tol=tol_ls)
+    else:
+        solver = LandauSolver2D(lam, mu, plane_mode, solver_kw.get("anderson_m", 5))
+        # one e33 state for the whole run (the reference's e33_state dict)
+        ps = plane_mode == "plane_stress"
+        solve = lambda eb, warm: solver.solve(prm, eb, None, warm=warm, max_iter=max_iter_ls, tol=tol_ls,
+                                              persist_e33=ps)
+    E_avg, nu_avg = _medium_compliance(solver.lam0, solver.mu0)
+    return _nl_stepping(d, solver, solve, E_avg, nu_avg, target_strain_mask, target_values, n_steps, tol_macro,
+                        max_iter_macro, store, True, solver_kw.get("info"))
+
+
+def landau_elastic_simulation_2d(lam, mu, v1, v2, v3, g1, g2, g3, g4, target_strain_mask, target_values, n_steps=20,
+                                 pixel=1.0, tol_ls=1e-6, max_iter_ls=400, tol_macro=1e-4, max_iter_macro=20,
+                                 store=True, plane_mode="plane_strain", log_path=None, global_log_path=None,
+                                 driving_component=(0, 0), enable_console=True, vtk_path=None, vtk_interval="none",
+                                 **solver_kw):
+    """2D mixed-BC load stepping with the Landau law
+    (mgkmc/linear_elastic_simulator.py:2055-2284) on the GPU: secant-slope
+    macro corrections, Anderson depth 5, one plane-stress e33 state for the
+    run.  Capping options and ``solver`` come through solver_kw as in the
+    reference; ``info`` (extension) receives the LS iteration counts."""
+    return _landau_simulation(2, lam, mu, (v1, v2, v3, g1, g2, g3, g4), target_strain_mask, target_values, n_steps,
+                              tol_ls, max_iter_ls, tol_macro, max_iter_macro, store, plane_mode, solver_kw)
+
+
+def landau_elastic_simulation_3d(lam, mu, v1, v2, v3, g1, g2, g3, g4, target_strain_mask, target_values, n_steps=20,
+                                 pixel=1.0, tol_ls=1e-6, max_iter_ls=400, tol_macro=1e-4, max_iter_macro=20,
+                                 store=True, log_path=None, global_log_path=None, driving_component=(0, 0),
+                                 enable_console=True, vtk_path=None, vtk_interval="none", **solver_kw):
+    """3D mixed-BC load stepping with the Landau law
+    (mgkmc/linear_elastic_simulator.py:2287-2516) on the GPU; ``precision``
+    (extension) selects FP64 or FP32 fields."""
+    return _landau_simulation(3, lam, mu, (v1, v2, v3, g1, g2, g3, g4), target_strain_mask, target_values, n_steps,
+                              tol_ls, max_iter_ls, tol_macro, max_iter_macro, store, "plane_strain", solver_kw)

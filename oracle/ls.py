@@ -758,23 +758,17 @@ def spectral_solver_rose_2d(lam, mu, eta, mu_floor_fraction=0.1, eps_bar=None, e
                       eps_bar, eps_plastic, max_iter, tol, pixel, Gamma, eps_init, info)
 
 
-def secant_simulation(d, law, lam, mu, p1, p2, target_strain_mask, target_values, n_steps=20, pixel=1.0, tol_ls=1e-6,
-                      max_iter_ls=400, tol_macro=1e-4, max_iter_macro=20, store=True, plane_mode="plane_strain"):
-    """Mixed stress/strain load stepping of the secant-modulus laws
-    (linear_elastic_simulator.py:1353-1600 secant 2D, :1603-1855 Rose 2D,
-    :1857-2052 secant 3D): law "secant" (p1, p2 = d, k) or "rose" (eta,
-    floor).  As linear_elastic_simulation, except that the compliance of the
-    stress correction comes from the reference medium: E_avg = m (3 l + 2 m) /
-    (l + m), nu_avg = l / (2 (l + m)) with (l, m) = (mean lam, mean mu), m =
-    mu_ref for the Rose law.  Returns (eps_macro, sig_macro, eps list, sig
-    list, LS iteration counts per solve)."""
-    lam = np.asarray(lam, float)
-    mu = np.asarray(mu, float)
-    la, ma = float(lam.mean()), float(mu.mean())
-    if law == "rose":
-        ma = 0.5 * (ma + p2 * ma)
-    E_avg = ma * (3.0 * la + 2.0 * ma) / (la + ma)
-    nu_avg = la / (2.0 * (la + ma))
+def _macro_stepping(d, solve, E_avg, nu_avg, target_strain_mask, target_values, n_steps, tol_macro, max_iter_macro,
+                    store, secant_slope):
+    """Load stepping shared by the nonlinear drivers: targets ramp over
+    n_steps + 1 steps; each macro iteration sets the strain-controlled
+    entries, solves from the previous step's field and corrects the
+    stress-controlled entries by the reference-medium compliance
+    (err - nu_avg tr(err) I) / E_avg; with secant_slope (the Landau drivers,
+    linear_elastic_simulator.py:2155-2205, :2390-2438) an entry whose
+    last two macro iterates give a finite slope above 0.01 E_avg (and a strain
+    change above 1e-14) takes err / slope instead; the history restarts each
+    step."""
     mask = np.asarray(target_strain_mask, bool)
     smask = ~mask
     tv = np.asarray(target_values, float)
@@ -783,28 +777,29 @@ def secant_simulation(d, law, lam, mu, p1, p2, target_strain_mask, target_values
     warm = None
     for s in range(n_steps + 1):
         tgt = tv * (s / n_steps)
+        prev_e = prev_s = None
         for _ in range(max_iter_macro):
             eb[mask] = tgt[mask]
-            info = {}
-            if law == "rose":
-                eps, sig, epsM, sigM = spectral_solver_rose_2d(lam, mu, p1, p2, eps_bar=eb, max_iter=max_iter_ls,
-                                                               tol=tol_ls, pixel=pixel, plane_mode=plane_mode,
-                                                               eps_init=warm, info=info)
-            elif d == 3:
-                eps, sig, epsM, sigM = spectral_solver_secant_3d(lam, mu, p1, p2, eb, max_iter=max_iter_ls,
-                                                                 tol=tol_ls, pixel=pixel, eps_init=warm, info=info)
-            else:
-                eps, sig, epsM, sigM = spectral_solver_secant_2d(lam, mu, p1, p2, eb, max_iter=max_iter_ls,
-                                                                 tol=tol_ls, pixel=pixel, plane_mode=plane_mode,
-                                                                 eps_init=warm, info=info)
-            its.append(info["iters"])
+            eps, sig, epsM, sigM, it = solve(eb, warm)
+            its.append(it)
             err = np.zeros((d, d))
             err[smask] = tgt[smask] - sigM[smask]
             mx = np.max(np.abs(err[smask])) if np.any(smask) else 0.0
             if mx < tol_macro:
                 break
             de = (err - nu_avg * np.trace(err) * np.eye(d)) / E_avg
-            eb[smask] += de[smask]
+            dsc = de[smask]
+            if secant_slope:
+                e_sc = eb[smask].copy()
+                s_sc = sigM[smask].copy()
+                if prev_e is not None:
+                    dh = e_sc - prev_e
+                    with np.errstate(divide="ignore", invalid="ignore"):
+                        slope = (s_sc - prev_s) / dh
+                    ok = np.isfinite(slope) & (np.abs(dh) > 1e-14) & (slope > 0.01 * E_avg)
+                    dsc[ok] = err[smask][ok] / slope[ok]
+                prev_e, prev_s = e_sc, s_sc
+            eb[smask] += dsc
         warm = eps
         em_l.append(epsM)
         sm_l.append(sigM)
@@ -812,3 +807,61 @@ def secant_simulation(d, law, lam, mu, p1, p2, target_strain_mask, target_values
             e_l.append(eps)
             s_l.append(sig)
     return np.array(em_l), np.array(sm_l), e_l, s_l, its
+
+
+def _medium_compliance(la, ma):
+    return ma * (3.0 * la + 2.0 * ma) / (la + ma), la / (2.0 * (la + ma))
+
+
+def secant_simulation(d, law, lam, mu, p1, p2, target_strain_mask, target_values, n_steps=20, pixel=1.0, tol_ls=1e-6,
+                      max_iter_ls=400, tol_macro=1e-4, max_iter_macro=20, store=True, plane_mode="plane_strain"):
+    """Mixed stress/strain load stepping of the secant-modulus laws
+    (linear_elastic_simulator.py:1353-1600 secant 2D, :1603-1855 Rose 2D,
+    :1857-2052 secant 3D): law "secant" (p1, p2 = d, k) or "rose" (eta,
+    floor); compliance of the reference medium (mean lam, mean mu), mu_ref for
+    the Rose law.  Returns (eps_macro, sig_macro, eps list, sig list, LS
+    iteration counts per solve)."""
+    lam = np.asarray(lam, float)
+    mu = np.asarray(mu, float)
+    la, ma = float(lam.mean()), float(mu.mean())
+    if law == "rose":
+        ma = 0.5 * (ma + p2 * ma)
+    E_avg, nu_avg = _medium_compliance(la, ma)
+
+    def solve(eb, warm):
+        info = {}
+        kw = dict(max_iter=max_iter_ls, tol=tol_ls, pixel=pixel, eps_init=warm, info=info)
+        if law == "rose":
+            r = spectral_solver_rose_2d(lam, mu, p1, p2, eps_bar=eb, plane_mode=plane_mode, **kw)
+        elif d == 3:
+            r = spectral_solver_secant_3d(lam, mu, p1, p2, eb, **kw)
+        else:
+            r = spectral_solver_secant_2d(lam, mu, p1, p2, eb, plane_mode=plane_mode, **kw)
+        return r + (info["iters"],)
+    return _macro_stepping(d, solve, E_avg, nu_avg, target_strain_mask, target_values, n_steps, tol_macro,
+                           max_iter_macro, store, False)
+
+
+def landau_simulation(d, lam, mu, v1, v2, v3, g1, g2, g3, g4, target_strain_mask, target_values, n_steps=20,
+                      pixel=1.0, tol_ls=1e-6, max_iter_ls=400, tol_macro=1e-4, max_iter_macro=20, store=True,
+                      plane_mode="plane_strain", anderson_m=5, **cap):
+    """Mixed stress/strain load stepping of the Landau law
+    (linear_elastic_simulator.py:2055-2284 in 2D, :2287-2516 in 3D): secant
+    slope correction, compliance of (mean lam, mean mu); in 2D plane stress
+    one e33 state carries across every solve of the run."""
+    lam = np.asarray(lam, float)
+    mu = np.asarray(mu, float)
+    E_avg, nu_avg = _medium_compliance(float(lam.mean()), float(mu.mean()))
+    e33_state = {} if (d == 2 and plane_mode == "plane_stress") else None
+
+    def solve(eb, warm):
+        info = {}
+        kw = dict(max_iter=max_iter_ls, tol=tol_ls, pixel=pixel, eps_init=warm, info=info, **cap)
+        if d == 3:
+            r = spectral_solver_landau_3d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eb, **kw)
+        else:
+            r = spectral_solver_landau_2d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eb, plane_mode=plane_mode,
+                                          e33_state=e33_state, anderson_m=anderson_m, **kw)
+        return r + (info["iters"],)
+    return _macro_stepping(d, solve, E_avg, nu_avg, target_strain_mask, target_values, n_steps, tol_macro,
+                           max_iter_macro, store, True)

@@ -535,3 +535,36 @@ def test_secant_driver_parity(gpu, law, d, plane_mode):
     assert _rel(got[0], ref[0]) < 1e-10 and _rel(got[1], ref[1]) < 1e-10
     for a, b in zip(got[2] + got[3], ref[2] + ref[3]):
         assert _rel(a, b) < 1e-10
+
+
+@pytest.mark.parametrize("d,plane_mode,cap", [(3, "plane_strain", {}), (2, "plane_stress", {}),
+                                              (2, "plane_strain", dict(strain_capping_enabled=True,
+                                                                       strain_capping_limit=0.03))])
+def test_landau_driver_parity(gpu, d, plane_mode, cap):
+    # Landau mixed-BC load stepping (secant-slope macro corrections, Anderson
+    # in 2D, e33 state across the run): macro lists and fields <= 1e-9
+    # relative, LS iteration counts per solve equal to the oracle driver's
+    from mgkmc_amd import linear_elastic_simulator as les
+    print(f"[landau driver] {d}D {plane_mode} {cap}", flush=True)
+    shape = (10, 8, 6) if d == 3 else (16, 12)
+    rng = np.random.default_rng(25)
+    E = 70e9 * rng.uniform(0.85, 1.15, shape)
+    nu = np.full(shape, 0.3)
+    lam, mu = ol.compute_lame_3d(E, nu) if d == 3 else ol.compute_lame_2d(E, nu, "plane_strain")
+    mask = ~np.eye(d, dtype=bool)
+    mask[0, 0] = True
+    tv = np.zeros((d, d))
+    tv[0, 0] = 0.02
+    ref = ol.landau_simulation(d, lam, mu, **LANDAU_PA, target_strain_mask=mask, target_values=tv, n_steps=3,
+                               tol_ls=1e-9, tol_macro=1e3, plane_mode=plane_mode, **cap)
+    info = {}
+    fn = les.landau_elastic_simulation_3d if d == 3 else les.landau_elastic_simulation_2d
+    kw = {} if d == 3 else dict(plane_mode=plane_mode)
+    got = fn(lam, mu, **LANDAU_PA, target_strain_mask=mask, target_values=tv, n_steps=3, tol_ls=1e-9, tol_macro=1e3,
+             info=info, **kw, **cap)
+    print(f"[landau driver] LS iterations {info['ls_iters']} (oracle {ref[4]})", flush=True)
+    assert len(info["ls_iters"]) == len(ref[4])
+    assert all(abs(a - b) <= 1 for a, b in zip(info["ls_iters"], ref[4]))
+    assert _rel(got[0], ref[0]) < 1e-9 and _rel(got[1], ref[1]) < 1e-9
+    for a, b in zip(got[2] + got[3], ref[2] + ref[3]):
+        assert _rel(a, b) < 1e-9

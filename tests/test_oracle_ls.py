@@ -492,3 +492,36 @@ def test_secant_driver_uniaxial_closed_form(law, d, plane_mode):
             assert abs(sm[s][k, k]) < 1.0
         assert abs(sm[s][0, 0] - stress(ex, t)[1]) < rt * abs(sm[s][0, 0])
     assert len(el) == 3 and el[-1].shape == shape + (d, d)
+
+
+@pytest.mark.parametrize("d,plane_mode", [(3, "plane_strain"), (2, "plane_strain"), (2, "plane_stress")])
+def test_landau_driver_uniaxial_closed_form(d, plane_mode):
+    # homogeneous Landau medium: the secant-slope macro loop reaches the
+    # uniaxial state whose transverse strain t zeroes sigma_yy of the law
+    # (brentq on the single-voxel law)
+    from scipy.optimize import brentq
+    lam0, mu0 = 45e9, 27e9
+    shape = (6,) * d
+    lam = np.full(shape, lam0)
+    mu = np.full(shape, mu0)
+    mask = ~np.eye(d, dtype=bool)
+    mask[0, 0] = True
+    tv = np.zeros((d, d))
+    tv[0, 0] = 0.02
+    em, sm, el, sl, its = ol.landau_simulation(d, lam, mu, **LANDAU_PA, target_strain_mask=mask, target_values=tv,
+                                               n_steps=2, tol_ls=1e-10, tol_macro=1.0, plane_mode=plane_mode)
+
+    def stress(ex, t):
+        e = np.diag([ex] + [t] * (d - 1)).reshape((1,) * d + (d, d))
+        one = np.ones((1,) * d)
+        if d == 3:
+            return ol.stress_landau_3d(e, lam0 * one, mu0 * one, **LANDAU_PA).reshape(d, d)
+        return ol.stress_landau_2d(e, lam0 * one, mu0 * one, **LANDAU_PA, plane_mode=plane_mode).reshape(d, d)
+
+    for s, ex in ((1, 0.01), (2, 0.02)):
+        t = brentq(lambda t: stress(ex, t)[1, 1], -0.05, 0.05, xtol=1e-18)
+        assert abs(em[s][0, 0] - ex) < 1e-15
+        for k in range(1, d):
+            assert abs(em[s][k, k] - t) < 1e-7 * abs(t)
+            assert abs(sm[s][k, k]) < 1.0
+        assert abs(sm[s][0, 0] - stress(ex, t)[0, 0]) < 1e-7 * abs(sm[s][0, 0])
