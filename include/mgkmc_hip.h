@@ -208,10 +208,13 @@ int mgk_find_below(long long n, const double* a, double thr, long long* idx_out,
 /* apply_flip_soa (mgkmc/stz/cascade.py:43-113; 2D kmc_simulator_functions.py:183-218)
  * on device state; also sets prev (if not NULL) to the mode, as the callers do
  * (mgkmc/aqs.py:999, kmc_simulator.py:1026).  epsp_shadow (FP64 epsp only, may be
- * NULL): an FP32 copy of epsp whose flipped voxel receives the rounded new values. */
+ * NULL): an FP32 copy of epsp whose flipped voxel receives the rounded new values.
+ * eps_div: 1 for the flip itself; N > 1 adds mode / N to epsp (the last sub-step of a
+ * sub-stepped flip, mgkmc/kmc_simulator.py:268-385, which also applies the softening,
+ * time and prev updates); -N adds mode / N only (the earlier sub-steps). */
 int mgk_kmc_flip(int ndim, const int* shape, int M, int cat_kind, long long v, int m, double time, double jp,
                  double jt, double gmax, double jn, const double* cat, void* epsp, int precision, float* epsp_shadow,
-                 double* soft, double* tlast, double* prev, double* gamma_out, void* stream);
+                 double* soft, double* tlast, double* prev, double* gamma_out, int eps_div, void* stream);
 
 /* Redraw of one voxel's modes (mgkmc/aqs.py:1007-1019): newcat [m][k], newq0 [m]. */
 int mgk_kmc_set_voxel(long long nvox, int M, int CK, long long v, const double* newcat, const double* newq0,

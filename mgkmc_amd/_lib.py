@@ -101,7 +101,7 @@ SIGNATURES = {
     "mgk_kmc_select_array": (_c_i, [ctypes.c_longlong, _c_p, _c_d, _c_p, _c_p, ctypes.POINTER(MgkKmcSel)]),
     "mgk_find_below": (_c_i, [ctypes.c_longlong, _c_p, _c_d, _c_p, _c_p, ctypes.POINTER(ctypes.c_longlong)]),
     "mgk_kmc_flip": (_c_i, [_c_i, ctypes.POINTER(_c_i), _c_i, _c_i, ctypes.c_longlong, _c_i, _c_d, _c_d, _c_d,
-                            _c_d, _c_d, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
+                            _c_d, _c_d, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i, _c_p]),
     "mgk_kmc_set_voxel": (_c_i, [ctypes.c_longlong, _c_i, _c_i, ctypes.c_longlong, _c_p, _c_p, _c_p, _c_p,
                                  _c_p]),
     "mgk_kmc_init_modes": (_c_i, [ctypes.c_longlong, _c_i, ctypes.c_uint64, _c_d, _c_d, _c_i, _c_d, _c_i, _c_d, _c_d,

@@ -31,15 +31,25 @@ class ThermalSimulation(DeviceKmc):
                           ["use_3d_barriers"] if use_3d_barriers else [])
         self.output_dir = output_dir
         self.strain_assumption = strain_assumption
+        self.hyperelastic_model = hyperelastic_model
+        self.v1, self.v2, self.v3, self.g1, self.g2, self.g3, self.g4 = v1, v2, v3, g1, g2, g3, g4
+        cap = dict(strain_capping_enabled=strain_capping_enabled, strain_capping_limit=strain_capping_limit,
+                   strain_capping_tangent_ratio=strain_capping_tangent_ratio,
+                   strain_capping_type=strain_capping_type, strain_capping_smooth_power=strain_capping_smooth_power)
+        for key, val in cap.items():
+            setattr(self, key, val)
+        # small strain: only "landau" leaves the linear solve (mgkmc/aqs.py:330-352)
+        law = ("landau", (v1, v2, v3, g1, g2, g3, g4), cap) if hyperelastic_model == "landau" else None
         self._setup((nx, ny, nz), M, gamma0, E_field, nu_field, pixel, barrier_generator, barrier_kwargs,
                     softening_scheme, softening_cap, jp, jt, neighbor_softening_fraction, temperature, strain_rate,
                     stability_threshold, redraw_directions, redraw_barriers, nu0, q_act_temp, cascade_mode,
                     scale_rate_by_volume, "plane_strain", ls_precision, ls_warm_start, rng, selection_stream,
-                    init_modes, dict(mean=2.0, std=0.6))
+                    init_modes, dict(mean=2.0, std=0.6), law=law)
 
     def update_stresses(self):
         """LS re-equilibration with the current eps_plastic and eps_macro
-        (mgkmc/aqs.py:345-351: update_stress_fft_full, cold start)."""
+        (mgkmc/aqs.py:330-351: update_stress_fft_full, cold start, or the
+        Landau solver for hyperelastic_model="landau")."""
         return self._solve(self.eps_macro)
 
     def mixed_bc(self, stress_targets, component, mixed_tol, mixed_max_iter):

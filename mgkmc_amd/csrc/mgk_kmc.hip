@@ -554,6 +554,7 @@ struct KmcFlip {
   long long v;
   int m;
   double time, jp, jt, gmax, jn;
+  int eps_div;  // eps_p += mode / |eps_div|; eps_div < 0: that increment only
 };
 
 template <int D, int CK, typename TE>
@@ -609,12 +610,14 @@ __global__ void k_kmc_flip(KmcFlip F, const double* __restrict__ cat, TE* __rest
       add[2] = 0.5 * (c[1] + c[2]);
     }
     constexpr int CS = (D == 3) ? 6 : 3;
+    const double div = (double)(F.eps_div < 0 ? -F.eps_div : F.eps_div);
 #pragma unroll
     for (int k = 0; k < CS; ++k) {
-      const TE nv = (TE)((double)epsp[k * NV + v] + add[k]);
+      const TE nv = (TE)((double)epsp[k * NV + v] + add[k] / div);
       epsp[k * NV + v] = nv;
       if (shadow) shadow[k * NV + v] = (float)nv;
     }
+    if (F.eps_div < 0) return;
     double gp = soft[v] + F.jp * s;
     if (F.gmax > 0 && gp > F.gmax) gp = F.gmax;
     soft[v] = gp;
@@ -629,7 +632,7 @@ __global__ void k_kmc_flip(KmcFlip F, const double* __restrict__ cat, TE* __rest
       for (int k = 0; k < DD; ++k) gamma_out[k] = c[k];
     }
   }
-  if (t == 0 && F.jn > 0.0) {
+  if (t == 0 && F.jn > 0.0 && F.eps_div > 0) {
     // neighbours in the reference's (dx, dy[, dz]) loop order, sequentially
     // (on small periodic grids several offsets hit the same voxel)
     for (int dx = -1; dx <= 1; ++dx)
@@ -939,8 +942,9 @@ extern "C" int mgk_find_below(long long n, const double* a, double thr, long lon
 extern "C" int mgk_kmc_flip(int ndim, const int* shape, int M, int cat_kind, long long v, int m, double time,
                             double jp, double jt, double gmax, double jn, const double* cat, void* epsp,
                             int precision, float* epsp_shadow, double* soft, double* tlast, double* prev, double* gamma_out,
-                            void* stream) {
-  if ((ndim != 2 && ndim != 3) || !shape || !cat || !epsp || !soft || !tlast || (precision != 0 && epsp_shadow)) {
+                            int eps_div, void* stream) {
+  if ((ndim != 2 && ndim != 3) || !shape || !cat || !epsp || !soft || !tlast || (precision != 0 && epsp_shadow) ||
+      eps_div == 0) {
     mgk_set_error("mgk_kmc_flip: bad argument", __FILE__, __LINE__);
     return MGK_ERR_ARG;
   }
@@ -956,6 +960,7 @@ extern "C" int mgk_kmc_flip(int ndim, const int* shape, int M, int cat_kind, lon
   F.jt = jt;
   F.gmax = gmax;
   F.jn = jn;
+  F.eps_div = eps_div;
   const long long NV = (long long)F.nx * F.ny * F.nz;
   if (v < 0 || v >= NV || m < 0 || m >= M) {
     mgk_set_error("mgk_kmc_flip: voxel/mode out of range", __FILE__, __LINE__);

@@ -165,13 +165,15 @@ class KmcState:
         _lib.check(rc, "mgk_kmc_select")
         return int(self.sel.item), int(self.sel.mode)
 
-    def flip(self, v, m, time, jp, jt, gmax, jn):
+    def flip(self, v, m, time, jp, jt, gmax, jn, eps_div=1):
+        """Flip mode m of voxel v; eps_div = N > 1 adds mode / N (last
+        sub-step of a sub-stepped flip), -N adds mode / N and nothing else."""
         rc = _lib.load().mgk_kmc_flip(self.d, self._shape_c, self.M, self.cat_kind, int(v), int(m), float(time),
                                       float(jp), float(jt), float(gmax), float(jn), _lib.dptr(self.cat),
                                       _lib.dptr(self.eps_p), self.eps_precision, _lib.dptr(self.eps_p32),
                                       _lib.dptr(self.soft),
                                       _lib.dptr(self.tlast), _lib.dptr(self.prev), _lib.dptr(self.gamma),
-                                      ctypes.c_void_p(dev.stream_ptr()))
+                                      int(eps_div), ctypes.c_void_p(dev.stream_ptr()))
         _lib.check(rc, "mgk_kmc_flip")
 
     def set_voxel(self, v, newcat=None, newq0=None):

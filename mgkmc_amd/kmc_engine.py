@@ -20,9 +20,11 @@ the random streams:
 with a counter-based generator instead (no reference counterpart: used by the
 serving loop, where host draws of M modes per voxel would dominate start-up).
 
-Supported: small strain, linear elasticity, KMC mode (cascade_mode=False),
-isotropic or directional softening, redraw of directions and barriers,
-planar (pure-shear) catalogs.  Options the device path does not cover raise
+Supported: small strain, KMC mode (cascade_mode=False), isotropic or
+directional softening, redraw of directions and barriers, planar (pure-shear)
+catalogs; linear elasticity, and the nonlinear laws of the reference's small-
+strain branches (3D Landau; 2D Landau, secant degradation and Rose, whose
+flips are sub-stepped on solver failure as the reference does).  Options the device path does not cover raise
 NotImplementedError instead of falling back to a CPU path.
 """
 import ctypes
@@ -32,7 +34,9 @@ import numpy as np
 from . import _lib
 from . import device as dev
 from .kmc_device import KmcState, params as kmc_params, SIG_SYM64, SIG_SYM32, modes_from_device
-from .linear_elastic_simulator import LsSolver, FP64, FP32, MIXED
+from .linear_elastic_simulator import (LsSolver, FP64, FP32, MIXED, LandauSolver, LandauSolver2D, SecantSolver,
+                                       SECANT_EXP, SECANT_ROSE, landau_params)
+from .elasticity import compute_lame_2d, compute_lame_3d
 from .stz import kmc as _kmcmod
 
 KB = 8.617e-5
@@ -101,8 +105,6 @@ class DeviceKmc:
             bad.append("enable_thermal")
         if strain_assumption != "small_strain":
             bad.append("finite_strain KMC")
-        if hyperelastic_model in ("landau", "secant_degradation", "rose"):
-            bad.append(f"hyperelastic_model={hyperelastic_model}")
         if bad:
             raise NotImplementedError("device KMC path does not cover: " + ", ".join(bad))
 
@@ -110,7 +112,7 @@ class DeviceKmc:
                softening_scheme, softening_cap, jp, jt, neighbor_softening_fraction, temperature, strain_rate,
                stability_threshold, redraw_directions, redraw_barriers, nu0, q_act_temp, cascade_mode,
                scale_rate_by_volume, plane_mode, ls_precision, ls_warm_start, rng, selection_stream, init_modes,
-               barrier_defaults):
+               barrier_defaults, law=None):
         self.rng = np.random if rng is None else rng
         self.selection_stream = _kmcmod.SELECTION_STREAM if selection_stream is None else selection_stream
         self.M, self.gamma0 = M, gamma0
@@ -134,9 +136,9 @@ class DeviceKmc:
         else:
             self.tau = np.inf
         self.grid_shape = tuple(int(s) for s in shape)
-        self.d = len(self.grid_shape)
+        self.ndim = len(self.grid_shape)
         self.total_voxels = int(np.prod(self.grid_shape))
-        d = self.d
+        d = self.ndim
         E = np.asarray(E_field, dtype=np.float64)
         if E.ndim == 0:
             E = np.full(self.grid_shape, float(E))
@@ -158,7 +160,13 @@ class DeviceKmc:
             raise ValueError(f"ls_precision must be one of {sorted(LS_PRECISIONS)}")
         self.precision = LS_PRECISIONS[ls_precision]
         self.sig_kind = SIG_SYM32 if self.precision == FP32 else SIG_SYM64
-        self.ls = LsSolver(self.E_field, self.nu_field, plane_mode=plane_mode, precision=self.precision)
+        self.law = law
+        if law is None:
+            self.ls = LsSolver(self.E_field, self.nu_field, plane_mode=plane_mode, precision=self.precision)
+        else:
+            if self.precision != FP64:
+                raise NotImplementedError("the nonlinear small-strain laws run with ls_precision='fp64'")
+            self.ls = self._law_solver(law)
         directional = softening_scheme == "directional"
         eps_prec = FP32 if self.precision == FP32 else FP64
         if init_modes is not None and init_modes[0] == "device":
@@ -186,21 +194,21 @@ class DeviceKmc:
     # ----- state mirrors (synchronised on access) --------------------------
     @property
     def sig_field(self):
-        return dev.from_sym_soa(dev.download(self.ls.sig), self.d)
+        return dev.from_sym_soa(dev.download(self.ls.sig), self.ndim)
 
     @property
     def eps_field(self):
-        return dev.from_sym_soa(dev.download(self.ls.eps), self.d)
+        return dev.from_sym_soa(dev.download(self.ls.eps), self.ndim)
 
     @property
     def eps_plastic(self):
-        return dev.from_sym_soa(dev.download(self.kmc.eps_p), self.d)
+        return dev.from_sym_soa(dev.download(self.kmc.eps_p), self.ndim)
 
     @eps_plastic.setter
     def eps_plastic(self, value):
         """Set the eigenstrain field from a host (..., d, d) array (its
         symmetric part is stored)."""
-        sym, _ = dev.to_sym_soa(np.asarray(value, dtype=np.float64), self.d)
+        sym, _ = dev.to_sym_soa(np.asarray(value, dtype=np.float64), self.ndim)
         self.kmc.eps_p.copy_(dev.upload(sym, FP64).to(self.kmc.eps_p.dtype))
         if self.kmc.eps_p32 is not None:
             self.kmc.eps_p32.copy_(self.kmc.eps_p.to(dev.torch.float32))
@@ -219,13 +227,35 @@ class DeviceKmc:
     @property
     def soft_prop(self):
         s = dev.download(self.kmc.soft)
-        out = np.zeros(self.grid_shape + ((4,) if self.d == 3 else (2,)))
+        out = np.zeros(self.grid_shape + ((4,) if self.ndim == 3 else (2,)))
         out[..., 0] = s[0].reshape(self.grid_shape)
         out[..., 1] = s[1].reshape(self.grid_shape)
         return out
 
+    def _law_solver(self, law):
+        """Device solver of a nonlinear small-strain law (the reference's
+        elastic_run / update_stresses branches, mgkmc/kmc_simulator.py:449-512,
+        mgkmc/aqs.py:331-348): ("landau", params) in 3D or 2D, ("secant", d, k),
+        ("rose", eta, floor).  Secant takes the Lame pair of the plane mode,
+        Rose and Landau the plane-strain (3D) pair."""
+        kind = law[0]
+        if self.ndim == 3:
+            if kind != "landau":
+                raise NotImplementedError(f"3D hyperelastic_model {kind!r} on the small-strain KMC path")
+            lam, mu = compute_lame_3d(self.E_field, self.nu_field)
+            return LandauSolver(lam, mu, FP64)
+        if kind == "secant":
+            lam, mu = compute_lame_2d(self.E_field, self.nu_field, plane_mode=self.plane_mode)
+            return SecantSolver(lam, mu, SECANT_EXP, law[1], law[2], self.plane_mode, FP64)
+        lam, mu = compute_lame_2d(self.E_field, self.nu_field, plane_mode="plane_strain")
+        if kind == "rose":
+            return SecantSolver(lam, mu, SECANT_ROSE, law[1], law[2], self.plane_mode, FP64)
+        return LandauSolver2D(lam, mu, self.plane_mode, 5)
+
     # ----- seams -----------------------------------------------------------
     def _solve(self, eps_macro):
+        if self.law is not None:
+            return self._solve_law(eps_macro)
         warm = self.ls_warm_start and self._solved_once
         args = dict(max_iter=200, tol=1e-6)
         args.update(self.solver_args)
@@ -234,8 +264,30 @@ class DeviceKmc:
         self._solved_once = True
         return self.ls_info
 
+    def _solve_law(self, eps_macro):
+        """One nonlinear solve with the reference's defaults (max_iter 400,
+        tol 1e-6) overridden by solver_args: Landau 3D and the secant laws
+        start cold, Landau 2D from the current field with one e33 state for
+        the run; a non-finite Landau update raises FloatingPointError."""
+        args = dict(max_iter=400, tol=1e-6)
+        args.update(self.solver_args)
+        kind = self.law[0]
+        kw = dict(max_iter=args["max_iter"], tol=args["tol"])
+        if kind == "landau":
+            if not hasattr(self, "_law_params"):
+                self._law_params = landau_params(*self.law[1], mu=self.ls.mu_host, **self.law[2])
+            if self.ndim == 3:
+                self.ls_info = self.ls.solve(self._law_params, eps_macro, self.kmc.eps_p, warm=False, **kw)
+            else:
+                self.ls_info = self.ls.solve(self._law_params, eps_macro, self.kmc.eps_p, warm=True, persist_e33=True,
+                                             **kw)
+        else:
+            self.ls_info = self.ls.solve(eps_macro, self.kmc.eps_p, warm=False, **kw)
+        self._solved_once = True
+        return self.ls_info
+
     def sig_mean(self):
-        d = self.d
+        d = self.ndim
         return np.array(self.ls_info.sig_mean[: d * d]).reshape(d, d)
 
     def _params(self):
@@ -263,16 +315,11 @@ class DeviceKmc:
                 v, m = self.kmc.select(us, self.ls.sig, self.sig_kind, p)
                 coords = tuple(int(c) for c in self.kmc.voxel_coords(v))
                 is_instab = self.kmc.sel.q <= self.stability_threshold
-                self.kmc.flip(v, m, self.time, self.jp, self.jt, self.softening_cap, self.neighbor_softening_fraction)
-                if not (self.redraw_directions or self.redraw_barriers):
-                    raise NotImplementedError("single-mode redraw")
-                newcat = newq0 = None
-                if self.redraw_directions:
-                    newcat = planar_to_full(planar_catalog(self.M, self.gamma0, self.rng), self.d)
-                if self.redraw_barriers:
-                    newq0 = self.barrier_generator((self.M,))
-                self.kmc.set_voxel(v, newcat, newq0)
-                self._after_flip_solve()
+                if self.law is not None and self.ndim == 2:
+                    self._flip_substepped(v, m)
+                else:
+                    self._flip(v, m, 1)
+                    self._after_flip_solve()
                 ev = coords + (m, self.time, is_instab)
                 self.flip_event_history.append(ev)
                 return ev, remaining_time
@@ -280,12 +327,70 @@ class DeviceKmc:
         self.time += remaining_time
         return None, 0.0
 
+    def _flip(self, v, m, eps_div):
+        """Flip (eps_div = 1), or the last sub-step of a sub-stepped flip
+        (eps_div = N: mode / N), then the redraw of the voxel's modes."""
+        self.kmc.flip(v, m, self.time, self.jp, self.jt, self.softening_cap, self.neighbor_softening_fraction,
+                      eps_div)
+        if not (self.redraw_directions or self.redraw_barriers):
+            raise NotImplementedError("single-mode redraw")
+        newcat = newq0 = None
+        if self.redraw_directions:
+            newcat = planar_to_full(planar_catalog(self.M, self.gamma0, self.rng), self.ndim)
+        if self.redraw_barriers:
+            newq0 = self.barrier_generator((self.M,))
+        self.kmc.set_voxel(v, newcat, newq0)
+
+    def _mode_host(self, v, m):
+        c = dev.download(self.kmc.cat[:, m, v])
+        if self.kmc.planar:
+            return planar_to_full(c[None, :], self.ndim)[0]
+        return c.reshape(self.ndim, self.ndim)
+
+    def _flip_substepped(self, v, m):
+        """Flip with the reference's small-strain sub-stepping
+        (mgkmc/kmc_simulator.py:263-385): one step first; if the re-solve
+        raises ValueError or FloatingPointError, restore the state (eigenstrain,
+        softening, event times, prev, modes, fields) and retry with N sub-steps
+        of mode / N, N = max(N_h, 2) .. +19 (N_h = min(max(2, ceil(sqrt(C:C/2) /
+        0.02)), 40)), softening, prev and redraws at the last, a re-solve after
+        each.  Every attempt draws its redraws anew from the host stream."""
+        k = self.kmc
+        live = [k.eps_p, k.soft, k.tlast, k.prev, k.cat, k.Q0, self.ls.eps, self.ls.sig]
+        backup = [None if a is None else a.clone() for a in live]
+        schedule = [1]
+        for N in schedule:
+            if N > 1:
+                for a, b in zip(live, backup):
+                    if a is not None:
+                        a.copy_(b)
+            try:
+                for i in range(1, N + 1):
+                    if i < N:
+                        k.flip(v, m, self.time, self.jp, self.jt, self.softening_cap,
+                               self.neighbor_softening_fraction, -N)
+                    else:
+                        self._flip(v, m, N)
+                    self.update_stresses()
+                self.last_substeps = N
+                return
+            except (ValueError, FloatingPointError):
+                if N == 1:
+                    for a, b in zip(live, backup):
+                        if a is not None:
+                            a.copy_(b)
+                    C = self._mode_host(v, m)
+                    nh = min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.02))), 40)
+                    schedule.extend(range(max(nh, 2), max(nh, 2) + 20))
+        raise ValueError("Mechanical solver failed to converge under small-strain sub-stepping.")
+
     def footprint_bytes(self):
         """HBM bytes held by this simulation's device state."""
         t = 0
         for x in (self.ls.lam, self.ls.mu, self.ls.eps, self.ls.sig, self.ls.ws,
                   getattr(self.ls, "lam32", None), getattr(self.ls, "mu32", None), getattr(self.ls, "eps32", None),
-                  getattr(self.ls, "ws32", None), self.kmc.Q0, self.kmc.Q, self.kmc.cat, self.kmc.soft,
+                  getattr(self.ls, "ws32", None), getattr(self.ls, "tau", None), getattr(self.ls, "e33", None),
+                  getattr(self.ls, "aa", None), self.kmc.Q0, self.kmc.Q, self.kmc.cat, self.kmc.soft,
                   self.kmc.tlast, self.kmc.prev, self.kmc.eps_p, self.kmc.eps_p32, self.kmc.ws.buf):
             if x is not None:
                 t += x.numel() * x.element_size()

@@ -3,10 +3,11 @@ KmcSimulation2D).
 
 Same constructor and run_simulation signatures as the reference
 (mgkmc/kmc_simulator.py:14-35, :796-803).  The per-event work of its KMC loop
-(:871-1101, small strain, linear elasticity) runs on the GPU through
-``mgkmc_amd.kmc_engine.DeviceKmc``; elastic_run (:387-525, the linear branch)
-is the device LS solve with the reference's plane mode and its
-unstable-stress check; the mixed-BC loop (:1106-1143) keeps the reference's
+(:871-1101, small strain) runs on the GPU through
+``mgkmc_amd.kmc_engine.DeviceKmc``; elastic_run (:387-525, the small-strain
+branches: linear, secant degradation, Rose, Landau) is the device LS solve of
+that law with the reference's plane mode and its unstable-stress check, and
+flips under a nonlinear law are sub-stepped on solver failure (:263-385); the mixed-BC loop (:1106-1143) keeps the reference's
 compliance-then-secant update on the host.  Extensions as in
 ``mgkmc_amd.aqs.ThermalSimulation``.
 """
@@ -36,16 +37,26 @@ class KmcSimulation2D(DeviceKmc):
         self.strain_assumption = strain_assumption
         self.hyperelastic_model = hyperelastic_model
         self.stz_mode = stz_mode
+        # the reference's names: d, k (secant degradation), eta, mu_floor_fraction (Rose), v*, g* (Landau)
+        self.d, self.k, self.eta, self.mu_floor_fraction = d, k, eta, mu_floor_fraction
+        self.v1, self.v2, self.v3, self.g1, self.g2, self.g3, self.g4 = v1, v2, v3, g1, g2, g3, g4
+        cap = dict(strain_capping_enabled=strain_capping_enabled, strain_capping_limit=strain_capping_limit,
+                   strain_capping_tangent_ratio=strain_capping_tangent_ratio,
+                   strain_capping_type=strain_capping_type, strain_capping_smooth_power=strain_capping_smooth_power)
+        for key, val in cap.items():
+            setattr(self, key, val)
+        law = {"secant_degradation": ("secant", d, k), "rose": ("rose", eta, mu_floor_fraction),
+               "landau": ("landau", (v1, v2, v3, g1, g2, g3, g4), cap)}.get(hyperelastic_model)
         self._setup((nx, ny), M, gamma0, E_field, nu_field, pixel, barrier_generator, barrier_kwargs,
                     softening_scheme, softening_cap, jp, jt, neighbor_softening_fraction, temperature, strain_rate,
                     stability_threshold, redraw_directions, redraw_barriers, nu0, q_act_temp, cascade_mode,
                     scale_rate_by_volume, plane_mode, ls_precision, ls_warm_start, rng, selection_stream,
-                    init_modes, dict(mean=2.0, std=0.6, min_cutoff=0.1))
+                    init_modes, dict(mean=2.0, std=0.6, min_cutoff=0.1), law=law)
 
     def elastic_run(self, eps_macro):
-        """Small-strain linear elastic equilibrium (mgkmc/kmc_simulator.py:515-525):
-        LS solve, then ValueError on a NaN stress or |<sigma>| > 20 GPa.
-        Returns <sigma> (2, 2)."""
+        """Small-strain equilibrium (mgkmc/kmc_simulator.py:449-525): LS solve
+        of the linear or nonlinear law, then ValueError on a NaN stress or
+        |<sigma>| > 20 GPa.  Returns <sigma> (2, 2)."""
         inf = self._solve(eps_macro)
         sig_mean = self.sig_mean()
         if inf.status == _lib.MGK_ERR_NONFINITE or not np.all(np.isfinite(sig_mean)) or \

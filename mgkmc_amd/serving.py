@@ -163,7 +163,7 @@ def run_request(req):
     with torch.cuda.stream(s):
         sim = build_simulation(req)
         fp = sim.footprint_bytes()
-        d = sim.d
+        d = sim.ndim
         unit = np.zeros((d, d))
         unit[0, 0] = 1.0
         iters = 0

@@ -10,6 +10,17 @@ np.random.rand, selection, flip, prev_strain_dir, redraw of directions then
 barriers, re-solve) and :1103-1143 (mixed-BC loop with the compliance step and
 the per-component secant update).
 
+Nonlinear small-strain models (``model`` = "secant_degradation", "rose",
+"landau"): elastic_run calls the matching LS solver (:449-512: secant with the
+Lame pair of the plane mode, Rose and Landau with the plane-strain pair;
+Landau warm-started from the current field with one e33 state for the run),
+and a flip goes through the sub-stepping of :268-385 (first one step; if the
+re-solve raises ValueError or FloatingPointError, the state is restored and the
+flip is retried in N = max(N_h, 2) .. +19 sub-steps of C/N each, softening,
+prev and redraws at the last, N_h = min(max(2, ceil(sqrt(C:C/2)/0.02)), 40)).
+``fail_once_at`` (test hook) makes the first re-solve after that event's flip
+raise ValueError, to drive the sub-stepped path.
+
 Random streams: numpy's global legacy stream as the reference uses it, and a
 separate RandomState for the selection draw r = total * u (the reference's
 numba-internal draw, made seedable).
@@ -26,7 +37,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                   seed, sel_seed, barrier_kwargs=None, softening_scheme="isotropic", softening_cap=2.0, jp=10.0,
                   jt=30.0, jn=0.0, nu0=1e13, q_act_temp=0.37, plane_mode="plane_strain", ls_tol=1e-6,
                   stress_targets=None, mixed_tol=1e-4, mixed_max_iter=50, n_global_steps=10 ** 9,
-                  eps_plastic=None):
+                  eps_plastic=None, model="linear", model_params=None, fail_once_at=None):
     """Returns (events [(x, y, m, time, is_instab)], final state dict)."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
@@ -56,14 +67,88 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
     state = {}
     targets = dict(stress_targets or {})
 
+    mp = dict(model_params or {})
+    if model == "secant_degradation":
+        lam, mu = ol.compute_lame_2d(E, nu, plane_mode)
+    elif model in ("rose", "landau"):
+        lam, mu = ol.compute_lame_2d(E, nu, "plane_strain")
+    elif model != "linear":
+        raise ValueError(model)
+    state["eps"] = np.zeros((nx, ny, 2, 2))
+    e33_state = {}
+    inject = {"armed": False}
+
     def elastic_run():
-        e, s, _, _ = ol.spectral_solver_2d(E, nu, eps_macro, eps_plastic=ep, pixel=pixel, plane_mode=plane_mode,
-                                           tol=ls_tol)
+        if inject["armed"]:
+            inject["armed"] = False
+            raise ValueError("injected solver failure")
+        if model == "linear":
+            e, s, _, _ = ol.spectral_solver_2d(E, nu, eps_macro, eps_plastic=ep, pixel=pixel, plane_mode=plane_mode,
+                                               tol=ls_tol)
+        elif model == "secant_degradation":
+            e, s, _, _ = ol.spectral_solver_secant_2d(lam, mu, mp["d"], mp["k"], eps_macro, eps_plastic=ep,
+                                                      tol=ls_tol, pixel=pixel, plane_mode=plane_mode)
+        elif model == "rose":
+            e, s, _, _ = ol.spectral_solver_rose_2d(lam, mu, mp["eta"], mp["mu_floor_fraction"], eps_macro,
+                                                    eps_plastic=ep, tol=ls_tol, pixel=pixel, plane_mode=plane_mode)
+        else:
+            lp = {k: mp[k] for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4")}
+            cap = {k: v for k, v in mp.items() if k.startswith("strain_capping")}
+            e, s, _, _ = ol.spectral_solver_landau_2d(lam, mu, **lp, eps_bar=eps_macro, eps_plastic=ep, tol=ls_tol,
+                                                      pixel=pixel, plane_mode=plane_mode, e33_state=e33_state,
+                                                      eps_init=state["eps"], **cap)
+        state["eps"] = e
         state["sig"] = s
         sm = s.mean(axis=(0, 1))
         if np.any(np.isnan(s)) or np.max(np.abs(sm)) / 1e9 > 20.0:
             raise ValueError("Unstable stress in small-strain elastic_run")
         return sm
+
+    def soften(x, y, c):
+        sq = ok.vm_sq_2d(c)
+        gp = soft[x, y, 0] + jp * sq
+        if softening_cap > 0 and gp > softening_cap:
+            gp = softening_cap
+        soft[x, y, 0] = gp
+        soft[x, y, 1] = jt * sq
+        tl[x, y] = t
+        if jn > 0.0:
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    if dx == 0 and dy == 0:
+                        continue
+                    a, b = (x + dx + nx) % nx, (y + dy + ny) % ny
+                    g = soft[a, b, 0] + jn * jp * sq
+                    if softening_cap > 0 and g > softening_cap:
+                        g = softening_cap
+                    soft[a, b, 0] = g
+                    soft[a, b, 1] += jn * jt * sq
+
+    def flip_substepped(x, y, m):
+        arrays = (ep, soft, tl, prev, cat, Q0)
+        backup = [a.copy() for a in arrays]
+        b_eps, b_sig = state["eps"].copy(), state["sig"].copy()
+        c0 = cat[x, y, m]
+        nh = min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(c0 ** 2)) / 0.02))), 40)
+        for N in [1] + list(range(max(nh, 2), max(nh, 2) + 20)):
+            for a, b in zip(arrays, backup):
+                a[...] = b
+            state["eps"], state["sig"] = b_eps.copy(), b_sig.copy()
+            try:
+                for k in range(1, N + 1):
+                    c = cat[x, y, m].copy()
+                    ep[x, y] += c / N
+                    if k == N:
+                        soften(x, y, c)
+                        prev[x, y] = c
+                        cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
+                        Q0[x, y] = gen((M,))
+                    elastic_run()
+                state.setdefault("substeps", []).append(N)
+                return
+            except (ValueError, FloatingPointError):
+                continue
+        raise ValueError("Mechanical solver failed to converge under small-strain sub-stepping.")
 
     unit = np.zeros((2, 2))
     unit[component] = 1.0
@@ -75,7 +160,8 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
     E_avg, nu_avg = E.mean(), nu.mean()
 
     def result():
-        return events, dict(eps_plastic=ep, Q0=Q0, soft=soft, sig=state["sig"], eps_macro=eps_macro.copy(), time=t)
+        return events, dict(eps_plastic=ep, Q0=Q0, soft=soft, sig=state["sig"], eps_macro=eps_macro.copy(), time=t,
+                            substeps=state.get("substeps"))
 
     while done < n_global_steps:
         remaining += dt_step
@@ -91,12 +177,17 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     i = ok.select_event(rates, total, total * sel.random_sample())
                     x, y, m = ok.decode_index_2d(idx[i], ny, M)
                     inst = Q[x, y, m] <= 0.0
-                    C = cat[x, y, m].copy()
-                    ok.apply_flip_soa_2d(ep, soft, tl, cat, x, y, m, t, jp, jt, softening_cap, jn)
-                    prev[x, y] = C
-                    cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
-                    Q0[x, y] = gen((M,))
-                    elastic_run()
+                    if fail_once_at is not None and len(events) == fail_once_at:
+                        inject["armed"] = True
+                    if model == "linear":
+                        C = cat[x, y, m].copy()
+                        ok.apply_flip_soa_2d(ep, soft, tl, cat, x, y, m, t, jp, jt, softening_cap, jn)
+                        prev[x, y] = C
+                        cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
+                        Q0[x, y] = gen((M,))
+                        elastic_run()
+                    else:
+                        flip_substepped(x, y, m)
                     events.append((int(x), int(y), int(m), t, bool(inst)))
                     if len(events) >= n_events:
                         return result()

@@ -266,3 +266,91 @@ def test_kmc2d_athermal_aqs_with_eigenstrain_matches_oracle(gpu):
     assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
     assert np.abs(sim.eps_macro - st_ref["eps_macro"]).max() <= 1e-9 * np.abs(st_ref["eps_macro"]).max()
     assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-9 * np.abs(st_ref["sig"]).max()
+
+
+_NL_LANDAU = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9)
+
+
+@pytest.mark.parametrize("model,plane_mode,mp,fail", [
+    ("secant_degradation", "plane_strain", dict(d=0.5, k=30.0), None),
+    ("secant_degradation", "plane_stress", dict(d=0.5, k=30.0), 5),
+    ("rose", "plane_stress", dict(eta=25.0, mu_floor_fraction=0.1), None),
+    ("landau", "plane_strain", _NL_LANDAU, None),
+    ("landau", "plane_stress", _NL_LANDAU, 3),
+])
+def test_kmc2d_nonlinear_law_sequence_matches_oracle(gpu, model, plane_mode, mp, fail):
+    # KmcSimulation2D with a nonlinear small-strain law: the same 20 events as
+    # the oracle loop; with ``fail`` the one-step re-solve after that event's
+    # flip is made to fail (host-side injection, on both sides) and the flip is
+    # redone in sub-steps
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    from oracle.kmc2d_loop import run_events_2d
+    print(f"[kmc2d nonlinear] {model} {plane_mode} fail={fail}", flush=True)
+    n, M = 16, 20
+    rng = np.random.default_rng(2)
+    mu = 26.923e9 * rng.uniform(0.9, 1.1, (n, n))
+    nu = np.full((n, n), 0.35)
+    E = 2 * mu * (1 + nu)
+    kw = dict(mean=2.2, std=0.45, min_cutoff=0.0)
+    args = dict(softening_cap=0.916, jp=20.0, jt=0.0)
+    ev_ref, st_ref = run_events_2d(n, n, M, 0.14, E, nu, 0.7, 1276.5, 1e7, 1e-2, (0, 0), 20, seed=21, sel_seed=22,
+                                   barrier_kwargs=kw, plane_mode=plane_mode, stress_targets={(1, 1): 0.0},
+                                   mixed_tol=1e6, model=model, model_params=mp, fail_once_at=fail, ls_tol=1e-8,
+                                   **args)
+    sim = KmcSimulation2D(n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                          temperature=1276.5, strain_rate=1e7, plane_mode=plane_mode, hyperelastic_model=model,
+                          rng=np.random.RandomState(21), selection_stream=np.random.RandomState(22), **mp, **args)
+    sim.solver_args = {"tol": 1e-8}
+    armed = [False]
+    substeps = []
+    real_run, real_fs = sim.elastic_run, sim._flip_substepped
+
+    def elastic_run(eps_macro):
+        if armed[0]:
+            armed[0] = False
+            raise ValueError("injected solver failure")
+        return real_run(eps_macro)
+
+    def flip_substepped(v, m):
+        if fail is not None and len(sim.flip_event_history) == fail:
+            armed[0] = True
+        real_fs(v, m)
+        substeps.append(sim.last_substeps)
+    sim.elastic_run = elastic_run
+    sim._flip_substepped = flip_substepped
+    ev = sim.run_simulation(10 ** 9, 1e-2, component=(0, 0), stress_targets={(1, 1): 0.0}, mixed_tol=1e6,
+                            max_events=20)
+    print("events:", [e[:3] for e in ev[:8]], "substeps:", substeps, flush=True)
+    assert substeps == st_ref["substeps"]
+    assert [e[:3] for e in ev] == [e[:3] for e in ev_ref]
+    assert np.allclose([e[3] for e in ev], [e[3] for e in ev_ref], rtol=1e-9)
+    assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
+    assert np.abs(sim.eps_macro - st_ref["eps_macro"]).max() <= 1e-9 * np.abs(st_ref["eps_macro"]).max()
+    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-8 * np.abs(st_ref["sig"]).max()
+
+
+def test_kmc3d_landau_sequence_matches_linear_limit_and_runs(gpu):
+    # ThermalSimulation with hyperelastic_model="landau": with all Landau
+    # coefficients zero the law is linear, so the event sequence equals the
+    # linear simulation's; with the coefficients on, events run and the stress
+    # stays finite
+    from mgkmc_amd.aqs import ThermalSimulation
+    n, M = 12, 8
+    rng = np.random.default_rng(4)
+    E = 70e9 * rng.uniform(0.9, 1.1, (n, n, n))
+    nu = np.full((n, n, n), 0.3)
+    kw = dict(mean=2.0, std=0.5, min_cutoff=0.0)
+    base = dict(pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw, temperature=1200.0, strain_rate=1e7)
+    zero = {k: 0.0 for k in _NL_LANDAU}
+    runs = []
+    for model, prm in (("svk", zero), ("landau", zero)):
+        sim = ThermalSimulation(n, n, n, M, 0.1, E, nu, hyperelastic_model=model, rng=np.random.RandomState(3),
+                                selection_stream=np.random.RandomState(4), **prm, **base)
+        sim.solver_args = {"tol": 1e-10}
+        runs.append([e[:4] for e in sim.run_simulation(10 ** 9, 1e-3, max_events=15)])
+    print("[kmc3d landau] linear limit events:", runs[1][:5], flush=True)
+    assert runs[0] == runs[1]
+    sim = ThermalSimulation(n, n, n, M, 0.1, E, nu, hyperelastic_model="landau", rng=np.random.RandomState(3),
+                            selection_stream=np.random.RandomState(4), **_NL_LANDAU, **base)
+    ev = sim.run_simulation(10 ** 9, 1e-3, max_events=10)
+    assert len(ev) == 10 and np.all(np.isfinite(sim.sig_mean()))

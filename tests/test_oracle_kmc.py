@@ -232,3 +232,31 @@ def test_oracle_2d_loop_invariants():
     assert len(ev) == 6 and all(b > a for a, b in zip(times, times[1:]))
     assert abs(st["eps_macro"][0, 0] - 1e7 * st["time"]) <= 1e-12 * abs(st["eps_macro"][0, 0])
     assert np.all(np.abs(st["eps_plastic"]) < 1.0)
+
+
+def test_oracle_2d_substepped_flip():
+    # a nonlinear-law flip whose one-step re-solve fails (injected) is redone in
+    # N >= 2 sub-steps of C/N: the eigenstrain reaches the one-step value to
+    # round-off, softening and event times are identical, the voxel's new modes
+    # come from fresh draws (the failed attempt's draws are discarded)
+    from oracle.kmc2d_loop import run_events_2d
+    n, M = 8, 6
+    rng = np.random.default_rng(3)
+    mu = 26.9e9 * rng.uniform(0.9, 1.1, (n, n))
+    nu = np.full((n, n), 0.35)
+    E = 2 * mu * (1 + nu)
+    common = dict(barrier_kwargs=dict(mean=2.2, std=0.45, min_cutoff=0.0), softening_cap=0.916, jp=20.0, jt=0.0,
+                  model="secant_degradation", model_params=dict(d=0.5, k=30.0))
+    args = (n, n, M, 0.14, E, nu, 0.7, 1276.5, 1e7, 1e-2, (0, 0), 4)
+    ev_a, st_a = run_events_2d(*args, seed=4, sel_seed=5, **common)
+    ev_b, st_b = run_events_2d(*args, seed=4, sel_seed=5, fail_once_at=3, **common)
+    assert st_a["substeps"] == [1, 1, 1, 1]
+    assert st_b["substeps"][:3] == [1, 1, 1] and st_b["substeps"][3] >= 2
+    assert [e[:3] for e in ev_a] == [e[:3] for e in ev_b]
+    x, y = ev_a[3][:2]
+    assert np.allclose(st_a["eps_plastic"], st_b["eps_plastic"], rtol=0, atol=1e-16)
+    assert np.array_equal(st_a["soft"], st_b["soft"])
+    assert not np.array_equal(st_a["Q0"][x, y], st_b["Q0"][x, y])
+    mask = np.ones((n, n), bool)
+    mask[x, y] = False
+    assert np.array_equal(st_a["Q0"][mask], st_b["Q0"][mask])
