@@ -45,6 +45,7 @@ class ThermalSimulation(DeviceKmc):
                     stability_threshold, redraw_directions, redraw_barriers, nu0, q_act_temp, cascade_mode,
                     scale_rate_by_volume, "plane_strain", ls_precision, ls_warm_start, rng, selection_stream,
                     init_modes, dict(mean=2.0, std=0.6), law=law)
+        self.max_cascade_steps_pct = max_cascade_steps_pct
 
     def update_stresses(self):
         """LS re-equilibration with the current eps_plastic and eps_macro
@@ -74,10 +75,17 @@ class ThermalSimulation(DeviceKmc):
         return converged
 
     def run_simulation(self, n_global_steps, step_size, component=(0, 1), stress_targets={}, mixed_tol=1e-4,
-                       mixed_max_iter=50, max_kmc_steps_pct=0.3, max_events=None, **kwargs):
-        """KMC + elastic increments (mgkmc/aqs.py:682-1150, KMC mode).  Logging,
+                       mixed_max_iter=50, max_kmc_steps_pct=0.3, max_cascade_steps_pct=0.3, max_events=None,
+                       **kwargs):
+        """KMC + elastic increments (mgkmc/aqs.py:682-1150).  Logging,
         checkpoint, VTK and stress-drop options of the reference are accepted
-        and ignored.  ``max_events`` (extension) stops after that many events."""
+        and ignored.  In cascade_mode the initial relaxation is a cascade (no
+        solve before it) and every inner pass first runs a cascade while any
+        mode is unstable; a truncated cascade ends the run.  Cascade flips go
+        to ``cascade_history``.  ``max_events`` (extension) stops after that
+        many KMC events."""
+        if max_cascade_steps_pct is not None:
+            self.max_cascade_steps_pct = max_cascade_steps_pct
         targets = dict(stress_targets)
         if not targets:
             if component == (0, 0):
@@ -86,7 +94,11 @@ class ThermalSimulation(DeviceKmc):
                 targets = {(0, 0): 0.0, (1, 1): 0.0, (2, 2): 0.0}
         unit = np.zeros((3, 3))
         unit[component] = 1.0
-        self.update_stresses()
+        if self.cascade_mode:
+            if self._run_cascade(0)[2]:
+                return self.flip_event_history
+        else:
+            self.update_stresses()
         dt_step = abs(step_size) / self.strain_rate if self.strain_rate > 0 else 1.0
         max_seq = int(max_kmc_steps_pct * self.total_voxels)
         seq = 0
@@ -96,6 +108,10 @@ class ThermalSimulation(DeviceKmc):
         while done < n_global_steps:
             remaining += dt_step
             while remaining > 0:
+                if self.cascade_mode and self._cascade_pairs():
+                    if self._run_cascade(done + 1)[2]:
+                        return self.flip_event_history
+                    continue
                 ev, remaining = self.kmc_event(remaining, unit)
                 if ev is None:
                     break

@@ -153,6 +153,32 @@ class KmcState:
         _lib.check(rc, "mgk_kmc_barrier_rates")
         return tot.value
 
+    def unstable_modes(self, threshold):
+        """(v, m, q) host arrays of the modes with stored Q < threshold, in the
+        voxel-major scan order of find_unstable (mgkmc/stz/cascade.py:4-41):
+        one compaction per mode row, then a sort by v * M + m."""
+        torch = dev.torch
+        if getattr(self, "_fb", None) is None:
+            self._fb = torch.empty(self.NV, dtype=torch.int64, device="cuda")
+        cnt = ctypes.c_longlong()
+        vs, ms = [], []
+        for m in range(self.M):
+            rc = _lib.load().mgk_find_below(self.NV, _lib.dptr(self.Q[m]), float(threshold), _lib.dptr(self._fb),
+                                            ctypes.c_void_p(dev.stream_ptr()), ctypes.byref(cnt))
+            _lib.check(rc, "mgk_find_below")
+            if cnt.value:
+                vs.append(self._fb[: cnt.value].clone())
+                ms.append(torch.full((cnt.value,), m, dtype=torch.int64, device="cuda"))
+        if not vs:
+            e = np.zeros(0, dtype=np.int64)
+            return e, e, np.zeros(0)
+        v = torch.cat(vs)
+        m = torch.cat(ms)
+        order = torch.argsort(v * self.M + m)
+        v, m = v[order], m[order]
+        q = self.Q[m, v]
+        return v.cpu().numpy(), m.cpu().numpy(), q.cpu().numpy()
+
     def select(self, u, sig, sig_kind, p):
         """(voxel, mode) of the event for draw u in [0, 1): r = total * u, after
         barrier_rates with the same sig and parameters.  self.sel.q holds the

@@ -20,7 +20,7 @@ the random streams:
 with a counter-based generator instead (no reference counterpart: used by the
 serving loop, where host draws of M modes per voxel would dominate start-up).
 
-Supported: small strain, KMC mode (cascade_mode=False), isotropic or
+Supported: small strain, KMC mode and cascade mode, isotropic or
 directional softening, redraw of directions and barriers, planar (pure-shear)
 catalogs; linear elasticity, and the nonlinear laws of the reference's small-
 strain branches (3D Landau; 2D Landau, secant degradation and Rose, whose
@@ -97,8 +97,6 @@ class DeviceKmc:
     def _unsupported(self, cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model,
                      extra=()):
         bad = list(extra)
-        if cascade_mode:
-            bad.append("cascade_mode")
         if fast_patching and fast_patching.get("enabled", False):
             bad.append("fast_patching")
         if enable_thermal:
@@ -189,6 +187,8 @@ class DeviceKmc:
         self.time = 0.0
         self.solver_args = {}
         self.flip_event_history = []
+        self.cascade_history = []  # (global step, batch, x, y[, z], m) per cascade flip
+        self.max_cascade_steps_pct = 0.3
         self.ls_info = None
 
     # ----- state mirrors (synchronised on access) --------------------------
@@ -316,7 +316,7 @@ class DeviceKmc:
                 coords = tuple(int(c) for c in self.kmc.voxel_coords(v))
                 is_instab = self.kmc.sel.q <= self.stability_threshold
                 if self.law is not None and self.ndim == 2:
-                    self._flip_substepped(v, m)
+                    self._flip_substepped([(v, m)])
                 else:
                     self._flip(v, m, 1)
                     self._after_flip_solve()
@@ -347,14 +347,15 @@ class DeviceKmc:
             return planar_to_full(c[None, :], self.ndim)[0]
         return c.reshape(self.ndim, self.ndim)
 
-    def _flip_substepped(self, v, m):
-        """Flip with the reference's small-strain sub-stepping
+    def _flip_substepped(self, flips):
+        """Flips [(v, m), ...] with the reference's small-strain sub-stepping
         (mgkmc/kmc_simulator.py:263-385): one step first; if the re-solve
         raises ValueError or FloatingPointError, restore the state (eigenstrain,
         softening, event times, prev, modes, fields) and retry with N sub-steps
-        of mode / N, N = max(N_h, 2) .. +19 (N_h = min(max(2, ceil(sqrt(C:C/2) /
-        0.02)), 40)), softening, prev and redraws at the last, a re-solve after
-        each.  Every attempt draws its redraws anew from the host stream."""
+        of mode / N per flip, N = max(N_h, 2) .. +19 (N_h = min(max over the
+        flips of max(2, ceil(sqrt(C:C/2) / 0.02)), 40)), softening, prev and
+        redraws at the last, one re-solve per sub-step.  Every attempt draws
+        its redraws anew from the host stream."""
         k = self.kmc
         live = [k.eps_p, k.soft, k.tlast, k.prev, k.cat, k.Q0, self.ls.eps, self.ls.sig]
         backup = [None if a is None else a.clone() for a in live]
@@ -366,11 +367,12 @@ class DeviceKmc:
                         a.copy_(b)
             try:
                 for i in range(1, N + 1):
-                    if i < N:
-                        k.flip(v, m, self.time, self.jp, self.jt, self.softening_cap,
-                               self.neighbor_softening_fraction, -N)
-                    else:
-                        self._flip(v, m, N)
+                    for v, m in flips:
+                        if i < N:
+                            k.flip(v, m, self.time, self.jp, self.jt, self.softening_cap,
+                                   self.neighbor_softening_fraction, -N)
+                        else:
+                            self._flip(v, m, N)
                     self.update_stresses()
                 self.last_substeps = N
                 return
@@ -379,10 +381,57 @@ class DeviceKmc:
                     for a, b in zip(live, backup):
                         if a is not None:
                             a.copy_(b)
-                    C = self._mode_host(v, m)
-                    nh = min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.02))), 40)
+                    nh = 2
+                    for v, m in flips:
+                        C = self._mode_host(v, m)
+                        nh = max(nh, max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.02))))
+                    nh = min(nh, 40)
                     schedule.extend(range(max(nh, 2), max(nh, 2) + 20))
         raise ValueError("Mechanical solver failed to converge under small-strain sub-stepping.")
+
+    # ----- cascades (cascade_mode) -------------------------------------------
+    def _cascade_pairs(self):
+        """Modes to flip in one cascade batch (mgkmc/aqs.py:489-510,
+        kmc_simulator.py:581-597): barriers of the current state, modes with
+        Q < stability_threshold sorted by Q (np.argsort over the voxel-major
+        scan, as the reference sorts), the first (most unstable) per voxel."""
+        self.kmc.barrier_rates(self.ls.sig, self.sig_kind, self._params(), write_q=True)
+        v, m, q = self.kmc.unstable_modes(self.stability_threshold)
+        pairs, seen = [], set()
+        for i in np.argsort(q):
+            if v[i] in seen:
+                continue
+            seen.add(v[i])
+            pairs.append((int(v[i]), int(m[i])))
+        return pairs
+
+    def _run_cascade(self, step):
+        """Cascade batches until no mode is unstable (mgkmc/aqs.py:480-680,
+        kmc_simulator.py:577-794, small strain): each batch flips its modes in
+        order (flip, prev, redraw; sub-stepped together under a 2D nonlinear
+        law), then one re-solve.  Returns (batches, flips, truncated): 3D stops
+        with truncated = True when the batch count exceeds
+        int(max_cascade_steps_pct * voxels); 2D stops after nx * ny + 1 batches."""
+        local = total = 0
+        while True:
+            pairs = self._cascade_pairs()
+            if not pairs:
+                break
+            if self.law is not None and self.ndim == 2:
+                self._flip_substepped(pairs)
+            else:
+                for v, m in pairs:
+                    self._flip(v, m, 1)
+                self.update_stresses()
+            for v, m in pairs:
+                self.cascade_history.append((step, local) + tuple(int(c) for c in self.kmc.voxel_coords(v)) + (m,))
+            total += len(pairs)
+            local += 1
+            if self.ndim == 3 and local > int(self.max_cascade_steps_pct * self.total_voxels):
+                return local, total, True
+            if self.ndim == 2 and local > self.total_voxels:
+                break
+        return local, total, False
 
     def footprint_bytes(self):
         """HBM bytes held by this simulation's device state."""

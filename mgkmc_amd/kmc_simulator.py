@@ -102,11 +102,15 @@ class KmcSimulation2D(DeviceKmc):
                     self.eps_macro[idx] += step
 
     def run_simulation(self, n_global_steps, step_size, component=(0, 0), stress_targets={}, mixed_tol=1e-4,
-                       mixed_max_iter=50, max_kmc_steps_pct=0.3, max_events=None, **kwargs):
-        """KMC + elastic increments (mgkmc/kmc_simulator.py:796-1143, KMC mode).
-        Logging, checkpoint and VTK options are accepted and ignored.
-        ``max_events`` (extension) stops after that many events.  Returns the
-        event list (x, y, m, time, is_instab)."""
+                       mixed_max_iter=50, max_kmc_steps_pct=0.3, max_cascade_steps_pct=0.3, max_events=None,
+                       **kwargs):
+        """KMC + elastic increments (mgkmc/kmc_simulator.py:796-1143).  Logging,
+        checkpoint and VTK options are accepted and ignored.  In cascade_mode
+        every inner pass first runs a cascade while any mode is unstable; the
+        run ends when more than int(max_cascade_steps_pct nx ny) consecutive
+        cascades flipped something.  Cascade flips go to ``cascade_history``.
+        ``max_events`` (extension) stops after that many KMC events.  Returns
+        the event list (x, y, m, time, is_instab)."""
         self.driving_component = component
         self.stress_targets = stress_targets
         unit = np.zeros((2, 2))
@@ -114,6 +118,8 @@ class KmcSimulation2D(DeviceKmc):
         self.elastic_run(self.eps_macro)
         dt_step = abs(step_size) / self.strain_rate if self.strain_rate > 0 else 1.0
         max_seq = int(max_kmc_steps_pct * self.nx * self.ny)
+        max_casc = int(max_cascade_steps_pct * self.nx * self.ny)
+        n_casc = 0
         seq = 0
         remaining = 0.0
         done = 0
@@ -121,6 +127,14 @@ class KmcSimulation2D(DeviceKmc):
         while done < n_global_steps:
             remaining += dt_step
             while remaining > 0:
+                if self.cascade_mode:
+                    if self._cascade_pairs():
+                        if self._run_cascade(done + 1)[1] > 0:
+                            n_casc += 1
+                        if n_casc > max_casc:
+                            return self.flip_event_history
+                        continue
+                    n_casc = 0
                 ev, remaining = self.kmc_event(remaining, unit)
                 if ev is None:
                     break

@@ -18,6 +18,14 @@ and a flip goes through the sub-stepping of :268-385 (first one step; if the
 re-solve raises ValueError or FloatingPointError, the state is restored and the
 flip is retried in N = max(N_h, 2) .. +19 sub-steps of C/N each, softening,
 prev and redraws at the last, N_h = min(max(2, ceil(sqrt(C:C/2)/0.02)), 40)).
+cascade_mode (:577-794, :870-888): after the initial solve, each pass of the
+inner loop first runs a cascade when any Q < stability_threshold: repeat
+{barriers; unstable modes sorted by Q; first mode per voxel; flip, prev and
+redraw each (sub-stepped together under a nonlinear law); one re-solve} until
+none is unstable or the batch count exceeds nx ny; the run stops when more than
+int(max_cascade_steps_pct nx ny) consecutive cascades flipped something; rates
+then count Q > 0 only.
+
 ``fail_once_at`` (test hook) makes the first re-solve after that event's flip
 raise ValueError, to drive the sub-stepped path.
 
@@ -37,7 +45,8 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                   seed, sel_seed, barrier_kwargs=None, softening_scheme="isotropic", softening_cap=2.0, jp=10.0,
                   jt=30.0, jn=0.0, nu0=1e13, q_act_temp=0.37, plane_mode="plane_strain", ls_tol=1e-6,
                   stress_targets=None, mixed_tol=1e-4, mixed_max_iter=50, n_global_steps=10 ** 9,
-                  eps_plastic=None, model="linear", model_params=None, fail_once_at=None):
+                  eps_plastic=None, model="linear", model_params=None, fail_once_at=None, cascade_mode=False,
+                  stability_threshold=0.0, max_cascade_steps_pct=0.3):
     """Returns (events [(x, y, m, time, is_instab)], final state dict)."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
@@ -124,25 +133,25 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     soft[a, b, 0] = g
                     soft[a, b, 1] += jn * jt * sq
 
-    def flip_substepped(x, y, m):
+    def flip_substepped(flips):
         arrays = (ep, soft, tl, prev, cat, Q0)
         backup = [a.copy() for a in arrays]
         b_eps, b_sig = state["eps"].copy(), state["sig"].copy()
-        c0 = cat[x, y, m]
-        nh = min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(c0 ** 2)) / 0.02))), 40)
+        nh = min(max(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(cat[x, y, m] ** 2)) / 0.02))) for x, y, m in flips), 40)
         for N in [1] + list(range(max(nh, 2), max(nh, 2) + 20)):
             for a, b in zip(arrays, backup):
                 a[...] = b
             state["eps"], state["sig"] = b_eps.copy(), b_sig.copy()
             try:
                 for k in range(1, N + 1):
-                    c = cat[x, y, m].copy()
-                    ep[x, y] += c / N
-                    if k == N:
-                        soften(x, y, c)
-                        prev[x, y] = c
-                        cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
-                        Q0[x, y] = gen((M,))
+                    for x, y, m in flips:
+                        c = cat[x, y, m].copy()
+                        ep[x, y] += c / N
+                        if k == N:
+                            soften(x, y, c)
+                            prev[x, y] = c
+                            cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
+                            Q0[x, y] = gen((M,))
                     elastic_run()
                 state.setdefault("substeps", []).append(N)
                 return
@@ -150,9 +159,43 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                 continue
         raise ValueError("Mechanical solver failed to converge under small-strain sub-stepping.")
 
+    cascades = []
+
+    def run_cascade(step):
+        local = total = 0
+        while True:
+            ok.compute_barrier_2d(Q, Q0, state["sig"], cat, volume, soft, tl, t, prev, softening_cap, scheme, tau)
+            un = ok.find_unstable_2d(Q, stability_threshold)
+            if len(un) == 0:
+                break
+            un = un[np.argsort(Q[un[:, 0], un[:, 1], un[:, 2]])]
+            seen = set()
+            flips = []
+            for x, y, m in un:
+                if (x, y) in seen:
+                    continue
+                seen.add((x, y))
+                flips.append((x, y, m))
+            if model == "linear":
+                for x, y, m in flips:
+                    ok.apply_flip_soa_2d(ep, soft, tl, cat, x, y, m, t, jp, jt, softening_cap, jn)
+                    prev[x, y] = cat[x, y, m].copy()
+                    cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
+                    Q0[x, y] = gen((M,))
+                elastic_run()
+            else:
+                flip_substepped(flips)
+            cascades.extend((step, local, int(x), int(y), int(m)) for x, y, m in flips)
+            total += len(flips)
+            local += 1
+            if local > nx * ny:
+                break
+        return total
+
     unit = np.zeros((2, 2))
     unit[component] = 1.0
     elastic_run()
+    n_casc = 0
     dt_step = abs(step_size) / strain_rate if strain_rate > 0 else 1.0
     events = []
     remaining = 0.0
@@ -161,13 +204,21 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
 
     def result():
         return events, dict(eps_plastic=ep, Q0=Q0, soft=soft, sig=state["sig"], eps_macro=eps_macro.copy(), time=t,
-                            substeps=state.get("substeps"))
+                            substeps=state.get("substeps"), cascades=cascades)
 
     while done < n_global_steps:
         remaining += dt_step
         while remaining > 0:
             ok.compute_barrier_2d(Q, Q0, state["sig"], cat, volume, soft, tl, t, prev, softening_cap, scheme, tau)
-            rates, idx, total = ok.compute_rates_2d(Q, 1.0, temperature, nu0, cascade_mode=False)
+            if cascade_mode:
+                if len(ok.find_unstable_2d(Q, stability_threshold)) > 0:
+                    if run_cascade(done + 1) > 0:
+                        n_casc += 1
+                    if n_casc > int(max_cascade_steps_pct * nx * ny):
+                        return result()
+                    continue
+                n_casc = 0
+            rates, idx, total = ok.compute_rates_2d(Q, 1.0, temperature, nu0, cascade_mode=cascade_mode)
             if total > 0:
                 tw = -np.log(np.random.rand()) / total
                 if tw < remaining:
@@ -176,7 +227,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     remaining -= tw
                     i = ok.select_event(rates, total, total * sel.random_sample())
                     x, y, m = ok.decode_index_2d(idx[i], ny, M)
-                    inst = Q[x, y, m] <= 0.0
+                    inst = Q[x, y, m] <= stability_threshold
                     if fail_once_at is not None and len(events) == fail_once_at:
                         inject["armed"] = True
                     if model == "linear":
@@ -187,7 +238,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                         Q0[x, y] = gen((M,))
                         elastic_run()
                     else:
-                        flip_substepped(x, y, m)
+                        flip_substepped([(x, y, m)])
                     events.append((int(x), int(y), int(m), t, bool(inst)))
                     if len(events) >= n_events:
                         return result()

@@ -311,10 +311,10 @@ def test_kmc2d_nonlinear_law_sequence_matches_oracle(gpu, model, plane_mode, mp,
             raise ValueError("injected solver failure")
         return real_run(eps_macro)
 
-    def flip_substepped(v, m):
+    def flip_substepped(flips):
         if fail is not None and len(sim.flip_event_history) == fail:
             armed[0] = True
-        real_fs(v, m)
+        real_fs(flips)
         substeps.append(sim.last_substeps)
     sim.elastic_run = elastic_run
     sim._flip_substepped = flip_substepped
@@ -354,3 +354,68 @@ def test_kmc3d_landau_sequence_matches_linear_limit_and_runs(gpu):
                             selection_stream=np.random.RandomState(4), **_NL_LANDAU, **base)
     ev = sim.run_simulation(10 ** 9, 1e-3, max_events=10)
     assert len(ev) == 10 and np.all(np.isfinite(sim.sig_mean()))
+
+
+@pytest.mark.parametrize("T,mean,steps,model,mp", [
+    (400.0, 0.9, 4, "linear", {}),
+    (0.0, 0.55, 6, "linear", {}),
+    (400.0, 0.8, 4, "secant_degradation", dict(d=0.3, k=20.0)),
+])
+def test_kmc2d_cascade_mode_matches_oracle(gpu, T, mean, steps, model, mp):
+    # cascade_mode: the cascade batches (order of flips included), the KMC
+    # events between them (rates over Q > 0) and the final fields equal the
+    # oracle loop's
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    from oracle.kmc2d_loop import run_events_2d
+    print(f"[kmc2d cascade] T={T} mean={mean} {model}", flush=True)
+    n, M = 16, 20
+    rng = np.random.default_rng(0)
+    mu = 26.923e9 * rng.uniform(0.9, 1.1, (n, n))
+    nu = np.full((n, n), 0.35)
+    E = 2 * mu * (1 + nu)
+    kw = dict(mean=mean, std=0.15, min_cutoff=0.0)
+    args = dict(softening_cap=0.916, jp=20.0, jt=0.0)
+    ev_ref, st_ref = run_events_2d(n, n, M, 0.14, E, nu, 0.7, T, 1e7, 1e-3, (0, 0), 10 ** 9, seed=5, sel_seed=6,
+                                   barrier_kwargs=kw, stress_targets={(1, 1): 0.0}, mixed_tol=1e6, ls_tol=1e-8,
+                                   n_global_steps=steps, cascade_mode=True, max_cascade_steps_pct=0.02,
+                                   model=model, model_params=mp, **args)
+    hm = model if model != "linear" else "svk"
+    sim = KmcSimulation2D(n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                          temperature=T, strain_rate=1e7, cascade_mode=True, hyperelastic_model=hm,
+                          rng=np.random.RandomState(5), selection_stream=np.random.RandomState(6), **mp, **args)
+    sim.solver_args = {"tol": 1e-8}
+    ev = sim.run_simulation(steps, 1e-3, component=(0, 0), stress_targets={(1, 1): 0.0}, mixed_tol=1e6,
+                            max_cascade_steps_pct=0.02)
+    print(f"[kmc2d cascade] events {len(ev)} (oracle {len(ev_ref)}), cascade flips {len(sim.cascade_history)} "
+          f"(oracle {len(st_ref['cascades'])})", flush=True)
+    assert len(st_ref["cascades"]) > 0
+    assert sim.cascade_history == st_ref["cascades"]
+    assert [e[:3] for e in ev] == [e[:3] for e in ev_ref]
+    assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
+    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-8 * np.abs(st_ref["sig"]).max()
+
+
+def test_kmc3d_cascade_mode_matches_oracle(gpu):
+    from mgkmc_amd.aqs import ThermalSimulation
+    from oracle.aqs_loop import run_events
+    n, M = 8, 10
+    rng = np.random.default_rng(1)
+    E = 70e9 * rng.uniform(0.9, 1.1, (n, n, n))
+    nu = np.full((n, n, n), 0.3)
+    kw = dict(mean=0.5, std=0.15, min_cutoff=0.0)
+    args = dict(softening_cap=0.916, jp=20.0, jt=0.0)
+    ev_ref, st_ref = run_events(n, n, n, M, 0.14, E, nu, 0.7, 800.0, 1e7, 1e-3, (0, 0), 20, seed=7, sel_seed=8,
+                                barrier_kwargs=kw, mixed_tol=1e6, ls_tol=1e-8, n_global_steps=5, cascade_mode=True,
+                                **args)
+    sim = ThermalSimulation(n, n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                            temperature=800.0, strain_rate=1e7, cascade_mode=True, rng=np.random.RandomState(7),
+                            selection_stream=np.random.RandomState(8), **args)
+    sim.solver_args = {"tol": 1e-8}
+    ev = sim.run_simulation(5, 1e-3, component=(0, 0), mixed_tol=1e6, max_events=20)
+    print(f"[kmc3d cascade] events {len(ev)} (oracle {len(ev_ref)}), cascade flips {len(sim.cascade_history)} "
+          f"(oracle {len(st_ref['cascades'])})", flush=True)
+    assert len(st_ref["cascades"]) > 0
+    assert sim.cascade_history == st_ref["cascades"]
+    assert [e[:4] for e in ev] == [e[:4] for e in ev_ref]
+    assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
+    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-8 * np.abs(st_ref["sig"]).max()
