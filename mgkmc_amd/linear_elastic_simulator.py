@@ -544,8 +544,8 @@ def _secant_drop_in(d, lam, mu, law, p1, p2, eps_bar, eps_plastic, max_iter, tol
     lam = np.asarray(lam, dtype=np.float64)
     mu = np.asarray(mu, dtype=np.float64)
     shape = lam.shape if lam.ndim == d else mu.shape
-    lam = np.broadcast_to(lam, shape)
-    mu = np.broadcast_to(mu, shape)
+    lam = np.array(np.broadcast_to(lam, shape))
+    mu = np.array(np.broadcast_to(mu, shape))
     eps_bar = np.asarray(eps_bar, dtype=np.float64)
     if np.any(eps_bar != eps_bar.T):
         raise ValueError("the device secant solvers take a symmetric eps_bar")
