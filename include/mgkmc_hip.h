@@ -287,6 +287,79 @@ int mgk_fs_constitutive(long long nvox, int precision, const void* F, const void
 /* sigma = P F^T / det F (cauchy_from_P_3d, mgkmc/finite_strain_simulator.py:1892-1904). */
 int mgk_fs_cauchy(long long nvox, int precision, const void* P, const void* F, void* sig, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Native KMC event loop.  Runs the per-event sequence of the reference's KMC loops
+ * (mgkmc/aqs.py:855-1087, mgkmc/kmc_simulator.py:871-1101: small strain, linear
+ * elasticity, KMC mode, planar catalogs) in C++: fused barrier + rate pass, waiting
+ * time from the host stream, selection from the selection stream, flip, redraw of the
+ * voxel's modes (2M normals, then M clipped Gaussian barriers), LS re-solve.  Random
+ * streams are numpy legacy RandomState states, continued here and handed back.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  uint32_t key[624];
+  int pos;         /* index of the next word (624: regenerate first) */
+  int has_gauss;   /* a cached second polar-method normal             */
+  double gauss;
+} mgk_mt_state;
+
+/* n draws from the stream: kind 0 random_sample(), kind 1 loc + scale * normal. */
+int mgk_mt_fill(mgk_mt_state* state, int kind, double loc, double scale, long long n, double* out);
+
+typedef struct {
+  int ndim;
+  int64_t shape[3];
+  int M;
+  /* KMC state (device; planar catalog [2][M][NV]) */
+  double *Q0, *cat, *soft, *tlast, *prev; /* prev may be NULL (isotropic) */
+  void* epsp;
+  int eps_precision;
+  float* epsp_shadow;                     /* FP32 copy of an FP64 epsp (refined LS form), or NULL */
+  void* kmc_ws;                           /* mgk_kmc_workspace_bytes */
+  double* redraw_dev;                     /* 3 M doubles of device scratch */
+  mgk_kmc_params kp;                      /* time is taken from *time */
+  double jp, jt, jn, stability_threshold;
+  int redraw_directions, redraw_barriers;
+  double gamma0, q_mean, q_std, q_min, q_max;
+  int q_has_min, q_has_max;
+  /* LS: precision 0 FP64, 1 FP32, 2 FP32 + FP64 refinement (mgk_ls_solve_refined) */
+  int ls_precision;
+  int sig_kind;                           /* of sig for the barrier pass */
+  const void *lam, *mu;
+  const float *lam32, *mu32;
+  double lam0, mu0;
+  void *eps, *sig;
+  float* eps32;
+  void* ws;
+  size_t ws_bytes;
+  void* ws32;
+  size_t ws32_bytes;
+  int warm, max_iter;
+  double tol, tol32;
+  int check_unstable;                     /* 2D elastic_run: NaN or |<sigma>| > 20 GPa stops the run */
+  double strain_rate;
+  double unit[9];                         /* d*d driving direction */
+} mgk_kmc_loop;
+
+typedef struct {
+  long long v;
+  int m;
+  int is_instab;
+  double time;
+  int ls_iters, ls_iters32;
+} mgk_kmc_event;
+
+#define MGK_KMC_RUN_TIME 0       /* no event within *remaining: time advanced by it, *remaining = 0 */
+#define MGK_KMC_RUN_COUNT 1      /* max_events events done                                         */
+#define MGK_KMC_RUN_SEQ 2        /* *seq (consecutive is_instab events) exceeded max_seq           */
+#define MGK_KMC_RUN_UNSTABLE -6  /* unstable stress after an event's re-solve (2D ValueError)      */
+
+/* Runs events until one of the conditions above; *time, eps_macro (d*d), *remaining,
+ * *seq and both streams are updated in place; events[0..*n_events) describe the
+ * events; info / iters32 hold the last solve's result. */
+int mgk_kmc_run(const mgk_kmc_loop* loop, mgk_mt_state* rng, mgk_mt_state* selection, double* time,
+                double* eps_macro, double* remaining, int max_events, int max_seq, int* seq,
+                mgk_kmc_event* events, int* n_events, mgk_ls_info* info, int* iters32, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@ MGK_ERR_HIP = -2
 MGK_ERR_WORKSPACE = -3
 MGK_ERR_NONFINITE = -4
 MGK_ERR_SOLVER = -5
+MGK_KMC_RUN_TIME, MGK_KMC_RUN_COUNT, MGK_KMC_RUN_SEQ, MGK_KMC_RUN_UNSTABLE = 0, 1, 2, -6
 
 _lib = None
 
@@ -48,6 +49,54 @@ class MgkKmcSel(ctypes.Structure):
     _fields_ = [("item", ctypes.c_longlong), ("mode", ctypes.c_int), ("fallthrough", ctypes.c_int),
                 ("r", ctypes.c_double), ("cum_before", ctypes.c_double), ("cum_at", ctypes.c_double),
                 ("q", ctypes.c_double)]
+
+
+class MgkMtState(ctypes.Structure):
+    _fields_ = [("key", ctypes.c_uint32 * 624), ("pos", ctypes.c_int), ("has_gauss", ctypes.c_int),
+                ("gauss", ctypes.c_double)]
+
+
+class MgkKmcLoop(ctypes.Structure):
+    pass
+
+
+class MgkKmcEvent(ctypes.Structure):
+    _fields_ = [("v", ctypes.c_longlong), ("m", ctypes.c_int), ("is_instab", ctypes.c_int), ("time", ctypes.c_double),
+                ("ls_iters", ctypes.c_int), ("ls_iters32", ctypes.c_int)]
+
+
+MgkKmcLoop._fields_ = [
+    ("ndim", ctypes.c_int), ("shape", ctypes.c_int64 * 3), ("M", ctypes.c_int),
+    ("Q0", ctypes.c_void_p), ("cat", ctypes.c_void_p), ("soft", ctypes.c_void_p), ("tlast", ctypes.c_void_p),
+    ("prev", ctypes.c_void_p), ("epsp", ctypes.c_void_p), ("eps_precision", ctypes.c_int),
+    ("epsp_shadow", ctypes.c_void_p), ("kmc_ws", ctypes.c_void_p), ("redraw_dev", ctypes.c_void_p),
+    ("kp", MgkKmcParams), ("jp", ctypes.c_double), ("jt", ctypes.c_double), ("jn", ctypes.c_double),
+    ("stability_threshold", ctypes.c_double), ("redraw_directions", ctypes.c_int), ("redraw_barriers", ctypes.c_int),
+    ("gamma0", ctypes.c_double), ("q_mean", ctypes.c_double), ("q_std", ctypes.c_double), ("q_min", ctypes.c_double),
+    ("q_max", ctypes.c_double), ("q_has_min", ctypes.c_int), ("q_has_max", ctypes.c_int),
+    ("ls_precision", ctypes.c_int), ("sig_kind", ctypes.c_int), ("lam", ctypes.c_void_p), ("mu", ctypes.c_void_p),
+    ("lam32", ctypes.c_void_p), ("mu32", ctypes.c_void_p), ("lam0", ctypes.c_double), ("mu0", ctypes.c_double),
+    ("eps", ctypes.c_void_p), ("sig", ctypes.c_void_p), ("eps32", ctypes.c_void_p), ("ws", ctypes.c_void_p),
+    ("ws_bytes", ctypes.c_size_t), ("ws32", ctypes.c_void_p), ("ws32_bytes", ctypes.c_size_t), ("warm", ctypes.c_int),
+    ("max_iter", ctypes.c_int), ("tol", ctypes.c_double), ("tol32", ctypes.c_double),
+    ("check_unstable", ctypes.c_int), ("strain_rate", ctypes.c_double), ("unit", ctypes.c_double * 9)]
+
+
+def mt_from(rs):
+    """MgkMtState of a numpy RandomState (or the np.random module)."""
+    name, key, pos, has_gauss, gauss = rs.get_state()
+    if name != "MT19937":
+        raise ValueError("an MT19937 RandomState is required")
+    st = MgkMtState()
+    ctypes.memmove(st.key, np.ascontiguousarray(key, dtype=np.uint32).ctypes.data, 624 * 4)
+    st.pos, st.has_gauss, st.gauss = int(pos), int(has_gauss), float(gauss)
+    return st
+
+
+def mt_to(rs, st):
+    """Hand an MgkMtState back to a numpy RandomState (or the np.random module)."""
+    key = np.frombuffer(bytes(st.key), dtype=np.uint32).copy()
+    rs.set_state(("MT19937", key, int(st.pos), int(st.has_gauss), float(st.gauss)))
 
 
 class MgkFsMaterial(ctypes.Structure):
@@ -91,6 +140,10 @@ SIGNATURES = {
     "mgk_ls_solve_secant": (_c_i, [_c_i, _c_i64p, _c_i, _c_p, _c_p, _c_d, _c_d, ctypes.POINTER(MgkSecantParams), _c_dp,
                                    _c_p, _c_p, _c_p, _c_p, _c_i, _c_i, _c_d, _c_p, _c_sz, _c_p,
                                    ctypes.POINTER(MgkLsInfo)]),
+    "mgk_mt_fill": (_c_i, [ctypes.POINTER(MgkMtState), _c_i, _c_d, _c_d, ctypes.c_longlong, _c_dp]),
+    "mgk_kmc_run": (_c_i, [ctypes.POINTER(MgkKmcLoop), ctypes.POINTER(MgkMtState), ctypes.POINTER(MgkMtState),
+                           _c_dp, _c_dp, _c_dp, _c_i, _c_i, ctypes.POINTER(_c_i), ctypes.POINTER(MgkKmcEvent),
+                           ctypes.POINTER(_c_i), ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i), _c_p]),
     "mgk_kmc_workspace_bytes": (_c_i, [ctypes.c_longlong, ctypes.POINTER(_c_sz)]),
     "mgk_kmc_barrier_rates": (_c_i, [_c_i, ctypes.c_longlong, _c_i, _c_i, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p,
                                      _c_p, _c_p, ctypes.POINTER(MgkKmcParams), _c_p, _c_p, _c_p, _c_dp]),

@@ -1,0 +1,219 @@
+// Native KMC event loop: the per-event sequence of the reference's KMC loops
+// (mgkmc/aqs.py:855-1087, mgkmc/kmc_simulator.py:871-1101; small strain,
+// linear elasticity, KMC mode) driven from C++ over the library's own entry
+// points, with numpy's legacy RandomState streams (MT19937 words, 53-bit
+// doubles, polar-method Gaussians with one cached value) continued in C++
+// from the caller's state and handed back, so an event sequence does not
+// depend on which side of the boundary ran it.
+#include <cmath>
+#include <cstring>
+#include "mgk_host.h"
+#include "../../include/mgkmc_hip.h"
+
+namespace {
+
+// MT19937 (Matsumoto & Nishimura 1998) in numpy's legacy RandomState usage.
+struct Mt {
+  mgk_mt_state* s;
+  void regen() {
+    uint32_t* k = s->key;
+    const uint32_t up = 0x80000000u, lo = 0x7fffffffu, mag = 0x9908b0dfu;
+    int i = 0;
+    for (; i < 624 - 397; ++i) {
+      const uint32_t y = (k[i] & up) | (k[i + 1] & lo);
+      k[i] = k[i + 397] ^ (y >> 1) ^ ((y & 1u) ? mag : 0u);
+    }
+    for (; i < 623; ++i) {
+      const uint32_t y = (k[i] & up) | (k[i + 1] & lo);
+      k[i] = k[i + 397 - 624] ^ (y >> 1) ^ ((y & 1u) ? mag : 0u);
+    }
+    const uint32_t y = (k[623] & up) | (k[0] & lo);
+    k[623] = k[396] ^ (y >> 1) ^ ((y & 1u) ? mag : 0u);
+    s->pos = 0;
+  }
+  uint32_t word() {
+    if (s->pos >= 624) regen();
+    uint32_t y = s->key[s->pos++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+  }
+  // random_sample(): 27 + 26 bits
+  double uniform() {
+    const uint32_t a = word() >> 5, b = word() >> 6;
+    return (a * 67108864.0 + b) / 9007199254740992.0;
+  }
+  // standard normal (polar method; the second value is cached)
+  double gauss() {
+    if (s->has_gauss) {
+      const double t = s->gauss;
+      s->gauss = 0.0;
+      s->has_gauss = 0;
+      return t;
+    }
+    double x1, x2, r2;
+    do {
+      x1 = 2.0 * uniform() - 1.0;
+      x2 = 2.0 * uniform() - 1.0;
+      r2 = x1 * x1 + x2 * x2;
+    } while (r2 >= 1.0 || r2 == 0.0);
+    const double f = std::sqrt(-2.0 * std::log(r2) / r2);
+    s->gauss = f * x1;
+    s->has_gauss = 1;
+    return f * x2;
+  }
+};
+
+bool unstable_mean(const mgk_ls_info& inf, int d) {
+  double mx = 0.0;
+  for (int i = 0; i < d * d; ++i) {
+    const double v = inf.sig_mean[i];
+    if (!std::isfinite(v)) return true;
+    mx = std::fmax(mx, std::fabs(v));
+  }
+  return inf.status == MGK_ERR_NONFINITE || mx / 1e9 > 20.0;
+}
+
+}  // namespace
+
+extern "C" int mgk_mt_fill(mgk_mt_state* state, int kind, double loc, double scale, long long n, double* out) {
+  if (!state || !out || n < 0 || state->pos < 0 || state->pos > 624) {
+    mgk_set_error("mgk_mt_fill: bad argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  Mt r{state};
+  for (long long i = 0; i < n; ++i) out[i] = kind == 0 ? r.uniform() : loc + scale * r.gauss();
+  return MGK_OK;
+}
+
+extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_state* sel_rng, double* time,
+                           double* eps_macro, double* remaining, int max_events, int max_seq, int* seq,
+                           mgk_kmc_event* events, int* n_events, mgk_ls_info* info, int* iters32, void* stream) {
+  if (!L || !rng || !sel_rng || !time || !eps_macro || !remaining || !seq || !events || !n_events || !info ||
+      max_events < 1 || (L->ndim != 2 && L->ndim != 3) || L->M < 1 || !L->Q0 || !L->cat || !L->soft ||
+      !L->tlast || !L->epsp || !L->kmc_ws || !L->redraw_dev || !L->sig || !L->eps || !L->ws ||
+      (L->ls_precision == 2 && (!L->lam32 || !L->mu32 || !L->eps32 || !L->ws32)) ||
+      rng->pos < 0 || rng->pos > 624 || sel_rng->pos < 0 || sel_rng->pos > 624) {
+    mgk_set_error("mgk_kmc_run: bad argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int d = L->ndim, M = L->M;
+  long long NV = 1;
+  int shape_i[3] = {1, 1, 1};
+  for (int i = 0; i < d; ++i) {
+    NV *= L->shape[i];
+    shape_i[i] = (int)L->shape[i];
+  }
+  Mt R{rng}, S{sel_rng};
+  double* host = nullptr;
+  MGK_HIP(hipHostMalloc((void**)&host, sizeof(double) * 3 * (size_t)M, hipHostMallocDefault));
+  const double hg = 0.5 * L->gamma0;
+  int n = 0, rc = MGK_KMC_RUN_TIME;
+  *n_events = 0;
+  while (true) {
+    mgk_kmc_params p = L->kp;
+    p.time = *time;
+    double total = 0.0;
+    int e = mgk_kmc_barrier_rates(d, NV, M, L->sig_kind, 0, L->sig, L->Q0, L->cat, L->soft, L->tlast, L->prev,
+                                  nullptr, &p, nullptr, L->kmc_ws, stream, &total);
+    if (e != MGK_OK) {
+      rc = e;
+      break;
+    }
+    bool event = false;
+    if (total > 0) {
+      const double u = R.uniform();
+      const double tw = -std::log(u) / total;
+      if (tw < *remaining) {
+        event = true;
+        *time += tw;
+        for (int k = 0; k < d * d; ++k) eps_macro[k] += L->unit[k] * (L->strain_rate * tw);
+        *remaining -= tw;
+        const double us = S.uniform();
+        mgk_kmc_sel so;
+        e = mgk_kmc_select(d, NV, M, L->sig_kind, 0, us, L->sig, L->Q0, L->cat, L->soft, L->tlast, L->prev,
+                           nullptr, &p, L->kmc_ws, stream, &so);
+        if (e != MGK_OK) {
+          rc = e;
+          break;
+        }
+        const long long v = so.item;
+        const int m = so.mode;
+        const int instab = so.q <= L->stability_threshold ? 1 : 0;
+        e = mgk_kmc_flip(d, shape_i, M, 0, v, m, *time, L->jp, L->jt, L->kp.cap, L->jn, L->cat, L->epsp,
+                         L->eps_precision, L->epsp_shadow, L->soft, L->tlast, L->prev, nullptr, 1, stream);
+        if (e != MGK_OK) {
+          rc = e;
+          break;
+        }
+        // redraw: 2M normals (gxx, gxy per mode), then M barriers, from the host stream
+        if (L->redraw_directions)
+          for (int i = 0; i < 2 * M; ++i) host[i] = hg * R.gauss();
+        if (L->redraw_barriers)
+          for (int i = 0; i < M; ++i) {
+            double q = L->q_mean + L->q_std * R.gauss();
+            if (L->q_has_min) q = std::fmax(q, L->q_min);
+            if (L->q_has_max) q = std::fmin(q, L->q_max);
+            host[2 * M + i] = q;
+          }
+        MGK_HIP(hipMemcpyAsync(L->redraw_dev, host, sizeof(double) * 3 * (size_t)M, hipMemcpyHostToDevice, s));
+        e = mgk_kmc_set_voxel(NV, M, 2, v, L->redraw_directions ? L->redraw_dev : nullptr,
+                              L->redraw_barriers ? L->redraw_dev + 2 * M : nullptr, L->cat, L->Q0, stream);
+        if (e != MGK_OK) {
+          rc = e;
+          break;
+        }
+        int it32 = 0;
+        if (L->ls_precision == 2) {
+          e = mgk_ls_solve_refined(d, L->shape, L->lam32, L->mu32, L->lam, L->mu, L->lam0, L->mu0, eps_macro,
+                                   L->epsp_shadow, L->epsp, L->eps32, L->eps, L->sig, L->warm, L->max_iter,
+                                   L->tol, L->tol32, L->ws32, L->ws32_bytes, L->ws, L->ws_bytes, stream, info,
+                                   &it32);
+        } else {
+          e = mgk_ls_solve(d, L->shape, L->ls_precision, L->lam, L->mu, L->lam0, L->mu0, eps_macro, L->epsp,
+                           L->eps, L->sig, L->warm, L->max_iter, L->tol, L->ws, L->ws_bytes, stream, info);
+        }
+        if (iters32) *iters32 = it32;
+        if (e < 0 && e != MGK_ERR_NONFINITE) {
+          rc = e;
+          break;
+        }
+        if (L->check_unstable && unstable_mean(*info, d)) {
+          mgk_set_error("Unstable stress in small-strain elastic_run", __FILE__, __LINE__);
+          rc = MGK_KMC_RUN_UNSTABLE;
+          break;
+        }
+        mgk_kmc_event& ev = events[n++];
+        ev.v = v;
+        ev.m = m;
+        ev.is_instab = instab;
+        ev.time = *time;
+        ev.ls_iters = info->iters;
+        ev.ls_iters32 = it32;
+        *n_events = n;
+        *seq = instab ? *seq + 1 : 0;
+        if (*seq > max_seq) {
+          rc = MGK_KMC_RUN_SEQ;
+          break;
+        }
+        if (n >= max_events) {
+          rc = MGK_KMC_RUN_COUNT;
+          break;
+        }
+      }
+    }
+    if (!event) {
+      for (int k = 0; k < d * d; ++k) eps_macro[k] += L->unit[k] * (L->strain_rate * *remaining);
+      *time += *remaining;
+      *remaining = 0.0;
+      rc = MGK_KMC_RUN_TIME;
+      break;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipHostFree(host);
+  return rc;
+}

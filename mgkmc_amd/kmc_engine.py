@@ -54,6 +54,7 @@ def gaussian_generator(mean=2.0, std=0.6, min_cutoff=0.1, max_cutoff=None, rng=N
             b = np.clip(b, a_min=min_cutoff, a_max=max_cutoff)
         return b
     gen.params = dict(mean=mean, std=std, min_cutoff=min_cutoff, max_cutoff=max_cutoff)
+    gen.rng = rng
     return gen
 
 
@@ -182,6 +183,9 @@ class DeviceKmc:
             self.kmc = KmcState(self.grid_shape, M, Q0, cat, directional=directional, eps_precision=eps_prec,
                                 eps_shadow32=self.precision == MIXED)
         self.ls_warm_start = ls_warm_start
+        self.native_loop = True  # run eligible event sequences in the library's C++ loop
+        self.ls_iters_total = 0  # LS iterations (FP64 + FP32) of the events' re-solves
+        self._redraw_dev = None
         self._solved_once = False
         self.eps_macro = np.zeros((d, d))
         self.time = 0.0
@@ -320,6 +324,8 @@ class DeviceKmc:
                 else:
                     self._flip(v, m, 1)
                     self._after_flip_solve()
+                if self.ls_info is not None:
+                    self.ls_iters_total += self.ls_info.iters + getattr(self.ls, "iters32", 0)
                 ev = coords + (m, self.time, is_instab)
                 self.flip_event_history.append(ev)
                 return ev, remaining_time
@@ -432,6 +438,126 @@ class DeviceKmc:
             if self.ndim == 2 and local > self.total_voxels:
                 break
         return local, total, False
+
+    # ----- event sequences ----------------------------------------------------
+    def kmc_events(self, remaining_time, strain_unit_tensor, max_events=None, max_seq=None, seq=0):
+        """Events until the remaining time holds no further event ("time": the
+        time and macro strain then advance by what was left), max_events events
+        ("count") or more than max_seq consecutive unstable-mode events ("seq").
+        Returns (n events, remaining time, reason, seq).  Linear-law KMC-mode
+        simulations with the gaussian barrier generator run in the library's
+        C++ loop (mgk_kmc_run; same streams, same sequence); others event by
+        event through kmc_event."""
+        if self._native_ok():
+            return self._run_native(remaining_time, strain_unit_tensor, max_events, max_seq, seq)
+        n = 0
+        while True:
+            ev, remaining_time = self.kmc_event(remaining_time, strain_unit_tensor)
+            if ev is None:
+                return n, remaining_time, "time", seq
+            n += 1
+            seq = seq + 1 if ev[-1] else 0
+            if max_seq is not None and seq > max_seq:
+                return n, remaining_time, "seq", seq
+            if max_events is not None and n >= max_events:
+                return n, remaining_time, "count", seq
+
+    def _native_ok(self):
+        gen = self.barrier_generator
+        gp = getattr(gen, "params", None)
+        same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
+        return (self.native_loop and self.law is None and not self.cascade_mode and self.kmc.planar
+                and gp is not None and same_stream and (self.redraw_directions or self.redraw_barriers)
+                and hasattr(self.rng, "get_state") and hasattr(self.selection_stream, "get_state")
+                and (self._solved_once or not self.ls_warm_start))
+
+    def _loop_struct(self, unit):
+        k, ls, d = self.kmc, self.ls, self.ndim
+        L = _lib.MgkKmcLoop()
+        L.ndim, L.M = d, self.M
+        for i, n in enumerate(self.grid_shape):
+            L.shape[i] = n
+        if self._redraw_dev is None:
+            self._redraw_dev = dev.torch.zeros(3 * self.M, dtype=dev.torch.float64, device="cuda")
+        L.Q0, L.cat, L.soft, L.tlast, L.prev = (_lib.dptr(k.Q0), _lib.dptr(k.cat), _lib.dptr(k.soft),
+                                                _lib.dptr(k.tlast), _lib.dptr(k.prev))
+        L.epsp, L.eps_precision, L.epsp_shadow = _lib.dptr(k.eps_p), k.eps_precision, _lib.dptr(k.eps_p32)
+        L.kmc_ws, L.redraw_dev = k.ws.ptr, _lib.dptr(self._redraw_dev)
+        L.kp = self._params()
+        L.jp, L.jt, L.jn = self.jp, self.jt, self.neighbor_softening_fraction
+        L.stability_threshold = self.stability_threshold
+        L.redraw_directions, L.redraw_barriers = int(bool(self.redraw_directions)), int(bool(self.redraw_barriers))
+        gp = self.barrier_generator.params
+        L.gamma0, L.q_mean, L.q_std = self.gamma0, float(gp.get("mean", 2.0)), float(gp.get("std", 0.6))
+        lo, hi = gp.get("min_cutoff"), gp.get("max_cutoff")
+        L.q_has_min, L.q_min = int(lo is not None), float(lo if lo is not None else 0.0)
+        L.q_has_max, L.q_max = int(hi is not None), float(hi if hi is not None else 0.0)
+        L.ls_precision, L.sig_kind = {FP64: 0, FP32: 1, MIXED: 2}[self.precision], self.sig_kind
+        L.lam, L.mu, L.lam0, L.mu0 = _lib.dptr(ls.lam), _lib.dptr(ls.mu), ls.lam0, ls.mu0
+        L.eps, L.sig, L.ws, L.ws_bytes = _lib.dptr(ls.eps), _lib.dptr(ls.sig), _lib.dptr(ls.ws), ls.ws.numel()
+        if self.precision == MIXED:
+            L.lam32, L.mu32, L.eps32 = _lib.dptr(ls.lam32), _lib.dptr(ls.mu32), _lib.dptr(ls.eps32)
+            L.ws32, L.ws32_bytes = _lib.dptr(ls.ws32), ls.ws32.numel()
+        args = dict(max_iter=200, tol=1e-6)
+        args.update(self.solver_args)
+        L.warm = int(bool(self.ls_warm_start))
+        L.max_iter, L.tol = int(args["max_iter"]), float(args["tol"])
+        L.tol32 = max(float(args["tol"]), 1e-6)
+        L.check_unstable = int(d == 2)
+        L.strain_rate = float(self.strain_rate)
+        u = np.asarray(unit, dtype=np.float64).reshape(-1)
+        for i in range(d * d):
+            L.unit[i] = u[i]
+        return L
+
+    def _run_native(self, remaining, unit, max_events, max_seq, seq):
+        d = self.ndim
+        lib = _lib.load()
+        L = self._loop_struct(unit)
+        rng, sel = _lib.mt_from(self.rng), _lib.mt_from(self.selection_stream)
+        t = ctypes.c_double(self.time)
+        rem = ctypes.c_double(remaining)
+        em = (ctypes.c_double * 9)(*self.eps_macro.reshape(-1))
+        sq = ctypes.c_int(int(seq))
+        ms = 2 ** 31 - 1 if max_seq is None else int(min(max_seq, 2 ** 31 - 1))
+        n_total = 0
+        it32 = ctypes.c_int()
+        nb = ctypes.c_int()
+        reason = "time"
+        try:
+            while True:
+                chunk = 4096 if max_events is None else min(4096, max_events - n_total)
+                evs = (_lib.MgkKmcEvent * chunk)()
+                rc = lib.mgk_kmc_run(ctypes.byref(L), ctypes.byref(rng), ctypes.byref(sel), ctypes.byref(t), em,
+                                     ctypes.byref(rem), chunk, ms, ctypes.byref(sq), evs, ctypes.byref(nb),
+                                     ctypes.byref(self.ls.info), ctypes.byref(it32),
+                                     ctypes.c_void_p(dev.stream_ptr()))
+                for e in evs[: nb.value]:
+                    coords = tuple(int(c) for c in self.kmc.voxel_coords(e.v))
+                    self.flip_event_history.append(coords + (int(e.m), float(e.time), bool(e.is_instab)))
+                    self.ls_iters_total += e.ls_iters + e.ls_iters32
+                n_total += nb.value
+                if nb.value:
+                    self._solved_once = True
+                    self.ls_info = self.ls.info
+                    if self.precision == MIXED:
+                        self.ls.iters32 = it32.value
+                if rc == _lib.MGK_KMC_RUN_UNSTABLE:
+                    sm = np.array(self.ls.info.sig_mean[: d * d]).reshape(d, d)
+                    raise ValueError(f"Unstable stress in small-strain elastic_run: {sm / 1e9} GPa")
+                if rc < 0:
+                    _lib.check(rc, "mgk_kmc_run")
+                if rc == _lib.MGK_KMC_RUN_COUNT and (max_events is None or n_total < max_events):
+                    continue
+                reason = {_lib.MGK_KMC_RUN_TIME: "time", _lib.MGK_KMC_RUN_COUNT: "count",
+                          _lib.MGK_KMC_RUN_SEQ: "seq"}[rc]
+                break
+        finally:
+            _lib.mt_to(self.rng, rng)
+            _lib.mt_to(self.selection_stream, sel)
+            self.time = t.value
+            self.eps_macro[...] = np.array(em[: d * d]).reshape(d, d)
+        return n_total, rem.value, reason, sq.value
 
     def footprint_bytes(self):
         """HBM bytes held by this simulation's device state."""

@@ -135,13 +135,14 @@ class KmcSimulation2D(DeviceKmc):
                             return self.flip_event_history
                         continue
                     n_casc = 0
-                ev, remaining = self.kmc_event(remaining, unit)
-                if ev is None:
-                    break
-                events += 1
-                seq = seq + 1 if ev[-1] else 0
-                if seq > max_seq or (max_events is not None and events >= max_events):
+                # one event at a time in cascade mode (a cascade check follows each)
+                limit = 1 if self.cascade_mode else (None if max_events is None else max_events - events)
+                n, remaining, reason, seq = self.kmc_events(remaining, unit, limit, max_seq, seq)
+                events += n
+                if reason == "seq" or (max_events is not None and events >= max_events):
                     return self.flip_event_history
+                if reason == "time":
+                    break
             self.mixed_bc(stress_targets, mixed_tol, mixed_max_iter)
             done += 1
         return self.flip_event_history

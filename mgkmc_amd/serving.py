@@ -166,17 +166,13 @@ def run_request(req):
         d = sim.ndim
         unit = np.zeros((d, d))
         unit[0, 0] = 1.0
-        iters = 0
         err = None
         try:
             sim.update_stresses()
-            for _ in range(req.events):
-                ev, _ = sim.kmc_event(1e30, unit)
-                if ev is None:
-                    break
-                iters += sim.ls_info.iters + sim.ls.iters32
+            sim.kmc_events(1e30, unit, req.events)
         except (ValueError, FloatingPointError) as e:  # the simulators' solver-failure errors
             err = str(e)
+        iters = sim.ls_iters_total
         eps_p_sum = float(sim.kmc.eps_p.to(torch.float64).sum())
         res = SimResult(req.rid, tuple(req.shape), list(sim.flip_event_history), sim.eps_macro.copy(),
                         sim.sig_mean().copy() if sim.ls_info is not None else None, sim.time, eps_p_sum, iters, fp)

@@ -419,3 +419,42 @@ def test_kmc3d_cascade_mode_matches_oracle(gpu):
     assert [e[:4] for e in ev] == [e[:4] for e in ev_ref]
     assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
     assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-8 * np.abs(st_ref["sig"]).max()
+
+
+@pytest.mark.parametrize("d,prec,scheme", [(3, "fp64", "isotropic"), (3, "mixed", "directional"),
+                                           (2, "fp32", "directional"), (2, "mixed", "isotropic")])
+def test_native_loop_matches_python_loop(gpu, d, prec, scheme):
+    # the library's C++ event loop (mgk_kmc_run) against the per-event Python
+    # loop on the same seeds: same events, times, fields and stream states after
+    from mgkmc_amd.aqs import ThermalSimulation
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    print(f"[native loop] {d}D {prec} {scheme}", flush=True)
+    n, M = (12, 10) if d == 3 else (32, 20)
+    shape = (n,) * d
+    rng = np.random.default_rng(9)
+    E = 70e9 * rng.uniform(0.9, 1.1, shape)
+    nu = np.full(shape, 0.3)
+    kw = dict(mean=1.0, std=0.3, min_cutoff=0.0)
+    sims = []
+    for native in (True, False):
+        args = dict(pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw, temperature=900.0, strain_rate=1e7,
+                    softening_scheme=scheme, softening_cap=0.916, jp=20.0, jt=5.0, neighbor_softening_fraction=0.1,
+                    ls_precision=prec, rng=np.random.RandomState(31), selection_stream=np.random.RandomState(32))
+        sim = ThermalSimulation(n, n, n, M, 0.14, E, nu, **args) if d == 3 else KmcSimulation2D(n, n, M, 0.14, E, nu,
+                                                                                               **args)
+        sim.native_loop = native
+        assert sim._native_ok() == native
+        sim.run_simulation(3, 2e-3, component=(0, 0), mixed_tol=1e6, max_events=40)
+        sims.append(sim)
+    a, b = sims
+    print(f"[native loop] {len(a.flip_event_history)} events, first {a.flip_event_history[:3]}", flush=True)
+    assert len(a.flip_event_history) == 40
+    assert [e[:-2] for e in a.flip_event_history] == [e[:-2] for e in b.flip_event_history]
+    assert [e[-1] for e in a.flip_event_history] == [e[-1] for e in b.flip_event_history]
+    assert np.allclose([e[-2] for e in a.flip_event_history], [e[-2] for e in b.flip_event_history], rtol=1e-12)
+    assert a.ls_iters_total == b.ls_iters_total
+    assert np.array_equal(a.eps_plastic, b.eps_plastic)
+    assert np.abs(a.sig_field - b.sig_field).max() <= 1e-9 * np.abs(b.sig_field).max()
+    sa, sb = a.rng.get_state(), b.rng.get_state()
+    assert np.array_equal(sa[1], sb[1]) and sa[2:] == sb[2:]
+    assert a.selection_stream.get_state()[2] == b.selection_stream.get_state()[2]
