@@ -53,11 +53,16 @@ class ThermalSimulation(DeviceKmc):
         Landau solver for hyperelastic_model="landau")."""
         return self._solve(self.eps_macro)
 
-    def mixed_bc(self, stress_targets, component, mixed_tol, mixed_max_iter):
-        """Elastic relaxation loop with mixed BCs (mgkmc/aqs.py:1107-1142)."""
+    def mixed_bc(self, stress_targets, component, mixed_tol, mixed_max_iter, step=0):
+        """Elastic relaxation loop with mixed BCs (mgkmc/aqs.py:1103-1142): in
+        cascade_mode each iteration first runs a cascade (None is returned
+        when it is truncated); the corrections re-solve with the linear LS
+        (update_stress_fft_full) whatever the law, as the reference does."""
         E_avg, nu_avg = np.mean(self.E), np.mean(self.nu)
         converged = False
         for _ in range(mixed_max_iter):
+            if self.cascade_mode and self._run_cascade(step)[2]:
+                return None
             sig_curr = self.sig_mean()
             err = np.zeros((3, 3))
             max_err = 0.0
@@ -71,7 +76,7 @@ class ThermalSimulation(DeviceKmc):
             corr = (err - nu_avg * np.trace(err) * np.eye(3)) / E_avg
             corr[component] = 0.0
             self.eps_macro += corr
-            self.update_stresses()
+            self._solve_linear(self.eps_macro)
         return converged
 
     def run_simulation(self, n_global_steps, step_size, component=(0, 1), stress_targets={}, mixed_tol=1e-4,
@@ -105,22 +110,26 @@ class ThermalSimulation(DeviceKmc):
         remaining = 0.0
         done = 0
         events = 0
+        step = 1  # the reference's global step: +1 per KMC event and per elastic increment
         while done < n_global_steps:
             remaining += dt_step
             while remaining > 0:
                 if self.cascade_mode and self._cascade_pairs():
-                    if self._run_cascade(done + 1)[2]:
+                    if self._run_cascade(step)[2]:
                         return self.flip_event_history
                     continue
                 # one event at a time in cascade mode (a cascade check follows each)
                 limit = 1 if self.cascade_mode else (None if max_events is None else max_events - events)
                 n, remaining, reason, seq = self.kmc_events(remaining, unit, limit, max_seq, seq)
                 events += n
+                step += n
                 if reason == "seq" or (max_events is not None and events >= max_events):
                     return self.flip_event_history
                 if reason == "time":
                     break
             self.update_stresses()
-            self.mixed_bc(targets, component, mixed_tol, mixed_max_iter)
+            if self.mixed_bc(targets, component, mixed_tol, mixed_max_iter, step) is None:
+                return self.flip_event_history
             done += 1
+            step += 1
         return self.flip_event_history

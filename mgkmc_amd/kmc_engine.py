@@ -183,6 +183,8 @@ class DeviceKmc:
             self.kmc = KmcState(self.grid_shape, M, Q0, cat, directional=directional, eps_precision=eps_prec,
                                 eps_shadow32=self.precision == MIXED)
         self.ls_warm_start = ls_warm_start
+        self._lin = None  # linear solver of the 3D mixed-BC corrections under a nonlinear law
+        self._sig_owner = None  # solver whose stress field is current (None: self.ls)
         self.native_loop = True  # run eligible event sequences in the library's C++ loop
         self.ls_iters_total = 0  # LS iterations (FP64 + FP32) of the events' re-solves
         self._redraw_dev = None
@@ -197,12 +199,16 @@ class DeviceKmc:
 
     # ----- state mirrors (synchronised on access) --------------------------
     @property
+    def _stress(self):
+        return self.ls if self._sig_owner is None else self._sig_owner
+
+    @property
     def sig_field(self):
-        return dev.from_sym_soa(dev.download(self.ls.sig), self.ndim)
+        return dev.from_sym_soa(dev.download(self._stress.sig), self.ndim)
 
     @property
     def eps_field(self):
-        return dev.from_sym_soa(dev.download(self.ls.eps), self.ndim)
+        return dev.from_sym_soa(dev.download(self._stress.eps), self.ndim)
 
     @property
     def eps_plastic(self):
@@ -221,7 +227,7 @@ class DeviceKmc:
     def Q(self):
         """Barriers of the current state (evaluated on access; the event loop
         does not store them)."""
-        self.kmc.barrier_rates(self.ls.sig, self.sig_kind, self._params(), write_q=True)
+        self.kmc.barrier_rates(self._stress.sig, self.sig_kind, self._params(), write_q=True)
         return modes_from_device(self.kmc.Q, self.grid_shape)
 
     @property
@@ -258,6 +264,7 @@ class DeviceKmc:
 
     # ----- seams -----------------------------------------------------------
     def _solve(self, eps_macro):
+        self._sig_owner = None
         if self.law is not None:
             return self._solve_law(eps_macro)
         warm = self.ls_warm_start and self._solved_once
@@ -266,6 +273,21 @@ class DeviceKmc:
         self.ls_info = self.ls.solve(eps_macro, self.kmc.eps_p, warm=warm, max_iter=args["max_iter"],
                                      tol=args["tol"], eps_plastic32=self.kmc.eps_p32)
         self._solved_once = True
+        return self.ls_info
+
+    def _solve_linear(self, eps_macro):
+        """update_stress_fft_full itself (linear LS, cold start) whatever the
+        law, as the reference's 3D mixed-BC loop calls it (mgkmc/aqs.py:1137-1139);
+        under a nonlinear law its stress field becomes the current one."""
+        if self.law is None:
+            return self._solve(eps_macro)
+        if self._lin is None:
+            self._lin = LsSolver(self.E_field, self.nu_field, plane_mode=self.plane_mode, precision=FP64)
+        args = dict(max_iter=200, tol=1e-6)
+        args.update(self.solver_args)
+        self.ls_info = self._lin.solve(eps_macro, self.kmc.eps_p, warm=False, max_iter=args["max_iter"],
+                                       tol=args["tol"])
+        self._sig_owner = self._lin
         return self.ls_info
 
     def _solve_law(self, eps_macro):
@@ -307,7 +329,8 @@ class DeviceKmc:
         """One pass of the inner KMC loop body.  Returns (event or None,
         remaining_time); an event is (x, y[, z], m, time, is_instab)."""
         p = self._params()
-        total = self.kmc.barrier_rates(self.ls.sig, self.sig_kind, p, write_q=False)
+        sig = self._stress.sig
+        total = self.kmc.barrier_rates(sig, self.sig_kind, p, write_q=False)
         if total > 0:
             u = self.rng.random_sample()
             t_wait = -np.log(u) / total
@@ -316,7 +339,7 @@ class DeviceKmc:
                 self.eps_macro += strain_unit_tensor * (self.strain_rate * t_wait)
                 remaining_time -= t_wait
                 us = self.selection_stream.random_sample()
-                v, m = self.kmc.select(us, self.ls.sig, self.sig_kind, p)
+                v, m = self.kmc.select(us, sig, self.sig_kind, p)
                 coords = tuple(int(c) for c in self.kmc.voxel_coords(v))
                 is_instab = self.kmc.sel.q <= self.stability_threshold
                 if self.law is not None and self.ndim == 2:
@@ -401,7 +424,7 @@ class DeviceKmc:
         kmc_simulator.py:581-597): barriers of the current state, modes with
         Q < stability_threshold sorted by Q (np.argsort over the voxel-major
         scan, as the reference sorts), the first (most unstable) per voxel."""
-        self.kmc.barrier_rates(self.ls.sig, self.sig_kind, self._params(), write_q=True)
+        self.kmc.barrier_rates(self._stress.sig, self.sig_kind, self._params(), write_q=True)
         v, m, q = self.kmc.unstable_modes(self.stability_threshold)
         pairs, seen = [], set()
         for i in np.argsort(q):

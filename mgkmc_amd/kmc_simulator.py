@@ -124,12 +124,13 @@ class KmcSimulation2D(DeviceKmc):
         remaining = 0.0
         done = 0
         events = 0
+        step = 1  # the reference's global step: +1 per KMC event and per elastic increment
         while done < n_global_steps:
             remaining += dt_step
             while remaining > 0:
                 if self.cascade_mode:
                     if self._cascade_pairs():
-                        if self._run_cascade(done + 1)[1] > 0:
+                        if self._run_cascade(step)[1] > 0:
                             n_casc += 1
                         if n_casc > max_casc:
                             return self.flip_event_history
@@ -139,10 +140,12 @@ class KmcSimulation2D(DeviceKmc):
                 limit = 1 if self.cascade_mode else (None if max_events is None else max_events - events)
                 n, remaining, reason, seq = self.kmc_events(remaining, unit, limit, max_seq, seq)
                 events += n
+                step += n
                 if reason == "seq" or (max_events is not None and events >= max_events):
                     return self.flip_event_history
                 if reason == "time":
                     break
             self.mixed_bc(stress_targets, mixed_tol, mixed_max_iter)
             done += 1
+            step += 1
         return self.flip_event_history

@@ -7,13 +7,16 @@ update_stress_fft_full, cold start), :848-1087 (KMC mode inner loop) and
 :1105-1142 (mixed-BC relaxation), for small-strain linear elasticity,
 KMC mode, planar catalogs, redraw of directions and barriers.
 
-cascade_mode (:480-680 small-strain branch, :820-870): the initial relaxation
-is a cascade (no solve before it: barriers from the zero initial stress); each
-pass of the inner loop first runs a cascade when any Q < stability_threshold:
+cascade_mode (:480-680 small-strain branch, :820-870, :1103-1142): the
+initial relaxation is a cascade (no solve before it: barriers from the zero
+initial stress); each pass of the inner loop, and each iteration of the
+mixed-BC loop, first runs a cascade when any Q < stability_threshold:
 repeat {barriers; unstable modes sorted by Q (np.argsort over the C-order
 scan); first mode per voxel; flip, prev and redraw each in that order; one
 re-solve} until none is unstable, stopping the run when the number of batches
 exceeds int(max_cascade_steps_pct * voxels); rates then count Q > 0 only.
+Cascade flips are recorded as (step, batch, x, y, z, m) with the reference's
+global step (1 at the start, +1 per KMC event and per elastic increment).
 
 Random streams: numpy's global legacy stream exactly as the reference uses it
 (Q0, then the catalog voxel by voxel with stz_catalog_glass, then per event
@@ -121,13 +124,14 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
     dt_step = abs(step_size) / strain_rate if strain_rate > 0 else 1.0
     remaining = 0.0
     done = 0
+    step = 1
     E_avg, nu_avg = np.mean(E), np.mean(nu)
     while done < n_global_steps:
         remaining += dt_step
         while remaining > 0:
             ok.compute_barrier(Q, Q0, state["sig"], cat, volume, soft, tl, t, prev, softening_cap, scheme, tau)
             if cascade_mode and len(ok.find_unstable(Q, stability_threshold)) > 0:
-                if run_cascade(done + 1):
+                if run_cascade(step):
                     return result()
                 continue
             rates, idx, total = ok.compute_rates(Q, 1.0, temperature, nu0, cascade_mode=cascade_mode)
@@ -150,12 +154,15 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                     events.append((int(x), int(y), int(z), int(m), t, bool(inst)))
                     if len(events) >= n_events:
                         return result()
+                    step += 1
                     continue
             eps_macro += unit * (strain_rate * remaining)
             t += remaining
             remaining = 0.0
         update()
         for _ in range(mixed_max_iter):
+            if cascade_mode and run_cascade(step):
+                return result()
             sc = state["sm"]
             err = np.zeros((3, 3))
             mx = 0.0
@@ -169,4 +176,5 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
             eps_macro += corr
             update()
         done += 1
+        step += 1
     return result()

@@ -24,7 +24,9 @@ inner loop first runs a cascade when any Q < stability_threshold: repeat
 redraw each (sub-stepped together under a nonlinear law); one re-solve} until
 none is unstable or the batch count exceeds nx ny; the run stops when more than
 int(max_cascade_steps_pct nx ny) consecutive cascades flipped something; rates
-then count Q > 0 only.
+then count Q > 0 only.  Cascade flips are recorded as (step, batch, x, y, m)
+with the reference's global step (1 at the start, +1 per KMC event and per
+elastic increment).
 
 ``fail_once_at`` (test hook) makes the first re-solve after that event's flip
 raise ValueError, to drive the sub-stepped path.
@@ -196,6 +198,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
     unit[component] = 1.0
     elastic_run()
     n_casc = 0
+    step = 1
     dt_step = abs(step_size) / strain_rate if strain_rate > 0 else 1.0
     events = []
     remaining = 0.0
@@ -212,7 +215,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
             ok.compute_barrier_2d(Q, Q0, state["sig"], cat, volume, soft, tl, t, prev, softening_cap, scheme, tau)
             if cascade_mode:
                 if len(ok.find_unstable_2d(Q, stability_threshold)) > 0:
-                    if run_cascade(done + 1) > 0:
+                    if run_cascade(step) > 0:
                         n_casc += 1
                     if n_casc > int(max_cascade_steps_pct * nx * ny):
                         return result()
@@ -242,6 +245,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     events.append((int(x), int(y), int(m), t, bool(inst)))
                     if len(events) >= n_events:
                         return result()
+                    step += 1
                     continue
             eps_macro += unit * (strain_rate * remaining)
             t += remaining
@@ -272,4 +276,5 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     prev_s[k] = sm[k]
                     eps_macro[k] += st
         done += 1
+        step += 1
     return result()

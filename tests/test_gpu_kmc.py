@@ -395,23 +395,29 @@ def test_kmc2d_cascade_mode_matches_oracle(gpu, T, mean, steps, model, mp):
     assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-8 * np.abs(st_ref["sig"]).max()
 
 
-def test_kmc3d_cascade_mode_matches_oracle(gpu):
+@pytest.mark.parametrize("T,step_size,n_events,steps,mixed_tol", [
+    (800.0, 1e-3, 20, 5, 1e6),      # cascades between KMC events
+    (0.0, 2e-3, None, 4, 1e5),      # athermal: cascades inside the mixed-BC loop
+    (300.0, 2e-3, None, 4, 1e5),
+])
+def test_kmc3d_cascade_mode_matches_oracle(gpu, T, step_size, n_events, steps, mixed_tol):
     from mgkmc_amd.aqs import ThermalSimulation
     from oracle.aqs_loop import run_events
+    print(f"[kmc3d cascade] T={T} mixed_tol={mixed_tol}", flush=True)
     n, M = 8, 10
     rng = np.random.default_rng(1)
     E = 70e9 * rng.uniform(0.9, 1.1, (n, n, n))
     nu = np.full((n, n, n), 0.3)
     kw = dict(mean=0.5, std=0.15, min_cutoff=0.0)
     args = dict(softening_cap=0.916, jp=20.0, jt=0.0)
-    ev_ref, st_ref = run_events(n, n, n, M, 0.14, E, nu, 0.7, 800.0, 1e7, 1e-3, (0, 0), 20, seed=7, sel_seed=8,
-                                barrier_kwargs=kw, mixed_tol=1e6, ls_tol=1e-8, n_global_steps=5, cascade_mode=True,
-                                **args)
+    ev_ref, st_ref = run_events(n, n, n, M, 0.14, E, nu, 0.7, T, 1e7, step_size, (0, 0),
+                                10 ** 9 if n_events is None else n_events, seed=7, sel_seed=8, barrier_kwargs=kw,
+                                mixed_tol=mixed_tol, ls_tol=1e-8, n_global_steps=steps, cascade_mode=True, **args)
     sim = ThermalSimulation(n, n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
-                            temperature=800.0, strain_rate=1e7, cascade_mode=True, rng=np.random.RandomState(7),
+                            temperature=T, strain_rate=1e7, cascade_mode=True, rng=np.random.RandomState(7),
                             selection_stream=np.random.RandomState(8), **args)
     sim.solver_args = {"tol": 1e-8}
-    ev = sim.run_simulation(5, 1e-3, component=(0, 0), mixed_tol=1e6, max_events=20)
+    ev = sim.run_simulation(steps, step_size, component=(0, 0), mixed_tol=mixed_tol, max_events=n_events)
     print(f"[kmc3d cascade] events {len(ev)} (oracle {len(ev_ref)}), cascade flips {len(sim.cascade_history)} "
           f"(oracle {len(st_ref['cascades'])})", flush=True)
     assert len(st_ref["cascades"]) > 0
