@@ -211,7 +211,9 @@ int mgk_find_below(long long n, const double* a, double thr, long long* idx_out,
  * NULL): an FP32 copy of epsp whose flipped voxel receives the rounded new values.
  * eps_div: 1 for the flip itself; N > 1 adds mode / N to epsp (the last sub-step of a
  * sub-stepped flip, mgkmc/kmc_simulator.py:268-385, which also applies the softening,
- * time and prev updates); -N adds mode / N only (the earlier sub-steps). */
+ * time and prev updates); -N adds mode / N only (the earlier sub-steps); 0 leaves epsp
+ * alone (may be NULL): the softening, time and prev updates of a finite-strain flip,
+ * whose mode enters F_p instead (mgkmc/aqs.py:912-975). */
 int mgk_kmc_flip(int ndim, const int* shape, int M, int cat_kind, long long v, int m, double time, double jp,
                  double jt, double gmax, double jn, const double* cat, void* epsp, int precision, float* epsp_shadow,
                  double* soft, double* tlast, double* prev, double* gamma_out, int eps_div, void* stream);

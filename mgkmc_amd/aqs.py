@@ -27,8 +27,12 @@ class ThermalSimulation(DeviceKmc):
                  strain_capping_enabled=False, strain_capping_limit=None, strain_capping_tangent_ratio=0.1,
                  strain_capping_type="piecewise", strain_capping_smooth_power=1.0,
                  ls_precision="fp64", ls_warm_start=False, rng=None, selection_stream=None, init_modes=None):
-        self._unsupported(cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model,
-                          ["use_3d_barriers"] if use_3d_barriers else [])
+        # finite strain: the Newton-Krylov step (solver="newton_cg"), KMC mode
+        self._finite_ok = strain_assumption == "finite_strain" and solver == "newton_cg" and not cascade_mode
+        extra = ["use_3d_barriers"] if use_3d_barriers else []
+        if strain_assumption == "finite_strain" and solver != "newton_cg":
+            extra.append(f"finite-strain solver={solver!r} (the device step is solver='newton_cg')")
+        self._unsupported(cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model, extra)
         self.output_dir = output_dir
         self.strain_assumption = strain_assumption
         self.hyperelastic_model = hyperelastic_model
@@ -40,11 +44,17 @@ class ThermalSimulation(DeviceKmc):
             setattr(self, key, val)
         # small strain: only "landau" leaves the linear solve (mgkmc/aqs.py:330-352)
         law = ("landau", (v1, v2, v3, g1, g2, g3, g4), cap) if hyperelastic_model == "landau" else None
+        finite = None
+        self.solver, self.A_m, self.B_m, self.C_m = solver, A_m, B_m, C_m
+        if strain_assumption == "finite_strain":
+            law = None
+            finite = dict(model_type=hyperelastic_model, A_m=A_m, B_m=B_m, C_m=C_m, v1=v1, v2=v2, v3=v3, g1=g1,
+                          g2=g2, g3=g3, g4=g4, **cap)
         self._setup((nx, ny, nz), M, gamma0, E_field, nu_field, pixel, barrier_generator, barrier_kwargs,
                     softening_scheme, softening_cap, jp, jt, neighbor_softening_fraction, temperature, strain_rate,
                     stability_threshold, redraw_directions, redraw_barriers, nu0, q_act_temp, cascade_mode,
                     scale_rate_by_volume, "plane_strain", ls_precision, ls_warm_start, rng, selection_stream,
-                    init_modes, dict(mean=2.0, std=0.6), law=law)
+                    init_modes, dict(mean=2.0, std=0.6), law=law, finite=finite)
         self.max_cascade_steps_pct = max_cascade_steps_pct
 
     def update_stresses(self):
@@ -92,11 +102,13 @@ class ThermalSimulation(DeviceKmc):
         if max_cascade_steps_pct is not None:
             self.max_cascade_steps_pct = max_cascade_steps_pct
         targets = dict(stress_targets)
+        self.driving_component = component
         if not targets:
             if component == (0, 0):
                 targets = {(1, 1): 0.0, (2, 2): 0.0}
             elif component == (0, 1):
                 targets = {(0, 0): 0.0, (1, 1): 0.0, (2, 2): 0.0}
+        self.stress_targets = targets
         unit = np.zeros((3, 3))
         unit[component] = 1.0
         if self.cascade_mode:
@@ -128,7 +140,8 @@ class ThermalSimulation(DeviceKmc):
                 if reason == "time":
                     break
             self.update_stresses()
-            if self.mixed_bc(targets, component, mixed_tol, mixed_max_iter, step) is None:
+            # finite strain: the step itself enforces the stress targets (mgkmc/aqs.py:1089-1102)
+            if not self.finite and self.mixed_bc(targets, component, mixed_tol, mixed_max_iter, step) is None:
                 return self.flip_event_history
             done += 1
             step += 1

@@ -554,7 +554,7 @@ struct KmcFlip {
   long long v;
   int m;
   double time, jp, jt, gmax, jn;
-  int eps_div;  // eps_p += mode / |eps_div|; eps_div < 0: that increment only
+  int eps_div;  // eps_p += mode / |eps_div|; eps_div < 0: that increment only; 0: no eps_p update
 };
 
 template <int D, int CK, typename TE>
@@ -611,11 +611,13 @@ __global__ void k_kmc_flip(KmcFlip F, const double* __restrict__ cat, TE* __rest
     }
     constexpr int CS = (D == 3) ? 6 : 3;
     const double div = (double)(F.eps_div < 0 ? -F.eps_div : F.eps_div);
+    if (F.eps_div != 0) {
 #pragma unroll
-    for (int k = 0; k < CS; ++k) {
-      const TE nv = (TE)((double)epsp[k * NV + v] + add[k] / div);
-      epsp[k * NV + v] = nv;
-      if (shadow) shadow[k * NV + v] = (float)nv;
+      for (int k = 0; k < CS; ++k) {
+        const TE nv = (TE)((double)epsp[k * NV + v] + add[k] / div);
+        epsp[k * NV + v] = nv;
+        if (shadow) shadow[k * NV + v] = (float)nv;
+      }
     }
     if (F.eps_div < 0) return;
     double gp = soft[v] + F.jp * s;
@@ -632,7 +634,7 @@ __global__ void k_kmc_flip(KmcFlip F, const double* __restrict__ cat, TE* __rest
       for (int k = 0; k < DD; ++k) gamma_out[k] = c[k];
     }
   }
-  if (t == 0 && F.jn > 0.0 && F.eps_div > 0) {
+  if (t == 0 && F.jn > 0.0 && F.eps_div >= 0) {
     // neighbours in the reference's (dx, dy[, dz]) loop order, sequentially
     // (on small periodic grids several offsets hit the same voxel)
     for (int dx = -1; dx <= 1; ++dx)
@@ -943,8 +945,8 @@ extern "C" int mgk_kmc_flip(int ndim, const int* shape, int M, int cat_kind, lon
                             double jp, double jt, double gmax, double jn, const double* cat, void* epsp,
                             int precision, float* epsp_shadow, double* soft, double* tlast, double* prev, double* gamma_out,
                             int eps_div, void* stream) {
-  if ((ndim != 2 && ndim != 3) || !shape || !cat || !epsp || !soft || !tlast || (precision != 0 && epsp_shadow) ||
-      eps_div == 0) {
+  if ((ndim != 2 && ndim != 3) || !shape || !cat || (!epsp && eps_div != 0) || !soft || !tlast ||
+      (precision != 0 && epsp_shadow)) {
     mgk_set_error("mgk_kmc_flip: bad argument", __FILE__, __LINE__);
     return MGK_ERR_ARG;
   }

@@ -36,6 +36,7 @@ from . import device as dev
 from .kmc_device import KmcState, params as kmc_params, SIG_SYM64, SIG_SYM32, modes_from_device
 from .linear_elastic_simulator import (LsSolver, FP64, FP32, MIXED, LandauSolver, LandauSolver2D, SecantSolver,
                                        SECANT_EXP, SECANT_ROSE, landau_params)
+from .kmc_device import SIG_GEN64
 from .elasticity import compute_lame_2d, compute_lame_3d
 from .stz import kmc as _kmcmod
 
@@ -102,7 +103,7 @@ class DeviceKmc:
             bad.append("fast_patching")
         if enable_thermal:
             bad.append("enable_thermal")
-        if strain_assumption != "small_strain":
+        if strain_assumption != "small_strain" and not getattr(self, "_finite_ok", False):
             bad.append("finite_strain KMC")
         if bad:
             raise NotImplementedError("device KMC path does not cover: " + ", ".join(bad))
@@ -111,7 +112,7 @@ class DeviceKmc:
                softening_scheme, softening_cap, jp, jt, neighbor_softening_fraction, temperature, strain_rate,
                stability_threshold, redraw_directions, redraw_barriers, nu0, q_act_temp, cascade_mode,
                scale_rate_by_volume, plane_mode, ls_precision, ls_warm_start, rng, selection_stream, init_modes,
-               barrier_defaults, law=None):
+               barrier_defaults, law=None, finite=None):
         self.rng = np.random if rng is None else rng
         self.selection_stream = _kmcmod.SELECTION_STREAM if selection_stream is None else selection_stream
         self.M, self.gamma0 = M, gamma0
@@ -160,7 +161,21 @@ class DeviceKmc:
         self.precision = LS_PRECISIONS[ls_precision]
         self.sig_kind = SIG_SYM32 if self.precision == FP32 else SIG_SYM64
         self.law = law
-        if law is None:
+        self.finite = finite is not None
+        if self.finite:
+            # de Geus finite-strain state (mgkmc/aqs.py:205-215): F, F_plastic = I; the step's own macro BC
+            from .finite_strain_simulator import FsSolver, MIXED as FS_MIXED
+            if self.precision == FP32:
+                raise NotImplementedError("finite-strain KMC runs with ls_precision 'fp64' or 'mixed'")
+            lam, mu = compute_lame_3d(E, nu)
+            eye = np.broadcast_to(np.eye(3).reshape(3, 3, 1, 1, 1), (3, 3) + self.grid_shape)
+            self.fs = FsSolver(lam, mu, precision=FS_MIXED if self.precision == MIXED else 0, Fp=eye)
+            from .finite_strain_simulator import material
+            self._fs_mat = material(**finite, mu_mean=float(np.mean(mu)))
+            self.F_macro = np.eye(3)
+            self.sig_kind = SIG_GEN64
+            self.ls = None
+        elif law is None:
             self.ls = LsSolver(self.E_field, self.nu_field, plane_mode=plane_mode, precision=self.precision)
         else:
             if self.precision != FP64:
@@ -200,15 +215,30 @@ class DeviceKmc:
     # ----- state mirrors (synchronised on access) --------------------------
     @property
     def _stress(self):
+        if self.finite:
+            return self.fs
         return self.ls if self._sig_owner is None else self._sig_owner
 
     @property
     def sig_field(self):
+        if self.finite:
+            return np.einsum("ijxyz->xyzij", dev.download(self.fs.sig))
         return dev.from_sym_soa(dev.download(self._stress.sig), self.ndim)
 
     @property
     def eps_field(self):
+        if self.finite:  # Green-Lagrange strain of F, as the reference stores it
+            F = np.einsum("ijxyz->xyzij", dev.download(self.fs.F))
+            return 0.5 * (np.einsum("...ki,...kj->...ij", F, F) - np.eye(3))
         return dev.from_sym_soa(dev.download(self._stress.eps), self.ndim)
+
+    @property
+    def F_field(self):
+        return np.einsum("ijxyz->xyzij", dev.download(self.fs.F)) if self.finite else None
+
+    @property
+    def F_plastic(self):
+        return np.einsum("ijxyz->xyzij", dev.download(self.fs.Fp)) if self.finite else None
 
     @property
     def eps_plastic(self):
@@ -265,6 +295,8 @@ class DeviceKmc:
     # ----- seams -----------------------------------------------------------
     def _solve(self, eps_macro):
         self._sig_owner = None
+        if self.finite:
+            return self._solve_finite()
         if self.law is not None:
             return self._solve_law(eps_macro)
         warm = self.ls_warm_start and self._solved_once
@@ -274,6 +306,67 @@ class DeviceKmc:
                                      tol=args["tol"], eps_plastic32=self.kmc.eps_p32)
         self._solved_once = True
         return self.ls_info
+
+    def _solve_finite(self):
+        """Finite-strain update_stresses (mgkmc/aqs.py:276-330): macro BC from
+        the driven strain with F_macro as the initial F_bar, one Newton-Krylov
+        step from the current F with F_plastic; F_macro and the macro strain
+        (F_ii - 1, F_ij) follow the step; ValueError on a NaN field or
+        |<sigma>| > 20 GPa."""
+        from .finite_strain_simulator import build_finite_strain_bc_3d
+        drv = getattr(self, "driving_component", (0, 0))
+        tg = getattr(self, "stress_targets", {})
+        Fb, _, Pt, Pm = build_finite_strain_bc_3d(drv, self.eps_macro[drv], tg, F_bar_initial=self.F_macro)
+        info = self.fs.step(Fb, self._fs_mat, drv, Pt, Pm, E_avg=self.E_field.mean(), nu_avg=self.nu_field.mean())
+        self.F_macro = np.array(info.F_bar[:]).reshape(3, 3)
+        for i in range(3):
+            for j in range(3):
+                self.eps_macro[i, j] = self.F_macro[i, j] - 1.0 if i == j else self.F_macro[i, j]
+        self.ls_info = info
+        self._solved_once = True
+        sm = np.array(info.sig_mean[:9]).reshape(3, 3)
+        torch = dev.torch
+        if bool(torch.isnan(self.fs.sig).any()) or bool(torch.isnan(self.fs.F).any()) or \
+                np.max(np.abs(sm)) / 1e9 > 20.0:
+            raise ValueError(f"Simulation stopped due to unstable stress: {sm / 1e9} GPa")
+        return info
+
+    def _flip_finite(self, v, m):
+        """Finite-strain flip (mgkmc/aqs.py:887-996): F_p <- expm(C/N) F_p at
+        the voxel in N sub-steps, N from max(2, ceil(sqrt(C:C/2) / 0.005)) (at
+        most 40), retried with N + 1 .. N + 19 on ValueError from the state
+        before the flip; softening, prev and redraws at the last sub-step, a
+        finite-strain step after each."""
+        from scipy.linalg import expm
+        torch = dev.torch
+        C = self._mode_host(v, m)
+        k, fs = self.kmc, self.fs
+        live = [fs.F, fs.P, fs.sig, fs.Fp, k.soft, k.tlast, k.prev, k.cat, k.Q0]
+        backup = [None if a is None else a.clone() for a in live]
+        b_fm, b_em = self.F_macro.copy(), self.eps_macro.copy()
+        x, y, z = (int(c) for c in self.kmc.voxel_coords(v))
+        n0 = min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.005))), 40)
+        for N in range(n0, n0 + 20):
+            for a, b in zip(live, backup):
+                if a is not None:
+                    a.copy_(b)
+            self.F_macro = b_fm.copy()
+            self.eps_macro[...] = b_em
+            try:
+                for i in range(1, N + 1):
+                    dF = expm(C / N)
+                    fpv = fs.Fp[:, :, x, y, z].cpu().numpy()
+                    fs.Fp[:, :, x, y, z] = torch.from_numpy(np.dot(dF, fpv)).to(fs.Fp.device)
+                    if i == N:
+                        k.flip(v, m, self.time, self.jp, self.jt, self.softening_cap,
+                               self.neighbor_softening_fraction, 0)
+                        self._redraw(v)
+                    self.update_stresses()
+                self.last_substeps = N
+                return
+            except ValueError:
+                continue
+        raise ValueError("Mechanical solver failed to converge under sub-stepping.")
 
     def _solve_linear(self, eps_macro):
         """update_stress_fft_full itself (linear LS, cold start) whatever the
@@ -342,12 +435,14 @@ class DeviceKmc:
                 v, m = self.kmc.select(us, sig, self.sig_kind, p)
                 coords = tuple(int(c) for c in self.kmc.voxel_coords(v))
                 is_instab = self.kmc.sel.q <= self.stability_threshold
-                if self.law is not None and self.ndim == 2:
+                if self.finite:
+                    self._flip_finite(v, m)
+                elif self.law is not None and self.ndim == 2:
                     self._flip_substepped([(v, m)])
                 else:
                     self._flip(v, m, 1)
                     self._after_flip_solve()
-                if self.ls_info is not None:
+                if self.ls_info is not None and not self.finite:
                     self.ls_iters_total += self.ls_info.iters + getattr(self.ls, "iters32", 0)
                 ev = coords + (m, self.time, is_instab)
                 self.flip_event_history.append(ev)
@@ -361,6 +456,10 @@ class DeviceKmc:
         (eps_div = N: mode / N), then the redraw of the voxel's modes."""
         self.kmc.flip(v, m, self.time, self.jp, self.jt, self.softening_cap, self.neighbor_softening_fraction,
                       eps_div)
+        self._redraw(v)
+
+    def _redraw(self, v):
+        """New directions (2M normals) and barriers (M draws) for voxel v from the host stream."""
         if not (self.redraw_directions or self.redraw_barriers):
             raise NotImplementedError("single-mode redraw")
         newcat = newq0 = None
@@ -489,7 +588,7 @@ class DeviceKmc:
         gen = self.barrier_generator
         gp = getattr(gen, "params", None)
         same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
-        return (self.native_loop and self.law is None and not self.cascade_mode and self.kmc.planar
+        return (self.native_loop and self.law is None and not self.finite and not self.cascade_mode and self.kmc.planar
                 and gp is not None and same_stream and (self.redraw_directions or self.redraw_barriers)
                 and hasattr(self.rng, "get_state") and hasattr(self.selection_stream, "get_state")
                 and (self._solved_once or not self.ls_warm_start))
@@ -585,6 +684,14 @@ class DeviceKmc:
     def footprint_bytes(self):
         """HBM bytes held by this simulation's device state."""
         t = 0
+        if self.finite:
+            f = self.fs
+            for x in (f.lam, f.mu, f.F, f.P, f.sig, f.Fp, f.ws, getattr(f, "lam32", None), getattr(f, "mu32", None),
+                      getattr(f, "ws32", None), self.kmc.Q0, self.kmc.cat, self.kmc.soft, self.kmc.tlast,
+                      self.kmc.prev, self.kmc.eps_p):
+                if x is not None:
+                    t += x.numel() * x.element_size()
+            return t
         for x in (self.ls.lam, self.ls.mu, self.ls.eps, self.ls.sig, self.ls.ws,
                   getattr(self.ls, "lam32", None), getattr(self.ls, "mu32", None), getattr(self.ls, "eps32", None),
                   getattr(self.ls, "ws32", None), getattr(self.ls, "tau", None), getattr(self.ls, "e33", None),

@@ -18,6 +18,17 @@ exceeds int(max_cascade_steps_pct * voxels); rates then count Q > 0 only.
 Cascade flips are recorded as (step, batch, x, y, z, m) with the reference's
 global step (1 at the start, +1 per KMC event and per elastic increment).
 
+strain_assumption="finite_strain" (:273-330, :887-996, :1088-1102; svk
+and newton_cg): update() is the finite-strain step (oracle/finite_strain.py)
+from the current F with F_plastic, the macro BC built from the driven strain
+with F_macro as the initial F_bar; F_macro is the returned F_bar and the macro
+strain is read from it (F_ii - 1, F_ij); barriers use the Cauchy stress; a
+NaN field or |<sigma>| > 20 GPa raises ValueError.  A flip is applied in N
+sub-steps F_p <- expm(C/N) F_p (N from max(2, ceil(sqrt(C:C/2) / 0.005)), at
+most 40, then N + 1 .. on ValueError, state restored), with softening, prev
+and redraws at the last sub-step and a finite-strain step after each; no
+mixed-BC loop after an increment (the step enforces the stress targets).
+
 Random streams: numpy's global legacy stream exactly as the reference uses it
 (Q0, then the catalog voxel by voxel with stz_catalog_glass, then per event
 one uniform for the waiting time, 2M normals for the new modes and M normals
@@ -36,7 +47,8 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                n_events, seed, sel_seed, barrier_kwargs=None, softening_scheme="isotropic", softening_cap=2.0,
                jp=10.0, jt=30.0, jn=0.0, nu0=1e13, q_act_temp=0.37, ls_tol=1e-6, stress_targets=None,
                mixed_tol=1e-4, mixed_max_iter=50, n_global_steps=10 ** 9, cascade_mode=False,
-               stability_threshold=0.0, max_cascade_steps_pct=0.3, eps_plastic=None):
+               stability_threshold=0.0, max_cascade_steps_pct=0.3, eps_plastic=None,
+               strain_assumption="small_strain"):
     """Returns the list of events (x, y, z, m, time, is_instab) and final state."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
@@ -66,10 +78,80 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
     scheme = 0 if softening_scheme == "isotropic" else 1
     state = {}
 
+    finite = strain_assumption == "finite_strain"
+    if finite:
+        from scipy.linalg import expm
+        from . import finite_strain as ofs
+        I2, I4, I4rt, I4s, II = ofs.identity_tensors(nx, ny, nz)
+        Gh = ofs.build_ghat4(nx, ny, nz, nx * pixel, ny * pixel, nz * pixel)
+        C4 = ofs.build_C4(E, nu, I4s, II)
+        Ff = I2.copy()
+        Fp = I2.copy()
+        state["F_macro"] = np.eye(3)
+
     def update():
-        e, s, em, sm = ol.update_stress_fft_full(ep, eps_macro, E, nu, pixel=pixel, tol=ls_tol)
-        state["sig"] = s
-        state["sm"] = sm
+        if not finite:
+            e, s, em, sm = ol.update_stress_fft_full(ep, eps_macro, E, nu, pixel=pixel, tol=ls_tol)
+            state["sig"] = s
+            state["sm"] = sm
+            return
+        Fb, _, Pt, Pm = ofs.build_bc(component, eps_macro[component], targets, F_bar_initial=state["F_macro"])
+        Fo, Po, So, _, Fb = ofs.solver_step(Ff, Fb, Gh, C4, I2, I4, I4rt, Fp=Fp, driving_component=component,
+                                            P_target=Pt, P_mask=Pm, E_avg=E.mean(), nu_avg=nu.mean())
+        Ff[...] = Fo
+        state["F_macro"] = Fb
+        for i in range(3):
+            for j in range(3):
+                eps_macro[i, j] = Fb[i, j] - 1.0 if i == j else Fb[i, j]
+        state["sig"] = np.einsum("ijxyz->xyzij", So)
+        state["sm"] = state["sig"].mean(axis=(0, 1, 2))
+        if np.any(np.isnan(So)) or np.any(np.isnan(Fo)) or np.max(np.abs(state["sm"])) / 1e9 > 20.0:
+            raise ValueError("Simulation stopped due to unstable stress")
+
+    def flip_finite(x, y, z, m):
+        C = cat[x, y, z, m].copy()
+        arrays = (Ff, Fp, soft, tl, prev, cat, Q0)
+        backup = [a.copy() for a in arrays]
+        b_sig, b_sm, b_fm, b_em = state["sig"].copy(), state["sm"].copy(), state["F_macro"].copy(), eps_macro.copy()
+        n0 = min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.005))), 40)
+        for N in range(n0, n0 + 20):
+            for a, b in zip(arrays, backup):
+                a[...] = b
+            state["sig"], state["sm"], state["F_macro"] = b_sig.copy(), b_sm.copy(), b_fm.copy()
+            eps_macro[...] = b_em
+            try:
+                for k in range(1, N + 1):
+                    dF = expm(C / N)
+                    Fp[:, :, x, y, z] = np.dot(dF, Fp[:, :, x, y, z])
+                    if k == N:
+                        sq = ok.vm_sq_3d(C)
+                        gp = soft[x, y, z, 0] + jp * sq
+                        if softening_cap > 0 and gp > softening_cap:
+                            gp = softening_cap
+                        soft[x, y, z, 0] = gp
+                        soft[x, y, z, 1] = jt * sq
+                        tl[x, y, z] = t
+                        if jn > 0.0:
+                            for dx in (-1, 0, 1):
+                                for dy in (-1, 0, 1):
+                                    for dz in (-1, 0, 1):
+                                        if dx == 0 and dy == 0 and dz == 0:
+                                            continue
+                                        a, b, c = (x + dx + nx) % nx, (y + dy + ny) % ny, (z + dz + nz) % nz
+                                        g = soft[a, b, c, 0] + jn * jp * sq
+                                        if softening_cap > 0 and g > softening_cap:
+                                            g = softening_cap
+                                        soft[a, b, c, 0] = g
+                                        soft[a, b, c, 1] += jn * jt * sq
+                        prev[x, y, z] = C
+                        cat[x, y, z] = ok.stz_catalog_glass(M, gamma0, np.random)
+                        Q0[x, y, z] = gen((M,))
+                    update()
+                state.setdefault("substeps", []).append(N)
+                return
+            except ValueError:
+                continue
+        raise ValueError("Mechanical solver failed to converge under sub-stepping.")
 
     state["sig"] = np.zeros((nx, ny, nz, 3, 3))
     state["sm"] = np.zeros((3, 3))
@@ -104,8 +186,11 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
         return False
 
     def result():
-        return events, dict(eps_plastic=ep, Q=Q, Q0=Q0, soft=soft, sig=state["sig"], eps_macro=eps_macro, time=t,
-                            cascades=cascades)
+        out = dict(eps_plastic=ep, Q=Q, Q0=Q0, soft=soft, sig=state["sig"], eps_macro=eps_macro, time=t,
+                   cascades=cascades, substeps=state.get("substeps"))
+        if finite:
+            out.update(F=Ff, F_plastic=Fp, F_macro=state["F_macro"])
+        return events, out
 
     targets = dict(stress_targets or {})
     if not targets:
@@ -145,12 +230,15 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                     i = ok.select_event(rates, total, total * sel.random_sample())
                     x, y, z, m = ok.decode_index(idx[i], ny, nz, M)
                     inst = Q[x, y, z, m] <= stability_threshold
-                    C = cat[x, y, z, m].copy()
-                    ok.apply_flip_soa(ep, None, soft, tl, cat, x, y, z, m, t, jp, jt, softening_cap, jn)
-                    prev[x, y, z] = C
-                    cat[x, y, z] = ok.stz_catalog_glass(M, gamma0, np.random)
-                    Q0[x, y, z] = gen((M,))
-                    update()
+                    if finite:
+                        flip_finite(x, y, z, m)
+                    else:
+                        C = cat[x, y, z, m].copy()
+                        ok.apply_flip_soa(ep, None, soft, tl, cat, x, y, z, m, t, jp, jt, softening_cap, jn)
+                        prev[x, y, z] = C
+                        cat[x, y, z] = ok.stz_catalog_glass(M, gamma0, np.random)
+                        Q0[x, y, z] = gen((M,))
+                        update()
                     events.append((int(x), int(y), int(z), int(m), t, bool(inst)))
                     if len(events) >= n_events:
                         return result()
@@ -160,7 +248,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
             t += remaining
             remaining = 0.0
         update()
-        for _ in range(mixed_max_iter):
+        for _ in range(0 if finite else mixed_max_iter):
             if cascade_mode and run_cascade(step):
                 return result()
             sc = state["sm"]

@@ -464,3 +464,43 @@ def test_native_loop_matches_python_loop(gpu, d, prec, scheme):
     sa, sb = a.rng.get_state(), b.rng.get_state()
     assert np.array_equal(sa[1], sb[1]) and sa[2:] == sb[2:]
     assert a.selection_stream.get_state()[2] == b.selection_stream.get_state()[2]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_kmc3d_finite_strain_matches_oracle(gpu, prec):
+    # strain_assumption="finite_strain" (svk, newton_cg): every flip enters F_p
+    # as expm(C/N) over N sub-steps, each followed by a Newton-Krylov step with
+    # the stress targets; events, sub-step counts, F, F_p and F_macro equal
+    # the oracle loop's (FP64; the FP32-Krylov form within its Newton tolerance)
+    from mgkmc_amd.aqs import ThermalSimulation
+    from oracle.aqs_loop import run_events
+    print(f"[kmc3d finite] {prec}", flush=True)
+    n, M = 8, 6
+    rng = np.random.default_rng(3)
+    E = 70e9 * rng.uniform(0.9, 1.1, (n, n, n))
+    nu = np.full((n, n, n), 0.3)
+    kw = dict(mean=1.0, std=0.3, min_cutoff=0.0)
+    args = dict(softening_cap=0.916, jp=20.0, jt=5.0)
+    ev_ref, st_ref = run_events(n, n, n, M, 0.03, E, nu, 0.7, 900.0, 1e7, 2e-3, (0, 0), 8, seed=7, sel_seed=8,
+                                barrier_kwargs=kw, n_global_steps=3, strain_assumption="finite_strain", **args)
+    sim = ThermalSimulation(n, n, n, M, 0.03, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                            temperature=900.0, strain_rate=1e7, strain_assumption="finite_strain", solver="newton_cg",
+                            ls_precision=prec, rng=np.random.RandomState(7), selection_stream=np.random.RandomState(8),
+                            **args)
+    subs = []
+    real = sim._flip_finite
+
+    def flip_finite(v, m):
+        real(v, m)
+        subs.append(sim.last_substeps)
+    sim._flip_finite = flip_finite
+    ev = sim.run_simulation(3, 2e-3, component=(0, 0), max_events=8)
+    print(f"[kmc3d finite] events {[e[:4] for e in ev]} substeps {subs} (oracle {st_ref['substeps']})", flush=True)
+    assert [e[:4] for e in ev] == [e[:4] for e in ev_ref]
+    assert subs == st_ref["substeps"] and max(subs) >= 3
+    tol = 1e-9 if prec == "fp64" else 1e-5
+    Fr = np.einsum("ijxyz->xyzij", st_ref["F"])
+    assert np.abs(sim.F_field - Fr).max() <= tol
+    assert np.abs(sim.F_plastic - np.einsum("ijxyz->xyzij", st_ref["F_plastic"])).max() <= 1e-14
+    assert np.abs(sim.F_macro - st_ref["F_macro"]).max() <= tol
+    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= tol * 1e3 * np.abs(st_ref["sig"]).max()
