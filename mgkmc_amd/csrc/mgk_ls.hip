@@ -22,6 +22,21 @@
 #include "mgk_fftpass.h"
 #include "mgk_landau.h"
 #include "mgk_secant.h"
+
+// Compile-time x sweeps (tau + r2c, c2r + update): FP64 runs 512-thread
+// workgroups on tiles of 4096 points (twice the other FP64 tiles, 64 B of
+// each contiguous row per field instead of 32): tau_xfwd 0.635 -> 0.593 ms,
+// xinv_update 0.597 -> 0.581 ms at 256^3.  FP32 keeps 256 threads on 4096
+// points: its tau_xfwd needs 142 VGPRs, which 512-thread workgroups cannot
+// hold at two workgroups per CU (measured 1.3% slower).
+template <typename T>
+constexpr int x_tile() {
+  return ct_tile<T>() * (sizeof(T) == 8 ? 2 : 1);
+}
+template <typename T>
+constexpr int x_nt() {
+  return sizeof(T) == 8 ? 512 : MGK_NT;
+}
 #include <algorithm>
 
 struct LsState {
@@ -128,7 +143,7 @@ __device__ __forceinline__ void gamma2_apply(const T q0, const T q1, const cplx<
 // sweep 1: polarisation + forward x FFT
 // ---------------------------------------------------------------------------
 template <typename T, int D, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
-__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 ? 6 : (sizeof(T) == 4 ? 3 : 2)))) void k_ls_tau_xfwd(LsGeom<T> g, const T* __restrict__ eps,
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 ? (sizeof(T) == 4 ? 4 : 2) : (sizeof(T) == 4 ? 3 : 2)))) void k_ls_tau_xfwd(LsGeom<T> g, const T* __restrict__ eps,
                                                         const T* __restrict__ epsp, const T* __restrict__ lam,
                                                         const T* __restrict__ mu, T lam0, T mu0,
                                                         cplx<T>* __restrict__ spec, const LsState* __restrict__ st) {
@@ -433,7 +448,7 @@ __global__ __launch_bounds__(NTH) void k_ls_pencil(LsGeom<T> g, cplx<T>* __restr
 // sweep 5: inverse x FFT (complex-to-real) + update + FP64 partial norms
 // ---------------------------------------------------------------------------
 template <typename T, int D, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
-__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 ? 6 : (sizeof(T) == 4 ? 4 : 2)))) void k_ls_xinv_update(LsGeom<T> g, const cplx<T>* __restrict__ spec,
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 ? (sizeof(T) == 4 ? 4 : 2) : (sizeof(T) == 4 ? 4 : 2)))) void k_ls_xinv_update(LsGeom<T> g, const cplx<T>* __restrict__ spec,
                                                            T* __restrict__ eps, Sym6 ebar,
                                                            double* __restrict__ partials,
                                                            const LsState* __restrict__ st) {
@@ -699,8 +714,8 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_pol_tau(const T* __restrict__ eps
   }
 }
 
-template <typename T, int MAXV, int NN = 0, int BB = 0>
-__global__ __launch_bounds__(MGK_NT) void k_ls_xfwd_real(LsGeom<T> g, const T* __restrict__ f,
+template <typename T, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
+__global__ __launch_bounds__(NTH) void k_ls_xfwd_real(LsGeom<T> g, const T* __restrict__ f,
                                                          cplx<T>* __restrict__ spec, const LsState* __restrict__ st) {
   if (st->done) return;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -713,7 +728,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xfwd_real(LsGeom<T> g, const T* _
   const int q0 = (tid_lin % gridDim.x) * B, p = tid_lin / gridDim.x;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
-  for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
+  for (int t = threadIdx.x; t < nx; t += NTH) tw[t] = g.twx[t];
   const Tile tl{nx, logB, ldb, 1, 0};
   constexpr int VW = vw16<T>();
   constexpr bool VOK = CT && (BB % VW == 0);
@@ -726,7 +741,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xfwd_real(LsGeom<T> g, const T* _
     const T* fb = f + (size_t)(hasb ? 2 * pr + 1 : 2 * pr) * NV;
     if (vec) {
       constexpr int GPR = VOK ? BB / VW : 1;
-      for (int e = threadIdx.x; e < nx * GPR; e += MGK_NT) {
+      for (int e = threadIdx.x; e < nx * GPR; e += NTH) {
         const int bg = e % GPR, x = e / GPR;
         const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
         T a[VW], b[VW];
@@ -736,7 +751,7 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xfwd_real(LsGeom<T> g, const T* _
         for (int i = 0; i < VW; ++i) tile[x * ldb + bg * VW + i] = cmk(a[i], hasb ? b[i] : T(0));
       }
     } else {
-      for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
+      for (int e = threadIdx.x; e < nx * B; e += NTH) {
         const int b = e & (B - 1), x = e >> logB, q = q0 + b;
         cplx<T> z = cmk(T(0), T(0));
         if (q < Q) {
@@ -747,9 +762,9 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_xfwd_real(LsGeom<T> g, const T* _
       }
     }
     __syncthreads();
-    if constexpr (CT) tile_fft_ct<NN, BB, 1>(tile, tw, false);
+    if constexpr (CT) tile_fft_ct<NN, BB, 1, false, NTH>(tile, tw, false);
     else tile_fft<MAXV>(tile, tl, g.plx, tw, false);
-    store_half<T, NN, BB>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)(2 * pr) * cs,
+    store_half<T, NN, BB, NTH>(tile, vec, ldb, nx, g.nxh, logB, q0, Q, P, p, spec + (size_t)(2 * pr) * cs,
                           hasb ? spec + (size_t)(2 * pr + 1) * cs : (cplx<T>*)nullptr);
     __syncthreads();
   }
@@ -1163,10 +1178,11 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
   const int CTT = ct_tile<T>();
   auto ct_ok = [](int n) { return n == 64 || n == 128 || n == 256 || n == 512; };
   pl->ctx = pl->cty = pl->ctq = 0;
-  if (ct_ok(pl->nx) && pl->Q >= CTT / pl->nx && (size_t)(pl->nx + CTT) * cb <= lim) {
+  const int XTT = x_tile<T>();
+  if (ct_ok(pl->nx) && pl->Q >= XTT / pl->nx && (size_t)(pl->nx + XTT) * cb <= lim) {
     pl->ctx = 1;
-    pl->logBx = mgk_ilog2(CTT / pl->nx);
-    pl->lds_x = (size_t)(pl->nx + CTT) * cb;
+    pl->logBx = mgk_ilog2(XTT / pl->nx);
+    pl->lds_x = (size_t)(pl->nx + XTT) * cb;
   }
   if (ct_ok(pl->P) && pl->Q >= CTT / pl->P && (size_t)(pl->P + CTT) * cb <= lim) {
     pl->cty = 1;
@@ -1199,9 +1215,6 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
 // threads per workgroup of the compile-time mid-axis sweep (its tiles are
 // LDS-bound to four per CU, so wider workgroups give more waves per CU)
 constexpr int MID_NT = 512;
-// threads per workgroup of the compile-time x sweeps (512 measured slower:
-// the staged values per lane do not shrink enough to fit more waves)
-constexpr int X_NT = MGK_NT;
 
 template <typename T, int MV>
 void launch_mid(const LsPlan& pl, dim3 gy, hipStream_t stream, const FftPlan& ply, const cplx<T>* twy, cplx<T>* spec,
@@ -1252,6 +1265,8 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
                const L2Mode* l2 = nullptr, const SecLaw* sl = nullptr) {
   constexpr int MV = maxv<T>();
   constexpr int TT = ct_tile<T>();  // points per compile-time tile
+  constexpr int XT = x_tile<T>();   // points per compile-time x-sweep tile
+  constexpr int X_NT = x_nt<T>();   // threads per x-sweep workgroup
   constexpr int C = (D == 3) ? 6 : 3;
   const int prec = sizeof(T) == 8 ? 0 : 1;
   LsGeom<T> g;
@@ -1343,10 +1358,10 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
         k_l2_after_eval<<<1, 1, 0, stream>>>(st, l2->persist);
         if (pl.ctx) {
           switch (pl.nx) {
-            case 64: k_ls_xfwd_real<T, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
-            case 128: k_ls_xfwd_real<T, TT / MGK_NT, 128, TT / 128><<<gx, MGK_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
-            case 256: k_ls_xfwd_real<T, TT / MGK_NT, 256, TT / 256><<<gx, MGK_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
-            default: k_ls_xfwd_real<T, TT / MGK_NT, 512, TT / 512><<<gx, MGK_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
+            case 64: k_ls_xfwd_real<T, XT / X_NT, 64, XT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
+            case 128: k_ls_xfwd_real<T, XT / X_NT, 128, XT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
+            case 256: k_ls_xfwd_real<T, XT / X_NT, 256, XT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
+            default: k_ls_xfwd_real<T, XT / X_NT, 512, XT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
           }
         } else {
           k_ls_xfwd_real<T, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st);
@@ -1365,10 +1380,10 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
       }
       if (pl.ctx) {
         switch (pl.nx) {
-          case 64: k_ls_xfwd_real<T, TT / MGK_NT, 64, TT / 64><<<gx, MGK_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
-          case 128: k_ls_xfwd_real<T, TT / MGK_NT, 128, TT / 128><<<gx, MGK_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
-          case 256: k_ls_xfwd_real<T, TT / MGK_NT, 256, TT / 256><<<gx, MGK_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
-          default: k_ls_xfwd_real<T, TT / MGK_NT, 512, TT / 512><<<gx, MGK_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
+          case 64: k_ls_xfwd_real<T, XT / X_NT, 64, XT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
+          case 128: k_ls_xfwd_real<T, XT / X_NT, 128, XT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
+          case 256: k_ls_xfwd_real<T, XT / X_NT, 256, XT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
+          default: k_ls_xfwd_real<T, XT / X_NT, 512, XT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
         }
       } else {
         k_ls_xfwd_real<T, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, taubuf, spec, st);
@@ -1377,10 +1392,10 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
     if (pl.ctx) {
       switch (pl.nx) {
-        case 64: k_ls_tau_xfwd<T, D, TT / X_NT, 64, TT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-        case 128: k_ls_tau_xfwd<T, D, TT / X_NT, 128, TT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-        case 256: k_ls_tau_xfwd<T, D, TT / X_NT, 256, TT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-        default: k_ls_tau_xfwd<T, D, TT / X_NT, 512, TT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        case 64: k_ls_tau_xfwd<T, D, XT / X_NT, 64, XT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        case 128: k_ls_tau_xfwd<T, D, XT / X_NT, 128, XT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        case 256: k_ls_tau_xfwd<T, D, XT / X_NT, 256, XT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
+        default: k_ls_tau_xfwd<T, D, XT / X_NT, 512, XT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
       }
     } else {
       k_ls_tau_xfwd<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st);
@@ -1416,10 +1431,10 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
     if (pl.ctx) {
       switch (pl.nx) {
-        case 64: k_ls_xinv_update<T, D, TT / X_NT, 64, TT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-        case 128: k_ls_xinv_update<T, D, TT / X_NT, 128, TT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-        case 256: k_ls_xinv_update<T, D, TT / X_NT, 256, TT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-        default: k_ls_xinv_update<T, D, TT / X_NT, 512, TT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        case 64: k_ls_xinv_update<T, D, XT / X_NT, 64, XT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        case 128: k_ls_xinv_update<T, D, XT / X_NT, 128, XT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        case 256: k_ls_xinv_update<T, D, XT / X_NT, 256, XT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
+        default: k_ls_xinv_update<T, D, XT / X_NT, 512, XT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
       }
     } else {
       k_ls_xinv_update<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st);
