@@ -141,6 +141,17 @@ int mgk_ls_solve_secant(int ndim, const int64_t* shape, int precision, const voi
                         const void* eps_plastic, void* eps, void* sig, void* tau_buf, int warm, int max_iter,
                         double tol, void* workspace, size_t workspace_bytes, void* stream, mgk_ls_info* info);
 
+/* Heat conduction of the local temperature field, the thermal extension of the KMC
+ * simulators (heat_conducting_3d, mgkmc/aqs.py:462-478; heat_conducting_2d,
+ * mgkmc/kmc_simulator.py:559-575): T <- F^-1[exp(-|k|^2 Ddt) F[T]] with k the
+ * periodic wave numbers of box lengths lengths[ndim], then thermostat 1: T0 + (T - T0) *
+ * bath_factor, 2: T - (<T> - T0), and the cap min(T, cap) when cap > 0.  T: FP64,
+ * in place.  Workspace from mgk_heat_workspace_bytes. */
+int mgk_heat_workspace_bytes(int ndim, const int64_t* shape, size_t* bytes);
+int mgk_heat_step(int ndim, const int64_t* shape, double* T, const double* lengths, double Ddt, double T0,
+                  int thermostat, double bath_factor, double cap, void* workspace, size_t workspace_bytes,
+                  void* stream);
+
 /* ---------------------------------------------------------------------------
  * KMC: barriers, rates, selection, compaction, flips.
  * Device layouts: Q, Q0 [m][v]; catalog [k][m][v] with k over 2 planar

@@ -56,6 +56,8 @@ class ThermalSimulation(DeviceKmc):
                     scale_rate_by_volume, "plane_strain", ls_precision, ls_warm_start, rng, selection_stream,
                     init_modes, dict(mean=2.0, std=0.6), law=law, finite=finite)
         self.max_cascade_steps_pct = max_cascade_steps_pct
+        if enable_thermal:
+            self._setup_thermal(Cp, rho, thermal_diffusivity, thermal_coords, temperature_cap, thermostat, tau_bath)
 
     def update_stresses(self):
         """LS re-equilibration with the current eps_plastic and eps_macro

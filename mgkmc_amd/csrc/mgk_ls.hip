@@ -1135,7 +1135,7 @@ struct LsPlan {
 };
 
 template <typename T>
-int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
+int ls_plan(int ndim, const int64_t* shape, LsPlan* pl, bool allow_ct = true) {
   if (ndim != 2 && ndim != 3) return MGK_ERR_ARG;
   pl->D = ndim;
   pl->C = ndim == 3 ? 6 : 3;
@@ -1179,17 +1179,17 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl) {
   auto ct_ok = [](int n) { return n == 64 || n == 128 || n == 256 || n == 512; };
   pl->ctx = pl->cty = pl->ctq = 0;
   const int XTT = x_tile<T>();
-  if (ct_ok(pl->nx) && pl->Q >= XTT / pl->nx && (size_t)(pl->nx + XTT) * cb <= lim) {
+  if (allow_ct && ct_ok(pl->nx) && pl->Q >= XTT / pl->nx && (size_t)(pl->nx + XTT) * cb <= lim) {
     pl->ctx = 1;
     pl->logBx = mgk_ilog2(XTT / pl->nx);
     pl->lds_x = (size_t)(pl->nx + XTT) * cb;
   }
-  if (ct_ok(pl->P) && pl->Q >= CTT / pl->P && (size_t)(pl->P + CTT) * cb <= lim) {
+  if (allow_ct && ct_ok(pl->P) && pl->Q >= CTT / pl->P && (size_t)(pl->P + CTT) * cb <= lim) {
     pl->cty = 1;
     pl->logBy = mgk_ilog2(CTT / pl->P);
     pl->lds_y = (size_t)(pl->P + CTT) * cb;
   }
-  if (ct_ok(pl->Q)) {
+  if (allow_ct && ct_ok(pl->Q)) {
     int pp = 1;
     while (2 * pp * pl->C * pl->Q <= CTT) pp *= 2;
     if (npen >= pp && (size_t)(pl->Q + tpad_size(pl->C * pl->Q * pp)) * cb <= lim) {
@@ -1242,6 +1242,83 @@ int ls_iter_hint(const void* ws, int iters) {
   }
   auto it = last.find(ws);
   return it == last.end() ? 0 : it->second;
+}
+
+// ---------------------------------------------------------------------------
+// Heat conduction of the local temperature field (the thermal extension of
+// the KMC simulators, mgkmc/aqs.py:462-478, mgkmc/kmc_simulator.py:559-575):
+// T <- F^-1[exp(-|k|^2 D dt) F[T]], then the thermostat and the cap.  The x
+// pass is k_ls_xfwd_real with one component, the y pass k_fft_mid; this
+// pencil kernel multiplies along z, k_heat_xinv transforms back.
+// ---------------------------------------------------------------------------
+struct HeatPar {
+  double kx, ky, kz;  // wave number per frequency bin along each axis (2 pi / L)
+  double Ddt;         // diffusivity * dt
+  double scale;       // 1 / NV (the transforms are unnormalised)
+};
+
+template <typename T, int MAXV>
+__global__ __launch_bounds__(MGK_NT) void k_heat_pencil(LsGeom<T> g, cplx<T>* __restrict__ spec, HeatPar hp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* tile = tw + g.Q;
+  const int Q = g.Q, P = g.P, logPP = g.logPP, PP = 1 << logPP;
+  const int pid0 = blockIdx.x * PP, npen = g.npencil;
+  for (int t = threadIdx.x; t < Q; t += MGK_NT) tw[t] = g.twq[t];
+  for (int e = threadIdx.x; e < Q * PP; e += MGK_NT) {
+    const int pp = e / Q, q = e - pp * Q, pid = pid0 + pp;
+    tile[q * PP + pp] = (pid < npen) ? spec[(size_t)pid * Q + q] : cmk(T(0), T(0));
+  }
+  __syncthreads();
+  const Tile tl{Q, logPP, PP, 1, Q * PP};
+  tile_fft<MAXV>(tile, tl, g.plq, tw, false);
+  for (int e = threadIdx.x; e < Q * PP; e += MGK_NT) {
+    const int pp = e & (PP - 1), q = e >> logPP, pid = pid0 + pp;
+    if (pid >= npen) continue;
+    const int kx = pid / P, p = pid - kx * P;
+    const double ax = hp.kx * fbin(kx, g.nx), ay = hp.ky * fbin(p, P), az = hp.kz * fbin(q, Q);
+    const double k2 = (ax * ax + ay * ay) + az * az;
+    const T f = T(exp(-k2 * hp.Ddt) * hp.scale);
+    const cplx<T> v = tile[q * PP + pp];
+    tile[q * PP + pp] = cmk(v.x * f, v.y * f);
+  }
+  __syncthreads();
+  tile_fft<MAXV>(tile, tl, g.plq, tw, true);
+  for (int e = threadIdx.x; e < Q * PP; e += MGK_NT) {
+    const int pp = e / Q, q = e - pp * Q, pid = pid0 + pp;
+    if (pid < npen) spec[(size_t)pid * Q + q] = tile[q * PP + pp];
+  }
+}
+
+// inverse x transform of one half-spectrum component into a real field
+template <typename T, int MAXV>
+__global__ __launch_bounds__(MGK_NT) void k_heat_xinv(LsGeom<T> g, const cplx<T>* __restrict__ spec,
+                                                      T* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* tile = tw + g.nx;
+  const int nx = g.nx, P = g.P, Q = g.Q, logB = g.logBx, B = 1 << logB;
+  const int q0 = blockIdx.x * B, p = blockIdx.y;
+  for (int t = threadIdx.x; t < nx; t += MGK_NT) tw[t] = g.twx[t];
+  xmerge_load<T, MGK_NT>(tile, B, nx, g.nxh, logB, q0, Q, P, p, spec, (const cplx<T>*)nullptr);
+  const Tile tl{nx, logB, B, 1, 0};
+  tile_fft<MAXV>(tile, tl, g.plx, tw, true);
+  for (int e = threadIdx.x; e < nx * B; e += MGK_NT) {
+    const int b = e & (B - 1), x = e >> logB, q = q0 + b;
+    if (q < Q) out[((size_t)x * P + p) * Q + q] = tile[x * B + b].x;
+  }
+}
+
+// thermostat (mode 1: T0 + (T - T0) f; mode 2: T - (<T> - T0)) and cap (cap > 0)
+__global__ void k_heat_post(double* __restrict__ Tf, size_t NV, double T0, int mode, double f, double cap,
+                            const double* __restrict__ mean) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < NV; i += (size_t)gridDim.x * blockDim.x) {
+    double t = Tf[i];
+    if (mode == 1) t = T0 + (t - T0) * f;
+    else if (mode == 2) t = t - (mean[0] - T0);
+    if (cap > 0) t = fmin(t, cap);
+    Tf[i] = t;
+  }
 }
 
 // 2D Landau mode of the LS loop (FP64)
@@ -1739,6 +1816,99 @@ extern "C" int mgk_ls_solve_secant(int ndim, const int64_t* shape, int precision
   return ls_solve_t<float, 2>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar, (const float*)eps_plastic,
                               (float*)eps, (float*)sig, warm, max_iter, tol, (char*)workspace, s, info, true, nullptr,
                               (float*)tau_buf, nullptr, &L);
+}
+
+namespace {
+struct HeatWs {
+  size_t off_part, off_state, total;
+};
+HeatWs heat_ws(const LsPlan& pl) {
+  HeatWs w;
+  w.off_part = mgk_align((size_t)pl.nxh * pl.P * pl.Q * sizeof(cplx<double>));
+  w.off_state = w.off_part + mgk_align((size_t)pl.nblk_red * sizeof(double));
+  w.total = w.off_state + mgk_align(sizeof(LsState));
+  return w;
+}
+}  // namespace
+
+extern "C" int mgk_heat_workspace_bytes(int ndim, const int64_t* shape, size_t* bytes) {
+  LsPlan pl;
+  if (!shape || !bytes || ls_plan<double>(ndim, shape, &pl, false) != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the heat solver", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  *bytes = heat_ws(pl).total;
+  return MGK_OK;
+}
+
+extern "C" int mgk_heat_step(int ndim, const int64_t* shape, double* Tf, const double* lengths, double Ddt, double T0,
+                             int thermostat, double bath_factor, double cap, void* workspace, size_t workspace_bytes,
+                             void* stream) {
+  LsPlan pl;
+  if (!shape || !Tf || !lengths || !workspace || thermostat < 0 || thermostat > 2 ||
+      ls_plan<double>(ndim, shape, &pl, false) != MGK_OK) {
+    mgk_set_error("mgk_heat_step: bad argument or unsupported grid shape", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  const HeatWs w = heat_ws(pl);
+  if (workspace_bytes < w.total) {
+    mgk_set_error("workspace too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  constexpr int MV = maxv<double>();
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  cplx<double>* spec = (cplx<double>*)ws;
+  double* part = (double*)(ws + w.off_part);
+  LsState* st = (LsState*)(ws + w.off_state);
+  LsGeom<double> g;
+  g.nx = pl.nx;
+  g.P = pl.P;
+  g.Q = pl.Q;
+  g.nxh = pl.nxh;
+  g.C = 1;
+  g.NV = pl.NV;
+  g.logBx = pl.logBx;
+  g.logBy = pl.logBy;
+  g.logPP = pl.logPP;
+  g.npencil = pl.nxh * pl.P;
+  mgk_make_plan(pl.nx, &g.plx);
+  mgk_make_plan(pl.P, &g.ply);
+  mgk_make_plan(pl.Q, &g.plq);
+  g.twx = (const cplx<double>*)mgk_twiddles(pl.nx, 0);
+  g.twy = (const cplx<double>*)mgk_twiddles(pl.P, 0);
+  g.twq = (const cplx<double>*)mgk_twiddles(pl.Q, 0);
+  if (!g.twx || !g.twy || !g.twq) {
+    mgk_set_error("twiddle table allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
+  HeatPar hp;
+  const double two_pi = 6.283185307179586;
+  hp.kx = two_pi / lengths[0];
+  hp.ky = ndim == 3 ? two_pi / lengths[1] : 0.0;
+  hp.kz = two_pi / lengths[ndim - 1];
+  hp.Ddt = Ddt;
+  hp.scale = 1.0 / (double)pl.NV;
+  MGK_HIP(hipMemsetAsync(st, 0, sizeof(LsState), s));
+  const int nqx = (pl.Q + (1 << pl.logBx) - 1) >> pl.logBx;
+  const int nqy = (pl.Q + (1 << pl.logBy) - 1) >> pl.logBy;
+  const int npb = (g.npencil + (1 << pl.logPP) - 1) >> pl.logPP;
+  const dim3 gx(nqx, pl.P), gy(nqy, pl.nxh, 1);
+  k_ls_xfwd_real<double, MV><<<gx, MGK_NT, pl.lds_x, s>>>(g, Tf, spec, st);
+  if (pl.P > 1) launch_mid<double, MV>(pl, gy, s, g.ply, g.twy, spec, 0, nullptr);
+  k_heat_pencil<double, MV><<<npb, MGK_NT, pl.lds_q, s>>>(g, spec, hp);
+  if (pl.P > 1) launch_mid<double, MV>(pl, gy, s, g.ply, g.twy, spec, 1, nullptr);
+  k_heat_xinv<double, MV><<<gx, MGK_NT, pl.lds_x, s>>>(g, spec, Tf);
+  if (thermostat == 2) {
+    k_comp_sums<double><<<pl.nblk_red, MGK_NT, 0, s>>>(Tf, 1, (size_t)pl.NV, part);
+    k_means_finalize<<<1, MGK_NT, 0, s>>>(part, pl.nblk_red, 1, (double)pl.NV, st->mean);
+  }
+  if (thermostat != 0 || cap > 0) {
+    const int nb = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 4096);
+    k_heat_post<<<nb, MGK_NT, 0, s>>>(Tf, (size_t)pl.NV, T0, thermostat, bath_factor, cap, st->mean);
+  }
+  MGK_HIP(hipGetLastError());
+  return MGK_OK;
 }
 
 extern "C" int mgk_ls_solve_landau2d(const int64_t* shape, const double* lam, const double* mu, double lam0,

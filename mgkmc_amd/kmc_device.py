@@ -139,15 +139,16 @@ class KmcState:
         self._shape_c = (ctypes.c_int * d)(*self.shape)
         self.gamma = torch.zeros(d * d, dtype=torch.float64, device="cuda")
 
-    def barrier_rates(self, sig, sig_kind, p, write_q=True):
+    def barrier_rates(self, sig, sig_kind, p, write_q=True, Tfield=None):
         """Fused compute_barrier + compute_rates; returns the FP64 total rate.
-        write_q: also store the barriers in self.Q ([m][v] FP64)."""
+        write_q: also store the barriers in self.Q ([m][v] FP64); Tfield: a
+        per-voxel temperature (FP64 device field) instead of p.temperature."""
         if write_q and self.Q is None:
             self.Q = dev.torch.zeros_like(self.Q0)
         tot = ctypes.c_double()
         rc = _lib.load().mgk_kmc_barrier_rates(
             self.d, self.NV, self.M, sig_kind, self.cat_kind, _lib.dptr(sig), _lib.dptr(self.Q0),
-            _lib.dptr(self.cat), _lib.dptr(self.soft), _lib.dptr(self.tlast), _lib.dptr(self.prev), None,
+            _lib.dptr(self.cat), _lib.dptr(self.soft), _lib.dptr(self.tlast), _lib.dptr(self.prev), _lib.dptr(Tfield),
             ctypes.byref(p), _lib.dptr(self.Q) if write_q else None, self.ws.ptr,
             ctypes.c_void_p(dev.stream_ptr()), ctypes.byref(tot))
         _lib.check(rc, "mgk_kmc_barrier_rates")
@@ -179,15 +180,15 @@ class KmcState:
         q = self.Q[m, v]
         return v.cpu().numpy(), m.cpu().numpy(), q.cpu().numpy()
 
-    def select(self, u, sig, sig_kind, p):
+    def select(self, u, sig, sig_kind, p, Tfield=None):
         """(voxel, mode) of the event for draw u in [0, 1): r = total * u, after
-        barrier_rates with the same sig and parameters.  self.sel.q holds the
-        selected mode's barrier."""
+        barrier_rates with the same sig, parameters and temperature field.
+        self.sel.q holds the selected mode's barrier."""
         rc = _lib.load().mgk_kmc_select(self.d, self.NV, self.M, sig_kind, self.cat_kind, float(u),
                                         _lib.dptr(sig), _lib.dptr(self.Q0), _lib.dptr(self.cat),
-                                        _lib.dptr(self.soft), _lib.dptr(self.tlast), _lib.dptr(self.prev), None,
-                                        ctypes.byref(p), self.ws.ptr, ctypes.c_void_p(dev.stream_ptr()),
-                                        ctypes.byref(self.sel))
+                                        _lib.dptr(self.soft), _lib.dptr(self.tlast), _lib.dptr(self.prev),
+                                        _lib.dptr(Tfield), ctypes.byref(p), self.ws.ptr,
+                                        ctypes.c_void_p(dev.stream_ptr()), ctypes.byref(self.sel))
         _lib.check(rc, "mgk_kmc_select")
         return int(self.sel.item), int(self.sel.mode)
 

@@ -101,8 +101,9 @@ class DeviceKmc:
         bad = list(extra)
         if fast_patching and fast_patching.get("enabled", False):
             bad.append("fast_patching")
-        if enable_thermal:
-            bad.append("enable_thermal")
+        if enable_thermal and (strain_assumption != "small_strain" or
+                               hyperelastic_model in ("landau", "secant_degradation", "rose")):
+            bad.append("enable_thermal with finite strain or a nonlinear law")
         if strain_assumption != "small_strain" and not getattr(self, "_finite_ok", False):
             bad.append("finite_strain KMC")
         if bad:
@@ -207,10 +208,67 @@ class DeviceKmc:
         self.eps_macro = np.zeros((d, d))
         self.time = 0.0
         self.solver_args = {}
+        self.enable_thermal = False
+        self.Tfield = None
         self.flip_event_history = []
         self.cascade_history = []  # (global step, batch, x, y[, z], m) per cascade flip
         self.max_cascade_steps_pct = 0.3
         self.ls_info = None
+
+    # ----- thermal extension (local temperature, heat conduction) -----------
+    def _setup_thermal(self, Cp, rho, thermal_diffusivity, thermal_coords, temperature_cap, thermostat, tau_bath):
+        """Local temperature field on the device (mgkmc/aqs.py:83-99,
+        kmc_simulator.py:60-75): T = temperature everywhere; heat conduction
+        on the periodic box of n (pixel coordinates) or n * pixel (physical,
+        diffusivity m^2/s -> nm^2/s) per axis."""
+        self.enable_thermal = True
+        self.Cp, self.rho = float(Cp), float(rho)
+        self.thermal_diffusivity = float(thermal_diffusivity)
+        self.thermal_coords = thermal_coords.lower()
+        self.temperature_cap, self.thermostat, self.tau_bath = float(temperature_cap), bool(thermostat), float(tau_bath)
+        torch = dev.torch
+        self.Tfield = torch.full(self.grid_shape, float(self.temperature), dtype=torch.float64, device="cuda")
+        phys = self.thermal_coords != "pixel"
+        self._heat_L = (ctypes.c_double * 3)(*[float(n) * (self.pixel if phys else 1.0) for n in self.grid_shape])
+        self.diffusivity_scaled = self.thermal_diffusivity * (1e18 if phys else 1.0)
+        nb = ctypes.c_size_t()
+        self._heat_shape = _lib.shape_arg(self.grid_shape)
+        _lib.check(_lib.load().mgk_heat_workspace_bytes(self.ndim, self._heat_shape, ctypes.byref(nb)),
+                   "mgk_heat_workspace_bytes")
+        self._heat_ws = torch.empty(int(nb.value), dtype=torch.uint8, device="cuda")
+
+    @property
+    def Tlocal(self):
+        return dev.download(self.Tfield) if self.enable_thermal else None
+
+    def heat_conducting(self, dt):
+        """heat_conducting_3d / _2d (mgkmc/aqs.py:462-478, kmc_simulator.py:559-575)
+        on the device: spectral diffusion over dt, thermostat, cap."""
+        if not self.enable_thermal or dt <= 0:
+            return
+        mode = 0 if not self.thermostat else (1 if self.tau_bath > 0 else 2)
+        f = float(np.exp(-dt / self.tau_bath)) if mode == 1 else 1.0
+        rc = _lib.load().mgk_heat_step(self.ndim, self._heat_shape, _lib.dptr(self.Tfield), self._heat_L,
+                                       self.diffusivity_scaled * dt, float(self.temperature), mode, f,
+                                       self.temperature_cap, _lib.dptr(self._heat_ws), self._heat_ws.numel(),
+                                       ctypes.c_void_p(dev.stream_ptr()))
+        _lib.check(rc, "mgk_heat_step")
+
+    def _flip_heat(self, v, C):
+        """Heat of a flip at voxel v (mgkmc/aqs.py:1000-1005): T += |sigma:C| /
+        (rho Cp) with the stress before the re-solve, capped."""
+        if not self.enable_thermal:
+            return
+        d = self.ndim
+        s = dev.download(self._stress.sig[(slice(None),) + tuple(int(c) for c in self.kmc.voxel_coords(v))])
+        sig = dev.from_sym_soa(s.reshape((-1,) + (1,) * d), d).reshape(d, d) if self.sig_kind != 2 else \
+            s.reshape(d, d)
+        dT = abs(np.sum(sig * C)) / (self.rho * self.Cp)
+        flat = self.Tfield.view(-1)
+        t = float(flat[int(v)]) + dT
+        if self.temperature_cap > 0:
+            t = min(t, self.temperature_cap)
+        flat[int(v)] = t
 
     # ----- state mirrors (synchronised on access) --------------------------
     @property
@@ -257,7 +315,7 @@ class DeviceKmc:
     def Q(self):
         """Barriers of the current state (evaluated on access; the event loop
         does not store them)."""
-        self.kmc.barrier_rates(self._stress.sig, self.sig_kind, self._params(), write_q=True)
+        self.kmc.barrier_rates(self._stress.sig, self.sig_kind, self._params(), write_q=True, Tfield=self.Tfield)
         return modes_from_device(self.kmc.Q, self.grid_shape)
 
     @property
@@ -423,7 +481,7 @@ class DeviceKmc:
         remaining_time); an event is (x, y[, z], m, time, is_instab)."""
         p = self._params()
         sig = self._stress.sig
-        total = self.kmc.barrier_rates(sig, self.sig_kind, p, write_q=False)
+        total = self.kmc.barrier_rates(sig, self.sig_kind, p, write_q=False, Tfield=self.Tfield)
         if total > 0:
             u = self.rng.random_sample()
             t_wait = -np.log(u) / total
@@ -432,15 +490,21 @@ class DeviceKmc:
                 self.eps_macro += strain_unit_tensor * (self.strain_rate * t_wait)
                 remaining_time -= t_wait
                 us = self.selection_stream.random_sample()
-                v, m = self.kmc.select(us, sig, self.sig_kind, p)
+                v, m = self.kmc.select(us, sig, self.sig_kind, p, Tfield=self.Tfield)
                 coords = tuple(int(c) for c in self.kmc.voxel_coords(v))
                 is_instab = self.kmc.sel.q <= self.stability_threshold
+                # the reference conducts heat over t_wait before selecting, from
+                # rates computed before it: conducting after the selection is the same
+                self.heat_conducting(t_wait)
+                if self.enable_thermal:
+                    self._flip_heat_mode = self._mode_host(v, m)
                 if self.finite:
                     self._flip_finite(v, m)
                 elif self.law is not None and self.ndim == 2:
                     self._flip_substepped([(v, m)])
                 else:
                     self._flip(v, m, 1)
+                    self._flip_heat(v, getattr(self, "_flip_heat_mode", None))
                     self._after_flip_solve()
                 if self.ls_info is not None and not self.finite:
                     self.ls_iters_total += self.ls_info.iters + getattr(self.ls, "iters32", 0)
@@ -448,6 +512,7 @@ class DeviceKmc:
                 self.flip_event_history.append(ev)
                 return ev, remaining_time
         self.eps_macro += strain_unit_tensor * (self.strain_rate * remaining_time)
+        self.heat_conducting(remaining_time)
         self.time += remaining_time
         return None, 0.0
 
@@ -523,7 +588,7 @@ class DeviceKmc:
         kmc_simulator.py:581-597): barriers of the current state, modes with
         Q < stability_threshold sorted by Q (np.argsort over the voxel-major
         scan, as the reference sorts), the first (most unstable) per voxel."""
-        self.kmc.barrier_rates(self._stress.sig, self.sig_kind, self._params(), write_q=True)
+        self.kmc.barrier_rates(self._stress.sig, self.sig_kind, self._params(), write_q=True, Tfield=self.Tfield)
         v, m, q = self.kmc.unstable_modes(self.stability_threshold)
         pairs, seen = [], set()
         for i in np.argsort(q):
@@ -549,7 +614,9 @@ class DeviceKmc:
                 self._flip_substepped(pairs)
             else:
                 for v, m in pairs:
+                    C = self._mode_host(v, m) if self.enable_thermal else None
                     self._flip(v, m, 1)
+                    self._flip_heat(v, C)
                 self.update_stresses()
             for v, m in pairs:
                 self.cascade_history.append((step, local) + tuple(int(c) for c in self.kmc.voxel_coords(v)) + (m,))
@@ -588,7 +655,8 @@ class DeviceKmc:
         gen = self.barrier_generator
         gp = getattr(gen, "params", None)
         same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
-        return (self.native_loop and self.law is None and not self.finite and not self.cascade_mode and self.kmc.planar
+        return (self.native_loop and self.law is None and not self.finite and not self.cascade_mode
+                and not self.enable_thermal and self.kmc.planar
                 and gp is not None and same_stream and (self.redraw_directions or self.redraw_barriers)
                 and hasattr(self.rng, "get_state") and hasattr(self.selection_stream, "get_state")
                 and (self._solved_once or not self.ls_warm_start))

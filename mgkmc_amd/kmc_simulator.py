@@ -52,6 +52,8 @@ class KmcSimulation2D(DeviceKmc):
                     stability_threshold, redraw_directions, redraw_barriers, nu0, q_act_temp, cascade_mode,
                     scale_rate_by_volume, plane_mode, ls_precision, ls_warm_start, rng, selection_stream,
                     init_modes, dict(mean=2.0, std=0.6, min_cutoff=0.1), law=law)
+        if enable_thermal:
+            self._setup_thermal(Cp, rho, thermal_diffusivity, thermal_coords, temperature_cap, thermostat, tau_bath)
 
     def elastic_run(self, eps_macro):
         """Small-strain equilibrium (mgkmc/kmc_simulator.py:449-525): LS solve

@@ -29,6 +29,16 @@ most 40, then N + 1 .. on ValueError, state restored), with softening, prev
 and redraws at the last sub-step and a finite-strain step after each; no
 mixed-BC loop after an increment (the step enforces the stress targets).
 
+enable_thermal (:83-99, :462-478, :877, :1000-1005; also in cascades,
+:630-635): a local temperature field (initially the temperature) sets the
+rates; after a waiting time dt (and the remaining time of an increment) it
+diffuses spectrally, T <- ifft(fft(T) exp(-|k|^2 D dt)), with k of the box n
+(pixel coordinates) or n * pixel (physical, D in m^2/s -> nm^2/s), then the
+thermostat (T0 + (T - T0) exp(-dt/tau_bath), or T - (<T> - T0) for tau_bath
+<= 0) and the cap; a flip heats its voxel by |sigma:C| / (rho Cp) (stress
+before the re-solve), capped.  The rates of an event are those computed
+before its heat step.
+
 Random streams: numpy's global legacy stream exactly as the reference uses it
 (Q0, then the catalog voxel by voxel with stz_catalog_glass, then per event
 one uniform for the waiting time, 2M normals for the new modes and M normals
@@ -43,12 +53,44 @@ from . import ls as ol
 KB = 8.617e-5
 
 
+def heat_operator(shape, pixel, thermal_diffusivity, thermal_coords="pixel"):
+    """|k|^2 on the periodic box (n per axis in pixel coordinates, n * pixel in
+    physical ones) and the diffusivity in those units (m^2/s -> nm^2/s for
+    physical), as the simulators build them (mgkmc/aqs.py:93-99,
+    kmc_simulator.py:66-75)."""
+    phys = thermal_coords.lower() != "pixel"
+    Ls = [n * pixel if phys else float(n) for n in shape]
+    ks = np.meshgrid(*[2 * np.pi * np.fft.fftfreq(n, d=L / n) for n, L in zip(shape, Ls)], indexing="ij")
+    k2 = ks[0] ** 2 + ks[1] ** 2
+    if len(shape) == 3:
+        k2 = k2 + ks[2] ** 2
+    return k2, thermal_diffusivity * (1e18 if phys else 1.0)
+
+
+def heat_step(Tl, k2, Dsc, dt, temperature, thermostat, tau_bath, temperature_cap):
+    """heat_conducting_3d / _2d in place (mgkmc/aqs.py:462-478)."""
+    if dt <= 0:
+        return
+    Th = ol.fft_field(Tl)
+    Th *= np.exp(-k2 * Dsc * dt)
+    Tl[...] = ol.ifft_field(Th)
+    if thermostat:
+        if tau_bath > 0:
+            Tl[...] = temperature + (Tl - temperature) * np.exp(-dt / tau_bath)
+        else:
+            Tl[...] = Tl - (np.mean(Tl) - temperature)
+    if temperature_cap > 0:
+        Tl[...] = np.clip(Tl, a_min=None, a_max=temperature_cap)
+
+
 def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, step_size, component,
                n_events, seed, sel_seed, barrier_kwargs=None, softening_scheme="isotropic", softening_cap=2.0,
                jp=10.0, jt=30.0, jn=0.0, nu0=1e13, q_act_temp=0.37, ls_tol=1e-6, stress_targets=None,
                mixed_tol=1e-4, mixed_max_iter=50, n_global_steps=10 ** 9, cascade_mode=False,
                stability_threshold=0.0, max_cascade_steps_pct=0.3, eps_plastic=None,
-               strain_assumption="small_strain"):
+               strain_assumption="small_strain", enable_thermal=False, Cp=420.0, rho=6125.0,
+               thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
+               tau_bath=0.0):
     """Returns the list of events (x, y, z, m, time, is_instab) and final state."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
@@ -78,6 +120,21 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
     scheme = 0 if softening_scheme == "isotropic" else 1
     state = {}
 
+    Tl = np.full((nx, ny, nz), float(temperature))
+    if enable_thermal:
+        k2, Dsc = heat_operator((nx, ny, nz), pixel, thermal_diffusivity, thermal_coords)
+
+    def heat(dt):
+        if enable_thermal:
+            heat_step(Tl, k2, Dsc, dt, temperature, thermostat, tau_bath, temperature_cap)
+
+    def flip_heat(x, y, z, C):
+        if enable_thermal:
+            Tl[x, y, z] += abs(np.sum(state["sig"][x, y, z] * C)) / (rho * Cp)
+            if temperature_cap > 0:
+                Tl[x, y, z] = min(Tl[x, y, z], temperature_cap)
+
+    Tr = Tl if enable_thermal else temperature
     finite = strain_assumption == "finite_strain"
     if finite:
         from scipy.linalg import expm
@@ -176,6 +233,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
             for x, y, z, m in flips:
                 ok.apply_flip_soa(ep, None, soft, tl, cat, x, y, z, m, t, jp, jt, softening_cap, jn)
                 prev[x, y, z] = cat[x, y, z, m].copy()
+                flip_heat(x, y, z, prev[x, y, z])
                 cat[x, y, z] = ok.stz_catalog_glass(M, gamma0, np.random)
                 Q0[x, y, z] = gen((M,))
                 cascades.append((step, local, int(x), int(y), int(z), int(m)))
@@ -187,7 +245,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
 
     def result():
         out = dict(eps_plastic=ep, Q=Q, Q0=Q0, soft=soft, sig=state["sig"], eps_macro=eps_macro, time=t,
-                   cascades=cascades, substeps=state.get("substeps"))
+                   cascades=cascades, substeps=state.get("substeps"), Tlocal=Tl)
         if finite:
             out.update(F=Ff, F_plastic=Fp, F_macro=state["F_macro"])
         return events, out
@@ -219,12 +277,13 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                 if run_cascade(step):
                     return result()
                 continue
-            rates, idx, total = ok.compute_rates(Q, 1.0, temperature, nu0, cascade_mode=cascade_mode)
+            rates, idx, total = ok.compute_rates(Q, 1.0, Tr, nu0, cascade_mode=cascade_mode)
             if total > 0:
                 u = np.random.uniform()
                 tw = -np.log(u) / total
                 if tw < remaining:
                     t += tw
+                    heat(tw)
                     eps_macro += unit * (strain_rate * tw)
                     remaining -= tw
                     i = ok.select_event(rates, total, total * sel.random_sample())
@@ -236,6 +295,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                         C = cat[x, y, z, m].copy()
                         ok.apply_flip_soa(ep, None, soft, tl, cat, x, y, z, m, t, jp, jt, softening_cap, jn)
                         prev[x, y, z] = C
+                        flip_heat(x, y, z, C)
                         cat[x, y, z] = ok.stz_catalog_glass(M, gamma0, np.random)
                         Q0[x, y, z] = gen((M,))
                         update()
@@ -245,6 +305,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                     step += 1
                     continue
             eps_macro += unit * (strain_rate * remaining)
+            heat(remaining)
             t += remaining
             remaining = 0.0
         update()

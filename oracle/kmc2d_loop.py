@@ -28,6 +28,11 @@ then count Q > 0 only.  Cascade flips are recorded as (step, batch, x, y, m)
 with the reference's global step (1 at the start, +1 per KMC event and per
 elastic increment).
 
+enable_thermal (:60-75, :559-575, :749-753, :896, :1029-1033, :1099): a local temperature
+field sets the rates, diffuses over each waiting time and each remaining
+increment time (oracle/aqs_loop.py heat_step), and each flip heats its voxel
+by |sigma:C| / (rho Cp), capped (linear law).
+
 ``fail_once_at`` (test hook) makes the first re-solve after that event's flip
 raise ValueError, to drive the sub-stepped path.
 
@@ -48,7 +53,9 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                   jt=30.0, jn=0.0, nu0=1e13, q_act_temp=0.37, plane_mode="plane_strain", ls_tol=1e-6,
                   stress_targets=None, mixed_tol=1e-4, mixed_max_iter=50, n_global_steps=10 ** 9,
                   eps_plastic=None, model="linear", model_params=None, fail_once_at=None, cascade_mode=False,
-                  stability_threshold=0.0, max_cascade_steps_pct=0.3):
+                  stability_threshold=0.0, max_cascade_steps_pct=0.3, enable_thermal=False, Cp=420.0, rho=6125.0,
+                  thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
+                  tau_bath=0.0):
     """Returns (events [(x, y, m, time, is_instab)], final state dict)."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
@@ -161,6 +168,23 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                 continue
         raise ValueError("Mechanical solver failed to converge under small-strain sub-stepping.")
 
+    # thermal extension (:60-75, :559-575, :749-753, :1029-1033): as oracle/aqs_loop.py
+    from .aqs_loop import heat_operator, heat_step
+    Tl = np.full((nx, ny), float(temperature))
+    if enable_thermal:
+        k2, Dsc = heat_operator((nx, ny), pixel, thermal_diffusivity, thermal_coords)
+    Tr = Tl if enable_thermal else temperature
+
+    def heat(dt):
+        if enable_thermal:
+            heat_step(Tl, k2, Dsc, dt, temperature, thermostat, tau_bath, temperature_cap)
+
+    def flip_heat(x, y, C):
+        if enable_thermal:
+            Tl[x, y] += abs(np.sum(state["sig"][x, y] * C)) / (rho * Cp)
+            if temperature_cap > 0:
+                Tl[x, y] = min(Tl[x, y], temperature_cap)
+
     cascades = []
 
     def run_cascade(step):
@@ -182,6 +206,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                 for x, y, m in flips:
                     ok.apply_flip_soa_2d(ep, soft, tl, cat, x, y, m, t, jp, jt, softening_cap, jn)
                     prev[x, y] = cat[x, y, m].copy()
+                    flip_heat(x, y, prev[x, y])
                     cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
                     Q0[x, y] = gen((M,))
                 elastic_run()
@@ -207,7 +232,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
 
     def result():
         return events, dict(eps_plastic=ep, Q0=Q0, soft=soft, sig=state["sig"], eps_macro=eps_macro.copy(), time=t,
-                            substeps=state.get("substeps"), cascades=cascades)
+                            substeps=state.get("substeps"), cascades=cascades, Tlocal=Tl)
 
     while done < n_global_steps:
         remaining += dt_step
@@ -221,11 +246,12 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                         return result()
                     continue
                 n_casc = 0
-            rates, idx, total = ok.compute_rates_2d(Q, 1.0, temperature, nu0, cascade_mode=cascade_mode)
+            rates, idx, total = ok.compute_rates_2d(Q, 1.0, Tr, nu0, cascade_mode=cascade_mode)
             if total > 0:
                 tw = -np.log(np.random.rand()) / total
                 if tw < remaining:
                     t += tw
+                    heat(tw)
                     eps_macro += unit * (strain_rate * tw)
                     remaining -= tw
                     i = ok.select_event(rates, total, total * sel.random_sample())
@@ -237,6 +263,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                         C = cat[x, y, m].copy()
                         ok.apply_flip_soa_2d(ep, soft, tl, cat, x, y, m, t, jp, jt, softening_cap, jn)
                         prev[x, y] = C
+                        flip_heat(x, y, C)
                         cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
                         Q0[x, y] = gen((M,))
                         elastic_run()
@@ -248,6 +275,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     step += 1
                     continue
             eps_macro += unit * (strain_rate * remaining)
+            heat(remaining)
             t += remaining
             remaining = 0.0
         prev_e, prev_s = {}, {}

@@ -504,3 +504,57 @@ def test_kmc3d_finite_strain_matches_oracle(gpu, prec):
     assert np.abs(sim.F_plastic - np.einsum("ijxyz->xyzij", st_ref["F_plastic"])).max() <= 1e-14
     assert np.abs(sim.F_macro - st_ref["F_macro"]).max() <= tol
     assert np.abs(sim.sig_field - st_ref["sig"]).max() <= tol * 1e3 * np.abs(st_ref["sig"]).max()
+
+
+@pytest.mark.parametrize("d,thermostat,tau_bath,cascade,coords", [
+    (3, False, 0.0, False, "pixel"),
+    (3, True, 0.0, False, "physical"),
+    (2, True, 5e-13, False, "pixel"),
+    (2, False, 0.0, True, "pixel"),
+])
+def test_kmc_thermal_matches_oracle(gpu, d, thermostat, tau_bath, cascade, coords):
+    # enable_thermal: local temperatures in the rates, spectral heat conduction
+    # over every waiting time (device mgk_heat_step), flip heating, thermostat
+    # and cap: events and the final temperature field equal the oracle loop's
+    from mgkmc_amd.aqs import ThermalSimulation
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    from oracle.aqs_loop import run_events
+    from oracle.kmc2d_loop import run_events_2d
+    print(f"[kmc thermal] {d}D thermostat={thermostat} tau_bath={tau_bath} cascade={cascade} {coords}", flush=True)
+    n, M = (8, 8) if d == 3 else (16, 12)
+    shape = (n,) * d
+    rng = np.random.default_rng(6)
+    E = 70e9 * rng.uniform(0.9, 1.1, shape)
+    nu = np.full(shape, 0.3)
+    kw = dict(mean=0.8 if cascade else 1.0, std=0.3, min_cutoff=0.0)
+    th = dict(enable_thermal=True, Cp=420.0, rho=6125.0, thermal_diffusivity=3e-6 if coords == "physical" else 2e11,
+              thermal_coords=coords, temperature_cap=1500.0, thermostat=thermostat, tau_bath=tau_bath)
+    args = dict(softening_cap=0.916, jp=20.0, jt=5.0)
+    T, steps, step_size = 700.0, 3, 2e-3
+    if d == 3:
+        ev_ref, st_ref = run_events(n, n, n, M, 0.14, E, nu, 0.7, T, 1e7, step_size, (0, 0), 25, seed=3, sel_seed=4,
+                                    barrier_kwargs=kw, mixed_tol=1e6, ls_tol=1e-8, n_global_steps=steps,
+                                    cascade_mode=cascade, **args, **th)
+        sim = ThermalSimulation(n, n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                                temperature=T, strain_rate=1e7, cascade_mode=cascade, rng=np.random.RandomState(3),
+                                selection_stream=np.random.RandomState(4), **args, **th)
+    else:
+        ev_ref, st_ref = run_events_2d(n, n, M, 0.14, E, nu, 0.7, T, 1e7, step_size, (0, 0), 25, seed=3, sel_seed=4,
+                                       barrier_kwargs=kw, stress_targets={(1, 1): 0.0}, mixed_tol=1e6, ls_tol=1e-8,
+                                       n_global_steps=steps, cascade_mode=cascade, **args, **th)
+        sim = KmcSimulation2D(n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                              temperature=T, strain_rate=1e7, cascade_mode=cascade, rng=np.random.RandomState(3),
+                              selection_stream=np.random.RandomState(4), **args, **th)
+    sim.solver_args = {"tol": 1e-8}
+    kw_run = dict(mixed_tol=1e6, max_events=25)
+    if d == 2:
+        kw_run["stress_targets"] = {(1, 1): 0.0}
+    ev = sim.run_simulation(steps, step_size, component=(0, 0), **kw_run)
+    Tl = sim.Tlocal
+    print(f"[kmc thermal] events {len(ev)} (oracle {len(ev_ref)}), T range {Tl.min():.3f}..{Tl.max():.3f}", flush=True)
+    assert len(ev) > 0
+    assert [e[:d + 1] for e in ev] == [e[:d + 1] for e in ev_ref]
+    if cascade:
+        assert sim.cascade_history == st_ref["cascades"]
+    assert np.abs(Tl - st_ref["Tlocal"]).max() <= 1e-9 * st_ref["Tlocal"].max()
+    assert Tl.max() > T  # flips heated something

@@ -260,3 +260,33 @@ def test_oracle_2d_substepped_flip():
     mask = np.ones((n, n), bool)
     mask[x, y] = False
     assert np.array_equal(st_a["Q0"][mask], st_b["Q0"][mask])
+
+
+@pytest.mark.parametrize("coords", ["pixel", "physical"])
+def test_oracle_heat_step_closed_forms(coords):
+    # one Fourier mode decays as exp(-|k|^2 D dt) with k = 2 pi m / L
+    # (L = n pixels, or n * pixel nm with D in m^2/s -> nm^2/s); the mean is
+    # kept; thermostats and the cap act as written
+    from oracle.aqs_loop import heat_operator, heat_step
+    shape, pixel, D, dt = (16, 8, 12), 0.7, 3e-6, 2e-3
+    k2, Dsc = heat_operator(shape, pixel, D, coords)
+    L = [n * (pixel if coords == "physical" else 1.0) for n in shape]
+    x, y, z = np.meshgrid(*[np.arange(n) * (pixel if coords == "physical" else 1.0) for n in shape], indexing="ij")
+    T0, A = 300.0, 5.0
+    T = T0 + A * np.cos(2 * np.pi * 2 * x / L[0]) * np.cos(2 * np.pi * 3 * z / L[2])
+    kk = (2 * np.pi * 2 / L[0]) ** 2 + (2 * np.pi * 3 / L[2]) ** 2
+    Dn = D * (1e18 if coords == "physical" else 1.0)
+    dt = 0.3 / (kk * Dn)  # decay by e^-0.3
+    ref = T0 + (T - T0) * np.exp(-kk * Dn * dt)
+    Tl = T.copy()
+    heat_step(Tl, k2, Dsc, dt, T0, False, 0.0, 0.0)
+    assert np.allclose(Tl, ref, rtol=0, atol=1e-11 * T0)
+    Tl = T.copy() + 7.0
+    heat_step(Tl, k2, Dsc, dt, T0, True, 0.0, 0.0)  # mean reset to T0
+    assert abs(Tl.mean() - T0) < 1e-9
+    Tl = T.copy() + 7.0
+    heat_step(Tl, k2, Dsc, dt, T0, True, 2 * dt, 0.0)  # relaxation towards T0
+    assert np.allclose(Tl, T0 + (ref + 7.0 - T0) * np.exp(-0.5), rtol=0, atol=1e-9)
+    Tl = T.copy()
+    heat_step(Tl, k2, Dsc, dt, T0, False, 0.0, 302.0)
+    assert Tl.max() <= 302.0 and np.allclose(Tl, np.minimum(ref, 302.0), atol=1e-9)
