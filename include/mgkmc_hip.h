@@ -271,6 +271,18 @@ int mgk_fs_workspace_bytes(const int64_t* shape, int precision, size_t* bytes);
  * Fp optional; lam/mu per voxel (the reference reads C4[0,0,1,1], C4[0,1,0,1]);
  * F_bar, P_target row-major 9 (host); P_mask 9 ints (host).  BiCGSTAB follows
  * SciPy 1.15.3 (x0 = 0, maxiter 150, rtol = tol_CG). */
+/* One load step with the augmented-Lagrangian solver: replaces
+ * finite_strain_solver_step_3d, solver="al" (_al_step_3d, mgkmc/finite_strain_simulator.py:
+ * 2145-2242, inside the same macro loop, :3011-3165).  tol, max_iter = the reference's
+ * tol_NW, max_NW; L0 = {<lam + 2 mu>, <lam>, <mu>}, the voxel means of C4_iiii, C4_iijj,
+ * C4_ijij of the linear stiffness (its reference medium).  Other arguments as
+ * mgk_fs_solver_step; newton_iters counts AL iterations. */
+int mgk_fs_solver_step_al(const int64_t* shape, int precision, void* F, const void* Fp, const void* lam,
+                          const void* mu, const mgk_fs_material* mat, const double* F_bar, const double* P_target,
+                          const int* P_mask, int i_drv, int j_drv, double E_avg, double nu_avg, double tol,
+                          int max_iter, double tol_macro, int max_iter_macro, const double* L0, void* P_out,
+                          void* sig_out, void* workspace, size_t workspace_bytes, void* stream, mgk_fs_info* info);
+
 int mgk_fs_solver_step(const int64_t* shape, int precision, void* F, const void* Fp, const void* lam,
                        const void* mu, const mgk_fs_material* mat, const double* F_bar, const double* P_target,
                        const int* P_mask, int i_drv, int j_drv, double E_avg, double nu_avg, double tol_NW,

@@ -27,11 +27,11 @@ class ThermalSimulation(DeviceKmc):
                  strain_capping_enabled=False, strain_capping_limit=None, strain_capping_tangent_ratio=0.1,
                  strain_capping_type="piecewise", strain_capping_smooth_power=1.0,
                  ls_precision="fp64", ls_warm_start=False, rng=None, selection_stream=None, init_modes=None):
-        # finite strain: the Newton-Krylov step (solver="newton_cg"), KMC mode
-        self._finite_ok = strain_assumption == "finite_strain" and solver == "newton_cg" and not cascade_mode
+        # finite strain: the Newton-Krylov or augmented-Lagrangian step, KMC mode
+        self._finite_ok = strain_assumption == "finite_strain" and solver in ("newton_cg", "al") and not cascade_mode
         extra = ["use_3d_barriers"] if use_3d_barriers else []
-        if strain_assumption == "finite_strain" and solver != "newton_cg":
-            extra.append(f"finite-strain solver={solver!r} (the device step is solver='newton_cg')")
+        if strain_assumption == "finite_strain" and solver not in ("newton_cg", "al"):
+            extra.append(f"finite-strain solver={solver!r} (the device step is 'newton_cg' or 'al')")
         self._unsupported(cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model, extra)
         self.output_dir = output_dir
         self.strain_assumption = strain_assumption
@@ -49,7 +49,7 @@ class ThermalSimulation(DeviceKmc):
         if strain_assumption == "finite_strain":
             law = None
             finite = dict(model_type=hyperelastic_model, A_m=A_m, B_m=B_m, C_m=C_m, v1=v1, v2=v2, v3=v3, g1=g1,
-                          g2=g2, g3=g3, g4=g4, **cap)
+                          g2=g2, g3=g3, g4=g4, solver=solver, **cap)
         self._setup((nx, ny, nz), M, gamma0, E_field, nu_field, pixel, barrier_generator, barrier_kwargs,
                     softening_scheme, softening_cap, jp, jt, neighbor_softening_fraction, temperature, strain_rate,
                     stability_threshold, redraw_directions, redraw_barriers, nu0, q_act_temp, cascade_mode,

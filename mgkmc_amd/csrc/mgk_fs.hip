@@ -993,6 +993,45 @@ __global__ __launch_bounds__(MGK_NT) void k_norm2(size_t n, const T* __restrict_
   if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
 }
 
+// sum |x| (FP64) per block
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_abs_sum(size_t n, const T* __restrict__ x, double* __restrict__ partials) {
+  __shared__ double red[MGK_NT / 64];
+  double acc[1] = {0.0};
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT)
+    acc[0] += fabs((double)x[i]);
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
+}
+
+// Augmented-Lagrangian polarisation phi = lam_AL - L0:F with the voxel mean of
+// the linear stiffness, L0_iiii = A, L0_iijj = Bl (i != j), L0_ijij = L0_ijji = Cm
+// (i != j), contracted as L0_ijkl F_lk (_ddot42_3d)
+template <typename T>
+__global__ void k_fs_al_phi(size_t NV, const T* __restrict__ F, const T* __restrict__ lal, double A, double Bl,
+                            double Cm, T* __restrict__ phi) {
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < NV; v += (size_t)gridDim.x * blockDim.x) {
+    double f[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f[k] = (double)F[(size_t)k * NV + v];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double l0f;
+        if (i == j) {
+          l0f = 0.0;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) l0f += (k == i ? A : Bl) * f[4 * k];
+        } else {
+          l0f = Cm * f[3 * j + i] + Cm * f[3 * i + j];
+        }
+        const size_t o = (size_t)(3 * i + j) * NV + v;
+        phi[o] = (T)((double)lal[o] - l0f);
+      }
+  }
+}
+
 __global__ __launch_bounds__(MGK_NT) void k_sum_parts(const double* __restrict__ partials, int n, int nv,
                                                        double* __restrict__ out) {
   __shared__ double red[16 * (MGK_NT / 64)];
@@ -1464,7 +1503,8 @@ template <typename T>
 int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu, const mgk_fs_material* mt,
               const double* F_bar, const double* P_target, const int* P_mask, int i_drv, int j_drv, double E_avg,
               double nu_avg, double tol_NW, double tol_CG, int max_NW, double tol_macro, int max_iter_macro,
-              T* P_out, T* sig_out, char* ws, hipStream_t s, mgk_fs_info* info, FsCtx<float>* inner = nullptr) {
+              T* P_out, T* sig_out, char* ws, hipStream_t s, mgk_fs_info* info, FsCtx<float>* inner = nullptr,
+              const double* al_L0 = nullptr) {
   FsCtx<T> c;
   int rc = fs_ctx_init(c, shape, lam, mu, mt, Fp != nullptr, ws, s);
   if (rc) return rc;
@@ -1512,11 +1552,86 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
     Nine DbarF;
     for (int k = 0; k < 9; ++k) DbarF.v[k] = Fbc[k] - Fm[k];
     Nine zero{};
+    if (al_L0) {
+      // augmented Lagrangian (finite_strain_simulator.py:2145-2242): F + DbarF,
+      // lam = P; per iteration dF = -G(lam - L0:F), step beta in 1 .. 1/16 with
+      // the mean reset to F_bar (det F_e > 1e-4, finite P), lam = P, stop when
+      // |G(P)| / (sqrt(9 NV) (<|P|> + 1e-20)) < tol after the first iteration
+      rc = fs_const(c, F, (const T*)nullptr, DbarF, 1.0, 1, c.Ftr, c.P, res);
+      if (rc) return rc;
+      MGK_HIP(hipMemcpyAsync(F, c.Ftr, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
+      MGK_HIP(hipMemcpyAsync(c.b, c.P, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
+      for (int k = 0; k < 9; ++k) info->sig_mean[k] = res[3 + k];
+      const int nbv = fs_nblk<T>(pl.NV), nb9 = fs_nblk<T>(9 * pl.NV);
+      bool al_ok = false;
+      double resid = INFINITY;
+      for (int i = 0; i < max_NW; ++i) {
+        info->newton_iters += 1;
+        k_fs_al_phi<T><<<nbv, MGK_NT, 0, s>>>(NV, F, c.b, al_L0[0], al_L0[1], al_L0[2], c.dP);
+        fs_project(c, c.dP, c.x, T(-1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
+        double mF[9], mdF[9];
+        k_comp_means9<T><<<nbv, MGK_NT, 0, s>>>(NV, F, c.part);
+        rc = fs_read_scal(c, nbv, 9, mF);
+        if (rc) return rc;
+        k_comp_means9<T><<<nbv, MGK_NT, 0, s>>>(NV, c.x, c.part);
+        rc = fs_read_scal(c, nbv, 9, mdF);
+        if (rc) return rc;
+        bool step_ok = false;
+        for (double beta = 1.0; beta >= 0.0625; beta *= 0.5) {
+          Nine D2;  // mean reset, folded into the step: F + beta (dF + D2)
+          for (int k = 0; k < 9; ++k) D2.v[k] = (Fbc[k] - (mF[k] / (double)NV + beta * (mdF[k] / (double)NV))) / beta;
+          rc = fs_const(c, F, c.x, D2, beta, 1, c.Ftr, c.Ptr, res);
+          if (rc) return rc;
+          if (res[1] == 0.0 && res[0] > 1e-4) {
+            step_ok = true;
+            break;
+          }
+        }
+        if (!step_ok) break;
+        std::swap(c.Ptr, c.P);
+        MGK_HIP(hipMemcpyAsync(F, c.Ftr, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
+        MGK_HIP(hipMemcpyAsync(c.b, c.P, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
+        for (int k = 0; k < 9; ++k) info->sig_mean[k] = res[3 + k];
+        fs_project(c, c.P, c.r, T(1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
+        double g2, pa;
+        k_norm2<T><<<nb9, MGK_NT, 0, s>>>((size_t)9 * NV, c.r, c.part);
+        rc = fs_read_scal(c, nb9, 1, &g2);
+        if (rc) return rc;
+        k_abs_sum<T><<<nb9, MGK_NT, 0, s>>>((size_t)9 * NV, c.P, c.part);
+        rc = fs_read_scal(c, nb9, 1, &pa);
+        if (rc) return rc;
+        resid = sqrt(g2) / (sqrt(9.0 * (double)NV) * (pa / (9.0 * (double)NV) + 1e-20));
+        if (resid < tol_NW && i > 0) {
+          al_ok = true;
+          break;
+        }
+      }
+      if (!al_ok) {
+        if (!std::isfinite(resid)) {  // no accepted step: the residual of the current P
+          fs_project(c, c.P, c.r, T(1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
+          double g2, pa;
+          k_norm2<T><<<nb9, MGK_NT, 0, s>>>((size_t)9 * NV, c.r, c.part);
+          rc = fs_read_scal(c, nb9, 1, &g2);
+          if (rc) return rc;
+          k_abs_sum<T><<<nb9, MGK_NT, 0, s>>>((size_t)9 * NV, c.P, c.part);
+          rc = fs_read_scal(c, nb9, 1, &pa);
+          if (rc) return rc;
+          resid = sqrt(g2) / (sqrt(9.0 * (double)NV) * (pa / (9.0 * (double)NV) + 1e-20));
+        }
+        info->fail_kind = 2;
+        char msg[160];
+        snprintf(msg, sizeof msg, "AL solver did not converge or exploded. Final relative residual: %.2e", resid);
+        mgk_set_error(msg, __FILE__, __LINE__);
+        return MGK_ERR_SOLVER;
+      }
+    }
     // P at F_start
-    rc = fs_const(c, F, (const T*)nullptr, zero, 0.0, 0, (T*)nullptr, c.P, res);
-    if (rc) return rc;
-    bool newton_ok = false;
-    for (int i_NW = 0; i_NW < max_NW; ++i_NW) {
+    if (!al_L0) {
+      rc = fs_const(c, F, (const T*)nullptr, zero, 0.0, 0, (T*)nullptr, c.P, res);
+      if (rc) return rc;
+    }
+    bool newton_ok = al_L0 != nullptr;
+    for (int i_NW = 0; i_NW < (al_L0 ? 0 : max_NW); ++i_NW) {
       info->newton_iters += 1;
       // rhs
       if (i_NW == 0) {
@@ -1692,6 +1807,37 @@ extern "C" int mgk_fs_solver_step(const int64_t* shape, int precision, void* F, 
     rc = fs_step_t<float>(shape, (float*)F, (const float*)Fp, (const float*)lam, (const float*)mu, mat, F_bar,
                           P_target, P_mask, i_drv, j_drv, E_avg, nu_avg, tol_NW, tol_CG, max_NW, tol_macro,
                           max_iter_macro, (float*)P_out, (float*)sig_out, (char*)workspace, s, info);
+  info->status = rc;
+  return rc;
+}
+
+extern "C" int mgk_fs_solver_step_al(const int64_t* shape, int precision, void* F, const void* Fp, const void* lam,
+                                     const void* mu, const mgk_fs_material* mat, const double* F_bar,
+                                     const double* P_target, const int* P_mask, int i_drv, int j_drv, double E_avg,
+                                     double nu_avg, double tol, int max_iter, double tol_macro, int max_iter_macro,
+                                     const double* L0, void* P_out, void* sig_out, void* workspace,
+                                     size_t workspace_bytes, void* stream, mgk_fs_info* info) {
+  if (!shape || !F || !lam || !mu || !mat || !F_bar || !P_target || !P_mask || !workspace || !info || !L0) {
+    mgk_set_error("mgk_fs_solver_step_al: null argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  size_t need = 0;
+  int rc = mgk_fs_workspace_bytes(shape, precision, &need);
+  if (rc) return rc;
+  if (workspace_bytes < need) {
+    mgk_set_error("workspace too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  memset(info, 0, sizeof(*info));
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == 0)
+    rc = fs_step_t<double>(shape, (double*)F, (const double*)Fp, (const double*)lam, (const double*)mu, mat, F_bar,
+                           P_target, P_mask, i_drv, j_drv, E_avg, nu_avg, tol, 0.0, max_iter, tol_macro,
+                           max_iter_macro, (double*)P_out, (double*)sig_out, (char*)workspace, s, info, nullptr, L0);
+  else
+    rc = fs_step_t<float>(shape, (float*)F, (const float*)Fp, (const float*)lam, (const float*)mu, mat, F_bar,
+                          P_target, P_mask, i_drv, j_drv, E_avg, nu_avg, tol, 0.0, max_iter, tol_macro,
+                          max_iter_macro, (float*)P_out, (float*)sig_out, (char*)workspace, s, info, nullptr, L0);
   info->status = rc;
   return rc;
 }

@@ -172,6 +172,8 @@ class DeviceKmc:
             eye = np.broadcast_to(np.eye(3).reshape(3, 3, 1, 1, 1), (3, 3) + self.grid_shape)
             self.fs = FsSolver(lam, mu, precision=FS_MIXED if self.precision == MIXED else 0, Fp=eye)
             from .finite_strain_simulator import material
+            finite = dict(finite)
+            self._fs_solver = finite.pop("solver", "newton_cg")
             self._fs_mat = material(**finite, mu_mean=float(np.mean(mu)))
             self.F_macro = np.eye(3)
             self.sig_kind = SIG_GEN64
@@ -375,7 +377,8 @@ class DeviceKmc:
         drv = getattr(self, "driving_component", (0, 0))
         tg = getattr(self, "stress_targets", {})
         Fb, _, Pt, Pm = build_finite_strain_bc_3d(drv, self.eps_macro[drv], tg, F_bar_initial=self.F_macro)
-        info = self.fs.step(Fb, self._fs_mat, drv, Pt, Pm, E_avg=self.E_field.mean(), nu_avg=self.nu_field.mean())
+        info = self.fs.step(Fb, self._fs_mat, drv, Pt, Pm, E_avg=self.E_field.mean(), nu_avg=self.nu_field.mean(),
+                            solver=self._fs_solver)
         self.F_macro = np.array(info.F_bar[:]).reshape(3, 3)
         for i in range(3):
             for j in range(3):

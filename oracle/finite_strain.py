@@ -347,11 +347,52 @@ def build_bc(driving_component, eps_target_step, mixed_targets, F_bar_initial=No
     return F_bar, F_mask, P_target, P_mask
 
 
+def al_step(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, tol=1e-5, max_iter=200, model_type="svk", **mat):
+    """_al_step_3d (finite_strain_simulator.py:2145-2242): reference medium L0 =
+    voxel mean of C4; F + (F_bar - <F>); lam = P; per iteration dF = -G(lam -
+    L0:F), the first beta in 1, 1/2, .., 1/16 whose F + beta dF (mean reset to
+    F_bar) has det F_e > 1e-4 everywhere and a finite P; lam = P; stop when
+    |G(P)| / (sqrt(size) (<|P|> + 1e-20)) < tol after the first iteration.
+    Returns (F, P, iterations); ValueError otherwise."""
+    nx, ny, nz = F.shape[2:]
+    L0 = C4.mean(axis=(-3, -2, -1))
+    L0f = np.einsum("ijkl,xyz->ijklxyz", L0, np.ones((nx, ny, nz)))
+    Fpi = invert_Fp(Fp) if Fp is not None else None
+    F = F + np.einsum("ij,xyz->ijxyz", F_bar - F.mean(axis=(2, 3, 4)), np.ones((nx, ny, nz)))
+    P, _ = constitutive(F, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, need_tangent=False, **mat)
+    lam = P.copy()
+    for i in range(max_iter):
+        dF = -project(lam - _ddot42(L0f, F), Ghat4)
+        ok = False
+        for beta in (1.0, 0.5, 0.25, 0.125, 0.0625):
+            Ft = F + beta * dF
+            Ft = Ft + np.einsum("ij,xyz->ijxyz", F_bar - Ft.mean(axis=(2, 3, 4)), np.ones((nx, ny, nz)))
+            Fe = np.einsum("ijxyz,jkxyz->ikxyz", Ft, Fpi) if Fpi is not None else Ft
+            Je = (Fe[0, 0] * (Fe[1, 1] * Fe[2, 2] - Fe[1, 2] * Fe[2, 1])
+                  - Fe[0, 1] * (Fe[1, 0] * Fe[2, 2] - Fe[1, 2] * Fe[2, 0])
+                  + Fe[0, 2] * (Fe[1, 0] * Fe[2, 1] - Fe[1, 1] * Fe[2, 0]))
+            Pn, _ = constitutive(Ft, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, need_tangent=False, **mat)
+            if not np.any(np.isnan(Pn)) and np.all(Je > 1e-4):
+                F, P, ok = Ft, Pn, True
+                break
+        if not ok:
+            break
+        lam = P.copy()
+        GP = project(P, Ghat4)
+        res = np.linalg.norm(GP) / (np.sqrt(GP.size) * (np.mean(np.abs(P)) + 1e-20))
+        if res < tol and i > 0:
+            return F, P, i + 1
+    GP = project(P, Ghat4)
+    res = np.linalg.norm(GP) / (np.sqrt(GP.size) * (np.mean(np.abs(P)) + 1e-20))
+    raise ValueError(f"AL solver did not converge or exploded. Final relative residual: {res:.2e}")
+
+
 def solver_step(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, driving_component=(0, 0), P_target=None, P_mask=None,
                 E_avg=100e9, nu_avg=0.3, tol_NW=1e-5, tol_CG=1e-6, max_NW=50, tol_macro=1e6, max_iter_macro=20,
-                model_type="svk", stats=None, **mat):
-    """finite_strain_solver_step_3d, newton_cg branch (:2967-3168).  ``stats``
-    (oracle extension) collects Newton and BiCGSTAB iteration counts."""
+                model_type="svk", stats=None, solver="newton_cg", **mat):
+    """finite_strain_solver_step_3d, newton_cg branch (:2967-3168), or its "al"
+    branch (al_step in the same macro loop).  ``stats`` (oracle extension)
+    collects Newton and BiCGSTAB iteration counts (AL: "al" iterations)."""
     nx, ny, nz = F.shape[2:]
     P_target = np.zeros((3, 3)) if P_target is None else P_target
     P_mask = np.zeros((3, 3), bool) if P_mask is None else P_mask
@@ -366,55 +407,15 @@ def solver_step(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, driving_component=(0
     max_err = 0.0
     res = None
     for it_mac in range(max_iter_macro):
-        DbarF = Fbc - F_start.mean(axis=(2, 3, 4))
-        DbarF_grid = np.einsum("ij,xyz->ijxyz", DbarF, np.ones((nx, ny, nz)))
-        cur = {}
-        cur["P"], cur["K4"] = constitutive(F_start, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, **mat)
-
-        def G_op(A2):
-            return project(A2, Ghat4)
-
-        def K_dF(x):
-            return _trans2(_ddot42(cur["K4"], _trans2(x.reshape(3, 3, nx, ny, nz))))
-
-        ncg = {"n": 0}
-
-        def cb(xk):
-            ncg["n"] += 1
-
-        A_op = sp.LinearOperator(shape=(F_start.size,) * 2, matvec=lambda x: G_op(K_dF(x)).reshape(-1),
-                                 dtype="float64")
-        F_curr = F_start.copy()
-        Fpi = invert_Fp(Fp) if Fp is not None else None
-        for i_NW in range(max_NW):
-            rhs = (-G_op(K_dF(DbarF_grid.reshape(-1))).reshape(-1) if i_NW == 0 else -G_op(cur["P"]).reshape(-1))
-            dFm, _ = sp.bicgstab(A_op, rhs, rtol=tol_CG, maxiter=150, callback=cb)
-            dF = dFm.reshape(3, 3, nx, ny, nz)
-            dX = (DbarF_grid + dF) if i_NW == 0 else dF
-            alpha = 1.0
-            for _ in range(16):
-                Ft = F_curr + alpha * dX
-                Fe = np.einsum("ijxyz,jkxyz->ikxyz", Ft, Fpi) if Fpi is not None else Ft
-                Je = (Fe[0, 0] * (Fe[1, 1] * Fe[2, 2] - Fe[1, 2] * Fe[2, 1])
-                      - Fe[0, 1] * (Fe[1, 0] * Fe[2, 2] - Fe[1, 2] * Fe[2, 0])
-                      + Fe[0, 2] * (Fe[1, 0] * Fe[2, 1] - Fe[1, 1] * Fe[2, 0]))
-                if np.any(Je <= 1e-4) or np.any(np.isnan(Je)):
-                    alpha *= 0.5
-                    continue
-                Pt, Kt = constitutive(Ft, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, **mat)
-                if np.any(np.isnan(Pt)) or np.any(np.isnan(Kt)):
-                    alpha *= 0.5
-                    continue
-                F_curr, cur["P"], cur["K4"] = Ft, Pt, Kt
-                break
-            else:
-                raise ValueError("Newton-CG line search failed to find a valid step.")
+        if solver == "al":
+            F_curr, P_al, nit = al_step(F_start.copy(), Fbc, Ghat4, C4, I2, I4, I4rt, Fp=Fp, tol=tol_NW,
+                                        max_iter=max_NW, model_type=model_type, **mat)
             if stats is not None:
-                stats.setdefault("newton", []).append((i_NW, ncg["n"], alpha))
-            if np.linalg.norm(dFm) / (np.linalg.norm(F_curr) + 1e-20) < tol_NW and i_NW > 0:
-                break
+                stats.setdefault("al", []).append(nit)
+            cur = {"P": P_al, "K4": None}
         else:
-            raise ValueError(f"Newton-CG inner solver did not converge within {max_NW} iterations.")
+            F_curr, cur = _newton(F_start, Fbc, Ghat4, C4, I2, I4, I4rt, Fp, tol_NW, tol_CG, max_NW, model_type,
+                                  stats, mat)
         Sig = cauchy_from_P(cur["P"], F_curr)
         Sm = Sig.mean(axis=(2, 3, 4))
         res = (F_curr, cur["P"], Sig, cur["K4"])
@@ -444,3 +445,59 @@ def solver_step(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, driving_component=(0
         if np.any(P_mask):
             raise ValueError(f"Outer BC loop did not converge after {max_iter_macro} iterations.")
     return res[0], res[1], res[2], res[3], Fbc
+
+
+def _newton(F_start, Fbc, Ghat4, C4, I2, I4, I4rt, Fp, tol_NW, tol_CG, max_NW, model_type, stats, mat):
+    """The newton_cg step of one macro iteration (:3040-3119)."""
+    nx, ny, nz = F_start.shape[2:]
+    DbarF = Fbc - F_start.mean(axis=(2, 3, 4))
+    DbarF_grid = np.einsum("ij,xyz->ijxyz", DbarF, np.ones((nx, ny, nz)))
+    cur = {}
+    cur["P"], cur["K4"] = constitutive(F_start, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, **mat)
+
+    def G_op(A2):
+        return project(A2, Ghat4)
+
+    def K_dF(x):
+        return _trans2(_ddot42(cur["K4"], _trans2(x.reshape(3, 3, nx, ny, nz))))
+
+    ncg = {"n": 0}
+
+    def cb(xk):
+        ncg["n"] += 1
+
+    A_op = sp.LinearOperator(shape=(F_start.size,) * 2, matvec=lambda x: G_op(K_dF(x)).reshape(-1),
+                             dtype="float64")
+    F_curr = F_start.copy()
+    Fpi = invert_Fp(Fp) if Fp is not None else None
+    for i_NW in range(max_NW):
+        rhs = (-G_op(K_dF(DbarF_grid.reshape(-1))).reshape(-1) if i_NW == 0 else -G_op(cur["P"]).reshape(-1))
+        dFm, _ = sp.bicgstab(A_op, rhs, rtol=tol_CG, maxiter=150, callback=cb)
+        dF = dFm.reshape(3, 3, nx, ny, nz)
+        dX = (DbarF_grid + dF) if i_NW == 0 else dF
+        alpha = 1.0
+        for _ in range(16):
+            Ft = F_curr + alpha * dX
+            Fe = np.einsum("ijxyz,jkxyz->ikxyz", Ft, Fpi) if Fpi is not None else Ft
+            Je = (Fe[0, 0] * (Fe[1, 1] * Fe[2, 2] - Fe[1, 2] * Fe[2, 1])
+                  - Fe[0, 1] * (Fe[1, 0] * Fe[2, 2] - Fe[1, 2] * Fe[2, 0])
+                  + Fe[0, 2] * (Fe[1, 0] * Fe[2, 1] - Fe[1, 1] * Fe[2, 0]))
+            if np.any(Je <= 1e-4) or np.any(np.isnan(Je)):
+                alpha *= 0.5
+                continue
+            Pt, Kt = constitutive(Ft, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, **mat)
+            if np.any(np.isnan(Pt)) or np.any(np.isnan(Kt)):
+                alpha *= 0.5
+                continue
+            F_curr, cur["P"], cur["K4"] = Ft, Pt, Kt
+            break
+        else:
+            raise ValueError("Newton-CG line search failed to find a valid step.")
+        if stats is not None:
+            stats.setdefault("newton", []).append((i_NW, ncg["n"], alpha))
+        if np.linalg.norm(dFm) / (np.linalg.norm(F_curr) + 1e-20) < tol_NW and i_NW > 0:
+            break
+    else:
+        raise ValueError(f"Newton-CG inner solver did not converge within {max_NW} iterations.")
+    return F_curr, cur
+

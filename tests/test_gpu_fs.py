@@ -164,3 +164,26 @@ def test_newton_step_refined_form(gpu, sh, model, cap):
     print(f"[fs refined] {sh} {model} cap={cap}: newton {out[MIXED][2]} (fp64 {out[0][2]}), krylov "
           f"{out[MIXED][3]} (fp64 {out[0][3]}), relF={eF:.2e} relP={eP:.2e}", flush=True)
     assert eF < 1e-7 and eP < 1e-5
+
+
+def test_al_step_matches_oracle(gpu):
+    # solver="al": the homogeneous closed form in two iterations, and on a
+    # heterogeneous grid the same ValueError as the oracle (the reference's
+    # AL update explodes there)
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_3d
+    sh = (6, 4, 5)
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = fs.identity_tensors(*sh)
+    G = fs.build_ghat4(*sh, *sh)
+    Fbar = np.eye(3)
+    Fbar[0, 1] = 0.02
+    Fbar[0, 0] = 1.005
+    C4h = fs.build_C4(np.full(sh, 1.0), nu, I4s, II)
+    ref = fs.solver_step(I2.copy(), Fbar, G, C4h, I2, I4, I4rt, solver="al", tol_NW=1e-12)
+    got = finite_strain_solver_step_3d(I2.copy(), Fbar, G, C4h, I2, I4, I4rt, solver="al", tol_NW=1e-12)
+    assert _rel(got[0], ref[0]) < 1e-14 and _rel(got[1], ref[1]) < 1e-12
+    rng = np.random.default_rng(9)
+    C4 = fs.build_C4(70e9 * rng.uniform(0.85, 1.15, sh), nu, I4s, II)
+    for fn in (fs.solver_step, finite_strain_solver_step_3d):
+        with pytest.raises(ValueError, match="AL solver did not converge or exploded"):
+            fn(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="al")

@@ -189,3 +189,33 @@ def test_heterogeneous_solution_is_equilibrated():
     assert np.allclose(F.mean(axis=(2, 3, 4)), Fbar, atol=1e-12)
     GP = fs.project(P, G)
     assert np.abs(GP).max() < 1e-8 * np.abs(P).max()
+
+
+def test_al_step_closed_form_and_heterogeneous_failure():
+    # The AL update adds dF = -G(lam - L0:F), a stress-valued field, to F
+    # (finite_strain_simulator.py:2182-2186), G being the dimensionless
+    # compatibility projection.  In a homogeneous medium of O(1) moduli the
+    # polarisation has no fluctuation, dF = 0 and the step returns the closed
+    # form after the second iteration.  With any heterogeneity the fixed point
+    # diverges and the step raises ValueError; with moduli in Pa even the FFT
+    # round-off of the uniform L0:F (~1e11) perturbs F by ~1e-5 and it never
+    # meets the tolerance.
+    sh = (6, 4, 5)
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = fs.identity_tensors(*sh)
+    G = fs.build_ghat4(*sh, *sh)
+    Fbar = np.eye(3)
+    Fbar[0, 1] = 0.02
+    Fbar[0, 0] = 1.005
+    C4h = fs.build_C4(np.full(sh, 1.0), nu, I4s, II)
+    st = {}
+    Fh, Ph, _, _, _ = fs.solver_step(I2.copy(), Fbar, G, C4h, I2, I4, I4rt, solver="al", tol_NW=1e-12, stats=st)
+    assert st["al"] == [2]
+    assert np.allclose(Fh, Fbar[:, :, None, None, None], atol=1e-14)
+    lam, mu = 0.3 / (1.3 * 0.4), 1 / 2.6
+    Eg = 0.5 * (Fbar.T @ Fbar - np.eye(3))
+    assert np.allclose(Ph[:, :, 0, 0, 0], Fbar @ (lam * np.trace(Eg) * np.eye(3) + 2 * mu * Eg), rtol=1e-12)
+    rng = np.random.default_rng(9)
+    C4 = fs.build_C4(70e9 * rng.uniform(0.85, 1.15, sh), nu, I4s, II)
+    with pytest.raises(ValueError, match="AL solver did not converge or exploded"):
+        fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="al")
