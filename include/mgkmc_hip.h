@@ -271,6 +271,12 @@ int mgk_fs_workspace_bytes(const int64_t* shape, int precision, size_t* bytes);
  * Fp optional; lam/mu per voxel (the reference reads C4[0,0,1,1], C4[0,1,0,1]);
  * F_bar, P_target row-major 9 (host); P_mask 9 ints (host).  BiCGSTAB follows
  * SciPy 1.15.3 (x0 = 0, maxiter 150, rtol = tol_CG). */
+int mgk_fs_solver_step(const int64_t* shape, int precision, void* F, const void* Fp, const void* lam,
+                       const void* mu, const mgk_fs_material* mat, const double* F_bar, const double* P_target,
+                       const int* P_mask, int i_drv, int j_drv, double E_avg, double nu_avg, double tol_NW,
+                       double tol_CG, int max_NW, double tol_macro, int max_iter_macro, void* P_out,
+                       void* sig_out, void* workspace, size_t workspace_bytes, void* stream, mgk_fs_info* info);
+
 /* One load step with the augmented-Lagrangian solver: replaces
  * finite_strain_solver_step_3d, solver="al" (_al_step_3d, mgkmc/finite_strain_simulator.py:
  * 2145-2242, inside the same macro loop, :3011-3165).  tol, max_iter = the reference's
@@ -283,11 +289,21 @@ int mgk_fs_solver_step_al(const int64_t* shape, int precision, void* F, const vo
                           int max_iter, double tol_macro, int max_iter_macro, const double* L0, void* P_out,
                           void* sig_out, void* workspace, size_t workspace_bytes, void* stream, mgk_fs_info* info);
 
-int mgk_fs_solver_step(const int64_t* shape, int precision, void* F, const void* Fp, const void* lam,
-                       const void* mu, const mgk_fs_material* mat, const double* F_bar, const double* P_target,
-                       const int* P_mask, int i_drv, int j_drv, double E_avg, double nu_avg, double tol_NW,
-                       double tol_CG, int max_NW, double tol_macro, int max_iter_macro, void* P_out,
-                       void* sig_out, void* workspace, size_t workspace_bytes, void* stream, mgk_fs_info* info);
+/* One load step with the displacement-based FFT solver: replaces
+ * finite_strain_solver_step_3d, solver="dbfft" (_dbfft_step_3d and
+ * solve_dbfft_linear_system_3d, mgkmc/finite_strain_simulator.py:1212-1379,1508-1620, in the
+ * macro loop :3011-3165).  tol, max_iter = the reference's tol_NW, max_NW; tol_CG as
+ * mgk_fs_solver_step (BiCGSTAB maxiter 150, preconditioner (xi.<K4>.xi)^-1); pixel sets the
+ * wave vectors 2 pi fftfreq(n, pixel).  Grid sizes and pixel values for which the reference's
+ * Nyquist test misses its entry are rejected (MGK_ERR_ARG), and so is precision 1 (the
+ * reference's stop test |b_hat| / |P| scales with sqrt(NV) times the rounding of P).  newton_iters counts DBFFT
+ * Newton iterations; failures return MGK_ERR_SOLVER with the reference's messages. */
+int mgk_fs_solver_step_dbfft(const int64_t* shape, int precision, void* F, const void* Fp, const void* lam,
+                             const void* mu, const mgk_fs_material* mat, const double* F_bar,
+                             const double* P_target, const int* P_mask, int i_drv, int j_drv, double E_avg,
+                             double nu_avg, double tol, double tol_CG, int max_iter, double tol_macro,
+                             int max_iter_macro, double pixel, void* P_out, void* sig_out, void* workspace,
+                             size_t workspace_bytes, void* stream, mgk_fs_info* info);
 
 /* FP32 form of the same step (replaces the same reference function; the precision
  * split has no reference counterpart): the Newton loop, P, the residual -G:P, the line

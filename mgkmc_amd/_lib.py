@@ -168,6 +168,9 @@ SIGNATURES = {
     "mgk_fs_solver_step_al": (_c_i, [_c_i64p, _c_i, _c_p, _c_p, _c_p, _c_p, ctypes.POINTER(MgkFsMaterial), _c_dp,
                                      _c_dp, ctypes.POINTER(_c_i), _c_i, _c_i, _c_d, _c_d, _c_d, _c_i, _c_d, _c_i, _c_dp,
                                      _c_p, _c_p, _c_p, _c_sz, _c_p, ctypes.POINTER(MgkFsInfo)]),
+    "mgk_fs_solver_step_dbfft": (_c_i, [_c_i64p, _c_i, _c_p, _c_p, _c_p, _c_p, ctypes.POINTER(MgkFsMaterial), _c_dp,
+                                        _c_dp, ctypes.POINTER(_c_i), _c_i, _c_i, _c_d, _c_d, _c_d, _c_d, _c_i, _c_d, _c_i,
+                                        _c_d, _c_p, _c_p, _c_p, _c_sz, _c_p, ctypes.POINTER(MgkFsInfo)]),
     "mgk_fs_solver_step_refined": (_c_i, [_c_i64p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, ctypes.POINTER(MgkFsMaterial),
                                           _c_dp, _c_dp, ctypes.POINTER(_c_i), _c_i, _c_i, _c_d, _c_d, _c_d, _c_d, _c_i,
                                           _c_d, _c_i, _c_p, _c_p, _c_p, _c_sz, _c_p, _c_sz, _c_p,

@@ -27,11 +27,12 @@ class ThermalSimulation(DeviceKmc):
                  strain_capping_enabled=False, strain_capping_limit=None, strain_capping_tangent_ratio=0.1,
                  strain_capping_type="piecewise", strain_capping_smooth_power=1.0,
                  ls_precision="fp64", ls_warm_start=False, rng=None, selection_stream=None, init_modes=None):
-        # finite strain: the Newton-Krylov or augmented-Lagrangian step, KMC mode
-        self._finite_ok = strain_assumption == "finite_strain" and solver in ("newton_cg", "al") and not cascade_mode
+        # finite strain: the Newton-Krylov, augmented-Lagrangian or DBFFT step, KMC mode
+        fs_solvers = ("newton_cg", "al", "dbfft")
+        self._finite_ok = strain_assumption == "finite_strain" and solver in fs_solvers and not cascade_mode
         extra = ["use_3d_barriers"] if use_3d_barriers else []
-        if strain_assumption == "finite_strain" and solver not in ("newton_cg", "al"):
-            extra.append(f"finite-strain solver={solver!r} (the device step is 'newton_cg' or 'al')")
+        if strain_assumption == "finite_strain" and solver not in fs_solvers:
+            extra.append(f"finite-strain solver={solver!r} (the device step is 'newton_cg', 'al' or 'dbfft')")
         self._unsupported(cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model, extra)
         self.output_dir = output_dir
         self.strain_assumption = strain_assumption

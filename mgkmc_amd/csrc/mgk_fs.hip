@@ -286,7 +286,7 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_const(size_t NV, const T* __restr
                                                      Nine dXu, double alpha, int trial, T* __restrict__ Fout,
                                                      const T* __restrict__ Fpi, const T* __restrict__ lam,
                                                      const T* __restrict__ mu, FsMat m, T* __restrict__ P,
-                                                     double* __restrict__ partials) {
+                                                     double* __restrict__ partials, int chk_tan) {
   __shared__ double red[FS_NPART * (MGK_NT / 64)];
   double acc[FS_NPART];
 #pragma unroll
@@ -317,7 +317,8 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_const(size_t NV, const T* __restr
       bad = bad || isnan((double)Pv[k]);
     }
     // the reference also rejects a NaN tangent; its entries come from these coefficients
-    bad = bad || isnan((double)s.al) || isnan((double)s.B) || isnan((double)s.C) || isnan((double)s.cw);
+    // (callers whose reference evaluates P only pass chk_tan = 0)
+    if (chk_tan) bad = bad || isnan((double)s.al) || isnan((double)s.B) || isnan((double)s.C) || isnan((double)s.cw);
     acc[1] += bad ? 1.0 : 0.0;
     double f2 = 0.0;
 #pragma unroll
@@ -433,6 +434,7 @@ struct FsGeom {
   FftPlan plq;
   const cplx<T>* twq;
   int npencil;
+  int nc;  // real components of the field the x sweeps transform (9, or 3 for displacements)
 };
 
 template <typename T, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
@@ -448,7 +450,7 @@ __global__ __launch_bounds__(NTH) void k_fs_xr2c(FsGeom<T> g, const T* __restric
   const int tid_lin = xcd_tile((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x,
                                gridDim.x * gridDim.y * gridDim.z);
   const int q0 = (tid_lin % gridDim.x) * B, p = (tid_lin / gridDim.x) % gridDim.y, pr = tid_lin / (gridDim.x * gridDim.y);
-  const int ca = 2 * pr, cb = (2 * pr + 1 < 9) ? 2 * pr + 1 : -1;
+  const int ca = 2 * pr, cb = (2 * pr + 1 < g.nc) ? 2 * pr + 1 : -1;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
   for (int t = threadIdx.x; t < nx; t += NTH) tw[t] = g.twx[t];
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(NTH) void k_fs_xc2r(FsGeom<T> g, const cplx<T>* __r
   const int tid_lin = xcd_tile((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x,
                                gridDim.x * gridDim.y * gridDim.z);
   const int q0 = (tid_lin % gridDim.x) * B, p = (tid_lin / gridDim.x) % gridDim.y, pr = tid_lin / (gridDim.x * gridDim.y);
-  const int ca = 2 * pr, cb = (2 * pr + 1 < 9) ? 2 * pr + 1 : -1;
+  const int ca = 2 * pr, cb = (2 * pr + 1 < g.nc) ? 2 * pr + 1 : -1;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
   for (int t = threadIdx.x; t < nx; t += NTH) tw[t] = g.twx[t];
@@ -678,6 +680,224 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_proj(FsGeom<T> g, cplx<T>* __rest
 }
 
 // ---------------------------------------------------------------------------
+// DBFFT spectral operators (finite_strain_simulator.py:1212-1379) on z pencils.
+// The reference keeps displacements and its Krylov vectors as full complex
+// spectra; all of them are spectra of real fields (Hermitian), on which
+// vdot(a, b) = NV * (real-space dot) and Re(ifftn) is exact, so here they are
+// real fields and each operator is one r2c -> pencil -> c2r pass.
+//   grad:   out_(i,j) = i xi_j u_i            3 -> 9 comps
+//   div:    out_i = i sum_j xi_j a_(i,j)      9 -> 3 comps (0 at k = 0)
+//   compat: out_(i,j) = xi_j sum_k xi_k a_(i,k) / |xi|^2 (|xi|^2 < 1e-14 -> 1)
+//   prec:   out = (xi . Kbar . xi)^-1 a, adjugate / det (|det| < 1e-14 -> 1), 0 at k = 0
+// xi_a = 2 pi (k_a val_a) as numpy's 2 pi fftfreq(n, L / n), its Nyquist
+// component zeroed (the host checks that the reference's equality test
+// matches it, which keeps every operator real).  With tr the 9-comp field is
+// stored transposed (comp 3j+i holds (i, j)).
+// ---------------------------------------------------------------------------
+enum { DB_GRAD = 0, DB_DIV = 1, DB_COMPAT = 2, DB_PREC = 3 };
+
+struct DbPar {
+  double val[3];  // fftfreq spacing 1 / (n d) per axis
+  const double* Kbar;
+};
+
+__device__ __forceinline__ void db_xi(const DbPar& d, int nx, int P, int Q, int kx, int p, int q, double* xi) {
+  const int i[3] = {kx, p, q}, n[3] = {nx, P, Q};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    xi[a] = fnyq_fs(i[a], n[a]) ? 0.0 : 6.283185307179586 * ((double)fbin_fs(i[a], n[a]) * d.val[a]);
+}
+
+template <typename T, int MAXV, int NN = 0, int BB = 0>
+__global__ __launch_bounds__(MGK_NT) void k_db_pencil(FsGeom<T> g, DbPar d, int op, int tr, T scale,
+                                                      const cplx<T>* __restrict__ sin, cplx<T>* __restrict__ sout,
+                                                      const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* tile = tw + g.Q;
+  constexpr bool CT = NN > 0;
+  const int Q = CT ? NN : g.Q, P = g.P, logPP = CT ? ct_log2(BB) : g.logPP, PP = CT ? BB : 1 << logPP;
+  const int ldb = PP, cstride = Q * PP;
+  const int pid0 = blockIdx.x * PP;
+  const int row = blockIdx.y;
+  const int npen = g.npencil;
+  for (int t = threadIdx.x; t < Q; t += MGK_NT) tw[t] = g.twq[t];
+  auto ti = [&](int c, int q, int pp) { return tpad<CT>(c * cstride + q * ldb + pp); };
+  // component maps of this row
+  int cin[3], cout[3];
+  const int nin = op == DB_GRAD ? 1 : 3, nout = op == DB_DIV ? 1 : 3;
+  for (int j = 0; j < 3; ++j) {
+    const int c9 = tr ? 3 * j + row : 3 * row + j;
+    if (op == DB_GRAD) {
+      cin[j] = row;
+      cout[j] = c9;
+    } else if (op == DB_DIV) {
+      cin[j] = c9;
+      cout[j] = row;
+    } else if (op == DB_COMPAT) {
+      cin[j] = cout[j] = 3 * row + j;
+    } else {
+      cin[j] = cout[j] = j;
+    }
+  }
+  const int chunk = PP * Q;
+  for (int e = threadIdx.x; e < nin * chunk; e += MGK_NT) {
+    const int c = e / chunk;
+    const int r = e - c * chunk;
+    const int pp = r / Q;
+    const int q = r - pp * Q;
+    const int pid = pid0 + pp;
+    tile[ti(c, q, pp)] = (pid < npen) ? sin[((size_t)cin[c] * npen + pid) * Q + q] : cmk(T(0), T(0));
+  }
+  __syncthreads();
+  if constexpr (CT) {
+    if (nin == 1) tile_fft_ct<NN, BB, 1, true>(tile, tw, false);
+    else tile_fft_ct<NN, BB, 3, true>(tile, tw, false);
+  } else {
+    tile_fft<MAXV>(tile, Tile{Q, logPP, ldb, nin, cstride}, g.plq, tw, false);
+  }
+  for (int e = threadIdx.x; e < Q * PP; e += MGK_NT) {
+    const int pp = e & (PP - 1), q = e >> logPP;
+    const int pid = pid0 + pp;
+    if (pid >= npen) continue;
+    const int kx = pid / P, p = pid - kx * P;
+    const bool dc = kx == 0 && p == 0 && q == 0;
+    double xi[3];
+    db_xi(d, g.nx, P, Q, kx, p, q, xi);
+    cplx<T> a[3];
+    for (int c = 0; c < nin; ++c) a[c] = tile[ti(c, q, pp)];
+    double ore[3] = {0, 0, 0}, oim[3] = {0, 0, 0};
+    if (op == DB_GRAD) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        ore[j] = -xi[j] * (double)a[0].y;
+        oim[j] = xi[j] * (double)a[0].x;
+      }
+    } else if (op == DB_DIV) {
+      if (!dc) {
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          sr += xi[j] * (double)a[j].x;
+          si += xi[j] * (double)a[j].y;
+        }
+        ore[0] = -si;
+        oim[0] = sr;
+      }
+    } else if (op == DB_COMPAT) {
+      if (!dc) {
+        double x2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+        if (x2 < 1e-14) x2 = 1.0;
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          sr += xi[k] * (double)a[k].x;
+          si += xi[k] * (double)a[k].y;
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          ore[j] = xi[j] * sr / x2;
+          oim[j] = xi[j] * si / x2;
+        }
+      }
+    } else if (!dc) {
+      const double* K = d.Kbar;
+      double A[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          double acc = 0.0;
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) acc += xi[j] * K[27 * i + 9 * j + 3 * l + k] * xi[l];
+          A[3 * i + k] = acc;
+        }
+      double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                   A[2] * (A[3] * A[7] - A[4] * A[6]);
+      if (fabs(det) < 1e-14) det = 1.0;
+      double M[9];
+      M[0] = (A[4] * A[8] - A[5] * A[7]) / det;
+      M[1] = -(A[1] * A[8] - A[2] * A[7]) / det;
+      M[2] = (A[1] * A[5] - A[2] * A[4]) / det;
+      M[3] = -(A[3] * A[8] - A[5] * A[6]) / det;
+      M[4] = (A[0] * A[8] - A[2] * A[6]) / det;
+      M[5] = -(A[0] * A[5] - A[2] * A[3]) / det;
+      M[6] = (A[3] * A[7] - A[4] * A[6]) / det;
+      M[7] = -(A[0] * A[7] - A[1] * A[6]) / det;
+      M[8] = (A[0] * A[4] - A[1] * A[3]) / det;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        ore[i] = M[3 * i] * (double)a[0].x + M[3 * i + 1] * (double)a[1].x + M[3 * i + 2] * (double)a[2].x;
+        oim[i] = M[3 * i] * (double)a[0].y + M[3 * i + 1] * (double)a[1].y + M[3 * i + 2] * (double)a[2].y;
+      }
+    }
+    for (int c = 0; c < nout; ++c) tile[ti(c, q, pp)] = cmk(T(ore[c] * (double)scale), T(oim[c] * (double)scale));
+  }
+  __syncthreads();
+  if constexpr (CT) {
+    if (nout == 1) tile_fft_ct<NN, BB, 1, true>(tile, tw, true);
+    else tile_fft_ct<NN, BB, 3, true>(tile, tw, true);
+  } else {
+    tile_fft<MAXV>(tile, Tile{Q, logPP, ldb, nout, cstride}, g.plq, tw, true);
+  }
+  for (int e = threadIdx.x; e < nout * chunk; e += MGK_NT) {
+    const int c = e / chunk;
+    const int r = e - c * chunk;
+    const int pp = r / Q;
+    const int q = r - pp * Q;
+    const int pid = pid0 + pp;
+    if (pid < npen) sout[((size_t)cout[c] * npen + pid) * Q + q] = tile[ti(c, q, pp)];
+  }
+}
+
+// per-block sums of N(e_kl) (the tangent action on the unit direction kl,
+// blockIdx.y = kl) for the voxel mean of the reference's K4:
+// K4_abkl = N(e_kl)_ba under its (K4 : dF^T)^T matvec
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_fs_kmean(size_t NV, const T* __restrict__ F, const T* __restrict__ Fpi,
+                                                     const T* __restrict__ lam, const T* __restrict__ mu, FsMat m,
+                                                     double* __restrict__ partials) {
+  __shared__ double red[9 * (MGK_NT / 64)];
+  const int kl = blockIdx.y;
+  double acc[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+  for (size_t v = blockIdx.x * (size_t)MGK_NT + threadIdx.x; v < NV; v += (size_t)gridDim.x * MGK_NT) {
+    T f[9], fpi[9], d[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      f[k] = F[(size_t)k * NV + v];
+      if (Fpi) fpi[k] = Fpi[(size_t)k * NV + v];
+      d[k] = k == kl ? T(1) : T(0);
+    }
+    FsVox<T> st;
+    fs_state(f, Fpi ? fpi : (const T*)nullptr, lam[v], mu[v], m, st, (T*)nullptr);
+    T o[9];
+    fs_tangent(st, m, d, Fpi != nullptr, o);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc[k] += (double)o[k];
+  }
+  block_sum<9>(acc, red);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) partials[((size_t)kl * gridDim.x + blockIdx.x) * 9 + k] = acc[k];
+}
+
+__global__ void k_fs_kmean_fin(const double* __restrict__ partials, int nb, double inv_nv, double* __restrict__ Kbar) {
+  const int idx = blockIdx.x;  // 27 a + 9 b + 3 k + l
+  const int a = idx / 27, b = (idx / 9) % 3, kl = idx % 9;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partials[((size_t)kl * nb + i) * 9 + 3 * b + a];
+  __shared__ double red[MGK_NT / 64];
+  double v[1] = {acc};
+  block_sum<1>(v, red);
+  if (threadIdx.x == 0) Kbar[idx] = v[0] * inv_nv;
+}
+
+// ---------------------------------------------------------------------------
 // BiCGSTAB (SciPy 1.15.3 recurrence) with device-resident scalars
 // ---------------------------------------------------------------------------
 struct KryState {
@@ -688,7 +908,7 @@ struct KryState {
   int iters;
   int maxiter;
   int pad0, pad1;
-  double rtol, atol, bnrm2, rho, rho_prev, alpha, omega, beta, rv, ts, tt, rnorm, snorm, rhotol;
+  double rtol, atol, bnrm2, rho, rho_prev, alpha, omega, beta, rv, ts, tt, rnorm, snorm, rhotol, omtol;
 };
 
 #define KRY_NBLK 1024
@@ -880,11 +1100,13 @@ __global__ __launch_bounds__(MGK_NT) void k_kry_fin_t(const double* __restrict__
   }
 }
 
-// x += alpha p; x += omega s; r = s - omega t; partials |r|^2, (rtilde, r)
+// x += alpha p; x += omega s; r = s - omega t; partials |r|^2, (rtilde, r).
+// Preconditioned: p is M p and sh = M s replaces s in the x update.
 template <typename T>
 __global__ __launch_bounds__(MGK_NT) void k_kry_xr(size_t n, const KryState* __restrict__ st, T* __restrict__ x,
                                                    const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ t,
-                                                   const T* __restrict__ rt, double* __restrict__ partials) {
+                                                   const T* __restrict__ rt, double* __restrict__ partials,
+                                                   const T* __restrict__ sh) {
   if (st->done) return;
   __shared__ double red[2 * (MGK_NT / 64)];
   const T al = (T)st->alpha, om = (T)st->omega;
@@ -900,14 +1122,15 @@ __global__ __launch_bounds__(MGK_NT) void k_kry_xr(size_t n, const KryState* __r
 #pragma unroll
       for (int k = 0; k < VW; ++k) xv[k] = xv[k] + al * pv[k];
     } else {
-      T sv[VW], tv[VW], rtv[VW];
+      T sv[VW], tv[VW], rtv[VW], hv[VW];
       vload16(r + j * VW, sv);
       vload16(t + j * VW, tv);
       vload16(rt + j * VW, rtv);
+      if (sh) vload16(sh + j * VW, hv);
 #pragma unroll
       for (int k = 0; k < VW; ++k) {
         T q = xv[k] + al * pv[k];
-        xv[k] = q + om * sv[k];
+        xv[k] = q + om * (sh ? hv[k] : sv[k]);
         const T rn = sv[k] - om * tv[k];
         sv[k] = rn;
         acc[0] += (double)rn * (double)rn;
@@ -923,7 +1146,7 @@ __global__ __launch_bounds__(MGK_NT) void k_kry_xr(size_t n, const KryState* __r
     } else {
       const T s = r[i];
       T xv = x[i] + al * p[i];
-      xv = xv + om * s;
+      xv = xv + om * (sh ? sh[i] : s);
       x[i] = xv;
       const T rn = s - om * t[i];
       r[i] = rn;
@@ -964,7 +1187,7 @@ __global__ __launch_bounds__(MGK_NT) void k_kry_fin_xr(const double* __restrict_
     } else if (fabs(rho_new) < st->rhotol) {
       st->done = 1;
       st->status = -10;
-    } else if (fabs(st->omega) < st->rhotol) {
+    } else if (fabs(st->omega) < st->omtol) {
       st->done = 1;
       st->status = -11;
     } else {
@@ -1104,7 +1327,7 @@ struct FsPlan {
   int ctx, cty, ctq;
   size_t lds_x, lds_y, lds_q;
   int nqx, nqy, npb;
-  size_t fld, spec;  // bytes per 9-comp field and spectrum
+  size_t fld, spec, spec3;  // bytes per 9-comp field, 9-comp and 3-comp spectrum
   int nred;
   size_t total;
 };
@@ -1169,10 +1392,11 @@ int fs_plan(const int64_t* shape, FsPlan* pl) {
   pl->npb = (npen + (1 << pl->logPP) - 1) >> pl->logPP;
   pl->fld = mgk_align((size_t)9 * pl->NV * sizeof(T));
   pl->spec = mgk_align((size_t)9 * pl->nxh * pl->P * pl->Q * cb);
+  pl->spec3 = mgk_align((size_t)3 * pl->nxh * pl->P * pl->Q * cb);
   pl->nred = (int)std::min<long long>((9 * pl->NV + MGK_NT - 1) / MGK_NT, KRY_NBLK);
-  // fields: Ftrial P Ptrial b x r rt p v t dP Fpi  (12) + spectrum + partials + states
+  // fields: Ftrial P Ptrial b x r rt p v t dP Fpi (12) + spectra + partials + states + mean tangent
   const size_t nparts = std::max<size_t>((size_t)pl->nqx * pl->P * 5 * 2, (size_t)KRY_NBLK * FS_NPART);
-  pl->total = 12 * pl->fld + pl->spec + mgk_align(nparts * sizeof(double)) + 4096;
+  pl->total = 12 * pl->fld + pl->spec + pl->spec3 + mgk_align(nparts * sizeof(double)) + 4096 + 1024;
   return MGK_OK;
 }
 
@@ -1183,6 +1407,8 @@ struct FsCtx {
   hipStream_t s;
   T *Ftr, *P, *Ptr, *b, *x, *r, *rt, *p, *v, *t, *dP, *Fpi;
   cplx<T>* spec;
+  cplx<T>* spec3;
+  double* Kbar;  // 81 voxel means of the reference's K4 (DBFFT preconditioner)
   double* part;
   KryState* kst;
   double* scal;  // 16 doubles of reductions
@@ -1194,8 +1420,8 @@ struct FsCtx {
 
 template <typename T, int MV>
 void fs_launch_mid(const FsPlan& pl, hipStream_t s, const FftPlan& ply, const cplx<T>* twy, cplx<T>* spec, int inv,
-                   const int* skip) {
-  const dim3 gy(pl.nqy, pl.nxh, 9);
+                   const int* skip, int nc = 9) {
+  const dim3 gy(pl.nqy, pl.nxh, nc);
   if (pl.cty) {
     switch (pl.P) {
       case 64: k_fft_mid<T, ct_tile<T>() / MGK_NT, 64, ct_tile<T>() / 64, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
@@ -1303,6 +1529,46 @@ int kry_hint(const void* key, int iters) {
   return it == last.end() ? 0 : it->second;
 }
 
+// Enqueues BiCGSTAB iterations in growing chunks and polls the device state
+// one chunk behind, so the queue never drains while the host waits
+template <typename Enq>
+int kry_drive(KryState* kst, hipStream_t s, int maxiter, Enq&& enqueue, int* iters, int* status, int* enq_out) {
+  KryState* hs = (KryState*)mgk_pinned(2 * sizeof(KryState));
+  hipEvent_t ev[2];
+  MGK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  MGK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  // first chunk: the iteration count of the previous solve on this workspace
+  // (Newton steps of one load path need similar Krylov counts)
+  const int hint = kry_hint(kst, -1);
+  int enq = 0, chunk = hint > 0 ? hint + 1 : 4, slot = 0;
+  bool pending = false, stop = false, first = true;
+  while (!stop && enq < maxiter) {
+    const int k = std::min(chunk, maxiter - enq);
+    for (int i = 0; i < k; ++i) enqueue();
+    enq += k;
+    MGK_HIP(hipGetLastError());
+    MGK_HIP(hipMemcpyAsync(&hs[slot], kst, sizeof(KryState), hipMemcpyDeviceToHost, s));
+    MGK_HIP(hipEventRecord(ev[slot], s));
+    if (pending) {
+      MGK_HIP(hipEventSynchronize(ev[slot ^ 1]));
+      if (hs[slot ^ 1].done) stop = true;
+    }
+    pending = true;
+    slot ^= 1;
+    chunk = first ? 2 : std::min(chunk * 2, 32);
+    first = false;
+  }
+  MGK_HIP(hipMemcpyAsync(&hs[0], kst, sizeof(KryState), hipMemcpyDeviceToHost, s));
+  MGK_HIP(hipStreamSynchronize(s));
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  *iters = hs[0].iters;
+  *status = hs[0].status;
+  kry_hint(kst, hs[0].iters);
+  *enq_out = enq;
+  return MGK_OK;
+}
+
 // SciPy bicgstab(A, b, x0=None, rtol, maxiter) -> c.x
 template <typename T>
 int fs_bicgstab(FsCtx<T>& c, const T* Fcur, double rtol, int maxiter, int* iters, int* status) {
@@ -1313,6 +1579,7 @@ int fs_bicgstab(FsCtx<T>& c, const T* Fcur, double rtol, int maxiter, int* iters
   init.rtol = rtol;
   init.maxiter = maxiter;
   init.rhotol = 4.930380657631324e-32;  // eps(double)^2, as SciPy for float64
+  init.omtol = init.rhotol;
   MGK_HIP(hipMemcpyAsync(c.kst, &init, sizeof(KryState), hipMemcpyHostToDevice, c.s));
   k_kry_init<T><<<nb, MGK_NT, 0, c.s>>>(n, c.b, c.x, c.r, c.rt, c.part);
   k_kry_fin_init<<<1, MGK_NT, 0, c.s>>>(c.part, nb, c.kst);
@@ -1328,46 +1595,17 @@ int fs_bicgstab(FsCtx<T>& c, const T* Fcur, double rtol, int maxiter, int* iters
     fs_matvec(c, Fcur, c.r, c.t, c.r, 1, c.part, &c.kst->done2);
     k_kry_fin_t<<<1, MGK_NT, 0, c.s>>>(c.part, nxp, c.kst);
     const int h2 = c.prof->begin(KID_FS_VEC, c.kry_it, 6.0 * n * sizeof(T));
-    k_kry_xr<T><<<nb, MGK_NT, 0, c.s>>>(n, c.kst, c.x, c.p, c.r, c.t, c.rt, c.part);
+    k_kry_xr<T><<<nb, MGK_NT, 0, c.s>>>(n, c.kst, c.x, c.p, c.r, c.t, c.rt, c.part, (const T*)nullptr);
     c.prof->end(h2);
     k_kry_fin_xr<<<1, MGK_NT, 0, c.s>>>(c.part, nb, c.kst);
     ++c.kry_it;
   };
-  KryState* hs = (KryState*)mgk_pinned(2 * sizeof(KryState));
-  hipEvent_t ev[2];
-  MGK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  MGK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   const int start_it = c.kry_it;
-  // first chunk: the iteration count of the previous solve on this workspace
-  // (Newton steps of one load path need similar Krylov counts)
-  const int hint = kry_hint(c.kst, -1);
-  int enq = 0, chunk = hint > 0 ? hint + 1 : 4, slot = 0;
-  bool pending = false, stop = false, first = true;
-  while (!stop && enq < maxiter) {
-    const int k = std::min(chunk, maxiter - enq);
-    for (int i = 0; i < k; ++i) enqueue();
-    enq += k;
-    MGK_HIP(hipGetLastError());
-    MGK_HIP(hipMemcpyAsync(&hs[slot], c.kst, sizeof(KryState), hipMemcpyDeviceToHost, c.s));
-    MGK_HIP(hipEventRecord(ev[slot], c.s));
-    if (pending) {
-      MGK_HIP(hipEventSynchronize(ev[slot ^ 1]));
-      if (hs[slot ^ 1].done) stop = true;
-    }
-    pending = true;
-    slot ^= 1;
-    chunk = first ? 2 : std::min(chunk * 2, 32);
-    first = false;
-  }
-  MGK_HIP(hipMemcpyAsync(&hs[0], c.kst, sizeof(KryState), hipMemcpyDeviceToHost, c.s));
-  MGK_HIP(hipStreamSynchronize(c.s));
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
-  *iters = hs[0].iters;
-  *status = hs[0].status;
-  kry_hint(c.kst, hs[0].iters);
+  int enq = 0;
+  const int rc = kry_drive(c.kst, c.s, maxiter, enqueue, iters, status, &enq);
+  if (rc) return rc;
   // launches queued after the stop exited at once: not timed
-  c.prof->invalidate(start_it + hs[0].iters + 1, start_it + enq);
+  c.prof->invalidate(start_it + *iters + 1, start_it + enq);
   return MGK_OK;
 }
 
@@ -1384,10 +1622,10 @@ int fs_read_scal(FsCtx<T>& c, int nparts, int nv, double* out) {
 // constitutive pass; returns [minJe, nonfinite, |F|^2, sig mean x9] (means divided)
 template <typename T>
 int fs_const(FsCtx<T>& c, const T* Fb, const T* dX, const Nine& dXu, double alpha, int trial, T* Fout, T* P,
-             double* res) {
+             double* res, int chk_tan = 1) {
   const int nb = fs_nblk<T>(c.pl.NV);
   k_fs_const<T><<<nb, MGK_NT, 0, c.s>>>((size_t)c.pl.NV, Fb, dX, dXu, alpha, trial, Fout, c.Fpi, c.lam, c.mu, c.mat,
-                                        P, c.part);
+                                        P, c.part, chk_tan);
   MGK_HIP(hipGetLastError());
   // min over blocks of column 0 is needed, the rest are sums
   std::vector<double> hp((size_t)nb * FS_NPART);
@@ -1445,6 +1683,7 @@ int fs_ctx_init(FsCtx<T>& c, const int64_t* shape, const T* lam, const T* mu, co
   g.logBy = pl.logBy;
   g.logPP = pl.logPP;
   g.npencil = pl.nxh * pl.P;
+  g.nc = 9;
   mgk_make_plan(pl.nx, &g.plx);
   mgk_make_plan(pl.P, &g.ply);
   mgk_make_plan(pl.Q, &g.plq);
@@ -1463,11 +1702,15 @@ int fs_ctx_init(FsCtx<T>& c, const int64_t* shape, const T* lam, const T* mu, co
   }
   c.spec = (cplx<T>*)w;
   w += pl.spec;
+  c.spec3 = (cplx<T>*)w;
+  w += pl.spec3;
   c.part = (double*)w;
   w += mgk_align(std::max<size_t>((size_t)pl.nqx * pl.P * 5 * 2, (size_t)KRY_NBLK * FS_NPART) * sizeof(double));
   c.kst = (KryState*)w;
   w += mgk_align(sizeof(KryState));
   c.scal = (double*)w;
+  w += 4096 - mgk_align(sizeof(KryState));
+  c.Kbar = (double*)w;
   if (!has_fp) c.Fpi = nullptr;
   return MGK_OK;
 }
@@ -1499,12 +1742,215 @@ int fs_bicgstab_f32(FsCtx<double>& c, FsCtx<float>& c32, const double* Fcur, dou
   return MGK_OK;
 }
 
+// ---------------------------------------------------------------------------
+// DBFFT Newton step (finite_strain_simulator.py:1508-1620)
+// ---------------------------------------------------------------------------
+template <typename T>
+void fs_launch_xr2c(FsCtx<T>& c, int nc, const T* in, cplx<T>* spec, const int* skip) {
+  constexpr int MV = fs_maxv<T>();
+  constexpr int TT = ct_tile<T>();
+  const FsPlan& pl = c.pl;
+  FsGeom<T> g = c.g;
+  g.nc = nc;
+  const dim3 gx(pl.nqx, pl.P, (nc + 1) / 2);
+  if (pl.ctx) {
+    switch (pl.nx) {
+      case 64: k_fs_xr2c<T, TT / 512, 64, TT / 64, 512><<<gx, 512, pl.lds_x, c.s>>>(g, in, spec, skip); break;
+      case 128: k_fs_xr2c<T, TT / 512, 128, TT / 128, 512><<<gx, 512, pl.lds_x, c.s>>>(g, in, spec, skip); break;
+      case 256: k_fs_xr2c<T, TT / 512, 256, TT / 256, 512><<<gx, 512, pl.lds_x, c.s>>>(g, in, spec, skip); break;
+      default: k_fs_xr2c<T, TT / 512, 512, TT / 512, 512><<<gx, 512, pl.lds_x, c.s>>>(g, in, spec, skip); break;
+    }
+  } else {
+    k_fs_xr2c<T, MV><<<gx, MGK_NT, pl.lds_x, c.s>>>(g, in, spec, skip);
+  }
+}
+
+template <typename T>
+void fs_launch_xc2r(FsCtx<T>& c, int nc, const cplx<T>* spec, T* out, const T* dota, int selfdot,
+                    double* partials, const int* skip) {
+  constexpr int MV = fs_maxv<T>();
+  constexpr int TT = ct_tile<T>();
+  const FsPlan& pl = c.pl;
+  FsGeom<T> g = c.g;
+  g.nc = nc;
+  const dim3 gx(pl.nqx, pl.P, (nc + 1) / 2);
+  if (pl.ctx) {
+    switch (pl.nx) {
+      case 64: k_fs_xc2r<T, TT / 512, 64, TT / 64, 512><<<gx, 512, pl.lds_x, c.s>>>(g, spec, out, dota, selfdot, partials, skip); break;
+      case 128: k_fs_xc2r<T, TT / 512, 128, TT / 128, 512><<<gx, 512, pl.lds_x, c.s>>>(g, spec, out, dota, selfdot, partials, skip); break;
+      case 256: k_fs_xc2r<T, TT / 512, 256, TT / 256, 512><<<gx, 512, pl.lds_x, c.s>>>(g, spec, out, dota, selfdot, partials, skip); break;
+      default: k_fs_xc2r<T, TT / 512, 512, TT / 512, 512><<<gx, 512, pl.lds_x, c.s>>>(g, spec, out, dota, selfdot, partials, skip); break;
+    }
+  } else {
+    k_fs_xc2r<T, MV><<<gx, MGK_NT, pl.lds_x, c.s>>>(g, spec, out, dota, selfdot, partials, skip);
+  }
+}
+
+// one DBFFT operator: x r2c, y FFT, z pencil with the operator, back
+template <typename T>
+void fs_db_apply(FsCtx<T>& c, const DbPar& d, int op, int tr, double scale, const T* in, T* out, const T* dota,
+                 int selfdot, double* partials, const int* skip) {
+  constexpr int MV = fs_maxv<T>();
+  constexpr int TT = ct_tile<T>();
+  const FsPlan& pl = c.pl;
+  const int nin = (op == DB_GRAD || op == DB_PREC) ? 3 : 9;
+  const int nout = (op == DB_DIV || op == DB_PREC) ? 3 : 9;
+  cplx<T>* si = nin == 3 ? c.spec3 : c.spec;
+  cplx<T>* so = nout == 3 ? c.spec3 : c.spec;
+  fs_launch_xr2c(c, nin, in, si, skip);
+  if (pl.P > 1) fs_launch_mid<T, MV>(pl, c.s, c.g.ply, c.g.twy, si, 0, skip, nin);
+  const dim3 gq(pl.npb, op == DB_PREC ? 1 : 3);
+  const T sc = (T)scale;
+  if (pl.ctq) {
+    switch (pl.Q) {
+      case 64: k_db_pencil<T, TT / MGK_NT, 64, ct_pp(TT, 3, 64)><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, d, op, tr, sc, si, so, skip); break;
+      case 128: k_db_pencil<T, TT / MGK_NT, 128, ct_pp(TT, 3, 128)><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, d, op, tr, sc, si, so, skip); break;
+      case 256: k_db_pencil<T, TT / MGK_NT, 256, ct_pp(TT, 3, 256)><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, d, op, tr, sc, si, so, skip); break;
+      default: k_db_pencil<T, TT / MGK_NT, 512, ct_pp(TT, 3, 512)><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, d, op, tr, sc, si, so, skip); break;
+    }
+  } else {
+    k_db_pencil<T, MV><<<gq, MGK_NT, pl.lds_q, c.s>>>(c.g, d, op, tr, sc, si, so, skip);
+  }
+  if (pl.P > 1) fs_launch_mid<T, MV>(pl, c.s, c.g.ply, c.g.twy, so, 1, skip, nout);
+  fs_launch_xc2r(c, nout, so, out, dota, selfdot, partials, skip);
+}
+
+// A w = div(K4 : grad w) as the reference's A_op: grad stored transposed, the
+// tangent action N = (K4 : X^T)^T, div reading N transposed
+template <typename T>
+void fs_db_matvec(FsCtx<T>& c, const DbPar& d, const T* Fcur, const T* in, T* out, const T* dota, int selfdot,
+                  double* partials, const int* skip) {
+  const size_t NV = (size_t)c.pl.NV;
+  const double inv = 1.0 / (double)NV;
+  fs_db_apply(c, d, DB_GRAD, 1, inv, in, c.Ftr, (const T*)nullptr, 0, (double*)nullptr, skip);
+  Nine z{};
+  k_fs_dP<T><<<fs_nblk<T>(c.pl.NV), MGK_NT, 0, c.s>>>(NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, c.Ftr, z, c.dP, skip);
+  fs_db_apply(c, d, DB_DIV, 1, inv, c.dP, out, dota, selfdot, partials, skip);
+}
+
+// SciPy bicgstab(A, b, M=M, rtol, maxiter) on real fields -> c.x (3 comps).
+// rho is compared with eps^2 / NV: the reference's vdot over full spectra is
+// NV times the real-space dot; every other test is scale-free.
+template <typename T>
+int fs_db_bicgstab(FsCtx<T>& c, const DbPar& d, const T* Fcur, double rtol, int maxiter, int* iters, int* status) {
+  const FsPlan& pl = c.pl;
+  const size_t NV = (size_t)pl.NV, n = 3 * NV;
+  const int nb = fs_nblk<T>((long long)n);
+  const double inv = 1.0 / (double)NV;
+  KryState init{};
+  init.rtol = rtol;
+  init.maxiter = maxiter;
+  init.rhotol = 4.930380657631324e-32 / (double)NV;
+  init.omtol = 4.930380657631324e-32;
+  MGK_HIP(hipMemcpyAsync(c.kst, &init, sizeof(KryState), hipMemcpyHostToDevice, c.s));
+  k_kry_init<T><<<nb, MGK_NT, 0, c.s>>>(n, c.b, c.x, c.r, c.rt, c.part);
+  k_kry_fin_init<<<1, MGK_NT, 0, c.s>>>(c.part, nb, c.kst);
+  T* ph = c.Ptr;  // M p
+  T* sh = c.b;    // M s (b is read once, by the init)
+  const int nxp = pl.nqx * pl.P * 2;
+  auto enqueue = [&]() {
+    k_kry_p<T><<<nb, MGK_NT, 0, c.s>>>(n, c.kst, c.p, c.r, c.v);
+    fs_db_apply(c, d, DB_PREC, 0, inv, c.p, ph, (const T*)nullptr, 0, (double*)nullptr, &c.kst->done);
+    fs_db_matvec(c, d, Fcur, ph, c.v, c.rt, 0, c.part, &c.kst->done);
+    k_kry_fin_rv<<<1, MGK_NT, 0, c.s>>>(c.part, nxp, c.kst);
+    k_kry_s<T><<<nb, MGK_NT, 0, c.s>>>(n, c.kst, c.r, c.v, c.part);
+    k_kry_fin_s<<<1, MGK_NT, 0, c.s>>>(c.part, nb, c.kst);
+    fs_db_apply(c, d, DB_PREC, 0, inv, c.r, sh, (const T*)nullptr, 0, (double*)nullptr, &c.kst->done2);
+    fs_db_matvec(c, d, Fcur, sh, c.t, c.r, 1, c.part, &c.kst->done2);
+    k_kry_fin_t<<<1, MGK_NT, 0, c.s>>>(c.part, nxp, c.kst);
+    k_kry_xr<T><<<nb, MGK_NT, 0, c.s>>>(n, c.kst, c.x, ph, c.r, c.t, c.rt, c.part, (const T*)sh);
+    k_kry_fin_xr<<<1, MGK_NT, 0, c.s>>>(c.part, nb, c.kst);
+  };
+  int enq = 0;
+  return kry_drive(c.kst, c.s, maxiter, enqueue, iters, status, &enq);
+}
+
+// _dbfft_step_3d: F_curr = F_bar + grad u(F) (the compatible part of F), then
+// Newton: b = -div P, stop when |b_hat| / |P| < tol after the first
+// iteration, preconditioned BiCGSTAB for du, line search alpha = 1 .. 2^-15
+// on det F_e > 1e-4 and a finite P.  F and c.P hold the result.
+template <typename T>
+int fs_db_newton(FsCtx<T>& c, DbPar& d, T* F, const double* Fbc, double tol, double tol_CG, int max_iter,
+                 mgk_fs_info* info) {
+  const FsPlan& pl = c.pl;
+  const size_t NV = (size_t)pl.NV;
+  const double inv = 1.0 / (double)NV;
+  double res[FS_NPART];
+  int rc;
+  fs_db_apply(c, d, DB_COMPAT, 0, inv, F, c.Ftr, (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
+  Nine fb;
+  for (int k = 0; k < 9; ++k) fb.v[k] = Fbc[k];
+  rc = fs_const(c, c.Ftr, (const T*)nullptr, fb, 1.0, 1, F, c.P, res, 0);
+  if (rc) return rc;
+  for (int k = 0; k < 9; ++k) info->sig_mean[k] = res[3 + k];
+  const int nb9 = fs_nblk<T>(9 * pl.NV), nb3 = fs_nblk<T>(3 * pl.NV);
+  const int nxp = pl.nqx * pl.P * 2;
+  const int nbk = std::min(fs_nblk<T>(pl.NV), 128);
+  d.Kbar = c.Kbar;
+  double rel = 0.0;
+  const Nine zero{};
+  for (int i = 0; i < max_iter; ++i) {
+    info->newton_iters += 1;
+    fs_db_apply(c, d, DB_DIV, 0, -inv, c.P, c.b, (const T*)nullptr, 1, c.part, (const int*)nullptr);
+    double bb[2], pp;
+    rc = fs_read_scal(c, nxp, 2, bb);
+    if (rc) return rc;
+    k_norm2<T><<<nb9, MGK_NT, 0, c.s>>>(9 * NV, c.P, c.part);
+    rc = fs_read_scal(c, nb9, 1, &pp);
+    if (rc) return rc;
+    rel = sqrt((double)NV * bb[1]) / (sqrt(pp) + 1e-20);
+    if (rel < tol && i > 0) return MGK_OK;
+    k_fs_kmean<T><<<dim3(nbk, 9), MGK_NT, 0, c.s>>>(NV, F, c.Fpi, c.lam, c.mu, c.mat, c.part);
+    k_fs_kmean_fin<<<81, MGK_NT, 0, c.s>>>(c.part, nbk, inv, c.Kbar);
+    int kit = 0, kst = 0;
+    rc = fs_db_bicgstab(c, d, F, tol_CG, 150, &kit, &kst);
+    if (rc) return rc;
+    info->krylov_iters += kit;
+    double xn;
+    k_norm2<T><<<nb3, MGK_NT, 0, c.s>>>(3 * NV, c.x, c.part);
+    rc = fs_read_scal(c, nb3, 1, &xn);
+    if (rc) return rc;
+    if (!std::isfinite(xn)) {
+      info->fail_kind = 1;
+      mgk_set_error("Linear solver returned NaN or Inf (BiCGSTAB breakdown).", __FILE__, __LINE__);
+      return MGK_ERR_SOLVER;
+    }
+    fs_db_apply(c, d, DB_GRAD, 0, inv, c.x, c.dP, (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
+    double alpha = 1.0;
+    bool accepted = false;
+    for (int ls = 0; ls < 16; ++ls) {
+      rc = fs_const(c, F, c.dP, zero, alpha, 1, c.Ftr, c.Ptr, res, 0);
+      if (rc) return rc;
+      if (res[0] > 1e-4 && res[1] == 0.0) {
+        accepted = true;
+        break;
+      }
+      alpha *= 0.5;
+    }
+    if (!accepted) {
+      info->fail_kind = 1;
+      mgk_set_error("Constitutive solver failed to converge in line search (potential negative Jacobian/NaN).",
+                    __FILE__, __LINE__);
+      return MGK_ERR_SOLVER;
+    }
+    std::swap(c.Ptr, c.P);
+    MGK_HIP(hipMemcpyAsync(F, c.Ftr, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, c.s));
+    for (int k = 0; k < 9; ++k) info->sig_mean[k] = res[3 + k];
+  }
+  info->fail_kind = 2;
+  char msg[160];
+  snprintf(msg, sizeof msg, "DBFFT solver did not converge within %d iterations. Final relative residual: %.2e",
+           max_iter, rel);
+  mgk_set_error(msg, __FILE__, __LINE__);
+  return MGK_ERR_SOLVER;
+}
+
 template <typename T>
 int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu, const mgk_fs_material* mt,
               const double* F_bar, const double* P_target, const int* P_mask, int i_drv, int j_drv, double E_avg,
               double nu_avg, double tol_NW, double tol_CG, int max_NW, double tol_macro, int max_iter_macro,
               T* P_out, T* sig_out, char* ws, hipStream_t s, mgk_fs_info* info, FsCtx<float>* inner = nullptr,
-              const double* al_L0 = nullptr) {
+              const double* al_L0 = nullptr, const DbPar* db = nullptr) {
   FsCtx<T> c;
   int rc = fs_ctx_init(c, shape, lam, mu, mt, Fp != nullptr, ws, s);
   if (rc) return rc;
@@ -1538,6 +1984,7 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
   info->fail_kind = 0;
   int it_mac = 0;
   bool macro_ok = false;
+  const bool alt = al_L0 || db;
   for (it_mac = 0; it_mac < max_iter_macro; ++it_mac) {
     info->macro_iters = it_mac + 1;
     // DbarF = F_bar - mean(F_start)
@@ -1557,7 +2004,7 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
       // lam = P; per iteration dF = -G(lam - L0:F), step beta in 1 .. 1/16 with
       // the mean reset to F_bar (det F_e > 1e-4, finite P), lam = P, stop when
       // |G(P)| / (sqrt(9 NV) (<|P|> + 1e-20)) < tol after the first iteration
-      rc = fs_const(c, F, (const T*)nullptr, DbarF, 1.0, 1, c.Ftr, c.P, res);
+      rc = fs_const(c, F, (const T*)nullptr, DbarF, 1.0, 1, c.Ftr, c.P, res, 0);
       if (rc) return rc;
       MGK_HIP(hipMemcpyAsync(F, c.Ftr, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
       MGK_HIP(hipMemcpyAsync(c.b, c.P, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
@@ -1580,7 +2027,7 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
         for (double beta = 1.0; beta >= 0.0625; beta *= 0.5) {
           Nine D2;  // mean reset, folded into the step: F + beta (dF + D2)
           for (int k = 0; k < 9; ++k) D2.v[k] = (Fbc[k] - (mF[k] / (double)NV + beta * (mdF[k] / (double)NV))) / beta;
-          rc = fs_const(c, F, c.x, D2, beta, 1, c.Ftr, c.Ptr, res);
+          rc = fs_const(c, F, c.x, D2, beta, 1, c.Ftr, c.Ptr, res, 0);
           if (rc) return rc;
           if (res[1] == 0.0 && res[0] > 1e-4) {
             step_ok = true;
@@ -1625,13 +2072,18 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
         return MGK_ERR_SOLVER;
       }
     }
+    if (db) {
+      DbPar dp = *db;
+      rc = fs_db_newton(c, dp, F, Fbc, tol_NW, tol_CG, max_NW, info);
+      if (rc) return rc;
+    }
     // P at F_start
-    if (!al_L0) {
+    if (!alt) {
       rc = fs_const(c, F, (const T*)nullptr, zero, 0.0, 0, (T*)nullptr, c.P, res);
       if (rc) return rc;
     }
-    bool newton_ok = al_L0 != nullptr;
-    for (int i_NW = 0; i_NW < (al_L0 ? 0 : max_NW); ++i_NW) {
+    bool newton_ok = alt;
+    for (int i_NW = 0; i_NW < (alt ? 0 : max_NW); ++i_NW) {
       info->newton_iters += 1;
       // rhs
       if (i_NW == 0) {
@@ -1838,6 +2290,61 @@ extern "C" int mgk_fs_solver_step_al(const int64_t* shape, int precision, void* 
     rc = fs_step_t<float>(shape, (float*)F, (const float*)Fp, (const float*)lam, (const float*)mu, mat, F_bar,
                           P_target, P_mask, i_drv, j_drv, E_avg, nu_avg, tol, 0.0, max_iter, tol_macro,
                           max_iter_macro, (float*)P_out, (float*)sig_out, (char*)workspace, s, info, nullptr, L0);
+  info->status = rc;
+  return rc;
+}
+
+extern "C" int mgk_fs_solver_step_dbfft(const int64_t* shape, int precision, void* F, const void* Fp,
+                                        const void* lam, const void* mu, const mgk_fs_material* mat,
+                                        const double* F_bar, const double* P_target, const int* P_mask, int i_drv,
+                                        int j_drv, double E_avg, double nu_avg, double tol, double tol_CG,
+                                        int max_iter, double tol_macro, int max_iter_macro, double pixel,
+                                        void* P_out, void* sig_out, void* workspace, size_t workspace_bytes,
+                                        void* stream, mgk_fs_info* info) {
+  if (!shape || !F || !lam || !mu || !mat || !F_bar || !P_target || !P_mask || !workspace || !info) {
+    mgk_set_error("mgk_fs_solver_step_dbfft: null argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  if (!(pixel > 0.0) || !std::isfinite(pixel)) {
+    mgk_set_error("mgk_fs_solver_step_dbfft: pixel must be positive", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  // the reference's stop test |b_hat| / |P| grows with sqrt(NV) times the
+  // rounding of P, which FP32 keeps above any useful tolerance
+  if (precision != 0) {
+    mgk_set_error("mgk_fs_solver_step_dbfft: FP64 only (precision = 0)", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  size_t need = 0;
+  int rc = mgk_fs_workspace_bytes(shape, precision, &need);
+  if (rc) return rc;
+  if (workspace_bytes < need) {
+    mgk_set_error("workspace too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  // wave vectors as _get_frequencies_3d: 2 pi fftfreq(n, d = L / n), L = n pixel,
+  // the Nyquist entry zeroed where it equals -pi n / L
+  DbPar dp{};
+  for (int a = 0; a < 3; ++a) {
+    const int n = (int)shape[a];
+    const double L = (double)n * pixel, dd = L / (double)n;
+    dp.val[a] = 1.0 / ((double)n * dd);
+    if (n % 2 == 0) {
+      const double xn = 6.283185307179586 * ((double)(-(n / 2)) * dp.val[a]);
+      if (!(xn == -3.141592653589793 * (double)n / L)) {
+        mgk_set_error("mgk_fs_solver_step_dbfft: for this pixel size the reference keeps the Nyquist wave vector "
+                      "(its spectra are then not those of real fields); not supported",
+                      __FILE__, __LINE__);
+        return MGK_ERR_ARG;
+      }
+    }
+  }
+  memset(info, 0, sizeof(*info));
+  hipStream_t s = (hipStream_t)stream;
+  rc = fs_step_t<double>(shape, (double*)F, (const double*)Fp, (const double*)lam, (const double*)mu, mat, F_bar,
+                         P_target, P_mask, i_drv, j_drv, E_avg, nu_avg, tol, tol_CG, max_iter, tol_macro,
+                         max_iter_macro, (double*)P_out, (double*)sig_out, (char*)workspace, s, info, nullptr, nullptr,
+                         &dp);
   info->status = rc;
   return rc;
 }

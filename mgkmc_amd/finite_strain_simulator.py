@@ -5,7 +5,7 @@ tensor-first (3, 3, nx, ny, nz), which is exactly the library's 9-component
 SoA layout, so no transposition happens at the seam.
 
 ``finite_strain_solver_step_3d`` keeps the reference signature
-(:2967-2982) for solver="newton_cg" and "al"; Ghat4, C4, I2, I4, I4rt are accepted for
+(:2967-2982) for solver="newton_cg", "al" and "dbfft"; Ghat4, C4, I2, I4, I4rt are accepted for
 compatibility and only lam = C4[0,0,1,1], mu = C4[0,1,0,1] are read (the
 projection and the tangent are matrix-free on the device).  K4 is returned as
 None, as no caller consumes it (mgkmc/aqs.py:289-303).
@@ -109,16 +109,24 @@ class FsSolver:
         self.info = _lib.MgkFsInfo()
 
     def step(self, F_bar, mat, driving_component=(0, 0), P_target=None, P_mask=None, E_avg=100e9, nu_avg=0.3,
-             tol_NW=1e-5, tol_CG=1e-6, max_NW=50, tol_macro=1e6, max_iter_macro=20, solver="newton_cg"):
+             tol_NW=1e-5, tol_CG=1e-6, max_NW=50, tol_macro=1e6, max_iter_macro=20, solver="newton_cg", pixel=1.0):
         """One load step with solver "newton_cg" (FP64, FP32 or the FP32-Krylov
-        form) or "al" (augmented Lagrangian, FP64 or FP32)."""
+        form), "al" (augmented Lagrangian, FP64 or FP32) or "dbfft"
+        (displacement-based FFT, FP64; ``pixel`` sets its wave vectors)."""
         Fb, Fbp = _lib.host_dbl(np.asarray(F_bar, dtype=np.float64).reshape(9))
         Pt, Ptp = _lib.host_dbl(np.zeros(9) if P_target is None else np.asarray(P_target, float).reshape(9))
         pm = np.zeros(9, dtype=np.int32) if P_mask is None else np.asarray(P_mask, bool).reshape(9).astype(np.int32)
         pm_c = (ctypes.c_int * 9)(*pm.tolist())
-        if solver not in ("newton_cg", "al"):
-            raise NotImplementedError(f"solver={solver!r}: the device path implements 'newton_cg' and 'al'")
-        if solver == "al":
+        if solver not in ("newton_cg", "al", "dbfft"):
+            raise NotImplementedError(f"solver={solver!r}: the device path implements 'newton_cg', 'al' and 'dbfft'")
+        if solver == "dbfft":
+            rc = _lib.load().mgk_fs_solver_step_dbfft(
+                self._shape_c, self.precision, _lib.dptr(self.F), _lib.dptr(self.Fp), _lib.dptr(self.lam),
+                _lib.dptr(self.mu), ctypes.byref(mat), Fbp, Ptp, pm_c, int(driving_component[0]),
+                int(driving_component[1]), float(E_avg), float(nu_avg), float(tol_NW), float(tol_CG), int(max_NW),
+                float(tol_macro), int(max_iter_macro), float(pixel), _lib.dptr(self.P), _lib.dptr(self.sig),
+                _lib.dptr(self.ws), self.ws.numel(), ctypes.c_void_p(dev.stream_ptr()), ctypes.byref(self.info))
+        elif solver == "al":
             rc = _lib.load().mgk_fs_solver_step_al(
                 self._shape_c, self.precision, _lib.dptr(self.F), _lib.dptr(self.Fp), _lib.dptr(self.lam),
                 _lib.dptr(self.mu), ctypes.byref(mat), Fbp, Ptp, pm_c, int(driving_component[0]),
@@ -153,9 +161,9 @@ def finite_strain_solver_step_3d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, dri
                                  g2=0.0, g3=0.0, g4=0.0, strain_capping_enabled=False, strain_capping_limit=None,
                                  strain_capping_tangent_ratio=0.1, strain_capping_type="piecewise",
                                  strain_capping_smooth_power=1.0):
-    """One finite-strain load step (solver "newton_cg" or "al") -> (F, P, Sig, None, F_bar)."""
-    if solver not in ("newton_cg", "al"):
-        raise NotImplementedError(f"solver={solver!r}: the device path implements solver='newton_cg' and 'al'")
+    """One finite-strain load step (solver "newton_cg", "al" or "dbfft") -> (F, P, Sig, None, F_bar)."""
+    if solver not in ("newton_cg", "al", "dbfft"):
+        raise NotImplementedError(f"solver={solver!r}: the device path implements 'newton_cg', 'al' and 'dbfft'")
     C4 = np.asarray(C4)
     lam = C4[0, 0, 1, 1]
     mu = C4[0, 1, 0, 1]
@@ -165,7 +173,7 @@ def finite_strain_solver_step_3d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, dri
                    strain_capping_limit, strain_capping_tangent_ratio, strain_capping_type,
                    strain_capping_smooth_power, mu_mean=float(np.mean(mu)))
     info = s.step(F_bar, mat, driving_component, P_target, P_mask, E_avg, nu_avg, tol_NW, tol_CG, max_NW, tol_macro,
-                  max_iter_macro, solver=solver)
+                  max_iter_macro, solver=solver, pixel=pixel)
     Fo = dev.download(s.F)
     Po = dev.download(s.P)
     So = dev.download(s.sig)
@@ -269,8 +277,8 @@ def finite_strain_simulation_3d(E, nu, driving_component, eps_target, n_steps, m
     Logs, checkpoints and VTK output of the reference are accepted and
     ignored.  Returns (F_macro_list, Sig_macro_list, P_macro_list, F_list,
     Sig_list); the field lists are filled only when store=True."""
-    if solver not in ("newton_cg", "al"):
-        raise NotImplementedError("the device path implements solver='newton_cg' and 'al'")
+    if solver not in ("newton_cg", "al", "dbfft"):
+        raise NotImplementedError("the device path implements solver='newton_cg', 'al' and 'dbfft'")
     mixed_targets = {} if mixed_targets is None else mixed_targets
     E = np.asarray(E, dtype=np.float64)
     nu = np.asarray(nu, dtype=np.float64)
@@ -287,7 +295,7 @@ def finite_strain_simulation_3d(E, nu, driving_component, eps_target, n_steps, m
         F_bar, F_mask, P_tgt, P_mask = build_finite_strain_bc_3d(driving_component, eps_s, mixed_targets,
                                                                  F_bar_initial=F_bar_init)
         info = s.step(F_bar, mat, driving_component, P_tgt, P_mask, E.mean(), nu.mean(), tol_NW, tol_CG, max_NW,
-                      tol_macro, max_iter_macro, solver=solver)
+                      tol_macro, max_iter_macro, solver=solver, pixel=pixel)
         F_bar_init = np.array(info.F_bar[:]).reshape(3, 3)
         Fm.append(np.array(info.F_mean[:]).reshape(3, 3))
         Sm.append(np.array(info.sig_mean[:]).reshape(3, 3))

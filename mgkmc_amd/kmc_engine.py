@@ -378,7 +378,7 @@ class DeviceKmc:
         tg = getattr(self, "stress_targets", {})
         Fb, _, Pt, Pm = build_finite_strain_bc_3d(drv, self.eps_macro[drv], tg, F_bar_initial=self.F_macro)
         info = self.fs.step(Fb, self._fs_mat, drv, Pt, Pm, E_avg=self.E_field.mean(), nu_avg=self.nu_field.mean(),
-                            solver=self._fs_solver)
+                            solver=self._fs_solver, pixel=self.pixel)
         self.F_macro = np.array(info.F_bar[:]).reshape(3, 3)
         for i in range(3):
             for j in range(3):

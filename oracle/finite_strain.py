@@ -14,7 +14,10 @@ Follows mgkmc/finite_strain_simulator.py:
                 capping, Fp, Mandel and einsum tangents)
   :1892-1904    cauchy_from_P_3d
   :2852-2899    build_finite_strain_bc_3d
-  :2967-3168    finite_strain_solver_step_3d, solver="newton_cg" branch
+  :2145-2242    _al_step_3d (augmented Lagrangian)
+  :1212-1379,1508-1620  _get_frequencies_3d, DBFFT helpers, solve_dbfft_linear_system_3d,
+                _dbfft_step_3d (displacement-based Newton with the (xi.<K4>.xi)^-1 preconditioner)
+  :2967-3168    finite_strain_solver_step_3d, solver="newton_cg" / "al" / "dbfft"
   :3175-3413    finite_strain_simulation_3d (load stepping, store=False)
 The Krylov solver is SciPy's own scipy.sparse.linalg.bicgstab (the reference's
 dependency, SciPy 1.15.3 here), called exactly as the reference calls it.
@@ -24,7 +27,11 @@ against the assembled tangent (no Fp), the Mandel assembly against the einsum
 ("reference") assembly (tests/test_landau_tangent_mandel.py of the reference),
 the SVK limit of Landau, projector identities of G (G:G = G, G annihilates
 compatible... fields with zero mean), the converged Newton solution
-(G:P = 0 to tolerance, <F> = F_bar), and a homogeneous closed form.
+(G:P = 0 to tolerance, <F> = F_bar), and a homogeneous closed form.  AL: the
+homogeneous closed form after two iterations and the reference's failure on
+heterogeneous grids.  DBFFT: the homogeneous closed form, the same equilibrium
+as the Newton-CG solution, an exactly compatible F - F_bar, and its operators
+in the real-field form the device uses (grad / div with K4 transposed).
 """
 import numpy as np
 import scipy.fft as _sfft
@@ -387,12 +394,153 @@ def al_step(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, tol=1e-5, max_iter=200, 
     raise ValueError(f"AL solver did not converge or exploded. Final relative residual: {res:.2e}")
 
 
+def frequencies(nx, ny, nz, pixel=1.0):
+    """_get_frequencies_3d (finite_strain_simulator.py:1212-1229) with L = n pixel:
+    2 pi fftfreq(n, L / n) per axis, entries equal to -pi n / L set to zero."""
+    ns = (nx, ny, nz)
+    Ls = tuple(n * pixel for n in ns)
+    X = list(np.meshgrid(*[2.0 * np.pi * np.fft.fftfreq(n, d=L / n) for n, L in zip(ns, Ls)], indexing="ij"))
+    for a in range(3):
+        if ns[a] % 2 == 0:
+            X[a][X[a] == -np.pi * ns[a] / Ls[a]] = 0.0
+    return np.stack(X, axis=0)
+
+
+def _fftn(A):
+    return _sfft.fftn(A, axes=(-3, -2, -1), workers=FFT_WORKERS)
+
+
+def _ifftn_real(A):
+    return _sfft.ifftn(A, axes=(-3, -2, -1), workers=FFT_WORKERS).real
+
+
+def inv3_field(A):
+    """_invert_matrix_field_3d (:1250-1276): adjugate / det, det -> 1 where
+    |det| < 1e-14, zero at k = 0."""
+    det = (A[0, 0] * (A[1, 1] * A[2, 2] - A[1, 2] * A[2, 1]) - A[0, 1] * (A[1, 0] * A[2, 2] - A[1, 2] * A[2, 0])
+           + A[0, 2] * (A[1, 0] * A[2, 1] - A[1, 1] * A[2, 0]))
+    d = np.where(np.abs(det) < 1e-14, 1.0, det)
+    M = np.empty_like(A)
+    M[0, 0] = (A[1, 1] * A[2, 2] - A[1, 2] * A[2, 1]) / d
+    M[0, 1] = -(A[0, 1] * A[2, 2] - A[0, 2] * A[2, 1]) / d
+    M[0, 2] = (A[0, 1] * A[1, 2] - A[0, 2] * A[1, 1]) / d
+    M[1, 0] = -(A[1, 0] * A[2, 2] - A[1, 2] * A[2, 0]) / d
+    M[1, 1] = (A[0, 0] * A[2, 2] - A[0, 2] * A[2, 0]) / d
+    M[1, 2] = -(A[0, 0] * A[1, 2] - A[0, 2] * A[1, 0]) / d
+    M[2, 0] = (A[1, 0] * A[2, 1] - A[1, 1] * A[2, 0]) / d
+    M[2, 1] = -(A[0, 0] * A[2, 1] - A[0, 1] * A[2, 0]) / d
+    M[2, 2] = (A[0, 0] * A[1, 1] - A[0, 1] * A[1, 0]) / d
+    M[:, :, 0, 0, 0] = 0.0
+    return M
+
+
+def u_from_F(F, F_bar, Xi):
+    """_reconstruct_u_from_F_3d (:1294-1306)."""
+    gh = _fftn(F - F_bar[:, :, None, None, None])
+    xi2 = np.sum(Xi ** 2, axis=0)
+    xi2 = np.where(xi2 < 1e-14, 1.0, xi2)
+    uh = -1j * np.einsum("jxyz,ijxyz->ixyz", Xi, gh) / xi2
+    uh[:, 0, 0, 0] = 0.0
+    return _ifftn_real(uh)
+
+
+def grad_u(u, Xi):
+    """get_grad_u_3d (:1315-1318): grad_ij = ifft(i xi_j u_hat_i).real."""
+    return _ifftn_real(1j * Xi[None] * _fftn(u)[:, None])
+
+
+def dbfft_operators(Xi, K4, Kavg):
+    """(A, M) of solve_dbfft_linear_system_3d (:1351-1379) on flattened full
+    complex spectra: A u_hat = i xi . fft(K4 : ifft(i xi u_hat).real), zero at
+    k = 0; M = (xi . Kavg . xi)^-1."""
+    sh = Xi.shape[1:]
+    Minv = inv3_field(np.einsum("jxyz,ijlk,lxyz->ikxyz", Xi, Kavg, Xi))
+
+    def A(uf):
+        uh = uf.reshape((3,) + sh)
+        g = _ifftn_real(1j * Xi[None] * uh[:, None])
+        dh = _fftn(_ddot42(K4, g))
+        out = 1j * np.einsum("jxyz,ijxyz->ixyz", Xi, dh)
+        out[:, 0, 0, 0] = 0.0
+        return out.reshape(-1)
+
+    def M(rf):
+        return np.einsum("ijxyz,jxyz->ixyz", Minv, rf.reshape((3,) + sh)).reshape(-1)
+
+    return A, M
+
+
+def dbfft_step(F, F_bar, Xi, C4, I2, I4, I4rt, Fp=None, tol=1e-5, tol_CG=1e-6, max_iter=20, model_type="svk",
+               stats=None, **mat):
+    """_dbfft_step_3d (:1508-1620): u from F, then Newton on the displacement:
+    F = F_bar + grad u; b = -i xi . fft(P); stop when |b| / (|P| + 1e-20) < tol
+    after the first iteration; BiCGSTAB (SciPy, M = (xi.<K4>.xi)^-1, maxiter
+    150) for du; line search alpha = 1 .. 2^-15 on det F_e > 1e-4 and a NaN-free
+    P, whose P is reused by the next iteration.  Returns (F, P, iterations)."""
+    n = F.shape[2:]
+    Fbg = np.einsum("ij,xyz->ijxyz", F_bar, np.ones(n))
+    u = u_from_F(F, F_bar, Xi)
+    Fpi = invert_Fp(Fp) if Fp is not None else None
+    carry = None
+    rel = np.nan
+    for i in range(max_iter):
+        Fc = Fbg + grad_u(u, Xi)
+        if carry is not None:
+            P, K4 = carry, None
+            carry = None
+        else:
+            P, K4 = constitutive(Fc, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, **mat)
+        bh = -1j * np.einsum("jxyz,ijxyz->ixyz", Xi, _fftn(P))
+        rel = np.linalg.norm(bh) / (np.linalg.norm(P) + 1e-20)
+        if rel < tol and i > 0:
+            return Fc, P, i + 1
+        if K4 is None:
+            P, K4 = constitutive(Fc, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, **mat)
+        A, M = dbfft_operators(Xi, K4, K4.mean(axis=(-3, -2, -1)))
+        nk = {"n": 0}
+
+        def cb(xk):
+            nk["n"] += 1
+
+        N = bh.size
+        duh, _ = sp.bicgstab(sp.LinearOperator((N, N), matvec=A, dtype="complex128"), bh.reshape(-1),
+                             M=sp.LinearOperator((N, N), matvec=M, dtype="complex128"), rtol=tol_CG, maxiter=150,
+                             callback=cb)
+        if stats is not None:
+            stats.setdefault("dbfft_krylov", []).append(nk["n"])
+        if np.any(np.isnan(duh)) or np.any(np.isinf(duh)):
+            raise ValueError("Linear solver returned NaN or Inf (BiCGSTAB breakdown).")
+        du = _ifftn_real(duh.reshape((3,) + n))
+        alpha = 1.0
+        for _ in range(16):
+            ut = u + alpha * du
+            Ft = Fbg + grad_u(ut, Xi)
+            Fe = np.einsum("ijxyz,jkxyz->ikxyz", Ft, Fpi) if Fpi is not None else Ft
+            Je = (Fe[0, 0] * (Fe[1, 1] * Fe[2, 2] - Fe[1, 2] * Fe[2, 1])
+                  - Fe[0, 1] * (Fe[1, 0] * Fe[2, 2] - Fe[1, 2] * Fe[2, 0])
+                  + Fe[0, 2] * (Fe[1, 0] * Fe[2, 1] - Fe[1, 1] * Fe[2, 0]))
+            if np.any(Je <= 1e-4) or np.any(np.isnan(Je)):
+                alpha *= 0.5
+                continue
+            Pt, _ = constitutive(Ft, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, need_tangent=False, **mat)
+            if np.any(np.isnan(Pt)):
+                alpha *= 0.5
+                continue
+            u = ut
+            carry = Pt
+            break
+        else:
+            raise ValueError("Constitutive solver failed to converge in line search (potential negative Jacobian/NaN).")
+    raise ValueError(f"DBFFT solver did not converge within {max_iter} iterations. Final relative residual: {rel:.2e}")
+
+
 def solver_step(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, driving_component=(0, 0), P_target=None, P_mask=None,
                 E_avg=100e9, nu_avg=0.3, tol_NW=1e-5, tol_CG=1e-6, max_NW=50, tol_macro=1e6, max_iter_macro=20,
-                model_type="svk", stats=None, solver="newton_cg", **mat):
+                model_type="svk", stats=None, solver="newton_cg", pixel=1.0, **mat):
     """finite_strain_solver_step_3d, newton_cg branch (:2967-3168), or its "al"
-    branch (al_step in the same macro loop).  ``stats`` (oracle extension)
-    collects Newton and BiCGSTAB iteration counts (AL: "al" iterations)."""
+    / "dbfft" branch (al_step / dbfft_step in the same macro loop).  ``stats``
+    (oracle extension) collects Newton and BiCGSTAB iteration counts (AL: "al"
+    iterations; DBFFT: "dbfft" iterations and "dbfft_krylov" counts)."""
     nx, ny, nz = F.shape[2:]
     P_target = np.zeros((3, 3)) if P_target is None else P_target
     P_mask = np.zeros((3, 3), bool) if P_mask is None else P_mask
@@ -413,6 +561,13 @@ def solver_step(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, driving_component=(0
             if stats is not None:
                 stats.setdefault("al", []).append(nit)
             cur = {"P": P_al, "K4": None}
+        elif solver == "dbfft":
+            F_curr, P_db, nit = dbfft_step(F_start.copy(), Fbc, frequencies(nx, ny, nz, pixel), C4, I2, I4, I4rt,
+                                           Fp=Fp, tol=tol_NW, tol_CG=tol_CG, max_iter=max_NW, model_type=model_type,
+                                           stats=stats, **mat)
+            if stats is not None:
+                stats.setdefault("dbfft", []).append(nit)
+            cur = {"P": P_db, "K4": None}
         else:
             F_curr, cur = _newton(F_start, Fbc, Ghat4, C4, I2, I4, I4rt, Fp, tol_NW, tol_CG, max_NW, model_type,
                                   stats, mat)

@@ -187,3 +187,84 @@ def test_al_step_matches_oracle(gpu):
     for fn in (fs.solver_step, finite_strain_solver_step_3d):
         with pytest.raises(ValueError, match="AL solver did not converge or exploded"):
             fn(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="al")
+
+
+def _db_pair(sh, seed, model, Fbar, F0=None, Fp=None, P_target=None, P_mask=None, drv=(0, 1), tol=1e-8,
+             pixel=1.0, precision=0, kw=None):
+    from mgkmc_amd import device as dev
+    from mgkmc_amd.finite_strain_simulator import FsSolver, material
+    rng, E, nu, I2, I4, I4rt, C4, Fr = _grid(sh, seed)
+    F0 = I2.copy() if F0 is None else (Fr if F0 == "random" else F0)
+    G = fs.build_ghat4(*sh, *sh)
+    args = dict(kw or {})
+    if model == "landau":
+        args.update(LANDAU)
+    st = {}
+    ref = fs.solver_step(F0.copy(), Fbar, G, C4, I2, I4, I4rt, Fp=Fp, driving_component=drv, P_target=P_target,
+                         P_mask=P_mask, E_avg=E.mean(), nu_avg=nu.mean(), tol_NW=tol, tol_CG=tol, model_type=model,
+                         stats=st, solver="dbfft", pixel=pixel, **args)
+    lam, mu = C4[0, 0, 1, 1], C4[0, 1, 0, 1]
+    s = FsSolver(lam, mu, precision=precision, Fp=Fp)
+    s.F.copy_(dev.upload(np.ascontiguousarray(F0), precision))
+    mat = material(model, args.get("A_m", 0.0), args.get("B_m", 0.0), args.get("C_m", 0.0),
+                   *[args.get(k, 0.0) for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4")],
+                   args.get("strain_capping_enabled", False), args.get("strain_capping_limit"),
+                   args.get("strain_capping_tangent_ratio", 0.1), args.get("strain_capping_type", "piecewise"),
+                   args.get("strain_capping_smooth_power", 1.0), mu_mean=float(np.mean(mu)))
+    inf = s.step(Fbar, mat, drv, P_target, P_mask, E.mean(), nu.mean(), tol, tol, 50, 1e6, 20, solver="dbfft",
+                 pixel=pixel)
+    got = (s.F.double().cpu().numpy(), s.P.double().cpu().numpy(), np.array(inf.F_bar[:]).reshape(3, 3))
+    return ref, got, st, inf
+
+
+@pytest.mark.parametrize("sh,model,F0,fp,pixel", [
+    ((6, 4, 5), "svk", None, False, 1.0), ((5, 3, 9), "landau", None, False, 1.0),
+    ((8, 8, 8), "svk", "random", True, 1.0), ((16, 12, 10), "landau", "random", False, 0.5),
+    ((64, 8, 64), "svk", None, False, 1.0)])
+def test_dbfft_step_parity(gpu, sh, model, F0, fp, pixel):
+    # solver="dbfft": F, P and the Newton / Krylov iteration counts of the
+    # oracle (the reference's complex-spectrum algorithm with SciPy's BiCGSTAB)
+    Fbar = np.eye(3)
+    Fbar[0, 1] = 0.02
+    Fbar[2, 2] = 0.997
+    Fp = None
+    if fp:
+        rng = np.random.default_rng(8)
+        Fp = fs.identity_tensors(*sh)[0] + 0.01 * rng.standard_normal((3, 3) + sh)
+    print(f"[dbfft] shape={sh} model={model} F0={F0} Fp={fp} pixel={pixel}", flush=True)
+    ref, got, st, inf = _db_pair(sh, 3, model, Fbar, F0=F0, Fp=Fp, pixel=pixel)
+    eF, eP = _rel(got[0], ref[0]), _rel(got[1], ref[1])
+    print(f"[dbfft] oracle newton={st['dbfft']} krylov={st['dbfft_krylov']}; device newton={inf.newton_iters} "
+          f"krylov={inf.krylov_iters}; relF={eF:.2e} relP={eP:.2e}", flush=True)
+    assert inf.newton_iters == sum(st["dbfft"]) and inf.krylov_iters == sum(st["dbfft_krylov"])
+    assert eF < 1e-12 and eP < 1e-9
+
+
+def test_dbfft_mixed_bc_and_fp64_only(gpu):
+    sh = (8, 6, 8)
+    Fbar = np.eye(3)
+    Fbar[0, 0] = 1.01
+    Pt = np.zeros((3, 3))
+    Pm = np.zeros((3, 3), bool)
+    Pm[1, 1] = Pm[2, 2] = True
+    ref, got, st, inf = _db_pair(sh, 4, "svk", Fbar, P_target=Pt, P_mask=Pm, drv=(0, 0), tol=1e-6)
+    print(f"[dbfft mixed] Fbar ref={np.diag(ref[4])} got={np.diag(got[2])} newton={st['dbfft']} "
+          f"device={inf.newton_iters} macro={inf.macro_iters}", flush=True)
+    assert inf.newton_iters == sum(st["dbfft"])
+    assert np.abs(got[2] - ref[4]).max() < 1e-12
+    assert _rel(got[0], ref[0]) < 1e-12
+    with pytest.raises(RuntimeError, match="FP64 only"):
+        _db_pair((8, 8, 8), 5, "svk", np.eye(3), precision=1)
+
+
+def test_dbfft_rejects_pixel_sizes_the_reference_mishandles(gpu):
+    # for n = 10 and pixel = 0.3 numpy's Nyquist entry of 2 pi fftfreq differs
+    # from -pi n / L in the last bit, so the reference keeps it (non-real spectra)
+    xi = 2.0 * np.pi * np.fft.fftfreq(10, d=10 * 0.3 / 10)
+    assert not np.any(xi == -np.pi * 10 / (10 * 0.3))
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_3d
+    sh = (10, 5, 5)
+    I2, I4, I4rt, I4s, II = fs.identity_tensors(*sh)
+    C4 = fs.build_C4(np.full(sh, 70e9), np.full(sh, 0.3), I4s, II)
+    with pytest.raises(RuntimeError, match="Nyquist"):
+        finite_strain_solver_step_3d(I2.copy(), np.eye(3), None, C4, I2, I4, I4rt, solver="dbfft", pixel=0.3)

@@ -219,3 +219,84 @@ def test_al_step_closed_form_and_heterogeneous_failure():
     C4 = fs.build_C4(70e9 * rng.uniform(0.85, 1.15, sh), nu, I4s, II)
     with pytest.raises(ValueError, match="AL solver did not converge or exploded"):
         fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="al")
+
+
+def _db_grid(sh, seed=3, spread=0.2):
+    rng = np.random.default_rng(seed)
+    E = 70e9 * rng.uniform(1 - spread, 1 + spread, sh)
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = fs.identity_tensors(*sh)
+    return I2, I4, I4rt, fs.build_C4(E, nu, I4s, II)
+
+
+def test_dbfft_homogeneous_closed_form():
+    sh = (6, 4, 5)
+    I2, I4, I4rt, C4 = _db_grid(sh, spread=0.0)
+    G = fs.build_ghat4(*sh, *sh)
+    Fbar = np.eye(3)
+    Fbar[0, 1] = 0.02
+    Fbar[2, 2] = 0.99
+    st = {}
+    F, P, _, _, _ = fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="dbfft", stats=st)
+    # u = 0, b = 0 (BiCGSTAB returns du = 0), accepted at alpha = 1, converged at the second check
+    assert st["dbfft"] == [2]
+    assert np.abs(F - Fbar[:, :, None, None, None]).max() < 1e-15
+    lam, mu = 70e9 * 0.3 / (1.3 * 0.4), 70e9 / 2.6
+    Eg = 0.5 * (Fbar.T @ Fbar - np.eye(3))
+    assert np.allclose(P[:, :, 1, 2, 3], Fbar @ (lam * np.trace(Eg) * np.eye(3) + 2 * mu * Eg), rtol=1e-12)
+
+
+@pytest.mark.parametrize("sh", [(5, 3, 7), (7, 5, 5)])
+def test_dbfft_reaches_the_newton_equilibrium(sh):
+    # on odd grids (no Nyquist planes) the displacement discretisation and the
+    # de Geus projection describe the same discrete problem: both solvers
+    # must end at one equilibrium, and DBFFT's F - F_bar is exactly compatible
+    I2, I4, I4rt, C4 = _db_grid(sh)
+    G = fs.build_ghat4(*sh, *sh)
+    Fbar = np.eye(3)
+    Fbar[0, 1] = 0.02
+    Fbar[1, 1] = 0.995
+    st = {}
+    Fd, Pd, _, _, _ = fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="dbfft", tol_NW=1e-9, tol_CG=1e-9,
+                                     stats=st)
+    Fn, Pn, _, _, _ = fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="newton_cg", tol_NW=1e-9,
+                                     tol_CG=1e-9)
+    assert len(st["dbfft_krylov"]) == st["dbfft"][0] - 1
+    assert np.abs(Fd - Fn).max() < 1e-11
+    assert np.abs(Pd - Pn).max() < 1e-9 * np.abs(Pn).max()
+    dF = Fd - Fbar[:, :, None, None, None]
+    assert np.abs(fs.project(dF, G) - dF).max() < 1e-14
+    assert np.allclose(Fd.mean(axis=(2, 3, 4)), Fbar, atol=1e-15)
+
+
+def test_dbfft_operators_in_real_field_form():
+    # The device keeps the DBFFT Krylov vectors as real fields: A w =
+    # div(D(grad w)) with D(g) = (N(g^T))^T, N the Newton tangent action
+    # (K4 : X^T)^T, and M as a real-to-real filter.  Against the reference's
+    # complex operators on an even grid (Nyquist entries of xi zeroed):
+    # A fft(w) = fft(A_real w), ifft(A fft(w)) and ifft(M fft(r)) are real, and
+    # the full-spectrum vdot is NV times the real dot.
+    sh = (6, 4, 5)
+    I2, I4, I4rt, C4 = _db_grid(sh)
+    rng = np.random.default_rng(11)
+    F = I2 + 0.02 * rng.standard_normal((3, 3) + sh)
+    P, K4 = fs.constitutive(F, C4, I2, I4, I4rt)
+    Xi = fs.frequencies(*sh)
+    A, M = fs.dbfft_operators(Xi, K4, K4.mean(axis=(-3, -2, -1)))
+    w = rng.standard_normal((3,) + sh)
+    r = rng.standard_normal((3,) + sh)
+    fft = lambda a: np.fft.fftn(a, axes=(-3, -2, -1))
+    ifft = lambda a: np.fft.ifftn(a, axes=(-3, -2, -1))
+    g = ifft(1j * Xi[None] * fft(w)[:, None]).real            # grad_ij = d_j w_i
+    N = lambda X: fs._trans2(fs._ddot42(K4, fs._trans2(X)))   # Newton tangent action
+    D = fs._trans2(N(fs._trans2(g)))
+    Areal = ifft(1j * np.einsum("jxyz,ijxyz->ixyz", Xi, fft(D))).real
+    Aw = A(fft(w).reshape(-1)).reshape((3,) + sh)
+    assert np.abs(Aw - fft(Areal)).max() < 1e-12 * np.abs(Aw).max()
+    assert np.abs(ifft(Aw).imag).max() < 1e-12 * np.abs(Areal).max()
+    Mr = ifft(M(fft(r).reshape(-1)).reshape((3,) + sh))
+    assert np.abs(Mr.imag).max() < 1e-12 * np.abs(Mr.real).max()
+    assert np.isclose(np.vdot(fft(w), fft(r)).real, w.size // 3 * np.sum(w * r), rtol=1e-12)
+    # the transposes matter: the Newton action on grad w itself is another operator
+    Awrong = ifft(1j * np.einsum("jxyz,ijxyz->ixyz", Xi, fft(N(g)))).real
+    assert np.abs(fft(Awrong) - Aw).max() > 1e-3 * np.abs(Aw).max()
