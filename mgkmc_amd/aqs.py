@@ -7,7 +7,14 @@ on the host in the reference's order).  Extensions (keyword-only in use):
 ``ls_precision`` ("fp64", "fp32", "mixed" = FP32 + FP64 refinement),
 ``ls_warm_start``, ``rng`` / ``selection_stream`` (isolated random streams),
 ``init_modes=("device", seed)`` (initial modes drawn on the GPU).
+
+Checkpoints (save_checkpoint / load_checkpoint, mgkmc/aqs.py:1207-1323) hold
+the reference's groups, attributes and datasets under their HDF5 paths as keys
+("metadata/nx", "fields/eps_field", ...) of a NumPy .npz container written
+under the reference's file name, since h5py is not part of this stack.
 """
+from datetime import datetime
+
 import numpy as np
 
 from .kmc_engine import (DeviceKmc, LS_PRECISIONS, gaussian_generator, planar_catalog,  # noqa: F401
@@ -57,6 +64,10 @@ class ThermalSimulation(DeviceKmc):
                     scale_rate_by_volume, "plane_strain", ls_precision, ls_warm_start, rng, selection_stream,
                     init_modes, dict(mean=2.0, std=0.6), law=law, finite=finite)
         self.max_cascade_steps_pct = max_cascade_steps_pct
+        # thermal parameters are kept (and checkpointed) whether or not the extension is on
+        self.Cp, self.rho, self.thermal_diffusivity = Cp, rho, thermal_diffusivity
+        self.thermal_coords, self.temperature_cap = thermal_coords, temperature_cap
+        self.thermostat, self.tau_bath = thermostat, tau_bath
         if enable_thermal:
             self._setup_thermal(Cp, rho, thermal_diffusivity, thermal_coords, temperature_cap, thermostat, tau_bath)
 
@@ -95,9 +106,10 @@ class ThermalSimulation(DeviceKmc):
     def run_simulation(self, n_global_steps, step_size, component=(0, 1), stress_targets={}, mixed_tol=1e-4,
                        mixed_max_iter=50, max_kmc_steps_pct=0.3, max_cascade_steps_pct=0.3, max_events=None,
                        **kwargs):
-        """KMC + elastic increments (mgkmc/aqs.py:682-1150).  Logging,
-        checkpoint, VTK and stress-drop options of the reference are accepted
-        and ignored.  In cascade_mode the initial relaxation is a cascade (no
+        """KMC + elastic increments (mgkmc/aqs.py:682-1150).  Checkpoints
+        follow the reference's checkpoint_interval ("current", an int of
+        global steps, "last"), checkpoint_path and checkpoint_elastic_only;
+        logging, VTK and stress-drop options are accepted and ignored.  In cascade_mode the initial relaxation is a cascade (no
         solve before it) and every inner pass first runs a cascade while any
         mode is unstable; a truncated cascade ends the run.  Cascade flips go
         to ``cascade_history``.  ``max_events`` (extension) stops after that
@@ -119,6 +131,15 @@ class ThermalSimulation(DeviceKmc):
                 return self.flip_event_history
         else:
             self.update_stresses()
+        cp_int = kwargs.get("checkpoint_interval")
+        cp_path = kwargs.get("checkpoint_path", "checkpoint")
+        cp_elastic = kwargs.get("checkpoint_elastic_only", False)
+        cp_id = 0
+        if cp_int is not None and cp_int not in ("none", "last"):
+            name = f"{cp_path}.h5" if cp_int == "current" else \
+                (f"{cp_path}_elastic_{cp_id:06d}.h5" if cp_elastic else f"{cp_path}_000000.h5")
+            cp_id += int(bool(cp_elastic))
+            self.save_checkpoint(name, step=0)
         dt_step = abs(step_size) / self.strain_rate if self.strain_rate > 0 else 1.0
         max_seq = int(max_kmc_steps_pct * self.total_voxels)
         seq = 0
@@ -146,6 +167,82 @@ class ThermalSimulation(DeviceKmc):
             # finite strain: the step itself enforces the stress targets (mgkmc/aqs.py:1089-1102)
             if not self.finite and self.mixed_bc(targets, component, mixed_tol, mixed_max_iter, step) is None:
                 return self.flip_event_history
+            if cp_int is not None and cp_int not in ("none", "last"):
+                if cp_int == "current":
+                    self.save_checkpoint(f"{cp_path}.h5", step=step)
+                elif isinstance(cp_int, int) and step % cp_int == 0:
+                    self.save_checkpoint(f"{cp_path}_elastic_{cp_id:06d}.h5" if cp_elastic else
+                                         f"{cp_path}_{step:06d}.h5", step=step)
+                    cp_id += int(bool(cp_elastic))
             done += 1
             step += 1
+        if cp_int == "last" or (isinstance(cp_int, int) and (step - 1) % cp_int != 0):
+            self.save_checkpoint(f"{cp_path}_final.h5", step=step - 1)
         return self.flip_event_history
+
+    _CKPT_META = ("nu0", "q_act_temp", "temperature", "strain_rate", "stability_threshold", "jp", "jt",
+                  "neighbor_softening_fraction", "softening_cap", "softening_scheme", "scale_rate_by_volume",
+                  "enable_thermal", "Cp", "thermostat", "tau_bath", "rho", "thermal_diffusivity", "thermal_coords",
+                  "temperature_cap")
+
+    def save_checkpoint(self, path, step=None):
+        """The reference's checkpoint content (mgkmc/aqs.py:1207-1262): metadata
+        attributes, fields (eps, sig, E, nu, Tlocal), grid (eps_plastic,
+        soft_prop, Q, Q0, catalog, last_event_time), state/eps_macro and
+        history/flips (the cascade flips, (step, batch, x, y, z, m), int32)."""
+        if not path.endswith(".h5"):
+            path += ".h5"
+        d = {}
+        nx, ny, nz = self.grid_shape
+        meta = dict(nx=nx, ny=ny, nz=nz, M=self.M, gamma0=self.gamma0, pixel=self.pixel,
+                    timestamp=datetime.now().isoformat(), step=step if step is not None else 0, sim_time=self.time)
+        meta.update({k: getattr(self, k) for k in self._CKPT_META})
+        for k, v in meta.items():
+            if v is not None:
+                d["metadata/" + k] = np.asarray(v)
+        T = self.Tlocal if self.enable_thermal else np.full(self.grid_shape, float(self.temperature))
+        d.update({"fields/eps_field": self.eps_field, "fields/sig_field": self.sig_field,
+                  "fields/E_field": self.E_field, "fields/nu_field": self.nu_field, "fields/Tlocal": T,
+                  "grid/eps_plastic": self.eps_plastic, "grid/soft_prop": self.soft_prop, "grid/Q": self.Q,
+                  "grid/Q0": self.Q0, "grid/catalog": self.catalog, "grid/last_event_time": self.last_event_time,
+                  "state/eps_macro": np.asarray(self.eps_macro)})
+        if len(self.cascade_history) > 0:
+            d["history/flips"] = np.array(self.cascade_history, dtype=np.int32)
+        with open(path, "wb") as f:
+            np.savez_compressed(f, **d)
+
+    @classmethod
+    def load_checkpoint(cls, path):
+        """A new simulation from a checkpoint (mgkmc/aqs.py:1264-1323): built
+        from the stored metadata and material fields (drawing fresh modes as
+        the reference's constructor does), then the stored state uploaded."""
+        with np.load(path, allow_pickle=False) as f:
+            m = {k.split("/", 1)[1]: f[k][()] for k in f.files if k.startswith("metadata/")}
+            sim = cls(int(m["nx"]), int(m["ny"]), int(m["nz"]), M=int(m["M"]), gamma0=float(m["gamma0"]),
+                      E_field=f["fields/E_field"], nu_field=f["fields/nu_field"], pixel=float(m["pixel"]),
+                      temperature=float(m["temperature"]), strain_rate=float(m["strain_rate"]), jp=float(m["jp"]),
+                      jt=float(m["jt"]), neighbor_softening_fraction=float(m.get("neighbor_softening_fraction", 0.0)),
+                      softening_cap=float(m["softening_cap"]), softening_scheme=str(m["softening_scheme"]),
+                      nu0=float(m.get("nu0", 1e13)), q_act_temp=float(m.get("q_act_temp", 0.37)),
+                      scale_rate_by_volume=bool(m.get("scale_rate_by_volume", True)),
+                      enable_thermal=bool(m.get("enable_thermal", False)), Cp=float(m.get("Cp", 420.0)),
+                      rho=float(m.get("rho", 6125.0)), thermal_diffusivity=float(m.get("thermal_diffusivity", 3.0e-6)),
+                      thermal_coords=str(m.get("thermal_coords", "pixel")),
+                      temperature_cap=float(m.get("temperature_cap", 1000.0)),
+                      thermostat=bool(m.get("thermostat", False)), tau_bath=float(m.get("tau_bath", 0.0)))
+            sim.time = float(m.get("sim_time", 0.0))
+            sim.eps_field = f["fields/eps_field"]
+            sim.sig_field = f["fields/sig_field"]
+            if sim.enable_thermal and "fields/Tlocal" in f.files:
+                sim.Tlocal = f["fields/Tlocal"]
+            sim.eps_plastic = f["grid/eps_plastic"]
+            sim.soft_prop = f["grid/soft_prop"]
+            sim.Q0 = f["grid/Q0"]
+            sim.catalog = f["grid/catalog"]
+            sim.last_event_time = f["grid/last_event_time"]
+            if "state/eps_macro" in f.files:
+                sim.eps_macro = np.array(f["state/eps_macro"])
+            if "history/flips" in f.files:
+                sim.cascade_history = [tuple(r) for r in f["history/flips"].tolist()]
+        print(f"Loaded checkpoint from {path} (SimTime={sim.time:.4e}s, Step={m['step']})")
+        return sim

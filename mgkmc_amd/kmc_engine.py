@@ -36,7 +36,7 @@ from . import device as dev
 from .kmc_device import KmcState, params as kmc_params, SIG_SYM64, SIG_SYM32, modes_from_device
 from .linear_elastic_simulator import (LsSolver, FP64, FP32, MIXED, LandauSolver, LandauSolver2D, SecantSolver,
                                        SECANT_EXP, SECANT_ROSE, landau_params)
-from .kmc_device import SIG_GEN64
+from .kmc_device import SIG_GEN64, CAT_PLANAR, CAT_GENERAL, is_planar, catalog_to_device, modes_to_device
 from .elasticity import compute_lame_2d, compute_lame_3d
 from .stz import kmc as _kmcmod
 
@@ -243,6 +243,12 @@ class DeviceKmc:
     def Tlocal(self):
         return dev.download(self.Tfield) if self.enable_thermal else None
 
+    @Tlocal.setter
+    def Tlocal(self, value):
+        if not self.enable_thermal:
+            raise ValueError("Tlocal is only stored with enable_thermal=True")
+        self.Tfield.copy_(dev.upload(np.array(np.broadcast_to(np.asarray(value, dtype=np.float64), self.grid_shape)), 0))
+
     def heat_conducting(self, dt):
         """heat_conducting_3d / _2d (mgkmc/aqs.py:462-478, kmc_simulator.py:559-575)
         on the device: spectral diffusion over dt, thermostat, cap."""
@@ -285,12 +291,28 @@ class DeviceKmc:
             return np.einsum("ijxyz->xyzij", dev.download(self.fs.sig))
         return dev.from_sym_soa(dev.download(self._stress.sig), self.ndim)
 
+    @sig_field.setter
+    def sig_field(self, value):
+        self._set_sym_field(self._stress.sig, value)
+
+    def _set_sym_field(self, t, value):
+        """Store a host (*grid, d, d) field into a small-strain solver's
+        symmetric-component array (its symmetric part)."""
+        if self.finite:
+            raise NotImplementedError("finite-strain states are set through F_field / F_plastic")
+        sym, _ = dev.to_sym_soa(np.asarray(value, dtype=np.float64), self.ndim)
+        t.copy_(dev.upload(sym, FP64).to(t.dtype).reshape(t.shape))
+
     @property
     def eps_field(self):
         if self.finite:  # Green-Lagrange strain of F, as the reference stores it
             F = np.einsum("ijxyz->xyzij", dev.download(self.fs.F))
             return 0.5 * (np.einsum("...ki,...kj->...ij", F, F) - np.eye(3))
         return dev.from_sym_soa(dev.download(self._stress.eps), self.ndim)
+
+    @eps_field.setter
+    def eps_field(self, value):
+        self._set_sym_field(self._stress.eps, value)
 
     @property
     def F_field(self):
@@ -324,6 +346,38 @@ class DeviceKmc:
     def Q0(self):
         return modes_from_device(self.kmc.Q0, self.grid_shape)
 
+    @Q0.setter
+    def Q0(self, value):
+        self.kmc.Q0.copy_(modes_to_device(value))
+
+    @property
+    def catalog(self):
+        """(*grid, M, d, d) strain modes of every voxel."""
+        d = self.ndim
+        c = dev.download(self.kmc.cat).transpose(2, 1, 0)  # (NV, M, CK)
+        full = planar_to_full(c, d) if self.kmc.planar else c.reshape(c.shape[:2] + (d, d))
+        return np.ascontiguousarray(full).reshape(self.grid_shape + (self.M, d, d))
+
+    @catalog.setter
+    def catalog(self, value):
+        d = self.ndim
+        planar = is_planar(value, d)
+        if planar == self.kmc.planar:
+            self.kmc.cat.copy_(catalog_to_device(value, d, planar))
+        else:
+            self.kmc.cat = catalog_to_device(value, d, planar)
+            self.kmc.planar = planar
+            self.kmc.cat_kind = CAT_PLANAR if planar else CAT_GENERAL
+            self.kmc.CK = 2 if planar else d * d
+
+    @property
+    def last_event_time(self):
+        return dev.download(self.kmc.tlast).reshape(self.grid_shape)
+
+    @last_event_time.setter
+    def last_event_time(self, value):
+        self.kmc.tlast.copy_(dev.upload(np.asarray(value, dtype=np.float64).reshape(-1), 0))
+
     @property
     def soft_prop(self):
         s = dev.download(self.kmc.soft)
@@ -331,6 +385,12 @@ class DeviceKmc:
         out[..., 0] = s[0].reshape(self.grid_shape)
         out[..., 1] = s[1].reshape(self.grid_shape)
         return out
+
+    @soft_prop.setter
+    def soft_prop(self, value):
+        """g_p, g_t from columns 0 and 1 (the others are unused)."""
+        v = np.asarray(value, dtype=np.float64).reshape(self.total_voxels, -1)
+        self.kmc.soft.copy_(dev.upload(np.ascontiguousarray(v[:, :2].T), 0))
 
     def _law_solver(self, law):
         """Device solver of a nonlinear small-strain law (the reference's

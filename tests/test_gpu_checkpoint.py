@@ -1,0 +1,67 @@
+"""Checkpoint round trip of the device 3D simulator (mgkmc/aqs.py:1207-1323)
+and the reference's checkpoint tests (tests/test_checkpoint.py,
+tests/test_thermal.py:83-115), plus an exact resume: with the random streams
+restored, a run continued from the loaded state repeats the original's."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim(**kw):
+    from mgkmc_amd.aqs import ThermalSimulation
+    nx, ny, nz = 8, 8, 4
+    args = dict(M=6, gamma0=0.14, E_field=np.full((nx, ny, nz), 70.0), nu_field=np.full((nx, ny, nz), 0.3),
+                pixel=0.7, jp=11, jt=33, strain_rate=1.0e7, temperature=300.0)
+    args.update(kw)
+    return ThermalSimulation(nx, ny, nz, **args)
+
+
+def test_checkpoint_round_trip_and_resume(gpu, tmp_path):
+    from mgkmc_amd.aqs import ThermalSimulation
+    from mgkmc_amd.stz import kmc as K
+    np.random.seed(42)
+    K.seed_selection(7)
+    run = dict(step_size=0.001, component=(0, 0), stress_targets={(1, 1): 0.0, (2, 2): 0.0})
+    sim1 = _sim(enable_thermal=True, temperature_cap=1200.0, strain_rate=1.0e4)
+    sim1.run_simulation(n_global_steps=2, max_events=12, **run)
+    n1 = len(sim1.flip_event_history)
+    path = str(tmp_path / "cp")
+    sim1.save_checkpoint(path, step=42)
+    S, T = np.random.get_state(), K.SELECTION_STREAM.get_state()
+    sim2 = ThermalSimulation.load_checkpoint(path + ".h5")
+    # tests/test_checkpoint.py: the mean stress survives; tests/test_thermal.py: thermal metadata and T
+    assert abs(sim1.sig_field.mean(axis=(0, 1, 2))[0, 0] - sim2.sig_field.mean(axis=(0, 1, 2))[0, 0]) < 1e-5
+    for k in ("enable_thermal", "Cp", "rho", "thermal_diffusivity", "thermal_coords", "temperature_cap", "time"):
+        assert getattr(sim2, k) == getattr(sim1, k), k
+    for k in ("Tlocal", "eps_field", "sig_field", "eps_plastic", "Q0", "catalog", "last_event_time", "Q"):
+        assert np.array_equal(getattr(sim2, k), getattr(sim1, k)), k
+    assert np.array_equal(sim2.soft_prop[..., :2], sim1.soft_prop[..., :2])
+    assert np.array_equal(sim2.eps_macro, sim1.eps_macro)
+    # resume: same streams, same continuation
+    np.random.set_state(S)
+    K.SELECTION_STREAM.set_state(T)
+    sim1.run_simulation(n_global_steps=2, max_events=10, **run)
+    np.random.set_state(S)
+    K.SELECTION_STREAM.set_state(T)
+    sim2.run_simulation(n_global_steps=2, max_events=10, **run)
+    tail = sim1.flip_event_history[n1:]
+    print(f"[ckpt] events before {n1}, after {len(tail)}", flush=True)
+    assert n1 > 0 and len(tail) > 0 and tail == sim2.flip_event_history
+    for k in ("eps_plastic", "sig_field", "Tlocal", "soft_prop", "Q0", "catalog"):
+        assert np.array_equal(getattr(sim2, k), getattr(sim1, k)), k
+
+
+def test_checkpoint_intervals_of_run_simulation(gpu, tmp_path):
+    # athermal, small increments: no events, the global step counts the increments
+    np.random.seed(3)
+    sim = _sim(temperature=0.0)
+    base = str(tmp_path / "run")
+    sim.run_simulation(n_global_steps=3, step_size=0.0005, component=(0, 0), checkpoint_interval=2,
+                       checkpoint_path=base)
+    assert len(sim.flip_event_history) == 0
+    names = sorted(p.name for p in tmp_path.iterdir())
+    # step 0, every global step divisible by 2, and the final step when not already saved
+    assert names == ["run_000000.h5", "run_000002.h5", "run_final.h5"], names
+    with np.load(tmp_path / "run_final.h5", allow_pickle=False) as f:
+        assert int(f["metadata/step"]) == 3 and f["fields/eps_field"].shape == (8, 8, 4, 3, 3)

@@ -268,3 +268,30 @@ def test_dbfft_rejects_pixel_sizes_the_reference_mishandles(gpu):
     C4 = fs.build_C4(np.full(sh, 70e9), np.full(sh, 0.3), I4s, II)
     with pytest.raises(RuntimeError, match="Nyquist"):
         finite_strain_solver_step_3d(I2.copy(), np.eye(3), None, C4, I2, I4, I4rt, solver="dbfft", pixel=0.3)
+
+
+def test_dbfft_vs_newton_cg_reference_case(gpu):
+    # tests/test_finite_strain_kmc.py:368-417 of the reference on the device:
+    # both device solvers equal the oracle's, and differ from each other by
+    # less than the reference's bounds (F 1e-5, P 1e-2)
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_3d
+    sh = (8, 8, 8)
+    E = np.full(sh, 70.0e9)
+    E[2:6, 2:6, 2:6] = 68.0e9
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = fs.identity_tensors(*sh)
+    C4 = fs.build_C4(E, nu, I4s, II)
+    G = fs.build_ghat4(*sh, *sh)
+    Fbar = np.diag([1.0005, 0.9995, 1.0])
+    kw = dict(E_avg=E.mean(), nu_avg=nu.mean(), tol_NW=1e-8, tol_CG=1e-9, max_NW=30)
+    out = {}
+    for solver in ("newton_cg", "dbfft"):
+        ref = fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver=solver, **kw)
+        got = finite_strain_solver_step_3d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver=solver, **kw)
+        assert _rel(got[0], ref[0]) < 1e-12 and _rel(got[1], ref[1]) < 1e-9
+        out[solver] = got
+    Fn, Pn, Fd, Pd = out["newton_cg"][0], out["newton_cg"][1], out["dbfft"][0], out["dbfft"][1]
+    dF = np.linalg.norm(Fd - Fn) / np.linalg.norm(Fn)
+    dP = np.linalg.norm(Pd - Pn) / np.linalg.norm(Pn)
+    print(f"[dbfft vs ncg] F {dF:.3e} P {dP:.3e}", flush=True)
+    assert dF < 1e-5 and dP < 1e-2

@@ -300,3 +300,27 @@ def test_dbfft_operators_in_real_field_form():
     # the transposes matter: the Newton action on grad w itself is another operator
     Awrong = ifft(1j * np.einsum("jxyz,ijxyz->ixyz", Xi, fft(N(g)))).real
     assert np.abs(fft(Awrong) - Aw).max() > 1e-3 * np.abs(Aw).max()
+
+
+def _inclusion_problem():
+    # tests/test_finite_strain_kmc.py:368-417 of the reference: 8^3, a 4^3 cube
+    # at 68 GPa in 70 GPa, F_bar = diag(1.0005, 0.9995, 1), tol_NW 1e-8, tol_CG 1e-9
+    sh = (8, 8, 8)
+    E = np.full(sh, 70.0e9)
+    E[2:6, 2:6, 2:6] = 68.0e9
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = fs.identity_tensors(*sh)
+    C4 = fs.build_C4(E, nu, I4s, II)
+    G = fs.build_ghat4(*sh, *sh)
+    Fbar = np.diag([1.0005, 0.9995, 1.0])
+    kw = dict(E_avg=E.mean(), nu_avg=nu.mean(), tol_NW=1e-8, tol_CG=1e-9, max_NW=30)
+    return I2, I4, I4rt, C4, G, Fbar, kw
+
+
+def test_dbfft_vs_newton_cg_reference_case():
+    # the reference's own cross-check: relative F difference < 1e-5, P < 1e-2
+    I2, I4, I4rt, C4, G, Fbar, kw = _inclusion_problem()
+    Fn, Pn, _, _, _ = fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="newton_cg", **kw)
+    Fd, Pd, _, _, _ = fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="dbfft", **kw)
+    assert np.linalg.norm(Fd - Fn) / np.linalg.norm(Fn) < 1e-5
+    assert np.linalg.norm(Pd - Pn) / np.linalg.norm(Pn) < 1e-2
