@@ -90,8 +90,9 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                stability_threshold=0.0, max_cascade_steps_pct=0.3, eps_plastic=None,
                strain_assumption="small_strain", enable_thermal=False, Cp=420.0, rho=6125.0,
                thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
-               tau_bath=0.0):
-    """Returns the list of events (x, y, z, m, time, is_instab) and final state."""
+               tau_bath=0.0, solver="newton_cg"):
+    """Returns the list of events (x, y, z, m, time, is_instab) and final state.
+    ``solver``: the finite-strain step's solver ("newton_cg", "al", "dbfft")."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
     bk = dict(mean=2.0, std=0.6, min_cutoff=0.1) if barrier_kwargs is None else dict(barrier_kwargs)
@@ -154,7 +155,8 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
             return
         Fb, _, Pt, Pm = ofs.build_bc(component, eps_macro[component], targets, F_bar_initial=state["F_macro"])
         Fo, Po, So, _, Fb = ofs.solver_step(Ff, Fb, Gh, C4, I2, I4, I4rt, Fp=Fp, driving_component=component,
-                                            P_target=Pt, P_mask=Pm, E_avg=E.mean(), nu_avg=nu.mean())
+                                            P_target=Pt, P_mask=Pm, E_avg=E.mean(), nu_avg=nu.mean(),
+                                            solver=solver, pixel=pixel)
         Ff[...] = Fo
         state["F_macro"] = Fb
         for i in range(3):

@@ -466,15 +466,15 @@ def test_native_loop_matches_python_loop(gpu, d, prec, scheme):
     assert a.selection_stream.get_state()[2] == b.selection_stream.get_state()[2]
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_kmc3d_finite_strain_matches_oracle(gpu, prec):
-    # strain_assumption="finite_strain" (svk, newton_cg): every flip enters F_p
+@pytest.mark.parametrize("prec,solver", [("fp64", "newton_cg"), ("mixed", "newton_cg"), ("fp64", "dbfft")])
+def test_kmc3d_finite_strain_matches_oracle(gpu, prec, solver):
+    # strain_assumption="finite_strain" (svk, newton_cg or dbfft): every flip enters F_p
     # as expm(C/N) over N sub-steps, each followed by a Newton-Krylov step with
     # the stress targets; events, sub-step counts, F, F_p and F_macro equal
     # the oracle loop's (FP64; the FP32-Krylov form within its Newton tolerance)
     from mgkmc_amd.aqs import ThermalSimulation
     from oracle.aqs_loop import run_events
-    print(f"[kmc3d finite] {prec}", flush=True)
+    print(f"[kmc3d finite] {prec} {solver}", flush=True)
     n, M = 8, 6
     rng = np.random.default_rng(3)
     E = 70e9 * rng.uniform(0.9, 1.1, (n, n, n))
@@ -482,9 +482,10 @@ def test_kmc3d_finite_strain_matches_oracle(gpu, prec):
     kw = dict(mean=1.0, std=0.3, min_cutoff=0.0)
     args = dict(softening_cap=0.916, jp=20.0, jt=5.0)
     ev_ref, st_ref = run_events(n, n, n, M, 0.03, E, nu, 0.7, 900.0, 1e7, 2e-3, (0, 0), 8, seed=7, sel_seed=8,
-                                barrier_kwargs=kw, n_global_steps=3, strain_assumption="finite_strain", **args)
+                                barrier_kwargs=kw, n_global_steps=3, strain_assumption="finite_strain", solver=solver,
+                                **args)
     sim = ThermalSimulation(n, n, n, M, 0.03, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
-                            temperature=900.0, strain_rate=1e7, strain_assumption="finite_strain", solver="newton_cg",
+                            temperature=900.0, strain_rate=1e7, strain_assumption="finite_strain", solver=solver,
                             ls_precision=prec, rng=np.random.RandomState(7), selection_stream=np.random.RandomState(8),
                             **args)
     subs = []
