@@ -250,6 +250,10 @@ typedef struct {
   int model;       /* 0 svk, 1 landau, 2 murnaghan                          */
   int capping;     /* landau strain capping on/off                          */
   int cap_smooth;  /* 1 smooth (tanh), 0 piecewise                          */
+  int plane;       /* 0 3D; a 2D problem on an (nx, 1, ny) grid: 1 plane strain,
+                      2 plane-stress SVK (lam = the plane-stress lam*); the
+                      Newton stop test then uses the in-plane |F| and Cauchy
+                      J = det F * F33 (finite_strain_simulator.py:2356-2572)  */
   double v1, v2, v3, g1, g2, g3, g4;  /* Landau constants in Pa             */
   double Am, Bm, Cm;                  /* Murnaghan constants                */
   double cap_limit;                   /* resolved capping limit E_cap       */

@@ -31,7 +31,7 @@ struct FsMat {
   int model;  // 0 svk, 1 landau, 2 murnaghan
   int capping;
   int cap_smooth;
-  int pad;
+  int plane;  // 0 3D, 1 plane strain, 2 plane-stress SVK (2D on an (nx, 1, ny) grid)
   double v1, v2, v3, g1, g2, g3, g4;
   double Am, Bm, Cm;
   double Ecap, cap_p, cap_ratio;
@@ -274,6 +274,16 @@ __device__ void fs_tangent(const FsVox<T>& s, const FsMat& m, const T* dF, bool 
   }
 }
 
+// plane-stress SVK out-of-plane stretch (finite_strain_simulator.py:1062-1068):
+// E33 = -(lam* / (2 mu)) tr E_2d, F33 = sqrt(max(1e-12, 1 + 2 E33)); the 2D
+// problem sits on an (nx, 1, ny) grid, so tr E_2d = E_xx + E_zz
+template <typename T>
+__device__ __forceinline__ T fs_plane_f33(const T* E, T lam, T mu) {
+  const T trE = E[0] + E[8];
+  const T e33 = -(lam / (T(2) * mu)) * trE;
+  return sqrt(fmax(T(1e-12), T(1) + T(2) * e33));
+}
+
 struct Nine {
   double v[9];
 };
@@ -324,8 +334,9 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_const(size_t NV, const T* __restr
 #pragma unroll
     for (int k = 0; k < 9; ++k) f2 += (double)F[k] * (double)F[k];
     acc[2] += f2;
-    // Cauchy sigma = P F^T / J(F)  (cauchy_from_P_3d)
+    // Cauchy sigma = P F^T / J(F)  (cauchy_from_P_3d; 2D plane stress: J = det F * F33)
     double J = (double)det3(F);
+    if (m.plane == 2) J *= (double)fs_plane_f33(s.E, s.lam, s.mu);
     if (fabs(J) < 1e-14) J = 1e-14;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -361,9 +372,12 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_const(size_t NV, const T* __restr
   }
 }
 
-// Cauchy stress field (for the returned Sig)
+// Cauchy stress field (for the returned Sig); plane == 2 multiplies J by the
+// plane-stress F33 of F_e (Fpi optional)
 template <typename T>
-__global__ void k_fs_cauchy(size_t NV, const T* __restrict__ P, const T* __restrict__ F, T* __restrict__ sig) {
+__global__ void k_fs_cauchy(size_t NV, const T* __restrict__ P, const T* __restrict__ F, T* __restrict__ sig,
+                            int plane, const T* __restrict__ Fpi, const T* __restrict__ lam,
+                            const T* __restrict__ mu) {
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < NV; v += (size_t)gridDim.x * blockDim.x) {
     T f[9], p[9];
 #pragma unroll
@@ -372,6 +386,21 @@ __global__ void k_fs_cauchy(size_t NV, const T* __restrict__ P, const T* __restr
       p[k] = P[(size_t)k * NV + v];
     }
     T J = det3(f);
+    if (plane == 2) {
+      T fe[9], fpi[9], ce[9], e[9];
+      if (Fpi) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) fpi[k] = Fpi[(size_t)k * NV + v];
+        mm3(f, fpi, fe);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) fe[k] = f[k];
+      }
+      mtm3(fe, fe, ce);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) e[k] = T(0.5) * (ce[k] - ((k % 4 == 0) ? T(1) : T(0)));
+      J = J * fs_plane_f33(e, lam[v], mu[v]);
+    }
     if (fabs((double)J) < 1e-14) J = T(1e-14);
     T o[9];
     mmt3(p, f, o);
@@ -1294,6 +1323,7 @@ typedef struct {
   int model;  // 0 svk, 1 landau, 2 murnaghan
   int capping;
   int cap_smooth;
+  int plane;  // 0 3D; 2D on an (nx, 1, ny) grid: 1 plane strain, 2 plane-stress SVK
   double v1, v2, v3, g1, g2, g3, g4;  // Pa (the caller applies the GPa autoconversion)
   double Am, Bm, Cm;
   double cap_limit;  // resolved E_cap
@@ -1652,6 +1682,10 @@ int fs_ctx_init(FsCtx<T>& c, const int64_t* shape, const T* lam, const T* mu, co
     return rc;
   }
   const FsPlan& pl = c.pl;
+  if (mt->plane != 0 && pl.P != 1) {
+    mgk_set_error("plane problems run on an (nx, 1, ny) grid", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
   c.s = s;
   c.lam = lam;
   c.mu = mu;
@@ -1660,6 +1694,7 @@ int fs_ctx_init(FsCtx<T>& c, const int64_t* shape, const T* lam, const T* mu, co
   m.model = mt->model;
   m.capping = mt->capping;
   m.cap_smooth = mt->cap_smooth;
+  m.plane = mt->plane;
   m.v1 = mt->v1;
   m.v2 = mt->v2;
   m.v3 = mt->v3;
@@ -2133,7 +2168,8 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
       }
       std::swap(c.Ptr, c.P);
       MGK_HIP(hipMemcpyAsync(F, c.Ftr, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
-      const double fnorm = sqrt(res[2]);
+      // a 2D problem: |F| over the in-plane block (F_yy = 1 exactly, F_iy = F_yi = 0)
+      const double fnorm = sqrt(res[2] - (c.mat.plane ? (double)NV : 0.0));
       for (int k = 0; k < 9; ++k) info->sig_mean[k] = res[3 + k];
       if (sqrt(dn2) / (fnorm + 1e-20) < tol_NW && i_NW > 0) {
         newton_ok = true;
@@ -2201,7 +2237,7 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
     return MGK_ERR_SOLVER;
   }
   if (P_out) MGK_HIP(hipMemcpyAsync(P_out, c.P, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
-  if (sig_out) k_fs_cauchy<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, c.P, F, sig_out);
+  if (sig_out) k_fs_cauchy<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, c.P, F, sig_out, c.mat.plane, c.Fpi, c.lam, c.mu);
   {
     const int nb = fs_nblk<T>(pl.NV);
     k_comp_means9<T><<<nb, MGK_NT, 0, s>>>(NV, F, c.part);
@@ -2391,7 +2427,7 @@ static FsMat to_fsmat(const mgk_fs_material* mt) {
   m.model = mt->model;
   m.capping = mt->capping;
   m.cap_smooth = mt->cap_smooth;
-  m.pad = 0;
+  m.plane = mt->plane;
   m.v1 = mt->v1;
   m.v2 = mt->v2;
   m.v3 = mt->v3;
@@ -2472,9 +2508,11 @@ extern "C" int mgk_fs_cauchy(long long nvox, int precision, const void* P, const
   const int nb = (int)std::min<long long>((nvox + 255) / 256, 4096);
   hipStream_t s = (hipStream_t)stream;
   if (precision == 0)
-    k_fs_cauchy<double><<<nb, 256, 0, s>>>((size_t)nvox, (const double*)P, (const double*)F, (double*)sig);
+    k_fs_cauchy<double><<<nb, 256, 0, s>>>((size_t)nvox, (const double*)P, (const double*)F, (double*)sig, 0,
+                                           (const double*)nullptr, (const double*)nullptr, (const double*)nullptr);
   else
-    k_fs_cauchy<float><<<nb, 256, 0, s>>>((size_t)nvox, (const float*)P, (const float*)F, (float*)sig);
+    k_fs_cauchy<float><<<nb, 256, 0, s>>>((size_t)nvox, (const float*)P, (const float*)F, (float*)sig, 0,
+                                          (const float*)nullptr, (const float*)nullptr, (const float*)nullptr);
   MGK_HIP(hipGetLastError());
   return MGK_OK;
 }

@@ -180,6 +180,70 @@ def finite_strain_solver_step_3d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, dri
     return Fo, Po, So, None, np.array(info.F_bar[:]).reshape(3, 3)
 
 
+_PLANE_AXES = (0, 2)  # in-plane axes of the (nx, 1, ny) grid a 2D problem runs on
+
+
+def _embed2(A, fill=1.0):
+    """(2, 2, nx, ny) [or (2, 2)] -> (3, 3, nx, 1, ny) [(3, 3)], out-of-plane diagonal = fill."""
+    A = np.asarray(A, dtype=np.float64)
+    grid = A.shape[2:]
+    out = np.zeros((3, 3) + ((grid[0], 1, grid[1]) if grid else ()))
+    for a in range(2):
+        for b in range(2):
+            out[_PLANE_AXES[a], _PLANE_AXES[b]] = A[a, b].reshape(out.shape[2:]) if grid else A[a, b]
+    out[1, 1] = fill
+    return out
+
+
+def _extract2(A3):
+    """(3, 3, nx, 1, ny) [or (3, 3)] -> (2, 2, nx, ny) [(2, 2)]."""
+    g = A3.shape[2:]
+    return np.stack([np.stack([A3[_PLANE_AXES[a], _PLANE_AXES[b]].reshape((g[0], g[2]) if g else ())
+                               for b in range(2)]) for a in range(2)])
+
+
+def finite_strain_solver_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, driving_component=(0, 0),
+                                 P_target=None, P_mask=None, E_avg=100e9, nu_avg=0.3, tol_NW=1e-5, tol_CG=1e-6,
+                                 max_NW=50, tol_macro=1e6, max_iter_macro=20, enable_console=True, model_type="svk",
+                                 plane_mode="plane_strain", A_m=0.0, B_m=0.0, C_m=0.0, solver="al", pixel=1.0,
+                                 v1=0.0, v2=0.0, v3=0.0, g1=0.0, g2=0.0, g3=0.0, g4=0.0,
+                                 strain_capping_enabled=False, strain_capping_limit=None,
+                                 strain_capping_tangent_ratio=0.1, strain_capping_type="piecewise",
+                                 strain_capping_smooth_power=1.0):
+    """One 2D finite-strain load step (finite_strain_simulator.py:2356-2572),
+    solver "newton_cg" -> (F, P, Sig, None, F_bar).  The plane problem runs
+    through the 3D device step on an (nx, 1, ny) grid: F_yy = 1, no
+    out-of-plane shear, the projection's y wave number 0, so the in-plane
+    fields, the Krylov iterates and the Newton / macro decisions are those of
+    the 2D step (its stop test over the in-plane |F|, and for plane-stress SVK
+    Cauchy's J = det F * F33).  Laws: svk (plane strain and plane stress),
+    landau and murnaghan (plane strain); lam = C4[0,0,1,1] and mu =
+    C4[0,1,0,1] of the 2D stiffness are read."""
+    if solver != "newton_cg":
+        raise NotImplementedError(f"solver={solver!r}: the 2D device step implements solver='newton_cg'")
+    if model_type not in ("svk", "landau", "murnaghan") or (model_type != "svk" and plane_mode != "plane_strain"):
+        raise NotImplementedError(f"2D {model_type} in {plane_mode}: the device step covers svk (both plane modes), "
+                                  "landau and murnaghan (plane strain)")
+    C4 = np.asarray(C4)
+    nx, ny = C4.shape[4:]
+    sh3 = (nx, 1, ny)
+    lam = np.ascontiguousarray(C4[0, 0, 1, 1]).reshape(sh3)
+    mu = np.ascontiguousarray(C4[0, 1, 0, 1]).reshape(sh3)
+    s = FsSolver(lam, mu, precision=0, Fp=None if Fp is None else _embed2(Fp))
+    s.F.copy_(dev.upload(np.ascontiguousarray(_embed2(F)), 0))
+    mat = material(model_type, A_m, B_m, C_m, v1, v2, v3, g1, g2, g3, g4, strain_capping_enabled,
+                   strain_capping_limit, strain_capping_tangent_ratio, strain_capping_type,
+                   strain_capping_smooth_power, mu_mean=float(np.mean(mu)))
+    mat.plane = 2 if plane_mode == "plane_stress" else 1
+    Pt = None if P_target is None else _embed2(P_target, 0.0)
+    Pm = None if P_mask is None else _embed2(np.asarray(P_mask, dtype=float), 0.0).astype(bool)
+    drv = (_PLANE_AXES[driving_component[0]], _PLANE_AXES[driving_component[1]])
+    info = s.step(_embed2(F_bar), mat, drv, Pt, Pm, E_avg, nu_avg, tol_NW, tol_CG, max_NW, tol_macro,
+                  max_iter_macro, solver="newton_cg")
+    return (_extract2(dev.download(s.F)), _extract2(dev.download(s.P)), _extract2(dev.download(s.sig)), None,
+            _extract2(np.array(info.F_bar[:]).reshape(3, 3)))
+
+
 def constitutive_hyperelastic_3d(F, C4, I2, I4, I4rt, Fp=None, model_type="svk", A_m=0.0, B_m=0.0, C_m=0.0, v1=0.0,
                                  v2=0.0, v3=0.0, g1=0.0, g2=0.0, g3=0.0, g4=0.0, strain_capping_enabled=False,
                                  strain_capping_limit=None, strain_capping_tangent_ratio=0.1,

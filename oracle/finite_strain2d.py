@@ -1,0 +1,321 @@
+"""Oracle: 2D finite-strain Newton-Krylov step (CPU restatement).
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Follows mgkmc/finite_strain_simulator.py:
+  :125-140      _make_identity_tensors_2d
+  :147-199      build_ghat4_2d (zero at DC and on even-axis Nyquist lines)
+  :206-225      _fft2_tensor / _ifft2_tensor / _project
+  :233-261      build_C4_2d (plane strain: 3D lambda; plane stress: E nu / (1 - nu^2))
+  :591-1091     constitutive_hyperelastic_2d: svk (both plane modes, F33 of the
+                plane-stress E33 = -lam* tr E / (2 mu)), landau and murnaghan
+                in plane strain (E33 = 0; the einsum tangent of the 3D law cut
+                to the in-plane block), Fp, the push-forward to K4
+  :1094-1114    cauchy_from_P (J = det F * F33)
+  :2356-2572    finite_strain_solver_step_2d, solver="newton_cg"
+  :2580-2850    finite_strain_simulation_2d (load stepping, store=False)
+The Krylov solver is SciPy's own bicgstab, called as the reference calls it.
+Not restated (the device step does not offer them): neo-Hookean, landau and
+murnaghan in plane stress, the AL and DBFFT 2D solvers.
+
+Pinned by (tests/test_oracle_fs2d.py): the homogeneous closed form, the 3D
+oracle on the same problem embedded as an (nx, 1, ny) grid (plane strain:
+identical fields and Newton / Krylov counts), the plane-stress SVK law against
+its closed form (sigma_33 = 0 of the linearised law), central differences of
+P(F) against K4, and equilibrium of the converged step.
+"""
+import numpy as np
+import scipy.fft as _sfft
+import scipy.sparse.linalg as sp
+
+from .finite_strain import landau_constants
+
+
+def identity_tensors_2d(nx, ny):
+    i = np.eye(2)
+    ones = np.ones((nx, ny))
+    I2 = np.einsum("ij,xy->ijxy", i, ones)
+    I4 = np.einsum("ijkl,xy->ijklxy", np.einsum("il,jk", i, i), ones)
+    I4rt = np.einsum("ijkl,xy->ijklxy", np.einsum("ik,jl", i, i), ones)
+    I4s = 0.5 * (I4 + I4rt)
+    II = np.einsum("ijxy,klxy->ijklxy", I2, I2)
+    return I2, I4, I4rt, I4s, II
+
+
+def build_ghat4_2d(nx, ny, Lx, Ly):
+    X = np.meshgrid(np.fft.fftfreq(nx, d=Lx / nx), np.fft.fftfreq(ny, d=Ly / ny), indexing="ij")
+    xi = np.stack(X, 0)
+    xi2 = X[0] ** 2 + X[1] ** 2
+    nyq = np.zeros((nx, ny), bool)
+    if nx % 2 == 0:
+        nyq |= X[0] == -nx / (2.0 * Lx)
+    if ny % 2 == 0:
+        nyq |= X[1] == -ny / (2.0 * Ly)
+    s = np.where(xi2 == 0, 1.0, xi2)
+    G = np.zeros((2, 2, 2, 2, nx, ny))
+    d = np.eye(2)
+    for i in range(2):
+        for j in range(2):
+            for l in range(2):
+                for m in range(2):
+                    v = d[i, m] * xi[j] * xi[l] / s
+                    v[xi2 == 0] = 0.0
+                    v[nyq] = 0.0
+                    G[i, j, l, m] = v
+    return G
+
+
+def project2d(A2, G):
+    Ah = _sfft.fftn(A2, axes=(-2, -1))
+    return _sfft.ifftn(np.einsum("ijklxy,lkxy->ijxy", G, Ah), axes=(-2, -1)).real
+
+
+def build_C4_2d(E, nu, I4s, II, plane_mode="plane_strain"):
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / (1.0 - nu ** 2) if plane_mode == "plane_stress" else E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    return lam[None, None, None, None] * II + 2.0 * mu[None, None, None, None] * I4s
+
+
+def _trans2(A):
+    return np.einsum("ijxy->jixy", A)
+
+
+def _ddot42(A4, B2):
+    return np.einsum("ijklxy,lkxy->ijxy", A4, B2)
+
+
+def _inv2(A):
+    det = A[0, 0] * A[1, 1] - A[0, 1] * A[1, 0]
+    ds = np.where(np.abs(det) < 1e-14, 1e-14, det)
+    out = np.empty_like(A)
+    out[0, 0], out[1, 1] = A[1, 1] / ds, A[0, 0] / ds
+    out[0, 1], out[1, 0] = -A[0, 1] / ds, -A[1, 0] / ds
+    return out
+
+
+def _i4s3(sh):
+    i3 = np.eye(3)
+    t = 0.5 * (np.einsum("ik,jl->ijkl", i3, i3) + np.einsum("il,jk->ijkl", i3, i3))
+    return np.broadcast_to(t[..., None, None], (3, 3, 3, 3) + sh)
+
+
+def _law3d(E_GL, lam, mu, model_type, mat, need_tangent):
+    """S (3D) and dS/dE (3D einsum form) of the landau / murnaghan laws at a
+    plane-strain E_GL (finite_strain_simulator.py:700-964)."""
+    sh = lam.shape
+    I3 = np.broadcast_to(np.eye(3)[..., None, None], (3, 3) + sh)
+    II3 = np.einsum("ijxy,klxy->ijklxy", I3, I3)
+    trE = E_GL[0, 0] + E_GL[1, 1] + E_GL[2, 2]
+    trE2 = np.einsum("ijxy,jixy->xy", E_GL, E_GL)
+    E2 = np.einsum("ijxy,jkxy->ikxy", E_GL, E_GL)
+    KE = 0.5 * (np.einsum("ikxy,jlxy->ijklxy", I3, E_GL) + np.einsum("ikxy,jlxy->ijklxy", E_GL, I3)
+                + np.einsum("ilxy,jkxy->ijklxy", I3, E_GL) + np.einsum("ilxy,jkxy->ijklxy", E_GL, I3))
+    if model_type == "murnaghan":
+        A, B, C = (np.full(sh, float(mat.get(k, 0.0))) for k in ("A_m", "B_m", "C_m"))
+        S = ((lam * trE + A * trE ** 2 + B * trE2)[None, None] * I3 + 2.0 * (mu + B * trE)[None, None] * E_GL
+             + C[None, None] * E2)
+        if not need_tangent:
+            return S, None, None
+        C4 = ((lam + 2.0 * A * trE)[None, None, None, None] * II3
+              + (2.0 * B)[None, None, None, None] * (np.einsum("ijxy,klxy->ijklxy", I3, E_GL)
+                                                     + np.einsum("ijxy,klxy->ijklxy", E_GL, I3))
+              + (2.0 * (mu + B * trE))[None, None, None, None] * _i4s3(sh) + C[None, None, None, None] * KE)
+        return S, C4, None
+    v1, v2, v3, g1, g2, g3, g4 = (np.full(sh, c) for c in landau_constants(
+        [mat.get(k, 0.0) for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4")]))
+    cap = mat.get("strain_capping_enabled", False)
+    if cap:
+        lim = mat.get("strain_capping_limit")
+        g4m, mum = g4.mean(), mu.mean()
+        Ecap = lim if (lim is not None and lim > 0) else (np.sqrt(0.2 * mum / abs(g4m)) if g4m < 0 else 999.0)
+        tr3 = trE / 3.0
+        Ed = E_GL.copy()
+        for i in range(3):
+            Ed[i, i] -= tr3
+        Eeq = np.sqrt(np.maximum(0.0, (2.0 / 3.0) * np.einsum("ijxy,jixy->xy", Ed, Ed)))
+        if mat.get("strain_capping_type", "piecewise") == "smooth":
+            p = mat.get("strain_capping_smooth_power", 1.0)
+            f = np.ones_like(Eeq)
+            nz = Eeq > 1e-12
+            f[nz] = (Ecap / Eeq[nz]) * np.tanh(np.minimum(100.0, (Eeq[nz] / Ecap) ** p)) ** (1.0 / p)
+            w = np.tanh(np.minimum(100.0, (Eeq / Ecap) ** p)) ** 2
+        else:
+            capd = Eeq > Ecap
+            f = np.ones_like(Eeq)
+            np.divide(Ecap, Eeq, out=f, where=capd)
+            w = capd.astype(float)
+        Eorig = E_GL
+        E_GL = Ed * f[None, None]
+        for i in range(3):
+            E_GL[i, i] += tr3
+        trE = E_GL[0, 0] + E_GL[1, 1] + E_GL[2, 2]
+        trE2 = np.einsum("ijxy,jixy->xy", E_GL, E_GL)
+        E2 = np.einsum("ijxy,jkxy->ikxy", E_GL, E_GL)
+        KE = 0.5 * (np.einsum("ikxy,jlxy->ijklxy", I3, E_GL) + np.einsum("ikxy,jlxy->ijklxy", E_GL, I3)
+                    + np.einsum("ilxy,jkxy->ijklxy", I3, E_GL) + np.einsum("ilxy,jkxy->ijklxy", E_GL, I3))
+    trE3 = np.einsum("ijxy,jkxy,kixy->xy", E_GL, E_GL, E_GL)
+    A = lam * trE + 0.5 * v1 * trE ** 2 + v2 * trE2 + g1 * trE ** 3 / 6.0 + g2 * trE * trE2 + (4.0 / 3.0) * g3 * trE3
+    B = 2.0 * (mu + v2 * trE + 0.5 * g2 * trE ** 2 + g4 * trE2)
+    C = 4.0 * (v3 + g3 * trE)
+    S = A[None, None] * I3 + B[None, None] * E_GL + C[None, None] * E2
+    Gt = mat.get("strain_capping_tangent_ratio", 0.1) * mu if cap else None
+    if cap:
+        S = S + 2.0 * Gt[None, None] * (Eorig - E_GL) * w[None, None]
+    if not need_tangent:
+        return S, None, None
+    dA = ((lam + v1 * trE + 0.5 * g1 * trE ** 2 + g2 * trE2)[None, None] * I3
+          + 2.0 * (v2 + g2 * trE)[None, None] * E_GL + 4.0 * g3[None, None] * E2)
+    dB = 2.0 * (v2 + g2 * trE)[None, None] * I3 + 4.0 * g4[None, None] * E_GL
+    dC = 4.0 * g3[None, None] * I3
+    C4 = (np.einsum("ijxy,klxy->ijklxy", I3, dA) + np.einsum("ijxy,klxy->ijklxy", E_GL, dB)
+          + np.einsum("ijxy,klxy->ijklxy", E2, dC) + B[None, None, None, None] * _i4s3(sh)
+          + C[None, None, None, None] * KE)
+    if cap:
+        C4 = C4 + (2.0 * Gt * w)[None, None, None, None] * (_i4s3(sh) - II3 / 3.0)
+    return S, C4, None
+
+
+def constitutive_2d(F, C4, I2, I4, I4rt, Fp=None, model_type="svk", plane_mode="plane_strain", need_tangent=True,
+                    **mat):
+    """(P, K4, F33) of constitutive_hyperelastic_2d (:591-1091) for the laws
+    listed in the module docstring."""
+    if model_type not in ("svk", "landau", "murnaghan") or (model_type != "svk" and plane_mode == "plane_stress"):
+        raise NotImplementedError(f"{model_type} / {plane_mode}")
+    Fpi = _inv2(Fp) if Fp is not None else None
+    Fe = np.einsum("ijxy,jkxy->ikxy", F, Fpi) if Fpi is not None else F
+    lam, mu = C4[0, 0, 1, 1], C4[0, 1, 0, 1]
+    E2d = 0.5 * (np.einsum("jixy,jkxy->ikxy", Fe, Fe) - I2)
+    if model_type == "svk":
+        S = np.einsum("ijklxy,lkxy->ijxy", C4, E2d)
+        C4e = C4
+        if plane_mode == "plane_stress":
+            F33 = np.sqrt(np.maximum(1e-12, 1.0 + 2.0 * (-(lam / (2.0 * mu)) * (E2d[0, 0] + E2d[1, 1]))))
+        else:
+            F33 = np.ones_like(lam)
+    else:
+        EG = np.zeros((3, 3) + lam.shape)
+        EG[:2, :2] = E2d
+        S3, C43, _ = _law3d(EG, lam, mu, model_type, mat, need_tangent)
+        S = S3[:2, :2]
+        C4e = C43[:2, :2, :2, :2] if need_tangent else None
+        F33 = np.ones_like(lam)
+    if Fpi is not None:
+        P = np.einsum("ikxy,klxy,jlxy->ijxy", Fe, S, Fpi)
+        if not need_tangent:
+            return P, None, F33
+        Sref = np.einsum("mkxy,klxy,jlxy->mjxy", Fpi, S, Fpi)
+        A = np.einsum("klmnxy,jlxy->kjmnxy", C4e, Fpi)
+        B = np.einsum("kjmnxy,bmxy->kjbnxy", A, Fpi)
+        Cc = np.einsum("kjbnxy,ikxy->ijbnxy", B, Fe)
+        K4 = np.einsum("bjxy,ia->ijabxy", Sref, np.eye(2)) + np.einsum("ijbnxy,anxy->ijabxy", Cc, Fe)
+        return P, K4, F33
+    P = np.einsum("ijxy,jkxy->ikxy", Fe, S)
+    if not need_tangent:
+        return P, None, F33
+    FC4 = np.einsum("ijxy,jkmnxy->ikmnxy", Fe, C4e)
+    FC4Ft = np.einsum("ijklxy,lmxy->ijkmxy", FC4, _trans2(Fe))
+    K4 = np.ascontiguousarray(FC4Ft.transpose(1, 0, 3, 2, 4, 5))
+    for k in range(2):
+        K4[:, k, k] += S
+    return P, K4, F33
+
+
+def cauchy_2d(P, F, F33=None):
+    """cauchy_from_P (:1094-1114): sigma = P F^T / (det F * F33)."""
+    J = F[0, 0] * F[1, 1] - F[0, 1] * F[1, 0]
+    if F33 is not None:
+        J = J * F33
+    J = np.where(np.abs(J) < 1e-14, 1e-14, J)
+    return np.einsum("ijxy,kjxy->ikxy", P, F) / J[None, None]
+
+
+def solver_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, driving_component=(0, 0), P_target=None, P_mask=None,
+                   E_avg=100e9, nu_avg=0.3, tol_NW=1e-5, tol_CG=1e-6, max_NW=50, tol_macro=1e6, max_iter_macro=20,
+                   model_type="svk", plane_mode="plane_strain", stats=None, **mat):
+    """finite_strain_solver_step_2d, newton_cg branch (:2356-2572) ->
+    (F, P, Sig, K4, F_bar).  ``stats`` collects (Newton iteration, Krylov
+    iterations so far in this macro iteration, alpha) per Newton step under
+    "newton", as the 3D oracle does."""
+    nx, ny = F.shape[2:]
+    P_target = np.zeros((2, 2)) if P_target is None else P_target
+    P_mask = np.zeros((2, 2), bool) if P_mask is None else P_mask
+    F_start = F.copy()
+    Fbc = np.array(F_bar, dtype=float)
+    i_drv, j_drv = driving_component
+    upd = np.zeros((2, 2), bool)
+    for a in range(2):
+        if P_mask[a, a] and (a, a) != (i_drv, j_drv):
+            upd[a, a] = True
+    prev_F = prev_s = None
+    Fpi = _inv2(Fp) if Fp is not None else None
+    res = None
+    max_err = 0.0
+    for _ in range(max_iter_macro):
+        DbarF = Fbc - F_start.mean(axis=(2, 3))
+        Dg = np.einsum("ij,xy->ijxy", DbarF, np.ones((nx, ny)))
+        cur = {}
+        cur["P"], cur["K4"], cur["F33"] = constitutive_2d(F_start, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type,
+                                                          plane_mode=plane_mode, **mat)
+        G_op = lambda A2: project2d(A2, Ghat4)
+        K_dF = lambda x: _trans2(_ddot42(cur["K4"], _trans2(x.reshape(2, 2, nx, ny))))
+        nk = {"n": 0}
+
+        def cb(xk):
+            nk["n"] += 1
+
+        A_op = sp.LinearOperator((F_start.size,) * 2, matvec=lambda x: G_op(K_dF(x)).reshape(-1), dtype="float64")
+        F_curr = F_start.copy()
+        for i_NW in range(max_NW):
+            rhs = -G_op(K_dF(Dg.reshape(-1))).reshape(-1) if i_NW == 0 else -G_op(cur["P"]).reshape(-1)
+            dFm, _ = sp.bicgstab(A_op, rhs, rtol=tol_CG, maxiter=150, callback=cb)
+            dF = dFm.reshape(2, 2, nx, ny)
+            dX = Dg + dF if i_NW == 0 else dF
+            alpha = 1.0
+            for _ in range(16):
+                Ft = F_curr + alpha * dX
+                Fe = np.einsum("ijxy,jkxy->ikxy", Ft, Fpi) if Fpi is not None else Ft
+                Je = Fe[0, 0] * Fe[1, 1] - Fe[0, 1] * Fe[1, 0]
+                if np.any(Je <= 1e-4) or np.any(np.isnan(Je)):
+                    alpha *= 0.5
+                    continue
+                Pt, Kt, F33t = constitutive_2d(Ft, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type,
+                                               plane_mode=plane_mode, **mat)
+                if np.any(np.isnan(Pt)) or np.any(np.isnan(Kt)):
+                    alpha *= 0.5
+                    continue
+                F_curr, cur["P"], cur["K4"], cur["F33"] = Ft, Pt, Kt, F33t
+                break
+            else:
+                raise ValueError("Newton-CG line search failed to find a valid step.")
+            if stats is not None:
+                stats.setdefault("newton", []).append((i_NW, nk["n"], alpha))
+            if np.linalg.norm(dFm) / (np.linalg.norm(F_curr) + 1e-20) < tol_NW and i_NW > 0:
+                break
+        else:
+            raise ValueError(f"Newton-CG inner solver did not converge within {max_NW} iterations.")
+        Sig = cauchy_2d(cur["P"], F_curr, F33=cur["F33"])
+        Sm = Sig.mean(axis=(2, 3))
+        res = (F_curr, cur["P"], Sig, cur["K4"])
+        if not np.any(P_mask):
+            break
+        err = np.zeros((2, 2))
+        err[P_mask] = P_target[P_mask] - Sm[P_mask]
+        max_err = np.max(np.abs(err[P_mask]))
+        if max_err < tol_macro:
+            break
+        dFm_ = np.clip((err - nu_avg * np.trace(err) * np.eye(2)) / E_avg, -0.01, 0.01)
+        dsc = dFm_[upd]
+        Fsc, ssc = Fbc[upd].copy(), Sm[upd].copy()
+        if prev_F is not None:
+            dh, dsh = Fsc - prev_F, ssc - prev_s
+            with np.errstate(divide="ignore", invalid="ignore"):
+                slope = dsh / dh
+            ok = np.isfinite(slope) & (np.abs(dh) > 1e-14) & (slope > 0.01 * E_avg)
+            dsc[ok] = np.clip(err[upd][ok] / slope[ok], -0.01, 0.01)
+        prev_F, prev_s = Fsc, ssc
+        Fbc[upd] += dsc
+        F_start = F_curr
+    else:
+        if np.any(P_mask):
+            raise ValueError(f"Outer BC loop did not converge after {max_iter_macro} iterations.")
+    return res[0], res[1], res[2], res[3], Fbc

@@ -1,0 +1,84 @@
+"""GPU parity of the 2D finite-strain step (finite_strain_solver_step_2d,
+newton_cg) with the 2D oracle: svk in both plane modes, landau (plain,
+capped, with Fp) and murnaghan in plane strain, mixed BCs."""
+import numpy as np
+import pytest
+
+from oracle import finite_strain2d as f2
+
+pytestmark = pytest.mark.gpu
+
+LANDAU = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("sh,model,kw,fp,mode", [
+    ((8, 6), "svk", {}, False, "plane_strain"), ((8, 6), "svk", {}, True, "plane_strain"),
+    ((8, 6), "svk", {}, False, "plane_stress"), ((8, 6), "landau", LANDAU, False, "plane_strain"),
+    ((8, 6), "landau", LANDAU, True, "plane_strain"),
+    ((12, 10), "landau", dict(LANDAU, strain_capping_enabled=True, strain_capping_type="smooth"), False,
+     "plane_strain"),
+    ((8, 6), "murnaghan", dict(A_m=-100e9, B_m=-100e9, C_m=-100e9), False, "plane_strain"),
+    ((64, 48), "svk", {}, False, "plane_stress")])
+def test_2d_step_parity(gpu, sh, model, kw, fp, mode):
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_2d
+    rng = np.random.default_rng(4)
+    E = 70e9 * rng.uniform(0.85, 1.15, sh)
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    C4 = f2.build_C4_2d(E, nu, I4s, II, mode)
+    G = f2.build_ghat4_2d(*sh, *sh)
+    Fbar = np.eye(2)
+    Fbar[0, 1] = 0.02
+    Fbar[1, 1] = 0.996
+    Fp = None
+    if fp:
+        Fp = I2 + 0.01 * np.random.default_rng(5).standard_normal((2, 2) + sh)
+    print(f"[fs2d] {sh} {model} {mode} Fp={fp}", flush=True)
+    st = {}
+    ref = f2.solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, Fp=Fp, model_type=model, plane_mode=mode,
+                            tol_NW=1e-8, tol_CG=1e-8, stats=st, **kw)
+    got = finite_strain_solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, Fp=Fp, model_type=model,
+                                       plane_mode=mode, tol_NW=1e-8, tol_CG=1e-8, solver="newton_cg", **kw)
+    eF, eP, eS = _rel(got[0], ref[0]), _rel(got[1], ref[1]), _rel(got[2], ref[2])
+    print(f"[fs2d] newton={st['newton']} relF={eF:.2e} relP={eP:.2e} relSig={eS:.2e}", flush=True)
+    assert eF < 1e-12 and eP < 1e-9 and eS < 1e-9
+
+
+def test_2d_mixed_bc_parity(gpu):
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_2d
+    sh = (10, 8)
+    rng = np.random.default_rng(9)
+    E = 70e9 * rng.uniform(0.7, 1.3, sh)
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    for mode in ("plane_strain", "plane_stress"):
+        C4 = f2.build_C4_2d(E, nu, I4s, II, mode)
+        G = f2.build_ghat4_2d(*sh, *sh)
+        Fbar = np.eye(2)
+        Fbar[0, 0] = 1.01
+        Pt, Pm = np.zeros((2, 2)), np.zeros((2, 2), bool)
+        Pm[1, 1] = True
+        # tolerances the reference's own loop completes at (at 1e-7 its plane-stress
+        # macro loop ends in a line-search failure, which the oracle reproduces)
+        kw = dict(P_target=Pt, P_mask=Pm, E_avg=E.mean(), nu_avg=0.3, tol_NW=1e-6, tol_CG=1e-6, tol_macro=1e3,
+                  plane_mode=mode)
+        ref = f2.solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, **kw)
+        got = finite_strain_solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="newton_cg", **kw)
+        print(f"[fs2d mixed] {mode} Fbar ref={np.diag(ref[4])} got={np.diag(got[4])}", flush=True)
+        assert np.abs(got[4] - ref[4]).max() < 1e-12
+        assert _rel(got[0], ref[0]) < 1e-11 and _rel(got[2], ref[2]) < 1e-8
+
+
+def test_2d_unsupported_laws_raise(gpu):
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_2d
+    sh = (4, 4)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    C4 = f2.build_C4_2d(np.full(sh, 70e9), np.full(sh, 0.3), I4s, II)
+    for kw in (dict(model_type="neo_hookean", solver="newton_cg"), dict(solver="al"),
+               dict(model_type="landau", plane_mode="plane_stress", solver="newton_cg")):
+        with pytest.raises(NotImplementedError):
+            finite_strain_solver_step_2d(I2.copy(), np.eye(2), None, C4, I2, I4, I4rt, **kw)

@@ -1,0 +1,124 @@
+"""Pins of the 2D finite-strain oracle (oracle/finite_strain2d.py): the 3D
+oracle on the same plane problem embedded as an (nx, 1, ny) grid, closed
+forms, finite differences of P(F), equilibrium."""
+import numpy as np
+import pytest
+
+from oracle import finite_strain as fs
+from oracle import finite_strain2d as f2
+
+LANDAU = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9)
+M3 = [0, 2]  # in-plane axes of the embedding (the singleton is y)
+
+
+def embed(A2, fill=1.0):
+    """(2, 2, nx, ny) -> (3, 3, nx, 1, ny); the out-of-plane diagonal = fill."""
+    nx, ny = A2.shape[2:]
+    A3 = np.zeros((3, 3, nx, 1, ny))
+    for a in range(2):
+        for b in range(2):
+            A3[M3[a], M3[b], :, 0, :] = A2[a, b]
+    A3[1, 1] = fill
+    return A3
+
+
+def extract(A3):
+    return np.stack([np.stack([A3[M3[a], M3[b], :, 0, :] for b in range(2)]) for a in range(2)])
+
+
+def _problem(sh, seed, plane_mode="plane_strain", spread=0.15):
+    rng = np.random.default_rng(seed)
+    E = 70e9 * rng.uniform(1 - spread, 1 + spread, sh)
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    C4 = f2.build_C4_2d(E, nu, I4s, II, plane_mode)
+    lam, mu = C4[0, 0, 1, 1], C4[0, 1, 0, 1]
+    sh3 = (sh[0], 1, sh[1])
+    I2_3, I4_3, I4rt_3, I4s_3, II_3 = fs.identity_tensors(*sh3)
+    C4_3 = lam.reshape(sh3)[None, None, None, None] * II_3 + 2.0 * mu.reshape(sh3)[None, None, None, None] * I4s_3
+    return E, nu, (I2, I4, I4rt, C4), (I2_3, I4_3, I4rt_3, C4_3), sh3
+
+
+@pytest.mark.parametrize("model,kw,fp,mode", [
+    ("svk", {}, False, "plane_strain"), ("svk", {}, True, "plane_strain"), ("svk", {}, False, "plane_stress"),
+    ("landau", LANDAU, False, "plane_strain"), ("landau", LANDAU, True, "plane_strain"),
+    ("landau", dict(LANDAU, strain_capping_enabled=True, strain_capping_type="smooth"), False, "plane_strain"),
+    ("murnaghan", dict(A_m=-100e9, B_m=-100e9, C_m=-100e9), False, "plane_strain")])
+def test_2d_step_equals_the_3d_oracle_on_the_embedded_grid(model, kw, fp, mode):
+    sh = (8, 6)
+    E, nu, t2, t3, sh3 = _problem(sh, 4, mode)
+    Fbar = np.eye(2)
+    Fbar[0, 1] = 0.02
+    Fbar[1, 1] = 0.996
+    Fp = None
+    if fp:
+        rng = np.random.default_rng(5)
+        Fp = t2[0] + 0.01 * rng.standard_normal((2, 2) + sh)
+    s2, s3 = {}, {}
+    F2, P2, S2, _, Fb2 = f2.solver_step_2d(t2[0].copy(), Fbar, f2.build_ghat4_2d(*sh, *sh), t2[3], *t2[:3], Fp=Fp,
+                                           model_type=model, plane_mode=mode, tol_NW=1e-8, tol_CG=1e-8, stats=s2,
+                                           **kw)
+    F3, P3, S3, _, Fb3 = fs.solver_step(t3[0].copy(), embed(Fbar[:, :, None, None])[:, :, 0, 0, 0],
+                                        fs.build_ghat4(*sh3, *sh3), t3[3], *t3[:3],
+                                        Fp=None if Fp is None else embed(Fp), model_type=model, tol_NW=1e-8,
+                                        tol_CG=1e-8, stats=s3, **kw)
+    assert [n[:2] for n in s2["newton"]] == [n[:2] for n in s3["newton"]]
+    assert np.abs(F2 - extract(F3)).max() < 1e-13
+    assert np.abs(P2 - extract(P3)).max() < 1e-12 * np.abs(P2).max()
+    if mode == "plane_strain":
+        assert np.abs(S2 - extract(S3)).max() < 1e-12 * np.abs(S2).max()
+    # the out-of-plane block of the embedded solution stays untouched
+    assert np.all(F3[1, 1] == 1.0) and not np.any(F3[1, [0, 2]]) and not np.any(F3[[0, 2], 1])
+
+
+def test_2d_plane_stress_svk_closed_form():
+    # homogeneous plane stress: E33 = -lam* tr E / (2 mu) makes sigma_33 = 0 of
+    # the 3D SVK law with E_33 = E33 (lam* = 2 lam mu / (lam + 2 mu))
+    sh = (4, 4)
+    E, nu = np.full(sh, 70e9), np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    C4 = f2.build_C4_2d(E, nu, I4s, II, "plane_stress")
+    F = I2.copy()
+    F[0, 0] += 0.01
+    F[0, 1] += 0.004
+    P, K4, F33 = f2.constitutive_2d(F, C4, I2, I4, I4rt, plane_mode="plane_stress")
+    lam3, mu = 70e9 * 0.3 / (1.3 * 0.4), 70e9 / 2.6
+    Eg = 0.5 * (F[:, :, 0, 0].T @ F[:, :, 0, 0] - np.eye(2))
+    E33 = 0.5 * (F33[0, 0] ** 2 - 1.0)
+    assert abs(lam3 * (np.trace(Eg) + E33) + 2 * mu * E33) < 1e-9 * lam3 * abs(np.trace(Eg))
+    S = lam3 * (np.trace(Eg) + E33) * np.eye(2) + 2 * mu * Eg
+    assert np.allclose(P[:, :, 1, 2], F[:, :, 0, 0] @ S, rtol=1e-12)
+    sig = f2.cauchy_2d(P, F, F33)[:, :, 0, 0]
+    J = np.linalg.det(F[:, :, 0, 0]) * F33[0, 0]
+    assert np.allclose(sig, F[:, :, 0, 0] @ S @ F[:, :, 0, 0].T / J, rtol=1e-12)
+
+
+@pytest.mark.parametrize("model,kw", [("landau", LANDAU), ("murnaghan", dict(A_m=-100e9, B_m=-100e9, C_m=-100e9))])
+def test_2d_tangent_is_the_derivative_of_P(model, kw):
+    sh = (3, 2)
+    E, nu, t2, _, _ = _problem(sh, 7)
+    rng = np.random.default_rng(2)
+    F = t2[0] + 0.02 * rng.standard_normal((2, 2) + sh)
+    dF = rng.standard_normal((2, 2) + sh)
+    P, K4, _ = f2.constitutive_2d(F, t2[3], *t2[:3], model_type=model, **kw)
+    h = 1e-6
+    Pp = f2.constitutive_2d(F + h * dF, t2[3], *t2[:3], model_type=model, need_tangent=False, **kw)[0]
+    Pm = f2.constitutive_2d(F - h * dF, t2[3], *t2[:3], model_type=model, need_tangent=False, **kw)[0]
+    fd = (Pp - Pm) / (2 * h)
+    lin = f2._trans2(f2._ddot42(K4, f2._trans2(dF)))
+    assert np.abs(fd - lin).max() < 1e-6 * np.abs(lin).max()
+
+
+def test_2d_step_is_equilibrated_and_keeps_the_mean():
+    sh = (10, 8)
+    E, nu, t2, _, _ = _problem(sh, 9, spread=0.3)
+    G = f2.build_ghat4_2d(*sh, *sh)
+    Fbar = np.eye(2)
+    Fbar[0, 0] = 1.01
+    Pt, Pm = np.zeros((2, 2)), np.zeros((2, 2), bool)
+    Pm[1, 1] = True
+    F, P, S, _, Fb = f2.solver_step_2d(t2[0].copy(), Fbar, G, t2[3], *t2[:3], P_target=Pt, P_mask=Pm,
+                                       E_avg=E.mean(), nu_avg=0.3, tol_NW=1e-7, tol_CG=1e-7, tol_macro=1e3)
+    assert np.allclose(F.mean(axis=(2, 3)), Fb, atol=1e-14)
+    assert np.abs(f2.project2d(P, G)).max() < 1e-6 * np.abs(P).max()
+    assert abs(S.mean(axis=(2, 3))[1, 1]) < 1e3 and Fb[1, 1] < 1.0
