@@ -328,6 +328,61 @@ def build_finite_strain_bc_3d(driving_component, eps_target_step, mixed_targets,
     return F_bar, F_mask, P_target, P_mask
 
 
+def build_finite_strain_bc(driving_component, eps_target_step, mixed_targets, plane_mode, ndim=2,
+                           F_bar_initial=None):
+    """2D macro boundary conditions of one load step (:1912-1974): the same
+    rules as the 3D builder (plane_mode does not enter them)."""
+    return build_finite_strain_bc_3d(driving_component, eps_target_step, mixed_targets, ndim=ndim,
+                                     F_bar_initial=F_bar_initial)
+
+
+def finite_strain_simulation_2d(E, nu, driving_component, eps_target, n_steps, mixed_targets=None,
+                                plane_mode="plane_strain", pixel=1.0, tol_NW=1e-5, tol_CG=1e-6, max_NW=20,
+                                tol_macro=1e6, max_iter_macro=20, store=True, log_path=None, global_log_path=None,
+                                enable_console=True, checkpoint_interval="none", checkpoint_path=None,
+                                vtk_interval="none", vtk_path=None, model_type="svk", A_m=0.0, B_m=0.0, C_m=0.0,
+                                solver="al", v1=0.0, v2=0.0, v3=0.0, g1=0.0, g2=0.0, g3=0.0, g4=0.0,
+                                strain_capping_enabled=False, strain_capping_limit=None,
+                                strain_capping_tangent_ratio=0.1, strain_capping_type="piecewise",
+                                strain_capping_smooth_power=1.0):
+    """2D incremental load stepping (:2580-2845) with F resident on the device
+    (the plane problem on an (nx, 1, ny) grid, as finite_strain_solver_step_2d).
+    Logs, checkpoints and VTK output are accepted and ignored.  Returns
+    (F_macro, Sig_macro, P_macro, F_list, Sig_list) in 2D shapes."""
+    if solver != "newton_cg":
+        raise NotImplementedError(f"solver={solver!r}: the 2D device step implements solver='newton_cg'")
+    if model_type not in ("svk", "landau", "murnaghan") or (model_type != "svk" and plane_mode != "plane_strain"):
+        raise NotImplementedError(f"2D {model_type} in {plane_mode}")
+    mixed_targets = {} if mixed_targets is None else mixed_targets
+    E = np.asarray(E, dtype=np.float64)
+    nu = np.asarray(nu, dtype=np.float64)
+    nx, ny = E.shape
+    sh3 = (nx, 1, ny)
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / (1.0 - nu ** 2) if plane_mode == "plane_stress" else E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    s = FsSolver(lam.reshape(sh3), mu.reshape(sh3), precision=0)
+    mat = material(model_type, A_m, B_m, C_m, v1, v2, v3, g1, g2, g3, g4, strain_capping_enabled,
+                   strain_capping_limit, strain_capping_tangent_ratio, strain_capping_type,
+                   strain_capping_smooth_power, mu_mean=float(mu.mean()))
+    mat.plane = 2 if plane_mode == "plane_stress" else 1
+    drv = (_PLANE_AXES[driving_component[0]], _PLANE_AXES[driving_component[1]])
+    Fm, Sm, Pm, Fl, Sl = [], [], [], [], []
+    F_bar_init = None
+    for eps_s in np.linspace(0.0, eps_target, n_steps + 1):
+        F_bar, _, P_tgt, P_mask = build_finite_strain_bc(driving_component, eps_s, mixed_targets, plane_mode,
+                                                         F_bar_initial=F_bar_init)
+        info = s.step(_embed2(F_bar), mat, drv, _embed2(P_tgt, 0.0), _embed2(P_mask.astype(float), 0.0).astype(bool),
+                      E.mean(), nu.mean(), tol_NW, tol_CG, max_NW, tol_macro, max_iter_macro, solver="newton_cg")
+        F_bar_init = _extract2(np.array(info.F_bar[:]).reshape(3, 3))
+        Fm.append(_extract2(np.array(info.F_mean[:]).reshape(3, 3)))
+        Sm.append(_extract2(np.array(info.sig_mean[:]).reshape(3, 3)))
+        Pm.append(_extract2(s.P.double().mean(dim=(2, 3, 4)).cpu().numpy()))
+        if store:
+            Fl.append(_extract2(dev.download(s.F)))
+            Sl.append(_extract2(dev.download(s.sig)))
+    return np.array(Fm), np.array(Sm), np.array(Pm), Fl, Sl
+
+
 def finite_strain_simulation_3d(E, nu, driving_component, eps_target, n_steps, mixed_targets=None, pixel=1.0,
                                 tol_NW=1e-5, tol_CG=1e-6, max_NW=20, tol_macro=1e6, max_iter_macro=20, store=True,
                                 log_path=None, global_log_path=None, enable_console=True,

@@ -319,3 +319,54 @@ def solver_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, driving_component
         if np.any(P_mask):
             raise ValueError(f"Outer BC loop did not converge after {max_iter_macro} iterations.")
     return res[0], res[1], res[2], res[3], Fbc
+
+
+def build_bc_2d(driving_component, eps, mixed_targets, F_bar_initial=None):
+    """build_finite_strain_bc (:1912-1974) -> (F_bar, P_target, P_mask): the
+    driven component (1 + eps on the diagonal, eps off it), other shears 0,
+    undriven diagonals stress-controlled when targeted, else 1."""
+    Fb = np.eye(2) if F_bar_initial is None else np.array(F_bar_initial, dtype=float)
+    Pt, Pm = np.zeros((2, 2)), np.zeros((2, 2), bool)
+    i, j = driving_component
+    Fb[i, j] = 1.0 + eps if i == j else eps
+    for a in range(2):
+        for b in range(2):
+            if a != b and (a, b) != (i, j):
+                Fb[a, b] = 0.0
+    for k in range(2):
+        if k == i and i == j:
+            continue
+        if (k, k) in mixed_targets:
+            Pt[k, k] = mixed_targets[(k, k)]
+            Pm[k, k] = True
+        else:
+            Fb[k, k] = 1.0
+    return Fb, Pt, Pm
+
+
+def simulation_2d(E, nu, driving_component, eps_target, n_steps, mixed_targets=None, plane_mode="plane_strain",
+                  pixel=1.0, tol_NW=1e-5, tol_CG=1e-6, max_NW=20, tol_macro=1e6, max_iter_macro=20,
+                  model_type="svk", **mat):
+    """finite_strain_simulation_2d (:2580-2845), store=False: load steps over
+    linspace(0, eps_target, n_steps + 1) with build_finite_strain_bc (:1912-1974)
+    from the previous F_bar -> (F_macro, Sig_macro, P_macro)."""
+    mixed_targets = {} if mixed_targets is None else mixed_targets
+    nx, ny = E.shape
+    I2, I4, I4rt, I4s, II = identity_tensors_2d(nx, ny)
+    C4 = build_C4_2d(E, nu, I4s, II, plane_mode)
+    G = build_ghat4_2d(nx, ny, nx * pixel, ny * pixel)
+    F = I2.copy()
+    Fb_init = None
+    Fm, Sm, Pm = [], [], []
+    for eps_s in np.linspace(0.0, eps_target, n_steps + 1):
+        Fb, Pt, Pmask = build_bc_2d(driving_component, eps_s, mixed_targets, Fb_init)
+        F, P, Sig, _, Fb = solver_step_2d(F, Fb, G, C4, I2, I4, I4rt, driving_component=driving_component,
+                                          P_target=Pt, P_mask=Pmask, E_avg=E.mean(), nu_avg=nu.mean(),
+                                          tol_NW=tol_NW, tol_CG=tol_CG, max_NW=max_NW, tol_macro=tol_macro,
+                                          max_iter_macro=max_iter_macro, model_type=model_type,
+                                          plane_mode=plane_mode, **mat)
+        Fb_init = Fb
+        Fm.append(F.mean(axis=(2, 3)))
+        Sm.append(Sig.mean(axis=(2, 3)))
+        Pm.append(P.mean(axis=(2, 3)))
+    return np.array(Fm), np.array(Sm), np.array(Pm)

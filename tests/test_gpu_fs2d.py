@@ -82,3 +82,23 @@ def test_2d_unsupported_laws_raise(gpu):
                dict(model_type="landau", plane_mode="plane_stress", solver="newton_cg")):
         with pytest.raises(NotImplementedError):
             finite_strain_solver_step_2d(I2.copy(), np.eye(2), None, C4, I2, I4, I4rt, **kw)
+
+
+@pytest.mark.parametrize("model,mode,kw,drv,targets", [
+    ("svk", "plane_stress", {}, (0, 0), {(1, 1): 0.0}),
+    ("landau", "plane_strain", dict(LANDAU, strain_capping_enabled=True), (0, 1), {(0, 0): 0.0, (1, 1): 0.0})])
+def test_2d_simulation_parity(gpu, model, mode, kw, drv, targets):
+    # finite_strain_simulation_2d: the macro F, Cauchy and P lists of every
+    # load step equal the oracle driver's
+    from mgkmc_amd.finite_strain_simulator import finite_strain_simulation_2d
+    sh = (16, 12)
+    rng = np.random.default_rng(2)
+    E = 70e9 * rng.uniform(0.8, 1.2, sh)
+    nu = np.full(sh, 0.3)
+    args = dict(mixed_targets=targets, plane_mode=mode, tol_NW=1e-6, tol_CG=1e-6, tol_macro=1e4, model_type=model,
+                **kw)
+    ref = f2.simulation_2d(E, nu, drv, 0.01, 3, **args)
+    got = finite_strain_simulation_2d(E, nu, drv, 0.01, 3, solver="newton_cg", store=False, **args)
+    print(f"[fs2d sim] {model} {mode} F_macro ref={ref[0][-1].ravel()} got={got[0][-1].ravel()}", flush=True)
+    for a, b in zip(got[:3], ref):
+        assert np.abs(a - b).max() <= 1e-10 * max(1.0, np.abs(b).max())
