@@ -45,6 +45,16 @@ KB = 8.617e-5
 LS_PRECISIONS = {"fp64": FP64, "fp32": FP32, "mixed": MIXED}
 
 
+class _SigView:
+    """The in-plane 2x2 Cauchy field (4, NV) of a 2D problem solved on an
+    (nx, 1, ny) grid, the layout the 2D barrier kernels read."""
+
+    sig = None
+
+    def sync(self, sig3):
+        self.sig = sig3[[0, 2]][:, [0, 2]].reshape(4, -1).contiguous()
+
+
 def gaussian_generator(mean=2.0, std=0.6, min_cutoff=0.1, max_cutoff=None, rng=None):
     """normal + clip on the legacy stream (mgkmc/stz/barrier_generators.py:4-30,
     kmc_simulator_functions.py:222-229)."""
@@ -164,18 +174,33 @@ class DeviceKmc:
         self.law = law
         self.finite = finite is not None
         if self.finite:
-            # de Geus finite-strain state (mgkmc/aqs.py:205-215): F, F_plastic = I; the step's own macro BC
-            from .finite_strain_simulator import FsSolver, MIXED as FS_MIXED
+            # de Geus finite-strain state (mgkmc/aqs.py:205-215, kmc_simulator.py:156-168):
+            # F, F_plastic = I; the step's own macro BC.  2D: the plane problem on an
+            # (nx, 1, ny) grid of the 3D step, the lam of the plane mode
+            from .finite_strain_simulator import FsSolver, MIXED as FS_MIXED, material
             if self.precision == FP32:
                 raise NotImplementedError("finite-strain KMC runs with ls_precision 'fp64' or 'mixed'")
-            lam, mu = compute_lame_3d(E, nu)
-            eye = np.broadcast_to(np.eye(3).reshape(3, 3, 1, 1, 1), (3, 3) + self.grid_shape)
-            self.fs = FsSolver(lam, mu, precision=FS_MIXED if self.precision == MIXED else 0, Fp=eye)
-            from .finite_strain_simulator import material
             finite = dict(finite)
             self._fs_solver = finite.pop("solver", "newton_cg")
+            if d == 3:
+                lam, mu = compute_lame_3d(E, nu)
+                fs_grid = self.grid_shape
+            else:
+                if self._fs_solver != "newton_cg" or self.precision != FP64:
+                    raise NotImplementedError("2D finite-strain KMC runs solver='newton_cg' in FP64")
+                lam, mu = compute_lame_2d(E, nu, plane_mode=plane_mode)
+                fs_grid = (self.grid_shape[0], 1, self.grid_shape[1])
+                lam, mu = lam.reshape(fs_grid), mu.reshape(fs_grid)
+            eye = np.broadcast_to(np.eye(3).reshape(3, 3, 1, 1, 1), (3, 3) + fs_grid)
+            self.fs = FsSolver(lam, mu, precision=FS_MIXED if self.precision == MIXED else 0, Fp=eye)
             self._fs_mat = material(**finite, mu_mean=float(np.mean(mu)))
-            self.F_macro = np.eye(3)
+            if d == 2:
+                model = finite.get("model_type", "svk")
+                if model not in ("svk", "landau", "murnaghan") or (model != "svk" and plane_mode != "plane_strain"):
+                    raise NotImplementedError(f"2D finite-strain {model} in {plane_mode}")
+                self._fs_mat.plane = 2 if plane_mode == "plane_stress" else 1
+                self._fs_sig2 = _SigView()
+            self.F_macro = np.eye(d)
             self.sig_kind = SIG_GEN64
             self.ls = None
         elif law is None:
@@ -282,13 +307,22 @@ class DeviceKmc:
     @property
     def _stress(self):
         if self.finite:
-            return self.fs
+            return self.fs if self.ndim == 3 else self._fs_sig2
         return self.ls if self._sig_owner is None else self._sig_owner
+
+    def _fs_field(self, t):
+        """(3, 3, *fs grid) device field -> host (*grid, d, d)."""
+        a = dev.download(t)
+        if self.ndim == 2:
+            from .finite_strain_simulator import _extract2
+            a = _extract2(a)
+            return np.einsum("ijxy->xyij", a)
+        return np.einsum("ijxyz->xyzij", a)
 
     @property
     def sig_field(self):
         if self.finite:
-            return np.einsum("ijxyz->xyzij", dev.download(self.fs.sig))
+            return self._fs_field(self.fs.sig)
         return dev.from_sym_soa(dev.download(self._stress.sig), self.ndim)
 
     @sig_field.setter
@@ -306,8 +340,8 @@ class DeviceKmc:
     @property
     def eps_field(self):
         if self.finite:  # Green-Lagrange strain of F, as the reference stores it
-            F = np.einsum("ijxyz->xyzij", dev.download(self.fs.F))
-            return 0.5 * (np.einsum("...ki,...kj->...ij", F, F) - np.eye(3))
+            F = self._fs_field(self.fs.F)
+            return 0.5 * (np.einsum("...ki,...kj->...ij", F, F) - np.eye(self.ndim))
         return dev.from_sym_soa(dev.download(self._stress.eps), self.ndim)
 
     @eps_field.setter
@@ -316,11 +350,11 @@ class DeviceKmc:
 
     @property
     def F_field(self):
-        return np.einsum("ijxyz->xyzij", dev.download(self.fs.F)) if self.finite else None
+        return self._fs_field(self.fs.F) if self.finite else None
 
     @property
     def F_plastic(self):
-        return np.einsum("ijxyz->xyzij", dev.download(self.fs.Fp)) if self.finite else None
+        return self._fs_field(self.fs.Fp) if self.finite else None
 
     @property
     def eps_plastic(self):
@@ -433,19 +467,28 @@ class DeviceKmc:
         step from the current F with F_plastic; F_macro and the macro strain
         (F_ii - 1, F_ij) follow the step; ValueError on a NaN field or
         |<sigma>| > 20 GPa."""
-        from .finite_strain_simulator import build_finite_strain_bc_3d
+        from .finite_strain_simulator import build_finite_strain_bc_3d, _embed2, _extract2, _PLANE_AXES
+        d = self.ndim
         drv = getattr(self, "driving_component", (0, 0))
         tg = getattr(self, "stress_targets", {})
-        Fb, _, Pt, Pm = build_finite_strain_bc_3d(drv, self.eps_macro[drv], tg, F_bar_initial=self.F_macro)
-        info = self.fs.step(Fb, self._fs_mat, drv, Pt, Pm, E_avg=self.E_field.mean(), nu_avg=self.nu_field.mean(),
+        Fb, _, Pt, Pm = build_finite_strain_bc_3d(drv, self.eps_macro[drv], tg, ndim=d, F_bar_initial=self.F_macro)
+        drv3 = drv
+        if d == 2:  # kmc_simulator.py:389-445 through the (nx, 1, ny) embedding
+            Fb, Pt, Pm = _embed2(Fb), _embed2(Pt, 0.0), _embed2(Pm.astype(float), 0.0).astype(bool)
+            drv3 = (_PLANE_AXES[drv[0]], _PLANE_AXES[drv[1]])
+        info = self.fs.step(Fb, self._fs_mat, drv3, Pt, Pm, E_avg=self.E_field.mean(), nu_avg=self.nu_field.mean(),
                             solver=self._fs_solver, pixel=self.pixel)
-        self.F_macro = np.array(info.F_bar[:]).reshape(3, 3)
-        for i in range(3):
-            for j in range(3):
+        Fm = np.array(info.F_bar[:]).reshape(3, 3)
+        sm = np.array(info.sig_mean[:9]).reshape(3, 3)
+        if d == 2:
+            Fm, sm = _extract2(Fm), _extract2(sm)
+            self._fs_sig2.sync(self.fs.sig)
+        self.F_macro = Fm
+        for i in range(d):
+            for j in range(d):
                 self.eps_macro[i, j] = self.F_macro[i, j] - 1.0 if i == j else self.F_macro[i, j]
         self.ls_info = info
         self._solved_once = True
-        sm = np.array(info.sig_mean[:9]).reshape(3, 3)
         torch = dev.torch
         if bool(torch.isnan(self.fs.sig).any()) or bool(torch.isnan(self.fs.F).any()) or \
                 np.max(np.abs(sm)) / 1e9 > 20.0:
@@ -453,10 +496,11 @@ class DeviceKmc:
         return info
 
     def _flip_finite(self, v, m):
-        """Finite-strain flip (mgkmc/aqs.py:887-996): F_p <- expm(C/N) F_p at
-        the voxel in N sub-steps, N from max(2, ceil(sqrt(C:C/2) / 0.005)) (at
-        most 40), retried with N + 1 .. N + 19 on ValueError from the state
-        before the flip; softening, prev and redraws at the last sub-step, a
+        """Finite-strain flip (mgkmc/aqs.py:887-996, 2D kmc_simulator.py:895-1000):
+        F_p <- expm(C/N) F_p at the voxel in N sub-steps, retried from the state
+        before the flip on ValueError; 3D: N = n0 .. n0 + 19, n0 =
+        min(max(2, ceil(sqrt(C:C/2) / 0.005)), 40); 2D: N = 1, then n0 .. n0 + 19
+        with the step 0.02.  Softening, prev and redraws at the last sub-step, a
         finite-strain step after each."""
         from scipy.linalg import expm
         torch = dev.torch
@@ -465,19 +509,28 @@ class DeviceKmc:
         live = [fs.F, fs.P, fs.sig, fs.Fp, k.soft, k.tlast, k.prev, k.cat, k.Q0]
         backup = [None if a is None else a.clone() for a in live]
         b_fm, b_em = self.F_macro.copy(), self.eps_macro.copy()
-        x, y, z = (int(c) for c in self.kmc.voxel_coords(v))
-        n0 = min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.005))), 40)
-        for N in range(n0, n0 + 20):
+        c = tuple(int(q) for q in self.kmc.voxel_coords(v))
+        if self.ndim == 3:
+            n0 = min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.005))), 40)
+            schedule = list(range(n0, n0 + 20))
+            sel = (slice(None), slice(None)) + c
+        else:
+            n0 = max(min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.02))), 40), 2)
+            schedule = [1] + list(range(n0, n0 + 20))
+            sel = ([[0], [2]], [0, 2], c[0], 0, c[1])  # the in-plane block of the (nx, 1, ny) grid
+        for N in schedule:
             for a, b in zip(live, backup):
                 if a is not None:
                     a.copy_(b)
             self.F_macro = b_fm.copy()
             self.eps_macro[...] = b_em
+            if self.ndim == 2:
+                self._fs_sig2.sync(fs.sig)
             try:
                 for i in range(1, N + 1):
                     dF = expm(C / N)
-                    fpv = fs.Fp[:, :, x, y, z].cpu().numpy()
-                    fs.Fp[:, :, x, y, z] = torch.from_numpy(np.dot(dF, fpv)).to(fs.Fp.device)
+                    fpv = fs.Fp[sel].cpu().numpy()
+                    fs.Fp[sel] = torch.from_numpy(np.dot(dF, fpv)).to(fs.Fp.device)
                     if i == N:
                         k.flip(v, m, self.time, self.jp, self.jt, self.softening_cap,
                                self.neighbor_softening_fraction, 0)
@@ -528,6 +581,12 @@ class DeviceKmc:
 
     def sig_mean(self):
         d = self.ndim
+        if self.finite:
+            sm = np.array(self.ls_info.sig_mean[:9]).reshape(3, 3)
+            if d == 2:
+                from .finite_strain_simulator import _extract2
+                sm = _extract2(sm)
+            return sm
         return np.array(self.ls_info.sig_mean[: d * d]).reshape(d, d)
 
     def _params(self):

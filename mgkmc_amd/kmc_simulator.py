@@ -8,7 +8,9 @@ Same constructor and run_simulation signatures as the reference
 branches: linear, secant degradation, Rose, Landau) is the device LS solve of
 that law with the reference's plane mode and its unstable-stress check, and
 flips under a nonlinear law are sub-stepped on solver failure (:263-385); the mixed-BC loop (:1106-1143) keeps the reference's
-compliance-then-secant update on the host.  Extensions as in
+compliance-then-secant update on the host.  strain_assumption="finite_strain"
+(:156-168, :389-445, :895-1000) runs the 2D Newton-CG step (solver="newton_cg")
+with F_p updated by expm(C/N) sub-steps.  Extensions as in
 ``mgkmc_amd.aqs.ThermalSimulation``.
 """
 import numpy as np
@@ -30,8 +32,12 @@ class KmcSimulation2D(DeviceKmc):
                  strain_capping_enabled=False, strain_capping_limit=None, strain_capping_tangent_ratio=0.1,
                  strain_capping_type="piecewise", strain_capping_smooth_power=1.0,
                  ls_precision="fp64", ls_warm_start=False, rng=None, selection_stream=None, init_modes=None):
-        self._unsupported(cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model,
-                          [f"stz_mode={stz_mode}"] if stz_mode != "pure_shear" else [])
+        # finite strain: the Newton-CG step of the plane problem, KMC mode
+        self._finite_ok = strain_assumption == "finite_strain" and solver == "newton_cg" and not cascade_mode
+        extra = [f"stz_mode={stz_mode}"] if stz_mode != "pure_shear" else []
+        if strain_assumption == "finite_strain" and solver != "newton_cg":
+            extra.append(f"2D finite-strain solver={solver!r} (the device step is 'newton_cg')")
+        self._unsupported(cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model, extra)
         self.nx, self.ny = nx, ny
         self.output_dir = output_dir
         self.strain_assumption = strain_assumption
@@ -47,11 +53,17 @@ class KmcSimulation2D(DeviceKmc):
             setattr(self, key, val)
         law = {"secant_degradation": ("secant", d, k), "rose": ("rose", eta, mu_floor_fraction),
                "landau": ("landau", (v1, v2, v3, g1, g2, g3, g4), cap)}.get(hyperelastic_model)
+        finite = None
+        self.solver, self.A_m, self.B_m, self.C_m = solver, A_m, B_m, C_m
+        if strain_assumption == "finite_strain":
+            law = None
+            finite = dict(model_type=hyperelastic_model, A_m=A_m, B_m=B_m, C_m=C_m, v1=v1, v2=v2, v3=v3, g1=g1,
+                          g2=g2, g3=g3, g4=g4, solver=solver, **cap)
         self._setup((nx, ny), M, gamma0, E_field, nu_field, pixel, barrier_generator, barrier_kwargs,
                     softening_scheme, softening_cap, jp, jt, neighbor_softening_fraction, temperature, strain_rate,
                     stability_threshold, redraw_directions, redraw_barriers, nu0, q_act_temp, cascade_mode,
                     scale_rate_by_volume, plane_mode, ls_precision, ls_warm_start, rng, selection_stream,
-                    init_modes, dict(mean=2.0, std=0.6, min_cutoff=0.1), law=law)
+                    init_modes, dict(mean=2.0, std=0.6, min_cutoff=0.1), law=law, finite=finite)
         if enable_thermal:
             self._setup_thermal(Cp, rho, thermal_diffusivity, thermal_coords, temperature_cap, thermostat, tau_bath)
 
@@ -147,7 +159,10 @@ class KmcSimulation2D(DeviceKmc):
                     return self.flip_event_history
                 if reason == "time":
                     break
-            self.mixed_bc(stress_targets, mixed_tol, mixed_max_iter)
+            if self.finite:  # the step enforces the stress targets itself (kmc_simulator.py:1103-1104)
+                self.elastic_run(self.eps_macro)
+            else:
+                self.mixed_bc(stress_targets, mixed_tol, mixed_max_iter)
             done += 1
             step += 1
         return self.flip_event_history

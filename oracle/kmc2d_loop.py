@@ -33,6 +33,17 @@ field sets the rates, diffuses over each waiting time and each remaining
 increment time (oracle/aqs_loop.py heat_step), and each flip heats its voxel
 by |sigma:C| / (rho Cp), capped (linear law).
 
+strain_assumption="finite_strain" (:156-168, :389-445, :895-1000, :1103-1104):
+elastic_run is the 2D Newton-CG step (oracle/finite_strain2d.py solver_step_2d
+with the reference's default tolerances; ``model`` = "svk", "landau" or
+"murnaghan", constants in ``model_params``) from the current F with F_plastic,
+macro BC from the driven strain with F_macro as the initial F_bar, F_macro and
+the macro strain from the returned F_bar, ValueError on NaN or |<sigma>| > 20
+GPa; a flip multiplies F_p by expm(C/N) N times (N = 1, then N_h .. N_h + 19
+on ValueError from the restored state), softening, prev and redraws at the
+last, a step after each; elastic increments end with one elastic_run (no
+mixed loop).
+
 ``fail_once_at`` (test hook) makes the first re-solve after that event's flip
 raise ValueError, to drive the sub-stepped path.
 
@@ -55,7 +66,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                   eps_plastic=None, model="linear", model_params=None, fail_once_at=None, cascade_mode=False,
                   stability_threshold=0.0, max_cascade_steps_pct=0.3, enable_thermal=False, Cp=420.0, rho=6125.0,
                   thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
-                  tau_bath=0.0):
+                  tau_bath=0.0, strain_assumption="small_strain"):
     """Returns (events [(x, y, m, time, is_instab)], final state dict)."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
@@ -86,20 +97,50 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
     targets = dict(stress_targets or {})
 
     mp = dict(model_params or {})
-    if model == "secant_degradation":
+    finite = strain_assumption == "finite_strain"
+    if finite:
+        from . import finite_strain2d as f2
+        from scipy.linalg import expm
+        fI2, fI4, fI4rt, fI4s, fII = f2.identity_tensors_2d(nx, ny)
+        fC4 = f2.build_C4_2d(E, nu, fI4s, fII, plane_mode)
+        fG = f2.build_ghat4_2d(nx, ny, nx * pixel, ny * pixel)
+        state["F"] = fI2.copy()
+        state["Fp"] = fI2.copy()
+        state["F_macro"] = np.eye(2)
+    elif model == "secant_degradation":
         lam, mu = ol.compute_lame_2d(E, nu, plane_mode)
     elif model in ("rose", "landau"):
         lam, mu = ol.compute_lame_2d(E, nu, "plane_strain")
     elif model != "linear":
         raise ValueError(model)
+    if finite and (cascade_mode or enable_thermal):
+        raise NotImplementedError("finite strain with cascades or thermal coupling")
     state["eps"] = np.zeros((nx, ny, 2, 2))
     e33_state = {}
     inject = {"armed": False}
+
+    def elastic_run_finite():
+        Fb, Pt, Pm = f2.build_bc_2d(component, eps_macro[component], targets, state["F_macro"])
+        F, P, Sig, _, Fb = f2.solver_step_2d(state["F"], Fb, fG, fC4, fI2, fI4, fI4rt, Fp=state["Fp"],
+                                             driving_component=component, P_target=Pt, P_mask=Pm, E_avg=E.mean(),
+                                             nu_avg=nu.mean(), model_type=model, plane_mode=plane_mode, **mp)
+        state["F"] = F
+        state["sig"] = np.einsum("ijxy->xyij", Sig)
+        state["F_macro"] = Fb
+        for i in range(2):
+            for j in range(2):
+                eps_macro[i, j] = Fb[i, j] - 1.0 if i == j else Fb[i, j]
+        sm = state["sig"].mean(axis=(0, 1))
+        if np.any(np.isnan(state["sig"])) or np.any(np.isnan(F)) or np.max(np.abs(sm)) / 1e9 > 20.0:
+            raise ValueError("Simulation stopped due to unstable stress")
+        return sm
 
     def elastic_run():
         if inject["armed"]:
             inject["armed"] = False
             raise ValueError("injected solver failure")
+        if finite:
+            return elastic_run_finite()
         if model == "linear":
             e, s, _, _ = ol.spectral_solver_2d(E, nu, eps_macro, eps_plastic=ep, pixel=pixel, plane_mode=plane_mode,
                                                tol=ls_tol)
@@ -168,6 +209,34 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                 continue
         raise ValueError("Mechanical solver failed to converge under small-strain sub-stepping.")
 
+    def flip_finite(x, y, m):
+        C = cat[x, y, m].copy()
+        arrays = (soft, tl, prev, cat, Q0)
+        backup = [a.copy() for a in arrays]
+        b = {k: state[k].copy() for k in ("F", "Fp", "sig", "F_macro")}
+        b_em = eps_macro.copy()
+        nh = min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.02))), 40)
+        for N in [1] + list(range(max(nh, 2), max(nh, 2) + 20)):
+            for a, bb in zip(arrays, backup):
+                a[...] = bb
+            for k in b:
+                state[k] = b[k].copy()
+            eps_macro[...] = b_em
+            try:
+                for k in range(1, N + 1):
+                    state["Fp"][:, :, x, y] = expm(C / N) @ state["Fp"][:, :, x, y]
+                    if k == N:
+                        soften(x, y, C)
+                        prev[x, y] = C
+                        cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
+                        Q0[x, y] = gen((M,))
+                    elastic_run()
+                state.setdefault("substeps", []).append(N)
+                return
+            except ValueError:
+                continue
+        raise ValueError("Mechanical solver failed to converge under sub-stepping.")
+
     # thermal extension (:60-75, :559-575, :749-753, :1029-1033): as oracle/aqs_loop.py
     from .aqs_loop import heat_operator, heat_step
     Tl = np.full((nx, ny), float(temperature))
@@ -232,7 +301,8 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
 
     def result():
         return events, dict(eps_plastic=ep, Q0=Q0, soft=soft, sig=state["sig"], eps_macro=eps_macro.copy(), time=t,
-                            substeps=state.get("substeps"), cascades=cascades, Tlocal=Tl)
+                            substeps=state.get("substeps"), cascades=cascades, Tlocal=Tl, F=state.get("F"),
+                            F_plastic=state.get("Fp"), F_macro=state.get("F_macro"))
 
     while done < n_global_steps:
         remaining += dt_step
@@ -259,7 +329,9 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     inst = Q[x, y, m] <= stability_threshold
                     if fail_once_at is not None and len(events) == fail_once_at:
                         inject["armed"] = True
-                    if model == "linear":
+                    if finite:
+                        flip_finite(x, y, m)
+                    elif model == "linear":
                         C = cat[x, y, m].copy()
                         ok.apply_flip_soa_2d(ep, soft, tl, cat, x, y, m, t, jp, jt, softening_cap, jn)
                         prev[x, y] = C
@@ -278,8 +350,10 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
             heat(remaining)
             t += remaining
             remaining = 0.0
+        if finite:
+            elastic_run()
         prev_e, prev_s = {}, {}
-        for _ in range(mixed_max_iter):
+        for _ in range(0 if finite else mixed_max_iter):
             sm = elastic_run()
             err = np.zeros((2, 2))
             mx = 0.0
