@@ -36,7 +36,7 @@ class ThermalSimulation(DeviceKmc):
                  ls_precision="fp64", ls_warm_start=False, rng=None, selection_stream=None, init_modes=None):
         # finite strain: the Newton-Krylov, augmented-Lagrangian or DBFFT step, KMC mode
         fs_solvers = ("newton_cg", "al", "dbfft")
-        self._finite_ok = strain_assumption == "finite_strain" and solver in fs_solvers and not cascade_mode
+        self._finite_ok = strain_assumption == "finite_strain" and solver in fs_solvers
         extra = ["use_3d_barriers"] if use_3d_barriers else []
         if strain_assumption == "finite_strain" and solver not in fs_solvers:
             extra.append(f"finite-strain solver={solver!r} (the device step is 'newton_cg', 'al' or 'dbfft')")

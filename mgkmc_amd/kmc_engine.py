@@ -200,6 +200,7 @@ class DeviceKmc:
                     raise NotImplementedError(f"2D finite-strain {model} in {plane_mode}")
                 self._fs_mat.plane = 2 if plane_mode == "plane_stress" else 1
                 self._fs_sig2 = _SigView()
+                self._fs_sig2.sync(self.fs.sig)
             self.F_macro = np.eye(d)
             self.sig_kind = SIG_GEN64
             self.ls = None
@@ -496,28 +497,36 @@ class DeviceKmc:
         return info
 
     def _flip_finite(self, v, m):
-        """Finite-strain flip (mgkmc/aqs.py:887-996, 2D kmc_simulator.py:895-1000):
-        F_p <- expm(C/N) F_p at the voxel in N sub-steps, retried from the state
-        before the flip on ValueError; 3D: N = n0 .. n0 + 19, n0 =
-        min(max(2, ceil(sqrt(C:C/2) / 0.005)), 40); 2D: N = 1, then n0 .. n0 + 19
-        with the step 0.02.  Softening, prev and redraws at the last sub-step, a
-        finite-strain step after each."""
+        """One KMC event's finite-strain flip (see _flip_finite_pairs)."""
+        self._flip_finite_pairs([(v, m)])
+
+    def _flip_finite_pairs(self, pairs):
+        """Finite-strain flips of one event or one cascade batch (mgkmc/aqs.py:512-615,
+        887-996; 2D kmc_simulator.py:605-700, 895-1000): F_p <- expm(C/N) F_p at
+        every flipped voxel in N sub-steps, retried from the state before the
+        flips on ValueError; 3D: N = n0 .. n0 + 19, n0 = min(max over the
+        flips of max(2, ceil(sqrt(C:C/2) / 0.005)), 40); 2D: N = 1, then
+        n0 .. n0 + 19 with the step 0.02.  Softening, prev and redraws of each
+        voxel at the last sub-step, one finite-strain step after each sub-step."""
         from scipy.linalg import expm
         torch = dev.torch
-        C = self._mode_host(v, m)
+        Cs = [self._mode_host(v, m) for v, m in pairs]
         k, fs = self.kmc, self.fs
         live = [fs.F, fs.P, fs.sig, fs.Fp, k.soft, k.tlast, k.prev, k.cat, k.Q0]
         backup = [None if a is None else a.clone() for a in live]
         b_fm, b_em = self.F_macro.copy(), self.eps_macro.copy()
-        c = tuple(int(q) for q in self.kmc.voxel_coords(v))
+        step = 0.005 if self.ndim == 3 else 0.02
+        n0 = min(max(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / step))) for C in Cs), 40)
         if self.ndim == 3:
-            n0 = min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.005))), 40)
             schedule = list(range(n0, n0 + 20))
-            sel = (slice(None), slice(None)) + c
         else:
-            n0 = max(min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.02))), 40), 2)
+            n0 = max(n0, 2)
             schedule = [1] + list(range(n0, n0 + 20))
-            sel = ([[0], [2]], [0, 2], c[0], 0, c[1])  # the in-plane block of the (nx, 1, ny) grid
+        sels = []
+        for v, _ in pairs:
+            c = tuple(int(q) for q in self.kmc.voxel_coords(v))
+            # 2D: the in-plane block of the (nx, 1, ny) grid
+            sels.append((slice(None), slice(None)) + c if self.ndim == 3 else ([[0], [2]], [0, 2], c[0], 0, c[1]))
         for N in schedule:
             for a, b in zip(live, backup):
                 if a is not None:
@@ -528,13 +537,14 @@ class DeviceKmc:
                 self._fs_sig2.sync(fs.sig)
             try:
                 for i in range(1, N + 1):
-                    dF = expm(C / N)
-                    fpv = fs.Fp[sel].cpu().numpy()
-                    fs.Fp[sel] = torch.from_numpy(np.dot(dF, fpv)).to(fs.Fp.device)
-                    if i == N:
-                        k.flip(v, m, self.time, self.jp, self.jt, self.softening_cap,
-                               self.neighbor_softening_fraction, 0)
-                        self._redraw(v)
+                    for (v, m), C, sel in zip(pairs, Cs, sels):
+                        dF = expm(C / N)
+                        fpv = fs.Fp[sel].cpu().numpy()
+                        fs.Fp[sel] = torch.from_numpy(np.dot(dF, fpv)).to(fs.Fp.device)
+                        if i == N:
+                            k.flip(v, m, self.time, self.jp, self.jt, self.softening_cap,
+                                   self.neighbor_softening_fraction, 0)
+                            self._redraw(v)
                     self.update_stresses()
                 self.last_substeps = N
                 return
@@ -732,7 +742,9 @@ class DeviceKmc:
             pairs = self._cascade_pairs()
             if not pairs:
                 break
-            if self.law is not None and self.ndim == 2:
+            if self.finite:
+                self._flip_finite_pairs(pairs)
+            elif self.law is not None and self.ndim == 2:
                 self._flip_substepped(pairs)
             else:
                 for v, m in pairs:

@@ -32,8 +32,8 @@ class KmcSimulation2D(DeviceKmc):
                  strain_capping_enabled=False, strain_capping_limit=None, strain_capping_tangent_ratio=0.1,
                  strain_capping_type="piecewise", strain_capping_smooth_power=1.0,
                  ls_precision="fp64", ls_warm_start=False, rng=None, selection_stream=None, init_modes=None):
-        # finite strain: the Newton-CG step of the plane problem, KMC mode
-        self._finite_ok = strain_assumption == "finite_strain" and solver == "newton_cg" and not cascade_mode
+        # finite strain: the Newton-CG step of the plane problem, KMC or cascade mode
+        self._finite_ok = strain_assumption == "finite_strain" and solver == "newton_cg"
         extra = [f"stz_mode={stz_mode}"] if stz_mode != "pure_shear" else []
         if strain_assumption == "finite_strain" and solver != "newton_cg":
             extra.append(f"2D finite-strain solver={solver!r} (the device step is 'newton_cg')")

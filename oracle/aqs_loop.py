@@ -167,12 +167,13 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
         if np.any(np.isnan(So)) or np.any(np.isnan(Fo)) or np.max(np.abs(state["sm"])) / 1e9 > 20.0:
             raise ValueError("Simulation stopped due to unstable stress")
 
-    def flip_finite(x, y, z, m):
-        C = cat[x, y, z, m].copy()
+    def flip_finite(flips):
+        # one KMC event or one cascade batch (:512-615, :887-996)
+        Cs = [cat[x, y, z, m].copy() for x, y, z, m in flips]
         arrays = (Ff, Fp, soft, tl, prev, cat, Q0)
         backup = [a.copy() for a in arrays]
         b_sig, b_sm, b_fm, b_em = state["sig"].copy(), state["sm"].copy(), state["F_macro"].copy(), eps_macro.copy()
-        n0 = min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.005))), 40)
+        n0 = min(max(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.005))) for C in Cs), 40)
         for N in range(n0, n0 + 20):
             for a, b in zip(arrays, backup):
                 a[...] = b
@@ -180,6 +181,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
             eps_macro[...] = b_em
             try:
                 for k in range(1, N + 1):
+                  for (x, y, z, m), C in zip(flips, Cs):
                     dF = expm(C / N)
                     Fp[:, :, x, y, z] = np.dot(dF, Fp[:, :, x, y, z])
                     if k == N:
@@ -205,7 +207,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                         prev[x, y, z] = C
                         cat[x, y, z] = ok.stz_catalog_glass(M, gamma0, np.random)
                         Q0[x, y, z] = gen((M,))
-                    update()
+                  update()
                 state.setdefault("substeps", []).append(N)
                 return
             except ValueError:
@@ -232,14 +234,18 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                     continue
                 seen.add((x, y, z))
                 flips.append((x, y, z, m))
-            for x, y, z, m in flips:
-                ok.apply_flip_soa(ep, None, soft, tl, cat, x, y, z, m, t, jp, jt, softening_cap, jn)
-                prev[x, y, z] = cat[x, y, z, m].copy()
-                flip_heat(x, y, z, prev[x, y, z])
-                cat[x, y, z] = ok.stz_catalog_glass(M, gamma0, np.random)
-                Q0[x, y, z] = gen((M,))
-                cascades.append((step, local, int(x), int(y), int(z), int(m)))
-            update()
+            if finite:
+                flip_finite(flips)
+                cascades.extend((step, local, int(x), int(y), int(z), int(m)) for x, y, z, m in flips)
+            else:
+                for x, y, z, m in flips:
+                    ok.apply_flip_soa(ep, None, soft, tl, cat, x, y, z, m, t, jp, jt, softening_cap, jn)
+                    prev[x, y, z] = cat[x, y, z, m].copy()
+                    flip_heat(x, y, z, prev[x, y, z])
+                    cat[x, y, z] = ok.stz_catalog_glass(M, gamma0, np.random)
+                    Q0[x, y, z] = gen((M,))
+                    cascades.append((step, local, int(x), int(y), int(z), int(m)))
+                update()
             local += 1
             if local > int(max_cascade_steps_pct * nx * ny * nz):
                 return True
@@ -292,7 +298,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                     x, y, z, m = ok.decode_index(idx[i], ny, nz, M)
                     inst = Q[x, y, z, m] <= stability_threshold
                     if finite:
-                        flip_finite(x, y, z, m)
+                        flip_finite([(x, y, z, m)])
                     else:
                         C = cat[x, y, z, m].copy()
                         ok.apply_flip_soa(ep, None, soft, tl, cat, x, y, z, m, t, jp, jt, softening_cap, jn)

@@ -113,8 +113,8 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
         lam, mu = ol.compute_lame_2d(E, nu, "plane_strain")
     elif model != "linear":
         raise ValueError(model)
-    if finite and (cascade_mode or enable_thermal):
-        raise NotImplementedError("finite strain with cascades or thermal coupling")
+    if finite and enable_thermal:
+        raise NotImplementedError("finite strain with thermal coupling")
     state["eps"] = np.zeros((nx, ny, 2, 2))
     e33_state = {}
     inject = {"armed": False}
@@ -209,13 +209,14 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                 continue
         raise ValueError("Mechanical solver failed to converge under small-strain sub-stepping.")
 
-    def flip_finite(x, y, m):
-        C = cat[x, y, m].copy()
+    def flip_finite(flips):
+        # one KMC event or one cascade batch (:605-700, :895-1000)
+        Cs = [cat[x, y, m].copy() for x, y, m in flips]
         arrays = (soft, tl, prev, cat, Q0)
         backup = [a.copy() for a in arrays]
         b = {k: state[k].copy() for k in ("F", "Fp", "sig", "F_macro")}
         b_em = eps_macro.copy()
-        nh = min(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.02))), 40)
+        nh = min(max(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.02))) for C in Cs), 40)
         for N in [1] + list(range(max(nh, 2), max(nh, 2) + 20)):
             for a, bb in zip(arrays, backup):
                 a[...] = bb
@@ -224,12 +225,13 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
             eps_macro[...] = b_em
             try:
                 for k in range(1, N + 1):
-                    state["Fp"][:, :, x, y] = expm(C / N) @ state["Fp"][:, :, x, y]
-                    if k == N:
-                        soften(x, y, C)
-                        prev[x, y] = C
-                        cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
-                        Q0[x, y] = gen((M,))
+                    for (x, y, m), C in zip(flips, Cs):
+                        state["Fp"][:, :, x, y] = expm(C / N) @ state["Fp"][:, :, x, y]
+                        if k == N:
+                            soften(x, y, C)
+                            prev[x, y] = C
+                            cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
+                            Q0[x, y] = gen((M,))
                     elastic_run()
                 state.setdefault("substeps", []).append(N)
                 return
@@ -271,7 +273,9 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     continue
                 seen.add((x, y))
                 flips.append((x, y, m))
-            if model == "linear":
+            if finite:
+                flip_finite(flips)
+            elif model == "linear":
                 for x, y, m in flips:
                     ok.apply_flip_soa_2d(ep, soft, tl, cat, x, y, m, t, jp, jt, softening_cap, jn)
                     prev[x, y] = cat[x, y, m].copy()
@@ -330,7 +334,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     if fail_once_at is not None and len(events) == fail_once_at:
                         inject["armed"] = True
                     if finite:
-                        flip_finite(x, y, m)
+                        flip_finite([(x, y, m)])
                     elif model == "linear":
                         C = cat[x, y, m].copy()
                         ok.apply_flip_soa_2d(ep, soft, tl, cat, x, y, m, t, jp, jt, softening_cap, jn)

@@ -603,3 +603,53 @@ def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode):
     assert np.abs(sim.F_plastic - np.einsum("ijxy->xyij", st_ref["F_plastic"])).max() <= 1e-14
     assert np.abs(sim.F_macro - st_ref["F_macro"]).max() <= 1e-9
     assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-6 * np.abs(st_ref["sig"]).max()
+
+
+@pytest.mark.parametrize("d,T,mean,g0", [(2, 400.0, 0.3, 0.05), (2, 0.0, 0.5, 0.1), (3, 800.0, 0.5, 0.03)])
+def test_finite_strain_cascade_mode_matches_oracle(gpu, d, T, mean, g0):
+    # cascade_mode with finite strain: every cascade batch goes through the
+    # expm sub-stepped F_p update (one finite-strain step per sub-step); the
+    # cascade histories, the KMC events, sub-step counts, F and F_p equal the
+    # oracle loops'
+    from mgkmc_amd.aqs import ThermalSimulation
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    from oracle.aqs_loop import run_events
+    from oracle.kmc2d_loop import run_events_2d
+    print(f"[finite cascade] {d}D T={T}", flush=True)
+    kw = dict(mean=mean, std=0.15, min_cutoff=0.0)
+    args = dict(softening_cap=0.916, jp=20.0, jt=0.0)
+    if d == 2:
+        n, M = 12, 10
+        E = 70e9 * np.random.default_rng(0).uniform(0.9, 1.1, (n, n))
+        nu = np.full((n, n), 0.3)
+        ev_ref, st_ref = run_events_2d(n, n, M, g0, E, nu, 0.7, T, 1e7, 2e-3, (0, 0), 10, seed=5, sel_seed=6,
+                                       barrier_kwargs=kw, stress_targets={(1, 1): 0.0}, n_global_steps=4,
+                                       cascade_mode=True, max_cascade_steps_pct=0.05, model="svk",
+                                       strain_assumption="finite_strain", **args)
+        sim = KmcSimulation2D(n, n, M, g0, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                              temperature=T, strain_rate=1e7, cascade_mode=True, strain_assumption="finite_strain",
+                              solver="newton_cg", rng=np.random.RandomState(5),
+                              selection_stream=np.random.RandomState(6), **args)
+        ev = sim.run_simulation(4, 2e-3, component=(0, 0), stress_targets={(1, 1): 0.0},
+                                max_cascade_steps_pct=0.05, max_events=10)
+        Fr, Fpr = np.einsum("ijxy->xyij", st_ref["F"]), np.einsum("ijxy->xyij", st_ref["F_plastic"])
+    else:
+        n, M = 8, 10
+        E = 70e9 * np.random.default_rng(1).uniform(0.9, 1.1, (n, n, n))
+        nu = np.full((n, n, n), 0.3)
+        ev_ref, st_ref = run_events(n, n, n, M, g0, E, nu, 0.7, T, 1e7, 1e-3, (0, 0), 10, seed=7, sel_seed=8,
+                                    barrier_kwargs=kw, n_global_steps=3, cascade_mode=True,
+                                    strain_assumption="finite_strain", **args)
+        sim = ThermalSimulation(n, n, n, M, g0, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                                temperature=T, strain_rate=1e7, cascade_mode=True, strain_assumption="finite_strain",
+                                solver="newton_cg", rng=np.random.RandomState(7),
+                                selection_stream=np.random.RandomState(8), **args)
+        ev = sim.run_simulation(3, 1e-3, component=(0, 0), max_events=10)
+        Fr, Fpr = np.einsum("ijxyz->xyzij", st_ref["F"]), np.einsum("ijxyz->xyzij", st_ref["F_plastic"])
+    print(f"[finite cascade] events {len(ev)} (oracle {len(ev_ref)}), cascade flips {len(sim.cascade_history)} "
+          f"(oracle {len(st_ref['cascades'])})", flush=True)
+    assert len(st_ref["cascades"]) > 0
+    assert sim.cascade_history == st_ref["cascades"]
+    assert [e[:d + 1] for e in ev] == [e[:d + 1] for e in ev_ref]
+    assert np.abs(sim.F_field - Fr).max() <= 1e-9
+    assert np.abs(sim.F_plastic - Fpr).max() <= 1e-13
