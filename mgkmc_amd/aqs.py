@@ -37,7 +37,8 @@ class ThermalSimulation(DeviceKmc):
         # finite strain: the Newton-Krylov, augmented-Lagrangian or DBFFT step, KMC mode
         fs_solvers = ("newton_cg", "al", "dbfft")
         self._finite_ok = strain_assumption == "finite_strain" and solver in fs_solvers
-        extra = ["use_3d_barriers"] if use_3d_barriers else []
+        extra = []
+        self.use_3d_barriers = bool(use_3d_barriers)  # fully 3D mode catalogs (stz/catalog.py:47-82)
         if strain_assumption == "finite_strain" and solver not in fs_solvers:
             extra.append(f"finite-strain solver={solver!r} (the device step is 'newton_cg', 'al' or 'dbfft')")
         self._unsupported(cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model, extra)

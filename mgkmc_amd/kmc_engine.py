@@ -79,6 +79,27 @@ def planar_catalog(n_modes_total, gamma0, rng=None):
     return 0.5 * gamma0 * g
 
 
+def catalog_3d(n_modes, gamma0, rng=None):
+    """n fully 3D modes in the stream order of stz_catalog_glass_3d
+    (mgkmc/stz/catalog.py:47-82): per mode 9 normals (row-major 3x3), the
+    symmetric deviatoric part G scaled to equivalent strain 1 (1 / (sqrt 2 |G|);
+    a zero mode and no further draw when |G| < 1e-12), times the Rayleigh
+    magnitude gamma0 sqrt(-2 ln u), u = uniform(1e-15, 1).  Mode by mode: the
+    normals' cached second value carries over between modes."""
+    r = np.random if rng is None else rng
+    out = np.zeros((n_modes, 3, 3))
+    for i in range(n_modes):
+        a = r.normal(size=(3, 3))
+        sym = 0.5 * (a + a.T)
+        g = sym - (1.0 / 3.0) * np.trace(sym) * np.eye(3)
+        nrm = np.sqrt(np.sum(g ** 2))
+        if nrm < 1e-12:
+            continue
+        g = g * (1.0 / (np.sqrt(2.0) * nrm))
+        out[i] = g * (gamma0 * np.sqrt(-2.0 * np.log(r.uniform(1e-15, 1.0))))
+    return out
+
+
 def planar_to_full(p, d=3):
     out = np.zeros(p.shape[:-1] + (d, d))
     out[..., 0, 0] = p[..., 0]
@@ -222,8 +243,7 @@ class DeviceKmc:
         else:
             # the reference's order: Q0, then the catalog voxel by voxel
             Q0 = self.barrier_generator(self.grid_shape + (M,))
-            cat = planar_to_full(planar_catalog(self.total_voxels * M, gamma0, rng), d).reshape(
-                self.grid_shape + (M, d, d))
+            cat = self._draw_modes(self.total_voxels * M).reshape(self.grid_shape + (M, d, d))
             self.kmc = KmcState(self.grid_shape, M, Q0, cat, directional=directional, eps_precision=eps_prec,
                                 eps_shadow32=self.precision == MIXED)
         self.ls_warm_start = ls_warm_start
@@ -661,10 +681,17 @@ class DeviceKmc:
             raise NotImplementedError("single-mode redraw")
         newcat = newq0 = None
         if self.redraw_directions:
-            newcat = planar_to_full(planar_catalog(self.M, self.gamma0, self.rng), self.ndim)
+            newcat = self._draw_modes(self.M)
         if self.redraw_barriers:
             newq0 = self.barrier_generator((self.M,))
         self.kmc.set_voxel(v, newcat, newq0)
+
+    def _draw_modes(self, n):
+        """n catalog modes from the host stream: planar pure shear, or fully 3D
+        modes with use_3d_barriers."""
+        if getattr(self, "use_3d_barriers", False):
+            return catalog_3d(n, self.gamma0, self.rng)
+        return planar_to_full(planar_catalog(n, self.gamma0, self.rng), self.ndim)
 
     def _mode_host(self, v, m):
         c = dev.download(self.kmc.cat[:, m, v])

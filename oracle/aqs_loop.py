@@ -90,7 +90,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                stability_threshold=0.0, max_cascade_steps_pct=0.3, eps_plastic=None,
                strain_assumption="small_strain", enable_thermal=False, Cp=420.0, rho=6125.0,
                thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
-               tau_bath=0.0, solver="newton_cg"):
+               tau_bath=0.0, solver="newton_cg", use_3d_barriers=False):
     """Returns the list of events (x, y, z, m, time, is_instab) and final state.
     ``solver``: the finite-strain step's solver ("newton_cg", "al", "dbfft")."""
     np.random.seed(seed)
@@ -105,12 +105,16 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
     nu = np.asarray(nu, float)
     volume = pixel ** 3
     tau = 1.0 / (nu0 * np.exp(-q_act_temp / (KB * temperature))) if temperature > 0 else np.inf
+    def catalog(m):
+        # stz_catalog_glass, or stz_catalog_glass_3d with use_3d_barriers (aqs.py:162-168)
+        return (ok.stz_catalog_glass_3d if use_3d_barriers else ok.stz_catalog_glass)(m, gamma0, np.random)
+
     Q0 = gen((nx, ny, nz, M))
     cat = np.zeros((nx, ny, nz, M, 3, 3))
     for x in range(nx):
         for y in range(ny):
             for z in range(nz):
-                cat[x, y, z] = ok.stz_catalog_glass(M, gamma0, np.random)
+                cat[x, y, z] = catalog(M)
     Q = np.zeros((nx, ny, nz, M))
     soft = np.zeros((nx, ny, nz, 4))
     tl = np.full((nx, ny, nz), -np.inf)
@@ -205,7 +209,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                                         soft[a, b, c, 0] = g
                                         soft[a, b, c, 1] += jn * jt * sq
                         prev[x, y, z] = C
-                        cat[x, y, z] = ok.stz_catalog_glass(M, gamma0, np.random)
+                        cat[x, y, z] = catalog(M)
                         Q0[x, y, z] = gen((M,))
                   update()
                 state.setdefault("substeps", []).append(N)
@@ -242,7 +246,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                     ok.apply_flip_soa(ep, None, soft, tl, cat, x, y, z, m, t, jp, jt, softening_cap, jn)
                     prev[x, y, z] = cat[x, y, z, m].copy()
                     flip_heat(x, y, z, prev[x, y, z])
-                    cat[x, y, z] = ok.stz_catalog_glass(M, gamma0, np.random)
+                    cat[x, y, z] = catalog(M)
                     Q0[x, y, z] = gen((M,))
                     cascades.append((step, local, int(x), int(y), int(z), int(m)))
                 update()
@@ -304,7 +308,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                         ok.apply_flip_soa(ep, None, soft, tl, cat, x, y, z, m, t, jp, jt, softening_cap, jn)
                         prev[x, y, z] = C
                         flip_heat(x, y, z, C)
-                        cat[x, y, z] = ok.stz_catalog_glass(M, gamma0, np.random)
+                        cat[x, y, z] = catalog(M)
                         Q0[x, y, z] = gen((M,))
                         update()
                     events.append((int(x), int(y), int(z), int(m), t, bool(inst)))

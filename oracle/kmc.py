@@ -255,6 +255,24 @@ def stz_catalog_glass(M, gamma0, rs):
     return cat
 
 
+def stz_catalog_glass_3d(M, gamma0, rs):
+    """M fully 3D modes (stz/catalog.py:47-82): per mode a 3x3 standard-normal
+    matrix, its symmetric deviatoric part G scaled by 1 / (sqrt(2) |G|) (zero,
+    with no further draw, when |G| < 1e-12), times gamma0 sqrt(-2 ln u) with
+    u = rs.uniform(1e-15, 1)."""
+    cat = np.zeros((M, 3, 3))
+    for i in range(M):
+        A = rs.normal(size=(3, 3))
+        S = 0.5 * (A + A.T)
+        G = S - (1.0 / 3.0) * np.trace(S) * np.eye(3)
+        nrm = np.sqrt(np.sum(G ** 2))
+        if nrm < 1e-12:
+            continue
+        G = G * (1.0 / (np.sqrt(2.0) * nrm))
+        cat[i] = G * (gamma0 * np.sqrt(-2.0 * np.log(rs.uniform(1e-15, 1.0))))
+    return cat
+
+
 def stz_catalog_glass_2d(M, gamma0, rs):
     """pure-shear 2x2 modes (kmc_simulator_functions.py:78-98)."""
     cat = np.zeros((M, 2, 2))

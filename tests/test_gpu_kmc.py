@@ -653,3 +653,31 @@ def test_finite_strain_cascade_mode_matches_oracle(gpu, d, T, mean, g0):
     assert [e[:d + 1] for e in ev] == [e[:d + 1] for e in ev_ref]
     assert np.abs(sim.F_field - Fr).max() <= 1e-9
     assert np.abs(sim.F_plastic - Fpr).max() <= 1e-13
+
+
+@pytest.mark.parametrize("scheme", ["isotropic", "directional"])
+def test_kmc3d_3d_barriers_match_oracle(gpu, scheme):
+    # use_3d_barriers: fully 3D mode catalogs (9 normals and a Rayleigh
+    # magnitude per mode, in the reference's stream order), general-mode
+    # barriers, flips and redraws on the device; events and fields equal the
+    # oracle loop's
+    from mgkmc_amd.aqs import ThermalSimulation
+    from oracle.aqs_loop import run_events
+    print(f"[kmc3d 3d barriers] {scheme}", flush=True)
+    n, M = 8, 6
+    rng = np.random.default_rng(4)
+    E = 70e9 * rng.uniform(0.9, 1.1, (n, n, n))
+    nu = np.full((n, n, n), 0.3)
+    kw = dict(mean=1.0, std=0.3, min_cutoff=0.0)
+    args = dict(softening_cap=0.916, jp=20.0, jt=5.0, softening_scheme=scheme)
+    ev_ref, st_ref = run_events(n, n, n, M, 0.1, E, nu, 0.7, 700.0, 1e7, 2e-3, (0, 0), 15, seed=7, sel_seed=8,
+                                barrier_kwargs=kw, n_global_steps=3, use_3d_barriers=True, **args)
+    sim = ThermalSimulation(n, n, n, M, 0.1, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                            temperature=700.0, strain_rate=1e7, use_3d_barriers=True, rng=np.random.RandomState(7),
+                            selection_stream=np.random.RandomState(8), **args)
+    assert not sim.kmc.planar
+    ev = sim.run_simulation(3, 2e-3, component=(0, 0), max_events=15)
+    print(f"[kmc3d 3d barriers] events {[e[:4] for e in ev][:6]}", flush=True)
+    assert [e[:4] for e in ev] == [e[:4] for e in ev_ref]
+    assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
+    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-8 * np.abs(st_ref["sig"]).max()

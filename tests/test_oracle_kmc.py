@@ -290,3 +290,21 @@ def test_oracle_heat_step_closed_forms(coords):
     Tl = T.copy()
     heat_step(Tl, k2, Dsc, dt, T0, False, 0.0, 302.0)
     assert Tl.max() <= 302.0 and np.allclose(Tl, np.minimum(ref, 302.0), atol=1e-9)
+
+
+def test_3d_catalog_modes_are_deviatoric_with_rayleigh_magnitudes():
+    # stz_catalog_glass_3d: symmetric, traceless, equivalent strain
+    # sqrt(2 G:G) = gamma0 sqrt(-2 ln u) with u the uniform drawn after the
+    # mode's 9 normals (so the magnitudes are reproduced from the stream)
+    import numpy as np
+    from oracle import kmc as ok
+    rs = np.random.RandomState(5)
+    cat = ok.stz_catalog_glass_3d(40, 0.1, rs)
+    assert np.allclose(cat, cat.transpose(0, 2, 1)) and np.abs(np.trace(cat, axis1=1, axis2=2)).max() < 1e-16
+    eq = np.sqrt(2.0 * np.sum(cat ** 2, axis=(1, 2)))
+    rs2 = np.random.RandomState(5)
+    mags = []
+    for _ in range(40):
+        rs2.normal(size=(3, 3))
+        mags.append(0.1 * np.sqrt(-2.0 * np.log(rs2.uniform(1e-15, 1.0))))
+    assert np.allclose(eq, mags, rtol=1e-12)
