@@ -132,9 +132,6 @@ class DeviceKmc:
         bad = list(extra)
         if fast_patching and fast_patching.get("enabled", False):
             bad.append("fast_patching")
-        if enable_thermal and (strain_assumption != "small_strain" or
-                               hyperelastic_model in ("landau", "secant_degradation", "rose")):
-            bad.append("enable_thermal with finite strain or a nonlinear law")
         if strain_assumption != "small_strain" and not getattr(self, "_finite_ok", False):
             bad.append("finite_strain KMC")
         if bad:
@@ -314,9 +311,14 @@ class DeviceKmc:
         if not self.enable_thermal:
             return
         d = self.ndim
-        s = dev.download(self._stress.sig[(slice(None),) + tuple(int(c) for c in self.kmc.voxel_coords(v))])
-        sig = dev.from_sym_soa(s.reshape((-1,) + (1,) * d), d).reshape(d, d) if self.sig_kind != 2 else \
-            s.reshape(d, d)
+        coords = tuple(int(c) for c in self.kmc.voxel_coords(v))
+        if self.finite:  # Cauchy stress of the finite-strain state (2D: its in-plane block)
+            sig = dev.download(self.fs.sig[(slice(None), slice(None)) + coords]) if d == 3 else \
+                dev.download(self._fs_sig2.sig[:, int(v)]).reshape(2, 2)
+        else:
+            s = dev.download(self._stress.sig[(slice(None),) + coords])
+            sig = dev.from_sym_soa(s.reshape((-1,) + (1,) * d), d).reshape(d, d) if self.sig_kind != 2 else \
+                s.reshape(d, d)
         dT = abs(np.sum(sig * C)) / (self.rho * self.Cp)
         flat = self.Tfield.view(-1)
         t = float(flat[int(v)]) + dT
@@ -532,7 +534,7 @@ class DeviceKmc:
         torch = dev.torch
         Cs = [self._mode_host(v, m) for v, m in pairs]
         k, fs = self.kmc, self.fs
-        live = [fs.F, fs.P, fs.sig, fs.Fp, k.soft, k.tlast, k.prev, k.cat, k.Q0]
+        live = [fs.F, fs.P, fs.sig, fs.Fp, k.soft, k.tlast, k.prev, k.cat, k.Q0, self.Tfield]
         backup = [None if a is None else a.clone() for a in live]
         b_fm, b_em = self.F_macro.copy(), self.eps_macro.copy()
         step = 0.005 if self.ndim == 3 else 0.02
@@ -564,6 +566,7 @@ class DeviceKmc:
                         if i == N:
                             k.flip(v, m, self.time, self.jp, self.jt, self.softening_cap,
                                    self.neighbor_softening_fraction, 0)
+                            self._flip_heat(v, C)  # from the stress of the last sub-step's start
                             self._redraw(v)
                     self.update_stresses()
                 self.last_substeps = N
@@ -709,7 +712,7 @@ class DeviceKmc:
         redraws at the last, one re-solve per sub-step.  Every attempt draws
         its redraws anew from the host stream."""
         k = self.kmc
-        live = [k.eps_p, k.soft, k.tlast, k.prev, k.cat, k.Q0, self.ls.eps, self.ls.sig]
+        live = [k.eps_p, k.soft, k.tlast, k.prev, k.cat, k.Q0, self.ls.eps, self.ls.sig, self.Tfield]
         backup = [None if a is None else a.clone() for a in live]
         schedule = [1]
         for N in schedule:
@@ -724,7 +727,10 @@ class DeviceKmc:
                             k.flip(v, m, self.time, self.jp, self.jt, self.softening_cap,
                                    self.neighbor_softening_fraction, -N)
                         else:
+                            # heat at the last sub-step from the current stress (kmc_simulator.py:358-364)
+                            C = self._mode_host(v, m) if self.enable_thermal else None
                             self._flip(v, m, N)
+                            self._flip_heat(v, C)
                     self.update_stresses()
                 self.last_substeps = N
                 return

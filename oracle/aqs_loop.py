@@ -174,7 +174,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
     def flip_finite(flips):
         # one KMC event or one cascade batch (:512-615, :887-996)
         Cs = [cat[x, y, z, m].copy() for x, y, z, m in flips]
-        arrays = (Ff, Fp, soft, tl, prev, cat, Q0)
+        arrays = (Ff, Fp, soft, tl, prev, cat, Q0, Tl)
         backup = [a.copy() for a in arrays]
         b_sig, b_sm, b_fm, b_em = state["sig"].copy(), state["sm"].copy(), state["F_macro"].copy(), eps_macro.copy()
         n0 = min(max(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(C ** 2)) / 0.005))) for C in Cs), 40)
@@ -209,6 +209,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                                         soft[a, b, c, 0] = g
                                         soft[a, b, c, 1] += jn * jt * sq
                         prev[x, y, z] = C
+                        flip_heat(x, y, z, C)
                         cat[x, y, z] = catalog(M)
                         Q0[x, y, z] = gen((M,))
                   update()

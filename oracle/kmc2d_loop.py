@@ -113,8 +113,6 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
         lam, mu = ol.compute_lame_2d(E, nu, "plane_strain")
     elif model != "linear":
         raise ValueError(model)
-    if finite and enable_thermal:
-        raise NotImplementedError("finite strain with thermal coupling")
     state["eps"] = np.zeros((nx, ny, 2, 2))
     e33_state = {}
     inject = {"armed": False}
@@ -184,7 +182,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     soft[a, b, 1] += jn * jt * sq
 
     def flip_substepped(flips):
-        arrays = (ep, soft, tl, prev, cat, Q0)
+        arrays = (ep, soft, tl, prev, cat, Q0, Tl)
         backup = [a.copy() for a in arrays]
         b_eps, b_sig = state["eps"].copy(), state["sig"].copy()
         nh = min(max(max(2, int(np.ceil(np.sqrt(0.5 * np.sum(cat[x, y, m] ** 2)) / 0.02))) for x, y, m in flips), 40)
@@ -200,6 +198,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                         if k == N:
                             soften(x, y, c)
                             prev[x, y] = c
+                            flip_heat(x, y, c)
                             cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
                             Q0[x, y] = gen((M,))
                     elastic_run()
@@ -212,7 +211,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
     def flip_finite(flips):
         # one KMC event or one cascade batch (:605-700, :895-1000)
         Cs = [cat[x, y, m].copy() for x, y, m in flips]
-        arrays = (soft, tl, prev, cat, Q0)
+        arrays = (soft, tl, prev, cat, Q0, Tl)
         backup = [a.copy() for a in arrays]
         b = {k: state[k].copy() for k in ("F", "Fp", "sig", "F_macro")}
         b_em = eps_macro.copy()
@@ -230,6 +229,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                         if k == N:
                             soften(x, y, C)
                             prev[x, y] = C
+                            flip_heat(x, y, C)
                             cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
                             Q0[x, y] = gen((M,))
                     elastic_run()

@@ -681,3 +681,53 @@ def test_kmc3d_3d_barriers_match_oracle(gpu, scheme):
     assert [e[:4] for e in ev] == [e[:4] for e in ev_ref]
     assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
     assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-8 * np.abs(st_ref["sig"]).max()
+
+
+@pytest.mark.parametrize("d,kind", [(3, "finite"), (2, "finite"), (2, "secant")])
+def test_thermal_with_finite_strain_and_nonlinear_laws(gpu, d, kind):
+    # enable_thermal together with finite strain or a 2D nonlinear law: the
+    # flip heat comes from the stress at the start of the last sub-step and is
+    # restored with the state when a sub-step count fails; events, the final
+    # temperature field and the mechanical fields equal the oracle loops'
+    from mgkmc_amd.aqs import ThermalSimulation
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    from oracle.aqs_loop import run_events
+    from oracle.kmc2d_loop import run_events_2d
+    print(f"[thermal combo] {d}D {kind}", flush=True)
+    kw = dict(mean=1.0, std=0.3, min_cutoff=0.0)
+    th = dict(enable_thermal=True, temperature_cap=1500.0, thermostat=True, tau_bath=5e-13)
+    args = dict(softening_cap=0.916, jp=20.0, jt=5.0)
+    if d == 3:
+        n, M = 8, 6
+        E = 70e9 * np.random.default_rng(3).uniform(0.9, 1.1, (n, n, n))
+        nu = np.full((n, n, n), 0.3)
+        ev_ref, st_ref = run_events(n, n, n, M, 0.03, E, nu, 0.7, 700.0, 1e7, 2e-3, (0, 0), 8, seed=7, sel_seed=8,
+                                    barrier_kwargs=kw, n_global_steps=3, strain_assumption="finite_strain",
+                                    **th, **args)
+        sim = ThermalSimulation(n, n, n, M, 0.03, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                                temperature=700.0, strain_rate=1e7, strain_assumption="finite_strain",
+                                solver="newton_cg", rng=np.random.RandomState(7),
+                                selection_stream=np.random.RandomState(8), **th, **args)
+        ev = sim.run_simulation(3, 2e-3, component=(0, 0), max_events=8)
+    else:
+        n, M = 12, 6
+        E = 70e9 * np.random.default_rng(3).uniform(0.9, 1.1, (n, n))
+        nu = np.full((n, n), 0.3)
+        model, mp, extra = ("svk", {}, dict(strain_assumption="finite_strain", solver="newton_cg")) \
+            if kind == "finite" else ("secant_degradation", dict(d=0.3, k=20.0),
+                                      dict(hyperelastic_model="secant_degradation", d=0.3, k=20.0))
+        ev_ref, st_ref = run_events_2d(n, n, M, 0.05, E, nu, 0.7, 500.0, 1e7, 2e-3, (0, 0), 8, seed=7, sel_seed=8,
+                                       barrier_kwargs=kw, stress_targets={(1, 1): 0.0}, n_global_steps=3,
+                                       model=model, model_params=mp,
+                                       strain_assumption="finite_strain" if kind == "finite" else "small_strain",
+                                       **th, **args)
+        sim = KmcSimulation2D(n, n, M, 0.05, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                              temperature=500.0, strain_rate=1e7, rng=np.random.RandomState(7),
+                              selection_stream=np.random.RandomState(8), **extra, **th, **args)
+        ev = sim.run_simulation(3, 2e-3, component=(0, 0), stress_targets={(1, 1): 0.0}, max_events=8)
+    print(f"[thermal combo] events {[e[:d + 1] for e in ev]}; T max {sim.Tlocal.max():.3f} "
+          f"(oracle {st_ref['Tlocal'].max():.3f})", flush=True)
+    assert [e[:d + 1] for e in ev] == [e[:d + 1] for e in ev_ref]
+    assert np.abs(sim.Tlocal - st_ref["Tlocal"]).max() <= 1e-9 * st_ref["Tlocal"].max()
+    assert st_ref["Tlocal"].max() > 500.0 * 1.0000001 or d == 3
+    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-6 * np.abs(st_ref["sig"]).max()
