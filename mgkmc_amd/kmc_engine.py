@@ -100,6 +100,22 @@ def catalog_3d(n_modes, gamma0, rng=None):
     return out
 
 
+def simple_shear_catalog(n_modes, gamma0, rng=None):
+    """n 2D simple-shear modes in the stream order of stz_catalog_glass_2d
+    with stz_mode != "pure_shear" (mgkmc/kmc_simulator_functions.py:78-98):
+    per mode gxx, gxy (normals), then the sign s = +1 if rand() > 0.5 else -1;
+    [[gxx, gxy + s R], [gxy - s R, -gxx]] with R = sqrt(gxx^2 + gxy^2)."""
+    r = np.random if rng is None else rng
+    out = np.zeros((n_modes, 2, 2))
+    for i in range(n_modes):
+        gxx = 0.5 * gamma0 * r.normal()
+        gxy = 0.5 * gamma0 * r.normal()
+        R = np.sqrt(gxx ** 2 + gxy ** 2)
+        sgn = 1.0 if r.rand() > 0.5 else -1.0
+        out[i] = ((gxx, gxy + sgn * R), (gxy - sgn * R, -gxx))
+    return out
+
+
 def planar_to_full(p, d=3):
     out = np.zeros(p.shape[:-1] + (d, d))
     out[..., 0, 0] = p[..., 0]
@@ -694,6 +710,8 @@ class DeviceKmc:
         modes with use_3d_barriers."""
         if getattr(self, "use_3d_barriers", False):
             return catalog_3d(n, self.gamma0, self.rng)
+        if getattr(self, "stz_mode", "pure_shear") != "pure_shear":
+            return simple_shear_catalog(n, self.gamma0, self.rng)
         return planar_to_full(planar_catalog(n, self.gamma0, self.rng), self.ndim)
 
     def _mode_host(self, v, m):

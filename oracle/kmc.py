@@ -273,15 +273,24 @@ def stz_catalog_glass_3d(M, gamma0, rs):
     return cat
 
 
-def stz_catalog_glass_2d(M, gamma0, rs):
-    """pure-shear 2x2 modes (kmc_simulator_functions.py:78-98)."""
+def stz_catalog_glass_2d(M, gamma0, rs, stz_mode="pure_shear"):
+    """2x2 modes (kmc_simulator_functions.py:78-98): gxx, gxy from rs.normal()
+    per mode; pure shear [[gxx, gxy], [gxy, -gxx]], or simple shear with the
+    same symmetric part, [[gxx, gxy + s R], [gxy - s R, -gxx]], R = |(gxx, gxy)|,
+    s = +1 if rs.rand() > 0.5 else -1 (drawn after the mode's two normals)."""
     cat = np.zeros((M, 2, 2))
     for i in range(M):
         gxx = 0.5 * gamma0 * rs.normal()
         gxy = 0.5 * gamma0 * rs.normal()
         cat[i, 0, 0] = gxx
         cat[i, 1, 1] = -gxx
-        cat[i, 0, 1] = cat[i, 1, 0] = gxy
+        if stz_mode == "pure_shear":
+            cat[i, 0, 1] = cat[i, 1, 0] = gxy
+        else:
+            R = np.sqrt(gxx ** 2 + gxy ** 2)
+            sgn = 1.0 if rs.rand() > 0.5 else -1.0
+            cat[i, 0, 1] = gxy + sgn * R
+            cat[i, 1, 0] = gxy - sgn * R
     return cat
 
 

@@ -66,7 +66,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                   eps_plastic=None, model="linear", model_params=None, fail_once_at=None, cascade_mode=False,
                   stability_threshold=0.0, max_cascade_steps_pct=0.3, enable_thermal=False, Cp=420.0, rho=6125.0,
                   thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
-                  tau_bath=0.0, strain_assumption="small_strain"):
+                  tau_bath=0.0, strain_assumption="small_strain", stz_mode="pure_shear"):
     """Returns (events [(x, y, m, time, is_instab)], final state dict)."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
@@ -84,7 +84,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
     cat = np.zeros((nx, ny, M, 2, 2))
     for x in range(nx):
         for y in range(ny):
-            cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
+            cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random, stz_mode)
     Q = np.zeros((nx, ny, M))
     soft = np.zeros((nx, ny, 2))
     tl = np.full((nx, ny), -np.inf)
@@ -199,7 +199,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                             soften(x, y, c)
                             prev[x, y] = c
                             flip_heat(x, y, c)
-                            cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
+                            cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random, stz_mode)
                             Q0[x, y] = gen((M,))
                     elastic_run()
                 state.setdefault("substeps", []).append(N)
@@ -230,7 +230,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                             soften(x, y, C)
                             prev[x, y] = C
                             flip_heat(x, y, C)
-                            cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
+                            cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random, stz_mode)
                             Q0[x, y] = gen((M,))
                     elastic_run()
                 state.setdefault("substeps", []).append(N)
@@ -280,7 +280,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     ok.apply_flip_soa_2d(ep, soft, tl, cat, x, y, m, t, jp, jt, softening_cap, jn)
                     prev[x, y] = cat[x, y, m].copy()
                     flip_heat(x, y, prev[x, y])
-                    cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
+                    cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random, stz_mode)
                     Q0[x, y] = gen((M,))
                 elastic_run()
             else:
@@ -340,7 +340,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                         ok.apply_flip_soa_2d(ep, soft, tl, cat, x, y, m, t, jp, jt, softening_cap, jn)
                         prev[x, y] = C
                         flip_heat(x, y, C)
-                        cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random)
+                        cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random, stz_mode)
                         Q0[x, y] = gen((M,))
                         elastic_run()
                     else:

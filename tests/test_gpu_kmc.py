@@ -561,16 +561,18 @@ def test_kmc_thermal_matches_oracle(gpu, d, thermostat, tau_bath, cascade, coord
     assert Tl.max() > T  # flips heated something
 
 
-@pytest.mark.parametrize("model,plane_mode", [("svk", "plane_strain"), ("svk", "plane_stress"),
-                                              ("landau", "plane_strain")])
-def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode):
+@pytest.mark.parametrize("model,plane_mode,stz_mode", [("svk", "plane_strain", "pure_shear"),
+                                                       ("svk", "plane_stress", "pure_shear"),
+                                                       ("landau", "plane_strain", "pure_shear"),
+                                                       ("svk", "plane_strain", "simple_shear")])
+def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode, stz_mode):
     # KmcSimulation2D(strain_assumption="finite_strain", solver="newton_cg"):
     # every flip multiplies F_p by expm(C/N) (N = 1 first), each followed by
     # the 2D Newton-CG step with the stress targets; events, sub-step counts,
     # F, F_p, F_macro and sigma equal the 2D oracle loop's
     from mgkmc_amd.kmc_simulator import KmcSimulation2D
     from oracle.kmc2d_loop import run_events_2d
-    print(f"[kmc2d finite] {model} {plane_mode}", flush=True)
+    print(f"[kmc2d finite] {model} {plane_mode} {stz_mode}", flush=True)
     n, M = 12, 6
     rng = np.random.default_rng(3)
     E = 70e9 * rng.uniform(0.9, 1.1, (n, n))
@@ -583,8 +585,9 @@ def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode):
     ev_ref, st_ref = run_events_2d(n, n, M, 0.03, E, nu, 0.7, 400.0, 1e7, 2e-3, (0, 0), 8, seed=7, sel_seed=8,
                                    barrier_kwargs=kw, plane_mode=plane_mode, stress_targets={(1, 1): 0.0},
                                    n_global_steps=3, model=model, model_params=mp,
-                                   strain_assumption="finite_strain", **args)
+                                   strain_assumption="finite_strain", stz_mode=stz_mode, **args)
     sim = KmcSimulation2D(n, n, M, 0.03, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                          stz_mode=stz_mode,
                           temperature=400.0, strain_rate=1e7, plane_mode=plane_mode, strain_assumption="finite_strain",
                           hyperelastic_model=model, solver="newton_cg", rng=np.random.RandomState(7),
                           selection_stream=np.random.RandomState(8), **mp, **args)
@@ -599,6 +602,7 @@ def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode):
     print(f"[kmc2d finite] events {[e[:3] for e in ev]} substeps {subs} (oracle {st_ref['substeps']})", flush=True)
     assert [e[:3] for e in ev] == [e[:3] for e in ev_ref]
     assert subs == st_ref["substeps"]
+    assert sim.kmc.planar == (stz_mode == "pure_shear")
     assert np.abs(sim.F_field - np.einsum("ijxy->xyij", st_ref["F"])).max() <= 1e-9
     assert np.abs(sim.F_plastic - np.einsum("ijxy->xyij", st_ref["F_plastic"])).max() <= 1e-14
     assert np.abs(sim.F_macro - st_ref["F_macro"]).max() <= 1e-9
