@@ -583,7 +583,7 @@ class DeviceKmc:
                             k.flip(v, m, self.time, self.jp, self.jt, self.softening_cap,
                                    self.neighbor_softening_fraction, 0)
                             self._flip_heat(v, C)  # from the stress of the last sub-step's start
-                            self._redraw(v)
+                            self._redraw(v, m)
                     self.update_stresses()
                 self.last_substeps = N
                 return
@@ -692,17 +692,24 @@ class DeviceKmc:
         (eps_div = N: mode / N), then the redraw of the voxel's modes."""
         self.kmc.flip(v, m, self.time, self.jp, self.jt, self.softening_cap, self.neighbor_softening_fraction,
                       eps_div)
-        self._redraw(v)
+        self._redraw(v, m)
 
-    def _redraw(self, v):
-        """New directions (2M normals) and barriers (M draws) for voxel v from the host stream."""
-        if not (self.redraw_directions or self.redraw_barriers):
-            raise NotImplementedError("single-mode redraw")
+    def _redraw(self, v, m):
+        """New modes and barriers of voxel v from the host stream (directions
+        first, then barriers, each when its flag is on); with both flags off
+        only the flipped mode m gets a new direction and barrier
+        (mgkmc/aqs.py:594-606, kmc_simulator.py:1000-1008)."""
         newcat = newq0 = None
-        if self.redraw_directions:
-            newcat = self._draw_modes(self.M)
-        if self.redraw_barriers:
-            newq0 = self.barrier_generator((self.M,))
+        if self.redraw_directions or self.redraw_barriers:
+            if self.redraw_directions:
+                newcat = self._draw_modes(self.M)
+            if self.redraw_barriers:
+                newq0 = self.barrier_generator((self.M,))
+        else:
+            newcat = np.stack([self._mode_host(v, mm) for mm in range(self.M)])
+            newcat[m] = self._draw_modes(1)[0]
+            newq0 = dev.download(self.kmc.Q0[:, int(v)]).copy()
+            newq0[m] = self.barrier_generator((1,))[0]
         self.kmc.set_voxel(v, newcat, newq0)
 
     def _draw_modes(self, n):

@@ -90,7 +90,8 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                stability_threshold=0.0, max_cascade_steps_pct=0.3, eps_plastic=None,
                strain_assumption="small_strain", enable_thermal=False, Cp=420.0, rho=6125.0,
                thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
-               tau_bath=0.0, solver="newton_cg", use_3d_barriers=False):
+               tau_bath=0.0, solver="newton_cg", use_3d_barriers=False, redraw_directions=True,
+               redraw_barriers=True):
     """Returns the list of events (x, y, z, m, time, is_instab) and final state.
     ``solver``: the finite-strain step's solver ("newton_cg", "al", "dbfft")."""
     np.random.seed(seed)
@@ -108,6 +109,17 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
     def catalog(m):
         # stz_catalog_glass, or stz_catalog_glass_3d with use_3d_barriers (aqs.py:162-168)
         return (ok.stz_catalog_glass_3d if use_3d_barriers else ok.stz_catalog_glass)(m, gamma0, np.random)
+
+    def redraw(x, y, z, m):
+        # the voxel's modes and barriers, or only the flipped mode's (aqs.py:594-606)
+        if redraw_directions or redraw_barriers:
+            if redraw_directions:
+                cat[x, y, z] = catalog(M)
+            if redraw_barriers:
+                Q0[x, y, z] = gen((M,))
+        else:
+            cat[x, y, z, m] = catalog(1)[0]
+            Q0[x, y, z, m] = gen((1,))[0]
 
     Q0 = gen((nx, ny, nz, M))
     cat = np.zeros((nx, ny, nz, M, 3, 3))
@@ -210,8 +222,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                                         soft[a, b, c, 1] += jn * jt * sq
                         prev[x, y, z] = C
                         flip_heat(x, y, z, C)
-                        cat[x, y, z] = catalog(M)
-                        Q0[x, y, z] = gen((M,))
+                        redraw(x, y, z, m)
                   update()
                 state.setdefault("substeps", []).append(N)
                 return
@@ -247,8 +258,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                     ok.apply_flip_soa(ep, None, soft, tl, cat, x, y, z, m, t, jp, jt, softening_cap, jn)
                     prev[x, y, z] = cat[x, y, z, m].copy()
                     flip_heat(x, y, z, prev[x, y, z])
-                    cat[x, y, z] = catalog(M)
-                    Q0[x, y, z] = gen((M,))
+                    redraw(x, y, z, m)
                     cascades.append((step, local, int(x), int(y), int(z), int(m)))
                 update()
             local += 1
@@ -309,8 +319,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                         ok.apply_flip_soa(ep, None, soft, tl, cat, x, y, z, m, t, jp, jt, softening_cap, jn)
                         prev[x, y, z] = C
                         flip_heat(x, y, z, C)
-                        cat[x, y, z] = catalog(M)
-                        Q0[x, y, z] = gen((M,))
+                        redraw(x, y, z, m)
                         update()
                     events.append((int(x), int(y), int(z), int(m), t, bool(inst)))
                     if len(events) >= n_events:

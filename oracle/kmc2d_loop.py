@@ -66,7 +66,8 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                   eps_plastic=None, model="linear", model_params=None, fail_once_at=None, cascade_mode=False,
                   stability_threshold=0.0, max_cascade_steps_pct=0.3, enable_thermal=False, Cp=420.0, rho=6125.0,
                   thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
-                  tau_bath=0.0, strain_assumption="small_strain", stz_mode="pure_shear"):
+                  tau_bath=0.0, strain_assumption="small_strain", stz_mode="pure_shear", redraw_directions=True,
+                  redraw_barriers=True):
     """Returns (events [(x, y, m, time, is_instab)], final state dict)."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
@@ -161,6 +162,17 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
             raise ValueError("Unstable stress in small-strain elastic_run")
         return sm
 
+    def redraw(x, y, m):
+        # the voxel's modes and barriers, or only the flipped mode's (kmc_simulator.py:1000-1008)
+        if redraw_directions or redraw_barriers:
+            if redraw_directions:
+                cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random, stz_mode)
+            if redraw_barriers:
+                Q0[x, y] = gen((M,))
+        else:
+            cat[x, y, m] = ok.stz_catalog_glass_2d(1, gamma0, np.random, stz_mode)[0]
+            Q0[x, y, m] = gen((1,))[0]
+
     def soften(x, y, c):
         sq = ok.vm_sq_2d(c)
         gp = soft[x, y, 0] + jp * sq
@@ -199,8 +211,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                             soften(x, y, c)
                             prev[x, y] = c
                             flip_heat(x, y, c)
-                            cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random, stz_mode)
-                            Q0[x, y] = gen((M,))
+                            redraw(x, y, m)
                     elastic_run()
                 state.setdefault("substeps", []).append(N)
                 return
@@ -230,8 +241,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                             soften(x, y, C)
                             prev[x, y] = C
                             flip_heat(x, y, C)
-                            cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random, stz_mode)
-                            Q0[x, y] = gen((M,))
+                            redraw(x, y, m)
                     elastic_run()
                 state.setdefault("substeps", []).append(N)
                 return
@@ -280,8 +290,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     ok.apply_flip_soa_2d(ep, soft, tl, cat, x, y, m, t, jp, jt, softening_cap, jn)
                     prev[x, y] = cat[x, y, m].copy()
                     flip_heat(x, y, prev[x, y])
-                    cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random, stz_mode)
-                    Q0[x, y] = gen((M,))
+                    redraw(x, y, m)
                 elastic_run()
             else:
                 flip_substepped(flips)
@@ -340,8 +349,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                         ok.apply_flip_soa_2d(ep, soft, tl, cat, x, y, m, t, jp, jt, softening_cap, jn)
                         prev[x, y] = C
                         flip_heat(x, y, C)
-                        cat[x, y] = ok.stz_catalog_glass_2d(M, gamma0, np.random, stz_mode)
-                        Q0[x, y] = gen((M,))
+                        redraw(x, y, m)
                         elastic_run()
                     else:
                         flip_substepped([(x, y, m)])

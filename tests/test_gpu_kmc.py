@@ -735,3 +735,43 @@ def test_thermal_with_finite_strain_and_nonlinear_laws(gpu, d, kind):
     assert np.abs(sim.Tlocal - st_ref["Tlocal"]).max() <= 1e-9 * st_ref["Tlocal"].max()
     assert st_ref["Tlocal"].max() > 500.0 * 1.0000001 or d == 3
     assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-6 * np.abs(st_ref["sig"]).max()
+
+
+@pytest.mark.parametrize("d,rd,rb", [(3, False, False), (2, False, False), (3, False, True), (2, True, False)])
+def test_kmc_redraw_options_match_oracle(gpu, d, rd, rb):
+    # redraw_directions / redraw_barriers: the voxel's modes and barriers as
+    # the flags say, or (both off) only the flipped mode's direction and
+    # barrier; events and fields equal the oracle loops'
+    from mgkmc_amd.aqs import ThermalSimulation
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    from oracle.aqs_loop import run_events
+    from oracle.kmc2d_loop import run_events_2d
+    print(f"[redraw] {d}D directions={rd} barriers={rb}", flush=True)
+    kw = dict(mean=1.0, std=0.3, min_cutoff=0.0)
+    args = dict(softening_cap=0.916, jp=20.0, jt=5.0)
+    if d == 3:
+        n, M = 8, 6
+        E = 70e9 * np.random.default_rng(4).uniform(0.9, 1.1, (n, n, n))
+        nu = np.full((n, n, n), 0.3)
+        ev_ref, st_ref = run_events(n, n, n, M, 0.1, E, nu, 0.7, 700.0, 1e7, 2e-3, (0, 0), 15, seed=7, sel_seed=8,
+                                    barrier_kwargs=kw, n_global_steps=3, redraw_directions=rd, redraw_barriers=rb,
+                                    **args)
+        sim = ThermalSimulation(n, n, n, M, 0.1, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                                temperature=700.0, strain_rate=1e7, redraw_directions=rd, redraw_barriers=rb,
+                                rng=np.random.RandomState(7), selection_stream=np.random.RandomState(8), **args)
+        ev = sim.run_simulation(3, 2e-3, component=(0, 0), max_events=15)
+    else:
+        n, M = 16, 8
+        E = 70e9 * np.random.default_rng(4).uniform(0.9, 1.1, (n, n))
+        nu = np.full((n, n), 0.3)
+        ev_ref, st_ref = run_events_2d(n, n, M, 0.1, E, nu, 0.7, 700.0, 1e7, 2e-3, (0, 0), 15, seed=7, sel_seed=8,
+                                       barrier_kwargs=kw, stress_targets={(1, 1): 0.0}, n_global_steps=3,
+                                       redraw_directions=rd, redraw_barriers=rb, **args)
+        sim = KmcSimulation2D(n, n, M, 0.1, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                              temperature=700.0, strain_rate=1e7, redraw_directions=rd, redraw_barriers=rb,
+                              rng=np.random.RandomState(7), selection_stream=np.random.RandomState(8), **args)
+        ev = sim.run_simulation(3, 2e-3, component=(0, 0), stress_targets={(1, 1): 0.0}, max_events=15)
+    print(f"[redraw] events {[e[:d + 1] for e in ev][:6]}", flush=True)
+    assert [e[:d + 1] for e in ev] == [e[:d + 1] for e in ev_ref]
+    assert np.allclose(sim.Q0, st_ref["Q0"], rtol=0, atol=0)
+    assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
