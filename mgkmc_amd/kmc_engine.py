@@ -79,6 +79,80 @@ def planar_catalog(n_modes_total, gamma0, rng=None):
     return 0.5 * gamma0 * g
 
 
+def _clip(b, lo, hi):
+    return np.clip(b, a_min=lo, a_max=hi) if (lo is not None or hi is not None) else b
+
+
+def rayleigh_generator(loc=1.0, scale=0.5, min_cutoff=0.1, max_cutoff=None, rng=None):
+    """scipy.stats.rayleigh(loc, scale).rvs + clip (mgkmc/stz/barrier_generators.py:32-58),
+    on the given stream (numpy's global one by default)."""
+    from scipy.stats import rayleigh
+
+    def gen(shape):
+        r = np.random.mtrand._rand if rng is None else rng
+        return _clip(rayleigh(loc=loc, scale=scale).rvs(size=shape, random_state=r), min_cutoff, max_cutoff)
+    gen.rng = rng
+    return gen
+
+
+def _modified_rayleigh(n, mean, std, r):
+    """Rejection sampling of f(x) ~ x exp(-(x - mean)^2 / (2 std^2)) on
+    [max(0, mean - 6 std), mean + 6 std] in batches of max(3 n_needed, 100)
+    proposals (x then u uniform), first accepted kept in order
+    (barrier_generators.py:60-107)."""
+    out = np.empty(n)
+    got = 0
+    a, b = max(0.0, mean - 6 * std), mean + 6 * std
+    x_max = (mean + np.sqrt(mean ** 2 + 4 * std ** 2)) / 2.0
+    c_max = x_max * np.exp(-(x_max - mean) ** 2 / (2 * std ** 2))
+    while got < n:
+        need = n - got
+        m = max(int(need * 3), 100)
+        x = r.uniform(a, b, m)
+        u = r.uniform(0, c_max, m)
+        acc = x[u < x * np.exp(-(x - mean) ** 2 / (2 * std ** 2))]
+        take = min(len(acc), need)
+        out[got:got + take] = acc[:take]
+        got += take
+    return out
+
+
+def modified_rayleigh_generator(mean=2.0, std=0.6, min_cutoff=0.1, max_cutoff=None, rng=None):
+    """barrier_generators.py:60-107 on the given stream."""
+    def gen(shape):
+        r = np.random if rng is None else rng
+        n = shape if isinstance(shape, int) else int(np.prod(shape))
+        return _clip(_modified_rayleigh(n, mean, std, r).reshape(shape), min_cutoff, max_cutoff)
+    gen.rng = rng
+    return gen
+
+
+def modified_rayleigh_exp_generator(mean=2.0, std=0.6, epsilon=0.1, ratio=0.8, min_cutoff=0.1, max_cutoff=None,
+                                    rng=None):
+    """Mixture of modified Rayleigh and exponential(epsilon) barriers
+    (barrier_generators.py:111-170): one uniform per sample picks the
+    exponential when below ratio; the Rayleigh samples are drawn first, then
+    the exponential ones."""
+    def gen(shape):
+        r = np.random if rng is None else rng
+        n = shape if isinstance(shape, int) else int(np.prod(shape))
+        out = np.empty(n)
+        exp_mask = r.uniform(0.0, 1.0, n) < ratio
+        n_ray = int(np.sum(~exp_mask))
+        if n_ray > 0:
+            out[~exp_mask] = _modified_rayleigh(n_ray, mean, std, r)
+        if n - n_ray > 0:
+            out[exp_mask] = r.exponential(scale=epsilon, size=n - n_ray)
+        return _clip(out.reshape(shape), min_cutoff, max_cutoff)
+    gen.rng = rng
+    return gen
+
+
+BARRIER_GENERATORS = {"gaussian": gaussian_generator, "rayleigh": rayleigh_generator,
+                      "modified_rayleigh": modified_rayleigh_generator,
+                      "modified_rayleigh_with_exponential": modified_rayleigh_exp_generator}
+
+
 def catalog_3d(n_modes, gamma0, rng=None):
     """n fully 3D modes in the stream order of stz_catalog_glass_3d
     (mgkmc/stz/catalog.py:47-82): per mode 9 normals (row-major 3x3), the
@@ -196,9 +270,10 @@ class DeviceKmc:
         if barrier_generator is None:
             self.barrier_generator = gaussian_generator(**barrier_defaults, rng=rng)
         elif isinstance(barrier_generator, str):
-            if barrier_generator.lower() != "gaussian":
-                raise NotImplementedError(f"barrier generator {barrier_generator!r}")
-            self.barrier_generator = gaussian_generator(**barrier_kwargs, rng=rng)
+            make = BARRIER_GENERATORS.get(barrier_generator.lower())
+            if make is None:
+                raise ValueError(f"Unknown barrier generator type: {barrier_generator}")
+            self.barrier_generator = make(**barrier_kwargs, rng=rng)
         else:
             self.barrier_generator = barrier_generator
         if ls_precision not in LS_PRECISIONS:

@@ -91,16 +91,21 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                strain_assumption="small_strain", enable_thermal=False, Cp=420.0, rho=6125.0,
                thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
                tau_bath=0.0, solver="newton_cg", use_3d_barriers=False, redraw_directions=True,
-               redraw_barriers=True):
+               redraw_barriers=True, barrier_generator="gaussian"):
     """Returns the list of events (x, y, z, m, time, is_instab) and final state.
     ``solver``: the finite-strain step's solver ("newton_cg", "al", "dbfft")."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
     bk = dict(mean=2.0, std=0.6, min_cutoff=0.1) if barrier_kwargs is None else dict(barrier_kwargs)
-    gen = lambda shape: ok.gaussian_barriers(shape, np.random, **{"mean": bk.get("mean", 2.0),
-                                                                  "std": bk.get("std", 0.6),
-                                                                  "min_cutoff": bk.get("min_cutoff", 0.1),
-                                                                  "max_cutoff": bk.get("max_cutoff", None)})
+    if barrier_generator == "gaussian":
+        gen = lambda shape: ok.gaussian_barriers(shape, np.random, **{"mean": bk.get("mean", 2.0),
+                                                                      "std": bk.get("std", 0.6),
+                                                                      "min_cutoff": bk.get("min_cutoff", 0.1),
+                                                                      "max_cutoff": bk.get("max_cutoff", None)})
+    else:  # barrier_generators.py:32-170 on numpy's global stream
+        fn = {"rayleigh": ok.rayleigh_barriers, "modified_rayleigh": ok.modified_rayleigh_barriers,
+              "modified_rayleigh_with_exponential": ok.modified_rayleigh_exp_barriers}[barrier_generator]
+        gen = lambda shape: fn(shape, np.random.mtrand._rand, **bk)
     E = np.asarray(E, float)
     E = E * 1e9 if E.mean() < 1e6 else E
     nu = np.asarray(nu, float)

@@ -300,3 +300,56 @@ def gaussian_barriers(shape, rs, mean=2.0, std=0.6, min_cutoff=0.1, max_cutoff=N
     if min_cutoff is not None or max_cutoff is not None:
         b = np.clip(b, a_min=min_cutoff, a_max=max_cutoff)
     return b
+
+
+def rayleigh_barriers(shape, rs, loc=1.0, scale=0.5, min_cutoff=0.1, max_cutoff=None):
+    """scipy.stats.rayleigh(loc, scale).rvs then clip (barrier_generators.py:32-58)."""
+    from scipy.stats import rayleigh
+    b = rayleigh(loc=loc, scale=scale).rvs(size=shape, random_state=rs)
+    if min_cutoff is not None or max_cutoff is not None:
+        b = np.clip(b, a_min=min_cutoff, a_max=max_cutoff)
+    return b
+
+
+def _mod_rayleigh(N, mean, std, rs):
+    samples = np.empty(N)
+    k = 0
+    lo, hi = max(0.0, mean - 6 * std), mean + 6 * std
+    xm = (mean + np.sqrt(mean ** 2 + 4 * std ** 2)) / 2.0
+    cm = xm * np.exp(-(xm - mean) ** 2 / (2 * std ** 2))
+    while k < N:
+        need = N - k
+        m = max(int(need * 3), 100)
+        x = rs.uniform(lo, hi, m)
+        u = rs.uniform(0, cm, m)
+        ok_ = x[u < x * np.exp(-(x - mean) ** 2 / (2 * std ** 2))]
+        t = min(len(ok_), need)
+        samples[k:k + t] = ok_[:t]
+        k += t
+    return samples
+
+
+def modified_rayleigh_barriers(shape, rs, mean=2.0, std=0.6, min_cutoff=0.1, max_cutoff=None):
+    """Rejection sampling of x exp(-(x - mean)^2 / (2 std^2)) (barrier_generators.py:60-107)."""
+    N = shape if isinstance(shape, int) else int(np.prod(shape))
+    b = _mod_rayleigh(N, mean, std, rs).reshape(shape)
+    if min_cutoff is not None or max_cutoff is not None:
+        b = np.clip(b, a_min=min_cutoff, a_max=max_cutoff)
+    return b
+
+
+def modified_rayleigh_exp_barriers(shape, rs, mean=2.0, std=0.6, epsilon=0.1, ratio=0.8, min_cutoff=0.1,
+                                   max_cutoff=None):
+    """Modified Rayleigh / exponential mixture (barrier_generators.py:111-170)."""
+    N = shape if isinstance(shape, int) else int(np.prod(shape))
+    b = np.empty(N)
+    m_exp = rs.uniform(0.0, 1.0, N) < ratio
+    n_ray = int(np.sum(~m_exp))
+    if n_ray > 0:
+        b[~m_exp] = _mod_rayleigh(n_ray, mean, std, rs)
+    if N - n_ray > 0:
+        b[m_exp] = rs.exponential(scale=epsilon, size=N - n_ray)
+    b = b.reshape(shape)
+    if min_cutoff is not None or max_cutoff is not None:
+        b = np.clip(b, a_min=min_cutoff, a_max=max_cutoff)
+    return b

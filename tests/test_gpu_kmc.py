@@ -775,3 +775,28 @@ def test_kmc_redraw_options_match_oracle(gpu, d, rd, rb):
     assert [e[:d + 1] for e in ev] == [e[:d + 1] for e in ev_ref]
     assert np.allclose(sim.Q0, st_ref["Q0"], rtol=0, atol=0)
     assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("gen,kw", [("rayleigh", dict(loc=0.5, scale=0.4, min_cutoff=0.1)),
+                                    ("modified_rayleigh", dict(mean=1.0, std=0.3, min_cutoff=0.1)),
+                                    ("modified_rayleigh_with_exponential",
+                                     dict(mean=1.2, std=0.3, epsilon=0.3, ratio=0.3, min_cutoff=0.05))])
+def test_kmc3d_barrier_generators_match_oracle(gpu, gen, kw):
+    # the reference's other barrier generators (initial Q0 and every redraw
+    # from the same host stream): events and fields equal the oracle loop's
+    from mgkmc_amd.aqs import ThermalSimulation
+    from oracle.aqs_loop import run_events
+    print(f"[generators] {gen}", flush=True)
+    n, M = 8, 6
+    E = 70e9 * np.random.default_rng(4).uniform(0.9, 1.1, (n, n, n))
+    nu = np.full((n, n, n), 0.3)
+    args = dict(softening_cap=0.916, jp=20.0, jt=5.0)
+    ev_ref, st_ref = run_events(n, n, n, M, 0.1, E, nu, 0.7, 700.0, 1e7, 2e-3, (0, 0), 12, seed=7, sel_seed=8,
+                                barrier_kwargs=kw, n_global_steps=3, barrier_generator=gen, **args)
+    sim = ThermalSimulation(n, n, n, M, 0.1, E, nu, pixel=0.7, barrier_generator=gen, barrier_kwargs=kw,
+                            temperature=700.0, strain_rate=1e7, rng=np.random.RandomState(7),
+                            selection_stream=np.random.RandomState(8), **args)
+    ev = sim.run_simulation(3, 2e-3, component=(0, 0), max_events=12)
+    print(f"[generators] events {[e[:4] for e in ev][:5]} (of {len(ev)})", flush=True)
+    assert len(ev) > 0 and [e[:4] for e in ev] == [e[:4] for e in ev_ref]
+    assert np.array_equal(sim.Q0, st_ref["Q0"])

@@ -308,3 +308,26 @@ def test_3d_catalog_modes_are_deviatoric_with_rayleigh_magnitudes():
         rs2.normal(size=(3, 3))
         mags.append(0.1 * np.sqrt(-2.0 * np.log(rs2.uniform(1e-15, 1.0))))
     assert np.allclose(eq, mags, rtol=1e-12)
+
+
+def test_barrier_generators_follow_their_distributions():
+    # modified Rayleigh: support [max(0, m - 6s), m + 6s] and the density
+    # x exp(-(x - m)^2 / (2 s^2)) (histogram vs normalised pdf); the mixture
+    # puts a fraction ~ratio on the exponential (mean epsilon); Rayleigh
+    # samples have mean loc + scale sqrt(pi / 2)
+    import numpy as np
+    from oracle import kmc as ok
+    rs = np.random.RandomState(1)
+    m, s = 1.0, 0.3
+    x = ok.modified_rayleigh_barriers(200000, rs, mean=m, std=s, min_cutoff=None)
+    assert x.min() >= max(0.0, m - 6 * s) and x.max() <= m + 6 * s
+    h, e = np.histogram(x, bins=40, range=(0.0, m + 6 * s), density=True)
+    c = 0.5 * (e[1:] + e[:-1])
+    pdf = c * np.exp(-(c - m) ** 2 / (2 * s * s))
+    pdf /= np.sum(pdf) * (e[1] - e[0])
+    assert np.abs(h - pdf).max() < 0.05 * pdf.max()
+    y = ok.modified_rayleigh_exp_barriers(200000, rs, mean=m, std=s, epsilon=0.1, ratio=0.8, min_cutoff=None)
+    small = y < 0.25
+    assert abs(small.mean() - 0.8 * (1 - np.exp(-2.5))) < 0.01
+    r = ok.rayleigh_barriers(200000, rs, loc=1.0, scale=0.5, min_cutoff=None)
+    assert abs(r.mean() - (1.0 + 0.5 * np.sqrt(np.pi / 2))) < 0.01 and r.min() >= 1.0
