@@ -176,6 +176,31 @@ __device__ void fs_state(const T* F, const T* Fpi, T lam, T mu, const FsMat& m, 
     s.ga = T(4) * g3;
     s.B = B;
     s.C = C;
+  } else if (m.model == 3) {
+    // neo-Hookean of the plane problem (finite_strain_simulator.py:686-712):
+    // S = mu I + (-mu + lam ln J) C^-1, J = max(1e-14, det F_e), C = F_e^T F_e
+    // (|det C| < 1e-14 -> 1e-14); E2 keeps C^-1, al = mu - lam ln J, B = lam
+    T Ce[9];
+    mtm3(s.Fe, s.Fe, Ce);
+    const T J = fmax(T(1e-14), det3(s.Fe));
+    const T lnJ = log(J);
+    T dC = det3(Ce);
+    if (fabs((double)dC) < 1e-14) dC = T(1e-14);
+    s.E2[0] = (Ce[4] * Ce[8] - Ce[5] * Ce[7]) / dC;
+    s.E2[1] = -(Ce[1] * Ce[8] - Ce[2] * Ce[7]) / dC;
+    s.E2[2] = (Ce[1] * Ce[5] - Ce[2] * Ce[4]) / dC;
+    s.E2[3] = -(Ce[3] * Ce[8] - Ce[5] * Ce[6]) / dC;
+    s.E2[4] = (Ce[0] * Ce[8] - Ce[2] * Ce[6]) / dC;
+    s.E2[5] = -(Ce[0] * Ce[5] - Ce[2] * Ce[3]) / dC;
+    s.E2[6] = (Ce[3] * Ce[7] - Ce[4] * Ce[6]) / dC;
+    s.E2[7] = -(Ce[0] * Ce[7] - Ce[1] * Ce[6]) / dC;
+    s.E2[8] = (Ce[0] * Ce[4] - Ce[1] * Ce[3]) / dC;
+    const T coeff = -mu + lam * lnJ;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s.S[k] = ((k % 4 == 0) ? mu : T(0)) + coeff * s.E2[k];
+    s.al = mu - lam * lnJ;
+    s.B = lam;
+    s.trE = s.E[0] + s.E[4] + s.E[8];
   } else {
     const T* E = s.E;
     mm3(E, E, s.E2);
@@ -240,6 +265,18 @@ __device__ void fs_tangent(const FsVox<T>& s, const FsMat& m, const T* dF, bool 
       dS[k] = s.al * trX * dia + s.be * (trX * s.E[k] + EX * dia) + s.ga * (trX * s.E2[k] + E2X * dia) +
               T(4) * g4 * EX * s.E[k] + s.B * X[k] + s.C * (EXm[k] + XEm[k]) + s.cw * (X[k] - trX / T(3) * dia);
     }
+  } else if (m.model == 3) {
+    // dS = lam (C^-1 : X) C^-1 + (mu - lam ln J)(C^-1_ik C^-1_jl + C^-1_il C^-1_jk) X_kl
+    //    = B (C^-1 : X) C^-1 + 2 al C^-1 X C^-1   (X symmetric)
+    const T* Ci = s.E2;
+    T cx = T(0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) cx += Ci[k] * X[k];
+    T t1[9], t2[9];
+    mm3(Ci, X, t1);
+    mm3(t1, Ci, t2);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dS[k] = s.B * cx * Ci[k] + T(2) * s.al * t2[k];
   } else {
     T EX = T(0);
 #pragma unroll

@@ -20,7 +20,7 @@ import numpy as np
 from . import _lib
 from . import device as dev
 
-MODELS = {"svk": 0, "landau": 1, "murnaghan": 2}
+MODELS = {"svk": 0, "landau": 1, "murnaghan": 2, "neo_hookean": 3}  # neo-Hookean: 2D plane strain only
 
 
 def _scalar_const(x, name):
@@ -44,10 +44,14 @@ def landau_pa(x, name):
 
 def material(model_type="svk", A_m=0.0, B_m=0.0, C_m=0.0, v1=0.0, v2=0.0, v3=0.0, g1=0.0, g2=0.0, g3=0.0,
              g4=0.0, strain_capping_enabled=False, strain_capping_limit=None, strain_capping_tangent_ratio=0.1,
-             strain_capping_type="piecewise", strain_capping_smooth_power=1.0, mu_mean=None):
-    """mgk_fs_material for the device kernels (constants resolved on the host)."""
+             strain_capping_type="piecewise", strain_capping_smooth_power=1.0, mu_mean=None, ndim=3):
+    """mgk_fs_material for the device kernels (constants resolved on the host).
+    neo_hookean exists in the 2D law only; the 3D law treats it, like any
+    other unknown name, as SVK."""
     m = _lib.MgkFsMaterial()
     m.model = MODELS.get(model_type, 0)  # any other name falls through to SVK, as the reference
+    if model_type == "neo_hookean" and ndim == 3:
+        m.model = 0
     if model_type == "landau":
         m.v1, m.v2, m.v3 = landau_pa(v1, "v1"), landau_pa(v2, "v2"), landau_pa(v3, "v3")
         m.g1, m.g2, m.g3, m.g4 = landau_pa(g1, "g1"), landau_pa(g2, "g2"), landau_pa(g3, "g3"), landau_pa(g4, "g4")
@@ -221,9 +225,10 @@ def finite_strain_solver_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, dri
     C4[0,1,0,1] of the 2D stiffness are read."""
     if solver != "newton_cg":
         raise NotImplementedError(f"solver={solver!r}: the 2D device step implements solver='newton_cg'")
-    if model_type not in ("svk", "landau", "murnaghan") or (model_type != "svk" and plane_mode != "plane_strain"):
+    if model_type not in ("svk", "landau", "murnaghan", "neo_hookean") or \
+            (model_type != "svk" and plane_mode != "plane_strain"):
         raise NotImplementedError(f"2D {model_type} in {plane_mode}: the device step covers svk (both plane modes), "
-                                  "landau and murnaghan (plane strain)")
+                                  "landau, murnaghan and neo_hookean (plane strain)")
     C4 = np.asarray(C4)
     nx, ny = C4.shape[4:]
     sh3 = (nx, 1, ny)
@@ -233,7 +238,7 @@ def finite_strain_solver_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, dri
     s.F.copy_(dev.upload(np.ascontiguousarray(_embed2(F)), 0))
     mat = material(model_type, A_m, B_m, C_m, v1, v2, v3, g1, g2, g3, g4, strain_capping_enabled,
                    strain_capping_limit, strain_capping_tangent_ratio, strain_capping_type,
-                   strain_capping_smooth_power, mu_mean=float(np.mean(mu)))
+                   strain_capping_smooth_power, mu_mean=float(np.mean(mu)), ndim=2)
     mat.plane = 2 if plane_mode == "plane_stress" else 1
     Pt = None if P_target is None else _embed2(P_target, 0.0)
     Pm = None if P_mask is None else _embed2(np.asarray(P_mask, dtype=float), 0.0).astype(bool)
@@ -351,7 +356,8 @@ def finite_strain_simulation_2d(E, nu, driving_component, eps_target, n_steps, m
     (F_macro, Sig_macro, P_macro, F_list, Sig_list) in 2D shapes."""
     if solver != "newton_cg":
         raise NotImplementedError(f"solver={solver!r}: the 2D device step implements solver='newton_cg'")
-    if model_type not in ("svk", "landau", "murnaghan") or (model_type != "svk" and plane_mode != "plane_strain"):
+    if model_type not in ("svk", "landau", "murnaghan", "neo_hookean") or \
+            (model_type != "svk" and plane_mode != "plane_strain"):
         raise NotImplementedError(f"2D {model_type} in {plane_mode}")
     mixed_targets = {} if mixed_targets is None else mixed_targets
     E = np.asarray(E, dtype=np.float64)
@@ -363,7 +369,7 @@ def finite_strain_simulation_2d(E, nu, driving_component, eps_target, n_steps, m
     s = FsSolver(lam.reshape(sh3), mu.reshape(sh3), precision=0)
     mat = material(model_type, A_m, B_m, C_m, v1, v2, v3, g1, g2, g3, g4, strain_capping_enabled,
                    strain_capping_limit, strain_capping_tangent_ratio, strain_capping_type,
-                   strain_capping_smooth_power, mu_mean=float(mu.mean()))
+                   strain_capping_smooth_power, mu_mean=float(mu.mean()), ndim=2)
     mat.plane = 2 if plane_mode == "plane_stress" else 1
     drv = (_PLANE_AXES[driving_component[0]], _PLANE_AXES[driving_component[1]])
     Fm, Sm, Pm, Fl, Sl = [], [], [], [], []

@@ -302,10 +302,11 @@ class DeviceKmc:
                 lam, mu = lam.reshape(fs_grid), mu.reshape(fs_grid)
             eye = np.broadcast_to(np.eye(3).reshape(3, 3, 1, 1, 1), (3, 3) + fs_grid)
             self.fs = FsSolver(lam, mu, precision=FS_MIXED if self.precision == MIXED else 0, Fp=eye)
-            self._fs_mat = material(**finite, mu_mean=float(np.mean(mu)))
+            self._fs_mat = material(**finite, mu_mean=float(np.mean(mu)), ndim=d)
             if d == 2:
                 model = finite.get("model_type", "svk")
-                if model not in ("svk", "landau", "murnaghan") or (model != "svk" and plane_mode != "plane_strain"):
+                if model not in ("svk", "landau", "murnaghan", "neo_hookean") or \
+                        (model != "svk" and plane_mode != "plane_strain"):
                     raise NotImplementedError(f"2D finite-strain {model} in {plane_mode}")
                 self._fs_mat.plane = 2 if plane_mode == "plane_stress" else 1
                 self._fs_sig2 = _SigView()

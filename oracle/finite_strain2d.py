@@ -10,7 +10,9 @@ Follows mgkmc/finite_strain_simulator.py:
   :591-1091     constitutive_hyperelastic_2d: svk (both plane modes, F33 of the
                 plane-stress E33 = -lam* tr E / (2 mu)), landau and murnaghan
                 in plane strain (E33 = 0; the einsum tangent of the 3D law cut
-                to the in-plane block), Fp, the push-forward to K4
+                to the in-plane block), neo-Hookean in plane strain (:686-712:
+                S = mu I + (-mu + lam ln J) C^-1 and its tangent), Fp, the
+                push-forward to K4
   :1094-1114    cauchy_from_P (J = det F * F33)
   :2356-2572    finite_strain_solver_step_2d, solver="newton_cg"
   :2580-2850    finite_strain_simulation_2d (load stepping, store=False)
@@ -179,13 +181,25 @@ def constitutive_2d(F, C4, I2, I4, I4rt, Fp=None, model_type="svk", plane_mode="
                     **mat):
     """(P, K4, F33) of constitutive_hyperelastic_2d (:591-1091) for the laws
     listed in the module docstring."""
-    if model_type not in ("svk", "landau", "murnaghan") or (model_type != "svk" and plane_mode == "plane_stress"):
+    if model_type not in ("svk", "landau", "murnaghan", "neo_hookean") or \
+            (model_type != "svk" and plane_mode == "plane_stress"):
         raise NotImplementedError(f"{model_type} / {plane_mode}")
     Fpi = _inv2(Fp) if Fp is not None else None
     Fe = np.einsum("ijxy,jkxy->ikxy", F, Fpi) if Fpi is not None else F
     lam, mu = C4[0, 0, 1, 1], C4[0, 1, 0, 1]
     E2d = 0.5 * (np.einsum("jixy,jkxy->ikxy", Fe, Fe) - I2)
-    if model_type == "svk":
+    if model_type == "neo_hookean":
+        Ce = np.einsum("jixy,jkxy->ikxy", Fe, Fe)
+        lnJ = np.log(np.maximum(1e-14, Fe[0, 0] * Fe[1, 1] - Fe[0, 1] * Fe[1, 0]))
+        Ci = _inv2(Ce)
+        S = mu[None, None] * I2 + (-mu + lam * lnJ)[None, None] * Ci
+        C4e = None
+        if need_tangent:
+            C4e = (lam[None, None, None, None] * np.einsum("ijxy,klxy->ijklxy", Ci, Ci)
+                   + (mu - lam * lnJ)[None, None, None, None] * (np.einsum("ikxy,jlxy->ijklxy", Ci, Ci)
+                                                                 + np.einsum("ilxy,jkxy->ijklxy", Ci, Ci)))
+        F33 = np.ones_like(lam)
+    elif model_type == "svk":
         S = np.einsum("ijklxy,lkxy->ijxy", C4, E2d)
         C4e = C4
         if plane_mode == "plane_stress":

@@ -22,7 +22,8 @@ def _rel(a, b):
     ((12, 10), "landau", dict(LANDAU, strain_capping_enabled=True, strain_capping_type="smooth"), False,
      "plane_strain"),
     ((8, 6), "murnaghan", dict(A_m=-100e9, B_m=-100e9, C_m=-100e9), False, "plane_strain"),
-    ((64, 48), "svk", {}, False, "plane_stress")])
+    ((64, 48), "svk", {}, False, "plane_stress"), ((8, 6), "neo_hookean", {}, False, "plane_strain"),
+    ((10, 12), "neo_hookean", {}, True, "plane_strain")])
 def test_2d_step_parity(gpu, sh, model, kw, fp, mode):
     from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_2d
     rng = np.random.default_rng(4)
@@ -78,7 +79,7 @@ def test_2d_unsupported_laws_raise(gpu):
     sh = (4, 4)
     I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
     C4 = f2.build_C4_2d(np.full(sh, 70e9), np.full(sh, 0.3), I4s, II)
-    for kw in (dict(model_type="neo_hookean", solver="newton_cg"), dict(solver="al"),
+    for kw in (dict(model_type="neo_hookean", plane_mode="plane_stress", solver="newton_cg"), dict(solver="al"),
                dict(model_type="landau", plane_mode="plane_stress", solver="newton_cg")):
         with pytest.raises(NotImplementedError):
             finite_strain_solver_step_2d(I2.copy(), np.eye(2), None, C4, I2, I4, I4rt, **kw)

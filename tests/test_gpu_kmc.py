@@ -564,7 +564,8 @@ def test_kmc_thermal_matches_oracle(gpu, d, thermostat, tau_bath, cascade, coord
 @pytest.mark.parametrize("model,plane_mode,stz_mode", [("svk", "plane_strain", "pure_shear"),
                                                        ("svk", "plane_stress", "pure_shear"),
                                                        ("landau", "plane_strain", "pure_shear"),
-                                                       ("svk", "plane_strain", "simple_shear")])
+                                                       ("svk", "plane_strain", "simple_shear"),
+                                                       ("neo_hookean", "plane_strain", "simple_shear")])
 def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode, stz_mode):
     # KmcSimulation2D(strain_assumption="finite_strain", solver="newton_cg"):
     # every flip multiplies F_p by expm(C/N) (N = 1 first), each followed by

@@ -93,7 +93,8 @@ def test_2d_plane_stress_svk_closed_form():
     assert np.allclose(sig, F[:, :, 0, 0] @ S @ F[:, :, 0, 0].T / J, rtol=1e-12)
 
 
-@pytest.mark.parametrize("model,kw", [("landau", LANDAU), ("murnaghan", dict(A_m=-100e9, B_m=-100e9, C_m=-100e9))])
+@pytest.mark.parametrize("model,kw", [("landau", LANDAU), ("murnaghan", dict(A_m=-100e9, B_m=-100e9, C_m=-100e9)),
+                                      ("neo_hookean", {})])
 def test_2d_tangent_is_the_derivative_of_P(model, kw):
     sh = (3, 2)
     E, nu, t2, _, _ = _problem(sh, 7)
@@ -122,3 +123,28 @@ def test_2d_step_is_equilibrated_and_keeps_the_mean():
     assert np.allclose(F.mean(axis=(2, 3)), Fb, atol=1e-14)
     assert np.abs(f2.project2d(P, G)).max() < 1e-6 * np.abs(P).max()
     assert abs(S.mean(axis=(2, 3))[1, 1]) < 1e3 and Fb[1, 1] < 1.0
+
+
+def test_2d_neo_hookean_limits():
+    # S = 0 at F = I; at small strain the law is linear plane-strain
+    # elasticity (S = lam tr E I + 2 mu E to O(E^2)); under a pure rotation
+    # P = R S(I) = 0
+    sh = (3, 2)
+    E, nu = np.full(sh, 70e9), np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    C4 = f2.build_C4_2d(E, nu, I4s, II)
+    lam, mu = 70e9 * 0.3 / (1.3 * 0.4), 70e9 / 2.6
+    P0 = f2.constitutive_2d(I2.copy(), C4, I2, I4, I4rt, model_type="neo_hookean", need_tangent=False)[0]
+    assert np.abs(P0).max() < 1e-3
+    F = I2.copy()
+    F[0, 0] += 1e-5
+    F[0, 1] += 4e-6
+    P = f2.constitutive_2d(F, C4, I2, I4, I4rt, model_type="neo_hookean", need_tangent=False)[0][:, :, 0, 0]
+    Eg = 0.5 * (F[:, :, 0, 0].T @ F[:, :, 0, 0] - np.eye(2))
+    lin = F[:, :, 0, 0] @ (lam * np.trace(Eg) * np.eye(2) + 2 * mu * Eg)
+    assert np.abs(P - lin).max() < 1e-4 * np.abs(lin).max()
+    th = 0.3
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    Fr = np.einsum("ij,xy->ijxy", R, np.ones(sh))
+    Pr = f2.constitutive_2d(Fr, C4, I2, I4, I4rt, model_type="neo_hookean", need_tangent=False)[0]
+    assert np.abs(Pr).max() < 1e-2
