@@ -1970,6 +1970,14 @@ int fs_db_newton(FsCtx<T>& c, DbPar& d, T* F, const double* Fbc, double tol, dou
     k_norm2<T><<<nb9, MGK_NT, 0, c.s>>>(9 * NV, c.P, c.part);
     rc = fs_read_scal(c, nb9, 1, &pp);
     if (rc) return rc;
+    if (c.mat.plane) {  // 2D (_dbfft_step_2d): |P| over the in-plane block, without P_yy of the (nx, 1, ny) grid
+      double pyy;
+      const int nb1 = fs_nblk<T>(pl.NV);
+      k_norm2<T><<<nb1, MGK_NT, 0, c.s>>>(NV, c.P + 4 * NV, c.part);
+      rc = fs_read_scal(c, nb1, 1, &pyy);
+      if (rc) return rc;
+      pp -= pyy;
+    }
     rel = sqrt((double)NV * bb[1]) / (sqrt(pp) + 1e-20);
     if (rel < tol && i > 0) return MGK_OK;
     k_fs_kmean<T><<<dim3(nbk, 9), MGK_NT, 0, c.s>>>(NV, F, c.Fpi, c.lam, c.mu, c.mat, c.part);

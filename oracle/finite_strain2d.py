@@ -14,7 +14,9 @@ Follows mgkmc/finite_strain_simulator.py:
                 S = mu I + (-mu + lam ln J) C^-1 and its tangent), Fp, the
                 push-forward to K4
   :1094-1114    cauchy_from_P (J = det F * F33)
-  :2356-2572    finite_strain_solver_step_2d, solver="newton_cg"
+  :2356-2572    finite_strain_solver_step_2d, solver="newton_cg" or "dbfft"
+  :1195-1210,1232-1248,1279-1313,1321-1349,1381-1507  the 2D DBFFT step (frequencies,
+                matrix-field inverse, u from F, grad u, the preconditioned system)
   :2580-2850    finite_strain_simulation_2d (load stepping, store=False)
 The Krylov solver is SciPy's own bicgstab, called as the reference calls it.
 Not restated (the device step does not offer them): neo-Hookean, landau and
@@ -234,6 +236,105 @@ def constitutive_2d(F, C4, I2, I4, I4rt, Fp=None, model_type="svk", plane_mode="
     return P, K4, F33
 
 
+def frequencies_2d(nx, ny, pixel=1.0):
+    """_get_frequencies_2d (:1195-1210): 2 pi fftfreq(n, L / n), L = n pixel,
+    entries equal to -pi n / L set to zero."""
+    ns, Ls = (nx, ny), (nx * pixel, ny * pixel)
+    X = list(np.meshgrid(*[2.0 * np.pi * np.fft.fftfreq(n, d=L / n) for n, L in zip(ns, Ls)], indexing="ij"))
+    for a in range(2):
+        if ns[a] % 2 == 0:
+            X[a][X[a] == -np.pi * ns[a] / Ls[a]] = 0.0
+    return np.stack(X, 0)
+
+
+def dbfft_step_2d(F, F_bar, Xi, C4, I2, I4, I4rt, Fp=None, tol=1e-5, tol_CG=1e-6, max_iter=20, model_type="svk",
+                  plane_mode="plane_strain", stats=None, **mat):
+    """_dbfft_step_2d (:1381-1507) with solve_dbfft_linear_system_2d
+    (:1321-1349): u from F; Newton on u with b = -i xi . fft(P), stop on
+    |b| / (|P| + 1e-20) < tol after the first iteration, SciPy BiCGSTAB with
+    M = (xi . <K4> . xi)^-1 (det -> 1 below 1e-14, zero at k = 0), line search
+    1 .. 2^-15 on det F_e > 1e-4 and a NaN-free P (reused with its F33).
+    Returns (F, P, F33, iterations)."""
+    n = F.shape[2:]
+    fft = lambda a: _sfft.fftn(a, axes=(-2, -1))
+    ifft = lambda a: _sfft.ifftn(a, axes=(-2, -1)).real
+    Fbg = np.einsum("ij,xy->ijxy", F_bar, np.ones(n))
+    gh = fft(F - Fbg)
+    xi2 = np.sum(Xi ** 2, axis=0)
+    uh = -1j * np.einsum("jxy,ijxy->ixy", Xi, gh) / np.where(xi2 < 1e-14, 1.0, xi2)
+    uh[:, 0, 0] = 0.0
+    u = ifft(uh)
+    grad = lambda w: ifft(1j * Xi[None] * fft(w)[:, None])
+    Fpi = _inv2(Fp) if Fp is not None else None
+    carry = None
+    rel = np.nan
+    for i in range(max_iter):
+        Fc = Fbg + grad(u)
+        if carry is not None:
+            (P, F33), K4 = carry, None
+            carry = None
+        else:
+            P, K4, F33 = constitutive_2d(Fc, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, plane_mode=plane_mode,
+                                         **mat)
+        bh = -1j * np.einsum("jxy,ijxy->ixy", Xi, fft(P))
+        rel = np.linalg.norm(bh) / (np.linalg.norm(P) + 1e-20)
+        if rel < tol and i > 0:
+            return Fc, P, F33, i + 1
+        if K4 is None:
+            P, K4, F33 = constitutive_2d(Fc, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, plane_mode=plane_mode,
+                                         **mat)
+        A = np.einsum("jxy,ijlk,lxy->ikxy", Xi, K4.mean(axis=(-2, -1)), Xi)
+        det = A[0, 0] * A[1, 1] - A[0, 1] * A[1, 0]
+        ds = np.where(np.abs(det) < 1e-14, 1.0, det)
+        Mi = np.empty_like(A)
+        Mi[0, 0], Mi[1, 1] = A[1, 1] / ds, A[0, 0] / ds
+        Mi[0, 1], Mi[1, 0] = -A[0, 1] / ds, -A[1, 0] / ds
+        Mi[:, :, 0, 0] = 0.0
+
+        def A_op(uf):
+            g = ifft(1j * Xi[None] * uf.reshape((2,) + n)[:, None])
+            out = 1j * np.einsum("jxy,ijxy->ixy", Xi, fft(_ddot42(K4, g)))
+            out[:, 0, 0] = 0.0
+            return out.reshape(-1)
+
+        N = bh.size
+        nk = {"n": 0}
+
+        def cb(xk):
+            nk["n"] += 1
+
+        duh, _ = sp.bicgstab(sp.LinearOperator((N, N), matvec=A_op, dtype="complex128"), bh.reshape(-1),
+                             M=sp.LinearOperator((N, N), dtype="complex128",
+                                                 matvec=lambda r: np.einsum("ijxy,jxy->ixy", Mi,
+                                                                            r.reshape((2,) + n)).reshape(-1)),
+                             rtol=tol_CG, maxiter=150, callback=cb)
+        if stats is not None:
+            stats.setdefault("dbfft_krylov", []).append(nk["n"])
+        if np.any(np.isnan(duh)) or np.any(np.isinf(duh)):
+            raise ValueError("Linear solver returned NaN or Inf (BiCGSTAB breakdown).")
+        du = ifft(duh.reshape((2,) + n))
+        alpha = 1.0
+        for _ in range(16):
+            ut = u + alpha * du
+            Ft = Fbg + grad(ut)
+            Fe = np.einsum("ijxy,jkxy->ikxy", Ft, Fpi) if Fpi is not None else Ft
+            Je = Fe[0, 0] * Fe[1, 1] - Fe[0, 1] * Fe[1, 0]
+            if np.any(Je <= 1e-4) or np.any(np.isnan(Je)):
+                alpha *= 0.5
+                continue
+            Pt, _, F33t = constitutive_2d(Ft, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, plane_mode=plane_mode,
+                                          need_tangent=False, **mat)
+            if np.any(np.isnan(Pt)):
+                alpha *= 0.5
+                continue
+            u = ut
+            carry = (Pt, F33t)
+            break
+        else:
+            raise ValueError("Constitutive solver failed to converge in line search (potential negative Jacobian/NaN).")
+    raise ValueError(f"DBFFT solver did not converge within {max_iter} iterations. Final relative residual: {rel:.2e}")
+
+
 def cauchy_2d(P, F, F33=None):
     """cauchy_from_P (:1094-1114): sigma = P F^T / (det F * F33)."""
     J = F[0, 0] * F[1, 1] - F[0, 1] * F[1, 0]
@@ -245,8 +346,8 @@ def cauchy_2d(P, F, F33=None):
 
 def solver_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, driving_component=(0, 0), P_target=None, P_mask=None,
                    E_avg=100e9, nu_avg=0.3, tol_NW=1e-5, tol_CG=1e-6, max_NW=50, tol_macro=1e6, max_iter_macro=20,
-                   model_type="svk", plane_mode="plane_strain", stats=None, **mat):
-    """finite_strain_solver_step_2d, newton_cg branch (:2356-2572) ->
+                   model_type="svk", plane_mode="plane_strain", stats=None, solver="newton_cg", pixel=1.0, **mat):
+    """finite_strain_solver_step_2d, newton_cg or dbfft branch (:2356-2572) ->
     (F, P, Sig, K4, F_bar).  ``stats`` collects (Newton iteration, Krylov
     iterations so far in this macro iteration, alpha) per Newton step under
     "newton", as the 3D oracle does."""
@@ -265,48 +366,56 @@ def solver_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, driving_component
     res = None
     max_err = 0.0
     for _ in range(max_iter_macro):
-        DbarF = Fbc - F_start.mean(axis=(2, 3))
-        Dg = np.einsum("ij,xy->ijxy", DbarF, np.ones((nx, ny)))
-        cur = {}
-        cur["P"], cur["K4"], cur["F33"] = constitutive_2d(F_start, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type,
-                                                          plane_mode=plane_mode, **mat)
-        G_op = lambda A2: project2d(A2, Ghat4)
-        K_dF = lambda x: _trans2(_ddot42(cur["K4"], _trans2(x.reshape(2, 2, nx, ny))))
-        nk = {"n": 0}
-
-        def cb(xk):
-            nk["n"] += 1
-
-        A_op = sp.LinearOperator((F_start.size,) * 2, matvec=lambda x: G_op(K_dF(x)).reshape(-1), dtype="float64")
-        F_curr = F_start.copy()
-        for i_NW in range(max_NW):
-            rhs = -G_op(K_dF(Dg.reshape(-1))).reshape(-1) if i_NW == 0 else -G_op(cur["P"]).reshape(-1)
-            dFm, _ = sp.bicgstab(A_op, rhs, rtol=tol_CG, maxiter=150, callback=cb)
-            dF = dFm.reshape(2, 2, nx, ny)
-            dX = Dg + dF if i_NW == 0 else dF
-            alpha = 1.0
-            for _ in range(16):
-                Ft = F_curr + alpha * dX
-                Fe = np.einsum("ijxy,jkxy->ikxy", Ft, Fpi) if Fpi is not None else Ft
-                Je = Fe[0, 0] * Fe[1, 1] - Fe[0, 1] * Fe[1, 0]
-                if np.any(Je <= 1e-4) or np.any(np.isnan(Je)):
-                    alpha *= 0.5
-                    continue
-                Pt, Kt, F33t = constitutive_2d(Ft, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type,
-                                               plane_mode=plane_mode, **mat)
-                if np.any(np.isnan(Pt)) or np.any(np.isnan(Kt)):
-                    alpha *= 0.5
-                    continue
-                F_curr, cur["P"], cur["K4"], cur["F33"] = Ft, Pt, Kt, F33t
-                break
-            else:
-                raise ValueError("Newton-CG line search failed to find a valid step.")
+        if solver == "dbfft":
+            F_curr, P_db, F33_db, nit = dbfft_step_2d(F_start.copy(), Fbc, frequencies_2d(nx, ny, pixel), C4, I2, I4,
+                                                      I4rt, Fp=Fp, tol=tol_NW, tol_CG=tol_CG, max_iter=max_NW,
+                                                      model_type=model_type, plane_mode=plane_mode, stats=stats, **mat)
             if stats is not None:
-                stats.setdefault("newton", []).append((i_NW, nk["n"], alpha))
-            if np.linalg.norm(dFm) / (np.linalg.norm(F_curr) + 1e-20) < tol_NW and i_NW > 0:
-                break
+                stats.setdefault("dbfft", []).append(nit)
+            cur = {"P": P_db, "K4": None, "F33": F33_db}
         else:
-            raise ValueError(f"Newton-CG inner solver did not converge within {max_NW} iterations.")
+            DbarF = Fbc - F_start.mean(axis=(2, 3))
+            Dg = np.einsum("ij,xy->ijxy", DbarF, np.ones((nx, ny)))
+            cur = {}
+            cur["P"], cur["K4"], cur["F33"] = constitutive_2d(F_start, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type,
+                                                              plane_mode=plane_mode, **mat)
+            G_op = lambda A2: project2d(A2, Ghat4)
+            K_dF = lambda x: _trans2(_ddot42(cur["K4"], _trans2(x.reshape(2, 2, nx, ny))))
+            nk = {"n": 0}
+
+            def cb(xk):
+                nk["n"] += 1
+
+            A_op = sp.LinearOperator((F_start.size,) * 2, matvec=lambda x: G_op(K_dF(x)).reshape(-1), dtype="float64")
+            F_curr = F_start.copy()
+            for i_NW in range(max_NW):
+                rhs = -G_op(K_dF(Dg.reshape(-1))).reshape(-1) if i_NW == 0 else -G_op(cur["P"]).reshape(-1)
+                dFm, _ = sp.bicgstab(A_op, rhs, rtol=tol_CG, maxiter=150, callback=cb)
+                dF = dFm.reshape(2, 2, nx, ny)
+                dX = Dg + dF if i_NW == 0 else dF
+                alpha = 1.0
+                for _ in range(16):
+                    Ft = F_curr + alpha * dX
+                    Fe = np.einsum("ijxy,jkxy->ikxy", Ft, Fpi) if Fpi is not None else Ft
+                    Je = Fe[0, 0] * Fe[1, 1] - Fe[0, 1] * Fe[1, 0]
+                    if np.any(Je <= 1e-4) or np.any(np.isnan(Je)):
+                        alpha *= 0.5
+                        continue
+                    Pt, Kt, F33t = constitutive_2d(Ft, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type,
+                                                   plane_mode=plane_mode, **mat)
+                    if np.any(np.isnan(Pt)) or np.any(np.isnan(Kt)):
+                        alpha *= 0.5
+                        continue
+                    F_curr, cur["P"], cur["K4"], cur["F33"] = Ft, Pt, Kt, F33t
+                    break
+                else:
+                    raise ValueError("Newton-CG line search failed to find a valid step.")
+                if stats is not None:
+                    stats.setdefault("newton", []).append((i_NW, nk["n"], alpha))
+                if np.linalg.norm(dFm) / (np.linalg.norm(F_curr) + 1e-20) < tol_NW and i_NW > 0:
+                    break
+            else:
+                raise ValueError(f"Newton-CG inner solver did not converge within {max_NW} iterations.")
         Sig = cauchy_2d(cur["P"], F_curr, F33=cur["F33"])
         Sm = Sig.mean(axis=(2, 3))
         res = (F_curr, cur["P"], Sig, cur["K4"])

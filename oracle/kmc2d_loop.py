@@ -67,7 +67,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                   stability_threshold=0.0, max_cascade_steps_pct=0.3, enable_thermal=False, Cp=420.0, rho=6125.0,
                   thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
                   tau_bath=0.0, strain_assumption="small_strain", stz_mode="pure_shear", redraw_directions=True,
-                  redraw_barriers=True):
+                  redraw_barriers=True, solver="newton_cg"):
     """Returns (events [(x, y, m, time, is_instab)], final state dict)."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
@@ -122,7 +122,8 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
         Fb, Pt, Pm = f2.build_bc_2d(component, eps_macro[component], targets, state["F_macro"])
         F, P, Sig, _, Fb = f2.solver_step_2d(state["F"], Fb, fG, fC4, fI2, fI4, fI4rt, Fp=state["Fp"],
                                              driving_component=component, P_target=Pt, P_mask=Pm, E_avg=E.mean(),
-                                             nu_avg=nu.mean(), model_type=model, plane_mode=plane_mode, **mp)
+                                             nu_avg=nu.mean(), model_type=model, plane_mode=plane_mode,
+                                             solver=solver, pixel=pixel, **mp)
         state["F"] = F
         state["sig"] = np.einsum("ijxy->xyij", Sig)
         state["F_macro"] = Fb

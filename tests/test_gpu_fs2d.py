@@ -103,3 +103,53 @@ def test_2d_simulation_parity(gpu, model, mode, kw, drv, targets):
     print(f"[fs2d sim] {model} {mode} F_macro ref={ref[0][-1].ravel()} got={got[0][-1].ravel()}", flush=True)
     for a, b in zip(got[:3], ref):
         assert np.abs(a - b).max() <= 1e-10 * max(1.0, np.abs(b).max())
+
+
+@pytest.mark.parametrize("sh,model,mode,fp", [((8, 6), "svk", "plane_strain", False),
+                                              ((8, 6), "svk", "plane_stress", True),
+                                              ((5, 9), "landau", "plane_strain", False),
+                                              ((10, 12), "neo_hookean", "plane_strain", True)])
+def test_2d_dbfft_parity(gpu, sh, model, mode, fp):
+    # solver="dbfft" in 2D (the displacement step on the (nx, 1, ny) grid, its
+    # stop test over the in-plane |P|): F, P, sigma of the oracle
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_2d
+    rng = np.random.default_rng(4)
+    E = 70e9 * rng.uniform(0.85, 1.15, sh)
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    C4 = f2.build_C4_2d(E, nu, I4s, II, mode)
+    G = f2.build_ghat4_2d(*sh, *sh)
+    Fbar = np.eye(2)
+    Fbar[0, 1] = 0.02
+    Fbar[1, 1] = 0.996
+    Fp = I2 + 0.01 * np.random.default_rng(5).standard_normal((2, 2) + sh) if fp else None
+    kw = dict(LANDAU) if model == "landau" else {}
+    print(f"[fs2d dbfft] {sh} {model} {mode} Fp={fp}", flush=True)
+    st = {}
+    ref = f2.solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, Fp=Fp, model_type=model, plane_mode=mode,
+                            tol_NW=1e-8, tol_CG=1e-8, solver="dbfft", stats=st, **kw)
+    got = finite_strain_solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, Fp=Fp, model_type=model,
+                                       plane_mode=mode, tol_NW=1e-8, tol_CG=1e-8, solver="dbfft", **kw)
+    eF, eP, eS = _rel(got[0], ref[0]), _rel(got[1], ref[1]), _rel(got[2], ref[2])
+    print(f"[fs2d dbfft] oracle newton={st['dbfft']} krylov={st['dbfft_krylov']} relF={eF:.2e} relP={eP:.2e} "
+          f"relSig={eS:.2e}", flush=True)
+    assert eF < 1e-12 and eP < 1e-9 and eS < 1e-9
+
+
+def test_2d_dbfft_vs_newton_cg_reference_case(gpu):
+    # tests/test_finite_strain_kmc.py:314-365 of the reference on the device
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_2d
+    sh = (16, 16)
+    E = np.full(sh, 70.0e9)
+    E[4:12, 4:12] = 68.0e9
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    C4 = f2.build_C4_2d(E, nu, I4s, II, "plane_strain")
+    Fbar = np.diag([1.0005, 0.9995])
+    kw = dict(E_avg=E.mean(), nu_avg=0.3, tol_NW=1e-8, tol_CG=1e-9, max_NW=30)
+    Fn, Pn = finite_strain_solver_step_2d(I2.copy(), Fbar, None, C4, I2, I4, I4rt, solver="newton_cg", **kw)[:2]
+    Fd, Pd = finite_strain_solver_step_2d(I2.copy(), Fbar, None, C4, I2, I4, I4rt, solver="dbfft", **kw)[:2]
+    dF = np.linalg.norm(Fd - Fn) / np.linalg.norm(Fn)
+    dP = np.linalg.norm(Pd - Pn) / np.linalg.norm(Pn)
+    print(f"[fs2d dbfft vs ncg] F {dF:.3e} P {dP:.3e}", flush=True)
+    assert dF < 1e-5 and dP < 1e-2

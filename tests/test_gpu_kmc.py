@@ -561,12 +561,18 @@ def test_kmc_thermal_matches_oracle(gpu, d, thermostat, tau_bath, cascade, coord
     assert Tl.max() > T  # flips heated something
 
 
+@pytest.mark.parametrize("model,plane_mode,stz_mode,solver", [("svk", "plane_strain", "pure_shear", "dbfft"),
+                                                              ("svk", "plane_stress", "pure_shear", "dbfft")])
+def test_kmc2d_finite_strain_dbfft_matches_oracle(gpu, model, plane_mode, stz_mode, solver):
+    test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode, stz_mode, solver)
+
+
 @pytest.mark.parametrize("model,plane_mode,stz_mode", [("svk", "plane_strain", "pure_shear"),
                                                        ("svk", "plane_stress", "pure_shear"),
                                                        ("landau", "plane_strain", "pure_shear"),
                                                        ("svk", "plane_strain", "simple_shear"),
                                                        ("neo_hookean", "plane_strain", "simple_shear")])
-def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode, stz_mode):
+def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode, stz_mode, solver="newton_cg"):
     # KmcSimulation2D(strain_assumption="finite_strain", solver="newton_cg"):
     # every flip multiplies F_p by expm(C/N) (N = 1 first), each followed by
     # the 2D Newton-CG step with the stress targets; events, sub-step counts,
@@ -586,11 +592,11 @@ def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode, stz_mode):
     ev_ref, st_ref = run_events_2d(n, n, M, 0.03, E, nu, 0.7, 400.0, 1e7, 2e-3, (0, 0), 8, seed=7, sel_seed=8,
                                    barrier_kwargs=kw, plane_mode=plane_mode, stress_targets={(1, 1): 0.0},
                                    n_global_steps=3, model=model, model_params=mp,
-                                   strain_assumption="finite_strain", stz_mode=stz_mode, **args)
+                                   strain_assumption="finite_strain", stz_mode=stz_mode, solver=solver, **args)
     sim = KmcSimulation2D(n, n, M, 0.03, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
                           stz_mode=stz_mode,
                           temperature=400.0, strain_rate=1e7, plane_mode=plane_mode, strain_assumption="finite_strain",
-                          hyperelastic_model=model, solver="newton_cg", rng=np.random.RandomState(7),
+                          hyperelastic_model=model, solver=solver, rng=np.random.RandomState(7),
                           selection_stream=np.random.RandomState(8), **mp, **args)
     subs = []
     real = sim._flip_finite

@@ -148,3 +148,42 @@ def test_2d_neo_hookean_limits():
     Fr = np.einsum("ij,xy->ijxy", R, np.ones(sh))
     Pr = f2.constitutive_2d(Fr, C4, I2, I4, I4rt, model_type="neo_hookean", need_tangent=False)[0]
     assert np.abs(Pr).max() < 1e-2
+
+
+def test_2d_dbfft_vs_newton_cg_reference_case():
+    # tests/test_finite_strain_kmc.py:314-365 of the reference: 16^2, an 8^2
+    # square at 68 GPa in 70 GPa, F_bar = diag(1.0005, 0.9995): relative F
+    # difference < 1e-5, P < 1e-2; and the homogeneous closed form in two
+    # DBFFT iterations
+    sh = (16, 16)
+    E = np.full(sh, 70.0e9)
+    E[4:12, 4:12] = 68.0e9
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    C4 = f2.build_C4_2d(E, nu, I4s, II, "plane_strain")
+    G = f2.build_ghat4_2d(*sh, *sh)
+    Fbar = np.diag([1.0005, 0.9995])
+    kw = dict(E_avg=E.mean(), nu_avg=0.3, tol_NW=1e-8, tol_CG=1e-9, max_NW=30)
+    Fn, Pn, _, _, _ = f2.solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="newton_cg", **kw)
+    Fd, Pd, _, _, _ = f2.solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="dbfft", **kw)
+    assert np.linalg.norm(Fd - Fn) / np.linalg.norm(Fn) < 1e-5
+    assert np.linalg.norm(Pd - Pn) / np.linalg.norm(Pn) < 1e-2
+    C4h = f2.build_C4_2d(np.full(sh, 70e9), nu, I4s, II, "plane_strain")
+    st = {}
+    Fh, _, _, _, _ = f2.solver_step_2d(I2.copy(), Fbar, G, C4h, I2, I4, I4rt, solver="dbfft", stats=st)
+    assert st["dbfft"] == [2] and np.abs(Fh - Fbar[:, :, None, None]).max() < 1e-15
+
+
+@pytest.mark.parametrize("mode", ["plane_strain", "plane_stress"])
+def test_2d_dbfft_reaches_the_newton_equilibrium(mode):
+    # odd grid: the displacement and projection discretisations coincide, so
+    # both 2D solvers end at one equilibrium
+    sh = (7, 9)
+    E, nu, t2, _, _ = _problem(sh, 6, mode)
+    G = f2.build_ghat4_2d(*sh, *sh)
+    Fbar = np.eye(2)
+    Fbar[0, 1] = 0.02
+    kw = dict(plane_mode=mode, tol_NW=1e-9, tol_CG=1e-9)
+    Fn = f2.solver_step_2d(t2[0].copy(), Fbar, G, t2[3], *t2[:3], solver="newton_cg", **kw)[0]
+    Fd = f2.solver_step_2d(t2[0].copy(), Fbar, G, t2[3], *t2[:3], solver="dbfft", **kw)[0]
+    assert np.abs(Fd - Fn).max() < 1e-11
