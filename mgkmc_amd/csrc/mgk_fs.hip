@@ -2090,6 +2090,26 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
       MGK_HIP(hipMemcpyAsync(c.b, c.P, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
       for (int k = 0; k < 9; ++k) info->sig_mean[k] = res[3 + k];
       const int nbv = fs_nblk<T>(pl.NV), nb9 = fs_nblk<T>(9 * pl.NV);
+      // |G(P)| / (sqrt(size) (<|P|> + 1e-20)); a 2D problem counts the in-plane
+      // block only (_al_step_2d): size 4 NV, <|P|> without P_yy
+      auto al_resid = [&](double* out) -> int {
+        fs_project(c, c.P, c.r, T(1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
+        double g2, pa, pyy = 0.0;
+        k_norm2<T><<<nb9, MGK_NT, 0, s>>>((size_t)9 * NV, c.r, c.part);
+        int r = fs_read_scal(c, nb9, 1, &g2);
+        if (r) return r;
+        k_abs_sum<T><<<nb9, MGK_NT, 0, s>>>((size_t)9 * NV, c.P, c.part);
+        r = fs_read_scal(c, nb9, 1, &pa);
+        if (r) return r;
+        if (c.mat.plane) {
+          k_abs_sum<T><<<nbv, MGK_NT, 0, s>>>(NV, c.P + 4 * NV, c.part);
+          r = fs_read_scal(c, nbv, 1, &pyy);
+          if (r) return r;
+        }
+        const double sz = (c.mat.plane ? 4.0 : 9.0) * (double)NV;
+        *out = sqrt(g2) / (sqrt(sz) * ((pa - pyy) / sz + 1e-20));
+        return MGK_OK;
+      };
       bool al_ok = false;
       double resid = INFINITY;
       for (int i = 0; i < max_NW; ++i) {
@@ -2119,15 +2139,8 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
         MGK_HIP(hipMemcpyAsync(F, c.Ftr, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
         MGK_HIP(hipMemcpyAsync(c.b, c.P, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
         for (int k = 0; k < 9; ++k) info->sig_mean[k] = res[3 + k];
-        fs_project(c, c.P, c.r, T(1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
-        double g2, pa;
-        k_norm2<T><<<nb9, MGK_NT, 0, s>>>((size_t)9 * NV, c.r, c.part);
-        rc = fs_read_scal(c, nb9, 1, &g2);
+        rc = al_resid(&resid);
         if (rc) return rc;
-        k_abs_sum<T><<<nb9, MGK_NT, 0, s>>>((size_t)9 * NV, c.P, c.part);
-        rc = fs_read_scal(c, nb9, 1, &pa);
-        if (rc) return rc;
-        resid = sqrt(g2) / (sqrt(9.0 * (double)NV) * (pa / (9.0 * (double)NV) + 1e-20));
         if (resid < tol_NW && i > 0) {
           al_ok = true;
           break;
@@ -2135,15 +2148,8 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
       }
       if (!al_ok) {
         if (!std::isfinite(resid)) {  // no accepted step: the residual of the current P
-          fs_project(c, c.P, c.r, T(1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
-          double g2, pa;
-          k_norm2<T><<<nb9, MGK_NT, 0, s>>>((size_t)9 * NV, c.r, c.part);
-          rc = fs_read_scal(c, nb9, 1, &g2);
+          rc = al_resid(&resid);
           if (rc) return rc;
-          k_abs_sum<T><<<nb9, MGK_NT, 0, s>>>((size_t)9 * NV, c.P, c.part);
-          rc = fs_read_scal(c, nb9, 1, &pa);
-          if (rc) return rc;
-          resid = sqrt(g2) / (sqrt(9.0 * (double)NV) * (pa / (9.0 * (double)NV) + 1e-20));
         }
         info->fail_kind = 2;
         char msg[160];

@@ -215,7 +215,7 @@ def finite_strain_solver_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, dri
                                  strain_capping_tangent_ratio=0.1, strain_capping_type="piecewise",
                                  strain_capping_smooth_power=1.0):
     """One 2D finite-strain load step (finite_strain_simulator.py:2356-2572),
-    solver "newton_cg" or "dbfft" (:1381-1507) -> (F, P, Sig, None, F_bar).  The plane problem runs
+    solver "newton_cg", "dbfft" (:1381-1507) or "al" (:2007-2140) -> (F, P, Sig, None, F_bar).  The plane problem runs
     through the 3D device step on an (nx, 1, ny) grid: F_yy = 1, no
     out-of-plane shear, the projection's y wave number 0, so the in-plane
     fields, the Krylov iterates and the Newton / macro decisions are those of
@@ -223,8 +223,8 @@ def finite_strain_solver_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, dri
     Cauchy's J = det F * F33).  Laws: svk (plane strain and plane stress),
     landau and murnaghan (plane strain); lam = C4[0,0,1,1] and mu =
     C4[0,1,0,1] of the 2D stiffness are read."""
-    if solver not in ("newton_cg", "dbfft"):
-        raise NotImplementedError(f"solver={solver!r}: the 2D device step implements 'newton_cg' and 'dbfft'")
+    if solver not in ("newton_cg", "dbfft", "al"):
+        raise NotImplementedError(f"solver={solver!r}: the 2D device step implements 'newton_cg', 'al' and 'dbfft'")
     if model_type not in ("svk", "landau", "murnaghan", "neo_hookean") or \
             (model_type != "svk" and plane_mode != "plane_strain"):
         raise NotImplementedError(f"2D {model_type} in {plane_mode}: the device step covers svk (both plane modes), "
@@ -354,8 +354,8 @@ def finite_strain_simulation_2d(E, nu, driving_component, eps_target, n_steps, m
     (the plane problem on an (nx, 1, ny) grid, as finite_strain_solver_step_2d).
     Logs, checkpoints and VTK output are accepted and ignored.  Returns
     (F_macro, Sig_macro, P_macro, F_list, Sig_list) in 2D shapes."""
-    if solver not in ("newton_cg", "dbfft"):
-        raise NotImplementedError(f"solver={solver!r}: the 2D device step implements 'newton_cg' and 'dbfft'")
+    if solver not in ("newton_cg", "dbfft", "al"):
+        raise NotImplementedError(f"solver={solver!r}: the 2D device step implements 'newton_cg', 'al' and 'dbfft'")
     if model_type not in ("svk", "landau", "murnaghan", "neo_hookean") or \
             (model_type != "svk" and plane_mode != "plane_strain"):
         raise NotImplementedError(f"2D {model_type} in {plane_mode}")

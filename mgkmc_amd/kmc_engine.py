@@ -295,8 +295,8 @@ class DeviceKmc:
                 lam, mu = compute_lame_3d(E, nu)
                 fs_grid = self.grid_shape
             else:
-                if self._fs_solver not in ("newton_cg", "dbfft") or self.precision != FP64:
-                    raise NotImplementedError("2D finite-strain KMC runs solver='newton_cg' or 'dbfft' in FP64")
+                if self._fs_solver not in ("newton_cg", "dbfft", "al") or self.precision != FP64:
+                    raise NotImplementedError("2D finite-strain KMC runs solver='newton_cg', 'al' or 'dbfft' in FP64")
                 lam, mu = compute_lame_2d(E, nu, plane_mode=plane_mode)
                 fs_grid = (self.grid_shape[0], 1, self.grid_shape[1])
                 lam, mu = lam.reshape(fs_grid), mu.reshape(fs_grid)

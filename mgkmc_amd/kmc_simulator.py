@@ -33,13 +33,13 @@ class KmcSimulation2D(DeviceKmc):
                  strain_capping_type="piecewise", strain_capping_smooth_power=1.0,
                  ls_precision="fp64", ls_warm_start=False, rng=None, selection_stream=None, init_modes=None):
         # finite strain: the Newton-CG step of the plane problem, KMC or cascade mode
-        self._finite_ok = strain_assumption == "finite_strain" and solver in ("newton_cg", "dbfft")
+        self._finite_ok = strain_assumption == "finite_strain" and solver in ("newton_cg", "dbfft", "al")
         # simple-shear modes are non-symmetric: finite strain only (expm(C/N) F_p)
         extra = [f"stz_mode={stz_mode} with small strain"] if stz_mode != "pure_shear" and \
             strain_assumption != "finite_strain" else []
         self.stz_mode = stz_mode
-        if strain_assumption == "finite_strain" and solver not in ("newton_cg", "dbfft"):
-            extra.append(f"2D finite-strain solver={solver!r} (the device step is 'newton_cg' or 'dbfft')")
+        if strain_assumption == "finite_strain" and solver not in ("newton_cg", "dbfft", "al"):
+            extra.append(f"2D finite-strain solver={solver!r} (the device steps are 'newton_cg', 'al', 'dbfft')")
         self._unsupported(cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model, extra)
         self.nx, self.ny = nx, ny
         self.output_dir = output_dir

@@ -335,6 +335,45 @@ def dbfft_step_2d(F, F_bar, Xi, C4, I2, I4, I4rt, Fp=None, tol=1e-5, tol_CG=1e-6
     raise ValueError(f"DBFFT solver did not converge within {max_iter} iterations. Final relative residual: {rel:.2e}")
 
 
+def al_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, tol=1e-5, max_iter=200, model_type="svk",
+               plane_mode="plane_strain", **mat):
+    """_al_step_2d (:2007-2140): L0 = grid mean of C4; F + (F_bar - <F>); lam
+    = P; per iteration dF = -G(lam - L0:F), the first beta in 1 .. 1/16 whose
+    trial (mean reset to F_bar) has det F_e > 1e-4 and a NaN-free P; lam = P;
+    stop when |G(P)| / (sqrt(size) (<|P|> + 1e-20)) < tol after the first
+    iteration.  Returns (F, P, F33, iterations); ValueError otherwise."""
+    n = F.shape[2:]
+    L0 = np.einsum("ijkl,xy->ijklxy", C4.mean(axis=(-2, -1)), np.ones(n))
+    Fpi = _inv2(Fp) if Fp is not None else None
+    F = F + np.einsum("ij,xy->ijxy", F_bar - F.mean(axis=(2, 3)), np.ones(n))
+    P, _, F33 = constitutive_2d(F, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, plane_mode=plane_mode,
+                                need_tangent=False, **mat)
+    lam = P.copy()
+    for i in range(max_iter):
+        dF = -project2d(lam - _ddot42(L0, F), Ghat4)
+        ok = False
+        for beta in (1.0, 0.5, 0.25, 0.125, 0.0625):
+            Ft = F + beta * dF
+            Ft = Ft + np.einsum("ij,xy->ijxy", F_bar - Ft.mean(axis=(2, 3)), np.ones(n))
+            Fe = np.einsum("ijxy,jkxy->ikxy", Ft, Fpi) if Fpi is not None else Ft
+            Je = Fe[0, 0] * Fe[1, 1] - Fe[0, 1] * Fe[1, 0]
+            Pn, _, F33n = constitutive_2d(Ft, C4, I2, I4, I4rt, Fp=Fp, model_type=model_type, plane_mode=plane_mode,
+                                          need_tangent=False, **mat)
+            if not np.any(np.isnan(Pn)) and np.all(Je > 1e-4):
+                F, P, F33, ok = Ft, Pn, F33n, True
+                break
+        if not ok:
+            break
+        lam = P.copy()
+        GP = project2d(P, Ghat4)
+        res = np.linalg.norm(GP) / (np.sqrt(GP.size) * (np.mean(np.abs(P)) + 1e-20))
+        if res < tol and i > 0:
+            return F, P, F33, i + 1
+    GP = project2d(P, Ghat4)
+    res = np.linalg.norm(GP) / (np.sqrt(GP.size) * (np.mean(np.abs(P)) + 1e-20))
+    raise ValueError(f"AL solver did not converge or exploded. Final relative residual: {res:.2e}")
+
+
 def cauchy_2d(P, F, F33=None):
     """cauchy_from_P (:1094-1114): sigma = P F^T / (det F * F33)."""
     J = F[0, 0] * F[1, 1] - F[0, 1] * F[1, 0]
@@ -366,7 +405,14 @@ def solver_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, driving_component
     res = None
     max_err = 0.0
     for _ in range(max_iter_macro):
-        if solver == "dbfft":
+        if solver == "al":
+            F_curr, P_al, F33_al, nit = al_step_2d(F_start.copy(), Fbc, Ghat4, C4, I2, I4, I4rt, Fp=Fp, tol=tol_NW,
+                                                   max_iter=max_NW, model_type=model_type, plane_mode=plane_mode,
+                                                   **mat)
+            if stats is not None:
+                stats.setdefault("al", []).append(nit)
+            cur = {"P": P_al, "K4": None, "F33": F33_al}
+        elif solver == "dbfft":
             F_curr, P_db, F33_db, nit = dbfft_step_2d(F_start.copy(), Fbc, frequencies_2d(nx, ny, pixel), C4, I2, I4,
                                                       I4rt, Fp=Fp, tol=tol_NW, tol_CG=tol_CG, max_iter=max_NW,
                                                       model_type=model_type, plane_mode=plane_mode, stats=stats, **mat)

@@ -79,7 +79,7 @@ def test_2d_unsupported_laws_raise(gpu):
     sh = (4, 4)
     I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
     C4 = f2.build_C4_2d(np.full(sh, 70e9), np.full(sh, 0.3), I4s, II)
-    for kw in (dict(model_type="neo_hookean", plane_mode="plane_stress", solver="newton_cg"), dict(solver="al"),
+    for kw in (dict(model_type="neo_hookean", plane_mode="plane_stress", solver="newton_cg"), dict(solver="bogus"),
                dict(model_type="landau", plane_mode="plane_stress", solver="newton_cg")):
         with pytest.raises(NotImplementedError):
             finite_strain_solver_step_2d(I2.copy(), np.eye(2), None, C4, I2, I4, I4rt, **kw)
@@ -153,3 +153,25 @@ def test_2d_dbfft_vs_newton_cg_reference_case(gpu):
     dP = np.linalg.norm(Pd - Pn) / np.linalg.norm(Pn)
     print(f"[fs2d dbfft vs ncg] F {dF:.3e} P {dP:.3e}", flush=True)
     assert dF < 1e-5 and dP < 1e-2
+
+
+@pytest.mark.parametrize("mode", ["plane_strain", "plane_stress"])
+def test_2d_al_matches_oracle(gpu, mode):
+    # solver="al" in 2D: the homogeneous closed form as the oracle (in-plane
+    # residual), and the reference's ValueError on a heterogeneous grid
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_2d
+    sh = (6, 4)
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    G = f2.build_ghat4_2d(*sh, *sh)
+    Fbar = np.eye(2)
+    Fbar[0, 1] = 0.02
+    C4h = f2.build_C4_2d(np.full(sh, 1.0), nu, I4s, II, mode)
+    ref = f2.solver_step_2d(I2.copy(), Fbar, G, C4h, I2, I4, I4rt, solver="al", plane_mode=mode, tol_NW=1e-12)
+    got = finite_strain_solver_step_2d(I2.copy(), Fbar, G, C4h, I2, I4, I4rt, solver="al", plane_mode=mode,
+                                       tol_NW=1e-12)
+    assert _rel(got[0], ref[0]) < 1e-14 and _rel(got[1], ref[1]) < 1e-12
+    C4 = f2.build_C4_2d(70e9 * np.random.default_rng(9).uniform(0.85, 1.15, sh), nu, I4s, II, mode)
+    for fn in (f2.solver_step_2d, finite_strain_solver_step_2d):
+        with pytest.raises(ValueError, match="AL solver did not converge or exploded"):
+            fn(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="al", plane_mode=mode)

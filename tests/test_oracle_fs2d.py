@@ -187,3 +187,21 @@ def test_2d_dbfft_reaches_the_newton_equilibrium(mode):
     Fn = f2.solver_step_2d(t2[0].copy(), Fbar, G, t2[3], *t2[:3], solver="newton_cg", **kw)[0]
     Fd = f2.solver_step_2d(t2[0].copy(), Fbar, G, t2[3], *t2[:3], solver="dbfft", **kw)[0]
     assert np.abs(Fd - Fn).max() < 1e-11
+
+
+def test_2d_al_closed_form_and_heterogeneous_failure():
+    # the AL update adds G of a stress to F: homogeneous O(1) moduli give the
+    # closed form after two iterations, any heterogeneity diverges (ValueError)
+    sh = (6, 4)
+    nu = np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    G = f2.build_ghat4_2d(*sh, *sh)
+    Fbar = np.eye(2)
+    Fbar[0, 1] = 0.02
+    C4h = f2.build_C4_2d(np.full(sh, 1.0), nu, I4s, II)
+    st = {}
+    F, P, _, _, _ = f2.solver_step_2d(I2.copy(), Fbar, G, C4h, I2, I4, I4rt, solver="al", tol_NW=1e-12, stats=st)
+    assert st["al"] == [2] and np.allclose(F, Fbar[:, :, None, None], atol=1e-14)
+    C4 = f2.build_C4_2d(70e9 * np.random.default_rng(9).uniform(0.85, 1.15, sh), nu, I4s, II)
+    with pytest.raises(ValueError, match="AL solver did not converge or exploded"):
+        f2.solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="al")
