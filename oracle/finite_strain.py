@@ -656,3 +656,32 @@ def _newton(F_start, Fbc, Ghat4, C4, I2, I4, I4rt, Fp, tol_NW, tol_CG, max_NW, m
         raise ValueError(f"Newton-CG inner solver did not converge within {max_NW} iterations.")
     return F_curr, cur
 
+
+
+def simulation(E, nu, driving_component, eps_target, n_steps, mixed_targets=None, pixel=1.0, tol_NW=1e-5,
+               tol_CG=1e-6, max_NW=20, tol_macro=1e6, max_iter_macro=20, model_type="svk", solver="al", **mat):
+    """finite_strain_simulation_3d (finite_strain_simulator.py:3175-3413),
+    store=False: load steps over linspace(0, eps_target, n_steps + 1)
+    (:3270), build_bc from the previous step's F_bar (:3276-3279), one
+    solver_step per load step (:3281-3298) -> (F_macro, Sig_macro, P_macro)
+    as voxel means (:3301-3303)."""
+    mixed_targets = {} if mixed_targets is None else mixed_targets
+    nx, ny, nz = E.shape
+    I2, I4, I4rt, I4s, II = identity_tensors(nx, ny, nz)
+    C4 = build_C4(E, nu, I4s, II)
+    G = build_ghat4(nx, ny, nz, nx * pixel, ny * pixel, nz * pixel)
+    F = I2.copy()
+    Fb_init = None
+    Fm, Sm, Pm = [], [], []
+    for eps_s in np.linspace(0.0, eps_target, n_steps + 1):
+        Fb, _, Pt, Pmask = build_bc(driving_component, eps_s, mixed_targets, Fb_init)
+        F, P, Sig, _, Fb = solver_step(F, Fb, G, C4, I2, I4, I4rt, driving_component=driving_component,
+                                       P_target=Pt, P_mask=Pmask, E_avg=E.mean(), nu_avg=nu.mean(), tol_NW=tol_NW,
+                                       tol_CG=tol_CG, max_NW=max_NW, tol_macro=tol_macro,
+                                       max_iter_macro=max_iter_macro, model_type=model_type, solver=solver,
+                                       pixel=pixel, **mat)
+        Fb_init = Fb
+        Fm.append(F.mean(axis=(2, 3, 4)))
+        Sm.append(Sig.mean(axis=(2, 3, 4)))
+        Pm.append(P.mean(axis=(2, 3, 4)))
+    return np.array(Fm), np.array(Sm), np.array(Pm)

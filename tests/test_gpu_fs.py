@@ -295,3 +295,36 @@ def test_dbfft_vs_newton_cg_reference_case(gpu):
     dP = np.linalg.norm(Pd - Pn) / np.linalg.norm(Pn)
     print(f"[dbfft vs ncg] F {dF:.3e} P {dP:.3e}", flush=True)
     assert dF < 1e-5 and dP < 1e-2
+
+
+REF_LANDAU = dict(v1=-100.0e9, v2=-50.0e9, v3=-10.0e9, g1=-1000.0e9, g2=500.0e9, g3=-100.0e9, g4=-200.0e9)
+
+
+@pytest.mark.parametrize("model,solver,kw,sh,drv,targets,tols", [
+    # the reference's own case (tests/test_landau_finite_strain.py:69-100)
+    ("landau", "al", REF_LANDAU, (4, 4, 4), (0, 0), {(1, 1): 0.0, (2, 2): 0.0}, dict(tol_macro=1e5)),
+    ("svk", "newton_cg", {}, (8, 6, 5), (0, 0), {(1, 1): 0.0, (2, 2): 0.0}, dict(tol_NW=1e-6, tol_macro=1e5)),
+    ("landau", "dbfft", LANDAU, (6, 4, 5), (0, 1), {(0, 0): 0.0}, dict(tol_NW=1e-8, tol_CG=1e-8, tol_macro=1e4))])
+def test_simulation_3d_parity(gpu, model, solver, kw, sh, drv, targets, tols):
+    # finite_strain_simulation_3d: the macro F, Cauchy and P of every load
+    # step equal the oracle driver's; the stored field lists hold every step
+    from mgkmc_amd.finite_strain_simulator import finite_strain_simulation_3d
+    n = 5 if solver == "al" else 3
+    if solver == "al":
+        E = np.full(sh, 70e9)
+    else:
+        E = 70e9 * np.random.default_rng(3).uniform(0.8, 1.2, sh)
+    nu = np.full(sh, 0.3)
+    args = dict(mixed_targets=targets, model_type=model, solver=solver, **tols, **kw)
+    ref = fs.simulation(E, nu, drv, 0.01, n, **args)
+    got = finite_strain_simulation_3d(E, nu, drv, 0.01, n, enable_console=False, **args)
+    print(f"[fs3d sim] {model} {solver} {sh} F_macro ref={ref[0][-1].ravel()} got={got[0][-1].ravel()}", flush=True)
+    for a, b in zip(got[:3], ref):
+        assert a.shape == (n + 1, 3, 3)
+        assert np.abs(a - b).max() <= 1e-10 * max(1.0, np.abs(b).max())
+    assert len(got[3]) == len(got[4]) == n + 1 and got[3][-1].shape == (3, 3) + sh
+    if solver == "al":
+        # the reference test's own assertions
+        np.testing.assert_allclose(got[0][-1, 0, 0], 1.01, atol=1e-5)
+        np.testing.assert_allclose(got[1][-1, 1, 1], 0.0, atol=2e5)
+        np.testing.assert_allclose(got[1][-1, 2, 2], 0.0, atol=2e5)

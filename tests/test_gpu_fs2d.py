@@ -175,3 +175,20 @@ def test_2d_al_matches_oracle(gpu, mode):
     for fn in (f2.solver_step_2d, finite_strain_solver_step_2d):
         with pytest.raises(ValueError, match="AL solver did not converge or exploded"):
             fn(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="al", plane_mode=mode)
+
+
+def test_2d_reference_landau_al_case(gpu):
+    # the reference's plane-strain Landau AL simulation
+    # (tests/test_landau_finite_strain.py:4-34): parity with the oracle driver
+    # and the reference test's own assertions
+    from mgkmc_amd.finite_strain_simulator import finite_strain_simulation_2d
+    L = dict(v1=-100.0e9, v2=-50.0e9, v3=-10.0e9, g1=-1000.0e9, g2=500.0e9, g3=-100.0e9, g4=-200.0e9)
+    E = np.full((8, 8), 70.0e9)
+    nu = np.full((8, 8), 0.3)
+    ref = f2.simulation_2d(E, nu, (0, 0), 0.01, 5, plane_mode="plane_strain", model_type="landau", solver="al", **L)
+    got = finite_strain_simulation_2d(E, nu, (0, 0), 0.01, 5, plane_mode="plane_strain", model_type="landau",
+                                      solver="al", enable_console=False, **L)
+    for a, b in zip(got[:3], ref):
+        assert a.shape == (6, 2, 2)
+        assert np.abs(a - b).max() <= 1e-10 * max(1.0, np.abs(b).max())
+    np.testing.assert_allclose(got[0][-1, 0, 0], 1.01, atol=1e-5)

@@ -324,3 +324,25 @@ def test_dbfft_vs_newton_cg_reference_case():
     Fd, Pd, _, _, _ = fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="dbfft", **kw)
     assert np.linalg.norm(Fd - Fn) / np.linalg.norm(Fn) < 1e-5
     assert np.linalg.norm(Pd - Pn) / np.linalg.norm(Pn) < 1e-2
+
+
+REF_LANDAU = dict(v1=-100.0e9, v2=-50.0e9, v3=-10.0e9, g1=-1000.0e9, g2=500.0e9, g3=-100.0e9, g4=-200.0e9)
+
+
+def test_simulation_driver_meets_the_reference_tests_assertions():
+    # the reference's 3D Landau AL simulation with yy = zz = 0 traction
+    # (tests/test_landau_finite_strain.py:69-100): shapes, the driven F_xx and
+    # the free stresses within the reference's own tolerances; and the
+    # uniaxial-stress svk closed form: with P_yy = P_zz = 0 and a homogeneous
+    # grid the lateral stretch makes sig_yy vanish while sig_xx > 0
+    E = np.full((4, 4, 4), 70.0e9)
+    nu = np.full((4, 4, 4), 0.3)
+    Fm, Sm, Pm = fs.simulation(E, nu, (0, 0), 0.01, 5, mixed_targets={(1, 1): 0.0, (2, 2): 0.0},
+                               model_type="landau", solver="al", tol_macro=1e5, **REF_LANDAU)
+    assert Fm.shape == Sm.shape == Pm.shape == (6, 3, 3)
+    np.testing.assert_allclose(Fm[-1, 0, 0], 1.01, atol=1e-5)
+    np.testing.assert_allclose(Sm[-1, 1, 1], 0.0, atol=2e5)
+    np.testing.assert_allclose(Sm[-1, 2, 2], 0.0, atol=2e5)
+    np.testing.assert_allclose(Fm[:, 0, 0], 1.0 + np.linspace(0.0, 0.01, 6), atol=1e-12)
+    assert np.all(np.diff(Sm[:, 0, 0]) > 0)
+    assert np.all(Fm[1:, 1, 1] < 1.0) and np.allclose(Fm[:, 1, 1], Fm[:, 2, 2])

@@ -205,3 +205,19 @@ def test_2d_al_closed_form_and_heterogeneous_failure():
     C4 = f2.build_C4_2d(70e9 * np.random.default_rng(9).uniform(0.85, 1.15, sh), nu, I4s, II)
     with pytest.raises(ValueError, match="AL solver did not converge or exploded"):
         f2.solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="al")
+
+
+def test_2d_simulation_driver_meets_the_reference_tests_assertions():
+    # the reference's plane-strain Landau AL simulation
+    # (tests/test_landau_finite_strain.py:4-34): shapes and the driven F_xx;
+    # plane strain keeps F_yy = 1 (no mixed target), so sig_yy > 0 from the
+    # Poisson coupling
+    L = dict(v1=-100.0e9, v2=-50.0e9, v3=-10.0e9, g1=-1000.0e9, g2=500.0e9, g3=-100.0e9, g4=-200.0e9)
+    E = np.full((8, 8), 70.0e9)
+    nu = np.full((8, 8), 0.3)
+    Fm, Sm, Pm = f2.simulation_2d(E, nu, (0, 0), 0.01, 5, plane_mode="plane_strain", model_type="landau",
+                                  solver="al", **L)
+    assert Fm.shape == Sm.shape == Pm.shape == (6, 2, 2)
+    np.testing.assert_allclose(Fm[-1, 0, 0], 1.01, atol=1e-5)
+    np.testing.assert_allclose(Fm[:, 1, 1], 1.0, atol=1e-12)
+    assert Sm[-1, 1, 1] > 0 and Sm[-1, 0, 0] > Sm[-1, 1, 1]
