@@ -251,8 +251,11 @@ typedef struct {
   int capping;     /* landau strain capping on/off                          */
   int cap_smooth;  /* 1 smooth (tanh), 0 piecewise                          */
   int plane;       /* 0 3D; a 2D problem on an (nx, 1, ny) grid: 1 plane strain,
-                      2 plane-stress SVK (lam = the plane-stress lam*); the
-                      Newton stop test then uses the in-plane |F| and Cauchy
+                      2 plane stress (lam = the plane-stress lam*): svk, and
+                      uncapped landau / murnaghan at their out-of-plane E33
+                      with the condensed in-plane tangent
+                      (finite_strain_simulator.py:742-1031); the Newton stop
+                      test then uses the in-plane |F| and Cauchy
                       J = det F * F33 (finite_strain_simulator.py:2356-2572)  */
   double v1, v2, v3, g1, g2, g3, g4;  /* Landau constants in Pa             */
   double Am, Bm, Cm;                  /* Murnaghan constants                */

@@ -31,7 +31,7 @@ struct FsMat {
   int model;  // 0 svk, 1 landau, 2 murnaghan
   int capping;
   int cap_smooth;
-  int plane;  // 0 3D, 1 plane strain, 2 plane-stress SVK (2D on an (nx, 1, ny) grid)
+  int plane;  // 0 3D, 1 plane strain, 2 plane stress (2D on an (nx, 1, ny) grid)
   double v1, v2, v3, g1, g2, g3, g4;
   double Am, Bm, Cm;
   double Ecap, cap_p, cap_ratio;
@@ -86,6 +86,53 @@ __device__ __forceinline__ void inv3(const T* A, T* R) {
   R[8] = (A[0] * A[4] - A[1] * A[3]) / d;
 }
 
+// out-of-plane Green-Lagrange strain E33 of a plane-stress state from the
+// in-plane block of E on the (nx, 1, ny) grid (E_xx = E[0], E_xy = E[2],
+// E_yx = E[6], E_yy = E[8]); lam2 is lam* = C4_xxyy of the plane-stress
+// stiffness, the nonlinear laws use the 3D lam = 2 mu lam* / max(1e-14, 2 mu - lam*)
+//  svk (finite_strain_simulator.py:1039-1044): -(lam* / (2 mu)) tr E_2d
+//  landau, uncapped (:762-811): S33(E33) = 0 by Newton from the linear guess
+//    -(lam / (lam + 2 mu)) tr E_2d, at most 20 iterations, |C3333| < 1e-12 ->
+//    1e-12; a voxel stops once its own update is below 1e-12 (the reference
+//    stops on the grid maximum: Newton's later iterates move by round-off)
+//  murnaghan (:931-954): the root of a E33^2 + b E33 + c = 0 picked by
+//    (-b + sign(b) sqrt(max(0, disc))) / (2 a), -c / max(1e-14, b) for |a| < 1e-12
+template <typename T>
+__device__ T fs_plane_e33(const T* E, T lam2, T mu, const FsMat& m) {
+  const T t = E[0] + E[8];
+  if (m.model == 0) return -(lam2 / (T(2) * mu)) * t;
+  const T lam = T(2) * mu * lam2 / fmax(T(1e-14), T(2) * mu - lam2);
+  const T q = E[0] * E[0] + E[8] * E[8] + T(2) * E[2] * E[6];
+  if (m.model == 2) {
+    const T Am = T(m.Am), Bm = T(m.Bm), Cm = T(m.Cm);
+    const T a = Am + T(3) * Bm + Cm;
+    const T b = lam + T(2) * mu + T(2) * (Am + Bm) * t;
+    const T c = lam * t + Am * t * t + Bm * q;
+    if (fabs(a) < T(1e-12)) return -c / fmax(T(1e-14), b);
+    const T disc = fmax(T(0), b * b - T(4) * a * c);
+    return (-b + ((b < T(0)) ? -sqrt(disc) : sqrt(disc))) / (T(2) * a);
+  }
+  const T r = E[0] * E[0] * E[0] + E[8] * E[8] * E[8] + T(3) * E[2] * E[6] * t;  // tr(E_2d^3)
+  const T v1 = T(m.v1), v2 = T(m.v2), v3 = T(m.v3), g1 = T(m.g1), g2 = T(m.g2), g3 = T(m.g3), g4 = T(m.g4);
+  T e = -(lam / (lam + T(2) * mu)) * t;
+  for (int it = 0; it < 20; ++it) {
+    const T I1 = t + e, I2 = q + e * e, I3 = r + e * e * e;
+    const T A = lam * I1 + T(0.5) * v1 * I1 * I1 + v2 * I2 + (T(1) / T(6)) * g1 * I1 * I1 * I1 + g2 * I1 * I2 +
+                (T(4) / T(3)) * g3 * I3;
+    const T B = T(2) * (mu + v2 * I1 + T(0.5) * g2 * I1 * I1 + g4 * I2);
+    const T C = T(4) * (v3 + g3 * I1);
+    const T S33 = A + B * e + C * e * e;
+    const T dA = lam + v1 * I1 + T(0.5) * g1 * I1 * I1 + g2 * I2 + T(2) * (v2 + g2 * I1) * e + T(4) * g3 * e * e;
+    const T dB = T(2) * (v2 + g2 * I1) + T(4) * g4 * e;
+    T K = dA + dB * e + T(4) * g3 * e * e + B + T(2) * C * e;
+    if (fabs(K) < T(1e-12)) K = T(1e-12);
+    const T de = -S33 / K;
+    e += de;
+    if (fabs(de) < T(1e-12)) break;
+  }
+  return e;
+}
+
 // stress P and the tangent coefficients at deformation F
 template <typename T>
 __device__ void fs_state(const T* F, const T* Fpi, T lam, T mu, const FsMat& m, FsVox<T>& s, T* P) {
@@ -103,6 +150,12 @@ __device__ void fs_state(const T* F, const T* Fpi, T lam, T mu, const FsMat& m, 
   mtm3(s.Fe, s.Fe, Ce);
 #pragma unroll
   for (int k = 0; k < 9; ++k) s.E[k] = T(0.5) * (Ce[k] - ((k % 4 == 0) ? T(1) : T(0)));
+  if (m.plane == 2 && m.model != 0) {
+    // plane-stress landau / murnaghan: the 3D law at E_yy = E33 of the (nx, 1, ny) grid
+    s.E[4] = fs_plane_e33(s.E, lam, mu, m);
+    lam = T(2) * mu * lam / fmax(T(1e-14), T(2) * mu - lam);
+    s.lam = lam;
+  }
   s.al = s.be = s.ga = s.B = s.C = s.cw = T(0);
   if (m.model == 0) {
     const T trE = s.E[0] + s.E[4] + s.E[8];
@@ -228,23 +281,10 @@ __device__ void fs_state(const T* F, const T* Fpi, T lam, T mu, const FsMat& m, 
   }
 }
 
-// dP for direction dF at the state s
+// material tangent dS = C : X at the state s for a symmetric direction X
 template <typename T>
-__device__ void fs_tangent(const FsVox<T>& s, const FsMat& m, const T* dF, bool hasFp, T* dP) {
-  T dFe[9];
-  if (hasFp) mm3(dF, s.Fpi, dFe);
-  else {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) dFe[k] = dF[k];
-  }
-  T Y[9], X[9];
-  mtm3(s.Fe, dFe, Y);
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) X[3 * i + j] = T(0.5) * (Y[3 * i + j] + Y[3 * j + i]);
+__device__ __forceinline__ void fs_dS(const FsVox<T>& s, const FsMat& m, const T* X, T* dS) {
   const T trX = X[0] + X[4] + X[8];
-  T dS[9];
   if (m.model == 0) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) dS[k] = T(2) * s.mu * X[k] + ((k % 4 == 0) ? s.lam * trX : T(0));
@@ -292,6 +332,39 @@ __device__ void fs_tangent(const FsVox<T>& s, const FsMat& m, const T* dF, bool 
               T(2) * (s.mu + Bm * s.trE) * X[k] + Cm * (EXm[k] + XEm[k]);
     }
   }
+}
+
+// dP for direction dF at the state s
+template <typename T>
+__device__ void fs_tangent(const FsVox<T>& s, const FsMat& m, const T* dF, bool hasFp, T* dP) {
+  T dFe[9];
+  if (hasFp) mm3(dF, s.Fpi, dFe);
+  else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dFe[k] = dF[k];
+  }
+  T Y[9], X[9];
+  mtm3(s.Fe, dFe, Y);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) X[3 * i + j] = T(0.5) * (Y[3 * i + j] + Y[3 * j + i]);
+  T dS[9];
+  fs_dS(s, m, X, dS);
+  if (m.plane == 2 && m.model != 0) {
+    // plane-stress landau / murnaghan: the in-plane tangent condensed on the
+    // out-of-plane slot (finite_strain_simulator.py:389-395, :1026-1031),
+    // C_eff = C_abcd - C_ab33 C_cd33 / max(1e-14, C_3333); with the major
+    // symmetry of C, C_cd33 X_cd = dS(X)_33 and C_ab33 = dS(e_33)_ab, where
+    // the out-of-plane slot is the y axis of the (nx, 1, ny) grid
+    T e33[9], d33[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) e33[k] = (k == 4) ? T(1) : T(0);
+    fs_dS(s, m, e33, d33);
+    const T r = dS[4] / fmax(T(1e-14), d33[4]);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dS[k] -= d33[k] * r;
+  }
   T a1[9], a2[9], sum[9];
   mm3(dFe, s.S, a1);
   mm3(s.Fe, dS, a2);
@@ -311,14 +384,11 @@ __device__ void fs_tangent(const FsVox<T>& s, const FsMat& m, const T* dF, bool 
   }
 }
 
-// plane-stress SVK out-of-plane stretch (finite_strain_simulator.py:1062-1068):
-// E33 = -(lam* / (2 mu)) tr E_2d, F33 = sqrt(max(1e-12, 1 + 2 E33)); the 2D
-// problem sits on an (nx, 1, ny) grid, so tr E_2d = E_xx + E_zz
+// plane-stress out-of-plane stretch F33 = sqrt(max(1e-12, 1 + 2 E33))
+// (finite_strain_simulator.py:814, :974, :1044) from E_2d and lam* (fs_plane_e33)
 template <typename T>
-__device__ __forceinline__ T fs_plane_f33(const T* E, T lam, T mu) {
-  const T trE = E[0] + E[8];
-  const T e33 = -(lam / (T(2) * mu)) * trE;
-  return sqrt(fmax(T(1e-12), T(1) + T(2) * e33));
+__device__ __forceinline__ T fs_plane_f33(const T* E, T lam2, T mu, const FsMat& m) {
+  return sqrt(fmax(T(1e-12), T(1) + T(2) * fs_plane_e33(E, lam2, mu, m)));
 }
 
 struct Nine {
@@ -373,7 +443,9 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_const(size_t NV, const T* __restr
     acc[2] += f2;
     // Cauchy sigma = P F^T / J(F)  (cauchy_from_P_3d; 2D plane stress: J = det F * F33)
     double J = (double)det3(F);
-    if (m.plane == 2) J *= (double)fs_plane_f33(s.E, s.lam, s.mu);
+    if (m.plane == 2)
+      J *= (double)((m.model == 0) ? fs_plane_f33(s.E, s.lam, s.mu, m)
+                                   : sqrt(fmax(T(1e-12), T(1) + T(2) * s.E[4])));
     if (fabs(J) < 1e-14) J = 1e-14;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -409,11 +481,11 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_const(size_t NV, const T* __restr
   }
 }
 
-// Cauchy stress field (for the returned Sig); plane == 2 multiplies J by the
+// Cauchy stress field (for the returned Sig); m.plane == 2 multiplies J by the
 // plane-stress F33 of F_e (Fpi optional)
 template <typename T>
 __global__ void k_fs_cauchy(size_t NV, const T* __restrict__ P, const T* __restrict__ F, T* __restrict__ sig,
-                            int plane, const T* __restrict__ Fpi, const T* __restrict__ lam,
+                            FsMat m, const T* __restrict__ Fpi, const T* __restrict__ lam,
                             const T* __restrict__ mu) {
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < NV; v += (size_t)gridDim.x * blockDim.x) {
     T f[9], p[9];
@@ -423,7 +495,7 @@ __global__ void k_fs_cauchy(size_t NV, const T* __restrict__ P, const T* __restr
       p[k] = P[(size_t)k * NV + v];
     }
     T J = det3(f);
-    if (plane == 2) {
+    if (m.plane == 2) {
       T fe[9], fpi[9], ce[9], e[9];
       if (Fpi) {
 #pragma unroll
@@ -436,7 +508,7 @@ __global__ void k_fs_cauchy(size_t NV, const T* __restrict__ P, const T* __restr
       mtm3(fe, fe, ce);
 #pragma unroll
       for (int k = 0; k < 9; ++k) e[k] = T(0.5) * (ce[k] - ((k % 4 == 0) ? T(1) : T(0)));
-      J = J * fs_plane_f33(e, lam[v], mu[v]);
+      J = J * fs_plane_f33(e, lam[v], mu[v], m);
     }
     if (fabs((double)J) < 1e-14) J = T(1e-14);
     T o[9];
@@ -1360,7 +1432,7 @@ typedef struct {
   int model;  // 0 svk, 1 landau, 2 murnaghan
   int capping;
   int cap_smooth;
-  int plane;  // 0 3D; 2D on an (nx, 1, ny) grid: 1 plane strain, 2 plane-stress SVK
+  int plane;  // 0 3D; 2D on an (nx, 1, ny) grid: 1 plane strain, 2 plane stress
   double v1, v2, v3, g1, g2, g3, g4;  // Pa (the caller applies the GPa autoconversion)
   double Am, Bm, Cm;
   double cap_limit;  // resolved E_cap
@@ -2288,7 +2360,7 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
     return MGK_ERR_SOLVER;
   }
   if (P_out) MGK_HIP(hipMemcpyAsync(P_out, c.P, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
-  if (sig_out) k_fs_cauchy<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, c.P, F, sig_out, c.mat.plane, c.Fpi, c.lam, c.mu);
+  if (sig_out) k_fs_cauchy<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, c.P, F, sig_out, c.mat, c.Fpi, c.lam, c.mu);
   {
     const int nb = fs_nblk<T>(pl.NV);
     k_comp_means9<T><<<nb, MGK_NT, 0, s>>>(NV, F, c.part);
@@ -2559,10 +2631,10 @@ extern "C" int mgk_fs_cauchy(long long nvox, int precision, const void* P, const
   const int nb = (int)std::min<long long>((nvox + 255) / 256, 4096);
   hipStream_t s = (hipStream_t)stream;
   if (precision == 0)
-    k_fs_cauchy<double><<<nb, 256, 0, s>>>((size_t)nvox, (const double*)P, (const double*)F, (double*)sig, 0,
+    k_fs_cauchy<double><<<nb, 256, 0, s>>>((size_t)nvox, (const double*)P, (const double*)F, (double*)sig, FsMat{},
                                            (const double*)nullptr, (const double*)nullptr, (const double*)nullptr);
   else
-    k_fs_cauchy<float><<<nb, 256, 0, s>>>((size_t)nvox, (const float*)P, (const float*)F, (float*)sig, 0,
+    k_fs_cauchy<float><<<nb, 256, 0, s>>>((size_t)nvox, (const float*)P, (const float*)F, (float*)sig, FsMat{},
                                           (const float*)nullptr, (const float*)nullptr, (const float*)nullptr);
   MGK_HIP(hipGetLastError());
   return MGK_OK;

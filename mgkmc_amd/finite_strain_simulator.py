@@ -187,6 +187,20 @@ def finite_strain_solver_step_3d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, dri
 _PLANE_AXES = (0, 2)  # in-plane axes of the (nx, 1, ny) grid a 2D problem runs on
 
 
+def plane_law_supported(model_type, plane_mode, capping=False):
+    """Whether the 2D device step covers a law in a plane mode: svk,
+    murnaghan and landau in both modes, neo_hookean in plane strain; not
+    capped landau in plane stress, whose E33 iteration in the reference
+    (finite_strain_simulator.py:486-585) stops on the grid-wide maximum
+    update with linear convergence, so a per-voxel stop would not match it
+    to round-off."""
+    if model_type not in ("svk", "landau", "murnaghan", "neo_hookean"):
+        return False
+    if plane_mode == "plane_strain":
+        return True
+    return model_type in ("svk", "murnaghan") or (model_type == "landau" and not capping)
+
+
 def _embed2(A, fill=1.0):
     """(2, 2, nx, ny) [or (2, 2)] -> (3, 3, nx, 1, ny) [(3, 3)], out-of-plane diagonal = fill."""
     A = np.asarray(A, dtype=np.float64)
@@ -219,16 +233,18 @@ def finite_strain_solver_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, dri
     through the 3D device step on an (nx, 1, ny) grid: F_yy = 1, no
     out-of-plane shear, the projection's y wave number 0, so the in-plane
     fields, the Krylov iterates and the Newton / macro decisions are those of
-    the 2D step (its stop test over the in-plane |F|, and for plane-stress SVK
-    Cauchy's J = det F * F33).  Laws: svk (plane strain and plane stress),
-    landau and murnaghan (plane strain); lam = C4[0,0,1,1] and mu =
-    C4[0,1,0,1] of the 2D stiffness are read."""
+    the 2D step (its stop test over the in-plane |F|, and in plane stress
+    Cauchy's J = det F * F33).  Laws: svk, landau and murnaghan in both plane
+    modes (plane-stress landau uncapped: the reference's capped E33 iteration
+    stops on a grid-wide 1e-9 with linear convergence), neo_hookean in plane
+    strain; lam = C4[0,0,1,1] and mu = C4[0,1,0,1] of the 2D stiffness are
+    read."""
     if solver not in ("newton_cg", "dbfft", "al"):
         raise NotImplementedError(f"solver={solver!r}: the 2D device step implements 'newton_cg', 'al' and 'dbfft'")
-    if model_type not in ("svk", "landau", "murnaghan", "neo_hookean") or \
-            (model_type != "svk" and plane_mode != "plane_strain"):
-        raise NotImplementedError(f"2D {model_type} in {plane_mode}: the device step covers svk (both plane modes), "
-                                  "landau, murnaghan and neo_hookean (plane strain)")
+    if not plane_law_supported(model_type, plane_mode, strain_capping_enabled):
+        raise NotImplementedError(f"2D {model_type} in {plane_mode}: the device step covers svk, murnaghan and "
+                                  "landau (both plane modes; plane-stress landau uncapped) and neo_hookean "
+                                  "(plane strain)")
     C4 = np.asarray(C4)
     nx, ny = C4.shape[4:]
     sh3 = (nx, 1, ny)
@@ -356,8 +372,7 @@ def finite_strain_simulation_2d(E, nu, driving_component, eps_target, n_steps, m
     (F_macro, Sig_macro, P_macro, F_list, Sig_list) in 2D shapes."""
     if solver not in ("newton_cg", "dbfft", "al"):
         raise NotImplementedError(f"solver={solver!r}: the 2D device step implements 'newton_cg', 'al' and 'dbfft'")
-    if model_type not in ("svk", "landau", "murnaghan", "neo_hookean") or \
-            (model_type != "svk" and plane_mode != "plane_strain"):
+    if not plane_law_supported(model_type, plane_mode, strain_capping_enabled):
         raise NotImplementedError(f"2D {model_type} in {plane_mode}")
     mixed_targets = {} if mixed_targets is None else mixed_targets
     E = np.asarray(E, dtype=np.float64)

@@ -286,7 +286,7 @@ class DeviceKmc:
             # de Geus finite-strain state (mgkmc/aqs.py:205-215, kmc_simulator.py:156-168):
             # F, F_plastic = I; the step's own macro BC.  2D: the plane problem on an
             # (nx, 1, ny) grid of the 3D step, the lam of the plane mode
-            from .finite_strain_simulator import FsSolver, MIXED as FS_MIXED, material
+            from .finite_strain_simulator import FsSolver, MIXED as FS_MIXED, material, plane_law_supported
             if self.precision == FP32:
                 raise NotImplementedError("finite-strain KMC runs with ls_precision 'fp64' or 'mixed'")
             finite = dict(finite)
@@ -305,8 +305,7 @@ class DeviceKmc:
             self._fs_mat = material(**finite, mu_mean=float(np.mean(mu)), ndim=d)
             if d == 2:
                 model = finite.get("model_type", "svk")
-                if model not in ("svk", "landau", "murnaghan", "neo_hookean") or \
-                        (model != "svk" and plane_mode != "plane_strain"):
+                if not plane_law_supported(model, plane_mode, finite.get("strain_capping_enabled", False)):
                     raise NotImplementedError(f"2D finite-strain {model} in {plane_mode}")
                 self._fs_mat.plane = 2 if plane_mode == "plane_stress" else 1
                 self._fs_sig2 = _SigView()

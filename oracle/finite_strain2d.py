@@ -9,24 +9,32 @@ Follows mgkmc/finite_strain_simulator.py:
   :233-261      build_C4_2d (plane strain: 3D lambda; plane stress: E nu / (1 - nu^2))
   :591-1091     constitutive_hyperelastic_2d: svk (both plane modes, F33 of the
                 plane-stress E33 = -lam* tr E / (2 mu)), landau and murnaghan
-                in plane strain (E33 = 0; the einsum tangent of the 3D law cut
-                to the in-plane block), neo-Hookean in plane strain (:686-712:
-                S = mu I + (-mu + lam ln J) C^-1 and its tangent), Fp, the
-                push-forward to K4
+                (plane strain: E33 = 0; plane stress: the 3D lam from lam*,
+                E33 of S33 = 0 by landau's Newton (:762-811, uncapped) or
+                murnaghan's quadratic root (:931-954), the tangent condensed
+                on the 33 slot (:1026-1031)), the einsum tangent of the 3D law
+                cut to the in-plane block, neo-Hookean in plane strain
+                (:686-712: S = mu I + (-mu + lam ln J) C^-1 and its tangent),
+                Fp, the push-forward to K4
   :1094-1114    cauchy_from_P (J = det F * F33)
-  :2356-2572    finite_strain_solver_step_2d, solver="newton_cg" or "dbfft"
+  :2356-2572    finite_strain_solver_step_2d, solver "newton_cg", "dbfft" or "al"
   :1195-1210,1232-1248,1279-1313,1321-1349,1381-1507  the 2D DBFFT step (frequencies,
                 matrix-field inverse, u from F, grad u, the preconditioned system)
+  :2007-2140    _al_step_2d
   :2580-2850    finite_strain_simulation_2d (load stepping, store=False)
 The Krylov solver is SciPy's own bicgstab, called as the reference calls it.
-Not restated (the device step does not offer them): neo-Hookean, landau and
-murnaghan in plane stress, the AL and DBFFT 2D solvers.
+Not restated (the device step does not offer them): neo-Hookean and capped
+landau in plane stress.
 
 Pinned by (tests/test_oracle_fs2d.py): the homogeneous closed form, the 3D
 oracle on the same problem embedded as an (nx, 1, ny) grid (plane strain:
 identical fields and Newton / Krylov counts), the plane-stress SVK law against
-its closed form (sigma_33 = 0 of the linearised law), central differences of
-P(F) against K4, and equilibrium of the converged step.
+its closed form (sigma_33 = 0 of the linearised law), the plane-stress landau
+and murnaghan laws against the 3D oracle's law at F_33 = sqrt(1 + 2 E33)
+(P_33 = 0 and the same in-plane P), central differences of P(F) against K4
+(in plane stress: the derivative through E33(F), which pins the condensation),
+equilibrium of the converged step, the reference's DBFFT-vs-NCG and Landau AL
+simulation cases.
 """
 import numpy as np
 import scipy.fft as _sfft
@@ -179,12 +187,57 @@ def _law3d(E_GL, lam, mu, model_type, mat, need_tangent):
     return S, C4, None
 
 
+def _plane_stress_e33(E2d, lam, mu, model_type, mat):
+    """Out-of-plane E33 of a plane-stress state (lam the 3D lam).  landau,
+    uncapped (finite_strain_simulator.py:762-811): Newton on S33(E33) = 0 from
+    -(lam / (lam + 2 mu)) tr E_2d, C3333 -> 1e-12 where |C3333| < 1e-12, at
+    most 20 iterations, stop when the grid's max |dE33| < 1e-12.  murnaghan
+    (:931-954): a = A + 3B + C, b = lam + 2 mu + 2 (A + B) tr E_2d,
+    c = lam tr E_2d + A tr^2 + B tr(E_2d^2); E33 = -c / max(1e-14, b) where
+    |a| < 1e-12, else (-b + sign(b) sqrt(max(0, b^2 - 4ac))) / (2a), sign 0 -> 1."""
+    t = E2d[0, 0] + E2d[1, 1]
+    q = E2d[0, 0] ** 2 + E2d[1, 1] ** 2 + 2.0 * E2d[0, 1] * E2d[1, 0]
+    sh = lam.shape
+    if model_type == "murnaghan":
+        A, B, C = (np.full(sh, float(mat.get(k, 0.0))) for k in ("A_m", "B_m", "C_m"))
+        a = A + 3.0 * B + C
+        b = lam + 2.0 * mu + 2.0 * (A + B) * t
+        c = lam * t + A * t ** 2 + B * q
+        e = np.zeros(sh)
+        sm = np.abs(a) < 1e-12
+        e[sm] = -c[sm] / np.maximum(1e-14, b[sm])
+        disc = np.maximum(0.0, b[~sm] ** 2 - 4.0 * a[~sm] * c[~sm])
+        sg = np.sign(b[~sm])
+        sg[sg == 0] = 1.0
+        e[~sm] = (-b[~sm] + sg * np.sqrt(disc)) / (2.0 * a[~sm])
+        return e
+    v1, v2, v3, g1, g2, g3, g4 = (np.full(sh, c) for c in landau_constants(
+        [mat.get(k, 0.0) for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4")]))
+    r = np.einsum("ijxy,jkxy,kixy->xy", E2d, E2d, E2d)
+    e = -(lam / (lam + 2.0 * mu)) * t
+    for _ in range(20):
+        I1, I2, I3 = t + e, q + e ** 2, r + e ** 3
+        A = lam * I1 + 0.5 * v1 * I1 ** 2 + v2 * I2 + g1 * I1 ** 3 / 6.0 + g2 * I1 * I2 + (4.0 / 3.0) * g3 * I3
+        B = 2.0 * (mu + v2 * I1 + 0.5 * g2 * I1 ** 2 + g4 * I2)
+        C = 4.0 * (v3 + g3 * I1)
+        dA = lam + v1 * I1 + 0.5 * g1 * I1 ** 2 + g2 * I2 + 2.0 * (v2 + g2 * I1) * e + 4.0 * g3 * e ** 2
+        dB = 2.0 * (v2 + g2 * I1) + 4.0 * g4 * e
+        K = dA + dB * e + 4.0 * g3 * e ** 2 + B + 2.0 * C * e
+        K = np.where(np.abs(K) < 1e-12, 1e-12, K)
+        de = -(A + B * e + C * e ** 2) / K
+        e = e + de
+        if np.max(np.abs(de)) < 1e-12:
+            break
+    return e
+
+
 def constitutive_2d(F, C4, I2, I4, I4rt, Fp=None, model_type="svk", plane_mode="plane_strain", need_tangent=True,
                     **mat):
     """(P, K4, F33) of constitutive_hyperelastic_2d (:591-1091) for the laws
     listed in the module docstring."""
     if model_type not in ("svk", "landau", "murnaghan", "neo_hookean") or \
-            (model_type != "svk" and plane_mode == "plane_stress"):
+            (plane_mode == "plane_stress" and (model_type == "neo_hookean" or
+                                               (model_type == "landau" and mat.get("strain_capping_enabled")))):
         raise NotImplementedError(f"{model_type} / {plane_mode}")
     Fpi = _inv2(Fp) if Fp is not None else None
     Fe = np.einsum("ijxy,jkxy->ikxy", F, Fpi) if Fpi is not None else F
@@ -211,10 +264,20 @@ def constitutive_2d(F, C4, I2, I4, I4rt, Fp=None, model_type="svk", plane_mode="
     else:
         EG = np.zeros((3, 3) + lam.shape)
         EG[:2, :2] = E2d
+        F33 = np.ones_like(lam)
+        if plane_mode == "plane_stress":
+            lam = (2.0 * mu * lam) / np.maximum(1e-14, 2.0 * mu - lam)
+            EG[2, 2] = _plane_stress_e33(E2d, lam, mu, model_type, mat)
+            F33 = np.sqrt(np.maximum(1e-12, 1.0 + 2.0 * EG[2, 2]))
         S3, C43, _ = _law3d(EG, lam, mu, model_type, mat, need_tangent)
         S = S3[:2, :2]
-        C4e = C43[:2, :2, :2, :2] if need_tangent else None
-        F33 = np.ones_like(lam)
+        C4e = None
+        if need_tangent and plane_mode == "plane_stress":
+            C2222 = np.maximum(1e-14, C43[2, 2, 2, 2])
+            C22 = C43[:, :, 2, 2]
+            C4e = C43[:2, :2, :2, :2] - C22[:2, :2, None, None] * C22[None, None, :2, :2] / C2222
+        elif need_tangent:
+            C4e = C43[:2, :2, :2, :2]
     if Fpi is not None:
         P = np.einsum("ikxy,klxy,jlxy->ijxy", Fe, S, Fpi)
         if not need_tangent:

@@ -23,7 +23,10 @@ def _rel(a, b):
      "plane_strain"),
     ((8, 6), "murnaghan", dict(A_m=-100e9, B_m=-100e9, C_m=-100e9), False, "plane_strain"),
     ((64, 48), "svk", {}, False, "plane_stress"), ((8, 6), "neo_hookean", {}, False, "plane_strain"),
-    ((10, 12), "neo_hookean", {}, True, "plane_strain")])
+    ((10, 12), "neo_hookean", {}, True, "plane_strain"), ((8, 6), "landau", LANDAU, False, "plane_stress"),
+    ((10, 12), "landau", LANDAU, True, "plane_stress"),
+    ((8, 6), "murnaghan", dict(A_m=-100e9, B_m=-100e9, C_m=-100e9), False, "plane_stress"),
+    ((64, 48), "landau", LANDAU, False, "plane_stress")])
 def test_2d_step_parity(gpu, sh, model, kw, fp, mode):
     from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_2d
     rng = np.random.default_rng(4)
@@ -80,14 +83,17 @@ def test_2d_unsupported_laws_raise(gpu):
     I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
     C4 = f2.build_C4_2d(np.full(sh, 70e9), np.full(sh, 0.3), I4s, II)
     for kw in (dict(model_type="neo_hookean", plane_mode="plane_stress", solver="newton_cg"), dict(solver="bogus"),
-               dict(model_type="landau", plane_mode="plane_stress", solver="newton_cg")):
+               dict(model_type="landau", plane_mode="plane_stress", solver="newton_cg",
+                    strain_capping_enabled=True)):
         with pytest.raises(NotImplementedError):
             finite_strain_solver_step_2d(I2.copy(), np.eye(2), None, C4, I2, I4, I4rt, **kw)
 
 
 @pytest.mark.parametrize("model,mode,kw,drv,targets", [
     ("svk", "plane_stress", {}, (0, 0), {(1, 1): 0.0}),
-    ("landau", "plane_strain", dict(LANDAU, strain_capping_enabled=True), (0, 1), {(0, 0): 0.0, (1, 1): 0.0})])
+    ("landau", "plane_strain", dict(LANDAU, strain_capping_enabled=True), (0, 1), {(0, 0): 0.0, (1, 1): 0.0}),
+    ("landau", "plane_stress", LANDAU, (0, 1), {(0, 0): 0.0, (1, 1): 0.0}),
+    ("murnaghan", "plane_stress", dict(A_m=-100e9, B_m=-100e9, C_m=-100e9), (0, 1), {(0, 0): 0.0, (1, 1): 0.0})])
 def test_2d_simulation_parity(gpu, model, mode, kw, drv, targets):
     # finite_strain_simulation_2d: the macro F, Cauchy and P lists of every
     # load step equal the oracle driver's
@@ -108,7 +114,9 @@ def test_2d_simulation_parity(gpu, model, mode, kw, drv, targets):
 @pytest.mark.parametrize("sh,model,mode,fp", [((8, 6), "svk", "plane_strain", False),
                                               ((8, 6), "svk", "plane_stress", True),
                                               ((5, 9), "landau", "plane_strain", False),
-                                              ((10, 12), "neo_hookean", "plane_strain", True)])
+                                              ((10, 12), "neo_hookean", "plane_strain", True),
+                                              ((8, 6), "landau", "plane_stress", False),
+                                              ((10, 12), "landau", "plane_stress", True)])
 def test_2d_dbfft_parity(gpu, sh, model, mode, fp):
     # solver="dbfft" in 2D (the displacement step on the (nx, 1, ny) grid, its
     # stop test over the in-plane |P|): F, P, sigma of the oracle
@@ -192,3 +200,22 @@ def test_2d_reference_landau_al_case(gpu):
         assert a.shape == (6, 2, 2)
         assert np.abs(a - b).max() <= 1e-10 * max(1.0, np.abs(b).max())
     np.testing.assert_allclose(got[0][-1, 0, 0], 1.01, atol=1e-5)
+
+
+def test_2d_reference_landau_plane_stress_al_case(gpu):
+    # the reference's plane-stress Landau AL simulation with sigma_yy = 0
+    # (tests/test_landau_finite_strain.py:36-67): parity with the oracle driver
+    # and the reference test's own assertions
+    from mgkmc_amd.finite_strain_simulator import finite_strain_simulation_2d
+    L = dict(v1=-100.0e9, v2=-50.0e9, v3=-10.0e9, g1=-1000.0e9, g2=500.0e9, g3=-100.0e9, g4=-200.0e9)
+    E = np.full((8, 8), 70.0e9)
+    nu = np.full((8, 8), 0.3)
+    args = dict(mixed_targets={(1, 1): 0.0}, plane_mode="plane_stress", model_type="landau", solver="al",
+                tol_macro=1e5, **L)
+    ref = f2.simulation_2d(E, nu, (0, 0), 0.01, 5, **args)
+    got = finite_strain_simulation_2d(E, nu, (0, 0), 0.01, 5, enable_console=False, **args)
+    for a, b in zip(got[:3], ref):
+        assert a.shape == (6, 2, 2)
+        assert np.abs(a - b).max() <= 1e-10 * max(1.0, np.abs(b).max())
+    np.testing.assert_allclose(got[0][-1, 0, 0], 1.01, atol=1e-5)
+    np.testing.assert_allclose(got[1][-1, 1, 1], 0.0, atol=2e5)

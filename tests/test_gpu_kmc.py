@@ -570,6 +570,7 @@ def test_kmc2d_finite_strain_dbfft_matches_oracle(gpu, model, plane_mode, stz_mo
 @pytest.mark.parametrize("model,plane_mode,stz_mode", [("svk", "plane_strain", "pure_shear"),
                                                        ("svk", "plane_stress", "pure_shear"),
                                                        ("landau", "plane_strain", "pure_shear"),
+                                                       ("landau", "plane_stress", "pure_shear"),
                                                        ("svk", "plane_strain", "simple_shear"),
                                                        ("neo_hookean", "plane_strain", "simple_shear")])
 def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode, stz_mode, solver="newton_cg"):

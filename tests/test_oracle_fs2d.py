@@ -93,21 +93,57 @@ def test_2d_plane_stress_svk_closed_form():
     assert np.allclose(sig, F[:, :, 0, 0] @ S @ F[:, :, 0, 0].T / J, rtol=1e-12)
 
 
-@pytest.mark.parametrize("model,kw", [("landau", LANDAU), ("murnaghan", dict(A_m=-100e9, B_m=-100e9, C_m=-100e9)),
-                                      ("neo_hookean", {})])
-def test_2d_tangent_is_the_derivative_of_P(model, kw):
+MURN = dict(A_m=-100e9, B_m=-100e9, C_m=-100e9)
+
+
+@pytest.mark.parametrize("model,kw,mode", [("landau", LANDAU, "plane_strain"), ("murnaghan", MURN, "plane_strain"),
+                                           ("neo_hookean", {}, "plane_strain"), ("landau", LANDAU, "plane_stress"),
+                                           ("murnaghan", MURN, "plane_stress")])
+def test_2d_tangent_is_the_derivative_of_P(model, kw, mode):
+    # in plane stress P depends on F also through E33(F): the condensed
+    # tangent is that total derivative
     sh = (3, 2)
-    E, nu, t2, _, _ = _problem(sh, 7)
+    E, nu, t2, _, _ = _problem(sh, 7, plane_mode=mode)
     rng = np.random.default_rng(2)
     F = t2[0] + 0.02 * rng.standard_normal((2, 2) + sh)
     dF = rng.standard_normal((2, 2) + sh)
-    P, K4, _ = f2.constitutive_2d(F, t2[3], *t2[:3], model_type=model, **kw)
+    P, K4, _ = f2.constitutive_2d(F, t2[3], *t2[:3], model_type=model, plane_mode=mode, **kw)
     h = 1e-6
-    Pp = f2.constitutive_2d(F + h * dF, t2[3], *t2[:3], model_type=model, need_tangent=False, **kw)[0]
-    Pm = f2.constitutive_2d(F - h * dF, t2[3], *t2[:3], model_type=model, need_tangent=False, **kw)[0]
+    Pp = f2.constitutive_2d(F + h * dF, t2[3], *t2[:3], model_type=model, plane_mode=mode, need_tangent=False,
+                            **kw)[0]
+    Pm = f2.constitutive_2d(F - h * dF, t2[3], *t2[:3], model_type=model, plane_mode=mode, need_tangent=False,
+                            **kw)[0]
     fd = (Pp - Pm) / (2 * h)
     lin = f2._trans2(f2._ddot42(K4, f2._trans2(dF)))
     assert np.abs(fd - lin).max() < 1e-6 * np.abs(lin).max()
+
+
+@pytest.mark.parametrize("model,kw", [("landau", LANDAU), ("murnaghan", MURN)])
+def test_2d_plane_stress_laws_are_the_3d_law_at_zero_s33(model, kw):
+    # plane stress: with F_33 = sqrt(1 + 2 E33) the 3D oracle's law (its 3D
+    # lam = E nu / ((1 + nu)(1 - 2 nu)) equals the one recovered from lam*)
+    # gives P_33 = 0 and the same in-plane P; Cauchy's J carries F33
+    sh = (4, 3)
+    rng = np.random.default_rng(5)
+    E, nu = 70e9 * rng.uniform(0.8, 1.2, sh), np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    C4 = f2.build_C4_2d(E, nu, I4s, II, "plane_stress")
+    F = I2 + 0.03 * rng.standard_normal((2, 2) + sh)
+    P, _, F33 = f2.constitutive_2d(F, C4, I2, I4, I4rt, model_type=model, plane_mode="plane_stress",
+                                   need_tangent=False, **kw)
+    sh3 = sh + (1,)
+    F3 = np.zeros((3, 3) + sh3)
+    F3[:2, :2, ..., 0] = F
+    F3[2, 2, ..., 0] = F33
+    i2, i4, i4rt, i4s, ii = fs.identity_tensors(*sh3)
+    C43 = fs.build_C4(E.reshape(sh3), nu.reshape(sh3), i4s, ii)
+    P3 = fs.constitutive(F3, C43, i2, i4, i4rt, model_type=model, need_tangent=False, **kw)[0]
+    scale = np.abs(P).max()
+    assert np.abs(P3[2, 2]).max() < 1e-9 * scale
+    assert np.abs(P3[:2, :2, ..., 0] - P).max() < 1e-9 * scale
+    sig = f2.cauchy_2d(P, F, F33)
+    sig3 = fs.cauchy_from_P(P3, F3)[:2, :2, ..., 0]
+    assert np.abs(sig - sig3).max() < 1e-9 * np.abs(sig3).max()
 
 
 def test_2d_step_is_equilibrated_and_keeps_the_mean():
@@ -221,3 +257,21 @@ def test_2d_simulation_driver_meets_the_reference_tests_assertions():
     np.testing.assert_allclose(Fm[-1, 0, 0], 1.01, atol=1e-5)
     np.testing.assert_allclose(Fm[:, 1, 1], 1.0, atol=1e-12)
     assert Sm[-1, 1, 1] > 0 and Sm[-1, 0, 0] > Sm[-1, 1, 1]
+
+
+def test_2d_plane_stress_landau_simulation_is_the_3d_uniaxial_one():
+    # the reference's plane-stress Landau AL case (tests/test_landau_finite_strain.py:36-67):
+    # its assertions, and on a homogeneous grid uniaxial plane stress is the
+    # 3D uniaxial state (sigma_yy = sigma_zz = 0), so sigma_xx matches the 3D
+    # driver's to the macro tolerance
+    L = dict(v1=-100.0e9, v2=-50.0e9, v3=-10.0e9, g1=-1000.0e9, g2=500.0e9, g3=-100.0e9, g4=-200.0e9)
+    Fm, Sm, _ = f2.simulation_2d(np.full((8, 8), 70e9), np.full((8, 8), 0.3), (0, 0), 0.01, 5,
+                                 mixed_targets={(1, 1): 0.0}, plane_mode="plane_stress", model_type="landau",
+                                 solver="al", tol_macro=1e5, **L)
+    np.testing.assert_allclose(Fm[-1, 0, 0], 1.01, atol=1e-5)
+    np.testing.assert_allclose(Sm[-1, 1, 1], 0.0, atol=2e5)
+    F3, S3, _ = fs.simulation(np.full((4, 4, 4), 70e9), np.full((4, 4, 4), 0.3), (0, 0), 0.01, 5,
+                              mixed_targets={(1, 1): 0.0, (2, 2): 0.0}, model_type="landau", solver="al",
+                              tol_macro=1e5, **L)
+    assert abs(Sm[-1, 0, 0] - S3[-1, 0, 0]) < 1e-4 * S3[-1, 0, 0]
+    assert abs(Fm[-1, 1, 1] - F3[-1, 1, 1]) < 1e-5
