@@ -37,6 +37,9 @@ struct FsMat {
   double Ecap, cap_p, cap_ratio;
 };
 
+// plane-stress landau / murnaghan: E33 solve and condensed tangent needed
+static inline bool fs_ps(const FsMat& m) { return m.plane == 2 && m.model != 0; }
+
 template <typename T>
 struct FsVox {
   T Fe[9], S[9], E[9], E2[9], Fpi[9];
@@ -133,8 +136,9 @@ __device__ T fs_plane_e33(const T* E, T lam2, T mu, const FsMat& m) {
   return e;
 }
 
-// stress P and the tangent coefficients at deformation F
-template <typename T>
+// stress P and the tangent coefficients at deformation F; PS = false compiles
+// the plane-stress landau / murnaghan branch out (the 3D matvec instantiation)
+template <typename T, bool PS = true>
 __device__ void fs_state(const T* F, const T* Fpi, T lam, T mu, const FsMat& m, FsVox<T>& s, T* P) {
   s.lam = lam;
   s.mu = mu;
@@ -150,7 +154,7 @@ __device__ void fs_state(const T* F, const T* Fpi, T lam, T mu, const FsMat& m, 
   mtm3(s.Fe, s.Fe, Ce);
 #pragma unroll
   for (int k = 0; k < 9; ++k) s.E[k] = T(0.5) * (Ce[k] - ((k % 4 == 0) ? T(1) : T(0)));
-  if (m.plane == 2 && m.model != 0) {
+  if (PS && m.plane == 2 && m.model != 0) {
     // plane-stress landau / murnaghan: the 3D law at E_yy = E33 of the (nx, 1, ny) grid
     s.E[4] = fs_plane_e33(s.E, lam, mu, m);
     lam = T(2) * mu * lam / fmax(T(1e-14), T(2) * mu - lam);
@@ -334,8 +338,8 @@ __device__ __forceinline__ void fs_dS(const FsVox<T>& s, const FsMat& m, const T
   }
 }
 
-// dP for direction dF at the state s
-template <typename T>
+// dP for direction dF at the state s (PS as in fs_state)
+template <typename T, bool PS = true>
 __device__ void fs_tangent(const FsVox<T>& s, const FsMat& m, const T* dF, bool hasFp, T* dP) {
   T dFe[9];
   if (hasFp) mm3(dF, s.Fpi, dFe);
@@ -351,7 +355,7 @@ __device__ void fs_tangent(const FsVox<T>& s, const FsMat& m, const T* dF, bool 
     for (int j = 0; j < 3; ++j) X[3 * i + j] = T(0.5) * (Y[3 * i + j] + Y[3 * j + i]);
   T dS[9];
   fs_dS(s, m, X, dS);
-  if (m.plane == 2 && m.model != 0) {
+  if (PS && m.plane == 2 && m.model != 0) {
     // plane-stress landau / murnaghan: the in-plane tangent condensed on the
     // out-of-plane slot (finite_strain_simulator.py:389-395, :1026-1031),
     // C_eff = C_abcd - C_ab33 C_cd33 / max(1e-14, C_3333); with the major
@@ -518,8 +522,9 @@ __global__ void k_fs_cauchy(size_t NV, const T* __restrict__ P, const T* __restr
   }
 }
 
-// tangent action on a field (or a uniform direction dXu when dX == NULL)
-template <typename T>
+// tangent action on a field (or a uniform direction dXu when dX == NULL);
+// PS = fs_ps(m), so 3D and plane-strain launches carry no plane-stress code
+template <typename T, bool PS>
 __global__ __launch_bounds__(MGK_NT) void k_fs_dP(size_t NV, const T* __restrict__ F, const T* __restrict__ Fpi,
                                                   const T* __restrict__ lam, const T* __restrict__ mu, FsMat m,
                                                   const T* __restrict__ dX, Nine dXu, T* __restrict__ dP,
@@ -534,9 +539,9 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_dP(size_t NV, const T* __restrict
       d[k] = dX ? dX[(size_t)k * NV + v] : (T)dXu.v[k];
     }
     FsVox<T> s;
-    fs_state(f, Fpi ? fpi : (const T*)nullptr, lam[v], mu[v], m, s, (T*)nullptr);
+    fs_state<T, PS>(f, Fpi ? fpi : (const T*)nullptr, lam[v], mu[v], m, s, (T*)nullptr);
     T o[9];
-    fs_tangent(s, m, d, Fpi != nullptr, o);
+    fs_tangent<T, PS>(s, m, d, Fpi != nullptr, o);
 #pragma unroll
     for (int k = 0; k < 9; ++k) dP[(size_t)k * NV + v] = o[k];
   }
@@ -1649,8 +1654,12 @@ void fs_matvec(FsCtx<T>& c, const T* Fcur, const T* in, T* out, const T* dota, i
   const FsPlan& pl = c.pl;
   Nine z{};
   const int h = c.prof->begin(KID_FS_CONST, c.kry_it, (double)pl.NV * sizeof(T) * (27 + 2 + (c.Fpi ? 9 : 0)));
-  k_fs_dP<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, c.s>>>((size_t)pl.NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, in, z, c.dP,
-                                                    skip);
+  if (fs_ps(c.mat))
+    k_fs_dP<T, true><<<fs_nblk<T>(pl.NV), MGK_NT, 0, c.s>>>((size_t)pl.NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, in, z,
+                                                           c.dP, skip);
+  else
+    k_fs_dP<T, false><<<fs_nblk<T>(pl.NV), MGK_NT, 0, c.s>>>((size_t)pl.NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, in, z,
+                                                            c.dP, skip);
   c.prof->end(h);
   fs_project(c, c.dP, out, T(1.0 / (double)pl.NV), dota, selfdot, partials, skip);
 }
@@ -1968,7 +1977,10 @@ void fs_db_matvec(FsCtx<T>& c, const DbPar& d, const T* Fcur, const T* in, T* ou
   const double inv = 1.0 / (double)NV;
   fs_db_apply(c, d, DB_GRAD, 1, inv, in, c.Ftr, (const T*)nullptr, 0, (double*)nullptr, skip);
   Nine z{};
-  k_fs_dP<T><<<fs_nblk<T>(c.pl.NV), MGK_NT, 0, c.s>>>(NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, c.Ftr, z, c.dP, skip);
+  if (fs_ps(c.mat))
+    k_fs_dP<T, true><<<fs_nblk<T>(c.pl.NV), MGK_NT, 0, c.s>>>(NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, c.Ftr, z, c.dP, skip);
+  else
+    k_fs_dP<T, false><<<fs_nblk<T>(c.pl.NV), MGK_NT, 0, c.s>>>(NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, c.Ftr, z, c.dP, skip);
   fs_db_apply(c, d, DB_DIV, 1, inv, c.dP, out, dota, selfdot, partials, skip);
 }
 
@@ -2245,8 +2257,12 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
       info->newton_iters += 1;
       // rhs
       if (i_NW == 0) {
-        k_fs_dP<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, F, c.Fpi, c.lam, c.mu, c.mat, nullptr, DbarF, c.dP,
-                                                       nullptr);
+        if (fs_ps(c.mat))
+          k_fs_dP<T, true><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, F, c.Fpi, c.lam, c.mu, c.mat, nullptr, DbarF,
+                                                                c.dP, nullptr);
+        else
+          k_fs_dP<T, false><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, F, c.Fpi, c.lam, c.mu, c.mat, nullptr, DbarF,
+                                                                 c.dP, nullptr);
         fs_project(c, c.dP, c.b, T(-1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
       } else {
         fs_project(c, c.P, c.b, T(-1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
