@@ -226,6 +226,7 @@ def gpu_config3(n, events):
                             barrier_kwargs=dict(mean=2.2, std=0.45, min_cutoff=0.0), softening_scheme="directional",
                             softening_cap=0.916, jp=20.0, jt=0.0, temperature=0.03 * 2.2 / 8.617e-5,
                             strain_rate=1e7, ls_precision="mixed", rng=np.random.RandomState(3),
+                            output_dir=os.path.join(ROOT, "gpurun_out", "config3"),
                             selection_stream=np.random.RandomState(4), init_modes=("device", 3))
     del E, nu
     NV = n ** 3
@@ -240,7 +241,8 @@ def gpu_config3(n, events):
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     ev = sim.run_simulation(10 ** 9, 1e-4, component=(0, 0), stress_targets={(1, 1): 0.0, (2, 2): 0.0},
-                            mixed_tol=1e6, max_events=events)
+                            mixed_tol=1e6, enable_console_log=False, enable_summary_log=False,
+                            enable_global_log=False, enable_cascade_log=False, max_events=events)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     out = {"events": len(ev), "seconds": dt, "kmc_steps_per_s": len(ev) / dt, "setup_s": t_setup,

@@ -247,7 +247,8 @@ int mgk_kmc_init_modes(long long nvox, int M, uint64_t seed, double mean, double
  * Finite strain (de Geus Newton-Krylov).  Fields: 9 components F[3*i+j][v].
  * ------------------------------------------------------------------------- */
 typedef struct {
-  int model;       /* 0 svk, 1 landau, 2 murnaghan                          */
+  int model;       /* 0 svk, 1 landau, 2 murnaghan, 3 neo-Hookean (2D plane
+                      strain only: the (nx, 1, ny) grid, plane = 1)        */
   int capping;     /* landau strain capping on/off                          */
   int cap_smooth;  /* 1 smooth (tanh), 0 piecewise                          */
   int plane;       /* 0 3D; a 2D problem on an (nx, 1, ny) grid: 1 plane strain,
@@ -256,7 +257,9 @@ typedef struct {
                       with the condensed in-plane tangent
                       (finite_strain_simulator.py:742-1031); the Newton stop
                       test then uses the in-plane |F| and Cauchy
-                      J = det F * F33 (finite_strain_simulator.py:2356-2572)  */
+                      J = det F * F33 (finite_strain_simulator.py:2356-2572).
+                      Plane stress with capped landau or neo-Hookean, a model
+                      outside 0..3 or a plane outside 0..2 return MGK_ERR_ARG. */
   double v1, v2, v3, g1, g2, g3, g4;  /* Landau constants in Pa             */
   double Am, Bm, Cm;                  /* Murnaghan constants                */
   double cap_limit;                   /* resolved capping limit E_cap       */
@@ -394,6 +397,8 @@ typedef struct {
   int is_instab;
   double time;
   int ls_iters, ls_iters32;
+  double eps_macro[9];  /* macro strain after the event (d*d row-major)          */
+  double sig_mean[9];   /* <sig> after the event's re-solve (d*d row-major)      */
 } mgk_kmc_event;
 
 #define MGK_KMC_RUN_TIME 0       /* no event within *remaining: time advanced by it, *remaining = 0 */

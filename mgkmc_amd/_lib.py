@@ -62,7 +62,8 @@ class MgkKmcLoop(ctypes.Structure):
 
 class MgkKmcEvent(ctypes.Structure):
     _fields_ = [("v", ctypes.c_longlong), ("m", ctypes.c_int), ("is_instab", ctypes.c_int), ("time", ctypes.c_double),
-                ("ls_iters", ctypes.c_int), ("ls_iters32", ctypes.c_int)]
+                ("ls_iters", ctypes.c_int), ("ls_iters32", ctypes.c_int), ("eps_macro", ctypes.c_double * 9),
+                ("sig_mean", ctypes.c_double * 9)]
 
 
 MgkKmcLoop._fields_ = [

@@ -13,6 +13,7 @@ the reference's groups, attributes and datasets under their HDF5 paths as keys
 ("metadata/nx", "fields/eps_field", ...) of a NumPy .npz container written
 under the reference's file name, since h5py is not part of this stack.
 """
+import os
 from datetime import datetime
 
 import numpy as np
@@ -43,6 +44,7 @@ class ThermalSimulation(DeviceKmc):
             extra.append(f"finite-strain solver={solver!r} (the device step is 'newton_cg', 'al' or 'dbfft')")
         self._unsupported(cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model, extra)
         self.output_dir = output_dir
+        os.makedirs(output_dir, exist_ok=True)  # as the reference's constructor
         self.strain_assumption = strain_assumption
         self.hyperelastic_model = hyperelastic_model
         self.v1, self.v2, self.v3, self.g1, self.g2, self.g3, self.g4 = v1, v2, v3, g1, g2, g3, g4
@@ -78,16 +80,21 @@ class ThermalSimulation(DeviceKmc):
         Landau solver for hyperelastic_model="landau")."""
         return self._solve(self.eps_macro)
 
-    def mixed_bc(self, stress_targets, component, mixed_tol, mixed_max_iter, step=0):
+    def mixed_bc(self, stress_targets, component, mixed_tol, mixed_max_iter, step=0, eps_target=None, log_cb=None,
+                 track=False):
         """Elastic relaxation loop with mixed BCs (mgkmc/aqs.py:1103-1142): in
         cascade_mode each iteration first runs a cascade (None is returned
-        when it is truncated); the corrections re-solve with the linear LS
-        (update_stress_fft_full) whatever the law, as the reference does."""
+        when it is truncated or reaches eps_target); the corrections re-solve
+        with the linear LS (update_stress_fft_full) whatever the law, as the
+        reference does.  Returns (converged, last max error)."""
         E_avg, nu_avg = np.mean(self.E), np.mean(self.nu)
         converged = False
+        max_err = 0.0
         for _ in range(mixed_max_iter):
-            if self.cascade_mode and self._run_cascade(step)[2]:
-                return None
+            if self.cascade_mode:
+                r = self._run_cascade(step, eps_target=eps_target, component=component, log_cb=log_cb, track=track)
+                if r[2] or r[3]:
+                    return None
             sig_curr = self.sig_mean()
             err = np.zeros((3, 3))
             max_err = 0.0
@@ -102,84 +109,165 @@ class ThermalSimulation(DeviceKmc):
             corr[component] = 0.0
             self.eps_macro += corr
             self._solve_linear(self.eps_macro)
-        return converged
+        return converged, max_err
 
     def run_simulation(self, n_global_steps, step_size, component=(0, 1), stress_targets={}, mixed_tol=1e-4,
-                       mixed_max_iter=50, max_kmc_steps_pct=0.3, max_cascade_steps_pct=0.3, max_events=None,
-                       **kwargs):
-        """KMC + elastic increments (mgkmc/aqs.py:682-1150).  Checkpoints
-        follow the reference's checkpoint_interval ("current", an int of
-        global steps, "last"), checkpoint_path and checkpoint_elastic_only;
-        logging, VTK and stress-drop options are accepted and ignored.  In cascade_mode the initial relaxation is a cascade (no
-        solve before it) and every inner pass first runs a cascade while any
-        mode is unstable; a truncated cascade ends the run.  Cascade flips go
-        to ``cascade_history``.  ``max_events`` (extension) stops after that
-        many KMC events."""
+                       mixed_max_iter=50, checkpoint_interval=None, checkpoint_path="checkpoint",
+                       stop_on_stress_drop=None, stress_drop_component=None, stop_post_drop_steps=20,
+                       vtk_interval="none", vtk_elastic_only=True, track_cascades=False, enable_console_log=True,
+                       summary_filename="summary_log.txt", enable_summary_log=True, enable_global_log=True,
+                       enable_cascade_log=True, enable_kmc_log=True, checkpoint_elastic_only=False,
+                       enable_save_q=False, save_q_interval=None, save_q_elastic_only=False, max_kmc_steps_pct=0.3,
+                       max_cascade_steps_pct=0.3, ignore_drop_steps=0, stress_drop_lookback=1, append_logs=False,
+                       eps_target=None, cascade_mode=False, max_events=None, **kwargs):
+        """KMC + elastic increments (mgkmc/aqs.py:682-1200), the reference's
+        parameters in its order; ``max_events`` (extension, last) stops after
+        that many KMC events.  Honoured as the reference does: checkpoints
+        (checkpoint_interval "current" / int / "last", checkpoint_path,
+        checkpoint_elastic_only), the stress-drop stop (stop_on_stress_drop,
+        stress_drop_component, stop_post_drop_steps, ignore_drop_steps,
+        stress_drop_lookback), the cascade eps_target stop, track_cascades
+        (cascade flips appended to flip_event_history), enable_save_q /
+        save_q_interval (Q_step_XXXXXX.npy after elastic increments), the text
+        logs (summary_filename, enable_summary_log, enable_global_log,
+        enable_cascade_log, append_logs, enable_console_log; the reference
+        writes no kmc_log lines).  ``cascade_mode`` is read from the
+        constructor, as the reference does.  VTK output (vtk_interval other
+        than "none") is post-processing outside this build and raises
+        NotImplementedError.  The default stress targets are filled into a
+        copy, not into the caller's dict.  Returns kmc_event_history
+        (x, y, z, m, time, is_instab per KMC event; the reference returns None)."""
+        from .runlog import RunLog, StressDrop, save_q
+        if vtk_interval not in (None, "none"):
+            raise NotImplementedError("VTK export (mgkmc/analysis/vtk.py) is not part of the device build; "
+                                      "pass vtk_interval='none'")
         if max_cascade_steps_pct is not None:
             self.max_cascade_steps_pct = max_cascade_steps_pct
+        sdc = component if stress_drop_component is None else tuple(stress_drop_component)
         targets = dict(stress_targets)
-        self.driving_component = component
         if not targets:
             if component == (0, 0):
                 targets = {(1, 1): 0.0, (2, 2): 0.0}
             elif component == (0, 1):
                 targets = {(0, 0): 0.0, (1, 1): 0.0, (2, 2): 0.0}
+        self.driving_component = component
         self.stress_targets = targets
+        log = RunLog(self.output_dir, 3, summary_filename, enable_summary_log, enable_global_log, enable_cascade_log,
+                     enable_console_log, append_logs, thermal=self.enable_thermal,
+                     t_avg=lambda: float(np.mean(self.Tlocal)))
+        log.console_header()
         unit = np.zeros((3, 3))
         unit[component] = 1.0
-        if self.cascade_mode:
-            if self._run_cascade(0)[2]:
-                return self.flip_event_history
-        else:
-            self.update_stresses()
-        cp_int = kwargs.get("checkpoint_interval")
-        cp_path = kwargs.get("checkpoint_path", "checkpoint")
-        cp_elastic = kwargs.get("checkpoint_elastic_only", False)
-        cp_id = 0
-        if cp_int is not None and cp_int not in ("none", "last"):
-            name = f"{cp_path}.h5" if cp_int == "current" else \
-                (f"{cp_path}_elastic_{cp_id:06d}.h5" if cp_elastic else f"{cp_path}_000000.h5")
-            cp_id += int(bool(cp_elastic))
-            self.save_checkpoint(name, step=0)
-        dt_step = abs(step_size) / self.strain_rate if self.strain_rate > 0 else 1.0
-        max_seq = int(max_kmc_steps_pct * self.total_voxels)
-        seq = 0
-        remaining = 0.0
-        done = 0
-        events = 0
-        step = 1  # the reference's global step: +1 per KMC event and per elastic increment
-        while done < n_global_steps:
-            remaining += dt_step
-            while remaining > 0:
-                if self.cascade_mode and self._cascade_pairs():
-                    if self._run_cascade(step)[2]:
-                        return self.flip_event_history
-                    continue
-                # one event at a time in cascade mode (a cascade check follows each)
-                limit = 1 if self.cascade_mode else (None if max_events is None else max_events - events)
-                n, remaining, reason, seq = self.kmc_events(remaining, unit, limit, max_seq, seq)
-                events += n
-                step += n
-                if reason == "seq" or (max_events is not None and events >= max_events):
-                    return self.flip_event_history
-                if reason == "time":
-                    break
-            self.update_stresses()
-            # finite strain: the step itself enforces the stress targets (mgkmc/aqs.py:1089-1102)
-            if not self.finite and self.mixed_bc(targets, component, mixed_tol, mixed_max_iter, step) is None:
-                return self.flip_event_history
+        c = dict(elastic=0, kmc=0, step=1, events=0)
+        casc_count = 0  # the reference's cascade counter (its _run_cascade reports 0 flips, so it stays 0)
+
+        def do_log(step, kind, cid, flips, eps=None, sig=None):
+            eps = self.eps_macro.copy() if eps is None else eps
+            sig = self.sig_mean() if sig is None else sig
+            log.log_global(step, c["elastic"], c["kmc"], self.time, eps, sig, cid, flips)
+            self.history_global.append((eps[0, 0], sig[0, 0] / 1e9))
+            log.summary(step, kind, eps[sdc], sig[sdc], c["kmc"], cid, flips, self.time)
+            return sig[sdc]
+
+        def cascade_cb(local, nflips):
+            do_log(c["step"], "cascade", casc_count, nflips)
+
+        def cascade(step):
+            r = self._run_cascade(step, eps_target=eps_target, component=component, log_cb=cascade_cb,
+                                  track=track_cascades)
+            return r[2] or r[3]
+
+        try:
+            if self.cascade_mode:
+                if cascade(0):
+                    return self.kmc_event_history
+            else:
+                self.update_stresses()
+            drop = StressDrop(stop_on_stress_drop, stop_post_drop_steps, ignore_drop_steps, stress_drop_lookback,
+                              self.sig_mean()[sdc])
+            cp_int = checkpoint_interval
+            cp_id = 0
             if cp_int is not None and cp_int not in ("none", "last"):
-                if cp_int == "current":
-                    self.save_checkpoint(f"{cp_path}.h5", step=step)
-                elif isinstance(cp_int, int) and step % cp_int == 0:
-                    self.save_checkpoint(f"{cp_path}_elastic_{cp_id:06d}.h5" if cp_elastic else
-                                         f"{cp_path}_{step:06d}.h5", step=step)
-                    cp_id += int(bool(cp_elastic))
-            done += 1
-            step += 1
-        if cp_int == "last" or (isinstance(cp_int, int) and (step - 1) % cp_int != 0):
-            self.save_checkpoint(f"{cp_path}_final.h5", step=step - 1)
-        return self.flip_event_history
+                name = f"{checkpoint_path}.h5" if cp_int == "current" else \
+                    (f"{checkpoint_path}_elastic_{cp_id:06d}.h5" if checkpoint_elastic_only else
+                     f"{checkpoint_path}_000000.h5")
+                cp_id += int(bool(checkpoint_elastic_only))
+                self.save_checkpoint(name, step=0)
+            dt_step = abs(step_size) / self.strain_rate if self.strain_rate > 0 else 1.0
+            max_seq = int(max_kmc_steps_pct * self.total_voxels)
+            seq = 0
+            remaining = 0.0
+            stopped = []
+
+            def on_event(ev, eps, sig, over):
+                # aqs.py:1058-1072: log, count, unstable-sequence stop, history, step
+                val = do_log(c["step"], "kmc_instab" if ev[-1] else "kmc", casc_count if self.cascade_mode else 0,
+                             1, eps, sig)
+                c["kmc"] += 1
+                c["events"] += 1
+                if over:
+                    print(f"\n[TERMINATE] KMC instability sequence limit reached! {max_seq + 1} steps.")
+                    stopped.append(True)
+                    return
+                drop.add(val)
+                c["step"] += 1
+
+            while c["elastic"] < n_global_steps:
+                remaining += dt_step
+                while remaining > 0:
+                    if self.cascade_mode and self._cascade_pairs():
+                        if cascade(c["step"]):
+                            return self.kmc_event_history
+                        continue
+                    # one event at a time in cascade mode (a cascade check follows each)
+                    limit = 1 if self.cascade_mode else (None if max_events is None else max_events - c["events"])
+                    n, remaining, reason, seq = self.kmc_events(remaining, unit, limit, max_seq, seq, on_event)
+                    if reason == "seq" or stopped or (max_events is not None and c["events"] >= max_events):
+                        return self.kmc_event_history
+                    if reason == "time":
+                        break
+                step = c["step"]
+                self.update_stresses()
+                converged, max_err = True, 0.0
+                if self.finite:
+                    # aqs.py:1089-1102: a cascade, or a second finite-strain step from the converged state
+                    if self.cascade_mode:
+                        if cascade(step):
+                            return self.kmc_event_history
+                    else:
+                        self.update_stresses()
+                else:
+                    r = self.mixed_bc(targets, component, mixed_tol, mixed_max_iter, step, eps_target, cascade_cb,
+                                      track_cascades)
+                    if r is None:
+                        return self.kmc_event_history
+                    converged, max_err = r
+                if not converged:
+                    print(f"Warning: Mixed loop did not converge at step {step} (Err={max_err:.2e})")
+                c["elastic"] += 1
+                val = do_log(step, "elastic", casc_count, 0)
+                drop.after_increment(step, val)
+                if cp_int is not None and cp_int not in ("none", "last"):
+                    if cp_int == "current":
+                        self.save_checkpoint(f"{checkpoint_path}.h5", step=step)
+                    elif isinstance(cp_int, int) and step % cp_int == 0:
+                        self.save_checkpoint(f"{checkpoint_path}_elastic_{cp_id:06d}.h5" if checkpoint_elastic_only
+                                             else f"{checkpoint_path}_{step:06d}.h5", step=step)
+                        cp_id += int(bool(checkpoint_elastic_only))
+                if enable_save_q and save_q_interval and step % save_q_interval == 0:
+                    save_q(self.output_dir, step, self.Q)
+                if drop.stop_now():
+                    break
+                c["step"] += 1
+            last = c["step"] - 1
+            if cp_int == "last" or (isinstance(cp_int, int) and last % cp_int != 0):
+                self.save_checkpoint(f"{checkpoint_path}_final.h5", step=last)
+            log.finish()
+        finally:
+            log.close()
+        self.stress_drop_step = drop.trigger_step
+        self.last_step = c["step"]  # the global step at which the loop ended
+        return self.kmc_event_history
 
     _CKPT_META = ("nu0", "q_act_temp", "temperature", "strain_rate", "stability_threshold", "jp", "jt",
                   "neighbor_softening_fraction", "softening_cap", "softening_scheme", "scale_rate_by_volume",
@@ -190,7 +278,8 @@ class ThermalSimulation(DeviceKmc):
         """The reference's checkpoint content (mgkmc/aqs.py:1207-1262): metadata
         attributes, fields (eps, sig, E, nu, Tlocal), grid (eps_plastic,
         soft_prop, Q, Q0, catalog, last_event_time), state/eps_macro and
-        history/flips (the cascade flips, (step, batch, x, y, z, m), int32)."""
+        history/flips (flip_event_history: the cascade flips tracked with
+        track_cascades, (step, batch, x, y, z, m), int32)."""
         if not path.endswith(".h5"):
             path += ".h5"
         d = {}
@@ -207,8 +296,8 @@ class ThermalSimulation(DeviceKmc):
                   "grid/eps_plastic": self.eps_plastic, "grid/soft_prop": self.soft_prop, "grid/Q": self.Q,
                   "grid/Q0": self.Q0, "grid/catalog": self.catalog, "grid/last_event_time": self.last_event_time,
                   "state/eps_macro": np.asarray(self.eps_macro)})
-        if len(self.cascade_history) > 0:
-            d["history/flips"] = np.array(self.cascade_history, dtype=np.int32)
+        if len(self.flip_event_history) > 0:
+            d["history/flips"] = np.array(self.flip_event_history, dtype=np.int32)
         with open(path, "wb") as f:
             np.savez_compressed(f, **d)
 
@@ -244,6 +333,6 @@ class ThermalSimulation(DeviceKmc):
             if "state/eps_macro" in f.files:
                 sim.eps_macro = np.array(f["state/eps_macro"])
             if "history/flips" in f.files:
-                sim.cascade_history = [tuple(r) for r in f["history/flips"].tolist()]
+                sim.flip_event_history = f["history/flips"].tolist()
         print(f"Loaded checkpoint from {path} (SimTime={sim.time:.4e}s, Step={m['step']})")
         return sim

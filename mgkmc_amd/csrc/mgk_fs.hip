@@ -1790,10 +1790,26 @@ int fs_const(FsCtx<T>& c, const T* Fb, const T* dX, const Nine& dXu, double alph
   return MGK_OK;
 }
 
+// Material combinations the device laws implement: models 0..3, plane 0..2;
+// plane stress (plane 2) covers svk and the uncapped landau / murnaghan laws
+// (capped landau and neo-Hookean in plane stress are not built).
+static int fs_mat_check(const mgk_fs_material* mt) {
+  if (mt->model < 0 || mt->model > 3 || mt->plane < 0 || mt->plane > 2) {
+    mgk_set_error("finite-strain material: model must be 0..3 and plane 0..2", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  if (mt->plane == 2 && ((mt->model == 1 && mt->capping) || mt->model == 3)) {
+    mgk_set_error("plane-stress capped landau and neo-Hookean are not supported", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  return MGK_OK;
+}
+
 template <typename T>
 int fs_ctx_init(FsCtx<T>& c, const int64_t* shape, const T* lam, const T* mu, const mgk_fs_material* mt,
                 bool has_fp, char* ws, hipStream_t s) {
   constexpr int prec = sizeof(T) == 8 ? 0 : 1;
+  if (fs_mat_check(mt) != MGK_OK) return MGK_ERR_ARG;
   int rc = fs_plan<T>(shape, &c.pl);
   if (rc != MGK_OK) {
     mgk_set_error("unsupported grid shape for the finite-strain solver", __FILE__, __LINE__);
@@ -2624,6 +2640,7 @@ extern "C" int mgk_fs_constitutive(long long nvox, int precision, const void* F,
     mgk_set_error("mgk_fs_constitutive: bad argument", __FILE__, __LINE__);
     return MGK_ERR_ARG;
   }
+  if (fs_mat_check(mat) != MGK_OK) return MGK_ERR_ARG;
   const FsMat m = to_fsmat(mat);
   const int nb = (int)std::min<long long>((nvox + 255) / 256, 4096);
   hipStream_t s = (hipStream_t)stream;

@@ -66,6 +66,26 @@ struct Mt {
   }
 };
 
+// Pinned host staging for the redraw uploads, per thread, grown on demand and
+// kept until the process ends (a hipHostFree per call would synchronise the
+// whole device and stall the other streams of a serving process).
+struct RedrawStage {
+  double* p = nullptr;
+  size_t n = 0;
+};
+thread_local RedrawStage g_stage;
+
+double* redraw_stage(size_t count) {
+  if (g_stage.n >= count) return g_stage.p;
+  double* np_ = nullptr;
+  if (hipHostMalloc((void**)&np_, sizeof(double) * count, hipHostMallocDefault) != hipSuccess) return nullptr;
+  // a smaller earlier buffer is dropped rather than freed (hipHostFree would
+  // synchronise the device); growth happens only when M grows
+  g_stage.p = np_;
+  g_stage.n = count;
+  return np_;
+}
+
 bool unstable_mean(const mgk_ls_info& inf, int d) {
   double mx = 0.0;
   for (int i = 0; i < d * d; ++i) {
@@ -108,8 +128,11 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
     shape_i[i] = (int)L->shape[i];
   }
   Mt R{rng}, S{sel_rng};
-  double* host = nullptr;
-  MGK_HIP(hipHostMalloc((void**)&host, sizeof(double) * 3 * (size_t)M, hipHostMallocDefault));
+  double* host = redraw_stage(3 * (size_t)M);
+  if (!host) {
+    mgk_set_error("mgk_kmc_run: pinned host allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
   const double hg = 0.5 * L->gamma0;
   int n = 0, rc = MGK_KMC_RUN_TIME;
   *n_events = 0;
@@ -159,7 +182,12 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
             if (L->q_has_max) q = std::fmin(q, L->q_max);
             host[2 * M + i] = q;
           }
-        MGK_HIP(hipMemcpyAsync(L->redraw_dev, host, sizeof(double) * 3 * (size_t)M, hipMemcpyHostToDevice, s));
+        if (hipMemcpyAsync(L->redraw_dev, host, sizeof(double) * 3 * (size_t)M, hipMemcpyHostToDevice, s) !=
+            hipSuccess) {
+          mgk_set_error("mgk_kmc_run: redraw upload failed", __FILE__, __LINE__);
+          rc = MGK_ERR_HIP;
+          break;
+        }
         e = mgk_kmc_set_voxel(NV, M, 2, v, L->redraw_directions ? L->redraw_dev : nullptr,
                               L->redraw_barriers ? L->redraw_dev + 2 * M : nullptr, L->cat, L->Q0, stream);
         if (e != MGK_OK) {
@@ -193,6 +221,10 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
         ev.time = *time;
         ev.ls_iters = info->iters;
         ev.ls_iters32 = it32;
+        for (int k = 0; k < 9; ++k) {
+          ev.eps_macro[k] = k < d * d ? eps_macro[k] : 0.0;
+          ev.sig_mean[k] = info->sig_mean[k];
+        }
         *n_events = n;
         *seq = instab ? *seq + 1 : 0;
         if (*seq > max_seq) {
@@ -214,6 +246,5 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
     }
   }
   (void)hipStreamSynchronize(s);
-  (void)hipHostFree(host);
   return rc;
 }

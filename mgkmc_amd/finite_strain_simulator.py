@@ -178,6 +178,7 @@ def finite_strain_solver_step_3d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, dri
                    strain_capping_smooth_power, mu_mean=float(np.mean(mu)))
     info = s.step(F_bar, mat, driving_component, P_target, P_mask, E_avg, nu_avg, tol_NW, tol_CG, max_NW, tol_macro,
                   max_iter_macro, solver=solver, pixel=pixel)
+    finite_strain_solver_step_3d.last_info = info  # iteration counts of the last call (extension)
     Fo = dev.download(s.F)
     Po = dev.download(s.P)
     So = dev.download(s.sig)

@@ -346,8 +346,11 @@ class DeviceKmc:
         self.solver_args = {}
         self.enable_thermal = False
         self.Tfield = None
-        self.flip_event_history = []
+        self.kmc_event_history = []  # (x, y[, z], m, time, is_instab) per KMC event (extension)
+        self.flip_event_history = []  # tracked cascade flips (global step, batch, x, y, z, m), as the reference's
         self.cascade_history = []  # (global step, batch, x, y[, z], m) per cascade flip
+        self.history_global = []  # (eps_xx, sig_xx / 1e9) per logged step (mgkmc/aqs.py:760)
+        self._flip_log = None  # 2D flipped_voxels_log writer during a run
         self.max_cascade_steps_pct = 0.3
         self.ls_info = None
 
@@ -705,6 +708,8 @@ class DeviceKmc:
 
     def sig_mean(self):
         d = self.ndim
+        if self.ls_info is None:  # no solve yet (cascade mode before its first batch): the zero initial stress
+            return np.zeros((d, d))
         if self.finite:
             sm = np.array(self.ls_info.sig_mean[:9]).reshape(3, 3)
             if d == 2:
@@ -739,6 +744,8 @@ class DeviceKmc:
                 v, m = self.kmc.select(us, sig, self.sig_kind, p, Tfield=self.Tfield)
                 coords = tuple(int(c) for c in self.kmc.voxel_coords(v))
                 is_instab = self.kmc.sel.q <= self.stability_threshold
+                if self._flip_log is not None:
+                    self._log_flip(v, m, self.kmc.sel.q, "kmc")
                 # the reference conducts heat over t_wait before selecting, from
                 # rates computed before it: conducting after the selection is the same
                 self.heat_conducting(t_wait)
@@ -755,7 +762,7 @@ class DeviceKmc:
                 if self.ls_info is not None and not self.finite:
                     self.ls_iters_total += self.ls_info.iters + getattr(self.ls, "iters32", 0)
                 ev = coords + (m, self.time, is_instab)
-                self.flip_event_history.append(ev)
+                self.kmc_event_history.append(ev)
                 return ev, remaining_time
         self.eps_macro += strain_unit_tensor * (self.strain_rate * remaining_time)
         self.heat_conducting(remaining_time)
@@ -856,56 +863,102 @@ class DeviceKmc:
         self.kmc.barrier_rates(self._stress.sig, self.sig_kind, self._params(), write_q=True, Tfield=self.Tfield)
         v, m, q = self.kmc.unstable_modes(self.stability_threshold)
         pairs, seen = [], set()
+        self._pair_q = {}
         for i in np.argsort(q):
             if v[i] in seen:
                 continue
             seen.add(v[i])
             pairs.append((int(v[i]), int(m[i])))
+            self._pair_q[pairs[-1]] = float(q[i])
         return pairs
 
-    def _run_cascade(self, step):
+    def _run_cascade(self, step, eps_target=None, component=(0, 0), log_cb=None, track=False):
         """Cascade batches until no mode is unstable (mgkmc/aqs.py:480-680,
         kmc_simulator.py:577-794, small strain): each batch flips its modes in
         order (flip, prev, redraw; sub-stepped together under a 2D nonlinear
-        law), then one re-solve.  Returns (batches, flips, truncated): 3D stops
-        with truncated = True when the batch count exceeds
-        int(max_cascade_steps_pct * voxels); 2D stops after nx * ny + 1 batches."""
+        law), then one re-solve.  Returns (batches, flips, truncated,
+        stopped_by_eps): 3D stops with truncated = True when the batch count
+        exceeds int(max_cascade_steps_pct * voxels), and with stopped_by_eps =
+        True once eps_macro[component] >= eps_target after a batch (checked
+        after ``log_cb(batch, n_flips)``, aqs.py:660-666); ``track`` appends
+        the flips to flip_event_history as the reference's track_cascades does;
+        2D stops after nx * ny + 1 batches."""
         local = total = 0
         while True:
             pairs = self._cascade_pairs()
             if not pairs:
                 break
+            flog = self._flip_log is not None and self.ndim == 2
             if self.finite:
+                if flog:
+                    for v, m in pairs:
+                        self._log_flip(v, m, self._pair_q[(v, m)], "cascade")
                 self._flip_finite_pairs(pairs)
             elif self.law is not None and self.ndim == 2:
+                if flog:
+                    for v, m in pairs:
+                        self._log_flip(v, m, self._pair_q[(v, m)], "cascade")
                 self._flip_substepped(pairs)
             else:
                 for v, m in pairs:
+                    if flog:
+                        self._log_flip(v, m, self._pair_q[(v, m)], "cascade")
                     C = self._mode_host(v, m) if self.enable_thermal else None
                     self._flip(v, m, 1)
                     self._flip_heat(v, C)
                 self.update_stresses()
             for v, m in pairs:
-                self.cascade_history.append((step, local) + tuple(int(c) for c in self.kmc.voxel_coords(v)) + (m,))
+                rec = (step, local) + tuple(int(c) for c in self.kmc.voxel_coords(v)) + (m,)
+                self.cascade_history.append(rec)
+                if track and self.ndim == 3:
+                    self.flip_event_history.append(tuple(int(r) for r in rec))
             total += len(pairs)
+            if log_cb is not None:
+                log_cb(local, len(pairs))
+            if self.ndim == 3 and eps_target is not None and self.eps_macro[component] >= eps_target:
+                return local + 1, total, False, True
             local += 1
             if self.ndim == 3 and local > int(self.max_cascade_steps_pct * self.total_voxels):
-                return local, total, True
+                print(f"Cascade limit reached ({local} steps > {self.max_cascade_steps_pct * 100:.1f}% of "
+                      f"{self.total_voxels} voxels)")
+                return local, total, True, False
             if self.ndim == 2 and local > self.total_voxels:
                 break
-        return local, total, False
+        return local, total, False, False
+
+    def _log_flip(self, v, m, q, flip_type):
+        """One flipped_voxels_log line (mgkmc/kmc_simulator.py:535-557): the
+        voxel's softening g_p + g_t (decayed over the time since its last
+        event, capped) and the mode's barrier at selection."""
+        soft = dev.download(self.kmc.soft[:, int(v)])
+        tl = float(dev.download(self.kmc.tlast[int(v)]))
+        gp, gt = float(soft[0]), float(soft[1])
+        gtc = 0.0
+        if gt > 0:
+            if self.tau == np.inf:
+                gtc = gt
+            elif tl != -np.inf:
+                gtc = gt * np.exp(-max(self.time - tl, 0.0) / self.tau)
+        sft = gp + gtc
+        if self.softening_cap > 0 and sft > self.softening_cap:
+            sft = self.softening_cap
+        x, y = (int(c) for c in self.kmc.voxel_coords(v))
+        self._flip_log.write(self._flip_log_step, x, y, int(m), sft, float(q), flip_type)
 
     # ----- event sequences ----------------------------------------------------
-    def kmc_events(self, remaining_time, strain_unit_tensor, max_events=None, max_seq=None, seq=0):
+    def kmc_events(self, remaining_time, strain_unit_tensor, max_events=None, max_seq=None, seq=0, on_event=None):
         """Events until the remaining time holds no further event ("time": the
         time and macro strain then advance by what was left), max_events events
         ("count") or more than max_seq consecutive unstable-mode events ("seq").
-        Returns (n events, remaining time, reason, seq).  Linear-law KMC-mode
+        Returns (n events, remaining time, reason, seq).  ``on_event(event,
+        eps_macro, sig_mean, over_seq)`` is called after each event with the
+        macro strain and mean stress after its re-solve (over_seq: this event
+        ends the run on the unstable-sequence limit).  Linear-law KMC-mode
         simulations with the gaussian barrier generator run in the library's
         C++ loop (mgk_kmc_run; same streams, same sequence); others event by
         event through kmc_event."""
         if self._native_ok():
-            return self._run_native(remaining_time, strain_unit_tensor, max_events, max_seq, seq)
+            return self._run_native(remaining_time, strain_unit_tensor, max_events, max_seq, seq, on_event)
         n = 0
         while True:
             ev, remaining_time = self.kmc_event(remaining_time, strain_unit_tensor)
@@ -913,7 +966,10 @@ class DeviceKmc:
                 return n, remaining_time, "time", seq
             n += 1
             seq = seq + 1 if ev[-1] else 0
-            if max_seq is not None and seq > max_seq:
+            over = max_seq is not None and seq > max_seq
+            if on_event is not None:
+                on_event(ev, self.eps_macro.copy(), self.sig_mean(), over)
+            if over:
                 return n, remaining_time, "seq", seq
             if max_events is not None and n >= max_events:
                 return n, remaining_time, "count", seq
@@ -923,7 +979,7 @@ class DeviceKmc:
         gp = getattr(gen, "params", None)
         same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
         return (self.native_loop and self.law is None and not self.finite and not self.cascade_mode
-                and not self.enable_thermal and self.kmc.planar
+                and not self.enable_thermal and self.kmc.planar and self._flip_log is None
                 and gp is not None and same_stream and (self.redraw_directions or self.redraw_barriers)
                 and hasattr(self.rng, "get_state") and hasattr(self.selection_stream, "get_state")
                 and (self._solved_once or not self.ls_warm_start))
@@ -967,7 +1023,7 @@ class DeviceKmc:
             L.unit[i] = u[i]
         return L
 
-    def _run_native(self, remaining, unit, max_events, max_seq, seq):
+    def _run_native(self, remaining, unit, max_events, max_seq, seq, on_event=None):
         d = self.ndim
         lib = _lib.load()
         L = self._loop_struct(unit)
@@ -989,10 +1045,15 @@ class DeviceKmc:
                                      ctypes.byref(rem), chunk, ms, ctypes.byref(sq), evs, ctypes.byref(nb),
                                      ctypes.byref(self.ls.info), ctypes.byref(it32),
                                      ctypes.c_void_p(dev.stream_ptr()))
-                for e in evs[: nb.value]:
+                for i, e in enumerate(evs[: nb.value]):
                     coords = tuple(int(c) for c in self.kmc.voxel_coords(e.v))
-                    self.flip_event_history.append(coords + (int(e.m), float(e.time), bool(e.is_instab)))
+                    ev = coords + (int(e.m), float(e.time), bool(e.is_instab))
+                    self.kmc_event_history.append(ev)
                     self.ls_iters_total += e.ls_iters + e.ls_iters32
+                    if on_event is not None:
+                        over = rc == _lib.MGK_KMC_RUN_SEQ and i == nb.value - 1
+                        on_event(ev, np.array(e.eps_macro[: d * d]).reshape(d, d),
+                                 np.array(e.sig_mean[: d * d]).reshape(d, d), over)
                 n_total += nb.value
                 if nb.value:
                     self._solved_once = True

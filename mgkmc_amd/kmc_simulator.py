@@ -13,6 +13,8 @@ compliance-then-secant update on the host.  strain_assumption="finite_strain"
 with F_p updated by expm(C/N) sub-steps.  Extensions as in
 ``mgkmc_amd.aqs.ThermalSimulation``.
 """
+import os
+
 import numpy as np
 
 from . import _lib
@@ -43,6 +45,7 @@ class KmcSimulation2D(DeviceKmc):
         self._unsupported(cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model, extra)
         self.nx, self.ny = nx, ny
         self.output_dir = output_dir
+        os.makedirs(output_dir, exist_ok=True)  # as the reference's constructor
         self.strain_assumption = strain_assumption
         self.hyperelastic_model = hyperelastic_model
         self.stz_mode = stz_mode
@@ -119,53 +122,99 @@ class KmcSimulation2D(DeviceKmc):
                     self.eps_macro[idx] += step
 
     def run_simulation(self, n_global_steps, step_size, component=(0, 0), stress_targets={}, mixed_tol=1e-4,
-                       mixed_max_iter=50, max_kmc_steps_pct=0.3, max_cascade_steps_pct=0.3, max_events=None,
+                       mixed_max_iter=50, checkpoint_interval="none", checkpoint_path="checkpoint",
+                       vtk_interval="none", vtk_elastic_only=True, track_cascades=False, enable_console_log=True,
+                       summary_filename="summary_log.txt", enable_summary_log=True, enable_global_log=True,
+                       max_kmc_steps_pct=0.3, max_cascade_steps_pct=0.3, enable_kmc_log=False, max_events=None,
                        **kwargs):
-        """KMC + elastic increments (mgkmc/kmc_simulator.py:796-1143).  Logging,
-        checkpoint and VTK options are accepted and ignored.  In cascade_mode
-        every inner pass first runs a cascade while any mode is unstable; the
-        run ends when more than int(max_cascade_steps_pct nx ny) consecutive
-        cascades flipped something.  Cascade flips go to ``cascade_history``.
-        ``max_events`` (extension) stops after that many KMC events.  Returns
-        the event list (x, y, m, time, is_instab)."""
+        """KMC + elastic increments (mgkmc/kmc_simulator.py:796-1160), the
+        reference's parameters in its order; ``max_events`` (extension, last)
+        stops after that many KMC events.  Logs as the reference writes them:
+        summary (summary_filename, enable_summary_log), global_log.txt
+        (enable_global_log), flipped_voxels_log.txt (enable_kmc_log: one line
+        per flipped mode, its softening and barrier at selection), console
+        lines (enable_console_log).  checkpoint_interval / checkpoint_path and
+        track_cascades are accepted without effect, as in the reference's 2D
+        loop; VTK output (vtk_interval other than "none") raises
+        NotImplementedError.  In cascade_mode every inner pass first runs a
+        cascade while any mode is unstable; the run ends when more than
+        int(max_cascade_steps_pct nx ny) consecutive cascades flipped
+        something.  Cascade flips go to ``cascade_history``.  Returns
+        kmc_event_history (x, y, m, time, is_instab per KMC event)."""
+        from .runlog import RunLog, FlipLog
+        if vtk_interval not in (None, "none"):
+            raise NotImplementedError("VTK export (mgkmc/analysis/vtk.py) is not part of the device build; "
+                                      "pass vtk_interval='none'")
         self.driving_component = component
         self.stress_targets = stress_targets
-        unit = np.zeros((2, 2))
-        unit[component] = 1.0
-        self.elastic_run(self.eps_macro)
-        dt_step = abs(step_size) / self.strain_rate if self.strain_rate > 0 else 1.0
-        max_seq = int(max_kmc_steps_pct * self.nx * self.ny)
-        max_casc = int(max_cascade_steps_pct * self.nx * self.ny)
-        n_casc = 0
-        seq = 0
-        remaining = 0.0
-        done = 0
-        events = 0
-        step = 1  # the reference's global step: +1 per KMC event and per elastic increment
-        while done < n_global_steps:
-            remaining += dt_step
-            while remaining > 0:
-                if self.cascade_mode:
-                    if self._cascade_pairs():
-                        if self._run_cascade(step)[1] > 0:
-                            n_casc += 1
-                        if n_casc > max_casc:
-                            return self.flip_event_history
-                        continue
-                    n_casc = 0
-                # one event at a time in cascade mode (a cascade check follows each)
-                limit = 1 if self.cascade_mode else (None if max_events is None else max_events - events)
-                n, remaining, reason, seq = self.kmc_events(remaining, unit, limit, max_seq, seq)
-                events += n
-                step += n
-                if reason == "seq" or (max_events is not None and events >= max_events):
-                    return self.flip_event_history
-                if reason == "time":
-                    break
-            if self.finite:  # the step enforces the stress targets itself (kmc_simulator.py:1103-1104)
-                self.elastic_run(self.eps_macro)
-            else:
-                self.mixed_bc(stress_targets, mixed_tol, mixed_max_iter)
-            done += 1
-            step += 1
-        return self.flip_event_history
+        log = RunLog(self.output_dir, 2, summary_filename, enable_summary_log, enable_global_log,
+                     enable_console_log=enable_console_log, thermal=self.enable_thermal,
+                     t_avg=lambda: float(np.mean(self.Tlocal)), plane_mode=self.plane_mode,
+                     nu_avg=float(self.nu_field.mean()))
+        self._flip_log = FlipLog(self.output_dir) if enable_kmc_log else None
+        c = dict(elastic=0, kmc=0, step=1, events=0, casc=0)
+
+        def do_log(step, kind, cid, flips, eps=None, sig=None):
+            eps = self.eps_macro.copy() if eps is None else eps
+            sig = self.sig_mean() if sig is None else sig
+            log.log_global(step, c["elastic"], c["kmc"], self.time, eps, sig, cid, flips)
+            log.summary(step, kind, eps[component], sig[component], c["kmc"], cid, flips, self.time,
+                        first_2d=step == 0)
+
+        try:
+            unit = np.zeros((2, 2))
+            unit[component] = 1.0
+            self.elastic_run(self.eps_macro)
+            do_log(0, "INIT", 0, 0)
+            dt_step = abs(step_size) / self.strain_rate if self.strain_rate > 0 else 1.0
+            max_seq = int(max_kmc_steps_pct * self.nx * self.ny)
+            max_casc = int(max_cascade_steps_pct * self.nx * self.ny)
+            seq = 0
+            remaining = 0.0
+
+            def on_event(ev, eps, sig, over):
+                # kmc_simulator.py:1081-1092: unstable-sequence stop before the log line
+                if over:
+                    print(f"\n[TERMINATE] KMC instability sequence limit reached! {max_seq + 1} steps.")
+                    return
+                c["kmc"] += 1
+                c["events"] += 1
+                do_log(c["step"], "KMC_INSTAB" if ev[-1] else "KMC", c["casc"] if self.cascade_mode else 0, 1,
+                       eps, sig)
+                c["step"] += 1
+                self._flip_log_step = c["step"]
+
+            self._flip_log_step = c["step"]
+            while c["elastic"] < n_global_steps:
+                remaining += dt_step
+                while remaining > 0:
+                    if self.cascade_mode:
+                        if self._cascade_pairs():
+                            if self._run_cascade(c["step"])[1] > 0:
+                                c["casc"] += 1
+                            if c["casc"] > max_casc:
+                                print(f"\n[TERMINATE] Cascade instability sequence limit reached! "
+                                      f"{c['casc']} batches.")
+                                return self.kmc_event_history
+                            continue
+                        c["casc"] = 0
+                    # one event at a time in cascade mode (a cascade check follows each)
+                    limit = 1 if self.cascade_mode else (None if max_events is None else max_events - c["events"])
+                    n, remaining, reason, seq = self.kmc_events(remaining, unit, limit, max_seq, seq, on_event)
+                    if reason == "seq" or (max_events is not None and c["events"] >= max_events):
+                        return self.kmc_event_history
+                    if reason == "time":
+                        break
+                if self.finite:  # the step enforces the stress targets itself (kmc_simulator.py:1103-1104)
+                    self.elastic_run(self.eps_macro)
+                else:
+                    self.mixed_bc(stress_targets, mixed_tol, mixed_max_iter)
+                c["elastic"] += 1
+                do_log(c["step"], "ELAST", c["casc"], 0)
+                c["step"] += 1
+                self._flip_log_step = c["step"]
+            log.finish()
+        finally:
+            log.close()
+            self._flip_log = None
+        return self.kmc_event_history

@@ -174,7 +174,7 @@ def run_request(req):
             err = str(e)
         iters = sim.ls_iters_total
         eps_p_sum = float(sim.kmc.eps_p.to(torch.float64).sum())
-        res = SimResult(req.rid, tuple(req.shape), list(sim.flip_event_history), sim.eps_macro.copy(),
+        res = SimResult(req.rid, tuple(req.shape), list(sim.kmc_event_history), sim.eps_macro.copy(),
                         sim.sig_mean().copy() if sim.ls_info is not None else None, sim.time, eps_p_sum, iters, fp)
         if err is not None:
             res.extra["error"] = err

@@ -22,7 +22,9 @@ strain_assumption="finite_strain" (:273-330, :887-996, :1088-1102; svk
 and newton_cg): update() is the finite-strain step (oracle/finite_strain.py)
 from the current F with F_plastic, the macro BC built from the driven strain
 with F_macro as the initial F_bar; F_macro is the returned F_bar and the macro
-strain is read from it (F_ii - 1, F_ij); barriers use the Cauchy stress; a
+strain is read from it (F_ii - 1, F_ij); after each elastic increment's step
+the reference runs a cascade (cascade_mode) or a second step from the
+converged state (:1089-1102); barriers use the Cauchy stress; a
 NaN field or |<sigma>| > 20 GPa raises ValueError.  A flip is applied in N
 sub-steps F_p <- expm(C/N) F_p (N from max(2, ceil(sqrt(C:C/2) / 0.005)), at
 most 40, then N + 1 .. on ValueError, state restored), with softening, prev
@@ -91,9 +93,19 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                strain_assumption="small_strain", enable_thermal=False, Cp=420.0, rho=6125.0,
                thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
                tau_bath=0.0, solver="newton_cg", use_3d_barriers=False, redraw_directions=True,
-               redraw_barriers=True, barrier_generator="gaussian"):
+               redraw_barriers=True, barrier_generator="gaussian", stop_on_stress_drop=None,
+               stress_drop_component=None, stop_post_drop_steps=20, ignore_drop_steps=0, stress_drop_lookback=1,
+               eps_target=None):
     """Returns the list of events (x, y, z, m, time, is_instab) and final state.
-    ``solver``: the finite-strain step's solver ("newton_cg", "al", "dbfft")."""
+    ``solver``: the finite-strain step's solver ("newton_cg", "al", "dbfft").
+    Stress-drop stop (aqs.py:834, :1070, :1149-1181): the driven stress
+    history holds the mean stress after the initial relaxation, after each
+    event and after each increment; after an increment with step >
+    ignore_drop_steps a drop (|s_ref| - |s|) / |s_ref| against the entry
+    stress_drop_lookback back above stop_on_stress_drop arms the stop, and the
+    run ends after stop_post_drop_steps further increments (state
+    "drop_step", "stress_history").  eps_target (aqs.py:660-666): a cascade
+    batch after which eps_macro[component] >= eps_target ends the run."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
     bk = dict(mean=2.0, std=0.6, min_cutoff=0.1) if barrier_kwargs is None else dict(barrier_kwargs)
@@ -266,14 +278,20 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                     redraw(x, y, z, m)
                     cascades.append((step, local, int(x), int(y), int(z), int(m)))
                 update()
+            if eps_target is not None and eps_macro[component] >= eps_target:
+                state["cascade_stop"] = "eps_target"
+                return True
             local += 1
             if local > int(max_cascade_steps_pct * nx * ny * nz):
+                state["cascade_stop"] = "limit"
                 return True
         return False
 
     def result():
         out = dict(eps_plastic=ep, Q=Q, Q0=Q0, soft=soft, sig=state["sig"], eps_macro=eps_macro, time=t,
-                   cascades=cascades, substeps=state.get("substeps"), Tlocal=Tl)
+                   cascades=cascades, substeps=state.get("substeps"), Tlocal=Tl,
+                   stress_history=state.get("stress_history"), drop_step=state.get("drop_step"),
+                   last_step=state.get("last_step"), cascade_stop=state.get("cascade_stop"))
         if finite:
             out.update(F=Ff, F_plastic=Fp, F_macro=state["F_macro"])
         return events, out
@@ -287,11 +305,16 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
     unit = np.zeros((3, 3))
     unit[component] = 1.0
     events = []
+    sdc = component if stress_drop_component is None else tuple(stress_drop_component)
     if cascade_mode:
         if run_cascade(0):
             return result()
     else:
         update()
+    hist = [state["sm"][sdc]]
+    state["stress_history"] = hist
+    state["drop_step"] = None
+    countdown = stop_post_drop_steps
     dt_step = abs(step_size) / strain_rate if strain_rate > 0 else 1.0
     remaining = 0.0
     done = 0
@@ -327,6 +350,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                         redraw(x, y, z, m)
                         update()
                     events.append((int(x), int(y), int(z), int(m), t, bool(inst)))
+                    hist.append(state["sm"][sdc])
                     if len(events) >= n_events:
                         return result()
                     step += 1
@@ -336,6 +360,13 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
             t += remaining
             remaining = 0.0
         update()
+        if finite:
+            # aqs.py:1089-1102: a cascade, or a second finite-strain step from the converged state
+            if cascade_mode:
+                if run_cascade(step):
+                    return result()
+            else:
+                update()
         for _ in range(0 if finite else mixed_max_iter):
             if cascade_mode and run_cascade(step):
                 return result()
@@ -352,5 +383,20 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
             eps_macro += corr
             update()
         done += 1
+        cur = state["sm"][sdc]
+        hist.append(cur)
+        if stop_on_stress_drop is not None and state["drop_step"] is None and step > ignore_drop_steps:
+            lb = max(1, stress_drop_lookback)
+            if len(hist) >= lb:
+                ref = hist[-lb]
+                drop = (abs(ref) - abs(cur)) / abs(ref) if abs(ref) > 1e-6 else 0.0
+                if drop > stop_on_stress_drop:
+                    state["drop_step"] = step
+        if state["drop_step"] is not None:
+            if countdown > 0:
+                countdown -= 1
+            else:
+                break
         step += 1
+    state["last_step"] = step
     return result()

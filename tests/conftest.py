@@ -27,3 +27,11 @@ def gpu():
     from mgkmc_amd import _lib
     _lib.load()
     return True
+
+
+@pytest.fixture(autouse=True)
+def _run_in_scratch_dir(tmp_path_factory, monkeypatch):
+    """The simulators write their run logs under output_dir ("output",
+    relative to the working directory); each test runs in a scratch directory
+    of its own so that nothing lands in the repository."""
+    monkeypatch.chdir(tmp_path_factory.mktemp("cwd"))

@@ -25,7 +25,7 @@ def test_checkpoint_round_trip_and_resume(gpu, tmp_path):
     run = dict(step_size=0.001, component=(0, 0), stress_targets={(1, 1): 0.0, (2, 2): 0.0})
     sim1 = _sim(enable_thermal=True, temperature_cap=1200.0, strain_rate=1.0e4)
     sim1.run_simulation(n_global_steps=2, max_events=12, **run)
-    n1 = len(sim1.flip_event_history)
+    n1 = len(sim1.kmc_event_history)
     path = str(tmp_path / "cp")
     sim1.save_checkpoint(path, step=42)
     S, T = np.random.get_state(), K.SELECTION_STREAM.get_state()
@@ -45,9 +45,9 @@ def test_checkpoint_round_trip_and_resume(gpu, tmp_path):
     np.random.set_state(S)
     K.SELECTION_STREAM.set_state(T)
     sim2.run_simulation(n_global_steps=2, max_events=10, **run)
-    tail = sim1.flip_event_history[n1:]
+    tail = sim1.kmc_event_history[n1:]
     print(f"[ckpt] events before {n1}, after {len(tail)}", flush=True)
-    assert n1 > 0 and len(tail) > 0 and tail == sim2.flip_event_history
+    assert n1 > 0 and len(tail) > 0 and tail == sim2.kmc_event_history
     for k in ("eps_plastic", "sig_field", "Tlocal", "soft_prop", "Q0", "catalog"):
         assert np.array_equal(getattr(sim2, k), getattr(sim1, k)), k
 
@@ -59,7 +59,7 @@ def test_checkpoint_intervals_of_run_simulation(gpu, tmp_path):
     base = str(tmp_path / "run")
     sim.run_simulation(n_global_steps=3, step_size=0.0005, component=(0, 0), checkpoint_interval=2,
                        checkpoint_path=base)
-    assert len(sim.flip_event_history) == 0
+    assert len(sim.kmc_event_history) == 0
     names = sorted(p.name for p in tmp_path.iterdir())
     # step 0, every global step divisible by 2, and the final step when not already saved
     assert names == ["run_000000.h5", "run_000002.h5", "run_final.h5"], names

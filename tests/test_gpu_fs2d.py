@@ -49,7 +49,7 @@ def test_2d_step_parity(gpu, sh, model, kw, fp, mode):
                                        plane_mode=mode, tol_NW=1e-8, tol_CG=1e-8, solver="newton_cg", **kw)
     eF, eP, eS = _rel(got[0], ref[0]), _rel(got[1], ref[1]), _rel(got[2], ref[2])
     print(f"[fs2d] newton={st['newton']} relF={eF:.2e} relP={eP:.2e} relSig={eS:.2e}", flush=True)
-    assert eF < 1e-12 and eP < 1e-9 and eS < 1e-9
+    assert eF < 1e-12 and eP < 1e-10 and eS < 1e-10
 
 
 def test_2d_mixed_bc_parity(gpu):
@@ -74,7 +74,7 @@ def test_2d_mixed_bc_parity(gpu):
         got = finite_strain_solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="newton_cg", **kw)
         print(f"[fs2d mixed] {mode} Fbar ref={np.diag(ref[4])} got={np.diag(got[4])}", flush=True)
         assert np.abs(got[4] - ref[4]).max() < 1e-12
-        assert _rel(got[0], ref[0]) < 1e-11 and _rel(got[2], ref[2]) < 1e-8
+        assert _rel(got[0], ref[0]) < 1e-11 and _rel(got[2], ref[2]) < 1e-10
 
 
 def test_2d_unsupported_laws_raise(gpu):
@@ -141,7 +141,7 @@ def test_2d_dbfft_parity(gpu, sh, model, mode, fp):
     eF, eP, eS = _rel(got[0], ref[0]), _rel(got[1], ref[1]), _rel(got[2], ref[2])
     print(f"[fs2d dbfft] oracle newton={st['dbfft']} krylov={st['dbfft_krylov']} relF={eF:.2e} relP={eP:.2e} "
           f"relSig={eS:.2e}", flush=True)
-    assert eF < 1e-12 and eP < 1e-9 and eS < 1e-9
+    assert eF < 1e-12 and eP < 1e-10 and eS < 1e-10
 
 
 def test_2d_dbfft_vs_newton_cg_reference_case(gpu):

@@ -153,11 +153,11 @@ def test_kmc_event_sequence_matches_oracle(gpu):
     ref = [e[:4] for e in ev_ref]
     print("events:", got[:10], flush=True)
     assert got == ref
-    assert np.allclose([e[4] for e in ev], [e[4] for e in ev_ref], rtol=1e-9)
+    assert np.allclose([e[4] for e in ev], [e[4] for e in ev_ref], rtol=1e-10)
     ep = sim.eps_plastic
     assert np.allclose(ep, st_ref["eps_plastic"], rtol=0, atol=1e-15)
     sig = sim.sig_field
-    assert np.abs(sig - st_ref["sig"]).max() <= 1e-9 * np.abs(st_ref["sig"]).max()
+    assert np.abs(sig - st_ref["sig"]).max() <= 1e-10 * np.abs(st_ref["sig"]).max()
 
 
 @pytest.mark.parametrize("scheme,plane_mode", [("isotropic", "plane_strain"), ("directional", "plane_stress")])
@@ -187,12 +187,12 @@ def test_kmc2d_event_sequence_matches_oracle(gpu, scheme, plane_mode):
     ref = [e[:3] for e in ev_ref]
     print("events:", got[:8], flush=True)
     assert got == ref
-    assert np.allclose([e[3] for e in ev], [e[3] for e in ev_ref], rtol=1e-9)
+    assert np.allclose([e[3] for e in ev], [e[3] for e in ev_ref], rtol=1e-10)
     assert [e[4] for e in ev] == [e[4] for e in ev_ref]
     assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
-    assert np.abs(sim.eps_macro - st_ref["eps_macro"]).max() <= 1e-9 * np.abs(st_ref["eps_macro"]).max()
+    assert np.abs(sim.eps_macro - st_ref["eps_macro"]).max() <= 1e-10 * np.abs(st_ref["eps_macro"]).max()
     sig = sim.sig_field
-    assert np.abs(sig - st_ref["sig"]).max() <= 1e-9 * np.abs(st_ref["sig"]).max()
+    assert np.abs(sig - st_ref["sig"]).max() <= 1e-10 * np.abs(st_ref["sig"]).max()
 
 
 def test_device_init_modes_match_oracle(gpu):
@@ -264,8 +264,8 @@ def test_kmc2d_athermal_aqs_with_eigenstrain_matches_oracle(gpu):
     assert len(ev_ref) > 0
     assert [e[:3] for e in ev] == [e[:3] for e in ev_ref]
     assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
-    assert np.abs(sim.eps_macro - st_ref["eps_macro"]).max() <= 1e-9 * np.abs(st_ref["eps_macro"]).max()
-    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-9 * np.abs(st_ref["sig"]).max()
+    assert np.abs(sim.eps_macro - st_ref["eps_macro"]).max() <= 1e-10 * np.abs(st_ref["eps_macro"]).max()
+    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-10 * np.abs(st_ref["sig"]).max()
 
 
 _NL_LANDAU = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9)
@@ -312,7 +312,7 @@ def test_kmc2d_nonlinear_law_sequence_matches_oracle(gpu, model, plane_mode, mp,
         return real_run(eps_macro)
 
     def flip_substepped(flips):
-        if fail is not None and len(sim.flip_event_history) == fail:
+        if fail is not None and len(sim.kmc_event_history) == fail:
             armed[0] = True
         real_fs(flips)
         substeps.append(sim.last_substeps)
@@ -323,10 +323,10 @@ def test_kmc2d_nonlinear_law_sequence_matches_oracle(gpu, model, plane_mode, mp,
     print("events:", [e[:3] for e in ev[:8]], "substeps:", substeps, flush=True)
     assert substeps == st_ref["substeps"]
     assert [e[:3] for e in ev] == [e[:3] for e in ev_ref]
-    assert np.allclose([e[3] for e in ev], [e[3] for e in ev_ref], rtol=1e-9)
+    assert np.allclose([e[3] for e in ev], [e[3] for e in ev_ref], rtol=1e-10)
     assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
-    assert np.abs(sim.eps_macro - st_ref["eps_macro"]).max() <= 1e-9 * np.abs(st_ref["eps_macro"]).max()
-    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-8 * np.abs(st_ref["sig"]).max()
+    assert np.abs(sim.eps_macro - st_ref["eps_macro"]).max() <= 1e-10 * np.abs(st_ref["eps_macro"]).max()
+    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-10 * np.abs(st_ref["sig"]).max()
 
 
 def test_kmc3d_landau_sequence_matches_linear_limit_and_runs(gpu):
@@ -392,7 +392,7 @@ def test_kmc2d_cascade_mode_matches_oracle(gpu, T, mean, steps, model, mp):
     assert sim.cascade_history == st_ref["cascades"]
     assert [e[:3] for e in ev] == [e[:3] for e in ev_ref]
     assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
-    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-8 * np.abs(st_ref["sig"]).max()
+    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-10 * np.abs(st_ref["sig"]).max()
 
 
 @pytest.mark.parametrize("T,step_size,n_events,steps,mixed_tol", [
@@ -424,7 +424,7 @@ def test_kmc3d_cascade_mode_matches_oracle(gpu, T, step_size, n_events, steps, m
     assert sim.cascade_history == st_ref["cascades"]
     assert [e[:4] for e in ev] == [e[:4] for e in ev_ref]
     assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
-    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-8 * np.abs(st_ref["sig"]).max()
+    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-10 * np.abs(st_ref["sig"]).max()
 
 
 @pytest.mark.parametrize("d,prec,scheme", [(3, "fp64", "isotropic"), (3, "mixed", "directional"),
@@ -453,14 +453,14 @@ def test_native_loop_matches_python_loop(gpu, d, prec, scheme):
         sim.run_simulation(3, 2e-3, component=(0, 0), mixed_tol=1e6, max_events=40)
         sims.append(sim)
     a, b = sims
-    print(f"[native loop] {len(a.flip_event_history)} events, first {a.flip_event_history[:3]}", flush=True)
-    assert len(a.flip_event_history) == 40
-    assert [e[:-2] for e in a.flip_event_history] == [e[:-2] for e in b.flip_event_history]
-    assert [e[-1] for e in a.flip_event_history] == [e[-1] for e in b.flip_event_history]
-    assert np.allclose([e[-2] for e in a.flip_event_history], [e[-2] for e in b.flip_event_history], rtol=1e-12)
+    print(f"[native loop] {len(a.kmc_event_history)} events, first {a.kmc_event_history[:3]}", flush=True)
+    assert len(a.kmc_event_history) == 40
+    assert [e[:-2] for e in a.kmc_event_history] == [e[:-2] for e in b.kmc_event_history]
+    assert [e[-1] for e in a.kmc_event_history] == [e[-1] for e in b.kmc_event_history]
+    assert np.allclose([e[-2] for e in a.kmc_event_history], [e[-2] for e in b.kmc_event_history], rtol=1e-12)
     assert a.ls_iters_total == b.ls_iters_total
     assert np.array_equal(a.eps_plastic, b.eps_plastic)
-    assert np.abs(a.sig_field - b.sig_field).max() <= 1e-9 * np.abs(b.sig_field).max()
+    assert np.abs(a.sig_field - b.sig_field).max() <= 1e-10 * np.abs(b.sig_field).max()
     sa, sb = a.rng.get_state(), b.rng.get_state()
     assert np.array_equal(sa[1], sb[1]) and sa[2:] == sb[2:]
     assert a.selection_stream.get_state()[2] == b.selection_stream.get_state()[2]
@@ -499,12 +499,16 @@ def test_kmc3d_finite_strain_matches_oracle(gpu, prec, solver):
     print(f"[kmc3d finite] events {[e[:4] for e in ev]} substeps {subs} (oracle {st_ref['substeps']})", flush=True)
     assert [e[:4] for e in ev] == [e[:4] for e in ev_ref]
     assert subs == st_ref["substeps"] and max(subs) >= 3
-    tol = 1e-9 if prec == "fp64" else 1e-5
+    # FP64: fields to 1e-10 relative; FP32 Krylov: within its Newton tolerance
+    tol, stol = (1e-10, 1e-10) if prec == "fp64" else (1e-5, 1e-2)
     Fr = np.einsum("ijxyz->xyzij", st_ref["F"])
-    assert np.abs(sim.F_field - Fr).max() <= tol
+    eF = np.abs(sim.F_field - Fr).max() / np.abs(Fr).max()
+    eS = np.abs(sim.sig_field - st_ref["sig"]).max() / np.abs(st_ref["sig"]).max()
+    print(f"[kmc3d finite] relF={eF:.2e} relSig={eS:.2e}", flush=True)
+    assert eF <= tol
     assert np.abs(sim.F_plastic - np.einsum("ijxyz->xyzij", st_ref["F_plastic"])).max() <= 1e-14
     assert np.abs(sim.F_macro - st_ref["F_macro"]).max() <= tol
-    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= tol * 1e3 * np.abs(st_ref["sig"]).max()
+    assert eS <= stol
 
 
 @pytest.mark.parametrize("d,thermostat,tau_bath,cascade,coords", [
@@ -557,7 +561,7 @@ def test_kmc_thermal_matches_oracle(gpu, d, thermostat, tau_bath, cascade, coord
     assert [e[:d + 1] for e in ev] == [e[:d + 1] for e in ev_ref]
     if cascade:
         assert sim.cascade_history == st_ref["cascades"]
-    assert np.abs(Tl - st_ref["Tlocal"]).max() <= 1e-9 * st_ref["Tlocal"].max()
+    assert np.abs(Tl - st_ref["Tlocal"]).max() <= 1e-10 * st_ref["Tlocal"].max()
     assert Tl.max() > T  # flips heated something
 
 
@@ -611,10 +615,13 @@ def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode, stz_mode, so
     assert [e[:3] for e in ev] == [e[:3] for e in ev_ref]
     assert subs == st_ref["substeps"]
     assert sim.kmc.planar == (stz_mode == "pure_shear")
-    assert np.abs(sim.F_field - np.einsum("ijxy->xyij", st_ref["F"])).max() <= 1e-9
+    eF = np.abs(sim.F_field - np.einsum("ijxy->xyij", st_ref["F"])).max()
+    eS = np.abs(sim.sig_field - st_ref["sig"]).max() / np.abs(st_ref["sig"]).max()
+    print(f"[kmc2d finite] relF={eF:.2e} relSig={eS:.2e}", flush=True)
+    assert eF <= 1e-10
     assert np.abs(sim.F_plastic - np.einsum("ijxy->xyij", st_ref["F_plastic"])).max() <= 1e-14
-    assert np.abs(sim.F_macro - st_ref["F_macro"]).max() <= 1e-9
-    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-6 * np.abs(st_ref["sig"]).max()
+    assert np.abs(sim.F_macro - st_ref["F_macro"]).max() <= 1e-10
+    assert eS <= 1e-10
 
 
 @pytest.mark.parametrize("d,T,mean,g0", [(2, 400.0, 0.3, 0.05), (2, 0.0, 0.5, 0.1), (3, 800.0, 0.5, 0.03)])
@@ -663,7 +670,8 @@ def test_finite_strain_cascade_mode_matches_oracle(gpu, d, T, mean, g0):
     assert len(st_ref["cascades"]) > 0
     assert sim.cascade_history == st_ref["cascades"]
     assert [e[:d + 1] for e in ev] == [e[:d + 1] for e in ev_ref]
-    assert np.abs(sim.F_field - Fr).max() <= 1e-9
+    print(f"[finite cascade] relF={np.abs(sim.F_field - Fr).max():.2e}", flush=True)
+    assert np.abs(sim.F_field - Fr).max() <= 1e-10
     assert np.abs(sim.F_plastic - Fpr).max() <= 1e-13
 
 
@@ -692,7 +700,7 @@ def test_kmc3d_3d_barriers_match_oracle(gpu, scheme):
     print(f"[kmc3d 3d barriers] events {[e[:4] for e in ev][:6]}", flush=True)
     assert [e[:4] for e in ev] == [e[:4] for e in ev_ref]
     assert np.allclose(sim.eps_plastic, st_ref["eps_plastic"], rtol=0, atol=1e-15)
-    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-8 * np.abs(st_ref["sig"]).max()
+    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-10 * np.abs(st_ref["sig"]).max()
 
 
 @pytest.mark.parametrize("d,kind", [(3, "finite"), (2, "finite"), (2, "secant")])
@@ -740,9 +748,11 @@ def test_thermal_with_finite_strain_and_nonlinear_laws(gpu, d, kind):
     print(f"[thermal combo] events {[e[:d + 1] for e in ev]}; T max {sim.Tlocal.max():.3f} "
           f"(oracle {st_ref['Tlocal'].max():.3f})", flush=True)
     assert [e[:d + 1] for e in ev] == [e[:d + 1] for e in ev_ref]
-    assert np.abs(sim.Tlocal - st_ref["Tlocal"]).max() <= 1e-9 * st_ref["Tlocal"].max()
+    assert np.abs(sim.Tlocal - st_ref["Tlocal"]).max() <= 1e-10 * st_ref["Tlocal"].max()
     assert st_ref["Tlocal"].max() > 500.0 * 1.0000001 or d == 3
-    assert np.abs(sim.sig_field - st_ref["sig"]).max() <= 1e-6 * np.abs(st_ref["sig"]).max()
+    eS = np.abs(sim.sig_field - st_ref["sig"]).max() / np.abs(st_ref["sig"]).max()
+    print(f"[thermal combo] relSig={eS:.2e}", flush=True)
+    assert eS <= 1e-10
 
 
 @pytest.mark.parametrize("d,rd,rb", [(3, False, False), (2, False, False), (3, False, True), (2, True, False)])

@@ -1,0 +1,167 @@
+"""Run-control options of the simulators' run loops on the device, against the
+oracle loop: the stress-drop stop (mgkmc/aqs.py:1151-1181), the cascade
+eps_target stop (:660-666), the text logs (:353-415, :750-778;
+mgkmc/kmc_simulator.py:177-224, :535-557), Q output (:1174-1177) and the
+finite-strain cascade after each increment (:1089-1102)."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(n, seed=1):
+    rng = np.random.default_rng(seed)
+    E = 70e9 * rng.uniform(0.9, 1.1, (n, n, n))
+    return E, np.full((n, n, n), 0.3)
+
+
+def test_stress_drop_stop_matches_oracle(gpu, tmp_path):
+    # 600 K, barriers 1.0 +- 0.15 eV: ~90 events per 2e-3 increment relax the
+    # stress; a 5% drop against the entry 4 back arms the stop, which ends the
+    # run two increments later
+    from mgkmc_amd.aqs import ThermalSimulation
+    from oracle.aqs_loop import run_events
+    n, M = 8, 10
+    E, nu = _fields(n)
+    kw = dict(mean=1.0, std=0.15, min_cutoff=0.0)
+    drop = dict(stop_on_stress_drop=0.05, stop_post_drop_steps=2, stress_drop_lookback=4, ignore_drop_steps=2)
+    ev_ref, st = run_events(n, n, n, M, 0.14, E, nu, 0.7, 600.0, 1e7, 2e-3, (0, 0), 10 ** 9, seed=7, sel_seed=8,
+                            barrier_kwargs=kw, mixed_tol=1e5, ls_tol=1e-8, n_global_steps=12, softening_cap=0.916,
+                            jp=20.0, jt=0.0, **drop)
+    sim = ThermalSimulation(n, n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                            temperature=600.0, strain_rate=1e7, softening_cap=0.916, jp=20.0, jt=0.0,
+                            output_dir=str(tmp_path), rng=np.random.RandomState(7),
+                            selection_stream=np.random.RandomState(8))
+    sim.solver_args = {"tol": 1e-8}
+    ev = sim.run_simulation(12, 2e-3, component=(0, 0), mixed_tol=1e5, enable_console_log=False, **drop)
+    print(f"[stress drop] events {len(ev)} (oracle {len(ev_ref)}), drop step {sim.stress_drop_step} "
+          f"(oracle {st['drop_step']}), last step {sim.last_step} (oracle {st['last_step']})", flush=True)
+    assert st["drop_step"] is not None and st["last_step"] < len(ev_ref) + 12
+    assert sim.stress_drop_step == st["drop_step"]
+    assert sim.last_step == st["last_step"]
+    assert [e[:4] for e in ev] == [e[:4] for e in ev_ref]
+    # the driven stress history: the global log's lines are the events and increments in order
+    hist = np.array([h[1] for h in sim.history_global]) * 1e9
+    ref = np.array(st["stress_history"][1:])
+    assert hist.shape == ref.shape
+    assert np.abs(hist - ref).max() <= 1e-9 * np.abs(ref).max()
+    lines = (tmp_path / "global_log.txt").read_text().splitlines()
+    assert len(lines) == 1 + len(ref)
+    kinds = [ln.split()[4] for ln in (tmp_path / "summary_log.txt").read_text().splitlines()[2:]
+             if ln[:1].isdigit()]
+    assert kinds.count("ELASTIC") == st["last_step"] - len(ev_ref)
+    assert kinds.count("KMC") + kinds.count("KMC_INSTAB") == len(ev_ref)
+    assert "Total Duration" in (tmp_path / "summary_log.txt").read_text()
+
+
+def test_cascade_eps_target_stop_matches_oracle(gpu, tmp_path):
+    # athermal cascade mode: the batch after which eps_xx >= eps_target ends the run
+    from mgkmc_amd.aqs import ThermalSimulation
+    from oracle.aqs_loop import run_events
+    n, M = 8, 10
+    E, nu = _fields(n)
+    kw = dict(mean=0.5, std=0.15, min_cutoff=0.0)
+    _, st = run_events(n, n, n, M, 0.14, E, nu, 0.7, 0.0, 1e7, 2e-3, (0, 0), 10 ** 9, seed=7, sel_seed=8,
+                       barrier_kwargs=kw, mixed_tol=1e5, ls_tol=1e-8, n_global_steps=6, cascade_mode=True,
+                       softening_cap=0.916, jp=20.0, jt=0.0, eps_target=0.005)
+    sim = ThermalSimulation(n, n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                            temperature=0.0, strain_rate=1e7, cascade_mode=True, softening_cap=0.916, jp=20.0,
+                            jt=0.0, output_dir=str(tmp_path), rng=np.random.RandomState(7),
+                            selection_stream=np.random.RandomState(8))
+    sim.solver_args = {"tol": 1e-8}
+    sim.run_simulation(6, 2e-3, component=(0, 0), mixed_tol=1e5, eps_target=0.005, track_cascades=True,
+                       enable_console_log=False)
+    print(f"[eps_target] cascade flips {len(sim.cascade_history)} (oracle {len(st['cascades'])}), "
+          f"eps_xx {sim.eps_macro[0, 0]:.4g}", flush=True)
+    assert st["cascade_stop"] == "eps_target"
+    assert sim.cascade_history == st["cascades"]
+    assert [tuple(r) for r in sim.flip_event_history] == st["cascades"]  # track_cascades
+    assert abs(sim.eps_macro[0, 0] - st["eps_macro"][0, 0]) <= 1e-12
+    # cascade batches are logged to the summary as "CASCADE" lines at the current global step
+    kinds = [ln.split()[4] for ln in (tmp_path / "summary_log.txt").read_text().splitlines()[2:]
+             if ln[:1].isdigit()]
+    assert kinds.count("CASCADE") == len({(c[0], c[1]) for c in st["cascades"]})
+
+
+def test_save_q_and_cascade_log_header(gpu, tmp_path):
+    from mgkmc_amd.aqs import ThermalSimulation
+    n = 8
+    E, nu = _fields(n)
+    sim = ThermalSimulation(n, n, n, 6, 0.14, E, nu, pixel=0.7, temperature=0.0, strain_rate=1e7,
+                            output_dir=str(tmp_path), rng=np.random.RandomState(3),
+                            selection_stream=np.random.RandomState(4))
+    sim.run_simulation(4, 5e-4, component=(0, 0), enable_save_q=True, save_q_interval=2, enable_console_log=False)
+    names = sorted(p.name for p in tmp_path.iterdir())
+    assert names == ["Q_step_000002.npy", "Q_step_000004.npy", "cascade_log.txt", "global_log.txt",
+                     "summary_log.txt"], names
+    q = np.load(tmp_path / "Q_step_000004.npy")
+    assert q.shape == (n, n, n, 6) and np.array_equal(q, sim.Q)
+    assert (tmp_path / "cascade_log.txt").read_text().split() == ["GlobalStep", "LocalStep", "NumUnstable",
+                                                                  "FlippedVoxels(x,y,z,mode)"]
+    with pytest.raises(NotImplementedError):
+        sim.run_simulation(1, 5e-4, vtk_interval=1, enable_console_log=False)
+
+
+def test_kmc2d_logs_and_flip_log(gpu, tmp_path):
+    # 2D: INIT line, one line per event and increment, flipped_voxels_log with
+    # one line per flip (its softening and barrier at selection)
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    from oracle.kmc2d_loop import run_events_2d
+    n, M = 16, 20
+    rng = np.random.default_rng(2)
+    mu = 26.923e9 * rng.uniform(0.9, 1.1, (n, n))
+    nu = np.full((n, n), 0.35)
+    E = 2 * mu * (1 + nu)
+    kw = dict(mean=2.2, std=0.45, min_cutoff=0.0)
+    args = dict(softening_cap=0.916, jp=20.0, jt=0.0)
+    ev_ref, _ = run_events_2d(n, n, M, 0.14, E, nu, 0.7, 1276.5, 1e7, 1e-2, (0, 0), 30, seed=21, sel_seed=22,
+                              barrier_kwargs=kw, stress_targets={(1, 1): 0.0}, mixed_tol=1e6, **args)
+    sim = KmcSimulation2D(n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                          temperature=1276.5, strain_rate=1e7, output_dir=str(tmp_path),
+                          rng=np.random.RandomState(21), selection_stream=np.random.RandomState(22), **args)
+    ev = sim.run_simulation(10 ** 9, 1e-2, component=(0, 0), stress_targets={(1, 1): 0.0}, mixed_tol=1e6,
+                            enable_kmc_log=True, enable_console_log=False, max_events=30)
+    assert [e[:3] for e in ev] == [e[:3] for e in ev_ref]
+    fl = (tmp_path / "flipped_voxels_log.txt").read_text().splitlines()
+    assert fl[0].split()[:4] == ["Step", "X", "Y", "Mode"]
+    rows = [ln.split() for ln in fl[1:]]
+    assert len(rows) == 30
+    assert [(int(r[1]), int(r[2]), int(r[3])) for r in rows] == [e[:3] for e in ev]
+    assert [int(r[0]) for r in rows] == list(range(1, 31))  # no increment ends inside the first 30 events
+    assert all(r[6] == "kmc" for r in rows) and all(0.0 <= float(r[4]) <= 0.916 for r in rows)
+    sm = [ln.split() for ln in (tmp_path / "summary_log.txt").read_text().splitlines()[2:] if ln[:1].isdigit()]
+    assert sm[0][4] == "INIT" and sum(r[4] in ("KMC", "KMC_INSTAB") for r in sm[1:]) == 30
+    g = (tmp_path / "global_log.txt").read_text().splitlines()
+    assert len(g) == 1 + 31
+    # plane strain: the logged sigma_zz = <nu> (sigma_xx + sigma_yy)
+    f = g[-1].split()
+    assert abs(float(f[11]) - 0.35 * (float(f[9]) + float(f[10]))) <= 2e-3
+
+
+def test_finite_strain_cascade_after_each_increment_matches_oracle(gpu, tmp_path):
+    # finite strain + cascade mode: the cascade after each increment's step
+    # (aqs.py:1089-1101): the history's step labels and the state after the
+    # last increment equal the oracle loop's
+    from mgkmc_amd.aqs import ThermalSimulation
+    from oracle.aqs_loop import run_events
+    n, M = 6, 8
+    E, nu = _fields(n, 4)
+    kw = dict(mean=0.05, std=0.02, min_cutoff=0.0)  # gamma0 0.03: ~100 flips over the three increments
+    common = dict(softening_cap=0.916, jp=20.0, jt=0.0)
+    ev_ref, st = run_events(n, n, n, M, 0.03, E, nu, 0.7, 0.0, 1e7, 4e-3, (0, 0), 10 ** 9, seed=5, sel_seed=6,
+                            barrier_kwargs=kw, n_global_steps=3, cascade_mode=True, strain_assumption="finite_strain",
+                            **common)
+    sim = ThermalSimulation(n, n, n, M, 0.03, E, nu, pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw,
+                            temperature=0.0, strain_rate=1e7, cascade_mode=True, strain_assumption="finite_strain",
+                            solver="newton_cg", output_dir=str(tmp_path), rng=np.random.RandomState(5),
+                            selection_stream=np.random.RandomState(6), **common)
+    sim.run_simulation(3, 4e-3, component=(0, 0), enable_console_log=False)
+    print(f"[fs cascade] flips {len(sim.cascade_history)} (oracle {len(st['cascades'])}); steps "
+          f"{sorted({c[0] for c in st['cascades']})}", flush=True)
+    assert len(st["cascades"]) > 0 and max(c[0] for c in st["cascades"]) == 3
+    assert sim.cascade_history == st["cascades"]
+    F = np.einsum("xyzij->ijxyz", sim.F_field)
+    assert np.abs(F - st["F"]).max() <= 1e-10 * np.abs(st["F"]).max()
+    assert np.abs(sim.F_macro - st["F_macro"]).max() <= 1e-12
