@@ -69,6 +69,22 @@ int mgk_ls_solve(int ndim, const int64_t* shape, int precision, const void* lam,
                  void* sig, int warm, int max_iter, double tol, void* workspace, size_t workspace_bytes,
                  void* stream, mgk_ls_info* info);
 
+/* No-allocation variant of mgk_ls_solve (no counterpart in the reference: for
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as mgk_ls_solve without the workspace: the workspace and the FFT twiddle
+ * tables live in a static device array of MGK_LS_NOALLOC_BYTES per GPU (held until
+ * the process ends; its device address is looked up once per device).  Grids whose
+ * workspace plus twiddles exceed it (for example 3D FP64 above ~100^3, FP32 above
+ * 128^3) and axes longer than 4096 return MGK_ERR_WORKSPACE / MGK_ERR_ARG before any
+ * launch.  Calls no hipMalloc, hipFree, hipHostMalloc or event creation.  All calls
+ * on one GPU share the array: calls that overlap on one GPU (other streams or
+ * threads) race.  Blocks until the solve has finished on `stream`. */
+#define MGK_LS_NOALLOC_BYTES ((size_t)64 << 20)
+int mgk_ls_solve_noalloc(int ndim, const int64_t* shape, int precision, const void* lam, const void* mu,
+                         double lam0, double mu0, const double* eps_bar, const void* eps_plastic, void* eps,
+                         void* sig, int warm, int max_iter, double tol, void* stream, mgk_ls_info* info);
+
 /* FP32 form with FP64 refinement of the same solve (replaces the same reference
  * functions as mgk_ls_solve; the precision split has no reference counterpart).
  * Stage 1: FP32 fixed point on (lam32, mu32, epsp32) from eps32 (warm != 0: eps32 is

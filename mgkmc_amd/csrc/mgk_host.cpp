@@ -24,32 +24,33 @@ bool mgk_make_plan(int n, FftPlan* p) {
     m /= 2;
     ++twos;
   }
-  std::vector<int> r;
+  int r[64];  // radices (a fixed array: planning allocates nothing)
+  int nr = 0;
   while (twos >= 3) {
-    r.push_back(8);
+    r[nr++] = 8;
     twos -= 3;
   }
-  if (twos == 2) r.push_back(4);
+  if (twos == 2) r[nr++] = 4;
   if (twos == 1) {
     // prefer 8*4*4 style splits over a trailing radix-2 when possible
-    if (!r.empty() && r.back() == 8) {
-      r.back() = 4;
-      r.push_back(4);
+    if (nr > 0 && r[nr - 1] == 8) {
+      r[nr - 1] = 4;
+      r[nr++] = 4;
     } else {
-      r.push_back(2);
+      r[nr++] = 2;
     }
   }
-  while (m % 3 == 0) {
-    r.push_back(3);
+  while (m % 3 == 0 && nr < 64) {
+    r[nr++] = 3;
     m /= 3;
   }
-  while (m % 5 == 0) {
-    r.push_back(5);
+  while (m % 5 == 0 && nr < 64) {
+    r[nr++] = 5;
     m /= 5;
   }
-  if (m != 1 || r.size() > 12) return false;
-  for (size_t i = 0; i < r.size(); ++i) p->radix[i] = r[i];
-  p->npass = (int)r.size();
+  if (m != 1 || nr > 12) return false;
+  for (int i = 0; i < nr; ++i) p->radix[i] = r[i];
+  p->npass = nr;
   return true;
 }
 
@@ -64,8 +65,24 @@ struct TwKey {
 };
 std::mutex g_tw_mu;
 std::map<TwKey, void*> g_tw;
-std::map<int, size_t> g_lds;
+constexpr int kLdsDevices = 64;
+size_t g_lds[kLdsDevices];  // per device, 0 = not read yet (a fixed table: no allocation)
 }  // namespace
+
+void mgk_twiddle_values(int n, int prec, void* out) {
+  // exp(-2 pi i t/n) from long double angles, rounded once to the target type
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int t = 0; t < n; ++t) {
+    const long double a = 2.0L * pi * (long double)t / (long double)n;
+    if (prec == 0) {
+      ((double*)out)[2 * t] = (double)cosl(a);
+      ((double*)out)[2 * t + 1] = (double)(-sinl(a));
+    } else {
+      ((float*)out)[2 * t] = (float)cosl(a);
+      ((float*)out)[2 * t + 1] = (float)(-sinl(a));
+    }
+  }
+}
 
 const void* mgk_twiddles(int n, int prec) {
   int dev = 0;
@@ -74,28 +91,12 @@ const void* mgk_twiddles(int n, int prec) {
   TwKey k{dev, n, prec};
   auto it = g_tw.find(k);
   if (it != g_tw.end()) return it->second;
-  // exp(-2 pi i t/n) from long double angles, rounded once to the target type
-  const long double pi = 3.141592653589793238462643383279502884L;
+  const size_t bytes = (size_t)2 * n * (prec == 0 ? sizeof(double) : sizeof(float));
+  std::vector<double> h((bytes + sizeof(double) - 1) / sizeof(double));
+  mgk_twiddle_values(n, prec, h.data());
   void* d = nullptr;
-  if (prec == 0) {
-    std::vector<double> h(2 * n);
-    for (int t = 0; t < n; ++t) {
-      long double a = 2.0L * pi * (long double)t / (long double)n;
-      h[2 * t] = (double)cosl(a);
-      h[2 * t + 1] = (double)(-sinl(a));
-    }
-    if (hipMalloc(&d, h.size() * sizeof(double)) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
-  } else {
-    std::vector<float> h(2 * n);
-    for (int t = 0; t < n; ++t) {
-      long double a = 2.0L * pi * (long double)t / (long double)n;
-      h[2 * t] = (float)cosl(a);
-      h[2 * t + 1] = (float)(-sinl(a));
-    }
-    if (hipMalloc(&d, h.size() * sizeof(float)) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
-  }
+  if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
   g_tw[k] = d;
   return d;
 }
@@ -104,11 +105,10 @@ size_t mgk_lds_limit() {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 65536;
   std::lock_guard<std::mutex> lk(g_tw_mu);
-  auto it = g_lds.find(dev);
-  if (it != g_lds.end()) return it->second;
+  if (dev >= 0 && dev < kLdsDevices && g_lds[dev]) return g_lds[dev];
   int v = 65536;
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
-  g_lds[dev] = (size_t)v;
+  if (dev >= 0 && dev < kLdsDevices) g_lds[dev] = (size_t)v;
   return (size_t)v;
 }
 

@@ -25,6 +25,10 @@ void mgk_set_error(const char* msg, const char* file, int line);
 // prime factor other than 2, 3, 5 or needs more than 12 passes.
 bool mgk_make_plan(int n, FftPlan* p);
 
+// Host values of the twiddle table exp(-2 pi i t/n), t in [0,n): 2n doubles
+// (prec 0) or 2n floats (prec 1) written to out.
+void mgk_twiddle_values(int n, int prec, void* out);
+
 // Device twiddle table exp(-2 pi i t/n), t in [0,n), for the current device.
 // prec 0 = double, 1 = float.  Owned by the library until process exit.
 const void* mgk_twiddles(int n, int prec);
@@ -62,7 +66,7 @@ bool mgk_prof_on();
 // counters after the stream has been synchronised.
 class MgkProfRec {
  public:
-  explicit MgkProfRec(hipStream_t s) : s_(s), on_(mgk_prof_on()) {}
+  explicit MgkProfRec(hipStream_t s, bool allow = true) : s_(s), on_(allow && mgk_prof_on()) {}
   ~MgkProfRec();
   int begin(int kid, int iter, double bytes);
   void end(int h);

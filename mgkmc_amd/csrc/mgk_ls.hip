@@ -38,6 +38,7 @@ constexpr int x_nt() {
   return sizeof(T) == 8 ? 512 : MGK_NT;
 }
 #include <algorithm>
+#include <cstdio>
 
 struct LsState {
   int done;    // set once the solve has stopped
@@ -1090,6 +1091,12 @@ __global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, siz
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = a[i];
 }
 
+// clears n 32-bit words (a fill kernel rather than hipMemsetAsync, so the
+// enqueue sequence stays a plain kernel sequence)
+__global__ void k_zero_words(unsigned int* __restrict__ p, size_t n) {
+  for (size_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
 __global__ void k_widen(const float* __restrict__ a, double* __restrict__ b, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     b[i] = (double)a[i];
@@ -1231,6 +1238,13 @@ void launch_mid(const LsPlan& pl, dim3 gy, hipStream_t stream, const FftPlan& pl
   }
 }
 
+// Resources of a solve that allocates nothing: twiddle tables already in
+// device memory and a host buffer for the state reads (two LsState).
+struct NoAllocCtx {
+  const void* tw[3];
+  LsState* host_state;
+};
+
 // last iteration count per workspace (0 = unknown); set when iters > 0
 int ls_iter_hint(const void* ws, int iters) {
   static std::mutex mu;
@@ -1339,7 +1353,7 @@ template <typename T, int D>
 int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double mu0, const double* eps_bar,
                const T* epsp, T* eps, T* sig, int warm, int max_iter, double tol, char* ws, hipStream_t stream,
                mgk_ls_info* info, bool final_stress = true, const LandauPar* lp = nullptr, T* taubuf = nullptr,
-               const L2Mode* l2 = nullptr, const SecLaw* sl = nullptr) {
+               const L2Mode* l2 = nullptr, const SecLaw* sl = nullptr, const NoAllocCtx* na = nullptr) {
   constexpr int MV = maxv<T>();
   constexpr int TT = ct_tile<T>();  // points per compile-time tile
   constexpr int XT = x_tile<T>();   // points per compile-time x-sweep tile
@@ -1360,9 +1374,15 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   mgk_make_plan(pl.nx, &g.plx);
   mgk_make_plan(pl.P, &g.ply);
   mgk_make_plan(pl.Q, &g.plq);
-  g.twx = (const cplx<T>*)mgk_twiddles(pl.nx, prec);
-  g.twy = (const cplx<T>*)mgk_twiddles(pl.P, prec);
-  g.twq = (const cplx<T>*)mgk_twiddles(pl.Q, prec);
+  if (na) {  // twiddle tables staged in the caller's scratch (nothing allocated here)
+    g.twx = (const cplx<T>*)na->tw[0];
+    g.twy = (const cplx<T>*)na->tw[1];
+    g.twq = (const cplx<T>*)na->tw[2];
+  } else {
+    g.twx = (const cplx<T>*)mgk_twiddles(pl.nx, prec);
+    g.twy = (const cplx<T>*)mgk_twiddles(pl.P, prec);
+    g.twq = (const cplx<T>*)mgk_twiddles(pl.Q, prec);
+  }
   if (!g.twx || !g.twy || !g.twq) {
     mgk_set_error("twiddle table allocation failed", __FILE__, __LINE__);
     return MGK_ERR_HIP;
@@ -1395,7 +1415,7 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   const double e2_extra = anti2 * (double)pl.NV;
   const size_t NV = (size_t)pl.NV;
 
-  MGK_HIP(hipMemsetAsync(st, 0, sizeof(LsState), stream));
+  k_zero_words<<<1, 64, 0, stream>>>((unsigned int*)st, sizeof(LsState) / 4);
   if (l2) k_l2_init<<<1, 1, 0, stream>>>(st, l2->persist && l2->e33_have ? 1 : 0);
   const int nfill = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 4096);
   if (!warm) {
@@ -1422,7 +1442,7 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   const double b_tau = fieldB * ((epsp ? 2.0 * C : (double)C) + 2.0) + specB;
   const double b_fft = 2.0 * specB;
   const double b_upd = specB + 2.0 * C * fieldB;
-  MgkProfRec prof(stream);
+  MgkProfRec prof(stream, na == nullptr);
   int iter_idx = 0;
   const int nel = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 8192);
   auto enqueue_iter = [&]() {
@@ -1533,19 +1553,23 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     ++iter_idx;
   };
 
-  LsState* hst = (LsState*)mgk_pinned(2 * sizeof(LsState));
+  LsState* hst = na ? na->host_state : (LsState*)mgk_pinned(2 * sizeof(LsState));
   if (!hst) {
     mgk_set_error("pinned host allocation failed", __FILE__, __LINE__);
     return MGK_ERR_HIP;
   }
-  hipEvent_t ev[2];
-  MGK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  MGK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  if (!na) {
+    MGK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    MGK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  }
   // Iterations are enqueued in chunks; the stop flag of chunk i is read while
   // chunk i+1 runs.  The first chunk is the iteration count of the previous
   // solve on this workspace (solves in a KMC loop converge alike), so at most
-  // one chunk of early-exit launches trails the converged iteration.
-  const int hint = ls_iter_hint(ws, 0);
+  // one chunk of early-exit launches trails the converged iteration.  The
+  // no-allocation form keeps no per-workspace history and reads the flag
+  // after a stream synchronisation per chunk instead of events.
+  const int hint = na ? 0 : ls_iter_hint(ws, 0);
   int enq = 0, chunk = hint > 0 ? hint : 2, slot = 0;
   bool pending = false, stop = false;
   bool first = true;
@@ -1555,13 +1579,18 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     enq += k;
     MGK_HIP(hipGetLastError());
     MGK_HIP(hipMemcpyAsync(&hst[slot], st, sizeof(LsState), hipMemcpyDeviceToHost, stream));
-    MGK_HIP(hipEventRecord(ev[slot], stream));
-    if (pending) {
-      MGK_HIP(hipEventSynchronize(ev[slot ^ 1]));
-      if (hst[slot ^ 1].done) stop = true;
+    if (na) {
+      MGK_HIP(hipStreamSynchronize(stream));
+      if (hst[slot].done) stop = true;
+    } else {
+      MGK_HIP(hipEventRecord(ev[slot], stream));
+      if (pending) {
+        MGK_HIP(hipEventSynchronize(ev[slot ^ 1]));
+        if (hst[slot ^ 1].done) stop = true;
+      }
+      pending = true;
+      slot ^= 1;
     }
-    pending = true;
-    slot ^= 1;
     chunk = first ? 1 : std::min(chunk * 2, 16);
     first = false;
   }
@@ -1597,9 +1626,11 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   MGK_HIP(hipMemcpyAsync(&hst[0], st, sizeof(LsState), hipMemcpyDeviceToHost, stream));
   MGK_HIP(hipStreamSynchronize(stream));
   prof.commit(hst[0].iters);
-  ls_iter_hint(ws, hst[0].iters);
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
+  if (!na) {
+    ls_iter_hint(ws, hst[0].iters);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+  }
   const LsState& s = hst[0];
   if (info) {
     info->iters = s.iters;
@@ -1677,6 +1708,117 @@ extern "C" int mgk_ls_solve(int ndim, const int64_t* shape, int precision, const
                                 (const float*)eps_plastic, (float*)eps, (float*)sig, warm, max_iter, tol, ws, s, info);
   return ls_solve_t<float, 2>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar, (const float*)eps_plastic,
                               (float*)eps, (float*)sig, warm, max_iter, tol, ws, s, info);
+}
+
+// ---------------------------------------------------------------------------
+// No-allocation form of mgk_ls_solve.  The workspace and the FFT twiddle
+// tables live in `scratch`, a file-scope device array of a fixed size that
+// each GPU holds for the life of the process.  Its device address is looked
+// up once per device (hipGetSymbolAddress with that device current) and only
+// that address is handed to kernels and copies: the array's name is a
+// host-side stand-in, not a device pointer.  Every call on one GPU shares its
+// array, so calls that overlap on one GPU (two streams, two threads) race;
+// the solve state is cleared by a kernel at the start of every call.
+// ---------------------------------------------------------------------------
+#define MGK_LS_NOALLOC_BYTES ((size_t)64 << 20)
+static __device__ __align__(256) unsigned char scratch[MGK_LS_NOALLOC_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+constexpr int kMaxAxis = 4096;  // longest axis the compile-time and generic FFT passes take
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+
+// device address of this device's scratch array (looked up on first use)
+int scratch_address(void** out) {
+  int dev = 0;
+  MGK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    mgk_set_error("device index out of range for the no-allocation scratch", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    MGK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      mgk_set_error("hipGetSymbolAddress returned no address for the scratch array", __FILE__, __LINE__);
+      return MGK_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MGK_OK;
+}
+
+// per-thread host staging: three twiddle tables (doubles cover both
+// precisions) and the two solve-state reads
+thread_local double g_na_tw[3][2 * kMaxAxis];
+thread_local LsState g_na_state[2];
+}  // namespace
+
+extern "C" int mgk_ls_solve_noalloc(int ndim, const int64_t* shape, int precision, const void* lam,
+                                    const void* mu, double lam0, double mu0, const double* eps_bar,
+                                    const void* eps_plastic, void* eps, void* sig, int warm, int max_iter,
+                                    double tol, void* stream, mgk_ls_info* info) {
+  if (!lam || !mu || !eps || !eps_bar || !shape || max_iter < 1 || (precision != 0 && precision != 1)) {
+    mgk_set_error("null argument, max_iter < 1 or precision not 0/1", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  LsPlan pl;
+  int rc = precision == 0 ? ls_plan<double>(ndim, shape, &pl) : ls_plan<float>(ndim, shape, &pl);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the LS solver", __FILE__, __LINE__);
+    return rc;
+  }
+  const int axis[3] = {pl.nx, pl.P, pl.Q};
+  const size_t cb = precision == 0 ? sizeof(cplx<double>) : sizeof(cplx<float>);
+  size_t off = 0, tw_off[3];
+  for (int a = 0; a < 3; ++a) {
+    if (axis[a] > kMaxAxis) {
+      mgk_set_error("axis longer than the no-allocation path takes (4096)", __FILE__, __LINE__);
+      return MGK_ERR_ARG;
+    }
+    tw_off[a] = off;
+    off += mgk_align((size_t)axis[a] * cb);
+  }
+  const size_t need = off + pl.total;
+  if (need > MGK_LS_NOALLOC_BYTES) {
+    char msg[160];
+    snprintf(msg, sizeof(msg), "grid needs %zu bytes of scratch; the no-allocation path holds %zu", need,
+             (size_t)MGK_LS_NOALLOC_BYTES);
+    mgk_set_error(msg, __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  char* base = nullptr;
+  rc = scratch_address((void**)&base);
+  if (rc != MGK_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  NoAllocCtx na;
+  for (int a = 0; a < 3; ++a) {
+    mgk_twiddle_values(axis[a], precision, g_na_tw[a]);
+    MGK_HIP(hipMemcpyAsync(base + tw_off[a], g_na_tw[a], (size_t)axis[a] * cb, hipMemcpyHostToDevice, s));
+    na.tw[a] = base + tw_off[a];
+  }
+  na.host_state = g_na_state;
+  char* ws = base + off;
+  warm = warm ? 1 : 0;
+  if (precision == 0) {
+    if (ndim == 3)
+      return ls_solve_t<double, 3>(pl, (const double*)lam, (const double*)mu, lam0, mu0, eps_bar,
+                                   (const double*)eps_plastic, (double*)eps, (double*)sig, warm, max_iter, tol, ws, s,
+                                   info, true, nullptr, nullptr, nullptr, nullptr, &na);
+    return ls_solve_t<double, 2>(pl, (const double*)lam, (const double*)mu, lam0, mu0, eps_bar,
+                                 (const double*)eps_plastic, (double*)eps, (double*)sig, warm, max_iter, tol, ws, s,
+                                 info, true, nullptr, nullptr, nullptr, nullptr, &na);
+  }
+  if (ndim == 3)
+    return ls_solve_t<float, 3>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar,
+                                (const float*)eps_plastic, (float*)eps, (float*)sig, warm, max_iter, tol, ws, s, info,
+                                true, nullptr, nullptr, nullptr, nullptr, &na);
+  return ls_solve_t<float, 2>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar, (const float*)eps_plastic,
+                              (float*)eps, (float*)sig, warm, max_iter, tol, ws, s, info, true, nullptr, nullptr,
+                              nullptr, nullptr, &na);
 }
 
 extern "C" int mgk_ls_solve_refined(int ndim, const int64_t* shape, const void* lam32, const void* mu32,
