@@ -139,7 +139,7 @@ __device__ T fs_plane_e33(const T* E, T lam2, T mu, const FsMat& m) {
 // stress P and the tangent coefficients at deformation F; PS = false compiles
 // the plane-stress landau / murnaghan branch out (the 3D matvec instantiation)
 template <typename T, bool PS = true>
-__device__ void fs_state(const T* F, const T* Fpi, T lam, T mu, const FsMat& m, FsVox<T>& s, T* P) {
+__device__ __forceinline__ void fs_state(const T* F, const T* Fpi, T lam, T mu, const FsMat& m, FsVox<T>& s, T* P) {
   s.lam = lam;
   s.mu = mu;
   if (Fpi) {
@@ -340,7 +340,7 @@ __device__ __forceinline__ void fs_dS(const FsVox<T>& s, const FsMat& m, const T
 
 // dP for direction dF at the state s (PS as in fs_state)
 template <typename T, bool PS = true>
-__device__ void fs_tangent(const FsVox<T>& s, const FsMat& m, const T* dF, bool hasFp, T* dP) {
+__device__ __forceinline__ void fs_tangent(const FsVox<T>& s, const FsMat& m, const T* dF, bool hasFp, T* dP) {
   T dFe[9];
   if (hasFp) mm3(dF, s.Fpi, dFe);
   else {
@@ -545,6 +545,70 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_dP(size_t NV, const T* __restrict
 #pragma unroll
     for (int k = 0; k < 9; ++k) dP[(size_t)k * NV + v] = o[k];
   }
+}
+
+// The tangent action specialised on the law (MODEL, capping CAP: 0 off,
+// 1 piecewise, 2 smooth) and on F_p (FP): the runtime material switch of
+// k_fs_dP keeps every law's registers live (FP64: 256 VGPRs, one wave per
+// SIMD); with the law fixed at compile time each instantiation holds only its
+// own.  Same arithmetic as k_fs_dP.
+template <typename T, bool PS, int MODEL, int CAP, bool FP>
+__global__ __launch_bounds__(MGK_NT) void k_fs_dP_law(size_t NV, const T* __restrict__ F, const T* __restrict__ Fpi,
+                                                      const T* __restrict__ lam, const T* __restrict__ mu, FsMat m0,
+                                                      const T* __restrict__ dX, Nine dXu, T* __restrict__ dP,
+                                                      const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  FsMat m = m0;
+  m.model = MODEL;
+  m.capping = CAP != 0;
+  m.cap_smooth = CAP == 2;
+  if (!PS) m.plane = m0.plane == 2 ? 1 : m0.plane;  // only the plane-stress branch reads plane == 2
+  for (size_t v = blockIdx.x * (size_t)MGK_NT + threadIdx.x; v < NV; v += (size_t)gridDim.x * MGK_NT) {
+    T f[9], fpi[9], d[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      f[k] = F[(size_t)k * NV + v];
+      if (FP) fpi[k] = Fpi[(size_t)k * NV + v];
+      d[k] = dX ? dX[(size_t)k * NV + v] : (T)dXu.v[k];
+    }
+    FsVox<T> s;
+    fs_state<T, PS>(f, FP ? fpi : (const T*)nullptr, lam[v], mu[v], m, s, (T*)nullptr);
+    T o[9];
+    fs_tangent<T, PS>(s, m, d, FP, o);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dP[(size_t)k * NV + v] = o[k];
+  }
+}
+
+// launches the tangent action dP = K : dX (dX == NULL: the uniform direction dXu)
+template <typename T>
+void launch_dP(int nblk, hipStream_t st, size_t NV, const T* F, const T* Fpi, const T* lam, const T* mu,
+               const FsMat& m, const T* dX, const Nine& dXu, T* dP, const int* skip) {
+  const bool ps = fs_ps(m);
+  const int cap = m.model == 1 && m.capping ? (m.cap_smooth ? 2 : 1) : 0;
+#define MGK_DP(PSV, MD, CP)                                                                                    \
+  do {                                                                                                         \
+    if (Fpi)                                                                                                   \
+      k_fs_dP_law<T, PSV, MD, CP, true><<<nblk, MGK_NT, 0, st>>>(NV, F, Fpi, lam, mu, m, dX, dXu, dP, skip);   \
+    else                                                                                                       \
+      k_fs_dP_law<T, PSV, MD, CP, false><<<nblk, MGK_NT, 0, st>>>(NV, F, Fpi, lam, mu, m, dX, dXu, dP, skip);  \
+  } while (0)
+  if (ps) {
+    if (m.model == 1) MGK_DP(true, 1, 0);
+    else MGK_DP(true, 2, 0);
+    return;
+  }
+  switch (m.model) {
+    case 0: MGK_DP(false, 0, 0); break;
+    case 1:
+      if (cap == 0) MGK_DP(false, 1, 0);
+      else if (cap == 1) MGK_DP(false, 1, 1);
+      else MGK_DP(false, 1, 2);
+      break;
+    case 2: MGK_DP(false, 2, 0); break;
+    default: MGK_DP(false, 3, 0); break;
+  }
+#undef MGK_DP
 }
 
 // per-voxel inverse of Fp
@@ -1654,12 +1718,7 @@ void fs_matvec(FsCtx<T>& c, const T* Fcur, const T* in, T* out, const T* dota, i
   const FsPlan& pl = c.pl;
   Nine z{};
   const int h = c.prof->begin(KID_FS_CONST, c.kry_it, (double)pl.NV * sizeof(T) * (27 + 2 + (c.Fpi ? 9 : 0)));
-  if (fs_ps(c.mat))
-    k_fs_dP<T, true><<<fs_nblk<T>(pl.NV), MGK_NT, 0, c.s>>>((size_t)pl.NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, in, z,
-                                                           c.dP, skip);
-  else
-    k_fs_dP<T, false><<<fs_nblk<T>(pl.NV), MGK_NT, 0, c.s>>>((size_t)pl.NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, in, z,
-                                                            c.dP, skip);
+  launch_dP<T>(fs_nblk<T>(pl.NV), c.s, (size_t)pl.NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, in, z, c.dP, skip);
   c.prof->end(h);
   fs_project(c, c.dP, out, T(1.0 / (double)pl.NV), dota, selfdot, partials, skip);
 }
@@ -1993,10 +2052,7 @@ void fs_db_matvec(FsCtx<T>& c, const DbPar& d, const T* Fcur, const T* in, T* ou
   const double inv = 1.0 / (double)NV;
   fs_db_apply(c, d, DB_GRAD, 1, inv, in, c.Ftr, (const T*)nullptr, 0, (double*)nullptr, skip);
   Nine z{};
-  if (fs_ps(c.mat))
-    k_fs_dP<T, true><<<fs_nblk<T>(c.pl.NV), MGK_NT, 0, c.s>>>(NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, c.Ftr, z, c.dP, skip);
-  else
-    k_fs_dP<T, false><<<fs_nblk<T>(c.pl.NV), MGK_NT, 0, c.s>>>(NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, c.Ftr, z, c.dP, skip);
+  launch_dP<T>(fs_nblk<T>(c.pl.NV), c.s, NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, c.Ftr, z, c.dP, skip);
   fs_db_apply(c, d, DB_DIV, 1, inv, c.dP, out, dota, selfdot, partials, skip);
 }
 
@@ -2273,12 +2329,7 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
       info->newton_iters += 1;
       // rhs
       if (i_NW == 0) {
-        if (fs_ps(c.mat))
-          k_fs_dP<T, true><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, F, c.Fpi, c.lam, c.mu, c.mat, nullptr, DbarF,
-                                                                c.dP, nullptr);
-        else
-          k_fs_dP<T, false><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, F, c.Fpi, c.lam, c.mu, c.mat, nullptr, DbarF,
-                                                                 c.dP, nullptr);
+        launch_dP<T>(fs_nblk<T>(pl.NV), s, NV, F, c.Fpi, c.lam, c.mu, c.mat, (const T*)nullptr, DbarF, c.dP, nullptr);
         fs_project(c, c.dP, c.b, T(-1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
       } else {
         fs_project(c, c.P, c.b, T(-1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
