@@ -245,6 +245,7 @@ class ThermalSimulation(DeviceKmc):
                 if not converged:
                     print(f"Warning: Mixed loop did not converge at step {step} (Err={max_err:.2e})")
                 c["elastic"] += 1
+                self.increment_history.append((self.eps_macro.copy(), self.sig_mean().copy()))
                 val = do_log(step, "elastic", casc_count, 0)
                 drop.after_increment(step, val)
                 if cp_int is not None and cp_int not in ("none", "last"):

@@ -350,6 +350,7 @@ class DeviceKmc:
         self.flip_event_history = []  # tracked cascade flips (global step, batch, x, y, z, m), as the reference's
         self.cascade_history = []  # (global step, batch, x, y[, z], m) per cascade flip
         self.history_global = []  # (eps_xx, sig_xx / 1e9) per logged step (mgkmc/aqs.py:760)
+        self.increment_history = []  # (eps_macro, <sigma>) after each elastic increment (extension)
         self._flip_log = None  # 2D flipped_voxels_log writer during a run
         self.max_cascade_steps_pct = 0.3
         self.ls_info = None

@@ -210,6 +210,7 @@ class KmcSimulation2D(DeviceKmc):
                 else:
                     self.mixed_bc(stress_targets, mixed_tol, mixed_max_iter)
                 c["elastic"] += 1
+                self.increment_history.append((self.eps_macro.copy(), self.sig_mean().copy()))
                 do_log(c["step"], "ELAST", c["casc"], 0)
                 c["step"] += 1
                 self._flip_log_step = c["step"]

@@ -85,6 +85,27 @@ def heat_step(Tl, k2, Dsc, dt, temperature, thermostat, tau_bath, temperature_ca
         Tl[...] = np.clip(Tl, a_min=None, a_max=temperature_cap)
 
 
+class _PlanarCatalog:
+    """Planar catalogs [[gxx, gxy, 0], [gxy, -gxx, 0], [0, 0, 0]] stored as
+    their two parameters (..., M, 2) with the indexing of the full
+    (..., M, 3, 3) array the loop uses (fast mode: 128^3 x 20 modes would
+    take 3 GB as full tensors)."""
+
+    def __init__(self, shape, M):
+        self.p = np.zeros(tuple(shape) + (M, 2))
+
+    def __getitem__(self, k):
+        g = self.p[k]
+        out = np.zeros(g.shape[:-1] + (3, 3))
+        out[..., 0, 0], out[..., 1, 1] = g[..., 0], -g[..., 0]
+        out[..., 0, 1] = out[..., 1, 0] = g[..., 1]
+        return out
+
+    def __setitem__(self, k, v):
+        v = np.asarray(v)
+        self.p[k] = np.stack([v[..., 0, 0], v[..., 0, 1]], axis=-1)
+
+
 def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, step_size, component,
                n_events, seed, sel_seed, barrier_kwargs=None, softening_scheme="isotropic", softening_cap=2.0,
                jp=10.0, jt=30.0, jn=0.0, nu0=1e13, q_act_temp=0.37, ls_tol=1e-6, stress_targets=None,
@@ -95,7 +116,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                tau_bath=0.0, solver="newton_cg", use_3d_barriers=False, redraw_directions=True,
                redraw_barriers=True, barrier_generator="gaussian", stop_on_stress_drop=None,
                stress_drop_component=None, stop_post_drop_steps=20, ignore_drop_steps=0, stress_drop_lookback=1,
-               eps_target=None):
+               eps_target=None, fast=False, progress=None):
     """Returns the list of events (x, y, z, m, time, is_instab) and final state.
     ``solver``: the finite-strain step's solver ("newton_cg", "al", "dbfft").
     Stress-drop stop (aqs.py:834, :1070, :1149-1181): the driven stress
@@ -105,7 +126,17 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
     stress_drop_lookback back above stop_on_stress_drop arms the stop, and the
     run ends after stop_post_drop_steps further increments (state
     "drop_step", "stress_history").  eps_target (aqs.py:660-666): a cascade
-    batch after which eps_macro[component] >= eps_target ends the run."""
+    batch after which eps_macro[component] >= eps_target ends the run.
+
+    ``fast`` (large grids; small strain, KMC mode, isotropic softening, planar
+    catalogs, gaussian barriers): the LS solves run matrix-free
+    (ol.spectral_solver_3d_mf, equal to the materialised solver to round-off
+    with the same iteration counts), the barriers from the two planar
+    parameters (ok.compute_barrier_planar, bit-identical), and the initial
+    catalog is drawn in one vectorised call that consumes the stream like the
+    per-voxel loop; the event sequence is the standard mode's
+    (tests/test_oracle_kmc.py).  ``progress(n_events)`` is called after each
+    event."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
     bk = dict(mean=2.0, std=0.6, min_cutoff=0.1) if barrier_kwargs is None else dict(barrier_kwargs)
@@ -139,11 +170,19 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
             Q0[x, y, z, m] = gen((1,))[0]
 
     Q0 = gen((nx, ny, nz, M))
-    cat = np.zeros((nx, ny, nz, M, 3, 3))
-    for x in range(nx):
-        for y in range(ny):
-            for z in range(nz):
-                cat[x, y, z] = catalog(M)
+    if fast:
+        if (strain_assumption != "small_strain" or cascade_mode or enable_thermal or use_3d_barriers
+                or softening_scheme != "isotropic" or barrier_generator != "gaussian"):
+            raise ValueError("fast mode covers small-strain KMC runs with isotropic softening and planar catalogs")
+        cat = _PlanarCatalog((nx, ny, nz), M)
+        # one call: the legacy stream's normals in the per-mode order (gxx, gxy) of stz_catalog_glass
+        cat.p[...] = 0.5 * gamma0 * np.random.normal(size=nx * ny * nz * M * 2).reshape(nx, ny, nz, M, 2)
+    else:
+        cat = np.zeros((nx, ny, nz, M, 3, 3))
+        for x in range(nx):
+            for y in range(ny):
+                for z in range(nz):
+                    cat[x, y, z] = catalog(M)
     Q = np.zeros((nx, ny, nz, M))
     soft = np.zeros((nx, ny, nz, 4))
     tl = np.full((nx, ny, nz), -np.inf)
@@ -181,6 +220,11 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
         state["F_macro"] = np.eye(3)
 
     def update():
+        if fast:
+            e, s, em, sm = ol.spectral_solver_3d_mf(E, nu, eps_macro, eps_plastic=ep, pixel=pixel, tol=ls_tol)
+            state["sig"] = s
+            state["sm"] = sm
+            return
         if not finite:
             e, s, em, sm = ol.update_stress_fft_full(ep, eps_macro, E, nu, pixel=pixel, tol=ls_tol)
             state["sig"] = s
@@ -323,7 +367,11 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
     while done < n_global_steps:
         remaining += dt_step
         while remaining > 0:
-            ok.compute_barrier(Q, Q0, state["sig"], cat, volume, soft, tl, t, prev, softening_cap, scheme, tau)
+            if fast:
+                ok.compute_barrier_planar(Q, Q0, state["sig"], cat.p[..., 0], cat.p[..., 1], volume, soft, tl, t,
+                                          softening_cap, tau)
+            else:
+                ok.compute_barrier(Q, Q0, state["sig"], cat, volume, soft, tl, t, prev, softening_cap, scheme, tau)
             if cascade_mode and len(ok.find_unstable(Q, stability_threshold)) > 0:
                 if run_cascade(step):
                     return result()
@@ -351,6 +399,8 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                         update()
                     events.append((int(x), int(y), int(z), int(m), t, bool(inst)))
                     hist.append(state["sm"][sdc])
+                    if progress is not None:
+                        progress(len(events))
                     if len(events) >= n_events:
                         return result()
                     step += 1

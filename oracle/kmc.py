@@ -98,6 +98,37 @@ def _barrier(Q, Q0, sigma, catalog, volume, soft_prop, last_event_time, current_
     Q[...] = Q0 * np.exp(-g_eff) - w_val
 
 
+def compute_barrier_planar(Q_field, Q0_field, sigma_field, gxx, gxy, volume, soft_prop, last_event_time,
+                           current_time, softening_cap, tau=np.inf):
+    """compute_barrier (stz/barriers.py:4-100) for the isotropic scheme and
+    planar catalogs [[gxx, gxy, 0], [gxy, -gxx, 0], [0, 0, 0]] held as their
+    two parameters (..., M): the work sum keeps the reference's (i, j) order
+    over the nonzero entries (the zero entries add exact zeros), so Q is the
+    same, bit for bit, as compute_barrier on the full catalog
+    (tests/test_oracle_kmc.py), without the (..., M, 3, 3) array."""
+    g_t = soft_prop[..., 1]
+    t_last = last_event_time
+    if tau == np.inf:
+        g_tc = np.where(g_t > 0, g_t, 0.0)
+    else:
+        dt = current_time - t_last
+        dt = np.where(dt < 0, 0.0, dt)
+        with np.errstate(invalid="ignore", over="ignore"):
+            dec = g_t * np.exp(-dt / tau)
+        g_tc = np.where(g_t > 0, np.where(t_last == -np.inf, 0.0, dec), 0.0)
+    g_base = soft_prop[..., 0] + g_tc
+    if softening_cap > 0:
+        g_base = np.where(g_base > softening_cap, softening_cap, g_base)
+    s = sigma_field
+    w = s[..., None, 0, 0] * gxx
+    w += s[..., None, 0, 1] * gxy
+    w += s[..., None, 1, 0] * gxy
+    w += s[..., None, 1, 1] * (-gxx)
+    w_val = 0.5 * volume * (w / 1e9) * GPA_NM3_TO_EV
+    np.multiply(Q0_field, np.exp(-(1.0 * g_base[..., None])), out=Q_field)
+    Q_field -= w_val
+
+
 def compute_barrier(Q_field, Q0_field, sigma_field, catalog, volume, soft_prop, last_event_time,
                     current_time, prev_strain_dir, softening_cap, softening_scheme_idx=0, tau=np.inf):
     """3D, in place on Q_field (stz/barriers.py:4-100)."""

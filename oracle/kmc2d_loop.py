@@ -68,7 +68,8 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                   thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
                   tau_bath=0.0, strain_assumption="small_strain", stz_mode="pure_shear", redraw_directions=True,
                   redraw_barriers=True, solver="newton_cg"):
-    """Returns (events [(x, y, m, time, is_instab)], final state dict)."""
+    """Returns (events [(x, y, m, time, is_instab)], final state dict; its
+    "increments" holds (eps_macro, <sigma>) after each elastic increment)."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
     bk = dict(mean=2.0, std=0.6, min_cutoff=0.1) if barrier_kwargs is None else dict(barrier_kwargs)
@@ -309,12 +310,14 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
     step = 1
     dt_step = abs(step_size) / strain_rate if strain_rate > 0 else 1.0
     events = []
+    increments = []  # (eps_macro, <sigma>) after each elastic increment
     remaining = 0.0
     done = 0
     E_avg, nu_avg = E.mean(), nu.mean()
 
     def result():
         return events, dict(eps_plastic=ep, Q0=Q0, soft=soft, sig=state["sig"], eps_macro=eps_macro.copy(), time=t,
+                            increments=increments,
                             substeps=state.get("substeps"), cascades=cascades, Tlocal=Tl, F=state.get("F"),
                             F_plastic=state.get("Fp"), F_macro=state.get("F_macro"))
 
@@ -390,6 +393,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     prev_e[k] = eps_macro[k]
                     prev_s[k] = sm[k]
                     eps_macro[k] += st
+        increments.append((eps_macro.copy(), state["sig"].mean(axis=(0, 1))))
         done += 1
         step += 1
     return result()

@@ -209,6 +209,108 @@ def update_stress_fft_full(eps_plastic_field, eps_macro, E, nu, pixel=1.0,
                               max_iter=max_iter, tol=tol, verbose=verbose, pixel=pixel)
 
 
+_SYM3 = ((0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1))
+
+
+def _gamma_sym3(q, t, A2, Bc):
+    """-Gamma0(q):t for symmetric t given as its 6 components (xx, yy, zz, yz,
+    xz, xy) on a spectrum: (q_h (t q)_k + q_k (t q)_h) / (2 mu0 |q|^2) -
+    B q_k q_h (q.t.q) / |q|^4 (elasticity.py:70-105 contracted with a
+    symmetric tau), zero at q = 0."""
+    k2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2]
+    zero = k2 == 0
+    k2 = np.where(zero, 1.0, k2)
+    a = A2 / k2
+    bb = Bc / (k2 * k2)
+    T = {}
+    for c, (i, j) in enumerate(_SYM3):
+        T[(i, j)] = T[(j, i)] = t[c]
+    u = [sum(T[(k, j)] * q[j] for j in range(3)) for k in range(3)]
+    qtq = (q[0] * u[0] + q[1] * u[1] + q[2] * u[2]) * bb
+    out = []
+    for k, h in _SYM3:
+        r = -(a * (q[h] * u[k] + q[k] * u[h]) - q[k] * q[h] * qtq)
+        out.append(np.where(zero, 0.0, r))
+    return out
+
+
+def spectral_solver_3d_mf(E, nu, eps_bar, eps_plastic=None, max_iter=200, tol=1e-6, pixel=1.0, info=None):
+    """Matrix-free restatement of spectral_solver_3d (linear_elastic_simulator.py:
+    274-376) for large grids: the same iteration (sigma = C:(eps - eps_p),
+    sigma0 = C0:eps, tau = sigma - sigma0, eps~ = -F^-1[Gamma0:F[tau]] minus its
+    mean, eps_new = eps_bar + eps~, diff = |eps_new - eps| / (|eps| + 1e-20),
+    cold start) on the 6 symmetric components, with Gamma0 applied from q on
+    real-to-complex spectra.  The reference keeps .real of a full-spectrum
+    inverse, which for a real tau is the inverse of the Hermitian part
+    1/2 (Gamma0(q) + Gamma0(R q)) tau_hat (R negates the Nyquist components of
+    q); that average is applied here.  Equal to spectral_solver_3d to
+    round-off with the same iteration count (tests/test_oracle_ls.py);
+    materialises no Gamma0 and no (d, d) spectra.  Inputs and outputs in the
+    reference's (nx, ny, nz, 3, 3) layout."""
+    E = np.asarray(E, float)
+    nu = np.asarray(nu, float)
+    shape = E.shape
+    lam, mu = compute_lame_3d(E, nu)
+    lam0, mu0 = compute_lame_3d(E.mean(), nu.mean())
+    A2 = 1.0 / (2.0 * mu0)
+    Bc = (lam0 + mu0) / (mu0 * (lam0 + 2.0 * mu0))
+    # the half axis keeps numpy's fftfreq sign convention (-1/2 at an even
+    # length's Nyquist bin), which the Nyquist average pairs with R q
+    qs = [np.fft.fftfreq(n) for n in shape[:2]] + [np.fft.fftfreq(shape[2])[: shape[2] // 2 + 1]]
+    q = np.meshgrid(*qs, indexing="ij", sparse=True)
+    nyq = []
+    for a, n in enumerate(shape):
+        v = (np.arange(len(qs[a])) == n // 2) & (n % 2 == 0)
+        sh = [1, 1, 1]
+        sh[a] = len(qs[a])
+        nyq.append(v.reshape(sh))
+    qR = [np.where(nyq[a], -q[a], q[a]) for a in range(3)]
+    eb = [float(eps_bar[i][j]) if i == j else 0.5 * (eps_bar[i][j] + eps_bar[j][i]) for i, j in _SYM3]
+    anti = np.asarray(eps_bar, float) - np.asarray(eps_bar, float).T
+    e = [np.full(shape, v) for v in eb]
+    ep = [np.zeros(shape)] * 6 if eps_plastic is None else \
+        [0.5 * (eps_plastic[..., i, j] + eps_plastic[..., j, i]) for i, j in _SYM3]
+    w = [1.0, 1.0, 1.0, 2.0, 2.0, 2.0]
+    ax = (0, 1, 2)
+    it_done, diff = 0, np.inf
+    for it in range(max_iter):
+        d = [e[c] - ep[c] for c in range(6)]
+        tr = d[0] + d[1] + d[2]
+        tr0 = e[0] + e[1] + e[2]
+        tau = [(2 * mu * d[c] + (lam * tr if c < 3 else 0.0)) - (2 * mu0 * e[c] + (lam0 * tr0 if c < 3 else 0.0))
+               for c in range(6)]
+        th = [_sfft.rfftn(t, axes=ax, workers=FFT_WORKERS) for t in tau]
+        o1 = _gamma_sym3(q, th, A2, Bc)
+        o2 = _gamma_sym3(qR, th, A2, Bc)
+        new = []
+        for c in range(6):
+            et = _sfft.irfftn(0.5 * (o1[c] + o2[c]), s=shape, axes=ax, workers=FFT_WORKERS)
+            et -= et.mean()
+            new.append(eb[c] + et)
+        num = sum(w[c] * np.sum((new[c] - e[c]) ** 2) for c in range(6))
+        den = sum(w[c] * np.sum(e[c] ** 2) for c in range(6)) + 0.25 * np.sum(anti ** 2) * E.size
+        diff = np.sqrt(num) / (np.sqrt(den) + 1e-20)
+        e = new
+        it_done = it + 1
+        if diff < tol:
+            break
+    d = [e[c] - ep[c] for c in range(6)]
+    tr = d[0] + d[1] + d[2]
+    sig = [2 * mu * d[c] + (lam * tr if c < 3 else 0.0) for c in range(6)]
+    eps = np.zeros(shape + (3, 3))
+    sg = np.zeros(shape + (3, 3))
+    for c, (i, j) in enumerate(_SYM3):
+        eps[..., i, j] = e[c]
+        sg[..., i, j] = sig[c]
+        if i != j:
+            eps[..., j, i] = e[c]
+            sg[..., j, i] = sig[c]
+    eps = eps + 0.5 * anti
+    if info is not None:
+        info.update(iters=it_done, diff=diff)
+    return eps, sg, eps.mean(axis=ax), sg.mean(axis=ax)
+
+
 def equilibrium_residual(sig, pixel=1.0):
     """FP64 equilibrium residual of a stress field (nx,..,d,d):
     sqrt(sum_k |q_j sig_hat_ij|^2 / |q|^2) / sqrt(sum_k |sig_hat|^2), k != 0 and

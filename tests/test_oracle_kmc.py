@@ -331,3 +331,49 @@ def test_barrier_generators_follow_their_distributions():
     assert abs(small.mean() - 0.8 * (1 - np.exp(-2.5))) < 0.01
     r = ok.rayleigh_barriers(200000, rs, loc=1.0, scale=0.5, min_cutoff=None)
     assert abs(r.mean() - (1.0 + 0.5 * np.sqrt(np.pi / 2))) < 0.01 and r.min() >= 1.0
+
+
+def test_planar_barrier_equals_full_catalog_barrier():
+    # compute_barrier_planar: the same Q, bit for bit, as compute_barrier on
+    # the full planar catalog (isotropic scheme, with and without decay)
+    import numpy as np
+    from oracle import kmc as ok
+    rng = np.random.default_rng(1)
+    sh, M = (6, 5, 4), 7
+    Q0 = rng.normal(2.2, 0.45, sh + (M,))
+    gx, gy = rng.standard_normal(sh + (M,)) * 0.07, rng.standard_normal(sh + (M,)) * 0.07
+    cat = np.zeros(sh + (M, 3, 3))
+    cat[..., 0, 0], cat[..., 1, 1], cat[..., 0, 1], cat[..., 1, 0] = gx, -gx, gy, gy
+    g = rng.standard_normal(sh + (3, 3)) * 5e8
+    sig = 0.5 * (g + np.swapaxes(g, -1, -2))
+    soft = np.abs(rng.standard_normal(sh + (4,))) * 0.3
+    tl = rng.uniform(0, 1e-6, sh)
+    tl.reshape(-1)[::3] = -np.inf
+    for tau in (np.inf, 3e-7):
+        q1, q2 = np.zeros(sh + (M,)), np.zeros(sh + (M,))
+        ok.compute_barrier(q1, Q0, sig, cat, 0.343, soft, tl, 5e-7, np.zeros(sh + (3, 3)), 0.916, 0, tau)
+        ok.compute_barrier_planar(q2, Q0, sig, gx, gy, 0.343, soft, tl, 5e-7, 0.916, tau)
+        assert np.array_equal(q1, q2)
+
+
+def test_fast_loop_equals_standard_loop():
+    # fast mode (matrix-free LS, planar barriers, vectorised initial catalog)
+    # reproduces the standard loop: the same events, fields to round-off
+    import numpy as np
+    from oracle.aqs_loop import run_events
+    n, M = 8, 10
+    rng = np.random.default_rng(1)
+    mu = 26.923e9 * rng.uniform(0.9, 1.1, (n, n, n))
+    nu = np.full((n, n, n), 0.35)
+    E = 2 * mu * (1 + nu)
+    g = rng.standard_normal((n, n, n, 3, 3)) * 1e-3
+    ep = 0.5 * (g + np.swapaxes(g, -1, -2))
+    kw = dict(barrier_kwargs=dict(mean=2.2, std=0.45, min_cutoff=0.0), softening_cap=0.916, jp=20.0, jt=0.0,
+              eps_plastic=ep)
+    out = [run_events(n, n, n, M, 0.14, E, nu, 0.7, 1276.5, 1e7, 1e-4, (0, 0), 40, seed=1, sel_seed=1, fast=f, **kw)
+           for f in (False, True)]
+    (ev0, st0), (ev1, st1) = out
+    assert [e[:4] for e in ev0] == [e[:4] for e in ev1]
+    assert np.allclose([e[4] for e in ev0], [e[4] for e in ev1], rtol=1e-12)
+    assert np.abs(st0["sig"] - st1["sig"]).max() <= 1e-12 * np.abs(st0["sig"]).max()
+    assert np.array_equal(st0["eps_plastic"], st1["eps_plastic"])

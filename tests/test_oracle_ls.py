@@ -525,3 +525,25 @@ def test_landau_driver_uniaxial_closed_form(d, plane_mode):
             assert abs(em[s][k, k] - t) < 1e-7 * abs(t)
             assert abs(sm[s][k, k]) < 1.0
         assert abs(sm[s][0, 0] - stress(ex, t)[0, 0]) < 1e-7 * abs(sm[s][0, 0])
+
+
+def test_matrix_free_ls_equals_materialised_ls():
+    # spectral_solver_3d_mf: the reference's iteration with Gamma0 applied on
+    # r2c spectra (Nyquist: the Hermitian average) = the materialised
+    # full-spectrum solver to round-off, with the same iteration count; odd,
+    # even and mixed grids (two even axes exercise the Nyquist sign pairing)
+    import numpy as np
+    from oracle import ls as ol
+    for shape in [(8, 6, 10), (7, 9, 5), (6, 7, 4), (5, 8, 8)]:
+        rng = np.random.default_rng(3)
+        E = 70e9 * rng.uniform(0.9, 1.1, shape)
+        nu = np.full(shape, 0.3)
+        g = rng.standard_normal(shape + (3, 3)) * 2e-3
+        ep = 0.5 * (g + np.swapaxes(g, -1, -2))
+        eb = np.array([[0.01, 0.002, 0.0], [0.002, -0.003, 0.001], [0.0, 0.001, 0.0]])
+        i0, i1 = {}, {}
+        r0 = ol.spectral_solver_3d(E, nu, eb, eps_plastic=ep, tol=1e-10, info=i0)
+        r1 = ol.spectral_solver_3d_mf(E, nu, eb, eps_plastic=ep, tol=1e-10, info=i1)
+        assert i0["iters"] == i1["iters"], shape
+        for a, b in zip(r1, r0):
+            assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max(), shape
