@@ -172,7 +172,8 @@ def gpu_fs(n, increments, mixed=False):
     with smooth strain capping (limit 0.10), simple shear F_xy ramped by 0.1% per
     increment (the first `increments` of the 50 to 5%), tol_NW 1e-5; FP64 with
     tol_CG 1e-8, or (mixed) the FP64 Newton loop with FP32 Krylov solves at
-    tol_CG 1e-6.  voxel-matvec = one Krylov operator application over one voxel."""
+    tol_CG 1e-6.  voxel-matvec = one Krylov operator application over one voxel
+    (BiCGSTAB: two per Krylov iteration, plus one per Newton right-hand side)."""
     import torch
     from mgkmc_amd import _lib
     from mgkmc_amd.finite_strain_simulator import FsSolver, material, MIXED
@@ -183,7 +184,7 @@ def gpu_fs(n, increments, mixed=False):
     tcg = 1e-6 if mixed else 1e-8
     mat = material("landau", **LANDAU, strain_capping_enabled=True, strain_capping_type="smooth",
                    strain_capping_limit=0.10, mu_mean=float(mu.mean()))
-    eps = np.linspace(0.0, 0.05, 51)
+    eps = np.linspace(0.0, 0.05, 51)  # F_xy ramp to 5 % in 50 increments
     Fb = np.eye(3)
     s.step(Fb, mat, (0, 1), tol_NW=1e-5, tol_CG=tcg)  # increment 0 (F = I), warm-up
     torch.cuda.synchronize()
@@ -261,14 +262,58 @@ def gpu_serving(n_requests, rate, events):
     from mgkmc_amd.serving import SimServer, request_stream, summarize
     reqs = request_stream(n_requests, rate, 4, events)
     srv = SimServer(max_active=8)
-    res = srv.serve(reqs)
+    done = [0]
+
+    def progress(_r):
+        done[0] += 1
+        if done[0] % 64 == 0:
+            log(f"serving: {done[0]}/{n_requests} requests done")
+    res = srv.serve(reqs, on_done=progress)
     out = summarize(res, srv.wall)
     out.update(slots=8, rate_per_s=rate, events_per_request=events, peak_active=srv.peak_active,
                peak_hbm_gb=srv.peak_used / 1e9)
     return out
 
 
-def cpu_baseline(n_sample=128, iters=3):
+def copy_ceiling(gib=2.0, reps=10):
+    """Streaming device copy (mgk_stream_copy, 16-byte accesses) of `gib` GiB,
+    timed with events on the stream it runs on: the practical read + write HBM
+    ceiling the sweeps are compared against."""
+    import ctypes
+    import torch
+    from mgkmc_amd import _lib
+    from mgkmc_amd import device as dev
+    n = int(gib * 2 ** 30)
+    a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    lib = _lib.load()
+    s = ctypes.c_void_p(dev.stream_ptr())
+    for _ in range(3):
+        _lib.check(lib.mgk_stream_copy(_lib.dptr(a), _lib.dptr(b), n, s), "mgk_stream_copy")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        lib.mgk_stream_copy(_lib.dptr(a), _lib.dptr(b), n, s)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    assert bool(torch.equal(a, b))
+    del a, b
+    torch.cuda.empty_cache()
+    return {"GB_s": 2.0 * n / (ms * 1e-3) / 1e9, "bytes_per_copy": 2 * n, "avg_ms": ms}
+
+
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n_sample=128, iters=3, repeats=3):
     """The oracle (reference structure: materialised Gamma, per-component
     complex FFTs) on this host: LS iterations on a bounded 128^3 sample."""
     from oracle import ls as ol
@@ -282,12 +327,17 @@ def cpu_baseline(n_sample=128, iters=3):
     eb = np.diag([0.01, 0.0, 0.0])
     lam0, mu0 = ol.compute_lame_3d(E.mean(), nu.mean())
     G = ol.green_operator(ol.wave_vectors(E.shape), lam0, mu0)
-    t0 = time.perf_counter()
-    ol.spectral_solver_3d(E, nu, eb, eps_plastic=epa, max_iter=iters, tol=0.0, Gamma=G)
-    dt = time.perf_counter() - t0
+    ol.spectral_solver_3d(E, nu, eb, eps_plastic=epa, max_iter=1, tol=0.0, Gamma=G)  # warm-up
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        ol.spectral_solver_3d(E, nu, eb, eps_plastic=epa, max_iter=iters, tol=0.0, Gamma=G)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     return {"value": iters * n_sample ** 3 / dt, "unit": "voxel-iter/s", "cores": cores, "kind": "port",
-            "sample": f"{n_sample}^3 LS, {iters} iterations of the oracle (materialised Gamma0 built "
-                      f"beforehand, scipy.fft workers={cores})"}
+            "cpu_model": cpu_model(), "repeats_s": times,
+            "sample": f"{n_sample}^3 LS, {iters} iterations of the oracle per run (materialised Gamma0 built "
+                      f"beforehand, scipy.fft workers={cores}), one warm-up iteration, median of {repeats} runs"}
 
 
 def pmc_workload(n):
@@ -333,12 +383,14 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kmc-events", type=int, default=20)
-    ap.add_argument("--fs-increments", type=int, default=2)
-    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--fs-increments", type=int, default=50)
+    ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-events", type=int, default=100)
-    ap.add_argument("--config3-events", type=int, default=0,
-                    help="run only the 512^3 KMC leg with this many events and print its JSON")
+    ap.add_argument("--config3-events", type=int, default=100,
+                    help="events of the 512^3 mixed-BC KMC leg (0: skip it)")
+    ap.add_argument("--config3-only", action="store_true",
+                    help="run only the 512^3 KMC leg and print its JSON")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run only the short LS workload used for rocprofv3 --pmc counter runs")
     a = ap.parse_args()
@@ -349,9 +401,11 @@ def main():
     if a.pmc_workload:
         pmc_workload(a.n)
         return
-    if a.config3_events > 0:
+    if a.config3_only:
         print(json.dumps(gpu_config3(512, a.config3_events)))
         return
+    log("copy ceiling")
+    copy = copy_ceiling()
     log(f"LS forms at {a.n}^3")
     res = gpu_run(a.n, a.steps, a.warmup)
     rmx, r64, r32 = res[2], res[0], res[1]
@@ -371,14 +425,23 @@ def main():
     fsm = gpu_fs(a.n, a.fs_increments, mixed=True) if a.fs_increments > 0 else None
     if fsm is not None:
         fsm["F_err_vs_fp64"] = float((fsm["F"] - fsr["F"]).abs().max() / fsr["F"].abs().max())
+    log("512^3 mixed-BC KMC")
+    cfg3 = gpu_config3(512, a.config3_events) if a.config3_events > 0 else None
     log("serving stream")
     srv = gpu_serving(a.serve_requests, a.serve_rate, a.serve_events) if a.serve_requests > 0 else None
     roof = None
     if dom:
         name, (ms, by, cnt) = dom
         ach = (by / cnt) / ((ms / cnt) * 1e-3)
+        nv = a.n ** 3
+        # the value's workload on the three-sweep model's bytes: 200 B per FP32 and 400 B per FP64 voxel-iter
+        it_bytes = (rmx["iters32"] * 200.0 + rmx["iters64"] * 400.0) * nv
         roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": ach / HBM_PEAK, "traffic": measured_traffic(name, a.n), "kernel": name,
+                "copy_ceiling_GB_s": copy["GB_s"], "frac_of_copy_ceiling": ach / (copy["GB_s"] * 1e9),
+                "iteration_frac": it_bytes / rmx["seconds"] / HBM_PEAK,
+                "iteration_model": "200 B per FP32 / 400 B per FP64 voxel-iter (three-sweep model), over the "
+                                   "refined solves' wall time",
                 "bytes_per_launch": by / cnt, "avg_launch_ms": ms / cnt,
                 "per_kernel": {k: {"avg_ms": v[0] / v[2], "GB_s": (v[1] / v[2]) / (v[0] / v[2] * 1e-3) / 1e9}
                                for k, v in prof.items()}}
@@ -424,13 +487,16 @@ def main():
             "matvecs": fsr["matvecs"], "seconds_per_increment": fsr["seconds_per_increment"],
             "per_kernel": {k: {"avg_ms": v[0] / v[2], "GB_s": (v[1] / v[2]) / (v[0] / v[2] * 1e-3) / 1e9}
                            for k, v in fsr["prof"].items()},
-            "workload": f"{a.n}^3 Landau + smooth capping, simple shear F_xy 0.1%/increment, tol_CG 1e-8"},
+            "roofline_128_words": {"bytes_per_voxel_matvec": 1024, "frac": fsr["voxel_matvec_per_s"] * 1024 / HBM_PEAK},
+            "workload": f"{a.n}^3 Landau + smooth capping, simple shear F_xy to {0.001 * a.fs_increments:.1%} in "
+                        f"{a.fs_increments} increments of 0.1%, tol_NW 1e-5, tol_CG 1e-8"},
         "finite_strain_refined": None if fsm is None else {
             "voxel_matvec_per_s": fsm["voxel_matvec_per_s"], "matvecs": fsm["matvecs"],
             "seconds_per_increment": fsm["seconds_per_increment"],
             "time_to_solution_fp64_over_refined": fsr["seconds_per_increment"] / fsm["seconds_per_increment"],
             "F_err_vs_fp64": fsm["F_err_vs_fp64"],
             "workload": "same increments, FP64 Newton loop + FP32 BiCGSTAB at tol_CG 1e-6"},
+        "config3_512": cfg3,
         "serving": srv,
         "roofline": roof,
         "cpu_baseline": cpu,

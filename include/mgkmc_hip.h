@@ -38,6 +38,11 @@ int mgk_device_info(int* lds_per_block, int* cu_count);
  * 5 kmc_select, 6 fs_constitutive, 7 fs_xfwd, 8 fs_fft_mid, 9 fs_pencil_proj,
  * 10 fs_xinv, 11 fs_vector.  total_bytes: algorithmic HBM bytes of the timed
  * launches. */
+/* Streaming device copy of `bytes` (a multiple of 16, 16-byte aligned pointers) on
+ * `stream`: the practical HBM read + write ceiling that bench.py reports beside the
+ * kernels' roofline (no counterpart in the reference). */
+int mgk_stream_copy(const void* src, void* dst, size_t bytes, void* stream);
+
 void mgk_profile_enable(int on);
 void mgk_profile_reset(void);
 int mgk_profile_read(int kid, double* total_ms, double* total_bytes, long long* launches);

@@ -131,6 +131,7 @@ SIGNATURES = {
     "mgk_ls_workspace_bytes": (_c_i, [_c_i, _c_i64p, _c_i, ctypes.POINTER(_c_sz)]),
     "mgk_ls_solve": (_c_i, [_c_i, _c_i64p, _c_i, _c_p, _c_p, _c_d, _c_d, _c_dp, _c_p, _c_p, _c_p,
                             _c_i, _c_i, _c_d, _c_p, _c_sz, _c_p, ctypes.POINTER(MgkLsInfo)]),
+    "mgk_stream_copy": (_c_i, [_c_p, _c_p, _c_sz, _c_p]),
     "mgk_ls_solve_noalloc": (_c_i, [_c_i, _c_i64p, _c_i, _c_p, _c_p, _c_d, _c_d, _c_dp, _c_p, _c_p, _c_p,
                                     _c_i, _c_i, _c_d, _c_p, ctypes.POINTER(MgkLsInfo)]),
     "mgk_ls_solve_refined": (_c_i, [_c_i, _c_i64p, _c_p, _c_p, _c_p, _c_p, _c_d, _c_d, _c_dp, _c_p, _c_p, _c_p,
