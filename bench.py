@@ -255,13 +255,15 @@ def gpu_config3(n, events):
     return out
 
 
-def gpu_serving(n_requests, rate, events):
+def gpu_serving(n_requests, rate, events, slots=16, chunk=10):
     """Serving stream (Poisson arrivals at `rate`/s, seed 4; grids uniform over
     64^2..512^2 and 64^3..192^3; kT/E0 ~ U[0.01, 0.08]; `events` KMC events
-    per request with FP32 + FP64 refinement LS), 8 slots, HBM admission."""
+    per request with FP32 + FP64 refinement LS): HBM admission of up to
+    `slots` resident simulations, continuous batching per grid shape
+    (`chunk` events per member between joins)."""
     from mgkmc_amd.serving import SimServer, request_stream, summarize
     reqs = request_stream(n_requests, rate, 4, events)
-    srv = SimServer(max_active=8)
+    srv = SimServer(max_active=slots, chunk_events=chunk)
     done = [0]
 
     def progress(_r):
@@ -270,8 +272,8 @@ def gpu_serving(n_requests, rate, events):
             log(f"serving: {done[0]}/{n_requests} requests done")
     res = srv.serve(reqs, on_done=progress)
     out = summarize(res, srv.wall)
-    out.update(slots=8, rate_per_s=rate, events_per_request=events, peak_active=srv.peak_active,
-               peak_hbm_gb=srv.peak_used / 1e9)
+    out.update(slots=slots, batched=True, chunk_events=chunk, rate_per_s=rate, events_per_request=events,
+               peak_active=srv.peak_active, peak_batch=srv.peak_batch, peak_hbm_gb=srv.peak_used / 1e9)
     return out
 
 

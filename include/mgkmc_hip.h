@@ -105,6 +105,43 @@ int mgk_ls_solve_refined(int ndim, const int64_t* shape, const void* lam32, cons
                          double tol32, void* ws32, size_t ws32_bytes, void* ws64, size_t ws64_bytes, void* stream,
                          mgk_ls_info* info, int* iters32);
 
+/* Batched forms of mgk_ls_solve and mgk_ls_solve_refined (no counterpart in the
+ * reference: the serving form, in which several resident simulations of one grid
+ * shape are solved with one launch per sweep).  Each member carries its own fields,
+ * reference medium (lam0, mu0), macro strain and workspace (sized as for the
+ * unbatched call); the *32 fields are read by the refined form only.  A member's
+ * fields, stress, iteration counts and status are bit-identical to the unbatched
+ * call on the same inputs: it runs the same kernels on its own buffers and stops on
+ * its own stop test, while the batch continues until every member has stopped.
+ * info (and iters32) are arrays of nb.  The return value is the first negative
+ * member status, else MGK_NOT_CONVERGED if any member did not converge, else
+ * MGK_OK.  Blocks until done. */
+#define MGK_LS_MAX_BATCH 8
+typedef struct {
+  const void* lam;           /* FP64 (or the precision of mgk_ls_solve_batch) */
+  const void* mu;
+  double lam0, mu0;
+  const double* eps_bar;     /* d*d row-major, host                           */
+  const void* eps_plastic;   /* or NULL                                       */
+  void* eps;                 /* in (warm) / out                               */
+  void* sig;                 /* out or NULL                                   */
+  void* workspace;
+  size_t workspace_bytes;
+  const void* lam32;         /* refined form: FP32 copies and stage-1 field   */
+  const void* mu32;
+  const void* eps_plastic32;
+  void* eps32;
+  void* workspace32;
+  size_t workspace32_bytes;
+} mgk_ls_member;
+
+int mgk_ls_solve_batch(int ndim, const int64_t* shape, int precision, int nb, const mgk_ls_member* members,
+                       int warm, int max_iter, double tol, void* stream, mgk_ls_info* info);
+
+int mgk_ls_solve_refined_batch(int ndim, const int64_t* shape, int nb, const mgk_ls_member* members, int warm,
+                               int max_iter, double tol, double tol32, void* stream, mgk_ls_info* info,
+                               int* iters32);
+
 /* Landau small-strain model (3D).  Replaces spectral_solver_landau_3d, default
  * "dbfft" path (mgkmc/linear_elastic_simulator.py:1231-1350) with the stress of
  * stress_from_strain_landau_3d (mgkmc/elasticity.py:297-335, :811-861).  Constants in
@@ -433,6 +470,25 @@ typedef struct {
 int mgk_kmc_run(const mgk_kmc_loop* loop, mgk_mt_state* rng, mgk_mt_state* selection, double* time,
                 double* eps_macro, double* remaining, int max_events, int max_seq, int* seq,
                 mgk_kmc_event* events, int* n_events, mgk_ls_info* info, int* iters32, void* stream);
+
+/* Batched form of mgk_kmc_run (no counterpart in the reference: the serving form's
+ * continuous batching).  nb (1..MGK_LS_MAX_BATCH) simulations of one grid shape and
+ * one LS configuration (ls_precision, warm, max_iter, tol, tol32) step in lockstep on
+ * `stream`: one synchronisation reads every member's total rate, one every member's
+ * selection, and one batched LS solve (mgk_ls_solve_batch / _refined_batch)
+ * re-equilibrates every member that had an event.  Each member runs mgk_kmc_run's
+ * kernels and draws on its own buffers and streams, so its events, fields and stream
+ * states are bit-identical to mgk_kmc_run's on the same inputs.  Per-member arrays:
+ * loop, rng, selection, time, remaining, max_events (each 1..cap), seq, n_events,
+ * status (the member's MGK_KMC_RUN_* stop reason or a negative error), info, iters32
+ * (nb each), eps_macro (9 per member, d*d used), events (cap per member: member b's
+ * at events + b*cap).  The call returns once every member has stopped: MGK_OK, or a
+ * negative error if the batch itself failed (then every member still running
+ * carries that error as its status). */
+int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* loop, mgk_mt_state* rng, mgk_mt_state* selection, double* time,
+                      double* eps_macro, double* remaining, const int* max_events, int max_seq, int* seq,
+                      mgk_kmc_event* events, int cap, int* n_events, int* status, mgk_ls_info* info, int* iters32,
+                      void* stream);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,18 @@ class MgkLsInfo(ctypes.Structure):
                 ("eps_mean", ctypes.c_double * 9), ("sig_mean", ctypes.c_double * 9)]
 
 
+class MgkLsMember(ctypes.Structure):
+    _fields_ = [("lam", ctypes.c_void_p), ("mu", ctypes.c_void_p), ("lam0", ctypes.c_double),
+                ("mu0", ctypes.c_double), ("eps_bar", ctypes.POINTER(ctypes.c_double)),
+                ("eps_plastic", ctypes.c_void_p), ("eps", ctypes.c_void_p), ("sig", ctypes.c_void_p),
+                ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t), ("lam32", ctypes.c_void_p),
+                ("mu32", ctypes.c_void_p), ("eps_plastic32", ctypes.c_void_p), ("eps32", ctypes.c_void_p),
+                ("workspace32", ctypes.c_void_p), ("workspace32_bytes", ctypes.c_size_t)]
+
+
+MGK_LS_MAX_BATCH = 8
+
+
 class MgkLandauParams(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4")] + \
                [("capping", ctypes.c_int), ("cap_smooth", ctypes.c_int), ("cap_limit", ctypes.c_double),
@@ -137,6 +149,10 @@ SIGNATURES = {
     "mgk_ls_solve_refined": (_c_i, [_c_i, _c_i64p, _c_p, _c_p, _c_p, _c_p, _c_d, _c_d, _c_dp, _c_p, _c_p, _c_p,
                                     _c_p, _c_p, _c_i, _c_i, _c_d, _c_d, _c_p, _c_sz, _c_p, _c_sz, _c_p,
                                     ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i)]),
+    "mgk_ls_solve_batch": (_c_i, [_c_i, _c_i64p, _c_i, _c_i, ctypes.POINTER(MgkLsMember), _c_i, _c_i, _c_d, _c_p,
+                                  ctypes.POINTER(MgkLsInfo)]),
+    "mgk_ls_solve_refined_batch": (_c_i, [_c_i, _c_i64p, _c_i, ctypes.POINTER(MgkLsMember), _c_i, _c_i, _c_d, _c_d,
+                                          _c_p, ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i)]),
     "mgk_ls_solve_landau": (_c_i, [_c_i64p, _c_i, _c_p, _c_p, _c_d, _c_d, ctypes.POINTER(MgkLandauParams), _c_dp, _c_p,
                                    _c_p, _c_p, _c_p, _c_i, _c_i, _c_d, _c_p, _c_sz, _c_p, ctypes.POINTER(MgkLsInfo)]),
     "mgk_ls_solve_landau2d": (_c_i, [_c_i64p, _c_p, _c_p, _c_d, _c_d, ctypes.POINTER(MgkLandauParams), _c_i, _c_dp,
@@ -151,6 +167,10 @@ SIGNATURES = {
     "mgk_kmc_run": (_c_i, [ctypes.POINTER(MgkKmcLoop), ctypes.POINTER(MgkMtState), ctypes.POINTER(MgkMtState),
                            _c_dp, _c_dp, _c_dp, _c_i, _c_i, ctypes.POINTER(_c_i), ctypes.POINTER(MgkKmcEvent),
                            ctypes.POINTER(_c_i), ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i), _c_p]),
+    "mgk_kmc_run_batch": (_c_i, [_c_i, ctypes.POINTER(MgkKmcLoop), ctypes.POINTER(MgkMtState),
+                                 ctypes.POINTER(MgkMtState), _c_dp, _c_dp, _c_dp, ctypes.POINTER(_c_i), _c_i,
+                                 ctypes.POINTER(_c_i), ctypes.POINTER(MgkKmcEvent), _c_i, ctypes.POINTER(_c_i),
+                                 ctypes.POINTER(_c_i), ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i), _c_p]),
     "mgk_kmc_workspace_bytes": (_c_i, [ctypes.c_longlong, ctypes.POINTER(_c_sz)]),
     "mgk_kmc_barrier_rates": (_c_i, [_c_i, ctypes.c_longlong, _c_i, _c_i, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p,
                                      _c_p, _c_p, ctypes.POINTER(MgkKmcParams), _c_p, _c_p, _c_p, _c_dp]),

@@ -366,6 +366,9 @@ __device__ __forceinline__ void tile_fft_ct(cplx<T>* __restrict__ buf, const cpl
 // being fetched from HBM once per XCD.
 // ---------------------------------------------------------------------------
 #define MGK_NXCD 8
+
+// most simulations one batched LS launch serves (kernel-argument tables)
+#define MGK_MAXB 8
 __device__ __forceinline__ int xcd_tile(int w, int n) {
   const int per = n / MGK_NXCD;
   if (w >= per * MGK_NXCD) return w;

@@ -102,15 +102,35 @@ __device__ __forceinline__ void xmerge_load(cplx<T>* tile, int ldb, int nx, int 
   __syncthreads();
 }
 
+// Spectra of up to MGK_MAXB simulations of one grid shape for one mid-axis
+// launch: member m's component c is grid slice z = m * ncomp + c.
+template <typename T>
+struct MidPtrs {
+  cplx<T>* spec[MGK_MAXB];
+  const int* done[MGK_MAXB];  // per-member stop flag (NULL: always run)
+  int ncomp;
+};
+
+template <typename T>
+__host__ __device__ inline MidPtrs<T> mid_one(cplx<T>* spec, const int* done, int ncomp) {
+  MidPtrs<T> m{};
+  m.spec[0] = spec;
+  m.done[0] = done;
+  m.ncomp = ncomp;
+  return m;
+}
+
 // ---------------------------------------------------------------------------
 // sweep 2/4: complex FFT along the middle axis (length P, stride Q)
-// grid (ceil(Q/B), nxh, C)
+// grid (ceil(Q/B), nxh, C * members)
 // ---------------------------------------------------------------------------
 template <typename T, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
 __global__ __launch_bounds__(NTH) void k_fft_mid(int P, int Q, int nxh, int logB, FftPlan pl,
-                                                    const cplx<T>* __restrict__ twg, cplx<T>* __restrict__ spec,
-                                                    int inv, const int* __restrict__ done) {
+                                                    const cplx<T>* __restrict__ twg, MidPtrs<T> mp, int inv) {
+  const int mem = blockIdx.z / mp.ncomp;
+  const int* __restrict__ done = mp.done[mem];
   if (done && *done) return;
+  cplx<T>* __restrict__ spec = mp.spec[mem];
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* tile = tw + P;
@@ -120,7 +140,7 @@ __global__ __launch_bounds__(NTH) void k_fft_mid(int P, int Q, int nxh, int logB
     logB = ct_log2(BB);
   }
   const int B = CT ? BB : 1 << logB, ldb = B;
-  const int q0 = blockIdx.x * B, kx = blockIdx.y, c = blockIdx.z;
+  const int q0 = blockIdx.x * B, kx = blockIdx.y, c = blockIdx.z - mem * mp.ncomp;
   cplx<T>* base = spec + ((size_t)c * nxh + kx) * (size_t)P * Q;
   for (int t = threadIdx.x; t < P; t += NTH) tw[t] = twg[t];
   // FP32 with compile-time tiles: two complex values (16 bytes) per lane and

@@ -1496,30 +1496,7 @@ __global__ __launch_bounds__(MGK_NT) void k_comp_means9(size_t NV, const T* __re
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-extern "C" {
-typedef struct {
-  int model;  // 0 svk, 1 landau, 2 murnaghan
-  int capping;
-  int cap_smooth;
-  int plane;  // 0 3D; 2D on an (nx, 1, ny) grid: 1 plane strain, 2 plane stress
-  double v1, v2, v3, g1, g2, g3, g4;  // Pa (the caller applies the GPa autoconversion)
-  double Am, Bm, Cm;
-  double cap_limit;  // resolved E_cap
-  double cap_power, cap_ratio;
-} mgk_fs_material;
-
-typedef struct {
-  int status;        // 0 ok; MGK_ERR_SOLVER with fail_kind set
-  int fail_kind;     // 1 line search, 2 Newton not converged, 3 macro BC not converged
-  int newton_iters;  // total over macro iterations
-  int krylov_iters;  // total BiCGSTAB iterations
-  int macro_iters;
-  double max_err;
-  double F_mean[9];
-  double sig_mean[9];
-  double F_bar[9];
-} mgk_fs_info;
-}
+// the C ABI's structs (mgk_fs_material, mgk_fs_info) come from the public header
 
 namespace {
 
@@ -1632,13 +1609,13 @@ void fs_launch_mid(const FsPlan& pl, hipStream_t s, const FftPlan& ply, const cp
   const dim3 gy(pl.nqy, pl.nxh, nc);
   if (pl.cty) {
     switch (pl.P) {
-      case 64: k_fft_mid<T, ct_tile<T>() / MGK_NT, 64, ct_tile<T>() / 64, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
-      case 128: k_fft_mid<T, ct_tile<T>() / MGK_NT, 128, ct_tile<T>() / 128, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
-      case 256: k_fft_mid<T, ct_tile<T>() / MGK_NT, 256, ct_tile<T>() / 256, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
-      default: k_fft_mid<T, ct_tile<T>() / MGK_NT, 512, ct_tile<T>() / 512, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip); break;
+      case 64: k_fft_mid<T, ct_tile<T>() / MGK_NT, 64, ct_tile<T>() / 64, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mid_one(spec, skip, (int)gy.z), inv); break;
+      case 128: k_fft_mid<T, ct_tile<T>() / MGK_NT, 128, ct_tile<T>() / 128, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mid_one(spec, skip, (int)gy.z), inv); break;
+      case 256: k_fft_mid<T, ct_tile<T>() / MGK_NT, 256, ct_tile<T>() / 256, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mid_one(spec, skip, (int)gy.z), inv); break;
+      default: k_fft_mid<T, ct_tile<T>() / MGK_NT, 512, ct_tile<T>() / 512, 512><<<gy, 512, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mid_one(spec, skip, (int)gy.z), inv); break;
     }
   } else {
-    k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, skip);
+    k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, s>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mid_one(spec, skip, (int)gy.z), inv);
   }
 }
 

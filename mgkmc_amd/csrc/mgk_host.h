@@ -9,6 +9,7 @@
 #include <vector>
 #include <cmath>
 #include "mgk_common.h"
+#include "../../include/mgkmc_hip.h"
 
 #define MGK_HIP(call)                                  \
   do {                                                 \
@@ -38,6 +39,15 @@ size_t mgk_lds_limit();
 
 // Pinned host scratch (per thread, grown on demand) for small D2H reads.
 void* mgk_pinned(size_t bytes);
+
+// Rate pass and selection of one simulation of the event loop, enqueued on s
+// with their small results copied into pinned host memory (host_total: one
+// double; host_rec: mgk_kmc_sel_rec_bytes() bytes, read with
+// mgk_kmc_sel_unpack after the stream has been synchronised).
+int mgk_kmc_rates_async(const mgk_kmc_loop* L, double time, double* host_total, hipStream_t s);
+size_t mgk_kmc_sel_rec_bytes();
+int mgk_kmc_select_async(const mgk_kmc_loop* L, double time, double u, void* host_rec, hipStream_t s);
+void mgk_kmc_sel_unpack(const void* host_rec, mgk_kmc_sel* out);
 
 // ---------------------------------------------------------------------------
 // Live per-kernel timing (HIP events on the launching stream).  Off by default;

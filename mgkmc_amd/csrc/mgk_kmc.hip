@@ -670,22 +670,8 @@ __global__ void k_kmc_set_voxel(size_t NV, int M, int CK, long long v, const dou
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-extern "C" {
-typedef struct {
-  double volume, rate_volume, nu0, temperature, cap, tau, time;
-  int scheme, cascade;
-} mgk_kmc_params;
-
-typedef struct {
-  long long item;
-  int mode;
-  int fallthrough;
-  double r;
-  double cum_before;
-  double cum_at;
-  double q;
-} mgk_kmc_sel;
-}
+// the C ABI's structs (mgk_kmc_params, mgk_kmc_sel, mgk_kmc_loop)
+#include "../../include/mgkmc_hip.h"
 
 static KmcParams to_params(const mgk_kmc_params* p) {
   KmcParams P;
@@ -848,6 +834,58 @@ extern "C" int mgk_kmc_select(int ndim, long long nvox, int M, int sig_kind, int
   out->cum_at = hp->cum_at;
   out->q = hp->q;
   return MGK_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Asynchronous forms of the rate pass and the selection for the batched event
+// loop: each is enqueued with a copy of its small result into pinned host
+// memory, and the caller synchronises once for all members of a batch.  The
+// kernels and their arguments are those of mgk_kmc_barrier_rates /
+// mgk_kmc_select as the single-simulation loop calls them (planar catalog, no
+// temperature field, no Q output).
+// ---------------------------------------------------------------------------
+int mgk_kmc_rates_async(const mgk_kmc_loop* L, double time, double* host_total, hipStream_t s) {
+  const int ndim = L->ndim, cat_kind = 0, sig_kind = L->sig_kind;
+  long long nvox = 1;
+  for (int i = 0; i < ndim; ++i) nvox *= L->shape[i];
+  mgk_kmc_params p = L->kp;
+  p.time = time;
+  const KmcParams P = to_params(&p);
+  const KmcWs w = kmc_ws(L->kmc_ws, (size_t)nvox);
+  KMC_DISPATCH(launch_rates, (size_t)nvox, L->M, L->sig, L->Q0, L->cat, L->soft, L->tlast, L->prev, nullptr, P,
+               nullptr, w, s);
+  MGK_HIP(hipGetLastError());
+  MGK_HIP(hipMemcpyAsync(host_total, w.total, sizeof(double), hipMemcpyDeviceToHost, s));
+  return MGK_OK;
+}
+
+size_t mgk_kmc_sel_rec_bytes() { return sizeof(KmcSel); }
+
+int mgk_kmc_select_async(const mgk_kmc_loop* L, double time, double u, void* host_rec, hipStream_t s) {
+  const int ndim = L->ndim, cat_kind = 0, sig_kind = L->sig_kind;
+  long long nvox = 1;
+  for (int i = 0; i < ndim; ++i) nvox *= L->shape[i];
+  mgk_kmc_params p = L->kp;
+  p.time = time;
+  const KmcParams P = to_params(&p);
+  const KmcWs w = kmc_ws(L->kmc_ws, (size_t)nvox);
+  KMC_DISPATCH(launch_select, (size_t)nvox, L->M, u, L->sig, L->Q0, L->cat, L->soft, L->tlast, L->prev, nullptr, P,
+               w, s);
+  MGK_HIP(hipGetLastError());
+  MGK_HIP(hipMemcpyAsync(host_rec, w.sel, sizeof(KmcSel), hipMemcpyDeviceToHost, s));
+  return MGK_OK;
+}
+
+void mgk_kmc_sel_unpack(const void* host_rec, mgk_kmc_sel* out) {
+  const KmcSel* h = (const KmcSel*)host_rec;
+  out->item = h->item;
+  out->mode = h->mode;
+  out->fallthrough = h->fallthrough;
+  out->r = h->r;
+  out->cum_before = h->cum_before;
+  out->cum_at = h->cum_at;
+  out->q = h->q;
 }
 
 // Rates of a Q field in the reference (v, m) layout: full per-mode rates

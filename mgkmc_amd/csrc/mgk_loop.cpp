@@ -5,6 +5,7 @@
 // doubles, polar-method Gaussians with one cached value) continued in C++
 // from the caller's state and handed back, so an event sequence does not
 // depend on which side of the boundary ran it.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include "mgk_host.h"
@@ -86,6 +87,43 @@ double* redraw_stage(size_t count) {
   return np_;
 }
 
+// the voxel's redraw from the host stream: 2M normals (gxx, gxy per mode),
+// then M clipped Gaussian barriers (3M doubles into host)
+void redraw_draws(const mgk_kmc_loop* L, Mt& R, double* host) {
+  const int M = L->M;
+  const double hg = 0.5 * L->gamma0;
+  if (L->redraw_directions)
+    for (int i = 0; i < 2 * M; ++i) host[i] = hg * R.gauss();
+  if (L->redraw_barriers)
+    for (int i = 0; i < M; ++i) {
+      double q = L->q_mean + L->q_std * R.gauss();
+      if (L->q_has_min) q = std::fmax(q, L->q_min);
+      if (L->q_has_max) q = std::fmin(q, L->q_max);
+      host[2 * M + i] = q;
+    }
+}
+
+bool loop_ok(const mgk_kmc_loop* L, const mgk_mt_state* rng, const mgk_mt_state* sel_rng) {
+  return L && rng && sel_rng && (L->ndim == 2 || L->ndim == 3) && L->M >= 1 && L->Q0 && L->cat && L->soft &&
+         L->tlast && L->epsp && L->kmc_ws && L->redraw_dev && L->sig && L->eps && L->ws &&
+         (L->ls_precision != 2 || (L->lam32 && L->mu32 && L->eps32 && L->ws32)) && rng->pos >= 0 &&
+         rng->pos <= 624 && sel_rng->pos >= 0 && sel_rng->pos <= 624;
+}
+
+void fill_event(mgk_kmc_event& ev, long long v, int m, int instab, double time, const mgk_ls_info& inf, int it32,
+                const double* eps_macro, int d) {
+  ev.v = v;
+  ev.m = m;
+  ev.is_instab = instab;
+  ev.time = time;
+  ev.ls_iters = inf.iters;
+  ev.ls_iters32 = it32;
+  for (int k = 0; k < 9; ++k) {
+    ev.eps_macro[k] = k < d * d ? eps_macro[k] : 0.0;
+    ev.sig_mean[k] = inf.sig_mean[k];
+  }
+}
+
 bool unstable_mean(const mgk_ls_info& inf, int d) {
   double mx = 0.0;
   for (int i = 0; i < d * d; ++i) {
@@ -111,11 +149,8 @@ extern "C" int mgk_mt_fill(mgk_mt_state* state, int kind, double loc, double sca
 extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_state* sel_rng, double* time,
                            double* eps_macro, double* remaining, int max_events, int max_seq, int* seq,
                            mgk_kmc_event* events, int* n_events, mgk_ls_info* info, int* iters32, void* stream) {
-  if (!L || !rng || !sel_rng || !time || !eps_macro || !remaining || !seq || !events || !n_events || !info ||
-      max_events < 1 || (L->ndim != 2 && L->ndim != 3) || L->M < 1 || !L->Q0 || !L->cat || !L->soft ||
-      !L->tlast || !L->epsp || !L->kmc_ws || !L->redraw_dev || !L->sig || !L->eps || !L->ws ||
-      (L->ls_precision == 2 && (!L->lam32 || !L->mu32 || !L->eps32 || !L->ws32)) ||
-      rng->pos < 0 || rng->pos > 624 || sel_rng->pos < 0 || sel_rng->pos > 624) {
+  if (!loop_ok(L, rng, sel_rng) || !time || !eps_macro || !remaining || !seq || !events || !n_events || !info ||
+      max_events < 1) {
     mgk_set_error("mgk_kmc_run: bad argument", __FILE__, __LINE__);
     return MGK_ERR_ARG;
   }
@@ -133,7 +168,6 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
     mgk_set_error("mgk_kmc_run: pinned host allocation failed", __FILE__, __LINE__);
     return MGK_ERR_HIP;
   }
-  const double hg = 0.5 * L->gamma0;
   int n = 0, rc = MGK_KMC_RUN_TIME;
   *n_events = 0;
   while (true) {
@@ -172,16 +206,7 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
           rc = e;
           break;
         }
-        // redraw: 2M normals (gxx, gxy per mode), then M barriers, from the host stream
-        if (L->redraw_directions)
-          for (int i = 0; i < 2 * M; ++i) host[i] = hg * R.gauss();
-        if (L->redraw_barriers)
-          for (int i = 0; i < M; ++i) {
-            double q = L->q_mean + L->q_std * R.gauss();
-            if (L->q_has_min) q = std::fmax(q, L->q_min);
-            if (L->q_has_max) q = std::fmin(q, L->q_max);
-            host[2 * M + i] = q;
-          }
+        redraw_draws(L, R, host);
         if (hipMemcpyAsync(L->redraw_dev, host, sizeof(double) * 3 * (size_t)M, hipMemcpyHostToDevice, s) !=
             hipSuccess) {
           mgk_set_error("mgk_kmc_run: redraw upload failed", __FILE__, __LINE__);
@@ -214,17 +239,7 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
           rc = MGK_KMC_RUN_UNSTABLE;
           break;
         }
-        mgk_kmc_event& ev = events[n++];
-        ev.v = v;
-        ev.m = m;
-        ev.is_instab = instab;
-        ev.time = *time;
-        ev.ls_iters = info->iters;
-        ev.ls_iters32 = it32;
-        for (int k = 0; k < 9; ++k) {
-          ev.eps_macro[k] = k < d * d ? eps_macro[k] : 0.0;
-          ev.sig_mean[k] = info->sig_mean[k];
-        }
+        fill_event(events[n++], v, m, instab, *time, *info, it32, eps_macro, d);
         *n_events = n;
         *seq = instab ? *seq + 1 : 0;
         if (*seq > max_seq) {
@@ -247,4 +262,234 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
   }
   (void)hipStreamSynchronize(s);
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Batched event loop: nb simulations of one grid shape and LS configuration
+// stepped in lockstep on one stream.  Per step: every running member's rate
+// pass is enqueued and one synchronisation reads all totals; the members'
+// waiting times and selection draws come from their own streams; their
+// selections are enqueued and read with one synchronisation; flips and
+// redraws are enqueued; one batched LS solve re-equilibrates every member
+// that had an event.  Each member runs the kernels and draws of mgk_kmc_run
+// on its own buffers, so its events, fields and streams are bit-identical to
+// mgk_kmc_run's; a member stops on its own condition while the others go on.
+// ---------------------------------------------------------------------------
+namespace {
+struct BatchStage {
+  void* p = nullptr;
+  size_t n = 0;
+};
+thread_local BatchStage g_bstage;
+
+// pinned host staging of the batch loop (grown on demand; a smaller earlier
+// block is dropped, never freed, as a hipHostFree would synchronise the device)
+void* batch_stage(size_t bytes) {
+  if (g_bstage.n >= bytes) return g_bstage.p;
+  void* np_ = nullptr;
+  if (hipHostMalloc(&np_, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+  g_bstage.p = np_;
+  g_bstage.n = bytes;
+  return np_;
+}
+}  // namespace
+
+extern "C" int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_state* sel_rng,
+                                 double* time, double* eps_macro, double* remaining, const int* max_events,
+                                 int max_seq, int* seq, mgk_kmc_event* events, int cap, int* n_events, int* status,
+                                 mgk_ls_info* info, int* iters32, void* stream) {
+  if (nb < 1 || nb > MGK_LS_MAX_BATCH || !L || !rng || !sel_rng || !time || !eps_macro || !remaining ||
+      !max_events || !seq || !events || cap < 1 || !n_events || !status || !info) {
+    mgk_set_error("mgk_kmc_run_batch: bad argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  const mgk_kmc_loop& L0 = L[0];
+  const int d = L0.ndim;
+  int Mmax = 0;
+  for (int b = 0; b < nb; ++b) {
+    const mgk_kmc_loop& x = L[b];
+    bool same = loop_ok(&x, &rng[b], &sel_rng[b]) && x.ndim == d && x.ls_precision == L0.ls_precision &&
+                (x.warm != 0) == (L0.warm != 0) && x.max_iter == L0.max_iter && x.tol == L0.tol &&
+                x.tol32 == L0.tol32 && max_events[b] >= 1 && max_events[b] <= cap;
+    for (int i = 0; i < d; ++i) same = same && x.shape[i] == L0.shape[i];
+    if (!same) {
+      mgk_set_error("mgk_kmc_run_batch: a member is invalid or differs from the first in grid shape, LS precision "
+                    "or LS settings, or its max_events is outside 1..cap",
+                    __FILE__, __LINE__);
+      return MGK_ERR_ARG;
+    }
+    Mmax = std::max(Mmax, x.M);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  long long NV = 1;
+  int shape_i[3] = {1, 1, 1};
+  for (int i = 0; i < d; ++i) {
+    NV *= L0.shape[i];
+    shape_i[i] = (int)L0.shape[i];
+  }
+  const size_t rec = (mgk_kmc_sel_rec_bytes() + 63) & ~(size_t)63;
+  const size_t off_rec = 64 * (size_t)MGK_LS_MAX_BATCH, off_red = off_rec + rec * MGK_LS_MAX_BATCH;
+  char* stage = (char*)batch_stage(off_red + sizeof(double) * 3 * (size_t)Mmax * MGK_LS_MAX_BATCH);
+  if (!stage) {
+    mgk_set_error("mgk_kmc_run_batch: pinned host allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
+  double* tot = (double*)stage;
+  bool run[MGK_LS_MAX_BATCH], ev[MGK_LS_MAX_BATCH];
+  double t0[MGK_LS_MAX_BATCH], us[MGK_LS_MAX_BATCH];
+  for (int b = 0; b < nb; ++b) {
+    run[b] = true;
+    n_events[b] = 0;
+    status[b] = MGK_KMC_RUN_TIME;
+    if (iters32) iters32[b] = 0;
+  }
+  int rc_all = MGK_OK;
+  auto stop = [&](int b, int code) {
+    run[b] = false;
+    status[b] = code;
+  };
+  auto fail_running = [&](int code) {
+    for (int b = 0; b < nb; ++b)
+      if (run[b]) stop(b, code);
+    rc_all = code;
+  };
+  while (true) {
+    int nrun = 0;
+    for (int b = 0; b < nb; ++b) nrun += run[b];
+    if (!nrun) break;
+    // rate passes, one read of all totals
+    bool ok = true;
+    for (int b = 0; b < nb && ok; ++b)
+      if (run[b]) ok = mgk_kmc_rates_async(&L[b], time[b], &tot[b], s) == MGK_OK;
+    if (!ok || hipStreamSynchronize(s) != hipSuccess) {
+      if (ok) mgk_set_error("mgk_kmc_run_batch: stream synchronisation failed", __FILE__, __LINE__);
+      fail_running(MGK_ERR_HIP);
+      break;
+    }
+    int nev = 0;
+    for (int b = 0; b < nb; ++b) {
+      ev[b] = false;
+      if (!run[b]) continue;
+      Mt R{&rng[b]}, S{&sel_rng[b]};
+      const mgk_kmc_loop& x = L[b];
+      double* em = eps_macro + 9 * b;
+      t0[b] = time[b];
+      if (tot[b] > 0) {
+        const double u = R.uniform();
+        const double tw = -std::log(u) / tot[b];
+        if (tw < remaining[b]) {
+          ev[b] = true;
+          ++nev;
+          time[b] += tw;
+          for (int k = 0; k < d * d; ++k) em[k] += x.unit[k] * (x.strain_rate * tw);
+          remaining[b] -= tw;
+          us[b] = S.uniform();
+        }
+      }
+      if (!ev[b]) {
+        for (int k = 0; k < d * d; ++k) em[k] += x.unit[k] * (x.strain_rate * remaining[b]);
+        time[b] += remaining[b];
+        remaining[b] = 0.0;
+        stop(b, MGK_KMC_RUN_TIME);
+      }
+    }
+    if (!nev) continue;
+    // selections, one read of all records
+    for (int b = 0; b < nb && ok; ++b)
+      if (ev[b]) ok = mgk_kmc_select_async(&L[b], t0[b], us[b], stage + off_rec + rec * b, s) == MGK_OK;
+    if (!ok || hipStreamSynchronize(s) != hipSuccess) {
+      if (ok) mgk_set_error("mgk_kmc_run_batch: stream synchronisation failed", __FILE__, __LINE__);
+      fail_running(MGK_ERR_HIP);
+      break;
+    }
+    // flips and redraws
+    long long vv[MGK_LS_MAX_BATCH];
+    int mm[MGK_LS_MAX_BATCH], ins[MGK_LS_MAX_BATCH];
+    for (int b = 0; b < nb && ok; ++b) {
+      if (!ev[b]) continue;
+      const mgk_kmc_loop& x = L[b];
+      mgk_kmc_sel so;
+      mgk_kmc_sel_unpack(stage + off_rec + rec * b, &so);
+      vv[b] = so.item;
+      mm[b] = so.mode;
+      ins[b] = so.q <= x.stability_threshold ? 1 : 0;
+      int e = mgk_kmc_flip(d, shape_i, x.M, 0, vv[b], mm[b], time[b], x.jp, x.jt, x.kp.cap, x.jn, x.cat, x.epsp,
+                           x.eps_precision, x.epsp_shadow, x.soft, x.tlast, x.prev, nullptr, 1, stream);
+      double* host = (double*)(stage + off_red) + 3 * (size_t)Mmax * b;
+      Mt R{&rng[b]};
+      redraw_draws(&x, R, host);
+      if (e == MGK_OK && hipMemcpyAsync(x.redraw_dev, host, sizeof(double) * 3 * (size_t)x.M,
+                                        hipMemcpyHostToDevice, s) != hipSuccess) {
+        mgk_set_error("mgk_kmc_run_batch: redraw upload failed", __FILE__, __LINE__);
+        e = MGK_ERR_HIP;
+      }
+      if (e == MGK_OK)
+        e = mgk_kmc_set_voxel(NV, x.M, 2, vv[b], x.redraw_directions ? x.redraw_dev : nullptr,
+                              x.redraw_barriers ? x.redraw_dev + 2 * x.M : nullptr, x.cat, x.Q0, stream);
+      if (e != MGK_OK) {
+        fail_running(e);
+        ok = false;
+      }
+    }
+    if (!ok) break;
+    // one batched re-solve of every member with an event
+    mgk_ls_member mem[MGK_LS_MAX_BATCH] = {};
+    int idx[MGK_LS_MAX_BATCH], k = 0;
+    for (int b = 0; b < nb; ++b) {
+      if (!ev[b]) continue;
+      const mgk_kmc_loop& x = L[b];
+      mgk_ls_member& m = mem[k];
+      m.lam = x.lam;
+      m.mu = x.mu;
+      m.lam0 = x.lam0;
+      m.mu0 = x.mu0;
+      m.eps_bar = eps_macro + 9 * b;
+      m.eps_plastic = x.epsp;
+      m.eps = x.eps;
+      m.sig = x.sig;
+      m.workspace = x.ws;
+      m.workspace_bytes = x.ws_bytes;
+      m.lam32 = x.lam32;
+      m.mu32 = x.mu32;
+      m.eps_plastic32 = x.epsp_shadow;
+      m.eps32 = x.eps32;
+      m.workspace32 = x.ws32;
+      m.workspace32_bytes = x.ws32_bytes;
+      idx[k++] = b;
+    }
+    mgk_ls_info inf[MGK_LS_MAX_BATCH] = {};
+    int it32[MGK_LS_MAX_BATCH] = {};
+    int e;
+    if (L0.ls_precision == 2)
+      e = mgk_ls_solve_refined_batch(d, L0.shape, k, mem, L0.warm, L0.max_iter, L0.tol, L0.tol32, stream, inf, it32);
+    else
+      e = mgk_ls_solve_batch(d, L0.shape, L0.ls_precision, k, mem, L0.warm, L0.max_iter, L0.tol, stream, inf);
+    if (e < 0 && e != MGK_ERR_NONFINITE) {  // the batch itself failed
+      fail_running(e);
+      break;
+    }
+    for (int j = 0; j < k; ++j) {
+      const int b = idx[j];
+      info[b] = inf[j];
+      if (iters32) iters32[b] = it32[j];
+      const int eb = inf[j].status;
+      if (eb < 0 && eb != MGK_ERR_NONFINITE) {
+        stop(b, eb);
+        continue;
+      }
+      if (L[b].check_unstable && unstable_mean(inf[j], d)) {
+        mgk_set_error("Unstable stress in small-strain elastic_run", __FILE__, __LINE__);
+        stop(b, MGK_KMC_RUN_UNSTABLE);
+        continue;
+      }
+      fill_event(events[(size_t)b * cap + n_events[b]], vv[b], mm[b], ins[b], time[b], inf[j], it32[j],
+                 eps_macro + 9 * b, d);
+      n_events[b] += 1;
+      seq[b] = ins[b] ? seq[b] + 1 : 0;
+      if (seq[b] > max_seq) stop(b, MGK_KMC_RUN_SEQ);
+      else if (n_events[b] >= max_events[b]) stop(b, MGK_KMC_RUN_COUNT);
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  return rc_all < 0 ? rc_all : MGK_OK;
 }

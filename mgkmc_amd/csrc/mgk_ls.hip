@@ -74,6 +74,37 @@ struct Sym6 {
   double v[6];
 };
 
+// One simulation of a batched LS launch: its fields, spectrum, partial sums,
+// solve state and constants.  The sweep kernels take a table of up to
+// MGK_MAXB members of one grid shape by value and pick their member from the
+// grid (x sweeps: blockIdx.z, pencils: blockIdx.y, mid sweeps: blockIdx.z /
+// components); each member stops on its own state's done flag, so a member's
+// arithmetic and iteration count are those of a solve alone.
+template <typename T>
+struct LsMem {
+  const T* lam;
+  const T* mu;
+  const T* epsp;
+  T* eps;
+  cplx<T>* spec;
+  double* part;
+  LsState* st;
+  Sym6 eb;
+  double lam0, mu0, e2_extra;
+};
+
+template <typename T>
+struct LsBatch {
+  LsMem<T> m[MGK_MAXB];
+};
+
+// what k_ls_finalize needs of each member
+struct LsFin {
+  const double* part[MGK_MAXB];
+  LsState* st[MGK_MAXB];
+  double e2_extra[MGK_MAXB];
+};
+
 // ---------------------------------------------------------------------------
 // frequency helpers
 // ---------------------------------------------------------------------------
@@ -144,11 +175,15 @@ __device__ __forceinline__ void gamma2_apply(const T q0, const T q1, const cplx<
 // sweep 1: polarisation + forward x FFT
 // ---------------------------------------------------------------------------
 template <typename T, int D, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
-__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 ? (sizeof(T) == 4 ? 4 : 2) : (sizeof(T) == 4 ? 3 : 2)))) void k_ls_tau_xfwd(LsGeom<T> g, const T* __restrict__ eps,
-                                                        const T* __restrict__ epsp, const T* __restrict__ lam,
-                                                        const T* __restrict__ mu, T lam0, T mu0,
-                                                        cplx<T>* __restrict__ spec, const LsState* __restrict__ st) {
-  if (st->done) return;
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 ? (sizeof(T) == 4 ? 4 : 2) : (sizeof(T) == 4 ? 3 : 2)))) void k_ls_tau_xfwd(LsGeom<T> g, LsBatch<T> bt) {
+  const LsMem<T>& M = bt.m[blockIdx.z];
+  if (M.st->done) return;
+  const T* __restrict__ eps = M.eps;
+  const T* __restrict__ epsp = M.epsp;
+  const T* __restrict__ lam = M.lam;
+  const T* __restrict__ mu = M.mu;
+  const T lam0 = T(M.lam0), mu0 = T(M.mu0);
+  cplx<T>* __restrict__ spec = M.spec;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* tile = tw + g.nx;
@@ -332,9 +367,12 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
 // grid ceil(npencil/PP); pencil id = kx*P + p
 // ---------------------------------------------------------------------------
 template <typename T, int D, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
-__global__ __launch_bounds__(NTH) void k_ls_pencil(LsGeom<T> g, cplx<T>* __restrict__ spec, T A2, T Bc, T scale,
-                                                      const LsState* __restrict__ st) {
-  if (st->done) return;
+__global__ __launch_bounds__(NTH) void k_ls_pencil(LsGeom<T> g, LsBatch<T> bt, T scale) {
+  const LsMem<T>& M = bt.m[blockIdx.y];
+  if (M.st->done) return;
+  cplx<T>* __restrict__ spec = M.spec;
+  const T A2 = T(1.0 / (2.0 * M.mu0));
+  const T Bc = T((M.lam0 + M.mu0) / (M.mu0 * (M.lam0 + 2.0 * M.mu0)));
   constexpr int C = (D == 3) ? 6 : 3;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
@@ -449,11 +487,13 @@ __global__ __launch_bounds__(NTH) void k_ls_pencil(LsGeom<T> g, cplx<T>* __restr
 // sweep 5: inverse x FFT (complex-to-real) + update + FP64 partial norms
 // ---------------------------------------------------------------------------
 template <typename T, int D, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
-__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 ? (sizeof(T) == 4 ? 4 : 2) : (sizeof(T) == 4 ? 4 : 2)))) void k_ls_xinv_update(LsGeom<T> g, const cplx<T>* __restrict__ spec,
-                                                           T* __restrict__ eps, Sym6 ebar,
-                                                           double* __restrict__ partials,
-                                                           const LsState* __restrict__ st) {
-  if (st->done) return;
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 ? (sizeof(T) == 4 ? 4 : 2) : (sizeof(T) == 4 ? 4 : 2)))) void k_ls_xinv_update(LsGeom<T> g, LsBatch<T> bt) {
+  const LsMem<T>& M = bt.m[blockIdx.z];
+  if (M.st->done) return;
+  const cplx<T>* __restrict__ spec = M.spec;
+  T* __restrict__ eps = M.eps;
+  const Sym6 ebar = M.eb;
+  double* __restrict__ partials = M.part;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* tile = tw + g.nx;
@@ -560,10 +600,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
   }
 }
 
-__global__ __launch_bounds__(MGK_NT) void k_ls_finalize(const double* __restrict__ partials, int nparts,
-                                                        LsState* __restrict__ st, double tol, int max_iter,
-                                                        double e2_extra) {
+__global__ __launch_bounds__(MGK_NT) void k_ls_finalize(LsFin f, int nparts, double tol, int max_iter) {
+  LsState* __restrict__ st = f.st[blockIdx.x];
   if (st->done) return;
+  const double* __restrict__ partials = f.part[blockIdx.x];
+  const double e2_extra = f.e2_extra[blockIdx.x];
   __shared__ double red[2 * (MGK_NT / 64)];
   double acc[2] = {0.0, 0.0};
   for (int i = threadIdx.x; i < nparts; i += MGK_NT) {
@@ -1105,24 +1146,8 @@ __global__ void k_widen(const float* __restrict__ a, double* __restrict__ b, siz
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
-extern "C" {
-typedef struct {
-  int iters;
-  int status;
-  double diff;
-  double eps_mean[9];  // full d x d row-major (symmetric part + constant antisymmetric part)
-  double sig_mean[9];
-} mgk_ls_info;
-typedef struct {
-  double v1, v2, v3, g1, g2, g3, g4;
-  int capping, cap_smooth;
-  double cap_limit, cap_power, cap_ratio;
-} mgk_landau_params;
-typedef struct {
-  int law, plane_stress;
-  double p1, p2;
-} mgk_secant_params;
-}
+// the C ABI's structs (mgk_ls_info, mgk_landau_params, mgk_secant_params, mgk_ls_member)
+#include "../../include/mgkmc_hip.h"
 
 namespace {
 
@@ -1224,17 +1249,17 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl, bool allow_ct = true) {
 constexpr int MID_NT = 512;
 
 template <typename T, int MV>
-void launch_mid(const LsPlan& pl, dim3 gy, hipStream_t stream, const FftPlan& ply, const cplx<T>* twy, cplx<T>* spec,
-                int inv, const int* done) {
+void launch_mid(const LsPlan& pl, dim3 gy, hipStream_t stream, const FftPlan& ply, const cplx<T>* twy,
+                const MidPtrs<T>& mp, int inv) {
   if (pl.cty) {
     switch (pl.P) {
-      case 64: k_fft_mid<T, ct_tile<T>() / MGK_NT, 64, ct_tile<T>() / 64, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
-      case 128: k_fft_mid<T, ct_tile<T>() / MGK_NT, 128, ct_tile<T>() / 128, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
-      case 256: k_fft_mid<T, ct_tile<T>() / MGK_NT, 256, ct_tile<T>() / 256, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
-      default: k_fft_mid<T, ct_tile<T>() / MGK_NT, 512, ct_tile<T>() / 512, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done); break;
+      case 64: k_fft_mid<T, ct_tile<T>() / MGK_NT, 64, ct_tile<T>() / 64, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mp, inv); break;
+      case 128: k_fft_mid<T, ct_tile<T>() / MGK_NT, 128, ct_tile<T>() / 128, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mp, inv); break;
+      case 256: k_fft_mid<T, ct_tile<T>() / MGK_NT, 256, ct_tile<T>() / 256, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mp, inv); break;
+      default: k_fft_mid<T, ct_tile<T>() / MGK_NT, 512, ct_tile<T>() / 512, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mp, inv); break;
     }
   } else {
-    k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, spec, inv, done);
+    k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mp, inv);
   }
 }
 
@@ -1349,17 +1374,14 @@ struct L2Mode {
 
 __global__ void k_l2_init(LsState* __restrict__ st, int e33_valid) { st->e33_valid = e33_valid; }
 
-template <typename T, int D>
-int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double mu0, const double* eps_bar,
-               const T* epsp, T* eps, T* sig, int warm, int max_iter, double tol, char* ws, hipStream_t stream,
-               mgk_ls_info* info, bool final_stress = true, const LandauPar* lp = nullptr, T* taubuf = nullptr,
-               const L2Mode* l2 = nullptr, const SecLaw* sl = nullptr, const NoAllocCtx* na = nullptr) {
-  constexpr int MV = maxv<T>();
-  constexpr int TT = ct_tile<T>();  // points per compile-time tile
-  constexpr int XT = x_tile<T>();   // points per compile-time x-sweep tile
-  constexpr int X_NT = x_nt<T>();   // threads per x-sweep workgroup
-  constexpr int C = (D == 3) ? 6 : 3;
-  const int prec = sizeof(T) == 8 ? 0 : 1;
+
+// ---------------------------------------------------------------------------
+// Launches of the linear sweeps over a table of nb members of one grid shape.
+// The grid of each sweep gains the member dimension; a member's workgroups
+// run the same code on its own buffers as in a solve of that member alone.
+// ---------------------------------------------------------------------------
+template <typename T>
+LsGeom<T> ls_geom(const LsPlan& pl, int C) {
   LsGeom<T> g;
   g.nx = pl.nx;
   g.P = pl.P;
@@ -1374,6 +1396,149 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   mgk_make_plan(pl.nx, &g.plx);
   mgk_make_plan(pl.P, &g.ply);
   mgk_make_plan(pl.Q, &g.plq);
+  g.twx = g.twy = g.twq = nullptr;
+  return g;
+}
+
+// symmetric part of the macro strain in component order and the squared
+// norm of its antisymmetric part (per voxel)
+template <int D>
+Sym6 ls_sym_bar(const double* e, double* anti2) {
+  Sym6 eb{};
+  if (D == 3) {
+    eb.v[0] = e[0];
+    eb.v[1] = e[4];
+    eb.v[2] = e[8];
+    eb.v[3] = 0.5 * (e[5] + e[7]);
+    eb.v[4] = 0.5 * (e[2] + e[6]);
+    eb.v[5] = 0.5 * (e[1] + e[3]);
+    const double a1 = 0.5 * (e[5] - e[7]), a2 = 0.5 * (e[2] - e[6]), a3 = 0.5 * (e[1] - e[3]);
+    *anti2 = 2.0 * (a1 * a1 + a2 * a2 + a3 * a3);
+  } else {
+    eb.v[0] = e[0];
+    eb.v[1] = e[3];
+    eb.v[2] = 0.5 * (e[1] + e[2]);
+    const double a = 0.5 * (e[1] - e[2]);
+    *anti2 = 2.0 * a * a;
+  }
+  return eb;
+}
+
+// macro means of a finished solve into the caller's info record
+template <int D>
+void ls_fill_info(const LsState& s, const double* eps_bar, mgk_ls_info* info) {
+  constexpr int C = (D == 3) ? 6 : 3;
+  info->iters = s.iters;
+  info->status = s.status;
+  info->diff = s.diff;
+  const double* em = s.mean;
+  const double* sm = s.mean + C;
+  if (D == 3) {
+    const double a1 = 0.5 * (eps_bar[5] - eps_bar[7]), a2 = 0.5 * (eps_bar[2] - eps_bar[6]),
+                 a3 = 0.5 * (eps_bar[1] - eps_bar[3]);
+    const double E[9] = {em[0], em[5] + a3, em[4] + a2, em[5] - a3, em[1], em[3] + a1, em[4] - a2, em[3] - a1, em[2]};
+    const double S[9] = {sm[0], sm[5], sm[4], sm[5], sm[1], sm[3], sm[4], sm[3], sm[2]};
+    for (int i = 0; i < 9; ++i) {
+      info->eps_mean[i] = E[i];
+      info->sig_mean[i] = S[i];
+    }
+  } else {
+    const double a = 0.5 * (eps_bar[1] - eps_bar[2]);
+    const double E[4] = {em[0], em[2] + a, em[2] - a, em[1]};
+    const double S[4] = {sm[0], sm[2], sm[2], sm[1]};
+    for (int i = 0; i < 9; ++i) {
+      info->eps_mean[i] = i < 4 ? E[i] : 0.0;
+      info->sig_mean[i] = i < 4 ? S[i] : 0.0;
+    }
+  }
+}
+
+template <typename T, int D>
+void launch_tau(const LsPlan& pl, const LsGeom<T>& g, dim3 gx, hipStream_t stream, const LsBatch<T>& bt) {
+  constexpr int MV = maxv<T>();
+  constexpr int XT = x_tile<T>();
+  constexpr int X_NT = x_nt<T>();
+  if (pl.ctx) {
+    switch (pl.nx) {
+      case 64: k_ls_tau_xfwd<T, D, XT / X_NT, 64, XT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
+      case 128: k_ls_tau_xfwd<T, D, XT / X_NT, 128, XT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
+      case 256: k_ls_tau_xfwd<T, D, XT / X_NT, 256, XT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
+      default: k_ls_tau_xfwd<T, D, XT / X_NT, 512, XT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
+    }
+  } else {
+    k_ls_tau_xfwd<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, bt);
+  }
+}
+
+template <typename T, int D>
+void launch_pencil(const LsPlan& pl, const LsGeom<T>& g, dim3 gp, hipStream_t stream, const LsBatch<T>& bt,
+                   T scale) {
+  constexpr int MV = maxv<T>();
+  constexpr int TT = ct_tile<T>();
+  constexpr int C = (D == 3) ? 6 : 3;
+  if (pl.ctq) {
+    switch (pl.Q) {
+      case 64: k_ls_pencil<T, D, TT / MGK_NT, 64, ct_pp(TT, C, 64)><<<gp, MGK_NT, pl.lds_q, stream>>>(g, bt, scale); break;
+      case 128: k_ls_pencil<T, D, TT / MGK_NT, 128, ct_pp(TT, C, 128)><<<gp, MGK_NT, pl.lds_q, stream>>>(g, bt, scale); break;
+      case 256: k_ls_pencil<T, D, TT / MGK_NT, 256, ct_pp(TT, C, 256)><<<gp, MGK_NT, pl.lds_q, stream>>>(g, bt, scale); break;
+      default: k_ls_pencil<T, D, TT / MGK_NT, 512, ct_pp(TT, C, 512)><<<gp, MGK_NT, pl.lds_q, stream>>>(g, bt, scale); break;
+    }
+  } else {
+    k_ls_pencil<T, D, MV><<<gp, MGK_NT, pl.lds_q, stream>>>(g, bt, scale);
+  }
+}
+
+template <typename T, int D>
+void launch_xinv(const LsPlan& pl, const LsGeom<T>& g, dim3 gx, hipStream_t stream, const LsBatch<T>& bt) {
+  constexpr int MV = maxv<T>();
+  constexpr int XT = x_tile<T>();
+  constexpr int X_NT = x_nt<T>();
+  if (pl.ctx) {
+    switch (pl.nx) {
+      case 64: k_ls_xinv_update<T, D, XT / X_NT, 64, XT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
+      case 128: k_ls_xinv_update<T, D, XT / X_NT, 128, XT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
+      case 256: k_ls_xinv_update<T, D, XT / X_NT, 256, XT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
+      default: k_ls_xinv_update<T, D, XT / X_NT, 512, XT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
+    }
+  } else {
+    k_ls_xinv_update<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, bt);
+  }
+}
+
+template <typename T>
+MidPtrs<T> mid_table(const LsBatch<T>& bt, int nb, int C) {
+  MidPtrs<T> mp{};
+  for (int b = 0; b < nb; ++b) {
+    mp.spec[b] = bt.m[b].spec;
+    mp.done[b] = &bt.m[b].st->done;
+  }
+  mp.ncomp = C;
+  return mp;
+}
+
+template <typename T>
+LsFin fin_table(const LsBatch<T>& bt, int nb) {
+  LsFin f{};
+  for (int b = 0; b < nb; ++b) {
+    f.part[b] = bt.m[b].part;
+    f.st[b] = bt.m[b].st;
+    f.e2_extra[b] = bt.m[b].e2_extra;
+  }
+  return f;
+}
+
+template <typename T, int D>
+int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double mu0, const double* eps_bar,
+               const T* epsp, T* eps, T* sig, int warm, int max_iter, double tol, char* ws, hipStream_t stream,
+               mgk_ls_info* info, bool final_stress = true, const LandauPar* lp = nullptr, T* taubuf = nullptr,
+               const L2Mode* l2 = nullptr, const SecLaw* sl = nullptr, const NoAllocCtx* na = nullptr) {
+  constexpr int MV = maxv<T>();
+  constexpr int TT = ct_tile<T>();  // points per compile-time tile
+  constexpr int XT = x_tile<T>();   // points per compile-time x-sweep tile
+  constexpr int X_NT = x_nt<T>();   // threads per x-sweep workgroup
+  constexpr int C = (D == 3) ? 6 : 3;
+  const int prec = sizeof(T) == 8 ? 0 : 1;
+  LsGeom<T> g = ls_geom<T>(pl, C);
   if (na) {  // twiddle tables staged in the caller's scratch (nothing allocated here)
     g.twx = (const cplx<T>*)na->tw[0];
     g.twy = (const cplx<T>*)na->tw[1];
@@ -1392,26 +1557,8 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   LsState* st = (LsState*)(ws + pl.off_state);
 
   // symmetric part of eps_bar in component order, antisymmetric norm extra
-  Sym6 eb{};
   double anti2 = 0.0;
-  if (D == 3) {
-    const double* e = eps_bar;
-    eb.v[0] = e[0];
-    eb.v[1] = e[4];
-    eb.v[2] = e[8];
-    eb.v[3] = 0.5 * (e[5] + e[7]);
-    eb.v[4] = 0.5 * (e[2] + e[6]);
-    eb.v[5] = 0.5 * (e[1] + e[3]);
-    const double a1 = 0.5 * (e[5] - e[7]), a2 = 0.5 * (e[2] - e[6]), a3 = 0.5 * (e[1] - e[3]);
-    anti2 = 2.0 * (a1 * a1 + a2 * a2 + a3 * a3);
-  } else {
-    const double* e = eps_bar;
-    eb.v[0] = e[0];
-    eb.v[1] = e[3];
-    eb.v[2] = 0.5 * (e[1] + e[2]);
-    const double a = 0.5 * (e[1] - e[2]);
-    anti2 = 2.0 * a * a;
-  }
+  const Sym6 eb = ls_sym_bar<D>(eps_bar, &anti2);
   const double e2_extra = anti2 * (double)pl.NV;
   const size_t NV = (size_t)pl.NV;
 
@@ -1427,13 +1574,16 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   }
   MGK_HIP(hipGetLastError());
 
-  const T A2 = T(1.0 / (2.0 * mu0));
-  const T Bc = T((lam0 + mu0) / (mu0 * (lam0 + 2.0 * mu0)));
   const T scale = T(1.0 / (double)pl.NV);
   const int nqx = (pl.Q + (1 << pl.logBx) - 1) >> pl.logBx;
   const int nqy = (pl.Q + (1 << pl.logBy) - 1) >> pl.logBy;
   const int npb = (g.npencil + (1 << pl.logPP) - 1) >> pl.logPP;
-  const dim3 gx(nqx, pl.P), gy(nqy, pl.nxh, C);
+  const dim3 gx(nqx, pl.P), gy(nqy, pl.nxh, C), gp(npb);
+  // this solve as a one-member table of the linear sweeps
+  LsBatch<T> bt{};
+  bt.m[0] = LsMem<T>{lam, mu, epsp, eps, spec, part, st, eb, lam0, mu0, e2_extra};
+  const MidPtrs<T> mp = mid_table(bt, 1, C);
+  const LsFin fin = fin_table(bt, 1);
 
   // algorithmic HBM bytes per launch (what each sweep must read + write)
   const double sT = (double)sizeof(T);
@@ -1487,38 +1637,20 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
       }
     } else {
     h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
-    if (pl.ctx) {
-      switch (pl.nx) {
-        case 64: k_ls_tau_xfwd<T, D, XT / X_NT, 64, XT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-        case 128: k_ls_tau_xfwd<T, D, XT / X_NT, 128, XT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-        case 256: k_ls_tau_xfwd<T, D, XT / X_NT, 256, XT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-        default: k_ls_tau_xfwd<T, D, XT / X_NT, 512, XT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st); break;
-      }
-    } else {
-      k_ls_tau_xfwd<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, eps, epsp, lam, mu, T(lam0), T(mu0), spec, st);
-    }
+    launch_tau<T, D>(pl, g, gx, stream, bt);
     prof.end(h);
     }
     if (D == 3 && pl.P > 1) {
       h = prof.begin(KID_LS_MID, iter_idx, b_fft);
-      launch_mid<T, MV>(pl, gy, stream, g.ply, g.twy, spec, 0, &st->done);
+      launch_mid<T, MV>(pl, gy, stream, g.ply, g.twy, mp, 0);
       prof.end(h);
     }
     h = prof.begin(KID_LS_PENCIL, iter_idx, b_fft);
-    if (pl.ctq) {
-      switch (pl.Q) {
-        case 64: k_ls_pencil<T, D, TT / MGK_NT, 64, ct_pp(TT, C, 64)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
-        case 128: k_ls_pencil<T, D, TT / MGK_NT, 128, ct_pp(TT, C, 128)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
-        case 256: k_ls_pencil<T, D, TT / MGK_NT, 256, ct_pp(TT, C, 256)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
-        default: k_ls_pencil<T, D, TT / MGK_NT, 512, ct_pp(TT, C, 512)><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st); break;
-      }
-    } else {
-      k_ls_pencil<T, D, MV><<<npb, MGK_NT, pl.lds_q, stream>>>(g, spec, A2, Bc, scale, st);
-    }
+    launch_pencil<T, D>(pl, g, gp, stream, bt, scale);
     prof.end(h);
     if (D == 3 && pl.P > 1) {
       h = prof.begin(KID_LS_MID, iter_idx, b_fft);
-      launch_mid<T, MV>(pl, gy, stream, g.ply, g.twy, spec, 1, &st->done);
+      launch_mid<T, MV>(pl, gy, stream, g.ply, g.twy, mp, 1);
       prof.end(h);
     }
     if constexpr (sizeof(T) == 8) {
@@ -1526,18 +1658,9 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
         k_copy<<<nel, MGK_NT, 0, stream>>>(eps, l2->eps_old, (size_t)C * NV, st);
     }
     h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
-    if (pl.ctx) {
-      switch (pl.nx) {
-        case 64: k_ls_xinv_update<T, D, XT / X_NT, 64, XT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-        case 128: k_ls_xinv_update<T, D, XT / X_NT, 128, XT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-        case 256: k_ls_xinv_update<T, D, XT / X_NT, 256, XT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-        default: k_ls_xinv_update<T, D, XT / X_NT, 512, XT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st); break;
-      }
-    } else {
-      k_ls_xinv_update<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, spec, eps, eb, part, st);
-    }
+    launch_xinv<T, D>(pl, g, gx, stream, bt);
     prof.end(h);
-    k_ls_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_upd, st, tol, max_iter, e2_extra);
+    k_ls_finalize<<<1, MGK_NT, 0, stream>>>(fin, pl.nblk_upd, tol, max_iter);
     if constexpr (sizeof(T) == 8) {
       if (l2 && l2->aa_m > 0) {
         const size_t n3 = (size_t)C * NV;
@@ -1632,34 +1755,175 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     (void)hipEventDestroy(ev[1]);
   }
   const LsState& s = hst[0];
-  if (info) {
-    info->iters = s.iters;
-    info->status = s.status;
-    info->diff = s.diff;
-    const double* em = s.mean;
-    const double* sm = s.mean + C;
-    if (D == 3) {
-      const double a1 = 0.5 * (eps_bar[5] - eps_bar[7]), a2 = 0.5 * (eps_bar[2] - eps_bar[6]),
-                   a3 = 0.5 * (eps_bar[1] - eps_bar[3]);
-      const double E[9] = {em[0], em[5] + a3, em[4] + a2, em[5] - a3, em[1], em[3] + a1, em[4] - a2, em[3] - a1, em[2]};
-      const double S[9] = {sm[0], sm[5], sm[4], sm[5], sm[1], sm[3], sm[4], sm[3], sm[2]};
-      for (int i = 0; i < 9; ++i) {
-        info->eps_mean[i] = E[i];
-        info->sig_mean[i] = S[i];
-      }
-    } else {
-      const double a = 0.5 * (eps_bar[1] - eps_bar[2]);
-      const double E[4] = {em[0], em[2] + a, em[2] - a, em[1]};
-      const double S[4] = {sm[0], sm[2], sm[2], sm[1]};
-      for (int i = 0; i < 9; ++i) {
-        info->eps_mean[i] = i < 4 ? E[i] : 0.0;
-        info->sig_mean[i] = i < 4 ? S[i] : 0.0;
-      }
-    }
-  }
+  if (info) ls_fill_info<D>(s, eps_bar, info);
   return s.status;
 }
 
+
+// ---------------------------------------------------------------------------
+// Batched linear solve: nb (<= MGK_MAXB) simulations of one grid shape and
+// precision, each with its own fields, reference medium, macro strain and
+// workspace, iterated together with one launch per sweep.  Every member runs
+// the kernel sequence of a solve alone (same kernels, same partial-sum
+// order) and stops on its own state, so its fields, iteration count and
+// status are bit-identical to mgk_ls_solve on the same inputs; the batch
+// polls until every member has stopped.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct LsMemArgs {
+  const T* lam;
+  const T* mu;
+  double lam0, mu0;
+  const double* eps_bar;
+  const T* epsp;
+  T* eps;
+  T* sig;
+  char* ws;
+};
+
+// pinned host copies of the members' solve states, per thread (two chunks
+// in flight); allocated once and kept until the process ends
+LsState* batch_host_states() {
+  thread_local LsState* p = nullptr;
+  if (!p && hipHostMalloc((void**)&p, 2 * MGK_MAXB * sizeof(LsState), hipHostMallocDefault) != hipSuccess)
+    p = nullptr;
+  return p;
+}
+
+template <typename T, int D>
+int ls_solve_batch_t(const LsPlan& pl, int nb, const LsMemArgs<T>* a, int warm, int max_iter, double tol,
+                     hipStream_t stream, mgk_ls_info* info, bool final_stress) {
+  constexpr int MV = maxv<T>();
+  constexpr int C = (D == 3) ? 6 : 3;
+  const int prec = sizeof(T) == 8 ? 0 : 1;
+  LsGeom<T> g = ls_geom<T>(pl, C);
+  g.twx = (const cplx<T>*)mgk_twiddles(pl.nx, prec);
+  g.twy = (const cplx<T>*)mgk_twiddles(pl.P, prec);
+  g.twq = (const cplx<T>*)mgk_twiddles(pl.Q, prec);
+  if (!g.twx || !g.twy || !g.twq) {
+    mgk_set_error("twiddle table allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
+  const size_t NV = (size_t)pl.NV;
+  const int nfill = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 4096);
+  LsBatch<T> bt{};
+  int hint = 0;
+  for (int b = 0; b < nb; ++b) {
+    cplx<T>* spec = (cplx<T>*)(a[b].ws + pl.off_spec);
+    double* part = (double*)(a[b].ws + pl.off_part);
+    LsState* st = (LsState*)(a[b].ws + pl.off_state);
+    double anti2 = 0.0;
+    const Sym6 eb = ls_sym_bar<D>(a[b].eps_bar, &anti2);
+    bt.m[b] = LsMem<T>{a[b].lam, a[b].mu, a[b].epsp, a[b].eps, spec, part, st, eb, a[b].lam0, a[b].mu0,
+                       anti2 * (double)pl.NV};
+    k_zero_words<<<1, 64, 0, stream>>>((unsigned int*)st, sizeof(LsState) / 4);
+    if (!warm) {
+      k_ls_fill<T><<<nfill, MGK_NT, 0, stream>>>(a[b].eps, C, NV, eb);
+    } else if (warm == 1) {
+      k_comp_sums<T><<<pl.nblk_red, MGK_NT, 0, stream>>>(a[b].eps, C, NV, part);
+      k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, C, (double)NV, st->mean);
+      k_ls_warm_adjust<T><<<nfill, MGK_NT, 0, stream>>>(a[b].eps, C, NV, eb, st->mean);
+    }
+    hint = std::max(hint, ls_iter_hint(a[b].ws, 0));
+  }
+  MGK_HIP(hipGetLastError());
+
+  const T scale = T(1.0 / (double)pl.NV);
+  const int nqx = (pl.Q + (1 << pl.logBx) - 1) >> pl.logBx;
+  const int nqy = (pl.Q + (1 << pl.logBy) - 1) >> pl.logBy;
+  const int npb = (g.npencil + (1 << pl.logPP) - 1) >> pl.logPP;
+  const dim3 gx(nqx, pl.P, nb), gy(nqy, pl.nxh, C * nb), gp(npb, nb);
+  const MidPtrs<T> mp = mid_table(bt, nb, C);
+  const LsFin fin = fin_table(bt, nb);
+
+  const double sT = (double)sizeof(T);
+  const double specB = (double)C * pl.nxh * pl.P * pl.Q * 2.0 * sT;
+  const double fieldB = (double)pl.NV * sT;
+  const double b_tau = nb * (fieldB * ((a[0].epsp ? 2.0 * C : (double)C) + 2.0) + specB);
+  const double b_fft = nb * 2.0 * specB;
+  const double b_upd = nb * (specB + 2.0 * C * fieldB);
+  MgkProfRec prof(stream);
+  int iter_idx = 0;
+  auto enqueue_iter = [&]() {
+    int h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
+    launch_tau<T, D>(pl, g, gx, stream, bt);
+    prof.end(h);
+    if (D == 3 && pl.P > 1) {
+      h = prof.begin(KID_LS_MID, iter_idx, b_fft);
+      launch_mid<T, MV>(pl, gy, stream, g.ply, g.twy, mp, 0);
+      prof.end(h);
+    }
+    h = prof.begin(KID_LS_PENCIL, iter_idx, b_fft);
+    launch_pencil<T, D>(pl, g, gp, stream, bt, scale);
+    prof.end(h);
+    if (D == 3 && pl.P > 1) {
+      h = prof.begin(KID_LS_MID, iter_idx, b_fft);
+      launch_mid<T, MV>(pl, gy, stream, g.ply, g.twy, mp, 1);
+      prof.end(h);
+    }
+    h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
+    launch_xinv<T, D>(pl, g, gx, stream, bt);
+    prof.end(h);
+    k_ls_finalize<<<nb, MGK_NT, 0, stream>>>(fin, pl.nblk_upd, tol, max_iter);
+    ++iter_idx;
+  };
+
+  LsState* hst = batch_host_states();
+  if (!hst) {
+    mgk_set_error("pinned host allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  MGK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  MGK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  // chunked enqueue as in the single solve; the batch stops once every
+  // member's state reads done
+  int enq = 0, chunk = hint > 0 ? hint : 2, slot = 0;
+  bool pending = false, stop = false, first = true;
+  while (!stop && enq < max_iter) {
+    const int k = std::min(chunk, max_iter - enq);
+    for (int i = 0; i < k; ++i) enqueue_iter();
+    enq += k;
+    MGK_HIP(hipGetLastError());
+    for (int b = 0; b < nb; ++b)
+      MGK_HIP(hipMemcpyAsync(&hst[slot * MGK_MAXB + b], bt.m[b].st, sizeof(LsState), hipMemcpyDeviceToHost, stream));
+    MGK_HIP(hipEventRecord(ev[slot], stream));
+    if (pending) {
+      MGK_HIP(hipEventSynchronize(ev[slot ^ 1]));
+      bool all = true;
+      for (int b = 0; b < nb; ++b) all = all && hst[(slot ^ 1) * MGK_MAXB + b].done;
+      if (all) stop = true;
+    }
+    pending = true;
+    slot ^= 1;
+    chunk = first ? 1 : std::min(chunk * 2, 16);
+    first = false;
+  }
+  for (int b = 0; b < nb; ++b) {
+    if (final_stress) {
+      double* part = bt.m[b].part;
+      k_ls_stress<T, D><<<pl.nblk_red, MGK_NT, 0, stream>>>(a[b].eps, a[b].epsp, a[b].lam, a[b].mu, a[b].sig, NV,
+                                                            part);
+      k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, 2 * C, (double)NV, bt.m[b].st->mean);
+    }
+    MGK_HIP(hipMemcpyAsync(&hst[b], bt.m[b].st, sizeof(LsState), hipMemcpyDeviceToHost, stream));
+  }
+  MGK_HIP(hipGetLastError());
+  MGK_HIP(hipStreamSynchronize(stream));
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  int iters_max = 0, rc = MGK_OK;
+  for (int b = 0; b < nb; ++b) {
+    const LsState& sb = hst[b];
+    ls_iter_hint(a[b].ws, sb.iters);
+    iters_max = std::max(iters_max, sb.iters);
+    if (info) ls_fill_info<D>(sb, a[b].eps_bar, &info[b]);
+    if (sb.status < 0 && rc >= 0) rc = sb.status;
+    else if (sb.status > 0 && rc == MGK_OK) rc = sb.status;
+  }
+  prof.commit(iters_max);
+  return rc;
+}
 }  // namespace
 
 extern "C" int mgk_ls_workspace_bytes(int ndim, const int64_t* shape, int precision, size_t* bytes) {
@@ -1720,7 +1984,6 @@ extern "C" int mgk_ls_solve(int ndim, const int64_t* shape, int precision, const
 // array, so calls that overlap on one GPU (two streams, two threads) race;
 // the solve state is cleared by a kernel at the start of every call.
 // ---------------------------------------------------------------------------
-#define MGK_LS_NOALLOC_BYTES ((size_t)64 << 20)
 static __device__ __align__(256) unsigned char scratch[MGK_LS_NOALLOC_BYTES];
 
 namespace {
@@ -1873,6 +2136,136 @@ extern "C" int mgk_ls_solve_refined(int ndim, const int64_t* shape, const void* 
   return ls_solve_t<double, 2>(p64, (const double*)lam64, (const double*)mu64, lam0, mu0, eps_bar,
                                (const double*)epsp64, (double*)eps64, (double*)sig64, 2, max_iter, tol, (char*)ws64, s,
                                info);
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched entry points (no counterpart in the reference: the serving form's
+// one-launch-per-sweep solve of several simulations of one grid shape).
+// ---------------------------------------------------------------------------
+static_assert(MGK_MAXB == MGK_LS_MAX_BATCH, "kernel member tables and the C ABI batch limit differ");
+
+namespace {
+template <typename T>
+int ls_batch_args(const LsPlan& pl, int nb, const mgk_ls_member* m, int refined32, LsMemArgs<T>* a) {
+  for (int b = 0; b < nb; ++b) {
+    const mgk_ls_member& x = m[b];
+    const void* lam = refined32 ? x.lam32 : x.lam;
+    const void* mu = refined32 ? x.mu32 : x.mu;
+    const void* epsp = refined32 ? x.eps_plastic32 : x.eps_plastic;
+    void* eps = refined32 ? x.eps32 : x.eps;
+    void* ws = refined32 ? x.workspace32 : x.workspace;
+    const size_t wsb = refined32 ? x.workspace32_bytes : x.workspace_bytes;
+    if (!lam || !mu || !eps || !x.eps_bar || !ws) {
+      mgk_set_error("batch member with a null field, macro strain or workspace", __FILE__, __LINE__);
+      return MGK_ERR_ARG;
+    }
+    if (wsb < pl.total) {
+      mgk_set_error("batch member workspace too small", __FILE__, __LINE__);
+      return MGK_ERR_WORKSPACE;
+    }
+    a[b] = LsMemArgs<T>{(const T*)lam, (const T*)mu, x.lam0, x.mu0, x.eps_bar, (const T*)epsp, (T*)eps,
+                        refined32 ? nullptr : (T*)x.sig, (char*)ws};
+  }
+  return MGK_OK;
+}
+}  // namespace
+
+extern "C" int mgk_ls_solve_batch(int ndim, const int64_t* shape, int precision, int nb, const mgk_ls_member* members,
+                                  int warm, int max_iter, double tol, void* stream, mgk_ls_info* info) {
+  if (!members || !shape || nb < 1 || nb > MGK_MAXB || max_iter < 1 || (precision != 0 && precision != 1)) {
+    mgk_set_error("null argument, nb outside 1..MGK_LS_MAX_BATCH, max_iter < 1 or precision not 0/1", __FILE__,
+                  __LINE__);
+    return MGK_ERR_ARG;
+  }
+  LsPlan pl;
+  int rc = precision == 0 ? ls_plan<double>(ndim, shape, &pl) : ls_plan<float>(ndim, shape, &pl);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the LS solver", __FILE__, __LINE__);
+    return rc;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  warm = warm ? 1 : 0;
+  if (precision == 0) {
+    LsMemArgs<double> a[MGK_MAXB];
+    rc = ls_batch_args<double>(pl, nb, members, 0, a);
+    if (rc != MGK_OK) return rc;
+    return ndim == 3 ? ls_solve_batch_t<double, 3>(pl, nb, a, warm, max_iter, tol, s, info, true)
+                     : ls_solve_batch_t<double, 2>(pl, nb, a, warm, max_iter, tol, s, info, true);
+  }
+  LsMemArgs<float> a[MGK_MAXB];
+  rc = ls_batch_args<float>(pl, nb, members, 0, a);
+  if (rc != MGK_OK) return rc;
+  return ndim == 3 ? ls_solve_batch_t<float, 3>(pl, nb, a, warm, max_iter, tol, s, info, true)
+                   : ls_solve_batch_t<float, 2>(pl, nb, a, warm, max_iter, tol, s, info, true);
+}
+
+extern "C" int mgk_ls_solve_refined_batch(int ndim, const int64_t* shape, int nb, const mgk_ls_member* members,
+                                          int warm, int max_iter, double tol, double tol32, void* stream,
+                                          mgk_ls_info* info, int* iters32) {
+  if (!members || !shape || nb < 1 || nb > MGK_MAXB || max_iter < 1) {
+    mgk_set_error("null argument, nb outside 1..MGK_LS_MAX_BATCH or max_iter < 1", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  for (int b = 0; b < nb; ++b) {
+    if ((members[b].eps_plastic32 == nullptr) != (members[b].eps_plastic == nullptr) || !members[b].eps32 ||
+        !members[b].lam32 || !members[b].mu32) {
+      mgk_set_error("batch member without its FP32 fields, or with only one eps_plastic copy", __FILE__, __LINE__);
+      return MGK_ERR_ARG;
+    }
+  }
+  LsPlan p32, p64;
+  int rc = ls_plan<float>(ndim, shape, &p32);
+  if (rc == MGK_OK) rc = ls_plan<double>(ndim, shape, &p64);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the LS solver", __FILE__, __LINE__);
+    return rc;
+  }
+  LsMemArgs<float> a32[MGK_MAXB];
+  LsMemArgs<double> a64[MGK_MAXB];
+  rc = ls_batch_args<float>(p32, nb, members, 1, a32);
+  if (rc == MGK_OK) rc = ls_batch_args<double>(p64, nb, members, 0, a64);
+  if (rc != MGK_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  warm = warm ? 1 : 0;
+  // stage 1 (FP32 to tol32, no stress), widen, stage 2 (FP64 from the widened
+  // field, warm = 2), as mgk_ls_solve_refined does for one member
+  mgk_ls_info i32[MGK_MAXB] = {};
+  rc = ndim == 3 ? ls_solve_batch_t<float, 3>(p32, nb, a32, warm, max_iter, tol32, s, i32, false)
+                 : ls_solve_batch_t<float, 2>(p32, nb, a32, warm, max_iter, tol32, s, i32, false);
+  for (int b = 0; b < nb; ++b)
+    if (iters32) iters32[b] = i32[b].iters;
+  if (rc < 0 && rc != MGK_ERR_NONFINITE) {  // the batch itself failed
+    if (info)
+      for (int b = 0; b < nb; ++b) info[b] = i32[b];
+    return rc;
+  }
+  // members whose FP32 stage failed keep its record (as the single call
+  // returns it); the others continue in FP64
+  LsMemArgs<double> a2[MGK_MAXB];
+  int idx[MGK_MAXB], n2 = 0, rc1 = MGK_OK;
+  for (int b = 0; b < nb; ++b) {
+    if (i32[b].status < 0) {
+      if (info) info[b] = i32[b];
+      if (rc1 == MGK_OK) rc1 = i32[b].status;
+    } else {
+      idx[n2] = b;
+      a2[n2++] = a64[b];
+    }
+  }
+  if (n2 == 0) return rc1;
+  const size_t n = (size_t)p64.C * (size_t)p64.NV;
+  const int nw = (int)std::min<size_t>((n + MGK_NT - 1) / MGK_NT, 8192);
+  for (int k = 0; k < n2; ++k) k_widen<<<nw, MGK_NT, 0, s>>>(a32[idx[k]].eps, a2[k].eps, n);
+  MGK_HIP(hipGetLastError());
+  mgk_ls_info i64[MGK_MAXB] = {};
+  const int rc2 = ndim == 3 ? ls_solve_batch_t<double, 3>(p64, n2, a2, 2, max_iter, tol, s, i64, true)
+                            : ls_solve_batch_t<double, 2>(p64, n2, a2, 2, max_iter, tol, s, i64, true);
+  if (info)
+    for (int k = 0; k < n2; ++k) info[idx[k]] = i64[k];
+  if (rc2 < 0 && rc2 != MGK_ERR_NONFINITE) return rc2;
+  if (rc1 < 0) return rc1;
+  return rc2;
 }
 
 extern "C" int mgk_ls_solve_landau(const int64_t* shape, int precision, const void* lam, const void* mu, double lam0,
@@ -2037,9 +2430,9 @@ extern "C" int mgk_heat_step(int ndim, const int64_t* shape, double* Tf, const d
   const int npb = (g.npencil + (1 << pl.logPP) - 1) >> pl.logPP;
   const dim3 gx(nqx, pl.P), gy(nqy, pl.nxh, 1);
   k_ls_xfwd_real<double, MV><<<gx, MGK_NT, pl.lds_x, s>>>(g, Tf, spec, st);
-  if (pl.P > 1) launch_mid<double, MV>(pl, gy, s, g.ply, g.twy, spec, 0, nullptr);
+  if (pl.P > 1) launch_mid<double, MV>(pl, gy, s, g.ply, g.twy, mid_one(spec, (const int*)nullptr, (int)gy.z), 0);
   k_heat_pencil<double, MV><<<npb, MGK_NT, pl.lds_q, s>>>(g, spec, hp);
-  if (pl.P > 1) launch_mid<double, MV>(pl, gy, s, g.ply, g.twy, spec, 1, nullptr);
+  if (pl.P > 1) launch_mid<double, MV>(pl, gy, s, g.ply, g.twy, mid_one(spec, (const int*)nullptr, (int)gy.z), 1);
   k_heat_xinv<double, MV><<<gx, MGK_NT, pl.lds_x, s>>>(g, spec, Tf);
   if (thermostat == 2) {
     k_comp_sums<double><<<pl.nblk_red, MGK_NT, 0, s>>>(Tf, 1, (size_t)pl.NV, part);

@@ -1024,6 +1024,25 @@ class DeviceKmc:
             L.unit[i] = u[i]
         return L
 
+    def _take_native_events(self, evs, rc, it32, on_event=None):
+        """Record events returned by the native loop (history, LS iteration
+        totals, on_event callbacks) for this simulation."""
+        d = self.ndim
+        for i, e in enumerate(evs):
+            coords = tuple(int(c) for c in self.kmc.voxel_coords(e.v))
+            ev = coords + (int(e.m), float(e.time), bool(e.is_instab))
+            self.kmc_event_history.append(ev)
+            self.ls_iters_total += e.ls_iters + e.ls_iters32
+            if on_event is not None:
+                over = rc == _lib.MGK_KMC_RUN_SEQ and i == len(evs) - 1
+                on_event(ev, np.array(e.eps_macro[: d * d]).reshape(d, d),
+                         np.array(e.sig_mean[: d * d]).reshape(d, d), over)
+        if len(evs):
+            self._solved_once = True
+            self.ls_info = self.ls.info
+            if self.precision == MIXED:
+                self.ls.iters32 = it32
+
     def _run_native(self, remaining, unit, max_events, max_seq, seq, on_event=None):
         d = self.ndim
         lib = _lib.load()
@@ -1046,21 +1065,8 @@ class DeviceKmc:
                                      ctypes.byref(rem), chunk, ms, ctypes.byref(sq), evs, ctypes.byref(nb),
                                      ctypes.byref(self.ls.info), ctypes.byref(it32),
                                      ctypes.c_void_p(dev.stream_ptr()))
-                for i, e in enumerate(evs[: nb.value]):
-                    coords = tuple(int(c) for c in self.kmc.voxel_coords(e.v))
-                    ev = coords + (int(e.m), float(e.time), bool(e.is_instab))
-                    self.kmc_event_history.append(ev)
-                    self.ls_iters_total += e.ls_iters + e.ls_iters32
-                    if on_event is not None:
-                        over = rc == _lib.MGK_KMC_RUN_SEQ and i == nb.value - 1
-                        on_event(ev, np.array(e.eps_macro[: d * d]).reshape(d, d),
-                                 np.array(e.sig_mean[: d * d]).reshape(d, d), over)
+                self._take_native_events(evs[: nb.value], rc, it32.value, on_event)
                 n_total += nb.value
-                if nb.value:
-                    self._solved_once = True
-                    self.ls_info = self.ls.info
-                    if self.precision == MIXED:
-                        self.ls.iters32 = it32.value
                 if rc == _lib.MGK_KMC_RUN_UNSTABLE:
                     sm = np.array(self.ls.info.sig_mean[: d * d]).reshape(d, d)
                     raise ValueError(f"Unstable stress in small-strain elastic_run: {sm / 1e9} GPa")
@@ -1098,3 +1104,80 @@ class DeviceKmc:
                 t += x.numel() * x.element_size()
         return t
 
+
+def kmc_events_batch(sims, remaining, units, max_events, max_seq=None, seqs=None, stream=None):
+    """Events of several simulations of one grid shape and LS configuration in
+    lockstep (mgk_kmc_run_batch, at most MGK_LS_MAX_BATCH): per step one
+    synchronisation for all rate totals, one for all selections and one
+    batched LS re-solve.  Member i runs until its own stop (no event within
+    remaining[i], max_events[i] events, or more than max_seq consecutive
+    unstable-mode events); its events, fields, random streams, time and macro
+    strain come out bit-identical to sims[i].kmc_events(remaining[i],
+    units[i], max_events[i], max_seq, seqs[i]).  Returns one (n events,
+    remaining time, reason, seq, error) tuple per member: reason "time",
+    "count" or "seq", or "error" with the exception the single-simulation
+    call would raise (the others are unaffected by it)."""
+    n = len(sims)
+    if not 1 <= n <= _lib.MGK_LS_MAX_BATCH:
+        raise ValueError(f"batch of {n} simulations (1..{_lib.MGK_LS_MAX_BATCH})")
+    for sm in sims:
+        if not sm._native_ok():
+            raise ValueError("batched events need simulations that run in the native loop "
+                             "(linear law, small strain, KMC mode, gaussian barriers, planar catalog)")
+    lib = _lib.load()
+    seqs = [0] * n if seqs is None else list(seqs)
+    cap = max(int(m) for m in max_events)
+    loops = (_lib.MgkKmcLoop * n)()
+    rngs = (_lib.MgkMtState * n)()
+    sels = (_lib.MgkMtState * n)()
+    t = (ctypes.c_double * n)()
+    em = (ctypes.c_double * (9 * n))()
+    rem = (ctypes.c_double * n)()
+    mx = (ctypes.c_int * n)()
+    sq = (ctypes.c_int * n)()
+    for i, sm in enumerate(sims):
+        loops[i] = sm._loop_struct(units[i])
+        rngs[i] = _lib.mt_from(sm.rng)
+        sels[i] = _lib.mt_from(sm.selection_stream)
+        t[i] = sm.time
+        e = sm.eps_macro.reshape(-1)
+        for k in range(e.size):
+            em[9 * i + k] = float(e[k])
+        rem[i] = float(remaining[i])
+        mx[i] = int(max_events[i])
+        sq[i] = int(seqs[i])
+    evs = (_lib.MgkKmcEvent * (n * cap))()
+    nev = (ctypes.c_int * n)()
+    status = (ctypes.c_int * n)()
+    infos = (_lib.MgkLsInfo * n)()
+    it32 = (ctypes.c_int * n)()
+    ms = 2 ** 31 - 1 if max_seq is None else int(min(max_seq, 2 ** 31 - 1))
+    s = dev.stream_ptr() if stream is None else stream
+    rc = lib.mgk_kmc_run_batch(n, loops, rngs, sels, t, em, rem, mx, ms, sq, evs, cap, nev, status, infos, it32,
+                               ctypes.c_void_p(s))
+    out = []
+    for i, sm in enumerate(sims):
+        d = sm.ndim
+        err = None
+        _lib.mt_to(sm.rng, rngs[i])
+        _lib.mt_to(sm.selection_stream, sels[i])
+        sm.time = t[i]
+        sm.eps_macro[...] = np.array(em[9 * i: 9 * i + d * d]).reshape(d, d)
+        if nev[i] or status[i] == _lib.MGK_KMC_RUN_UNSTABLE:
+            ctypes.memmove(ctypes.byref(sm.ls.info), ctypes.byref(infos[i]), ctypes.sizeof(infos[i]))
+        sm._take_native_events(evs[i * cap: i * cap + nev[i]], status[i], int(it32[i]))
+        st = status[i]
+        if st == _lib.MGK_KMC_RUN_UNSTABLE:
+            smn = np.array(sm.ls.info.sig_mean[: d * d]).reshape(d, d)
+            err = ValueError(f"Unstable stress in small-strain elastic_run: {smn / 1e9} GPa")
+            reason = "error"
+        elif st < 0:
+            err = RuntimeError(f"mgk_kmc_run_batch member {i} failed with status {st}: {_lib.last_error()}")
+            reason = "error"
+        else:
+            reason = {_lib.MGK_KMC_RUN_TIME: "time", _lib.MGK_KMC_RUN_COUNT: "count",
+                      _lib.MGK_KMC_RUN_SEQ: "seq"}[st]
+        out.append((int(nev[i]), float(rem[i]), reason, int(sq[i]), err))
+    if rc < 0:
+        _lib.check(rc, "mgk_kmc_run_batch")
+    return out

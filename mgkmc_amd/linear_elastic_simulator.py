@@ -135,6 +135,69 @@ class LsSolver:
         return self.info
 
 
+
+def solve_batch(solvers, eps_bars, eps_plastics=None, warm=False, max_iter=200, tol=1e-6, stream=None,
+                eps_plastics32=None, tol32=None):
+    """Solve several LsSolvers of one grid shape and precision together: one
+    launch per sweep for all of them (mgk_ls_solve_batch /
+    mgk_ls_solve_refined_batch, at most MGK_LS_MAX_BATCH).  Each solver's
+    fields, info and iteration counts come out bit-identical to its own
+    LsSolver.solve with the same arguments.  Returns the list of info structs
+    (copies)."""
+    n = len(solvers)
+    if not 1 <= n <= _lib.MGK_LS_MAX_BATCH:
+        raise ValueError(f"batch of {n} solvers (1..{_lib.MGK_LS_MAX_BATCH})")
+    s0 = solvers[0]
+    if any(x.shape != s0.shape or x.precision != s0.precision for x in solvers):
+        raise ValueError("a batch needs one grid shape and one precision")
+    eps_plastics = [None] * n if eps_plastics is None else list(eps_plastics)
+    eps_plastics32 = [None] * n if eps_plastics32 is None else list(eps_plastics32)
+    mem = (_lib.MgkLsMember * n)()
+    keep = []
+    for i, (x, eb, ep) in enumerate(zip(solvers, eps_bars, eps_plastics)):
+        _, ebp = _lib.host_dbl(np.asarray(eb, dtype=np.float64).reshape(x.d, x.d))
+        keep.append(ebp)
+        if ep is not None:
+            if ep.dtype != x.eps.dtype or tuple(ep.shape) != tuple(x.eps.shape):
+                raise ValueError("eps_plastic must be a device tensor of shape (C, ...) and solver dtype")
+            ep = ep.contiguous()
+            keep.append(ep)
+        m = mem[i]
+        m.lam, m.mu, m.lam0, m.mu0 = _lib.dptr(x.lam), _lib.dptr(x.mu), x.lam0, x.mu0
+        m.eps_bar = ebp
+        m.eps_plastic, m.eps, m.sig = _lib.dptr(ep), _lib.dptr(x.eps), _lib.dptr(x.sig)
+        m.workspace, m.workspace_bytes = _lib.dptr(x.ws), x.ws.numel()
+        if x.precision == MIXED:
+            ep32 = eps_plastics32[i]
+            if ep is not None and ep32 is None:
+                ep32 = ep.to(dev.torch.float32)
+            keep.append(ep32)
+            m.lam32, m.mu32, m.eps_plastic32 = _lib.dptr(x.lam32), _lib.dptr(x.mu32), _lib.dptr(ep32)
+            m.eps32, m.workspace32, m.workspace32_bytes = _lib.dptr(x.eps32), _lib.dptr(x.ws32), x.ws32.numel()
+    s = dev.stream_ptr() if stream is None else stream
+    infos = (_lib.MgkLsInfo * n)()
+    lib = _lib.load()
+    if s0.precision == MIXED:
+        it32 = (ctypes.c_int * n)()
+        t32 = max(float(tol), 1e-6) if tol32 is None else float(tol32)
+        rc = lib.mgk_ls_solve_refined_batch(s0.d, s0._shape_c, n, mem, 1 if warm else 0, int(max_iter), float(tol),
+                                            t32, ctypes.c_void_p(s), infos, it32)
+        _lib.check(rc, "mgk_ls_solve_refined_batch")
+        for x, v in zip(solvers, it32):
+            x.iters32 = int(v)
+    else:
+        rc = lib.mgk_ls_solve_batch(s0.d, s0._shape_c, s0.precision, n, mem, 1 if warm else 0, int(max_iter),
+                                    float(tol), ctypes.c_void_p(s), infos)
+        _lib.check(rc, "mgk_ls_solve_batch")
+    out = []
+    for x, inf in zip(solvers, infos):
+        ctypes.memmove(ctypes.byref(x.info), ctypes.byref(inf), ctypes.sizeof(inf))
+        c = _lib.MgkLsInfo()
+        ctypes.memmove(ctypes.byref(c), ctypes.byref(inf), ctypes.sizeof(inf))
+        out.append(c)
+    return out
+
+
 _cache = {}
 
 

@@ -1,23 +1,32 @@
-"""Serving loop: many KMC simulations resident on one GPU, continuously admitted.
+"""Serving loop: many KMC simulations resident on one GPU, continuously batched.
 
 A request is one simulation (2D or 3D grid, temperature, seed) that runs a
 fixed number of KMC events, each with its LS re-equilibration in the FP32 +
 FP64 refinement form.  The server
 
   * admits arrived requests in arrival order while the estimated HBM footprint
-    of the admitted set stays within the budget and a slot is free (requests
-    that arrive while others run are admitted as soon as slots and memory free
-    up: continuous batching, not a queue in front of one simulation);
-  * runs every admitted simulation on its own worker thread and HIP stream, so
-    the kernels of different simulations overlap on the device (small grids
-    are launch-latency bound and leave most CUs idle on their own);
+    of the admitted set stays within the budget and fewer than max_active
+    simulations are resident;
+  * groups the resident simulations by grid shape: each shape in use has one
+    runner thread and HIP stream, which steps its simulations together in the
+    batched native event loop (mgk_kmc_run_batch: one synchronisation per
+    step for all rate totals, one for all selections, one LS solve with one
+    launch per sweep for every member that had an event), a chunk of events
+    at a time;
+  * lets requests join and leave a running group between chunks: a request
+    that arrives while others of its shape run is built on the group's stream
+    and joins at the next chunk boundary, and a finished one leaves at the
+    end of its chunk (continuous batching, not a queue in front of one
+    simulation); groups of different shapes overlap on the device;
   * keeps each simulation's fields, modes and KMC state in HBM for its whole
     life; its random streams are private (numpy RandomState for waiting times,
-    selection and redraws; Philox on the device for the initial modes), so a
-    request's result does not depend on what else is running.
+    selection and redraws; Philox on the device for the initial modes), and a
+    member of a batch runs the kernels of a simulation alone on its own
+    buffers, so a request's result is bit-identical alone and in the mix.
 
 There is no counterpart in the reference (its simulators run one at a time).
 """
+import queue
 import threading
 import time
 from collections import deque
@@ -172,52 +181,176 @@ def run_request(req):
             sim.kmc_events(1e30, unit, req.events)
         except (ValueError, FloatingPointError) as e:  # the simulators' solver-failure errors
             err = str(e)
-        iters = sim.ls_iters_total
-        eps_p_sum = float(sim.kmc.eps_p.to(torch.float64).sum())
-        res = SimResult(req.rid, tuple(req.shape), list(sim.kmc_event_history), sim.eps_macro.copy(),
-                        sim.sig_mean().copy() if sim.ls_info is not None else None, sim.time, eps_p_sum, iters, fp)
-        if err is not None:
-            res.extra["error"] = err
+        res = _result(req, sim, fp, err)
         s.synchronize()
         del sim
     return res
 
 
-class SimServer:
-    """Admission by HBM footprint, one worker thread + HIP stream per slot."""
+def _result(req, sim, fp, err=None):
+    torch = dev.torch
+    eps_p_sum = float(sim.kmc.eps_p.to(torch.float64).sum())
+    res = SimResult(req.rid, tuple(req.shape), list(sim.kmc_event_history), sim.eps_macro.copy(),
+                    sim.sig_mean().copy() if sim.ls_info is not None else None, sim.time, eps_p_sum,
+                    sim.ls_iters_total, fp)
+    if err is not None:
+        res.extra["error"] = err
+    return res
 
-    def __init__(self, hbm_budget=None, max_active=8):
+
+class _Member:
+    __slots__ = ("req", "sim", "fp", "left", "rem", "seq", "unit")
+
+    def __init__(self, req, sim, fp):
+        self.req, self.sim, self.fp = req, sim, fp
+        self.left = req.events
+        self.rem = 1e30
+        self.seq = 0
+        d = sim.ndim
+        self.unit = np.zeros((d, d))
+        self.unit[0, 0] = 1.0
+
+
+class _Group:
+    """Resident simulations of one grid shape and their runner's inbox."""
+
+    def __init__(self):
+        self.lock = threading.Lock()
+        self.inbox = deque()
+        self.running = False
+
+
+class SimServer:
+    """Admission by HBM footprint; continuous batching per grid shape (one
+    runner thread + HIP stream per shape in use, at most MGK_LS_MAX_BATCH
+    members stepping together, chunk_events events per member between
+    joins).  batch=False runs every request alone on its own worker thread
+    and stream instead (no batching; kept for comparison)."""
+
+    def __init__(self, hbm_budget=None, max_active=16, batch=True, chunk_events=10):
         dev.require_gpu()
         if hbm_budget is None:
             free, _total = dev.torch.cuda.mem_get_info()
             hbm_budget = int(0.85 * free)
         self.hbm_budget = int(hbm_budget)
         self.max_active = int(max_active)
+        self.batch = bool(batch)
+        self.chunk_events = int(chunk_events)
         self.peak_used = 0
         self.peak_active = 0
+        self.peak_batch = 0
+        self.batched_steps = 0
 
-    def serve(self, requests, time_scale=1.0, on_done=None):
-        """Replay `requests` against the wall clock (arrival * time_scale) and
-        return their SimResults in completion order."""
-        pending = deque(sorted(requests, key=lambda r: r.arrival))
-        queue = deque()
+    # ----- batched runner ------------------------------------------------------
+    def _runner(self, grp, done_q):
+        from .kmc_engine import kmc_events_batch
+        torch = dev.torch
+        s = torch.cuda.Stream()  # default priority
+        members = []
+        with torch.cuda.stream(s):
+            while True:
+                new = []
+                with grp.lock:
+                    while grp.inbox and len(members) + len(new) < _lib.MGK_LS_MAX_BATCH:
+                        new.append(grp.inbox.popleft())
+                    if not members and not new:
+                        grp.running = False
+                        return
+                for req, fp in new:
+                    try:
+                        sim = build_simulation(req)
+                        sim.update_stresses()
+                        members.append(_Member(req, sim, fp))
+                    except (ValueError, FloatingPointError) as e:
+                        done_q.put((req, fp, SimResult(req.rid, tuple(req.shape), [], None, None, 0.0, 0.0, 0, fp,
+                                                       extra={"error": str(e)})))
+                if not members:
+                    continue
+                self.peak_batch = max(self.peak_batch, len(members))
+                k = [min(self.chunk_events, m.left) for m in members]
+                outs = kmc_events_batch([m.sim for m in members], [m.rem for m in members],
+                                        [m.unit for m in members], k, seqs=[m.seq for m in members])
+                self.batched_steps += max(k)
+                keep = []
+                for m, (n, rem, reason, seq, err) in zip(members, outs):
+                    m.left -= n
+                    m.rem, m.seq = rem, seq
+                    if reason == "count" and m.left > 0:
+                        keep.append(m)
+                        continue
+                    res = _result(m.req, m.sim, m.fp, None if err is None else str(err))
+                    s.synchronize()
+                    m.sim = None
+                    done_q.put((m.req, m.fp, res))
+                members = keep
+
+    def _serve_batched(self, pending, t0, on_done, results, time_scale):
+        done_q = queue.Queue()
+        groups = {}
+        queue_ = deque()
+        used = 0
+        active = 0
+        while pending or queue_ or active:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival * time_scale <= now:
+                queue_.append(pending.popleft())
+            while queue_ and active < self.max_active:
+                fp = footprint_bytes(queue_[0])
+                if used + fp > self.hbm_budget:
+                    break
+                r = queue_.popleft()
+                used += fp
+                active += 1
+                r._t_admit = time.perf_counter() - t0
+                g = groups.setdefault(tuple(r.shape), _Group())
+                with g.lock:
+                    g.inbox.append((r, fp))
+                    start = not g.running
+                    g.running = True
+                if start:
+                    threading.Thread(target=self._runner, args=(g, done_q), daemon=True).start()
+            self.peak_used = max(self.peak_used, used)
+            self.peak_active = max(self.peak_active, active)
+            timeout = None
+            if pending:
+                timeout = max(0.0, pending[0].arrival * time_scale - (time.perf_counter() - t0))
+            if active:
+                try:
+                    item = done_q.get(timeout=timeout)
+                except queue.Empty:
+                    continue
+                while True:
+                    r, fp, res = item
+                    used -= fp
+                    active -= 1
+                    res.t_arrival = r.arrival * time_scale
+                    res.t_admit = r._t_admit
+                    res.t_done = time.perf_counter() - t0
+                    results.append(res)
+                    if on_done is not None:
+                        on_done(res)
+                    try:
+                        item = done_q.get_nowait()
+                    except queue.Empty:
+                        break
+            elif timeout:
+                time.sleep(timeout)
+
+    # ----- one worker per request (no batching) --------------------------------
+    def _serve_unbatched(self, pending, t0, on_done, results, time_scale):
+        queue_ = deque()
         active = {}
         used = 0
-        results = []
-        for r in requests:
-            if footprint_bytes(r) > self.hbm_budget:
-                raise ValueError(f"request {r.rid} ({r.shape}) needs more than the HBM budget")
-        t0 = time.perf_counter()
         with ThreadPoolExecutor(max_workers=self.max_active) as pool:
-            while pending or queue or active:
+            while pending or queue_ or active:
                 now = time.perf_counter() - t0
                 while pending and pending[0].arrival * time_scale <= now:
-                    queue.append(pending.popleft())
-                while queue and len(active) < self.max_active:
-                    fp = footprint_bytes(queue[0])
+                    queue_.append(pending.popleft())
+                while queue_ and len(active) < self.max_active:
+                    fp = footprint_bytes(queue_[0])
                     if used + fp > self.hbm_budget:
                         break
-                    r = queue.popleft()
+                    r = queue_.popleft()
                     used += fp
                     fut = pool.submit(run_request, r)
                     active[fut] = (r, fp, time.perf_counter() - t0)
@@ -240,6 +373,20 @@ class SimServer:
                             on_done(res)
                 elif timeout:
                     time.sleep(timeout)
+
+    def serve(self, requests, time_scale=1.0, on_done=None):
+        """Replay `requests` against the wall clock (arrival * time_scale) and
+        return their SimResults in completion order."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        for r in requests:
+            if footprint_bytes(r) > self.hbm_budget:
+                raise ValueError(f"request {r.rid} ({r.shape}) needs more than the HBM budget")
+        results = []
+        t0 = time.perf_counter()
+        if self.batch:
+            self._serve_batched(pending, t0, on_done, results, time_scale)
+        else:
+            self._serve_unbatched(pending, t0, on_done, results, time_scale)
         self.wall = time.perf_counter() - t0
         return results
 
@@ -264,10 +411,12 @@ def main(argv=None):
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--events", type=int, default=100)
-    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--slots", type=int, default=16)
+    ap.add_argument("--no-batch", action="store_true", help="one worker per request, no batching")
+    ap.add_argument("--chunk", type=int, default=10, help="events per member between joins")
     a = ap.parse_args(argv)
     reqs = request_stream(a.requests, a.rate, a.seed, a.events)
-    srv = SimServer(max_active=a.slots)
+    srv = SimServer(max_active=a.slots, batch=not a.no_batch, chunk_events=a.chunk)
     done = [0]
 
     def progress(res):
@@ -277,7 +426,8 @@ def main(argv=None):
     res = srv.serve(reqs, on_done=progress)
     out = summarize(res, srv.wall)
     out.update(slots=a.slots, rate=a.rate, events_per_request=a.events, peak_active=srv.peak_active,
-               peak_hbm_bytes=srv.peak_used, hbm_budget=srv.hbm_budget)
+               peak_hbm_bytes=srv.peak_used, hbm_budget=srv.hbm_budget, batched=srv.batch,
+               peak_batch=srv.peak_batch, chunk_events=srv.chunk_events)
     print(json.dumps(out), flush=True)
 
 

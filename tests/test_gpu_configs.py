@@ -234,7 +234,7 @@ def test_cfg4_serving_stream_properties(gpu):
     # stays within the budget, and sampled results are those of a solo run
     from mgkmc_amd.serving import SimServer, request_stream, run_request
     reqs = request_stream(512, 4.0, 4, 100)
-    srv = SimServer(max_active=8)
+    srv = SimServer()
     res = srv.serve(reqs, time_scale=0.125)
     by_id = {r.rid: r for r in res}
     lat = np.array([r.latency for r in res])

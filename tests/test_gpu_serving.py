@@ -16,14 +16,17 @@ def _same(a, b):
 
 
 def test_solo_vs_mix_identical(gpu):
+    # two shapes only, so requests of one shape share a batch and join it
+    # while it runs (chunks of 4 events); every result equals a solo run
     from mgkmc_amd.serving import SimServer, request_stream
-    reqs = request_stream(12, rate=40.0, seed=7, events=15, sizes=SIZES)
+    reqs = request_stream(12, rate=40.0, seed=7, events=15, sizes=((32, 24, 20), (64, 48)))
     print("[serving] mix of", [r.shape for r in reqs], flush=True)
-    srv = SimServer(max_active=4)
+    srv = SimServer(max_active=8, chunk_events=4)
     mix = {r.rid: r for r in srv.serve(reqs)}
-    assert len(mix) == len(reqs) and srv.peak_active >= 2
+    print(f"[serving] peak active {srv.peak_active}, peak batch {srv.peak_batch}", flush=True)
+    assert len(mix) == len(reqs) and srv.peak_active >= 2 and srv.peak_batch >= 2
     for q in reqs[:6]:
-        solo = SimServer(max_active=1).serve([q])[0]
+        solo = SimServer(max_active=1, batch=False).serve([q])[0]
         m = mix[q.rid]
         print(f"[serving] rid={q.rid} shape={q.shape} events={len(m.events)} solo==mix: {_same(solo, m)}",
               flush=True)
@@ -42,3 +45,13 @@ def test_admission_by_footprint(gpu):
     res = srv.serve(reqs)
     assert len(res) == 6
     assert srv.peak_used <= srv.hbm_budget and srv.peak_active == 2
+
+
+def test_unbatched_mode_matches_batched(gpu):
+    from mgkmc_amd.serving import SimServer, request_stream
+    reqs = request_stream(6, rate=40.0, seed=11, events=8, sizes=SIZES)
+    a = {r.rid: r for r in SimServer(max_active=6).serve(reqs)}
+    b = {r.rid: r for r in SimServer(max_active=6, batch=False).serve(reqs)}
+    assert set(a) == set(b)
+    for k in a:
+        assert _same(a[k], b[k])
