@@ -11,6 +11,7 @@
 //  * Half spectra (r2c along x) are complex, f[c][kx][p][q] with
 //    kx in [0, nx/2].  y and z are full complex axes.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
@@ -268,17 +269,46 @@ __device__ void tile_fft(cplx<T>* __restrict__ buf, const Tile& tl, const FftPla
 }
 
 // ---------------------------------------------------------------------------
-// Compile-time specialised tile FFT for power-of-two lengths.  Same Stockham
-// passes and data layout as above with row stride B and batch stride N*B, but
-// every extent is a template constant, so the index arithmetic folds into
-// shifts/immediates and the staged values stay in few registers.
+// Compile-time specialised tile FFT for lengths 2^k and 3 * 2^k.  Same
+// Stockham passes and data layout as above with row stride B and batch
+// stride N*B, but every extent is a template constant, so the index
+// arithmetic folds into shifts, immediates and constant divisions and the
+// staged values stay in few registers.
 // ---------------------------------------------------------------------------
 constexpr int ct_log2(int v) { return v <= 1 ? 0 : 1 + ct_log2(v / 2); }
 
-// radix of the pass that starts at sub-transform length L (8s, then 4/2)
+// radix of the pass that starts at sub-transform length L: a radix-3 pass
+// first for lengths 3 * 2^k, then 8s, then 4/2
 constexpr int ct_radix(int N, int L) {
+  if ((N / L) % 3 == 0) return 3;
   const int k = ct_log2(N / L);
   return (k % 3 == 0) ? 8 : ((k % 3 == 2) ? 4 : (k >= 4 ? 4 : 2));
+}
+
+// Axis lengths with compile-time sweeps: 64..512 in powers of two and the
+// mixed lengths 96, 192, 384 (3 * 2^k; one radix-3 pass).
+constexpr bool ct_len(int n) {
+  return n == 64 || n == 96 || n == 128 || n == 192 || n == 256 || n == 384 || n == 512;
+}
+
+// columns of a compile-time tile of at most `tile` points along an axis of
+// length n: the largest power of two c with n * c <= tile
+constexpr int ct_cols(int tile, int n) { return (2 * n <= tile) ? 2 * ct_cols(tile / 2, n) : 1; }
+
+// Calls f(std::integral_constant<int, n>{}) for a length with compile-time
+// sweeps; false (and no call) for any other length.
+template <typename F>
+bool ct_dispatch(int n, F&& f) {
+  switch (n) {
+    case 64: f(std::integral_constant<int, 64>{}); return true;
+    case 96: f(std::integral_constant<int, 96>{}); return true;
+    case 128: f(std::integral_constant<int, 128>{}); return true;
+    case 192: f(std::integral_constant<int, 192>{}); return true;
+    case 256: f(std::integral_constant<int, 256>{}); return true;
+    case 384: f(std::integral_constant<int, 384>{}); return true;
+    case 512: f(std::integral_constant<int, 512>{}); return true;
+    default: return false;
+  }
 }
 
 // points per compile-time tile: 32 KB of LDS in FP32, 32 KB in FP64 (so four

@@ -1206,9 +1206,10 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl, bool allow_ct = true) {
   while (PP > 1 && (size_t)(pl->Q + pl->C * pl->Q * PP) * cb > lim) PP /= 2;
   pl->logPP = mgk_ilog2(PP);
   pl->lds_q = (size_t)(pl->Q + pl->C * pl->Q * PP) * cb;
-  // compile-time specialised sweeps (power-of-two lengths 64..512, 4096-point tiles)
+  // compile-time specialised sweeps (lengths of ct_len: 64..512 in powers of two
+  // and 96, 192, 384; tiles of up to 4096 points, power-of-two column counts)
   const int CTT = ct_tile<T>();
-  auto ct_ok = [](int n) { return n == 64 || n == 128 || n == 256 || n == 512; };
+  auto ct_ok = [](int n) { return ct_len(n); };
   pl->ctx = pl->cty = pl->ctq = 0;
   const int XTT = x_tile<T>();
   if (allow_ct && ct_ok(pl->nx) && pl->Q >= XTT / pl->nx && (size_t)(pl->nx + XTT) * cb <= lim) {
@@ -1252,12 +1253,11 @@ template <typename T, int MV>
 void launch_mid(const LsPlan& pl, dim3 gy, hipStream_t stream, const FftPlan& ply, const cplx<T>* twy,
                 const MidPtrs<T>& mp, int inv) {
   if (pl.cty) {
-    switch (pl.P) {
-      case 64: k_fft_mid<T, ct_tile<T>() / MGK_NT, 64, ct_tile<T>() / 64, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mp, inv); break;
-      case 128: k_fft_mid<T, ct_tile<T>() / MGK_NT, 128, ct_tile<T>() / 128, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mp, inv); break;
-      case 256: k_fft_mid<T, ct_tile<T>() / MGK_NT, 256, ct_tile<T>() / 256, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mp, inv); break;
-      default: k_fft_mid<T, ct_tile<T>() / MGK_NT, 512, ct_tile<T>() / 512, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mp, inv); break;
-    }
+    ct_dispatch(pl.P, [&](auto nn) {
+      constexpr int NN = decltype(nn)::value, BB = ct_cols(ct_tile<T>(), NN);
+      k_fft_mid<T, ct_tile<T>() / MGK_NT, NN, BB, MID_NT><<<gy, MID_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh,
+                                                                                          pl.logBy, ply, twy, mp, inv);
+    });
   } else {
     k_fft_mid<T, MV><<<gy, MGK_NT, pl.lds_y, stream>>>(pl.P, pl.Q, pl.nxh, pl.logBy, ply, twy, mp, inv);
   }
@@ -1459,12 +1459,10 @@ void launch_tau(const LsPlan& pl, const LsGeom<T>& g, dim3 gx, hipStream_t strea
   constexpr int XT = x_tile<T>();
   constexpr int X_NT = x_nt<T>();
   if (pl.ctx) {
-    switch (pl.nx) {
-      case 64: k_ls_tau_xfwd<T, D, XT / X_NT, 64, XT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
-      case 128: k_ls_tau_xfwd<T, D, XT / X_NT, 128, XT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
-      case 256: k_ls_tau_xfwd<T, D, XT / X_NT, 256, XT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
-      default: k_ls_tau_xfwd<T, D, XT / X_NT, 512, XT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
-    }
+    ct_dispatch(pl.nx, [&](auto nn) {
+      constexpr int NN = decltype(nn)::value, BB = ct_cols(XT, NN);
+      k_ls_tau_xfwd<T, D, NN * BB / X_NT, NN, BB, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt);
+    });
   } else {
     k_ls_tau_xfwd<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, bt);
   }
@@ -1477,12 +1475,10 @@ void launch_pencil(const LsPlan& pl, const LsGeom<T>& g, dim3 gp, hipStream_t st
   constexpr int TT = ct_tile<T>();
   constexpr int C = (D == 3) ? 6 : 3;
   if (pl.ctq) {
-    switch (pl.Q) {
-      case 64: k_ls_pencil<T, D, TT / MGK_NT, 64, ct_pp(TT, C, 64)><<<gp, MGK_NT, pl.lds_q, stream>>>(g, bt, scale); break;
-      case 128: k_ls_pencil<T, D, TT / MGK_NT, 128, ct_pp(TT, C, 128)><<<gp, MGK_NT, pl.lds_q, stream>>>(g, bt, scale); break;
-      case 256: k_ls_pencil<T, D, TT / MGK_NT, 256, ct_pp(TT, C, 256)><<<gp, MGK_NT, pl.lds_q, stream>>>(g, bt, scale); break;
-      default: k_ls_pencil<T, D, TT / MGK_NT, 512, ct_pp(TT, C, 512)><<<gp, MGK_NT, pl.lds_q, stream>>>(g, bt, scale); break;
-    }
+    ct_dispatch(pl.Q, [&](auto nn) {
+      constexpr int NN = decltype(nn)::value;
+      k_ls_pencil<T, D, TT / MGK_NT, NN, ct_pp(TT, C, NN)><<<gp, MGK_NT, pl.lds_q, stream>>>(g, bt, scale);
+    });
   } else {
     k_ls_pencil<T, D, MV><<<gp, MGK_NT, pl.lds_q, stream>>>(g, bt, scale);
   }
@@ -1494,14 +1490,29 @@ void launch_xinv(const LsPlan& pl, const LsGeom<T>& g, dim3 gx, hipStream_t stre
   constexpr int XT = x_tile<T>();
   constexpr int X_NT = x_nt<T>();
   if (pl.ctx) {
-    switch (pl.nx) {
-      case 64: k_ls_xinv_update<T, D, XT / X_NT, 64, XT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
-      case 128: k_ls_xinv_update<T, D, XT / X_NT, 128, XT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
-      case 256: k_ls_xinv_update<T, D, XT / X_NT, 256, XT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
-      default: k_ls_xinv_update<T, D, XT / X_NT, 512, XT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt); break;
-    }
+    ct_dispatch(pl.nx, [&](auto nn) {
+      constexpr int NN = decltype(nn)::value, BB = ct_cols(XT, NN);
+      k_ls_xinv_update<T, D, NN * BB / X_NT, NN, BB, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt);
+    });
   } else {
     k_ls_xinv_update<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, bt);
+  }
+}
+
+// r2c along x of a C-component real field (the nonlinear laws' polarisation)
+template <typename T>
+void launch_xfwd_real(const LsPlan& pl, const LsGeom<T>& g, dim3 gx, hipStream_t stream, const T* tau,
+                      cplx<T>* spec, const LsState* st) {
+  constexpr int MV = maxv<T>();
+  constexpr int XT = x_tile<T>();
+  constexpr int X_NT = x_nt<T>();
+  if (pl.ctx) {
+    ct_dispatch(pl.nx, [&](auto nn) {
+      constexpr int NN = decltype(nn)::value, BB = ct_cols(XT, NN);
+      k_ls_xfwd_real<T, NN * BB / X_NT, NN, BB, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, tau, spec, st);
+    });
+  } else {
+    k_ls_xfwd_real<T, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, tau, spec, st);
   }
 }
 
@@ -1603,16 +1614,7 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
         k_l2_stress<<<nel, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, lam0, mu0, l2->P, l2->plane_stress, l2->e33,
                                                 l2->tau, nullptr, nullptr, NV, st, 0);
         k_l2_after_eval<<<1, 1, 0, stream>>>(st, l2->persist);
-        if (pl.ctx) {
-          switch (pl.nx) {
-            case 64: k_ls_xfwd_real<T, XT / X_NT, 64, XT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
-            case 128: k_ls_xfwd_real<T, XT / X_NT, 128, XT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
-            case 256: k_ls_xfwd_real<T, XT / X_NT, 256, XT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
-            default: k_ls_xfwd_real<T, XT / X_NT, 512, XT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st); break;
-          }
-        } else {
-          k_ls_xfwd_real<T, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, l2->tau, spec, st);
-        }
+        launch_xfwd_real<T>(pl, g, gx, stream, l2->tau, spec, st);
       }
     }
     if (l2) {
@@ -1625,16 +1627,7 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
         k_ls_pol_tau<T, LandauLaw><<<nel, MGK_NT, 0, stream>>>(eps, epsp, lam, mu, T(lam0), T(mu0), LandauLaw{*lp},
                                                                taubuf, NV, st);
       }
-      if (pl.ctx) {
-        switch (pl.nx) {
-          case 64: k_ls_xfwd_real<T, XT / X_NT, 64, XT / 64, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
-          case 128: k_ls_xfwd_real<T, XT / X_NT, 128, XT / 128, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
-          case 256: k_ls_xfwd_real<T, XT / X_NT, 256, XT / 256, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
-          default: k_ls_xfwd_real<T, XT / X_NT, 512, XT / 512, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, taubuf, spec, st); break;
-        }
-      } else {
-        k_ls_xfwd_real<T, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, taubuf, spec, st);
-      }
+      launch_xfwd_real<T>(pl, g, gx, stream, taubuf, spec, st);
     } else {
     h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
     launch_tau<T, D>(pl, g, gx, stream, bt);

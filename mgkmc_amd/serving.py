@@ -7,12 +7,14 @@ FP64 refinement form.  The server
   * admits arrived requests in arrival order while the estimated HBM footprint
     of the admitted set stays within the budget and fewer than max_active
     simulations are resident;
-  * groups the resident simulations by grid shape: each shape in use has one
+  * groups the resident simulations by grid shape: each group has one
     runner thread and HIP stream, which steps its simulations together in the
     batched native event loop (mgk_kmc_run_batch: one synchronisation per
     step for all rate totals, one for all selections, one LS solve with one
     launch per sweep for every member that had an event), a chunk of events
-    at a time;
+    at a time.  Small grids, whose launches are latency bound, share a group
+    (up to 8 members); grids that fill the GPU alone get groups of one or
+    two, so same-shape large requests still overlap on separate streams;
   * lets requests join and leave a running group between chunks: a request
     that arrives while others of its shape run is built on the group's stream
     and joins at the next chunk boundary, and a finished one leaves at the
@@ -121,7 +123,9 @@ def footprint_bytes(req):
 def build_simulation(req):
     """The request's simulation with the serving recipe: mu ~ U[0.9, 1.1] *
     26.92 GPa, nu = 0.35, zero-mean Gaussian eigenstrain (std 1e-3 per stored
-    component), M modes, gamma0 0.14, pixel 0.7 nm, Gaussian barriers (2.2,
+    component, drawn on the device from the request's seed: a 192^3 field is
+    42M normals, most of a request's set-up time if drawn on the host), M
+    modes, gamma0 0.14, pixel 0.7 nm, Gaussian barriers (2.2,
     0.45, clipped at 0) drawn on the device, jp 20, jt 0, cap 0.916, and a
     fixed macro strain eps_xx = APPLIED_STRAIN (strain rate 0: thermally
     activated relaxation under load; a driven strain rate would strain a
@@ -135,8 +139,6 @@ def build_simulation(req):
     nu = np.full(shape, 0.35)
     E = 2.0 * mu * (1.0 + nu)
     C = 6 if d == 3 else 3
-    ep = rng.standard_normal((C,) + shape) * 1e-3
-    ep -= ep.mean(axis=tuple(range(1, d + 1)), keepdims=True)
     T = req.kT * E0_EV / KB
     common = dict(pixel=0.7, barrier_generator="gaussian", barrier_kwargs=dict(mean=2.2, std=0.45, min_cutoff=0.0),
                   softening_cap=0.916, jp=20.0, jt=0.0, temperature=T, strain_rate=0.0,
@@ -146,8 +148,14 @@ def build_simulation(req):
         sim = ThermalSimulation(*shape, req.M, 0.14, E, nu, **common)
     else:
         sim = KmcSimulation2D(*shape, req.M, 0.14, E, nu, **common)
+    torch = dev.torch
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(req.seed)
+    ep = torch.randn((C,) + shape, generator=gen, dtype=torch.float64, device="cuda") * 1e-3
+    ep -= ep.mean(dim=tuple(range(1, d + 1)), keepdim=True)
     prec = sim.kmc.eps_p.dtype
-    sim.kmc.eps_p.copy_(dev.torch.from_numpy(ep).to(device="cuda", dtype=prec))
+    sim.kmc.eps_p.copy_(ep.to(prec))
+    del ep
     if sim.kmc.eps_p32 is not None:
         sim.kmc.eps_p32.copy_(sim.kmc.eps_p.to(dev.torch.float32))
     sim.eps_macro[0, 0] = APPLIED_STRAIN
@@ -212,22 +220,36 @@ class _Member:
 
 
 class _Group:
-    """Resident simulations of one grid shape and their runner's inbox."""
+    """Resident simulations of one grid shape stepping together (at most cap)
+    and their runner's inbox; n counts the members assigned and not yet
+    reported done."""
 
-    def __init__(self):
+    def __init__(self, cap):
         self.lock = threading.Lock()
         self.inbox = deque()
         self.running = False
+        self.cap = cap
+        self.n = 0
+
+
+def batch_cap(shape, voxels=1 << 21, limit=None):
+    """Members per batch for a grid shape: batching pays where a member's
+    launches are latency bound (small grids); a grid whose sweeps fill the
+    GPU on their own gains nothing from it and would only step in lockstep
+    with its peers, so up to `voxels` voxels per batch step, at least one."""
+    limit = _lib.MGK_LS_MAX_BATCH if limit is None else limit
+    nv = int(np.prod(shape))
+    return max(1, min(limit, voxels // nv))
 
 
 class SimServer:
-    """Admission by HBM footprint; continuous batching per grid shape (one
-    runner thread + HIP stream per shape in use, at most MGK_LS_MAX_BATCH
-    members stepping together, chunk_events events per member between
-    joins).  batch=False runs every request alone on its own worker thread
-    and stream instead (no batching; kept for comparison)."""
+    """Admission by HBM footprint; continuous batching per grid shape (groups
+    of up to batch_cap(shape) members, each group with its own runner thread
+    and HIP stream, chunk_events events per member between joins).
+    batch=False runs every request alone on its own worker thread and stream
+    instead (no batching; kept for comparison)."""
 
-    def __init__(self, hbm_budget=None, max_active=16, batch=True, chunk_events=10):
+    def __init__(self, hbm_budget=None, max_active=16, batch=True, chunk_events=10, batch_voxels=1 << 21):
         dev.require_gpu()
         if hbm_budget is None:
             free, _total = dev.torch.cuda.mem_get_info()
@@ -236,6 +258,7 @@ class SimServer:
         self.max_active = int(max_active)
         self.batch = bool(batch)
         self.chunk_events = int(chunk_events)
+        self.batch_voxels = int(batch_voxels)
         self.peak_used = 0
         self.peak_active = 0
         self.peak_batch = 0
@@ -251,7 +274,7 @@ class SimServer:
             while True:
                 new = []
                 with grp.lock:
-                    while grp.inbox and len(members) + len(new) < _lib.MGK_LS_MAX_BATCH:
+                    while grp.inbox and len(members) + len(new) < grp.cap:
                         new.append(grp.inbox.popleft())
                     if not members and not new:
                         grp.running = False
@@ -302,7 +325,17 @@ class SimServer:
                 used += fp
                 active += 1
                 r._t_admit = time.perf_counter() - t0
-                g = groups.setdefault(tuple(r.shape), _Group())
+                # a group of this shape with room (a running one first), else a new group
+                gl = groups.setdefault(tuple(r.shape), [])
+                free = [g for g in gl if g.n < g.cap]
+                free.sort(key=lambda g: not g.running)
+                if free:
+                    g = free[0]
+                else:
+                    g = _Group(batch_cap(r.shape, self.batch_voxels))
+                    gl.append(g)
+                g.n += 1
+                r._group = g
                 with g.lock:
                     g.inbox.append((r, fp))
                     start = not g.running
@@ -323,6 +356,7 @@ class SimServer:
                     r, fp, res = item
                     used -= fp
                     active -= 1
+                    r._group.n -= 1
                     res.t_arrival = r.arrival * time_scale
                     res.t_admit = r._t_admit
                     res.t_done = time.perf_counter() - t0

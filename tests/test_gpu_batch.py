@@ -168,3 +168,17 @@ def test_kmc_batch_bitwise(gpu, shape):
               f"{_equal_state(solo[i], got)}", flush=True)
         assert len(got["events"]) == want[i]
         assert _equal_state(solo[i], got)
+
+
+def test_ls_batch_mixed_radix_bitwise(gpu):
+    # a 96^3 batch runs the compile-time radix-3 sweeps with a member dimension
+    import torch
+    from mgkmc_amd.linear_elastic_simulator import solve_batch, MIXED
+    shape = (96, 96, 96)
+    ebs, eps = _inputs(shape, 2, 13, torch.float64)
+    solo = _solvers(shape, MIXED, 2, 6)
+    infos = [_copy_info(s.solve(eb, ep, tol=1e-8)) for s, eb, ep in zip(solo, ebs, eps)]
+    bat = _solvers(shape, MIXED, 2, 6)
+    got = solve_batch(bat, ebs, eps, tol=1e-8)
+    for s, b, i0, i1 in zip(solo, bat, infos, got):
+        assert torch.equal(s.eps, b.eps) and torch.equal(s.sig, b.sig) and _same_info(i0, i1)

@@ -65,9 +65,12 @@ def test_ls3d_parity(gpu, shape):
     assert inf["iters"] == info["iters"]
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 256), (256, 256), (512, 128), (8, 4, 512)])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 256), (256, 256), (512, 128), (8, 4, 512),
+                                   (96, 96, 96), (192, 64, 96), (64, 96, 192), (384, 32, 48), (96, 192),
+                                   (384, 96)])
 def test_ls_specialised_sizes_parity(gpu, shape):
-    # power-of-two extents 64..512 run the compile-time specialised sweeps
+    # extents 64..512 in powers of two and 96, 192, 384 (a radix-3 pass, then
+    # powers of two) run the compile-time specialised sweeps
     from mgkmc_amd.linear_elastic_simulator import spectral_solver_3d, spectral_solver_2d
     print(f"[ls ct] shape={shape}", flush=True)
     E, nu, ep = _case(shape, 11)
@@ -83,7 +86,8 @@ def test_ls_specialised_sizes_parity(gpu, shape):
         assert _rel(a, b) < 1e-10, _rel(a, b)
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 256), (64, 64, 75), (256, 256), (512, 128)])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 256), (64, 64, 75), (256, 256), (512, 128),
+                                   (96, 96, 96), (192, 64, 96), (96, 192), (384, 96)])
 def test_ls_fp32_specialised_sizes(gpu, shape):
     # FP32 sweeps (vectorised 16-byte tile loads/stores on full, even rows;
     # per-element on partial or odd rows) agree with the FP64 oracle to FP32
