@@ -177,6 +177,24 @@ int mgk_ls_solve_landau2d(const int64_t* shape, const double* lam, const double*
                           int e33_have, double* tau_buf, int anderson_m, double* aa_buf, int warm, int max_iter,
                           double tol, void* workspace, size_t workspace_bytes, void* stream, mgk_ls_info* info);
 
+/* The same model's Newton-CG solver: spectral_solver_landau_2d(solver="newton_cg")
+ * (mgkmc/linear_elastic_simulator.py:1011-1117).  Newton steps on eps; each solves
+ * A dx = b with BiCGSTAB (SciPy's recurrence, rtol = tol, maxiter 150, x0 = 0) where
+ * A = G C_h G, G = -Gamma0 of (lam0, mu0) and C_h d = (sigma(eps_el + h d) -
+ * sigma(eps_el - h d)) / 2h, h = 1e-7, every stress evaluation with a cold plane-stress
+ * e33 Newton; b = -G C_h (eps_bar - <eps>) at the first step, -G sigma after it; eps +=
+ * G dx (plus eps_bar - <eps> at the first step); stop when |dx| / (|eps| + 1e-20) < tol
+ * after the first step (norms over the full 2x2 fields).  info->iters = Newton steps,
+ * info->diff = the last |dx| / |eps|, *krylov_iters = BiCGSTAB iterations in total;
+ * MGK_NOT_CONVERGED (silent) after max_iter steps, MGK_ERR_NONFINITE on a non-finite
+ * stress or update.  sig: out (3 NV) or NULL.  Workspace: the bytes below. */
+int mgk_ls_landau2d_ncg_workspace_bytes(const int64_t* shape, size_t* bytes);
+int mgk_ls_solve_landau2d_ncg(const int64_t* shape, const double* lam, const double* mu, double lam0, double mu0,
+                              const mgk_landau_params* params, int plane_stress, const double* eps_bar,
+                              const double* eps_plastic, double* eps, double* sig, int warm, int max_iter, double tol,
+                              void* workspace, size_t workspace_bytes, void* stream, mgk_ls_info* info,
+                              int* krylov_iters);
+
 /* Secant-modulus small-strain laws.  Replaces spectral_solver_secant_2d/3d
  * (mgkmc/linear_elastic_simulator.py:614-850; stress_from_strain_secant_2d/3d,
  * mgkmc/elasticity.py:641-810) for law 0, and spectral_solver_rose_2d

@@ -158,6 +158,10 @@ SIGNATURES = {
     "mgk_ls_solve_landau2d": (_c_i, [_c_i64p, _c_p, _c_p, _c_d, _c_d, ctypes.POINTER(MgkLandauParams), _c_i, _c_dp,
                                      _c_p, _c_p, _c_p, _c_p, _c_i, _c_i, _c_p, _c_i, _c_p, _c_i, _c_i, _c_d, _c_p,
                                      _c_sz, _c_p, ctypes.POINTER(MgkLsInfo)]),
+    "mgk_ls_landau2d_ncg_workspace_bytes": (_c_i, [_c_i64p, ctypes.POINTER(_c_sz)]),
+    "mgk_ls_solve_landau2d_ncg": (_c_i, [_c_i64p, _c_p, _c_p, _c_d, _c_d, ctypes.POINTER(MgkLandauParams), _c_i, _c_dp,
+                                         _c_p, _c_p, _c_p, _c_i, _c_i, _c_d, _c_p, _c_sz, _c_p,
+                                         ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i)]),
     "mgk_ls_solve_secant": (_c_i, [_c_i, _c_i64p, _c_i, _c_p, _c_p, _c_d, _c_d, ctypes.POINTER(MgkSecantParams), _c_dp,
                                    _c_p, _c_p, _c_p, _c_p, _c_i, _c_i, _c_d, _c_p, _c_sz, _c_p,
                                    ctypes.POINTER(MgkLsInfo)]),

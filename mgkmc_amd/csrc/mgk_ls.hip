@@ -22,6 +22,7 @@
 #include "mgk_fftpass.h"
 #include "mgk_landau.h"
 #include "mgk_secant.h"
+#include "mgk_krylov.h"
 
 // Compile-time x sweeps (tau + r2c, c2r + update): FP64 runs 512-thread
 // workgroups on tiles of 4096 points (twice the other FP64 tiles, 64 B of
@@ -1917,6 +1918,367 @@ int ls_solve_batch_t(const LsPlan& pl, int nb, const LsMemArgs<T>* a, int warm, 
   prof.commit(iters_max);
   return rc;
 }
+
+// ---------------------------------------------------------------------------
+// 2D Landau small strain, Newton-CG (the reference's solver="newton_cg",
+// mgkmc/linear_elastic_simulator.py:1011-1117).  Newton steps on eps with
+// BiCGSTAB (SciPy's recurrence, rtol = tol, maxiter 150) on A = G C_h G,
+// G = the reference medium's -Gamma0 and C_h d = (sigma(eps_el + h d) -
+// sigma(eps_el - h d)) / 2h with h = 1e-7 (central differences of the
+// Landau stress, each evaluation with its own cold plane-stress e33 Newton).
+// The Krylov vectors hold the reference's full 2x2 field per voxel (xx, yy,
+// xy, yx), so their dot products are those of the reference's flattened
+// vectors; G reads the symmetric part and writes xy into both slots.
+// ---------------------------------------------------------------------------
+constexpr double kNcgH = 1e-7;
+
+__global__ void k_ncg_sym(size_t NV, const double* __restrict__ p4, double* __restrict__ q3,
+                          const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < NV; i += (size_t)gridDim.x * blockDim.x) {
+    q3[i] = p4[i];
+    q3[NV + i] = p4[NV + i];
+    q3[2 * NV + i] = 0.5 * (p4[2 * NV + i] + p4[3 * NV + i]);
+  }
+}
+
+__global__ void k_ncg_expand(size_t NV, const double* __restrict__ a3, double sgn, double* __restrict__ p4,
+                             const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < NV; i += (size_t)gridDim.x * blockDim.x) {
+    const double xy = sgn * a3[2 * NV + i];
+    p4[i] = sgn * a3[i];
+    p4[NV + i] = sgn * a3[NV + i];
+    p4[2 * NV + i] = xy;
+    p4[3 * NV + i] = xy;
+  }
+}
+
+// (eps - eps_p) + h d  (sign > 0) or (eps - eps_p) - h d, as the reference forms them
+__global__ void k_ncg_pert(size_t n3, const double* __restrict__ eps, const double* __restrict__ epsp,
+                           const double* __restrict__ d, double h, int sign, double* __restrict__ out,
+                           const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+    const double el = eps[i] - (epsp ? epsp[i] : 0.0);
+    const double hd = h * d[i];
+    out[i] = sign > 0 ? el + hd : el - hd;
+  }
+}
+
+__global__ void k_ncg_fd(size_t n3, const double* __restrict__ sp, const double* __restrict__ sm, double h2,
+                         double* __restrict__ out, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (sp[i] - sm[i]) / h2;
+}
+
+// constant field of the three symmetric components of a 2x2 matrix
+__global__ void k_ncg_const(size_t NV, Sym6 v, double* __restrict__ out) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < NV; i += (size_t)gridDim.x * blockDim.x) {
+    out[i] = v.v[0];
+    out[NV + i] = v.v[1];
+    out[2 * NV + i] = v.v[2];
+  }
+}
+
+// eps <- (eps + dbar) + g  (first Newton step) or eps + g
+__global__ void k_ncg_update(size_t NV, double* __restrict__ eps, const double* __restrict__ g, int with_dbar,
+                             Sym6 dbar) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < NV; i += (size_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double e = eps[(size_t)c * NV + i];
+      if (with_dbar) e = e + dbar.v[c];
+      eps[(size_t)c * NV + i] = e + g[(size_t)c * NV + i];
+    }
+  }
+}
+
+// per-block (a, b) and (c, d) dot products, stride 2 (the Krylov stop kernels' layout)
+__global__ __launch_bounds__(MGK_NT) void k_ncg_dot2(size_t n, const double* __restrict__ a,
+                                                     const double* __restrict__ b, const double* __restrict__ c,
+                                                     const double* __restrict__ d, double* __restrict__ partials,
+                                                     const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double red[2 * (MGK_NT / 64)];
+  double acc[2] = {0.0, 0.0};
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * MGK_NT) {
+    acc[0] += a[i] * b[i];
+    if (c) acc[1] += c[i] * d[i];
+  }
+  block_sum<2>(acc, red);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = acc[0];
+    partials[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+// sum over voxels of the full 2x2 squared norm (xy counted twice) of a 3-component field
+__global__ __launch_bounds__(MGK_NT) void k_ncg_norm2w(size_t NV, const double* __restrict__ f,
+                                                       double* __restrict__ partials) {
+  __shared__ double red[MGK_NT / 64];
+  double acc[1] = {0.0};
+  for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < NV; i += (size_t)gridDim.x * MGK_NT) {
+    const double a = f[i], b = f[NV + i], c = f[2 * NV + i];
+    acc[0] += (a * a + b * b) + 2.0 * (c * c);
+  }
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
+}
+
+// the law and Gamma kernels of the Krylov matvec skip once the solve is done
+__global__ void k_ncg_gate(const int* __restrict__ flag, LsState* __restrict__ st) { st->done = flag ? *flag : 0; }
+
+struct NcgCtx {
+  LsPlan pl;
+  LsGeom<double> g;
+  hipStream_t s;
+  size_t NV;
+  const double *lam, *mu, *epsp;
+  double lam0, mu0;
+  LandauPar P;
+  int plane_stress;
+  LsBatch<double> bt;  // member 0: the Gamma application's spectrum, partials, state
+  LsState* lst;        // Gamma and law state (done gate, e33 count, status)
+  double* e33;
+  double *q3, *dc3, *pp3, *sp3, *sm3, *dt3, *o3;
+  dim3 gx, gp;
+  int nel;
+};
+
+// out3 = -Gamma0 : in3 (r2c along x, pencil along y with the Hermitian Nyquist part, c2r)
+void ncg_gamma(NcgCtx& c, const double* in3, double* out3) {
+  launch_xfwd_real<double>(c.pl, c.g, c.gx, c.s, in3, c.bt.m[0].spec, c.lst);
+  launch_pencil<double, 2>(c.pl, c.g, c.gp, c.s, c.bt, 1.0 / (double)c.pl.NV);
+  LsBatch<double> b = c.bt;
+  b.m[0].eps = out3;
+  launch_xinv<double, 2>(c.pl, c.g, c.gx, c.s, b);
+}
+
+// sig3 = sigma_Landau(strain3) with a cold plane-stress e33 Newton
+void ncg_sigma(NcgCtx& c, const double* strain3, double* sig3) {
+  if (c.plane_stress)
+    k_l2_count<<<c.nel, MGK_NT, 0, c.s>>>(strain3, nullptr, c.lam, c.mu, c.P, c.e33, c.NV, c.lst, 0);
+  k_l2_stress<<<c.nel, MGK_NT, 0, c.s>>>(strain3, nullptr, c.lam, c.mu, c.lam0, c.mu0, c.P, c.plane_stress, c.e33,
+                                         nullptr, sig3, nullptr, c.NV, c.lst, 0);
+  k_l2_after_eval<<<1, 1, 0, c.s>>>(c.lst, 0);
+}
+
+// out3 = C_h d3 at the current eps
+void ncg_cfd(NcgCtx& c, const double* eps, const double* d3, double* out3) {
+  const size_t n3 = 3 * c.NV;
+  const int* sk = &c.lst->done;
+  k_ncg_pert<<<c.nel, MGK_NT, 0, c.s>>>(n3, eps, c.epsp, d3, kNcgH, 1, c.pp3, sk);
+  ncg_sigma(c, c.pp3, c.sp3);
+  k_ncg_pert<<<c.nel, MGK_NT, 0, c.s>>>(n3, eps, c.epsp, d3, kNcgH, -1, c.pp3, sk);
+  ncg_sigma(c, c.pp3, c.sm3);
+  k_ncg_fd<<<c.nel, MGK_NT, 0, c.s>>>(n3, c.sp3, c.sm3, 2.0 * kNcgH, out3, sk);
+}
+
+// out4 = G C_h G p4
+void ncg_matvec(NcgCtx& c, const double* eps, const double* p4, double* out4) {
+  const int* sk = &c.lst->done;
+  k_ncg_sym<<<c.nel, MGK_NT, 0, c.s>>>(c.NV, p4, c.q3, sk);
+  ncg_gamma(c, c.q3, c.dc3);
+  ncg_cfd(c, eps, c.dc3, c.dt3);
+  ncg_gamma(c, c.dt3, c.o3);
+  k_ncg_expand<<<c.nel, MGK_NT, 0, c.s>>>(c.NV, c.o3, 1.0, out4, sk);
+}
+
+int ncg_read_sum(double* part, int n, hipStream_t s, double* out) {
+  double* h = (double*)mgk_pinned((size_t)n * sizeof(double));
+  if (!h) return MGK_ERR_HIP;
+  MGK_HIP(hipMemcpyAsync(h, part, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
+  MGK_HIP(hipStreamSynchronize(s));
+  double t = 0.0;
+  for (int i = 0; i < n; ++i) t += h[i];
+  *out = t;
+  return MGK_OK;
+}
+
+size_t ncg_extra_bytes(const LsPlan& pl) {
+  const size_t f3 = mgk_align((size_t)3 * pl.NV * sizeof(double)), f4 = mgk_align((size_t)4 * pl.NV * sizeof(double));
+  return 7 * f4 + 9 * f3 + mgk_align((size_t)pl.NV * sizeof(double)) + mgk_align((size_t)KRY_NBLK * 2 * sizeof(double)) +
+         mgk_align(sizeof(KryState)) + mgk_align(sizeof(LsState));
+}
+
+int l2_newton_cg(const LsPlan& pl, const double* lam, const double* mu, double lam0, double mu0, const LandauPar& P,
+                 int plane_stress, const double* eps_bar, const double* epsp, double* eps, double* sig, int warm,
+                 int max_iter, double tol, char* ws, hipStream_t s, mgk_ls_info* info, int* krylov_total) {
+  NcgCtx c;
+  c.pl = pl;
+  c.g = ls_geom<double>(pl, 3);
+  c.g.twx = (const cplx<double>*)mgk_twiddles(pl.nx, 0);
+  c.g.twy = (const cplx<double>*)mgk_twiddles(pl.P, 0);
+  c.g.twq = (const cplx<double>*)mgk_twiddles(pl.Q, 0);
+  if (!c.g.twx || !c.g.twy || !c.g.twq) {
+    mgk_set_error("twiddle table allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
+  c.s = s;
+  c.NV = (size_t)pl.NV;
+  c.lam = lam;
+  c.mu = mu;
+  c.epsp = epsp;
+  c.lam0 = lam0;
+  c.mu0 = mu0;
+  c.P = P;
+  c.plane_stress = plane_stress;
+  const size_t NV = c.NV, n3 = 3 * NV, n4 = 4 * NV;
+  cplx<double>* spec = (cplx<double>*)(ws + pl.off_spec);
+  double* part = (double*)(ws + pl.off_part);
+  LsState* gst_unused = (LsState*)(ws + pl.off_state);
+  (void)gst_unused;
+  char* w = ws + pl.total;
+  const size_t f3 = mgk_align(n3 * sizeof(double)), f4 = mgk_align(n4 * sizeof(double));
+  double* kv[7];
+  for (int i = 0; i < 7; ++i) {
+    kv[i] = (double*)w;
+    w += f4;
+  }
+  double *kb = kv[0], *kx = kv[1], *kr = kv[2], *krt = kv[3], *kp = kv[4], *kvv = kv[5], *kt = kv[6];
+  double* f[9];
+  for (int i = 0; i < 9; ++i) {
+    f[i] = (double*)w;
+    w += f3;
+  }
+  c.q3 = f[0];
+  c.dc3 = f[1];
+  c.pp3 = f[2];
+  c.sp3 = f[3];
+  c.sm3 = f[4];
+  c.dt3 = f[5];
+  c.o3 = f[6];
+  double* sigc = sig ? sig : f[7];  // the current stress
+  double* dbar3 = f[8];
+  c.e33 = (double*)w;
+  w += mgk_align(NV * sizeof(double));
+  double* kpart = (double*)w;
+  w += mgk_align((size_t)KRY_NBLK * 2 * sizeof(double));
+  KryState* kst = (KryState*)w;
+  w += mgk_align(sizeof(KryState));
+  c.lst = (LsState*)w;
+  c.bt = LsBatch<double>{};
+  c.bt.m[0] = LsMem<double>{lam, mu, nullptr, nullptr, spec, part, c.lst, Sym6{}, lam0, mu0, 0.0};
+  const int nqx = (pl.Q + (1 << pl.logBx) - 1) >> pl.logBx;
+  const int npb = (c.g.npencil + (1 << pl.logPP) - 1) >> pl.logPP;
+  c.gx = dim3(nqx, pl.P);
+  c.gp = dim3(npb);
+  c.nel = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 8192);
+  const int nb = (int)std::min<size_t>((n4 + MGK_NT - 1) / MGK_NT, KRY_NBLK);
+
+  // fields start defined: the Gamma output buffers are read by the update sweep
+  // before it overwrites them
+  k_zero_words<<<1, 64, 0, s>>>((unsigned int*)c.lst, sizeof(LsState) / 4);
+  for (int i = 0; i < 9; ++i) k_ncg_const<<<c.nel, MGK_NT, 0, s>>>(NV, Sym6{}, f[i]);
+  double anti2 = 0.0;
+  const Sym6 eb = ls_sym_bar<2>(eps_bar, &anti2);
+  if (!warm) {
+    k_ls_fill<double><<<c.nel, MGK_NT, 0, s>>>(eps, 3, NV, eb);
+  } else {
+    k_comp_sums<double><<<pl.nblk_red, MGK_NT, 0, s>>>(eps, 3, NV, part);
+    k_means_finalize<<<1, MGK_NT, 0, s>>>(part, pl.nblk_red, 3, (double)NV, c.lst->mean);
+    k_ls_warm_adjust<double><<<c.nel, MGK_NT, 0, s>>>(eps, 3, NV, eb, c.lst->mean);
+  }
+  // Dbar = eps_bar - <eps>
+  k_comp_sums<double><<<pl.nblk_red, MGK_NT, 0, s>>>(eps, 3, NV, part);
+  k_means_finalize<<<1, MGK_NT, 0, s>>>(part, pl.nblk_red, 3, (double)NV, c.lst->mean);
+  LsState* hst = (LsState*)mgk_pinned(sizeof(LsState));
+  if (!hst) return MGK_ERR_HIP;
+  MGK_HIP(hipMemcpyAsync(hst, c.lst, sizeof(LsState), hipMemcpyDeviceToHost, s));
+  MGK_HIP(hipStreamSynchronize(s));
+  Sym6 dbar{};
+  for (int k = 0; k < 3; ++k) dbar.v[k] = eb.v[k] - hst->mean[k];
+  k_ncg_const<<<c.nel, MGK_NT, 0, s>>>(NV, dbar, dbar3);
+  // sigma of the start field
+  k_ncg_pert<<<c.nel, MGK_NT, 0, s>>>(n3, eps, epsp, dbar3, 0.0, 1, c.pp3, nullptr);
+  ncg_sigma(c, c.pp3, sigc);
+  MGK_HIP(hipGetLastError());
+
+  int newton = 0, kry_total = 0, status = MGK_NOT_CONVERGED;
+  double res = 0.0;
+  for (int it = 0; it < max_iter; ++it) {
+    // right-hand side: -G C_h Dbar (first step) or -G sigma
+    k_ncg_gate<<<1, 1, 0, s>>>(nullptr, c.lst);
+    if (it == 0) {
+      ncg_cfd(c, eps, dbar3, c.dt3);
+      ncg_gamma(c, c.dt3, c.o3);
+    } else {
+      ncg_gamma(c, sigc, c.o3);
+    }
+    k_ncg_expand<<<c.nel, MGK_NT, 0, s>>>(NV, c.o3, -1.0, kb, nullptr);
+    // BiCGSTAB(A, b, rtol = tol, maxiter = 150) from x = 0
+    KryState init{};
+    init.rtol = tol;
+    init.maxiter = 150;
+    init.rhotol = 4.930380657631324e-32;  // eps(double)^2, as SciPy for float64
+    init.omtol = init.rhotol;
+    MGK_HIP(hipMemcpyAsync(kst, &init, sizeof(KryState), hipMemcpyHostToDevice, s));
+    k_kry_init<double><<<nb, MGK_NT, 0, s>>>(n4, kb, kx, kr, krt, kpart);
+    k_kry_fin_init<<<1, MGK_NT, 0, s>>>(kpart, nb, kst);
+    auto enqueue = [&]() {
+      k_kry_p<double><<<nb, MGK_NT, 0, s>>>(n4, kst, kp, kr, kvv);
+      k_ncg_gate<<<1, 1, 0, s>>>(&kst->done, c.lst);
+      ncg_matvec(c, eps, kp, kvv);
+      k_ncg_dot2<<<nb, MGK_NT, 0, s>>>(n4, krt, kvv, nullptr, nullptr, kpart, &kst->done);
+      k_kry_fin_rv<<<1, MGK_NT, 0, s>>>(kpart, nb, kst);
+      k_kry_s<double><<<nb, MGK_NT, 0, s>>>(n4, kst, kr, kvv, kpart);
+      k_kry_fin_s<<<1, MGK_NT, 0, s>>>(kpart, nb, kst);
+      k_ncg_gate<<<1, 1, 0, s>>>(&kst->done2, c.lst);
+      ncg_matvec(c, eps, kr, kt);
+      k_ncg_dot2<<<nb, MGK_NT, 0, s>>>(n4, kt, kr, kt, kt, kpart, &kst->done2);
+      k_kry_fin_t<<<1, MGK_NT, 0, s>>>(kpart, nb, kst);
+      k_kry_xr<double><<<nb, MGK_NT, 0, s>>>(n4, kst, kx, kp, kr, kt, krt, kpart, (const double*)nullptr);
+      k_kry_fin_xr<<<1, MGK_NT, 0, s>>>(kpart, nb, kst);
+    };
+    int kit = 0, kstat = 0, enq = 0;
+    int rc = kry_drive(kst, s, 150, enqueue, &kit, &kstat, &enq);
+    if (rc != MGK_OK) return rc;
+    kry_total += kit;
+    // eps update: (eps + Dbar) + G dx (first step) or eps + G dx
+    k_ncg_gate<<<1, 1, 0, s>>>(nullptr, c.lst);
+    k_ncg_sym<<<c.nel, MGK_NT, 0, s>>>(NV, kx, c.q3, nullptr);
+    ncg_gamma(c, c.q3, c.o3);
+    k_ncg_update<<<c.nel, MGK_NT, 0, s>>>(NV, eps, c.o3, it == 0 ? 1 : 0, dbar);
+    k_ncg_pert<<<c.nel, MGK_NT, 0, s>>>(n3, eps, epsp, dbar3, 0.0, 1, c.pp3, nullptr);
+    ncg_sigma(c, c.pp3, sigc);
+    // |dx| / (|eps| + 1e-20) over the full 2x2 fields
+    k_norm2<double><<<nb, MGK_NT, 0, s>>>(n4, kx, kpart);
+    k_ncg_norm2w<<<nb, MGK_NT, 0, s>>>(NV, eps, kpart + KRY_NBLK);
+    MGK_HIP(hipGetLastError());
+    double nx2 = 0.0, ne2 = 0.0;
+    rc = ncg_read_sum(kpart, nb, s, &nx2);
+    if (rc == MGK_OK) rc = ncg_read_sum(kpart + KRY_NBLK, nb, s, &ne2);
+    if (rc != MGK_OK) return rc;
+    MGK_HIP(hipMemcpyAsync(hst, c.lst, sizeof(LsState), hipMemcpyDeviceToHost, s));
+    MGK_HIP(hipStreamSynchronize(s));
+    newton = it + 1;
+    res = std::sqrt(nx2) / (std::sqrt(ne2) + 1e-20);
+    if (hst->status == MGK_ERR_NONFINITE || !std::isfinite(res)) {
+      status = MGK_ERR_NONFINITE;
+      break;
+    }
+    if (res < tol && it > 0) {
+      status = MGK_OK;
+      break;
+    }
+  }
+  // macro means of eps and sigma
+  k_comp_sums<double><<<pl.nblk_red, MGK_NT, 0, s>>>(eps, 3, NV, part);
+  k_means_finalize<<<1, MGK_NT, 0, s>>>(part, pl.nblk_red, 3, (double)NV, c.lst->mean);
+  k_comp_sums<double><<<pl.nblk_red, MGK_NT, 0, s>>>(sigc, 3, NV, part);
+  k_means_finalize<<<1, MGK_NT, 0, s>>>(part, pl.nblk_red, 3, (double)NV, c.lst->mean + 3);
+  MGK_HIP(hipMemcpyAsync(hst, c.lst, sizeof(LsState), hipMemcpyDeviceToHost, s));
+  MGK_HIP(hipStreamSynchronize(s));
+  LsState out = *hst;
+  out.iters = newton;
+  out.status = status;
+  out.diff = res;
+  if (info) ls_fill_info<2>(out, eps_bar, info);
+  if (krylov_total) *krylov_total = kry_total;
+  return status;
+}
 }  // namespace
 
 extern "C" int mgk_ls_workspace_bytes(int ndim, const int64_t* shape, int precision, size_t* bytes) {
@@ -2302,6 +2664,54 @@ extern "C" int mgk_ls_solve_landau(const int64_t* shape, int precision, const vo
   return ls_solve_t<float, 3>(pl, (const float*)lam, (const float*)mu, lam0, mu0, eps_bar, (const float*)eps_plastic,
                               (float*)eps, (float*)sig, warm, max_iter, tol, (char*)workspace, s, info, true, &P,
                               (float*)tau_buf);
+}
+
+
+extern "C" int mgk_ls_landau2d_ncg_workspace_bytes(const int64_t* shape, size_t* bytes) {
+  LsPlan pl;
+  int rc = ls_plan<double>(2, shape, &pl);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the LS solver", __FILE__, __LINE__);
+    return rc;
+  }
+  *bytes = pl.total + ncg_extra_bytes(pl);
+  return MGK_OK;
+}
+
+extern "C" int mgk_ls_solve_landau2d_ncg(const int64_t* shape, const double* lam, const double* mu, double lam0,
+                                         double mu0, const mgk_landau_params* lpar, int plane_stress,
+                                         const double* eps_bar, const double* eps_plastic, double* eps, double* sig,
+                                         int warm, int max_iter, double tol, void* workspace, size_t workspace_bytes,
+                                         void* stream, mgk_ls_info* info, int* krylov_iters) {
+  if (!lam || !mu || !eps || !eps_bar || !workspace || !lpar || max_iter < 1) {
+    mgk_set_error("bad argument (null pointer or max_iter < 1)", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  LsPlan pl;
+  int rc = ls_plan<double>(2, shape, &pl);
+  if (rc != MGK_OK) {
+    mgk_set_error("unsupported grid shape for the LS solver", __FILE__, __LINE__);
+    return rc;
+  }
+  if (workspace_bytes < pl.total + ncg_extra_bytes(pl)) {
+    mgk_set_error("workspace too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  LandauPar P;
+  P.v1 = lpar->v1;
+  P.v2 = lpar->v2;
+  P.v3 = lpar->v3;
+  P.g1 = lpar->g1;
+  P.g2 = lpar->g2;
+  P.g3 = lpar->g3;
+  P.g4 = lpar->g4;
+  P.Ecap = lpar->cap_limit;
+  P.cap_p = lpar->cap_power;
+  P.cap_ratio = lpar->cap_ratio;
+  P.capping = lpar->capping;
+  P.cap_smooth = lpar->cap_smooth;
+  return l2_newton_cg(pl, lam, mu, lam0, mu0, P, plane_stress ? 1 : 0, eps_bar, eps_plastic, eps, sig, warm ? 1 : 0,
+                      max_iter, tol, (char*)workspace, (hipStream_t)stream, info, krylov_iters);
 }
 
 extern "C" int mgk_ls_solve_secant(int ndim, const int64_t* shape, int precision, const void* lam, const void* mu,

@@ -499,6 +499,32 @@ class LandauSolver2D:
                 "consider strain capping or smaller load steps).")
         return self.info
 
+    def solve_ncg(self, params, eps_bar, eps_plastic=None, warm=False, max_iter=400, tol=1e-6, stream=None):
+        """The reference's solver="newton_cg" (mgkmc/linear_elastic_simulator.py:1011-1117):
+        Newton steps with BiCGSTAB on G C_h G (central differences of the Landau
+        stress, h = 1e-7), stop when |dx| / |eps| < tol after the first step.
+        info.iters = Newton steps; self.krylov_iters = BiCGSTAB iterations."""
+        t = dev.torch
+        lib = _lib.load()
+        if getattr(self, "ws_ncg", None) is None:
+            nb = ctypes.c_size_t()
+            _lib.check(lib.mgk_ls_landau2d_ncg_workspace_bytes(self._shape_c, ctypes.byref(nb)),
+                       "mgk_ls_landau2d_ncg_workspace_bytes")
+            self.ws_ncg = t.empty(int(nb.value), dtype=t.uint8, device="cuda")
+        eb, ebp = _lib.host_dbl(np.asarray(eps_bar, dtype=np.float64).reshape(2, 2))
+        s = dev.stream_ptr() if stream is None else stream
+        kit = ctypes.c_int()
+        rc = lib.mgk_ls_solve_landau2d_ncg(
+            self._shape_c, _lib.dptr(self.lam), _lib.dptr(self.mu), self.lam0, self.mu0, ctypes.byref(params),
+            1 if self.plane_stress else 0, ebp, _lib.dptr(eps_plastic), _lib.dptr(self.eps), _lib.dptr(self.sig),
+            1 if warm else 0, int(max_iter), float(tol), _lib.dptr(self.ws_ncg), self.ws_ncg.numel(),
+            ctypes.c_void_p(s), ctypes.byref(self.info), ctypes.byref(kit))
+        _lib.check(rc, "mgk_ls_solve_landau2d_ncg")
+        self.krylov_iters = kit.value
+        if rc == _lib.MGK_ERR_NONFINITE:
+            raise FloatingPointError("spectral_solver_landau_2d (newton_cg): non-finite stress or update")
+        return self.info
+
 
 def spectral_solver_landau_2d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_plastic=None, max_iter=400, tol=1e-6,
                               verbose=False, pixel=1.0, plane_mode="plane_strain", Gamma=None, eps_init=None,
@@ -508,9 +534,11 @@ def spectral_solver_landau_2d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_
     """2D nonlinear LS with the Landau stress and Anderson mixing on the GPU
     (mgkmc/linear_elastic_simulator.py:952-1228, "dbfft" path); e33_state
     (dict) carries the plane-stress e33 warm start across calls as in the
-    reference.  Strains must be symmetric."""
-    if solver == "newton_cg":
-        raise NotImplementedError("the device path implements the dbfft (fixed point + Anderson) solver")
+    reference.  solver="newton_cg" runs the reference's Newton-CG branch
+    (:1011-1117; e33_state and anderson_m unused there, as in the reference).
+    Strains must be symmetric."""
+    if solver not in ("dbfft", "newton_cg"):
+        raise ValueError(f"unknown solver {solver!r}")
     lam = np.asarray(lam, dtype=np.float64)
     mu = np.asarray(mu, dtype=np.float64)
     eps_bar = np.asarray(eps_bar, dtype=np.float64)
@@ -529,12 +557,17 @@ def spectral_solver_landau_2d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_
         if np.any(a):
             raise ValueError("the device Landau solver takes a symmetric eps_init")
         sol.eps.copy_(dev.upload(sym, FP64))
+    prm = landau_params(v1, v2, v3, g1, g2, g3, g4, mu, strain_capping_enabled, strain_capping_limit,
+                        strain_capping_tangent_ratio, strain_capping_type, strain_capping_smooth_power)
+    if solver == "newton_cg":
+        inf = sol.solve_ncg(prm, eps_bar, ep, warm=warm, max_iter=max_iter, tol=tol)
+        eps = dev.from_sym_soa(dev.download(sol.eps), 2)
+        sig = dev.from_sym_soa(dev.download(sol.sig), 2)
+        return eps, sig, np.array(inf.eps_mean[:4]).reshape(2, 2), np.array(inf.sig_mean[:4]).reshape(2, 2)
     persist = e33_state is not None
     if persist and e33_state.get("e33") is not None and np.shape(e33_state["e33"]) == lam.shape:
         sol.e33.copy_(dev.upload(np.ascontiguousarray(e33_state["e33"], dtype=np.float64), FP64))
         sol.e33_have = True
-    prm = landau_params(v1, v2, v3, g1, g2, g3, g4, mu, strain_capping_enabled, strain_capping_limit,
-                        strain_capping_tangent_ratio, strain_capping_type, strain_capping_smooth_power)
     inf = sol.solve(prm, eps_bar, ep, warm=warm, max_iter=max_iter, tol=tol, persist_e33=persist)
     if persist and sol.plane_stress:
         e33_state["e33"] = dev.download(sol.e33)
@@ -776,8 +809,9 @@ def rose_elastic_simulation_2d(lam, mu, eta, mu_floor_fraction=0.1, target_strai
 def _landau_simulation(d, lam, mu, v, target_strain_mask, target_values, n_steps, tol_ls, max_iter_ls, tol_macro,
                        max_iter_macro, store, plane_mode, solver_kw):
     solver_name = solver_kw.get("solver", "dbfft")
-    if solver_name == "newton_cg":
-        raise NotImplementedError("the device path implements the dbfft (fixed point + Anderson) solver")
+    if solver_name == "newton_cg" and d == 3:
+        raise NotImplementedError("Newton-CG solver is not yet implemented for 3D small-strain. Please use 2D or "
+                                  "default to dbfft solver.")
     lam = np.ascontiguousarray(lam, dtype=np.float64)
     mu = np.ascontiguousarray(mu, dtype=np.float64)
     if lam.ndim != d:
@@ -788,6 +822,9 @@ def _landau_simulation(d, lam, mu, v, target_strain_mask, target_values, n_steps
     if d == 3:
         solver = LandauSolver(lam, mu, solver_kw.get("precision", FP64))
         solve = lambda eb, warm: solver.solve(prm, eb, None, warm=warm, max_iter=max_iter_ls, tol=tol_ls)
+    elif solver_name == "newton_cg":
+        solver = LandauSolver2D(lam, mu, plane_mode, 0)
+        solve = lambda eb, warm: solver.solve_ncg(prm, eb, None, warm=warm, max_iter=max_iter_ls, tol=tol_ls)
     else:
         solver = LandauSolver2D(lam, mu, plane_mode, solver_kw.get("anderson_m", 5))
         # one e33 state for the whole run (the reference's e33_state dict)

@@ -646,6 +646,10 @@ def spectral_solver_landau_2d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_
                strain_capping_limit=strain_capping_limit, strain_capping_tangent_ratio=strain_capping_tangent_ratio,
                strain_capping_type=strain_capping_type, strain_capping_smooth_power=strain_capping_smooth_power,
                e33_state=e33_state)
+    if solver == "newton_cg":
+        law = dict(cap, e33_state=None)
+        return _landau_2d_newton_cg(lam, mu, (v1, v2, v3, g1, g2, g3, g4), eps_bar, eps_plastic, eps, Gamma,
+                                    max_iter, tol, law, info)
     aa_g, aa_f = [], []
     it_done = 0
     diff = np.inf
@@ -698,6 +702,76 @@ def spectral_solver_landau_2d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eps_bar, eps_
     if info is not None:
         info.update(iters=it_done, diff=diff)
     return eps, sig_out, eps.mean(axis=(0, 1)), sig_out.mean(axis=(0, 1))
+
+
+def _landau_2d_newton_cg(lam, mu, coef, eps_bar, eps_plastic, eps, Gamma, max_iter, tol, law, info):
+    """Newton-CG branch of spectral_solver_landau_2d (linear_elastic_simulator.py:
+    1011-1117): Newton steps with SciPy's bicgstab (rtol = tol, maxiter 150) on
+    A = G C_h G, G the -Gamma0 convolution with the mean subtracted, C_h the
+    central difference (h = 1e-7) of the Landau stress with a cold plane-stress
+    e33 Newton per evaluation; b = -G C_h (eps_bar - <eps>) first, then -G sigma;
+    stop when |dx| / (|eps| + 1e-20) < tol after the first step.  info gets the
+    Newton steps, the BiCGSTAB iterations in total and the last ratio."""
+    import scipy.sparse.linalg as spla
+    shape = lam.shape
+    nx, ny = shape
+
+    def sigma(e_el):
+        return stress_landau_2d(e_el, lam, mu, *coef, **law)
+
+    def G(tau):
+        th = np.zeros(shape + (2, 2), dtype=complex)
+        for i in range(2):
+            for j in range(2):
+                th[..., i, j] = fft_field(tau[..., i, j])
+        et_hat = -np.einsum("xykhij,xyij->xykh", Gamma, th)
+        et = np.zeros(shape + (2, 2))
+        for i in range(2):
+            for j in range(2):
+                et[..., i, j] = ifft_field(et_hat[..., i, j])
+                et[..., i, j] -= et[..., i, j].mean()
+        return et
+
+    h = 1e-7
+    state = {"eps": eps}
+
+    def C_h(d):
+        e_el = state["eps"] - eps_plastic if eps_plastic is not None else state["eps"]
+        return (sigma(e_el + h * d) - sigma(e_el - h * d)) / (2.0 * h)
+
+    def matvec(x):
+        dc = G(x.reshape(nx, ny, 2, 2))
+        return G(C_h(dc)).reshape(-1)
+
+    n = eps.size
+    A = spla.LinearOperator(shape=(n, n), matvec=matvec, dtype="float64")
+    dbar = eps_bar - eps.mean(axis=(0, 1))
+    dbar_grid = np.einsum("ij,xy->xyij", dbar, np.ones((nx, ny)))
+    e_el = eps - eps_plastic if eps_plastic is not None else eps
+    sig = sigma(e_el)
+    krylov = 0
+    newton = 0
+    res = np.inf
+    for it in range(max_iter):
+        rhs = -G(C_h(dbar_grid)).reshape(-1) if it == 0 else -G(sig).reshape(-1)
+        count = [0]
+
+        def cb(_xk):
+            count[0] += 1
+        dx, _ = spla.bicgstab(A, rhs, rtol=tol, maxiter=150, callback=cb)
+        krylov += count[0]
+        d = dx.reshape(nx, ny, 2, 2)
+        eps = eps + dbar_grid + G(d) if it == 0 else eps + G(d)
+        state["eps"] = eps
+        e_el = eps - eps_plastic if eps_plastic is not None else eps
+        sig = sigma(e_el)
+        res = np.linalg.norm(dx) / (np.linalg.norm(eps) + 1e-20)
+        newton = it + 1
+        if res < tol and it > 0:
+            break
+    if info is not None:
+        info.update(iters=newton, krylov=krylov, diff=res)
+    return eps, sig, eps.mean(axis=(0, 1)), sig.mean(axis=(0, 1))
 
 
 # ---------------------------------------------------------------------------
@@ -946,7 +1020,7 @@ def secant_simulation(d, law, lam, mu, p1, p2, target_strain_mask, target_values
 
 def landau_simulation(d, lam, mu, v1, v2, v3, g1, g2, g3, g4, target_strain_mask, target_values, n_steps=20,
                       pixel=1.0, tol_ls=1e-6, max_iter_ls=400, tol_macro=1e-4, max_iter_macro=20, store=True,
-                      plane_mode="plane_strain", anderson_m=5, **cap):
+                      plane_mode="plane_strain", anderson_m=5, solver="dbfft", **cap):
     """Mixed stress/strain load stepping of the Landau law
     (linear_elastic_simulator.py:2055-2284 in 2D, :2287-2516 in 3D): secant
     slope correction, compliance of (mean lam, mean mu); in 2D plane stress
@@ -963,7 +1037,7 @@ def landau_simulation(d, lam, mu, v1, v2, v3, g1, g2, g3, g4, target_strain_mask
             r = spectral_solver_landau_3d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eb, **kw)
         else:
             r = spectral_solver_landau_2d(lam, mu, v1, v2, v3, g1, g2, g3, g4, eb, plane_mode=plane_mode,
-                                          e33_state=e33_state, anderson_m=anderson_m, **kw)
+                                          e33_state=e33_state, anderson_m=anderson_m, solver=solver, **kw)
         return r + (info["iters"],)
     return _macro_stepping(d, solve, E_avg, nu_avg, target_strain_mask, target_values, n_steps, tol_macro,
                            max_iter_macro, store, True)

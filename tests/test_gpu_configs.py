@@ -63,26 +63,35 @@ def test_cfg0_aqs_64_matches_fixture(gpu, tmp_path):
     assert abs(sim.eps_macro[0, 0] - 0.02) < 1e-12
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_cfg1_kmc_128_matches_fixture(gpu, tmp_path, prec):
-    # FP64: the oracle's 500-event sequence exactly, fields to 1e-10; the FP32 +
-    # FP64-refinement form: its field error against the FP64 oracle stated
+def _cfg1_run(prec, out_dir, max_events):
     import cfg_inputs as ci
     from mgkmc_amd.aqs import ThermalSimulation
-    f = _fixture("cfg1_kmc_128.npz")
     c = ci.CFG1
     n = c["n"]
     E, nu, ep = ci.cfg1_inputs(n)
     sim = ThermalSimulation(n, n, n, c["M"], c["gamma0"], E, nu, pixel=c["pixel"], barrier_generator="gaussian",
                             barrier_kwargs=ci.BARRIERS, temperature=c["temperature"], strain_rate=c["strain_rate"],
-                            output_dir=str(tmp_path), ls_precision=prec, rng=np.random.RandomState(c["seed"]),
+                            output_dir=str(out_dir), ls_precision=prec, rng=np.random.RandomState(c["seed"]),
                             selection_stream=np.random.RandomState(c["sel_seed"]), **ci.STZ)
     sim.eps_plastic = ep
     sim.solver_args = {"tol": c["ls_tol"]}
-    nev = len(f["events"])
     ev = sim.run_simulation(10 ** 9, c["step_size"], component=(0, 0), mixed_tol=c["mixed_tol"],
                             enable_console_log=False, enable_summary_log=False, enable_global_log=False,
-                            enable_cascade_log=False, max_events=nev)
+                            enable_cascade_log=False, max_events=max_events)
+    return sim, ev
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_cfg1_kmc_128_matches_fixture(gpu, tmp_path, prec):
+    # FP64: the oracle's 500-event sequence exactly, fields to 1e-10.  The FP32 +
+    # FP64-refinement form stops each solve at the FP64 test with tol 1e-6 while
+    # the fixture's solves run to 1e-8, so its fields differ by O(tol) and a
+    # selection whose draw falls within that of a boundary can differ; from
+    # there the sequences part.  Stated for it: the events it shares with the
+    # oracle (at least 50) and its field error against the FP64 path over them.
+    f = _fixture("cfg1_kmc_128.npz")
+    nev = len(f["events"])
+    sim, ev = _cfg1_run(prec, tmp_path, nev)
     got = [e[:4] for e in ev]
     ref = [tuple(int(v) for v in r) for r in f["events"]]
     first_diff = next((i for i, (a, b) in enumerate(zip(got, ref)) if a != b), None)
@@ -93,6 +102,7 @@ def test_cfg1_kmc_128_matches_fixture(gpu, tmp_path, prec):
     print(f"[cfg1 {prec}] {len(ev)} events (oracle {nev}), first difference at {first_diff}; "
           f"sampled sigma {e_samp:.2e}, mean sigma {e_mean:.2e}, |sigma| {abs(np.sqrt(np.sum(sig ** 2)) / f['sig_norm'] - 1):.2e}",
           flush=True)
+    assert len(ev) == nev
     if prec == "fp64":
         assert got == ref
         assert np.allclose([e[4] for e in ev], f["times"], rtol=1e-10, atol=0)
@@ -102,10 +112,16 @@ def test_cfg1_kmc_128_matches_fixture(gpu, tmp_path, prec):
         assert np.abs(sim.eps_macro - f["eps_macro"]).max() <= 1e-12 * max(1e-30, np.abs(f["eps_macro"]).max())
         assert np.abs(sim.eps_plastic.sum(axis=(0, 1, 2)) - f["eps_plastic_sum"]).max() <= 1e-9
     else:
-        # the refined LS stops at the FP64 test with tol 1e-6: the field differs by O(tol) from
-        # the FP64 path, so selections whose draw falls within that of a boundary may differ
-        assert len(ev) == nev
-        assert e_samp <= 1e-5
+        shared = nev if first_diff is None else first_diff
+        assert shared >= 50
+        k = min(shared, 100)
+        del sim
+        s64, ev64 = _cfg1_run("fp64", tmp_path, k)
+        smx, evmx = _cfg1_run("mixed", tmp_path, k)
+        assert [e[:4] for e in ev64] == [e[:4] for e in evmx] == ref[:k]
+        err = _rel(smx.sig_field, s64.sig_field)
+        print(f"[cfg1 mixed] field error vs the FP64 path after {k} shared events: sigma {err:.2e}", flush=True)
+        assert err <= 1e-5
 
 
 def _fs_residual(P):

@@ -590,3 +590,80 @@ def test_update_stress_fft_full_matches_oracle(gpu, pixel):
     assert got[0].shape == shape + (3, 3) and got[1].shape == shape + (3, 3)
     for a, b in zip(got, ref):
         assert _rel(a, b) <= 1e-10, _rel(a, b)
+
+
+# Landau 2D Newton-CG (mgkmc/linear_elastic_simulator.py:1011-1117): reduced
+# units and odd grids, where the reference's fixed-step difference tangent and
+# its Krylov solves work (the oracle's pins say why); the Krylov iterates carry
+# the difference quotient's round-off (about 1e-16 |eps| / h relative to the
+# step), which differs between the host and the device, so fields agree to the
+# solve's tolerance rather than to 1e-10
+LANDAU_RED = {k: v / 70e9 for k, v in LANDAU_PA.items()}
+
+
+@pytest.mark.parametrize("plane_mode,cap", [
+    ("plane_strain", {}),
+    ("plane_stress", {}),
+    ("plane_stress", dict(strain_capping_enabled=True, strain_capping_limit=0.03)),
+    ("plane_strain", dict(strain_capping_enabled=True, strain_capping_type="smooth", strain_capping_smooth_power=2.0)),
+])
+def test_landau2d_newton_cg_parity(gpu, plane_mode, cap):
+    from mgkmc_amd.linear_elastic_simulator import spectral_solver_landau_2d
+    shape = (25, 15)  # odd extents with factors 3 and 5 (the FFT radices)
+    rng = np.random.default_rng(41)
+    lam, mu = ol.compute_lame_2d(rng.uniform(0.8, 1.2, shape), np.full(shape, 0.3), "plane_strain")
+    g = rng.standard_normal(shape + (2, 2)) * 2e-3
+    ep = 0.5 * (g + np.swapaxes(g, -1, -2))
+    eb = np.array([[0.01, 0.02], [0.02, -0.004]])
+    info = {}
+    ref = ol.spectral_solver_landau_2d(lam, mu, **LANDAU_RED, eps_bar=eb, eps_plastic=ep, tol=1e-6, plane_mode=plane_mode,
+                                       solver="newton_cg", info=info, **cap)
+    got = spectral_solver_landau_2d(lam, mu, **LANDAU_RED, eps_bar=eb, eps_plastic=ep, tol=1e-6, plane_mode=plane_mode,
+                                    solver="newton_cg", **cap)
+    e = [_rel(a, b) for a, b in zip(got, ref)]
+    print(f"[landau2d ncg] {plane_mode} {cap}: Newton {info['iters']}, Krylov {info['krylov']}; "
+          f"eps {e[0]:.2e} sig {e[1]:.2e}", flush=True)
+    assert e[0] < 1e-6 and e[1] < 1e-6 and e[2] < 1e-12 and e[3] < 1e-6
+    # warm start from the solution with a larger load
+    ref2 = ol.spectral_solver_landau_2d(lam, mu, **LANDAU_RED, eps_bar=1.2 * eb, eps_plastic=ep, tol=1e-6,
+                                        plane_mode=plane_mode, solver="newton_cg", eps_init=ref[0], **cap)
+    got2 = spectral_solver_landau_2d(lam, mu, **LANDAU_RED, eps_bar=1.2 * eb, eps_plastic=ep, tol=1e-6,
+                                     plane_mode=plane_mode, solver="newton_cg", eps_init=got[0], **cap)
+    assert _rel(got2[0], ref2[0]) < 1e-6 and _rel(got2[1], ref2[1]) < 1e-6
+
+
+def test_landau2d_newton_cg_counts_and_driver(gpu):
+    # Newton steps equal to the oracle's, Krylov totals within 2x; the 2D
+    # load-stepping driver routes solver="newton_cg" to this solver
+    from mgkmc_amd import linear_elastic_simulator as les
+    from mgkmc_amd.linear_elastic_simulator import LandauSolver2D, landau_params
+    from mgkmc_amd import device as dev
+    shape = (15, 27)
+    rng = np.random.default_rng(43)
+    lam, mu = ol.compute_lame_2d(rng.uniform(0.8, 1.2, shape), np.full(shape, 0.3), "plane_strain")
+    eb = np.array([[0.012, 0.01], [0.01, 0.0]])
+    info = {}
+    ol.spectral_solver_landau_2d(lam, mu, **LANDAU_RED, eps_bar=eb, tol=1e-6, solver="newton_cg", info=info)
+    s = LandauSolver2D(lam, mu, "plane_strain", 0)
+    inf = s.solve_ncg(landau_params(**LANDAU_RED, mu=mu), eb, None, tol=1e-6)
+    print(f"[landau2d ncg] Newton {inf.iters} (oracle {info['iters']}), Krylov {s.krylov_iters} "
+          f"(oracle {info['krylov']})", flush=True)
+    # Krylov totals: each solve runs until the difference quotient's round-off
+    # stalls the residual near rtol, so the counts follow that round-off
+    assert inf.status == 0 and inf.iters == info["iters"]
+    assert 0.5 * info["krylov"] <= s.krylov_iters <= 2 * info["krylov"]
+    # the load-stepping driver with solver="newton_cg" solves each step with this
+    # solver, warm-started from the last step's field: the same device
+    # arithmetic as the solver called step by step (stress targets would drive
+    # the macro corrections into right-hand sides so small that the difference
+    # quotient along G(s) underflows, in the reference too)
+    mask = np.ones((2, 2), dtype=bool)
+    tv = np.array([[0.02, 0.005], [0.005, -0.01]])
+    got = les.landau_elastic_simulation_2d(lam, mu, **LANDAU_RED, target_strain_mask=mask, target_values=tv, n_steps=1,
+                                           tol_ls=1e-6, tol_macro=1e-5, solver="newton_cg")
+    prm = landau_params(**LANDAU_RED, mu=mu)
+    t = LandauSolver2D(lam, mu, "plane_strain", 0)
+    t.solve_ncg(prm, 0.0 * tv, None, tol=1e-6)
+    inf1 = t.solve_ncg(prm, tv, None, warm=True, tol=1e-6)
+    assert np.array_equal(got[2][1], dev.from_sym_soa(dev.download(t.eps), 2))
+    assert np.array_equal(got[1][1], np.array(inf1.sig_mean[:4]).reshape(2, 2))

@@ -261,6 +261,90 @@ def test_landau_2d_anderson_reaches_the_picard_fixed_point():
     assert np.abs(a[1] - b[1]).max() < 1e-7 * np.abs(b[1]).max()
 
 
+# The Newton-CG branch differentiates the stress by central differences with a
+# fixed h = 1e-7 along G(p); with moduli in Pa, G(p) of a Krylov vector is
+# ~1e-11 of p, so h G(p) falls below the rounding unit of the strain and the
+# differenced stress is zero (the reference's BiCGSTAB then stalls at maxiter).
+# Its tests therefore run in reduced units (moduli and Landau constants / 70 GPa)
+# and on odd grids: on an even grid -Gamma0 at the Nyquist frequencies is not a
+# projector (G(C0 G(t)) != -G(t) there), so the Newton-CG and the fixed point
+# settle on different fields and the Krylov solves stall.
+LANDAU_RED = {k: v / 70e9 for k, v in LANDAU_PA.items()}
+
+
+def _landau_red_case(seed, shape):
+    rng = np.random.default_rng(seed)
+    E = rng.uniform(0.8, 1.2, shape)
+    lam, mu = ol.compute_lame_2d(E, np.full(shape, 0.3), "plane_strain")
+    g = rng.standard_normal(shape + (2, 2)) * 2e-3
+    return lam, mu, 0.5 * (g + np.swapaxes(g, -1, -2))
+
+
+@pytest.mark.parametrize("plane_mode,cap", [("plane_strain", {}),
+                                            ("plane_stress", dict(strain_capping_enabled=True,
+                                                                  strain_capping_limit=0.03))])
+def test_landau_2d_newton_cg_reaches_the_fixed_point(plane_mode, cap):
+    # the Newton-CG branch and the plain fixed point solve one equilibrium
+    # problem: at tight tolerances they agree to the fixed point's accuracy,
+    # with far fewer Newton steps than Picard iterations; the Newton-CG field
+    # is equilibrated (-Gamma0 : sigma = 0) and keeps the prescribed mean
+    shape = (11, 9)
+    lam, mu, ep = _landau_red_case(8, shape)
+    eb = np.array([[0.01, 0.02], [0.02, -0.004]])
+    kw = dict(eps_plastic=ep, plane_mode=plane_mode, **cap)
+    inc, ip = {}, {}
+    a = ol.spectral_solver_landau_2d(lam, mu, **LANDAU_RED, eps_bar=eb, tol=1e-6, solver="newton_cg", info=inc, **kw)
+    b = ol.spectral_solver_landau_2d(lam, mu, **LANDAU_RED, eps_bar=eb, tol=1e-13, anderson_m=0, info=ip,
+                                     e33_state={}, **kw)
+    assert inc["iters"] < ip["iters"] and 0 < inc["krylov"] < 100
+    assert np.abs(a[0] - b[0]).max() < 1e-6 * np.abs(b[0]).max()
+    assert np.abs(a[1] - b[1]).max() < 1e-6 * np.abs(b[1]).max()
+    assert np.abs(a[2] - eb).max() < 1e-14
+    Gam = ol.green_operator(ol.wave_vectors(shape, 1.0), float(lam.mean()), float(mu.mean()))
+    th = np.stack([np.stack([ol.fft_field(a[1][..., i, j]) for j in range(2)], -1) for i in range(2)], -2)
+    t0 = np.stack([np.stack([ol.fft_field(ol.stress_landau_2d(np.broadcast_to(eb, shape + (2, 2)) - ep, lam, mu,
+                                                              **LANDAU_RED, plane_mode=plane_mode, **cap)[..., i, j])
+                             for j in range(2)], -1) for i in range(2)], -2)
+    r = np.einsum("xykhij,xyij->xykh", Gam, th)
+    r0 = np.einsum("xykhij,xyij->xykh", Gam, t0)
+    assert np.linalg.norm(r) < 1e-6 * np.linalg.norm(r0)
+
+
+def test_landau_2d_newton_cg_linear_limit():
+    # zero Landau coefficients: the law is linear, C_h is exact up to the
+    # differencing round-off, and one Newton step after the (empty) first
+    # reaches the linear LS solution
+    shape = (9, 13)
+    lam, mu, ep = _landau_red_case(4, shape)
+    eb = np.array([[0.01, 0.0], [0.0, 0.0]])
+    zero = {k: 0.0 for k in LANDAU_PA}
+    info = {}
+    a = ol.spectral_solver_landau_2d(lam, mu, **zero, eps_bar=eb, eps_plastic=ep, tol=1e-6, solver="newton_cg",
+                                     info=info)
+    b = ol.spectral_solver_landau_2d(lam, mu, **zero, eps_bar=eb, eps_plastic=ep, tol=1e-13, anderson_m=0)
+    assert info["iters"] <= 3
+    assert np.abs(a[0] - b[0]).max() < 1e-7 * np.abs(b[0]).max()
+
+
+def test_landau_2d_newton_cg_fd_underflow_in_pa_units():
+    # the reference's fixed-step difference along G(p) with moduli in Pa: the
+    # step h G(p) is below the strain's rounding unit and C_h returns zeros
+    shape = (8, 6)
+    rng = np.random.default_rng(2)
+    lam, mu = ol.compute_lame_2d(70e9 * rng.uniform(0.8, 1.2, shape), np.full(shape, 0.3), "plane_strain")
+    Gam = ol.green_operator(ol.wave_vectors(shape, 1.0), float(lam.mean()), float(mu.mean()))
+    e = np.broadcast_to(np.array([[0.01, 0.0], [0.0, 0.0]]), shape + (2, 2)).copy()
+    p = rng.standard_normal(shape + (2, 2)) * 1e-2
+    p = 0.5 * (p + np.swapaxes(p, -1, -2))
+    th = np.stack([np.stack([ol.fft_field(p[..., i, j]) for j in range(2)], -1) for i in range(2)], -2)
+    gh = -np.einsum("xykhij,xyij->xykh", Gam, th)
+    d = np.stack([np.stack([ol.ifft_field(gh[..., i, j]) for j in range(2)], -1) for i in range(2)], -2)
+    assert np.abs(1e-7 * d).max() < 0.5 * np.spacing(0.01)
+    sp = ol.stress_landau_2d(e + 1e-7 * d, lam, mu, **LANDAU_PA)
+    sm = ol.stress_landau_2d(e - 1e-7 * d, lam, mu, **LANDAU_PA)
+    assert np.all((sp - sm)[..., 0, 0] == 0.0)
+
+
 # ---------------------------------------------------------------------------
 # secant-degradation and Rose laws (oracle/ls.py stress_secant, stress_rose_2d)
 # ---------------------------------------------------------------------------
