@@ -277,6 +277,26 @@ def gpu_serving(n_requests, rate, events, slots=16, chunk=10):
     return out
 
 
+def gpu_serving_burst(shape=(64, 64), n=64, events=100):
+    """n requests of one small grid arriving at once (refined LS, 100 events
+    each), served twice: continuously batched (groups of up to 8 stepping in
+    lockstep, one launch per sweep and pass) and unbatched (one worker thread
+    and stream per request, 16 at a time): the throughput batching adds where
+    single simulations are launch-latency bound."""
+    from mgkmc_amd.serving import SimServer, SimRequest, summarize
+    out = {"workload": f"{n} requests of {'x'.join(map(str, shape))} at t = 0, {events} events each, refined LS"}
+    for mode in ("batched", "unbatched"):
+        reqs = [SimRequest(i, 0.0, tuple(shape), 0.02 + 0.01 * (i % 7), 500 + 31 * i, events=events) for i in range(n)]
+        srv = SimServer(max_active=16, batch=(mode == "batched"))
+        res = srv.serve(reqs)
+        sm = summarize(res, srv.wall)
+        out[mode] = {"kmc_events_per_s": sm["kmc_events_per_s"], "wall_s": sm["wall_s"],
+                     "latency_p50_s": sm["latency_p50_s"], "latency_p99_s": sm["latency_p99_s"],
+                     "failed": sm["failed"]}
+    out["batched_over_unbatched"] = out["batched"]["kmc_events_per_s"] / out["unbatched"]["kmc_events_per_s"]
+    return out
+
+
 def copy_ceiling(gib=2.0, reps=10):
     """Streaming device copy (mgk_stream_copy, 16-byte accesses) of `gib` GiB,
     timed with events on the stream it runs on: the practical read + write HBM
@@ -431,6 +451,7 @@ def main():
     cfg3 = gpu_config3(512, a.config3_events) if a.config3_events > 0 else None
     log("serving stream")
     srv = gpu_serving(a.serve_requests, a.serve_rate, a.serve_events) if a.serve_requests > 0 else None
+    burst = gpu_serving_burst() if a.serve_requests > 0 else None
     roof = None
     if dom:
         name, (ms, by, cnt) = dom
@@ -500,6 +521,7 @@ def main():
             "workload": "same increments, FP64 Newton loop + FP32 BiCGSTAB at tol_CG 1e-6"},
         "config3_512": cfg3,
         "serving": srv,
+        "serving_burst": burst,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -124,6 +124,25 @@ struct Pinned {
 thread_local Pinned g_pin;
 }  // namespace
 
+namespace {
+struct DevStage {
+  void* p[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t n[4] = {0, 0, 0, 0};
+};
+thread_local DevStage g_dstage;
+}  // namespace
+
+void* mgk_device_stage(int slot, size_t bytes) {
+  if (slot < 0 || slot > 3) return nullptr;
+  if (g_dstage.n[slot] >= bytes) return g_dstage.p[slot];
+  void* np_ = nullptr;
+  const size_t want = bytes < 4096 ? 4096 : bytes;
+  if (hipMalloc(&np_, want) != hipSuccess) return nullptr;
+  g_dstage.p[slot] = np_;
+  g_dstage.n[slot] = want;
+  return np_;
+}
+
 void* mgk_pinned(size_t bytes) {
   if (g_pin.n >= bytes) return g_pin.p;
   if (g_pin.p) (void)hipHostFree(g_pin.p);

@@ -40,14 +40,26 @@ size_t mgk_lds_limit();
 // Pinned host scratch (per thread, grown on demand) for small D2H reads.
 void* mgk_pinned(size_t bytes);
 
-// Rate pass and selection of one simulation of the event loop, enqueued on s
-// with their small results copied into pinned host memory (host_total: one
-// double; host_rec: mgk_kmc_sel_rec_bytes() bytes, read with
-// mgk_kmc_sel_unpack after the stream has been synchronised).
-int mgk_kmc_rates_async(const mgk_kmc_loop* L, double time, double* host_total, hipStream_t s);
+// Device staging for the batched paths' small arrays (totals, selections,
+// stop flags, solve states, redraws): per thread and slot (0..3), grown on
+// demand; a smaller earlier block is dropped rather than freed, since hipFree
+// would synchronise the device under other streams' work.
+void* mgk_device_stage(int slot, size_t bytes);
+
+// Batched passes of the event step (members share ndim, shape, M, sig_kind):
+// the rate pass with every member's total into dev_tot[k]; the selection with
+// every member's record (mgk_kmc_sel_rec_bytes() each, read back with
+// mgk_kmc_sel_unpack) into dev_rec; the flips of (v[k], m[k]) at times[k] and
+// the redraw of those voxels from dev_redraw + k * stride (2M directions, then
+// M barriers).  Enqueued on s; nothing is synchronised.
+int mgk_kmc_rates_batch(int nmem, const mgk_kmc_loop* const* L, const double* times, double* dev_tot,
+                        hipStream_t s);
 size_t mgk_kmc_sel_rec_bytes();
-int mgk_kmc_select_async(const mgk_kmc_loop* L, double time, double u, void* host_rec, hipStream_t s);
+int mgk_kmc_select_batch(int nmem, const mgk_kmc_loop* const* L, const double* times, const double* u,
+                         void* dev_rec, hipStream_t s);
 void mgk_kmc_sel_unpack(const void* host_rec, mgk_kmc_sel* out);
+int mgk_kmc_flip_redraw_batch(int nmem, const mgk_kmc_loop* const* L, const long long* v, const int* m,
+                              const double* times, const double* dev_redraw, size_t stride, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Live per-kernel timing (HIP events on the launching stream).  Off by default;

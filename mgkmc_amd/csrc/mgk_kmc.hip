@@ -159,10 +159,10 @@ __device__ __forceinline__ double kmc_rate(double q, double beta, const KmcParam
 // written only when requested (the drop-in compute_barrier needs it; the
 // device KMC loop does not).
 template <int D, int CK, int SIGK, typename TS>
-__global__ __launch_bounds__(MGK_NT) void k_kmc_barrier_rates(
+__device__ __forceinline__ void kmc_rates_block(
     size_t NV, int M, const TS* __restrict__ sig, const double* __restrict__ Q0, const double* __restrict__ cat,
     const double* __restrict__ soft, const double* __restrict__ tlast, const double* __restrict__ prev,
-    const double* __restrict__ Tfield, KmcParams P, double* __restrict__ Q, double* __restrict__ Rv,
+    const double* __restrict__ Tfield, const KmcParams& P, double* __restrict__ Q, double* __restrict__ Rv,
     double* __restrict__ blocksums) {
   const size_t MNV = (size_t)M * NV;
   __shared__ double red[MGK_NT / 64];
@@ -187,6 +187,15 @@ __global__ __launch_bounds__(MGK_NT) void k_kmc_barrier_rates(
   double acc[1] = {bsum};
   block_sum<1>(acc, red);
   if (threadIdx.x == 0) blocksums[blockIdx.x] = acc[0];
+}
+
+template <int D, int CK, int SIGK, typename TS>
+__global__ __launch_bounds__(MGK_NT) void k_kmc_barrier_rates(
+    size_t NV, int M, const TS* __restrict__ sig, const double* __restrict__ Q0, const double* __restrict__ cat,
+    const double* __restrict__ soft, const double* __restrict__ tlast, const double* __restrict__ prev,
+    const double* __restrict__ Tfield, KmcParams P, double* __restrict__ Q, double* __restrict__ Rv,
+    double* __restrict__ blocksums) {
+  kmc_rates_block<D, CK, SIGK, TS>(NV, M, sig, Q0, cat, soft, tlast, prev, Tfield, P, Q, Rv, blocksums);
 }
 
 // Rates of an existing Q field (drop-in compute_rates): per-mode rate array in
@@ -376,11 +385,12 @@ __device__ void first_reaching(const double* __restrict__ a, size_t n, const dou
 // same code as the fused pass.  r = total * u with total read from `total`
 // (device) so the host and device use one value.
 template <int D, int CK, int SIGK, typename TS>
-__global__ __launch_bounds__(MGK_NT) void k_kmc_select_voxel(
+__device__ __forceinline__ void kmc_select_block(
     size_t NV, int M, const double* __restrict__ Rv, const double* __restrict__ bs, int nb,
     const double* __restrict__ total, double u, const TS* __restrict__ sig, const double* __restrict__ Q0,
     const double* __restrict__ cat, const double* __restrict__ soft, const double* __restrict__ tlast,
-    const double* __restrict__ prev, const double* __restrict__ Tfield, KmcParams P, KmcSel* __restrict__ out) {
+    const double* __restrict__ prev, const double* __restrict__ Tfield, const KmcParams& P,
+    KmcSel* __restrict__ out, KmcSel* __restrict__ out2) {
   __shared__ long long item;
   __shared__ double before;
   double r = (*total) * u;
@@ -427,13 +437,26 @@ __global__ __launch_bounds__(MGK_NT) void k_kmc_select_voxel(
     ca = c;
     fall = fall | 2;
   }
-  out->item = v;
-  out->mode = mode;
-  out->fallthrough = fall;
-  out->r = r;
-  out->cum_before = cb;
-  out->cum_at = ca;
-  out->q = qsel;
+  KmcSel o;
+  o.item = v;
+  o.mode = mode;
+  o.fallthrough = fall;
+  o.r = r;
+  o.cum_before = cb;
+  o.cum_at = ca;
+  o.q = qsel;
+  *out = o;
+  if (out2) *out2 = o;
+}
+
+template <int D, int CK, int SIGK, typename TS>
+__global__ __launch_bounds__(MGK_NT) void k_kmc_select_voxel(
+    size_t NV, int M, const double* __restrict__ Rv, const double* __restrict__ bs, int nb,
+    const double* __restrict__ total, double u, const TS* __restrict__ sig, const double* __restrict__ Q0,
+    const double* __restrict__ cat, const double* __restrict__ soft, const double* __restrict__ tlast,
+    const double* __restrict__ prev, const double* __restrict__ Tfield, KmcParams P, KmcSel* __restrict__ out) {
+  kmc_select_block<D, CK, SIGK, TS>(NV, M, Rv, bs, nb, total, u, sig, Q0, cat, soft, tlast, prev, Tfield, P, out,
+                                    nullptr);
 }
 
 // Selection over a plain rates array (drop-in select_event): r given.
@@ -558,9 +581,10 @@ struct KmcFlip {
 };
 
 template <int D, int CK, typename TE>
-__global__ void k_kmc_flip(KmcFlip F, const double* __restrict__ cat, TE* __restrict__ epsp,
-                           float* __restrict__ shadow, double* __restrict__ soft, double* __restrict__ tlast, double* __restrict__ prev,
-                           double* __restrict__ gamma_out) {
+__device__ __forceinline__ void kmc_flip_body(const KmcFlip& F, const double* __restrict__ cat, TE* __restrict__ epsp,
+                                              float* __restrict__ shadow, double* __restrict__ soft,
+                                              double* __restrict__ tlast, double* __restrict__ prev,
+                                              double* __restrict__ gamma_out) {
   constexpr int DD = D * D;
   const size_t NV = (size_t)F.nx * F.ny * F.nz;
   const size_t MNV = (size_t)F.M * NV;
@@ -652,12 +676,20 @@ __global__ void k_kmc_flip(KmcFlip F, const double* __restrict__ cat, TE* __rest
   }
 }
 
+template <int D, int CK, typename TE>
+__global__ void k_kmc_flip(KmcFlip F, const double* __restrict__ cat, TE* __restrict__ epsp,
+                           float* __restrict__ shadow, double* __restrict__ soft, double* __restrict__ tlast,
+                           double* __restrict__ prev, double* __restrict__ gamma_out) {
+  kmc_flip_body<D, CK, TE>(F, cat, epsp, shadow, soft, tlast, prev, gamma_out);
+}
+
 #pragma clang fp contract(on)
 
 // write new catalog parameters and Q0 for all modes of voxel v
-__global__ void k_kmc_set_voxel(size_t NV, int M, int CK, long long v, const double* __restrict__ newcat,
-                                const double* __restrict__ newq0, double* __restrict__ cat,
-                                double* __restrict__ Q0) {
+__device__ __forceinline__ void kmc_set_voxel_body(size_t NV, int M, int CK, long long v,
+                                                   const double* __restrict__ newcat,
+                                                   const double* __restrict__ newq0, double* __restrict__ cat,
+                                                   double* __restrict__ Q0) {
   const size_t MNV = (size_t)M * NV;
   for (int i = threadIdx.x; i < M * CK; i += blockDim.x) {
     const int k = i / M, m = i % M;
@@ -665,6 +697,79 @@ __global__ void k_kmc_set_voxel(size_t NV, int M, int CK, long long v, const dou
   }
   for (int m = threadIdx.x; m < M; m += blockDim.x)
     if (newq0) Q0[(size_t)m * NV + v] = newq0[m];
+}
+
+__global__ void k_kmc_set_voxel(size_t NV, int M, int CK, long long v, const double* __restrict__ newcat,
+                                const double* __restrict__ newq0, double* __restrict__ cat,
+                                double* __restrict__ Q0) {
+  kmc_set_voxel_body(NV, M, CK, v, newcat, newq0, cat, Q0);
+}
+
+// ---------------------------------------------------------------------------
+// Batched event-step passes: up to MGK_MAXB simulations of one grid, one
+// launch per pass, member from the grid (rates: blockIdx.y, the others:
+// blockIdx.x).  Each member runs the body of the single kernel on its own
+// buffers, so its results equal those of the single launches bit for bit;
+// small results also go to contiguous staging arrays (one copy for all).
+// ---------------------------------------------------------------------------
+struct KmcMem {
+  const void* sig;
+  const double *Q0, *cat, *soft, *tlast, *prev;
+  KmcParams P;
+  double *Rv, *bs, *total;
+  KmcSel* sel;
+  double u;
+  // flip and redraw of the selected mode
+  KmcFlip F;
+  void* epsp;
+  float* shadow;
+  double* softw;
+  double* tlastw;
+  double* prevw;
+  const double *newcat, *newq0;
+  double *catw, *Q0w;
+};
+
+struct KmcBatch {
+  KmcMem m[MGK_MAXB];
+};
+
+template <int D, int CK, int SIGK, typename TS>
+__global__ __launch_bounds__(MGK_NT) void k_kmc_barrier_rates_b(size_t NV, int M, KmcBatch b) {
+  const KmcMem& x = b.m[blockIdx.y];
+  kmc_rates_block<D, CK, SIGK, TS>(NV, M, (const TS*)x.sig, x.Q0, x.cat, x.soft, x.tlast, x.prev, nullptr, x.P,
+                                   nullptr, x.Rv, x.bs);
+}
+
+__global__ __launch_bounds__(MGK_NT) void k_kmc_total_b(int nb, KmcBatch b, double* __restrict__ tot_out) {
+  const KmcMem& x = b.m[blockIdx.x];
+  __shared__ double red[MGK_NT / 64];
+  double acc[1] = {0.0};
+  for (int i = threadIdx.x; i < nb; i += MGK_NT) acc[0] += x.bs[i];
+  block_sum<1>(acc, red);
+  if (threadIdx.x == 0) {
+    x.total[0] = acc[0];
+    tot_out[blockIdx.x] = acc[0];
+  }
+}
+
+template <int D, int CK, int SIGK, typename TS>
+__global__ __launch_bounds__(MGK_NT) void k_kmc_select_voxel_b(size_t NV, int M, int nb, KmcBatch b,
+                                                                KmcSel* __restrict__ sel_out) {
+  const KmcMem& x = b.m[blockIdx.x];
+  kmc_select_block<D, CK, SIGK, TS>(NV, M, x.Rv, x.bs, nb, x.total, x.u, (const TS*)x.sig, x.Q0, x.cat, x.soft,
+                                    x.tlast, x.prev, nullptr, x.P, x.sel, sel_out + blockIdx.x);
+}
+
+template <int D, int CK, typename TE>
+__global__ void k_kmc_flip_b(KmcBatch b) {
+  const KmcMem& x = b.m[blockIdx.x];
+  kmc_flip_body<D, CK, TE>(x.F, x.catw, (TE*)x.epsp, x.shadow, x.softw, x.tlastw, x.prevw, nullptr);
+}
+
+__global__ void k_kmc_set_voxel_b(size_t NV, int M, int CK, KmcBatch b) {
+  const KmcMem& x = b.m[blockIdx.x];
+  kmc_set_voxel_body(NV, M, CK, x.F.v, x.newcat, x.newq0, x.catw, x.Q0w);
 }
 
 // ---------------------------------------------------------------------------
@@ -838,44 +943,134 @@ extern "C" int mgk_kmc_select(int ndim, long long nvox, int M, int sig_kind, int
 
 
 // ---------------------------------------------------------------------------
-// Asynchronous forms of the rate pass and the selection for the batched event
-// loop: each is enqueued with a copy of its small result into pinned host
-// memory, and the caller synchronises once for all members of a batch.  The
-// kernels and their arguments are those of mgk_kmc_barrier_rates /
-// mgk_kmc_select as the single-simulation loop calls them (planar catalog, no
-// temperature field, no Q output).
+// Batched passes of the event step for mgk_kmc_run_batch: every member shares
+// ndim, shape, M and sig_kind (the caller checks); planar catalogs, no
+// temperature field, no Q output, as the single loop calls them.
 // ---------------------------------------------------------------------------
-int mgk_kmc_rates_async(const mgk_kmc_loop* L, double time, double* host_total, hipStream_t s) {
-  const int ndim = L->ndim, cat_kind = 0, sig_kind = L->sig_kind;
+static KmcMem kmc_member(const mgk_kmc_loop* L, double time) {
   long long nvox = 1;
-  for (int i = 0; i < ndim; ++i) nvox *= L->shape[i];
+  for (int i = 0; i < L->ndim; ++i) nvox *= L->shape[i];
   mgk_kmc_params p = L->kp;
   p.time = time;
-  const KmcParams P = to_params(&p);
   const KmcWs w = kmc_ws(L->kmc_ws, (size_t)nvox);
-  KMC_DISPATCH(launch_rates, (size_t)nvox, L->M, L->sig, L->Q0, L->cat, L->soft, L->tlast, L->prev, nullptr, P,
-               nullptr, w, s);
+  KmcMem x{};
+  x.sig = L->sig;
+  x.Q0 = L->Q0;
+  x.cat = L->cat;
+  x.soft = L->soft;
+  x.tlast = L->tlast;
+  x.prev = L->prev;
+  x.P = to_params(&p);
+  x.Rv = w.Rv;
+  x.bs = w.bs;
+  x.total = w.total;
+  x.sel = w.sel;
+  return x;
+}
+
+#define KMC_BATCH_DISPATCH(FN, ...)                                                   \
+  do {                                                                                \
+    if (ndim == 3) {                                                                  \
+      if (sig_kind == 0) FN<3, 2, KMC_SIG_SYM, double>(__VA_ARGS__);                  \
+      else if (sig_kind == 1) FN<3, 2, KMC_SIG_SYM, float>(__VA_ARGS__);              \
+      else FN<3, 2, KMC_SIG_GEN, double>(__VA_ARGS__);                                \
+    } else {                                                                          \
+      if (sig_kind == 0) FN<2, 2, KMC_SIG_SYM, double>(__VA_ARGS__);                  \
+      else if (sig_kind == 1) FN<2, 2, KMC_SIG_SYM, float>(__VA_ARGS__);              \
+      else FN<2, 2, KMC_SIG_GEN, double>(__VA_ARGS__);                                \
+    }                                                                                 \
+  } while (0)
+
+template <int D, int CK, int SIGK, typename TS>
+static void launch_rates_b(size_t NV, int M, int nmem, const KmcBatch& b, double* tot, hipStream_t s) {
+  const int nb = kmc_nblocks(NV);
+  k_kmc_barrier_rates_b<D, CK, SIGK, TS><<<dim3(nb, nmem), MGK_NT, 0, s>>>(NV, M, b);
+  k_kmc_total_b<<<nmem, MGK_NT, 0, s>>>(nb, b, tot);
+}
+
+template <int D, int CK, int SIGK, typename TS>
+static void launch_select_b(size_t NV, int M, int nmem, const KmcBatch& b, KmcSel* out, hipStream_t s) {
+  k_kmc_select_voxel_b<D, CK, SIGK, TS><<<nmem, MGK_NT, 0, s>>>(NV, M, kmc_nblocks(NV), b, out);
+}
+
+int mgk_kmc_rates_batch(int nmem, const mgk_kmc_loop* const* L, const double* times, double* dev_tot,
+                        hipStream_t s) {
+  const int ndim = L[0]->ndim, sig_kind = L[0]->sig_kind;
+  long long nvox = 1;
+  for (int i = 0; i < ndim; ++i) nvox *= L[0]->shape[i];
+  KmcBatch b{};
+  for (int k = 0; k < nmem; ++k) b.m[k] = kmc_member(L[k], times[k]);
+  KMC_BATCH_DISPATCH(launch_rates_b, (size_t)nvox, L[0]->M, nmem, b, dev_tot, s);
   MGK_HIP(hipGetLastError());
-  MGK_HIP(hipMemcpyAsync(host_total, w.total, sizeof(double), hipMemcpyDeviceToHost, s));
+  return MGK_OK;
+}
+
+int mgk_kmc_select_batch(int nmem, const mgk_kmc_loop* const* L, const double* times, const double* u,
+                         void* dev_rec, hipStream_t s) {
+  const int ndim = L[0]->ndim, sig_kind = L[0]->sig_kind;
+  long long nvox = 1;
+  for (int i = 0; i < ndim; ++i) nvox *= L[0]->shape[i];
+  KmcBatch b{};
+  for (int k = 0; k < nmem; ++k) {
+    b.m[k] = kmc_member(L[k], times[k]);
+    b.m[k].u = u[k];
+  }
+  KMC_BATCH_DISPATCH(launch_select_b, (size_t)nvox, L[0]->M, nmem, b, (KmcSel*)dev_rec, s);
+  MGK_HIP(hipGetLastError());
+  return MGK_OK;
+}
+
+int mgk_kmc_flip_redraw_batch(int nmem, const mgk_kmc_loop* const* L, const long long* v, const int* m,
+                              const double* times, const double* dev_redraw, size_t stride, hipStream_t s) {
+  const int ndim = L[0]->ndim, M = L[0]->M, prec = L[0]->eps_precision;
+  long long nvox = 1;
+  for (int i = 0; i < ndim; ++i) nvox *= L[0]->shape[i];
+  KmcBatch b{};
+  for (int k = 0; k < nmem; ++k) {
+    const mgk_kmc_loop& x = *L[k];
+    if (x.eps_precision != prec || (x.eps_precision != 0 && x.epsp_shadow) || v[k] < 0 || v[k] >= nvox ||
+        m[k] < 0 || m[k] >= M) {
+      mgk_set_error("mgk_kmc_run_batch: flip of a member out of range or with mixed eps_p precisions", __FILE__,
+                    __LINE__);
+      return MGK_ERR_ARG;
+    }
+    KmcMem& y = b.m[k];
+    KmcFlip& F = y.F;
+    F.nx = (int)x.shape[0];
+    F.ny = (int)x.shape[1];
+    F.nz = ndim == 3 ? (int)x.shape[2] : 1;
+    F.M = M;
+    F.v = v[k];
+    F.m = m[k];
+    F.time = times[k];
+    F.jp = x.jp;
+    F.jt = x.jt;
+    F.gmax = x.kp.cap;
+    F.jn = x.jn;
+    F.eps_div = 1;
+    y.epsp = x.epsp;
+    y.shadow = x.epsp_shadow;
+    y.softw = x.soft;
+    y.tlastw = x.tlast;
+    y.prevw = x.prev;
+    y.catw = x.cat;
+    y.Q0w = x.Q0;
+    y.newcat = x.redraw_directions ? dev_redraw + (size_t)k * stride : nullptr;
+    y.newq0 = x.redraw_barriers ? dev_redraw + (size_t)k * stride + 2 * (size_t)M : nullptr;
+  }
+  if (ndim == 3) {
+    if (prec == 0) k_kmc_flip_b<3, 2, double><<<nmem, 64, 0, s>>>(b);
+    else k_kmc_flip_b<3, 2, float><<<nmem, 64, 0, s>>>(b);
+  } else {
+    if (prec == 0) k_kmc_flip_b<2, 2, double><<<nmem, 64, 0, s>>>(b);
+    else k_kmc_flip_b<2, 2, float><<<nmem, 64, 0, s>>>(b);
+  }
+  k_kmc_set_voxel_b<<<nmem, 64, 0, s>>>((size_t)nvox, M, 2, b);
+  MGK_HIP(hipGetLastError());
   return MGK_OK;
 }
 
 size_t mgk_kmc_sel_rec_bytes() { return sizeof(KmcSel); }
-
-int mgk_kmc_select_async(const mgk_kmc_loop* L, double time, double u, void* host_rec, hipStream_t s) {
-  const int ndim = L->ndim, cat_kind = 0, sig_kind = L->sig_kind;
-  long long nvox = 1;
-  for (int i = 0; i < ndim; ++i) nvox *= L->shape[i];
-  mgk_kmc_params p = L->kp;
-  p.time = time;
-  const KmcParams P = to_params(&p);
-  const KmcWs w = kmc_ws(L->kmc_ws, (size_t)nvox);
-  KMC_DISPATCH(launch_select, (size_t)nvox, L->M, u, L->sig, L->Q0, L->cat, L->soft, L->tlast, L->prev, nullptr, P,
-               w, s);
-  MGK_HIP(hipGetLastError());
-  MGK_HIP(hipMemcpyAsync(host_rec, w.sel, sizeof(KmcSel), hipMemcpyDeviceToHost, s));
-  return MGK_OK;
-}
 
 void mgk_kmc_sel_unpack(const void* host_rec, mgk_kmc_sel* out) {
   const KmcSel* h = (const KmcSel*)host_rec;

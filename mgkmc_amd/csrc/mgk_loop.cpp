@@ -320,6 +320,12 @@ extern "C" int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* L, mgk_mt_state* rn
     }
     Mmax = std::max(Mmax, x.M);
   }
+  for (int b = 1; b < nb; ++b) {
+    if (L[b].M != L0.M || L[b].sig_kind != L0.sig_kind || L[b].eps_precision != L0.eps_precision) {
+      mgk_set_error("mgk_kmc_run_batch: members differ in M, sig_kind or eps_p precision", __FILE__, __LINE__);
+      return MGK_ERR_ARG;
+    }
+  }
   hipStream_t s = (hipStream_t)stream;
   long long NV = 1;
   int shape_i[3] = {1, 1, 1};
@@ -327,14 +333,21 @@ extern "C" int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* L, mgk_mt_state* rn
     NV *= L0.shape[i];
     shape_i[i] = (int)L0.shape[i];
   }
-  const size_t rec = (mgk_kmc_sel_rec_bytes() + 63) & ~(size_t)63;
-  const size_t off_rec = 64 * (size_t)MGK_LS_MAX_BATCH, off_red = off_rec + rec * MGK_LS_MAX_BATCH;
-  char* stage = (char*)batch_stage(off_red + sizeof(double) * 3 * (size_t)Mmax * MGK_LS_MAX_BATCH);
-  if (!stage) {
-    mgk_set_error("mgk_kmc_run_batch: pinned host allocation failed", __FILE__, __LINE__);
+  // staging, host (pinned) and device with one layout: totals, selection
+  // records (side by side), redraws (3 Mmax per member)
+  const size_t rec = mgk_kmc_sel_rec_bytes();
+  const size_t off_rec = 64 * (size_t)MGK_LS_MAX_BATCH, off_red = off_rec + ((rec * MGK_LS_MAX_BATCH + 255) & ~(size_t)255);
+  const size_t red_stride = 3 * (size_t)Mmax;
+  const size_t stage_bytes = off_red + sizeof(double) * red_stride * MGK_LS_MAX_BATCH;
+  char* stage = (char*)batch_stage(stage_bytes);
+  char* dstage = (char*)mgk_device_stage(1, stage_bytes);
+  if (!stage || !dstage) {
+    mgk_set_error("mgk_kmc_run_batch: staging allocation failed", __FILE__, __LINE__);
     return MGK_ERR_HIP;
   }
   double* tot = (double*)stage;
+  const mgk_kmc_loop* lp[MGK_LS_MAX_BATCH];
+  int who[MGK_LS_MAX_BATCH];
   bool run[MGK_LS_MAX_BATCH], ev[MGK_LS_MAX_BATCH];
   double t0[MGK_LS_MAX_BATCH], us[MGK_LS_MAX_BATCH];
   for (int b = 0; b < nb; ++b) {
@@ -357,15 +370,25 @@ extern "C" int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* L, mgk_mt_state* rn
     int nrun = 0;
     for (int b = 0; b < nb; ++b) nrun += run[b];
     if (!nrun) break;
-    // rate passes, one read of all totals
+    // one rate pass for every running member, one read of all totals
     bool ok = true;
-    for (int b = 0; b < nb && ok; ++b)
-      if (run[b]) ok = mgk_kmc_rates_async(&L[b], time[b], &tot[b], s) == MGK_OK;
+    double tm[MGK_LS_MAX_BATCH];
+    int nr = 0;
+    for (int b = 0; b < nb; ++b)
+      if (run[b]) {
+        lp[nr] = &L[b];
+        tm[nr] = time[b];
+        who[nr++] = b;
+      }
+    ok = mgk_kmc_rates_batch(nr, lp, tm, (double*)dstage, s) == MGK_OK &&
+         hipMemcpyAsync(tot, dstage, nr * sizeof(double), hipMemcpyDeviceToHost, s) == hipSuccess;
     if (!ok || hipStreamSynchronize(s) != hipSuccess) {
       if (ok) mgk_set_error("mgk_kmc_run_batch: stream synchronisation failed", __FILE__, __LINE__);
       fail_running(MGK_ERR_HIP);
       break;
     }
+    double totb[MGK_LS_MAX_BATCH];
+    for (int k = 0; k < nr; ++k) totb[who[k]] = tot[k];
     int nev = 0;
     for (int b = 0; b < nb; ++b) {
       ev[b] = false;
@@ -374,9 +397,9 @@ extern "C" int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* L, mgk_mt_state* rn
       const mgk_kmc_loop& x = L[b];
       double* em = eps_macro + 9 * b;
       t0[b] = time[b];
-      if (tot[b] > 0) {
+      if (totb[b] > 0) {
         const double u = R.uniform();
-        const double tw = -std::log(u) / tot[b];
+        const double tw = -std::log(u) / totb[b];
         if (tw < remaining[b]) {
           ev[b] = true;
           ++nev;
@@ -394,42 +417,48 @@ extern "C" int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* L, mgk_mt_state* rn
       }
     }
     if (!nev) continue;
-    // selections, one read of all records
-    for (int b = 0; b < nb && ok; ++b)
-      if (ev[b]) ok = mgk_kmc_select_async(&L[b], t0[b], us[b], stage + off_rec + rec * b, s) == MGK_OK;
+    // one selection pass and one read of all records
+    int ne = 0;
+    double usel[MGK_LS_MAX_BATCH];
+    for (int b = 0; b < nb; ++b)
+      if (ev[b]) {
+        lp[ne] = &L[b];
+        tm[ne] = t0[b];
+        usel[ne] = us[b];
+        who[ne++] = b;
+      }
+    ok = mgk_kmc_select_batch(ne, lp, tm, usel, dstage + off_rec, s) == MGK_OK &&
+         hipMemcpyAsync(stage + off_rec, dstage + off_rec, ne * rec, hipMemcpyDeviceToHost, s) == hipSuccess;
     if (!ok || hipStreamSynchronize(s) != hipSuccess) {
       if (ok) mgk_set_error("mgk_kmc_run_batch: stream synchronisation failed", __FILE__, __LINE__);
       fail_running(MGK_ERR_HIP);
       break;
     }
-    // flips and redraws
-    long long vv[MGK_LS_MAX_BATCH];
-    int mm[MGK_LS_MAX_BATCH], ins[MGK_LS_MAX_BATCH];
-    for (int b = 0; b < nb && ok; ++b) {
-      if (!ev[b]) continue;
+    // flips and redraws: the draws on the host, one upload, one flip pass and
+    // one redraw pass
+    long long vv[MGK_LS_MAX_BATCH], vk[MGK_LS_MAX_BATCH];
+    int mm[MGK_LS_MAX_BATCH], mk[MGK_LS_MAX_BATCH], ins[MGK_LS_MAX_BATCH];
+    double* hred = (double*)(stage + off_red);
+    for (int k = 0; k < ne; ++k) {
+      const int b = who[k];
       const mgk_kmc_loop& x = L[b];
       mgk_kmc_sel so;
-      mgk_kmc_sel_unpack(stage + off_rec + rec * b, &so);
-      vv[b] = so.item;
-      mm[b] = so.mode;
+      mgk_kmc_sel_unpack(stage + off_rec + rec * k, &so);
+      vv[b] = vk[k] = so.item;
+      mm[b] = mk[k] = so.mode;
       ins[b] = so.q <= x.stability_threshold ? 1 : 0;
-      int e = mgk_kmc_flip(d, shape_i, x.M, 0, vv[b], mm[b], time[b], x.jp, x.jt, x.kp.cap, x.jn, x.cat, x.epsp,
-                           x.eps_precision, x.epsp_shadow, x.soft, x.tlast, x.prev, nullptr, 1, stream);
-      double* host = (double*)(stage + off_red) + 3 * (size_t)Mmax * b;
+      tm[k] = time[b];
       Mt R{&rng[b]};
-      redraw_draws(&x, R, host);
-      if (e == MGK_OK && hipMemcpyAsync(x.redraw_dev, host, sizeof(double) * 3 * (size_t)x.M,
-                                        hipMemcpyHostToDevice, s) != hipSuccess) {
-        mgk_set_error("mgk_kmc_run_batch: redraw upload failed", __FILE__, __LINE__);
-        e = MGK_ERR_HIP;
-      }
-      if (e == MGK_OK)
-        e = mgk_kmc_set_voxel(NV, x.M, 2, vv[b], x.redraw_directions ? x.redraw_dev : nullptr,
-                              x.redraw_barriers ? x.redraw_dev + 2 * x.M : nullptr, x.cat, x.Q0, stream);
-      if (e != MGK_OK) {
-        fail_running(e);
-        ok = false;
-      }
+      redraw_draws(&x, R, hred + red_stride * k);
+    }
+    ok = hipMemcpyAsync(dstage + off_red, hred, sizeof(double) * red_stride * ne, hipMemcpyHostToDevice, s) ==
+         hipSuccess;
+    if (!ok) mgk_set_error("mgk_kmc_run_batch: redraw upload failed", __FILE__, __LINE__);
+    int e = ok ? mgk_kmc_flip_redraw_batch(ne, lp, vk, mk, tm, (const double*)(dstage + off_red), red_stride, s)
+               : MGK_ERR_HIP;
+    if (e != MGK_OK) {
+      fail_running(e);
+      ok = false;
     }
     if (!ok) break;
     // one batched re-solve of every member with an event
@@ -459,7 +488,6 @@ extern "C" int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* L, mgk_mt_state* rn
     }
     mgk_ls_info inf[MGK_LS_MAX_BATCH] = {};
     int it32[MGK_LS_MAX_BATCH] = {};
-    int e;
     if (L0.ls_precision == 2)
       e = mgk_ls_solve_refined_batch(d, L0.shape, k, mem, L0.warm, L0.max_iter, L0.tol, L0.tol32, stream, inf, it32);
     else

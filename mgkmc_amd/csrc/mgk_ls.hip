@@ -92,6 +92,7 @@ struct LsMem {
   LsState* st;
   Sym6 eb;
   double lam0, mu0, e2_extra;
+  T* sig;  // final stress (batched solves), or NULL
 };
 
 template <typename T>
@@ -632,20 +633,32 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_finalize(LsFin f, int nparts, dou
   }
 }
 
+// the members' solve states side by side (one read for the whole batch)
+__global__ void k_ls_gather_states(LsFin f, LsState* __restrict__ out) {
+  const unsigned int* src = (const unsigned int*)f.st[blockIdx.x];
+  unsigned int* dst = (unsigned int*)(out + blockIdx.x);
+  for (int i = threadIdx.x; i < (int)(sizeof(LsState) / 4); i += blockDim.x) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // init / warm start / stress / means
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_ls_fill(T* __restrict__ eps, int C, size_t NV, Sym6 ebar) {
+__device__ __forceinline__ void ls_fill_body(T* __restrict__ eps, int C, size_t NV, const Sym6& ebar) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < NV; i += (size_t)gridDim.x * blockDim.x) {
     for (int c = 0; c < C; ++c) eps[(size_t)c * NV + i] = T(ebar.v[c]);
   }
 }
 
+template <typename T>
+__global__ void k_ls_fill(T* __restrict__ eps, int C, size_t NV, Sym6 ebar) {
+  ls_fill_body<T>(eps, C, NV, ebar);
+}
+
 // per-block FP64 sums of C components (fixed grid => deterministic)
 template <typename T>
-__global__ __launch_bounds__(MGK_NT) void k_comp_sums(const T* __restrict__ f, int C, size_t NV,
-                                                      double* __restrict__ partials) {
+__device__ __forceinline__ void comp_sums_body(const T* __restrict__ f, int C, size_t NV,
+                                               double* __restrict__ partials) {
   __shared__ double red[6 * (MGK_NT / 64)];
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (size_t i = blockIdx.x * (size_t)MGK_NT + threadIdx.x; i < NV; i += (size_t)gridDim.x * MGK_NT) {
@@ -658,9 +671,15 @@ __global__ __launch_bounds__(MGK_NT) void k_comp_sums(const T* __restrict__ f, i
     for (int c = 0; c < C; ++c) partials[(size_t)blockIdx.x * C + c] = acc[c];
 }
 
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_comp_sums(const T* __restrict__ f, int C, size_t NV,
+                                                      double* __restrict__ partials) {
+  comp_sums_body<T>(f, C, NV, partials);
+}
+
 // means[off + c] = sum(partials[:, c]) / NV
-__global__ __launch_bounds__(MGK_NT) void k_means_finalize(const double* __restrict__ partials, int nparts, int C,
-                                                           double NV, double* __restrict__ out) {
+__device__ __forceinline__ void means_finalize_body(const double* __restrict__ partials, int nparts, int C,
+                                                    double NV, double* __restrict__ out) {
   __shared__ double red[12 * (MGK_NT / 64)];
   double acc[12];
 #pragma unroll
@@ -675,9 +694,14 @@ __global__ __launch_bounds__(MGK_NT) void k_means_finalize(const double* __restr
     for (int c = 0; c < C; ++c) out[c] = acc[c] / NV;
 }
 
+__global__ __launch_bounds__(MGK_NT) void k_means_finalize(const double* __restrict__ partials, int nparts, int C,
+                                                           double NV, double* __restrict__ out) {
+  means_finalize_body(partials, nparts, C, NV, out);
+}
+
 template <typename T>
-__global__ void k_ls_warm_adjust(T* __restrict__ eps, int C, size_t NV, Sym6 ebar,
-                                 const double* __restrict__ means) {
+__device__ __forceinline__ void ls_warm_adjust_body(T* __restrict__ eps, int C, size_t NV, const Sym6& ebar,
+                                                    const double* __restrict__ means) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < NV; i += (size_t)gridDim.x * blockDim.x) {
     for (int c = 0; c < C; ++c) {
       const T add = T(ebar.v[c] - means[c]);
@@ -686,12 +710,18 @@ __global__ void k_ls_warm_adjust(T* __restrict__ eps, int C, size_t NV, Sym6 eba
   }
 }
 
+template <typename T>
+__global__ void k_ls_warm_adjust(T* __restrict__ eps, int C, size_t NV, Sym6 ebar,
+                                 const double* __restrict__ means) {
+  ls_warm_adjust_body<T>(eps, C, NV, ebar, means);
+}
+
 // sig = 2 mu (eps-eps_p) + lam tr(eps-eps_p) I  (elasticity.py:12-18, 62-67),
 // plus per-block FP64 sums of eps and sig components.
 template <typename T, int D>
-__global__ __launch_bounds__(MGK_NT) void k_ls_stress(const T* __restrict__ eps, const T* __restrict__ epsp,
-                                                      const T* __restrict__ lam, const T* __restrict__ mu,
-                                                      T* __restrict__ sig, size_t NV, double* __restrict__ partials) {
+__device__ __forceinline__ void ls_stress_body(const T* __restrict__ eps, const T* __restrict__ epsp,
+                                               const T* __restrict__ lam, const T* __restrict__ mu,
+                                               T* __restrict__ sig, size_t NV, double* __restrict__ partials) {
   constexpr int C = (D == 3) ? 6 : 3;
   __shared__ double red[2 * C * (MGK_NT / 64)];
   double acc[2 * C];
@@ -718,6 +748,13 @@ __global__ __launch_bounds__(MGK_NT) void k_ls_stress(const T* __restrict__ eps,
   block_sum<2 * C>(acc, red);
   if (threadIdx.x == 0)
     for (int c = 0; c < 2 * C; ++c) partials[(size_t)blockIdx.x * 2 * C + c] = acc[c];
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(MGK_NT) void k_ls_stress(const T* __restrict__ eps, const T* __restrict__ epsp,
+                                                      const T* __restrict__ lam, const T* __restrict__ mu,
+                                                      T* __restrict__ sig, size_t NV, double* __restrict__ partials) {
+  ls_stress_body<T, D>(eps, epsp, lam, mu, sig, NV, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -1140,6 +1177,55 @@ __global__ void k_zero_words(unsigned int* __restrict__ p, size_t n) {
 }
 
 __global__ void k_widen(const float* __restrict__ a, double* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = (double)a[i];
+}
+
+// ---------------------------------------------------------------------------
+// Batched forms of the set-up, mean, stress and widening passes (member
+// blockIdx.y, or blockIdx.x for the one-block passes): each member runs the
+// single pass's body on its own buffers.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_ls_init_b(LsBatch<T> bt, int C, size_t NV, int fill) {
+  const LsMem<T>& M = bt.m[blockIdx.y];
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < (int)(sizeof(LsState) / 4); i += blockDim.x) ((unsigned int*)M.st)[i] = 0u;
+  if (fill) ls_fill_body<T>(M.eps, C, NV, M.eb);
+}
+
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_comp_sums_b(LsBatch<T> bt, int C, size_t NV) {
+  const LsMem<T>& M = bt.m[blockIdx.y];
+  comp_sums_body<T>(M.eps, C, NV, M.part);
+}
+
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_means_finalize_b(LsBatch<T> bt, int nparts, int C, double NV) {
+  const LsMem<T>& M = bt.m[blockIdx.x];
+  means_finalize_body(M.part, nparts, C, NV, M.st->mean);
+}
+
+template <typename T>
+__global__ void k_ls_warm_adjust_b(LsBatch<T> bt, int C, size_t NV) {
+  const LsMem<T>& M = bt.m[blockIdx.y];
+  ls_warm_adjust_body<T>(M.eps, C, NV, M.eb, M.st->mean);
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(MGK_NT) void k_ls_stress_b(LsBatch<T> bt, size_t NV) {
+  const LsMem<T>& M = bt.m[blockIdx.y];
+  ls_stress_body<T, D>(M.eps, M.epsp, M.lam, M.mu, M.sig, NV, M.part);
+}
+
+struct WidenBatch {
+  const float* a[MGK_MAXB];
+  double* b[MGK_MAXB];
+};
+
+__global__ void k_widen_b(WidenBatch w, size_t n) {
+  const float* __restrict__ a = w.a[blockIdx.y];
+  double* __restrict__ b = w.b[blockIdx.y];
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     b[i] = (double)a[i];
 }
@@ -1798,27 +1884,31 @@ int ls_solve_batch_t(const LsPlan& pl, int nb, const LsMemArgs<T>* a, int warm, 
     mgk_set_error("twiddle table allocation failed", __FILE__, __LINE__);
     return MGK_ERR_HIP;
   }
+  // device staging: the members' states side by side, two polling slots + the final read
+  LsState* dst = (LsState*)mgk_device_stage(0, 3 * MGK_MAXB * sizeof(LsState));
+  LsState* hst = batch_host_states();
+  if (!dst || !hst) {
+    mgk_set_error("batch staging allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
   const size_t NV = (size_t)pl.NV;
   const int nfill = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 4096);
   LsBatch<T> bt{};
   int hint = 0;
   for (int b = 0; b < nb; ++b) {
-    cplx<T>* spec = (cplx<T>*)(a[b].ws + pl.off_spec);
-    double* part = (double*)(a[b].ws + pl.off_part);
-    LsState* st = (LsState*)(a[b].ws + pl.off_state);
     double anti2 = 0.0;
     const Sym6 eb = ls_sym_bar<D>(a[b].eps_bar, &anti2);
-    bt.m[b] = LsMem<T>{a[b].lam, a[b].mu, a[b].epsp, a[b].eps, spec, part, st, eb, a[b].lam0, a[b].mu0,
-                       anti2 * (double)pl.NV};
-    k_zero_words<<<1, 64, 0, stream>>>((unsigned int*)st, sizeof(LsState) / 4);
-    if (!warm) {
-      k_ls_fill<T><<<nfill, MGK_NT, 0, stream>>>(a[b].eps, C, NV, eb);
-    } else if (warm == 1) {
-      k_comp_sums<T><<<pl.nblk_red, MGK_NT, 0, stream>>>(a[b].eps, C, NV, part);
-      k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, C, (double)NV, st->mean);
-      k_ls_warm_adjust<T><<<nfill, MGK_NT, 0, stream>>>(a[b].eps, C, NV, eb, st->mean);
-    }
+    bt.m[b] = LsMem<T>{a[b].lam, a[b].mu, a[b].epsp, a[b].eps, (cplx<T>*)(a[b].ws + pl.off_spec),
+                       (double*)(a[b].ws + pl.off_part), (LsState*)(a[b].ws + pl.off_state), eb, a[b].lam0,
+                       a[b].mu0, anti2 * (double)pl.NV, a[b].sig};
     hint = std::max(hint, ls_iter_hint(a[b].ws, 0));
+  }
+  // set-up: state cleared, eps filled with eps_bar (cold) or shifted to mean eps_bar (warm 1)
+  k_ls_init_b<T><<<dim3(nfill, nb), MGK_NT, 0, stream>>>(bt, C, NV, warm == 0 ? 1 : 0);
+  if (warm == 1) {
+    k_comp_sums_b<T><<<dim3(pl.nblk_red, nb), MGK_NT, 0, stream>>>(bt, C, NV);
+    k_means_finalize_b<T><<<nb, MGK_NT, 0, stream>>>(bt, pl.nblk_red, C, (double)NV);
+    k_ls_warm_adjust_b<T><<<dim3(nfill, nb), MGK_NT, 0, stream>>>(bt, C, NV);
   }
   MGK_HIP(hipGetLastError());
 
@@ -1862,25 +1952,21 @@ int ls_solve_batch_t(const LsPlan& pl, int nb, const LsMemArgs<T>* a, int warm, 
     ++iter_idx;
   };
 
-  LsState* hst = batch_host_states();
-  if (!hst) {
-    mgk_set_error("pinned host allocation failed", __FILE__, __LINE__);
-    return MGK_ERR_HIP;
-  }
   hipEvent_t ev[2] = {nullptr, nullptr};
   MGK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   MGK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  // chunked enqueue as in the single solve; the batch stops once every
-  // member's state reads done
+  // chunked enqueue as in the single solve; each chunk ends with one read of
+  // every member's state, and the batch stops once all of them read done
   int enq = 0, chunk = hint > 0 ? hint : 2, slot = 0;
   bool pending = false, stop = false, first = true;
   while (!stop && enq < max_iter) {
     const int k = std::min(chunk, max_iter - enq);
     for (int i = 0; i < k; ++i) enqueue_iter();
     enq += k;
+    k_ls_gather_states<<<nb, 64, 0, stream>>>(fin, dst + slot * MGK_MAXB);
     MGK_HIP(hipGetLastError());
-    for (int b = 0; b < nb; ++b)
-      MGK_HIP(hipMemcpyAsync(&hst[slot * MGK_MAXB + b], bt.m[b].st, sizeof(LsState), hipMemcpyDeviceToHost, stream));
+    MGK_HIP(hipMemcpyAsync(&hst[slot * MGK_MAXB], dst + slot * MGK_MAXB, nb * sizeof(LsState),
+                           hipMemcpyDeviceToHost, stream));
     MGK_HIP(hipEventRecord(ev[slot], stream));
     if (pending) {
       MGK_HIP(hipEventSynchronize(ev[slot ^ 1]));
@@ -1893,16 +1979,13 @@ int ls_solve_batch_t(const LsPlan& pl, int nb, const LsMemArgs<T>* a, int warm, 
     chunk = first ? 1 : std::min(chunk * 2, 16);
     first = false;
   }
-  for (int b = 0; b < nb; ++b) {
-    if (final_stress) {
-      double* part = bt.m[b].part;
-      k_ls_stress<T, D><<<pl.nblk_red, MGK_NT, 0, stream>>>(a[b].eps, a[b].epsp, a[b].lam, a[b].mu, a[b].sig, NV,
-                                                            part);
-      k_means_finalize<<<1, MGK_NT, 0, stream>>>(part, pl.nblk_red, 2 * C, (double)NV, bt.m[b].st->mean);
-    }
-    MGK_HIP(hipMemcpyAsync(&hst[b], bt.m[b].st, sizeof(LsState), hipMemcpyDeviceToHost, stream));
+  if (final_stress) {
+    k_ls_stress_b<T, D><<<dim3(pl.nblk_red, nb), MGK_NT, 0, stream>>>(bt, NV);
+    k_means_finalize_b<T><<<nb, MGK_NT, 0, stream>>>(bt, pl.nblk_red, 2 * C, (double)NV);
   }
+  k_ls_gather_states<<<nb, 64, 0, stream>>>(fin, dst + 2 * MGK_MAXB);
   MGK_HIP(hipGetLastError());
+  MGK_HIP(hipMemcpyAsync(hst, dst + 2 * MGK_MAXB, nb * sizeof(LsState), hipMemcpyDeviceToHost, stream));
   MGK_HIP(hipStreamSynchronize(stream));
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
@@ -2611,7 +2694,12 @@ extern "C" int mgk_ls_solve_refined_batch(int ndim, const int64_t* shape, int nb
   if (n2 == 0) return rc1;
   const size_t n = (size_t)p64.C * (size_t)p64.NV;
   const int nw = (int)std::min<size_t>((n + MGK_NT - 1) / MGK_NT, 8192);
-  for (int k = 0; k < n2; ++k) k_widen<<<nw, MGK_NT, 0, s>>>(a32[idx[k]].eps, a2[k].eps, n);
+  WidenBatch wb{};
+  for (int k = 0; k < n2; ++k) {
+    wb.a[k] = a32[idx[k]].eps;
+    wb.b[k] = a2[k].eps;
+  }
+  k_widen_b<<<dim3(nw, n2), MGK_NT, 0, s>>>(wb, n);
   MGK_HIP(hipGetLastError());
   mgk_ls_info i64[MGK_MAXB] = {};
   const int rc2 = ndim == 3 ? ls_solve_batch_t<double, 3>(p64, n2, a2, 2, max_iter, tol, s, i64, true)
