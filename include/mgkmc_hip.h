@@ -489,6 +489,38 @@ int mgk_kmc_run(const mgk_kmc_loop* loop, mgk_mt_state* rng, mgk_mt_state* selec
                 double* eps_macro, double* remaining, int max_events, int max_seq, int* seq,
                 mgk_kmc_event* events, int* n_events, mgk_ls_info* info, int* iters32, void* stream);
 
+/* Native cascade loop: cascade batches until no mode is unstable (mgkmc/aqs.py:480-680,
+ * mgkmc/kmc_simulator.py:577-794; small strain, linear elasticity, planar catalogs, at
+ * least one of redraw_directions / redraw_barriers).  Per batch: the barriers of the
+ * current state into Q (M*NV device doubles), the modes with Q < stability_threshold in
+ * voxel-major order, sorted by Q (ties by that order) keeping the first mode per voxel,
+ * their flips at `time` in that order with the redraws of their voxels from the host
+ * stream `rng`, one LS re-solve at eps_macro (the loop's LS form).  Stops: no unstable
+ * mode (MGK_CASCADE_DONE); after the batch that leaves eps_macro[eps_comp] >= eps_target
+ * (eps_comp < 0: never; MGK_CASCADE_EPS_TARGET); once the batch count exceeds max_batches
+ * (MGK_CASCADE_TRUNCATED if truncate, else MGK_CASCADE_DONE); the unstable-stress check of
+ * 2D loops (MGK_CASCADE_UNSTABLE); before a batch whose flips would exceed cap records or
+ * batch_cap batches (MGK_CASCADE_CAPACITY: call again to continue, the state is
+ * consistent).  flips[0..*n_flips) hold (batch, v, m), batch_sizes[0..*n_batches) the
+ * flips per batch, batch_sig (optional, 9 doubles per batch) each batch's <sig> after
+ * its re-solve (the state a per-batch log reads); info / iters32 the last re-solve. */
+typedef struct {
+  int batch;
+  long long v;
+  int m;
+} mgk_cascade_flip;
+
+#define MGK_CASCADE_DONE 0
+#define MGK_CASCADE_TRUNCATED 1
+#define MGK_CASCADE_EPS_TARGET 2
+#define MGK_CASCADE_UNSTABLE 3
+#define MGK_CASCADE_CAPACITY 4
+
+int mgk_kmc_cascade(const mgk_kmc_loop* loop, mgk_mt_state* rng, double* Q, double time, const double* eps_macro,
+                    int max_batches, int truncate, int eps_comp, double eps_target, mgk_cascade_flip* flips, int cap,
+                    int* n_flips, int* batch_sizes, double* batch_sig, int batch_cap, int* n_batches, int* reason,
+                    mgk_ls_info* info, int* iters32, void* stream);
+
 /* Batched form of mgk_kmc_run (no counterpart in the reference: the serving form's
  * continuous batching).  nb (1..MGK_LS_MAX_BATCH) simulations of one grid shape and
  * one LS configuration (ls_precision, warm, max_iter, tol, tol32) step in lockstep on

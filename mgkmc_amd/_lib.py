@@ -19,6 +19,8 @@ MGK_ERR_WORKSPACE = -3
 MGK_ERR_NONFINITE = -4
 MGK_ERR_SOLVER = -5
 MGK_KMC_RUN_TIME, MGK_KMC_RUN_COUNT, MGK_KMC_RUN_SEQ, MGK_KMC_RUN_UNSTABLE = 0, 1, 2, -6
+(MGK_CASCADE_DONE, MGK_CASCADE_TRUNCATED, MGK_CASCADE_EPS_TARGET, MGK_CASCADE_UNSTABLE,
+ MGK_CASCADE_CAPACITY) = 0, 1, 2, 3, 4
 
 _lib = None
 
@@ -76,6 +78,10 @@ class MgkKmcEvent(ctypes.Structure):
     _fields_ = [("v", ctypes.c_longlong), ("m", ctypes.c_int), ("is_instab", ctypes.c_int), ("time", ctypes.c_double),
                 ("ls_iters", ctypes.c_int), ("ls_iters32", ctypes.c_int), ("eps_macro", ctypes.c_double * 9),
                 ("sig_mean", ctypes.c_double * 9)]
+
+
+class MgkCascadeFlip(ctypes.Structure):
+    _fields_ = [("batch", ctypes.c_int), ("v", ctypes.c_longlong), ("m", ctypes.c_int)]
 
 
 MgkKmcLoop._fields_ = [
@@ -175,6 +181,10 @@ SIGNATURES = {
                                  ctypes.POINTER(MgkMtState), _c_dp, _c_dp, _c_dp, ctypes.POINTER(_c_i), _c_i,
                                  ctypes.POINTER(_c_i), ctypes.POINTER(MgkKmcEvent), _c_i, ctypes.POINTER(_c_i),
                                  ctypes.POINTER(_c_i), ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i), _c_p]),
+    "mgk_kmc_cascade": (_c_i, [ctypes.POINTER(MgkKmcLoop), ctypes.POINTER(MgkMtState), _c_p, _c_d, _c_dp, _c_i,
+                               _c_i, _c_i, _c_d, ctypes.POINTER(MgkCascadeFlip), _c_i, ctypes.POINTER(_c_i),
+                               ctypes.POINTER(_c_i), _c_dp, _c_i, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i),
+                               ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i), _c_p]),
     "mgk_kmc_workspace_bytes": (_c_i, [ctypes.c_longlong, ctypes.POINTER(_c_sz)]),
     "mgk_kmc_barrier_rates": (_c_i, [_c_i, ctypes.c_longlong, _c_i, _c_i, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p,
                                      _c_p, _c_p, ctypes.POINTER(MgkKmcParams), _c_p, _c_p, _c_p, _c_dp]),

@@ -169,8 +169,8 @@ class ThermalSimulation(DeviceKmc):
             log.summary(step, kind, eps[sdc], sig[sdc], c["kmc"], cid, flips, self.time)
             return sig[sdc]
 
-        def cascade_cb(local, nflips):
-            do_log(c["step"], "cascade", casc_count, nflips)
+        def cascade_cb(local, nflips, sig=None):
+            do_log(c["step"], "cascade", casc_count, nflips, sig=sig)
 
         def cascade(step):
             r = self._run_cascade(step, eps_target=eps_target, component=component, log_cb=cascade_cb,

@@ -135,8 +135,11 @@ thread_local DevStage g_dstage;
 void* mgk_device_stage(int slot, size_t bytes) {
   if (slot < 0 || slot > 3) return nullptr;
   if (g_dstage.n[slot] >= bytes) return g_dstage.p[slot];
+  // geometric growth: the blocks dropped on the way (not freed: hipFree would
+  // synchronise the device under other threads' work) add up to less than the last
   void* np_ = nullptr;
-  const size_t want = bytes < 4096 ? 4096 : bytes;
+  size_t want = bytes < 4096 ? 4096 : bytes;
+  if (want < 2 * g_dstage.n[slot]) want = 2 * g_dstage.n[slot];
   if (hipMalloc(&np_, want) != hipSuccess) return nullptr;
   g_dstage.p[slot] = np_;
   g_dstage.n[slot] = want;

@@ -61,6 +61,21 @@ void mgk_kmc_sel_unpack(const void* host_rec, mgk_kmc_sel* out);
 int mgk_kmc_flip_redraw_batch(int nmem, const mgk_kmc_loop* const* L, const long long* v, const int* m,
                               const double* times, const double* dev_redraw, size_t stride, hipStream_t s);
 
+// Cascade batches (native cascade loop): count of the modes with stored
+// Q < thr (counts: one per 256 voxels; offsets: their exclusive scan with the
+// total at offsets[nblocks]), then their (v, m, q) in voxel-major order (at
+// most cap); the flips of a batch of n pairs at one time in nlev levels
+// (level l: the pairs dlv/dlm[dloff[l] .. dloff[l+1]), disjoint neighbourhoods
+// within a level), then the redraws of the voxels dv[i] (pair order) from
+// dred + i * stride.  Enqueued on s.
+int mgk_kmc_below_all(const mgk_kmc_loop* L, const double* Q, double thr, long long* counts, long long* offsets,
+                      hipStream_t s);
+int mgk_kmc_below_write(const mgk_kmc_loop* L, const double* Q, double thr, const long long* offsets, long long cap,
+                        long long* vv, int* mm, double* qq, hipStream_t s);
+int mgk_kmc_flip_redraw_levels(const mgk_kmc_loop* L, int n, int nlev, const int* dloff, const long long* dlv,
+                               const int* dlm, const long long* dv, double time, const double* dred, size_t stride,
+                               hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // Live per-kernel timing (HIP events on the launching stream).  Off by default;
 // bench.py turns it on around its timed region.

@@ -580,11 +580,13 @@ struct KmcFlip {
   int eps_div;  // eps_p += mode / |eps_div|; eps_div < 0: that increment only; 0: no eps_p update
 };
 
+// the flip by the calling thread when `lead` (every update of one flip is done by
+// one thread, in order)
 template <int D, int CK, typename TE>
-__device__ __forceinline__ void kmc_flip_body(const KmcFlip& F, const double* __restrict__ cat, TE* __restrict__ epsp,
-                                              float* __restrict__ shadow, double* __restrict__ soft,
-                                              double* __restrict__ tlast, double* __restrict__ prev,
-                                              double* __restrict__ gamma_out) {
+__device__ __forceinline__ void kmc_flip_apply(const KmcFlip& F, const double* __restrict__ cat, TE* __restrict__ epsp,
+                                               float* __restrict__ shadow, double* __restrict__ soft,
+                                               double* __restrict__ tlast, double* __restrict__ prev,
+                                               double* __restrict__ gamma_out, bool lead) {
   constexpr int DD = D * D;
   const size_t NV = (size_t)F.nx * F.ny * F.nz;
   const size_t MNV = (size_t)F.M * NV;
@@ -614,8 +616,7 @@ __device__ __forceinline__ void kmc_flip_body(const KmcFlip& F, const double* __
   const int x = (int)(F.v / ((long long)F.ny * F.nz));
   const int y = (int)((F.v / F.nz) % F.ny);
   const int z = (int)(F.v % F.nz);
-  const int t = threadIdx.x;
-  if (t == 0) {
+  if (lead) {
     const size_t v = (size_t)F.v;
     // eps_p += gamma (symmetric SoA storage)
     // symmetric part of the mode per stored component (exact for symmetric
@@ -658,7 +659,7 @@ __device__ __forceinline__ void kmc_flip_body(const KmcFlip& F, const double* __
       for (int k = 0; k < DD; ++k) gamma_out[k] = c[k];
     }
   }
-  if (t == 0 && F.jn > 0.0 && F.eps_div >= 0) {
+  if (lead && F.jn > 0.0 && F.eps_div >= 0) {
     // neighbours in the reference's (dx, dy[, dz]) loop order, sequentially
     // (on small periodic grids several offsets hit the same voxel)
     for (int dx = -1; dx <= 1; ++dx)
@@ -674,6 +675,14 @@ __device__ __forceinline__ void kmc_flip_body(const KmcFlip& F, const double* __
           soft[NV + w] = soft[NV + w] + F.jn * F.jt * s;
         }
   }
+}
+
+template <int D, int CK, typename TE>
+__device__ __forceinline__ void kmc_flip_body(const KmcFlip& F, const double* __restrict__ cat, TE* __restrict__ epsp,
+                                              float* __restrict__ shadow, double* __restrict__ soft,
+                                              double* __restrict__ tlast, double* __restrict__ prev,
+                                              double* __restrict__ gamma_out) {
+  kmc_flip_apply<D, CK, TE>(F, cat, epsp, shadow, soft, tlast, prev, gamma_out, threadIdx.x == 0);
 }
 
 template <int D, int CK, typename TE>
@@ -941,6 +950,178 @@ extern "C" int mgk_kmc_select(int ndim, long long nvox, int M, int sig_kind, int
   return MGK_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Cascade batch support (the native cascade loop): every mode with stored
+// Q < thr, in the voxel-major scan order of find_unstable (v * M + m), in one
+// count pass, one scan and one write pass; the batch's flips in levels of
+// flips with disjoint neighbourhoods (levels assigned on the host); the
+// redraws of the batch's voxels.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(MGK_NT) void k_below_count(size_t NV, int M, const double* __restrict__ Q, double thr,
+                                                        long long* __restrict__ counts) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  const size_t v = (size_t)blockIdx.x * MGK_NT + threadIdx.x;
+  int c = 0;
+  if (v < NV)
+    for (int m = 0; m < M; ++m) c += Q[(size_t)m * NV + v] < thr ? 1 : 0;
+  if (c) atomicAdd(&cnt, c);
+  __syncthreads();
+  if (threadIdx.x == 0) counts[blockIdx.x] = cnt;
+}
+
+// inclusive prefix sum over the workgroup (Hillis-Steele in LDS; NT threads)
+template <int NT, typename T>
+__device__ __forceinline__ T block_incl_scan(T x, T* buf) {
+  const int t = threadIdx.x;
+  buf[t] = x;
+  __syncthreads();
+  for (int off = 1; off < NT; off <<= 1) {
+    const T y = t >= off ? buf[t - off] : (T)0;
+    __syncthreads();
+    buf[t] += y;
+    __syncthreads();
+  }
+  return buf[t];
+}
+
+// exclusive offsets of the block counts, and the total at offsets[nb]: each of
+// the 1024 threads sums a contiguous run of counts, one scan over the runs
+constexpr int kScanNT = 1024;
+__global__ __launch_bounds__(kScanNT) void k_below_scan(const long long* __restrict__ counts, int nb,
+                                                        long long* __restrict__ offsets) {
+  __shared__ long long buf[kScanNT];
+  const int per = (nb + kScanNT - 1) / kScanNT;
+  const int b0 = min((int)threadIdx.x * per, nb), b1 = min(b0 + per, nb);
+  long long c = 0;
+  for (int b = b0; b < b1; ++b) c += counts[b];
+  const long long inc = block_incl_scan<kScanNT>(c, buf);
+  long long o = inc - c;
+  for (int b = b0; b < b1; ++b) {
+    offsets[b] = o;
+    o += counts[b];
+  }
+  if (threadIdx.x == kScanNT - 1) offsets[nb] = inc;
+}
+
+__global__ __launch_bounds__(MGK_NT) void k_below_write(size_t NV, int M, const double* __restrict__ Q, double thr,
+                                                        const long long* __restrict__ offsets, long long cap,
+                                                        long long* __restrict__ vv, int* __restrict__ mm,
+                                                        double* __restrict__ qq) {
+  __shared__ int pre[MGK_NT];
+  const size_t v = (size_t)blockIdx.x * MGK_NT + threadIdx.x;
+  int c = 0;
+  if (v < NV)
+    for (int m = 0; m < M; ++m) c += Q[(size_t)m * NV + v] < thr ? 1 : 0;
+  const int inc = block_incl_scan<MGK_NT>(c, pre);
+  if (v >= NV || !c) return;
+  long long o = offsets[blockIdx.x] + (inc - c);
+  for (int m = 0; m < M; ++m) {
+    const double q = Q[(size_t)m * NV + v];
+    if (q < thr) {
+      if (o < cap) {
+        vv[o] = (long long)v;
+        mm[o] = m;
+        qq[o] = q;
+      }
+      ++o;
+    }
+  }
+}
+
+// the batch's flips level by level: flips of one level touch disjoint voxels (the
+// flipped voxel and the neighbours it softens), and every voxel takes the updates of
+// the flips that touch it in pair order, since a later flip touching a voxel that an
+// earlier one touched sits on a higher level; each flip is done by one thread
+constexpr int kLevelNT = 1024;
+template <int D, int CK, typename TE>
+__global__ __launch_bounds__(kLevelNT) void k_kmc_flip_levels(KmcFlip F, int nlev, const int* __restrict__ loff,
+                                                              const long long* __restrict__ vs,
+                                                              const int* __restrict__ ms,
+                                                              const double* __restrict__ cat, TE* __restrict__ epsp,
+                                                              float* __restrict__ shadow, double* __restrict__ soft,
+                                                              double* __restrict__ tlast, double* __restrict__ prev) {
+  for (int l = 0; l < nlev; ++l) {
+    const int e = loff[l + 1];
+    for (int i = loff[l] + (int)threadIdx.x; i < e; i += kLevelNT) {
+      KmcFlip f = F;
+      f.v = vs[i];
+      f.m = ms[i];
+      kmc_flip_apply<D, CK, TE>(f, cat, epsp, shadow, soft, tlast, prev, nullptr, true);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_kmc_set_voxels(size_t NV, int M, int CK, const long long* __restrict__ vs,
+                                 const double* __restrict__ red, size_t stride, int dirs, int bars,
+                                 double* __restrict__ cat, double* __restrict__ Q0) {
+  const double* r = red + (size_t)blockIdx.x * stride;
+  kmc_set_voxel_body(NV, M, CK, vs[blockIdx.x], dirs ? r : nullptr, bars ? r + 2 * (size_t)M : nullptr, cat, Q0);
+}
+
+int mgk_kmc_below_all(const mgk_kmc_loop* L, const double* Q, double thr, long long* counts, long long* offsets,
+                      hipStream_t s) {
+  long long nvox = 1;
+  for (int i = 0; i < L->ndim; ++i) nvox *= L->shape[i];
+  const int nb = (int)((nvox + MGK_NT - 1) / MGK_NT);
+  k_below_count<<<nb, MGK_NT, 0, s>>>((size_t)nvox, L->M, Q, thr, counts);
+  k_below_scan<<<1, kScanNT, 0, s>>>(counts, nb, offsets);
+  MGK_HIP(hipGetLastError());
+  return MGK_OK;
+}
+
+int mgk_kmc_below_write(const mgk_kmc_loop* L, const double* Q, double thr, const long long* offsets, long long cap,
+                        long long* vv, int* mm, double* qq, hipStream_t s) {
+  long long nvox = 1;
+  for (int i = 0; i < L->ndim; ++i) nvox *= L->shape[i];
+  const int nb = (int)((nvox + MGK_NT - 1) / MGK_NT);
+  k_below_write<<<nb, MGK_NT, 0, s>>>((size_t)nvox, L->M, Q, thr, offsets, cap, vv, mm, qq);
+  MGK_HIP(hipGetLastError());
+  return MGK_OK;
+}
+
+int mgk_kmc_flip_redraw_levels(const mgk_kmc_loop* L, int n, int nlev, const int* dloff, const long long* dlv,
+                               const int* dlm, const long long* dv, double time, const double* dred, size_t stride,
+                               hipStream_t s) {
+  const int ndim = L->ndim;
+  long long nvox = 1;
+  for (int i = 0; i < ndim; ++i) nvox *= L->shape[i];
+  KmcFlip F;
+  F.nx = (int)L->shape[0];
+  F.ny = (int)L->shape[1];
+  F.nz = ndim == 3 ? (int)L->shape[2] : 1;
+  F.M = L->M;
+  F.v = 0;
+  F.m = 0;
+  F.time = time;
+  F.jp = L->jp;
+  F.jt = L->jt;
+  F.gmax = L->kp.cap;
+  F.jn = L->jn;
+  F.eps_div = 1;
+  if (n < 1 || nlev < 1 || nlev > n || (L->eps_precision != 0 && L->epsp_shadow)) {
+    mgk_set_error("cascade flips: bad batch or an FP32 eps_p with an FP32 shadow", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+#define MGK_LEVELS(DD, TE)                                                                                         \
+  k_kmc_flip_levels<DD, 2, TE><<<1, kLevelNT, 0, s>>>(F, nlev, dloff, dlv, dlm, L->cat, (TE*)L->epsp, L->epsp_shadow, \
+                                                      L->soft, L->tlast, L->prev)
+  if (ndim == 3) {
+    if (L->eps_precision == 0) MGK_LEVELS(3, double);
+    else MGK_LEVELS(3, float);
+  } else {
+    if (L->eps_precision == 0) MGK_LEVELS(2, double);
+    else MGK_LEVELS(2, float);
+  }
+#undef MGK_LEVELS
+  k_kmc_set_voxels<<<n, 64, 0, s>>>((size_t)nvox, L->M, 2, dv, dred, stride, L->redraw_directions,
+                                    L->redraw_barriers, L->cat, L->Q0);
+  MGK_HIP(hipGetLastError());
+  return MGK_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Batched passes of the event step for mgk_kmc_run_batch: every member shares

@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_set>
+#include <vector>
 #include "mgk_host.h"
 #include "../../include/mgkmc_hip.h"
 
@@ -520,4 +522,248 @@ extern "C" int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* L, mgk_mt_state* rn
   }
   (void)hipStreamSynchronize(s);
   return rc_all < 0 ? rc_all : MGK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Native cascade loop (mgkmc/aqs.py:480-680, mgkmc/kmc_simulator.py:577-794;
+// small strain, linear elasticity, planar catalogs): per batch one barrier
+// pass storing Q, one scan for the modes with Q < stability_threshold in
+// voxel-major order, a host sort by Q (ties by scan position) keeping the
+// first mode per voxel, the batch's flips (levels of flips with disjoint
+// neighbourhoods, each voxel's updates in pair order) and their redraws from
+// the host stream in that order, one LS re-solve.  Each batch costs three host
+// synchronisations (the count, the list, the solve), where the per-mode
+// scans of the Python loop took one per mode.
+// ---------------------------------------------------------------------------
+namespace {
+struct CascadeStage {
+  void* h = nullptr;
+  size_t hn = 0;
+};
+thread_local CascadeStage g_cstage;
+
+void* cascade_host(size_t bytes) {
+  if (g_cstage.hn >= bytes) return g_cstage.h;
+  void* np_ = nullptr;
+  const size_t want = std::max(bytes, 2 * g_cstage.hn);
+  if (hipHostMalloc(&np_, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+  // the smaller earlier block is dropped (hipHostFree would synchronise the device);
+  // growth doubles, so the dropped blocks add up to less than the current one
+  g_cstage.h = np_;
+  g_cstage.hn = want;
+  return np_;
+}
+
+// levels of a batch's flips for k_kmc_flip_levels: a flip touches its voxel and,
+// with neighbour softening on, the voxels one periodic step away along every
+// axis; its level is one more than the highest level of the earlier flips that
+// touch any voxel it touches, so flips of one level touch disjoint voxels and each
+// voxel sees its flips in pair order.  touch (NV entries, -1 between batches)
+// holds the highest level touching each voxel.
+thread_local std::vector<int> g_touch;
+
+template <typename Fn>
+void for_hood(const mgk_kmc_loop* L, long long v, bool nbrs, Fn&& fn) {
+  if (!nbrs) {
+    fn(v);
+    return;
+  }
+  const long long nx = L->shape[0], ny = L->shape[1], nz = L->ndim == 3 ? L->shape[2] : 1;
+  const long long x = v / (ny * nz), y = (v / nz) % ny, z = v % nz;
+  const int r = L->ndim == 3 ? 1 : 0;
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = -r; dz <= r; ++dz)
+        fn((((x + dx + nx) % nx) * ny + (y + dy + ny) % ny) * nz + (z + dz + nz) % nz);
+}
+
+int flip_levels(const mgk_kmc_loop* L, long long NV, const std::vector<long long>& pv, std::vector<int>& lev) {
+  if ((long long)g_touch.size() < NV) g_touch.assign((size_t)NV, -1);
+  const bool nbrs = L->jn > 0.0;
+  int nlev = 0;
+  lev.resize(pv.size());
+  for (size_t k = 0; k < pv.size(); ++k) {
+    int hi = -1;
+    for_hood(L, pv[k], nbrs, [&](long long w) { hi = std::max(hi, g_touch[(size_t)w]); });
+    lev[k] = hi + 1;
+    for_hood(L, pv[k], nbrs, [&](long long w) { g_touch[(size_t)w] = hi + 1; });
+    nlev = std::max(nlev, hi + 2);
+  }
+  for (long long v : pv) for_hood(L, v, nbrs, [&](long long w) { g_touch[(size_t)w] = -1; });
+  return nlev;
+}
+}  // namespace
+
+extern "C" int mgk_kmc_cascade(const mgk_kmc_loop* L, mgk_mt_state* rng, double* Q, double time,
+                               const double* eps_macro, int max_batches, int truncate, int eps_comp,
+                               double eps_target, mgk_cascade_flip* flips, int cap, int* n_flips, int* batch_sizes,
+                               double* batch_sig, int batch_cap, int* n_batches, int* reason, mgk_ls_info* info,
+                               int* iters32, void* stream) {
+  mgk_mt_state sel_dummy{};
+  sel_dummy.pos = 624;
+  if (!loop_ok(L, rng, &sel_dummy) || !Q || !eps_macro || !flips || cap < 1 || !n_flips || !batch_sizes ||
+      batch_cap < 1 || !n_batches || !reason || !info || max_batches < 0 ||
+      (!L->redraw_directions && !L->redraw_barriers)) {
+    mgk_set_error("mgk_kmc_cascade: bad argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int d = L->ndim, M = L->M;
+  long long NV = 1;
+  for (int i = 0; i < d; ++i) NV *= L->shape[i];
+  const int nblk = (int)((NV + MGK_NT - 1) / MGK_NT);
+  Mt R{rng};
+  *n_flips = 0;
+  *n_batches = 0;
+  *reason = MGK_CASCADE_DONE;
+  if (iters32) *iters32 = 0;
+  // device: block counts, offsets (+ total), then the list (grown on demand)
+  char* dcnt = (char*)mgk_device_stage(2, sizeof(long long) * (2 * (size_t)nblk + 1));
+  if (!dcnt) {
+    mgk_set_error("mgk_kmc_cascade: staging allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
+  long long* counts = (long long*)dcnt;
+  long long* offsets = counts + nblk;
+  const size_t stride = 3 * (size_t)M;
+  int local = 0, nf = 0;
+  while (true) {
+    mgk_kmc_params p = L->kp;
+    p.time = time;
+    int e = mgk_kmc_barrier_rates(d, NV, M, L->sig_kind, 0, L->sig, L->Q0, L->cat, L->soft, L->tlast, L->prev,
+                                  nullptr, &p, Q, L->kmc_ws, stream, nullptr);
+    if (e == MGK_OK) e = mgk_kmc_below_all(L, Q, L->stability_threshold, counts, offsets, s);
+    if (e != MGK_OK) return e;
+    long long* htot = (long long*)cascade_host(4096);
+    if (!htot) {
+      mgk_set_error("mgk_kmc_cascade: staging allocation failed", __FILE__, __LINE__);
+      return MGK_ERR_HIP;
+    }
+    if (hipMemcpyAsync(htot, offsets + nblk, sizeof(long long), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      mgk_set_error("mgk_kmc_cascade: count read failed", __FILE__, __LINE__);
+      return MGK_ERR_HIP;
+    }
+    const long long total = *htot;
+    if (total == 0) break;
+    // the list: (v, m, q) per unstable mode in voxel-major order
+    const size_t lb = ((size_t)total * (sizeof(long long) + sizeof(int) + sizeof(double)) + 255) & ~(size_t)255;
+    const size_t npmax = (size_t)std::min<long long>(total, NV);
+    const size_t sbytes = lb + npmax * (stride * sizeof(double) + 2 * sizeof(long long) + 2 * sizeof(int)) + 512;
+    char* dl = (char*)mgk_device_stage(3, sbytes);
+    char* hl = (char*)cascade_host(sbytes);
+    if (!dl || !hl) {
+      mgk_set_error("mgk_kmc_cascade: list allocation failed", __FILE__, __LINE__);
+      return MGK_ERR_HIP;
+    }
+    long long* dvv = (long long*)dl;
+    double* dqq = (double*)(dvv + total);
+    int* dmm = (int*)(dqq + total);
+    e = mgk_kmc_below_write(L, Q, L->stability_threshold, offsets, total, dvv, dmm, dqq, s);
+    if (e != MGK_OK) return e;
+    if (hipMemcpyAsync(hl, dl, (size_t)total * (sizeof(long long) + sizeof(double) + sizeof(int)),
+                       hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      mgk_set_error("mgk_kmc_cascade: list read failed", __FILE__, __LINE__);
+      return MGK_ERR_HIP;
+    }
+    const long long* hvv = (const long long*)hl;
+    const double* hqq = (const double*)(hvv + total);
+    const int* hmm = (const int*)(hqq + total);
+    // by barrier, ties by scan position; the first (most unstable) mode per voxel
+    std::vector<long long> ord((size_t)total);
+    for (long long i = 0; i < total; ++i) ord[(size_t)i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](long long a, long long b) { return hqq[a] < hqq[b]; });
+    std::unordered_set<long long> seen;
+    seen.reserve((size_t)total * 2);
+    std::vector<long long> pv;
+    std::vector<int> pm;
+    for (long long i : ord) {
+      const long long v = hvv[i];
+      if (!seen.insert(v).second) continue;
+      pv.push_back(v);
+      pm.push_back(hmm[i]);
+    }
+    const int np = (int)pv.size();
+    if (nf + np > cap || local >= batch_cap) {
+      *reason = MGK_CASCADE_CAPACITY;  // records full: the caller continues with a new call
+      break;
+    }
+    std::vector<int> lev;
+    const int nlev = flip_levels(L, NV, pv, lev);
+    // staged after the list, one upload: the voxels in pair order, the pairs
+    // grouped by level, the level offsets, the redraws in pair order
+    char* hp = hl + lb;
+    long long* hpv = (long long*)hp;
+    long long* hlv = hpv + np;
+    int* hlm = (int*)(hlv + np);
+    int* hloff = hlm + np;
+    double* hred = (double*)(((uintptr_t)(hloff + nlev + 1) + 15) & ~(uintptr_t)15);
+    for (int l = 0; l <= nlev; ++l) hloff[l] = 0;
+    for (int k = 0; k < np; ++k) ++hloff[lev[k] + 1];
+    for (int l = 0; l < nlev; ++l) hloff[l + 1] += hloff[l];
+    {
+      std::vector<int> fill(hloff, hloff + nlev);
+      for (int k = 0; k < np; ++k) {
+        const int at = fill[lev[k]]++;
+        hlv[at] = pv[k];
+        hlm[at] = pm[k];
+      }
+    }
+    for (int k = 0; k < np; ++k) {
+      hpv[k] = pv[k];
+      redraw_draws(L, R, hred + stride * k);
+    }
+    const size_t pbytes = (size_t)((char*)(hred + stride * np) - hp);
+    char* dp = dl + lb;
+    if (hipMemcpyAsync(dp, hp, pbytes, hipMemcpyHostToDevice, s) != hipSuccess) {
+      mgk_set_error("mgk_kmc_cascade: pair upload failed", __FILE__, __LINE__);
+      return MGK_ERR_HIP;
+    }
+    auto dev = [&](const void* h) { return dp + ((const char*)h - hp); };
+    e = mgk_kmc_flip_redraw_levels(L, np, nlev, (const int*)dev(hloff), (const long long*)dev(hlv),
+                                   (const int*)dev(hlm), (const long long*)dev(hpv), time,
+                                   (const double*)dev(hred), stride, s);
+    if (e != MGK_OK) return e;
+    int it32 = 0;
+    if (L->ls_precision == 2) {
+      e = mgk_ls_solve_refined(d, L->shape, L->lam32, L->mu32, L->lam, L->mu, L->lam0, L->mu0, eps_macro,
+                               L->epsp_shadow, L->epsp, L->eps32, L->eps, L->sig, L->warm, L->max_iter, L->tol,
+                               L->tol32, L->ws32, L->ws32_bytes, L->ws, L->ws_bytes, stream, info, &it32);
+    } else {
+      e = mgk_ls_solve(d, L->shape, L->ls_precision, L->lam, L->mu, L->lam0, L->mu0, eps_macro, L->epsp, L->eps,
+                       L->sig, L->warm, L->max_iter, L->tol, L->ws, L->ws_bytes, stream, info);
+    }
+    if (iters32) *iters32 = it32;
+    if (e < 0 && e != MGK_ERR_NONFINITE) return e;
+    for (int k = 0; k < np; ++k) {
+      flips[nf + k].batch = local;
+      flips[nf + k].v = pv[k];
+      flips[nf + k].m = pm[k];
+    }
+    nf += np;
+    batch_sizes[local] = np;
+    if (batch_sig)
+      for (int k = 0; k < 9; ++k) batch_sig[9 * (size_t)local + k] = info->sig_mean[k];
+    *n_flips = nf;
+    if (L->check_unstable && unstable_mean(*info, d)) {
+      mgk_set_error("Unstable stress in small-strain elastic_run", __FILE__, __LINE__);
+      *n_batches = local + 1;
+      *reason = MGK_CASCADE_UNSTABLE;
+      break;
+    }
+    if (eps_comp >= 0 && eps_macro[eps_comp] >= eps_target) {
+      *n_batches = local + 1;
+      *reason = MGK_CASCADE_EPS_TARGET;
+      break;
+    }
+    ++local;
+    *n_batches = local;
+    if (local > max_batches) {
+      *reason = truncate ? MGK_CASCADE_TRUNCATED : MGK_CASCADE_DONE;
+      break;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  return MGK_OK;
 }

@@ -883,7 +883,13 @@ class DeviceKmc:
         True once eps_macro[component] >= eps_target after a batch (checked
         after ``log_cb(batch, n_flips)``, aqs.py:660-666); ``track`` appends
         the flips to flip_event_history as the reference's track_cascades does;
-        2D stops after nx * ny + 1 batches."""
+        2D stops after nx * ny + 1 batches.  ``log_cb(batch, n_flips, sig)``
+        gets the mean stress after the batch's re-solve.  Linear-law
+        simulations with the gaussian barrier generator and redraws run in
+        the library's C++ cascade loop (mgk_kmc_cascade; same streams, same
+        batches)."""
+        if self._native_cascade_ok():
+            return self._run_cascade_native(step, eps_target, component, log_cb, track)
         local = total = 0
         while True:
             pairs = self._cascade_pairs()
@@ -915,7 +921,7 @@ class DeviceKmc:
                     self.flip_event_history.append(tuple(int(r) for r in rec))
             total += len(pairs)
             if log_cb is not None:
-                log_cb(local, len(pairs))
+                log_cb(local, len(pairs), self.sig_mean())
             if self.ndim == 3 and eps_target is not None and self.eps_macro[component] >= eps_target:
                 return local + 1, total, False, True
             local += 1
@@ -926,6 +932,92 @@ class DeviceKmc:
             if self.ndim == 2 and local > self.total_voxels:
                 break
         return local, total, False, False
+
+    # flip and batch records of one mgk_kmc_cascade call; a cascade that fills
+    # them continues in further calls
+    cascade_records = (1 << 16, 1024)
+
+    def _native_cascade_ok(self):
+        gen = self.barrier_generator
+        gp = getattr(gen, "params", None)
+        same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
+        return (self.native_loop and self.law is None and not self.finite and not self.enable_thermal
+                and self.kmc.planar and self._flip_log is None and self._sig_owner is None
+                and gp is not None and same_stream and (self.redraw_directions or self.redraw_barriers)
+                and hasattr(self.rng, "get_state") and (self._solved_once or not self.ls_warm_start))
+
+    def _run_cascade_native(self, step, eps_target, component, log_cb, track):
+        """_run_cascade through mgk_kmc_cascade: the library runs whole
+        batches (barriers, unstable list, sort, flips, redraws, re-solve) and
+        returns each batch's flips and mean stress; histories and per-batch
+        logs are written here in batch order.  A batch whose re-solve fails
+        the 2D stability check raises ValueError as update_stresses does,
+        with that batch left out of the histories and the log."""
+        d = self.ndim
+        lib = _lib.load()
+        L = self._loop_struct(np.zeros((d, d)))
+        if self.kmc.Q is None:
+            self.kmc.Q = dev.torch.zeros_like(self.kmc.Q0)
+        if d == 3:
+            maxb, trunc = int(self.max_cascade_steps_pct * self.total_voxels), 1
+            ec = component[0] * d + component[1] if eps_target is not None else -1
+        else:
+            maxb, trunc, ec = self.total_voxels, 0, -1
+        et = float(eps_target) if eps_target is not None else 0.0
+        rng = _lib.mt_from(self.rng)
+        em = (ctypes.c_double * 9)(*self.eps_macro.reshape(-1))
+        cap, bcap = min(self.total_voxels, self.cascade_records[0]), self.cascade_records[1]
+        flips = (_lib.MgkCascadeFlip * cap)()
+        sizes = (ctypes.c_int * bcap)()
+        bsig = (ctypes.c_double * (9 * bcap))()
+        nf, nb, why, it32 = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        local = total = 0
+        ran = False
+        try:
+            while True:
+                rc = lib.mgk_kmc_cascade(ctypes.byref(L), ctypes.byref(rng), _lib.dptr(self.kmc.Q), self.time, em,
+                                         maxb - local, trunc, ec, et, flips, cap, ctypes.byref(nf), sizes, bsig,
+                                         bcap, ctypes.byref(nb), ctypes.byref(why), ctypes.byref(self.ls.info),
+                                         ctypes.byref(it32), ctypes.c_void_p(dev.stream_ptr()))
+                if rc < 0:
+                    _lib.check(rc, "mgk_kmc_cascade")
+                ran = ran or nb.value > 0
+                n_ok = nb.value - int(why.value == _lib.MGK_CASCADE_UNSTABLE)
+                nrec = sum(sizes[:n_ok])
+                rec = np.frombuffer((ctypes.c_char * ctypes.sizeof(flips)).from_buffer(flips),
+                                    dtype=_CASCADE_FLIP)[:nrec]
+                cols = [np.full(nrec, step), rec["batch"] + local, *np.unravel_index(rec["v"], self.kmc.shape),
+                        rec["m"]]
+                recs = list(map(tuple, np.stack(cols, axis=1).astype(np.int64).tolist()))
+                self.cascade_history.extend(recs)
+                if track and d == 3:
+                    self.flip_event_history.extend(recs)
+                for b in range(n_ok):
+                    total += sizes[b]
+                    if log_cb is not None:
+                        log_cb(local, sizes[b], np.array(bsig[9 * b: 9 * b + d * d]).reshape(d, d))
+                    local += 1
+                if why.value == _lib.MGK_CASCADE_CAPACITY:
+                    if nb.value == 0:  # one batch holds more flips than the records: room for every voxel
+                        cap = self.total_voxels
+                        flips = (_lib.MgkCascadeFlip * cap)()
+                    continue
+                break
+        finally:
+            _lib.mt_to(self.rng, rng)
+            if ran:
+                self._solved_once = True
+                self.ls_info = self.ls.info
+                if self.precision == MIXED:
+                    self.ls.iters32 = it32.value
+        if why.value == _lib.MGK_CASCADE_UNSTABLE:
+            sm = np.array(self.ls.info.sig_mean[: d * d]).reshape(d, d)
+            raise ValueError(f"Unstable stress in small-strain elastic_run: {sm / 1e9} GPa")
+        if why.value == _lib.MGK_CASCADE_TRUNCATED:
+            print(f"Cascade limit reached ({local} steps > {self.max_cascade_steps_pct * 100:.1f}% of "
+                  f"{self.total_voxels} voxels)")
+            return local, total, True, False
+        return local, total, False, why.value == _lib.MGK_CASCADE_EPS_TARGET
 
     def _log_flip(self, v, m, q, flip_type):
         """One flipped_voxels_log line (mgkmc/kmc_simulator.py:535-557): the
@@ -1103,6 +1195,10 @@ class DeviceKmc:
             if x is not None:
                 t += x.numel() * x.element_size()
         return t
+
+
+_CASCADE_FLIP = np.dtype({"names": ["batch", "v", "m"], "formats": ["<i4", "<i8", "<i4"], "offsets": [0, 8, 16],
+                          "itemsize": 24})  # mgk_cascade_flip
 
 
 def kmc_events_batch(sims, remaining, units, max_events, max_seq=None, seqs=None, stream=None):

@@ -466,6 +466,61 @@ def test_native_loop_matches_python_loop(gpu, d, prec, scheme):
     assert a.selection_stream.get_state()[2] == b.selection_stream.get_state()[2]
 
 
+@pytest.mark.parametrize("d,prec,T,pct,records", [
+    (3, "fp64", 0.0, 0.3, None),        # athermal: cascades inside the mixed-BC loop, logged per batch
+    (3, "mixed", 0.0, 0.002, None),     # batch limit: the cascade is truncated
+    (3, "fp64", 300.0, 0.3, (3, 2)),    # small records: each cascade continues over several calls
+    (2, "fp64", 400.0, 0.02, None),
+    (2, "fp32", 0.0, 0.02, (2, 1)),
+])
+def test_native_cascade_matches_python_cascade(gpu, tmp_path, monkeypatch, d, prec, T, pct, records):
+    # the library's C++ cascade loop (mgk_kmc_cascade) against the per-batch
+    # Python loop on the same seeds: same batches and flips, per-batch logs,
+    # events, fields and stream state
+    from mgkmc_amd.aqs import ThermalSimulation
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    print(f"[native cascade] {d}D {prec} T={T} pct={pct} records={records}", flush=True)
+    monkeypatch.chdir(tmp_path)
+    n, M = (8, 10) if d == 3 else (16, 20)
+    shape = (n,) * d
+    rng = np.random.default_rng(1)
+    E = 70e9 * rng.uniform(0.9, 1.1, shape)
+    nu = np.full(shape, 0.3)
+    kw = dict(mean=0.5 if d == 3 else (0.55 if T == 0 else 0.6), std=0.15, min_cutoff=0.0)
+    sims = []
+    for native in (True, False):
+        args = dict(pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw, temperature=T, strain_rate=1e7,
+                    cascade_mode=True, softening_cap=0.916, jp=20.0, jt=0.0, ls_precision=prec,
+                    rng=np.random.RandomState(7), selection_stream=np.random.RandomState(8))
+        if d == 3:
+            sim = ThermalSimulation(n, n, n, M, 0.14, E, nu, **args)
+        else:
+            sim = KmcSimulation2D(n, n, M, 0.14, E, nu, **args)
+        sim.solver_args = {"tol": 1e-8}
+        sim.native_loop = native
+        if records is not None:
+            sim.cascade_records = records
+        assert sim._native_cascade_ok() == native
+        if d == 3:
+            sim.run_simulation(4, 2e-3, component=(0, 0), mixed_tol=1e5 if T == 0 else 1e6,
+                               max_cascade_steps_pct=pct, max_events=None if T == 0 else 20)
+        else:
+            sim.run_simulation(6 if T == 0 else 4, 1e-3, component=(0, 0), stress_targets={(1, 1): 0.0}, mixed_tol=1e6,
+                               max_cascade_steps_pct=pct, max_events=20)
+        sims.append(sim)
+    a, b = sims
+    print(f"[native cascade] flips {len(a.cascade_history)} (python {len(b.cascade_history)}), events "
+          f"{len(a.kmc_event_history)}, logged {len(a.history_global)}", flush=True)
+    assert len(b.cascade_history) > 0
+    assert a.cascade_history == b.cascade_history
+    assert a.kmc_event_history == b.kmc_event_history
+    assert a.history_global == b.history_global
+    assert np.array_equal(a.eps_plastic, b.eps_plastic)
+    assert np.array_equal(a.sig_field, b.sig_field)
+    sa, sb = a.rng.get_state(), b.rng.get_state()
+    assert np.array_equal(sa[1], sb[1]) and sa[2:] == sb[2:]
+
+
 @pytest.mark.parametrize("prec,solver", [("fp64", "newton_cg"), ("mixed", "newton_cg"), ("fp64", "dbfft")])
 def test_kmc3d_finite_strain_matches_oracle(gpu, prec, solver):
     # strain_assumption="finite_strain" (svk, newton_cg or dbfft): every flip enters F_p
