@@ -309,6 +309,23 @@ int mgk_kmc_flip(int ndim, const int* shape, int M, int cat_kind, long long v, i
 int mgk_kmc_set_voxel(long long nvox, int M, int CK, long long v, const double* newcat, const double* newq0,
                       double* cat, double* Q0, void* stream);
 
+/* Fast patching, the stress predictor that replaces a flip's re-solve in the
+ * reference's KMC loops (mgkmc/aqs.py:1021-1047, mgkmc/kmc_simulator.py:764-774,
+ * :1044-1066, :1094-1096).  In one pass over sig (symmetric SoA, sig_kind 0 FP64 or
+ * 1 FP32; 2D on the (nx, 1, ny) LS grid), per voxel and in this order:
+ * sig += sig_unit * a (if sig_unit, FP64, same layout: sigma_macro_unit over the
+ * waiting time); for every offset of the periodic box of radius radius[] (one per
+ * simulation axis, 0 on a flat axis) around voxel v that lands on the voxel, in
+ * (dx, dy, dz) order, sig += sum_b coef[b] * K_b (nbasis = 5 in 3D, 2 in 2D, 0 for
+ * no patch; kernels [b][c][2Rx+1][2Ry+1][2Rz+1] FP64); then sig += shift[c] (the
+ * combined missing mean).  Products and sums are rounded separately, as numpy
+ * does.  sig_mean (host, optional): the new field's mean, d*d row-major, after a
+ * stream synchronisation.  Workspace from mgk_kmc_patch_workspace_bytes. */
+int mgk_kmc_patch_workspace_bytes(size_t* bytes);
+int mgk_kmc_patch(int ndim, const int64_t* shape, int sig_kind, void* sig, const double* sig_unit, double a,
+                  long long v, const int* radius, int nbasis, const double* kernels, const double* coef,
+                  const double* shift, void* workspace, size_t workspace_bytes, void* stream, double* sig_mean);
+
 /* Initial modes drawn on the device (no counterpart in the reference, which draws
  * Q0 and the catalog on the host with the legacy stream: mgkmc/aqs.py:101-120,
  * kmc_simulator.py:117-127).  For callers that start many large simulations (the

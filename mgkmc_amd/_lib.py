@@ -198,6 +198,9 @@ SIGNATURES = {
                             _c_d, _c_d, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i, _c_p]),
     "mgk_kmc_set_voxel": (_c_i, [ctypes.c_longlong, _c_i, _c_i, ctypes.c_longlong, _c_p, _c_p, _c_p, _c_p,
                                  _c_p]),
+    "mgk_kmc_patch_workspace_bytes": (_c_i, [ctypes.POINTER(_c_sz)]),
+    "mgk_kmc_patch": (_c_i, [_c_i, _c_i64p, _c_i, _c_p, _c_p, _c_d, ctypes.c_longlong, ctypes.POINTER(_c_i), _c_i,
+                             _c_p, _c_dp, _c_dp, _c_p, _c_sz, _c_p, _c_dp]),
     "mgk_kmc_init_modes": (_c_i, [ctypes.c_longlong, _c_i, ctypes.c_uint64, _c_d, _c_d, _c_i, _c_d, _c_i, _c_d, _c_d,
                                   _c_p, _c_p, _c_p]),
     "mgk_fs_workspace_bytes": (_c_i, [_c_i64p, _c_i, ctypes.POINTER(_c_sz)]),

@@ -73,6 +73,9 @@ class ThermalSimulation(DeviceKmc):
         self.thermostat, self.tau_bath = thermostat, tau_bath
         if enable_thermal:
             self._setup_thermal(Cp, rho, thermal_diffusivity, thermal_coords, temperature_cap, thermostat, tau_bath)
+        # fast patching (mgkmc/aqs.py:196-218): off for finite strain and the nonlinear laws
+        self._setup_fast_patching(fast_patching, 3, strain_assumption == "finite_strain" or
+                                  hyperelastic_model in ("secant_degradation", "landau", "murnaghan"))
 
     def update_stresses(self):
         """LS re-equilibration with the current eps_plastic and eps_macro

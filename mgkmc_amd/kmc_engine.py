@@ -214,14 +214,20 @@ def device_init_modes(shape, M, gamma0, gen_params, seed, Q0, cat, stream=None):
     _lib.check(rc, "mgk_kmc_init_modes")
 
 
+# trace-free basis eigenstrains of the fast-patching kernels (mgkmc/aqs.py:224-231,
+# kmc_simulator.py:235-238)
+_PATCH_BASES_3D = [np.array([[1.0, 0, 0], [0, -1.0, 0], [0, 0, 0]]), np.array([[0, 0, 0], [0, 1.0, 0], [0, 0, -1.0]]),
+                   np.array([[0, 1.0, 0], [1.0, 0, 0], [0, 0, 0]]), np.array([[0, 0, 1.0], [0, 0, 0], [1.0, 0, 0]]),
+                   np.array([[0, 0, 0], [0, 0, 1.0], [0, 1.0, 0]])]
+_PATCH_BASES_2D = [np.array([[1.0, 0], [0, -1.0]]), np.array([[0, 1.0], [1.0, 0]])]
+
+
 class DeviceKmc:
     """Grid-dimension-independent state and per-event step."""
 
     def _unsupported(self, cascade_mode, fast_patching, enable_thermal, strain_assumption, hyperelastic_model,
                      extra=()):
         bad = list(extra)
-        if fast_patching and fast_patching.get("enabled", False):
-            bad.append("fast_patching")
         if strain_assumption != "small_strain" and not getattr(self, "_finite_ok", False):
             bad.append("finite_strain KMC")
         if bad:
@@ -354,6 +360,8 @@ class DeviceKmc:
         self._flip_log = None  # 2D flipped_voxels_log writer during a run
         self.max_cascade_steps_pct = 0.3
         self.ls_info = None
+        self.fast_patching_enabled = False
+        self._patched_mean = None  # <sigma> after a fast-patching update (None after a solve)
 
     # ----- thermal extension (local temperature, heat conduction) -----------
     def _setup_thermal(self, Cp, rho, thermal_diffusivity, thermal_coords, temperature_cap, thermostat, tau_bath):
@@ -564,9 +572,131 @@ class DeviceKmc:
             return SecantSolver(lam, mu, SECANT_ROSE, law[1], law[2], self.plane_mode, FP64)
         return LandauSolver2D(lam, mu, self.plane_mode, 5)
 
+    # ----- fast patching (the reference's predictor-corrector) -------------
+    def _setup_fast_patching(self, fast_patching, default_radius, disabled):
+        """mgkmc/aqs.py:196-218, kmc_simulator.py:76-78, 154-171: the options
+        under the reference's names; off for finite strain and the nonlinear
+        laws (``disabled``); the patch kernels are computed at construction."""
+        fp = fast_patching
+        self.fast_patching_enabled = fp.get("enabled", False) if fp else False
+        self.patch_radius = fp.get("patch_radius", default_radius) if fp else default_radius
+        self.sync_interval = fp.get("sync_interval", 100) if fp else 100
+        self.flips_since_sync = 0
+        self.sigma_macro_unit = None
+        if disabled:
+            self.fast_patching_enabled = False
+        if self.fast_patching_enabled:
+            self._precompute_patch_kernels()
+
+    def _precompute_patch_kernels(self):
+        """Truncated point solutions (mgkmc/aqs.py:220-256, kmc_simulator.py:229-261):
+        per trace-free basis mode (five in 3D, two in 2D) a cold FP64 LS solve with
+        that eigenstrain at the grid centre and the default solver arguments, the
+        stress cropped to the periodic box of radius patch_radius around the
+        centre (3D: one plane on a flat z axis), and the missing mean <sigma> -
+        (box sum) / voxels.  Kept on the device as K [b][c][wx][wy][wz]; the
+        host lists patch_kernels / patch_missing_mean hold the reference's
+        (2R+1, 2R+1[, W], d, d) crops and (d, d) means."""
+        torch = dev.torch
+        d, R = self.ndim, int(self.patch_radius)
+        if R < 0 or R > 64:
+            raise ValueError("patch_radius must lie in 0..64 on the device path")
+        shape = self.grid_shape
+        pairs = dev.sym_pairs(d)
+        C = len(pairs)
+        bases = _PATCH_BASES_3D if d == 3 else _PATCH_BASES_2D
+        self._patch_ls = LsSolver(self.E_field, self.nu_field, plane_mode=self.plane_mode, precision=FP64)
+        radii = [R, R, R if shape[2] > 1 else 0] if d == 3 else [R, R]
+        idx = [torch.tensor([(n // 2 + k) % n for k in range(-r, r + 1)], device="cuda")
+               for n, r in zip(shape, radii)]
+        nv = self.total_voxels
+        ep = torch.zeros((C,) + shape, dtype=torch.float64, device="cuda")
+        center = tuple(n // 2 for n in shape)
+        Ks, self.patch_kernels, self.patch_missing_mean = [], [], []
+        for P in bases:
+            ep.zero_()
+            for c, (i, j) in enumerate(pairs):
+                ep[(c,) + center] = float(P[i, j])
+            info = self._patch_ls.solve(np.zeros((d, d)), ep, warm=False, max_iter=200, tol=1e-6)
+            sig = self._patch_ls.sig
+            if d == 3:
+                crop = sig[:, idx[0][:, None, None], idx[1][None, :, None], idx[2][None, None, :]]
+            else:
+                crop = sig[:, idx[0][:, None], idx[1][None, :]]
+            crop = crop.contiguous()
+            Ks.append(crop)
+            host = dev.from_sym_soa(dev.download(crop), d)  # (wx, wy[, wz], d, d)
+            full_mean = np.array(info.sig_mean[: d * d]).reshape(d, d)
+            self.patch_kernels.append(host)
+            self.patch_missing_mean.append(full_mean - np.sum(host, axis=tuple(range(d))) / nv)
+        self._patch_K = torch.stack(Ks).contiguous()
+        self._patch_radius_c = (ctypes.c_int * d)(*radii[:d]) if d == 2 else (ctypes.c_int * 3)(*radii)
+        need = ctypes.c_size_t()
+        _lib.check(_lib.load().mgk_kmc_patch_workspace_bytes(ctypes.byref(need)), "mgk_kmc_patch_workspace_bytes")
+        self._patch_ws = torch.empty(int(need.value), dtype=torch.uint8, device="cuda")
+        self._patch_shape = _lib.shape_arg(shape)
+
+    def _patch_coefs(self, C):
+        # basis weights (mgkmc/aqs.py:1030, kmc_simulator.py:765, :1055)
+        if self.ndim == 3:
+            return C[0, 0], -C[2, 2], C[0, 1], C[0, 2], C[1, 2]
+        return C[0, 0], C[0, 1]
+
+    def _patch_call(self, v, coefs, shift, unit, a):
+        d = self.ndim
+        pairs = dev.sym_pairs(d)
+        sh = (ctypes.c_double * 6)(*[float(shift[i, j]) for i, j in pairs])
+        cf = (ctypes.c_double * 5)(*[float(c) for c in coefs])
+        sm = (ctypes.c_double * 9)()
+        sig = self._stress.sig
+        rc = _lib.load().mgk_kmc_patch(
+            d, self._patch_shape, SIG_SYM64 if sig.dtype == dev.torch.float64 else SIG_SYM32, _lib.dptr(sig),
+            _lib.dptr(unit), float(a), int(v), self._patch_radius_c, len(coefs), _lib.dptr(self._patch_K), cf, sh,
+            _lib.dptr(self._patch_ws), self._patch_ws.numel(), ctypes.c_void_p(dev.stream_ptr()), sm)
+        _lib.check(rc, "mgk_kmc_patch")
+        self._patched_mean = np.array(sm[: d * d]).reshape(d, d)
+
+    def _patch_stamp(self, v, C):
+        """The patch of mode C around voxel v and the missing mean
+        (mgkmc/aqs.py:1030-1047, kmc_simulator.py:764-774)."""
+        coefs = self._patch_coefs(C)
+        mm = self.patch_missing_mean
+        shift = coefs[0] * mm[0]
+        for c, m in zip(coefs[1:], mm[1:]):
+            shift = shift + c * m
+        self._patch_call(v, coefs, shift, None, 0.0)
+
+    def _patch_after_flip(self, v, C, t_wait, strain_unit_tensor):
+        """A KMC event's predictor (mgkmc/aqs.py:1021-1054, kmc_simulator.py:
+        1044-1078): the macro-unit stress (computed at the first flip with the
+        current solver arguments) scaled by strain_rate * t_wait, the patch and
+        the missing mean in one pass; every sync_interval flips the full
+        re-solve.  Returns True when it re-solved."""
+        if self.sigma_macro_unit is None:
+            args = dict(max_iter=200, tol=1e-6)
+            args.update(self.solver_args)
+            self._patch_ls.solve(np.asarray(strain_unit_tensor, float), None, warm=False,
+                                 max_iter=args["max_iter"], tol=args["tol"])
+            self.sigma_macro_unit = self._patch_ls.sig.clone()
+            self._patch_ls = None  # set-up done: its fields are not needed again
+        coefs = self._patch_coefs(C)
+        mm = self.patch_missing_mean
+        shift = coefs[0] * mm[0]
+        for c, m in zip(coefs[1:], mm[1:]):
+            shift = shift + c * m
+        self._patch_call(v, coefs, shift, self.sigma_macro_unit if t_wait > 0 else None,
+                         self.strain_rate * t_wait)
+        self.flips_since_sync += 1
+        if self.flips_since_sync >= self.sync_interval:
+            self.update_stresses()
+            self.flips_since_sync = 0
+            return True
+        return False
+
     # ----- seams -----------------------------------------------------------
     def _solve(self, eps_macro):
         self._sig_owner = None
+        self._patched_mean = None
         if self.finite:
             return self._solve_finite()
         if self.law is not None:
@@ -676,6 +806,7 @@ class DeviceKmc:
         under a nonlinear law its stress field becomes the current one."""
         if self.law is None:
             return self._solve(eps_macro)
+        self._patched_mean = None
         if self._lin is None:
             self._lin = LsSolver(self.E_field, self.nu_field, plane_mode=self.plane_mode, precision=FP64)
         args = dict(max_iter=200, tol=1e-6)
@@ -709,6 +840,8 @@ class DeviceKmc:
 
     def sig_mean(self):
         d = self.ndim
+        if self._patched_mean is not None:
+            return self._patched_mean.copy()
         if self.ls_info is None:  # no solve yet (cascade mode before its first batch): the zero initial stress
             return np.zeros((d, d))
         if self.finite:
@@ -756,15 +889,23 @@ class DeviceKmc:
                     self._flip_finite(v, m)
                 elif self.law is not None and self.ndim == 2:
                     self._flip_substepped([(v, m)])
+                elif self.fast_patching_enabled:
+                    C = self._mode_host(v, m)
+                    self._flip(v, m, 1)
+                    self._flip_heat(v, getattr(self, "_flip_heat_mode", None))
+                    solved = self._patch_after_flip(v, C, t_wait, strain_unit_tensor)
                 else:
                     self._flip(v, m, 1)
                     self._flip_heat(v, getattr(self, "_flip_heat_mode", None))
                     self._after_flip_solve()
-                if self.ls_info is not None and not self.finite:
+                if self.ls_info is not None and not self.finite and (not self.fast_patching_enabled or solved):
                     self.ls_iters_total += self.ls_info.iters + getattr(self.ls, "iters32", 0)
                 ev = coords + (m, self.time, is_instab)
                 self.kmc_event_history.append(ev)
                 return ev, remaining_time
+        if self.fast_patching_enabled and self.ndim == 2 and self.sigma_macro_unit is not None:
+            # kmc_simulator.py:1094-1096 (the 3D loop re-solves at the increment's end without it)
+            self._patch_call(0, (), np.zeros((2, 2)), self.sigma_macro_unit, self.strain_rate * remaining_time)
         self.eps_macro += strain_unit_tensor * (self.strain_rate * remaining_time)
         self.heat_conducting(remaining_time)
         self.time += remaining_time
@@ -907,13 +1048,18 @@ class DeviceKmc:
                         self._log_flip(v, m, self._pair_q[(v, m)], "cascade")
                 self._flip_substepped(pairs)
             else:
+                fp2 = self.fast_patching_enabled and self.ndim == 2
                 for v, m in pairs:
                     if flog:
                         self._log_flip(v, m, self._pair_q[(v, m)], "cascade")
                     C = self._mode_host(v, m) if self.enable_thermal else None
                     self._flip(v, m, 1)
                     self._flip_heat(v, C)
-                self.update_stresses()
+                    if fp2:
+                        # the catalog entry after its redraw (kmc_simulator.py:764-774)
+                        self._patch_stamp(v, self._mode_host(v, m))
+                if not fp2:
+                    self.update_stresses()
             for v, m in pairs:
                 rec = (step, local) + tuple(int(c) for c in self.kmc.voxel_coords(v)) + (m,)
                 self.cascade_history.append(rec)
@@ -931,6 +1077,8 @@ class DeviceKmc:
                 return local, total, True, False
             if self.ndim == 2 and local > self.total_voxels:
                 break
+        if self.fast_patching_enabled and self.ndim == 2 and not self.finite and self.law is None and total > 0:
+            self.update_stresses()  # the corrector after a cascade (kmc_simulator.py:790-792)
         return local, total, False, False
 
     # flip and batch records of one mgk_kmc_cascade call; a cascade that fills
@@ -942,6 +1090,7 @@ class DeviceKmc:
         gp = getattr(gen, "params", None)
         same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
         return (self.native_loop and self.law is None and not self.finite and not self.enable_thermal
+                and not (self.fast_patching_enabled and self.ndim == 2)
                 and self.kmc.planar and self._flip_log is None and self._sig_owner is None
                 and gp is not None and same_stream and (self.redraw_directions or self.redraw_barriers)
                 and hasattr(self.rng, "get_state") and (self._solved_once or not self.ls_warm_start))
@@ -1072,6 +1221,7 @@ class DeviceKmc:
         gp = getattr(gen, "params", None)
         same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
         return (self.native_loop and self.law is None and not self.finite and not self.cascade_mode
+                and not self.fast_patching_enabled
                 and not self.enable_thermal and self.kmc.planar and self._flip_log is None
                 and gp is not None and same_stream and (self.redraw_directions or self.redraw_barriers)
                 and hasattr(self.rng, "get_state") and hasattr(self.selection_stream, "get_state")

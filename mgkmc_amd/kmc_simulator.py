@@ -72,6 +72,9 @@ class KmcSimulation2D(DeviceKmc):
                     init_modes, dict(mean=2.0, std=0.6, min_cutoff=0.1), law=law, finite=finite)
         if enable_thermal:
             self._setup_thermal(Cp, rho, thermal_diffusivity, thermal_coords, temperature_cap, thermostat, tau_bath)
+        # fast patching (mgkmc/kmc_simulator.py:76-78, 154-171): off for finite strain and the nonlinear laws
+        self._setup_fast_patching(fast_patching, 5, strain_assumption == "finite_strain" or
+                                  hyperelastic_model in ("secant_degradation", "landau", "murnaghan", "rose"))
 
     def elastic_run(self, eps_macro):
         """Small-strain equilibrium (mgkmc/kmc_simulator.py:449-525): LS solve

@@ -116,7 +116,7 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                tau_bath=0.0, solver="newton_cg", use_3d_barriers=False, redraw_directions=True,
                redraw_barriers=True, barrier_generator="gaussian", stop_on_stress_drop=None,
                stress_drop_component=None, stop_post_drop_steps=20, ignore_drop_steps=0, stress_drop_lookback=1,
-               eps_target=None, fast=False, progress=None):
+               eps_target=None, fast=False, progress=None, fast_patching=None):
     """Returns the list of events (x, y, z, m, time, is_instab) and final state.
     ``solver``: the finite-strain step's solver ("newton_cg", "al", "dbfft").
     Stress-drop stop (aqs.py:834, :1070, :1149-1181): the driven stress
@@ -136,7 +136,16 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
     catalog is drawn in one vectorised call that consumes the stream like the
     per-voxel loop; the event sequence is the standard mode's
     (tests/test_oracle_kmc.py).  ``progress(n_events)`` is called after each
-    event."""
+    event.
+
+    ``fast_patching`` ({"enabled", "patch_radius" (3), "sync_interval" (100)},
+    mgkmc/aqs.py:196-218, :1021-1054; small strain only): after a KMC event's
+    flip, prev, heat and redraw the stress field is predicted by
+    oracle/patching.py (macro-unit stress over the waiting time, the patch
+    around the voxel, the missing mean) instead of re-solved; every
+    sync_interval such flips a full re-solve.  Cascades and elastic
+    increments re-solve as without it.  The driven-stress history and the
+    returned "sm" are the mean of the current (patched) field."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
     bk = dict(mean=2.0, std=0.6, min_cutoff=0.1) if barrier_kwargs is None else dict(barrier_kwargs)
@@ -243,6 +252,31 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
         state["sm"] = state["sig"].mean(axis=(0, 1, 2))
         if np.any(np.isnan(So)) or np.any(np.isnan(Fo)) or np.max(np.abs(state["sm"])) / 1e9 > 20.0:
             raise ValueError("Simulation stopped due to unstable stress")
+
+    fp = dict(fast_patching or {})
+    fp_on = bool(fp.get("enabled", False)) and not finite
+    if fp_on:
+        from . import patching as opt
+        fp_R = int(fp.get("patch_radius", 3))
+        fp_sync = int(fp.get("sync_interval", 100))
+        # at construction, with the default solver arguments (aqs.py:175, :217)
+        fp_K, fp_mm = opt.patch_kernels_3d(E, nu, fp_R, pixel=pixel)
+        state["fp_flips"] = 0
+        state["fp_unit"] = None
+
+    def patch_step(x, y, z, C, tw):
+        # mgkmc/aqs.py:1021-1054
+        if state["fp_unit"] is None:
+            _, state["fp_unit"], _, _ = ol.update_stress_fft_full(np.zeros_like(ep), unit, E, nu, pixel=pixel,
+                                                                  tol=ls_tol)
+        if tw > 0:
+            state["sig"] += state["fp_unit"] * (strain_rate * tw)
+        opt.stamp(state["sig"], (x, y, z), opt.coefficients_3d(C), fp_K, fp_mm, fp_R)
+        state["sm"] = state["sig"].mean(axis=(0, 1, 2))
+        state["fp_flips"] += 1
+        if state["fp_flips"] >= fp_sync:
+            update()
+            state["fp_flips"] = 0
 
     def flip_finite(flips):
         # one KMC event or one cascade batch (:512-615, :887-996)
@@ -396,7 +430,10 @@ def run_events(nx, ny, nz, M, gamma0, E, nu, pixel, temperature, strain_rate, st
                         prev[x, y, z] = C
                         flip_heat(x, y, z, C)
                         redraw(x, y, z, m)
-                        update()
+                        if fp_on:
+                            patch_step(x, y, z, C, tw)
+                        else:
+                            update()
                     events.append((int(x), int(y), int(z), int(m), t, bool(inst)))
                     hist.append(state["sm"][sdc])
                     if progress is not None:

@@ -67,9 +67,22 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                   stability_threshold=0.0, max_cascade_steps_pct=0.3, enable_thermal=False, Cp=420.0, rho=6125.0,
                   thermal_diffusivity=3.0e-6, thermal_coords="pixel", temperature_cap=1000.0, thermostat=False,
                   tau_bath=0.0, strain_assumption="small_strain", stz_mode="pure_shear", redraw_directions=True,
-                  redraw_barriers=True, solver="newton_cg"):
+                  redraw_barriers=True, solver="newton_cg", fast_patching=None, record=None):
     """Returns (events [(x, y, m, time, is_instab)], final state dict; its
-    "increments" holds (eps_macro, <sigma>) after each elastic increment)."""
+    "increments" holds (eps_macro, <sigma>) after each elastic increment).
+
+    ``fast_patching`` ({"enabled", "patch_radius" (5), "sync_interval" (100)},
+    mgkmc/kmc_simulator.py:76-78, :154-171, :229-261, :764-792, :1044-1098;
+    linear small strain only): KMC events and cascade flips predict the stress
+    with oracle/patching.py instead of re-solving (events: the macro-unit
+    stress over the waiting time first, weights from the flipped mode; cascade
+    flips: weights from the catalog entry after its redraw), a re-solve every
+    sync_interval event flips and after a cascade that flipped anything; once
+    the macro-unit stress exists the remaining time of an increment adds it
+    too.
+
+    ``record`` (a dict, test hook): its "cascades" list is the run's cascade
+    history as it grows, so a run that raises still shows how far it got."""
     np.random.seed(seed)
     sel = np.random.RandomState(sel_seed)
     bk = dict(mean=2.0, std=0.6, min_cutoff=0.1) if barrier_kwargs is None else dict(barrier_kwargs)
@@ -163,6 +176,20 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
         if np.any(np.isnan(s)) or np.max(np.abs(sm)) / 1e9 > 20.0:
             raise ValueError("Unstable stress in small-strain elastic_run")
         return sm
+
+    fp = dict(fast_patching or {})
+    fp_on = bool(fp.get("enabled", False)) and not finite and model == "linear"
+    if fp_on:
+        from . import patching as opt
+        fp_R = int(fp.get("patch_radius", 5))
+        fp_sync = int(fp.get("sync_interval", 100))
+        # at construction, with the default solver arguments (kmc_simulator.py:134, :171)
+        fp_K, fp_mm = opt.patch_kernels_2d(E, nu, fp_R, pixel=pixel, plane_mode=plane_mode)
+        state["fp_flips"] = 0
+        state["fp_unit"] = None
+
+    def patch(x, y, C):
+        opt.stamp(state["sig"], (x, y), opt.coefficients_2d(C), fp_K, fp_mm, fp_R)
 
     def redraw(x, y, m):
         # the voxel's modes and barriers, or only the flipped mode's (kmc_simulator.py:1000-1008)
@@ -268,7 +295,7 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
             if temperature_cap > 0:
                 Tl[x, y] = min(Tl[x, y], temperature_cap)
 
-    cascades = []
+    cascades = [] if record is None else record.setdefault("cascades", [])
 
     def run_cascade(step):
         local = total = 0
@@ -293,7 +320,10 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                     prev[x, y] = cat[x, y, m].copy()
                     flip_heat(x, y, prev[x, y])
                     redraw(x, y, m)
-                elastic_run()
+                    if fp_on:
+                        patch(x, y, cat[x, y, m])  # the entry after its redraw (kmc_simulator.py:764-765)
+                if not fp_on:
+                    elastic_run()
             else:
                 flip_substepped(flips)
             cascades.extend((step, local, int(x), int(y), int(m)) for x, y, m in flips)
@@ -301,6 +331,8 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
             local += 1
             if local > nx * ny:
                 break
+        if fp_on and total > 0:
+            elastic_run()  # kmc_simulator.py:790-792
         return total
 
     unit = np.zeros((2, 2))
@@ -354,7 +386,20 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                         prev[x, y] = C
                         flip_heat(x, y, C)
                         redraw(x, y, m)
-                        elastic_run()
+                        if fp_on:  # kmc_simulator.py:1044-1078
+                            if state["fp_unit"] is None:
+                                _, state["fp_unit"], _, _ = ol.spectral_solver_2d(
+                                    E, nu, unit, eps_plastic=np.zeros_like(ep), pixel=pixel, plane_mode=plane_mode,
+                                    tol=ls_tol)
+                            if tw > 0:
+                                state["sig"] += state["fp_unit"] * (strain_rate * tw)
+                            patch(x, y, C)
+                            state["fp_flips"] += 1
+                            if state["fp_flips"] >= fp_sync:
+                                elastic_run()
+                                state["fp_flips"] = 0
+                        else:
+                            elastic_run()
                     else:
                         flip_substepped([(x, y, m)])
                     events.append((int(x), int(y), int(m), t, bool(inst)))
@@ -362,6 +407,8 @@ def run_events_2d(nx, ny, M, gamma0, E, nu, pixel, temperature, strain_rate, ste
                         return result()
                     step += 1
                     continue
+            if fp_on and state["fp_unit"] is not None:
+                state["sig"] += state["fp_unit"] * (strain_rate * remaining)  # kmc_simulator.py:1094-1096
             eps_macro += unit * (strain_rate * remaining)
             heat(remaining)
             t += remaining
