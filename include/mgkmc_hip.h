@@ -428,6 +428,14 @@ int mgk_fs_constitutive(long long nvox, int precision, const void* F, const void
                         const void* mu, const mgk_fs_material* mat, void* P, const void* dF, void* dP,
                         void* stream);
 
+/* The 2D law (constitutive_hyperelastic_2d, mgkmc/finite_strain_simulator.py:591-1091) on
+ * the (nx, 1, ny) embedding of the plane problem: mat->plane 1 (plane strain) or 2 (plane
+ * stress, with lam = the plane-stress lam*), P and optionally dP as mgk_fs_constitutive,
+ * F33 (optional, nvox values) the out-of-plane stretch the reference returns. */
+int mgk_fs_constitutive_2d(long long nvox, int precision, const void* F, const void* Fp, const void* lam,
+                           const void* mu, const mgk_fs_material* mat, void* P, const void* dF, void* dP, void* F33,
+                           void* stream);
+
 /* sigma = P F^T / det F (cauchy_from_P_3d, mgkmc/finite_strain_simulator.py:1892-1904). */
 int mgk_fs_cauchy(long long nvox, int precision, const void* P, const void* F, void* sig, void* stream);
 

@@ -219,6 +219,8 @@ SIGNATURES = {
                                           ctypes.POINTER(MgkFsInfo)]),
     "mgk_fs_constitutive": (_c_i, [ctypes.c_longlong, _c_i, _c_p, _c_p, _c_p, _c_p, ctypes.POINTER(MgkFsMaterial),
                                    _c_p, _c_p, _c_p, _c_p]),
+    "mgk_fs_constitutive_2d": (_c_i, [ctypes.c_longlong, _c_i, _c_p, _c_p, _c_p, _c_p, ctypes.POINTER(MgkFsMaterial),
+                                      _c_p, _c_p, _c_p, _c_p, _c_p]),
     "mgk_fs_cauchy": (_c_i, [ctypes.c_longlong, _c_i, _c_p, _c_p, _c_p, _c_p]),
 }
 

@@ -36,15 +36,20 @@ struct FsMat {
   double v1, v2, v3, g1, g2, g3, g4;
   double Am, Bm, Cm;
   double Ecap, cap_p, cap_ratio;
+  // capped plane-stress landau: device ints [iteration count K, per-iteration
+  // "some update still >= 1e-9" mask, first non-finite iteration, sticky
+  // non-finite flag] of the latest E33 count (fs_ps_count); else unused
+  int* psS;
 };
 
-// plane-stress landau / murnaghan: E33 solve and condensed tangent needed
+// plane-stress landau / murnaghan / neo-Hookean: out-of-plane solve needed
 static inline bool fs_ps(const FsMat& m) { return m.plane == 2 && m.model != 0; }
 
 template <typename T>
 struct FsVox {
   T Fe[9], S[9], E[9], E2[9], Fpi[9];
   T lam, mu, trE, al, be, ga, B, C, cw;
+  T x33;  // plane stress: F33^2 (neo-Hookean x; landau / murnaghan 1 + 2 E33)
 };
 
 template <typename T>
@@ -137,6 +142,75 @@ __device__ T fs_plane_e33(const T* E, T lam2, T mu, const FsMat& m) {
   return e;
 }
 
+// One update of the capped plane-stress landau E33 Newton
+// (finite_strain_simulator.py:523-572, _solve_E33_capped_landau_2d): the cap
+// map of the deviator of diag-block(E_2d, E33) inside the residual,
+//   S33 = S33_landau(capped strain) + 2 G_t (E33 - c33) w,
+// the derivative without the cap factor's chain rule, -S33 / (dS33 + 1e-12).
+// E11 = E[0], E22 = E[8], E12 = E[2] on the (nx, 1, ny) grid; lam the 3D lam.
+template <typename T>
+__device__ __forceinline__ T fs_cap_e33_delta(const T* E, T e, T lam, T mu, const FsMat& m) {
+  const T E11 = E[0], E22 = E[8], E12 = E[2];
+  const T tr = E11 + E22 + e, t3 = tr / T(3);
+  const T d11 = E11 - t3, d22 = E22 - t3, d33 = e - t3;
+  const T Eeq = sqrt(fmax(T(0), (T(2) / T(3)) * (d11 * d11 + d22 * d22 + d33 * d33 + T(2) * E12 * E12)));
+  const T Ecap = T(m.Ecap);
+  T f = T(1), w = T(0);
+  if (m.cap_smooth) {
+    const T p = T(m.cap_p);
+    if (Eeq > T(1e-12)) f = (Ecap / Eeq) * pow(tanh(fmin(T(100), pow(Eeq / Ecap, p))), T(1) / p);
+    const T tw = tanh(fmin(T(100), pow(Eeq / Ecap, p)));
+    w = tw * tw;
+  } else if (Eeq > Ecap) {
+    f = Ecap / Eeq;
+    w = T(1);
+  }
+  const T c11 = f * d11 + t3, c22 = f * d22 + t3, c33 = f * d33 + t3, c12 = f * E12;
+  const T I1 = tr;
+  const T I2 = c11 * c11 + c22 * c22 + c33 * c33 + T(2) * c12 * c12;
+  const T t2 = c11 + c22, det2 = c11 * c22 - c12 * c12;
+  const T I3 = (t2 * t2 * t2 - T(3) * t2 * det2) + c33 * c33 * c33;
+  const T v1 = T(m.v1), v2 = T(m.v2), v3 = T(m.v3), g1 = T(m.g1), g2 = T(m.g2), g3 = T(m.g3), g4 = T(m.g4);
+  const T A = lam * I1 + T(0.5) * v1 * I1 * I1 + v2 * I2 + (T(1) / T(6)) * g1 * I1 * I1 * I1 + g2 * I1 * I2 +
+              (T(4) / T(3)) * g3 * I3;
+  const T B = T(2) * (mu + v2 * I1 + T(0.5) * g2 * I1 * I1 + g4 * I2);
+  const T C = T(4) * (v3 + g3 * I1);
+  const T dA = lam + v1 * I1 + T(0.5) * g1 * I1 * I1 + g2 * I2 + T(2) * (v2 + g2 * I1) * c33 + T(4) * g3 * c33 * c33;
+  const T dB = T(2) * (v2 + g2 * I1) + T(4) * g4 * c33;
+  const T Gt = T(m.cap_ratio) * mu;
+  const T S33 = A + B * c33 + C * c33 * c33 + T(2) * Gt * (e - c33) * w;
+  const T dS33 = dA + dB * c33 + B + T(4) * g3 * c33 * c33 + T(2) * C * c33 + T(2) * Gt * w;
+  return -S33 / (dS33 + T(1e-12));
+}
+
+// capped plane-stress E33: K updates from the linear guess -(lam / (lam + 2 mu)) tr E_2d,
+// K the grid's iteration count (the reference stops when the grid's max |update| < 1e-9)
+template <typename T>
+__device__ __forceinline__ T fs_plane_e33_capped(const T* E, T lam2, T mu, const FsMat& m, int K) {
+  const T lam = T(2) * mu * lam2 / fmax(T(1e-14), T(2) * mu - lam2);
+  T e = -(lam / (lam + T(2) * mu)) * (E[0] + E[8]);
+  for (int it = 0; it < K; ++it) e += fs_cap_e33_delta(E, e, lam, mu, m);
+  return e;
+}
+
+// plane-stress neo-Hookean (finite_strain_simulator.py:652-663): x = F33^2 with
+// mu x + lam ln(x) / 2 = mu - lam ln(max(1e-14, J_e)) by Newton from x = 1, at most
+// 10 steps, x >= 1e-10; a voxel stops once its own |dx| < 1e-12 (the reference
+// stops on the grid maximum: quadratic convergence leaves round-off to later steps)
+template <typename T>
+__device__ __forceinline__ T fs_nh_x(T Je, T lam, T mu) {
+  const T target = mu - lam * log(fmax(T(1e-14), Je));
+  T x = T(1);
+  for (int it = 0; it < 10; ++it) {
+    const T f = mu * x + T(0.5) * lam * log(x) - target;
+    const T df = mu + T(0.5) * lam / x;
+    const T dx = -f / df;
+    x = fmax(T(1e-10), x + dx);
+    if (fabs(dx) < T(1e-12)) break;
+  }
+  return x;
+}
+
 // stress P and the tangent coefficients at deformation F; PS = false compiles
 // the plane-stress landau / murnaghan branch out (the 3D matvec instantiation)
 template <typename T, bool PS = true>
@@ -156,8 +230,12 @@ __device__ __forceinline__ void fs_state(const T* F, const T* Fpi, T lam, T mu, 
 #pragma unroll
   for (int k = 0; k < 9; ++k) s.E[k] = T(0.5) * (Ce[k] - ((k % 4 == 0) ? T(1) : T(0)));
   if (PS && m.plane == 2 && m.model != 0) {
-    // plane-stress landau / murnaghan: the 3D law at E_yy = E33 of the (nx, 1, ny) grid
-    s.E[4] = fs_plane_e33(s.E, lam, mu, m);
+    // plane-stress landau / murnaghan: the 3D law at E_yy = E33 of the (nx, 1, ny) grid;
+    // neo-Hookean keeps E_yy = 0 and solves for F33^2 below
+    if (m.model == 1 && m.capping) s.E[4] = fs_plane_e33_capped(s.E, lam, mu, m, m.psS[0]);
+    else if (m.model != 3) s.E[4] = fs_plane_e33(s.E, lam, mu, m);
+    // F33^2 from E33 before any capping of the strain (:814)
+    if (m.model != 3) s.x33 = T(1) + T(2) * s.E[4];
     lam = T(2) * mu * lam / fmax(T(1e-14), T(2) * mu - lam);
     s.lam = lam;
   }
@@ -253,11 +331,23 @@ __device__ __forceinline__ void fs_state(const T* F, const T* Fpi, T lam, T mu, 
     s.E2[6] = (Ce[3] * Ce[7] - Ce[4] * Ce[6]) / dC;
     s.E2[7] = -(Ce[0] * Ce[7] - Ce[1] * Ce[6]) / dC;
     s.E2[8] = (Ce[0] * Ce[4] - Ce[1] * Ce[3]) / dC;
-    const T coeff = -mu + lam * lnJ;
+    T coeff;
+    if (PS && m.plane == 2) {
+      // plane stress (finite_strain_simulator.py:631-685): S = mu I - mu x C^-1,
+      // tangent lam* (C^-1 : X) C^-1 + 2 mu x C^-1 X C^-1 with
+      // lam* = 2 lam mu x / max(1e-14, lam + 2 mu x), lam the 3D lam
+      const T x = fs_nh_x(det3(s.Fe), lam, mu);
+      s.x33 = x;
+      coeff = -mu * x;
+      s.al = mu * x;
+      s.B = (T(2) * lam * s.al) / fmax(T(1e-14), lam + T(2) * s.al);
+    } else {
+      coeff = -mu + lam * lnJ;
+      s.al = mu - lam * lnJ;
+      s.B = lam;
+    }
 #pragma unroll
     for (int k = 0; k < 9; ++k) s.S[k] = ((k % 4 == 0) ? mu : T(0)) + coeff * s.E2[k];
-    s.al = mu - lam * lnJ;
-    s.B = lam;
     s.trE = s.E[0] + s.E[4] + s.E[8];
   } else {
     const T* E = s.E;
@@ -356,7 +446,7 @@ __device__ __forceinline__ void fs_tangent(const FsVox<T>& s, const FsMat& m, co
     for (int j = 0; j < 3; ++j) X[3 * i + j] = T(0.5) * (Y[3 * i + j] + Y[3 * j + i]);
   T dS[9];
   fs_dS(s, m, X, dS);
-  if (PS && m.plane == 2 && m.model != 0) {
+  if (PS && m.plane == 2 && m.model != 0 && m.model != 3) {
     // plane-stress landau / murnaghan: the in-plane tangent condensed on the
     // out-of-plane slot (finite_strain_simulator.py:389-395, :1026-1031),
     // C_eff = C_abcd - C_ab33 C_cd33 / max(1e-14, C_3333); with the major
@@ -394,6 +484,15 @@ __device__ __forceinline__ void fs_tangent(const FsVox<T>& s, const FsMat& m, co
 template <typename T>
 __device__ __forceinline__ T fs_plane_f33(const T* E, T lam2, T mu, const FsMat& m) {
   return sqrt(fmax(T(1e-12), T(1) + T(2) * fs_plane_e33(E, lam2, mu, m)));
+}
+
+// plane-stress F33 of a state from fs_state (svk: from E_2d and lam*; neo-Hookean:
+// sqrt(max(1e-12, x)) (:664); landau / murnaghan: sqrt(max(1e-12, 1 + 2 E33)) with E33
+// the out-of-plane root before capping (:814, :974))
+template <typename T>
+__device__ __forceinline__ T fs_state_f33(const FsVox<T>& s, const FsMat& m) {
+  if (m.model == 0) return fs_plane_f33(s.E, s.lam, s.mu, m);
+  return sqrt(fmax(T(1e-12), s.x33));
 }
 
 struct Nine {
@@ -448,9 +547,7 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_const(size_t NV, const T* __restr
     acc[2] += f2;
     // Cauchy sigma = P F^T / J(F)  (cauchy_from_P_3d; 2D plane stress: J = det F * F33)
     double J = (double)det3(F);
-    if (m.plane == 2)
-      J *= (double)((m.model == 0) ? fs_plane_f33(s.E, s.lam, s.mu, m)
-                                   : sqrt(fmax(T(1e-12), T(1) + T(2) * s.E[4])));
+    if (m.plane == 2) J *= (double)fs_state_f33(s, m);
     if (fabs(J) < 1e-14) J = 1e-14;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -501,19 +598,13 @@ __global__ void k_fs_cauchy(size_t NV, const T* __restrict__ P, const T* __restr
     }
     T J = det3(f);
     if (m.plane == 2) {
-      T fe[9], fpi[9], ce[9], e[9];
-      if (Fpi) {
+      T fpi[9];
+      if (Fpi)
 #pragma unroll
         for (int k = 0; k < 9; ++k) fpi[k] = Fpi[(size_t)k * NV + v];
-        mm3(f, fpi, fe);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) fe[k] = f[k];
-      }
-      mtm3(fe, fe, ce);
-#pragma unroll
-      for (int k = 0; k < 9; ++k) e[k] = T(0.5) * (ce[k] - ((k % 4 == 0) ? T(1) : T(0)));
-      J = J * fs_plane_f33(e, lam[v], mu[v], m);
+      FsVox<T> st;
+      fs_state<T, true>(f, Fpi ? fpi : (const T*)nullptr, lam[v], mu[v], m, st, (T*)nullptr);
+      J = J * fs_state_f33(st, m);
     }
     if (fabs((double)J) < 1e-14) J = T(1e-14);
     T o[9];
@@ -595,8 +686,15 @@ void launch_dP(int nblk, hipStream_t st, size_t NV, const T* F, const T* Fpi, co
       k_fs_dP_law<T, PSV, MD, CP, false><<<nblk, MGK_NT, 0, st>>>(NV, F, Fpi, lam, mu, m, dX, dXu, dP, skip);  \
   } while (0)
   if (ps) {
-    if (m.model == 1) MGK_DP(true, 1, 0);
-    else MGK_DP(true, 2, 0);
+    if (m.model == 1) {
+      if (cap == 0) MGK_DP(true, 1, 0);
+      else if (cap == 1) MGK_DP(true, 1, 1);
+      else MGK_DP(true, 1, 2);
+    } else if (m.model == 2) {
+      MGK_DP(true, 2, 0);
+    } else {
+      MGK_DP(true, 3, 0);
+    }
     return;
   }
   switch (m.model) {
@@ -1370,6 +1468,108 @@ int fs_nblk(long long n) {
   return (int)std::min<long long>((n + MGK_NT - 1) / MGK_NT, KRY_NBLK);
 }
 
+// Capped plane-stress landau: the reference runs the E33 Newton on the whole
+// grid until the largest update is below 1e-9 (finite_strain_simulator.py:
+// 523-579), so every voxel takes the same number K of updates.  The count pass
+// runs each voxel's 20 updates from F (a trial F = Fb + alpha (dXu + dX) formed
+// as k_fs_const forms it) and ORs "update it is not below 1e-9" into bit it of
+// a grid mask; K is one past the first clear bit (20 if none).  A non-finite
+// update before K sets the sticky flag (the reference's FloatingPointError).
+__global__ void k_fs_ps_init(int* ps) {
+  ps[1] = 0;
+  ps[2] = 20;
+}
+
+__global__ void k_fs_ps_reset(int* ps) {
+  ps[0] = 20;
+  ps[1] = 0;
+  ps[2] = 20;
+  ps[3] = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(MGK_NT) void k_fs_ps_count(size_t NV, const T* __restrict__ Fb,
+                                                        const T* __restrict__ dX, Nine dXu, double alpha, int trial,
+                                                        const T* __restrict__ Fp, int fp_inverse,
+                                                        const T* __restrict__ lam, const T* __restrict__ mu,
+                                                        FsMat m) {
+  unsigned mask = 0;
+  int nf = 20;
+  for (size_t v = blockIdx.x * (size_t)MGK_NT + threadIdx.x; v < NV; v += (size_t)gridDim.x * MGK_NT) {
+    T F[9], Fe[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      T f = Fb[(size_t)k * NV + v];
+      if (trial) {
+        const T x = (T)dXu.v[k] + (dX ? dX[(size_t)k * NV + v] : T(0));
+        f = f + (T)alpha * x;
+      }
+      F[k] = f;
+    }
+    if (Fp) {
+      T a[9], fpi[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) a[k] = Fp[(size_t)k * NV + v];
+      if (fp_inverse) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) fpi[k] = a[k];
+      } else {
+        inv3(a, fpi);
+      }
+      mm3(F, fpi, Fe);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Fe[k] = F[k];
+    }
+    T Ce[9], E[9];
+    mtm3(Fe, Fe, Ce);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) E[k] = T(0.5) * (Ce[k] - ((k % 4 == 0) ? T(1) : T(0)));
+    const T mv = mu[v], l2 = lam[v];
+    const T l3 = T(2) * mv * l2 / fmax(T(1e-14), T(2) * mv - l2);
+    T e = -(l3 / (l3 + T(2) * mv)) * (E[0] + E[8]);
+    for (int it = 0; it < 20; ++it) {
+      const T d = fs_cap_e33_delta(E, e, l3, mv, m);
+      if (!isfinite((double)d) && nf > it) nf = it;
+      e += d;
+      if (!(fabs((double)d) < 1e-9)) mask |= 1u << it;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    mask |= (unsigned)__shfl_xor((int)mask, off, 64);
+    nf = min(nf, __shfl_xor(nf, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicOr(m.psS + 1, (int)mask);
+    atomicMin(m.psS + 2, nf);
+  }
+}
+
+__global__ void k_fs_ps_fin(int* ps) {
+  const unsigned mask = (unsigned)ps[1];
+  int K = 20;
+  for (int it = 0; it < 20; ++it)
+    if (!((mask >> it) & 1u)) {
+      K = it + 1;
+      break;
+    }
+  ps[0] = K;
+  if (ps[2] < K) ps[3] = 1;
+}
+
+// iteration count of the capped plane-stress E33 at F (no-op for other laws);
+// Fp: F_p^-1 when fp_inverse, else F_p
+template <typename T>
+void fs_ps_count(const FsMat& m, hipStream_t s, size_t NV, const T* Fb, const T* dX, const Nine& dXu, double alpha,
+                 int trial, const T* Fp, int fp_inverse, const T* lam, const T* mu) {
+  if (!(m.plane == 2 && m.model == 1 && m.capping)) return;
+  k_fs_ps_init<<<1, 1, 0, s>>>(m.psS);
+  k_fs_ps_count<T><<<fs_nblk<T>((long long)NV), MGK_NT, 0, s>>>(NV, Fb, dX, dXu, alpha, trial, Fp, fp_inverse, lam,
+                                                                 mu, m);
+  k_fs_ps_fin<<<1, 1, 0, s>>>(m.psS);
+}
+
 // A x = G((K : x^T)^T) (matrix-free); F = current deformation (Fcur)
 template <typename T>
 void fs_matvec(FsCtx<T>& c, const T* Fcur, const T* in, T* out, const T* dota, int selfdot, double* partials,
@@ -1385,6 +1585,7 @@ void fs_matvec(FsCtx<T>& c, const T* Fcur, const T* in, T* out, const T* dota, i
 // SciPy bicgstab(A, b, x0=None, rtol, maxiter) -> c.x
 template <typename T>
 int fs_bicgstab(FsCtx<T>& c, const T* Fcur, double rtol, int maxiter, int* iters, int* status) {
+  fs_ps_count<T>(c.mat, c.s, (size_t)c.pl.NV, Fcur, (const T*)nullptr, Nine{}, 0.0, 0, c.Fpi, 1, c.lam, c.mu);
   const FsPlan& pl = c.pl;
   const size_t n = (size_t)9 * pl.NV;
   const int nb = fs_nblk<T>((long long)n);
@@ -1437,13 +1638,22 @@ template <typename T>
 int fs_const(FsCtx<T>& c, const T* Fb, const T* dX, const Nine& dXu, double alpha, int trial, T* Fout, T* P,
              double* res, int chk_tan = 1) {
   const int nb = fs_nblk<T>(c.pl.NV);
+  const bool pscap = c.mat.plane == 2 && c.mat.model == 1 && c.mat.capping;
+  fs_ps_count<T>(c.mat, c.s, (size_t)c.pl.NV, Fb, dX, dXu, alpha, trial, c.Fpi, 1, c.lam, c.mu);
   k_fs_const<T><<<nb, MGK_NT, 0, c.s>>>((size_t)c.pl.NV, Fb, dX, dXu, alpha, trial, Fout, c.Fpi, c.lam, c.mu, c.mat,
                                         P, c.part, chk_tan);
   MGK_HIP(hipGetLastError());
   // min over blocks of column 0 is needed, the rest are sums
   std::vector<double> hp((size_t)nb * FS_NPART);
+  int psbad = 0;
   MGK_HIP(hipMemcpyAsync(hp.data(), c.part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, c.s));
+  if (pscap) MGK_HIP(hipMemcpyAsync(&psbad, c.mat.psS + 3, sizeof(int), hipMemcpyDeviceToHost, c.s));
   MGK_HIP(hipStreamSynchronize(c.s));
+  if (psbad) {
+    mgk_set_error("constitutive_hyperelastic_2d: non-finite capped E33 Newton update (Landau energy likely "
+                  "unstable at this strain)", __FILE__, __LINE__);
+    return MGK_ERR_NONFINITE;
+  }
   double mn = 1e300;
   for (int k = 0; k < FS_NPART; ++k) res[k] = 0.0;
   for (int i = 0; i < nb; ++i) {
@@ -1455,16 +1665,10 @@ int fs_const(FsCtx<T>& c, const T* Fb, const T* dX, const Nine& dXu, double alph
   return MGK_OK;
 }
 
-// Material combinations the device laws implement: models 0..3, plane 0..2;
-// plane stress (plane 2) covers svk and the uncapped landau / murnaghan laws
-// (capped landau and neo-Hookean in plane stress are not built).
+// Material combinations the device laws implement: models 0..3, plane 0..2.
 static int fs_mat_check(const mgk_fs_material* mt) {
   if (mt->model < 0 || mt->model > 3 || mt->plane < 0 || mt->plane > 2) {
     mgk_set_error("finite-strain material: model must be 0..3 and plane 0..2", __FILE__, __LINE__);
-    return MGK_ERR_ARG;
-  }
-  if (mt->plane == 2 && ((mt->model == 1 && mt->capping) || mt->model == 3)) {
-    mgk_set_error("plane-stress capped landau and neo-Hookean are not supported", __FILE__, __LINE__);
     return MGK_ERR_ARG;
   }
   return MGK_OK;
@@ -1545,6 +1749,8 @@ int fs_ctx_init(FsCtx<T>& c, const int64_t* shape, const T* lam, const T* mu, co
   c.scal = (double*)w;
   w += 4096 - mgk_align(sizeof(KryState));
   c.Kbar = (double*)w;
+  m.psS = (int*)(c.Kbar + 96);
+  if (m.plane == 2 && m.model == 1 && m.capping) k_fs_ps_reset<<<1, 1, 0, s>>>(m.psS);
   if (!has_fp) c.Fpi = nullptr;
   return MGK_OK;
 }
@@ -1667,6 +1873,7 @@ void fs_db_matvec(FsCtx<T>& c, const DbPar& d, const T* Fcur, const T* in, T* ou
 // NV times the real-space dot; every other test is scale-free.
 template <typename T>
 int fs_db_bicgstab(FsCtx<T>& c, const DbPar& d, const T* Fcur, double rtol, int maxiter, int* iters, int* status) {
+  fs_ps_count<T>(c.mat, c.s, (size_t)c.pl.NV, Fcur, (const T*)nullptr, Nine{}, 0.0, 0, c.Fpi, 1, c.lam, c.mu);
   const FsPlan& pl = c.pl;
   const size_t NV = (size_t)pl.NV, n = 3 * NV;
   const int nb = fs_nblk<T>((long long)n);
@@ -1742,6 +1949,7 @@ int fs_db_newton(FsCtx<T>& c, DbPar& d, T* F, const double* Fbc, double tol, dou
     }
     rel = sqrt((double)NV * bb[1]) / (sqrt(pp) + 1e-20);
     if (rel < tol && i > 0) return MGK_OK;
+    fs_ps_count<T>(c.mat, c.s, NV, F, (const T*)nullptr, Nine{}, 0.0, 0, c.Fpi, 1, c.lam, c.mu);
     k_fs_kmean<T><<<dim3(nbk, 9), MGK_NT, 0, c.s>>>(NV, F, c.Fpi, c.lam, c.mu, c.mat, c.part);
     k_fs_kmean_fin<<<81, MGK_NT, 0, c.s>>>(c.part, nbk, inv, c.Kbar);
     int kit = 0, kst = 0;
@@ -1935,6 +2143,7 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
       info->newton_iters += 1;
       // rhs
       if (i_NW == 0) {
+        fs_ps_count<T>(c.mat, s, NV, F, (const T*)nullptr, Nine{}, 0.0, 0, c.Fpi, 1, c.lam, c.mu);
         launch_dP<T>(fs_nblk<T>(pl.NV), s, NV, F, c.Fpi, c.lam, c.mu, c.mat, (const T*)nullptr, DbarF, c.dP, nullptr);
         fs_project(c, c.dP, c.b, T(-1.0 / (double)NV), (const T*)nullptr, 0, (double*)nullptr, (const int*)nullptr);
       } else {
@@ -2049,7 +2258,10 @@ int fs_step_t(const int64_t* shape, T* F, const T* Fp, const T* lam, const T* mu
     return MGK_ERR_SOLVER;
   }
   if (P_out) MGK_HIP(hipMemcpyAsync(P_out, c.P, 9 * NV * sizeof(T), hipMemcpyDeviceToDevice, s));
-  if (sig_out) k_fs_cauchy<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, c.P, F, sig_out, c.mat, c.Fpi, c.lam, c.mu);
+  if (sig_out) {
+    fs_ps_count<T>(c.mat, s, NV, F, (const T*)nullptr, Nine{}, 0.0, 0, c.Fpi, 1, c.lam, c.mu);
+    k_fs_cauchy<T><<<fs_nblk<T>(pl.NV), MGK_NT, 0, s>>>(NV, c.P, F, sig_out, c.mat, c.Fpi, c.lam, c.mu);
+  }
   {
     const int nb = fs_nblk<T>(pl.NV);
     k_comp_means9<T><<<nb, MGK_NT, 0, s>>>(NV, F, c.part);
@@ -2253,13 +2465,14 @@ static FsMat to_fsmat(const mgk_fs_material* mt) {
   m.Ecap = mt->cap_limit;
   m.cap_p = mt->cap_power;
   m.cap_ratio = mt->cap_ratio;
+  m.psS = nullptr;
   return m;
 }
 
 template <typename T>
 static __global__ void k_fs_pointwise(size_t NV, const T* __restrict__ F, const T* __restrict__ Fp,
                                       const T* __restrict__ lam, const T* __restrict__ mu, FsMat m, T* __restrict__ P,
-                                      const T* __restrict__ dF, T* __restrict__ dP) {
+                                      const T* __restrict__ dF, T* __restrict__ dP, T* __restrict__ F33) {
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < NV; v += (size_t)gridDim.x * blockDim.x) {
     T f[9], fpi[9], d[9];
 #pragma unroll
@@ -2276,6 +2489,7 @@ static __global__ void k_fs_pointwise(size_t NV, const T* __restrict__ F, const 
     if (P)
 #pragma unroll
       for (int k = 0; k < 9; ++k) P[(size_t)k * NV + v] = Pv[k];
+    if (F33) F33[v] = (m.plane == 2) ? fs_state_f33(s, m) : T(1);
     if (dF && dP) {
 #pragma unroll
       for (int k = 0; k < 9; ++k) d[k] = dF[(size_t)k * NV + v];
@@ -2290,25 +2504,67 @@ static __global__ void k_fs_pointwise(size_t NV, const T* __restrict__ F, const 
 // Pointwise constitutive model: P(F) and optionally the tangent action
 // dP = K:dF in the reference's (K4 : dF^T)^T convention
 // (constitutive_hyperelastic_3d, mgkmc/finite_strain_simulator.py:1647-1885).
-extern "C" int mgk_fs_constitutive(long long nvox, int precision, const void* F, const void* Fp, const void* lam,
-                                   const void* mu, const mgk_fs_material* mat, void* P, const void* dF, void* dP,
-                                   void* stream) {
+static int fs_constitutive_impl(long long nvox, int precision, const void* F, const void* Fp, const void* lam,
+                                const void* mu, const mgk_fs_material* mat, void* P, const void* dF, void* dP,
+                                void* F33, void* stream) {
   if (nvox < 1 || !F || !lam || !mu || !mat) {
     mgk_set_error("mgk_fs_constitutive: bad argument", __FILE__, __LINE__);
     return MGK_ERR_ARG;
   }
   if (fs_mat_check(mat) != MGK_OK) return MGK_ERR_ARG;
-  const FsMat m = to_fsmat(mat);
+  FsMat m = to_fsmat(mat);
   const int nb = (int)std::min<long long>((nvox + 255) / 256, 4096);
   hipStream_t s = (hipStream_t)stream;
+  const bool pscap = m.plane == 2 && m.model == 1 && m.capping;
+  if (pscap) {
+    // the grid's E33 iteration count for this F (a short-lived device int block)
+    MGK_HIP(hipMallocAsync((void**)&m.psS, 4 * sizeof(int), s));
+    k_fs_ps_reset<<<1, 1, 0, s>>>(m.psS);
+    if (precision == 0)
+      fs_ps_count<double>(m, s, (size_t)nvox, (const double*)F, (const double*)nullptr, Nine{}, 0.0, 0,
+                          (const double*)Fp, 0, (const double*)lam, (const double*)mu);
+    else
+      fs_ps_count<float>(m, s, (size_t)nvox, (const float*)F, (const float*)nullptr, Nine{}, 0.0, 0,
+                         (const float*)Fp, 0, (const float*)lam, (const float*)mu);
+  }
   if (precision == 0)
     k_fs_pointwise<double><<<nb, 256, 0, s>>>((size_t)nvox, (const double*)F, (const double*)Fp, (const double*)lam,
-                                              (const double*)mu, m, (double*)P, (const double*)dF, (double*)dP);
+                                              (const double*)mu, m, (double*)P, (const double*)dF, (double*)dP,
+                                              (double*)F33);
   else
     k_fs_pointwise<float><<<nb, 256, 0, s>>>((size_t)nvox, (const float*)F, (const float*)Fp, (const float*)lam,
-                                             (const float*)mu, m, (float*)P, (const float*)dF, (float*)dP);
+                                             (const float*)mu, m, (float*)P, (const float*)dF, (float*)dP,
+                                             (float*)F33);
   MGK_HIP(hipGetLastError());
+  if (pscap) {
+    int bad = 0;
+    MGK_HIP(hipMemcpyAsync(&bad, m.psS + 3, sizeof(int), hipMemcpyDeviceToHost, s));
+    MGK_HIP(hipFreeAsync(m.psS, s));
+    MGK_HIP(hipStreamSynchronize(s));
+    if (bad) {
+      mgk_set_error("constitutive_hyperelastic_2d: non-finite capped E33 Newton update", __FILE__, __LINE__);
+      return MGK_ERR_NONFINITE;
+    }
+  }
   return MGK_OK;
+}
+
+extern "C" int mgk_fs_constitutive(long long nvox, int precision, const void* F, const void* Fp, const void* lam,
+                                   const void* mu, const mgk_fs_material* mat, void* P, const void* dF, void* dP,
+                                   void* stream) {
+  return fs_constitutive_impl(nvox, precision, F, Fp, lam, mu, mat, P, dF, dP, nullptr, stream);
+}
+
+// The 2D law on the (nx, 1, ny) embedding with the out-of-plane stretch F33
+// (constitutive_hyperelastic_2d, mgkmc/finite_strain_simulator.py:591-1091).
+extern "C" int mgk_fs_constitutive_2d(long long nvox, int precision, const void* F, const void* Fp,
+                                      const void* lam, const void* mu, const mgk_fs_material* mat, void* P,
+                                      const void* dF, void* dP, void* F33, void* stream) {
+  if (!mat || mat->plane < 1) {
+    mgk_set_error("mgk_fs_constitutive_2d: the material must be a plane problem (plane 1 or 2)", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  return fs_constitutive_impl(nvox, precision, F, Fp, lam, mu, mat, P, dF, dP, F33, stream);
 }
 
 // Cauchy stress sigma = P F^T / det F per voxel (cauchy_from_P_3d,

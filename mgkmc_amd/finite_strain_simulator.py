@@ -154,6 +154,8 @@ class FsSolver:
                 self.ws.numel(), ctypes.c_void_p(dev.stream_ptr()), ctypes.byref(self.info))
         if rc == _lib.MGK_ERR_SOLVER:
             raise ValueError(_lib.last_error().split(" (")[0])
+        if rc == _lib.MGK_ERR_NONFINITE:  # the capped plane-stress E33 iteration (finite_strain_simulator.py:573-576)
+            raise FloatingPointError(_lib.last_error().split(" (/")[0])
         _lib.check(rc, "mgk_fs_solver_step")
         return self.info
 
@@ -190,16 +192,10 @@ _PLANE_AXES = (0, 2)  # in-plane axes of the (nx, 1, ny) grid a 2D problem runs 
 
 def plane_law_supported(model_type, plane_mode, capping=False):
     """Whether the 2D device step covers a law in a plane mode: svk,
-    murnaghan and landau in both modes, neo_hookean in plane strain; not
-    capped landau in plane stress, whose E33 iteration in the reference
-    (finite_strain_simulator.py:486-585) stops on the grid-wide maximum
-    update with linear convergence, so a per-voxel stop would not match it
-    to round-off."""
-    if model_type not in ("svk", "landau", "murnaghan", "neo_hookean"):
-        return False
-    if plane_mode == "plane_strain":
-        return True
-    return model_type in ("svk", "murnaghan") or (model_type == "landau" and not capping)
+    landau (capped or not), murnaghan and neo_hookean in both modes (the
+    reference's 2D laws, finite_strain_simulator.py:591-1091)."""
+    return model_type in ("svk", "landau", "murnaghan", "neo_hookean") and \
+        plane_mode in ("plane_strain", "plane_stress")
 
 
 def _embed2(A, fill=1.0):
@@ -243,9 +239,8 @@ def finite_strain_solver_step_2d(F, F_bar, Ghat4, C4, I2, I4, I4rt, Fp=None, dri
     if solver not in ("newton_cg", "dbfft", "al"):
         raise NotImplementedError(f"solver={solver!r}: the 2D device step implements 'newton_cg', 'al' and 'dbfft'")
     if not plane_law_supported(model_type, plane_mode, strain_capping_enabled):
-        raise NotImplementedError(f"2D {model_type} in {plane_mode}: the device step covers svk, murnaghan and "
-                                  "landau (both plane modes; plane-stress landau uncapped) and neo_hookean "
-                                  "(plane strain)")
+        raise NotImplementedError(f"2D {model_type} in {plane_mode}: the device step covers svk, landau, "
+                                  "murnaghan and neo_hookean")
     C4 = np.asarray(C4)
     nx, ny = C4.shape[4:]
     sh3 = (nx, 1, ny)
@@ -305,6 +300,61 @@ def constitutive_hyperelastic_3d(F, C4, I2, I4, I4rt, Fp=None, model_type="svk",
                 # the matvec convention: (K4 : dF^T)^T_ij = sum_kl K4[j,i,k,l] dF[k,l]
                 K4[:, :, k, l] = dev.download(dP).swapaxes(0, 1)
     return dev.download(P), K4
+
+
+def constitutive_hyperelastic_2d(F, C4, I2, I4, I4rt, Fp=None, model_type="svk", plane_mode="plane_strain",
+                                 A_m=0.0, B_m=0.0, C_m=0.0, v1=0.0, v2=0.0, v3=0.0, g1=0.0, g2=0.0, g3=0.0, g4=0.0,
+                                 strain_capping_enabled=False, strain_capping_limit=None,
+                                 strain_capping_tangent_ratio=0.1, strain_capping_type="piecewise",
+                                 strain_capping_smooth_power=1.0, need_tangent=True, tangent_method="mandel"):
+    """(P, K4, F33) of the reference's 2D law (:591-1091) on the device, through
+    the (nx, 1, ny) embedding of the plane problem (mgk_fs_constitutive_2d).  K4
+    (16 comps per voxel) is assembled from four tangent actions only when
+    need_tangent is True."""
+    torch = dev.torch
+    dev.require_gpu()
+    if not plane_law_supported(model_type, plane_mode):
+        raise NotImplementedError(f"2D {model_type} in {plane_mode}")
+    C4 = np.asarray(C4)
+    nx, ny = C4.shape[4:]
+    sh3 = (nx, 1, ny)
+    lam = np.ascontiguousarray(C4[0, 0, 1, 1]).reshape(sh3)
+    mu = np.ascontiguousarray(C4[0, 1, 0, 1]).reshape(sh3)
+    nv = nx * ny
+    mat = material(model_type, A_m, B_m, C_m, v1, v2, v3, g1, g2, g3, g4, strain_capping_enabled,
+                   strain_capping_limit, strain_capping_tangent_ratio, strain_capping_type,
+                   strain_capping_smooth_power, mu_mean=float(np.mean(mu)), ndim=2)
+    mat.plane = 2 if plane_mode == "plane_stress" else 1
+    Fd = dev.upload(np.ascontiguousarray(_embed2(F)), 0)
+    Fpd = None if Fp is None else dev.upload(np.ascontiguousarray(_embed2(Fp)), 0)
+    ld, md = dev.upload(lam, 0), dev.upload(mu, 0)
+    P = torch.empty((3, 3) + sh3, dtype=torch.float64, device="cuda")
+    F33 = torch.empty(sh3, dtype=torch.float64, device="cuda")
+    lib = _lib.load()
+    st = ctypes.c_void_p(dev.stream_ptr())
+
+    def call(Pout, e, dP, f33):
+        rc = lib.mgk_fs_constitutive_2d(nv, 0, _lib.dptr(Fd), _lib.dptr(Fpd), _lib.dptr(ld), _lib.dptr(md),
+                                        ctypes.byref(mat), _lib.dptr(Pout), _lib.dptr(e), _lib.dptr(dP),
+                                        _lib.dptr(f33), st)
+        if rc == _lib.MGK_ERR_NONFINITE:
+            raise FloatingPointError(_lib.last_error().split(" (/")[0])
+        _lib.check(rc, "mgk_fs_constitutive_2d")
+
+    call(P, None, None, F33)
+    K4 = None
+    if need_tangent:
+        K4 = np.empty((2, 2, 2, 2, nx, ny))
+        dP = torch.empty((3, 3) + sh3, dtype=torch.float64, device="cuda")
+        for k in range(2):
+            for l in range(2):
+                e2 = np.zeros((2, 2, nx, ny))
+                e2[k, l] = 1.0
+                e = dev.upload(np.ascontiguousarray(_embed2(e2, 0.0)), 0)
+                call(None, e, dP, None)
+                # the matvec convention: (K4 : dF^T)^T_ij = sum_kl K4[j,i,k,l] dF[k,l]
+                K4[:, :, k, l] = _extract2(dev.download(dP)).swapaxes(0, 1)
+    return _extract2(dev.download(P)), K4, dev.download(F33).reshape(nx, ny)
 
 
 def cauchy_from_P_3d(P, F):

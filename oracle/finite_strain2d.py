@@ -14,8 +14,18 @@ Follows mgkmc/finite_strain_simulator.py:
                 murnaghan's quadratic root (:931-954), the tangent condensed
                 on the 33 slot (:1026-1031)), the einsum tangent of the 3D law
                 cut to the in-plane block, neo-Hookean in plane strain
-                (:686-712: S = mu I + (-mu + lam ln J) C^-1 and its tangent),
-                Fp, the push-forward to K4
+                (:686-712: S = mu I + (-mu + lam ln J) C^-1 and its tangent)
+                and in plane stress (:631-685: x = F33^2 of
+                mu x + lam ln(x) / 2 = mu - lam ln J by Newton from x = 1,
+                at most 10 steps, x >= 1e-10, stop when the grid's max |dx| <
+                1e-12; S = mu I - mu x C^-1, tangent lam* C^-1 C^-1 +
+                mu x (C^-1 C^-1 sym) with lam* = 2 lam mu x / max(1e-14,
+                lam + 2 mu x), lam the 3D lam from lam*), capped landau in
+                plane stress (:486-585 _solve_E33_capped_landau_2d: Newton on
+                S33 of the capped law with the cap factor held fixed in the
+                derivative, at most 20 steps, stop when the grid's max |dE33| <
+                1e-9, FloatingPointError on a non-finite update), Fp, the
+                push-forward to K4
   :1094-1114    cauchy_from_P (J = det F * F33)
   :2356-2572    finite_strain_solver_step_2d, solver "newton_cg", "dbfft" or "al"
   :1195-1210,1232-1248,1279-1313,1321-1349,1381-1507  the 2D DBFFT step (frequencies,
@@ -23,8 +33,6 @@ Follows mgkmc/finite_strain_simulator.py:
   :2007-2140    _al_step_2d
   :2580-2850    finite_strain_simulation_2d (load stepping, store=False)
 The Krylov solver is SciPy's own bicgstab, called as the reference calls it.
-Not restated (the device step does not offer them): neo-Hookean and capped
-landau in plane stress.
 
 Pinned by (tests/test_oracle_fs2d.py): the homogeneous closed form, the 3D
 oracle on the same problem embedded as an (nx, 1, ny) grid (plane strain:
@@ -34,7 +42,12 @@ and murnaghan laws against the 3D oracle's law at F_33 = sqrt(1 + 2 E33)
 (P_33 = 0 and the same in-plane P), central differences of P(F) against K4
 (in plane stress: the derivative through E33(F), which pins the condensation),
 equilibrium of the converged step, the reference's DBFFT-vs-NCG and Landau AL
-simulation cases.
+simulation cases; plane-stress neo-Hookean against an independently written
+3D neo-Hookean law at F_33 = sqrt(x) (S_33 = 0, same in-plane S) and central
+differences of P(F) (its tangent is the exact derivative through x); capped
+plane-stress landau: S_33 of the capped 3D law at the returned E33 within
+what the 1e-9 step tolerance leaves, the uncapped root when the cap is out of
+reach.
 """
 import numpy as np
 import scipy.fft as _sfft
@@ -231,19 +244,100 @@ def _plane_stress_e33(E2d, lam, mu, model_type, mat):
     return e
 
 
+def _plane_stress_e33_capped(E2d, lam, mu, mat):
+    """E33 with S33 = 0 of the capped landau law (finite_strain_simulator.py:486-585):
+    from the linear guess -(lam / (lam + 2 mu)) tr E_2d, the cap map of the
+    deviator inside the residual (S33 of the landau law at the capped strain
+    plus 2 G_t (E33 - c33) w), its derivative without the cap factor's chain
+    rule, E33 += -S33 / (dS33 + 1e-12), at most 20 steps, stop when the grid's
+    max |update| < 1e-9."""
+    E11, E22, E12 = E2d[0, 0], E2d[1, 1], E2d[0, 1]
+    sh = lam.shape
+    v1, v2, v3, g1, g2, g3, g4 = (np.full(sh, c) for c in landau_constants(
+        [mat.get(k, 0.0) for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4")]))
+    lim = mat.get("strain_capping_limit")
+    g4m, mum = g4.mean(), mu.mean()
+    Ecap = lim if (lim is not None and lim > 0) else (np.sqrt(0.2 * mum / abs(g4m)) if g4m < 0 else 999.0)
+    Gt = mat.get("strain_capping_tangent_ratio", 0.1) * mu
+    smooth = mat.get("strain_capping_type", "piecewise") == "smooth"
+    p = mat.get("strain_capping_smooth_power", 1.0)
+    E33 = -(lam / (lam + 2.0 * mu)) * (E11 + E22)
+    for _ in range(20):
+        tr = E11 + E22 + E33
+        t3 = tr / 3.0
+        d11, d22, d33 = E11 - t3, E22 - t3, E33 - t3
+        Eeq = np.sqrt(np.maximum(0.0, (2.0 / 3.0) * (d11 ** 2 + d22 ** 2 + d33 ** 2 + 2.0 * E12 ** 2)))
+        if smooth:
+            f = np.ones_like(Eeq)
+            nz = Eeq > 1e-12
+            f[nz] = (Ecap / Eeq[nz]) * np.tanh(np.minimum(100.0, (Eeq[nz] / Ecap) ** p)) ** (1.0 / p)
+            w = np.tanh(np.minimum(100.0, (Eeq / Ecap) ** p)) ** 2
+        else:
+            capd = Eeq > Ecap
+            f = np.ones_like(Eeq)
+            np.divide(Ecap, Eeq, out=f, where=capd)
+            w = capd.astype(float)
+        c11, c22, c33, c12 = f * d11 + t3, f * d22 + t3, f * d33 + t3, f * E12
+        I1 = tr
+        I2 = c11 ** 2 + c22 ** 2 + c33 ** 2 + 2.0 * c12 ** 2
+        t2 = c11 + c22
+        det2 = c11 * c22 - c12 ** 2
+        I3 = (t2 ** 3 - 3.0 * t2 * det2) + c33 ** 3
+        A = lam * I1 + 0.5 * v1 * I1 ** 2 + v2 * I2 + g1 * I1 ** 3 / 6.0 + g2 * I1 * I2 + (4.0 / 3.0) * g3 * I3
+        B = 2.0 * (mu + v2 * I1 + 0.5 * g2 * I1 ** 2 + g4 * I2)
+        C = 4.0 * (v3 + g3 * I1)
+        S33 = A + B * c33 + C * c33 ** 2 + 2.0 * Gt * (E33 - c33) * w
+        dA = lam + v1 * I1 + 0.5 * g1 * I1 ** 2 + g2 * I2 + 2.0 * (v2 + g2 * I1) * c33 + 4.0 * g3 * c33 ** 2
+        dB = 2.0 * (v2 + g2 * I1) + 4.0 * g4 * c33
+        dS33 = dA + dB * c33 + B + 4.0 * g3 * c33 ** 2 + 2.0 * C * c33 + 2.0 * Gt * w
+        delta = -S33 / (dS33 + 1e-12)
+        if not np.all(np.isfinite(delta)):
+            raise FloatingPointError("constitutive_hyperelastic_2d: non-finite capped E33 Newton update")
+        E33 = E33 + delta
+        if np.max(np.abs(delta)) < 1e-9:
+            break
+    return E33
+
+
+def _nh_plane_stress_x(Je, lam3, mu):
+    """x = F33^2 of the plane-stress neo-Hookean law (finite_strain_simulator.py:652-663)."""
+    x = np.ones_like(Je)
+    target = mu - lam3 * np.log(np.maximum(1e-14, Je))
+    for _ in range(10):
+        f = mu * x + 0.5 * lam3 * np.log(x) - target
+        df = mu + 0.5 * lam3 / x
+        dx = -f / df
+        x = np.maximum(1e-10, x + dx)
+        if np.max(np.abs(dx)) < 1e-12:
+            break
+    return x
+
+
 def constitutive_2d(F, C4, I2, I4, I4rt, Fp=None, model_type="svk", plane_mode="plane_strain", need_tangent=True,
                     **mat):
     """(P, K4, F33) of constitutive_hyperelastic_2d (:591-1091) for the laws
     listed in the module docstring."""
-    if model_type not in ("svk", "landau", "murnaghan", "neo_hookean") or \
-            (plane_mode == "plane_stress" and (model_type == "neo_hookean" or
-                                               (model_type == "landau" and mat.get("strain_capping_enabled")))):
+    if model_type not in ("svk", "landau", "murnaghan", "neo_hookean"):
         raise NotImplementedError(f"{model_type} / {plane_mode}")
     Fpi = _inv2(Fp) if Fp is not None else None
     Fe = np.einsum("ijxy,jkxy->ikxy", F, Fpi) if Fpi is not None else F
     lam, mu = C4[0, 0, 1, 1], C4[0, 1, 0, 1]
     E2d = 0.5 * (np.einsum("jixy,jkxy->ikxy", Fe, Fe) - I2)
-    if model_type == "neo_hookean":
+    if model_type == "neo_hookean" and plane_mode == "plane_stress":
+        lam3 = (2.0 * mu * lam) / np.maximum(1e-14, 2.0 * mu - lam)
+        Ce = np.einsum("jixy,jkxy->ikxy", Fe, Fe)
+        x = _nh_plane_stress_x(Fe[0, 0] * Fe[1, 1] - Fe[0, 1] * Fe[1, 0], lam3, mu)
+        F33 = np.sqrt(np.maximum(1e-12, x))
+        Ci = _inv2(Ce)
+        S = mu[None, None] * I2 + (-mu * x)[None, None] * Ci
+        C4e = None
+        if need_tangent:
+            ct = mu * x
+            ls = (2.0 * lam3 * ct) / np.maximum(1e-14, lam3 + 2.0 * ct)
+            C4e = (ls[None, None, None, None] * np.einsum("ijxy,klxy->ijklxy", Ci, Ci)
+                   + ct[None, None, None, None] * (np.einsum("ikxy,jlxy->ijklxy", Ci, Ci)
+                                                   + np.einsum("ilxy,jkxy->ijklxy", Ci, Ci)))
+    elif model_type == "neo_hookean":
         Ce = np.einsum("jixy,jkxy->ikxy", Fe, Fe)
         lnJ = np.log(np.maximum(1e-14, Fe[0, 0] * Fe[1, 1] - Fe[0, 1] * Fe[1, 0]))
         Ci = _inv2(Ce)
@@ -267,7 +361,10 @@ def constitutive_2d(F, C4, I2, I4, I4rt, Fp=None, model_type="svk", plane_mode="
         F33 = np.ones_like(lam)
         if plane_mode == "plane_stress":
             lam = (2.0 * mu * lam) / np.maximum(1e-14, 2.0 * mu - lam)
-            EG[2, 2] = _plane_stress_e33(E2d, lam, mu, model_type, mat)
+            if model_type == "landau" and mat.get("strain_capping_enabled"):
+                EG[2, 2] = _plane_stress_e33_capped(E2d, lam, mu, mat)
+            else:
+                EG[2, 2] = _plane_stress_e33(E2d, lam, mu, model_type, mat)
             F33 = np.sqrt(np.maximum(1e-12, 1.0 + 2.0 * EG[2, 2]))
         S3, C43, _ = _law3d(EG, lam, mu, model_type, mat, need_tangent)
         S = S3[:2, :2]

@@ -631,7 +631,9 @@ def test_kmc2d_finite_strain_dbfft_matches_oracle(gpu, model, plane_mode, stz_mo
                                                        ("landau", "plane_strain", "pure_shear"),
                                                        ("landau", "plane_stress", "pure_shear"),
                                                        ("svk", "plane_strain", "simple_shear"),
-                                                       ("neo_hookean", "plane_strain", "simple_shear")])
+                                                       ("neo_hookean", "plane_strain", "simple_shear"),
+                                                       ("neo_hookean", "plane_stress", "pure_shear"),
+                                                       ("landau_capped", "plane_stress", "pure_shear")])
 def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode, stz_mode, solver="newton_cg"):
     # KmcSimulation2D(strain_assumption="finite_strain", solver="newton_cg"):
     # every flip multiplies F_p by expm(C/N) (N = 1 first), each followed by
@@ -646,7 +648,10 @@ def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode, stz_mode, so
     nu = np.full((n, n), 0.3)
     kw = dict(mean=1.0, std=0.3, min_cutoff=0.0)
     mp = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9) \
-        if model == "landau" else {}
+        if model.startswith("landau") else {}
+    if model == "landau_capped":
+        model = "landau"
+        mp.update(strain_capping_enabled=True, strain_capping_limit=0.003)
     args = dict(softening_cap=0.916, jp=20.0, jt=5.0)
     # kT low enough that the 8 events span an elastic increment (F_xx ~ 1.004)
     ev_ref, st_ref = run_events_2d(n, n, M, 0.03, E, nu, 0.7, 400.0, 1e7, 2e-3, (0, 0), 8, seed=7, sel_seed=8,

@@ -98,7 +98,7 @@ MURN = dict(A_m=-100e9, B_m=-100e9, C_m=-100e9)
 
 @pytest.mark.parametrize("model,kw,mode", [("landau", LANDAU, "plane_strain"), ("murnaghan", MURN, "plane_strain"),
                                            ("neo_hookean", {}, "plane_strain"), ("landau", LANDAU, "plane_stress"),
-                                           ("murnaghan", MURN, "plane_stress")])
+                                           ("murnaghan", MURN, "plane_stress"), ("neo_hookean", {}, "plane_stress")])
 def test_2d_tangent_is_the_derivative_of_P(model, kw, mode):
     # in plane stress P depends on F also through E33(F): the condensed
     # tangent is that total derivative
@@ -144,6 +144,69 @@ def test_2d_plane_stress_laws_are_the_3d_law_at_zero_s33(model, kw):
     sig = f2.cauchy_2d(P, F, F33)
     sig3 = fs.cauchy_from_P(P3, F3)[:2, :2, ..., 0]
     assert np.abs(sig - sig3).max() < 1e-9 * np.abs(sig3).max()
+
+
+def test_2d_plane_stress_neo_hookean_is_the_3d_law_at_zero_s33():
+    # an independently written 3D neo-Hookean law, S = mu I + (-mu + lam ln J) C^-1
+    # with J = det F3, at F_33 = sqrt(x): S_33 = 0 and the same in-plane P
+    sh = (4, 3)
+    rng = np.random.default_rng(6)
+    E, nu = 70e9 * rng.uniform(0.8, 1.2, sh), np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    C4 = f2.build_C4_2d(E, nu, I4s, II, "plane_stress")
+    for amp in (0.03, 0.2):
+        F = I2 + amp * rng.standard_normal((2, 2) + sh)
+        P, _, F33 = f2.constitutive_2d(F, C4, I2, I4, I4rt, model_type="neo_hookean", plane_mode="plane_stress",
+                                       need_tangent=False)
+        mu = E / (2 * (1 + nu))
+        lam3 = E * nu / ((1 + nu) * (1 - 2 * nu))
+        for x in range(sh[0]):
+            for y in range(sh[1]):
+                F3 = np.zeros((3, 3))
+                F3[:2, :2] = F[:, :, x, y]
+                F3[2, 2] = F33[x, y]
+                C = F3.T @ F3
+                S3 = mu[x, y] * np.eye(3) + (-mu[x, y] + lam3[x, y] * np.log(np.linalg.det(F3))) * np.linalg.inv(C)
+                P3 = F3 @ S3
+                scale = np.abs(P[:, :, x, y]).max()
+                assert abs(S3[2, 2]) < 1e-9 * mu[x, y]
+                assert np.abs(P3[:2, :2] - P[:, :, x, y]).max() < 1e-9 * scale
+
+
+@pytest.mark.parametrize("kind", ["piecewise", "smooth"])
+def test_2d_plane_stress_capped_landau_zero_s33_and_uncapped_limit(kind):
+    # capped plane stress: S_33 of the capped 3D law at F_33 = sqrt(1 + 2 E33)
+    # vanishes to what the 1e-9 step tolerance of the linearly converging
+    # iteration leaves; with the cap out of reach the uncapped root is found
+    sh = (5, 4)
+    rng = np.random.default_rng(8)
+    E, nu = 70e9 * rng.uniform(0.8, 1.2, sh), np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    C4 = f2.build_C4_2d(E, nu, I4s, II, "plane_stress")
+    F = I2 + 0.03 * rng.standard_normal((2, 2) + sh)
+    cap = dict(LANDAU, strain_capping_enabled=True, strain_capping_limit=0.012, strain_capping_type=kind,
+               strain_capping_smooth_power=2.0)
+    P, _, F33 = f2.constitutive_2d(F, C4, I2, I4, I4rt, model_type="landau", plane_mode="plane_stress",
+                                   need_tangent=False, **cap)
+    sh3 = sh + (1,)
+    F3 = np.zeros((3, 3) + sh3)
+    F3[:2, :2, ..., 0] = F
+    F3[2, 2, ..., 0] = F33
+    i2, i4, i4rt, i4s, ii = fs.identity_tensors(*sh3)
+    C43 = fs.build_C4(E.reshape(sh3), nu.reshape(sh3), i4s, ii)
+    P3 = fs.constitutive(F3, C43, i2, i4, i4rt, model_type="landau", need_tangent=False, **cap)[0]
+    scale = np.abs(P).max()
+    assert np.abs(P3[2, 2]).max() < 1e-6 * scale
+    assert np.abs(P3[:2, :2, ..., 0] - P).max() < 1e-6 * scale
+    # the cap was active somewhere (the uncapped stress differs)
+    Pu = f2.constitutive_2d(F, C4, I2, I4, I4rt, model_type="landau", plane_mode="plane_stress",
+                            need_tangent=False, **LANDAU)[0]
+    assert np.abs(Pu - P).max() > 1e-3 * scale
+    far = dict(cap, strain_capping_limit=10.0)
+    Pf = f2.constitutive_2d(F, C4, I2, I4, I4rt, model_type="landau", plane_mode="plane_stress",
+                            need_tangent=False, **far)[0]
+    if kind == "piecewise":
+        assert np.abs(Pf - Pu).max() < 1e-9 * scale
 
 
 def test_2d_step_is_equilibrated_and_keeps_the_mean():
