@@ -340,23 +340,28 @@ int mgk_kmc_init_modes(long long nvox, int M, uint64_t seed, double mean, double
  * Finite strain (de Geus Newton-Krylov).  Fields: 9 components F[3*i+j][v].
  * ------------------------------------------------------------------------- */
 typedef struct {
-  int model;       /* 0 svk, 1 landau, 2 murnaghan, 3 neo-Hookean (2D plane
-                      strain only: the (nx, 1, ny) grid, plane = 1)        */
+  int model;       /* 0 svk, 1 landau, 2 murnaghan, 3 neo-Hookean (2D only:
+                      the (nx, 1, ny) grid, plane 1 or 2)                  */
   int capping;     /* landau strain capping on/off                          */
   int cap_smooth;  /* 1 smooth (tanh), 0 piecewise                          */
   int plane;       /* 0 3D; a 2D problem on an (nx, 1, ny) grid: 1 plane strain,
-                      2 plane stress (lam = the plane-stress lam*): svk, and
-                      uncapped landau / murnaghan at their out-of-plane E33
-                      with the condensed in-plane tangent
-                      (finite_strain_simulator.py:742-1031); the Newton stop
-                      test then uses the in-plane |F| and Cauchy
-                      J = det F * F33 (finite_strain_simulator.py:2356-2572).
-                      Plane stress with capped landau or neo-Hookean, a model
-                      outside 0..3 or a plane outside 0..2 return MGK_ERR_ARG. */
+                      2 plane stress (lam = the plane-stress lam*): svk, landau
+                      (capped or not) and murnaghan at their out-of-plane E33
+                      with the condensed in-plane tangent, neo-Hookean at its
+                      out-of-plane stretch (finite_strain_simulator.py:486-1031);
+                      the Newton stop test then uses the in-plane |F| and
+                      Cauchy J = det F * F33 (finite_strain_simulator.py:
+                      2356-2572).  A model outside 0..3 or a plane outside 0..2
+                      returns MGK_ERR_ARG. */
   double v1, v2, v3, g1, g2, g3, g4;  /* Landau constants in Pa             */
   double Am, Bm, Cm;                  /* Murnaghan constants                */
   double cap_limit;                   /* resolved capping limit E_cap       */
   double cap_power, cap_ratio;        /* smooth power, tangent ratio        */
+  const double* landau_field;         /* NULL, or per-voxel landau constants
+                                         (7 x nvox device FP64 in Pa, v1 v2 v3 g1
+                                         g2 g3 g4) overriding v1..g4, as the
+                                         reference takes arrays
+                                         (finite_strain_simulator.py:1678-1690) */
 } mgk_fs_material;
 
 typedef struct {

@@ -122,7 +122,8 @@ class MgkFsMaterial(ctypes.Structure):
     _fields_ = [("model", ctypes.c_int), ("capping", ctypes.c_int), ("cap_smooth", ctypes.c_int),
                 ("plane", ctypes.c_int)] + \
                [(k, ctypes.c_double) for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4", "Am", "Bm", "Cm",
-                                               "cap_limit", "cap_power", "cap_ratio")]
+                                               "cap_limit", "cap_power", "cap_ratio")] + \
+               [("landau_field", ctypes.c_void_p)]
 
 
 class MgkFsInfo(ctypes.Structure):

@@ -40,10 +40,23 @@ struct FsMat {
   // "some update still >= 1e-9" mask, first non-finite iteration, sticky
   // non-finite flag] of the latest E33 count (fs_ps_count); else unused
   int* psS;
+  // landau constants per voxel (7 x NV device doubles in Pa) or NULL: the scalars
+  const double* lc;
 };
 
 // plane-stress landau / murnaghan / neo-Hookean: out-of-plane solve needed
 static inline bool fs_ps(const FsMat& m) { return m.plane == 2 && m.model != 0; }
+
+// per-voxel landau constants (lc: 7 x NV, v1 v2 v3 g1 g2 g3 g4 in Pa) of voxel v
+__device__ __forceinline__ void fs_mat_at(FsMat& m, size_t v, size_t NV) {
+  m.v1 = m.lc[v];
+  m.v2 = m.lc[NV + v];
+  m.v3 = m.lc[2 * NV + v];
+  m.g1 = m.lc[3 * NV + v];
+  m.g2 = m.lc[4 * NV + v];
+  m.g3 = m.lc[5 * NV + v];
+  m.g4 = m.lc[6 * NV + v];
+}
 
 template <typename T>
 struct FsVox {
@@ -506,7 +519,7 @@ template <typename T>
 __global__ __launch_bounds__(MGK_NT) void k_fs_const(size_t NV, const T* __restrict__ Fb, const T* __restrict__ dX,
                                                      Nine dXu, double alpha, int trial, T* __restrict__ Fout,
                                                      const T* __restrict__ Fpi, const T* __restrict__ lam,
-                                                     const T* __restrict__ mu, FsMat m, T* __restrict__ P,
+                                                     const T* __restrict__ mu, FsMat m0, T* __restrict__ P,
                                                      double* __restrict__ partials, int chk_tan) {
   __shared__ double red[FS_NPART * (MGK_NT / 64)];
   double acc[FS_NPART];
@@ -514,6 +527,8 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_const(size_t NV, const T* __restr
   for (int k = 0; k < FS_NPART; ++k) acc[k] = 0.0;
   double jmin = 1e300;
   for (size_t v = blockIdx.x * (size_t)MGK_NT + threadIdx.x; v < NV; v += (size_t)gridDim.x * MGK_NT) {
+    FsMat m = m0;
+    if (m0.lc) fs_mat_at(m, v, NV);
     T F[9], fpi[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -587,9 +602,11 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_const(size_t NV, const T* __restr
 // plane-stress F33 of F_e (Fpi optional)
 template <typename T>
 __global__ void k_fs_cauchy(size_t NV, const T* __restrict__ P, const T* __restrict__ F, T* __restrict__ sig,
-                            FsMat m, const T* __restrict__ Fpi, const T* __restrict__ lam,
+                            FsMat m0, const T* __restrict__ Fpi, const T* __restrict__ lam,
                             const T* __restrict__ mu) {
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < NV; v += (size_t)gridDim.x * blockDim.x) {
+    FsMat m = m0;
+    if (m0.lc) fs_mat_at(m, v, NV);
     T f[9], p[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -614,48 +631,26 @@ __global__ void k_fs_cauchy(size_t NV, const T* __restrict__ P, const T* __restr
   }
 }
 
-// tangent action on a field (or a uniform direction dXu when dX == NULL);
-// PS = fs_ps(m), so 3D and plane-strain launches carry no plane-stress code
-template <typename T, bool PS>
-__global__ __launch_bounds__(MGK_NT) void k_fs_dP(size_t NV, const T* __restrict__ F, const T* __restrict__ Fpi,
-                                                  const T* __restrict__ lam, const T* __restrict__ mu, FsMat m,
-                                                  const T* __restrict__ dX, Nine dXu, T* __restrict__ dP,
-                                                  const int* __restrict__ skip) {
-  if (skip && *skip) return;
-  for (size_t v = blockIdx.x * (size_t)MGK_NT + threadIdx.x; v < NV; v += (size_t)gridDim.x * MGK_NT) {
-    T f[9], fpi[9], d[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      f[k] = F[(size_t)k * NV + v];
-      if (Fpi) fpi[k] = Fpi[(size_t)k * NV + v];
-      d[k] = dX ? dX[(size_t)k * NV + v] : (T)dXu.v[k];
-    }
-    FsVox<T> s;
-    fs_state<T, PS>(f, Fpi ? fpi : (const T*)nullptr, lam[v], mu[v], m, s, (T*)nullptr);
-    T o[9];
-    fs_tangent<T, PS>(s, m, d, Fpi != nullptr, o);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) dP[(size_t)k * NV + v] = o[k];
-  }
-}
-
-// The tangent action specialised on the law (MODEL, capping CAP: 0 off,
-// 1 piecewise, 2 smooth) and on F_p (FP): the runtime material switch of
-// k_fs_dP keeps every law's registers live (FP64: 256 VGPRs, one wave per
-// SIMD); with the law fixed at compile time each instantiation holds only its
-// own.  Same arithmetic as k_fs_dP.
-template <typename T, bool PS, int MODEL, int CAP, bool FP>
+// The tangent action dP = K : dX (or the uniform direction dXu when dX == NULL)
+// specialised on the law (MODEL, capping CAP: 0 off, 1 piecewise, 2 smooth), on
+// F_p (FP) and on per-voxel landau constants (LC): a runtime material switch keeps
+// every law's registers live (FP64: 256 VGPRs, one wave per SIMD); with the law
+// fixed at compile time each instantiation holds only its own.  PS = fs_ps(m), so
+// 3D and plane-strain launches carry no plane-stress code.
+template <typename T, bool PS, int MODEL, int CAP, bool FP, bool LC = false>
 __global__ __launch_bounds__(MGK_NT) void k_fs_dP_law(size_t NV, const T* __restrict__ F, const T* __restrict__ Fpi,
                                                       const T* __restrict__ lam, const T* __restrict__ mu, FsMat m0,
                                                       const T* __restrict__ dX, Nine dXu, T* __restrict__ dP,
                                                       const int* __restrict__ skip) {
   if (skip && *skip) return;
-  FsMat m = m0;
-  m.model = MODEL;
-  m.capping = CAP != 0;
-  m.cap_smooth = CAP == 2;
-  if (!PS) m.plane = m0.plane == 2 ? 1 : m0.plane;  // only the plane-stress branch reads plane == 2
+  FsMat mc = m0;
+  mc.model = MODEL;
+  mc.capping = CAP != 0;
+  mc.cap_smooth = CAP == 2;
+  if (!PS) mc.plane = m0.plane == 2 ? 1 : m0.plane;  // only the plane-stress branch reads plane == 2
   for (size_t v = blockIdx.x * (size_t)MGK_NT + threadIdx.x; v < NV; v += (size_t)gridDim.x * MGK_NT) {
+    FsMat m = mc;
+    if (LC) fs_mat_at(m, v, NV);
     T f[9], fpi[9], d[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -678,12 +673,19 @@ void launch_dP(int nblk, hipStream_t st, size_t NV, const T* F, const T* Fpi, co
                const FsMat& m, const T* dX, const Nine& dXu, T* dP, const int* skip) {
   const bool ps = fs_ps(m);
   const int cap = m.model == 1 && m.capping ? (m.cap_smooth ? 2 : 1) : 0;
-#define MGK_DP(PSV, MD, CP)                                                                                    \
-  do {                                                                                                         \
-    if (Fpi)                                                                                                   \
-      k_fs_dP_law<T, PSV, MD, CP, true><<<nblk, MGK_NT, 0, st>>>(NV, F, Fpi, lam, mu, m, dX, dXu, dP, skip);   \
-    else                                                                                                       \
-      k_fs_dP_law<T, PSV, MD, CP, false><<<nblk, MGK_NT, 0, st>>>(NV, F, Fpi, lam, mu, m, dX, dXu, dP, skip);  \
+#define MGK_DP(PSV, MD, CP)                                                                                      \
+  do {                                                                                                           \
+    if (MD == 1 && m.lc) {                                                                                       \
+      if (Fpi)                                                                                                   \
+        k_fs_dP_law<T, PSV, MD, CP, true, true><<<nblk, MGK_NT, 0, st>>>(NV, F, Fpi, lam, mu, m, dX, dXu, dP, skip); \
+      else                                                                                                       \
+        k_fs_dP_law<T, PSV, MD, CP, false, true><<<nblk, MGK_NT, 0, st>>>(NV, F, Fpi, lam, mu, m, dX, dXu, dP,      \
+                                                                          skip);                                 \
+    } else if (Fpi) {                                                                                            \
+      k_fs_dP_law<T, PSV, MD, CP, true><<<nblk, MGK_NT, 0, st>>>(NV, F, Fpi, lam, mu, m, dX, dXu, dP, skip);     \
+    } else {                                                                                                     \
+      k_fs_dP_law<T, PSV, MD, CP, false><<<nblk, MGK_NT, 0, st>>>(NV, F, Fpi, lam, mu, m, dX, dXu, dP, skip);    \
+    }                                                                                                            \
   } while (0)
   if (ps) {
     if (m.model == 1) {
@@ -1164,7 +1166,7 @@ __global__ __launch_bounds__(MGK_NT) void k_db_pencil(FsGeom<T> g, DbPar d, int 
 // K4_abkl = N(e_kl)_ba under its (K4 : dF^T)^T matvec
 template <typename T>
 __global__ __launch_bounds__(MGK_NT) void k_fs_kmean(size_t NV, const T* __restrict__ F, const T* __restrict__ Fpi,
-                                                     const T* __restrict__ lam, const T* __restrict__ mu, FsMat m,
+                                                     const T* __restrict__ lam, const T* __restrict__ mu, FsMat m0,
                                                      double* __restrict__ partials) {
   __shared__ double red[9 * (MGK_NT / 64)];
   const int kl = blockIdx.y;
@@ -1172,6 +1174,8 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_kmean(size_t NV, const T* __restr
 #pragma unroll
   for (int k = 0; k < 9; ++k) acc[k] = 0.0;
   for (size_t v = blockIdx.x * (size_t)MGK_NT + threadIdx.x; v < NV; v += (size_t)gridDim.x * MGK_NT) {
+    FsMat m = m0;
+    if (m0.lc) fs_mat_at(m, v, NV);
     T f[9], fpi[9], d[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -1492,10 +1496,12 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_ps_count(size_t NV, const T* __re
                                                         const T* __restrict__ dX, Nine dXu, double alpha, int trial,
                                                         const T* __restrict__ Fp, int fp_inverse,
                                                         const T* __restrict__ lam, const T* __restrict__ mu,
-                                                        FsMat m) {
+                                                        FsMat m0) {
   unsigned mask = 0;
   int nf = 20;
   for (size_t v = blockIdx.x * (size_t)MGK_NT + threadIdx.x; v < NV; v += (size_t)gridDim.x * MGK_NT) {
+    FsMat m = m0;
+    if (m0.lc) fs_mat_at(m, v, NV);
     T F[9], Fe[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -1541,8 +1547,8 @@ __global__ __launch_bounds__(MGK_NT) void k_fs_ps_count(size_t NV, const T* __re
     nf = min(nf, __shfl_xor(nf, off, 64));
   }
   if ((threadIdx.x & 63) == 0) {
-    atomicOr(m.psS + 1, (int)mask);
-    atomicMin(m.psS + 2, nf);
+    atomicOr(m0.psS + 1, (int)mask);
+    atomicMin(m0.psS + 2, nf);
   }
 }
 
@@ -1711,6 +1717,7 @@ int fs_ctx_init(FsCtx<T>& c, const int64_t* shape, const T* lam, const T* mu, co
   m.Ecap = mt->cap_limit;
   m.cap_p = mt->cap_power;
   m.cap_ratio = mt->cap_ratio;
+  m.lc = mt->model == 1 ? mt->landau_field : nullptr;
   FsGeom<T>& g = c.g;
   g.nx = pl.nx;
   g.P = pl.P;
@@ -2466,14 +2473,17 @@ static FsMat to_fsmat(const mgk_fs_material* mt) {
   m.cap_p = mt->cap_power;
   m.cap_ratio = mt->cap_ratio;
   m.psS = nullptr;
+  m.lc = mt->model == 1 ? mt->landau_field : nullptr;
   return m;
 }
 
 template <typename T>
 static __global__ void k_fs_pointwise(size_t NV, const T* __restrict__ F, const T* __restrict__ Fp,
-                                      const T* __restrict__ lam, const T* __restrict__ mu, FsMat m, T* __restrict__ P,
+                                      const T* __restrict__ lam, const T* __restrict__ mu, FsMat m0, T* __restrict__ P,
                                       const T* __restrict__ dF, T* __restrict__ dP, T* __restrict__ F33) {
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < NV; v += (size_t)gridDim.x * blockDim.x) {
+    FsMat m = m0;
+    if (m0.lc) fs_mat_at(m, v, NV);
     T f[9], fpi[9], d[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) f[k] = F[(size_t)k * NV + v];

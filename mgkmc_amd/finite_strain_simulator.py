@@ -53,8 +53,21 @@ def material(model_type="svk", A_m=0.0, B_m=0.0, C_m=0.0, v1=0.0, v2=0.0, v3=0.0
     if model_type == "neo_hookean" and ndim == 3:
         m.model = 0
     if model_type == "landau":
-        m.v1, m.v2, m.v3 = landau_pa(v1, "v1"), landau_pa(v2, "v2"), landau_pa(v3, "v3")
-        m.g1, m.g2, m.g3, m.g4 = landau_pa(g1, "g1"), landau_pa(g2, "g2"), landau_pa(g3, "g3"), landau_pa(g4, "g4")
+        consts = [np.asarray(x, dtype=np.float64) for x in (v1, v2, v3, g1, g2, g3, g4)]
+        if any(a.ndim > 0 for a in consts):
+            # per-voxel constants (finite_strain_simulator.py:1678-1690): each element
+            # converted from GPa as the reference converts them, kept on the device
+            shape = np.broadcast_shapes(*[a.shape for a in consts])
+            field = np.stack([np.broadcast_to(a, shape) for a in consts]).reshape(7, -1).copy()
+            small = (np.abs(field) < 1e6) & (field != 0.0)
+            field[small] *= 1e9
+            m._landau_dev = dev.upload(field, 0)
+            m.landau_field = m._landau_dev.data_ptr()
+            m.v1, m.v2, m.v3, m.g1, m.g2, m.g3 = (float(field[k].mean()) for k in range(6))
+            m.g4 = float(field[6].mean())  # the capping limit reads <g4> (:1700-1703)
+        else:
+            m.v1, m.v2, m.v3 = landau_pa(v1, "v1"), landau_pa(v2, "v2"), landau_pa(v3, "v3")
+            m.g1, m.g2, m.g3, m.g4 = landau_pa(g1, "g1"), landau_pa(g2, "g2"), landau_pa(g3, "g3"), landau_pa(g4, "g4")
         m.capping = 1 if strain_capping_enabled else 0
         if strain_capping_enabled:
             if strain_capping_limit is None or strain_capping_limit <= 0:

@@ -124,6 +124,17 @@ def _i4s3(sh):
     return np.broadcast_to(t[..., None, None], (3, 3, 3, 3) + sh)
 
 
+def _landau_arrays(mat, sh):
+    """The seven landau constants as grid arrays, scalars broadcast, each element
+    converted from GPa (finite_strain_simulator.py:723-733)."""
+    out = []
+    for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4"):
+        x = mat.get(k, 0.0)
+        a = np.full(sh, float(x)) if np.isscalar(x) else np.array(x, dtype=float).reshape(sh)
+        out.append(landau_constants(a))
+    return out
+
+
 def _law3d(E_GL, lam, mu, model_type, mat, need_tangent):
     """S (3D) and dS/dE (3D einsum form) of the landau / murnaghan laws at a
     plane-strain E_GL (finite_strain_simulator.py:700-964)."""
@@ -146,8 +157,7 @@ def _law3d(E_GL, lam, mu, model_type, mat, need_tangent):
                                                      + np.einsum("ijxy,klxy->ijklxy", E_GL, I3))
               + (2.0 * (mu + B * trE))[None, None, None, None] * _i4s3(sh) + C[None, None, None, None] * KE)
         return S, C4, None
-    v1, v2, v3, g1, g2, g3, g4 = (np.full(sh, c) for c in landau_constants(
-        [mat.get(k, 0.0) for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4")]))
+    v1, v2, v3, g1, g2, g3, g4 = _landau_arrays(mat, sh)
     cap = mat.get("strain_capping_enabled", False)
     if cap:
         lim = mat.get("strain_capping_limit")
@@ -224,8 +234,7 @@ def _plane_stress_e33(E2d, lam, mu, model_type, mat):
         sg[sg == 0] = 1.0
         e[~sm] = (-b[~sm] + sg * np.sqrt(disc)) / (2.0 * a[~sm])
         return e
-    v1, v2, v3, g1, g2, g3, g4 = (np.full(sh, c) for c in landau_constants(
-        [mat.get(k, 0.0) for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4")]))
+    v1, v2, v3, g1, g2, g3, g4 = _landau_arrays(mat, sh)
     r = np.einsum("ijxy,jkxy,kixy->xy", E2d, E2d, E2d)
     e = -(lam / (lam + 2.0 * mu)) * t
     for _ in range(20):
@@ -253,8 +262,7 @@ def _plane_stress_e33_capped(E2d, lam, mu, mat):
     max |update| < 1e-9."""
     E11, E22, E12 = E2d[0, 0], E2d[1, 1], E2d[0, 1]
     sh = lam.shape
-    v1, v2, v3, g1, g2, g3, g4 = (np.full(sh, c) for c in landau_constants(
-        [mat.get(k, 0.0) for k in ("v1", "v2", "v3", "g1", "g2", "g3", "g4")]))
+    v1, v2, v3, g1, g2, g3, g4 = _landau_arrays(mat, sh)
     lim = mat.get("strain_capping_limit")
     g4m, mum = g4.mean(), mu.mean()
     Ecap = lim if (lim is not None and lim > 0) else (np.sqrt(0.2 * mum / abs(g4m)) if g4m < 0 else 999.0)

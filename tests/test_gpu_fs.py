@@ -354,3 +354,35 @@ def test_simulation_3d_parity(gpu, model, solver, kw, sh, drv, targets, tols):
         np.testing.assert_allclose(got[0][-1, 0, 0], 1.01, atol=1e-5)
         np.testing.assert_allclose(got[1][-1, 1, 1], 0.0, atol=2e5)
         np.testing.assert_allclose(got[1][-1, 2, 2], 0.0, atol=2e5)
+
+
+def test_per_voxel_landau_constants_parity(gpu):
+    # landau constants as grid arrays (finite_strain_simulator.py:1678-1690; one
+    # field in GPa, converted per element): the pointwise law and the Newton-CG
+    # step with identical Newton / Krylov counts, plain and capped (the capping
+    # limit from the mean of g4)
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_3d, constitutive_hyperelastic_3d
+    sh = (8, 6, 10)
+    rng, E, nu, I2, I4, I4rt, C4, F0 = _grid(sh, 21)
+    per = {k: v * np.random.default_rng(22).uniform(0.9, 1.1, sh) for k, v in LANDAU.items()}
+    per["g2"] = per["g2"] / 1e9
+    G = fs.build_ghat4(*sh, *sh)
+    Fbar = np.eye(3)
+    Fbar[0, 1] = 0.02
+    for cap in ({}, dict(strain_capping_enabled=True, strain_capping_type="smooth")):
+        F = I2 + 0.02 * rng.standard_normal((3, 3) + sh)
+        P0, K0 = fs.constitutive(F, C4, I2, I4, I4rt, model_type="landau", **per, **cap)
+        P1, K1 = constitutive_hyperelastic_3d(F, C4, I2, I4, I4rt, model_type="landau", **per, **cap)
+        assert _rel(P1, P0) < 1e-12 and _rel(K1, K0) < 1e-10
+        st = {}
+        ref = fs.solver_step(I2.copy(), Fbar, G, C4, I2, I4, I4rt, driving_component=(0, 1), E_avg=E.mean(),
+                             nu_avg=nu.mean(), tol_NW=1e-8, tol_CG=1e-8, model_type="landau", stats=st, **per, **cap)
+        got = finite_strain_solver_step_3d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, driving_component=(0, 1),
+                                           E_avg=E.mean(), nu_avg=nu.mean(), tol_NW=1e-8, tol_CG=1e-8,
+                                           model_type="landau", solver="newton_cg", **per, **cap)
+        inf = _last_info()
+        nw, kr = _oracle_counts(st)
+        print(f"[fs per-voxel] cap={bool(cap)} oracle {nw}/{kr} device {inf.newton_iters}/{inf.krylov_iters}",
+              flush=True)
+        assert inf.newton_iters == nw and inf.krylov_iters == kr
+        assert _rel(got[0], ref[0]) < 1e-10 and _rel(got[1], ref[1]) < 1e-10 and _rel(got[2], ref[2]) < 1e-10

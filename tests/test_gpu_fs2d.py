@@ -256,3 +256,33 @@ def test_2d_reference_landau_plane_stress_al_case(gpu):
         assert np.abs(a - b).max() <= 1e-10 * max(1.0, np.abs(b).max())
     np.testing.assert_allclose(got[0][-1, 0, 0], 1.01, atol=1e-5)
     np.testing.assert_allclose(got[1][-1, 1, 1], 0.0, atol=2e5)
+
+
+def test_2d_per_voxel_landau_constants_parity(gpu):
+    # the 2D law with landau constants as grid arrays: pointwise (P, K4, F33) and
+    # the plane-stress Newton-CG step, uncapped and capped
+    from mgkmc_amd.finite_strain_simulator import finite_strain_solver_step_2d, constitutive_hyperelastic_2d
+    sh = (12, 10)
+    rng = np.random.default_rng(23)
+    E = 70e9 * rng.uniform(0.85, 1.15, sh)
+    nu = np.full(sh, 0.3)
+    per = {k: v * rng.uniform(0.9, 1.1, sh) for k, v in LANDAU.items()}
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(*sh)
+    G = f2.build_ghat4_2d(*sh, *sh)
+    Fbar = np.eye(2)
+    Fbar[0, 1] = 0.02
+    Fbar[1, 1] = 0.996
+    for mode, cap in (("plane_strain", {}), ("plane_stress", {}),
+                      ("plane_stress", dict(strain_capping_enabled=True, strain_capping_limit=0.012))):
+        C4 = f2.build_C4_2d(E, nu, I4s, II, mode)
+        F = I2 + 0.03 * rng.standard_normal((2, 2) + sh)
+        P, K4, F33 = f2.constitutive_2d(F, C4, I2, I4, I4rt, model_type="landau", plane_mode=mode, **per, **cap)
+        Pg, K4g, F33g = constitutive_hyperelastic_2d(F, C4, I2, I4, I4rt, model_type="landau", plane_mode=mode,
+                                                     **per, **cap)
+        assert _rel(Pg, P) < 1e-12 and _rel(K4g, K4) < 1e-10 and _rel(F33g, F33) < 1e-13
+        args = dict(model_type="landau", plane_mode=mode, tol_NW=1e-8, tol_CG=1e-8, **per, **cap)
+        ref = f2.solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, **args)
+        got = finite_strain_solver_step_2d(I2.copy(), Fbar, G, C4, I2, I4, I4rt, solver="newton_cg", **args)
+        eF, eP, eS = _rel(got[0], ref[0]), _rel(got[1], ref[1]), _rel(got[2], ref[2])
+        print(f"[fs2d per-voxel] {mode} cap={bool(cap)} relF={eF:.2e} relP={eP:.2e} relSig={eS:.2e}", flush=True)
+        assert eF < 1e-12 and eP < 1e-10 and eS < 1e-10

@@ -346,3 +346,27 @@ def test_simulation_driver_meets_the_reference_tests_assertions():
     np.testing.assert_allclose(Fm[:, 0, 0], 1.0 + np.linspace(0.0, 0.01, 6), atol=1e-12)
     assert np.all(np.diff(Sm[:, 0, 0]) > 0)
     assert np.all(Fm[1:, 1, 1] < 1.0) and np.allclose(Fm[:, 1, 1], Fm[:, 2, 2])
+
+
+def test_per_voxel_landau_constants_are_pointwise():
+    # constants given as grid arrays (finite_strain_simulator.py:1678-1690): every
+    # voxel's P and tangent equal the law evaluated with that voxel's scalars
+    # (capping limit given, so no grid mean enters), GPa fields converted per element
+    sh = (3, 2, 2)
+    rng = np.random.default_rng(31)
+    E, nu = 70e9 * rng.uniform(0.9, 1.1, sh), np.full(sh, 0.3)
+    I2, I4, I4rt, I4s, II = fs.identity_tensors(*sh)
+    C4 = fs.build_C4(E, nu, I4s, II)
+    F = I2 + 0.03 * rng.standard_normal((3, 3) + sh)
+    per = {k: v * rng.uniform(0.8, 1.2, sh) for k, v in LANDAU.items()}
+    per["g3"] = per["g3"] / 1e9
+    cap = dict(strain_capping_enabled=True, strain_capping_limit=0.02, strain_capping_type="smooth")
+    P, K = fs.constitutive(F, C4, I2, I4, I4rt, model_type="landau", **per, **cap)
+    one = fs.identity_tensors(1, 1, 1)
+    for idx in np.ndindex(*sh):
+        sc = {k: float(v[idx]) for k, v in per.items()}
+        C41 = C4[(Ellipsis,) + idx].reshape(C4.shape[:4] + (1, 1, 1))
+        F1 = F[(Ellipsis,) + idx].reshape(3, 3, 1, 1, 1)
+        P1, K1 = fs.constitutive(F1, C41, one[0], one[1], one[2], model_type="landau", **sc, **cap)
+        assert np.allclose(P1[..., 0, 0, 0], P[(Ellipsis,) + idx], rtol=1e-13, atol=0)
+        assert np.allclose(K1[..., 0, 0, 0], K[(Ellipsis,) + idx], rtol=1e-12, atol=1e-3)
