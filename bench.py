@@ -433,9 +433,10 @@ def main():
     rmx, r64, r32 = res[2], res[0], res[1]
     prof = dict(r32["prof"] or {})
     log("KMC legs")
-    kmcmx = gpu_kmc(a.n, a.kmc_events, 2, "mixed")
-    kmc32 = gpu_kmc(a.n, a.kmc_events, 2, "fp32")
-    kmc64 = gpu_kmc(a.n, a.kmc_events, 2, "fp64")
+    none = {"kmc_steps_per_s": None, "ls_iters_per_event": None, "prof": {}}
+    kmcmx = gpu_kmc(a.n, a.kmc_events, 2, "mixed") if a.kmc_events > 0 else none
+    kmc32 = gpu_kmc(a.n, a.kmc_events, 2, "fp32") if a.kmc_events > 0 else none
+    kmc64 = gpu_kmc(a.n, a.kmc_events, 2, "fp64") if a.kmc_events > 0 else none
     # the roofline names the dominant kernel of the value's workload (the FP32
     # sweeps of the refined LS); the KMC kernels are listed beside it
     dom = max(prof.items(), key=lambda kv: kv[1][0]) if prof else None

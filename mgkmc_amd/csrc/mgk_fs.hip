@@ -796,6 +796,89 @@ __global__ __launch_bounds__(NTH) void k_fs_xr2c(FsGeom<T> g, const T* __restric
                         cb >= 0 ? spec + (size_t)cb * cs : (cplx<T>*)nullptr);
 }
 
+// Tangent action fused into the x r2c sweep of the projection: one workgroup
+// owns an x-row tile (NN x BB voxels of one plane) for all nine components.
+// Each voxel's dP = K : dX is evaluated in registers from F (and F_p^-1) as
+// k_fs_dP_law does; components 0..5 go straight into three packed complex LDS
+// tiles (pairs (0,1) (2,3) (4,5)) and 6..8 stay in registers while those three
+// are transformed and stored, then fill two tiles ((6,7), (8,0)) for the second
+// batch.  dP never reaches HBM: per voxel the sweep reads F and dX (and F_p^-1,
+// lam, mu) and writes the half spectra, saving the 18 words of the dP round trip.
+// Three tiles keep the LDS at ~48 KB, so three workgroups share a CU.
+template <typename T, bool PS, int MODEL, int CAP, bool FP, int NN, int BB, int NTH>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void k_fs_dP_xr2c(FsGeom<T> g, const T* __restrict__ F,
+                                                     const T* __restrict__ Fpi, const T* __restrict__ lam,
+                                                     const T* __restrict__ mu, FsMat m0, const T* __restrict__ dX,
+                                                     cplx<T>* __restrict__ spec, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* tile = tw + NN;
+  constexpr int NPT = NN * BB, VPT = (NPT + NTH - 1) / NTH, LOGB = ct_log2(BB);
+  FsMat m = m0;
+  m.model = MODEL;
+  m.capping = CAP != 0;
+  m.cap_smooth = CAP == 2;
+  if (!PS) m.plane = m0.plane == 2 ? 1 : m0.plane;
+  const int P = g.P, Q = g.Q;
+  const int tid_lin = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int q0 = (tid_lin % gridDim.x) * BB, p = tid_lin / gridDim.x;
+  const size_t NV = (size_t)g.NV;
+  const size_t cs = (size_t)g.nxh * P * Q;
+  for (int t = threadIdx.x; t < NN; t += NTH) tw[t] = g.twx[t];
+  T keep[VPT][3];
+#pragma unroll
+  for (int j = 0; j < VPT; ++j) {
+    const int e = threadIdx.x + j * NTH;
+    T o[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o[k] = T(0);
+    if (NPT % NTH == 0 || e < NPT) {
+      const int b = e & (BB - 1), x = e >> LOGB, q = q0 + b;
+      if (q < Q) {
+        const size_t v = ((size_t)x * P + p) * Q + q;
+        T f[9], fpi[9], d[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          f[k] = F[(size_t)k * NV + v];
+          if (FP) fpi[k] = Fpi[(size_t)k * NV + v];
+          d[k] = dX[(size_t)k * NV + v];
+        }
+        FsVox<T> st;
+        fs_state<T, PS>(f, FP ? fpi : (const T*)nullptr, lam[v], mu[v], m, st, (T*)nullptr);
+        fs_tangent<T, PS>(st, m, d, FP, o);
+      }
+      tile[x * BB + b] = cmk(o[0], o[1]);
+      tile[NPT + x * BB + b] = cmk(o[2], o[3]);
+      tile[2 * NPT + x * BB + b] = cmk(o[4], o[5]);
+    }
+    keep[j][0] = o[6];
+    keep[j][1] = o[7];
+    keep[j][2] = o[8];
+  }
+  __syncthreads();
+  const bool vec = q0 + BB <= Q && Q % vw16<T>() == 0;
+  tile_fft_ct<NN, BB, 3, false, NTH>(tile, tw, false);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    store_half<T, NN, BB, NTH>(tile + c * NPT, vec, BB, NN, g.nxh, LOGB, q0, Q, P, p, spec + (size_t)(2 * c) * cs,
+                               spec + (size_t)(2 * c + 1) * cs);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < VPT; ++j) {
+    const int e = threadIdx.x + j * NTH;
+    if (NPT % NTH == 0 || e < NPT) {
+      tile[e] = cmk(keep[j][0], keep[j][1]);
+      tile[NPT + e] = cmk(keep[j][2], T(0));
+    }
+  }
+  __syncthreads();
+  tile_fft_ct<NN, BB, 2, false, NTH>(tile, tw, false);
+  store_half<T, NN, BB, NTH>(tile, vec, BB, NN, g.nxh, LOGB, q0, Q, P, p, spec + (size_t)6 * cs, spec + (size_t)7 * cs);
+  store_half<T, NN, BB, NTH>(tile + NPT, vec, BB, NN, g.nxh, LOGB, q0, Q, P, p, spec + (size_t)8 * cs,
+                             (cplx<T>*)nullptr);
+}
+
 // c2r along x into real comps; optional fused dots: part[2*blk] += out.a,
 // part[2*blk+1] += out.out (FP64), blk over (x-tile, plane, pair)
 template <typename T, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
@@ -1404,6 +1487,9 @@ void fs_launch_mid(const FsPlan& pl, hipStream_t s, const FftPlan& ply, const cp
 }
 
 template <typename T>
+void fs_project_rest(FsCtx<T>& c, T* out, T scale, const T* dota, int selfdot, double* partials, const int* skip);
+
+template <typename T>
 void fs_project(FsCtx<T>& c, const T* in, T* out, T scale, const T* dota, int selfdot, double* partials,
                 const int* skip) {
   constexpr int MV = fs_maxv<T>();
@@ -1424,6 +1510,16 @@ void fs_project(FsCtx<T>& c, const T* in, T* out, T scale, const T* dota, int se
     }
   }
   c.prof->end(h);
+  fs_project_rest(c, out, scale, dota, selfdot, partials, skip);
+}
+
+// the projection after its x r2c sweep: y FFT, z pencils with G, y inverse, x c2r
+template <typename T>
+void fs_project_rest(FsCtx<T>& c, T* out, T scale, const T* dota, int selfdot, double* partials, const int* skip) {
+  constexpr int MV = fs_maxv<T>();
+  constexpr int TT = ct_tile<T>();
+  const FsPlan& pl = c.pl;
+  int h;
   if (pl.P > 1) {
     h = c.prof->begin(KID_FS_MID, c.kry_it, 2.0 * pl.spec);
     fs_launch_mid<T, MV>(pl, c.s, c.g.ply, c.g.twy, c.spec, 0, skip);
@@ -1576,11 +1672,75 @@ void fs_ps_count(const FsMat& m, hipStream_t s, size_t NV, const T* Fb, const T*
   k_fs_ps_fin<<<1, 1, 0, s>>>(m.psS);
 }
 
+// columns per tile of the fused tangent + x r2c sweep: three packed tiles of
+// NN x BB complex values in ~48 KB of LDS, so three workgroups share a CU
+template <typename T, int NN>
+constexpr int fused_bb() {
+  return (sizeof(T) == 8 ? 1024 : 2048) / NN;
+}
+
+// whether the fused sweep has an instance: x length 128, 256 or 512, 3D or
+// plane strain, svk / landau / murnaghan with scalar constants
+template <typename T>
+bool fused_dP_ok(const FsCtx<T>& c) {
+  const FsMat& m = c.mat;
+  const int nx = c.pl.nx;
+  return (nx == 128 || nx == 256 || nx == 512) && !fs_ps(m) && !m.lc && m.model != 3 &&
+         c.pl.Q >= (sizeof(T) == 8 ? 1024 : 2048) / nx;
+}
+
+template <typename T, int NN>
+void launch_dP_xr2c_n(FsCtx<T>& c, const T* Fcur, const T* in, const int* skip) {
+  const FsPlan& pl = c.pl;
+  const FsMat& m = c.mat;
+  constexpr int BB = fused_bb<T, NN>(), NTH = sizeof(T) == 8 ? 256 : 512;
+  const dim3 gx((pl.Q + BB - 1) / BB, pl.P);
+  const size_t lds = (size_t)(NN + 3 * NN * BB) * sizeof(cplx<T>);
+  const int cap = m.model == 1 && m.capping ? (m.cap_smooth ? 2 : 1) : 0;
+#define MGK_FX(MD, CP)                                                                                          \
+  do {                                                                                                          \
+    if (c.Fpi)                                                                                                  \
+      k_fs_dP_xr2c<T, false, MD, CP, true, NN, BB, NTH><<<gx, NTH, lds, c.s>>>(c.g, Fcur, c.Fpi, c.lam, c.mu, m, \
+                                                                               in, c.spec, skip);               \
+    else                                                                                                        \
+      k_fs_dP_xr2c<T, false, MD, CP, false, NN, BB, NTH><<<gx, NTH, lds, c.s>>>(c.g, Fcur, c.Fpi, c.lam, c.mu,   \
+                                                                                m, in, c.spec, skip);           \
+  } while (0)
+  switch (m.model) {
+    case 0: MGK_FX(0, 0); break;
+    case 1:
+      if (cap == 0) MGK_FX(1, 0);
+      else if (cap == 1) MGK_FX(1, 1);
+      else MGK_FX(1, 2);
+      break;
+    default: MGK_FX(2, 0); break;
+  }
+#undef MGK_FX
+}
+
+template <typename T>
+void launch_dP_xr2c(FsCtx<T>& c, const T* Fcur, const T* in, const int* skip) {
+  switch (c.pl.nx) {
+    case 128: launch_dP_xr2c_n<T, 128>(c, Fcur, in, skip); break;
+    case 256: launch_dP_xr2c_n<T, 256>(c, Fcur, in, skip); break;
+    default: launch_dP_xr2c_n<T, 512>(c, Fcur, in, skip); break;
+  }
+}
+
 // A x = G((K : x^T)^T) (matrix-free); F = current deformation (Fcur)
 template <typename T>
 void fs_matvec(FsCtx<T>& c, const T* Fcur, const T* in, T* out, const T* dota, int selfdot, double* partials,
                const int* skip) {
   const FsPlan& pl = c.pl;
+  if (fused_dP_ok(c)) {
+    // algorithmic bytes: F, dX, F_p^-1, lam, mu read, the 9-component half spectrum written
+    const double bytes = (double)pl.NV * sizeof(T) * (18 + 2 + (c.Fpi ? 9 : 0)) + pl.spec;
+    const int h = c.prof->begin(KID_FS_CONST, c.kry_it, bytes);
+    launch_dP_xr2c(c, Fcur, in, skip);
+    c.prof->end(h);
+    fs_project_rest(c, out, T(1.0 / (double)pl.NV), dota, selfdot, partials, skip);
+    return;
+  }
   Nine z{};
   const int h = c.prof->begin(KID_FS_CONST, c.kry_it, (double)pl.NV * sizeof(T) * (27 + 2 + (c.Fpi ? 9 : 0)));
   launch_dP<T>(fs_nblk<T>(pl.NV), c.s, (size_t)pl.NV, Fcur, c.Fpi, c.lam, c.mu, c.mat, in, z, c.dP, skip);

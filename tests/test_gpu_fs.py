@@ -83,7 +83,9 @@ def _step_pair(sh, seed, model, Fbar, Fp=None, P_target=None, P_mask=None, drv=(
 
 
 @pytest.mark.parametrize("sh,model", [((6, 4, 5), "svk"), ((8, 8, 8), "landau"), ((16, 16, 16), "landau"),
-                                      ((12, 6, 10), "svk"), ((64, 8, 64), "landau"), ((8, 64, 64), "svk")])
+                                      ((12, 6, 10), "svk"), ((64, 8, 64), "landau"), ((8, 64, 64), "svk"),
+                                      # x lengths of the fused tangent + x r2c sweep
+                                      ((256, 4, 8), "landau"), ((128, 6, 16), "svk"), ((512, 2, 4), "landau")])
 def test_newton_step_parity(gpu, sh, model):
     Fbar = np.eye(3)
     Fbar[0, 1] = 0.02
@@ -118,8 +120,25 @@ def test_newton_step_mixed_bc_parity(gpu):
     assert _rel(got[0], ref[0]) < 1e-10 and _rel(got[1], ref[1]) < 1e-10 and _rel(got[2], ref[2]) < 1e-10
 
 
-def test_newton_step_with_fp(gpu):
-    sh = (6, 6, 6)
+@pytest.mark.parametrize("sh,model,kw", [
+    ((128, 4, 8), "murnaghan", dict(A_m=-100.0e9, B_m=-100.0e9, C_m=-100.0e9)),
+    ((256, 2, 8), "landau", dict(strain_capping_enabled=True, strain_capping_limit=0.013)),
+    ((512, 2, 4), "landau", dict(strain_capping_enabled=True, strain_capping_type="smooth"))])
+def test_newton_step_fused_sweep_laws(gpu, sh, model, kw):
+    # every law instance of the fused tangent + x r2c sweep (x lengths 128-512)
+    Fbar = np.eye(3)
+    Fbar[0, 1] = 0.02
+    ref, got, st = _step_pair(sh, 7, model, Fbar, kw=kw)
+    inf = _last_info()
+    nw, kr = _oracle_counts(st)
+    print(f"[fs fused] {sh} {model}: oracle {nw}/{kr} device {inf.newton_iters}/{inf.krylov_iters}; "
+          f"relF={_rel(got[0], ref[0]):.2e} relP={_rel(got[1], ref[1]):.2e}", flush=True)
+    assert inf.newton_iters == nw and inf.krylov_iters == kr
+    assert _rel(got[0], ref[0]) < 1e-10 and _rel(got[1], ref[1]) < 1e-10 and _rel(got[2], ref[2]) < 1e-10
+
+
+@pytest.mark.parametrize("sh", [(6, 6, 6), (128, 4, 16)])
+def test_newton_step_with_fp(gpu, sh):
     rng = np.random.default_rng(8)
     I2 = fs.identity_tensors(*sh)[0]
     Fp = I2 + 0.01 * rng.standard_normal((3, 3) + sh)
@@ -165,7 +184,8 @@ def test_full_size_properties(gpu):
 
 
 @pytest.mark.parametrize("sh,model,cap", [((16, 16, 16), "landau", False), ((64, 8, 64), "landau", True),
-                                          ((12, 6, 10), "svk", False)])
+                                          ((12, 6, 10), "svk", False), ((256, 8, 8), "landau", True),
+                                          ((128, 4, 32), "svk", False)])
 def test_newton_step_refined_form(gpu, sh, model, cap):
     # FP64 Newton + FP32 Krylov (rtol 1e-6) vs the plain FP64 path at 1e-8:
     # the Newton stop test holds and the fields agree to the stated bounds
