@@ -286,3 +286,24 @@ def test_2d_per_voxel_landau_constants_parity(gpu):
         eF, eP, eS = _rel(got[0], ref[0]), _rel(got[1], ref[1]), _rel(got[2], ref[2])
         print(f"[fs2d per-voxel] {mode} cap={bool(cap)} relF={eF:.2e} relP={eP:.2e} relSig={eS:.2e}", flush=True)
         assert eF < 1e-12 and eP < 1e-10 and eS < 1e-10
+
+
+@pytest.mark.parametrize("kind", ["smooth", "piecewise"])
+@pytest.mark.parametrize("amp", [0.05, 0.15])
+def test_2d_capped_plane_stress_reference_case_on_device(gpu, kind, amp):
+    # the reference's own capped plane-stress check (tests/test_finite_strain_capped_plane_stress.py:
+    # 63-91) on the device law: S33 of the capped law at the returned E33 below 1e3 Pa, and
+    # P, K4, F33 equal to the oracle's
+    from mgkmc_amd.finite_strain_simulator import constitutive_hyperelastic_2d
+    from test_oracle_fs2d import REF_CAP_LANDAU, capped_s33, reference_capped_case
+    F, C4, t, cap = reference_capped_case(kind, amp)
+    P, K4, F33 = constitutive_hyperelastic_2d(F, C4, *t, model_type="landau", plane_mode="plane_stress",
+                                              **REF_CAP_LANDAU, **cap)
+    E33 = 0.5 * (F33 ** 2 - 1.0)
+    E2d = 0.5 * (np.einsum("jixy,jkxy->ikxy", F, F) - t[0])
+    s33 = np.abs(capped_s33(E2d, E33, cap, kind)).max()
+    print(f"[fs2d ref capped] {kind} amp={amp}: max|S33| = {s33:.3e} Pa", flush=True)
+    assert s33 < 1e3
+    Pr, K4r, F33r = f2.constitutive_2d(F, C4, *t, model_type="landau", plane_mode="plane_stress",
+                                       **REF_CAP_LANDAU, **cap)
+    assert _rel(P, Pr) < 1e-12 and _rel(F33, F33r) < 1e-13 and _rel(K4, K4r) < 1e-10

@@ -338,3 +338,69 @@ def test_2d_plane_stress_landau_simulation_is_the_3d_uniaxial_one():
                               tol_macro=1e5, **L)
     assert abs(Sm[-1, 0, 0] - S3[-1, 0, 0]) < 1e-4 * S3[-1, 0, 0]
     assert abs(Fm[-1, 1, 1] - F3[-1, 1, 1]) < 1e-5
+
+
+# the reference's capped plane-stress case (tests/test_finite_strain_capped_plane_stress.py:19-116):
+# its Landau constants, moduli and production capping; S33 of the capped law at the
+# returned E33 must stay below 1e3 Pa (1e-9 on E33 times O(100 GPa))
+REF_CAP_LANDAU = dict(v1=-97.4e9, v2=-67.0e9, v3=-19.1e9, g1=5472.5e9, g2=-1209.0e9, g3=348.7e9, g4=-238.3e9)
+REF_CAP_LAM, REF_CAP_MU = 80.9e9, 23.75e9
+REF_CAPPING = dict(strain_capping_enabled=True, strain_capping_limit=0.10, strain_capping_tangent_ratio=0.1,
+                   strain_capping_smooth_power=1.0)
+
+
+def capped_s33(E2d, E33, cap, kind):
+    """S33 of the capped landau law at diag-block(E_2d, E33), written from the law:
+    deviator scaled toward E_cap (smooth: tanh map), invariants of the capped strain,
+    plus the residual-stiffness term 2 G_t (E33 - c33) w."""
+    lam, mu, c = REF_CAP_LAM, REF_CAP_MU, REF_CAP_LANDAU
+    E = np.zeros(E33.shape + (3, 3))
+    E[..., 0, 0], E[..., 1, 1], E[..., 0, 1], E[..., 1, 0] = E2d[0, 0], E2d[1, 1], E2d[0, 1], E2d[1, 0]
+    E[..., 2, 2] = E33
+    tr = np.trace(E, axis1=-2, axis2=-1)
+    dev = E - tr[..., None, None] / 3.0 * np.eye(3)
+    eq = np.sqrt(np.maximum(0.0, 2.0 / 3.0 * np.einsum("...ij,...ij->...", dev, dev)))
+    Ecap = cap["strain_capping_limit"]
+    if kind == "smooth":
+        p = cap["strain_capping_smooth_power"]
+        r = np.where(eq > 1e-12, eq, 1.0) / Ecap
+        f = np.where(eq > 1e-12, np.tanh(np.minimum(100.0, r ** p)) ** (1.0 / p) / r, 1.0)
+        w = np.tanh(np.minimum(100.0, (eq / Ecap) ** p)) ** 2
+    else:
+        f = np.where(eq > Ecap, Ecap / np.maximum(eq, 1e-300), 1.0)
+        w = (eq > Ecap).astype(float)
+    Ec = dev * f[..., None, None] + tr[..., None, None] / 3.0 * np.eye(3)
+    I1 = np.trace(Ec, axis1=-2, axis2=-1)
+    E2 = Ec @ Ec
+    I2 = np.trace(E2, axis1=-2, axis2=-1)
+    I3 = np.trace(E2 @ Ec, axis1=-2, axis2=-1)
+    A = (lam * I1 + 0.5 * c["v1"] * I1 ** 2 + c["v2"] * I2 + c["g1"] * I1 ** 3 / 6.0 + c["g2"] * I1 * I2
+         + 4.0 / 3.0 * c["g3"] * I3)
+    B = 2.0 * (mu + c["v2"] * I1 + 0.5 * c["g2"] * I1 ** 2 + c["g4"] * I2)
+    C = 4.0 * (c["v3"] + c["g3"] * I1)
+    c33 = Ec[..., 2, 2]
+    return A + B * c33 + C * c33 ** 2 + 2.0 * cap["strain_capping_tangent_ratio"] * mu * (E33 - c33) * w
+
+
+def reference_capped_case(kind, amp, n=8):
+    rng = np.random.default_rng(3)
+    E = REF_CAP_MU * (3 * REF_CAP_LAM + 2 * REF_CAP_MU) / (REF_CAP_LAM + REF_CAP_MU)
+    nu = REF_CAP_LAM / (2 * (REF_CAP_LAM + REF_CAP_MU))
+    I2, I4, I4rt, I4s, II = f2.identity_tensors_2d(n, n)
+    C4 = f2.build_C4_2d(np.full((n, n), E), np.full((n, n), nu), I4s, II, "plane_stress")
+    F = I2.copy()
+    F[0, 0] += amp * rng.random((n, n))
+    F[1, 1] -= 0.3 * amp * rng.random((n, n))
+    F[0, 1] += 0.3 * amp * (2 * rng.random((n, n)) - 1)
+    return F, C4, (I2, I4, I4rt), dict(REF_CAPPING, strain_capping_type=kind)
+
+
+@pytest.mark.parametrize("kind", ["smooth", "piecewise"])
+@pytest.mark.parametrize("amp", [0.05, 0.15])
+def test_2d_capped_plane_stress_reference_case(kind, amp):
+    F, C4, t, cap = reference_capped_case(kind, amp)
+    P, K4, F33 = f2.constitutive_2d(F, C4, *t, model_type="landau", plane_mode="plane_stress", **REF_CAP_LANDAU,
+                                    **cap)
+    E33 = 0.5 * (F33 ** 2 - 1.0)
+    E2d = 0.5 * (np.einsum("jixy,jkxy->ikxy", F, F) - t[0])
+    assert np.abs(capped_s33(E2d, E33, cap, kind)).max() < 1e3
