@@ -462,6 +462,8 @@ typedef struct {
 /* n draws from the stream: kind 0 random_sample(), kind 1 loc + scale * normal. */
 int mgk_mt_fill(mgk_mt_state* state, int kind, double loc, double scale, long long n, double* out);
 
+struct mgk_kmc_patch_cfg;
+
 typedef struct {
   int ndim;
   int64_t shape[3];
@@ -495,7 +497,28 @@ typedef struct {
   int check_unstable;                     /* 2D elastic_run: NaN or |<sigma>| > 20 GPa stops the run */
   double strain_rate;
   double unit[9];                         /* d*d driving direction */
+  /* fast patching (mgkmc/aqs.py:1021-1054, kmc_simulator.py:1044-1098), or NULL: the
+   * event's re-solve is replaced by the predictor (mgk_kmc_patch with the flipped mode's
+   * weights read on the device), a re-solve every sync_interval flips; 2D loops add the
+   * macro-unit stress over the remaining increment time once a flip has happened */
+  const struct mgk_kmc_patch_cfg* patch;
 } mgk_kmc_loop;
+
+/* Fast-patching state of a native KMC loop (planar catalogs, sig_kind 0 or 1). */
+typedef struct mgk_kmc_patch_cfg {
+  int radius[3];                /* per simulation axis (0 on a flat axis)                */
+  int nbasis;                   /* 5 (3D) or 2 (2D)                                      */
+  const double* kernels;        /* device, as mgk_kmc_patch                              */
+  double missing_mean[5][6];    /* per basis, symmetric components                      */
+  const double* sig_unit;       /* device FP64 sigma_macro_unit                          */
+  void* workspace;              /* mgk_kmc_patch_workspace_bytes                         */
+  size_t workspace_bytes;
+  int sync_interval;
+  int* flips_since_sync;        /* host, in/out                                          */
+  int* unit_live;               /* host, in/out: 1 once a patched flip has happened      */
+  double* sig_mean;             /* host, out: d*d mean of the field after the last update */
+  int* mean_from_patch;         /* host, out: 1 if that update was a predictor step      */
+} mgk_kmc_patch_cfg;
 
 typedef struct {
   long long v;

@@ -98,7 +98,16 @@ MgkKmcLoop._fields_ = [
     ("eps", ctypes.c_void_p), ("sig", ctypes.c_void_p), ("eps32", ctypes.c_void_p), ("ws", ctypes.c_void_p),
     ("ws_bytes", ctypes.c_size_t), ("ws32", ctypes.c_void_p), ("ws32_bytes", ctypes.c_size_t), ("warm", ctypes.c_int),
     ("max_iter", ctypes.c_int), ("tol", ctypes.c_double), ("tol32", ctypes.c_double),
-    ("check_unstable", ctypes.c_int), ("strain_rate", ctypes.c_double), ("unit", ctypes.c_double * 9)]
+    ("check_unstable", ctypes.c_int), ("strain_rate", ctypes.c_double), ("unit", ctypes.c_double * 9),
+    ("patch", ctypes.c_void_p)]
+
+
+class MgkKmcPatchCfg(ctypes.Structure):
+    _fields_ = [("radius", ctypes.c_int * 3), ("nbasis", ctypes.c_int), ("kernels", ctypes.c_void_p),
+                ("missing_mean", (ctypes.c_double * 6) * 5), ("sig_unit", ctypes.c_void_p),
+                ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t), ("sync_interval", ctypes.c_int),
+                ("flips_since_sync", ctypes.POINTER(ctypes.c_int)), ("unit_live", ctypes.POINTER(ctypes.c_int)),
+                ("sig_mean", ctypes.POINTER(ctypes.c_double)), ("mean_from_patch", ctypes.POINTER(ctypes.c_int))]
 
 
 def mt_from(rs):

@@ -72,6 +72,10 @@ int mgk_kmc_below_all(const mgk_kmc_loop* L, const double* Q, double thr, long l
                       hipStream_t s);
 int mgk_kmc_below_write(const mgk_kmc_loop* L, const double* Q, double thr, const long long* offsets, long long cap,
                         long long* vv, int* mm, double* qq, hipStream_t s);
+int mgk_kmc_patch_mode(const mgk_kmc_loop* L, const mgk_kmc_patch_cfg* P, long long v, int m, double a,
+                       int with_unit, double* sig_mean, hipStream_t stream);
+int mgk_kmc_patch_unit(const mgk_kmc_loop* L, const mgk_kmc_patch_cfg* P, double a, double* sig_mean,
+                       hipStream_t stream);
 int mgk_kmc_flip_redraw_levels(const mgk_kmc_loop* L, int n, int nlev, const int* dloff, const long long* dlv,
                                const int* dlm, const long long* dv, double time, const double* dred, size_t stride,
                                hipStream_t s);

@@ -105,11 +105,17 @@ void redraw_draws(const mgk_kmc_loop* L, Mt& R, double* host) {
     }
 }
 
+bool patch_ok(const mgk_kmc_loop* L) {
+  const mgk_kmc_patch_cfg* P = L->patch;
+  return !P || (P->kernels && P->sig_unit && P->workspace && P->flips_since_sync && P->unit_live && P->sig_mean &&
+                P->mean_from_patch && P->sync_interval >= 1 && (L->sig_kind == 0 || L->sig_kind == 1));
+}
+
 bool loop_ok(const mgk_kmc_loop* L, const mgk_mt_state* rng, const mgk_mt_state* sel_rng) {
   return L && rng && sel_rng && (L->ndim == 2 || L->ndim == 3) && L->M >= 1 && L->Q0 && L->cat && L->soft &&
          L->tlast && L->epsp && L->kmc_ws && L->redraw_dev && L->sig && L->eps && L->ws &&
          (L->ls_precision != 2 || (L->lam32 && L->mu32 && L->eps32 && L->ws32)) && rng->pos >= 0 &&
-         rng->pos <= 624 && sel_rng->pos >= 0 && sel_rng->pos <= 624;
+         rng->pos <= 624 && sel_rng->pos >= 0 && sel_rng->pos <= 624 && patch_ok(L);
 }
 
 void fill_event(mgk_kmc_event& ev, long long v, int m, int instab, double time, const mgk_ls_info& inf, int it32,
@@ -208,6 +214,22 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
           rc = e;
           break;
         }
+        // fast patching: the predictor from the flipped mode (read before its redraw;
+        // the reference patches after the redraw with the mode it saved)
+        const mgk_kmc_patch_cfg* P = L->patch;
+        bool resolve = true;
+        if (P) {
+          e = mgk_kmc_patch_mode(L, P, v, m, L->strain_rate * tw, tw > 0 ? 1 : 0, P->sig_mean, s);
+          if (e != MGK_OK) {
+            rc = e;
+            break;
+          }
+          *P->unit_live = 1;
+          *P->mean_from_patch = 1;
+          *P->flips_since_sync += 1;
+          resolve = *P->flips_since_sync >= P->sync_interval;
+          if (resolve) *P->flips_since_sync = 0;
+        }
         redraw_draws(L, R, host);
         if (hipMemcpyAsync(L->redraw_dev, host, sizeof(double) * 3 * (size_t)M, hipMemcpyHostToDevice, s) !=
             hipSuccess) {
@@ -221,6 +243,25 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
           rc = e;
           break;
         }
+        if (!resolve) {
+          // a predicted event: no solve, the patched field's mean in the record
+          mgk_ls_info pin = *info;
+          pin.iters = 0;
+          for (int k = 0; k < 9; ++k) pin.sig_mean[k] = k < d * d ? P->sig_mean[k] : 0.0;
+          fill_event(events[n++], v, m, instab, *time, pin, 0, eps_macro, d);
+          *n_events = n;
+          *seq = instab ? *seq + 1 : 0;
+          if (*seq > max_seq) {
+            rc = MGK_KMC_RUN_SEQ;
+            break;
+          }
+          if (n >= max_events) {
+            rc = MGK_KMC_RUN_COUNT;
+            break;
+          }
+          continue;
+        }
+        if (P) *P->mean_from_patch = 0;
         int it32 = 0;
         if (L->ls_precision == 2) {
           e = mgk_ls_solve_refined(d, L->shape, L->lam32, L->mu32, L->lam, L->mu, L->lam0, L->mu0, eps_macro,
@@ -255,6 +296,16 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
       }
     }
     if (!event) {
+      const mgk_kmc_patch_cfg* P = L->patch;
+      if (P && d == 2 && *P->unit_live) {
+        // kmc_simulator.py:1094-1096 (the 3D loop re-solves at the increment's end without it)
+        const int e2 = mgk_kmc_patch_unit(L, P, L->strain_rate * *remaining, P->sig_mean, s);
+        if (e2 != MGK_OK) {
+          rc = e2;
+          break;
+        }
+        *P->mean_from_patch = 1;
+      }
       for (int k = 0; k < d * d; ++k) eps_macro[k] += L->unit[k] * (L->strain_rate * *remaining);
       *time += *remaining;
       *remaining = 0.0;
@@ -305,6 +356,11 @@ extern "C" int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* L, mgk_mt_state* rn
     mgk_set_error("mgk_kmc_run_batch: bad argument", __FILE__, __LINE__);
     return MGK_ERR_ARG;
   }
+  for (int b = 0; b < nb; ++b)
+    if (L[b].patch) {
+      mgk_set_error("mgk_kmc_run_batch: fast patching runs in mgk_kmc_run only", __FILE__, __LINE__);
+      return MGK_ERR_ARG;
+    }
   const mgk_kmc_loop& L0 = L[0];
   const int d = L0.ndim;
   int Mmax = 0;

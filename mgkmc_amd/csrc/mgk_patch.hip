@@ -33,7 +33,38 @@ struct PatchArgs {
   double shift[6];  // per component, the combined missing mean
   double a;         // scale of sig_unit (unused when sig_unit == NULL)
   const double* K;
+  // coefficients read on the device from the planar catalog (gxx, gxy of mode m at
+  // v, before its redraw) with the missing means mm: used when cat != NULL
+  const double* cat;
+  long long off_xx, off_xy;
+  double mm[5][6];
 };
+
+// basis weights and combined missing mean of the planar mode (gxx, gxy)
+// [[gxx, gxy, 0], [gxy, -gxx, 0], [0, 0, 0]] (2D: [[gxx, gxy], [gxy, -gxx]]), in
+// the reference's order (mgkmc/aqs.py:1030-1037, kmc_simulator.py:1055-1057)
+template <int C>
+__device__ __forceinline__ void patch_coefs_from_cat(const PatchArgs& p, double* coef, double* shift) {
+  const double gxx = p.cat[p.off_xx], gxy = p.cat[p.off_xy];
+  if (C == 6) {
+    coef[0] = gxx;
+    coef[1] = -0.0;  // -C22 of a planar mode
+    coef[2] = gxy;
+    coef[3] = 0.0;
+    coef[4] = 0.0;
+  } else {
+    coef[0] = gxx;
+    coef[1] = gxy;
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    double t = coef[0] * p.mm[0][c];
+#pragma unroll
+    for (int b = 1; b < 5; ++b)
+      if (b < p.nb) t = t + coef[b] * p.mm[b][c];
+    shift[c] = t;
+  }
+}
 
 // smallest offset d in [-R, R] with (c0 + d) mod n == c, or R + 1 if none
 __device__ __forceinline__ int first_off(int c, int c0, int n, int R) {
@@ -49,6 +80,15 @@ __global__ __launch_bounds__(MGK_NT) void k_kmc_patch(PatchArgs p, T* __restrict
   double acc[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) acc[c] = 0.0;
+  double coef[5], shift[C];
+  if (p.cat) {
+    patch_coefs_from_cat<C>(p, coef, shift);
+  } else {
+#pragma unroll
+    for (int b = 0; b < 5; ++b) coef[b] = p.coef[b];
+#pragma unroll
+    for (int c = 0; c < C; ++c) shift[c] = p.shift[c];
+  }
   const int wx = 2 * p.Rx + 1, wy = 2 * p.Ry + 1, wz = 2 * p.Rz + 1;
   const size_t cstride = (size_t)wx * wy * wz;
   for (long long v = blockIdx.x * (long long)MGK_NT + threadIdx.x; v < p.NV; v += (long long)gridDim.x * MGK_NT) {
@@ -71,17 +111,17 @@ __global__ __launch_bounds__(MGK_NT) void k_kmc_patch(PatchArgs p, T* __restrict
 #pragma unroll
             for (int c = 0; c < C; ++c) {
               const double* Kc = p.K + (size_t)c * cstride + o;
-              double q = p.coef[0] * Kc[0];
+              double q = coef[0] * Kc[0];
 #pragma unroll
               for (int b = 1; b < 5; ++b)
-                if (b < p.nb) q = q + p.coef[b] * Kc[(size_t)b * C * cstride];
+                if (b < p.nb) q = q + coef[b] * Kc[(size_t)b * C * cstride];
               s[c] = s[c] + q;
             }
           }
     }
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      s[c] = s[c] + p.shift[c];
+      s[c] = s[c] + shift[c];
       const T w = (T)s[c];
       sig[(size_t)c * p.NV + v] = w;
       acc[c] += (double)w;
@@ -114,6 +154,9 @@ __global__ __launch_bounds__(MGK_NT) void k_kmc_patch_mean(const double* __restr
 }
 
 }  // namespace
+
+static int patch_launch(PatchArgs& p, int ndim, int sig_kind, void* sig, const double* sig_unit, void* workspace,
+                        void* stream, double* sig_mean);
 
 extern "C" int mgk_kmc_patch_workspace_bytes(size_t* bytes) {
   if (!bytes) return MGK_ERR_ARG;
@@ -166,6 +209,13 @@ extern "C" int mgk_kmc_patch(int ndim, const int64_t* shape, int sig_kind, void*
   const int C = ndim == 3 ? 6 : 3;
   for (int c = 0; c < C; ++c) p.shift[c] = shift[c];
   p.a = a;
+  p.cat = nullptr;
+  return patch_launch(p, ndim, sig_kind, sig, sig_unit, workspace, stream, sig_mean);
+}
+
+static int patch_launch(PatchArgs& p, int ndim, int sig_kind, void* sig, const double* sig_unit, void* workspace,
+                        void* stream, double* sig_mean) {
+  const int C = ndim == 3 ? 6 : 3;
   hipStream_t s = (hipStream_t)stream;
   double* parts = (double*)workspace;
   double* means = parts + (size_t)kPatchBlocks * 6;
@@ -206,4 +256,54 @@ extern "C" int mgk_kmc_patch(int ndim, const int64_t* shape, int sig_kind, void*
     }
   }
   return MGK_OK;
+}
+
+// Predictor step of a KMC event inside the native loop: the weights of the
+// planar mode m of voxel v are read from the catalog on the device (before its
+// redraw), the combined missing mean is formed on the device in the reference's
+// order, then the one-pass stamp as mgk_kmc_patch.  sig_mean: host d*d mean.
+int mgk_kmc_patch_mode(const mgk_kmc_loop* L, const mgk_kmc_patch_cfg* P, long long v, int m, double a,
+                       int with_unit, double* sig_mean, hipStream_t stream) {
+  const int ndim = L->ndim;
+  PatchArgs p{};
+  p.nx = (int)L->shape[0];
+  p.ny = ndim == 3 ? (int)L->shape[1] : 1;
+  p.nz = ndim == 3 ? (int)L->shape[2] : (int)L->shape[1];
+  p.NV = (long long)p.nx * p.ny * p.nz;
+  if (v < 0 || v >= p.NV || m < 0 || m >= L->M || P->nbasis != (ndim == 3 ? 5 : 2)) {
+    mgk_set_error("mgk_kmc_patch_mode: bad voxel, mode or basis count", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  p.cz = (int)(v % p.nz);
+  p.cy = (int)((v / p.nz) % p.ny);
+  p.cx = (int)(v / ((long long)p.nz * p.ny));
+  p.Rx = P->radius[0];
+  p.Ry = ndim == 3 ? P->radius[1] : 0;
+  p.Rz = ndim == 3 ? P->radius[2] : P->radius[1];
+  p.nb = P->nbasis;
+  p.K = P->kernels;
+  p.a = a;
+  p.cat = L->cat;
+  p.off_xx = (long long)m * p.NV + v;
+  p.off_xy = ((long long)L->M + m) * p.NV + v;
+  for (int b = 0; b < 5; ++b)
+    for (int c = 0; c < 6; ++c) p.mm[b][c] = P->missing_mean[b][c];
+  return patch_launch(p, ndim, L->sig_kind, L->sig, with_unit ? P->sig_unit : nullptr, P->workspace, stream,
+                      sig_mean);
+}
+
+// the macro-unit stress over the remaining time of an increment (2D loops,
+// mgkmc/kmc_simulator.py:1094-1096): sig += sig_unit * a, nothing else
+int mgk_kmc_patch_unit(const mgk_kmc_loop* L, const mgk_kmc_patch_cfg* P, double a, double* sig_mean,
+                       hipStream_t stream) {
+  const int ndim = L->ndim;
+  PatchArgs p{};
+  p.nx = (int)L->shape[0];
+  p.ny = ndim == 3 ? (int)L->shape[1] : 1;
+  p.nz = ndim == 3 ? (int)L->shape[2] : (int)L->shape[1];
+  p.NV = (long long)p.nx * p.ny * p.nz;
+  p.nb = 0;
+  p.a = a;
+  p.cat = nullptr;
+  return patch_launch(p, ndim, L->sig_kind, L->sig, P->sig_unit, P->workspace, stream, sig_mean);
 }

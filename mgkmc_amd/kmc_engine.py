@@ -583,6 +583,7 @@ class DeviceKmc:
         self.sync_interval = fp.get("sync_interval", 100) if fp else 100
         self.flips_since_sync = 0
         self.sigma_macro_unit = None
+        self._su_live = False  # a patched flip has used sigma_macro_unit (2D remaining-time term)
         if disabled:
             self.fast_patching_enabled = False
         if self.fast_patching_enabled:
@@ -666,19 +667,27 @@ class DeviceKmc:
             shift = shift + c * m
         self._patch_call(v, coefs, shift, None, 0.0)
 
+    def _make_sigma_macro_unit(self, strain_unit_tensor):
+        """sigma_macro_unit (mgkmc/aqs.py:1022-1026, kmc_simulator.py:1045-1050): the
+        stress of the unit macro strain with no eigenstrain, a cold LS solve with the
+        current solver arguments, made once."""
+        if self.sigma_macro_unit is not None:
+            return
+        args = dict(max_iter=200, tol=1e-6)
+        args.update(self.solver_args)
+        self._patch_ls.solve(np.asarray(strain_unit_tensor, float), None, warm=False, max_iter=args["max_iter"],
+                             tol=args["tol"])
+        self.sigma_macro_unit = self._patch_ls.sig.clone()
+        self._patch_ls = None  # set-up done: its fields are not needed again
+
     def _patch_after_flip(self, v, C, t_wait, strain_unit_tensor):
         """A KMC event's predictor (mgkmc/aqs.py:1021-1054, kmc_simulator.py:
         1044-1078): the macro-unit stress (computed at the first flip with the
         current solver arguments) scaled by strain_rate * t_wait, the patch and
         the missing mean in one pass; every sync_interval flips the full
         re-solve.  Returns True when it re-solved."""
-        if self.sigma_macro_unit is None:
-            args = dict(max_iter=200, tol=1e-6)
-            args.update(self.solver_args)
-            self._patch_ls.solve(np.asarray(strain_unit_tensor, float), None, warm=False,
-                                 max_iter=args["max_iter"], tol=args["tol"])
-            self.sigma_macro_unit = self._patch_ls.sig.clone()
-            self._patch_ls = None  # set-up done: its fields are not needed again
+        self._make_sigma_macro_unit(strain_unit_tensor)
+        self._su_live = True
         coefs = self._patch_coefs(C)
         mm = self.patch_missing_mean
         shift = coefs[0] * mm[0]
@@ -903,7 +912,7 @@ class DeviceKmc:
                 ev = coords + (m, self.time, is_instab)
                 self.kmc_event_history.append(ev)
                 return ev, remaining_time
-        if self.fast_patching_enabled and self.ndim == 2 and self.sigma_macro_unit is not None:
+        if self.fast_patching_enabled and self.ndim == 2 and self._su_live:
             # kmc_simulator.py:1094-1096 (the 3D loop re-solves at the increment's end without it)
             self._patch_call(0, (), np.zeros((2, 2)), self.sigma_macro_unit, self.strain_rate * remaining_time)
         self.eps_macro += strain_unit_tensor * (self.strain_rate * remaining_time)
@@ -1157,6 +1166,7 @@ class DeviceKmc:
             if ran:
                 self._solved_once = True
                 self.ls_info = self.ls.info
+                self._patched_mean = None
                 if self.precision == MIXED:
                     self.ls.iters32 = it32.value
         if why.value == _lib.MGK_CASCADE_UNSTABLE:
@@ -1221,7 +1231,6 @@ class DeviceKmc:
         gp = getattr(gen, "params", None)
         same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
         return (self.native_loop and self.law is None and not self.finite and not self.cascade_mode
-                and not self.fast_patching_enabled
                 and not self.enable_thermal and self.kmc.planar and self._flip_log is None
                 and gp is not None and same_stream and (self.redraw_directions or self.redraw_barriers)
                 and hasattr(self.rng, "get_state") and hasattr(self.selection_stream, "get_state")
@@ -1264,6 +1273,26 @@ class DeviceKmc:
         u = np.asarray(unit, dtype=np.float64).reshape(-1)
         for i in range(d * d):
             L.unit[i] = u[i]
+        if self.fast_patching_enabled:
+            self._make_sigma_macro_unit(unit)
+            P = self._patch_cfg = _lib.MgkKmcPatchCfg()
+            for i in range(len(self._patch_radius_c)):
+                P.radius[i] = self._patch_radius_c[i]
+            P.nbasis = len(self.patch_missing_mean)
+            P.kernels = self._patch_K.data_ptr()
+            for b, mm in enumerate(self.patch_missing_mean):
+                for c, (i, j) in enumerate(dev.sym_pairs(d)):
+                    P.missing_mean[b][c] = float(mm[i, j])
+            P.sig_unit = self.sigma_macro_unit.data_ptr()
+            P.workspace, P.workspace_bytes = self._patch_ws.data_ptr(), self._patch_ws.numel()
+            P.sync_interval = int(self.sync_interval)
+            self._patch_io = (ctypes.c_int(int(self.flips_since_sync)), ctypes.c_int(int(self._su_live)),
+                              (ctypes.c_double * 9)(), ctypes.c_int(int(self._patched_mean is not None)))
+            P.flips_since_sync = ctypes.pointer(self._patch_io[0])
+            P.unit_live = ctypes.pointer(self._patch_io[1])
+            P.sig_mean = ctypes.cast(self._patch_io[2], ctypes.POINTER(ctypes.c_double))
+            P.mean_from_patch = ctypes.pointer(self._patch_io[3])
+            L.patch = ctypes.addressof(P)
         return L
 
     def _take_native_events(self, evs, rc, it32, on_event=None):
@@ -1324,6 +1353,14 @@ class DeviceKmc:
             _lib.mt_to(self.selection_stream, sel)
             self.time = t.value
             self.eps_macro[...] = np.array(em[: d * d]).reshape(d, d)
+            if self.fast_patching_enabled:
+                fl, live, sm, from_patch = self._patch_io
+                self.flips_since_sync = fl.value
+                self._su_live = bool(live.value)
+                if from_patch.value:
+                    self._patched_mean = np.array(sm[: d * d]).reshape(d, d)
+                elif n_total:
+                    self._patched_mean = None
         return n_total, rem.value, reason, sq.value
 
     def footprint_bytes(self):

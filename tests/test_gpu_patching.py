@@ -195,3 +195,41 @@ def test_patch_fp32_stress_field_matches_fp64(gpu):
     a, b = sims[0].sig_field, sims[1].sig_field
     assert np.abs(b - a).max() <= 2e-6 * np.abs(a).max()
     assert np.allclose(sims[1].sig_mean(), sims[0].sig_mean(), rtol=0, atol=2e-6 * np.abs(a).max())
+
+
+@pytest.mark.parametrize("d,prec", [(3, "fp64"), (2, "fp64"), (3, "mixed"), (2, "fp32")])
+def test_native_fast_patching_equals_python_loop(gpu, d, prec):
+    # the C++ event loop with the predictor (mgk_kmc_run with a patch
+    # configuration) against the per-event Python loop: events, fields, sync
+    # counters and stream states bit for bit
+    from mgkmc_amd.aqs import ThermalSimulation
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    n, M = (10, 20) if d == 3 else (24, 20)
+    shape = (n,) * d
+    E, nu = _moduli(shape, 9)
+    kw = dict(mean=2.2, std=0.45, min_cutoff=0.0)
+    fp = dict(enabled=True, patch_radius=3, sync_interval=7)
+    out = []
+    for native in (True, False):
+        args = dict(pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw, temperature=1276.5, strain_rate=1e7,
+                    fast_patching=fp, ls_precision=prec, rng=np.random.RandomState(41),
+                    selection_stream=np.random.RandomState(42), softening_cap=0.916, jp=20.0, jt=0.0)
+        if d == 3:
+            sim = ThermalSimulation(n, n, n, M, 0.14, E, nu, **args)
+            sim.native_loop = native
+            ev = sim.run_simulation(10 ** 9, 1e-4, component=(0, 0), max_events=25)
+        else:
+            sim = KmcSimulation2D(n, n, M, 0.14, E, nu, **args)
+            sim.native_loop = native
+            ev = sim.run_simulation(10 ** 9, 2e-4, component=(0, 0), stress_targets={(1, 1): 0.0}, mixed_tol=1e6,
+                                    max_events=25)
+        out.append((ev, sim.sig_field, sim.eps_plastic, sim.flips_since_sync, sim.rng.get_state()[2],
+                    sim.selection_stream.get_state()[2], sim.sig_mean(), list(sim.history_global)))
+    a, b = out
+    print(f"[patch native] d={d} {prec}: {len(a[0])} events, flips since sync {a[3]}", flush=True)
+    assert [e[:-2] for e in a[0]] == [e[:-2] for e in b[0]]
+    assert [e[-2] for e in a[0]] == [e[-2] for e in b[0]]
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert a[3] == b[3] and a[4] == b[4] and a[5] == b[5]
+    assert np.array_equal(a[6], b[6])
+    assert a[7] == b[7]
