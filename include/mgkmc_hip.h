@@ -574,6 +574,20 @@ int mgk_kmc_cascade(const mgk_kmc_loop* loop, mgk_mt_state* rng, double* Q, doub
                     int* n_flips, int* batch_sizes, double* batch_sig, int batch_cap, int* n_batches, int* reason,
                     mgk_ls_info* info, int* iters32, void* stream);
 
+/* Macro relaxation after an elastic increment with stress targets, linear law, in the
+ * loop's LS form (eps_macro d*d in/out; info holds the current field's mean on entry and
+ * the last solve's on return).  3D (mgkmc/aqs.py:1105-1142): per iteration the error of
+ * the n_targets components target_idx[2t], target_idx[2t+1] against the current mean;
+ * below tol: *converged = 1; else eps_macro += (err - nu_avg tr(err) I) / E_avg with the
+ * driving component (drv_i, drv_j) zeroed, and a re-solve.  2D (mgkmc/kmc_simulator.py:
+ * 1106-1143): a re-solve first (unstable-stress check: MGK_KMC_RUN_UNSTABLE), the error
+ * test, then per in-plane target in the given order the compliance step, replaced by
+ * err / slope once two iterates exist and the secant slope is finite and above
+ * 0.01 E_avg.  *iters: solves done; *max_err: the last error. */
+int mgk_kmc_mixed_bc(const mgk_kmc_loop* loop, int n_targets, const int* target_idx, const double* target_val,
+                     int drv_i, int drv_j, double E_avg, double nu_avg, double tol, int max_iter, double* eps_macro,
+                     int* converged, double* max_err, int* iters, mgk_ls_info* info, int* iters32, void* stream);
+
 /* Batched form of mgk_kmc_run (no counterpart in the reference: the serving form's
  * continuous batching).  nb (1..MGK_LS_MAX_BATCH) simulations of one grid shape and
  * one LS configuration (ls_precision, warm, max_iter, tol, tol32) step in lockstep on

@@ -184,6 +184,9 @@ SIGNATURES = {
     "mgk_heat_workspace_bytes": (_c_i, [_c_i, _c_i64p, ctypes.POINTER(_c_sz)]),
     "mgk_heat_step": (_c_i, [_c_i, _c_i64p, _c_p, _c_dp, _c_d, _c_d, _c_i, _c_d, _c_d, _c_p, _c_sz, _c_p]),
     "mgk_mt_fill": (_c_i, [ctypes.POINTER(MgkMtState), _c_i, _c_d, _c_d, ctypes.c_longlong, _c_dp]),
+    "mgk_kmc_mixed_bc": (_c_i, [ctypes.POINTER(MgkKmcLoop), _c_i, ctypes.POINTER(_c_i), _c_dp, _c_i, _c_i, _c_d,
+                                _c_d, _c_d, _c_i, _c_dp, ctypes.POINTER(_c_i), _c_dp, ctypes.POINTER(_c_i),
+                                ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i), _c_p]),
     "mgk_kmc_run": (_c_i, [ctypes.POINTER(MgkKmcLoop), ctypes.POINTER(MgkMtState), ctypes.POINTER(MgkMtState),
                            _c_dp, _c_dp, _c_dp, _c_i, _c_i, ctypes.POINTER(_c_i), ctypes.POINTER(MgkKmcEvent),
                            ctypes.POINTER(_c_i), ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i), _c_p]),

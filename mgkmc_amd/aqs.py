@@ -90,6 +90,8 @@ class ThermalSimulation(DeviceKmc):
         when it is truncated or reaches eps_target); the corrections re-solve
         with the linear LS (update_stress_fft_full) whatever the law, as the
         reference does.  Returns (converged, last max error)."""
+        if not self.cascade_mode and self._native_mixed_ok():
+            return self._run_mixed_native(stress_targets, component, mixed_tol, mixed_max_iter)
         E_avg, nu_avg = np.mean(self.E), np.mean(self.nu)
         converged = False
         max_err = 0.0

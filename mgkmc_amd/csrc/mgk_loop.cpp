@@ -132,6 +132,18 @@ void fill_event(mgk_kmc_event& ev, long long v, int m, int instab, double time, 
   }
 }
 
+// the loop's LS re-solve at eps_macro (FP64, FP32 or FP32 + FP64 refinement)
+int loop_solve(const mgk_kmc_loop* L, const double* eps_macro, mgk_ls_info* info, int* it32, void* stream) {
+  const int d = L->ndim;
+  *it32 = 0;
+  if (L->ls_precision == 2)
+    return mgk_ls_solve_refined(d, L->shape, L->lam32, L->mu32, L->lam, L->mu, L->lam0, L->mu0, eps_macro,
+                                L->epsp_shadow, L->epsp, L->eps32, L->eps, L->sig, L->warm, L->max_iter, L->tol,
+                                L->tol32, L->ws32, L->ws32_bytes, L->ws, L->ws_bytes, stream, info, it32);
+  return mgk_ls_solve(d, L->shape, L->ls_precision, L->lam, L->mu, L->lam0, L->mu0, eps_macro, L->epsp, L->eps,
+                      L->sig, L->warm, L->max_iter, L->tol, L->ws, L->ws_bytes, stream, info);
+}
+
 bool unstable_mean(const mgk_ls_info& inf, int d) {
   double mx = 0.0;
   for (int i = 0; i < d * d; ++i) {
@@ -821,5 +833,113 @@ extern "C" int mgk_kmc_cascade(const mgk_kmc_loop* L, mgk_mt_state* rng, double*
     }
   }
   (void)hipStreamSynchronize(s);
+  return MGK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Macro relaxation after an elastic increment (linear law, small strain).
+// 3D (mgkmc/aqs.py:1105-1142): per iteration the stress error of the targeted
+// components against the current mean; below tol: converged; else the
+// Poisson-corrected compliance step (err - nu tr(err) I) / E with the driving
+// component zeroed, then a re-solve.  2D (mgkmc/kmc_simulator.py:1106-1143):
+// per iteration a re-solve first (with the unstable-stress check), then the
+// error test, then per targeted in-plane component (in the given order) the
+// compliance step, replaced by err / slope once two iterates exist and the
+// secant slope ds / de is finite and above 0.01 E.
+// ---------------------------------------------------------------------------
+extern "C" int mgk_kmc_mixed_bc(const mgk_kmc_loop* L, int n_targets, const int* target_idx,
+                                const double* target_val, int drv_i, int drv_j, double E_avg, double nu_avg,
+                                double tol, int max_iter, double* eps_macro, int* converged, double* max_err,
+                                int* iters, mgk_ls_info* info, int* iters32, void* stream) {
+  if (!L || (L->ndim != 2 && L->ndim != 3) || !L->sig || !L->eps || !L->ws || n_targets < 0 || n_targets > 9 ||
+      (n_targets > 0 && (!target_idx || !target_val)) || !eps_macro || !converged || !max_err || !iters || !info ||
+      (L->ls_precision == 2 && !(L->lam32 && L->mu32 && L->eps32 && L->ws32))) {
+    mgk_set_error("mgk_kmc_mixed_bc: bad argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  const int d = L->ndim;
+  for (int t = 0; t < n_targets; ++t)
+    if (target_idx[2 * t] < 0 || target_idx[2 * t] > 2 || target_idx[2 * t + 1] < 0 || target_idx[2 * t + 1] > 2) {
+      mgk_set_error("mgk_kmc_mixed_bc: target index outside 0..2", __FILE__, __LINE__);
+      return MGK_ERR_ARG;
+    }
+  *converged = 0;
+  *max_err = 0.0;
+  *iters = 0;
+  int it32 = 0, e = MGK_OK;
+  if (d == 3) {
+    for (int it = 0; it < max_iter; ++it) {
+      double err[9] = {0};
+      double mx = 0.0;
+      for (int t = 0; t < n_targets; ++t) {
+        const int k = 3 * target_idx[2 * t] + target_idx[2 * t + 1];
+        err[k] = target_val[t] - info->sig_mean[k];
+        mx = std::fmax(mx, std::fabs(err[k]));
+      }
+      *max_err = mx;
+      if (mx < tol) {
+        *converged = 1;
+        break;
+      }
+      const double tr = err[0] + err[4] + err[8];
+      double corr[9];
+      for (int k = 0; k < 9; ++k) corr[k] = (err[k] - nu_avg * tr * ((k % 4 == 0) ? 1.0 : 0.0)) / E_avg;
+      corr[3 * drv_i + drv_j] = 0.0;
+      for (int k = 0; k < 9; ++k) eps_macro[k] += corr[k];
+      e = loop_solve(L, eps_macro, info, &it32, stream);
+      ++*iters;
+      if (e < 0 && e != MGK_ERR_NONFINITE) return e;
+    }
+  } else {
+    bool have_prev[4] = {false, false, false, false};
+    double prev_e[4] = {0}, prev_s[4] = {0};
+    for (int it = 0; it < max_iter; ++it) {
+      e = loop_solve(L, eps_macro, info, &it32, stream);
+      ++*iters;
+      if (e < 0 && e != MGK_ERR_NONFINITE) return e;
+      if (L->check_unstable && unstable_mean(*info, 2)) {
+        mgk_set_error("Unstable stress in small-strain elastic_run", __FILE__, __LINE__);
+        if (iters32) *iters32 = it32;
+        return MGK_KMC_RUN_UNSTABLE;
+      }
+      const double* sm = info->sig_mean;
+      double err[4] = {0};
+      double mx = 0.0;
+      for (int t = 0; t < n_targets; ++t) {
+        const int a = target_idx[2 * t], b = target_idx[2 * t + 1];
+        if (a < 2 && b < 2) {
+          err[2 * a + b] = target_val[t] - sm[2 * a + b];
+          mx = std::fmax(mx, std::fabs(err[2 * a + b]));
+        }
+      }
+      *max_err = mx;
+      if (mx < tol) {
+        *converged = 1;
+        break;
+      }
+      const double tr = err[0] + err[3];
+      double de[4];
+      for (int k = 0; k < 4; ++k) de[k] = (err[k] - nu_avg * tr * ((k == 0 || k == 3) ? 1.0 : 0.0)) / E_avg;
+      for (int t = 0; t < n_targets; ++t) {
+        const int a = target_idx[2 * t], b = target_idx[2 * t + 1];
+        if (a >= 2 || b >= 2) continue;
+        const int k = 2 * a + b;
+        double step = de[k];
+        if (have_prev[k]) {
+          const double dE = eps_macro[k] - prev_e[k];
+          const double dS = sm[k] - prev_s[k];
+          if (std::fabs(dE) > 1e-14 && std::isfinite(dS)) {
+            const double slope = dS / dE;
+            if (std::isfinite(slope) && slope > 0.01 * E_avg) step = err[k] / slope;
+          }
+        }
+        have_prev[k] = true;
+        prev_e[k] = eps_macro[k];
+        prev_s[k] = sm[k];
+        eps_macro[k] += step;
+      }
+    }
+  }
+  if (iters32) *iters32 = it32;
   return MGK_OK;
 }

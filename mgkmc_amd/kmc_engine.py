@@ -1236,7 +1236,7 @@ class DeviceKmc:
                 and hasattr(self.rng, "get_state") and hasattr(self.selection_stream, "get_state")
                 and (self._solved_once or not self.ls_warm_start))
 
-    def _loop_struct(self, unit):
+    def _loop_struct(self, unit, patch=True):
         k, ls, d = self.kmc, self.ls, self.ndim
         L = _lib.MgkKmcLoop()
         L.ndim, L.M = d, self.M
@@ -1252,7 +1252,7 @@ class DeviceKmc:
         L.jp, L.jt, L.jn = self.jp, self.jt, self.neighbor_softening_fraction
         L.stability_threshold = self.stability_threshold
         L.redraw_directions, L.redraw_barriers = int(bool(self.redraw_directions)), int(bool(self.redraw_barriers))
-        gp = self.barrier_generator.params
+        gp = getattr(self.barrier_generator, "params", None) or {}
         L.gamma0, L.q_mean, L.q_std = self.gamma0, float(gp.get("mean", 2.0)), float(gp.get("std", 0.6))
         lo, hi = gp.get("min_cutoff"), gp.get("max_cutoff")
         L.q_has_min, L.q_min = int(lo is not None), float(lo if lo is not None else 0.0)
@@ -1273,7 +1273,7 @@ class DeviceKmc:
         u = np.asarray(unit, dtype=np.float64).reshape(-1)
         for i in range(d * d):
             L.unit[i] = u[i]
-        if self.fast_patching_enabled:
+        if patch and self.fast_patching_enabled:
             self._make_sigma_macro_unit(unit)
             P = self._patch_cfg = _lib.MgkKmcPatchCfg()
             for i in range(len(self._patch_radius_c)):
@@ -1294,6 +1294,46 @@ class DeviceKmc:
             P.mean_from_patch = ctypes.pointer(self._patch_io[3])
             L.patch = ctypes.addressof(P)
         return L
+
+    def _native_mixed_ok(self):
+        """Whether the macro relaxation runs in the library (mgk_kmc_mixed_bc):
+        linear law, small strain, no cascade interleaving, the solver's fields
+        current."""
+        return (self.native_loop and self.law is None and not self.finite
+                and (not self.cascade_mode or self.ndim == 2)  # the 3D loop interleaves cascades
+                and self._sig_owner is None and (self._solved_once or not self.ls_warm_start))
+
+    def _run_mixed_native(self, stress_targets, component, mixed_tol, mixed_max_iter):
+        """The macro relaxation after an increment through mgk_kmc_mixed_bc (3D:
+        mgkmc/aqs.py:1105-1142; 2D: kmc_simulator.py:1106-1143).  Returns
+        (converged, last max error)."""
+        d = self.ndim
+        lib = _lib.load()
+        L = self._loop_struct(np.zeros((d, d)), patch=False)  # every iteration solves
+        items = list(stress_targets.items())
+        idx = (ctypes.c_int * max(1, 2 * len(items)))(*[int(i) for k, _ in items for i in k])
+        val = (ctypes.c_double * max(1, len(items)))(*[float(v) for _, v in items])
+        em = (ctypes.c_double * 9)(*self.eps_macro.reshape(-1))
+        conv, mx, it, it32 = ctypes.c_int(), ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        if self.ls_info is None:  # no solve yet: the zero initial stress
+            for k in range(9):
+                self.ls.info.sig_mean[k] = 0.0
+        rc = lib.mgk_kmc_mixed_bc(ctypes.byref(L), len(items), idx, val, int(component[0]), int(component[1]),
+                                  float(np.mean(self.E_field)), float(np.mean(self.nu_field)), float(mixed_tol),
+                                  int(mixed_max_iter), em, ctypes.byref(conv), ctypes.byref(mx), ctypes.byref(it),
+                                  ctypes.byref(self.ls.info), ctypes.byref(it32), ctypes.c_void_p(dev.stream_ptr()))
+        self.eps_macro[...] = np.array(em[: d * d]).reshape(d, d)
+        if it.value:
+            self._solved_once = True
+            self._patched_mean = None
+            self.ls_info = self.ls.info
+            if self.precision == MIXED:
+                self.ls.iters32 = it32.value
+        if rc == _lib.MGK_KMC_RUN_UNSTABLE:
+            sm = np.array(self.ls.info.sig_mean[: d * d]).reshape(d, d)
+            raise ValueError(f"Unstable stress in small-strain elastic_run: {sm / 1e9} GPa")
+        _lib.check(rc, "mgk_kmc_mixed_bc")
+        return bool(conv.value), mx.value
 
     def _take_native_events(self, evs, rc, it32, on_event=None):
         """Record events returned by the native loop (history, LS iteration

@@ -95,6 +95,9 @@ class KmcSimulation2D(DeviceKmc):
         solve, stop if every targeted stress is within mixed_tol, else the
         Poisson-corrected compliance step, replaced per component by the secant
         slope once two iterates exist (and the slope exceeds 0.01 E_avg)."""
+        if self._native_mixed_ok():
+            self._run_mixed_native(stress_targets, (0, 0), mixed_tol, mixed_max_iter)
+            return
         E_avg = self.E_field.mean()
         nu_avg = self.nu_field.mean()
         prev_eps, prev_sig = {}, {}

@@ -165,3 +165,40 @@ def test_finite_strain_cascade_after_each_increment_matches_oracle(gpu, tmp_path
     F = np.einsum("xyzij->ijxyz", sim.F_field)
     assert np.abs(F - st["F"]).max() <= 1e-10 * np.abs(st["F"]).max()
     assert np.abs(sim.F_macro - st["F_macro"]).max() <= 1e-12
+
+
+@pytest.mark.parametrize("d,prec,targets", [(3, "fp64", None), (3, "mixed", {(1, 1): 0.0, (2, 2): 2e7}),
+                                            (2, "fp64", {(1, 1): 0.0}), (2, "fp32", {(1, 1): 3e7, (0, 1): 0.0})])
+def test_native_mixed_bc_equals_python_loop(gpu, d, prec, targets):
+    # the macro relaxation in the library (mgk_kmc_mixed_bc) against the Python
+    # loops of both simulators: macro strains, fields, increment histories and
+    # the logged stresses bit for bit, with several corrections per increment
+    import numpy as np
+    from mgkmc_amd.aqs import ThermalSimulation
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    n, M = (10, 12) if d == 3 else (20, 12)
+    rng = np.random.default_rng(4)
+    mu = 26.923e9 * rng.uniform(0.8, 1.2, (n,) * d)
+    nu = np.full((n,) * d, 0.35)
+    E = 2 * mu * (1 + nu)
+    kw = dict(mean=2.2, std=0.45, min_cutoff=0.0)
+    out = []
+    for native in (True, False):
+        args = dict(pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw, temperature=900.0, strain_rate=1e7,
+                    ls_precision=prec, rng=np.random.RandomState(51), selection_stream=np.random.RandomState(52))
+        if d == 3:
+            sim = ThermalSimulation(n, n, n, M, 0.14, E, nu, **args)
+        else:
+            sim = KmcSimulation2D(n, n, M, 0.14, E, nu, **args)
+        sim.native_loop = native
+        ev = sim.run_simulation(6, 2e-5, component=(0, 0), stress_targets=dict(targets or {}), mixed_tol=1e4,
+                                mixed_max_iter=30, max_events=12)
+        out.append((ev, sim.eps_macro.copy(), sim.sig_field, [tuple(map(float, np.ravel(a))) + tuple(map(float, np.ravel(b)))
+                                                             for a, b in sim.increment_history],
+                    list(sim.history_global)))
+    a, b = out
+    print(f"[mixed native] d={d} {prec}: {len(a[0])} events, {len(a[3])} increments", flush=True)
+    assert len(a[3]) > 0
+    assert [e[:-2] for e in a[0]] == [e[:-2] for e in b[0]]
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert a[3] == b[3] and a[4] == b[4]
