@@ -177,12 +177,56 @@ __device__ __forceinline__ void dft8(cplx<T>* v, bool inv) {
   v[7] = b[3];
 }
 
+// w16^j = exp(-+2 pi i j / 16) times a (forward: -, inverse: +), j a compile-time
+// multiple of pi/8 (cos(pi/8) = 0.9238795325..., sin(pi/8) = 0.3826834323...)
+template <int J, typename T>
+__device__ __forceinline__ cplx<T> w16mul(cplx<T> a, bool inv) {
+  constexpr int j = J % 16;
+  const T c8 = T(0.92387953251128675612818318939679), s8 = T(0.38268343236508977172845998403040);
+  const T r = T(0.70710678118654752440084436210485);
+  // cos and sin of j pi / 8 for j = 0..15
+  const T cs[16] = {T(1), c8, r, s8, T(0), -s8, -r, -c8, T(-1), -c8, -r, -s8, T(0), s8, r, c8};
+  const T sn[16] = {T(0), s8, r, c8, T(1), c8, r, s8, T(0), -s8, -r, -c8, T(-1), -c8, -r, -s8};
+  const T c = cs[j], sg = inv ? sn[j] : -sn[j];
+  return cmk(a.x * c - a.y * sg, a.x * sg + a.y * c);
+}
+
+// 16-point DFT as 4 x 4: DFT4 over the four stride-4 subsequences, twiddles
+// w16^(b k1), DFT4 across them; natural order in and out
+template <typename T>
+__device__ __forceinline__ void dft16(cplx<T>* v, bool inv) {
+  cplx<T> y[4][4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) y[b][q] = v[4 * q + b];
+    dft4(y[b], inv);
+  }
+  y[1][1] = w16mul<1>(y[1][1], inv);
+  y[1][2] = w16mul<2>(y[1][2], inv);
+  y[1][3] = w16mul<3>(y[1][3], inv);
+  y[2][1] = w16mul<2>(y[2][1], inv);
+  y[2][2] = w16mul<4>(y[2][2], inv);
+  y[2][3] = w16mul<6>(y[2][3], inv);
+  y[3][1] = w16mul<3>(y[3][1], inv);
+  y[3][2] = w16mul<6>(y[3][2], inv);
+  y[3][3] = w16mul<9>(y[3][3], inv);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cplx<T> z[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+    dft4(z, inv);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = z[k2];
+  }
+}
+
 template <int R, typename T>
 __device__ __forceinline__ void dftR(cplx<T>* v, bool inv) {
   if constexpr (R == 2) dft2(v);
   else if constexpr (R == 3) dft3(v, inv);
   else if constexpr (R == 4) dft4(v, inv);
   else if constexpr (R == 5) dft5(v, inv);
+  else if constexpr (R == 16) dft16(v, inv);
   else dft8(v, inv);
 }
 
@@ -278,10 +322,14 @@ __device__ void tile_fft(cplx<T>* __restrict__ buf, const Tile& tl, const FftPla
 constexpr int ct_log2(int v) { return v <= 1 ? 0 : 1 + ct_log2(v / 2); }
 
 // radix of the pass that starts at sub-transform length L: a radix-3 pass
-// first for lengths 3 * 2^k, then 8s, then 4/2
-constexpr int ct_radix(int N, int L) {
+// first for lengths 3 * 2^k, then 8s, then 4/2.  With `wide` (FP32 tiles)
+// 256 = 16 * 16 and 128 = 16 * 8 take two passes instead of three; a
+// 16-point butterfly holds 64 VGPRs of FP64 values, which costs the FP64
+// sweeps more occupancy than the saved LDS round trip gains.
+constexpr int ct_radix(int N, int L, bool wide) {
   if ((N / L) % 3 == 0) return 3;
   const int k = ct_log2(N / L);
+  if (wide && (k == 8 || k == 7 || k == 4)) return 16;
   return (k % 3 == 0) ? 8 : ((k % 3 == 2) ? 4 : (k >= 4 ? 4 : 2));
 }
 
@@ -376,7 +424,7 @@ __device__ __forceinline__ void ct_pass(cplx<T>* __restrict__ buf, const cplx<T>
 template <int N, int B, int C, int L, bool PAD, int NT, typename T>
 __device__ __forceinline__ void ct_fft_rec(cplx<T>* __restrict__ buf, const cplx<T>* __restrict__ tw, bool inv) {
   if constexpr (L < N) {
-    constexpr int R = ct_radix(N, L);
+    constexpr int R = ct_radix(N, L, sizeof(T) == 4);
     ct_pass<N, B, C, R, L, PAD, NT>(buf, tw, inv);
     ct_fft_rec<N, B, C, L * R, PAD, NT>(buf, tw, inv);
   }
