@@ -93,7 +93,33 @@ struct LsMem {
   Sym6 eb;
   double lam0, mu0, e2_extra;
   T* sig;  // final stress (batched solves), or NULL
+  // FP64 solves started from an FP32 iterate: the first iteration's x sweeps
+  // read the starting field from here (exact widening) instead of eps, so no
+  // widened copy is written first; NULL otherwise
+  const float* eps_old32;
 };
+
+// eps reads of the x sweeps, from the FP32 starting field when one is given
+template <typename T>
+__device__ __forceinline__ void eps_vload(const T* __restrict__ eps, const float* __restrict__ e32, size_t off,
+                                          T (&o)[16 / sizeof(T)]) {
+  if constexpr (sizeof(T) == 8) {
+    if (e32) {
+      const float2 v = *reinterpret_cast<const float2*>(e32 + off);
+      o[0] = (T)v.x;
+      o[1] = (T)v.y;
+      return;
+    }
+  }
+  vload16(eps + off, o);
+}
+template <typename T>
+__device__ __forceinline__ T eps_ld(const T* __restrict__ eps, const float* __restrict__ e32, size_t off) {
+  if constexpr (sizeof(T) == 8) {
+    if (e32) return (T)e32[off];
+  }
+  return eps[off];
+}
 
 template <typename T>
 struct LsBatch {
@@ -181,6 +207,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
   const LsMem<T>& M = bt.m[blockIdx.z];
   if (M.st->done) return;
   const T* __restrict__ eps = M.eps;
+  const float* __restrict__ e32 = sizeof(T) == 8 ? M.eps_old32 : nullptr;
   const T* __restrict__ epsp = M.epsp;
   const T* __restrict__ lam = M.lam;
   const T* __restrict__ mu = M.mu;
@@ -216,11 +243,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
       const int bg = e % GPR, x = e / GPR;
       const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
       T exx[VW], eyy[VW], ezz[VW], pxx[VW], pyy[VW], pzz[VW], l[VW], m[VW];
-      vload16(eps + v, exx);
-      vload16(eps + NV + v, eyy);
+      eps_vload(eps, e32, v, exx);
+      eps_vload(eps, e32, NV + v, eyy);
       vload16(lam + v, l);
       vload16(mu + v, m);
-      if (D == 3) vload16(eps + 2 * NV + v, ezz);
+      if (D == 3) eps_vload(eps, e32, 2 * NV + v, ezz);
       if (epsp) {
         vload16(epsp + v, pxx);
         vload16(epsp + NV + v, pyy);
@@ -263,13 +290,13 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
         T txx = T(0), tyy = T(0);
         if (q < Q) {
           const size_t v = ((size_t)x * P + p) * Q + q;
-          const T exx = eps[v], eyy = eps[NV + v];
+          const T exx = eps_ld(eps, e32, v), eyy = eps_ld(eps, e32, NV + v);
           const T pxx = epsp ? epsp[v] : T(0), pyy = epsp ? epsp[NV + v] : T(0);
           const T l = lam[v], m = mu[v];
           const T dxx = exx - pxx, dyy = eyy - pyy;
           T tre, tr0, ezz = T(0), dzz = T(0);
           if (D == 3) {
-            ezz = eps[2 * NV + v];
+            ezz = eps_ld(eps, e32, 2 * NV + v);
             const T pzz = epsp ? epsp[2 * NV + v] : T(0);
             dzz = ezz - pzz;
             tre = (dxx + dyy) + dzz;
@@ -311,11 +338,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
         const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
         T ea[VW], pa[VW], eb[VW], pb[VW];
         if (!(D == 3 && pr == 0)) {
-          vload16(eps + (size_t)ca * NV + v, ea);
+          eps_vload(eps, e32, (size_t)ca * NV + v, ea);
           if (epsp) vload16(epsp + (size_t)ca * NV + v, pa);
         }
         if (cb >= 0) {
-          vload16(eps + (size_t)cb * NV + v, eb);
+          eps_vload(eps, e32, (size_t)cb * NV + v, eb);
           if (epsp) vload16(epsp + (size_t)cb * NV + v, pb);
         }
 #pragma unroll
@@ -341,12 +368,12 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
             if (D == 3 && pr == 0) {
               ta = keep[s];
             } else {
-              const T ea = eps[(size_t)ca * NV + v];
+              const T ea = eps_ld(eps, e32, (size_t)ca * NV + v);
               const T pa = epsp ? epsp[(size_t)ca * NV + v] : T(0);
               ta = m2 * (ea - pa) - m02 * ea;
             }
             if (cb >= 0) {
-              const T eb = eps[(size_t)cb * NV + v];
+              const T eb = eps_ld(eps, e32, (size_t)cb * NV + v);
               const T pb = epsp ? epsp[(size_t)cb * NV + v] : T(0);
               tb = m2 * (eb - pb) - m02 * eb;
             }
@@ -496,6 +523,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
   if (M.st->done) return;
   const cplx<T>* __restrict__ spec = M.spec;
   T* __restrict__ eps = M.eps;
+  const float* __restrict__ e32 = sizeof(T) == 8 ? M.eps_old32 : nullptr;
   const Sym6 ebar = M.eb;
   double* __restrict__ partials = M.part;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -544,8 +572,8 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
         const int bg = e % GPR, x = e / GPR;
         const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
         T oa[VW], na[VW], ob[VW], nb[VW];
-        vload16(eps + (size_t)ca * NV + v, oa);
-        if (cb >= 0) vload16(eps + (size_t)cb * NV + v, ob);
+        eps_vload(eps, e32, (size_t)ca * NV + v, oa);
+        if (cb >= 0) eps_vload(eps, e32, (size_t)cb * NV + v, ob);
 #pragma unroll
         for (int i = 0; i < VW; ++i) {
           const cplx<T> z = tile[x * ldb + bg * VW + i];
@@ -576,7 +604,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
           const cplx<T> z = tile[x * ldb + b];
           {
             const T nw = eba + z.x;
-            const T od = eps[(size_t)ca * NV + v];
+            const T od = eps_ld(eps, e32, (size_t)ca * NV + v);
             const double dd = (double)nw - (double)od;
             acc[0] += wa * dd * dd;
             acc[1] += wa * (double)od * (double)od;
@@ -584,7 +612,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
           }
           if (cb >= 0) {
             const T nw = ebb + z.y;
-            const T od = eps[(size_t)cb * NV + v];
+            const T od = eps_ld(eps, e32, (size_t)cb * NV + v);
             const double dd = (double)nw - (double)od;
             acc[0] += wb * dd * dd;
             acc[1] += wb * (double)od * (double)od;
@@ -1178,13 +1206,8 @@ __global__ void k_zero_words(unsigned int* __restrict__ p, size_t n) {
   for (size_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
-__global__ void k_widen(const float* __restrict__ a, double* __restrict__ b, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    b[i] = (double)a[i];
-}
-
 // ---------------------------------------------------------------------------
-// Batched forms of the set-up, mean, stress and widening passes (member
+// Batched forms of the set-up, mean and stress passes (member
 // blockIdx.y, or blockIdx.x for the one-block passes): each member runs the
 // single pass's body on its own buffers.
 // ---------------------------------------------------------------------------
@@ -1218,18 +1241,6 @@ template <typename T, int D>
 __global__ __launch_bounds__(MGK_NT) void k_ls_stress_b(LsBatch<T> bt, size_t NV) {
   const LsMem<T>& M = bt.m[blockIdx.y];
   ls_stress_body<T, D>(M.eps, M.epsp, M.lam, M.mu, M.sig, NV, M.part);
-}
-
-struct WidenBatch {
-  const float* a[MGK_MAXB];
-  double* b[MGK_MAXB];
-};
-
-__global__ void k_widen_b(WidenBatch w, size_t n) {
-  const float* __restrict__ a = w.a[blockIdx.y];
-  double* __restrict__ b = w.b[blockIdx.y];
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    b[i] = (double)a[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1631,7 +1642,8 @@ template <typename T, int D>
 int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double mu0, const double* eps_bar,
                const T* epsp, T* eps, T* sig, int warm, int max_iter, double tol, char* ws, hipStream_t stream,
                mgk_ls_info* info, bool final_stress = true, const LandauPar* lp = nullptr, T* taubuf = nullptr,
-               const L2Mode* l2 = nullptr, const SecLaw* sl = nullptr, const NoAllocCtx* na = nullptr) {
+               const L2Mode* l2 = nullptr, const SecLaw* sl = nullptr, const NoAllocCtx* na = nullptr,
+               const float* eps_from32 = nullptr) {
   constexpr int MV = maxv<T>();
   constexpr int TT = ct_tile<T>();  // points per compile-time tile
   constexpr int XT = x_tile<T>();   // points per compile-time x-sweep tile
@@ -1682,6 +1694,13 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   // this solve as a one-member table of the linear sweeps
   LsBatch<T> bt{};
   bt.m[0] = LsMem<T>{lam, mu, epsp, eps, spec, part, st, eb, lam0, mu0, e2_extra};
+  // an FP32 starting field is read by the first iteration's x sweeps only
+  // (the linear law's; the nonlinear laws read eps in their own kernels)
+  if (eps_from32 && (lp || sl || l2 || warm != 2)) {
+    mgk_set_error("an FP32 starting field needs the linear law and warm = 2", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  bt.m[0].eps_old32 = eps_from32;
   const MidPtrs<T> mp = mid_table(bt, 1, C);
   const LsFin fin = fin_table(bt, 1);
 
@@ -1742,6 +1761,7 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
     launch_xinv<T, D>(pl, g, gx, stream, bt);
     prof.end(h);
+    bt.m[0].eps_old32 = nullptr;  // eps now holds the first iterate
     k_ls_finalize<<<1, MGK_NT, 0, stream>>>(fin, pl.nblk_upd, tol, max_iter);
     if constexpr (sizeof(T) == 8) {
       if (l2 && l2->aa_m > 0) {
@@ -1861,6 +1881,7 @@ struct LsMemArgs {
   T* eps;
   T* sig;
   char* ws;
+  const float* eps_from32;  // FP32 starting field of an FP64 stage (warm = 2), or NULL
 };
 
 // pinned host copies of the members' solve states, per thread (two chunks
@@ -1902,7 +1923,7 @@ int ls_solve_batch_t(const LsPlan& pl, int nb, const LsMemArgs<T>* a, int warm, 
     const Sym6 eb = ls_sym_bar<D>(a[b].eps_bar, &anti2);
     bt.m[b] = LsMem<T>{a[b].lam, a[b].mu, a[b].epsp, a[b].eps, (cplx<T>*)(a[b].ws + pl.off_spec),
                        (double*)(a[b].ws + pl.off_part), (LsState*)(a[b].ws + pl.off_state), eb, a[b].lam0,
-                       a[b].mu0, anti2 * (double)pl.NV, a[b].sig};
+                       a[b].mu0, anti2 * (double)pl.NV, a[b].sig, a[b].eps_from32};
     hint = std::max(hint, ls_iter_hint(a[b].ws, 0));
   }
   // set-up: state cleared, eps filled with eps_bar (cold) or shifted to mean eps_bar (warm 1)
@@ -1950,6 +1971,7 @@ int ls_solve_batch_t(const LsPlan& pl, int nb, const LsMemArgs<T>* a, int warm, 
     h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
     launch_xinv<T, D>(pl, g, gx, stream, bt);
     prof.end(h);
+    for (int b = 0; b < nb; ++b) bt.m[b].eps_old32 = nullptr;  // eps now holds the first iterate
     k_ls_finalize<<<nb, MGK_NT, 0, stream>>>(fin, pl.nblk_upd, tol, max_iter);
     ++iter_idx;
   };
@@ -2561,21 +2583,19 @@ extern "C" int mgk_ls_solve_refined(int ndim, const int64_t* shape, const void* 
     if (info) *info = i32;
     return rc;
   }
-  // stage 2: widen and refine in FP64 with the reference's stop test at tol.
-  // The FP32 iterate already has mean eps_bar up to FP32 rounding, and every
-  // update restores the exact mean (Gamma0(0) = 0), so the stage starts from
-  // the widened field as is (warm = 2: no mean adjustment pass).
-  const size_t n = (size_t)p64.C * (size_t)p64.NV;
-  const int nb = (int)std::min<size_t>((n + MGK_NT - 1) / MGK_NT, 8192);
-  k_widen<<<nb, MGK_NT, 0, s>>>((const float*)eps32, (double*)eps64, n);
-  MGK_HIP(hipGetLastError());
+  // stage 2: refine in FP64 with the reference's stop test at tol, starting
+  // from the FP32 iterate widened exactly.  It already has mean eps_bar up to
+  // FP32 rounding, and every update restores the exact mean (Gamma0(0) = 0),
+  // so no mean adjustment pass runs (warm = 2); the first iteration's x
+  // sweeps read the FP32 field directly and its update writes eps64 whole.
+  const float* from32 = (const float*)eps32;
   if (ndim == 3)
     return ls_solve_t<double, 3>(p64, (const double*)lam64, (const double*)mu64, lam0, mu0, eps_bar,
                                  (const double*)epsp64, (double*)eps64, (double*)sig64, 2, max_iter, tol, (char*)ws64,
-                                 s, info);
+                                 s, info, true, nullptr, nullptr, nullptr, nullptr, nullptr, from32);
   return ls_solve_t<double, 2>(p64, (const double*)lam64, (const double*)mu64, lam0, mu0, eps_bar,
                                (const double*)epsp64, (double*)eps64, (double*)sig64, 2, max_iter, tol, (char*)ws64, s,
-                               info);
+                               info, true, nullptr, nullptr, nullptr, nullptr, nullptr, from32);
 }
 
 
@@ -2668,8 +2688,8 @@ extern "C" int mgk_ls_solve_refined_batch(int ndim, const int64_t* shape, int nb
   if (rc != MGK_OK) return rc;
   hipStream_t s = (hipStream_t)stream;
   warm = warm ? 1 : 0;
-  // stage 1 (FP32 to tol32, no stress), widen, stage 2 (FP64 from the widened
-  // field, warm = 2), as mgk_ls_solve_refined does for one member
+  // stage 1 (FP32 to tol32, no stress), stage 2 (FP64 from the FP32 iterate,
+  // warm = 2), as mgk_ls_solve_refined does for one member
   mgk_ls_info i32[MGK_MAXB] = {};
   rc = ndim == 3 ? ls_solve_batch_t<float, 3>(p32, nb, a32, warm, max_iter, tol32, s, i32, false)
                  : ls_solve_batch_t<float, 2>(p32, nb, a32, warm, max_iter, tol32, s, i32, false);
@@ -2694,15 +2714,7 @@ extern "C" int mgk_ls_solve_refined_batch(int ndim, const int64_t* shape, int nb
     }
   }
   if (n2 == 0) return rc1;
-  const size_t n = (size_t)p64.C * (size_t)p64.NV;
-  const int nw = (int)std::min<size_t>((n + MGK_NT - 1) / MGK_NT, 8192);
-  WidenBatch wb{};
-  for (int k = 0; k < n2; ++k) {
-    wb.a[k] = a32[idx[k]].eps;
-    wb.b[k] = a2[k].eps;
-  }
-  k_widen_b<<<dim3(nw, n2), MGK_NT, 0, s>>>(wb, n);
-  MGK_HIP(hipGetLastError());
+  for (int k = 0; k < n2; ++k) a2[k].eps_from32 = a32[idx[k]].eps;
   mgk_ls_info i64[MGK_MAXB] = {};
   const int rc2 = ndim == 3 ? ls_solve_batch_t<double, 3>(p64, n2, a2, 2, max_iter, tol, s, i64, true)
                             : ls_solve_batch_t<double, 2>(p64, n2, a2, 2, max_iter, tol, s, i64, true);
