@@ -24,19 +24,21 @@
 #include "mgk_secant.h"
 #include "mgk_krylov.h"
 
-// Compile-time x sweeps (tau + r2c, c2r + update): FP64 runs 512-thread
-// workgroups on tiles of 4096 points (twice the other FP64 tiles, 64 B of
-// each contiguous row per field instead of 32): tau_xfwd 0.635 -> 0.593 ms,
-// xinv_update 0.597 -> 0.581 ms at 256^3.  FP32 keeps 256 threads on 4096
-// points: its tau_xfwd needs 142 VGPRs, which 512-thread workgroups cannot
-// hold at two workgroups per CU (measured 1.3% slower).
-template <typename T>
+// Compile-time x sweeps (tau + r2c, c2r + update) run wide tiles: twice the
+// points of the other compile-time tiles on 512-thread workgroups, so each
+// contiguous row segment of a field is 64 B (FP64) / 128 B (FP32) instead of
+// 32 / 64 B.  FP64 4096 points: tau_xfwd 0.635 -> 0.593 ms, xinv_update
+// 0.597 -> 0.581 ms at 256^3.  FP32 8192 points (two workgroups per CU at
+// 128 VGPRs with the radix-16 passes): tau_xfwd -1%, xinv_update -3%.  FP32
+// grids with fewer than 8192 / nx rows along the last axis take narrow tiles
+// (4096 points, 256 threads).
+template <typename T, bool WIDE>
 constexpr int x_tile() {
-  return ct_tile<T>() * (sizeof(T) == 8 ? 2 : 1);
+  return ct_tile<T>() * (WIDE ? 2 : 1);
 }
-template <typename T>
+template <bool WIDE>
 constexpr int x_nt() {
-  return sizeof(T) == 8 ? 512 : MGK_NT;
+  return WIDE ? 512 : MGK_NT;
 }
 #include <algorithm>
 #include <cstdio>
@@ -1311,11 +1313,14 @@ int ls_plan(int ndim, const int64_t* shape, LsPlan* pl, bool allow_ct = true) {
   const int CTT = ct_tile<T>();
   auto ct_ok = [](int n) { return ct_len(n); };
   pl->ctx = pl->cty = pl->ctq = 0;
-  const int XTT = x_tile<T>();
-  if (allow_ct && ct_ok(pl->nx) && pl->Q >= XTT / pl->nx && (size_t)(pl->nx + XTT) * cb <= lim) {
-    pl->ctx = 1;
-    pl->logBx = mgk_ilog2(XTT / pl->nx);
-    pl->lds_x = (size_t)(pl->nx + XTT) * cb;
+  // x sweeps: wide tiles (ctx = 2), or narrow FP32 tiles (ctx = 1)
+  for (int wide = 1; wide >= (sizeof(T) == 4 ? 0 : 1) && !pl->ctx; --wide) {
+    const int XTT = wide ? x_tile<T, true>() : x_tile<T, false>();
+    if (allow_ct && ct_ok(pl->nx) && pl->Q >= XTT / pl->nx && (size_t)(pl->nx + XTT) * cb <= lim) {
+      pl->ctx = wide ? 2 : 1;
+      pl->logBx = mgk_ilog2(XTT / pl->nx);
+      pl->lds_x = (size_t)(pl->nx + XTT) * cb;
+    }
   }
   if (allow_ct && ct_ok(pl->P) && pl->Q >= CTT / pl->P && (size_t)(pl->P + CTT) * cb <= lim) {
     pl->cty = 1;
@@ -1553,15 +1558,32 @@ void ls_fill_info(const LsState& s, const double* eps_bar, mgk_ls_info* info) {
   }
 }
 
+// the compile-time x sweeps of length NN on wide or narrow tiles
+template <typename T, int D, int NN, bool WIDE>
+void launch_tau_ct(const LsPlan& pl, const LsGeom<T>& g, dim3 gx, hipStream_t stream, const LsBatch<T>& bt) {
+  constexpr int NT = x_nt<WIDE>(), BB = ct_cols(x_tile<T, WIDE>(), NN);
+  k_ls_tau_xfwd<T, D, NN * BB / NT, NN, BB, NT><<<gx, NT, pl.lds_x, stream>>>(g, bt);
+}
+template <typename T, int D, int NN, bool WIDE>
+void launch_xinv_ct(const LsPlan& pl, const LsGeom<T>& g, dim3 gx, hipStream_t stream, const LsBatch<T>& bt) {
+  constexpr int NT = x_nt<WIDE>(), BB = ct_cols(x_tile<T, WIDE>(), NN);
+  k_ls_xinv_update<T, D, NN * BB / NT, NN, BB, NT><<<gx, NT, pl.lds_x, stream>>>(g, bt);
+}
+template <typename T, int NN, bool WIDE>
+void launch_xfwd_real_ct(const LsPlan& pl, const LsGeom<T>& g, dim3 gx, hipStream_t stream, const T* tau,
+                         cplx<T>* spec, const LsState* st) {
+  constexpr int NT = x_nt<WIDE>(), BB = ct_cols(x_tile<T, WIDE>(), NN);
+  k_ls_xfwd_real<T, NN * BB / NT, NN, BB, NT><<<gx, NT, pl.lds_x, stream>>>(g, tau, spec, st);
+}
+
 template <typename T, int D>
 void launch_tau(const LsPlan& pl, const LsGeom<T>& g, dim3 gx, hipStream_t stream, const LsBatch<T>& bt) {
   constexpr int MV = maxv<T>();
-  constexpr int XT = x_tile<T>();
-  constexpr int X_NT = x_nt<T>();
   if (pl.ctx) {
     ct_dispatch(pl.nx, [&](auto nn) {
-      constexpr int NN = decltype(nn)::value, BB = ct_cols(XT, NN);
-      k_ls_tau_xfwd<T, D, NN * BB / X_NT, NN, BB, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt);
+      constexpr int NN = decltype(nn)::value;
+      if (pl.ctx == 2) launch_tau_ct<T, D, NN, true>(pl, g, gx, stream, bt);
+      else if constexpr (sizeof(T) == 4) launch_tau_ct<T, D, NN, false>(pl, g, gx, stream, bt);
     });
   } else {
     k_ls_tau_xfwd<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, bt);
@@ -1587,12 +1609,11 @@ void launch_pencil(const LsPlan& pl, const LsGeom<T>& g, dim3 gp, hipStream_t st
 template <typename T, int D>
 void launch_xinv(const LsPlan& pl, const LsGeom<T>& g, dim3 gx, hipStream_t stream, const LsBatch<T>& bt) {
   constexpr int MV = maxv<T>();
-  constexpr int XT = x_tile<T>();
-  constexpr int X_NT = x_nt<T>();
   if (pl.ctx) {
     ct_dispatch(pl.nx, [&](auto nn) {
-      constexpr int NN = decltype(nn)::value, BB = ct_cols(XT, NN);
-      k_ls_xinv_update<T, D, NN * BB / X_NT, NN, BB, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, bt);
+      constexpr int NN = decltype(nn)::value;
+      if (pl.ctx == 2) launch_xinv_ct<T, D, NN, true>(pl, g, gx, stream, bt);
+      else if constexpr (sizeof(T) == 4) launch_xinv_ct<T, D, NN, false>(pl, g, gx, stream, bt);
     });
   } else {
     k_ls_xinv_update<T, D, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, bt);
@@ -1604,12 +1625,11 @@ template <typename T>
 void launch_xfwd_real(const LsPlan& pl, const LsGeom<T>& g, dim3 gx, hipStream_t stream, const T* tau,
                       cplx<T>* spec, const LsState* st) {
   constexpr int MV = maxv<T>();
-  constexpr int XT = x_tile<T>();
-  constexpr int X_NT = x_nt<T>();
   if (pl.ctx) {
     ct_dispatch(pl.nx, [&](auto nn) {
-      constexpr int NN = decltype(nn)::value, BB = ct_cols(XT, NN);
-      k_ls_xfwd_real<T, NN * BB / X_NT, NN, BB, X_NT><<<gx, X_NT, pl.lds_x, stream>>>(g, tau, spec, st);
+      constexpr int NN = decltype(nn)::value;
+      if (pl.ctx == 2) launch_xfwd_real_ct<T, NN, true>(pl, g, gx, stream, tau, spec, st);
+      else if constexpr (sizeof(T) == 4) launch_xfwd_real_ct<T, NN, false>(pl, g, gx, stream, tau, spec, st);
     });
   } else {
     k_ls_xfwd_real<T, MV><<<gx, MGK_NT, pl.lds_x, stream>>>(g, tau, spec, st);
@@ -1645,9 +1665,6 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
                const L2Mode* l2 = nullptr, const SecLaw* sl = nullptr, const NoAllocCtx* na = nullptr,
                const float* eps_from32 = nullptr) {
   constexpr int MV = maxv<T>();
-  constexpr int TT = ct_tile<T>();  // points per compile-time tile
-  constexpr int XT = x_tile<T>();   // points per compile-time x-sweep tile
-  constexpr int X_NT = x_nt<T>();   // threads per x-sweep workgroup
   constexpr int C = (D == 3) ? 6 : 3;
   const int prec = sizeof(T) == 8 ? 0 : 1;
   LsGeom<T> g = ls_geom<T>(pl, C);
