@@ -36,8 +36,10 @@ int mgk_device_info(int* lds_per_block, int* cu_count);
  * counterpart in the reference; used by bench.py).  kid: 0 ls_tau_xfwd,
  * 1 ls_fft_mid, 2 ls_pencil_gamma, 3 ls_xinv_update, 4 kmc_barrier_rates,
  * 5 kmc_select, 6 fs_constitutive, 7 fs_xfwd, 8 fs_fft_mid, 9 fs_pencil_proj,
- * 10 fs_xinv, 11 fs_vector.  total_bytes: algorithmic HBM bytes of the timed
- * launches. */
+ * 10 fs_xinv, 11 fs_vector, 12 ls_tau_xfwd_start, 13 ls_xinv_update_start (the
+ * first LS iteration when it reads its starting field as a constant or in FP32;
+ * 0 and 3 then time the later iterations).  total_bytes: algorithmic HBM bytes
+ * of the timed launches. */
 /* Streaming device copy of `bytes` (a multiple of 16, 16-byte aligned pointers) on
  * `stream`: the practical HBM read + write ceiling that bench.py reports beside the
  * kernels' roofline (no counterpart in the reference). */

@@ -289,7 +289,8 @@ def host_dbl(a):
 # kernel ids of the live timing counters (mgk_host.h MgkKid)
 KID = {"ls_tau_xfwd": 0, "ls_fft_mid": 1, "ls_pencil_gamma": 2, "ls_xinv_update": 3,
        "kmc_barrier_rates": 4, "kmc_select": 5, "fs_constitutive": 6, "fs_xfwd": 7, "fs_fft_mid": 8,
-       "fs_pencil_proj": 9, "fs_xinv": 10, "fs_vector": 11}
+       "fs_pencil_proj": 9, "fs_xinv": 10, "fs_vector": 11, "ls_tau_xfwd_start": 12,
+       "ls_xinv_update_start": 13}
 
 
 def profile_read():

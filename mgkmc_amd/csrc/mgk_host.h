@@ -97,6 +97,10 @@ enum MgkKid {
   KID_FS_PENCIL,
   KID_FS_XINV,
   KID_FS_VEC,
+  // first LS iteration when it reads its starting field as a constant or in
+  // FP32 (fewer bytes than the steady-state launches above)
+  KID_LS_TAU_START,
+  KID_LS_UPDATE_START,
   KID_COUNT = 16
 };
 

@@ -95,33 +95,51 @@ struct LsMem {
   Sym6 eb;
   double lam0, mu0, e2_extra;
   T* sig;  // final stress (batched solves), or NULL
-  // FP64 solves started from an FP32 iterate: the first iteration's x sweeps
-  // read the starting field from here (exact widening) instead of eps, so no
-  // widened copy is written first; NULL otherwise
+  // The first iteration's x sweeps read the starting field from elsewhere,
+  // so it is never written out first:
+  // eps_old32: FP64 solves started from an FP32 iterate read it here (exact
+  // widening); NULL otherwise.
+  // eps_uniform: cold starts of the linear law, eps = eps_bar everywhere,
+  // read as that constant; 0 otherwise.
   const float* eps_old32;
+  int eps_uniform;
 };
 
-// eps reads of the x sweeps, from the FP32 starting field when one is given
+// eps reads of the x sweeps: component c at voxel v, from the FP32 starting
+// field or the uniform starting value when the member has one
 template <typename T>
-__device__ __forceinline__ void eps_vload(const T* __restrict__ eps, const float* __restrict__ e32, size_t off,
-                                          T (&o)[16 / sizeof(T)]) {
-  if constexpr (sizeof(T) == 8) {
-    if (e32) {
-      const float2 v = *reinterpret_cast<const float2*>(e32 + off);
-      o[0] = (T)v.x;
-      o[1] = (T)v.y;
+struct EpsRead {
+  const T* eps;
+  const float* e32;
+  Sym6 eb;  // the member's eps_bar components
+  size_t NV;
+  bool uni;
+  __device__ __forceinline__ void vload(int c, size_t v, T (&o)[16 / sizeof(T)]) const {
+    if (uni) {
+#pragma unroll
+      for (int i = 0; i < 16 / (int)sizeof(T); ++i) o[i] = T(eb.v[c]);
       return;
     }
+    const size_t off = (size_t)c * NV + v;
+    if constexpr (sizeof(T) == 8) {
+      if (e32) {
+        const float2 w = *reinterpret_cast<const float2*>(e32 + off);
+        o[0] = (T)w.x;
+        o[1] = (T)w.y;
+        return;
+      }
+    }
+    vload16(eps + off, o);
   }
-  vload16(eps + off, o);
-}
-template <typename T>
-__device__ __forceinline__ T eps_ld(const T* __restrict__ eps, const float* __restrict__ e32, size_t off) {
-  if constexpr (sizeof(T) == 8) {
-    if (e32) return (T)e32[off];
+  __device__ __forceinline__ T ld(int c, size_t v) const {
+    if (uni) return T(eb.v[c]);
+    const size_t off = (size_t)c * NV + v;
+    if constexpr (sizeof(T) == 8) {
+      if (e32) return (T)e32[off];
+    }
+    return eps[off];
   }
-  return eps[off];
-}
+};
 
 template <typename T>
 struct LsBatch {
@@ -208,8 +226,6 @@ template <typename T, int D, int MAXV, int NN = 0, int BB = 0, int NTH = MGK_NT>
 __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 ? (sizeof(T) == 4 ? 4 : 2) : (sizeof(T) == 4 ? 3 : 2)))) void k_ls_tau_xfwd(LsGeom<T> g, LsBatch<T> bt) {
   const LsMem<T>& M = bt.m[blockIdx.z];
   if (M.st->done) return;
-  const T* __restrict__ eps = M.eps;
-  const float* __restrict__ e32 = sizeof(T) == 8 ? M.eps_old32 : nullptr;
   const T* __restrict__ epsp = M.epsp;
   const T* __restrict__ lam = M.lam;
   const T* __restrict__ mu = M.mu;
@@ -225,6 +241,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
   const int q0 = (tid_lin % gridDim.x) * B, p = tid_lin / gridDim.x;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;  // spectrum component stride
+  const EpsRead<T> er{M.eps, sizeof(T) == 8 ? M.eps_old32 : nullptr, M.eb, NV, M.eps_uniform != 0};
   for (int t = threadIdx.x; t < nx; t += NTH) tw[t] = g.twx[t];
   const Tile tl{nx, logB, ldb, 1, 0};
   const T m02 = T(2) * mu0;
@@ -245,11 +262,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
       const int bg = e % GPR, x = e / GPR;
       const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
       T exx[VW], eyy[VW], ezz[VW], pxx[VW], pyy[VW], pzz[VW], l[VW], m[VW];
-      eps_vload(eps, e32, v, exx);
-      eps_vload(eps, e32, NV + v, eyy);
+      er.vload(0, v, exx);
+      er.vload(1, v, eyy);
       vload16(lam + v, l);
       vload16(mu + v, m);
-      if (D == 3) eps_vload(eps, e32, 2 * NV + v, ezz);
+      if (D == 3) er.vload(2, v, ezz);
       if (epsp) {
         vload16(epsp + v, pxx);
         vload16(epsp + NV + v, pyy);
@@ -292,13 +309,13 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
         T txx = T(0), tyy = T(0);
         if (q < Q) {
           const size_t v = ((size_t)x * P + p) * Q + q;
-          const T exx = eps_ld(eps, e32, v), eyy = eps_ld(eps, e32, NV + v);
+          const T exx = er.ld(0, v), eyy = er.ld(1, v);
           const T pxx = epsp ? epsp[v] : T(0), pyy = epsp ? epsp[NV + v] : T(0);
           const T l = lam[v], m = mu[v];
           const T dxx = exx - pxx, dyy = eyy - pyy;
           T tre, tr0, ezz = T(0), dzz = T(0);
           if (D == 3) {
-            ezz = eps_ld(eps, e32, 2 * NV + v);
+            ezz = er.ld(2, v);
             const T pzz = epsp ? epsp[2 * NV + v] : T(0);
             dzz = ezz - pzz;
             tre = (dxx + dyy) + dzz;
@@ -340,11 +357,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
         const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
         T ea[VW], pa[VW], eb[VW], pb[VW];
         if (!(D == 3 && pr == 0)) {
-          eps_vload(eps, e32, (size_t)ca * NV + v, ea);
+          er.vload(ca, v, ea);
           if (epsp) vload16(epsp + (size_t)ca * NV + v, pa);
         }
         if (cb >= 0) {
-          eps_vload(eps, e32, (size_t)cb * NV + v, eb);
+          er.vload(cb, v, eb);
           if (epsp) vload16(epsp + (size_t)cb * NV + v, pb);
         }
 #pragma unroll
@@ -370,12 +387,12 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
             if (D == 3 && pr == 0) {
               ta = keep[s];
             } else {
-              const T ea = eps_ld(eps, e32, (size_t)ca * NV + v);
+              const T ea = er.ld(ca, v);
               const T pa = epsp ? epsp[(size_t)ca * NV + v] : T(0);
               ta = m2 * (ea - pa) - m02 * ea;
             }
             if (cb >= 0) {
-              const T eb = eps_ld(eps, e32, (size_t)cb * NV + v);
+              const T eb = er.ld(cb, v);
               const T pb = epsp ? epsp[(size_t)cb * NV + v] : T(0);
               tb = m2 * (eb - pb) - m02 * eb;
             }
@@ -525,7 +542,6 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
   if (M.st->done) return;
   const cplx<T>* __restrict__ spec = M.spec;
   T* __restrict__ eps = M.eps;
-  const float* __restrict__ e32 = sizeof(T) == 8 ? M.eps_old32 : nullptr;
   const Sym6 ebar = M.eb;
   double* __restrict__ partials = M.part;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -538,6 +554,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
   const int q0 = (tid_lin % gridDim.x) * B, p = tid_lin / gridDim.x;
   const size_t NV = (size_t)g.NV;
   const size_t cs = (size_t)g.nxh * P * Q;
+  const EpsRead<T> er{eps, sizeof(T) == 8 ? M.eps_old32 : nullptr, ebar, NV, M.eps_uniform != 0};
   for (int t = threadIdx.x; t < nx; t += NTH) tw[t] = g.twx[t];
   const Tile tl{nx, logB, ldb, 1, 0};
   double acc[2] = {0.0, 0.0};
@@ -574,8 +591,8 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
         const int bg = e % GPR, x = e / GPR;
         const size_t v = ((size_t)x * P + p) * Q + q0 + bg * VW;
         T oa[VW], na[VW], ob[VW], nb[VW];
-        eps_vload(eps, e32, (size_t)ca * NV + v, oa);
-        if (cb >= 0) eps_vload(eps, e32, (size_t)cb * NV + v, ob);
+        er.vload(ca, v, oa);
+        if (cb >= 0) er.vload(cb, v, ob);
 #pragma unroll
         for (int i = 0; i < VW; ++i) {
           const cplx<T> z = tile[x * ldb + bg * VW + i];
@@ -606,7 +623,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
           const cplx<T> z = tile[x * ldb + b];
           {
             const T nw = eba + z.x;
-            const T od = eps_ld(eps, e32, (size_t)ca * NV + v);
+            const T od = er.ld(ca, v);
             const double dd = (double)nw - (double)od;
             acc[0] += wa * dd * dd;
             acc[1] += wa * (double)od * (double)od;
@@ -614,7 +631,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(NTH == 512 
           }
           if (cb >= 0) {
             const T nw = ebb + z.y;
-            const T od = eps_ld(eps, e32, (size_t)cb * NV + v);
+            const T od = er.ld(cb, v);
             const double dd = (double)nw - (double)od;
             acc[0] += wb * dd * dd;
             acc[1] += wb * (double)od * (double)od;
@@ -1214,11 +1231,9 @@ __global__ void k_zero_words(unsigned int* __restrict__ p, size_t n) {
 // single pass's body on its own buffers.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_ls_init_b(LsBatch<T> bt, int C, size_t NV, int fill) {
+__global__ void k_ls_init_b(LsBatch<T> bt) {
   const LsMem<T>& M = bt.m[blockIdx.y];
-  if (blockIdx.x == 0)
-    for (int i = threadIdx.x; i < (int)(sizeof(LsState) / 4); i += blockDim.x) ((unsigned int*)M.st)[i] = 0u;
-  if (fill) ls_fill_body<T>(M.eps, C, NV, M.eb);
+  for (int i = threadIdx.x; i < (int)(sizeof(LsState) / 4); i += blockDim.x) ((unsigned int*)M.st)[i] = 0u;
 }
 
 template <typename T>
@@ -1694,7 +1709,11 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   k_zero_words<<<1, 64, 0, stream>>>((unsigned int*)st, sizeof(LsState) / 4);
   if (l2) k_l2_init<<<1, 1, 0, stream>>>(st, l2->persist && l2->e33_have ? 1 : 0);
   const int nfill = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 4096);
-  if (!warm) {
+  // cold starts of the linear law read eps = eps_bar as a constant in the
+  // first iteration (nothing else reads eps before that iteration's update
+  // has written it); the nonlinear laws read eps in their own kernels
+  const bool uniform_start = !warm && !lp && !sl && !l2;
+  if (!warm && !uniform_start) {
     k_ls_fill<T><<<nfill, MGK_NT, 0, stream>>>(eps, C, NV, eb);
   } else if (warm == 1) {
     k_comp_sums<T><<<pl.nblk_red, MGK_NT, 0, stream>>>(eps, C, NV, part);
@@ -1718,6 +1737,7 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
     return MGK_ERR_ARG;
   }
   bt.m[0].eps_old32 = eps_from32;
+  bt.m[0].eps_uniform = uniform_start ? 1 : 0;
   const MidPtrs<T> mp = mid_table(bt, 1, C);
   const LsFin fin = fin_table(bt, 1);
 
@@ -1728,6 +1748,11 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
   const double b_tau = fieldB * ((epsp ? 2.0 * C : (double)C) + 2.0) + specB;
   const double b_fft = 2.0 * specB;
   const double b_upd = specB + 2.0 * C * fieldB;
+  // the first iteration reads its starting field as a constant (uniform
+  // start) or in FP32 (start from an FP32 iterate) in both x sweeps
+  const double eps_in = (double)C * fieldB;
+  const double eps_in0 = uniform_start ? 0.0 : (eps_from32 ? (double)C * pl.NV * 4.0 : eps_in);
+  const bool light0 = uniform_start || eps_from32;  // timed under the _START ids
   MgkProfRec prof(stream, na == nullptr);
   int iter_idx = 0;
   const int nel = (int)std::min<long long>((pl.NV + MGK_NT - 1) / MGK_NT, 8192);
@@ -1754,7 +1779,8 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
       }
       launch_xfwd_real<T>(pl, g, gx, stream, taubuf, spec, st);
     } else {
-    h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
+    if (iter_idx == 0 && light0) h = prof.begin(KID_LS_TAU_START, iter_idx, b_tau - eps_in + eps_in0);
+    else h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
     launch_tau<T, D>(pl, g, gx, stream, bt);
     prof.end(h);
     }
@@ -1775,10 +1801,12 @@ int ls_solve_t(const LsPlan& pl, const T* lam, const T* mu, double lam0, double 
       if (l2 && l2->aa_m > 0)
         k_copy<<<nel, MGK_NT, 0, stream>>>(eps, l2->eps_old, (size_t)C * NV, st);
     }
-    h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
+    if (iter_idx == 0 && light0) h = prof.begin(KID_LS_UPDATE_START, iter_idx, b_upd - eps_in + eps_in0);
+    else h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
     launch_xinv<T, D>(pl, g, gx, stream, bt);
     prof.end(h);
     bt.m[0].eps_old32 = nullptr;  // eps now holds the first iterate
+    bt.m[0].eps_uniform = 0;
     k_ls_finalize<<<1, MGK_NT, 0, stream>>>(fin, pl.nblk_upd, tol, max_iter);
     if constexpr (sizeof(T) == 8) {
       if (l2 && l2->aa_m > 0) {
@@ -1940,11 +1968,12 @@ int ls_solve_batch_t(const LsPlan& pl, int nb, const LsMemArgs<T>* a, int warm, 
     const Sym6 eb = ls_sym_bar<D>(a[b].eps_bar, &anti2);
     bt.m[b] = LsMem<T>{a[b].lam, a[b].mu, a[b].epsp, a[b].eps, (cplx<T>*)(a[b].ws + pl.off_spec),
                        (double*)(a[b].ws + pl.off_part), (LsState*)(a[b].ws + pl.off_state), eb, a[b].lam0,
-                       a[b].mu0, anti2 * (double)pl.NV, a[b].sig, a[b].eps_from32};
+                       a[b].mu0, anti2 * (double)pl.NV, a[b].sig, a[b].eps_from32, warm == 0 ? 1 : 0};
     hint = std::max(hint, ls_iter_hint(a[b].ws, 0));
   }
-  // set-up: state cleared, eps filled with eps_bar (cold) or shifted to mean eps_bar (warm 1)
-  k_ls_init_b<T><<<dim3(nfill, nb), MGK_NT, 0, stream>>>(bt, C, NV, warm == 0 ? 1 : 0);
+  // set-up: state cleared, eps shifted to mean eps_bar (warm 1); a cold start
+  // reads eps = eps_bar as a constant in the first iteration (eps_uniform)
+  k_ls_init_b<T><<<dim3(1, nb), 64, 0, stream>>>(bt);
   if (warm == 1) {
     k_comp_sums_b<T><<<dim3(pl.nblk_red, nb), MGK_NT, 0, stream>>>(bt, C, NV);
     k_means_finalize_b<T><<<nb, MGK_NT, 0, stream>>>(bt, pl.nblk_red, C, (double)NV);
@@ -1966,10 +1995,17 @@ int ls_solve_batch_t(const LsPlan& pl, int nb, const LsMemArgs<T>* a, int warm, 
   const double b_tau = nb * (fieldB * ((a[0].epsp ? 2.0 * C : (double)C) + 2.0) + specB);
   const double b_fft = nb * 2.0 * specB;
   const double b_upd = nb * (specB + 2.0 * C * fieldB);
+  // first iteration: starting fields read as constants (cold) or in FP32
+  double eps_in0 = 0.0;
+  for (int b = 0; b < nb; ++b)
+    eps_in0 += warm == 0 ? 0.0 : (a[b].eps_from32 ? (double)C * pl.NV * 4.0 : (double)C * fieldB);
+  const double eps_in = nb * (double)C * fieldB;
+  const bool light0 = eps_in0 < eps_in;  // timed under the _START ids
   MgkProfRec prof(stream);
   int iter_idx = 0;
   auto enqueue_iter = [&]() {
-    int h = prof.begin(KID_LS_TAU, iter_idx, b_tau);
+    int h = (iter_idx == 0 && light0) ? prof.begin(KID_LS_TAU_START, iter_idx, b_tau - eps_in + eps_in0)
+                                      : prof.begin(KID_LS_TAU, iter_idx, b_tau);
     launch_tau<T, D>(pl, g, gx, stream, bt);
     prof.end(h);
     if (D == 3 && pl.P > 1) {
@@ -1985,10 +2021,14 @@ int ls_solve_batch_t(const LsPlan& pl, int nb, const LsMemArgs<T>* a, int warm, 
       launch_mid<T, MV>(pl, gy, stream, g.ply, g.twy, mp, 1);
       prof.end(h);
     }
-    h = prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
+    h = (iter_idx == 0 && light0) ? prof.begin(KID_LS_UPDATE_START, iter_idx, b_upd - eps_in + eps_in0)
+                                  : prof.begin(KID_LS_UPDATE, iter_idx, b_upd);
     launch_xinv<T, D>(pl, g, gx, stream, bt);
     prof.end(h);
-    for (int b = 0; b < nb; ++b) bt.m[b].eps_old32 = nullptr;  // eps now holds the first iterate
+    for (int b = 0; b < nb; ++b) {  // eps now holds the first iterate
+      bt.m[b].eps_old32 = nullptr;
+      bt.m[b].eps_uniform = 0;
+    }
     k_ls_finalize<<<nb, MGK_NT, 0, stream>>>(fin, pl.nblk_upd, tol, max_iter);
     ++iter_idx;
   };
