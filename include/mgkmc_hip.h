@@ -609,6 +609,77 @@ int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* loop, mgk_mt_state* rng, mgk_m
                       mgk_kmc_event* events, int cap, int* n_events, int* status, mgk_ls_info* info, int* iters32,
                       void* stream);
 
+/* One finite-strain load step as data: which of mgk_fs_solver_step (precision 0/1),
+ * mgk_fs_solver_step_refined (mixed, solver 0), mgk_fs_solver_step_al and
+ * mgk_fs_solver_step_dbfft runs, with their arguments (grid, fields, material,
+ * tolerances, workspaces).  Plane problems use their (nx, 1, ny) grid. */
+typedef struct mgk_fs_stepper {
+  int solver;     /* 0 newton_cg, 1 al, 2 dbfft */
+  int precision;  /* 0 FP64, 1 FP32 */
+  int mixed;      /* newton_cg: FP64 Newton loop with FP32 Krylov solves */
+  int64_t shape[3];
+  void* F;
+  void* Fp;
+  const void* lam;
+  const void* mu;
+  const void* lam32;
+  const void* mu32;
+  const mgk_fs_material* mat;
+  void* P;
+  void* sig;
+  void* ws;
+  size_t ws_bytes;
+  void* ws32;
+  size_t ws32_bytes;
+  const double* L0; /* al: the reference medium (3 doubles) */
+  double E_avg, nu_avg, tol_NW, tol_CG, tol_macro, pixel;
+  int max_NW, max_iter_macro;
+} mgk_fs_stepper;
+
+/* What mgk_kmc_fs_flip needs besides the flips.  kmc: the catalog, softening, event
+ * times, redraw settings and redraw buffer (its LS fields are not read; planar
+ * catalogs).  Macro BC of every step (build_finite_strain_bc_3d, mgkmc/
+ * finite_strain_simulator.py:2852-2899): F_bar = F_macro with the driving component
+ * (drv_i, drv_j, d-dimensional) set from eps_macro, the other off-diagonals 0, and
+ * each diagonal either a stress target (target_k / target_val, P_kk) or 1.  live /
+ * live_bytes: the device state restored before every sub-stepping attempt (F, P, sig,
+ * F_p, softening, event times, prev, catalog, barriers), saved into `backup` (at least
+ * their total size, each at a 256-byte aligned offset).  sig2: plane problems' in-plane
+ * Cauchy block (4 x NV: xx, xy, yx, yy) refreshed after every step (NULL in 3D).  flag:
+ * one device int (the NaN check). */
+typedef struct mgk_kmc_fs_substep {
+  const mgk_kmc_loop* kmc;
+  const mgk_fs_stepper* step;
+  int drv_i, drv_j;
+  int n_targets;
+  int target_k[3];
+  double target_val[3];
+  int nlive;
+  void* live[12];
+  size_t live_bytes[12];
+  void* backup;
+  size_t backup_bytes;
+  double* sig2;
+  int* flag;
+} mgk_kmc_fs_substep;
+
+/* The finite-strain flips of one KMC event or one cascade batch with the reference's
+ * sub-stepping (mgkmc/aqs.py:512-615,887-996; 2D mgkmc/kmc_simulator.py:605-700,
+ * 895-1000): F_p <- expm(C/N) F_p at every flipped voxel (the in-plane block in 2D) in N
+ * sub-steps, softening, prev and the redraw of each voxel at the last sub-step, one
+ * finite-strain step after every sub-step; an attempt whose step fails to converge or
+ * leaves a NaN in sig or F or a mean stress above 20 GPa is undone (live state, F_macro,
+ * eps_macro) and retried with the next N: 3D N = n0 .. n0 + 19, n0 = min(max over the
+ * flips of max(2, ceil(sqrt(C:C/2) / 0.005)), 40); 2D N = 1, then max(n0, 2) .. + 19
+ * with the step 0.02.  Redraws are drawn from `rng` anew at every attempt.  F_macro and
+ * eps_macro (d*d) follow the steps; *substeps = the N that succeeded; info = the last
+ * step's.  Returns MGK_OK, MGK_ERR_SOLVER when every N failed ("Mechanical solver failed
+ * to converge under sub-stepping."), MGK_ERR_NONFINITE at once when a plane-stress E33
+ * iteration overflows (as the reference's FloatingPointError, not retried). */
+int mgk_kmc_fs_flip(const mgk_kmc_fs_substep* cfg, int nflips, const long long* voxels, const int* modes,
+                    double time, mgk_mt_state* rng, double* F_macro, double* eps_macro, int* substeps,
+                    mgk_fs_info* info, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

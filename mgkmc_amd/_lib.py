@@ -141,6 +141,26 @@ class MgkFsInfo(ctypes.Structure):
                 ("F_mean", ctypes.c_double * 9), ("sig_mean", ctypes.c_double * 9), ("F_bar", ctypes.c_double * 9)]
 
 
+class MgkFsStepper(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int), ("precision", ctypes.c_int), ("mixed", ctypes.c_int),
+                ("shape", ctypes.c_int64 * 3), ("F", ctypes.c_void_p), ("Fp", ctypes.c_void_p),
+                ("lam", ctypes.c_void_p), ("mu", ctypes.c_void_p), ("lam32", ctypes.c_void_p),
+                ("mu32", ctypes.c_void_p), ("mat", ctypes.c_void_p), ("P", ctypes.c_void_p),
+                ("sig", ctypes.c_void_p), ("ws", ctypes.c_void_p), ("ws_bytes", ctypes.c_size_t),
+                ("ws32", ctypes.c_void_p), ("ws32_bytes", ctypes.c_size_t), ("L0", ctypes.c_void_p),
+                ("E_avg", ctypes.c_double), ("nu_avg", ctypes.c_double), ("tol_NW", ctypes.c_double),
+                ("tol_CG", ctypes.c_double), ("tol_macro", ctypes.c_double), ("pixel", ctypes.c_double),
+                ("max_NW", ctypes.c_int), ("max_iter_macro", ctypes.c_int)]
+
+
+class MgkKmcFsSubstep(ctypes.Structure):
+    _fields_ = [("kmc", ctypes.c_void_p), ("step", ctypes.c_void_p), ("drv_i", ctypes.c_int),
+                ("drv_j", ctypes.c_int), ("n_targets", ctypes.c_int), ("target_k", ctypes.c_int * 3),
+                ("target_val", ctypes.c_double * 3), ("nlive", ctypes.c_int), ("live", ctypes.c_void_p * 12),
+                ("live_bytes", ctypes.c_size_t * 12), ("backup", ctypes.c_void_p), ("backup_bytes", ctypes.c_size_t),
+                ("sig2", ctypes.c_void_p), ("flag", ctypes.c_void_p)]
+
+
 _c_p = ctypes.c_void_p
 _c_i = ctypes.c_int
 _c_d = ctypes.c_double
@@ -184,6 +204,9 @@ SIGNATURES = {
     "mgk_heat_workspace_bytes": (_c_i, [_c_i, _c_i64p, ctypes.POINTER(_c_sz)]),
     "mgk_heat_step": (_c_i, [_c_i, _c_i64p, _c_p, _c_dp, _c_d, _c_d, _c_i, _c_d, _c_d, _c_p, _c_sz, _c_p]),
     "mgk_mt_fill": (_c_i, [ctypes.POINTER(MgkMtState), _c_i, _c_d, _c_d, ctypes.c_longlong, _c_dp]),
+    "mgk_kmc_fs_flip": (_c_i, [ctypes.POINTER(MgkKmcFsSubstep), _c_i, ctypes.POINTER(ctypes.c_longlong),
+                               ctypes.POINTER(_c_i), _c_d, ctypes.POINTER(MgkMtState), _c_dp, _c_dp,
+                               ctypes.POINTER(_c_i), ctypes.POINTER(MgkFsInfo), _c_p]),
     "mgk_kmc_mixed_bc": (_c_i, [ctypes.POINTER(MgkKmcLoop), _c_i, ctypes.POINTER(_c_i), _c_dp, _c_i, _c_i, _c_d,
                                 _c_d, _c_d, _c_i, _c_dp, ctypes.POINTER(_c_i), _c_dp, ctypes.POINTER(_c_i),
                                 ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i), _c_p]),

@@ -76,6 +76,13 @@ int mgk_kmc_patch_mode(const mgk_kmc_loop* L, const mgk_kmc_patch_cfg* P, long l
                        int with_unit, double* sig_mean, hipStream_t stream);
 int mgk_kmc_patch_unit(const mgk_kmc_loop* L, const mgk_kmc_patch_cfg* P, double a, double* sig_mean,
                        hipStream_t stream);
+
+// Native finite-strain flip pieces (mgk_fsflip.hip): F_p <- dF F_p at voxel v
+// (d = 3 all nine components, d = 2 the in-plane block of the (nx, 1, ny)
+// grid; dF d x d row-major), and the NaN test of two n-long FP64 fields
+// (*host = 1 on a NaN; synchronises the stream).
+int mgk_fp_left_mul(double* Fp, long long NV, long long v, int d, const double* dF, hipStream_t s);
+int mgk_any_nan2(const double* a, const double* b, size_t n, int* flag_dev, int* host, hipStream_t s);
 int mgk_kmc_flip_redraw_levels(const mgk_kmc_loop* L, int n, int nlev, const int* dloff, const long long* dlv,
                                const int* dlm, const long long* dv, double time, const double* dred, size_t stride,
                                hipStream_t s);

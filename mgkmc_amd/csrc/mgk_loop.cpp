@@ -943,3 +943,293 @@ extern "C" int mgk_kmc_mixed_bc(const mgk_kmc_loop* L, int n_targets, const int*
   if (iters32) *iters32 = it32;
   return MGK_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Native finite-strain flip: the sub-stepped plastic flip of one event or one
+// cascade batch (mgkmc/aqs.py:897-996, mgkmc/kmc_simulator.py:905-1000).
+// ---------------------------------------------------------------------------
+namespace {
+
+// expm of a d x d matrix (d <= 3): scaling by 2^-s until the 1-norm is at most
+// 1/4, the Taylor polynomial of degree 18 (Horner), then s squarings.  The
+// truncation error is below 1e-25 of the norm, so the result is exact to
+// rounding; the flips' increments C/N have norms of about 0.005.
+void expm_small(const double* A, int d, double* E) {
+  double nrm = 0.0;
+  for (int j = 0; j < d; ++j) {
+    double c = 0.0;
+    for (int i = 0; i < d; ++i) c += std::fabs(A[i * d + j]);
+    nrm = std::fmax(nrm, c);
+  }
+  int s = 0;
+  while (nrm > 0.25 && s < 60) {
+    nrm *= 0.5;
+    ++s;
+  }
+  const double sc = std::ldexp(1.0, -s);
+  double B[9], P[9], T[9];
+  for (int k = 0; k < d * d; ++k) B[k] = A[k] * sc;
+  // Horner: P = I + B/1 (I + B/2 (I + ... (I + B/18)))
+  for (int k = 0; k < d * d; ++k) P[k] = (k % (d + 1) == 0) ? 1.0 : 0.0;
+  for (int n = 18; n >= 1; --n) {
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j) {
+        double acc = 0.0;
+        for (int k = 0; k < d; ++k) acc += B[i * d + k] * P[k * d + j];
+        T[i * d + j] = acc / n + (i == j ? 1.0 : 0.0);
+      }
+    std::memcpy(P, T, sizeof(double) * d * d);
+  }
+  for (int q = 0; q < s; ++q) {
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j) {
+        double acc = 0.0;
+        for (int k = 0; k < d; ++k) acc += P[i * d + k] * P[k * d + j];
+        T[i * d + j] = acc;
+      }
+    std::memcpy(P, T, sizeof(double) * d * d);
+  }
+  std::memcpy(E, P, sizeof(double) * d * d);
+}
+
+// one finite-strain step with the stepper's solver
+int fs_step_once(const mgk_fs_stepper* st, const double* Fb, const double* Pt, const int* Pm, int i3, int j3,
+                 void* stream, mgk_fs_info* info) {
+  if (st->solver == 2)
+    return mgk_fs_solver_step_dbfft(st->shape, st->precision, st->F, st->Fp, st->lam, st->mu, st->mat, Fb, Pt, Pm,
+                                    i3, j3, st->E_avg, st->nu_avg, st->tol_NW, st->tol_CG, st->max_NW,
+                                    st->tol_macro, st->max_iter_macro, st->pixel, st->P, st->sig, st->ws,
+                                    st->ws_bytes, stream, info);
+  if (st->solver == 1)
+    return mgk_fs_solver_step_al(st->shape, st->precision, st->F, st->Fp, st->lam, st->mu, st->mat, Fb, Pt, Pm, i3,
+                                 j3, st->E_avg, st->nu_avg, st->tol_NW, st->max_NW, st->tol_macro,
+                                 st->max_iter_macro, st->L0, st->P, st->sig, st->ws, st->ws_bytes, stream, info);
+  if (st->mixed)
+    return mgk_fs_solver_step_refined(st->shape, st->F, st->Fp, st->lam, st->mu, st->lam32, st->mu32, st->mat, Fb,
+                                      Pt, Pm, i3, j3, st->E_avg, st->nu_avg, st->tol_NW, st->tol_CG, st->max_NW,
+                                      st->tol_macro, st->max_iter_macro, st->P, st->sig, st->ws, st->ws_bytes,
+                                      st->ws32, st->ws32_bytes, stream, info);
+  return mgk_fs_solver_step(st->shape, st->precision, st->F, st->Fp, st->lam, st->mu, st->mat, Fb, Pt, Pm, i3, j3,
+                            st->E_avg, st->nu_avg, st->tol_NW, st->tol_CG, st->max_NW, st->tol_macro,
+                            st->max_iter_macro, st->P, st->sig, st->ws, st->ws_bytes, stream, info);
+}
+
+const int kPlaneAx[2] = {0, 2};  // in-plane axes of the (nx, 1, ny) grid
+
+// plane problems: the in-plane Cauchy block (xx, xy, yx, yy) of the 3 x 3 field
+int fs_sig2_sync(const mgk_kmc_fs_substep* c, long long NV, hipStream_t s) {
+  if (!c->sig2) return MGK_OK;
+  const double* sig = (const double*)c->step->sig;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      MGK_HIP(hipMemcpyAsync(c->sig2 + (size_t)(2 * a + b) * NV, sig + (size_t)(3 * kPlaneAx[a] + kPlaneAx[b]) * NV,
+                             sizeof(double) * NV, hipMemcpyDeviceToDevice, s));
+  return MGK_OK;
+}
+
+// update_stresses of the finite-strain KMC state (mgkmc/aqs.py:276-330; 2D
+// mgkmc/kmc_simulator.py:389-445): the step's macro BC, one step, F_macro and
+// eps_macro from its F_bar, then the stability test.  0: done; 1: the
+// reference's ValueError (no convergence, a NaN field or a mean stress above
+// 20 GPa); < 0: an error to pass on.
+int fs_update(const mgk_kmc_fs_substep* c, int d, long long NV, double* F_macro, double* eps_macro, int* host_flag,
+              hipStream_t s, mgk_fs_info* info) {
+  const int i = c->drv_i, j = c->drv_j;
+  double Fb[9] = {0}, Pt[9] = {0};
+  int Pm[9] = {0};
+  for (int k = 0; k < d * d; ++k) Fb[k] = F_macro[k];
+  Fb[i * d + j] = (i == j ? 1.0 : 0.0) + eps_macro[i * d + j];
+  for (int a = 0; a < d; ++a)
+    for (int b = 0; b < d; ++b)
+      if (a != b && !(a == i && b == j)) Fb[a * d + b] = 0.0;
+  for (int k = 0; k < d; ++k) {
+    if (k == i && i == j) continue;
+    int t = -1;
+    for (int q = 0; q < c->n_targets; ++q)
+      if (c->target_k[q] == k) t = q;
+    if (t >= 0) {
+      Pt[k * d + k] = c->target_val[t];
+      Pm[k * d + k] = 1;
+    } else {
+      Fb[k * d + k] = 1.0;
+    }
+  }
+  double Fb3[9] = {0}, Pt3[9] = {0};
+  int Pm3[9] = {0}, i3 = i, j3 = j;
+  if (d == 3) {
+    for (int k = 0; k < 9; ++k) {
+      Fb3[k] = Fb[k];
+      Pt3[k] = Pt[k];
+      Pm3[k] = Pm[k];
+    }
+  } else {  // the plane problem: in-plane block, out-of-plane F 1, P and mask 0
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) {
+        const int q = 3 * kPlaneAx[a] + kPlaneAx[b];
+        Fb3[q] = Fb[2 * a + b];
+        Pt3[q] = Pt[2 * a + b];
+        Pm3[q] = Pm[2 * a + b];
+      }
+    Fb3[4] = 1.0;
+    i3 = kPlaneAx[i];
+    j3 = kPlaneAx[j];
+  }
+  const int rc = fs_step_once(c->step, Fb3, Pt3, Pm3, i3, j3, s, info);
+  if (rc == MGK_ERR_SOLVER) return 1;
+  if (rc != MGK_OK) return rc;
+  double Fm[9], sm[9];
+  for (int a = 0; a < d; ++a)
+    for (int b = 0; b < d; ++b) {
+      const int q = d == 3 ? 3 * a + b : 3 * kPlaneAx[a] + kPlaneAx[b];
+      Fm[a * d + b] = info->F_bar[q];
+      sm[a * d + b] = info->sig_mean[q];
+    }
+  int e = fs_sig2_sync(c, NV, s);
+  if (e != MGK_OK) return e;
+  for (int a = 0; a < d; ++a)
+    for (int b = 0; b < d; ++b) {
+      F_macro[a * d + b] = Fm[a * d + b];
+      eps_macro[a * d + b] = a == b ? Fm[a * d + b] - 1.0 : Fm[a * d + b];
+    }
+  const long long NV3 = c->step->shape[0] * c->step->shape[1] * c->step->shape[2];
+  e = mgk_any_nan2((const double*)c->step->sig, (const double*)c->step->F, 9 * (size_t)NV3, c->flag, host_flag, s);
+  if (e != MGK_OK) return e;
+  // np.max(np.abs(sig_mean)) propagates a NaN, and a NaN mean fails no comparison
+  bool nan_mean = false;
+  double mx = 0.0;
+  for (int k = 0; k < d * d; ++k) {
+    if (std::isnan(sm[k])) nan_mean = true;
+    mx = std::fmax(mx, std::fabs(sm[k]));
+  }
+  if (*host_flag || (!nan_mean && mx / 1e9 > 20.0)) return 1;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int mgk_kmc_fs_flip(const mgk_kmc_fs_substep* c, int nflips, const long long* voxels, const int* modes,
+                               double time, mgk_mt_state* rng, double* F_macro, double* eps_macro, int* substeps,
+                               mgk_fs_info* info, void* stream) {
+  const mgk_kmc_loop* L = c ? c->kmc : nullptr;
+  const mgk_fs_stepper* st = c ? c->step : nullptr;
+  if (!c || !L || !st || nflips < 1 || !voxels || !modes || !rng || rng->pos < 0 || rng->pos > 624 || !F_macro ||
+      !eps_macro || !substeps || !info || !c->flag || !c->backup || c->nlive < 0 || c->nlive > 12 ||
+      (L->ndim != 2 && L->ndim != 3) || L->M < 1 || !L->cat || !L->Q0 || !L->soft || !L->tlast || !L->redraw_dev ||
+      !st->F || !st->Fp || !st->sig || (L->ndim == 2) != (c->sig2 != nullptr) || c->drv_i < 0 ||
+      c->drv_i >= L->ndim || c->drv_j < 0 || c->drv_j >= L->ndim || c->n_targets < 0 || c->n_targets > 3 ||
+      st->precision != 0 || (!L->redraw_directions && !L->redraw_barriers)) {
+    mgk_set_error("mgk_kmc_fs_flip: bad argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int d = L->ndim, M = L->M;
+  long long NV = 1;
+  int shape_i[3] = {1, 1, 1};
+  for (int i = 0; i < d; ++i) {
+    NV *= L->shape[i];
+    shape_i[i] = (int)L->shape[i];
+  }
+  const long long NV3 = st->shape[0] * st->shape[1] * st->shape[2];
+  if (NV3 != NV) {
+    mgk_set_error("mgk_kmc_fs_flip: the finite-strain grid does not hold the KMC grid's voxels", __FILE__,
+                  __LINE__);
+    return MGK_ERR_ARG;
+  }
+  // the backup's layout: every live buffer at a 256-byte aligned offset
+  size_t off[12], total = 0;
+  for (int k = 0; k < c->nlive; ++k) {
+    off[k] = total;
+    total += (c->live_bytes[k] + 255) / 256 * 256;
+  }
+  if (total > c->backup_bytes) {
+    mgk_set_error("mgk_kmc_fs_flip: backup area too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  // pinned staging: redraws (3M), the flips' planar modes (2 per flip), the NaN flag
+  double* host = redraw_stage(3 * (size_t)M + 2 * (size_t)nflips + 1);
+  if (!host) {
+    mgk_set_error("mgk_kmc_fs_flip: pinned host allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
+  double* cm = host + 3 * (size_t)M;
+  int* host_flag = (int*)(cm + 2 * (size_t)nflips);
+  // the flipped modes C (mgkmc/aqs.py:900-905: the catalog entry before its redraw)
+  for (int f = 0; f < nflips; ++f) {
+    if (voxels[f] < 0 || voxels[f] >= NV || modes[f] < 0 || modes[f] >= M) {
+      mgk_set_error("mgk_kmc_fs_flip: flip outside the grid or the catalog", __FILE__, __LINE__);
+      return MGK_ERR_ARG;
+    }
+    for (int k = 0; k < 2; ++k)
+      MGK_HIP(hipMemcpyAsync(cm + 2 * f + k, L->cat + ((size_t)k * M + modes[f]) * NV + voxels[f], sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+  }
+  MGK_HIP(hipStreamSynchronize(s));
+  std::vector<double> C((size_t)nflips * d * d, 0.0);
+  const double step = d == 3 ? 0.005 : 0.02;
+  int n0 = 0;
+  for (int f = 0; f < nflips; ++f) {
+    double* Cf = C.data() + (size_t)f * d * d;
+    Cf[0] = cm[2 * f];
+    Cf[d + 1] = -cm[2 * f];
+    Cf[1] = cm[2 * f + 1];
+    Cf[d] = cm[2 * f + 1];
+    double ss = 0.0;
+    for (int k = 0; k < d * d; ++k) ss += Cf[k] * Cf[k];
+    n0 = std::max(n0, std::max(2, (int)std::ceil(std::sqrt(0.5 * ss) / step)));
+  }
+  n0 = std::min(n0, 40);
+  std::vector<int> schedule;
+  if (d == 2) {
+    schedule.push_back(1);
+    n0 = std::max(n0, 2);
+  }
+  for (int k = 0; k < 20; ++k) schedule.push_back(n0 + k);
+  for (int k = 0; k < c->nlive; ++k)
+    if (c->live[k] && c->live_bytes[k])
+      MGK_HIP(hipMemcpyAsync((char*)c->backup + off[k], c->live[k], c->live_bytes[k], hipMemcpyDeviceToDevice, s));
+  double bfm[9], bem[9];
+  std::memcpy(bfm, F_macro, sizeof(double) * d * d);
+  std::memcpy(bem, eps_macro, sizeof(double) * d * d);
+  Mt R{rng};
+  double dF[9];
+  for (int N : schedule) {
+    for (int k = 0; k < c->nlive; ++k)
+      if (c->live[k] && c->live_bytes[k])
+        MGK_HIP(hipMemcpyAsync(c->live[k], (char*)c->backup + off[k], c->live_bytes[k], hipMemcpyDeviceToDevice, s));
+    std::memcpy(F_macro, bfm, sizeof(double) * d * d);
+    std::memcpy(eps_macro, bem, sizeof(double) * d * d);
+    int e = fs_sig2_sync(c, NV, s);
+    if (e != MGK_OK) return e;
+    bool failed = false;
+    for (int it = 1; it <= N && !failed; ++it) {
+      for (int f = 0; f < nflips; ++f) {
+        double A[9];
+        for (int k = 0; k < d * d; ++k) A[k] = C[(size_t)f * d * d + k] / N;
+        expm_small(A, d, dF);
+        e = mgk_fp_left_mul((double*)st->Fp, NV, voxels[f], d, dF, s);
+        if (e != MGK_OK) return e;
+        if (it == N) {
+          e = mgk_kmc_flip(d, shape_i, M, 0, voxels[f], modes[f], time, L->jp, L->jt, L->kp.cap, L->jn, L->cat,
+                           L->epsp, L->eps_precision, L->epsp_shadow, L->soft, L->tlast, L->prev, nullptr, 0, stream);
+          if (e != MGK_OK) return e;
+          redraw_draws(L, R, host);
+          MGK_HIP(hipMemcpyAsync(L->redraw_dev, host, sizeof(double) * 3 * (size_t)M, hipMemcpyHostToDevice, s));
+          e = mgk_kmc_set_voxel(NV, M, 2, voxels[f], L->redraw_directions ? L->redraw_dev : nullptr,
+                                L->redraw_barriers ? L->redraw_dev + 2 * M : nullptr, L->cat, L->Q0, stream);
+          if (e != MGK_OK) return e;
+          // the staging buffer is refilled by the next voxel's draws: wait for the upload
+          MGK_HIP(hipStreamSynchronize(s));
+        }
+      }
+      e = fs_update(c, d, NV, F_macro, eps_macro, host_flag, s, info);
+      if (e < 0) return e;
+      failed = e == 1;
+    }
+    if (!failed) {
+      *substeps = N;
+      return MGK_OK;
+    }
+  }
+  mgk_set_error("Mechanical solver failed to converge under sub-stepping.", __FILE__, __LINE__);
+  return MGK_ERR_SOLVER;
+}

@@ -125,6 +125,31 @@ class FsSolver:
             self.ws32 = dev.torch.empty(int(n.value), dtype=dev.torch.uint8, device="cuda")
         self.info = _lib.MgkFsInfo()
 
+    def stepper(self, mat, E_avg=100e9, nu_avg=0.3, tol_NW=1e-5, tol_CG=1e-6, max_NW=50, tol_macro=1e6,
+                max_iter_macro=20, solver="newton_cg", pixel=1.0):
+        """The arguments of step() without the macro BC as an MgkFsStepper
+        (for the library's native sub-stepped flip); keeps mat alive."""
+        if solver not in ("newton_cg", "al", "dbfft"):
+            raise NotImplementedError(f"solver={solver!r}: the device path implements 'newton_cg', 'al' and 'dbfft'")
+        st = _lib.MgkFsStepper()
+        st.solver = {"newton_cg": 0, "al": 1, "dbfft": 2}[solver]
+        st.precision, st.mixed = self.precision, int(self.mixed and solver == "newton_cg")
+        for i, n in enumerate(self.shape):
+            st.shape[i] = n
+        st.F, st.Fp, st.lam, st.mu = (_lib.dptr(self.F), _lib.dptr(self.Fp), _lib.dptr(self.lam),
+                                      _lib.dptr(self.mu))
+        if self.mixed:
+            st.lam32, st.mu32 = _lib.dptr(self.lam32), _lib.dptr(self.mu32)
+            st.ws32, st.ws32_bytes = _lib.dptr(self.ws32), self.ws32.numel()
+        st.mat = ctypes.addressof(mat)
+        st.P, st.sig, st.ws, st.ws_bytes = _lib.dptr(self.P), _lib.dptr(self.sig), _lib.dptr(self.ws), self.ws.numel()
+        st.L0 = ctypes.addressof(self.L0)
+        st.E_avg, st.nu_avg, st.tol_NW, st.tol_CG = float(E_avg), float(nu_avg), float(tol_NW), float(tol_CG)
+        st.tol_macro, st.pixel, st.max_NW, st.max_iter_macro = float(tol_macro), float(pixel), int(max_NW), \
+            int(max_iter_macro)
+        st._keep = (mat, self)
+        return st
+
     def step(self, F_bar, mat, driving_component=(0, 0), P_target=None, P_mask=None, E_avg=100e9, nu_avg=0.3,
              tol_NW=1e-5, tol_CG=1e-6, max_NW=50, tol_macro=1e6, max_iter_macro=20, solver="newton_cg", pixel=1.0):
         """One load step with solver "newton_cg" (FP64, FP32 or the FP32-Krylov

@@ -52,7 +52,11 @@ class _SigView:
     sig = None
 
     def sync(self, sig3):
-        self.sig = sig3[[0, 2]][:, [0, 2]].reshape(4, -1).contiguous()
+        v = sig3[[0, 2]][:, [0, 2]].reshape(4, -1)
+        if self.sig is None or self.sig.shape != v.shape:
+            self.sig = v.contiguous()
+        else:  # in place: the native finite-strain flip writes this buffer
+            self.sig.copy_(v)
 
 
 def gaussian_generator(mean=2.0, std=0.6, min_cutoff=0.1, max_cutoff=None, rng=None):
@@ -764,6 +768,8 @@ class DeviceKmc:
         flips of max(2, ceil(sqrt(C:C/2) / 0.005)), 40); 2D: N = 1, then
         n0 .. n0 + 19 with the step 0.02.  Softening, prev and redraws of each
         voxel at the last sub-step, one finite-strain step after each sub-step."""
+        if self._native_fs_ok():
+            return self._flip_finite_native(pairs)
         from scipy.linalg import expm
         torch = dev.torch
         Cs = [self._mode_host(v, m) for v, m in pairs]
@@ -808,6 +814,76 @@ class DeviceKmc:
             except ValueError:
                 continue
         raise ValueError("Mechanical solver failed to converge under sub-stepping.")
+
+    def _native_fs_ok(self):
+        """Whether the sub-stepped finite-strain flip runs in the library
+        (mgk_kmc_fs_flip): planar catalogs redrawn from the Gaussian barrier
+        generator on the host stream, no thermal coupling or flip log, FP64
+        fields."""
+        gen = self.barrier_generator
+        gp = getattr(gen, "params", None)
+        same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
+        return (self.native_loop and self.finite and not self.enable_thermal and self.kmc.planar
+                and self._flip_log is None and gp is not None and same_stream
+                and (self.redraw_directions or self.redraw_barriers) and hasattr(self.rng, "get_state")
+                and self.fs.precision == 0)
+
+    def _flip_finite_native(self, pairs):
+        """_flip_finite_pairs through mgk_kmc_fs_flip: the same sub-stepping
+        schedule, redraws from the same host stream, F_macro / eps_macro and
+        the step's record handed back."""
+        d, fs, k = self.ndim, self.fs, self.kmc
+        torch = dev.torch
+        L = self._loop_struct(np.zeros((d, d)), patch=False)
+        st = fs.stepper(self._fs_mat, E_avg=self.E_field.mean(), nu_avg=self.nu_field.mean(),
+                        solver=self._fs_solver, pixel=self.pixel)
+        c = _lib.MgkKmcFsSubstep()
+        c.kmc, c.step = ctypes.addressof(L), ctypes.addressof(st)
+        drv = getattr(self, "driving_component", (0, 0))
+        c.drv_i, c.drv_j = int(drv[0]), int(drv[1])
+        # the step's BC reads the diagonal targets P_kk only (build_finite_strain_bc_3d)
+        tg = getattr(self, "stress_targets", {}) or {}
+        diag = [(int(a), float(v)) for (a, b), v in tg.items() if a == b and 0 <= int(a) < d]
+        c.n_targets = len(diag)
+        for q, (kk, v) in enumerate(diag):
+            c.target_k[q], c.target_val[q] = kk, v
+        live = [fs.F, fs.P, fs.sig, fs.Fp, k.soft, k.tlast, k.prev, k.cat, k.Q0]
+        c.nlive = len(live)
+        total = 0
+        for q, a in enumerate(live):  # buffers a scheme leaves out (prev) are None
+            c.live[q] = _lib.dptr(a)
+            c.live_bytes[q] = 0 if a is None else a.numel() * a.element_size()
+            total += (c.live_bytes[q] + 255) // 256 * 256
+        if getattr(self, "_fs_backup", None) is None or self._fs_backup.numel() < total:
+            self._fs_backup = torch.empty(total, dtype=torch.uint8, device="cuda")
+            self._fs_flag = torch.zeros(1, dtype=torch.int32, device="cuda")
+        c.backup, c.backup_bytes = _lib.dptr(self._fs_backup), self._fs_backup.numel()
+        c.flag = _lib.dptr(self._fs_flag)
+        c.sig2 = _lib.dptr(self._fs_sig2.sig) if d == 2 else None
+        n = len(pairs)
+        vv = (ctypes.c_longlong * n)(*[int(v) for v, _ in pairs])
+        mm = (ctypes.c_int * n)(*[int(m) for _, m in pairs])
+        rng = _lib.mt_from(self.rng)
+        fm = (ctypes.c_double * 9)(*self.F_macro.reshape(-1))
+        em = (ctypes.c_double * 9)(*self.eps_macro.reshape(-1))
+        nsub = ctypes.c_int()
+        info = _lib.MgkFsInfo()
+        try:
+            rc = _lib.load().mgk_kmc_fs_flip(ctypes.byref(c), n, vv, mm, float(self.time), ctypes.byref(rng), fm, em,
+                                             ctypes.byref(nsub), ctypes.byref(info),
+                                             ctypes.c_void_p(dev.stream_ptr()))
+        finally:
+            _lib.mt_to(self.rng, rng)
+        self.F_macro = np.array(fm[: d * d]).reshape(d, d)
+        self.eps_macro[...] = np.array(em[: d * d]).reshape(d, d)
+        self.ls_info = info
+        self._solved_once = True
+        if rc == _lib.MGK_ERR_SOLVER:
+            raise ValueError("Mechanical solver failed to converge under sub-stepping.")
+        if rc == _lib.MGK_ERR_NONFINITE:
+            raise FloatingPointError(_lib.last_error().split(" (/")[0])
+        _lib.check(rc, "mgk_kmc_fs_flip")
+        self.last_substeps = nsub.value
 
     def _solve_linear(self, eps_macro):
         """update_stress_fft_full itself (linear LS, cold start) whatever the
@@ -1258,6 +1334,8 @@ class DeviceKmc:
         L.q_has_min, L.q_min = int(lo is not None), float(lo if lo is not None else 0.0)
         L.q_has_max, L.q_max = int(hi is not None), float(hi if hi is not None else 0.0)
         L.ls_precision, L.sig_kind = {FP64: 0, FP32: 1, MIXED: 2}[self.precision], self.sig_kind
+        if ls is None:  # finite strain: the KMC fields only (mgk_kmc_fs_flip)
+            return L
         L.lam, L.mu, L.lam0, L.mu0 = _lib.dptr(ls.lam), _lib.dptr(ls.mu), ls.lam0, ls.mu0
         L.eps, L.sig, L.ws, L.ws_bytes = _lib.dptr(ls.eps), _lib.dptr(ls.sig), _lib.dptr(ls.ws), ls.ws.numel()
         if self.precision == MIXED:

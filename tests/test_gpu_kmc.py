@@ -466,6 +466,84 @@ def test_native_loop_matches_python_loop(gpu, d, prec, scheme):
     assert a.selection_stream.get_state()[2] == b.selection_stream.get_state()[2]
 
 
+@pytest.mark.parametrize("d,prec,solver,model,cascade", [
+    (3, "fp64", "newton_cg", "svk", False),
+    (3, "mixed", "newton_cg", "svk", False),
+    (3, "fp64", "dbfft", "svk", False),
+    (2, "fp64", "newton_cg", "landau", False),
+    (2, "fp64", "dbfft", "svk", False),
+    (2, "fp64", "newton_cg", "svk", True),
+])
+def test_native_finite_flip_matches_python_flip(gpu, d, prec, solver, model, cascade):
+    # mgk_kmc_fs_flip (the sub-stepped finite-strain flip in the library) against the
+    # per-sub-step Python path on the same seeds: the same events, sub-step counts,
+    # cascades and host stream states; F_p to 1e-14 and, in FP64, F, F_macro and sigma
+    # to 1e-10 relative (expm(C/N) is rounded differently: scipy's Pade form against a
+    # Taylor form, both exact to rounding)
+    from mgkmc_amd.aqs import ThermalSimulation
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    print(f"[native fs flip] {d}D {prec} {solver} {model} cascade={cascade}", flush=True)
+    # the settings of the finite-strain oracle tests (the cascade case: 2D cascade_mode's)
+    n, M = (8, 6) if d == 3 else ((12, 10) if cascade else (12, 6))
+    shape = (n,) * d
+    E = 70e9 * np.random.default_rng(0 if cascade else 3).uniform(0.9, 1.1, shape)
+    nu = np.full(shape, 0.3)
+    kw = dict(mean=0.3, std=0.15, min_cutoff=0.0) if cascade else dict(mean=1.0, std=0.3, min_cutoff=0.0)
+    mp = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9) \
+        if model == "landau" else {}
+    g0, seeds = (0.05, (5, 6)) if cascade else (0.03, (7, 8))
+    sims, used, subs = [], [], []
+    for native in (True, False):
+        args = dict(pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw, temperature=900.0 if d == 3 else 400.0,
+                    strain_rate=1e7, softening_cap=0.916, jp=20.0, jt=0.0 if cascade else 5.0,
+                    strain_assumption="finite_strain", solver=solver, ls_precision=prec, cascade_mode=cascade,
+                    rng=np.random.RandomState(seeds[0]), selection_stream=np.random.RandomState(seeds[1]))
+        if d == 3:
+            sim = ThermalSimulation(n, n, n, M, g0, E, nu, **args)
+        else:
+            sim = KmcSimulation2D(n, n, M, g0, E, nu, hyperelastic_model=model, **mp, **args)
+        sim.native_loop = native
+        assert sim._native_fs_ok() == native
+        calls, sub = [0], []
+        real_native, real_flip = sim._flip_finite_native, sim._flip_finite_pairs
+
+        def counted(pairs, real_native=real_native, calls=calls):
+            calls[0] += 1
+            return real_native(pairs)
+
+        def recorded(pairs, real_flip=real_flip, sim=sim, sub=sub):
+            real_flip(pairs)
+            sub.append(sim.last_substeps)
+        sim._flip_finite_native = counted
+        sim._flip_finite_pairs = recorded
+        kw_run = dict(component=(0, 0), max_events=10 if cascade else 8)
+        if d == 2:
+            kw_run["stress_targets"] = {(1, 1): 0.0}
+        if cascade:
+            kw_run["max_cascade_steps_pct"] = 0.05
+        sim.run_simulation(4 if cascade else 3, 2e-3, **kw_run)
+        sims.append(sim)
+        used.append(calls[0])
+        subs.append(sub)
+    a, b = sims
+    print(f"[native fs flip] {len(a.kmc_event_history)} events, flips {used}, substeps {subs[0]}", flush=True)
+    assert used[0] > 0 and used[1] == 0 and len(subs[0]) > 0
+    assert subs[0] == subs[1]
+    assert [e[:-2] for e in a.kmc_event_history] == [e[:-2] for e in b.kmc_event_history]
+    assert np.allclose([e[-2] for e in a.kmc_event_history], [e[-2] for e in b.kmc_event_history], rtol=1e-12)
+    if cascade:
+        assert len(a.cascade_history) == len(b.cascade_history) and len(a.cascade_history) > 0
+    # the FP32-Krylov form turns the rounding difference into FP32-level ones (its
+    # oracle test allows 1e-5 in F)
+    tol, stol = (1e-10, 1e-10) if prec == "fp64" else (1e-6, 1e-3)
+    assert np.abs(a.F_plastic - b.F_plastic).max() <= 1e-14
+    assert np.abs(a.F_field - b.F_field).max() <= tol * np.abs(b.F_field).max()
+    assert np.abs(a.F_macro - b.F_macro).max() <= tol
+    assert np.abs(a.sig_field - b.sig_field).max() <= stol * np.abs(b.sig_field).max()
+    sa, sb = a.rng.get_state(), b.rng.get_state()
+    assert np.array_equal(sa[1], sb[1]) and sa[2:] == sb[2:]
+
+
 @pytest.mark.parametrize("d,prec,T,pct,records", [
     (3, "fp64", 0.0, 0.3, None),        # athermal: cascades inside the mixed-BC loop, logged per batch
     (3, "mixed", 0.0, 0.002, None),     # batch limit: the cascade is truncated
