@@ -465,6 +465,7 @@ typedef struct {
 int mgk_mt_fill(mgk_mt_state* state, int kind, double loc, double scale, long long n, double* out);
 
 struct mgk_kmc_patch_cfg;
+struct mgk_kmc_fs_substep;
 
 typedef struct {
   int ndim;
@@ -504,6 +505,12 @@ typedef struct {
    * weights read on the device), a re-solve every sync_interval flips; 2D loops add the
    * macro-unit stress over the remaining increment time once a flip has happened */
   const struct mgk_kmc_patch_cfg* patch;
+  /* finite strain (mgkmc/aqs.py:887-996, kmc_simulator.py:895-1000), or NULL: each event's
+   * flip is the sub-stepped finite-strain flip (mgk_kmc_fs_flip with this description,
+   * F_macro in/out through it) instead of the flip, redraw and LS re-solve; sig is the
+   * Cauchy field the barrier pass reads (sig_kind 2; the in-plane block in 2D) and the
+   * LS fields are not read */
+  const struct mgk_kmc_fs_substep* fs;
 } mgk_kmc_loop;
 
 /* Fast-patching state of a native KMC loop (planar catalogs, sig_kind 0 or 1). */
@@ -530,6 +537,7 @@ typedef struct {
   int ls_iters, ls_iters32;
   double eps_macro[9];  /* macro strain after the event (d*d row-major)          */
   double sig_mean[9];   /* <sig> after the event's re-solve (d*d row-major)      */
+  int substeps;         /* finite strain: the N of the event's sub-stepped flip  */
 } mgk_kmc_event;
 
 #define MGK_KMC_RUN_TIME 0       /* no event within *remaining: time advanced by it, *remaining = 0 */
@@ -661,6 +669,9 @@ typedef struct mgk_kmc_fs_substep {
   size_t backup_bytes;
   double* sig2;
   int* flag;
+  /* native event loops (mgk_kmc_loop.fs): F_macro (d*d, in/out) and the last step's record */
+  double* F_macro;
+  mgk_fs_info* info;
 } mgk_kmc_fs_substep;
 
 /* The finite-strain flips of one KMC event or one cascade batch with the reference's

@@ -77,7 +77,7 @@ class MgkKmcLoop(ctypes.Structure):
 class MgkKmcEvent(ctypes.Structure):
     _fields_ = [("v", ctypes.c_longlong), ("m", ctypes.c_int), ("is_instab", ctypes.c_int), ("time", ctypes.c_double),
                 ("ls_iters", ctypes.c_int), ("ls_iters32", ctypes.c_int), ("eps_macro", ctypes.c_double * 9),
-                ("sig_mean", ctypes.c_double * 9)]
+                ("sig_mean", ctypes.c_double * 9), ("substeps", ctypes.c_int)]
 
 
 class MgkCascadeFlip(ctypes.Structure):
@@ -99,7 +99,7 @@ MgkKmcLoop._fields_ = [
     ("ws_bytes", ctypes.c_size_t), ("ws32", ctypes.c_void_p), ("ws32_bytes", ctypes.c_size_t), ("warm", ctypes.c_int),
     ("max_iter", ctypes.c_int), ("tol", ctypes.c_double), ("tol32", ctypes.c_double),
     ("check_unstable", ctypes.c_int), ("strain_rate", ctypes.c_double), ("unit", ctypes.c_double * 9),
-    ("patch", ctypes.c_void_p)]
+    ("patch", ctypes.c_void_p), ("fs", ctypes.c_void_p)]
 
 
 class MgkKmcPatchCfg(ctypes.Structure):
@@ -158,7 +158,8 @@ class MgkKmcFsSubstep(ctypes.Structure):
                 ("drv_j", ctypes.c_int), ("n_targets", ctypes.c_int), ("target_k", ctypes.c_int * 3),
                 ("target_val", ctypes.c_double * 3), ("nlive", ctypes.c_int), ("live", ctypes.c_void_p * 12),
                 ("live_bytes", ctypes.c_size_t * 12), ("backup", ctypes.c_void_p), ("backup_bytes", ctypes.c_size_t),
-                ("sig2", ctypes.c_void_p), ("flag", ctypes.c_void_p)]
+                ("sig2", ctypes.c_void_p), ("flag", ctypes.c_void_p), ("F_macro", ctypes.c_void_p),
+                ("info", ctypes.c_void_p)]
 
 
 _c_p = ctypes.c_void_p

@@ -105,6 +105,22 @@ void redraw_draws(const mgk_kmc_loop* L, Mt& R, double* host) {
     }
 }
 
+// the sub-stepped finite-strain flip (defined with mgk_kmc_fs_flip below)
+int fs_flip_core(const mgk_kmc_loop* L, const mgk_kmc_fs_substep* c, int nflips, const long long* voxels,
+                 const int* modes, double time, Mt& R, double* F_macro, double* eps_macro, int* substeps,
+                 mgk_fs_info* info, hipStream_t s);
+
+bool fs_ok(const mgk_kmc_loop* L, const mgk_kmc_fs_substep* c) {
+  if (!c) return false;
+  const mgk_fs_stepper* st = c->step;
+  return st && st->F && st->Fp && st->sig && st->precision == 0 && c->flag && c->backup && c->nlive >= 0 &&
+         c->nlive <= 12 && (L->ndim == 2) == (c->sig2 != nullptr) && c->drv_i >= 0 && c->drv_i < L->ndim &&
+         c->drv_j >= 0 && c->drv_j < L->ndim && c->n_targets >= 0 && c->n_targets <= 3 &&
+         (L->redraw_directions || L->redraw_barriers) &&
+         st->shape[0] * st->shape[1] * st->shape[2] ==
+             L->shape[0] * L->shape[1] * (L->ndim == 3 ? L->shape[2] : 1);
+}
+
 bool patch_ok(const mgk_kmc_loop* L) {
   const mgk_kmc_patch_cfg* P = L->patch;
   return !P || (P->kernels && P->sig_unit && P->workspace && P->flips_since_sync && P->unit_live && P->sig_mean &&
@@ -113,9 +129,10 @@ bool patch_ok(const mgk_kmc_loop* L) {
 
 bool loop_ok(const mgk_kmc_loop* L, const mgk_mt_state* rng, const mgk_mt_state* sel_rng) {
   return L && rng && sel_rng && (L->ndim == 2 || L->ndim == 3) && L->M >= 1 && L->Q0 && L->cat && L->soft &&
-         L->tlast && L->epsp && L->kmc_ws && L->redraw_dev && L->sig && L->eps && L->ws &&
-         (L->ls_precision != 2 || (L->lam32 && L->mu32 && L->eps32 && L->ws32)) && rng->pos >= 0 &&
-         rng->pos <= 624 && sel_rng->pos >= 0 && sel_rng->pos <= 624 && patch_ok(L);
+         L->tlast && L->epsp && L->kmc_ws && L->redraw_dev && L->sig &&
+         (L->fs ? (fs_ok(L, L->fs) && L->fs->F_macro && !L->patch)
+                : (L->eps && L->ws && (L->ls_precision != 2 || (L->lam32 && L->mu32 && L->eps32 && L->ws32)))) &&
+         rng->pos >= 0 && rng->pos <= 624 && sel_rng->pos >= 0 && sel_rng->pos <= 624 && patch_ok(L);
 }
 
 void fill_event(mgk_kmc_event& ev, long long v, int m, int instab, double time, const mgk_ls_info& inf, int it32,
@@ -126,6 +143,7 @@ void fill_event(mgk_kmc_event& ev, long long v, int m, int instab, double time, 
   ev.time = time;
   ev.ls_iters = inf.iters;
   ev.ls_iters32 = it32;
+  ev.substeps = 0;
   for (int k = 0; k < 9; ++k) {
     ev.eps_macro[k] = k < d * d ? eps_macro[k] : 0.0;
     ev.sig_mean[k] = inf.sig_mean[k];
@@ -220,6 +238,35 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
         const long long v = so.item;
         const int m = so.mode;
         const int instab = so.q <= L->stability_threshold ? 1 : 0;
+        if (L->fs) {
+          // finite strain: the sub-stepped flip (flip, redraw and the steps) replaces
+          // the flip, redraw and LS re-solve
+          int nsub = 0;
+          mgk_fs_info fsi{};
+          e = fs_flip_core(L, L->fs, 1, &v, &m, *time, R, L->fs->F_macro, eps_macro, &nsub, &fsi, s);
+          if (L->fs->info) *L->fs->info = fsi;
+          if (e != MGK_OK) {
+            rc = e;
+            break;
+          }
+          mgk_ls_info pin{};
+          for (int a = 0; a < d; ++a)
+            for (int b = 0; b < d; ++b)
+              pin.sig_mean[a * d + b] = fsi.sig_mean[d == 3 ? 3 * a + b : 3 * (2 * a) + 2 * b];
+          fill_event(events[n], v, m, instab, *time, pin, 0, eps_macro, d);
+          events[n++].substeps = nsub;
+          *n_events = n;
+          *seq = instab ? *seq + 1 : 0;
+          if (*seq > max_seq) {
+            rc = MGK_KMC_RUN_SEQ;
+            break;
+          }
+          if (n >= max_events) {
+            rc = MGK_KMC_RUN_COUNT;
+            break;
+          }
+          continue;
+        }
         e = mgk_kmc_flip(d, shape_i, M, 0, v, m, *time, L->jp, L->jt, L->kp.cap, L->jn, L->cat, L->epsp,
                          L->eps_precision, L->epsp_shadow, L->soft, L->tlast, L->prev, nullptr, 1, stream);
         if (e != MGK_OK) {
@@ -369,8 +416,9 @@ extern "C" int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* L, mgk_mt_state* rn
     return MGK_ERR_ARG;
   }
   for (int b = 0; b < nb; ++b)
-    if (L[b].patch) {
-      mgk_set_error("mgk_kmc_run_batch: fast patching runs in mgk_kmc_run only", __FILE__, __LINE__);
+    if (L[b].patch || L[b].fs) {
+      mgk_set_error("mgk_kmc_run_batch: fast patching and finite strain run in mgk_kmc_run only", __FILE__,
+                    __LINE__);
       return MGK_ERR_ARG;
     }
   const mgk_kmc_loop& L0 = L[0];
@@ -669,7 +717,7 @@ extern "C" int mgk_kmc_cascade(const mgk_kmc_loop* L, mgk_mt_state* rng, double*
                                int* iters32, void* stream) {
   mgk_mt_state sel_dummy{};
   sel_dummy.pos = 624;
-  if (!loop_ok(L, rng, &sel_dummy) || !Q || !eps_macro || !flips || cap < 1 || !n_flips || !batch_sizes ||
+  if (!loop_ok(L, rng, &sel_dummy) || L->fs || !Q || !eps_macro || !flips || cap < 1 || !n_flips || !batch_sizes ||
       batch_cap < 1 || !n_batches || !reason || !info || max_batches < 0 ||
       (!L->redraw_directions && !L->redraw_barriers)) {
     mgk_set_error("mgk_kmc_cascade: bad argument", __FILE__, __LINE__);
@@ -1107,21 +1155,11 @@ int fs_update(const mgk_kmc_fs_substep* c, int d, long long NV, double* F_macro,
 
 }  // namespace
 
-extern "C" int mgk_kmc_fs_flip(const mgk_kmc_fs_substep* c, int nflips, const long long* voxels, const int* modes,
-                               double time, mgk_mt_state* rng, double* F_macro, double* eps_macro, int* substeps,
-                               mgk_fs_info* info, void* stream) {
-  const mgk_kmc_loop* L = c ? c->kmc : nullptr;
-  const mgk_fs_stepper* st = c ? c->step : nullptr;
-  if (!c || !L || !st || nflips < 1 || !voxels || !modes || !rng || rng->pos < 0 || rng->pos > 624 || !F_macro ||
-      !eps_macro || !substeps || !info || !c->flag || !c->backup || c->nlive < 0 || c->nlive > 12 ||
-      (L->ndim != 2 && L->ndim != 3) || L->M < 1 || !L->cat || !L->Q0 || !L->soft || !L->tlast || !L->redraw_dev ||
-      !st->F || !st->Fp || !st->sig || (L->ndim == 2) != (c->sig2 != nullptr) || c->drv_i < 0 ||
-      c->drv_i >= L->ndim || c->drv_j < 0 || c->drv_j >= L->ndim || c->n_targets < 0 || c->n_targets > 3 ||
-      st->precision != 0 || (!L->redraw_directions && !L->redraw_barriers)) {
-    mgk_set_error("mgk_kmc_fs_flip: bad argument", __FILE__, __LINE__);
-    return MGK_ERR_ARG;
-  }
-  hipStream_t s = (hipStream_t)stream;
+namespace {
+int fs_flip_core(const mgk_kmc_loop* L, const mgk_kmc_fs_substep* c, int nflips, const long long* voxels,
+                 const int* modes, double time, Mt& R, double* F_macro, double* eps_macro, int* substeps,
+                 mgk_fs_info* info, hipStream_t s) {
+  const mgk_fs_stepper* st = c->step;
   const int d = L->ndim, M = L->M;
   long long NV = 1;
   int shape_i[3] = {1, 1, 1};
@@ -1190,7 +1228,6 @@ extern "C" int mgk_kmc_fs_flip(const mgk_kmc_fs_substep* c, int nflips, const lo
   double bfm[9], bem[9];
   std::memcpy(bfm, F_macro, sizeof(double) * d * d);
   std::memcpy(bem, eps_macro, sizeof(double) * d * d);
-  Mt R{rng};
   double dF[9];
   for (int N : schedule) {
     for (int k = 0; k < c->nlive; ++k)
@@ -1210,12 +1247,12 @@ extern "C" int mgk_kmc_fs_flip(const mgk_kmc_fs_substep* c, int nflips, const lo
         if (e != MGK_OK) return e;
         if (it == N) {
           e = mgk_kmc_flip(d, shape_i, M, 0, voxels[f], modes[f], time, L->jp, L->jt, L->kp.cap, L->jn, L->cat,
-                           L->epsp, L->eps_precision, L->epsp_shadow, L->soft, L->tlast, L->prev, nullptr, 0, stream);
+                           L->epsp, L->eps_precision, L->epsp_shadow, L->soft, L->tlast, L->prev, nullptr, 0, s);
           if (e != MGK_OK) return e;
           redraw_draws(L, R, host);
           MGK_HIP(hipMemcpyAsync(L->redraw_dev, host, sizeof(double) * 3 * (size_t)M, hipMemcpyHostToDevice, s));
           e = mgk_kmc_set_voxel(NV, M, 2, voxels[f], L->redraw_directions ? L->redraw_dev : nullptr,
-                                L->redraw_barriers ? L->redraw_dev + 2 * M : nullptr, L->cat, L->Q0, stream);
+                                L->redraw_barriers ? L->redraw_dev + 2 * M : nullptr, L->cat, L->Q0, s);
           if (e != MGK_OK) return e;
           // the staging buffer is refilled by the next voxel's draws: wait for the upload
           MGK_HIP(hipStreamSynchronize(s));
@@ -1232,4 +1269,24 @@ extern "C" int mgk_kmc_fs_flip(const mgk_kmc_fs_substep* c, int nflips, const lo
   }
   mgk_set_error("Mechanical solver failed to converge under sub-stepping.", __FILE__, __LINE__);
   return MGK_ERR_SOLVER;
+}
+}  // namespace
+
+extern "C" int mgk_kmc_fs_flip(const mgk_kmc_fs_substep* c, int nflips, const long long* voxels, const int* modes,
+                               double time, mgk_mt_state* rng, double* F_macro, double* eps_macro, int* substeps,
+                               mgk_fs_info* info, void* stream) {
+  const mgk_kmc_loop* L = c ? c->kmc : nullptr;
+  const mgk_fs_stepper* st = c ? c->step : nullptr;
+  if (!c || !L || !st || nflips < 1 || !voxels || !modes || !rng || rng->pos < 0 || rng->pos > 624 || !F_macro ||
+      !eps_macro || !substeps || !info || !c->flag || !c->backup || c->nlive < 0 || c->nlive > 12 ||
+      (L->ndim != 2 && L->ndim != 3) || L->M < 1 || !L->cat || !L->Q0 || !L->soft || !L->tlast || !L->redraw_dev ||
+      !st->F || !st->Fp || !st->sig || (L->ndim == 2) != (c->sig2 != nullptr) || c->drv_i < 0 ||
+      c->drv_i >= L->ndim || c->drv_j < 0 || c->drv_j >= L->ndim || c->n_targets < 0 || c->n_targets > 3 ||
+      st->precision != 0 || (!L->redraw_directions && !L->redraw_barriers)) {
+    mgk_set_error("mgk_kmc_fs_flip: bad argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  Mt R{rng};
+  return fs_flip_core(L, c, nflips, voxels, modes, time, R, F_macro, eps_macro, substeps, info,
+                      (hipStream_t)stream);
 }

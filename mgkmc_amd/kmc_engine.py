@@ -357,6 +357,7 @@ class DeviceKmc:
         self.enable_thermal = False
         self.Tfield = None
         self.kmc_event_history = []  # (x, y[, z], m, time, is_instab) per KMC event (extension)
+        self.kmc_substeps = []  # finite strain: the N of each KMC event's sub-stepped flip (extension)
         self.flip_event_history = []  # tracked cascade flips (global step, batch, x, y, z, m), as the reference's
         self.cascade_history = []  # (global step, batch, x, y[, z], m) per cascade flip
         self.history_global = []  # (eps_xx, sig_xx / 1e9) per logged step (mgkmc/aqs.py:760)
@@ -759,6 +760,7 @@ class DeviceKmc:
     def _flip_finite(self, v, m):
         """One KMC event's finite-strain flip (see _flip_finite_pairs)."""
         self._flip_finite_pairs([(v, m)])
+        self.kmc_substeps.append(self.last_substeps)
 
     def _flip_finite_pairs(self, pairs):
         """Finite-strain flips of one event or one cascade batch (mgkmc/aqs.py:512-615,
@@ -832,9 +834,45 @@ class DeviceKmc:
         """_flip_finite_pairs through mgk_kmc_fs_flip: the same sub-stepping
         schedule, redraws from the same host stream, F_macro / eps_macro and
         the step's record handed back."""
+        d = self.ndim
+        L = self._loop_struct(np.zeros((d, d)), patch=False)
+        c = self._fs_substep_cfg(L)
+        n = len(pairs)
+        vv = (ctypes.c_longlong * n)(*[int(v) for v, _ in pairs])
+        mm = (ctypes.c_int * n)(*[int(m) for _, m in pairs])
+        rng = _lib.mt_from(self.rng)
+        fm = (ctypes.c_double * 9)(*self.F_macro.reshape(-1))
+        em = (ctypes.c_double * 9)(*self.eps_macro.reshape(-1))
+        nsub = ctypes.c_int()
+        info = _lib.MgkFsInfo()
+        try:
+            rc = _lib.load().mgk_kmc_fs_flip(ctypes.byref(c), n, vv, mm, float(self.time), ctypes.byref(rng), fm, em,
+                                             ctypes.byref(nsub), ctypes.byref(info),
+                                             ctypes.c_void_p(dev.stream_ptr()))
+        finally:
+            _lib.mt_to(self.rng, rng)
+        self.F_macro = np.array(fm[: d * d]).reshape(d, d)
+        self.eps_macro[...] = np.array(em[: d * d]).reshape(d, d)
+        self.ls_info = info
+        self._solved_once = True
+        self._fs_raise(rc)
+        self.last_substeps = nsub.value
+
+    @staticmethod
+    def _fs_raise(rc):
+        """The reference's exceptions of a failed sub-stepped flip."""
+        if rc == _lib.MGK_ERR_SOLVER:
+            raise ValueError("Mechanical solver failed to converge under sub-stepping.")
+        if rc == _lib.MGK_ERR_NONFINITE:
+            raise FloatingPointError(_lib.last_error().split(" (/")[0])
+        _lib.check(rc, "mgk_kmc_fs_flip")
+
+    def _fs_substep_cfg(self, L):
+        """MgkKmcFsSubstep for loop description L: the step (solver, fields,
+        material, the reference's step defaults), the macro BC inputs, the
+        live device state and its backup area (kept on self)."""
         d, fs, k = self.ndim, self.fs, self.kmc
         torch = dev.torch
-        L = self._loop_struct(np.zeros((d, d)), patch=False)
         st = fs.stepper(self._fs_mat, E_avg=self.E_field.mean(), nu_avg=self.nu_field.mean(),
                         solver=self._fs_solver, pixel=self.pixel)
         c = _lib.MgkKmcFsSubstep()
@@ -860,30 +898,8 @@ class DeviceKmc:
         c.backup, c.backup_bytes = _lib.dptr(self._fs_backup), self._fs_backup.numel()
         c.flag = _lib.dptr(self._fs_flag)
         c.sig2 = _lib.dptr(self._fs_sig2.sig) if d == 2 else None
-        n = len(pairs)
-        vv = (ctypes.c_longlong * n)(*[int(v) for v, _ in pairs])
-        mm = (ctypes.c_int * n)(*[int(m) for _, m in pairs])
-        rng = _lib.mt_from(self.rng)
-        fm = (ctypes.c_double * 9)(*self.F_macro.reshape(-1))
-        em = (ctypes.c_double * 9)(*self.eps_macro.reshape(-1))
-        nsub = ctypes.c_int()
-        info = _lib.MgkFsInfo()
-        try:
-            rc = _lib.load().mgk_kmc_fs_flip(ctypes.byref(c), n, vv, mm, float(self.time), ctypes.byref(rng), fm, em,
-                                             ctypes.byref(nsub), ctypes.byref(info),
-                                             ctypes.c_void_p(dev.stream_ptr()))
-        finally:
-            _lib.mt_to(self.rng, rng)
-        self.F_macro = np.array(fm[: d * d]).reshape(d, d)
-        self.eps_macro[...] = np.array(em[: d * d]).reshape(d, d)
-        self.ls_info = info
-        self._solved_once = True
-        if rc == _lib.MGK_ERR_SOLVER:
-            raise ValueError("Mechanical solver failed to converge under sub-stepping.")
-        if rc == _lib.MGK_ERR_NONFINITE:
-            raise FloatingPointError(_lib.last_error().split(" (/")[0])
-        _lib.check(rc, "mgk_kmc_fs_flip")
-        self.last_substeps = nsub.value
+        c._keep = (st, L)
+        return c
 
     def _solve_linear(self, eps_macro):
         """update_stress_fft_full itself (linear LS, cold start) whatever the
@@ -1306,8 +1322,8 @@ class DeviceKmc:
         gen = self.barrier_generator
         gp = getattr(gen, "params", None)
         same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
-        return (self.native_loop and self.law is None and not self.finite and not self.cascade_mode
-                and not self.enable_thermal and self.kmc.planar and self._flip_log is None
+        return (self.native_loop and self.law is None and (not self.finite or self._native_fs_ok())
+                and not self.cascade_mode and not self.enable_thermal and self.kmc.planar and self._flip_log is None
                 and gp is not None and same_stream and (self.redraw_directions or self.redraw_barriers)
                 and hasattr(self.rng, "get_state") and hasattr(self.selection_stream, "get_state")
                 and (self._solved_once or not self.ls_warm_start))
@@ -1334,7 +1350,12 @@ class DeviceKmc:
         L.q_has_min, L.q_min = int(lo is not None), float(lo if lo is not None else 0.0)
         L.q_has_max, L.q_max = int(hi is not None), float(hi if hi is not None else 0.0)
         L.ls_precision, L.sig_kind = {FP64: 0, FP32: 1, MIXED: 2}[self.precision], self.sig_kind
-        if ls is None:  # finite strain: the KMC fields only (mgk_kmc_fs_flip)
+        if ls is None:  # finite strain: the KMC fields and the Cauchy field of the barrier pass
+            L.sig = _lib.dptr(self.fs.sig if d == 3 else self._fs_sig2.sig)
+            L.strain_rate = float(self.strain_rate)
+            u = np.asarray(unit, dtype=np.float64).reshape(-1)
+            for i in range(d * d):
+                L.unit[i] = u[i]
             return L
         L.lam, L.mu, L.lam0, L.mu0 = _lib.dptr(ls.lam), _lib.dptr(ls.mu), ls.lam0, ls.mu0
         L.eps, L.sig, L.ws, L.ws_bytes = _lib.dptr(ls.eps), _lib.dptr(ls.sig), _lib.dptr(ls.ws), ls.ws.numel()
@@ -1422,20 +1443,31 @@ class DeviceKmc:
             ev = coords + (int(e.m), float(e.time), bool(e.is_instab))
             self.kmc_event_history.append(ev)
             self.ls_iters_total += e.ls_iters + e.ls_iters32
+            if self.finite:
+                self.kmc_substeps.append(int(e.substeps))
+                self.last_substeps = int(e.substeps)
             if on_event is not None:
                 over = rc == _lib.MGK_KMC_RUN_SEQ and i == len(evs) - 1
                 on_event(ev, np.array(e.eps_macro[: d * d]).reshape(d, d),
                          np.array(e.sig_mean[: d * d]).reshape(d, d), over)
         if len(evs):
             self._solved_once = True
-            self.ls_info = self.ls.info
-            if self.precision == MIXED:
-                self.ls.iters32 = it32
+            if self.ls is not None:
+                self.ls_info = self.ls.info
+                if self.precision == MIXED:
+                    self.ls.iters32 = it32
 
     def _run_native(self, remaining, unit, max_events, max_seq, seq, on_event=None):
         d = self.ndim
         lib = _lib.load()
         L = self._loop_struct(unit)
+        if self.finite:  # each event's flip is the sub-stepped finite-strain flip
+            cfg = self._fs_substep_cfg(L)
+            fm = (ctypes.c_double * 9)(*self.F_macro.reshape(-1))
+            fsi = _lib.MgkFsInfo()
+            cfg.F_macro, cfg.info = ctypes.addressof(fm), ctypes.addressof(fsi)
+            L.fs = ctypes.addressof(cfg)
+        ls_info = self.ls.info if self.ls is not None else _lib.MgkLsInfo()
         rng, sel = _lib.mt_from(self.rng), _lib.mt_from(self.selection_stream)
         t = ctypes.c_double(self.time)
         rem = ctypes.c_double(remaining)
@@ -1452,10 +1484,15 @@ class DeviceKmc:
                 evs = (_lib.MgkKmcEvent * chunk)()
                 rc = lib.mgk_kmc_run(ctypes.byref(L), ctypes.byref(rng), ctypes.byref(sel), ctypes.byref(t), em,
                                      ctypes.byref(rem), chunk, ms, ctypes.byref(sq), evs, ctypes.byref(nb),
-                                     ctypes.byref(self.ls.info), ctypes.byref(it32),
+                                     ctypes.byref(ls_info), ctypes.byref(it32),
                                      ctypes.c_void_p(dev.stream_ptr()))
                 self._take_native_events(evs[: nb.value], rc, it32.value, on_event)
                 n_total += nb.value
+                if self.finite and (nb.value or rc < 0):
+                    self.F_macro = np.array(fm[: d * d]).reshape(d, d)
+                    self.ls_info = fsi
+                if self.finite and rc < 0:
+                    self._fs_raise(rc)
                 if rc == _lib.MGK_KMC_RUN_UNSTABLE:
                     sm = np.array(self.ls.info.sig_mean[: d * d]).reshape(d, d)
                     raise ValueError(f"Unstable stress in small-strain elastic_run: {sm / 1e9} GPa")
@@ -1522,7 +1559,7 @@ def kmc_events_batch(sims, remaining, units, max_events, max_seq=None, seqs=None
     if not 1 <= n <= _lib.MGK_LS_MAX_BATCH:
         raise ValueError(f"batch of {n} simulations (1..{_lib.MGK_LS_MAX_BATCH})")
     for sm in sims:
-        if not sm._native_ok():
+        if not sm._native_ok() or sm.finite:
             raise ValueError("batched events need simulations that run in the native loop "
                              "(linear law, small strain, KMC mode, gaussian barriers, planar catalog)")
     lib = _lib.load()

@@ -492,7 +492,7 @@ def test_native_finite_flip_matches_python_flip(gpu, d, prec, solver, model, cas
     mp = dict(v1=-236.5e9, v2=-27.4e9, v3=11.8e9, g1=-5640.7e9, g2=2207.1e9, g3=-332.5e9, g4=-305.2e9) \
         if model == "landau" else {}
     g0, seeds = (0.05, (5, 6)) if cascade else (0.03, (7, 8))
-    sims, used, subs = [], [], []
+    sims, used = [], []
     for native in (True, False):
         args = dict(pixel=0.7, barrier_generator="gaussian", barrier_kwargs=kw, temperature=900.0 if d == 3 else 400.0,
                     strain_rate=1e7, softening_cap=0.916, jp=20.0, jt=0.0 if cascade else 5.0,
@@ -504,18 +504,16 @@ def test_native_finite_flip_matches_python_flip(gpu, d, prec, solver, model, cas
             sim = KmcSimulation2D(n, n, M, g0, E, nu, hyperelastic_model=model, **mp, **args)
         sim.native_loop = native
         assert sim._native_fs_ok() == native
-        calls, sub = [0], []
-        real_native, real_flip = sim._flip_finite_native, sim._flip_finite_pairs
+        # KMC-mode runs: whole events in the C++ loop (mgk_kmc_run calling the flip);
+        # cascade mode: per-event Python with the flips (events and batches) in C++
+        assert sim._native_ok() == (native and not cascade)
+        calls = [0]
+        real_native = sim._flip_finite_native
 
         def counted(pairs, real_native=real_native, calls=calls):
             calls[0] += 1
             return real_native(pairs)
-
-        def recorded(pairs, real_flip=real_flip, sim=sim, sub=sub):
-            real_flip(pairs)
-            sub.append(sim.last_substeps)
         sim._flip_finite_native = counted
-        sim._flip_finite_pairs = recorded
         kw_run = dict(component=(0, 0), max_events=10 if cascade else 8)
         if d == 2:
             kw_run["stress_targets"] = {(1, 1): 0.0}
@@ -524,11 +522,11 @@ def test_native_finite_flip_matches_python_flip(gpu, d, prec, solver, model, cas
         sim.run_simulation(4 if cascade else 3, 2e-3, **kw_run)
         sims.append(sim)
         used.append(calls[0])
-        subs.append(sub)
     a, b = sims
-    print(f"[native fs flip] {len(a.kmc_event_history)} events, flips {used}, substeps {subs[0]}", flush=True)
-    assert used[0] > 0 and used[1] == 0 and len(subs[0]) > 0
-    assert subs[0] == subs[1]
+    print(f"[native fs flip] {len(a.kmc_event_history)} events, flips through the wrapper {used}, "
+          f"substeps {a.kmc_substeps}", flush=True)
+    assert used[1] == 0 and (used[0] > 0 if cascade else used[0] == 0)
+    assert len(a.kmc_substeps) > 0 and a.kmc_substeps == b.kmc_substeps
     assert [e[:-2] for e in a.kmc_event_history] == [e[:-2] for e in b.kmc_event_history]
     assert np.allclose([e[-2] for e in a.kmc_event_history], [e[-2] for e in b.kmc_event_history], rtol=1e-12)
     if cascade:
@@ -621,14 +619,8 @@ def test_kmc3d_finite_strain_matches_oracle(gpu, prec, solver):
                             temperature=900.0, strain_rate=1e7, strain_assumption="finite_strain", solver=solver,
                             ls_precision=prec, rng=np.random.RandomState(7), selection_stream=np.random.RandomState(8),
                             **args)
-    subs = []
-    real = sim._flip_finite
-
-    def flip_finite(v, m):
-        real(v, m)
-        subs.append(sim.last_substeps)
-    sim._flip_finite = flip_finite
     ev = sim.run_simulation(3, 2e-3, component=(0, 0), max_events=8)
+    subs = sim.kmc_substeps  # N of every KMC event's flip (the C++ event loop's or the Python one's)
     print(f"[kmc3d finite] events {[e[:4] for e in ev]} substeps {subs} (oracle {st_ref['substeps']})", flush=True)
     assert [e[:4] for e in ev] == [e[:4] for e in ev_ref]
     assert subs == st_ref["substeps"] and max(subs) >= 3
@@ -741,14 +733,8 @@ def test_kmc2d_finite_strain_matches_oracle(gpu, model, plane_mode, stz_mode, so
                           temperature=400.0, strain_rate=1e7, plane_mode=plane_mode, strain_assumption="finite_strain",
                           hyperelastic_model=model, solver=solver, rng=np.random.RandomState(7),
                           selection_stream=np.random.RandomState(8), **mp, **args)
-    subs = []
-    real = sim._flip_finite
-
-    def flip_finite(v, m):
-        real(v, m)
-        subs.append(sim.last_substeps)
-    sim._flip_finite = flip_finite
     ev = sim.run_simulation(3, 2e-3, component=(0, 0), stress_targets={(1, 1): 0.0}, max_events=8)
+    subs = sim.kmc_substeps
     print(f"[kmc2d finite] events {[e[:3] for e in ev]} substeps {subs} (oracle {st_ref['substeps']})", flush=True)
     assert [e[:3] for e in ev] == [e[:3] for e in ev_ref]
     assert subs == st_ref["substeps"]
