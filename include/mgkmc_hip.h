@@ -466,6 +466,7 @@ int mgk_mt_fill(mgk_mt_state* state, int kind, double loc, double scale, long lo
 
 struct mgk_kmc_patch_cfg;
 struct mgk_kmc_fs_substep;
+struct mgk_kmc_law_substep;
 
 typedef struct {
   int ndim;
@@ -511,6 +512,10 @@ typedef struct {
    * Cauchy field the barrier pass reads (sig_kind 2; the in-plane block in 2D) and the
    * LS fields are not read */
   const struct mgk_kmc_fs_substep* fs;
+  /* 2D nonlinear small-strain law (mgkmc/kmc_simulator.py:263-385), or NULL: each event's
+   * flip and re-solve are the sub-stepped flip of mgk_kmc_law_flip with this description
+   * (the LS fields above are the law solver's; ls_precision 0) */
+  const struct mgk_kmc_law_substep* law;
 } mgk_kmc_loop;
 
 /* Fast-patching state of a native KMC loop (planar catalogs, sig_kind 0 or 1). */
@@ -690,6 +695,57 @@ typedef struct mgk_kmc_fs_substep {
 int mgk_kmc_fs_flip(const mgk_kmc_fs_substep* cfg, int nflips, const long long* voxels, const int* modes,
                     double time, mgk_mt_state* rng, double* F_macro, double* eps_macro, int* substeps,
                     mgk_fs_info* info, void* stream);
+
+/* One 2D nonlinear small-strain solve as data (FP64): kind 0 mgk_ls_solve_landau2d (warm,
+ * the E33 state persisting: *e33_have in/out, set after a plane-stress solve), kind 1
+ * mgk_ls_solve_secant (secant or Rose law, cold), with the solver's fields and buffers
+ * and the reference's max_iter / tol. */
+typedef struct mgk_law_stepper {
+  int kind;
+  int64_t shape[2];
+  const double* lam;
+  const double* mu;
+  double lam0, mu0;
+  const mgk_landau_params* landau;
+  const mgk_secant_params* secant;
+  int plane_stress;
+  double* eps;
+  double* sig;
+  double* e33;
+  int* e33_have;
+  double* tau;
+  int anderson_m;
+  double* aa;
+  void* ws;
+  size_t ws_bytes;
+  int max_iter;
+  double tol;
+} mgk_law_stepper;
+
+/* The law's sub-stepped flip (mgk_kmc_law_flip): live / live_bytes / backup as in
+ * mgk_kmc_fs_substep (eigenstrain, softening, event times, prev, catalog, barriers, eps,
+ * sig). */
+typedef struct mgk_kmc_law_substep {
+  const mgk_law_stepper* step;
+  int nlive;
+  void* live[12];
+  size_t live_bytes[12];
+  void* backup;
+  size_t backup_bytes;
+} mgk_kmc_law_substep;
+
+/* The flips of one KMC event or cascade batch of a 2D nonlinear-law simulation with the
+ * reference's small-strain sub-stepping (mgkmc/kmc_simulator.py:263-385): one full flip
+ * and solve first; if the solve fails (a non-finite Landau update, a NaN mean stress or
+ * one above 20 GPa) the live state is restored and the flips retried as N sub-steps of
+ * mode / N each followed by a solve, softening, prev and redraws at the last, N =
+ * max(N_h, 2) .. + 19, N_h = min(max over the flips of max(2, ceil(sqrt(C:C/2) /
+ * 0.02)), 40).  Redraws come from `rng`, anew at every attempt.  *substeps = the N that
+ * succeeded; info = the last solve's.  MGK_ERR_SOLVER when every N failed ("Mechanical
+ * solver failed to converge under small-strain sub-stepping."). */
+int mgk_kmc_law_flip(const mgk_kmc_loop* loop, const mgk_kmc_law_substep* cfg, int nflips, const long long* voxels,
+                     const int* modes, double time, mgk_mt_state* rng, const double* eps_macro, int* substeps,
+                     mgk_ls_info* info, void* stream);
 
 #ifdef __cplusplus
 }

@@ -99,7 +99,7 @@ MgkKmcLoop._fields_ = [
     ("ws_bytes", ctypes.c_size_t), ("ws32", ctypes.c_void_p), ("ws32_bytes", ctypes.c_size_t), ("warm", ctypes.c_int),
     ("max_iter", ctypes.c_int), ("tol", ctypes.c_double), ("tol32", ctypes.c_double),
     ("check_unstable", ctypes.c_int), ("strain_rate", ctypes.c_double), ("unit", ctypes.c_double * 9),
-    ("patch", ctypes.c_void_p), ("fs", ctypes.c_void_p)]
+    ("patch", ctypes.c_void_p), ("fs", ctypes.c_void_p), ("law", ctypes.c_void_p)]
 
 
 class MgkKmcPatchCfg(ctypes.Structure):
@@ -162,6 +162,21 @@ class MgkKmcFsSubstep(ctypes.Structure):
                 ("info", ctypes.c_void_p)]
 
 
+class MgkLawStepper(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("shape", ctypes.c_int64 * 2), ("lam", ctypes.c_void_p),
+                ("mu", ctypes.c_void_p), ("lam0", ctypes.c_double), ("mu0", ctypes.c_double),
+                ("landau", ctypes.c_void_p), ("secant", ctypes.c_void_p), ("plane_stress", ctypes.c_int),
+                ("eps", ctypes.c_void_p), ("sig", ctypes.c_void_p), ("e33", ctypes.c_void_p),
+                ("e33_have", ctypes.c_void_p), ("tau", ctypes.c_void_p), ("anderson_m", ctypes.c_int),
+                ("aa", ctypes.c_void_p), ("ws", ctypes.c_void_p), ("ws_bytes", ctypes.c_size_t),
+                ("max_iter", ctypes.c_int), ("tol", ctypes.c_double)]
+
+
+class MgkKmcLawSubstep(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_void_p), ("nlive", ctypes.c_int), ("live", ctypes.c_void_p * 12),
+                ("live_bytes", ctypes.c_size_t * 12), ("backup", ctypes.c_void_p), ("backup_bytes", ctypes.c_size_t)]
+
+
 _c_p = ctypes.c_void_p
 _c_i = ctypes.c_int
 _c_d = ctypes.c_double
@@ -208,6 +223,10 @@ SIGNATURES = {
     "mgk_kmc_fs_flip": (_c_i, [ctypes.POINTER(MgkKmcFsSubstep), _c_i, ctypes.POINTER(ctypes.c_longlong),
                                ctypes.POINTER(_c_i), _c_d, ctypes.POINTER(MgkMtState), _c_dp, _c_dp,
                                ctypes.POINTER(_c_i), ctypes.POINTER(MgkFsInfo), _c_p]),
+    "mgk_kmc_law_flip": (_c_i, [ctypes.POINTER(MgkKmcLoop), ctypes.POINTER(MgkKmcLawSubstep), _c_i,
+                                ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_c_i), _c_d,
+                                ctypes.POINTER(MgkMtState), _c_dp, ctypes.POINTER(_c_i), ctypes.POINTER(MgkLsInfo),
+                                _c_p]),
     "mgk_kmc_mixed_bc": (_c_i, [ctypes.POINTER(MgkKmcLoop), _c_i, ctypes.POINTER(_c_i), _c_dp, _c_i, _c_i, _c_d,
                                 _c_d, _c_d, _c_i, _c_dp, ctypes.POINTER(_c_i), _c_dp, ctypes.POINTER(_c_i),
                                 ctypes.POINTER(MgkLsInfo), ctypes.POINTER(_c_i), _c_p]),

@@ -121,6 +121,23 @@ bool fs_ok(const mgk_kmc_loop* L, const mgk_kmc_fs_substep* c) {
              L->shape[0] * L->shape[1] * (L->ndim == 3 ? L->shape[2] : 1);
 }
 
+// the 2D nonlinear-law sub-stepped flip (defined with mgk_kmc_law_flip below)
+int law_flip_core(const mgk_kmc_loop* L, const mgk_kmc_law_substep* c, int nflips, const long long* voxels,
+                  const int* modes, double time, Mt& R, const double* eps_macro, int* substeps, mgk_ls_info* info,
+                  hipStream_t s);
+
+bool law_ok(const mgk_kmc_loop* L, const mgk_kmc_law_substep* c) {
+  if (!L || !c || L->ndim != 2 || L->M < 1 || !L->cat || !L->Q0 || !L->soft || !L->tlast || !L->epsp ||
+      L->eps_precision != 0 || !L->redraw_dev || !(L->redraw_directions || L->redraw_barriers) || !c->backup ||
+      c->nlive < 0 || c->nlive > 12)
+    return false;
+  const mgk_law_stepper* st = c->step;
+  return st && st->lam && st->mu && st->eps && st->sig && st->tau && st->ws &&
+         (st->kind == 0 ? (st->landau && st->e33 && (st->anderson_m == 0 || st->aa))
+                        : (st->kind == 1 && st->secant)) &&
+         st->shape[0] == L->shape[0] && st->shape[1] == L->shape[1];
+}
+
 bool patch_ok(const mgk_kmc_loop* L) {
   const mgk_kmc_patch_cfg* P = L->patch;
   return !P || (P->kernels && P->sig_unit && P->workspace && P->flips_since_sync && P->unit_live && P->sig_mean &&
@@ -130,8 +147,10 @@ bool patch_ok(const mgk_kmc_loop* L) {
 bool loop_ok(const mgk_kmc_loop* L, const mgk_mt_state* rng, const mgk_mt_state* sel_rng) {
   return L && rng && sel_rng && (L->ndim == 2 || L->ndim == 3) && L->M >= 1 && L->Q0 && L->cat && L->soft &&
          L->tlast && L->epsp && L->kmc_ws && L->redraw_dev && L->sig &&
-         (L->fs ? (fs_ok(L, L->fs) && L->fs->F_macro && !L->patch)
-                : (L->eps && L->ws && (L->ls_precision != 2 || (L->lam32 && L->mu32 && L->eps32 && L->ws32)))) &&
+         (L->fs ? (fs_ok(L, L->fs) && L->fs->F_macro && !L->patch && !L->law)
+                : L->law ? (law_ok(L, L->law) && !L->patch)
+                         : (L->eps && L->ws &&
+                            (L->ls_precision != 2 || (L->lam32 && L->mu32 && L->eps32 && L->ws32)))) &&
          rng->pos >= 0 && rng->pos <= 624 && sel_rng->pos >= 0 && sel_rng->pos <= 624 && patch_ok(L);
 }
 
@@ -254,6 +273,28 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
             for (int b = 0; b < d; ++b)
               pin.sig_mean[a * d + b] = fsi.sig_mean[d == 3 ? 3 * a + b : 3 * (2 * a) + 2 * b];
           fill_event(events[n], v, m, instab, *time, pin, 0, eps_macro, d);
+          events[n++].substeps = nsub;
+          *n_events = n;
+          *seq = instab ? *seq + 1 : 0;
+          if (*seq > max_seq) {
+            rc = MGK_KMC_RUN_SEQ;
+            break;
+          }
+          if (n >= max_events) {
+            rc = MGK_KMC_RUN_COUNT;
+            break;
+          }
+          continue;
+        }
+        if (L->law) {
+          // 2D nonlinear law: the sub-stepped flip (flip, redraw and the law's solves)
+          int nsub = 0;
+          e = law_flip_core(L, L->law, 1, &v, &m, *time, R, eps_macro, &nsub, info, s);
+          if (e != MGK_OK) {
+            rc = e;
+            break;
+          }
+          fill_event(events[n], v, m, instab, *time, *info, 0, eps_macro, d);
           events[n++].substeps = nsub;
           *n_events = n;
           *seq = instab ? *seq + 1 : 0;
@@ -416,7 +457,7 @@ extern "C" int mgk_kmc_run_batch(int nb, const mgk_kmc_loop* L, mgk_mt_state* rn
     return MGK_ERR_ARG;
   }
   for (int b = 0; b < nb; ++b)
-    if (L[b].patch || L[b].fs) {
+    if (L[b].patch || L[b].fs || L[b].law) {
       mgk_set_error("mgk_kmc_run_batch: fast patching and finite strain run in mgk_kmc_run only", __FILE__,
                     __LINE__);
       return MGK_ERR_ARG;
@@ -717,7 +758,7 @@ extern "C" int mgk_kmc_cascade(const mgk_kmc_loop* L, mgk_mt_state* rng, double*
                                int* iters32, void* stream) {
   mgk_mt_state sel_dummy{};
   sel_dummy.pos = 624;
-  if (!loop_ok(L, rng, &sel_dummy) || L->fs || !Q || !eps_macro || !flips || cap < 1 || !n_flips || !batch_sizes ||
+  if (!loop_ok(L, rng, &sel_dummy) || L->fs || L->law || !Q || !eps_macro || !flips || cap < 1 || !n_flips || !batch_sizes ||
       batch_cap < 1 || !n_batches || !reason || !info || max_batches < 0 ||
       (!L->redraw_directions && !L->redraw_barriers)) {
     mgk_set_error("mgk_kmc_cascade: bad argument", __FILE__, __LINE__);
@@ -1289,4 +1330,131 @@ extern "C" int mgk_kmc_fs_flip(const mgk_kmc_fs_substep* c, int nflips, const lo
   Mt R{rng};
   return fs_flip_core(L, c, nflips, voxels, modes, time, R, F_macro, eps_macro, substeps, info,
                       (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// Native small-strain sub-stepping of the 2D nonlinear laws (mgkmc/
+// kmc_simulator.py:263-385): one full flip first, then N sub-steps of mode / N.
+// ---------------------------------------------------------------------------
+namespace {
+
+// elastic_run with the law (mgkmc/kmc_simulator.py:449-525): 0 done, 1 the
+// reference's ValueError / FloatingPointError (retry), < 0 an error
+int law_update(const mgk_kmc_loop* L, const mgk_law_stepper* st, const double* eps_macro, mgk_ls_info* info,
+               hipStream_t s) {
+  int rc;
+  if (st->kind == 0) {
+    const int have = (st->plane_stress && st->e33_have && *st->e33_have) ? 1 : 0;
+    rc = mgk_ls_solve_landau2d(st->shape, st->lam, st->mu, st->lam0, st->mu0, st->landau, st->plane_stress,
+                               eps_macro, (const double*)L->epsp, st->eps, st->sig, st->e33, 1, have, st->tau,
+                               st->anderson_m, st->aa, 1, st->max_iter, st->tol, st->ws, st->ws_bytes, s, info);
+    if (st->plane_stress && st->e33_have) *st->e33_have = 1;
+    if (rc == MGK_ERR_NONFINITE) return 1;
+  } else {
+    rc = mgk_ls_solve_secant(2, st->shape, 0, st->lam, st->mu, st->lam0, st->mu0, st->secant, eps_macro,
+                             L->epsp, st->eps, st->sig, st->tau, 0, st->max_iter, st->tol, st->ws, st->ws_bytes, s,
+                             info);
+  }
+  if (rc < 0 && rc != MGK_ERR_NONFINITE) return rc;
+  return unstable_mean(*info, 2) ? 1 : 0;
+}
+
+int law_flip_core(const mgk_kmc_loop* L, const mgk_kmc_law_substep* c, int nflips, const long long* voxels,
+                  const int* modes, double time, Mt& R, const double* eps_macro, int* substeps, mgk_ls_info* info,
+                  hipStream_t s) {
+  const int d = 2, M = L->M;
+  const long long NV = L->shape[0] * L->shape[1];
+  int shape_i[3] = {(int)L->shape[0], (int)L->shape[1], 1};
+  size_t off[12], total = 0;
+  for (int k = 0; k < c->nlive; ++k) {
+    off[k] = total;
+    total += (c->live_bytes[k] + 255) / 256 * 256;
+  }
+  if (total > c->backup_bytes) {
+    mgk_set_error("mgk_kmc_law_flip: backup area too small", __FILE__, __LINE__);
+    return MGK_ERR_WORKSPACE;
+  }
+  for (int f = 0; f < nflips; ++f)
+    if (voxels[f] < 0 || voxels[f] >= NV || modes[f] < 0 || modes[f] >= M) {
+      mgk_set_error("mgk_kmc_law_flip: flip outside the grid or the catalog", __FILE__, __LINE__);
+      return MGK_ERR_ARG;
+    }
+  double* host = redraw_stage(3 * (size_t)M + 2 * (size_t)nflips);
+  if (!host) {
+    mgk_set_error("mgk_kmc_law_flip: pinned host allocation failed", __FILE__, __LINE__);
+    return MGK_ERR_HIP;
+  }
+  double* cm = host + 3 * (size_t)M;
+  for (int k = 0; k < c->nlive; ++k)
+    if (c->live[k] && c->live_bytes[k])
+      MGK_HIP(hipMemcpyAsync((char*)c->backup + off[k], c->live[k], c->live_bytes[k], hipMemcpyDeviceToDevice, s));
+  auto restore = [&]() -> int {
+    for (int k = 0; k < c->nlive; ++k)
+      if (c->live[k] && c->live_bytes[k])
+        MGK_HIP(hipMemcpyAsync(c->live[k], (char*)c->backup + off[k], c->live_bytes[k], hipMemcpyDeviceToDevice, s));
+    return MGK_OK;
+  };
+  std::vector<int> schedule{1};
+  for (size_t si = 0; si < schedule.size(); ++si) {
+    const int N = schedule[si];
+    int e;
+    if (N > 1 && (e = restore()) != MGK_OK) return e;
+    bool failed = false;
+    for (int it = 1; it <= N && !failed; ++it) {
+      for (int f = 0; f < nflips; ++f) {
+        e = mgk_kmc_flip(d, shape_i, M, 0, voxels[f], modes[f], time, L->jp, L->jt, L->kp.cap, L->jn, L->cat,
+                         L->epsp, L->eps_precision, L->epsp_shadow, L->soft, L->tlast, L->prev, nullptr,
+                         it < N ? -N : N, s);
+        if (e != MGK_OK) return e;
+        if (it == N) {
+          redraw_draws(L, R, host);
+          MGK_HIP(hipMemcpyAsync(L->redraw_dev, host, sizeof(double) * 3 * (size_t)M, hipMemcpyHostToDevice, s));
+          e = mgk_kmc_set_voxel(NV, M, 2, voxels[f], L->redraw_directions ? L->redraw_dev : nullptr,
+                                L->redraw_barriers ? L->redraw_dev + 2 * M : nullptr, L->cat, L->Q0, s);
+          if (e != MGK_OK) return e;
+          MGK_HIP(hipStreamSynchronize(s));
+        }
+      }
+      e = law_update(L, c->step, eps_macro, info, s);
+      if (e < 0) return e;
+      failed = e == 1;
+    }
+    if (!failed) {
+      *substeps = N;
+      return MGK_OK;
+    }
+    if (N == 1) {
+      // restore, then N_h from the modes as they were (mgkmc/kmc_simulator.py:290-300)
+      if ((e = restore()) != MGK_OK) return e;
+      for (int f = 0; f < nflips; ++f)
+        for (int k = 0; k < 2; ++k)
+          MGK_HIP(hipMemcpyAsync(cm + 2 * f + k, L->cat + ((size_t)k * M + modes[f]) * NV + voxels[f],
+                                 sizeof(double), hipMemcpyDeviceToHost, s));
+      MGK_HIP(hipStreamSynchronize(s));
+      int nh = 2;
+      for (int f = 0; f < nflips; ++f) {
+        const double g0 = cm[2 * f], g1 = cm[2 * f + 1];
+        const double ss = ((g0 * g0 + g1 * g1) + g1 * g1) + g0 * g0;  // C:C of [[g0, g1], [g1, -g0]]
+        nh = std::max(nh, std::max(2, (int)std::ceil(std::sqrt(0.5 * ss) / 0.02)));
+      }
+      nh = std::min(nh, 40);
+      for (int k = 0; k < 20; ++k) schedule.push_back(std::max(nh, 2) + k);
+    }
+  }
+  mgk_set_error("Mechanical solver failed to converge under small-strain sub-stepping.", __FILE__, __LINE__);
+  return MGK_ERR_SOLVER;
+}
+
+}  // namespace
+
+extern "C" int mgk_kmc_law_flip(const mgk_kmc_loop* L, const mgk_kmc_law_substep* c, int nflips,
+                                const long long* voxels, const int* modes, double time, mgk_mt_state* rng,
+                                const double* eps_macro, int* substeps, mgk_ls_info* info, void* stream) {
+  if (!law_ok(L, c) || nflips < 1 || !voxels || !modes || !rng || rng->pos < 0 || rng->pos > 624 || !eps_macro ||
+      !substeps || !info) {
+    mgk_set_error("mgk_kmc_law_flip: bad argument", __FILE__, __LINE__);
+    return MGK_ERR_ARG;
+  }
+  Mt R{rng};
+  return law_flip_core(L, c, nflips, voxels, modes, time, R, eps_macro, substeps, info, (hipStream_t)stream);
 }

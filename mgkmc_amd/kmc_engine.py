@@ -830,6 +830,92 @@ class DeviceKmc:
                 and (self.redraw_directions or self.redraw_barriers) and hasattr(self.rng, "get_state")
                 and self.fs.precision == 0)
 
+    def _native_law_ok(self):
+        """Whether the 2D nonlinear-law sub-stepped flip runs in the library
+        (mgk_kmc_law_flip; also inside mgk_kmc_run): Landau (fixed point),
+        secant or Rose, planar catalogs from the Gaussian generator on the host
+        stream, no thermal coupling or flip log."""
+        gen = self.barrier_generator
+        gp = getattr(gen, "params", None)
+        same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
+        return (self.native_loop and self.law is not None and self.ndim == 2 and not self.finite
+                and self.law[0] in ("landau", "secant", "rose") and not self.enable_thermal and self.kmc.planar
+                and self._flip_log is None and gp is not None and same_stream
+                and (self.redraw_directions or self.redraw_barriers) and hasattr(self.rng, "get_state")
+                and self.kmc.eps_precision == FP64)
+
+    def _law_substep_cfg(self):
+        """MgkKmcLawSubstep of the law solver: its fields and buffers, the
+        reference's solve arguments, the live state and its backup."""
+        ls, k = self.ls, self.kmc
+        torch = dev.torch
+        args = dict(max_iter=400, tol=1e-6)
+        args.update(self.solver_args)
+        st = _lib.MgkLawStepper()
+        st.shape[0], st.shape[1] = self.grid_shape[0], self.grid_shape[1]
+        st.lam, st.mu, st.lam0, st.mu0 = _lib.dptr(ls.lam), _lib.dptr(ls.mu), ls.lam0, ls.mu0
+        keep = [st]
+        if self.law[0] == "landau":
+            if not hasattr(self, "_law_params"):
+                self._law_params = landau_params(*self.law[1], mu=self.ls.mu_host, **self.law[2])
+            st.kind, st.landau, st.plane_stress = 0, ctypes.addressof(self._law_params), int(ls.plane_stress)
+            st.e33 = _lib.dptr(ls.e33)
+            flag = ctypes.c_int(int(ls.e33_have))
+            st.e33_have = ctypes.addressof(flag)
+            st.anderson_m, st.aa = ls.anderson_m, _lib.dptr(ls.aa)
+            keep += [flag, self._law_params]
+        else:
+            st.kind, st.secant = 1, ctypes.addressof(ls.params)
+        st.eps, st.sig, st.tau = _lib.dptr(ls.eps), _lib.dptr(ls.sig), _lib.dptr(ls.tau)
+        st.ws, st.ws_bytes = _lib.dptr(ls.ws), ls.ws.numel()
+        st.max_iter, st.tol = int(args["max_iter"]), float(args["tol"])
+        c = _lib.MgkKmcLawSubstep()
+        c.step = ctypes.addressof(st)
+        live = [k.eps_p, k.soft, k.tlast, k.prev, k.cat, k.Q0, ls.eps, ls.sig]
+        c.nlive = len(live)
+        total = 0
+        for q, a in enumerate(live):
+            c.live[q] = _lib.dptr(a)
+            c.live_bytes[q] = 0 if a is None else a.numel() * a.element_size()
+            total += (c.live_bytes[q] + 255) // 256 * 256
+        if getattr(self, "_law_backup", None) is None or self._law_backup.numel() < total:
+            self._law_backup = torch.empty(total, dtype=torch.uint8, device="cuda")
+        c.backup, c.backup_bytes = _lib.dptr(self._law_backup), self._law_backup.numel()
+        c._keep = keep
+        return c
+
+    def _law_after(self, c):
+        """Solver state after a native law flip: the persisting E33 flag and
+        the last solve's record."""
+        if self.law[0] == "landau":
+            self.ls.e33_have = bool(c._keep[1].value)
+        self.ls_info = self.ls.info
+        self._solved_once = True
+
+    def _flip_substepped_native(self, flips):
+        """_flip_substepped through mgk_kmc_law_flip (same schedule, same
+        host-stream redraws)."""
+        d = self.ndim
+        L = self._loop_struct(np.zeros((d, d)), patch=False)
+        c = self._law_substep_cfg()
+        n = len(flips)
+        vv = (ctypes.c_longlong * n)(*[int(v) for v, _ in flips])
+        mm = (ctypes.c_int * n)(*[int(m) for _, m in flips])
+        rng = _lib.mt_from(self.rng)
+        em = (ctypes.c_double * 9)(*self.eps_macro.reshape(-1))
+        nsub = ctypes.c_int()
+        try:
+            rc = _lib.load().mgk_kmc_law_flip(ctypes.byref(L), ctypes.byref(c), n, vv, mm, float(self.time),
+                                              ctypes.byref(rng), em, ctypes.byref(nsub), ctypes.byref(self.ls.info),
+                                              ctypes.c_void_p(dev.stream_ptr()))
+        finally:
+            _lib.mt_to(self.rng, rng)
+        self._law_after(c)
+        if rc == _lib.MGK_ERR_SOLVER:
+            raise ValueError("Mechanical solver failed to converge under small-strain sub-stepping.")
+        _lib.check(rc, "mgk_kmc_law_flip")
+        self.last_substeps = nsub.value
+
     def _flip_finite_native(self, pairs):
         """_flip_finite_pairs through mgk_kmc_fs_flip: the same sub-stepping
         schedule, redraws from the same host stream, F_macro / eps_macro and
@@ -1061,6 +1147,8 @@ class DeviceKmc:
         flips of max(2, ceil(sqrt(C:C/2) / 0.02)), 40)), softening, prev and
         redraws at the last, one re-solve per sub-step.  Every attempt draws
         its redraws anew from the host stream."""
+        if self._native_law_ok():
+            return self._flip_substepped_native(flips)
         k = self.kmc
         live = [k.eps_p, k.soft, k.tlast, k.prev, k.cat, k.Q0, self.ls.eps, self.ls.sig, self.Tfield]
         backup = [None if a is None else a.clone() for a in live]
@@ -1322,7 +1410,8 @@ class DeviceKmc:
         gen = self.barrier_generator
         gp = getattr(gen, "params", None)
         same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
-        return (self.native_loop and self.law is None and (not self.finite or self._native_fs_ok())
+        return (self.native_loop and (self.law is None or self._native_law_ok())
+                and (not self.finite or self._native_fs_ok())
                 and not self.cascade_mode and not self.enable_thermal and self.kmc.planar and self._flip_log is None
                 and gp is not None and same_stream and (self.redraw_directions or self.redraw_barriers)
                 and hasattr(self.rng, "get_state") and hasattr(self.selection_stream, "get_state")
@@ -1443,7 +1532,7 @@ class DeviceKmc:
             ev = coords + (int(e.m), float(e.time), bool(e.is_instab))
             self.kmc_event_history.append(ev)
             self.ls_iters_total += e.ls_iters + e.ls_iters32
-            if self.finite:
+            if self.finite or self.law is not None:
                 self.kmc_substeps.append(int(e.substeps))
                 self.last_substeps = int(e.substeps)
             if on_event is not None:
@@ -1461,6 +1550,9 @@ class DeviceKmc:
         d = self.ndim
         lib = _lib.load()
         L = self._loop_struct(unit)
+        if self.law is not None:  # each event's flip is the law's sub-stepped flip
+            lcfg = self._law_substep_cfg()
+            L.law = ctypes.addressof(lcfg)
         if self.finite:  # each event's flip is the sub-stepped finite-strain flip
             cfg = self._fs_substep_cfg(L)
             fm = (ctypes.c_double * 9)(*self.F_macro.reshape(-1))
@@ -1493,6 +1585,10 @@ class DeviceKmc:
                     self.ls_info = fsi
                 if self.finite and rc < 0:
                     self._fs_raise(rc)
+                if self.law is not None:
+                    self._law_after(lcfg)
+                    if rc == _lib.MGK_ERR_SOLVER:
+                        raise ValueError("Mechanical solver failed to converge under small-strain sub-stepping.")
                 if rc == _lib.MGK_KMC_RUN_UNSTABLE:
                     sm = np.array(self.ls.info.sig_mean[: d * d]).reshape(d, d)
                     raise ValueError(f"Unstable stress in small-strain elastic_run: {sm / 1e9} GPa")

@@ -301,6 +301,10 @@ def test_kmc2d_nonlinear_law_sequence_matches_oracle(gpu, model, plane_mode, mp,
                           temperature=1276.5, strain_rate=1e7, plane_mode=plane_mode, hyperelastic_model=model,
                           rng=np.random.RandomState(21), selection_stream=np.random.RandomState(22), **mp, **args)
     sim.solver_args = {"tol": 1e-8}
+    # the failure injection wraps the Python elastic_run: those cases run the Python
+    # sub-stepping; the others run the C++ event loop with the library's sub-stepping
+    sim.native_loop = fail is None
+    assert sim._native_ok() == (fail is None)
     armed = [False]
     substeps = []
     real_run, real_fs = sim.elastic_run, sim._flip_substepped
@@ -320,6 +324,9 @@ def test_kmc2d_nonlinear_law_sequence_matches_oracle(gpu, model, plane_mode, mp,
     sim._flip_substepped = flip_substepped
     ev = sim.run_simulation(10 ** 9, 1e-2, component=(0, 0), stress_targets={(1, 1): 0.0}, mixed_tol=1e6,
                             max_events=20)
+    if fail is None:
+        assert substeps == [] and len(sim.kmc_substeps) == len(ev)
+        substeps = sim.kmc_substeps
     print("events:", [e[:3] for e in ev[:8]], "substeps:", substeps, flush=True)
     assert substeps == st_ref["substeps"]
     assert [e[:3] for e in ev] == [e[:3] for e in ev_ref]
@@ -540,6 +547,41 @@ def test_native_finite_flip_matches_python_flip(gpu, d, prec, solver, model, cas
     assert np.abs(a.sig_field - b.sig_field).max() <= stol * np.abs(b.sig_field).max()
     sa, sb = a.rng.get_state(), b.rng.get_state()
     assert np.array_equal(sa[1], sb[1]) and sa[2:] == sb[2:]
+
+
+@pytest.mark.parametrize("model,mp", [("secant_degradation", dict(d=0.5, k=30.0)), ("landau", _NL_LANDAU)])
+def test_native_law_flip_retry_schedule_matches_python(gpu, model, mp):
+    # mgk_kmc_law_flip's retry schedule against the Python sub-stepping: a macro strain
+    # whose mean stress exceeds 20 GPa makes the one-step solve and every one of the 20
+    # sub-stepped attempts fail (forced from the host), so both raise the reference's
+    # ValueError after drawing the same redraws (21 attempts) and leave the same state
+    from mgkmc_amd.kmc_simulator import KmcSimulation2D
+    print(f"[native law retry] {model}", flush=True)
+    n, M = 16, 8
+    rng = np.random.default_rng(2)
+    mu = 26.923e9 * rng.uniform(0.9, 1.1, (n, n))
+    nu = np.full((n, n), 0.35)
+    E = 2 * mu * (1 + nu)
+    out = []
+    for native in (True, False):
+        sim = KmcSimulation2D(n, n, M, 0.14, E, nu, pixel=0.7, barrier_generator="gaussian",
+                              barrier_kwargs=dict(mean=2.2, std=0.45, min_cutoff=0.0), temperature=1276.5,
+                              strain_rate=1e7, plane_mode="plane_strain", hyperelastic_model=model,
+                              rng=np.random.RandomState(21), selection_stream=np.random.RandomState(22),
+                              softening_cap=0.916, jp=20.0, jt=0.0, **mp)
+        sim.native_loop = native
+        assert sim._native_law_ok() == native
+        sim.eps_macro[...] = np.diag([0.4, 0.0])  # <sigma_xx> far above 20 GPa
+        with pytest.raises(ValueError, match="sub-stepping"):
+            sim._flip_substepped([(37, 3), (200, 5)])
+        out.append(sim)
+    a, b = out
+    sa, sb = a.rng.get_state(), b.rng.get_state()
+    assert np.array_equal(sa[1], sb[1]) and sa[2:] == sb[2:]
+    for name in ("eps_p", "soft", "tlast", "cat", "Q0"):
+        xa, xb = getattr(a.kmc, name), getattr(b.kmc, name)
+        assert np.array_equal(xa.cpu().numpy(), xb.cpu().numpy()), name
+    print(f"[native law retry] last N {a.last_substeps if hasattr(a, 'last_substeps') else None}", flush=True)
 
 
 @pytest.mark.parametrize("d,prec,T,pct,records", [
