@@ -512,9 +512,10 @@ typedef struct {
    * Cauchy field the barrier pass reads (sig_kind 2; the in-plane block in 2D) and the
    * LS fields are not read */
   const struct mgk_kmc_fs_substep* fs;
-  /* 2D nonlinear small-strain law (mgkmc/kmc_simulator.py:263-385), or NULL: each event's
-   * flip and re-solve are the sub-stepped flip of mgk_kmc_law_flip with this description
-   * (the LS fields above are the law solver's; ls_precision 0) */
+  /* nonlinear small-strain law (mgkmc/kmc_simulator.py:263-385; 3D Landau mgkmc/aqs.py:
+   * 330-351), or NULL: each event's flip and re-solve are the sub-stepped flip of
+   * mgk_kmc_law_flip with this description (3D Landau: flip, redraw, cold Landau solve;
+   * the LS fields above are the law solver's; ls_precision 0) */
   const struct mgk_kmc_law_substep* law;
 } mgk_kmc_loop;
 
@@ -696,10 +697,12 @@ int mgk_kmc_fs_flip(const mgk_kmc_fs_substep* cfg, int nflips, const long long* 
                     double time, mgk_mt_state* rng, double* F_macro, double* eps_macro, int* substeps,
                     mgk_fs_info* info, void* stream);
 
-/* One 2D nonlinear small-strain solve as data (FP64): kind 0 mgk_ls_solve_landau2d (warm,
+/* One nonlinear small-strain solve as data (FP64): kind 0 mgk_ls_solve_landau2d (warm,
  * the E33 state persisting: *e33_have in/out, set after a plane-stress solve), kind 1
- * mgk_ls_solve_secant (secant or Rose law, cold), with the solver's fields and buffers
- * and the reference's max_iter / tol. */
+ * mgk_ls_solve_secant (2D secant or Rose law, cold), kind 2 mgk_ls_solve_landau (3D, cold;
+ * the loop's own grid; its events are not sub-stepped: flip, redraw, solve, and a
+ * non-finite update stops the loop with MGK_ERR_NONFINITE, mgkmc/aqs.py:330-351), with the
+ * solver's fields and buffers and the reference's max_iter / tol. */
 typedef struct mgk_law_stepper {
   int kind;
   int64_t shape[2];

@@ -127,13 +127,14 @@ int law_flip_core(const mgk_kmc_loop* L, const mgk_kmc_law_substep* c, int nflip
                   hipStream_t s);
 
 bool law_ok(const mgk_kmc_loop* L, const mgk_kmc_law_substep* c) {
-  if (!L || !c || L->ndim != 2 || L->M < 1 || !L->cat || !L->Q0 || !L->soft || !L->tlast || !L->epsp ||
+  if (!L || !c || (L->ndim != 2 && L->ndim != 3) || L->M < 1 || !L->cat || !L->Q0 || !L->soft || !L->tlast || !L->epsp ||
       L->eps_precision != 0 || !L->redraw_dev || !(L->redraw_directions || L->redraw_barriers) || !c->backup ||
       c->nlive < 0 || c->nlive > 12)
     return false;
   const mgk_law_stepper* st = c->step;
-  return st && st->lam && st->mu && st->eps && st->sig && st->tau && st->ws &&
-         (st->kind == 0 ? (st->landau && st->e33 && (st->anderson_m == 0 || st->aa))
+  if (!st || !st->lam || !st->mu || !st->eps || !st->sig || !st->tau || !st->ws) return false;
+  if (L->ndim == 3) return st->kind == 2 && st->landau;
+  return (st->kind == 0 ? (st->landau && st->e33 && (st->anderson_m == 0 || st->aa))
                         : (st->kind == 1 && st->secant)) &&
          st->shape[0] == L->shape[0] && st->shape[1] == L->shape[1];
 }
@@ -287,9 +288,38 @@ extern "C" int mgk_kmc_run(const mgk_kmc_loop* L, mgk_mt_state* rng, mgk_mt_stat
           continue;
         }
         if (L->law) {
-          // 2D nonlinear law: the sub-stepped flip (flip, redraw and the law's solves)
+          // nonlinear law: the 2D sub-stepped flip (flip, redraw and the law's solves),
+          // or the 3D Landau event (flip, redraw, cold solve)
           int nsub = 0;
-          e = law_flip_core(L, L->law, 1, &v, &m, *time, R, eps_macro, &nsub, info, s);
+          if (L->law->step->kind == 2) {
+            e = mgk_kmc_flip(d, shape_i, M, 0, v, m, *time, L->jp, L->jt, L->kp.cap, L->jn, L->cat, L->epsp,
+                             L->eps_precision, L->epsp_shadow, L->soft, L->tlast, L->prev, nullptr, 1, stream);
+            if (e == MGK_OK) {
+              redraw_draws(L, R, host);
+              if (hipMemcpyAsync(L->redraw_dev, host, sizeof(double) * 3 * (size_t)M, hipMemcpyHostToDevice, s) !=
+                  hipSuccess) {
+                mgk_set_error("mgk_kmc_run: redraw upload failed", __FILE__, __LINE__);
+                e = MGK_ERR_HIP;
+              }
+            }
+            if (e == MGK_OK)
+              e = mgk_kmc_set_voxel(NV, M, 2, v, L->redraw_directions ? L->redraw_dev : nullptr,
+                                    L->redraw_barriers ? L->redraw_dev + 2 * M : nullptr, L->cat, L->Q0, stream);
+            if (e == MGK_OK) {
+              const mgk_law_stepper* st = L->law->step;
+              e = mgk_ls_solve_landau(L->shape, 0, st->lam, st->mu, st->lam0, st->mu0, st->landau, eps_macro,
+                                      L->epsp, st->eps, st->sig, st->tau, 0, st->max_iter, st->tol, st->ws,
+                                      st->ws_bytes, stream, info);
+              if (e == MGK_ERR_NONFINITE)
+                mgk_set_error("spectral_solver_landau_3d: non-finite strain update in LS loop (Landau energy likely "
+                              "unstable at this strain; consider strain capping or smaller load steps).",
+                              __FILE__, __LINE__);
+              else if (e == MGK_NOT_CONVERGED)
+                e = MGK_OK;
+            }
+          } else {
+            e = law_flip_core(L, L->law, 1, &v, &m, *time, R, eps_macro, &nsub, info, s);
+          }
           if (e != MGK_OK) {
             rc = e;
             break;
@@ -1450,7 +1480,7 @@ int law_flip_core(const mgk_kmc_loop* L, const mgk_kmc_law_substep* c, int nflip
 extern "C" int mgk_kmc_law_flip(const mgk_kmc_loop* L, const mgk_kmc_law_substep* c, int nflips,
                                 const long long* voxels, const int* modes, double time, mgk_mt_state* rng,
                                 const double* eps_macro, int* substeps, mgk_ls_info* info, void* stream) {
-  if (!law_ok(L, c) || nflips < 1 || !voxels || !modes || !rng || rng->pos < 0 || rng->pos > 624 || !eps_macro ||
+  if (!law_ok(L, c) || c->step->kind == 2 || nflips < 1 || !voxels || !modes || !rng || rng->pos < 0 || rng->pos > 624 || !eps_macro ||
       !substeps || !info) {
     mgk_set_error("mgk_kmc_law_flip: bad argument", __FILE__, __LINE__);
     return MGK_ERR_ARG;

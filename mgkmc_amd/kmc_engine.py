@@ -838,8 +838,9 @@ class DeviceKmc:
         gen = self.barrier_generator
         gp = getattr(gen, "params", None)
         same_stream = getattr(gen, "rng", 0) is (None if self.rng is np.random else self.rng)
-        return (self.native_loop and self.law is not None and self.ndim == 2 and not self.finite
-                and self.law[0] in ("landau", "secant", "rose") and not self.enable_thermal and self.kmc.planar
+        return (self.native_loop and self.law is not None and not self.finite
+                and (self.law[0] in ("landau", "secant", "rose") if self.ndim == 2 else self.law[0] == "landau")
+                and not self.enable_thermal and self.kmc.planar
                 and self._flip_log is None and gp is not None and same_stream
                 and (self.redraw_directions or self.redraw_barriers) and hasattr(self.rng, "get_state")
                 and self.kmc.eps_precision == FP64)
@@ -852,10 +853,16 @@ class DeviceKmc:
         args = dict(max_iter=400, tol=1e-6)
         args.update(self.solver_args)
         st = _lib.MgkLawStepper()
-        st.shape[0], st.shape[1] = self.grid_shape[0], self.grid_shape[1]
+        if self.ndim == 2:
+            st.shape[0], st.shape[1] = self.grid_shape[0], self.grid_shape[1]
         st.lam, st.mu, st.lam0, st.mu0 = _lib.dptr(ls.lam), _lib.dptr(ls.mu), ls.lam0, ls.mu0
         keep = [st]
-        if self.law[0] == "landau":
+        if self.law[0] == "landau" and self.ndim == 3:  # cold solves, no sub-stepping
+            if not hasattr(self, "_law_params"):
+                self._law_params = landau_params(*self.law[1], mu=self.ls.mu_host, **self.law[2])
+            st.kind, st.landau = 2, ctypes.addressof(self._law_params)
+            keep += [None, self._law_params]
+        elif self.law[0] == "landau":
             if not hasattr(self, "_law_params"):
                 self._law_params = landau_params(*self.law[1], mu=self.ls.mu_host, **self.law[2])
             st.kind, st.landau, st.plane_stress = 0, ctypes.addressof(self._law_params), int(ls.plane_stress)
@@ -887,7 +894,7 @@ class DeviceKmc:
     def _law_after(self, c):
         """Solver state after a native law flip: the persisting E33 flag and
         the last solve's record."""
-        if self.law[0] == "landau":
+        if self.law[0] == "landau" and self.ndim == 2:
             self.ls.e33_have = bool(c._keep[1].value)
         self.ls_info = self.ls.info
         self._solved_once = True
@@ -1532,7 +1539,7 @@ class DeviceKmc:
             ev = coords + (int(e.m), float(e.time), bool(e.is_instab))
             self.kmc_event_history.append(ev)
             self.ls_iters_total += e.ls_iters + e.ls_iters32
-            if self.finite or self.law is not None:
+            if self.finite or (self.law is not None and d == 2):
                 self.kmc_substeps.append(int(e.substeps))
                 self.last_substeps = int(e.substeps)
             if on_event is not None:
@@ -1589,6 +1596,8 @@ class DeviceKmc:
                     self._law_after(lcfg)
                     if rc == _lib.MGK_ERR_SOLVER:
                         raise ValueError("Mechanical solver failed to converge under small-strain sub-stepping.")
+                    if rc == _lib.MGK_ERR_NONFINITE:  # 3D Landau (LandauSolver.solve's error)
+                        raise FloatingPointError(_lib.last_error().split(" (/")[0])
                 if rc == _lib.MGK_KMC_RUN_UNSTABLE:
                     sm = np.array(self.ls.info.sig_mean[: d * d]).reshape(d, d)
                     raise ValueError(f"Unstable stress in small-strain elastic_run: {sm / 1e9} GPa")

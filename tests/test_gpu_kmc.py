@@ -357,10 +357,20 @@ def test_kmc3d_landau_sequence_matches_linear_limit_and_runs(gpu):
         runs.append([e[:4] for e in sim.run_simulation(10 ** 9, 1e-3, max_events=15)])
     print("[kmc3d landau] linear limit events:", runs[1][:5], flush=True)
     assert runs[0] == runs[1]
-    sim = ThermalSimulation(n, n, n, M, 0.1, E, nu, hyperelastic_model="landau", rng=np.random.RandomState(3),
-                            selection_stream=np.random.RandomState(4), **_NL_LANDAU, **base)
-    ev = sim.run_simulation(10 ** 9, 1e-3, max_events=10)
-    assert len(ev) == 10 and np.all(np.isfinite(sim.sig_mean()))
+    sims = []
+    for native in (True, False):  # the C++ event loop (cold Landau solves) against the Python one
+        sim = ThermalSimulation(n, n, n, M, 0.1, E, nu, hyperelastic_model="landau", rng=np.random.RandomState(3),
+                                selection_stream=np.random.RandomState(4), **_NL_LANDAU, **base)
+        sim.native_loop = native
+        assert sim._native_ok() == native
+        ev = sim.run_simulation(10 ** 9, 1e-3, max_events=10)
+        assert len(ev) == 10 and np.all(np.isfinite(sim.sig_mean()))
+        sims.append(sim)
+    a, b = sims
+    assert a.kmc_event_history == b.kmc_event_history
+    assert np.array_equal(a.eps_plastic, b.eps_plastic) and np.array_equal(a.sig_field, b.sig_field)
+    sa, sb = a.rng.get_state(), b.rng.get_state()
+    assert np.array_equal(sa[1], sb[1]) and sa[2:] == sb[2:]
 
 
 @pytest.mark.parametrize("T,mean,steps,model,mp", [
